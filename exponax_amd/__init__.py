@@ -1,0 +1,14 @@
+"""exponax_amd — exponax's ETDRK pseudo-spectral step on AMD MI355X (gfx950).
+
+The stepper constructors, ``(C, N, ..., N)`` state layout, ``__call__`` /
+``step_fourier`` / ``rollout`` / ``repeat`` follow exponax 0.2.0; every step
+runs in hand-written HIP kernels behind the C ABI in ``include/exmi.h``.
+"""
+from . import nonlin_fun, spectral, stepper
+from ._base_stepper import BaseStepper, Ensemble, ensemble
+from ._lib import ExmiError, load_library
+from ._utils import make_grid, repeat, rollout, vmap
+from .spectral import fft, ifft
+
+__all__ = ["BaseStepper", "Ensemble", "ensemble", "ExmiError", "fft", "ifft", "load_library", "make_grid",
+           "nonlin_fun", "repeat", "rollout", "spectral", "stepper", "vmap"]

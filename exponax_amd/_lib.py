@@ -1,0 +1,185 @@
+"""ctypes binding of the gfx950 C-ABI library ``_lib/libexmi.so``.
+
+The library is the only compute path: when it is missing, or no HIP device
+is visible, every entry point raises instead of falling back to anything.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libexmi.so")
+
+EXMI_NL_ZERO = 0
+EXMI_NL_CONVECTION = 1
+EXMI_NL_GRADIENT_NORM = 2
+EXMI_NL_POLYNOMIAL = 3
+EXMI_NL_GENERAL = 4
+EXMI_NL_VORTICITY = 5
+EXMI_NL_KOLMOGOROV = 6
+EXMI_MAX_POLY = 8
+
+EXMI_ERR_INVALID = -1
+EXMI_ERR_HIP = -2
+EXMI_ERR_ALLOC = -3
+EXMI_ERR_UNSUPPORTED = -4
+
+# every symbol include/exmi.h declares
+EXPORTED = (
+    "exmi_plan_create", "exmi_plan_destroy", "exmi_step", "exmi_step_fourier", "exmi_nonlinear",
+    "exmi_rollout", "exmi_repeat", "exmi_rfft", "exmi_irfft", "exmi_plan_num_coefs", "exmi_plan_get_tables",
+    "exmi_last_error", "exmi_info",
+)
+
+
+class PlanDesc(C.Structure):
+    """Mirror of ``exmi_plan_desc`` in include/exmi.h."""
+
+    _fields_ = [
+        ("num_spatial_dims", C.c_int32),
+        ("num_points", C.c_int32),
+        ("num_channels", C.c_int32),
+        ("batch", C.c_int32),
+        ("order", C.c_int32),
+        ("nonlin_kind", C.c_int32),
+        ("conv_single_channel", C.c_int32),
+        ("conv_conservative", C.c_int32),
+        ("zero_mode_fix", C.c_int32),
+        ("num_poly", C.c_int32),
+        ("injection_mode", C.c_int32),
+        ("lin_batch", C.c_int32),
+        ("lin_channels", C.c_int32),
+        ("num_circle_points", C.c_int32),
+        ("domain_extent", C.c_double),
+        ("dt", C.c_double),
+        ("circle_radius", C.c_double),
+        ("dealiasing_fraction", C.c_double),
+        ("scale", C.c_double),
+        ("general_scales", C.c_double * 3),
+        ("poly", C.c_double * EXMI_MAX_POLY),
+        ("injection_scale", C.c_double),
+        ("host_linear_operator", C.POINTER(C.c_double)),
+    ]
+
+
+class ExmiError(RuntimeError):
+    pass
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def _declare(lib):
+    vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+    lib.exmi_plan_create.argtypes = [C.POINTER(PlanDesc), C.POINTER(vp)]
+    lib.exmi_plan_destroy.argtypes = [vp]
+    for name in ("exmi_step", "exmi_step_fourier", "exmi_nonlinear"):
+        getattr(lib, name).argtypes = [vp, vp, i64, vp, i64, vp]
+    lib.exmi_rollout.argtypes = [vp, vp, i64, vp, i32, i32, i32, vp]
+    lib.exmi_repeat.argtypes = [vp, vp, i64, vp, i32, i32, vp]
+    lib.exmi_rfft.argtypes = [i32, i32, i32, vp, vp, vp]
+    lib.exmi_irfft.argtypes = [i32, i32, i32, vp, vp, vp]
+    lib.exmi_plan_num_coefs.argtypes = [vp]
+    lib.exmi_plan_get_tables.argtypes = [vp, vp, i64]
+    lib.exmi_last_error.restype = C.c_char_p
+    lib.exmi_info.argtypes = [C.c_char_p, i32]
+    for name in EXPORTED:
+        if name != "exmi_last_error":
+            getattr(lib, name).restype = C.c_int
+
+
+def load_library(path: str = LIB_PATH):
+    """Load (once) and return the ctypes handle. Raises if the .so is absent."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(path):
+                raise ExmiError(
+                    f"exponax_amd native library not built: {path} is missing "
+                    "(run `python -m exponax_amd.build`)")
+            lib = C.CDLL(path)
+            _declare(lib)
+            _lib = lib
+    return _lib
+
+
+def lib():
+    """The library, after checking that a HIP device is present."""
+    L = load_library()
+    _require_device()
+    return L
+
+
+_device_checked = False
+
+
+def _require_device():
+    global _device_checked
+    if _device_checked:
+        return
+    import torch
+
+    if not torch.cuda.is_available():
+        raise ExmiError("exponax_amd needs a HIP device (torch.cuda.is_available() is False); "
+                        "there is no CPU fallback")
+    _device_checked = True
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = _lib.exmi_last_error().decode() if _lib is not None else ""
+        if rc == EXMI_ERR_INVALID:
+            raise ValueError(msg or what)
+        if rc == EXMI_ERR_UNSUPPORTED:
+            raise NotImplementedError(msg or what)
+        raise ExmiError(f"{what}: {msg} (status {rc})")
+
+
+def stream_ptr(stream=None) -> int:
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+class Plan:
+    """Owns one ``exmi_plan``; created from a PlanDesc plus a float64 L table."""
+
+    def __init__(self, desc: PlanDesc, linear_operator: np.ndarray):
+        L = lib()
+        self._L = np.ascontiguousarray(linear_operator, dtype=np.float64)
+        desc.host_linear_operator = self._L.ctypes.data_as(C.POINTER(C.c_double))
+        self.desc = desc
+        h = C.c_void_p()
+        check(L.exmi_plan_create(C.byref(desc), C.byref(h)), "exmi_plan_create")
+        self._h = h
+        self._lib = L
+
+    @property
+    def handle(self):
+        return self._h
+
+    def num_coefs(self) -> int:
+        return int(self._lib.exmi_plan_num_coefs(self._h))
+
+    def tables(self) -> np.ndarray:
+        d = self.desc
+        nr = (d.num_points ** (d.num_spatial_dims - 1)) * (d.num_points // 2 + 1)
+        n = d.lin_batch * d.lin_channels * self.num_coefs() * nr
+        out = np.empty(n, dtype=np.float32)
+        check(self._lib.exmi_plan_get_tables(self._h, out.ctypes.data, n), "exmi_plan_get_tables")
+        return out.reshape(d.lin_batch, d.lin_channels, self.num_coefs(), nr)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._lib.exmi_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None

@@ -1,0 +1,116 @@
+"""Rollout / repeat / vmap with exponax's semantics (``_utils.py:92-254``).
+
+For exponax_amd steppers (single or vmapped) the whole trajectory is one
+captured hipGraph of per-step launches (``exmi_rollout``); any other
+callable is looped on the host.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from ._base_stepper import BaseStepper, Ensemble
+
+
+class vmap:
+    """Batch a stepper over a leading axis: ``vmap(stepper)(u_batch)``
+    advances all states in one launch (the reference's ``jax.vmap``)."""
+
+    def __init__(self, stepper):
+        self.stepper = stepper
+
+    def __call__(self, u):
+        if isinstance(self.stepper, BaseStepper):
+            return self.stepper.step(u)
+        return _stack([self.stepper(x) for x in u])
+
+
+def _stack(xs):
+    import torch
+
+    if isinstance(xs[0], torch.Tensor):
+        return torch.stack(xs)
+    return np.stack(xs)
+
+
+def _native(stepper_fn):
+    if isinstance(stepper_fn, vmap) and isinstance(stepper_fn.stepper, BaseStepper):
+        return stepper_fn.stepper, True
+    if isinstance(stepper_fn, Ensemble):
+        return stepper_fn, True
+    if isinstance(stepper_fn, BaseStepper):
+        return stepper_fn, False
+    return None, False
+
+
+def rollout(stepper_fn, n: int, *, include_init: bool = False, takes_aux: bool = False, constant_aux: bool = True):
+    """``_utils.py:92-186``. Returns a function u0 -> trajectory (n[+1], ...)."""
+    if takes_aux:
+        def run_aux(u0, aux):
+            out = [u0] if include_init else []
+            u = u0
+            for i in range(n):
+                a = aux if constant_aux else _index(aux, i)
+                u = stepper_fn(u, a)
+                out.append(u)
+            return _stack(out)
+        return run_aux
+
+    st, batched = _native(stepper_fn)
+
+    def run(u0):
+        if st is not None:
+            if not batched:
+                shape = tuple(getattr(u0, "shape", ()))
+                if shape != st.state_shape:
+                    raise ValueError(f"Expected shape {st.state_shape}, got {shape}.")
+            return st.rollout_batch(u0, n, include_init=include_init)
+        out = [u0] if include_init else []
+        u = u0
+        for _ in range(n):
+            u = stepper_fn(u)
+            out.append(u)
+        return _stack(out)
+    return run
+
+
+def repeat(stepper_fn, n: int, *, takes_aux: bool = False, constant_aux: bool = True):
+    """``_utils.py:189-254``. Returns a function u0 -> state after n steps."""
+    if takes_aux:
+        def run_aux(u0, aux):
+            u = u0
+            for i in range(n):
+                u = stepper_fn(u, aux if constant_aux else _index(aux, i))
+            return u
+        return run_aux
+
+    st, batched = _native(stepper_fn)
+
+    def run(u0):
+        if st is not None:
+            if not batched:
+                shape = tuple(getattr(u0, "shape", ()))
+                if shape != st.state_shape:
+                    raise ValueError(f"Expected shape {st.state_shape}, got {shape}.")
+            return st.repeat_batch(u0, n)
+        u = u0
+        for _ in range(n):
+            u = stepper_fn(u)
+        return u
+    return run
+
+
+def _index(aux, i):
+    if isinstance(aux, (tuple, list)):
+        return type(aux)(_index(a, i) for a in aux)
+    if isinstance(aux, dict):
+        return {k: _index(v, i) for k, v in aux.items()}
+    return aux[i]
+
+
+def make_grid(num_spatial_dims, domain_extent, num_points, *, full=False, zero_centered=False):
+    """``_utils.py:10-60`` (host NumPy, float64)."""
+    g = (np.linspace(0, domain_extent, num_points + 1, endpoint=True) if full
+         else np.linspace(0, domain_extent, num_points, endpoint=False))
+    if zero_centered:
+        g = g - domain_extent / 2
+    return np.stack(np.meshgrid(*([g] * num_spatial_dims), indexing="ij"))

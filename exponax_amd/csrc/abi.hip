@@ -1,0 +1,504 @@
+// C-ABI layer: plan construction, dispatch to the fused kernels, hipGraph
+// capture of rollouts, error reporting. See include/exmi.h for the contract.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "exmi_internal.hpp"
+#include "fft1d.hpp"
+#include "step1d.hpp"
+
+namespace exmi {
+
+using StepFn = hipError_t (*)(const Step1DArgs&, hipStream_t);
+hipError_t launch_coef(const double* L, long long total, int nr, double dt, int order, int ncirc, double radius,
+                       float* out, hipStream_t st);
+
+#define EXMI_DECL_SIZE(M)                                                                              \
+  StepFn get_step1d_##M(int kind, int order);                                                          \
+  hipError_t launch_fft1d_##M(bool inverse, const float* in, float* out, int count, float inv_n, const cf* tw, \
+                              const cf* twr, hipStream_t st);
+EXMI_DECL_SIZE(8)
+EXMI_DECL_SIZE(16)
+EXMI_DECL_SIZE(32)
+EXMI_DECL_SIZE(64)
+EXMI_DECL_SIZE(128)
+EXMI_DECL_SIZE(256)
+EXMI_DECL_SIZE(512)
+EXMI_DECL_SIZE(1024)
+EXMI_DECL_SIZE(2048)
+
+static StepFn get_step1d(int M, int kind, int order) {
+  switch (M) {
+    case 8: return get_step1d_8(kind, order);
+    case 16: return get_step1d_16(kind, order);
+    case 32: return get_step1d_32(kind, order);
+    case 64: return get_step1d_64(kind, order);
+    case 128: return get_step1d_128(kind, order);
+    case 256: return get_step1d_256(kind, order);
+    case 512: return get_step1d_512(kind, order);
+    case 1024: return get_step1d_1024(kind, order);
+    case 2048: return get_step1d_2048(kind, order);
+    default: return nullptr;
+  }
+}
+
+static hipError_t launch_fft1d(int M, bool inverse, const float* in, float* out, int count, float inv_n,
+                               const cf* tw, const cf* twr, hipStream_t st) {
+  switch (M) {
+#define EXMI_CASE(MM) \
+  case MM: return launch_fft1d_##MM(inverse, in, out, count, inv_n, tw, twr, st);
+    EXMI_CASE(8)
+    EXMI_CASE(16)
+    EXMI_CASE(32)
+    EXMI_CASE(64)
+    EXMI_CASE(128)
+    EXMI_CASE(256)
+    EXMI_CASE(512)
+    EXMI_CASE(1024)
+    EXMI_CASE(2048)
+#undef EXMI_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return fail(EXMI_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------- twiddle table cache
+struct Twiddles {
+  cf* tw = nullptr;   // exp(-2 pi i m / M), m < M
+  cf* twr = nullptr;  // exp(-2 pi i k / (2M)), k <= M
+};
+
+static std::mutex g_tw_mu;
+static std::map<std::pair<int, int>, Twiddles> g_tw;  // (device, M)
+
+static int get_twiddles(int M, Twiddles* out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto key = std::make_pair(dev, M);
+  auto it = g_tw.find(key);
+  if (it != g_tw.end()) {
+    *out = it->second;
+    return EXMI_OK;
+  }
+  std::vector<cf> h(M), hr(M + 1);
+  const double pi = 3.14159265358979323846;
+  for (int m = 0; m < M; ++m) h[m] = cf{(float)std::cos(2 * pi * m / M), (float)-std::sin(2 * pi * m / M)};
+  for (int k = 0; k <= M; ++k)
+    hr[k] = cf{(float)std::cos(pi * k / M), (float)-std::sin(pi * k / M)};
+  Twiddles t;
+  if (hipMalloc(&t.tw, sizeof(cf) * M) != hipSuccess) return fail(EXMI_ERR_ALLOC, "twiddle allocation failed");
+  if (hipMalloc(&t.twr, sizeof(cf) * (M + 1)) != hipSuccess)
+    return fail(EXMI_ERR_ALLOC, "twiddle allocation failed");
+  HIP_TRY(hipMemcpy(t.tw, h.data(), sizeof(cf) * M, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(t.twr, hr.data(), sizeof(cf) * (M + 1), hipMemcpyHostToDevice));
+  g_tw[key] = t;
+  *out = t;
+  return EXMI_OK;
+}
+
+static bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// Dealias cutoff of nonlin_fun/_base.py:59-71: keep |k| <= frac*(N//2) - 1,
+// compared in float32 as the reference's float32 wavenumber grid is.
+static int dealias_kmax(int N, double frac) {
+  if (frac < 0) return 1 << 30;
+  const double cutoff = frac * (double)(N / 2) - 1.0;
+  const float cf32 = (float)cutoff;
+  if (cf32 < 0.f) return -1;
+  return (int)std::floor((double)cf32);
+}
+
+}  // namespace exmi
+
+using namespace exmi;
+
+struct GraphCache {
+  const void* a = nullptr;
+  const void* b = nullptr;
+  long long s = 0;
+  int n = -1, flag = -1, kind = -1;
+  hipGraphExec_t exec = nullptr;
+};
+
+struct exmi_plan {
+  exmi_plan_desc d{};
+  int M = 0, nr = 0, ncoef = 0, kind = 0;
+  int kmax = 0;
+  float kscale = 0.f;
+  int device = 0;
+  float* tables = nullptr;
+  float* work = nullptr;
+  size_t work_elems = 0;
+  Twiddles tw;
+  StepFn step = nullptr;
+  StepFn nonlin = nullptr;
+  hipStream_t capture_stream = nullptr;
+  GraphCache gc;
+};
+
+static void fill_args(const exmi_plan* p, Step1DArgs* a) {
+  memset(a, 0, sizeof *a);
+  a->ntrans = p->d.batch * p->d.num_channels;
+  a->channels = p->d.num_channels;
+  a->tables = p->tables;
+  a->lin_batch = p->d.lin_batch;
+  a->lin_channels = p->d.lin_channels;
+  a->ncoef = p->ncoef;
+  a->tw = p->tw.tw;
+  a->twr = p->tw.twr;
+  a->kscale = p->kscale;
+  a->kmax = p->kmax;
+  a->inv_n = 1.0f / (float)p->d.num_points;
+  a->scale = (float)p->d.scale;
+  a->g0 = (float)p->d.general_scales[0];
+  a->g1 = (float)p->d.general_scales[1];
+  a->g2 = (float)p->d.general_scales[2];
+  a->zero_mode_fix = p->d.zero_mode_fix;
+  a->npoly = p->d.num_poly;
+  for (int i = 0; i < EXMI_MAX_POLY; ++i) a->poly[i] = (float)p->d.poly[i];
+}
+
+static int kernel_kind(const exmi_plan_desc* d, int* kind) {
+  const int D = d->num_spatial_dims, C = d->num_channels;
+  switch (d->nonlin_kind) {
+    case EXMI_NL_ZERO: *kind = NL_ZERO; return EXMI_OK;
+    case EXMI_NL_CONVECTION:
+      if (d->conv_single_channel) {
+        if (C != 1) return fail(EXMI_ERR_UNSUPPORTED, "single-channel convection needs num_channels == 1");
+      } else if (C != D) {
+        // nonlin_fun/_convection.py:125-129
+        return fail(EXMI_ERR_INVALID, "Number of channels in u_hat should match number of spatial dimensions");
+      }
+      *kind = d->conv_conservative ? NL_CONV_CONS : NL_CONV_NONCONS;
+      return EXMI_OK;
+    case EXMI_NL_GRADIENT_NORM: *kind = NL_GRADNORM; return EXMI_OK;
+    case EXMI_NL_POLYNOMIAL:
+      if (d->num_poly < 0 || d->num_poly > EXMI_MAX_POLY)
+        return fail(EXMI_ERR_UNSUPPORTED, "at most %d polynomial coefficients", EXMI_MAX_POLY);
+      *kind = NL_POLY;
+      return EXMI_OK;
+    case EXMI_NL_GENERAL:
+      if (C != 1) return fail(EXMI_ERR_UNSUPPORTED, "general nonlinear function needs num_channels == 1");
+      *kind = NL_GENERAL;
+      return EXMI_OK;
+    case EXMI_NL_VORTICITY:
+    case EXMI_NL_KOLMOGOROV:
+      if (D != 2) return fail(EXMI_ERR_INVALID, "Expected num_spatial_dims = 2, got %d.", D);
+      *kind = NL_VORT;
+      return EXMI_OK;
+    default: return fail(EXMI_ERR_INVALID, "unknown nonlinear kind %d", d->nonlin_kind);
+  }
+}
+
+extern "C" {
+
+const char* exmi_last_error(void) { return g_err.c_str(); }
+
+int exmi_info(char* buf, int32_t buflen) {
+  int dev = 0, n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    snprintf(buf, buflen, "exmi: no HIP device");
+    return EXMI_OK;
+  }
+  hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  hipGetDeviceProperties(&prop, dev);
+  snprintf(buf, buflen, "exmi: device %d %s (%s), %d CUs", dev, prop.name, prop.gcnArchName,
+           prop.multiProcessorCount);
+  return EXMI_OK;
+}
+
+int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
+  g_err.clear();
+  if (!d || !out) return fail(EXMI_ERR_INVALID, "null argument");
+  *out = nullptr;
+  const int D = d->num_spatial_dims, N = d->num_points;
+  if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "num_spatial_dims=%d: this build supports 1", D);
+  if (!pow2(N) || N < 16 || N > 4096)
+    return fail(EXMI_ERR_UNSUPPORTED, "num_points=%d: supported sizes are powers of two 16..4096", N);
+  if (d->num_channels < 1 || d->batch < 1) return fail(EXMI_ERR_INVALID, "num_channels and batch must be >= 1");
+  if (d->order < 0 || d->order > 4) return fail(EXMI_ERR_UNSUPPORTED, "Order %d not implemented.", d->order);
+  if (d->lin_batch != 1 && d->lin_batch != d->batch)
+    return fail(EXMI_ERR_INVALID, "lin_batch must be 1 or batch (%d), got %d", d->batch, d->lin_batch);
+  if (d->lin_channels != 1 && d->lin_channels != d->num_channels)
+    return fail(EXMI_ERR_INVALID, "lin_channels must be 1 or num_channels (%d), got %d", d->num_channels,
+                d->lin_channels);
+  if (!d->host_linear_operator) return fail(EXMI_ERR_INVALID, "host_linear_operator is null");
+  if (d->order > 0 && d->num_circle_points < 1) return fail(EXMI_ERR_INVALID, "num_circle_points must be >= 1");
+  int kind = NL_ZERO;
+  if (d->order > 0) {
+    int rc = kernel_kind(d, &kind);
+    if (rc) return rc;
+  }
+  exmi_plan* p = new exmi_plan();
+  p->d = *d;
+  p->d.host_linear_operator = nullptr;
+  p->M = N / 2;
+  p->nr = p->M + 1;
+  p->ncoef = num_coefs(d->order);
+  p->kind = kind;
+  p->kmax = dealias_kmax(N, d->dealiasing_fraction);
+  p->kscale = (float)(2.0 * 3.14159265358979323846 / d->domain_extent);
+  hipGetDevice(&p->device);
+  p->step = get_step1d(p->M, kind, d->order);
+  p->nonlin = d->order > 0 ? get_step1d(p->M, kind, 1) : get_step1d(p->M, NL_ZERO, 1);
+  if (!p->step || !p->nonlin) {
+    delete p;
+    return fail(EXMI_ERR_UNSUPPORTED, "no kernel for N=%d kind=%d order=%d", N, kind, d->order);
+  }
+  int rc = get_twiddles(p->M, &p->tw);
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  const long long rows = (long long)d->lin_batch * d->lin_channels;
+  const long long total = rows * p->nr;
+  double* dL = nullptr;
+  if (hipMalloc(&dL, sizeof(double) * total) != hipSuccess ||
+      hipMalloc(&p->tables, sizeof(float) * total * p->ncoef) != hipSuccess) {
+    hipFree(dL);
+    exmi_plan_destroy(p);
+    return fail(EXMI_ERR_ALLOC, "table allocation failed (%lld modes)", total);
+  }
+  hipError_t e = hipMemcpy(dL, d->host_linear_operator, sizeof(double) * total, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_coef(dL, total, p->nr, d->dt, d->order, d->num_circle_points, d->circle_radius, p->tables, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  hipFree(dL);
+  if (e != hipSuccess) {
+    exmi_plan_destroy(p);
+    return fail(EXMI_ERR_HIP, "coefficient formation failed: %s", hipGetErrorString(e));
+  }
+  if (hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+    exmi_plan_destroy(p);
+    return fail(EXMI_ERR_HIP, "stream creation failed");
+  }
+  *out = p;
+  return EXMI_OK;
+}
+
+int exmi_plan_destroy(exmi_plan* p) {
+  if (!p) return EXMI_OK;
+  if (p->gc.exec) hipGraphExecDestroy(p->gc.exec);
+  if (p->capture_stream) hipStreamDestroy(p->capture_stream);
+  hipFree(p->tables);
+  hipFree(p->work);
+  delete p;
+  return EXMI_OK;
+}
+
+int exmi_plan_num_coefs(const exmi_plan* p) { return p ? p->ncoef : fail(EXMI_ERR_INVALID, "null plan"); }
+
+int exmi_plan_get_tables(const exmi_plan* p, float* host_out, int64_t host_count) {
+  if (!p || !host_out) return fail(EXMI_ERR_INVALID, "null argument");
+  const long long n = (long long)p->d.lin_batch * p->d.lin_channels * p->ncoef * p->nr;
+  if (host_count < n) return fail(EXMI_ERR_INVALID, "host buffer too small: need %lld floats", n);
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(host_out, p->tables, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return EXMI_OK;
+}
+
+static int run_step(exmi_plan* p, const void* in, long long in_bs, int in_f, void* out, long long out_bs, int out_f,
+                    int nonlin_only, hipStream_t st) {
+  Step1DArgs a;
+  fill_args(p, &a);
+  a.in = in;
+  a.out = out;
+  a.in_bstride = in_bs;
+  a.out_bstride = out_bs;
+  a.in_fourier = in_f;
+  a.out_fourier = out_f;
+  a.nonlin_only = nonlin_only;
+  hipError_t e = nonlin_only ? p->nonlin(a, st) : p->step(a, st);
+  if (e != hipSuccess) return fail(EXMI_ERR_HIP, "step launch failed: %s", hipGetErrorString(e));
+  return EXMI_OK;
+}
+
+int exmi_step(exmi_plan* p, const float* u_in, int64_t in_bs, float* u_out, int64_t out_bs, void* stream) {
+  if (!p || !u_in || !u_out) return fail(EXMI_ERR_INVALID, "null argument");
+  return run_step(p, u_in, in_bs, 0, u_out, out_bs, 0, 0, (hipStream_t)stream);
+}
+
+int exmi_step_fourier(exmi_plan* p, const float* uh_in, int64_t in_bs, float* uh_out, int64_t out_bs,
+                      void* stream) {
+  if (!p || !uh_in || !uh_out) return fail(EXMI_ERR_INVALID, "null argument");
+  return run_step(p, uh_in, in_bs, 1, uh_out, out_bs, 1, 0, (hipStream_t)stream);
+}
+
+int exmi_nonlinear(exmi_plan* p, const float* uh_in, int64_t in_bs, float* nh_out, int64_t out_bs, void* stream) {
+  if (!p || !uh_in || !nh_out) return fail(EXMI_ERR_INVALID, "null argument");
+  if (p->d.order == 0) return fail(EXMI_ERR_INVALID, "order-0 plan has no nonlinear function");
+  return run_step(p, uh_in, in_bs, 1, nh_out, out_bs, 1, 1, (hipStream_t)stream);
+}
+
+// strided batch copy: dst[b*ds + i] = src[b*ss + i], i < n
+__global__ void exmi_copy_kernel(const float* __restrict__ src, long long ss, float* __restrict__ dst, long long ds,
+                                 long long n, int B) {
+  for (int b = blockIdx.y; b < B; b += gridDim.y)
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+      dst[(long long)b * ds + i] = src[(long long)b * ss + i];
+}
+
+static hipError_t copy_batch(const float* src, long long ss, float* dst, long long ds, long long n, int B,
+                             hipStream_t st) {
+  int gx = (int)((n + 255) / 256);
+  if (gx > 64) gx = 64;
+  const int gy = B < 65535 ? B : 65535;
+  hipLaunchKernelGGL(exmi_copy_kernel, dim3(gx, gy), dim3(256), 0, st, src, ss, dst, ds, n, B);
+  return hipGetLastError();
+}
+
+static int enqueue_rollout(exmi_plan* p, const float* u0, long long u0_bs, float* traj, int n, int include_init,
+                           hipStream_t st) {
+  const long long sz = (long long)p->d.num_channels * p->d.num_points;  // 1-D state size
+  const long long tstride = (long long)(n + (include_init ? 1 : 0)) * sz;
+  int off = 0;
+  if (include_init) {
+    hipError_t e = copy_batch(u0, u0_bs, traj, tstride, sz, p->d.batch, st);
+    if (e != hipSuccess) return fail(EXMI_ERR_HIP, "copy launch failed: %s", hipGetErrorString(e));
+    off = 1;
+  }
+  for (int t = 0; t < n; ++t) {
+    const float* in = (t == 0 && !include_init) ? u0 : traj + (long long)(t - 1 + off) * sz;
+    const long long ibs = (t == 0 && !include_init) ? u0_bs : tstride;
+    float* out = traj + (long long)(t + off) * sz;
+    int rc = run_step(p, in, ibs, 0, out, tstride, 0, 0, st);
+    if (rc) return rc;
+  }
+  return EXMI_OK;
+}
+
+static int run_maybe_graph(exmi_plan* p, const void* a, const void* b, long long s, int n, int flag, int kind,
+                           int use_graph, hipStream_t st, int (*enq)(exmi_plan*, const void*, const void*, long long,
+                                                                    int, int, hipStream_t)) {
+  if (!use_graph) return enq(p, a, b, s, n, flag, st);
+  GraphCache& g = p->gc;
+  if (!(g.exec && g.a == a && g.b == b && g.s == s && g.n == n && g.flag == flag && g.kind == kind)) {
+    if (g.exec) {
+      hipGraphExecDestroy(g.exec);
+      g.exec = nullptr;
+    }
+    HIP_TRY(hipStreamBeginCapture(p->capture_stream, hipStreamCaptureModeThreadLocal));
+    int rc = enq(p, a, b, s, n, flag, p->capture_stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(p->capture_stream, &graph);
+    if (rc) {
+      if (graph) hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(EXMI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      g.exec = nullptr;
+      return fail(EXMI_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+    }
+    g.a = a;
+    g.b = b;
+    g.s = s;
+    g.n = n;
+    g.flag = flag;
+    g.kind = kind;
+  }
+  HIP_TRY(hipGraphLaunch(g.exec, st));
+  return EXMI_OK;
+}
+
+static int enq_rollout_cb(exmi_plan* p, const void* a, const void* b, long long s, int n, int flag, hipStream_t st) {
+  return enqueue_rollout(p, (const float*)a, s, (float*)b, n, flag, st);
+}
+
+static int enq_repeat_cb(exmi_plan* p, const void* a, const void* b, long long s, int n, int, hipStream_t st) {
+  const float* u0 = (const float*)a;
+  float* out = (float*)b;
+  const long long sz = (long long)p->d.num_channels * p->d.num_points;
+  if (n == 0) {
+    hipError_t e = copy_batch(u0, s, out, sz, sz, p->d.batch, st);
+    return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "copy failed: %s", hipGetErrorString(e));
+  }
+  for (int t = 0; t < n; ++t) {
+    const float* in = t == 0 ? u0 : (((n - t) % 2 == 0) ? out : p->work);
+    const long long ibs = t == 0 ? s : sz;
+    float* o = ((n - 1 - t) % 2 == 0) ? out : p->work;
+    int rc = run_step(p, in, ibs, 0, o, sz, 0, 0, st);
+    if (rc) return rc;
+  }
+  return EXMI_OK;
+}
+
+int exmi_rollout(exmi_plan* p, const float* u0, int64_t u0_bs, float* traj, int32_t n, int32_t include_init,
+                 int32_t use_graph, void* stream) {
+  if (!p || !u0 || !traj) return fail(EXMI_ERR_INVALID, "null argument");
+  if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
+  return run_maybe_graph(p, u0, traj, u0_bs, n, include_init ? 1 : 0, 1, use_graph, (hipStream_t)stream,
+                         enq_rollout_cb);
+}
+
+int exmi_repeat(exmi_plan* p, const float* u0, int64_t u0_bs, float* u_out, int32_t n, int32_t use_graph,
+                void* stream) {
+  if (!p || !u0 || !u_out) return fail(EXMI_ERR_INVALID, "null argument");
+  if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
+  const size_t need = (size_t)p->d.batch * p->d.num_channels * p->d.num_points;
+  if (n > 1 && p->work_elems < need) {
+    hipFree(p->work);
+    p->work = nullptr;
+    p->work_elems = 0;
+    if (hipMalloc(&p->work, need * sizeof(float)) != hipSuccess)
+      return fail(EXMI_ERR_ALLOC, "workspace allocation failed");
+    p->work_elems = need;
+  }
+  return run_maybe_graph(p, u0, u_out, u0_bs, n, 0, 2, use_graph, (hipStream_t)stream, enq_repeat_cb);
+}
+
+int exmi_rfft(int32_t D, int32_t N, int32_t count, const float* x, float* x_hat, void* stream) {
+  if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "exmi_rfft: num_spatial_dims=%d not supported", D);
+  if (!pow2(N) || N < 16 || N > 4096) return fail(EXMI_ERR_UNSUPPORTED, "exmi_rfft: num_points=%d", N);
+  if (count <= 0) return EXMI_OK;
+  Twiddles tw;
+  int rc = get_twiddles(N / 2, &tw);
+  if (rc) return rc;
+  hipError_t e = launch_fft1d(N / 2, false, x, x_hat, count, 1.0f / N, tw.tw, tw.twr, (hipStream_t)stream);
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "rfft launch failed: %s", hipGetErrorString(e));
+}
+
+int exmi_irfft(int32_t D, int32_t N, int32_t count, const float* x_hat, float* x, void* stream) {
+  if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "exmi_irfft: num_spatial_dims=%d not supported", D);
+  if (!pow2(N) || N < 16 || N > 4096) return fail(EXMI_ERR_UNSUPPORTED, "exmi_irfft: num_points=%d", N);
+  if (count <= 0) return EXMI_OK;
+  Twiddles tw;
+  int rc = get_twiddles(N / 2, &tw);
+  if (rc) return rc;
+  hipError_t e = launch_fft1d(N / 2, true, x_hat, x, count, 1.0f / N, tw.tw, tw.twr, (hipStream_t)stream);
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "irfft launch failed: %s", hipGetErrorString(e));
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+/*
+ * exmi.h — C ABI of the MI355X (gfx950) ETDRK pseudo-spectral stepper.
+ *
+ * Drop-in boundary for exponax 0.2.0's hot path: every entry point below
+ * replaces one reference interface (cited as file:line of exponax/). All
+ * pointers are device pointers unless named host_*; all calls are
+ * asynchronous on the given HIP stream (passed as void*; NULL = the default
+ * stream). Every function returns 0 (EXMI_OK) or a negative status, and
+ * exmi_last_error() returns a thread-local message for the last failure.
+ * No C++ exception crosses this boundary.
+ *
+ * State layout (reference _base_stepper.py:241-271, vmapped over a leading
+ * batch axis): float32 (B, C, N[, N]) row-major; spectra are complex64
+ * (B, C, [N,] N/2+1) with interleaved (re, im) floats, exactly
+ * jnp.fft.rfftn's layout (_spectral.py:614-656).
+ *
+ * Threading: a plan is not re-entrant across streams; calls that share a
+ * plan must be ordered on one stream.
+ */
+#ifndef EXMI_H
+#define EXMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum exmi_status {
+  EXMI_OK = 0,
+  EXMI_ERR_INVALID = -1,     /* bad argument / unsupported configuration   */
+  EXMI_ERR_HIP = -2,         /* a HIP runtime call failed                  */
+  EXMI_ERR_ALLOC = -3,       /* device allocation failed                   */
+  EXMI_ERR_UNSUPPORTED = -4  /* valid for the reference, not built here    */
+};
+
+/* Nonlinear term kinds (reference nonlin_fun/__init__.py:34-62). */
+enum exmi_nonlin_kind {
+  EXMI_NL_ZERO = 0,        /* ZeroNonlinearFun        nonlin_fun/_zero.py:7            */
+  EXMI_NL_CONVECTION = 1,  /* ConvectionNonlinearFun  nonlin_fun/_convection.py:7      */
+  EXMI_NL_GRADIENT_NORM = 2, /* GradientNormNonlinearFun nonlin_fun/_gradient_norm.py:8 */
+  EXMI_NL_POLYNOMIAL = 3,  /* PolynomialNonlinearFun  nonlin_fun/_polynomial.py:6      */
+  EXMI_NL_GENERAL = 4,     /* GeneralNonlinearFun     nonlin_fun/_general_nonlinear.py:9 */
+  EXMI_NL_VORTICITY = 5,   /* VorticityConvection2d   nonlin_fun/_vorticity_convection.py:8 */
+  EXMI_NL_KOLMOGOROV = 6   /* VorticityConvection2dKolmogorov  ..._vorticity_convection.py:102 */
+};
+
+#define EXMI_MAX_POLY 8
+
+/*
+ * Stepper description: the constructor arguments of BaseStepper
+ * (_base_stepper.py:27-157) plus the nonlinear function's own arguments and
+ * the batch. The linear operator is given as a host float64 table of the
+ * real-valued diagonal L (every config stepper's L is real: even powers of
+ * i*kappa plus a constant) of shape (lin_batch, lin_channels, Nr),
+ * Nr = N^(D-1) * (N/2+1). lin_batch is 1 (shared) or `batch`
+ * (per-trajectory coefficients, the eqx.filter_vmap case).
+ */
+typedef struct exmi_plan_desc {
+  int32_t num_spatial_dims;   /* D: 1 or 2                                   */
+  int32_t num_points;         /* N (power of two, 16..4096 in 1-D)           */
+  int32_t num_channels;       /* C                                           */
+  int32_t batch;              /* B                                           */
+  int32_t order;              /* ETDRK order 0..4                            */
+  int32_t nonlin_kind;        /* enum exmi_nonlin_kind                       */
+  int32_t conv_single_channel;
+  int32_t conv_conservative;
+  int32_t zero_mode_fix;
+  int32_t num_poly;           /* PolynomialNonlinearFun coefficient count    */
+  int32_t injection_mode;     /* Kolmogorov forcing wavenumber k_f           */
+  int32_t lin_batch;          /* 1 or batch                                  */
+  int32_t lin_channels;       /* 1 or num_channels                           */
+  int32_t num_circle_points;  /* contour points (reference default 16)       */
+  double domain_extent;       /* L                                           */
+  double dt;
+  double circle_radius;       /* contour radius (reference default 1.0)      */
+  double dealiasing_fraction; /* < 0: no dealiasing (reference None)         */
+  double scale;               /* convection / gradient-norm / vorticity b    */
+  double general_scales[3];   /* GeneralNonlinearFun scale_list              */
+  double poly[EXMI_MAX_POLY]; /* PolynomialNonlinearFun coefficients         */
+  double injection_scale;     /* Kolmogorov forcing amplitude                */
+  const double* host_linear_operator; /* (lin_batch, lin_channels, Nr) f64  */
+} exmi_plan_desc;
+
+typedef struct exmi_plan exmi_plan;
+
+/* Build a plan: validates the description, uploads L and forms exp(L dt)
+ * and the ETDRK phi-coefficients by the contour-integral mean in float64 on
+ * the device, stored as float32 tables.
+ * Replaces BaseStepper.__init__ (_base_stepper.py:27-157), BaseETDRK.__init__
+ * (etdrk/_base_etdrk.py:18-62) and ETDRK{1,2,3,4}.__init__ (etdrk/_etdrk_1.py:13-76,
+ * _etdrk_2.py:13-89, _etdrk_3.py:13-189, _etdrk_4.py:13-196). Synchronous. */
+int exmi_plan_create(const exmi_plan_desc* desc, exmi_plan** out);
+
+/* Release a plan and its device tables, workspace and cached graphs.
+ * (The reference's pytrees are garbage collected; no counterpart.) */
+int exmi_plan_destroy(exmi_plan* plan);
+
+/* One step on B physical states: u_out = step(u_in), (B, C, N^D) float32.
+ * in/out trajectory strides are in floats (C*N^D for contiguous batches);
+ * u_in may equal u_out.
+ * Replaces BaseStepper.__call__/step (_base_stepper.py:201-220, 241-271)
+ * under jax.vmap. */
+int exmi_step(exmi_plan* plan, const float* u_in, int64_t in_bstride, float* u_out,
+              int64_t out_bstride, void* stream);
+
+/* One step in Fourier space on (B, C, Nr) complex64 spectra (float pairs);
+ * strides in complex elements.
+ * Replaces BaseStepper.step_fourier (_base_stepper.py:222-239) /
+ * BaseETDRK.step_fourier (etdrk/_base_etdrk.py:64-80). */
+int exmi_step_fourier(exmi_plan* plan, const float* uh_in, int64_t in_bstride, float* uh_out,
+                      int64_t out_bstride, void* stream);
+
+/* Dealiased nonlinear term N(u_hat) on (B, C, Nr) complex64 spectra.
+ * Replaces BaseNonlinearFun.__call__ (nonlin_fun/_base.py:139-159) of the
+ * plan's nonlinear function. */
+int exmi_nonlinear(exmi_plan* plan, const float* uh_in, int64_t in_bstride, float* nh_out,
+                   int64_t out_bstride, void* stream);
+
+/* Trajectory: traj[b, t] for t = 0..n_steps-1 (+1 with include_init),
+ * traj float32 (B, n_steps[+include_init], C, N^D) contiguous; u0 (B, C, N^D)
+ * with stride u0_bstride floats. With use_graph != 0 the n_steps launches are
+ * captured once in a hipGraph (cached per plan for the same pointers and n)
+ * and replayed on `stream`.
+ * Replaces exponax.rollout (_utils.py:92-186, takes_aux=False). */
+int exmi_rollout(exmi_plan* plan, const float* u0, int64_t u0_bstride, float* traj, int32_t n_steps,
+                 int32_t include_init, int32_t use_graph, void* stream);
+
+/* Final state after n_steps: u_out (B, C, N^D) contiguous.
+ * Replaces exponax.repeat (_utils.py:189-254, takes_aux=False). */
+int exmi_repeat(exmi_plan* plan, const float* u0, int64_t u0_bstride, float* u_out, int32_t n_steps,
+                int32_t use_graph, void* stream);
+
+/* Batched real FFTs with the reference conventions: forward rfftn over the
+ * last D axes, norm="backward"; inverse irfftn with 1/N^D scaling, Im of the
+ * last axis' DC/Nyquist ignored. `count` transforms, contiguous.
+ * Replace exponax.fft / exponax.ifft (_spectral.py:614-656, 659-721). */
+int exmi_rfft(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x, float* x_hat,
+              void* stream);
+int exmi_irfft(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x_hat, float* x,
+               void* stream);
+
+/* Copy the plan's float32 operator tables to host memory:
+ * (lin_batch, lin_channels, ncoef, Nr) with ncoef = exmi_plan_num_coefs().
+ * Coefficient order: ETDRK0 [E]; ETDRK1 [E, c1]; ETDRK2 [E, c1, c2];
+ * ETDRK3 [E, E_half, c1..c5]; ETDRK4 [E, E_half, c1, c4, c5, c6].
+ * Exposes BaseETDRK._exp_term / ETDRK*._coef_* (etdrk/_etdrk_2.py:88-89 etc.)
+ * for inspection. Synchronous. */
+int exmi_plan_num_coefs(const exmi_plan* plan);
+int exmi_plan_get_tables(const exmi_plan* plan, float* host_out, int64_t host_count);
+
+/* Message for the last error on this thread ("" if none). */
+const char* exmi_last_error(void);
+
+/* Library / device information: writes a short description into buf. */
+int exmi_info(char* buf, int32_t buflen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EXMI_H */

@@ -1,0 +1,281 @@
+"""CPU restatement of exponax's stepper layer (test infrastructure only).
+
+``BaseStepper`` follows ``exponax/_base_stepper.py:17-271``; the concrete
+steppers follow ``exponax/stepper/_kuramoto_sivashinsky.py``,
+``_burgers.py`` and ``_navier_stokes.py:13-327``; ``rollout``/``repeat``
+follow ``exponax/_utils.py:92-254``; ``nrmse`` follows
+``exponax/metrics/_spatial.py:537-586``.
+
+Batching: a stepper maps a single state (C, N, ..., N). ``step_batch``
+applies it to (B, C, N, ..., N) by running the same vectorised NumPy code
+with a leading batch axis — identical arithmetic to a vmap.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import etdrk as et
+from . import nonlin as nl
+from . import spectral as sp
+
+
+class BaseStepper:
+    """``_base_stepper.py:17-157``."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, num_channels, order,
+                 num_circle_points=16, circle_radius=1.0, dtype=np.float64, coef_dtype=np.float64):
+        self.num_spatial_dims = num_spatial_dims
+        self.domain_extent = domain_extent
+        self.num_points = num_points
+        self.dt = dt
+        self.num_channels = num_channels
+        self.dx = domain_extent / num_points
+        self.dtype = np.dtype(dtype)
+        deriv = sp.build_derivative_operator(num_spatial_dims, domain_extent, num_points)
+        lin = self._build_linear_operator(deriv)
+        single = (1,) + sp.wavenumber_shape(num_spatial_dims, num_points)
+        multi = (num_channels,) + sp.wavenumber_shape(num_spatial_dims, num_points)
+        if lin.shape not in (single, multi):
+            raise ValueError(f"Expected linear operator to have shape {single} or {multi}, got {lin.shape}.")
+        self.linear_operator = lin
+        nonlin = self._build_nonlinear_fun(deriv)
+        kw = dict(dtype=dtype, coef_dtype=coef_dtype)
+        ckw = dict(num_circle_points=num_circle_points, circle_radius=circle_radius)
+        if order == 0:
+            self._integrator = et.ETDRK0(dt, lin, **kw)
+        elif order == 1:
+            self._integrator = et.ETDRK1(dt, lin, nonlin, **ckw, **kw)
+        elif order == 2:
+            self._integrator = et.ETDRK2(dt, lin, nonlin, **ckw, **kw)
+        elif order == 3:
+            self._integrator = et.ETDRK3(dt, lin, nonlin, **ckw, **kw)
+        elif order == 4:
+            self._integrator = et.ETDRK4(dt, lin, nonlin, **ckw, **kw)
+        else:
+            raise NotImplementedError(f"Order {order} not implemented.")
+        self.nonlinear_fun = nonlin
+
+    def step(self, u):
+        """``_base_stepper.py:201-220``: fft -> step_fourier -> ifft."""
+        u = np.asarray(u, dtype=self.dtype)
+        u_hat = sp.fft(u, num_spatial_dims=self.num_spatial_dims).astype(sp.complex_of(self.dtype))
+        nxt = self.step_fourier(u_hat)
+        out = sp.ifft(nxt, num_spatial_dims=self.num_spatial_dims, num_points=self.num_points)
+        return out.astype(self.dtype, copy=False)
+
+    def step_fourier(self, u_hat):
+        return self._integrator.step_fourier(u_hat)
+
+    def __call__(self, u):
+        """``_base_stepper.py:241-271``."""
+        expected = (self.num_channels,) + sp.spatial_shape(self.num_spatial_dims, self.num_points)
+        if u.shape != expected:
+            raise ValueError(f"Expected shape {expected}, got {u.shape}. For batched operation use "
+                             "`jax.vmap` on this function.")
+        return self.step(u)
+
+    def step_batch(self, ub):
+        """Vectorised step over a leading batch axis (the vmap of ``__call__``).
+
+        Every operator broadcasts over a leading axis because the reference's
+        operators carry a leading channel axis of size 1 or C; a batch axis in
+        front of that broadcasts identically.
+        """
+        return self.step(ub)
+
+
+class KuramotoSivashinskyConservative(BaseStepper):
+    """``stepper/_kuramoto_sivashinsky.py:171-315``."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, convection_scale=1.0,
+                 second_order_scale=1.0, fourth_order_scale=1.0, single_channel=False, conservative=True,
+                 dealiasing_fraction=2 / 3, order=2, num_circle_points=16, circle_radius=1.0, **kw):
+        self.convection_scale = convection_scale
+        self.second_order_scale = second_order_scale
+        self.fourth_order_scale = fourth_order_scale
+        self.single_channel = single_channel
+        self.conservative = conservative
+        self.dealiasing_fraction = dealiasing_fraction
+        self._kw_dtype = kw.get("dtype", np.float64)
+        nc = 1 if single_channel else num_spatial_dims
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=nc, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius, **kw)
+
+    def _build_linear_operator(self, d):
+        return (-self.second_order_scale * sp.build_laplace_operator(d, order=2)
+                - self.fourth_order_scale * sp.build_laplace_operator(d, order=4))
+
+    def _build_nonlinear_fun(self, d):
+        return nl.ConvectionNonlinearFun(self.num_spatial_dims, self.num_points, derivative_operator=d,
+                                         dealiasing_fraction=self.dealiasing_fraction,
+                                         scale=self.convection_scale, single_channel=self.single_channel,
+                                         conservative=self.conservative, dtype=self._kw_dtype)
+
+
+class KuramotoSivashinsky(BaseStepper):
+    """``stepper/_kuramoto_sivashinsky.py:8-168`` (combustion / gradient-norm form)."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, gradient_norm_scale=1.0,
+                 second_order_scale=1.0, fourth_order_scale=1.0, dealiasing_fraction=2 / 3, order=2,
+                 num_circle_points=16, circle_radius=1.0, **kw):
+        self.gradient_norm_scale = gradient_norm_scale
+        self.second_order_scale = second_order_scale
+        self.fourth_order_scale = fourth_order_scale
+        self.dealiasing_fraction = dealiasing_fraction
+        self._kw_dtype = kw.get("dtype", np.float64)
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=1, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius, **kw)
+
+    def _build_linear_operator(self, d):
+        return (-self.second_order_scale * sp.build_laplace_operator(d, order=2)
+                - self.fourth_order_scale * sp.build_laplace_operator(d, order=4))
+
+    def _build_nonlinear_fun(self, d):
+        return nl.GradientNormNonlinearFun(self.num_spatial_dims, self.num_points, derivative_operator=d,
+                                           dealiasing_fraction=self.dealiasing_fraction, zero_mode_fix=True,
+                                           scale=self.gradient_norm_scale, dtype=self._kw_dtype)
+
+
+class Burgers(BaseStepper):
+    """``stepper/_burgers.py:8-155``."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity=0.1,
+                 convection_scale=1.0, single_channel=False, conservative=False, order=2,
+                 dealiasing_fraction=2 / 3, num_circle_points=16, circle_radius=1.0, **kw):
+        self.diffusivity = diffusivity
+        self.convection_scale = convection_scale
+        self.single_channel = single_channel
+        self.conservative = conservative
+        self.dealiasing_fraction = dealiasing_fraction
+        self._kw_dtype = kw.get("dtype", np.float64)
+        nc = 1 if single_channel else num_spatial_dims
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=nc, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius, **kw)
+
+    def _build_linear_operator(self, d):
+        return self.diffusivity * sp.build_laplace_operator(d)
+
+    def _build_nonlinear_fun(self, d):
+        return nl.ConvectionNonlinearFun(self.num_spatial_dims, self.num_points, derivative_operator=d,
+                                         dealiasing_fraction=self.dealiasing_fraction,
+                                         scale=self.convection_scale, single_channel=self.single_channel,
+                                         conservative=self.conservative, dtype=self._kw_dtype)
+
+
+class NavierStokesVorticity(BaseStepper):
+    """``stepper/_navier_stokes.py:13-153``."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity=0.01,
+                 vorticity_convection_scale=1.0, drag=0.0, order=2, dealiasing_fraction=2 / 3,
+                 num_circle_points=16, circle_radius=1.0, **kw):
+        if num_spatial_dims != 2:
+            raise ValueError(f"Expected num_spatial_dims = 2, got {num_spatial_dims}.")
+        self.diffusivity = diffusivity
+        self.vorticity_convection_scale = vorticity_convection_scale
+        self.drag = drag
+        self.dealiasing_fraction = dealiasing_fraction
+        self._kw_dtype = kw.get("dtype", np.float64)
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=1, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius, **kw)
+
+    def _build_linear_operator(self, d):
+        return (self.diffusivity * sp.build_laplace_operator(d, order=2)
+                + self.drag * sp.build_laplace_operator(d, order=0))
+
+    def _build_nonlinear_fun(self, d):
+        return nl.VorticityConvection2d(self.num_spatial_dims, self.num_points,
+                                        convection_scale=self.vorticity_convection_scale, derivative_operator=d,
+                                        dealiasing_fraction=self.dealiasing_fraction, dtype=self._kw_dtype)
+
+
+class KolmogorovFlowVorticity(BaseStepper):
+    """``stepper/_navier_stokes.py:156-327``."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity=0.001,
+                 convection_scale=1.0, drag=-0.1, injection_mode=4, injection_scale=1.0, order=2,
+                 dealiasing_fraction=2 / 3, num_circle_points=16, circle_radius=1.0, **kw):
+        if num_spatial_dims != 2:
+            raise ValueError(f"Expected num_spatial_dims = 2, got {num_spatial_dims}.")
+        self.diffusivity = diffusivity
+        self.convection_scale = convection_scale
+        self.drag = drag
+        self.injection_mode = injection_mode
+        self.injection_scale = injection_scale
+        self.dealiasing_fraction = dealiasing_fraction
+        self._kw_dtype = kw.get("dtype", np.float64)
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=1, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius, **kw)
+
+    def _build_linear_operator(self, d):
+        return (self.diffusivity * sp.build_laplace_operator(d, order=2)
+                + self.drag * sp.build_laplace_operator(d, order=0))
+
+    def _build_nonlinear_fun(self, d):
+        return nl.VorticityConvection2dKolmogorov(
+            self.num_spatial_dims, self.num_points, convection_scale=self.convection_scale,
+            injection_mode=self.injection_mode, injection_scale=self.injection_scale, derivative_operator=d,
+            dealiasing_fraction=self.dealiasing_fraction, dtype=self._kw_dtype)
+
+
+class GeneralStepper(BaseStepper):
+    """Linear part sum_j a_j sum_d (i kappa_d)^j (even j here) + ``GeneralNonlinearFun``.
+
+    Mirrors the stepper families of ``stepper/generic/_convection.py`` /
+    ``_nonlinear.py`` restricted to even-order (real) linear terms.
+    """
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, linear_coefficients=(0.0, 0.0, 0.01),
+                 nonlinear_scales=(0.0, -1.0, 0.0), dealiasing_fraction=2 / 3, order=2, num_circle_points=16,
+                 circle_radius=1.0, **kw):
+        self.linear_coefficients = tuple(linear_coefficients)
+        self.nonlinear_scales = tuple(nonlinear_scales)
+        self.dealiasing_fraction = dealiasing_fraction
+        self._kw_dtype = kw.get("dtype", np.float64)
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=1, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius, **kw)
+
+    def _build_linear_operator(self, d):
+        out = np.zeros((1,) + d.shape[1:], dtype=np.complex128)
+        for j, a in enumerate(self.linear_coefficients):
+            out = out + a * sp.build_laplace_operator(d, order=j) if j % 2 == 0 else out + a * np.sum(
+                d**j, axis=0, keepdims=True)
+        return out
+
+    def _build_nonlinear_fun(self, d):
+        return nl.GeneralNonlinearFun(self.num_spatial_dims, self.num_points, derivative_operator=d,
+                                      dealiasing_fraction=self.dealiasing_fraction,
+                                      scale_list=self.nonlinear_scales, dtype=self._kw_dtype)
+
+
+def rollout(stepper_fn, n, *, include_init=False):
+    """``_utils.py:92-186`` (takes_aux=False): stack of n successive states."""
+    def run(u0):
+        out = [u0] if include_init else []
+        u = u0
+        for _ in range(n):
+            u = stepper_fn(u)
+            out.append(u)
+        return np.stack(out)
+    return run
+
+
+def repeat(stepper_fn, n):
+    """``_utils.py:189-254`` (takes_aux=False): final state after n steps."""
+    def run(u0):
+        u = u0
+        for _ in range(n):
+            u = stepper_fn(u)
+        return u
+    return run
+
+
+def nrmse(u_pred, u_ref):
+    """``metrics/_spatial.py:537-586`` with ``spatial_norm(mode="normalized")``:
+    sum over channels of ||u - u_ref||_2 / ||u_ref||_2 (the (L/N)^D factor
+    cancels). Computed in float64. Works on (C, ...) states."""
+    a = np.asarray(u_pred, dtype=np.float64)
+    b = np.asarray(u_ref, dtype=np.float64)
+    c = a.shape[0]
+    num = np.sqrt(np.sum((a - b).reshape(c, -1) ** 2, axis=1))
+    den = np.sqrt(np.sum(b.reshape(c, -1) ** 2, axis=1))
+    return float(np.sum(num / den))

@@ -1,0 +1,105 @@
+"""GPU parity of the fused 1-D ETDRK step against the FP32 oracle.
+
+Every (nonlinear kind, ETDRK order, N) runs a small seeded batch through the
+C ABI (via the stepper classes) and is compared with the oracle's FP32
+restatement of the reference step on the same inputs; rel-L2 (the
+reference's nRMSE) must be <= 1e-5 after one and after ten steps.
+"""
+import numpy as np
+import pytest
+
+import exponax_amd as ex
+from oracle import stepper as ost
+from tests.helpers import RTOL_STEP, ics, max_rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(kind, N, order, L=None, dt=None):
+    if kind == "ks_cons":
+        L = L or 60.0
+        dt = dt or 0.1
+        return (ex.stepper.KuramotoSivashinskyConservative(1, L, N, dt, order=order),
+                ost.KuramotoSivashinskyConservative(1, L, N, dt, order=order, dtype=np.float32), L)
+    if kind == "ks":
+        L = L or 60.0
+        dt = dt or 0.05
+        return (ex.stepper.KuramotoSivashinsky(1, L, N, dt, order=order),
+                ost.KuramotoSivashinsky(1, L, N, dt, order=order, dtype=np.float32), L)
+    if kind == "burgers":
+        L = L or 1.0
+        dt = dt or 1e-3
+        return (ex.stepper.Burgers(1, L, N, dt, diffusivity=3e-3, order=order),
+                ost.Burgers(1, L, N, dt, diffusivity=3e-3, order=order, dtype=np.float32), L)
+    if kind == "burgers_cons":
+        L = L or 1.0
+        dt = dt or 1e-3
+        return (ex.stepper.Burgers(1, L, N, dt, diffusivity=3e-3, order=order, conservative=True,
+                                   single_channel=True),
+                ost.Burgers(1, L, N, dt, diffusivity=3e-3, order=order, conservative=True, single_channel=True,
+                            dtype=np.float32), L)
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("N", [32, 64, 256, 1024, 2048])
+@pytest.mark.parametrize("kind", ["ks_cons", "ks", "burgers", "burgers_cons"])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_step_parity(device, kind, N, order):
+    print(f"[case] step kind={kind} N={N} order={order}", flush=True)
+    gpu, cpu, L = _pair(kind, N, order)
+    u0, _ = ics(4, 1, L, N, 5, seed=11)
+    u_g = ex.vmap(gpu)(u0)
+    u_c = cpu.step_batch(u0)
+    assert np.isfinite(u_g).all()
+    assert max_rel_l2(u_g, u_c) <= RTOL_STEP
+    for _ in range(9):
+        u_g = ex.vmap(gpu)(u_g)
+        u_c = cpu.step_batch(u_c)
+    assert max_rel_l2(u_g, u_c) <= RTOL_STEP
+
+
+@pytest.mark.parametrize("N", [64, 256, 1024])
+def test_etdrk0_linear_exact(device, N):
+    """ETDRK0 on diffusion reproduces the analytic decay (reference
+    tests/test_etdrk.py:20-45) and the oracle."""
+    L, nu, dt = 10.0, 0.1, 0.1
+
+    class Diffusion(ex.BaseStepper):
+        def __init__(self):
+            super().__init__(1, L, N, dt, num_channels=1, order=0)
+
+        def _build_linear_operator(self, d):
+            return nu * ex.spectral.build_laplace_operator(d)
+
+        def _build_nonlinear_fun(self, d):
+            return ex.nonlin_fun.ZeroNonlinearFun(1, N)
+
+    x = ex.make_grid(1, L, N)
+    k = 4 * 2 * np.pi / L
+    u0 = np.sin(k * x).astype(np.float32)
+    u1 = Diffusion()(u0)
+    exact = np.exp(-k * k * nu * dt) * np.sin(k * x)
+    assert np.max(np.abs(u1 - exact)) < 1e-5
+
+
+def test_fft_roundtrip_and_numpy(device):
+    rng = np.random.default_rng(0)
+    for N in [16, 64, 256, 1024, 4096]:
+        print(f"[case] fft N={N}", flush=True)
+        x = rng.standard_normal((3, N)).astype(np.float32)
+        X = ex.fft(x)
+        ref = np.fft.rfft(x.astype(np.float64))
+        assert np.max(np.abs(X - ref)) <= 2e-5 * np.max(np.abs(ref)) * np.log2(N)
+        y = ex.ifft(X, num_points=N)
+        assert np.max(np.abs(y - x)) < 1e-5 * np.log2(N)
+        # c2r ignores Im at DC and Nyquist
+        X2 = ref.astype(np.complex64).copy()
+        X2[:, 0] += 3j
+        X2[:, -1] -= 2j
+        assert np.max(np.abs(ex.ifft(X2, num_points=N) - np.fft.irfft(ref, n=N))) < 1e-5 * np.log2(N)
+
+
+def test_shape_error(device):
+    s = ex.stepper.Burgers(1, 1.0, 64, 0.01)
+    with pytest.raises(ValueError, match="Expected shape"):
+        s(np.zeros((2, 64), np.float32))
