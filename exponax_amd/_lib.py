@@ -22,6 +22,7 @@ EXMI_NL_GENERAL = 4
 EXMI_NL_VORTICITY = 5
 EXMI_NL_KOLMOGOROV = 6
 EXMI_MAX_POLY = 8
+EXMI_FLAG_EXACT_FORM = 1
 
 EXMI_ERR_INVALID = -1
 EXMI_ERR_HIP = -2
@@ -31,8 +32,8 @@ EXMI_ERR_UNSUPPORTED = -4
 # every symbol include/exmi.h declares
 EXPORTED = (
     "exmi_plan_create", "exmi_plan_destroy", "exmi_step", "exmi_step_fourier", "exmi_nonlinear",
-    "exmi_rollout", "exmi_repeat", "exmi_rfft", "exmi_irfft", "exmi_plan_num_coefs", "exmi_plan_get_tables",
-    "exmi_last_error", "exmi_info",
+    "exmi_rollout", "exmi_rollout_prepare", "exmi_repeat", "exmi_rfft", "exmi_irfft", "exmi_plan_num_coefs", "exmi_plan_get_tables",
+    "exmi_plan_rollout_strategy", "exmi_plan_set_rollout_strategy", "exmi_last_error", "exmi_info",
 )
 
 
@@ -54,6 +55,7 @@ class PlanDesc(C.Structure):
         ("lin_batch", C.c_int32),
         ("lin_channels", C.c_int32),
         ("num_circle_points", C.c_int32),
+        ("flags", C.c_int32),
         ("domain_extent", C.c_double),
         ("dt", C.c_double),
         ("circle_radius", C.c_double),
@@ -81,11 +83,14 @@ def _declare(lib):
     for name in ("exmi_step", "exmi_step_fourier", "exmi_nonlinear"):
         getattr(lib, name).argtypes = [vp, vp, i64, vp, i64, vp]
     lib.exmi_rollout.argtypes = [vp, vp, i64, vp, i32, i32, i32, vp]
+    lib.exmi_rollout_prepare.argtypes = [vp, vp, i64, vp, i32, i32]
     lib.exmi_repeat.argtypes = [vp, vp, i64, vp, i32, i32, vp]
     lib.exmi_rfft.argtypes = [i32, i32, i32, vp, vp, vp]
     lib.exmi_irfft.argtypes = [i32, i32, i32, vp, vp, vp]
     lib.exmi_plan_num_coefs.argtypes = [vp]
     lib.exmi_plan_get_tables.argtypes = [vp, vp, i64]
+    lib.exmi_plan_rollout_strategy.argtypes = [vp]
+    lib.exmi_plan_set_rollout_strategy.argtypes = [vp, i32]
     lib.exmi_last_error.restype = C.c_char_p
     lib.exmi_info.argtypes = [C.c_char_p, i32]
     for name in EXPORTED:
@@ -174,6 +179,12 @@ class Plan:
         out = np.empty(n, dtype=np.float32)
         check(self._lib.exmi_plan_get_tables(self._h, out.ctypes.data, n), "exmi_plan_get_tables")
         return out.reshape(d.lin_batch, d.lin_channels, self.num_coefs(), nr)
+
+    def rollout_strategy(self) -> int:
+        return int(self._lib.exmi_plan_rollout_strategy(self._h))
+
+    def set_rollout_strategy(self, strategy: int) -> None:
+        check(self._lib.exmi_plan_set_rollout_strategy(self._h, int(strategy)), "exmi_plan_set_rollout_strategy")
 
     def __del__(self):
         h = getattr(self, "_h", None)

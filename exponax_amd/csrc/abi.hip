@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -18,13 +19,15 @@
 namespace exmi {
 
 using StepFn = hipError_t (*)(const Step1DArgs&, hipStream_t);
+using RollFn = hipError_t (*)(const Roll1DArgs&, hipStream_t);
 hipError_t launch_coef(const double* L, long long total, int nr, double dt, int order, int ncirc, double radius,
                        float* out, hipStream_t st);
 
 #define EXMI_DECL_SIZE(M)                                                                              \
   StepFn get_step1d_##M(int kind, int order);                                                          \
-  hipError_t launch_fft1d_##M(bool inverse, const float* in, float* out, int count, float inv_n, const cf* tw, \
-                              const cf* twr, hipStream_t st);
+  RollFn get_roll1d_##M(int kind, int order, int nt);                                                  \
+  hipError_t launch_fft1d_##M(bool inverse, const float* in, float* out, int count, const RealTables& rt, \
+                              hipStream_t st);
 EXMI_DECL_SIZE(8)
 EXMI_DECL_SIZE(16)
 EXMI_DECL_SIZE(32)
@@ -50,11 +53,26 @@ static StepFn get_step1d(int M, int kind, int order) {
   }
 }
 
-static hipError_t launch_fft1d(int M, bool inverse, const float* in, float* out, int count, float inv_n,
-                               const cf* tw, const cf* twr, hipStream_t st) {
+static RollFn get_roll1d(int M, int kind, int order, int nt) {
+  switch (M) {
+    case 8: return get_roll1d_8(kind, order, nt);
+    case 16: return get_roll1d_16(kind, order, nt);
+    case 32: return get_roll1d_32(kind, order, nt);
+    case 64: return get_roll1d_64(kind, order, nt);
+    case 128: return get_roll1d_128(kind, order, nt);
+    case 256: return get_roll1d_256(kind, order, nt);
+    case 512: return get_roll1d_512(kind, order, nt);
+    case 1024: return get_roll1d_1024(kind, order, nt);
+    case 2048: return get_roll1d_2048(kind, order, nt);
+    default: return nullptr;
+  }
+}
+
+static hipError_t launch_fft1d(int M, bool inverse, const float* in, float* out, int count, const RealTables& rt,
+                               hipStream_t st) {
   switch (M) {
 #define EXMI_CASE(MM) \
-  case MM: return launch_fft1d_##MM(inverse, in, out, count, inv_n, tw, twr, st);
+  case MM: return launch_fft1d_##MM(inverse, in, out, count, rt, st);
     EXMI_CASE(8)
     EXMI_CASE(16)
     EXMI_CASE(32)
@@ -91,8 +109,9 @@ static int fail(int code, const char* fmt, ...) {
 
 // ---------------------------------------------------- twiddle table cache
 struct Twiddles {
-  cf* tw = nullptr;   // exp(-2 pi i m / M), m < M
-  cf* twr = nullptr;  // exp(-2 pi i k / (2M)), k <= M
+  cf* tw = nullptr;  // exp(-2 pi i m / M), m < M
+  cf* fw = nullptr;  // -i w_k, w_k = exp(-2 pi i k / (2M)), k < M
+  RealTables rt() const { return RealTables{tw, fw}; }
 };
 
 static std::mutex g_tw_mu;
@@ -108,17 +127,19 @@ static int get_twiddles(int M, Twiddles* out) {
     *out = it->second;
     return EXMI_OK;
   }
-  std::vector<cf> h(M), hr(M + 1);
+  // tables formed in double precision, rounded once to float
+  std::vector<cf> h(M), hf(M);
   const double pi = 3.14159265358979323846;
   for (int m = 0; m < M; ++m) h[m] = cf{(float)std::cos(2 * pi * m / M), (float)-std::sin(2 * pi * m / M)};
-  for (int k = 0; k <= M; ++k)
-    hr[k] = cf{(float)std::cos(pi * k / M), (float)-std::sin(pi * k / M)};
+  for (int k = 0; k < M; ++k) {
+    const double c = std::cos(pi * k / M), s = -std::sin(pi * k / M);  // w_k
+    hf[k] = cf{(float)s, (float)(-c)};                                  // -i w_k
+  }
   Twiddles t;
-  if (hipMalloc(&t.tw, sizeof(cf) * M) != hipSuccess) return fail(EXMI_ERR_ALLOC, "twiddle allocation failed");
-  if (hipMalloc(&t.twr, sizeof(cf) * (M + 1)) != hipSuccess)
+  if (hipMalloc(&t.tw, sizeof(cf) * M) != hipSuccess || hipMalloc(&t.fw, sizeof(cf) * M) != hipSuccess)
     return fail(EXMI_ERR_ALLOC, "twiddle allocation failed");
   HIP_TRY(hipMemcpy(t.tw, h.data(), sizeof(cf) * M, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(t.twr, hr.data(), sizeof(cf) * (M + 1), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(t.fw, hf.data(), sizeof(cf) * M, hipMemcpyHostToDevice));
   g_tw[key] = t;
   *out = t;
   return EXMI_OK;
@@ -160,6 +181,8 @@ struct exmi_plan {
   Twiddles tw;
   StepFn step = nullptr;
   StepFn nonlin = nullptr;
+  RollFn roll[3] = {nullptr, nullptr, nullptr};  // by transforms per lane group (1, 2)
+  int roll_nt = 0;                               // chosen by the autotuner (0 = not yet)
   hipStream_t capture_stream = nullptr;
   GraphCache gc;
 };
@@ -173,10 +196,9 @@ static void fill_args(const exmi_plan* p, Step1DArgs* a) {
   a->lin_channels = p->d.lin_channels;
   a->ncoef = p->ncoef;
   a->tw = p->tw.tw;
-  a->twr = p->tw.twr;
+  a->fw = p->tw.fw;
   a->kscale = p->kscale;
   a->kmax = p->kmax;
-  a->inv_n = 1.0f / (float)p->d.num_points;
   a->scale = (float)p->d.scale;
   a->g0 = (float)p->d.general_scales[0];
   a->g1 = (float)p->d.general_scales[1];
@@ -258,6 +280,13 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
     int rc = kernel_kind(d, &kind);
     if (rc) return rc;
   }
+  const int kmax = dealias_kmax(N, d->dealiasing_fraction);
+  int nl_kind = kind;  // kernel used for exmi_nonlinear: always the literal form
+  // Alias-free quadratic products (3 kmax < N, C = D = 1): u u_x and
+  // (1/2)(u^2)_x agree on every retained mode, see EXMI_FLAG_EXACT_FORM.
+  if (kind == NL_CONV_NONCONS && !(d->flags & EXMI_FLAG_EXACT_FORM) && d->num_channels == 1 && kmax >= 0 &&
+      3LL * kmax < N)
+    kind = NL_CONV_CONS;
   exmi_plan* p = new exmi_plan();
   p->d = *d;
   p->d.host_linear_operator = nullptr;
@@ -265,15 +294,19 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   p->nr = p->M + 1;
   p->ncoef = num_coefs(d->order);
   p->kind = kind;
-  p->kmax = dealias_kmax(N, d->dealiasing_fraction);
+  p->kmax = kmax;
   p->kscale = (float)(2.0 * 3.14159265358979323846 / d->domain_extent);
   hipGetDevice(&p->device);
   p->step = get_step1d(p->M, kind, d->order);
-  p->nonlin = d->order > 0 ? get_step1d(p->M, kind, 1) : get_step1d(p->M, NL_ZERO, 1);
+  p->nonlin = d->order > 0 ? get_step1d(p->M, nl_kind, 1) : get_step1d(p->M, NL_ZERO, 1);
   if (!p->step || !p->nonlin) {
     delete p;
     return fail(EXMI_ERR_UNSUPPORTED, "no kernel for N=%d kind=%d order=%d", N, kind, d->order);
   }
+  p->roll[1] = get_roll1d(p->M, kind, d->order, 1);
+  p->roll[2] = get_roll1d(p->M, kind, d->order, 2);
+  if (const char* e = getenv("EXMI_ROLL_NT")) p->roll_nt = atoi(e);  // 1, 2; -1 = per-step launches
+
   int rc = get_twiddles(p->M, &p->tw);
   if (rc) {
     delete p;
@@ -312,6 +345,20 @@ int exmi_plan_destroy(exmi_plan* p) {
   hipFree(p->tables);
   hipFree(p->work);
   delete p;
+  return EXMI_OK;
+}
+
+int exmi_plan_rollout_strategy(const exmi_plan* p) { return p ? p->roll_nt : fail(EXMI_ERR_INVALID, "null plan"); }
+
+int exmi_plan_set_rollout_strategy(exmi_plan* p, int32_t nt) {
+  if (!p) return fail(EXMI_ERR_INVALID, "null plan");
+  if (!(nt == -1 || nt == 0 || ((nt == 1 || nt == 2) && p->roll[nt])))
+    return fail(EXMI_ERR_UNSUPPORTED, "rollout strategy %d not available for this plan", nt);
+  p->roll_nt = nt;
+  if (p->gc.exec) {
+    hipGraphExecDestroy(p->gc.exec);
+    p->gc = GraphCache();
+  }
   return EXMI_OK;
 }
 
@@ -377,8 +424,25 @@ static hipError_t copy_batch(const float* src, long long ss, float* dst, long lo
   return hipGetLastError();
 }
 
-static int enqueue_rollout(exmi_plan* p, const float* u0, long long u0_bs, float* traj, int n, int include_init,
-                           hipStream_t st) {
+static int enqueue_roll(exmi_plan* p, int nt, const float* u0, long long u0_bs, float* out, long long out_bs,
+                        long long out_ts, int n, int final_only, hipStream_t st) {
+  Roll1DArgs ra;
+  fill_args(p, &ra.s);
+  ra.s.in = u0;
+  ra.s.in_bstride = u0_bs;
+  ra.traj = out;
+  ra.traj_bstride = out_bs;
+  ra.traj_tstride = out_ts;
+  ra.nsteps = n;
+  ra.final_only = final_only;
+  hipError_t e = p->roll[nt](ra, st);
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "rollout launch failed: %s", hipGetErrorString(e));
+}
+
+// Rollout as per-step launches (nt <= 0) or one multi-step launch carrying
+// nt transforms per lane group.
+static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u0_bs, float* traj, int n,
+                              int include_init, hipStream_t st) {
   const long long sz = (long long)p->d.num_channels * p->d.num_points;  // 1-D state size
   const long long tstride = (long long)(n + (include_init ? 1 : 0)) * sz;
   int off = 0;
@@ -387,6 +451,8 @@ static int enqueue_rollout(exmi_plan* p, const float* u0, long long u0_bs, float
     if (e != hipSuccess) return fail(EXMI_ERR_HIP, "copy launch failed: %s", hipGetErrorString(e));
     off = 1;
   }
+  if (n == 0) return EXMI_OK;
+  if (nt > 0 && p->roll[nt]) return enqueue_roll(p, nt, u0, u0_bs, traj + off * sz, tstride, sz, n, 0, st);
   for (int t = 0; t < n; ++t) {
     const float* in = (t == 0 && !include_init) ? u0 : traj + (long long)(t - 1 + off) * sz;
     const long long ibs = (t == 0 && !include_init) ? u0_bs : tstride;
@@ -397,9 +463,60 @@ static int enqueue_rollout(exmi_plan* p, const float* u0, long long u0_bs, float
   return EXMI_OK;
 }
 
+// Autotuner: time each available rollout strategy on a short probe into the
+// caller's own buffer (overwritten by the real run) and keep the fastest.
+static int autotune_rollout(exmi_plan* p, const float* u0, long long u0_bs, float* traj, int n, int include_init) {
+  if (p->roll_nt != 0) return EXMI_OK;
+  const int probe = n < 16 ? n : 16;
+  if (probe == 0) {
+    p->roll_nt = p->roll[2] ? 2 : 1;
+    return EXMI_OK;
+  }
+  hipStream_t st = p->capture_stream;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  float best = 1e30f;
+  int best_nt = -1;
+  const int cands[3] = {-1, 1, 2};
+  for (int ci = 0; ci < 3; ++ci) {
+    const int nt = cands[ci];
+    if (nt > 0 && !p->roll[nt]) continue;
+    float tmin = 1e30f;
+    for (int rep = 0; rep < 2; ++rep) {
+      HIP_TRY(hipEventRecord(e0, st));
+      int rc = enqueue_rollout_nt(p, nt, u0, u0_bs, traj, probe, include_init, st);
+      if (rc) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return rc;
+      }
+      HIP_TRY(hipEventRecord(e1, st));
+      HIP_TRY(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < tmin) tmin = ms;
+    }
+    if (tmin < best) {
+      best = tmin;
+      best_nt = nt;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  p->roll_nt = best_nt;
+  return EXMI_OK;
+}
+
+static int enqueue_rollout(exmi_plan* p, const float* u0, long long u0_bs, float* traj, int n, int include_init,
+                           hipStream_t st) {
+  return enqueue_rollout_nt(p, p->roll_nt, u0, u0_bs, traj, n, include_init, st);
+}
+
 static int run_maybe_graph(exmi_plan* p, const void* a, const void* b, long long s, int n, int flag, int kind,
                            int use_graph, hipStream_t st, int (*enq)(exmi_plan*, const void*, const void*, long long,
-                                                                    int, int, hipStream_t)) {
+                                                                    int, int, hipStream_t),
+                           bool launch = true) {
   if (!use_graph) return enq(p, a, b, s, n, flag, st);
   GraphCache& g = p->gc;
   if (!(g.exec && g.a == a && g.b == b && g.s == s && g.n == n && g.flag == flag && g.kind == kind)) {
@@ -429,7 +546,7 @@ static int run_maybe_graph(exmi_plan* p, const void* a, const void* b, long long
     g.flag = flag;
     g.kind = kind;
   }
-  HIP_TRY(hipGraphLaunch(g.exec, st));
+  if (launch) HIP_TRY(hipGraphLaunch(g.exec, st));
   return EXMI_OK;
 }
 
@@ -445,6 +562,8 @@ static int enq_repeat_cb(exmi_plan* p, const void* a, const void* b, long long s
     hipError_t e = copy_batch(u0, s, out, sz, sz, p->d.batch, st);
     return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "copy failed: %s", hipGetErrorString(e));
   }
+  const int nt = p->roll_nt != 0 ? p->roll_nt : (p->roll[2] ? 2 : 1);
+  if (nt > 0 && p->roll[nt]) return enqueue_roll(p, nt, u0, s, out, sz, 0, n, 1, st);
   for (int t = 0; t < n; ++t) {
     const float* in = t == 0 ? u0 : (((n - t) % 2 == 0) ? out : p->work);
     const long long ibs = t == 0 ? s : sz;
@@ -459,8 +578,26 @@ int exmi_rollout(exmi_plan* p, const float* u0, int64_t u0_bs, float* traj, int3
                  int32_t use_graph, void* stream) {
   if (!p || !u0 || !traj) return fail(EXMI_ERR_INVALID, "null argument");
   if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
+  if (p->roll_nt == 0) {
+    // the probe writes into traj; order it after the caller's pending work
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    int rc = autotune_rollout(p, u0, u0_bs, traj, n, include_init);
+    if (rc) return rc;
+  }
   return run_maybe_graph(p, u0, traj, u0_bs, n, include_init ? 1 : 0, 1, use_graph, (hipStream_t)stream,
                          enq_rollout_cb);
+}
+
+int exmi_rollout_prepare(exmi_plan* p, const float* u0, int64_t u0_bs, float* traj, int32_t n,
+                         int32_t include_init) {
+  if (!p || !u0 || !traj) return fail(EXMI_ERR_INVALID, "null argument");
+  if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
+  if (p->roll_nt == 0) {
+    HIP_TRY(hipDeviceSynchronize());
+    int rc = autotune_rollout(p, u0, u0_bs, traj, n, include_init);
+    if (rc) return rc;
+  }
+  return run_maybe_graph(p, u0, traj, u0_bs, n, include_init ? 1 : 0, 1, 1, nullptr, enq_rollout_cb, false);
 }
 
 int exmi_repeat(exmi_plan* p, const float* u0, int64_t u0_bs, float* u_out, int32_t n, int32_t use_graph,
@@ -486,7 +623,7 @@ int exmi_rfft(int32_t D, int32_t N, int32_t count, const float* x, float* x_hat,
   Twiddles tw;
   int rc = get_twiddles(N / 2, &tw);
   if (rc) return rc;
-  hipError_t e = launch_fft1d(N / 2, false, x, x_hat, count, 1.0f / N, tw.tw, tw.twr, (hipStream_t)stream);
+  hipError_t e = launch_fft1d(N / 2, false, x, x_hat, count, tw.rt(), (hipStream_t)stream);
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "rfft launch failed: %s", hipGetErrorString(e));
 }
 
@@ -497,7 +634,7 @@ int exmi_irfft(int32_t D, int32_t N, int32_t count, const float* x_hat, float* x
   Twiddles tw;
   int rc = get_twiddles(N / 2, &tw);
   if (rc) return rc;
-  hipError_t e = launch_fft1d(N / 2, true, x_hat, x, count, 1.0f / N, tw.tw, tw.twr, (hipStream_t)stream);
+  hipError_t e = launch_fft1d(N / 2, true, x_hat, x, count, tw.rt(), (hipStream_t)stream);
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "irfft launch failed: %s", hipGetErrorString(e));
 }
 

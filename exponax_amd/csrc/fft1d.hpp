@@ -66,27 +66,31 @@ struct Dft<4, INV> {
 
 template <bool INV>
 struct Dft<8, INV> {
+  // radix-2 split into two DFT4s; the odd outputs' W8^1 and W8^3 factors
+  // (h (1 -+ i), h (-1 -+ i)) are folded into fused multiply-adds of the
+  // final butterfly.
   __device__ __forceinline__ static void run(cf* v) {
     const float h = 0.70710678118654752440f;
     cf e[4] = {v[0], v[2], v[4], v[6]};
     cf o[4] = {v[1], v[3], v[5], v[7]};
     Dft<4, INV>::run(e);
     Dft<4, INV>::run(o);
-    // o[k] *= W8^k (forward W8 = exp(-i pi/4); inverse conjugate)
-    cf o1 = o[1], o3 = o[3];
-    if (!INV) {
-      o[1] = cf{h * (o1.re + o1.im), h * (o1.im - o1.re)};
-      o[3] = cf{h * (o3.im - o3.re), -h * (o3.re + o3.im)};
-    } else {
-      o[1] = cf{h * (o1.re - o1.im), h * (o1.im + o1.re)};
-      o[3] = cf{-h * (o3.re + o3.im), h * (o3.re - o3.im)};
-    }
-    o[2] = rot90<INV>(o[2]);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      v[k] = e[k] + o[k];
-      v[k + 4] = e[k] - o[k];
-    }
+    // forward: W8 o1 = h ((o1.re + o1.im) + i (o1.im - o1.re)),
+    //          W8^3 o3 = h ((o3.im - o3.re) - i (o3.re + o3.im));
+    // inverse: conjugate factors.
+    const float p1 = INV ? (o[1].re - o[1].im) : (o[1].re + o[1].im);
+    const float q1 = INV ? (o[1].im + o[1].re) : (o[1].im - o[1].re);
+    const float p3 = INV ? -(o[3].re + o[3].im) : (o[3].im - o[3].re);
+    const float q3 = INV ? (o[3].re - o[3].im) : -(o[3].re + o[3].im);
+    const cf o2 = rot90<INV>(o[2]);
+    v[0] = e[0] + o[0];
+    v[4] = e[0] - o[0];
+    v[1] = cf{fmaf(h, p1, e[1].re), fmaf(h, q1, e[1].im)};
+    v[5] = cf{fmaf(-h, p1, e[1].re), fmaf(-h, q1, e[1].im)};
+    v[2] = e[2] + o2;
+    v[6] = e[2] - o2;
+    v[3] = cf{fmaf(h, p3, e[3].re), fmaf(h, q3, e[3].im)};
+    v[7] = cf{fmaf(-h, p3, e[3].re), fmaf(-h, q3, e[3].im)};
   }
 };
 
@@ -152,54 +156,87 @@ struct LdsSize {
   static constexpr int value = M + (M >> 4) + 1;
 };
 
-// One Stockham pass on the lane's registers. `v` holds the lane's PT inputs
-// in lane-strided order (input x[j + s*T] in v[s]) and receives the outputs
-// y[j + s*T] in v[s] after the exchange through `lds`, except for the last
-// pass which needs no exchange.
-// `tw` is the table exp(-2 pi i m / M), m = 0..M-1.
-template <int M, int T, int R, int NS, bool INV, bool LAST>
-__device__ __forceinline__ void stockham_pass(cf* v, int j, cf* lds, const cf* __restrict__ tw) {
-  constexpr int PT = M / T;
-  constexpr int NB = PT / R;   // butterflies per lane
-  constexpr int MR = M / R;    // butterfly count
-  constexpr int TWS = M / (NS * R);
+// Per-lane LDS addressing. With pad(x) = x + x/16 and power-of-two sizes,
+// pad(base + c) = pad(base) + c + c/16 for every offset c the passes use
+// (c a multiple of the pass stride below one 16-slot line, or of 16), so each
+// access is one per-lane base register plus an immediate offset.
+__host__ __device__ constexpr int cpad(int c) { return c + c / 16; }
+
+template <int M, int PT>
+struct Lane {
+  static constexpr int T = M / PT;
+  int j;    // lane index within the transform
+  int rd;   // pad(j): base of the lane-strided reads j + s*T
+  int mir;  // pad(T - j): base of the mirrored reads M - j - s*T
+  __device__ __forceinline__ explicit Lane(int jj) : j(jj), rd(pad(jj)), mir(pad(T - jj)) {}
+  // LDS index of element j + s*T
+  __device__ __forceinline__ int at(int s) const { return rd + cpad(s * T); }
+  // LDS index of element (M - (j + s*T)) mod M; lane 0 mirrors onto its
+  // own slots (PT - s) mod PT.
+  __device__ __forceinline__ int mirror(int s) const {
+    return j == 0 ? cpad(((PT - s) % PT) * T) : mir + cpad((PT - 1 - s) * T);
+  }
+};
+
+// One Stockham pass on the lanes' registers, for NT independent transforms
+// handled together (their instruction streams interleave). v[n] holds the
+// lane's PT inputs of transform n in lane-strided order (x[j + s*T] in
+// v[n][s]) and receives the outputs y[j + s*T]. All NT transforms exchange
+// through the same LDS buffer, one after the other (write n, read n, write
+// n+1, ...): LDS operations of a wavefront complete in program order, so
+// transform n+1's stores cannot overtake transform n's loads. The last pass
+// needs no exchange. `pw` are the lane's twiddles of this pass (TwRegs).
+template <int T>
+__device__ __forceinline__ void xsync();
+
+template <int M, int PT, int R, int NS, bool INV, bool LAST, int NT>
+__device__ __forceinline__ void stockham_pass(cf (*v)[PT], const Lane<M, PT>& ln, cf* lds, const cf* pw) {
+  constexpr int T = M / PT;
+  constexpr int NB = PT / R;  // butterflies per lane
+  static_assert(NT == 1 || T <= 64, "interleaved transforms must live in one wavefront");
 #pragma unroll
-  for (int q = 0; q < NB; ++q) {
-    const int b = j + q * T;
-    cf x[R];
+  for (int n = 0; n < NT; ++n) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) x[r] = v[q + r * NB];  // x[b + r*MR] = v[(b + r*MR - j)/T]
-    if constexpr (NS > 1) {
-      const int k = b % NS;
+    for (int q = 0; q < NB; ++q) {
+      cf x[R];
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        cf w = tw[(r * k * TWS) & (M - 1)];
-        x[r] = INV ? cmulc(x[r], w) : cmul(x[r], w);
+      for (int r = 0; r < R; ++r) x[r] = v[n][q + r * NB];  // x[b + r*M/R], b = j + q*T
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const cf w = pw[q * (R - 1) + r - 1];
+          x[r] = INV ? cmulc(x[r], w) : cmul(x[r], w);
+        }
       }
-    }
-    Dft<R, INV>::run(x);
-    if constexpr (LAST) {
-      // NS*R == M: output index b + r*NS = j + (q + r*NB)*T
+      Dft<R, INV>::run(x);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q + r * NB] = x[r];
-    } else {
-      const int base = (b / NS) * NS * R + (b % NS);
-#pragma unroll
-      for (int r = 0; r < R; ++r) lds[pad(base + r * NS)] = x[r];
+      for (int r = 0; r < R; ++r) v[n][q + r * NB] = x[r];
     }
   }
-  (void)MR;
-}
-
-template <int M, int T>
-__device__ __forceinline__ void lds_load(cf* v, int j, const cf* lds) {
-  constexpr int PT = M / T;
+  if constexpr (!LAST) {
+    // output y[(b/NS)*NS*R + b%NS + r*NS] = v[n][q + r*NB]
 #pragma unroll
-  for (int s = 0; s < PT; ++s) v[s] = lds[pad(j + s * T)];
+    for (int n = 0; n < NT; ++n) {
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        const int b = ln.j + q * T;
+        const int wb = pad((b / NS) * NS * R + (b % NS));
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[wb + cpad(r * NS)] = v[n][q + r * NB];
+      }
+      xsync<T>();
+#pragma unroll
+      for (int s = 0; s < PT; ++s) v[n][s] = lds[ln.at(s)];
+      xsync<T>();
+    }
+  }
+  // LAST: NS*R == M, output index b + r*NS = j + (q + r*NB)*T is already the
+  // lane-strided slot q + r*NB.
 }
 
 // Sync among the lanes sharing one transform's LDS region. When a transform
-// fits inside one wavefront the exchange is wave-local.
+// fits inside one wavefront the exchange is wave-local (LDS operations of a
+// wave complete in order; the fences only stop compiler reordering).
 template <int T>
 __device__ __forceinline__ void xsync() {
   if constexpr (T <= 64) {
@@ -211,108 +248,145 @@ __device__ __forceinline__ void xsync() {
   }
 }
 
-template <int M, int PT, bool INV, int P>
+template <int M, int PT>
+struct TwRegs;
+
+template <int M, int PT, bool INV, int P, int NT>
 struct PassRunner {
   using PL = Plan<M, PT>;
   static constexpr int T = M / PT;
-  __device__ __forceinline__ static void run(cf* v, int j, cf* lds, const cf* __restrict__ tw) {
+  __device__ __forceinline__ static void run(cf (*v)[PT], const Lane<M, PT>& ln, cf* lds, const TwRegs<M, PT>& tr) {
     constexpr bool LAST = (P == PL::NPASS - 1);
-    stockham_pass<M, T, PL::radix(P), PL::stride(P), INV, LAST>(v, j, lds, tw);
-    if constexpr (!LAST) {
-      xsync<T>();
-      lds_load<M, T>(v, j, lds);
-      xsync<T>();
-      PassRunner<M, PT, INV, P + 1>::run(v, j, lds, tw);
+    stockham_pass<M, PT, PL::radix(P), PL::stride(P), INV, LAST, NT>(v, ln, lds, tr.pw[P]);
+    if constexpr (!LAST) PassRunner<M, PT, INV, P + 1, NT>::run(v, ln, lds, tr);
+  }
+};
+
+// Complex FFTs of M points, in place on lane-strided registers.
+template <int M, int PT, bool INV, int NT>
+__device__ __forceinline__ void cfft(cf (*v)[PT], const Lane<M, PT>& ln, cf* lds, const TwRegs<M, PT>& tr) {
+  PassRunner<M, PT, INV, 0, NT>::run(v, ln, lds, tr);
+}
+
+// Table of the real-transform pre/post-processing, per mode k = 0..M-1
+// (N = 2M, w_k = exp(-2 pi i k / N)): fw[k] = -i w_k. The transforms carry
+// power-of-two scale factors (exact in floating point) so that one table
+// serves both directions:
+//   forward  X2[k] = (Z[k] + Z*[M-k]) + fw[k] (Z[k] - Z*[M-k])  = 2 X[k]
+//   inverse  Z'[k] = (X[k] + X*[M-k]) + conj(fw[k]) (X[k] - X*[M-k])
+//            and the M-point inverse DFT of Z' gives N * irfft(X).
+struct RealTables {
+  const cf* tw;  // exp(-2 pi i m / M), m < M
+  const cf* fw;  // -i exp(-2 pi i k / N), k < M
+};
+
+// Per-lane twiddle registers: every twiddle a lane multiplies by in any pass,
+// and its pre/post-processing factors, loaded once per kernel so that the
+// transforms themselves issue no memory loads.
+template <int M, int PT>
+struct TwRegs {
+  using PL = Plan<M, PT>;
+  static constexpr int T = M / PT;
+  cf pw[PL::NPASS][PT];  // pass p, entry q*(R-1) + (r-1); pass 0 needs none
+  cf fw[PT];
+  __device__ __forceinline__ void load(int j, const RealTables& rt) {
+    load_pass<1>(j, rt.tw);
+#pragma unroll
+    for (int s = 0; s < PT; ++s) fw[s] = rt.fw[j + s * T];
+  }
+  template <int P>
+  __device__ __forceinline__ void load_pass(int j, const cf* __restrict__ tw) {
+    if constexpr (P < PL::NPASS) {
+      constexpr int R = PL::radix(P), NS = PL::stride(P), NB = PT / R, TWS = M / (NS * R);
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        const int k = (j + q * T) % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) pw[P][q * (R - 1) + r - 1] = tw[r * k * TWS];
+      }
+      load_pass<P + 1>(j, tw);
     }
   }
 };
 
-// Complex FFT of M points, in place on lane-strided registers.
-template <int M, int PT, bool INV>
-__device__ __forceinline__ void cfft(cf* v, int j, cf* lds, const cf* __restrict__ tw) {
-  PassRunner<M, PT, INV, 0>::run(v, j, lds, tw);
-}
-
 // ----------------------------------------------------------- real transforms
-// Spectrum layout: lane j, slot s holds X[j + s*T] (k = 0..M-1); `nyq` holds
-// X[M] (meaningful in lane 0 only, kept in every lane of the group).
+// Spectrum layout: lane j, slot s holds X[j + s*T] (k = 0..M-1); nyq holds
+// X[M] (meaningful in lane 0 only).
 //
-// r2c: z (packed even/odd samples, lane-strided) -> X. `twr` is the table
-// exp(-2 pi i k / N), k = 0..M.
-template <int M, int PT>
-__device__ __forceinline__ void rfft(cf* z, cf& nyq, int j, cf* lds, const cf* __restrict__ tw,
-                                     const cf* __restrict__ twr) {
+// r2c: z (packed even/odd samples x[2n] + i x[2n+1], lane-strided) -> 2 X.
+// Modes above `kcut` are not formed (set to zero): callers that dealias the
+// result pass the dealias cutoff; slots entirely above it are skipped with a
+// uniform branch.
+template <int M, int PT, int NT>
+__device__ __forceinline__ void rfft(cf (*z)[PT], cf* nyq, const Lane<M, PT>& ln, cf* lds, const TwRegs<M, PT>& tr,
+                                     int kcut = M) {
   constexpr int T = M / PT;
-  cfft<M, PT, false>(z, j, lds, tw);
-  // partner exchange Z[M-k]
+  cfft<M, PT, false, NT>(z, ln, lds, tr);
+  cf zm[NT][PT];
 #pragma unroll
-  for (int s = 0; s < PT; ++s) lds[pad(j + s * T)] = z[s];
-  xsync<T>();
-  cf zm[PT];
+  for (int n = 0; n < NT; ++n) {
 #pragma unroll
-  for (int s = 0; s < PT; ++s) {
-    const int k = j + s * T;
-    zm[s] = lds[pad((M - k) & (M - 1))];
+    for (int s = 0; s < PT; ++s) lds[ln.at(s)] = z[n][s];
+    xsync<T>();
+#pragma unroll
+    for (int s = 0; s < PT; ++s) zm[n][s] = lds[ln.mirror(s)];
+    xsync<T>();
   }
-  xsync<T>();
 #pragma unroll
-  for (int s = 0; s < PT; ++s) {
-    const int k = j + s * T;
-    const cf a = z[s], b = conj(zm[s]);
-    const cf e = 0.5f * (a + b);               // even-sample spectrum
-    const cf d = a - b;                        // 2i * odd-sample spectrum
-    const cf o = cf{0.5f * d.im, -0.5f * d.re};  // d / (2i)
-    const cf w = twr[k];
-    z[s] = e + cmul(w, o);
+  for (int n = 0; n < NT; ++n) {
+    // lane 0 slot 0 (k = 0): zm[0] = Z[0]; 2 X[M] = 2 (Re Z0 - Im Z0)
+    nyq[n] = cf{2.f * (zm[n][0].re - zm[n][0].im), 0.f};
+#pragma unroll
+    for (int s = 0; s < PT; ++s) {
+      if (s * T > kcut) {
+        z[n][s] = cf{0.f, 0.f};
+        continue;
+      }
+      const cf a = z[n][s], b = conj(zm[n][s]);
+      // at k = 0 this yields 2 X[0] = 2 (Re Z0 + Im Z0) since fw[0] = -i
+      z[n][s] = (a + b) + cmul(tr.fw[s], a - b);
+    }
   }
-  // k = 0 (lane 0, slot 0): zm[0] = Z[0]; X[0] = Re Z0 + Im Z0 and the
-  // Nyquist mode X[M] = Re Z0 - Im Z0, both real. `nyq` is meaningful in
-  // lane 0 only.
-  nyq = cf{zm[0].re - zm[0].im, 0.f};
-  if (j == 0) z[0] = cf{zm[0].re + zm[0].im, 0.f};
 }
 
 // c2r: X (lane-strided spectrum + nyq in lane 0) -> z packed physical samples
-// multiplied by `scale` (1/N for the numpy irfft convention). Imaginary parts
-// of X[0] and X[M] are ignored.
-template <int M, int PT>
-__device__ __forceinline__ void irfft(cf* z, cf nyq, int j, cf* lds, const cf* __restrict__ tw,
-                                      const cf* __restrict__ twr, float scale) {
+// times N (numpy's irfft without its 1/N). Imaginary parts of X[0] and X[M]
+// are ignored.
+template <int M, int PT, int NT>
+__device__ __forceinline__ void irfft(cf (*z)[PT], const cf* nyq, const Lane<M, PT>& ln, cf* lds,
+                                      const TwRegs<M, PT>& tr) {
   constexpr int T = M / PT;
-  if (j == 0) {
-    // pack X[0] (real) and X[M] (real) so that the generic formula at k = 0
-    // reads X[M-0] = X[M] from slot 0 after the exchange below.
-    z[0] = cf{z[0].re, nyq.re};
+  // pack X[0] (real) and X[M] (real) so that the generic formula at k = 0
+  // reads X[M] as the partner of X[0] after the exchange below
+  if (ln.j == 0) {
+#pragma unroll
+    for (int n = 0; n < NT; ++n) z[n][0] = cf{z[n][0].re, nyq[n].re};
   }
+  cf xm[NT][PT];
 #pragma unroll
-  for (int s = 0; s < PT; ++s) lds[pad(j + s * T)] = z[s];
-  xsync<T>();
-  cf xm[PT];
+  for (int n = 0; n < NT; ++n) {
 #pragma unroll
-  for (int s = 0; s < PT; ++s) {
-    const int k = j + s * T;
-    xm[s] = lds[pad((M - k) & (M - 1))];
+    for (int s = 0; s < PT; ++s) lds[ln.at(s)] = z[n][s];
+    xsync<T>();
+#pragma unroll
+    for (int s = 0; s < PT; ++s) xm[n][s] = lds[ln.mirror(s)];
+    xsync<T>();
   }
-  xsync<T>();
+  if (ln.j == 0) {
 #pragma unroll
-  for (int s = 0; s < PT; ++s) {
-    const int k = j + s * T;
-    cf a = z[s], b = xm[s];
-    if (k == 0) {
-      // a = (X0, XM) packed; X[M-0] := XM
-      const float x0 = a.re, xmv = a.im;
-      a = cf{x0, 0.f};
-      b = cf{xmv, 0.f};
+    for (int n = 0; n < NT; ++n) {
+      xm[n][0] = cf{z[n][0].im, 0.f};
+      z[n][0] = cf{z[n][0].re, 0.f};
     }
-    b = conj(b);
-    const cf e = a + b;                       // 2 * even spectrum
-    const cf d = a - b;                       // 2 * W^-k * odd spectrum / W^-k
-    const cf w = twr[k];                      // exp(-2 pi i k/N); need exp(+...)
-    const cf o = cmulc(d, w);                 // d * exp(+2 pi i k / N)
-    // Z = E + i O
-    z[s] = scale * cf{e.re - o.im, e.im + o.re};
   }
-  cfft<M, PT, true>(z, j, lds, tw);
+#pragma unroll
+  for (int n = 0; n < NT; ++n)
+#pragma unroll
+    for (int s = 0; s < PT; ++s) {
+      const cf a = z[n][s], b = conj(xm[n][s]);
+      z[n][s] = (a + b) + cmulc(a - b, tr.fw[s]);
+    }
+  cfft<M, PT, true, NT>(z, ln, lds, tr);
 }
 
 }  // namespace exmi

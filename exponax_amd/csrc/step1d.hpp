@@ -6,7 +6,13 @@
 // read once (float2 pairs, coalesced), transformed, every nonlinear
 // evaluation (dealias, derivative multiply, products, forward transform, post
 // dealias) and ETDRK stage combination happens on registers with LDS used
-// only for the FFT exchanges, and the new state is written once.
+// for the FFT exchanges and as lane-private storage of the operator tables,
+// and the new state is written once.
+//
+// Scaling: the forward transform yields 2 X and the inverse yields N x (see
+// fft1d.hpp). Inside the kernel the spectral state is carried as 2 u_hat and
+// every nonlinear term as 2 N_hat, so the ETDRK combinations are unchanged;
+// the nonlinear coefficients absorb the remaining powers of two (exact).
 //
 // Semantics follow exponax 0.2.0: BaseStepper.step (_base_stepper.py:201-220),
 // ETDRK0-4 step_fourier (etdrk/_etdrk_0.py:30-34, _etdrk_1.py:77-82,
@@ -30,10 +36,9 @@ struct Step1DArgs {
   const float* tables;   // (lin_batch, lin_channels, ncoef, M+1)
   int lin_batch, lin_channels, ncoef;
   const cf* tw;          // exp(-2 pi i m / M), m < M
-  const cf* twr;         // exp(-2 pi i k / N), k <= M
+  const cf* fw;          // -i exp(-2 pi i k / N), k < M
   float kscale;          // 2 pi / L (float32, as the reference's derivative operator)
   int kmax;              // dealias: keep |k| <= kmax (kmax >= M: keep all)
-  float inv_n;           // 1/N
   float scale;           // b of convection / gradient norm
   float g0, g1, g2;      // general nonlinear scales (b0, b1, b2)
   int zero_mode_fix;
@@ -41,23 +46,34 @@ struct Step1DArgs {
   float poly[EXMI_MAX_POLY];
 };
 
-template <int M, int PT>
-struct Xf {
-  static constexpr int T = M / PT;
-  int j;
-  cf* lds;
-  const cf* tw;
-  const cf* twr;
-};
+constexpr int ncoef_of(int order) {
+  return order == 0 ? 1 : (order == 1 ? 2 : (order == 2 ? 3 : (order == 3 ? 7 : 6)));
+}
 
 // ----------------------------------------------------------------- helpers
+// Dealias mask classes of slot s (modes j + s*T, j < T) for a uniform kmax:
+// every mode kept, every mode dropped, or a per-lane boundary. The branches
+// are uniform across the wave, so fully kept or dropped slots cost no VALU
+// selects.
+template <int T>
+__device__ __forceinline__ bool slot_kept(int s, int kmax) { return (s + 1) * T - 1 <= kmax; }
+template <int T>
+__device__ __forceinline__ bool slot_dropped(int s, int kmax) { return s * T > kmax; }
+
+// Z = mask * S (dealias before an inverse transform)
 template <int M, int PT>
 __device__ __forceinline__ void masked_copy(const cf* S, cf nyq, cf* Z, cf& zn, int j, int kmax) {
   constexpr int T = M / PT;
 #pragma unroll
   for (int s = 0; s < PT; ++s) {
-    const int k = j + s * T;
-    Z[s] = (k <= kmax) ? S[s] : cf{0.f, 0.f};
+    if (slot_kept<T>(s, kmax)) {
+      Z[s] = S[s];
+    } else if (slot_dropped<T>(s, kmax)) {
+      Z[s] = cf{0.f, 0.f};
+    } else {
+      const int k = j + s * T;
+      Z[s] = (k <= kmax) ? S[s] : cf{0.f, 0.f};
+    }
   }
   zn = (M <= kmax) ? nyq : cf{0.f, 0.f};
 }
@@ -67,153 +83,367 @@ template <int M, int PT>
 __device__ __forceinline__ void masked_deriv(const cf* S, cf nyq, cf* Z, cf& zn, int j, int kmax,
                                              float kscale) {
   constexpr int T = M / PT;
+  const float kj = kscale * (float)j;
 #pragma unroll
   for (int s = 0; s < PT; ++s) {
-    const int k = j + s * T;
-    const float kap = kscale * (float)k;
-    Z[s] = (k <= kmax) ? cf{-kap * S[s].im, kap * S[s].re} : cf{0.f, 0.f};
+    if (slot_dropped<T>(s, kmax)) {
+      Z[s] = cf{0.f, 0.f};
+      continue;
+    }
+    const float kap = fmaf(kscale, (float)(s * T), kj);  // kscale * k, k = j + s*T
+    const cf z = cf{-kap * S[s].im, kap * S[s].re};
+    if (slot_kept<T>(s, kmax)) {
+      Z[s] = z;
+    } else {
+      const int k = j + s * T;
+      Z[s] = (k <= kmax) ? z : cf{0.f, 0.f};
+    }
   }
   const float kn = kscale * (float)M;
   zn = (M <= kmax) ? cf{-kn * nyq.im, kn * nyq.re} : cf{0.f, 0.f};
 }
 
-// ------------------------------------------------------ nonlinear functions
-// All return the dealiased Fourier-space nonlinear term in (Nn, nn).
-template <int KIND, int M, int PT>
-__device__ __forceinline__ void nl_eval(const cf* S, cf nyq, cf* Nn, cf& nn, int j, cf* lds,
-                                        const cf* __restrict__ tw, const cf* __restrict__ twr,
-                                        const Step1DArgs& a) {
+// Nn = mask * f * X for a real per-mode factor f(k) (dealias after a forward
+// transform). Fully dropped slots are already zero from rfft's kcut.
+template <int M, int PT, class F>
+__device__ __forceinline__ void masked_scale(const cf* X, cf* Nn, int j, int kmax, F f) {
   constexpr int T = M / PT;
-  if constexpr (KIND == NL_ZERO) {
-    // nonlin_fun/_zero.py:33-38
 #pragma unroll
-    for (int s = 0; s < PT; ++s) Nn[s] = cf{0.f, 0.f};
-    nn = cf{0.f, 0.f};
-  } else if constexpr (KIND == NL_CONV_CONS) {
-    // -b/2 (i kappa) F[u^2]  (nonlin_fun/_convection.py:104-138, :173-202)
-    cf Z[PT];
-    cf zn;
-    masked_copy<M, PT>(S, nyq, Z, zn, j, a.kmax);
-    irfft<M, PT>(Z, zn, j, lds, tw, twr, a.inv_n);
-#pragma unroll
-    for (int s = 0; s < PT; ++s) Z[s] = cf{Z[s].re * Z[s].re, Z[s].im * Z[s].im};
-    rfft<M, PT>(Z, zn, j, lds, tw, twr);
-    const float c = -a.scale * 0.5f * a.kscale;
-#pragma unroll
-    for (int s = 0; s < PT; ++s) {
+  for (int s = 0; s < PT; ++s) {
+    if (slot_dropped<T>(s, kmax)) {
+      Nn[s] = cf{0.f, 0.f};
+      continue;
+    }
+    const cf y = f(s, X[s]);
+    if (slot_kept<T>(s, kmax)) {
+      Nn[s] = y;
+    } else {
       const int k = j + s * T;
-      const float f = (k <= a.kmax) ? c * (float)k : 0.f;
-      Nn[s] = cf{-f * Z[s].im, f * Z[s].re};
-    }
-    const float fn = (M <= a.kmax) ? c * (float)M : 0.f;
-    nn = cf{-fn * zn.im, fn * zn.re};
-  } else if constexpr (KIND == NL_CONV_NONCONS) {
-    // -b F[u u_x]  (nonlin_fun/_convection.py:140-171, :204-231)
-    cf U[PT], V[PT];
-    cf un, vn;
-    masked_copy<M, PT>(S, nyq, U, un, j, a.kmax);
-    masked_deriv<M, PT>(S, nyq, V, vn, j, a.kmax, a.kscale);
-    irfft<M, PT>(U, un, j, lds, tw, twr, a.inv_n);
-    irfft<M, PT>(V, vn, j, lds, tw, twr, a.inv_n);
-#pragma unroll
-    for (int s = 0; s < PT; ++s) U[s] = cf{U[s].re * V[s].re, U[s].im * V[s].im};
-    rfft<M, PT>(U, un, j, lds, tw, twr);
-    const float c = -a.scale;
-#pragma unroll
-    for (int s = 0; s < PT; ++s) {
-      const int k = j + s * T;
-      const float f = (k <= a.kmax) ? c : 0.f;
-      Nn[s] = f * U[s];
-    }
-    nn = ((M <= a.kmax) ? c : 0.f) * un;
-  } else if constexpr (KIND == NL_GRADNORM) {
-    // -b/2 F[(u_x)^2 - mean]  (nonlin_fun/_gradient_norm.py:84-101); the mean
-    // removal equals zeroing the DC bin of the forward transform.
-    cf V[PT];
-    cf vn;
-    masked_deriv<M, PT>(S, nyq, V, vn, j, a.kmax, a.kscale);
-    irfft<M, PT>(V, vn, j, lds, tw, twr, a.inv_n);
-#pragma unroll
-    for (int s = 0; s < PT; ++s) V[s] = cf{V[s].re * V[s].re, V[s].im * V[s].im};
-    rfft<M, PT>(V, vn, j, lds, tw, twr);
-    const float c = -a.scale * 0.5f;
-#pragma unroll
-    for (int s = 0; s < PT; ++s) {
-      const int k = j + s * T;
-      float f = (k <= a.kmax) ? c : 0.f;
-      if (k == 0 && a.zero_mode_fix) f = 0.f;
-      Nn[s] = f * V[s];
-    }
-    nn = ((M <= a.kmax) ? c : 0.f) * vn;
-  } else if constexpr (KIND == NL_POLY) {
-    // F[sum_i c_i u^i]  (nonlin_fun/_polynomial.py:64-76), same power loop
-    cf U[PT];
-    cf un;
-    masked_copy<M, PT>(S, nyq, U, un, j, a.kmax);
-    irfft<M, PT>(U, un, j, lds, tw, twr, a.inv_n);
-#pragma unroll
-    for (int s = 0; s < PT; ++s) {
-      float pr = 1.f, pi = 1.f, ar = 0.f, ai = 0.f;
-      for (int i = 0; i < a.npoly; ++i) {
-        ar += a.poly[i] * pr;
-        ai += a.poly[i] * pi;
-        pr *= U[s].re;
-        pi *= U[s].im;
-      }
-      U[s] = cf{ar, ai};
-    }
-    rfft<M, PT>(U, un, j, lds, tw, twr);
-#pragma unroll
-    for (int s = 0; s < PT; ++s) {
-      const int k = j + s * T;
-      Nn[s] = (k <= a.kmax) ? U[s] : cf{0.f, 0.f};
-    }
-    nn = (M <= a.kmax) ? un : cf{0.f, 0.f};
-  } else if constexpr (KIND == NL_GENERAL) {
-    // b0 F[u^2] + (b1/2)(i kappa) F[u^2] + (b2/2) F[u_x^2 - mean]
-    // (nonlin_fun/_general_nonlinear.py:89-119 with the sub-functions'
-    // signs folded in; F is linear so the u^2 transform is shared)
-    cf U[PT], V[PT];
-    cf un, vn;
-    masked_copy<M, PT>(S, nyq, U, un, j, a.kmax);
-    masked_deriv<M, PT>(S, nyq, V, vn, j, a.kmax, a.kscale);
-    irfft<M, PT>(U, un, j, lds, tw, twr, a.inv_n);
-    irfft<M, PT>(V, vn, j, lds, tw, twr, a.inv_n);
-#pragma unroll
-    for (int s = 0; s < PT; ++s) {
-      U[s] = cf{U[s].re * U[s].re, U[s].im * U[s].im};
-      V[s] = cf{V[s].re * V[s].re, V[s].im * V[s].im};
-    }
-    rfft<M, PT>(U, un, j, lds, tw, twr);
-    rfft<M, PT>(V, vn, j, lds, tw, twr);
-    const float hb1 = 0.5f * a.g1 * a.kscale, hb2 = 0.5f * a.g2;
-#pragma unroll
-    for (int s = 0; s < PT; ++s) {
-      const int k = j + s * T;
-      const bool keep = k <= a.kmax;
-      const float ck = hb1 * (float)k;
-      const float cg = (k == 0 && a.zero_mode_fix) ? 0.f : hb2;
-      cf r = cf{a.g0 * U[s].re - ck * U[s].im + cg * V[s].re, a.g0 * U[s].im + ck * U[s].re + cg * V[s].im};
-      Nn[s] = keep ? r : cf{0.f, 0.f};
-    }
-    {
-      const float ck = hb1 * (float)M;
-      cf r = cf{a.g0 * un.re - ck * un.im + hb2 * vn.re, a.g0 * un.im + ck * un.re + hb2 * vn.im};
-      nn = (M <= a.kmax) ? r : cf{0.f, 0.f};
+      Nn[s] = (k <= kmax) ? y : cf{0.f, 0.f};
     }
   }
 }
 
-// --------------------------------------------------------------- the kernel
+// ------------------------------------------------------ nonlinear functions
+// Input: 2 u_hat (lane-strided + Nyquist) for NT transforms. Output: 2 N_hat,
+// dealiased. With the transform scalings, a product of two inverse
+// transforms of 2 u_hat comes back from the forward transform as
+// 8 N^2 F[product]; the factor q2 = 2 / (8 N^2) restores 2 N_hat (powers of
+// two, exact).
+template <int KIND, int M, int PT, int NT>
+__device__ __forceinline__ void nl_eval(const cf (*S)[PT], const cf* nyq, cf (*Nn)[PT], cf* nn,
+                                        const Lane<M, PT>& ln, cf* lds, const TwRegs<M, PT>& rt,
+                                        const Step1DArgs& a) {
+  constexpr int T = M / PT;
+  constexpr float NN = 2.f * M;
+  constexpr float q2 = 2.f / (8.f * NN * NN);
+  const int j = ln.j;
+  const int kc = a.kmax < M ? a.kmax : M;  // forward transforms are dealiased
+  if constexpr (KIND == NL_ZERO) {
+    // nonlin_fun/_zero.py:33-38
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+#pragma unroll
+      for (int s = 0; s < PT; ++s) Nn[n][s] = cf{0.f, 0.f};
+      nn[n] = cf{0.f, 0.f};
+    }
+  } else if constexpr (KIND == NL_CONV_CONS) {
+    // -b/2 (i kappa) F[u^2]  (nonlin_fun/_convection.py:104-138, :173-202)
+    cf Z[NT][PT];
+    cf zn[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) masked_copy<M, PT>(S[n], nyq[n], Z[n], zn[n], j, a.kmax);
+    irfft<M, PT, NT>(Z, zn, ln, lds, rt);
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+      for (int s = 0; s < PT; ++s) Z[n][s] = cf{Z[n][s].re * Z[n][s].re, Z[n][s].im * Z[n][s].im};
+    rfft<M, PT, NT>(Z, zn, ln, lds, rt, kc);
+    const float c = -a.scale * 0.5f * a.kscale * q2;
+    const float cj = c * (float)j;
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      masked_scale<M, PT>(Z[n], Nn[n], j, a.kmax, [&](int s, cf x) {
+        const float f = fmaf(c, (float)(s * T), cj);  // c * k
+        return cf{-f * x.im, f * x.re};
+      });
+      const float fn = (M <= a.kmax) ? c * (float)M : 0.f;
+      nn[n] = cf{-fn * zn[n].im, fn * zn[n].re};
+    }
+  } else if constexpr (KIND == NL_CONV_NONCONS) {
+    // -b F[u u_x]  (nonlin_fun/_convection.py:140-171, :204-231)
+    cf U[NT][PT], V[NT][PT];
+    cf un[NT], vn[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      masked_copy<M, PT>(S[n], nyq[n], U[n], un[n], j, a.kmax);
+      masked_deriv<M, PT>(S[n], nyq[n], V[n], vn[n], j, a.kmax, a.kscale);
+    }
+    irfft<M, PT, NT>(U, un, ln, lds, rt);
+    irfft<M, PT, NT>(V, vn, ln, lds, rt);
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+      for (int s = 0; s < PT; ++s) U[n][s] = cf{U[n][s].re * V[n][s].re, U[n][s].im * V[n][s].im};
+    rfft<M, PT, NT>(U, un, ln, lds, rt, kc);
+    const float c = -a.scale * q2;
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      masked_scale<M, PT>(U[n], Nn[n], j, a.kmax, [&](int, cf x) { return c * x; });
+      nn[n] = ((M <= a.kmax) ? c : 0.f) * un[n];
+    }
+  } else if constexpr (KIND == NL_GRADNORM) {
+    // -b/2 F[(u_x)^2 - mean]  (nonlin_fun/_gradient_norm.py:84-101); the mean
+    // removal equals zeroing the DC bin of the forward transform.
+    cf V[NT][PT];
+    cf vn[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) masked_deriv<M, PT>(S[n], nyq[n], V[n], vn[n], j, a.kmax, a.kscale);
+    irfft<M, PT, NT>(V, vn, ln, lds, rt);
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+      for (int s = 0; s < PT; ++s) V[n][s] = cf{V[n][s].re * V[n][s].re, V[n][s].im * V[n][s].im};
+    rfft<M, PT, NT>(V, vn, ln, lds, rt, kc);
+    const float c = -a.scale * 0.5f * q2;
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      masked_scale<M, PT>(V[n], Nn[n], j, a.kmax, [&](int, cf x) { return c * x; });
+      if (j == 0 && a.zero_mode_fix) Nn[n][0] = cf{0.f, 0.f};
+      nn[n] = ((M <= a.kmax) ? c : 0.f) * vn[n];
+    }
+  } else if constexpr (KIND == NL_POLY) {
+    // F[sum_i c_i u^i]  (nonlin_fun/_polynomial.py:64-76), same power loop;
+    // the inverse transform of 2 u_hat is 2 N u, rescaled first.
+    cf U[NT][PT];
+    cf un[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) masked_copy<M, PT>(S[n], nyq[n], U[n], un[n], j, a.kmax);
+    irfft<M, PT, NT>(U, un, ln, lds, rt);
+    constexpr float r2n = 1.f / (2.f * NN);
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+      for (int s = 0; s < PT; ++s) {
+        const float ur = U[n][s].re * r2n, ui = U[n][s].im * r2n;
+        float pr = 1.f, pi = 1.f, ar = 0.f, ai = 0.f;
+        for (int i = 0; i < a.npoly; ++i) {
+          ar += a.poly[i] * pr;
+          ai += a.poly[i] * pi;
+          pr *= ur;
+          pi *= ui;
+        }
+        U[n][s] = cf{ar, ai};
+      }
+    rfft<M, PT, NT>(U, un, ln, lds, rt, kc);
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      masked_scale<M, PT>(U[n], Nn[n], j, a.kmax, [&](int, cf x) { return x; });
+      nn[n] = (M <= a.kmax) ? un[n] : cf{0.f, 0.f};
+    }
+  } else if constexpr (KIND == NL_GENERAL) {
+    // b0 F[u^2] + (b1/2)(i kappa) F[u^2] + (b2/2) F[u_x^2 - mean]
+    // (nonlin_fun/_general_nonlinear.py:89-119 with the sub-functions'
+    // signs folded in; F is linear so the u^2 transform is shared)
+    cf U[NT][PT], V[NT][PT];
+    cf un[NT], vn[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      masked_copy<M, PT>(S[n], nyq[n], U[n], un[n], j, a.kmax);
+      masked_deriv<M, PT>(S[n], nyq[n], V[n], vn[n], j, a.kmax, a.kscale);
+    }
+    irfft<M, PT, NT>(U, un, ln, lds, rt);
+    irfft<M, PT, NT>(V, vn, ln, lds, rt);
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+      for (int s = 0; s < PT; ++s) {
+        U[n][s] = cf{U[n][s].re * U[n][s].re, U[n][s].im * U[n][s].im};
+        V[n][s] = cf{V[n][s].re * V[n][s].re, V[n][s].im * V[n][s].im};
+      }
+    rfft<M, PT, NT>(U, un, ln, lds, rt, kc);
+    rfft<M, PT, NT>(V, vn, ln, lds, rt, kc);
+    const float g0 = a.g0 * q2, hb1 = 0.5f * a.g1 * a.kscale * q2, hb2 = 0.5f * a.g2 * q2;
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+#pragma unroll
+      for (int s = 0; s < PT; ++s) {
+        const int k = j + s * T;
+        const bool keep = k <= a.kmax;
+        const float ck = hb1 * (float)k;
+        const float cg = (k == 0 && a.zero_mode_fix) ? 0.f : hb2;
+        const cf u = U[n][s], v = V[n][s];
+        cf r = cf{g0 * u.re - ck * u.im + cg * v.re, g0 * u.im + ck * u.re + cg * v.im};
+        Nn[n][s] = keep ? r : cf{0.f, 0.f};
+      }
+      const float ck = hb1 * (float)M;
+      const cf u = un[n], v = vn[n];
+      cf r = cf{g0 * u.re - ck * u.im + hb2 * v.re, g0 * u.im + ck * u.re + hb2 * v.im};
+      nn[n] = (M <= a.kmax) ? r : cf{0.f, 0.f};
+    }
+  }
+}
+
+// ------------------------------------------------------- operator tables
+// Accessors co(n, i, s): coefficient i of transform n at slot s (s = PT is
+// the Nyquist entry, meaningful in lane 0).
+template <int NCO, int PT, int NT>
+struct CoefRegs {
+  float v[NT][NCO][PT + 1];
+  __device__ __forceinline__ float operator()(int n, int i, int s) const { return v[n][i][s]; }
+  __device__ __forceinline__ void load(int n, const float* __restrict__ tb, int j, int T, int M) {
+#pragma unroll
+    for (int i = 0; i < NCO; ++i) {
+#pragma unroll
+      for (int s = 0; s < PT; ++s) v[n][i][s] = tb[i * (M + 1) + j + s * T];
+      v[n][i][PT] = tb[i * (M + 1) + M];
+    }
+  }
+};
+
+// Lane-private LDS copy (frees the registers between the table load and its
+// uses).
+template <int M, int PT, int NCO>
+struct CoefLds {
+  static constexpr int T = M / PT;
+  static constexpr int FLOATS = NCO * (PT + 1) * T;  // per transform
+  float* base;
+  int j;
+  __device__ __forceinline__ float operator()(int n, int i, int s) const {
+    return base[((n * NCO + i) * (PT + 1) + s) * T + j];
+  }
+  __device__ __forceinline__ void put(int n, int i, int s, float x) const {
+    base[((n * NCO + i) * (PT + 1) + s) * T + j] = x;
+  }
+};
+
+// ------------------------------------------------------- one ETDRK step
+// S: 2 u_hat of NT transforms (in/out). etdrk/_etdrk_0.py:30-34,
+// _etdrk_1.py:77-82, _etdrk_2.py:91-102, _etdrk_3.py:191-212,
+// _etdrk_4.py:198-224. Table order: ETDRK0 [E]; ETDRK1 [E, c1];
+// ETDRK2 [E, c1, c2]; ETDRK3 [E, E/2, c1..c5]; ETDRK4 [E, E/2, c1, c4, c5, c6].
+template <int KIND, int ORDER, int M, int PT, int NT, class Coef>
+__device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co, const Lane<M, PT>& ln, cf* lds,
+                                           const TwRegs<M, PT>& rt, const Step1DArgs& a) {
+#define EXMI_EACH(body)                   \
+  _Pragma("unroll") for (int n = 0; n < NT; ++n) { \
+    _Pragma("unroll") for (int s = 0; s < PT; ++s) { body; } }
+  if constexpr (ORDER == 0) {
+    EXMI_EACH(S[n][s] = co(n, 0, s) * S[n][s]);
+#pragma unroll
+    for (int n = 0; n < NT; ++n) nyq[n] = co(n, 0, PT) * nyq[n];
+  } else if constexpr (ORDER == 1) {
+    cf N0[NT][PT];
+    cf n0[NT];
+    nl_eval<KIND, M, PT, NT>(S, nyq, N0, n0, ln, lds, rt, a);
+    EXMI_EACH(S[n][s] = co(n, 0, s) * S[n][s] + co(n, 1, s) * N0[n][s]);
+#pragma unroll
+    for (int n = 0; n < NT; ++n) nyq[n] = co(n, 0, PT) * nyq[n] + co(n, 1, PT) * n0[n];
+  } else if constexpr (ORDER == 2) {
+    cf N0[NT][PT], N1[NT][PT];
+    cf n0[NT], n1[NT];
+    nl_eval<KIND, M, PT, NT>(S, nyq, N0, n0, ln, lds, rt, a);
+    EXMI_EACH(S[n][s] = co(n, 0, s) * S[n][s] + co(n, 1, s) * N0[n][s]);
+#pragma unroll
+    for (int n = 0; n < NT; ++n) nyq[n] = co(n, 0, PT) * nyq[n] + co(n, 1, PT) * n0[n];
+    nl_eval<KIND, M, PT, NT>(S, nyq, N1, n1, ln, lds, rt, a);
+    EXMI_EACH(S[n][s] = S[n][s] + co(n, 2, s) * (N1[n][s] - N0[n][s]));
+#pragma unroll
+    for (int n = 0; n < NT; ++n) nyq[n] = nyq[n] + co(n, 2, PT) * (n1[n] - n0[n]);
+  } else if constexpr (ORDER == 3) {
+    cf N0[NT][PT], Nk[NT][PT], A[NT][PT], U0[NT][PT];
+    cf n0[NT], nk[NT], an[NT], u0n[NT];
+    EXMI_EACH(U0[n][s] = S[n][s]);
+#pragma unroll
+    for (int n = 0; n < NT; ++n) u0n[n] = nyq[n];
+    nl_eval<KIND, M, PT, NT>(S, nyq, N0, n0, ln, lds, rt, a);
+    // accumulate the final combination E u + c3 N0 + c4 N1 + c5 N2
+    EXMI_EACH(A[n][s] = co(n, 0, s) * U0[n][s] + co(n, 4, s) * N0[n][s];
+              S[n][s] = co(n, 1, s) * U0[n][s] + co(n, 2, s) * N0[n][s]);  // stage 1
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      an[n] = co(n, 0, PT) * u0n[n] + co(n, 4, PT) * n0[n];
+      nyq[n] = co(n, 1, PT) * u0n[n] + co(n, 2, PT) * n0[n];
+    }
+    nl_eval<KIND, M, PT, NT>(S, nyq, Nk, nk, ln, lds, rt, a);  // N1
+    EXMI_EACH(A[n][s] = A[n][s] + co(n, 5, s) * Nk[n][s];
+              S[n][s] = co(n, 0, s) * U0[n][s] + co(n, 3, s) * (2.f * Nk[n][s] - N0[n][s]));  // stage 2
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      an[n] = an[n] + co(n, 5, PT) * nk[n];
+      nyq[n] = co(n, 0, PT) * u0n[n] + co(n, 3, PT) * (2.f * nk[n] - n0[n]);
+    }
+    nl_eval<KIND, M, PT, NT>(S, nyq, Nk, nk, ln, lds, rt, a);  // N2
+    EXMI_EACH(S[n][s] = A[n][s] + co(n, 6, s) * Nk[n][s]);
+#pragma unroll
+    for (int n = 0; n < NT; ++n) nyq[n] = an[n] + co(n, 6, PT) * nk[n];
+  } else {
+    cf N0[NT][PT], Nk[NT][PT], A[NT][PT], U0[NT][PT], S1[NT][PT];
+    cf n0[NT], nk[NT], an[NT], u0n[NT], s1n[NT];
+    EXMI_EACH(U0[n][s] = S[n][s]);
+#pragma unroll
+    for (int n = 0; n < NT; ++n) u0n[n] = nyq[n];
+    nl_eval<KIND, M, PT, NT>(S, nyq, N0, n0, ln, lds, rt, a);
+    EXMI_EACH(A[n][s] = co(n, 0, s) * U0[n][s] + co(n, 3, s) * N0[n][s];
+              S1[n][s] = co(n, 1, s) * U0[n][s] + co(n, 2, s) * N0[n][s]);
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      an[n] = co(n, 0, PT) * u0n[n] + co(n, 3, PT) * n0[n];
+      s1n[n] = co(n, 1, PT) * u0n[n] + co(n, 2, PT) * n0[n];
+    }
+    nl_eval<KIND, M, PT, NT>(S1, s1n, Nk, nk, ln, lds, rt, a);  // N1
+    EXMI_EACH(A[n][s] = A[n][s] + (co(n, 4, s) * 2.f) * Nk[n][s];
+              S[n][s] = co(n, 1, s) * U0[n][s] + co(n, 2, s) * Nk[n][s]);  // stage 2
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      an[n] = an[n] + (co(n, 4, PT) * 2.f) * nk[n];
+      nyq[n] = co(n, 1, PT) * u0n[n] + co(n, 2, PT) * nk[n];
+    }
+    nl_eval<KIND, M, PT, NT>(S, nyq, Nk, nk, ln, lds, rt, a);  // N2
+    EXMI_EACH(A[n][s] = A[n][s] + (co(n, 4, s) * 2.f) * Nk[n][s];
+              S[n][s] = co(n, 1, s) * S1[n][s] + co(n, 2, s) * (2.f * Nk[n][s] - N0[n][s]));  // stage 3
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      an[n] = an[n] + (co(n, 4, PT) * 2.f) * nk[n];
+      nyq[n] = co(n, 1, PT) * s1n[n] + co(n, 2, PT) * (2.f * nk[n] - n0[n]);
+    }
+    nl_eval<KIND, M, PT, NT>(S, nyq, Nk, nk, ln, lds, rt, a);  // N3
+    EXMI_EACH(S[n][s] = A[n][s] + co(n, 5, s) * Nk[n][s]);
+#pragma unroll
+    for (int n = 0; n < NT; ++n) nyq[n] = an[n] + co(n, 5, PT) * nk[n];
+  }
+#undef EXMI_EACH
+}
+
+// ------------------------------------------------------------- indexing
+// Transform n of a workgroup: G groups of T lanes, each group NT transforms.
+template <int M, int PT, int NT>
+struct Slots {
+  static constexpr int T = M / PT;
+  static constexpr int WG = T > 64 ? T : 64;
+  static constexpr int G = WG / T;
+  static constexpr int PER_BLOCK = G * NT;
+};
+
+// --------------------------------------------------------------- kernels
+#ifndef EXMI_STEP_WAVES_PER_EU
+#define EXMI_STEP_WAVES_PER_EU 1
+#endif
+
+// One step (or one nonlinear evaluation) of every transform of the batch.
+// Tables are staged through lane-private LDS to keep register pressure low.
 template <int M, int PT, int KIND, int ORDER>
 __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
-    step1d_kernel(Step1DArgs a) {
+    __attribute__((amdgpu_waves_per_eu(EXMI_STEP_WAVES_PER_EU))) step1d_kernel(Step1DArgs a) {
   constexpr int T = M / PT;
-  constexpr int WG = T > 64 ? T : 64;
-  constexpr int G = WG / T;
+  constexpr int G = Slots<M, PT, 1>::G;
   constexpr int LDSN = LdsSize<M>::value;
   constexpr int NR = M + 1;
+  constexpr int NCO = ncoef_of(ORDER);
+  using CL = CoefLds<M, PT, NCO>;
+  constexpr float NN = 2.f * M;
   __shared__ cf lds_all[G * LDSN];
+  __shared__ float lds_coef[G * CL::FLOATS];
 
   const int g = threadIdx.x / T;
   const int j = threadIdx.x % T;
@@ -221,170 +451,183 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
   const bool active = t_raw < a.ntrans;
   const int t = active ? t_raw : 0;
   cf* lds = lds_all + g * LDSN;
+  const CL cl{lds_coef + g * CL::FLOATS, j};
   const int b = t / a.channels, c = t % a.channels;
 
-  cf S[PT];
-  cf nyq = cf{0.f, 0.f};
+  const Lane<M, PT> ln(j);
+  TwRegs<M, PT> rt;
+  rt.load(j, RealTables{a.tw, a.fw});
+  cf S[1][PT];
+  cf nyq[1] = {cf{0.f, 0.f}};
 
-  // ---- load
+  // ---- load the state (as 2 u_hat when spectral)
   if (a.in_fourier) {
     const cf* in = reinterpret_cast<const cf*>(a.in) + (long long)b * a.in_bstride + (long long)c * NR;
 #pragma unroll
-    for (int s = 0; s < PT; ++s) S[s] = in[j + s * T];
-    nyq = in[M];
+    for (int s = 0; s < PT; ++s) S[0][s] = 2.f * in[j + s * T];
+    nyq[0] = 2.f * in[M];
   } else {
     const float2* in = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(a.in) +
                                                        (long long)b * a.in_bstride + (long long)c * 2 * M);
 #pragma unroll
     for (int s = 0; s < PT; ++s) {
       float2 v = in[j + s * T];
-      S[s] = cf{v.x, v.y};
+      S[0][s] = cf{v.x, v.y};
     }
   }
 
-  // operator tables for this transform
-  const float* tb = a.tables + ((long long)(a.lin_batch > 1 ? b : 0) * a.lin_channels +
-                                (a.lin_channels > 1 ? c : 0)) * (long long)a.ncoef * NR;
-  float co[ORDER == 0 ? 1 : (ORDER == 1 ? 2 : (ORDER == 2 ? 3 : (ORDER == 3 ? 7 : 6)))][PT];
-  float con[ORDER == 0 ? 1 : (ORDER == 1 ? 2 : (ORDER == 2 ? 3 : (ORDER == 3 ? 7 : 6)))];
-  constexpr int NCO = ORDER == 0 ? 1 : (ORDER == 1 ? 2 : (ORDER == 2 ? 3 : (ORDER == 3 ? 7 : 6)));
+  // ---- operator tables: HBM -> registers -> lane-private LDS
   if (!a.nonlin_only) {
+    const float* tb = a.tables + ((long long)(a.lin_batch > 1 ? b : 0) * a.lin_channels +
+                                  (a.lin_channels > 1 ? c : 0)) * (long long)NCO * NR;
+    CoefRegs<NCO, PT, 1> cr;
+    cr.load(0, tb, j, T, M);
+    if (!a.in_fourier) rfft<M, PT, 1>(S, nyq, ln, lds, rt);
 #pragma unroll
-    for (int i = 0; i < NCO; ++i) {
+    for (int i = 0; i < NCO; ++i)
 #pragma unroll
-      for (int s = 0; s < PT; ++s) co[i][s] = tb[i * NR + j + s * T];
-      con[i] = tb[i * NR + M];
-    }
+      for (int s = 0; s <= PT; ++s) cl.put(0, i, s, cr(0, i, s));
+  } else if (!a.in_fourier) {
+    rfft<M, PT, 1>(S, nyq, ln, lds, rt);
   }
-
-  if (!a.in_fourier) rfft<M, PT>(S, nyq, j, lds, a.tw, a.twr);
 
   if (a.nonlin_only) {
-    cf Nn[PT];
-    cf nn;
-    nl_eval<KIND, M, PT>(S, nyq, Nn, nn, j, lds, a.tw, a.twr, a);
+    cf Nn[1][PT];
+    cf nn[1];
+    nl_eval<KIND, M, PT, 1>(S, nyq, Nn, nn, ln, lds, rt, a);
 #pragma unroll
-    for (int s = 0; s < PT; ++s) S[s] = Nn[s];
-    nyq = nn;
-  } else if constexpr (ORDER == 0) {
-    // etdrk/_etdrk_0.py:30-34
-#pragma unroll
-    for (int s = 0; s < PT; ++s) S[s] = co[0][s] * S[s];
-    nyq = con[0] * nyq;
-  } else if constexpr (ORDER == 1) {
-    // etdrk/_etdrk_1.py:77-82
-    cf N0[PT];
-    cf n0;
-    nl_eval<KIND, M, PT>(S, nyq, N0, n0, j, lds, a.tw, a.twr, a);
-#pragma unroll
-    for (int s = 0; s < PT; ++s) S[s] = co[0][s] * S[s] + co[1][s] * N0[s];
-    nyq = con[0] * nyq + con[1] * n0;
-  } else if constexpr (ORDER == 2) {
-    // etdrk/_etdrk_2.py:91-102
-    cf N0[PT], N1[PT];
-    cf n0, n1;
-    nl_eval<KIND, M, PT>(S, nyq, N0, n0, j, lds, a.tw, a.twr, a);
-#pragma unroll
-    for (int s = 0; s < PT; ++s) S[s] = co[0][s] * S[s] + co[1][s] * N0[s];
-    nyq = con[0] * nyq + con[1] * n0;
-    nl_eval<KIND, M, PT>(S, nyq, N1, n1, j, lds, a.tw, a.twr, a);
-#pragma unroll
-    for (int s = 0; s < PT; ++s) S[s] = S[s] + co[2][s] * (N1[s] - N0[s]);
-    nyq = nyq + con[2] * (n1 - n0);
-  } else if constexpr (ORDER == 3) {
-    // etdrk/_etdrk_3.py:191-212; tables: E, E/2, c1..c5
-    cf N0[PT], Nk[PT], A[PT], U0[PT];
-    cf n0, nk, an, u0n;
-#pragma unroll
-    for (int s = 0; s < PT; ++s) U0[s] = S[s];
-    u0n = nyq;
-    nl_eval<KIND, M, PT>(S, nyq, N0, n0, j, lds, a.tw, a.twr, a);
-    // accumulate the final combination E u + c3 N0 + c4 N1 + c5 N2
-#pragma unroll
-    for (int s = 0; s < PT; ++s) {
-      A[s] = co[0][s] * U0[s] + co[4][s] * N0[s];
-      S[s] = co[1][s] * U0[s] + co[2][s] * N0[s];  // stage 1
-    }
-    an = con[0] * u0n + con[4] * n0;
-    nyq = con[1] * u0n + con[2] * n0;
-    nl_eval<KIND, M, PT>(S, nyq, Nk, nk, j, lds, a.tw, a.twr, a);  // N1
-#pragma unroll
-    for (int s = 0; s < PT; ++s) {
-      A[s] = A[s] + co[5][s] * Nk[s];
-      S[s] = co[0][s] * U0[s] + co[3][s] * (2.f * Nk[s] - N0[s]);  // stage 2
-    }
-    an = an + con[5] * nk;
-    nyq = con[0] * u0n + con[3] * (2.f * nk - n0);
-    nl_eval<KIND, M, PT>(S, nyq, Nk, nk, j, lds, a.tw, a.twr, a);  // N2
-#pragma unroll
-    for (int s = 0; s < PT; ++s) S[s] = A[s] + co[6][s] * Nk[s];
-    nyq = an + con[6] * nk;
+    for (int s = 0; s < PT; ++s) S[0][s] = Nn[0][s];
+    nyq[0] = nn[0];
   } else {
-    // etdrk/_etdrk_4.py:198-224; tables: E, E/2, c1 (= c2 = c3), c4, c5, c6
-    cf N0[PT], Nk[PT], A[PT], U0[PT], S1[PT];
-    cf n0, nk, an, u0n, s1n;
-#pragma unroll
-    for (int s = 0; s < PT; ++s) U0[s] = S[s];
-    u0n = nyq;
-    nl_eval<KIND, M, PT>(S, nyq, N0, n0, j, lds, a.tw, a.twr, a);
-#pragma unroll
-    for (int s = 0; s < PT; ++s) {
-      A[s] = co[0][s] * U0[s] + co[3][s] * N0[s];
-      S1[s] = co[1][s] * U0[s] + co[2][s] * N0[s];
-    }
-    an = con[0] * u0n + con[3] * n0;
-    s1n = con[1] * u0n + con[2] * n0;
-    nl_eval<KIND, M, PT>(S1, s1n, Nk, nk, j, lds, a.tw, a.twr, a);  // N1
-#pragma unroll
-    for (int s = 0; s < PT; ++s) {
-      A[s] = A[s] + (co[4][s] * 2.f) * Nk[s];
-      S[s] = co[1][s] * U0[s] + co[2][s] * Nk[s];  // stage 2
-    }
-    an = an + (con[4] * 2.f) * nk;
-    nyq = con[1] * u0n + con[2] * nk;
-    nl_eval<KIND, M, PT>(S, nyq, Nk, nk, j, lds, a.tw, a.twr, a);  // N2
-#pragma unroll
-    for (int s = 0; s < PT; ++s) {
-      A[s] = A[s] + (co[4][s] * 2.f) * Nk[s];
-      S[s] = co[1][s] * S1[s] + co[2][s] * (2.f * Nk[s] - N0[s]);  // stage 3
-    }
-    an = an + (con[4] * 2.f) * nk;
-    nyq = con[1] * s1n + con[2] * (2.f * nk - n0);
-    nl_eval<KIND, M, PT>(S, nyq, Nk, nk, j, lds, a.tw, a.twr, a);  // N3
-#pragma unroll
-    for (int s = 0; s < PT; ++s) S[s] = A[s] + co[5][s] * Nk[s];
-    nyq = an + con[5] * nk;
+    etdrk_step<KIND, ORDER, M, PT, 1>(S, nyq, cl, ln, lds, rt, a);
   }
 
-  // ---- store
+  // ---- store (2 u_hat -> u_hat, or N * 2 u -> u)
   if (a.out_fourier) {
     if (active) {
       cf* out = reinterpret_cast<cf*>(a.out) + (long long)b * a.out_bstride + (long long)c * NR;
 #pragma unroll
-      for (int s = 0; s < PT; ++s) out[j + s * T] = S[s];
-      if (j == 0) out[M] = nyq;
+      for (int s = 0; s < PT; ++s) out[j + s * T] = 0.5f * S[0][s];
+      if (j == 0) out[M] = 0.5f * nyq[0];
     }
   } else {
-    irfft<M, PT>(S, nyq, j, lds, a.tw, a.twr, a.inv_n);
+    irfft<M, PT, 1>(S, nyq, ln, lds, rt);
     if (active) {
+      constexpr float r2n = 1.f / (2.f * NN);
       float2* out = reinterpret_cast<float2*>(reinterpret_cast<float*>(a.out) + (long long)b * a.out_bstride +
                                               (long long)c * 2 * M);
 #pragma unroll
-      for (int s = 0; s < PT; ++s) out[j + s * T] = make_float2(S[s].re, S[s].im);
+      for (int s = 0; s < PT; ++s) out[j + s * T] = make_float2(S[0][s].re * r2n, S[0][s].im * r2n);
     }
   }
 }
 
-// Lane-0 Nyquist broadcast is not needed: only lane 0 (k = 0 slot) reads and
-// writes `nyq`; other lanes carry a dead copy.
-
 template <int M, int PT, int KIND, int ORDER>
 inline hipError_t launch_step1d(const Step1DArgs& a, hipStream_t st) {
-  constexpr int T = M / PT;
-  constexpr int WG = T > 64 ? T : 64;
-  constexpr int G = WG / T;
+  constexpr int WG = Slots<M, PT, 1>::WG;
+  constexpr int G = Slots<M, PT, 1>::G;
   const int grid = (a.ntrans + G - 1) / G;
   hipLaunchKernelGGL((step1d_kernel<M, PT, KIND, ORDER>), dim3(grid), dim3(WG), 0, st, a);
+  return hipGetLastError();
+}
+
+// Multi-step rollout: each group of lanes keeps NT transforms' states,
+// operator tables and twiddles on chip for `nsteps` steps. Per step the
+// arithmetic is exactly that of step1d_kernel (physical state in float32 ->
+// forward transform -> ETDRK step -> inverse transform -> float32 state);
+// the new state is written to traj[b, t0 + step] (or only the final state
+// when `final_only`). Every wave runs the same bounded loop; there is no
+// inter-wave communication.
+struct Roll1DArgs {
+  Step1DArgs s;            // in, in_bstride: initial states (physical)
+  float* traj;             // output base
+  long long traj_bstride;  // floats between trajectories
+  long long traj_tstride;  // floats between consecutive outputs of one trajectory
+  int nsteps;
+  int final_only;          // 1: write only the last state (traj_tstride unused)
+};
+
+template <int M, int PT, int KIND, int ORDER, int NT>
+__global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(Roll1DArgs ra) {
+  const Step1DArgs& a = ra.s;
+  constexpr int T = M / PT;
+  constexpr int G = Slots<M, PT, NT>::G;
+  constexpr int LDSN = LdsSize<M>::value;
+  constexpr int NR = M + 1;
+  constexpr int NCO = ncoef_of(ORDER);
+  using CL = CoefLds<M, PT, NCO>;
+  constexpr float NN = 2.f * M;
+  constexpr float r2n = 1.f / (2.f * NN);
+  __shared__ cf lds_all[G * LDSN];
+  __shared__ float lds_coef[G * NT * CL::FLOATS];
+
+  const int g = threadIdx.x / T;
+  const int j = threadIdx.x % T;
+  cf* lds = lds_all + g * LDSN;
+  const CL cl{lds_coef + g * NT * CL::FLOATS, j};
+  const Lane<M, PT> ln(j);
+  TwRegs<M, PT> rt;
+  rt.load(j, RealTables{a.tw, a.fw});
+
+  int tb_[NT], cb_[NT];
+  bool act[NT];
+  cf S[NT][PT];
+  cf nyq[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    const int t_raw = (blockIdx.x * G + g) * NT + n;
+    act[n] = t_raw < a.ntrans;
+    const int t = act[n] ? t_raw : 0;
+    tb_[n] = t / a.channels;
+    cb_[n] = t % a.channels;
+    const float2* in = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(a.in) +
+                                                       (long long)tb_[n] * a.in_bstride + (long long)cb_[n] * 2 * M);
+#pragma unroll
+    for (int s = 0; s < PT; ++s) {
+      float2 v = in[j + s * T];
+      S[n][s] = cf{v.x, v.y};
+    }
+    const float* tb = a.tables + ((long long)(a.lin_batch > 1 ? tb_[n] : 0) * a.lin_channels +
+                                  (a.lin_channels > 1 ? cb_[n] : 0)) * (long long)NCO * NR;
+#pragma unroll
+    for (int i = 0; i < NCO; ++i) {
+#pragma unroll
+      for (int s = 0; s < PT; ++s) cl.put(n, i, s, tb[i * NR + j + s * T]);
+      cl.put(n, i, PT, tb[i * NR + M]);
+    }
+  }
+
+  for (int step = 0; step < ra.nsteps; ++step) {
+    rfft<M, PT, NT>(S, nyq, ln, lds, rt);
+    etdrk_step<KIND, ORDER, M, PT, NT>(S, nyq, cl, ln, lds, rt, a);
+    irfft<M, PT, NT>(S, nyq, ln, lds, rt);
+    // back to the float32 physical state, exactly as a stored-and-reloaded
+    // state of the single-step path
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+      for (int s = 0; s < PT; ++s) S[n][s] = cf{S[n][s].re * r2n, S[n][s].im * r2n};
+    if (!ra.final_only || step == ra.nsteps - 1) {
+      const long long tofs = ra.final_only ? 0 : (long long)step * ra.traj_tstride;
+#pragma unroll
+      for (int n = 0; n < NT; ++n) {
+        if (act[n]) {
+          float2* out = reinterpret_cast<float2*>(ra.traj + (long long)tb_[n] * ra.traj_bstride + tofs +
+                                                  (long long)cb_[n] * 2 * M);
+#pragma unroll
+          for (int s = 0; s < PT; ++s) out[j + s * T] = make_float2(S[n][s].re, S[n][s].im);
+        }
+      }
+    }
+  }
+}
+
+template <int M, int PT, int KIND, int ORDER, int NT>
+inline hipError_t launch_roll1d(const Roll1DArgs& ra, hipStream_t st) {
+  constexpr int WG = Slots<M, PT, NT>::WG;
+  constexpr int PB = Slots<M, PT, NT>::PER_BLOCK;
+  const int grid = (ra.s.ntrans + PB - 1) / PB;
+  hipLaunchKernelGGL((roll1d_kernel<M, PT, KIND, ORDER, NT>), dim3(grid), dim3(WG), 0, st, ra);
   return hipGetLastError();
 }
 

@@ -14,8 +14,7 @@ using StepFn = hipError_t (*)(const Step1DArgs&, hipStream_t);
 
 template <int M, int PT, bool INV>
 __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
-    fft1d_kernel(const float* __restrict__ in, float* __restrict__ out, int count, float inv_n, const cf* tw,
-                 const cf* twr) {
+    fft1d_kernel(const float* __restrict__ in, float* __restrict__ out, int count, RealTables rt) {
   constexpr int T = M / PT;
   constexpr int WG = T > 64 ? T : 64;
   constexpr int G = WG / T;
@@ -26,32 +25,36 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
   const bool active = t_raw < count;
   const int t = active ? t_raw : 0;
   cf* lds = lds_all + g * LDSN;
-  cf v[PT];
-  cf nyq = cf{0.f, 0.f};
+  const Lane<M, PT> ln(j);
+  TwRegs<M, PT> tr;
+  tr.load(j, rt);
+  cf v[1][PT];
+  cf nyq[1] = {cf{0.f, 0.f}};
   if (!INV) {
     const float2* x = reinterpret_cast<const float2*>(in + (long long)t * 2 * M);
 #pragma unroll
     for (int s = 0; s < PT; ++s) {
       float2 q = x[j + s * T];
-      v[s] = cf{q.x, q.y};
+      v[0][s] = cf{q.x, q.y};
     }
-    rfft<M, PT>(v, nyq, j, lds, tw, twr);
+    rfft<M, PT, 1>(v, nyq, ln, lds, tr);  // 2 X
     if (active) {
       cf* y = reinterpret_cast<cf*>(out) + (long long)t * (M + 1);
 #pragma unroll
-      for (int s = 0; s < PT; ++s) y[j + s * T] = v[s];
-      if (j == 0) y[M] = nyq;
+      for (int s = 0; s < PT; ++s) y[j + s * T] = 0.5f * v[0][s];
+      if (j == 0) y[M] = 0.5f * nyq[0];
     }
   } else {
     const cf* x = reinterpret_cast<const cf*>(in) + (long long)t * (M + 1);
 #pragma unroll
-    for (int s = 0; s < PT; ++s) v[s] = x[j + s * T];
-    nyq = x[M];
-    irfft<M, PT>(v, nyq, j, lds, tw, twr, inv_n);
+    for (int s = 0; s < PT; ++s) v[0][s] = x[j + s * T];
+    nyq[0] = x[M];
+    irfft<M, PT, 1>(v, nyq, ln, lds, tr);  // N x
     if (active) {
+      constexpr float rn = 1.f / (2.f * M);
       float2* y = reinterpret_cast<float2*>(out + (long long)t * 2 * M);
 #pragma unroll
-      for (int s = 0; s < PT; ++s) y[j + s * T] = make_float2(v[s].re, v[s].im);
+      for (int s = 0; s < PT; ++s) y[j + s * T] = make_float2(v[0][s].re * rn, v[0][s].im * rn);
     }
   }
 }
@@ -87,17 +90,54 @@ StepFn EXMI_CAT(get_step1d_, EXMI_M)(int kind, int order) {
 // Nonlinear-only evaluation shares the ORDER-1 instantiation's register plan;
 // it is dispatched through the same kernel with nonlin_only = 1.
 
-hipError_t EXMI_CAT(launch_fft1d_, EXMI_M)(bool inverse, const float* in, float* out, int count, float inv_n,
-                                           const cf* tw, const cf* twr, hipStream_t st) {
+using RollFn = hipError_t (*)(const Roll1DArgs&, hipStream_t);
+
+template <int M, int PT, int KIND, int NT>
+static RollFn pick_roll_order(int order) {
+  switch (order) {
+    case 1: return &launch_roll1d<M, PT, KIND, 1, NT>;
+    case 2: return &launch_roll1d<M, PT, KIND, 2, NT>;
+    case 3: return &launch_roll1d<M, PT, KIND, 3, NT>;
+    case 4: return &launch_roll1d<M, PT, KIND, 4, NT>;
+    default: return nullptr;
+  }
+}
+
+template <int M, int PT, int NT>
+static RollFn pick_roll(int kind, int order) {
+  if (order == 0) return &launch_roll1d<M, PT, NL_ZERO, 0, NT>;
+  switch (kind) {
+    case NL_ZERO: return pick_roll_order<M, PT, NL_ZERO, NT>(order);
+    case NL_CONV_CONS: return pick_roll_order<M, PT, NL_CONV_CONS, NT>(order);
+    case NL_CONV_NONCONS: return pick_roll_order<M, PT, NL_CONV_NONCONS, NT>(order);
+    case NL_GRADNORM: return pick_roll_order<M, PT, NL_GRADNORM, NT>(order);
+    case NL_POLY: return pick_roll_order<M, PT, NL_POLY, NT>(order);
+    case NL_GENERAL: return pick_roll_order<M, PT, NL_GENERAL, NT>(order);
+    default: return nullptr;
+  }
+}
+
+// Multi-step kernels carrying 1 or 2 transforms per lane group.
+RollFn EXMI_CAT(get_roll1d_, EXMI_M)(int kind, int order, int nt) {
+  constexpr int M = EXMI_M, PT = EXMI_PT;
+  if (nt == 1) return pick_roll<M, PT, 1>(kind, order);
+  if constexpr (M / PT <= 64) {
+    if (nt == 2) return pick_roll<M, PT, 2>(kind, order);
+  }
+  return nullptr;
+}
+
+hipError_t EXMI_CAT(launch_fft1d_, EXMI_M)(bool inverse, const float* in, float* out, int count,
+                                           const RealTables& rt, hipStream_t st) {
   constexpr int M = EXMI_M, PT = EXMI_PT;
   constexpr int T = M / PT;
   constexpr int WG = T > 64 ? T : 64;
   constexpr int G = WG / T;
   const int grid = (count + G - 1) / G;
   if (inverse)
-    hipLaunchKernelGGL((fft1d_kernel<M, PT, true>), dim3(grid), dim3(WG), 0, st, in, out, count, inv_n, tw, twr);
+    hipLaunchKernelGGL((fft1d_kernel<M, PT, true>), dim3(grid), dim3(WG), 0, st, in, out, count, rt);
   else
-    hipLaunchKernelGGL((fft1d_kernel<M, PT, false>), dim3(grid), dim3(WG), 0, st, in, out, count, inv_n, tw, twr);
+    hipLaunchKernelGGL((fft1d_kernel<M, PT, false>), dim3(grid), dim3(WG), 0, st, in, out, count, rt);
   return hipGetLastError();
 }
 

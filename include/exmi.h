@@ -47,6 +47,16 @@ enum exmi_nonlin_kind {
 
 #define EXMI_MAX_POLY 8
 
+/* Plan flags.
+ * EXMI_FLAG_EXACT_FORM: evaluate the non-conservative convection u u_x with
+ * its two inverse transforms as the reference does
+ * (nonlin_fun/_convection.py:140-171), even when the dealiasing is alias-free
+ * for quadratic products (3 * kmax < N). Without the flag such plans evaluate
+ * (1/2) d/dx (u^2) instead: on every retained mode both are the same exact
+ * (alias-free) convolution, so only the float32 rounding differs, and one
+ * inverse transform per nonlinear evaluation is saved. */
+#define EXMI_FLAG_EXACT_FORM 1
+
 /*
  * Stepper description: the constructor arguments of BaseStepper
  * (_base_stepper.py:27-157) plus the nonlinear function's own arguments and
@@ -71,6 +81,7 @@ typedef struct exmi_plan_desc {
   int32_t lin_batch;          /* 1 or batch                                  */
   int32_t lin_channels;       /* 1 or num_channels                           */
   int32_t num_circle_points;  /* contour points (reference default 16)       */
+  int32_t flags;              /* EXMI_FLAG_*                                 */
   double domain_extent;       /* L                                           */
   double dt;
   double circle_radius;       /* contour radius (reference default 1.0)      */
@@ -126,6 +137,13 @@ int exmi_nonlinear(exmi_plan* plan, const float* uh_in, int64_t in_bstride, floa
 int exmi_rollout(exmi_plan* plan, const float* u0, int64_t u0_bstride, float* traj, int32_t n_steps,
                  int32_t include_init, int32_t use_graph, void* stream);
 
+/* Capture and instantiate the rollout graph for these arguments without
+ * launching it, so that a later exmi_rollout(..., use_graph=1) with the same
+ * arguments only replays. Synchronous on the host; enqueues no GPU work.
+ * (No reference counterpart: jax.jit/lax.scan compile on first call.) */
+int exmi_rollout_prepare(exmi_plan* plan, const float* u0, int64_t u0_bstride, float* traj, int32_t n_steps,
+                         int32_t include_init);
+
 /* Final state after n_steps: u_out (B, C, N^D) contiguous.
  * Replaces exponax.repeat (_utils.py:189-254, takes_aux=False). */
 int exmi_repeat(exmi_plan* plan, const float* u0, int64_t u0_bstride, float* u_out, int32_t n_steps,
@@ -148,6 +166,15 @@ int exmi_irfft(int32_t num_spatial_dims, int32_t num_points, int32_t count, cons
  * for inspection. Synchronous. */
 int exmi_plan_num_coefs(const exmi_plan* plan);
 int exmi_plan_get_tables(const exmi_plan* plan, float* host_out, int64_t host_count);
+
+/* Rollout strategy of a plan: -1 = one launch per step (each a fused step
+ * kernel), 1 or 2 = one multi-step launch keeping 1 or 2 transforms per lane
+ * group on chip for all steps (same per-step arithmetic), 0 = not chosen
+ * yet (the first exmi_rollout / exmi_rollout_prepare times the candidates on
+ * a 4-step probe and keeps the fastest). Setting a strategy drops a cached
+ * graph. (No reference counterpart: tuning knobs of this implementation.) */
+int exmi_plan_rollout_strategy(const exmi_plan* plan);
+int exmi_plan_set_rollout_strategy(exmi_plan* plan, int32_t strategy);
 
 /* Message for the last error on this thread ("" if none). */
 const char* exmi_last_error(void);

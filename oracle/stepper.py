@@ -279,3 +279,34 @@ def nrmse(u_pred, u_ref):
     num = np.sqrt(np.sum((a - b).reshape(c, -1) ** 2, axis=1))
     den = np.sqrt(np.sum(b.reshape(c, -1) ** 2, axis=1))
     return float(np.sum(num / den))
+
+
+class Ensemble:
+    """Members that differ only in their linear operator, advanced together:
+    the vmap over stepper constructors of the reference
+    (docs/examples/performance_hints.ipynb cells 23-31). The members' linear
+    operators are stacked to (B, C, ...) and broadcast against (B, C, ...)
+    states; the nonlinear function is shared."""
+
+    def __init__(self, members, *, dtype=np.float64, coef_dtype=np.float64, order=2, num_circle_points=16,
+                 circle_radius=1.0):
+        m0 = members[0]
+        self.num_spatial_dims = m0.num_spatial_dims
+        self.num_points = m0.num_points
+        self.num_channels = m0.num_channels
+        self.dtype = np.dtype(dtype)
+        lin = np.stack([m.linear_operator for m in members])
+        kw = dict(dtype=dtype, coef_dtype=coef_dtype)
+        ckw = dict(num_circle_points=num_circle_points, circle_radius=circle_radius)
+        cls = {0: et.ETDRK0, 1: et.ETDRK1, 2: et.ETDRK2, 3: et.ETDRK3, 4: et.ETDRK4}[order]
+        if order == 0:
+            self._integrator = cls(m0.dt, lin, **kw)
+        else:
+            self._integrator = cls(m0.dt, lin, m0.nonlinear_fun, **ckw, **kw)
+
+    def step_batch(self, ub):
+        ub = np.asarray(ub, dtype=self.dtype)
+        uh = sp.fft(ub, num_spatial_dims=self.num_spatial_dims).astype(sp.complex_of(self.dtype))
+        nxt = self._integrator.step_fourier(uh)
+        return sp.ifft(nxt, num_spatial_dims=self.num_spatial_dims, num_points=self.num_points).astype(
+            self.dtype, copy=False)

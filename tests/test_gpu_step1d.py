@@ -103,3 +103,49 @@ def test_shape_error(device):
     s = ex.stepper.Burgers(1, 1.0, 64, 0.01)
     with pytest.raises(ValueError, match="Expected shape"):
         s(np.zeros((2, 64), np.float32))
+
+
+@pytest.mark.parametrize("N", [64, 256, 1024, 2048])
+@pytest.mark.parametrize("kind", ["ks_cons", "burgers"])
+@pytest.mark.parametrize("order", [2, 4])
+def test_rollout_strategies_agree(device, kind, N, order):
+    """The multi-step rollout kernels (1 or 2 transforms per lane group) and
+    the per-step launches compute the same steps; all match the oracle
+    rollout (reference _utils.py:92-186) at steps 1 and 10."""
+    import torch
+
+    print(f"[case] rollout kind={kind} N={N} order={order}", flush=True)
+    gpu, cpu, L = _pair(kind, N, order)
+    B, n = 5, 12
+    u0, _ = ics(B, 1, L, N, 5, seed=21)
+    x = torch.tensor(u0, device=device)
+    plan = gpu.plan(B)
+    outs = {}
+    for strat in (-1, 1, 2):
+        try:
+            plan.set_rollout_strategy(strat)
+        except NotImplementedError:
+            continue
+        outs[strat] = gpu.rollout_batch(x, n, include_init=True).cpu().numpy()
+    ref = outs[-1]
+    for strat, tr in outs.items():
+        assert tr.shape == (B, n + 1, 1, N)
+        np.testing.assert_array_equal(tr[:, 0], u0)
+        assert max_rel_l2(tr[:, 1:].reshape(B * n, 1, N), ref[:, 1:].reshape(B * n, 1, N)) <= 1e-6, strat
+    u = u0
+    for s in range(1, 11):
+        u = cpu.step_batch(u)
+        if s in (1, 10):
+            for tr in outs.values():
+                assert max_rel_l2(tr[:, s], u) <= RTOL_STEP
+
+
+def test_repeat_matches_rollout_last(device):
+    import torch
+
+    gpu, _, L = _pair("ks_cons", 256, 2)
+    u0, _ = ics(3, 1, L, 256, 5, seed=5)
+    x = torch.tensor(u0, device=device)
+    tr = ex.rollout(ex.vmap(gpu), 7)(x)
+    last = ex.repeat(ex.vmap(gpu), 7)(x)
+    assert max_rel_l2(last.cpu().numpy(), tr[:, -1].cpu().numpy()) <= 1e-6
