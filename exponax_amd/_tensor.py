@@ -7,8 +7,13 @@ import numpy as np
 
 
 def as_device_tensor(x):
-    """Return (CUDA tensor, restore-info)."""
+    """Return (CUDA tensor, restore-info). Raises ExmiError without a HIP
+    device: the kernels are the only compute path."""
     import torch
+
+    from . import _lib
+
+    _lib.lib()
 
     if isinstance(x, torch.Tensor):
         if x.is_cuda:

@@ -23,6 +23,8 @@ def domain_extent_of(derivative_operator, num_spatial_dims, num_points) -> float
     if d.shape != (num_spatial_dims,) + sp.wavenumber_shape(num_spatial_dims, num_points):
         raise ValueError(f"derivative_operator has shape {d.shape}")
     k1 = d[(num_spatial_dims - 1,) + (0,) * (num_spatial_dims - 1) + (1,)]
+    if not np.imag(k1) > 0:
+        raise NotImplementedError("only the standard spectral derivative operator is supported")
     L = 2 * np.pi / float(np.imag(k1))
     ref = sp.build_derivative_operator(num_spatial_dims, L, num_points)
     if not np.allclose(d, ref, rtol=1e-6, atol=1e-6 * np.max(np.abs(ref))):

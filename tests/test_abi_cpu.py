@@ -1,0 +1,102 @@
+"""CPU checks of the C-ABI boundary and the host-side mirror of the
+reference interface (no GPU compute): the shared library loads and exports
+every symbol include/exmi.h declares, the ctypes struct matches the C
+layout, the stepper constructors validate like the reference, and the
+product refuses to run without a HIP device (no CPU fallback)."""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import exponax_amd as ex
+from exponax_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "exmi.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(exmi_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load_library()
+    names = declared_symbols()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.EXPORTED)
+
+
+def test_plan_desc_layout_matches_c():
+    """Compile a tiny C program against include/exmi.h and compare sizeof and
+    every field offset with the ctypes mirror."""
+    fields = [f for f, _ in _lib.PlanDesc._fields_]
+    src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){",
+           'printf("%zu\\n", sizeof(exmi_plan_desc));']
+    for f in fields:
+        src.append(f'printf("%zu\\n", offsetof(exmi_plan_desc, {f}));')
+    src.append("return 0;}")
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "layout.c")
+        exe = os.path.join(d, "layout")
+        open(c, "w").write("\n".join(src))
+        subprocess.run(["gcc", "-std=c99", c, "-o", exe], check=True)
+        out = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(_lib.PlanDesc)
+    for f, off in zip(fields, out[1:]):
+        assert getattr(_lib.PlanDesc, f).offset == off, f
+
+
+def test_error_message_channel():
+    lib = _lib.load_library()
+    h = ctypes.c_void_p()
+    assert lib.exmi_plan_create(None, ctypes.byref(h)) == _lib.EXMI_ERR_INVALID
+    assert b"null" in lib.exmi_last_error()
+
+
+def test_stepper_constructors_validate_like_reference():
+    s = ex.stepper.KuramotoSivashinskyConservative(1, 100.0, 256, 0.1)
+    assert s.state_shape == (1, 256) and s.spectral_shape == (1, 129)
+    assert s.linear_operator.shape == (1, 129)
+    k = 2 * np.pi / 100.0 * np.arange(129)
+    np.testing.assert_allclose(s.linear_operator[0], k**2 - k**4)
+    b = ex.stepper.Burgers(1, 1.0, 64, 0.01, diffusivity=0.3)
+    np.testing.assert_allclose(b.linear_operator[0], -0.3 * (2 * np.pi * np.arange(33)) ** 2)
+    kf = ex.stepper.KolmogorovFlowVorticity(2, 2 * np.pi, 32, 0.01, diffusivity=0.01)
+    assert kf.linear_operator.shape == (1, 32, 17)
+    with pytest.raises(NotImplementedError):
+        ex.stepper.Burgers(1, 1.0, 64, 0.01, order=7)
+    with pytest.raises(ValueError):
+        ex.stepper.NavierStokesVorticity(1, 1.0, 64, 0.01)
+
+
+def test_ensemble_validation():
+    a = ex.stepper.Burgers(1, 1.0, 64, 0.01, diffusivity=0.1)
+    b = ex.stepper.Burgers(1, 1.0, 64, 0.02, diffusivity=0.2)
+    with pytest.raises(ValueError, match="dt"):
+        ex.Ensemble([a, b])
+    e = ex.ensemble(lambda nu: ex.stepper.Burgers(1, 1.0, 64, 0.01, diffusivity=nu), [0.1, 0.2, 0.3])
+    assert e.linear_operator.shape == (3, 1, 33)
+
+
+def test_custom_derivative_operator_rejected():
+    d = ex.spectral.build_derivative_operator(1, 2.0, 32) * 1.1j
+    with pytest.raises(NotImplementedError):
+        ex.nonlin_fun.ConvectionNonlinearFun(1, 32, derivative_operator=d)
+
+
+def test_no_cpu_fallback_without_device():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is present")
+    s = ex.stepper.Burgers(1, 1.0, 64, 0.01)
+    with pytest.raises(ex.ExmiError, match="no CPU fallback"):
+        s(np.zeros((1, 64), np.float32))

@@ -1,0 +1,161 @@
+"""Pins of the oracle's stepper layer (oracle/stepper.py) on the CPU:
+rollout/repeat semantics (reference tests/test_utils.py), shape errors,
+equivalences, the ensemble (vmap over constructors), and regression against
+the committed fixtures."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import ic
+from oracle import nonlin as nl
+from oracle import spectral as sp
+from oracle import stepper as st
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_rollout_equals_loop_and_repeat_equals_last():
+    """tests/test_utils.py:25-81."""
+    s = st.KuramotoSivashinskyConservative(1, 60.0, 64, 0.1)
+    u0 = ic.tfs(1, 60.0, 64, 5, np.random.default_rng(0))
+    tr = st.rollout(s, 5, include_init=True)(u0)
+    assert tr.shape == (6, 1, 64)
+    u = u0
+    for i in range(5):
+        u = s(u)
+        np.testing.assert_allclose(tr[i + 1], u, atol=1e-12)
+    np.testing.assert_allclose(st.repeat(s, 5)(u0), tr[-1], atol=1e-12)
+    assert st.rollout(s, 5)(u0).shape == (5, 1, 64)
+
+
+def test_call_shape_error():
+    """_base_stepper.py:263-270."""
+    s = st.Burgers(1, 1.0, 32, 0.01)
+    with pytest.raises(ValueError, match="Expected shape"):
+        s(np.zeros((2, 32)))
+
+
+def test_linear_operator_shape_error_and_order():
+    """_base_stepper.py:111-116 and :156-157."""
+
+    class Bad(st.BaseStepper):
+        def _build_linear_operator(self, d):
+            return np.zeros((3, 17))
+
+        def _build_nonlinear_fun(self, d):
+            return nl.ZeroNonlinearFun(1, 32)
+
+    with pytest.raises(ValueError, match="linear operator"):
+        Bad(1, 1.0, 32, 0.1, num_channels=1, order=0)
+    with pytest.raises(NotImplementedError):
+        st.Burgers(1, 1.0, 32, 0.1, order=5)
+
+
+def test_ks_conservative_equals_general_stepper():
+    """tests/test_builtin_solvers.py:126-222: KS-conservative ==
+    General(linear (0, 0, -1, 0, -1), nonlinear (0, -1, 0))."""
+    u0 = ic.tfs(1, 30.0, 64, 4, np.random.default_rng(2))
+    a = st.KuramotoSivashinskyConservative(1, 30.0, 64, 0.05)(u0)
+    b = st.GeneralStepper(1, 30.0, 64, 0.05, linear_coefficients=(0.0, 0.0, -1.0, 0.0, -1.0),
+                          nonlinear_scales=(0.0, -1.0, 0.0))(u0)
+    np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-12)
+
+
+def test_ks_equals_general_gradient_norm():
+    """tests/test_builtin_solvers.py:225-299."""
+    u0 = ic.tfs(1, 30.0, 64, 4, np.random.default_rng(3))
+    a = st.KuramotoSivashinsky(1, 30.0, 64, 0.05)(u0)
+    b = st.GeneralStepper(1, 30.0, 64, 0.05, linear_coefficients=(0.0, 0.0, -1.0, 0.0, -1.0),
+                          nonlinear_scales=(0.0, 0.0, -1.0))(u0)
+    np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-12)
+
+
+def test_navier_stokes_kolmogorov_without_forcing():
+    """KolmogorovFlowVorticity with zero injection equals NavierStokesVorticity
+    with the same drag (tests/test_builtin_solvers.py:878-895 style)."""
+    u0 = ic.tfs(2, 2 * np.pi, 32, 3, np.random.default_rng(4))
+    a = st.KolmogorovFlowVorticity(2, 2 * np.pi, 32, 0.01, diffusivity=0.01, injection_scale=0.0)(u0)
+    b = st.NavierStokesVorticity(2, 2 * np.pi, 32, 0.01, diffusivity=0.01, drag=-0.1)(u0)
+    np.testing.assert_allclose(a, b, atol=1e-12)
+
+
+def test_burgers_energy_decays():
+    """tests/test_builtin_solvers.py:424-489: viscous Burgers loses energy."""
+    u0 = ic.tfs(1, 1.0, 128, 4, np.random.default_rng(5))
+    s = st.Burgers(1, 1.0, 128, 1e-3, diffusivity=1e-2)
+    e = [np.sum(u0**2)]
+    u = u0
+    for _ in range(20):
+        u = s(u)
+        e.append(np.sum(u**2))
+    assert all(b < a for a, b in zip(e, e[1:]))
+
+
+def test_ensemble_equals_members():
+    """The vmap over constructors (per-trajectory viscosity) equals stepping
+    each member on its own."""
+    nus = [1e-3, 5e-3, 2e-2]
+    members = [st.Burgers(1, 1.0, 64, 1e-3, diffusivity=n) for n in nus]
+    ens = st.Ensemble(members)
+    u0, _ = ic.tfs_batch(3, 1, 1.0, 64, 4, 7)
+    out = ens.step_batch(u0)
+    for i, m in enumerate(members):
+        np.testing.assert_allclose(out[i], m(u0[i]), atol=1e-13)
+
+
+def test_batch_step_equals_single():
+    """step_batch is the vmap of __call__ for every nonlinear kind."""
+    u0, _ = ic.tfs_batch(3, 1, 20.0, 64, 4, 8)
+    for s in (st.KuramotoSivashinskyConservative(1, 20.0, 64, 0.05), st.KuramotoSivashinsky(1, 20.0, 64, 0.05),
+              st.Burgers(1, 20.0, 64, 0.05, diffusivity=0.5)):
+        b = s.step_batch(u0)
+        for i in range(3):
+            np.testing.assert_allclose(b[i], s(u0[i]), atol=1e-13)
+
+
+def test_nrmse():
+    """metrics/_spatial.py:537-586: sum over channels of relative L2 errors."""
+    a = np.ones((2, 8))
+    b = np.ones((2, 8)) * 2
+    assert st.nrmse(a, b) == pytest.approx(2 * 0.5)
+
+
+@pytest.mark.parametrize("name,cls,args,nu", [
+    ("ks_cons_1d", "KuramotoSivashinskyConservative", (1, 100.0, 256, 0.1), False),
+    ("ks_1d", "KuramotoSivashinsky", (1, 60.0, 128, 0.05), False),
+    ("burgers_1d", "Burgers", (1, 1.0, 1024, 1e-3), True),
+])
+def test_oracle_regression_fixtures(name, cls, args, nu):
+    """The oracle reproduces the committed fixtures (written by
+    tests/golden/make_golden.py) exactly in FP64 and to FP32 rounding."""
+    f = np.load(os.path.join(GOLDEN, f"oracle_{name}.npz"))
+    C = getattr(st, cls)
+    for tag, dtype in (("f64", np.float64), ("f32", np.float32)):
+        if nu:
+            s = st.Ensemble([C(*args, diffusivity=float(n), dtype=dtype) for n in f["nu"]], dtype=dtype)
+        else:
+            s = C(*args, dtype=dtype)
+        u = f["u0"].astype(dtype)
+        for k in range(1, 11):
+            u = s.step_batch(u)
+            if k in (1, 10):
+                np.testing.assert_allclose(u, f[f"{tag}_step{k}"], rtol=0, atol=1e-12 if tag == "f64" else 1e-5)
+    # FP32 vs FP64 at 10 steps: the FP32 rounding floor the GPU path is held to
+    for b in range(f["u0"].shape[0]):
+        assert st.nrmse(f["f32_step10"][b], f["f64_step10"][b]) < 1e-5
+
+
+def test_ic_recipe():
+    """Zero mean, unit max-abs, seeded and reproducible."""
+    u, _ = ic.tfs_batch(2, 1, 1.0, 256, 5, 1)
+    v, _ = ic.tfs_batch(2, 1, 1.0, 256, 5, 1)
+    np.testing.assert_array_equal(u, v)
+    assert np.allclose(u.mean(axis=-1), 0, atol=1e-12)
+    assert np.allclose(np.abs(u).max(axis=-1), 1)
+    u2 = ic.tfs(2, 1.0, 32, 3, np.random.default_rng(0))
+    assert u2.shape == (1, 32, 32) and abs(u2.mean()) < 1e-12
+    # band-limited to the cutoff
+    uh = sp.fft(u2, num_spatial_dims=2)
+    wn = sp.build_wavenumbers(2, 32)
+    assert np.abs(uh[0][np.max(np.abs(wn), axis=0) > 3]).max() < 1e-9
