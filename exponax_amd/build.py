@@ -25,6 +25,8 @@ LIBNAME = "libexmi.so"
 
 # transform size M = N/2 -> points per lane
 SIZES_1D = {8: 8, 16: 8, 32: 8, 64: 8, 128: 8, 256: 8, 512: 8, 1024: 16, 2048: 16}
+# 2-D grid sizes N (N x N)
+SIZES_2D = (32, 64, 128, 256, 512)
 
 
 def hipcc() -> str:
@@ -45,6 +47,10 @@ def _jobs():
     jobs = [("coef", os.path.join(CSRC, "coef.hip"), []), ("abi", os.path.join(CSRC, "abi.hip"), [])]
     for m, pt in SIZES_1D.items():
         jobs.append((f"step1d_{m}", os.path.join(CSRC, "step1d_inst.hip"), [f"-DEXMI_M={m}", f"-DEXMI_PT={pt}"]))
+    for n in SIZES_2D:
+        jobs.append((f"step2d_{n}", os.path.join(CSRC, "step2d_inst.hip"), [f"-DEXMI_N2={n}"]))
+    # longest compiles first
+    jobs.sort(key=lambda j: 0 if j[0] in ("step1d_1024", "step1d_2048", "step1d_512") else 1)
     return jobs
 
 

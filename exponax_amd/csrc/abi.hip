@@ -15,6 +15,7 @@
 #include "exmi_internal.hpp"
 #include "fft1d.hpp"
 #include "step1d.hpp"
+#include "step2d.hpp"
 
 namespace exmi {
 
@@ -37,6 +38,24 @@ EXMI_DECL_SIZE(256)
 EXMI_DECL_SIZE(512)
 EXMI_DECL_SIZE(1024)
 EXMI_DECL_SIZE(2048)
+
+using Launch2D = hipError_t (*)(const Step2DArgs&, int, hipStream_t);
+Launch2D get_2d_32(int kind, int is_row, int mode);
+Launch2D get_2d_64(int kind, int is_row, int mode);
+Launch2D get_2d_128(int kind, int is_row, int mode);
+Launch2D get_2d_256(int kind, int is_row, int mode);
+Launch2D get_2d_512(int kind, int is_row, int mode);
+
+static Launch2D get_2d(int N, int kind, int is_row, int mode) {
+  switch (N) {
+    case 32: return get_2d_32(kind, is_row, mode);
+    case 64: return get_2d_64(kind, is_row, mode);
+    case 128: return get_2d_128(kind, is_row, mode);
+    case 256: return get_2d_256(kind, is_row, mode);
+    case 512: return get_2d_512(kind, is_row, mode);
+    default: return nullptr;
+  }
+}
 
 static StepFn get_step1d(int M, int kind, int order) {
   switch (M) {
@@ -185,7 +204,35 @@ struct exmi_plan {
   int roll_nt = 0;                               // chosen by the autotuner (0 = not yet)
   hipStream_t capture_stream = nullptr;
   GraphCache gc;
+  // 2-D plans
+  int ndim = 1;
+  int nf = 0, kc = 0;
+  Launch2D row2d[3] = {nullptr, nullptr, nullptr}, col2d[3] = {nullptr, nullptr, nullptr};
+  Twiddles tw_col;
+  cf *S0 = nullptr, *N0 = nullptr, *S1 = nullptr, *F = nullptr, *T2 = nullptr;
+  float inj_value = 0.f;
+  int inj_mode = -1;
 };
+
+static void fill_args2d(const exmi_plan* p, Step2DArgs* a) {
+  memset(a, 0, sizeof *a);
+  a->S0 = p->S0;
+  a->N0 = p->N0;
+  a->S1 = p->S1;
+  a->F = p->F;
+  a->T = p->T2;
+  a->tables = p->tables;
+  a->lin_batch = p->d.lin_batch;
+  a->tw_row = p->tw.tw;
+  a->fw_row = p->tw.fw;
+  a->tw_col = p->tw_col.tw;
+  a->kscale = p->kscale;
+  a->kmax = p->kmax;
+  a->scale = (float)p->d.scale;
+  a->zero_mode_fix = p->d.zero_mode_fix;
+  a->inj_mode = p->inj_mode;
+  a->inj_value = p->inj_value;
+}
 
 static void fill_args(const exmi_plan* p, Step1DArgs* a) {
   memset(a, 0, sizeof *a);
@@ -240,6 +287,132 @@ static int kernel_kind(const exmi_plan_desc* d, int* kind) {
   }
 }
 
+int exmi_plan_destroy(exmi_plan* p);
+
+// 2-D plans: ETDRK2 with the vorticity (Kolmogorov) or gradient-norm
+// nonlinearity, one channel, N a power of two in 32..512.
+static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
+  const int N = d->num_points;
+  if (!pow2(N) || N < 32 || N > 512)
+    return fail(EXMI_ERR_UNSUPPORTED, "2-D num_points=%d: supported sizes are powers of two 32..512", N);
+  if (d->num_channels != 1) return fail(EXMI_ERR_UNSUPPORTED, "2-D plans need num_channels == 1");
+  if (d->batch < 1) return fail(EXMI_ERR_INVALID, "batch must be >= 1");
+  if (d->order != 2) return fail(EXMI_ERR_UNSUPPORTED, "2-D plans implement ETDRK order 2 (got %d)", d->order);
+  if (d->lin_batch != 1 && d->lin_batch != d->batch)
+    return fail(EXMI_ERR_INVALID, "lin_batch must be 1 or batch (%d), got %d", d->batch, d->lin_batch);
+  if (d->lin_channels != 1) return fail(EXMI_ERR_INVALID, "lin_channels must be 1");
+  if (!d->host_linear_operator) return fail(EXMI_ERR_INVALID, "host_linear_operator is null");
+  int kind = 0;
+  if (d->nonlin_kind == EXMI_NL_VORTICITY || d->nonlin_kind == EXMI_NL_KOLMOGOROV) {
+    kind = NL_VORT;
+  } else if (d->nonlin_kind == EXMI_NL_GRADIENT_NORM) {
+    kind = NL_GRADNORM;
+  } else {
+    return fail(EXMI_ERR_UNSUPPORTED, "2-D plans implement the vorticity and gradient-norm nonlinearities");
+  }
+  exmi_plan* p = new exmi_plan();
+  p->d = *d;
+  p->d.host_linear_operator = nullptr;
+  p->ndim = 2;
+  p->M = N / 2;
+  const int NY = N / 2 + 1;
+  p->nr = N * NY;
+  p->ncoef = num_coefs(2);
+  p->kind = kind;
+  p->kmax = dealias_kmax(N, d->dealiasing_fraction);
+  p->kc = p->kmax + 1 < NY ? p->kmax + 1 : NY;
+  if (p->kc < 1) p->kc = 1;
+  p->nf = kind == NL_VORT ? 4 : 2;
+  p->kscale = (float)(2.0 * 3.14159265358979323846 / d->domain_extent);
+  p->roll_nt = -1;
+  hipGetDevice(&p->device);
+  if (d->nonlin_kind == EXMI_NL_KOLMOGOROV) {
+    // _vorticity_convection.py:166-176: -k_f gamma scaling(coef_extraction)
+    // at wavenumber (0, k_f); scaling = N at k = 0 or N/2, else N/2 per axis
+    const int kf = d->injection_mode;
+    if (kf >= 0 && kf <= N / 2) {
+      const double sy = (kf == 0 || kf == N / 2) ? N : N / 2.0;
+      p->inj_mode = kf;
+      p->inj_value = (float)(-(double)kf * d->injection_scale * (double)N * sy);
+    }
+  }
+  for (int m = 0; m < 3; ++m) {
+    p->row2d[m] = get_2d(N, kind, 1, m);
+    p->col2d[m] = get_2d(N, kind, 0, m);
+    if (!p->row2d[m] || !p->col2d[m]) {
+      delete p;
+      return fail(EXMI_ERR_UNSUPPORTED, "no 2-D kernel for N=%d", N);
+    }
+  }
+  int rc = get_twiddles(N / 2, &p->tw);
+  if (!rc) rc = get_twiddles(N, &p->tw_col);
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  const long long B = d->batch, full = (long long)NY * N;
+  const long long rows = d->lin_batch;
+  // L arrives as the reference's [kx][ky] table; the column kernels read
+  // [ky][kx]
+  std::vector<double> Lt((size_t)(rows * full));
+  for (long long b = 0; b < rows; ++b)
+    for (int kx = 0; kx < N; ++kx)
+      for (int ky = 0; ky < NY; ++ky)
+        Lt[(size_t)(b * full + (long long)ky * N + kx)] = d->host_linear_operator[b * full + (long long)kx * NY + ky];
+  double* dL = nullptr;
+  bool ok = hipMalloc(&dL, sizeof(double) * rows * full) == hipSuccess &&
+            hipMalloc(&p->tables, sizeof(float) * rows * full * 3) == hipSuccess &&
+            hipMalloc(&p->S0, sizeof(cf) * B * full) == hipSuccess &&
+            hipMalloc(&p->S1, sizeof(cf) * B * full) == hipSuccess &&
+            hipMalloc(&p->T2, sizeof(cf) * B * full) == hipSuccess &&
+            hipMalloc(&p->N0, sizeof(cf) * B * p->kc * N) == hipSuccess &&
+            hipMalloc(&p->F, sizeof(cf) * B * p->nf * p->kc * N) == hipSuccess;
+  if (!ok) {
+    hipFree(dL);
+    exmi_plan_destroy(p);
+    return fail(EXMI_ERR_ALLOC, "2-D workspace allocation failed (batch %lld, N %d)", B, N);
+  }
+  hipError_t e = hipMemcpy(dL, Lt.data(), sizeof(double) * rows * full, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_coef(dL, rows * full, (int)full, d->dt, 2, d->num_circle_points, d->circle_radius, p->tables, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  hipFree(dL);
+  if (e != hipSuccess) {
+    exmi_plan_destroy(p);
+    return fail(EXMI_ERR_HIP, "coefficient formation failed: %s", hipGetErrorString(e));
+  }
+  if (hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+    exmi_plan_destroy(p);
+    return fail(EXMI_ERR_HIP, "stream creation failed");
+  }
+  *out = p;
+  return EXMI_OK;
+}
+
+// One 2-D ETDRK2 step: u_in (stride ibs) -> u_out (stride obs). With
+// have_T the row spectra of u_in are already in T (rollout); with write_T
+// the last launch also leaves the new state's row spectra in T.
+static int enqueue_step2d(exmi_plan* p, const float* u_in, long long ibs, float* u_out, long long obs, bool have_T,
+                          bool write_T, hipStream_t st) {
+  Step2DArgs a;
+  fill_args2d(p, &a);
+  a.u_in = u_in;
+  a.u_in_bstride = ibs;
+  a.u_out = u_out;
+  a.u_out_bstride = obs;
+  a.write_T = write_T ? 1 : 0;
+  const int B = p->d.batch;
+  hipError_t e = hipSuccess;
+  if (!have_T) e = p->row2d[ROW_RA](a, B, st);
+  if (e == hipSuccess) e = p->col2d[COL_CA](a, B, st);
+  if (e == hipSuccess) e = p->row2d[ROW_RB](a, B, st);
+  if (e == hipSuccess) e = p->col2d[COL_CB](a, B, st);
+  if (e == hipSuccess) e = p->row2d[ROW_RB](a, B, st);
+  if (e == hipSuccess) e = p->col2d[COL_CC](a, B, st);
+  if (e == hipSuccess) e = p->row2d[ROW_RC](a, B, st);
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "2-D step launch failed: %s", hipGetErrorString(e));
+}
+
 extern "C" {
 
 const char* exmi_last_error(void) { return g_err.c_str(); }
@@ -263,7 +436,8 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   if (!d || !out) return fail(EXMI_ERR_INVALID, "null argument");
   *out = nullptr;
   const int D = d->num_spatial_dims, N = d->num_points;
-  if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "num_spatial_dims=%d: this build supports 1", D);
+  if (D == 2) return plan_create_2d(d, out);
+  if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "num_spatial_dims=%d: this build supports 1 and 2", D);
   if (!pow2(N) || N < 16 || N > 4096)
     return fail(EXMI_ERR_UNSUPPORTED, "num_points=%d: supported sizes are powers of two 16..4096", N);
   if (d->num_channels < 1 || d->batch < 1) return fail(EXMI_ERR_INVALID, "num_channels and batch must be >= 1");
@@ -344,6 +518,11 @@ int exmi_plan_destroy(exmi_plan* p) {
   if (p->capture_stream) hipStreamDestroy(p->capture_stream);
   hipFree(p->tables);
   hipFree(p->work);
+  hipFree(p->S0);
+  hipFree(p->S1);
+  hipFree(p->N0);
+  hipFree(p->F);
+  hipFree(p->T2);
   delete p;
   return EXMI_OK;
 }
@@ -391,18 +570,21 @@ static int run_step(exmi_plan* p, const void* in, long long in_bs, int in_f, voi
 
 int exmi_step(exmi_plan* p, const float* u_in, int64_t in_bs, float* u_out, int64_t out_bs, void* stream) {
   if (!p || !u_in || !u_out) return fail(EXMI_ERR_INVALID, "null argument");
+  if (p->ndim == 2) return enqueue_step2d(p, u_in, in_bs, u_out, out_bs, false, false, (hipStream_t)stream);
   return run_step(p, u_in, in_bs, 0, u_out, out_bs, 0, 0, (hipStream_t)stream);
 }
 
 int exmi_step_fourier(exmi_plan* p, const float* uh_in, int64_t in_bs, float* uh_out, int64_t out_bs,
                       void* stream) {
   if (!p || !uh_in || !uh_out) return fail(EXMI_ERR_INVALID, "null argument");
+  if (p->ndim == 2) return fail(EXMI_ERR_UNSUPPORTED, "exmi_step_fourier: 2-D plans step physical states only");
   return run_step(p, uh_in, in_bs, 1, uh_out, out_bs, 1, 0, (hipStream_t)stream);
 }
 
 int exmi_nonlinear(exmi_plan* p, const float* uh_in, int64_t in_bs, float* nh_out, int64_t out_bs, void* stream) {
   if (!p || !uh_in || !nh_out) return fail(EXMI_ERR_INVALID, "null argument");
   if (p->d.order == 0) return fail(EXMI_ERR_INVALID, "order-0 plan has no nonlinear function");
+  if (p->ndim == 2) return fail(EXMI_ERR_UNSUPPORTED, "exmi_nonlinear: not built for 2-D plans");
   return run_step(p, uh_in, in_bs, 1, nh_out, out_bs, 1, 1, (hipStream_t)stream);
 }
 
@@ -443,7 +625,7 @@ static int enqueue_roll(exmi_plan* p, int nt, const float* u0, long long u0_bs, 
 // nt transforms per lane group.
 static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u0_bs, float* traj, int n,
                               int include_init, hipStream_t st) {
-  const long long sz = (long long)p->d.num_channels * p->d.num_points;  // 1-D state size
+  const long long sz = (long long)p->d.num_channels * p->d.num_points * (p->ndim == 2 ? p->d.num_points : 1);
   const long long tstride = (long long)(n + (include_init ? 1 : 0)) * sz;
   int off = 0;
   if (include_init) {
@@ -452,6 +634,13 @@ static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u
     off = 1;
   }
   if (n == 0) return EXMI_OK;
+  if (p->ndim == 2) {
+    for (int t = 0; t < n; ++t) {
+      int rc = enqueue_step2d(p, u0, u0_bs, traj + (long long)(t + off) * sz, tstride, t > 0, t < n - 1, st);
+      if (rc) return rc;
+    }
+    return EXMI_OK;
+  }
   if (nt > 0 && p->roll[nt]) return enqueue_roll(p, nt, u0, u0_bs, traj + off * sz, tstride, sz, n, 0, st);
   for (int t = 0; t < n; ++t) {
     const float* in = (t == 0 && !include_init) ? u0 : traj + (long long)(t - 1 + off) * sz;
@@ -557,10 +746,17 @@ static int enq_rollout_cb(exmi_plan* p, const void* a, const void* b, long long 
 static int enq_repeat_cb(exmi_plan* p, const void* a, const void* b, long long s, int n, int, hipStream_t st) {
   const float* u0 = (const float*)a;
   float* out = (float*)b;
-  const long long sz = (long long)p->d.num_channels * p->d.num_points;
+  const long long sz = (long long)p->d.num_channels * p->d.num_points * (p->ndim == 2 ? p->d.num_points : 1);
   if (n == 0) {
     hipError_t e = copy_batch(u0, s, out, sz, sz, p->d.batch, st);
     return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "copy failed: %s", hipGetErrorString(e));
+  }
+  if (p->ndim == 2) {
+    for (int t = 0; t < n; ++t) {
+      int rc = enqueue_step2d(p, u0, s, out, sz, t > 0, t < n - 1, st);
+      if (rc) return rc;
+    }
+    return EXMI_OK;
   }
   const int nt = p->roll_nt != 0 ? p->roll_nt : (p->roll[2] ? 2 : 1);
   if (nt > 0 && p->roll[nt]) return enqueue_roll(p, nt, u0, s, out, sz, 0, n, 1, st);
@@ -605,7 +801,7 @@ int exmi_repeat(exmi_plan* p, const float* u0, int64_t u0_bs, float* u_out, int3
   if (!p || !u0 || !u_out) return fail(EXMI_ERR_INVALID, "null argument");
   if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
   const size_t need = (size_t)p->d.batch * p->d.num_channels * p->d.num_points;
-  if (n > 1 && p->work_elems < need) {
+  if (p->ndim == 1 && n > 1 && p->work_elems < need) {
     hipFree(p->work);
     p->work = nullptr;
     p->work_elems = 0;

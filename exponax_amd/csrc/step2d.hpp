@@ -1,0 +1,402 @@
+// Fused 2-D ETDRK2 pseudo-spectral step for gfx950 (N x N periodic grid).
+//
+// A 2-D real transform is a real FFT along y (the last, contiguous axis) of
+// every row followed by a complex FFT along x of every kept column, the
+// reference's rfftn (_spectral.py:614-656) / irfftn (:659-721) order. One
+// ETDRK2 step (etdrk/_etdrk_2.py:91-102) is a short chain of launches, each
+// one pass over the field:
+//
+//   row RA : u rows -> r2c along y                        -> T [ky][x]
+//   col CA : T column -> c2c along x = u_hat; keep S0; the nonlinear
+//            evaluation's fields (dealiased derivatives of u_hat)
+//            -> inverse c2c along x                       -> F fields [ky][x]
+//   row RB : F fields -> c2r along y, nonlinear product, r2c -> P [ky][x]
+//   col CB : P -> c2c = N_hat0 (post dealias, scale, forcing); S1 = E S0 +
+//            c1 N0; fields of S1 -> inverse c2c          -> F fields
+//   row RB : as above                                     -> P
+//   col CC : P -> N_hat1; S' = S1 + c2 (N1 - N0) -> inverse c2c -> T
+//   row RC : T -> c2r along y -> u' rows (in a rollout also r2c of u' again
+//            -> T for the next step, i.e. RA fused)
+//
+// Row/column interface fields are column-major ([ky][x], one column
+// contiguous) and transposed through an LDS tile in the row kernels, so every
+// global access is coalesced. Only the ky <= kmax columns of the dealiased
+// fields exist. Between launches the fields stay in HBM, largely served from
+// the Infinity Cache at these sizes.
+//
+// Scaling as in 1-D: forward transforms yield 2 X, inverse ones N^2 x; the
+// spectral state is carried as 2 u_hat and the nonlinear terms as 2 N_hat;
+// the nonlinear coefficients absorb the powers of two (exact).
+//
+// Nonlinear functions: VorticityConvection2d(+Kolmogorov)
+// (nonlin_fun/_vorticity_convection.py:65-99, 178-182) and
+// GradientNormNonlinearFun in 2-D (nonlin_fun/_gradient_norm.py:84-101).
+#pragma once
+#include "exmi_internal.hpp"
+#include "fft1d.hpp"
+
+namespace exmi {
+
+struct Step2DArgs {
+  const float* u_in;         // physical (ninst, N, N)
+  long long u_in_bstride;    // floats between instances
+  float* u_out;              // physical output of RC
+  long long u_out_bstride;
+  cf* S0;                    // (ninst, Ny, N)   spectral state [ky][kx]
+  cf* N0;                    // (ninst, Kc, N)   first nonlinear term, ky < Kc
+  cf* S1;                    // (ninst, Ny, N)   stage state
+  cf* F;                     // (ninst, NF, Kc, N) nonlinear fields [ky][x]
+  cf* T;                     // (ninst, Ny, N)   row spectra / products [ky][x]
+  const float* tables;       // (lin_batch, 3, Ny, N) E, c1, c2 as [ky][kx]
+  int lin_batch;
+  const cf* tw_row;          // row real FFT (M = N/2) tables
+  const cf* fw_row;
+  const cf* tw_col;          // column complex FFT (N points) twiddles
+  float kscale;              // 2 pi / L
+  int kmax;                  // dealias cutoff (|kx|, ky <= kmax)
+  float scale;               // convection / gradient-norm scale b
+  int zero_mode_fix;
+  int inj_mode;              // Kolmogorov forcing wavenumber (ky index); -1: none
+  float inj_value;           // forcing value at (kx = 0, ky = inj_mode), reference scaling
+  int write_T;               // RC: also r2c the new state into T (rollout)
+};
+
+enum Col2DMode { COL_CA = 0, COL_CB = 1, COL_CC = 2 };
+enum Row2DMode { ROW_RA = 0, ROW_RB = 1, ROW_RC = 2 };
+
+// Nonlinear fields of one evaluation. Vorticity: (u, omega_x, v, omega_y),
+// product u omega_x + v omega_y. Gradient norm: (u_x, u_y), product
+// u_x^2 + u_y^2.
+template <int KIND>
+struct Fields2D;
+template <>
+struct Fields2D<NL_VORT> {
+  static constexpr int NF = 4;
+};
+template <>
+struct Fields2D<NL_GRADNORM> {
+  static constexpr int NF = 2;
+};
+
+// ------------------------------------------------------------ row kernels
+// R rows per workgroup; each row's real FFT of N points (M = N/2 complex) is
+// owned by T = M/PT lanes of one wavefront.
+template <int N>
+struct RowCfg {
+  static constexpr int M = N / 2;
+  static constexpr int PT = 8;
+  static constexpr int T = M / PT;
+  static constexpr int R = 16;  // rows per workgroup
+  static constexpr int WG = R * T;
+  static constexpr int NY = M + 1;
+  static constexpr int LD = R + 1;  // padded tile row
+  static_assert(T <= 64, "row transform must fit one wavefront");
+};
+
+// Tile [ky][r] (padded) of R rows <-> column-major global field rows x0..x0+R.
+template <int N>
+__device__ __forceinline__ void tile_load(cf* tl, const cf* __restrict__ g, int x0, int kc) {
+  using C = RowCfg<N>;
+  for (int i = threadIdx.x; i < C::NY * C::R; i += C::WG) {
+    const int ky = i / C::R, r = i % C::R;
+    tl[ky * C::LD + r] = ky < kc ? g[(long long)ky * N + x0 + r] : cf{0.f, 0.f};
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void tile_store(const cf* tl, cf* __restrict__ g, int x0, int kc) {
+  using C = RowCfg<N>;
+  for (int i = threadIdx.x; i < kc * C::R; i += C::WG) {
+    const int ky = i / C::R, r = i % C::R;
+    g[(long long)ky * N + x0 + r] = tl[ky * C::LD + r];
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void tile_to_row(const cf* tl, int r, int j, cf (*z)[RowCfg<N>::PT], cf* nyq) {
+  using C = RowCfg<N>;
+#pragma unroll
+  for (int s = 0; s < C::PT; ++s) z[0][s] = tl[(j + s * C::T) * C::LD + r];
+  nyq[0] = tl[C::M * C::LD + r];
+}
+
+template <int N>
+__device__ __forceinline__ void row_to_tile(cf* tl, int r, int j, const cf (*z)[RowCfg<N>::PT], const cf* nyq) {
+  using C = RowCfg<N>;
+#pragma unroll
+  for (int s = 0; s < C::PT; ++s) tl[(j + s * C::T) * C::LD + r] = z[0][s];
+  if (j == 0) tl[C::M * C::LD + r] = nyq[0];
+}
+
+template <int N, int MODE, int KIND>
+__global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
+  using C = RowCfg<N>;
+  constexpr int M = C::M, PT = C::PT, T = C::T, R = C::R;
+  constexpr int LDSN = LdsSize<M>::value;
+  __shared__ cf lds_x[R * LDSN];
+  __shared__ cf lds_t[C::NY * C::LD];
+
+  const int inst = blockIdx.y;
+  const int x0 = blockIdx.x * R;
+  const int r = threadIdx.x / T, j = threadIdx.x % T;
+  const int x = x0 + r;
+  cf* lds = lds_x + r * LDSN;
+  const Lane<M, PT> ln(j);
+  TwRegs<M, PT> tr;
+  tr.load(j, RealTables{a.tw_row, a.fw_row});
+  const long long fstride = (long long)C::NY * N;  // one full [ky][x] field
+  const int kc = a.kmax + 1 < C::NY ? a.kmax + 1 : C::NY;
+  constexpr float NN = (float)N;
+  constexpr float rn2 = 1.f / (2.f * NN * NN);  // inverse of 2 u_hat gives 2 N^2 u
+
+  cf z[1][PT];
+  cf nyq[1];
+  if constexpr (MODE == ROW_RA) {
+    const float2* u = reinterpret_cast<const float2*>(a.u_in + (long long)inst * a.u_in_bstride + (long long)x * N);
+#pragma unroll
+    for (int s = 0; s < PT; ++s) {
+      const float2 v = u[j + s * T];
+      z[0][s] = cf{v.x, v.y};
+    }
+    rfft<M, PT, 1>(z, nyq, ln, lds, tr);
+    row_to_tile<N>(lds_t, r, j, z, nyq);
+    __syncthreads();
+    tile_store<N>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
+  } else if constexpr (MODE == ROW_RB) {
+    constexpr int NF = Fields2D<KIND>::NF;
+    const cf* F = a.F + (long long)inst * NF * kc * N;
+    cf A[PT], P[PT];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      if (f > 0) __syncthreads();
+      tile_load<N>(lds_t, F + (long long)f * kc * N, x0, kc);
+      __syncthreads();
+      tile_to_row<N>(lds_t, r, j, z, nyq);
+      irfft<M, PT, 1>(z, nyq, ln, lds, tr);
+      if constexpr (KIND == NL_VORT) {
+#pragma unroll
+        for (int s = 0; s < PT; ++s) {
+          const cf q = z[0][s];
+          if (f == 0 || f == 2) {
+            A[s] = q;
+          } else if (f == 1) {
+            P[s] = cf{A[s].re * q.re, A[s].im * q.im};
+          } else {
+            P[s] = cf{fmaf(A[s].re, q.re, P[s].re), fmaf(A[s].im, q.im, P[s].im)};
+          }
+        }
+      } else {
+        static_assert(KIND == NL_VORT || KIND == NL_GRADNORM, "unsupported 2-D nonlinear kind");
+#pragma unroll
+        for (int s = 0; s < PT; ++s) {
+          const cf q = z[0][s];
+          P[s] = f == 0 ? cf{q.re * q.re, q.im * q.im} : cf{fmaf(q.re, q.re, P[s].re), fmaf(q.im, q.im, P[s].im)};
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < PT; ++s) z[0][s] = P[s];
+    (void)A;
+    rfft<M, PT, 1>(z, nyq, ln, lds, tr, a.kmax);  // only ky <= kmax are kept
+    __syncthreads();
+    row_to_tile<N>(lds_t, r, j, z, nyq);
+    __syncthreads();
+    tile_store<N>(lds_t, a.T + (long long)inst * fstride, x0, kc);
+  } else {
+    tile_load<N>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
+    __syncthreads();
+    tile_to_row<N>(lds_t, r, j, z, nyq);
+    irfft<M, PT, 1>(z, nyq, ln, lds, tr);
+#pragma unroll
+    for (int s = 0; s < PT; ++s) z[0][s] = cf{z[0][s].re * rn2, z[0][s].im * rn2};
+    float2* u = reinterpret_cast<float2*>(a.u_out + (long long)inst * a.u_out_bstride + (long long)x * N);
+#pragma unroll
+    for (int s = 0; s < PT; ++s) u[j + s * T] = make_float2(z[0][s].re, z[0][s].im);
+    if (a.write_T) {
+      // the next step's forward transform of the float32 state just written
+      rfft<M, PT, 1>(z, nyq, ln, lds, tr);
+      __syncthreads();
+      row_to_tile<N>(lds_t, r, j, z, nyq);
+      __syncthreads();
+      tile_store<N>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
+    }
+  }
+}
+
+template <int N, int MODE, int KIND>
+inline hipError_t launch_row2d(const Step2DArgs& a, int ninst, hipStream_t st) {
+  using C = RowCfg<N>;
+  hipLaunchKernelGGL((row2d_kernel<N, MODE, KIND>), dim3(N / C::R, ninst), dim3(C::WG), 0, st, a);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------- column kernels
+// Each column (fixed ky, all N x-values, contiguous) is a complex N-point FFT
+// owned by T = N/PT lanes; CPB columns per 256-thread workgroup.
+template <int N>
+struct ColCfg {
+  static constexpr int PT = 8;
+  static constexpr int T = N / PT;
+  static constexpr int WG = 256;
+  static constexpr int CPB = WG / T;  // columns per workgroup
+  static constexpr int NY = N / 2 + 1;
+  static_assert(T <= 64, "column transform must fit one wavefront");
+};
+
+template <int N, int MODE, int KIND>
+__global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
+  using C = ColCfg<N>;
+  constexpr int PT = C::PT, T = C::T;
+  constexpr int LDSN = LdsSize<N>::value;
+  constexpr int NF = Fields2D<KIND>::NF;
+  constexpr float NN = (float)N;
+  // 2 N_hat from the forward transform of the product (8 N^4 F[product])
+  constexpr float q2 = 2.f / (8.f * NN * NN * NN * NN);
+  __shared__ cf lds_x[C::CPB * LDSN];
+
+  const int inst = blockIdx.y;
+  const int c = threadIdx.x / T, j = threadIdx.x % T;
+  const int ky_raw = blockIdx.x * C::CPB + c;
+  const bool active = ky_raw < C::NY;
+  const int ky = active ? ky_raw : 0;  // inactive groups mirror column 0, store nothing
+  cf* lds = lds_x + c * LDSN;
+  const Lane<N, PT> ln(j);
+  TwRegs<N, PT> tr;
+  tr.template load_pass<1>(j, a.tw_col);
+
+  const int kc = a.kmax + 1 < C::NY ? a.kmax + 1 : C::NY;
+  const bool kept_col = ky < kc;
+  const long long full = (long long)C::NY * N;
+  const float kap_y = a.kscale * (float)ky;
+
+  // per-slot signed kx and the dealias predicate |kx| <= kmax
+  float kap_x[PT];
+  bool keep_x[PT];
+#pragma unroll
+  for (int s = 0; s < PT; ++s) {
+    const int idx = j + s * T;
+    const int kx = idx < N / 2 ? idx : idx - N;
+    kap_x[s] = a.kscale * (float)kx;
+    keep_x[s] = (kx < 0 ? -kx : kx) <= a.kmax;
+  }
+
+  // forcing (reference scaling) carried as 2 N_hat
+  auto forcing = [&](int s) -> cf {
+    return (a.inj_mode == ky && j + s * T == 0) ? cf{2.f * a.inj_value, 0.f} : cf{0.f, 0.f};
+  };
+
+  // nonlinear term 2 N_hat of this column from the product P (ky < kc)
+  auto nonlinear = [&](cf* Nh) {
+    if (kept_col) {
+      cf v[1][PT];
+      const cf* p = a.T + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) v[0][s] = p[j + s * T];
+      cfft<N, PT, false, 1>(v, ln, lds, tr);
+      const float cs = (KIND == NL_VORT) ? -a.scale * q2 : -a.scale * 0.5f * q2;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) {
+        Nh[s] = keep_x[s] ? cs * v[0][s] : cf{0.f, 0.f};
+        if (KIND == NL_GRADNORM && a.zero_mode_fix && ky == 0 && j + s * T == 0) Nh[s] = cf{0.f, 0.f};
+        Nh[s] = Nh[s] + forcing(s);
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < PT; ++s) Nh[s] = forcing(s);
+    }
+  };
+
+  // fields of the nonlinear evaluation of state Sx (2 u_hat), inverse
+  // transformed along x and stored (ky < kc only)
+  auto fields_out = [&](const cf* Sx) {
+    if (!kept_col || !active) return;
+    cf v[NF][PT];
+#pragma unroll
+    for (int s = 0; s < PT; ++s) {
+      const cf w = keep_x[s] ? Sx[s] : cf{0.f, 0.f};
+      const float kx = kap_x[s], kyv = kap_y;
+      if constexpr (KIND == NL_VORT) {
+        // psi = Laplace^-1 omega (1 at the mean mode), u = i ky psi,
+        // v = -i kx psi, omega_x = i kx omega, omega_y = i ky omega
+        const float lap = -(kx * kx + kyv * kyv);
+        const float inv = lap == 0.f ? 1.f : 1.f / lap;
+        const cf psi = inv * w;
+        v[0][s] = cf{-kyv * psi.im, kyv * psi.re};
+        v[1][s] = cf{-kx * w.im, kx * w.re};
+        v[2][s] = cf{kx * psi.im, -kx * psi.re};
+        v[3][s] = cf{-kyv * w.im, kyv * w.re};
+      } else {
+        v[0][s] = cf{-kx * w.im, kx * w.re};
+        v[1][s] = cf{-kyv * w.im, kyv * w.re};
+      }
+    }
+    cfft<N, PT, true, NF>(v, ln, lds, tr);
+    cf* F = a.F + (long long)inst * NF * kc * N;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      cf* o = F + ((long long)f * kc + ky) * N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) o[j + s * T] = v[f][s];
+    }
+  };
+
+  const float* tb = a.tables + (long long)(a.lin_batch > 1 ? inst : 0) * 3 * full;
+  auto coef = [&](int i, int s) { return tb[(long long)i * full + (long long)ky * N + j + s * T]; };
+
+  if constexpr (MODE == COL_CA) {
+    cf v[1][PT];
+    const cf* t = a.T + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+    for (int s = 0; s < PT; ++s) v[0][s] = t[j + s * T];
+    cfft<N, PT, false, 1>(v, ln, lds, tr);
+    if (active) {
+      cf* s0 = a.S0 + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) s0[j + s * T] = v[0][s];
+    }
+    fields_out(v[0]);
+  } else if constexpr (MODE == COL_CB) {
+    cf Nh[PT], S[PT];
+    nonlinear(Nh);
+    const cf* s0 = a.S0 + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+    for (int s = 0; s < PT; ++s) S[s] = coef(0, s) * s0[j + s * T] + coef(1, s) * Nh[s];
+    if (active) {
+      cf* s1 = a.S1 + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) s1[j + s * T] = S[s];
+      if (kept_col) {
+        cf* n0 = a.N0 + ((long long)inst * kc + ky) * N;
+#pragma unroll
+        for (int s = 0; s < PT; ++s) n0[j + s * T] = Nh[s];
+      }
+    }
+    fields_out(S);
+  } else {
+    cf Nh[PT], v[1][PT];
+    nonlinear(Nh);
+    const cf* s1 = a.S1 + (long long)inst * full + (long long)ky * N;
+    const cf* n0 = a.N0 + ((long long)inst * kc + ky) * N;
+#pragma unroll
+    for (int s = 0; s < PT; ++s) {
+      const cf N0s = kept_col ? n0[j + s * T] : forcing(s);
+      v[0][s] = s1[j + s * T] + coef(2, s) * (Nh[s] - N0s);
+    }
+    cfft<N, PT, true, 1>(v, ln, lds, tr);
+    if (active) {
+      cf* t = a.T + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
+    }
+  }
+}
+
+template <int N, int MODE, int KIND>
+inline hipError_t launch_col2d(const Step2DArgs& a, int ninst, hipStream_t st) {
+  using C = ColCfg<N>;
+  const int groups = (C::NY + C::CPB - 1) / C::CPB;
+  hipLaunchKernelGGL((col2d_kernel<N, MODE, KIND>), dim3(groups, ninst), dim3(C::WG), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace exmi

@@ -1,0 +1,42 @@
+// Instantiates the 2-D row and column kernels for one grid size, selected at
+// compile time with -DEXMI_N2=<N>. The build compiles this file once per
+// supported size.
+#include "step2d.hpp"
+
+#ifndef EXMI_N2
+#error "EXMI_N2 must be defined"
+#endif
+
+namespace exmi {
+
+using Launch2D = hipError_t (*)(const Step2DArgs&, int, hipStream_t);
+
+#define EXMI_CAT2(a, b) a##b
+#define EXMI_CAT(a, b) EXMI_CAT2(a, b)
+
+template <int N, int KIND>
+static Launch2D pick2d(int is_row, int mode) {
+  if (is_row) {
+    switch (mode) {
+      case ROW_RA: return &launch_row2d<N, ROW_RA, KIND>;
+      case ROW_RB: return &launch_row2d<N, ROW_RB, KIND>;
+      case ROW_RC: return &launch_row2d<N, ROW_RC, KIND>;
+    }
+  } else {
+    switch (mode) {
+      case COL_CA: return &launch_col2d<N, COL_CA, KIND>;
+      case COL_CB: return &launch_col2d<N, COL_CB, KIND>;
+      case COL_CC: return &launch_col2d<N, COL_CC, KIND>;
+    }
+  }
+  return nullptr;
+}
+
+Launch2D EXMI_CAT(get_2d_, EXMI_N2)(int kind, int is_row, int mode) {
+  constexpr int N = EXMI_N2;
+  if (kind == NL_VORT) return pick2d<N, NL_VORT>(is_row, mode);
+  if (kind == NL_GRADNORM) return pick2d<N, NL_GRADNORM>(is_row, mode);
+  return nullptr;
+}
+
+}  // namespace exmi
