@@ -1,21 +1,28 @@
 """Throughput benchmark of the fused ETDRK step on one MI355X.
 
-Workload (the batched Burgers rollout): Burgers 1-D, N=1024, L=1, dt=1e-3,
-ETDRK2, 4096 trajectories advanced together, per-trajectory viscosity
-nu log-uniform in [1e-4, 1e-2], float32 truncated-Fourier ICs (cutoff 8,
-seed 2), rollout of --steps steps captured in one hipGraph, every state
-written to the trajectory buffer (rollout semantics).
+Default workload (the batched Burgers rollout): Burgers 1-D, N=1024, L=1,
+dt=1e-3, ETDRK2, 4096 trajectories advanced together, per-trajectory
+viscosity nu log-uniform in [1e-4, 1e-2], float32 truncated-Fourier ICs
+(cutoff 8, seed 2), rollout of --steps steps captured in one hipGraph, every
+state written to the trajectory buffer (rollout semantics).
 
-A bench step is one ETDRK2 step of all 4096 trajectories; `value` is
-state-steps/s = 4096 * steps / wall time of the timed graph replay, with the
-ICs already resident in HBM. The rollout runs as the strategy the library's
-autotuner picks (one multi-step launch, or per-step launches). The roofline
-uses the algorithmic bytes of one launch: 4096 * steps-per-launch *
-(4*1024 read + 4*1024 write + 12*513 operator tables) B.
+A bench step is one ETDRK step of every trajectory of the workload; `value`
+is state-steps/s = trajectories * steps / wall time of the timed graph
+replay, with the ICs already resident in HBM. The rollout runs as the
+strategy the library's autotuner picks (one multi-step launch, or per-step
+launches). The roofline uses the algorithmic bytes of one launch:
+trajectories * steps-per-launch * (read + write of the state + 12 B per mode
+of per-trajectory ETDRK2 tables).
 
 The CPU baseline (rank 0 only) runs the NumPy/SciPy FP32 restatement of the
-same step on a bounded sample (all 4096 trajectories for 300 steps) and
-doubles as the parity check of the GPU trajectory at steps 1 and 10.
+same step on a bounded sample of the workload and doubles as the parity
+check of the GPU trajectory at steps 1 and 10.
+
+Other workloads (--workload): ks1d (Kuramoto-Sivashinsky conservative 1-D,
+N=256, one trajectory, 500 steps), kolmogorov2d (Kolmogorov vorticity,
+32 x 128^2 + 16 x 256^2 + 8 x 512^2 instances with per-instance Re, 2000
+steps in one graph), ks2d (Kuramoto-Sivashinsky 2-D 128^2, 32 trajectories,
+50 steps). They print the same JSON contract for their own workload.
 """
 from __future__ import annotations
 
@@ -29,81 +36,294 @@ import numpy as np
 
 METRIC = "state-steps/sec at rel-L2≤1e-5 vs ref; %HBM peak; learned-vs-plain speedup"
 HBM_PEAK_GBS = 8000.0
-N, L, DT = 1024, 1.0, 1e-3
-CUTOFF, SEED = 8, 2
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def truncated_fourier_ics(batch, n, length, cutoff, rng):
-    """The TFS recipe: modes k = 1..cutoff, a ~ N(0,1), phase ~ U[0, 2pi),
-    u = sum a cos(2 pi k x / L + phase), u /= max|u| (per trajectory, drawn
-    in trajectory order from one stream; identical to the test fixtures)."""
-    x = np.linspace(0.0, length, n, endpoint=False)
-    k = np.arange(1, cutoff + 1, dtype=np.float64)
-    out = np.empty((batch, 1, n))
+def tfs_ics(batch, d, n, length, cutoff, rng):
+    """The truncated-Fourier-series IC recipe (identical to the test
+    fixtures): modes with 1 <= max_d |k_d| <= cutoff from one half-plane,
+    a ~ N(0,1), phase ~ U[0, 2pi), u = sum a cos(2 pi k.x / L + phase),
+    u /= max|u|; trajectories drawn in order from one stream."""
+    import itertools
+
+    modes = [k for k in itertools.product(range(-cutoff, cutoff + 1), repeat=d)
+             if max(abs(c) for c in k) >= 1 and [c for c in reversed(k) if c != 0][0] > 0]
+    modes = np.array(modes, dtype=np.float64)
+    g = np.linspace(0.0, length, n, endpoint=False)
+    grids = np.meshgrid(*([g] * d), indexing="ij")
+    out = np.empty((batch, 1) + (n,) * d)
     for b in range(batch):
-        a = rng.standard_normal(cutoff)
-        ph = rng.uniform(0.0, 2 * np.pi, cutoff)
-        u = (a[:, None] * np.cos(2 * np.pi * k[:, None] * x[None, :] / length + ph[:, None])).sum(0)
+        a = rng.standard_normal(len(modes))
+        ph = rng.uniform(0.0, 2 * np.pi, len(modes))
+        u = np.zeros((n,) * d)
+        for m in range(len(modes)):
+            arg = sum(modes[m, i] * grids[i] for i in range(d))
+            u += a[m] * np.cos(2 * np.pi * arg / length + ph[m])
         out[b, 0] = u / np.max(np.abs(u))
     return out
 
 
-def read_traffic(workload, steps_per_launch):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary
-    (measured bytes per state-step, scaled to this run's launch)."""
+def read_traffic(workload, state_steps):
+    """Measured HBM bytes for this run's launch from the committed
+    rocprofv3 --pmc summary (bytes per state-step times state-steps)."""
     path = os.path.join(HERE, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         per_step = json.load(f).get("bytes_per_state_step")
-    return None if per_step is None else per_step * 4096 * steps_per_launch
+    return None if per_step is None else per_step * state_steps
 
 
-def cpu_baseline(u0, nus, gpu_traj, sample_b, sample_steps):
-    """Time the FP32 oracle on a bounded sample and check the GPU trajectory."""
-    sys.path.insert(0, HERE)
+def rel_l2_rows(g, ref):
+    """Per-trajectory sum over channels of ||g - ref|| / ||ref||
+    (metrics/_spatial.py:537-586), finite trajectories only."""
+    g = np.asarray(g, dtype=np.float64).reshape(g.shape[0], -1)
+    ref = np.asarray(ref, dtype=np.float64).reshape(ref.shape[0], -1)
+    ok = np.isfinite(g).all(axis=1) & np.isfinite(ref).all(axis=1)
+    return np.sqrt(((g[ok] - ref[ok]) ** 2).sum(1)) / np.sqrt((ref[ok] ** 2).sum(1))
+
+
+def timed_rollout(run, sync):
+    """Time one call of `run` (captured graph replay) with events on the
+    launching stream and host wall time, synchronized on both sides."""
+    import torch
+
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    sync()
+    t0 = time.perf_counter()
+    ev0.record()
+    run()
+    ev1.record()
+    sync()
+    return time.perf_counter() - t0, ev0.elapsed_time(ev1) / 1e3
+
+
+def oracle_baseline(make_oracle, u0, gpu_checks, sample_steps, label, fp64_maker=None):
+    """Time the FP32 oracle on a bounded sample; compare the GPU states at
+    the recorded steps; optionally the FP64 oracle error of the GPU."""
     from oracle import spectral as osp
-    from oracle import stepper as ost
 
     cores = os.cpu_count() or 1
     osp.set_fft_workers(cores)
-    members = [ost.Burgers(1, L, N, DT, diffusivity=float(nu), dtype=np.float32) for nu in nus[:sample_b]]
-    ens = ost.Ensemble(members, dtype=np.float32, order=2)
-    u = u0[:sample_b].astype(np.float32)
-    checks = {}
+    ens = make_oracle(np.float32)
+    u = u0.astype(np.float32)
+    ref = {}
     t0 = time.perf_counter()
-    with np.errstate(all="ignore"):  # the lowest-viscosity trajectories blow up, as in the reference
+    with np.errstate(all="ignore"):  # blown-up trajectories stay NaN, as in the reference
         for s in range(1, sample_steps + 1):
             u = ens.step_batch(u)
-            if s in (1, 10):
-                checks[s] = u.copy()
+            if s in gpu_checks:
+                ref[s] = u.copy()
     dt = time.perf_counter() - t0
     parity = {}
-    for s, ref in checks.items():
-        g = gpu_traj[s]
-        ok = np.isfinite(ref).all(axis=(1, 2)) & np.isfinite(g).all(axis=(1, 2))
-        num = np.sqrt(((g[ok].astype(np.float64) - ref[ok]) ** 2).sum(axis=(1, 2)))
-        den = np.sqrt((ref[ok].astype(np.float64) ** 2).sum(axis=(1, 2)))
-        r = num / den
-        parity[f"rel_l2_step{s}_max"] = float(r.max())
-        parity[f"rel_l2_step{s}_mean"] = float(r.mean())
-    parity["checked_trajectories"] = int(sample_b)
-    base = {"value": sample_b * sample_steps / dt, "unit": "state-steps/s", "cores": cores, "kind": "port",
-            "sample": f"NumPy/SciPy FP32 restatement, first {sample_b} trajectories x {sample_steps} steps "
-                      f"({dt:.1f} s), scipy.fft workers={cores}"}
+    for s, g in gpu_checks.items():
+        if s in ref:
+            r = rel_l2_rows(g, ref[s])
+            parity[f"rel_l2_step{s}_max"] = float(r.max()) if r.size else None
+            parity[f"rel_l2_step{s}_mean"] = float(r.mean()) if r.size else None
+    parity["checked_trajectories"] = int(u0.shape[0])
+    if fp64_maker is not None:
+        e64 = fp64_maker(np.float64)
+        u = u0.astype(np.float64)
+        with np.errstate(all="ignore"):
+            for s in range(1, max(gpu_checks) + 1):
+                u = e64.step_batch(u)
+                if s in gpu_checks:
+                    r = rel_l2_rows(gpu_checks[s], u)
+                    parity[f"fp64_rel_l2_step{s}_max"] = float(r.max()) if r.size else None
+    base = {"value": u0.shape[0] * sample_steps / dt, "unit": "state-steps/s", "cores": cores, "kind": "port",
+            "sample": f"NumPy/SciPy FP32 restatement, {label} ({dt:.1f} s), scipy.fft workers={cores}"}
     return base, parity
+
+
+def result_line(workload, config, value, K, W, wall, per_launch_s, alg_bytes, n_launches, kernel, traffic):
+    achieved = alg_bytes / per_launch_s / 1e9
+    return {
+        "metric": METRIC, "value": value, "unit": "state-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded truncated-Fourier ICs, seeded coefficients)",
+        "config": dict(workload=workload, **config),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes_per_launch": alg_bytes,
+                     "avg_launch_us": per_launch_s * 1e6, "launches": n_launches, "kernel": kernel},
+    }
+
+
+# ------------------------------------------------------------------ workloads
+def wl_burgers1d(args, ex, _lib, torch):
+    N, L, DT = 1024, 1.0, 1e-3
+    B, K, W = args.batch or 4096, args.steps or 1000, args.warmup
+    rng = np.random.default_rng(2)
+    u0 = tfs_ics(B, 1, N, L, 8, rng)
+    nus = 10.0 ** rng.uniform(-4.0, -2.0, B)
+    ens = ex.ensemble(lambda nu: ex.stepper.Burgers(1, L, N, DT, diffusivity=float(nu)), nus)
+    plan = ens.plan(B)
+    lib = _lib.lib()
+    dev = torch.device("cuda:0")
+    u0_d = torch.tensor(u0, dtype=torch.float32, device=dev)
+    stream = _lib.stream_ptr()
+    if W > 0:  # warmup: W steps through a captured graph of their own (includes the autotuner's probe)
+        tw = torch.empty((B, W, 1, N), dtype=torch.float32, device=dev)
+        _lib.check(lib.exmi_rollout(plan.handle, u0_d.data_ptr(), N, tw.data_ptr(), W, 0, 1, stream))
+        torch.cuda.synchronize()
+        del tw
+    traj = torch.empty((B, K, 1, N), dtype=torch.float32, device=dev)
+    _lib.check(lib.exmi_rollout_prepare(plan.handle, u0_d.data_ptr(), N, traj.data_ptr(), K, 0))
+    wall, gpu_s = timed_rollout(
+        lambda: _lib.check(lib.exmi_rollout(plan.handle, u0_d.data_ptr(), N, traj.data_ptr(), K, 0, 1, stream)),
+        torch.cuda.synchronize)
+    strategy = plan.rollout_strategy()
+    n_launches = K if strategy < 0 else 1
+    spl = K // n_launches
+    per_state = 4 * N + 4 * N + 12 * (N // 2 + 1)
+    kernel = ("step1d_kernel<512,8,NL_CONV_CONS,2>" if strategy < 0 else
+              f"roll1d_kernel<512,8,NL_CONV_CONS,2,{strategy}> ({spl} steps per launch)")
+    res = result_line("burgers1d_n1024_b4096_etdrk2_graph",
+                      {"equation": "Burgers", "num_points": N, "domain_extent": L, "dt": DT, "order": 2,
+                       "batch": B, "rollout_steps": K, "per_trajectory_viscosity": "log-uniform [1e-4, 1e-2]",
+                       "ic": "TFS cutoff 8 seed 2", "hipgraph": True},
+                      B * K / wall, K, W, wall, gpu_s / n_launches, B * spl * per_state, n_launches, kernel,
+                      read_traffic("burgers1d_n1024_b4096", B * spl))
+    res["roofline"]["alg_bytes_per_state_step"] = per_state
+    res["rollout_strategy"] = strategy
+    final = traj[:, K - 1]
+    res["nan_trajectories"] = int((~torch.isfinite(final).all(dim=-1).all(dim=-1)).sum().item())
+    if not args.no_cpu_baseline:
+        from oracle import stepper as ost
+
+        nb, ns = min(args.cpu_sample_batch or 4096, B), args.cpu_sample_steps or 300
+        checks = {s: traj[:nb, s - 1].cpu().numpy() for s in (1, 10) if s <= K}
+        mk = lambda dt: ost.Ensemble([ost.Burgers(1, L, N, DT, diffusivity=float(nu), dtype=dt)  # noqa: E731
+                                      for nu in nus[:nb]], dtype=dt)
+        res["cpu_baseline"], res["parity"] = oracle_baseline(mk, u0[:nb], checks, ns,
+                                                              f"first {nb} trajectories x {ns} steps")
+    return res
+
+
+def wl_ks1d(args, ex, _lib, torch):
+    N, L, DT = 256, 100.0, 0.1
+    K, W = args.steps or 500, args.warmup
+    u0 = tfs_ics(1, 1, N, L, 5, np.random.default_rng(1))
+    st = ex.stepper.KuramotoSivashinskyConservative(1, L, N, DT)
+    x = torch.tensor(u0, dtype=torch.float32, device="cuda:0")
+    if W > 0:
+        st.rollout_batch(x, W)
+    torch.cuda.synchronize()
+    out = {}
+    wall, gpu_s = timed_rollout(lambda: out.__setitem__("tr", st.rollout_batch(x, K)), torch.cuda.synchronize)
+    tr = out["tr"]
+    strategy = st.plan(1).rollout_strategy()
+    n_launches = K if strategy < 0 else 1
+    per_state = 8 * N  # operators shared: the state read and write
+    res = result_line("ks_cons1d_n256_b1_etdrk2",
+                      {"equation": "KuramotoSivashinskyConservative", "num_points": N, "domain_extent": L,
+                       "dt": DT, "order": 2, "batch": 1, "rollout_steps": K, "ic": "TFS cutoff 5 seed 1"},
+                      K / wall, K, W, wall, gpu_s / n_launches, (K // n_launches) * per_state, n_launches,
+                      "roll1d_kernel<128,8,NL_CONV_CONS,2,*>" if strategy > 0 else "step1d_kernel<128,8,...>",
+                      None)
+    res["rollout_strategy"] = strategy
+    if not args.no_cpu_baseline:
+        from oracle import stepper as ost
+
+        g = tr.cpu().numpy()
+        checks = {s: g[:, s - 1] for s in (1, 10, 100, K)}
+        mk = lambda dt: ost.KuramotoSivashinskyConservative(1, L, N, DT, dtype=dt)  # noqa: E731
+        res["cpu_baseline"], res["parity"] = oracle_baseline(mk, u0, checks, K, f"1 trajectory x {K} steps",
+                                                              fp64_maker=mk)
+    return res
+
+
+def wl_kolmogorov2d(args, ex, _lib, torch):
+    K, W = args.steps or 2000, args.warmup
+    groups = [(128, 32), (256, 16), (512, 8)]
+    rng = np.random.default_rng(3)
+    steppers, u0s, res_list = [], [], []
+    for n, b in groups:
+        re = rng.uniform(100.0, 1000.0, b)
+        u0 = tfs_ics(b, 2, n, 2 * np.pi, 6, rng)
+        steppers.append(ex.ensemble(lambda r, n=n: ex.stepper.KolmogorovFlowVorticity(2, 2 * np.pi, n, 0.01,
+                                                                                      diffusivity=1.0 / r), re))
+        u0s.append(torch.tensor(u0, dtype=torch.float32, device="cuda:0"))
+        res_list.append((n, re, u0))
+    if W > 0:
+        ex.rollout_group(steppers, u0s, W)
+    torch.cuda.synchronize()
+    out = {}
+    wall, gpu_s = timed_rollout(lambda: out.__setitem__("tr", ex.rollout_group(steppers, u0s, K)),
+                                torch.cuda.synchronize)
+    trs = out["tr"]
+    nst = sum(b for _, b in groups)
+    per_step_bytes = sum(b * (8 * n * n + 12 * n * (n // 2 + 1)) for n, b in groups)
+    res = result_line("kolmogorov2d_mixed_32x128_16x256_8x512_etdrk2",
+                      {"equation": "KolmogorovFlowVorticity", "instances": "32x128^2 + 16x256^2 + 8x512^2",
+                       "domain_extent": "2 pi", "dt": 0.01, "order": 2, "reynolds": "U[100, 1000]",
+                       "rollout_steps": K, "ic": "TFS cutoff 6 seed 3", "hipgraph": "one graph, 3 branches"},
+                      nst * K / wall, K, W, wall, gpu_s / K, per_step_bytes, K,
+                      "row2d/col2d kernels (6 launches per step per grid size)", None)
+    res["nan_trajectories"] = int(sum((~torch.isfinite(t[:, -1]).reshape(t.shape[0], -1).all(1)).sum().item()
+                                      for t in trs))
+    if not args.no_cpu_baseline:
+        from oracle import stepper as ost
+
+        par, base = {}, None
+        for (n, re, u0), tr in zip(res_list, trs):
+            nb = 2
+            g = tr[:nb].cpu().numpy()
+            checks = {s: g[:, s - 1] for s in (1, 10, 100) if s <= K}
+            mk = lambda dt, n=n, re=re: ost.Ensemble(  # noqa: E731
+                [ost.KolmogorovFlowVorticity(2, 2 * np.pi, n, 0.01, diffusivity=1.0 / r, dtype=dt)
+                 for r in re[:nb]], dtype=dt)
+            b_, p_ = oracle_baseline(mk, u0[:nb], checks, 100, f"{nb} x {n}^2 instances x 100 steps", fp64_maker=mk)
+            par[f"n{n}"] = p_
+            if n == 128:
+                base = b_
+        res["cpu_baseline"], res["parity"] = base, par
+    return res
+
+
+def wl_ks2d(args, ex, _lib, torch):
+    N, L, DT = 128, 60.0, 0.05
+    B, K, W = args.batch or 32, args.steps or 50, args.warmup
+    u0 = tfs_ics(B, 2, N, L, 5, np.random.default_rng(4))
+    st = ex.stepper.KuramotoSivashinsky(2, L, N, DT)
+    x = torch.tensor(u0, dtype=torch.float32, device="cuda:0")
+    if W > 0:
+        st.rollout_batch(x, W)
+    torch.cuda.synchronize()
+    out = {}
+    wall, gpu_s = timed_rollout(lambda: out.__setitem__("tr", st.rollout_batch(x, K)), torch.cuda.synchronize)
+    tr = out["tr"]
+    res = result_line("ks2d_n128_b32_etdrk2",
+                      {"equation": "KuramotoSivashinsky", "num_points": N, "domain_extent": L, "dt": DT,
+                       "order": 2, "batch": B, "rollout_steps": K, "ic": "TFS cutoff 5 seed 4"},
+                      B * K / wall, K, W, wall, gpu_s / K, B * 8 * N * N, K,
+                      "row2d/col2d kernels (6 launches per step)", None)
+    if not args.no_cpu_baseline:
+        from oracle import stepper as ost
+
+        nb = 4
+        g = tr[:nb].cpu().numpy()
+        checks = {s: g[:, s - 1] for s in (1, 10, K)}
+        mk = lambda dt: ost.KuramotoSivashinsky(2, L, N, DT, dtype=dt)  # noqa: E731
+        res["cpu_baseline"], res["parity"] = oracle_baseline(mk, u0[:nb], checks, K,
+                                                              f"{nb} trajectories x {K} steps", fp64_maker=mk)
+    return res
+
+
+WORKLOADS = {"burgers1d": wl_burgers1d, "ks1d": wl_ks1d, "kolmogorov2d": wl_kolmogorov2d, "ks2d": wl_ks2d}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the workload's)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="burgers1d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-batch", type=int, default=4096)
-    ap.add_argument("--cpu-sample-steps", type=int, default=300)
+    ap.add_argument("--cpu-sample-batch", type=int, default=0)
+    ap.add_argument("--cpu-sample-steps", type=int, default=0)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -128,85 +348,8 @@ def main():
     import exponax_amd as ex
     from exponax_amd import _lib
 
-    B, K, W = args.batch, args.steps, args.warmup
-    rng = np.random.default_rng(SEED)
-    u0 = truncated_fourier_ics(B, N, L, CUTOFF, rng)
-    nus = 10.0 ** rng.uniform(-4.0, -2.0, B)
-    ens = ex.ensemble(lambda nu: ex.stepper.Burgers(1, L, N, DT, diffusivity=float(nu)), nus)
-    plan = ens.plan(B)
-    lib = _lib.lib()
-    dev = torch.device("cuda:0")
-    u0_d = torch.tensor(u0, dtype=torch.float32, device=dev)
-    sz = N
-    stream = _lib.stream_ptr()
-
-    # warmup: W steps through a captured graph of their own
-    if W > 0:
-        tw = torch.empty((B, W, 1, N), dtype=torch.float32, device=dev)
-        _lib.check(lib.exmi_rollout(plan.handle, u0_d.data_ptr(), sz, tw.data_ptr(), W, 0, 1, stream))
-        torch.cuda.synchronize()
-        del tw
-    traj = torch.empty((B, K, 1, N), dtype=torch.float32, device=dev)
-    _lib.check(lib.exmi_rollout_prepare(plan.handle, u0_d.data_ptr(), sz, traj.data_ptr(), K, 0))
-    torch.cuda.synchronize()
-
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record()
-    _lib.check(lib.exmi_rollout(plan.handle, u0_d.data_ptr(), sz, traj.data_ptr(), K, 0, 1, stream))
-    ev1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = ev0.elapsed_time(ev1)
-
-    value = B * K / wall
-    strategy = plan.rollout_strategy()
-    # per-step launches (strategy -1) or one multi-step launch of K steps
-    n_launches = K if strategy < 0 else 1
-    steps_per_launch = K // n_launches
-    per_launch_s = gpu_ms / 1e3 / n_launches
-    bytes_per_state_step = 4 * N + 4 * N + 12 * (N // 2 + 1)
-    alg_bytes = B * steps_per_launch * bytes_per_state_step
-    achieved = alg_bytes / per_launch_s / 1e9
-    kernel = ("step1d_kernel<512,8,NL_CONV_CONS,2>" if strategy < 0 else
-              f"roll1d_kernel<512,8,NL_CONV_CONS,2,{strategy}> ({steps_per_launch} steps per launch)")
-    final = traj[:, K - 1].float()
-    nan_traj = int((~torch.isfinite(final).all(dim=-1).all(dim=-1)).sum().item())
-
-    result = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "state-steps/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": wall * 1e3 / K,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded truncated-Fourier ICs, log-uniform viscosities)",
-        "config": {"workload": "burgers1d_n1024_b4096_etdrk2_graph", "equation": "Burgers", "num_points": N,
-                   "domain_extent": L, "dt": DT, "order": 2, "batch": B, "rollout_steps": K,
-                   "per_trajectory_viscosity": "log-uniform [1e-4, 1e-2]", "ic": "TFS cutoff 8 seed 2",
-                   "hipgraph": True},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": read_traffic("burgers1d_n1024_b4096", steps_per_launch),
-                     "alg_bytes_per_launch": alg_bytes, "alg_bytes_per_state_step": bytes_per_state_step,
-                     "avg_launch_us": per_launch_s * 1e6, "launches": n_launches, "kernel": kernel},
-        "rollout_strategy": strategy,
-        "nan_trajectories": nan_traj,
-    }
-    if not args.no_cpu_baseline:
-        nb = min(args.cpu_sample_batch, B)
-        gsel = {s: traj[:nb, s - 1].cpu().numpy() for s in (1, 10) if s <= K}
-        base, parity = cpu_baseline(u0, nus, gsel, nb, args.cpu_sample_steps)
-        result["cpu_baseline"] = base
-        result["parity"] = parity
-    print(json.dumps(result), flush=True)
+    res = WORKLOADS[args.workload](args, ex, _lib, torch)
+    print(json.dumps(res), flush=True)
     return 0
 
 

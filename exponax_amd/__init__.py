@@ -7,8 +7,8 @@ runs in hand-written HIP kernels behind the C ABI in ``include/exmi.h``.
 from . import nonlin_fun, spectral, stepper
 from ._base_stepper import BaseStepper, Ensemble, ensemble
 from ._lib import ExmiError, load_library
-from ._utils import make_grid, repeat, rollout, vmap
+from ._utils import make_grid, repeat, rollout, rollout_group, vmap
 from .spectral import fft, ifft
 
 __all__ = ["BaseStepper", "Ensemble", "ensemble", "ExmiError", "fft", "ifft", "load_library", "make_grid",
-           "nonlin_fun", "repeat", "rollout", "spectral", "stepper", "vmap"]
+           "nonlin_fun", "repeat", "rollout", "rollout_group", "spectral", "stepper", "vmap"]

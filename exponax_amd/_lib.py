@@ -32,7 +32,7 @@ EXMI_ERR_UNSUPPORTED = -4
 # every symbol include/exmi.h declares
 EXPORTED = (
     "exmi_plan_create", "exmi_plan_destroy", "exmi_step", "exmi_step_fourier", "exmi_nonlinear",
-    "exmi_rollout", "exmi_rollout_prepare", "exmi_repeat", "exmi_rfft", "exmi_irfft", "exmi_plan_num_coefs", "exmi_plan_get_tables",
+    "exmi_rollout", "exmi_rollout_prepare", "exmi_rollout_group", "exmi_repeat", "exmi_rfft", "exmi_irfft", "exmi_plan_num_coefs", "exmi_plan_get_tables",
     "exmi_plan_rollout_strategy", "exmi_plan_set_rollout_strategy", "exmi_last_error", "exmi_info",
 )
 
@@ -84,6 +84,8 @@ def _declare(lib):
         getattr(lib, name).argtypes = [vp, vp, i64, vp, i64, vp]
     lib.exmi_rollout.argtypes = [vp, vp, i64, vp, i32, i32, i32, vp]
     lib.exmi_rollout_prepare.argtypes = [vp, vp, i64, vp, i32, i32]
+    lib.exmi_rollout_group.argtypes = [C.POINTER(vp), i32, C.POINTER(vp), C.POINTER(i64), C.POINTER(vp), i32, i32,
+                                       i32, vp]
     lib.exmi_repeat.argtypes = [vp, vp, i64, vp, i32, i32, vp]
     lib.exmi_rfft.argtypes = [i32, i32, i32, vp, vp, vp]
     lib.exmi_irfft.argtypes = [i32, i32, i32, vp, vp, vp]

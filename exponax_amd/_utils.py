@@ -114,3 +114,35 @@ def make_grid(num_spatial_dims, domain_extent, num_points, *, full=False, zero_c
     if zero_centered:
         g = g - domain_extent / 2
     return np.stack(np.meshgrid(*([g] * num_spatial_dims), indexing="ij"))
+
+
+def rollout_group(steppers, u0s, n: int, *, include_init: bool = False, use_graph: bool = True):
+    """Roll several steppers (e.g. ensembles on different grid sizes) in one
+    captured graph whose per-stepper launch chains run as parallel branches.
+    ``u0s[i]`` is a (B_i, C, N_i^D) batch for ``steppers[i]``; returns the
+    list of (B_i, n[+1], C, N_i^D) trajectories."""
+    import ctypes
+
+    import torch
+
+    from . import _lib
+    from ._tensor import as_device_tensor
+
+    plans, xs, outs, bs = [], [], [], []
+    for st, u0 in zip(steppers, u0s):
+        t, _ = as_device_tensor(u0)
+        lead, B = st._split(t, st.state_shape)
+        x = t.contiguous().to(torch.float32)
+        T = n + (1 if include_init else 0)
+        outs.append(torch.empty(lead + (T,) + st.state_shape, dtype=torch.float32, device=x.device))
+        xs.append(x)
+        plans.append(st.plan(B))
+        bs.append(int(np.prod(st.state_shape)))
+    k = len(plans)
+    P = (ctypes.c_void_p * k)(*[p.handle.value for p in plans])
+    U = (ctypes.c_void_p * k)(*[x.data_ptr() for x in xs])
+    S = (ctypes.c_int64 * k)(*bs)
+    O = (ctypes.c_void_p * k)(*[o.data_ptr() for o in outs])
+    _lib.check(_lib.lib().exmi_rollout_group(P, k, U, S, O, n, int(include_init), int(use_graph),
+                                             _lib.stream_ptr()), "exmi_rollout_group")
+    return outs

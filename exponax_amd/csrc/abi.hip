@@ -214,6 +214,16 @@ struct exmi_plan {
   int inj_mode = -1;
 };
 
+// ---- grouped rollouts: several plans (e.g. different grid sizes) advanced
+// together; in a graph each plan's launches form a parallel branch.
+struct GroupGraph {
+  std::vector<const void*> key;
+  int n = -1, flag = -1;
+  hipGraphExec_t exec = nullptr;
+};
+static std::mutex g_group_mu;
+static std::map<const exmi_plan*, GroupGraph> g_group_graphs;  // by leader plan
+
 static void fill_args2d(const exmi_plan* p, Step2DArgs* a) {
   memset(a, 0, sizeof *a);
   a->S0 = p->S0;
@@ -514,6 +524,19 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
 
 int exmi_plan_destroy(exmi_plan* p) {
   if (!p) return EXMI_OK;
+  {
+    std::lock_guard<std::mutex> lk(g_group_mu);
+    for (auto it = g_group_graphs.begin(); it != g_group_graphs.end();) {
+      bool uses = it->first == p;
+      for (size_t i = 0; i < it->second.key.size() && !uses; i += 4) uses = it->second.key[i] == p;
+      if (uses) {
+        if (it->second.exec) hipGraphExecDestroy(it->second.exec);
+        it = g_group_graphs.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
   if (p->gc.exec) hipGraphExecDestroy(p->gc.exec);
   if (p->capture_stream) hipStreamDestroy(p->capture_stream);
   hipFree(p->tables);
@@ -794,6 +817,88 @@ int exmi_rollout_prepare(exmi_plan* p, const float* u0, int64_t u0_bs, float* tr
     if (rc) return rc;
   }
   return run_maybe_graph(p, u0, traj, u0_bs, n, include_init ? 1 : 0, 1, 1, nullptr, enq_rollout_cb, false);
+}
+
+// ---- grouped rollouts (see exmi_rollout_group)
+
+static int enqueue_group(exmi_plan* const* plans, int np, const float* const* u0s, const int64_t* bs,
+                         float* const* trajs, int n, int include_init, hipStream_t main, bool fork) {
+  if (!fork || np == 1) {
+    for (int i = 0; i < np; ++i) {
+      int rc = enqueue_rollout(plans[i], u0s[i], bs[i], trajs[i], n, include_init, main);
+      if (rc) return rc;
+    }
+    return EXMI_OK;
+  }
+  std::vector<hipEvent_t> ev(np + 1);
+  for (auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  int rc = EXMI_OK;
+  HIP_TRY(hipEventRecord(ev[0], main));
+  for (int i = 1; i < np && !rc; ++i) {
+    hipStream_t s = plans[i]->capture_stream;
+    if (hipStreamWaitEvent(s, ev[0], 0) != hipSuccess) rc = fail(EXMI_ERR_HIP, "fork failed");
+    if (!rc) rc = enqueue_rollout(plans[i], u0s[i], bs[i], trajs[i], n, include_init, s);
+    if (!rc && hipEventRecord(ev[i], s) != hipSuccess) rc = fail(EXMI_ERR_HIP, "join record failed");
+  }
+  if (!rc) rc = enqueue_rollout(plans[0], u0s[0], bs[0], trajs[0], n, include_init, main);
+  for (int i = 1; i < np && !rc; ++i)
+    if (hipStreamWaitEvent(main, ev[i], 0) != hipSuccess) rc = fail(EXMI_ERR_HIP, "join failed");
+  for (auto& e : ev) hipEventDestroy(e);
+  return rc;
+}
+
+int exmi_rollout_group(exmi_plan* const* plans, int32_t np, const float* const* u0s, const int64_t* u0_bstrides,
+                       float* const* trajs, int32_t n, int32_t include_init, int32_t use_graph, void* stream) {
+  if (!plans || np < 1 || !u0s || !u0_bstrides || !trajs) return fail(EXMI_ERR_INVALID, "null argument");
+  if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
+  for (int i = 0; i < np; ++i) {
+    if (!plans[i] || !u0s[i] || !trajs[i]) return fail(EXMI_ERR_INVALID, "null plan or buffer in group");
+    for (int k = 0; k < i; ++k)
+      if (plans[k] == plans[i]) return fail(EXMI_ERR_INVALID, "a plan appears twice in the group");
+    if (plans[i]->ndim == 1 && plans[i]->roll_nt == 0) {
+      HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+      int rc = autotune_rollout(plans[i], u0s[i], u0_bstrides[i], trajs[i], n, include_init);
+      if (rc) return rc;
+    }
+  }
+  hipStream_t st = (hipStream_t)stream;
+  if (!use_graph) return enqueue_group(plans, np, u0s, u0_bstrides, trajs, n, include_init, st, false);
+  std::vector<const void*> key;
+  for (int i = 0; i < np; ++i) {
+    key.push_back(plans[i]);
+    key.push_back(u0s[i]);
+    key.push_back((const void*)(intptr_t)u0_bstrides[i]);
+    key.push_back(trajs[i]);
+  }
+  std::lock_guard<std::mutex> lk(g_group_mu);
+  GroupGraph& g = g_group_graphs[plans[0]];
+  if (!(g.exec && g.key == key && g.n == n && g.flag == include_init)) {
+    if (g.exec) {
+      hipGraphExecDestroy(g.exec);
+      g.exec = nullptr;
+    }
+    hipStream_t cs = plans[0]->capture_stream;
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_group(plans, np, u0s, u0_bstrides, trajs, n, include_init, cs, true);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(cs, &graph);
+    if (rc) {
+      if (graph) hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(EXMI_ERR_HIP, "group graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      g.exec = nullptr;
+      return fail(EXMI_ERR_HIP, "group graph instantiation failed: %s", hipGetErrorString(e));
+    }
+    g.key = key;
+    g.n = n;
+    g.flag = include_init;
+  }
+  HIP_TRY(hipGraphLaunch(g.exec, st));
+  return EXMI_OK;
 }
 
 int exmi_repeat(exmi_plan* p, const float* u0, int64_t u0_bs, float* u_out, int32_t n, int32_t use_graph,

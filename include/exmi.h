@@ -144,6 +144,17 @@ int exmi_rollout(exmi_plan* plan, const float* u0, int64_t u0_bstride, float* tr
 int exmi_rollout_prepare(exmi_plan* plan, const float* u0, int64_t u0_bstride, float* traj, int32_t n_steps,
                          int32_t include_init);
 
+/* Rollouts of several plans advanced together, e.g. instances of different
+ * grid sizes in one batch: plan i rolls u0s[i] (stride u0_bstrides[i]) into
+ * trajs[i] exactly as exmi_rollout would. With use_graph != 0 all of it is
+ * one captured hipGraph in which each plan's launches form a parallel branch
+ * (cached by the first plan for the same arguments). A plan may appear once.
+ * (The reference runs each grid size separately: arrays of different shapes
+ * cannot share one jax.vmap, docs/examples/performance_hints.ipynb cell 22.) */
+int exmi_rollout_group(exmi_plan* const* plans, int32_t n_plans, const float* const* u0s,
+                       const int64_t* u0_bstrides, float* const* trajs, int32_t n_steps, int32_t include_init,
+                       int32_t use_graph, void* stream);
+
 /* Final state after n_steps: u_out (B, C, N^D) contiguous.
  * Replaces exponax.repeat (_utils.py:189-254, takes_aux=False). */
 int exmi_repeat(exmi_plan* plan, const float* u0, int64_t u0_bstride, float* u_out, int32_t n_steps,
