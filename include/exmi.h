@@ -182,7 +182,7 @@ int exmi_plan_get_tables(const exmi_plan* plan, float* host_out, int64_t host_co
  * kernel), 1 or 2 = one multi-step launch keeping 1 or 2 transforms per lane
  * group on chip for all steps (same per-step arithmetic), 0 = not chosen
  * yet (the first exmi_rollout / exmi_rollout_prepare times the candidates on
- * a 4-step probe and keeps the fastest). Setting a strategy drops a cached
+ * probe of min(n_steps, 16) steps and keeps the fastest). Setting a strategy drops a cached
  * graph. (No reference counterpart: tuning knobs of this implementation.) */
 int exmi_plan_rollout_strategy(const exmi_plan* plan);
 int exmi_plan_set_rollout_strategy(exmi_plan* plan, int32_t strategy);
