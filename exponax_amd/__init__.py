@@ -4,7 +4,7 @@ The stepper constructors, ``(C, N, ..., N)`` state layout, ``__call__`` /
 ``step_fourier`` / ``rollout`` / ``repeat`` follow exponax 0.2.0; every step
 runs in hand-written HIP kernels behind the C ABI in ``include/exmi.h``.
 """
-from . import nonlin_fun, spectral, stepper
+from . import autograd, nonlin_fun, spectral, stepper
 from ._base_stepper import BaseStepper, Ensemble, ensemble
 from ._lib import ExmiError, load_library
 from ._utils import make_grid, repeat, rollout, rollout_group, vmap

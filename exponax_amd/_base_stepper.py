@@ -74,6 +74,7 @@ class BaseStepper:
         d.domain_extent = float(self.domain_extent)
         d.dt = float(self.dt)
         d.dealiasing_fraction = -1.0
+        d.flags = _lib.EXMI_FLAG_EXACT_FORM if getattr(self, "exact_form", False) else 0
         if self.order > 0:
             self.nonlinear_fun.fill(d)
         return d
@@ -162,6 +163,61 @@ class BaseStepper:
         p = self.plan(B)
         _lib.check(_lib.lib().exmi_repeat(p.handle, x.data_ptr(), sz, out.data_ptr(), n, int(use_graph),
                                           _lib.stream_ptr()), "exmi_repeat")
+        return restore_like(out, back)
+
+    # --------------------------------------------------------- derivatives
+    def step_vjp(self, u, g):
+        """(d step(u) / d u)^T g for one state or a batch (the reverse-mode
+        derivative jax.vjp gives for ``_base_stepper.py:241-271``)."""
+        import torch
+
+        t, back = as_device_tensor(u)
+        gt, _ = as_device_tensor(g)
+        lead, B = self._split(t, self.state_shape)
+        if tuple(gt.shape) != tuple(t.shape):
+            raise ValueError(f"cotangent shape {tuple(gt.shape)} != state shape {tuple(t.shape)}")
+        x = t.contiguous().to(torch.float32)
+        gx = gt.contiguous().to(torch.float32)
+        out = torch.empty_like(x)
+        n = int(np.prod(self.state_shape))
+        p = self.plan(B)
+        _lib.check(_lib.lib().exmi_step_vjp(p.handle, x.data_ptr(), n, gx.data_ptr(), n, out.data_ptr(), n,
+                                            _lib.stream_ptr()), "exmi_step_vjp")
+        return restore_like(out, back)
+
+    def rollout_vjp(self, u0, n: int, *, g_traj=None, g_final=None, include_init: bool = False,
+                    checkpoint_every: int = 10):
+        """Gradient through ``rollout(n, include_init)`` (cotangents g_traj of
+        the trajectory) and/or through ``repeat(n)`` (cotangent g_final of the
+        last state), recomputing segments of `checkpoint_every` steps."""
+        import torch
+
+        t, back = as_device_tensor(u0)
+        lead, B = self._split(t, self.state_shape)
+        x = t.contiguous().to(torch.float32)
+        sz = int(np.prod(self.state_shape))
+        T = n + (1 if include_init else 0)
+        gtp = gfp = None
+        keep = []
+        if g_traj is not None:
+            gt, _ = as_device_tensor(g_traj)
+            if tuple(gt.shape) != lead + (T,) + self.state_shape:
+                raise ValueError(f"g_traj shape {tuple(gt.shape)} != {lead + (T,) + self.state_shape}")
+            gt = gt.contiguous().to(torch.float32)
+            keep.append(gt)
+            gtp = gt.data_ptr()
+        if g_final is not None:
+            gf, _ = as_device_tensor(g_final)
+            if tuple(gf.shape) != tuple(t.shape):
+                raise ValueError(f"g_final shape {tuple(gf.shape)} != state shape {tuple(t.shape)}")
+            gf = gf.contiguous().to(torch.float32)
+            keep.append(gf)
+            gfp = gf.data_ptr()
+        out = torch.empty_like(x)
+        p = self.plan(B)
+        _lib.check(_lib.lib().exmi_rollout_vjp(p.handle, x.data_ptr(), sz, n, int(include_init), gtp, gfp,
+                                               out.data_ptr(), int(checkpoint_every), _lib.stream_ptr()),
+                   "exmi_rollout_vjp")
         return restore_like(out, back)
 
     def coefficient_tables(self, batch: int = 1) -> np.ndarray:

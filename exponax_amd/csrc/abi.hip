@@ -13,6 +13,8 @@
 #include <vector>
 
 #include "exmi_internal.hpp"
+#include "adjoint1d.hpp"
+#include "adjoint2d.hpp"
 #include "fft1d.hpp"
 #include "step1d.hpp"
 #include "step2d.hpp"
@@ -21,12 +23,14 @@ namespace exmi {
 
 using StepFn = hipError_t (*)(const Step1DArgs&, hipStream_t);
 using RollFn = hipError_t (*)(const Roll1DArgs&, hipStream_t);
+using VjpFn = hipError_t (*)(const Vjp1DArgs&, hipStream_t);
 hipError_t launch_coef(const double* L, long long total, int nr, double dt, int order, int ncirc, double radius,
                        float* out, hipStream_t st);
 
 #define EXMI_DECL_SIZE(M)                                                                              \
   StepFn get_step1d_##M(int kind, int order);                                                          \
   RollFn get_roll1d_##M(int kind, int order, int nt);                                                  \
+  VjpFn get_vjp1d_##M(int kind, int order);                                                            \
   hipError_t launch_fft1d_##M(bool inverse, const float* in, float* out, int count, const RealTables& rt, \
                               hipStream_t st);
 EXMI_DECL_SIZE(8)
@@ -45,6 +49,23 @@ Launch2D get_2d_64(int kind, int is_row, int mode);
 Launch2D get_2d_128(int kind, int is_row, int mode);
 Launch2D get_2d_256(int kind, int is_row, int mode);
 Launch2D get_2d_512(int kind, int is_row, int mode);
+using LaunchAdj2D = hipError_t (*)(const Adj2DArgs&, int, hipStream_t);
+LaunchAdj2D get_adj2d_32(int is_row, int mode);
+LaunchAdj2D get_adj2d_64(int is_row, int mode);
+LaunchAdj2D get_adj2d_128(int is_row, int mode);
+LaunchAdj2D get_adj2d_256(int is_row, int mode);
+LaunchAdj2D get_adj2d_512(int is_row, int mode);
+
+static LaunchAdj2D get_adj2d(int N, int is_row, int mode) {
+  switch (N) {
+    case 32: return get_adj2d_32(is_row, mode);
+    case 64: return get_adj2d_64(is_row, mode);
+    case 128: return get_adj2d_128(is_row, mode);
+    case 256: return get_adj2d_256(is_row, mode);
+    case 512: return get_adj2d_512(is_row, mode);
+    default: return nullptr;
+  }
+}
 
 static Launch2D get_2d(int N, int kind, int is_row, int mode) {
   switch (N) {
@@ -68,6 +89,21 @@ static StepFn get_step1d(int M, int kind, int order) {
     case 512: return get_step1d_512(kind, order);
     case 1024: return get_step1d_1024(kind, order);
     case 2048: return get_step1d_2048(kind, order);
+    default: return nullptr;
+  }
+}
+
+static VjpFn get_vjp1d(int M, int kind, int order) {
+  switch (M) {
+    case 8: return get_vjp1d_8(kind, order);
+    case 16: return get_vjp1d_16(kind, order);
+    case 32: return get_vjp1d_32(kind, order);
+    case 64: return get_vjp1d_64(kind, order);
+    case 128: return get_vjp1d_128(kind, order);
+    case 256: return get_vjp1d_256(kind, order);
+    case 512: return get_vjp1d_512(kind, order);
+    case 1024: return get_vjp1d_1024(kind, order);
+    case 2048: return get_vjp1d_2048(kind, order);
     default: return nullptr;
   }
 }
@@ -212,6 +248,13 @@ struct exmi_plan {
   cf *S0 = nullptr, *N0 = nullptr, *S1 = nullptr, *F = nullptr, *T2 = nullptr;
   float inj_value = 0.f;
   int inj_mode = -1;
+  // adjoint: 1-D kernel; 2-D modes and workspaces (allocated on first use)
+  VjpFn vjp = nullptr;
+  LaunchAdj2D rowA[3] = {nullptr, nullptr, nullptr}, colA[3] = {nullptr, nullptr, nullptr};
+  float* G = nullptr;
+  cf *LAM = nullptr, *RHO = nullptr, *H = nullptr, *P = nullptr;
+  float* ckpt = nullptr;  // rollout-gradient checkpoints, segment states, accumulator
+  size_t ckpt_elems = 0;
 };
 
 // ---- grouped rollouts: several plans (e.g. different grid sizes) advanced
@@ -298,6 +341,7 @@ static int kernel_kind(const exmi_plan_desc* d, int* kind) {
 }
 
 int exmi_plan_destroy(exmi_plan* p);
+static int ensure_work(exmi_plan* p);
 
 // 2-D plans: ETDRK2 with the vorticity (Kolmogorov) or gradient-norm
 // nonlinearity, one channel, N a power of two in 32..512.
@@ -352,6 +396,12 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     if (!p->row2d[m] || !p->col2d[m]) {
       delete p;
       return fail(EXMI_ERR_UNSUPPORTED, "no 2-D kernel for N=%d", N);
+    }
+  }
+  if (kind == NL_GRADNORM) {
+    for (int m = 0; m < 3; ++m) {
+      p->rowA[m] = get_adj2d(N, 1, m);
+      p->colA[m] = get_adj2d(N, 0, m);
     }
   }
   int rc = get_twiddles(N / 2, &p->tw);
@@ -487,6 +537,7 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
     delete p;
     return fail(EXMI_ERR_UNSUPPORTED, "no kernel for N=%d kind=%d order=%d", N, kind, d->order);
   }
+  p->vjp = d->order <= 2 ? get_vjp1d(p->M, d->order == 0 ? NL_ZERO : kind, d->order) : nullptr;
   p->roll[1] = get_roll1d(p->M, kind, d->order, 1);
   p->roll[2] = get_roll1d(p->M, kind, d->order, 2);
   if (const char* e = getenv("EXMI_ROLL_NT")) p->roll_nt = atoi(e);  // 1, 2; -1 = per-step launches
@@ -546,6 +597,12 @@ int exmi_plan_destroy(exmi_plan* p) {
   hipFree(p->N0);
   hipFree(p->F);
   hipFree(p->T2);
+  hipFree(p->G);
+  hipFree(p->LAM);
+  hipFree(p->RHO);
+  hipFree(p->H);
+  hipFree(p->P);
+  hipFree(p->ckpt);
   delete p;
   return EXMI_OK;
 }
@@ -905,14 +962,9 @@ int exmi_repeat(exmi_plan* p, const float* u0, int64_t u0_bs, float* u_out, int3
                 void* stream) {
   if (!p || !u0 || !u_out) return fail(EXMI_ERR_INVALID, "null argument");
   if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
-  const size_t need = (size_t)p->d.batch * p->d.num_channels * p->d.num_points;
-  if (p->ndim == 1 && n > 1 && p->work_elems < need) {
-    hipFree(p->work);
-    p->work = nullptr;
-    p->work_elems = 0;
-    if (hipMalloc(&p->work, need * sizeof(float)) != hipSuccess)
-      return fail(EXMI_ERR_ALLOC, "workspace allocation failed");
-    p->work_elems = need;
+  if (n > 1) {
+    int rc = ensure_work(p);
+    if (rc) return rc;
   }
   return run_maybe_graph(p, u0, u_out, u0_bs, n, 0, 2, use_graph, (hipStream_t)stream, enq_repeat_cb);
 }
@@ -937,6 +989,208 @@ int exmi_irfft(int32_t D, int32_t N, int32_t count, const float* x_hat, float* x
   if (rc) return rc;
   hipError_t e = launch_fft1d(N / 2, true, x_hat, x, count, tw.rt(), (hipStream_t)stream);
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "irfft launch failed: %s", hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------ adjoint
+__global__ void exmi_axpy_kernel(float* __restrict__ dst, long long ds, const float* __restrict__ src, long long ss,
+                                 long long n, int B) {
+  for (int b = blockIdx.y; b < B; b += gridDim.y)
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+      dst[(long long)b * ds + i] += src[(long long)b * ss + i];
+}
+
+__global__ void exmi_fill_kernel(float* __restrict__ dst, long long n, float v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = v;
+}
+
+static hipError_t axpy_batch(float* dst, long long ds, const float* src, long long ss, long long n, int B,
+                             hipStream_t st) {
+  int gx = (int)((n + 255) / 256);
+  if (gx > 64) gx = 64;
+  const int gy = B < 65535 ? B : 65535;
+  hipLaunchKernelGGL(exmi_axpy_kernel, dim3(gx, gy), dim3(256), 0, st, dst, ds, src, ss, n, B);
+  return hipGetLastError();
+}
+
+static hipError_t fill(float* dst, long long n, float v, hipStream_t st) {
+  long long gx = (n + 255) / 256;
+  if (gx > 4096) gx = 4096;
+  if (gx < 1) gx = 1;
+  hipLaunchKernelGGL(exmi_fill_kernel, dim3((unsigned)gx), dim3(256), 0, st, dst, n, v);
+  return hipGetLastError();
+}
+
+static bool has_adjoint(const exmi_plan* p) {
+  return p->ndim == 1 ? p->vjp != nullptr : p->rowA[0] != nullptr;
+}
+
+static int ensure_adjoint_ws(exmi_plan* p) {
+  if (p->ndim != 2 || p->G) return EXMI_OK;
+  const int N = p->d.num_points;
+  const long long B = p->d.batch, full = (long long)(N / 2 + 1) * N;
+  bool ok = hipMalloc(&p->G, sizeof(float) * B * 4 * N * N) == hipSuccess &&
+            hipMalloc(&p->LAM, sizeof(cf) * B * full) == hipSuccess &&
+            hipMalloc(&p->RHO, sizeof(cf) * B * full) == hipSuccess &&
+            hipMalloc(&p->H, sizeof(cf) * B * p->kc * N) == hipSuccess &&
+            hipMalloc(&p->P, sizeof(cf) * B * 2 * p->kc * N) == hipSuccess;
+  if (!ok) {
+    hipFree(p->G);
+    hipFree(p->LAM);
+    hipFree(p->RHO);
+    hipFree(p->H);
+    hipFree(p->P);
+    p->G = nullptr;
+    p->LAM = p->RHO = p->H = p->P = nullptr;
+    return fail(EXMI_ERR_ALLOC, "adjoint workspace allocation failed (batch %lld, N %d)", B, N);
+  }
+  return EXMI_OK;
+}
+
+// gout = (d step / d u)^T g at u; gout may alias g (g is read first).
+static int enqueue_step_vjp(exmi_plan* p, const float* u, long long ubs, const float* g, long long gbs, float* gout,
+                            long long obs, hipStream_t st) {
+  const int B = p->d.batch;
+  hipError_t e = hipSuccess;
+  if (p->ndim == 1) {
+    Vjp1DArgs va;
+    fill_args(p, &va.s);
+    va.s.in = u;
+    va.s.in_bstride = ubs;
+    va.g = g;
+    va.g_bstride = gbs;
+    va.gout = gout;
+    va.gout_bstride = obs;
+    e = p->vjp(va, st);
+  } else {
+    Adj2DArgs aa;
+    memset(&aa, 0, sizeof aa);
+    fill_args2d(p, &aa.s);
+    aa.G = p->G;
+    aa.LAM = p->LAM;
+    aa.RHO = p->RHO;
+    aa.H = p->H;
+    aa.P = p->P;
+    Step2DArgs& a = aa.s;
+    // recompute the forward stage fields at u
+    a.u_in = u;
+    a.u_in_bstride = ubs;
+    e = p->row2d[ROW_RA](a, B, st);
+    if (e == hipSuccess) e = p->col2d[COL_CA](a, B, st);
+    aa.stage = 0;
+    if (e == hipSuccess) e = p->rowA[ROWA_RBG](aa, B, st);
+    if (e == hipSuccess) e = p->col2d[COL_CB](a, B, st);
+    aa.stage = 1;
+    if (e == hipSuccess) e = p->rowA[ROWA_RG](aa, B, st);
+    // transposed step
+    a.u_in = g;
+    a.u_in_bstride = gbs;
+    if (e == hipSuccess) e = p->row2d[ROW_RA](a, B, st);
+    if (e == hipSuccess) e = p->colA[COLA_CL](aa, B, st);
+    aa.stage = 1;
+    if (e == hipSuccess) e = p->rowA[ROWA_RP](aa, B, st);
+    if (e == hipSuccess) e = p->colA[COLA_CD](aa, B, st);
+    aa.stage = 0;
+    if (e == hipSuccess) e = p->rowA[ROWA_RP](aa, B, st);
+    if (e == hipSuccess) e = p->colA[COLA_CE](aa, B, st);
+    a.u_out = gout;
+    a.u_out_bstride = obs;
+    a.write_T = 0;
+    if (e == hipSuccess) e = p->row2d[ROW_RC](a, B, st);
+  }
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "adjoint step launch failed: %s", hipGetErrorString(e));
+}
+
+static int ensure_work(exmi_plan* p) {
+  const size_t need = (size_t)p->d.batch * p->d.num_channels * p->d.num_points;
+  if (p->ndim != 1 || p->work_elems >= need) return EXMI_OK;
+  hipFree(p->work);
+  p->work = nullptr;
+  p->work_elems = 0;
+  if (hipMalloc(&p->work, need * sizeof(float)) != hipSuccess) return fail(EXMI_ERR_ALLOC, "workspace allocation failed");
+  p->work_elems = need;
+  return EXMI_OK;
+}
+
+int exmi_step_vjp(exmi_plan* p, const float* u, int64_t u_bs, const float* g, int64_t g_bs, float* g_in,
+                  int64_t gin_bs, void* stream) {
+  if (!p || !u || !g || !g_in) return fail(EXMI_ERR_INVALID, "null argument");
+  if (!has_adjoint(p))
+    return fail(EXMI_ERR_UNSUPPORTED, "no adjoint kernel for this plan (1-D: ETDRK orders 0-2 with convection or "
+                                      "gradient norm; 2-D: gradient norm)");
+  int rc = ensure_adjoint_ws(p);
+  if (rc) return rc;
+  return enqueue_step_vjp(p, u, u_bs, g, g_bs, g_in, gin_bs, (hipStream_t)stream);
+}
+
+int exmi_rollout_vjp(exmi_plan* p, const float* u0, int64_t u0_bs, int32_t n, int32_t include_init,
+                     const float* g_traj, const float* g_final, float* g_u0, int32_t checkpoint_every,
+                     void* stream) {
+  if (!p || !u0 || !g_u0) return fail(EXMI_ERR_INVALID, "null argument");
+  if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
+  if (!has_adjoint(p))
+    return fail(EXMI_ERR_UNSUPPORTED, "no adjoint kernel for this plan (1-D: ETDRK orders 0-2 with convection or "
+                                      "gradient norm; 2-D: gradient norm)");
+  hipStream_t st = (hipStream_t)stream;
+  const int B = p->d.batch;
+  const long long sz = (long long)p->d.num_channels * p->d.num_points * (p->ndim == 2 ? p->d.num_points : 1);
+  const long long bsz = (long long)B * sz;
+  const int off = include_init ? 1 : 0;
+  const long long gts = (long long)(n + off) * sz;  // g_traj batch stride
+  int K = checkpoint_every > 0 && checkpoint_every < n ? checkpoint_every : (n > 0 ? n : 1);
+  const int nseg = n > 0 ? (n + K - 1) / K : 0;
+  int rc = ensure_adjoint_ws(p);
+  if (!rc) rc = ensure_work(p);
+  if (rc) return rc;
+  // workspace: nseg checkpoints (B, sz) | one segment (B, K, sz) | accumulator (B, sz)
+  const size_t need = (size_t)((long long)nseg * bsz + (long long)K * bsz + bsz);
+  if (p->ckpt_elems < need) {
+    HIP_TRY(hipStreamSynchronize(st));
+    hipFree(p->ckpt);
+    p->ckpt = nullptr;
+    p->ckpt_elems = 0;
+    if (hipMalloc(&p->ckpt, need * sizeof(float)) != hipSuccess)
+      return fail(EXMI_ERR_ALLOC, "rollout-gradient workspace allocation failed (%zu floats)", need);
+    p->ckpt_elems = need;
+  }
+  float* cp = p->ckpt;
+  float* seg = cp + (long long)nseg * bsz;
+  float* acc = seg + (long long)K * bsz;
+  hipError_t e = g_final ? copy_batch(g_final, sz, acc, sz, sz, B, st) : fill(acc, bsz, 0.f, st);
+  if (e != hipSuccess) return fail(EXMI_ERR_HIP, "launch failed: %s", hipGetErrorString(e));
+  if (n > 0) {
+    // forward pass keeping every K-th state
+    e = copy_batch(u0, u0_bs, cp, sz, sz, B, st);
+    if (e != hipSuccess) return fail(EXMI_ERR_HIP, "launch failed: %s", hipGetErrorString(e));
+    for (int sidx = 1; sidx < nseg; ++sidx) {
+      rc = enq_repeat_cb(p, cp + (long long)(sidx - 1) * bsz, cp + (long long)sidx * bsz, sz, K, 0, st);
+      if (rc) return rc;
+    }
+    // backward over the segments, recomputing each segment's states
+    for (int sidx = nseg - 1; sidx >= 0; --sidx) {
+      const int t0 = sidx * K;
+      const int len = (n - t0) < K ? (n - t0) : K;
+      rc = enqueue_rollout_nt(p, p->roll_nt, cp + (long long)sidx * bsz, sz, seg, len - 1, 1, st);
+      if (rc) return rc;
+      const long long segs = (long long)len * sz;  // segment batch stride
+      for (int i = len - 1; i >= 0; --i) {
+        const int t = t0 + i;
+        if (g_traj) {
+          e = axpy_batch(acc, sz, g_traj + (long long)(t + off) * sz, gts, sz, B, st);
+          if (e != hipSuccess) return fail(EXMI_ERR_HIP, "launch failed: %s", hipGetErrorString(e));
+        }
+        rc = enqueue_step_vjp(p, seg + (long long)i * sz, segs, acc, sz, acc, sz, st);
+        if (rc) return rc;
+      }
+    }
+  }
+  if (g_traj && include_init) {
+    e = axpy_batch(acc, sz, g_traj, gts, sz, B, st);
+    if (e != hipSuccess) return fail(EXMI_ERR_HIP, "launch failed: %s", hipGetErrorString(e));
+  }
+  e = copy_batch(acc, sz, g_u0, sz, sz, B, st);
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "launch failed: %s", hipGetErrorString(e));
 }
 
 }  // extern "C"

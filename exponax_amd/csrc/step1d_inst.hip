@@ -1,7 +1,8 @@
-// Instantiates the fused 1-D step kernels and the standalone batched real
+// Instantiates the fused 1-D step and adjoint kernels and the standalone batched real
 // FFT kernels for one transform size, selected at compile time with
 // -DEXMI_M=<N/2> -DEXMI_PT=<points per lane>. The build compiles this file
 // once per supported size.
+#include "adjoint1d.hpp"
 #include "step1d.hpp"
 
 #ifndef EXMI_M
@@ -125,6 +126,31 @@ RollFn EXMI_CAT(get_roll1d_, EXMI_M)(int kind, int order, int nt) {
     if (nt == 2) return pick_roll<M, PT, 2>(kind, order);
   }
   return nullptr;
+}
+
+using VjpFn = hipError_t (*)(const Vjp1DArgs&, hipStream_t);
+
+template <int M, int PT, int KIND>
+static VjpFn pick_vjp_order(int order) {
+  switch (order) {
+    case 1: return &launch_vjp1d<M, PT, KIND, 1>;
+    case 2: return &launch_vjp1d<M, PT, KIND, 2>;
+    default: return nullptr;
+  }
+}
+
+// Adjoint step kernels: ETDRK orders 0-2, convection (both forms) and
+// gradient norm.
+VjpFn EXMI_CAT(get_vjp1d_, EXMI_M)(int kind, int order) {
+  constexpr int M = EXMI_M, PT = EXMI_PT;
+  if (order == 0) return &launch_vjp1d<M, PT, NL_ZERO, 0>;
+  switch (kind) {
+    case NL_ZERO: return pick_vjp_order<M, PT, NL_ZERO>(order);
+    case NL_CONV_CONS: return pick_vjp_order<M, PT, NL_CONV_CONS>(order);
+    case NL_CONV_NONCONS: return pick_vjp_order<M, PT, NL_CONV_NONCONS>(order);
+    case NL_GRADNORM: return pick_vjp_order<M, PT, NL_GRADNORM>(order);
+    default: return nullptr;
+  }
 }
 
 hipError_t EXMI_CAT(launch_fft1d_, EXMI_M)(bool inverse, const float* in, float* out, int count,

@@ -1,6 +1,7 @@
-// Instantiates the 2-D row and column kernels for one grid size, selected at
+// Instantiates the 2-D row and column kernels (step and adjoint) for one grid size, selected at
 // compile time with -DEXMI_N2=<N>. The build compiles this file once per
 // supported size.
+#include "adjoint2d.hpp"
 #include "step2d.hpp"
 
 #ifndef EXMI_N2
@@ -27,6 +28,27 @@ static Launch2D pick2d(int is_row, int mode) {
       case COL_CA: return &launch_col2d<N, COL_CA, KIND>;
       case COL_CB: return &launch_col2d<N, COL_CB, KIND>;
       case COL_CC: return &launch_col2d<N, COL_CC, KIND>;
+    }
+  }
+  return nullptr;
+}
+
+using LaunchAdj2D = hipError_t (*)(const Adj2DArgs&, int, hipStream_t);
+
+// adjoint modes (gradient-norm nonlinearity only)
+LaunchAdj2D EXMI_CAT(get_adj2d_, EXMI_N2)(int is_row, int mode) {
+  constexpr int N = EXMI_N2;
+  if (is_row) {
+    switch (mode) {
+      case ROWA_RBG: return &launch_row2d_adj<N, ROWA_RBG>;
+      case ROWA_RG: return &launch_row2d_adj<N, ROWA_RG>;
+      case ROWA_RP: return &launch_row2d_adj<N, ROWA_RP>;
+    }
+  } else {
+    switch (mode) {
+      case COLA_CL: return &launch_col2d_adj<N, COLA_CL>;
+      case COLA_CD: return &launch_col2d_adj<N, COLA_CD>;
+      case COLA_CE: return &launch_col2d_adj<N, COLA_CE>;
     }
   }
   return nullptr;

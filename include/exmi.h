@@ -160,6 +160,33 @@ int exmi_rollout_group(exmi_plan* const* plans, int32_t n_plans, const float* co
 int exmi_repeat(exmi_plan* plan, const float* u0, int64_t u0_bstride, float* u_out, int32_t n_steps,
                 int32_t use_graph, void* stream);
 
+/* Adjoint of one step: g_in = (d step(u) / d u)^T g_out at u = u_in, all
+ * physical (B, C, N^D) float32 with batch strides in floats; g_in may equal
+ * g_out (not u_in). Inner products are the plain sums over grid points, as
+ * jax.grad of a loss sum(step(u) * g_out) would give.
+ * Built for ETDRK orders 0-2 with convection or gradient norm in 1-D, and for
+ * the gradient-norm nonlinearity (ETDRK2) in 2-D; other plans return
+ * EXMI_ERR_UNSUPPORTED.
+ * Replaces jax.vjp of BaseStepper.__call__ (_base_stepper.py:241-271), i.e.
+ * the transposes JAX derives for etdrk/_etdrk_2.py:91-102 and the nonlinear
+ * functions (nonlin_fun/_convection.py, _gradient_norm.py). */
+int exmi_step_vjp(exmi_plan* plan, const float* u_in, int64_t in_bstride, const float* g_out, int64_t g_bstride,
+                  float* g_in, int64_t gin_bstride, void* stream);
+
+/* Gradient of a loss through a rollout with segment checkpointing:
+ * g_u0 = sum_t (d u_t / d u0)^T g_t, where u_t are the states that
+ * exmi_rollout(plan, u0, ..., n_steps, include_init) produces. g_traj holds
+ * the cotangents of that trajectory, (B, n_steps[+1], C, N^D) contiguous, or
+ * is NULL. g_final, (B, C, N^D) contiguous or NULL, is an extra cotangent of
+ * the final state (exponax.repeat). g_u0 is (B, C, N^D) contiguous.
+ * Every checkpoint_every-th state is kept (<= 0: all states, i.e. one
+ * segment). Each segment is recomputed from its checkpoint on the way back.
+ * Workspace: (ceil(n/K) + K + 1) * B * C * N^D floats, kept by the plan.
+ * Replaces jax.grad through exponax.rollout / repeat (_utils.py:92-254,
+ * lax.scan at :149-172, :239-251). */
+int exmi_rollout_vjp(exmi_plan* plan, const float* u0, int64_t u0_bstride, int32_t n_steps, int32_t include_init,
+                     const float* g_traj, const float* g_final, float* g_u0, int32_t checkpoint_every, void* stream);
+
 /* Batched real FFTs with the reference conventions: forward rfftn over the
  * last D axes, norm="backward"; inverse irfftn with 1/N^D scaling, Im of the
  * last axis' DC/Nyquist ignored. `count` transforms, contiguous.

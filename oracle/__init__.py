@@ -20,4 +20,4 @@ Pinning (each function's check lives in tests/test_oracle_*.py):
   tests/test_builtin_solvers.py; rollout == loop.
 Bit-level agreement with JAX's XLA FFT is not pinnable here (JAX is absent).
 """
-from . import etdrk, ic, nonlin, spectral, stepper  # noqa: F401
+from . import adjoint, etdrk, ic, nonlin, spectral, stepper  # noqa: F401

@@ -1,0 +1,230 @@
+"""GPU parity of the adjoint kernels (exmi_step_vjp, exmi_rollout_vjp).
+
+They are compared against the oracle's float64 VJP (oracle/adjoint.py).
+That VJP is pinned on the CPU by dot-product identities and central
+differences.
+
+The full differentiation case is:
+- KuramotoSivashinsky 2-D, 128^2, L = 60, dt = 0.05, ETDRK2, batch 32;
+- the gradient of MSE(u_50, target) with respect to u_0, checkpoints every 10 steps;
+- checked against the oracle gradient, and against float64 central differences
+  (eps = 1e-3) along random directions.
+
+Tolerances are relative L2 errors. Per step, 1e-5: the float32 rounding
+floor, as for the forward step. Through 50 steps, 1e-4.
+"""
+import numpy as np
+import pytest
+
+import exponax_amd as ex
+from oracle import adjoint as oad
+from oracle import stepper as ost
+from tests.helpers import ics
+
+pytestmark = pytest.mark.gpu
+
+RTOL_VJP = 1e-5
+RTOL_GRAD = 1e-4
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _vjp_case(gpu, cpu, u, g, tol=RTOL_VJP):
+    import torch
+
+    out = gpu.step_vjp(torch.tensor(u, device="cuda"), torch.tensor(g, device="cuda")).cpu().numpy()
+    ref = oad.step_vjp(cpu, u.astype(np.float64), g.astype(np.float64))
+    err = _rel(out, ref)
+    print(f"  step vjp rel-L2 {err:.2e}", flush=True)
+    assert np.isfinite(out).all()
+    assert err <= tol
+    return out
+
+
+CASES_1D = [
+    ("ks_cons", lambda o, dt: ex.stepper.KuramotoSivashinskyConservative(1, 100.0, 256, dt, order=o),
+     lambda o, dt: ost.KuramotoSivashinskyConservative(1, 100.0, 256, dt, order=o), 100.0, 256),
+    ("ks_gradnorm", lambda o, dt: ex.stepper.KuramotoSivashinsky(1, 60.0, 128, dt, order=o),
+     lambda o, dt: ost.KuramotoSivashinsky(1, 60.0, 128, dt, order=o), 60.0, 128),
+    ("burgers", lambda o, dt: ex.stepper.Burgers(1, 1.0, 1024, dt * 1e-2, diffusivity=3e-3, order=o),
+     lambda o, dt: ost.Burgers(1, 1.0, 1024, dt * 1e-2, diffusivity=3e-3, order=o), 1.0, 1024),
+]
+
+
+@pytest.mark.parametrize("order", [0, 1, 2])
+@pytest.mark.parametrize("name,mk_gpu,mk_cpu,L,N", CASES_1D, ids=[c[0] for c in CASES_1D])
+def test_step_vjp_1d(device, name, mk_gpu, mk_cpu, L, N, order):
+    print(f"[case] vjp1d {name} order={order}", flush=True)
+    gpu, cpu = mk_gpu(order, 0.1), mk_cpu(order, 0.1)
+    u, _ = ics(4, 1, L, N, 6, seed=21)
+    g, _ = ics(4, 1, L, N, 12, seed=22)
+    _vjp_case(ex.vmap(gpu), cpu, u, g)
+
+
+def test_step_vjp_burgers_exact_form(device):
+    """The literal u u_x evaluation (EXMI_FLAG_EXACT_FORM) has its own
+    transpose (two forward transforms)."""
+    print("[case] vjp1d burgers exact form", flush=True)
+    gpu = ex.stepper.Burgers(1, 1.0, 256, 1e-3, diffusivity=1e-2)
+    gpu.exact_form = True
+    cpu = ost.Burgers(1, 1.0, 256, 1e-3, diffusivity=1e-2)
+    u, _ = ics(3, 1, 1.0, 256, 6, seed=23)
+    g, _ = ics(3, 1, 1.0, 256, 20, seed=24)
+    _vjp_case(ex.vmap(gpu), cpu, u, g)
+
+
+def test_step_vjp_ensemble(device):
+    """Per-trajectory tables (Burgers with per-trajectory viscosity)."""
+    print("[case] vjp1d ensemble", flush=True)
+    nus = [2e-3, 5e-3, 1e-2, 3e-2]
+    gpu = ex.ensemble(lambda nu: ex.stepper.Burgers(1, 1.0, 512, 1e-3, diffusivity=nu), nus)
+    cpu = ost.Ensemble([ost.Burgers(1, 1.0, 512, 1e-3, diffusivity=nu) for nu in nus])
+    u, _ = ics(4, 1, 1.0, 512, 8, seed=25)
+    g, _ = ics(4, 1, 1.0, 512, 16, seed=26)
+    _vjp_case(gpu, cpu, u, g)
+
+
+@pytest.mark.parametrize("N", [32, 64, 128, 256])
+def test_step_vjp_ks2d(device, N):
+    print(f"[case] vjp2d ks N={N}", flush=True)
+    L = 60.0 * N / 128
+    gpu = ex.stepper.KuramotoSivashinsky(2, L, N, 0.05)
+    cpu = ost.KuramotoSivashinsky(2, L, N, 0.05)
+    u, _ = ics(3, 2, L, N, 5, seed=31)
+    g, _ = ics(3, 2, L, N, 9, seed=32)
+    _vjp_case(ex.vmap(gpu), cpu, u, g)
+
+
+def test_step_vjp_ks2d_ensemble_no_zero_mode_fix(device):
+    """Per-instance tables and the gradient norm without mean removal."""
+    import torch
+
+    print("[case] vjp2d ensemble", flush=True)
+    N = 64
+    scales = [0.8, 1.0, 1.3]
+    gpu = ex.ensemble(lambda s: ex.stepper.KuramotoSivashinsky(2, 30.0, N, 0.05, second_order_scale=s), scales)
+    cpu = ost.Ensemble([ost.KuramotoSivashinsky(2, 30.0, N, 0.05, second_order_scale=s) for s in scales])
+    u, _ = ics(3, 2, 30.0, N, 5, seed=33)
+    g, _ = ics(3, 2, 30.0, N, 9, seed=34)
+    _vjp_case(gpu, cpu, u, g)
+    # same kernel, zero_mode_fix off: compare with the oracle's variant
+    gk = ex.stepper.KuramotoSivashinsky(2, 30.0, N, 0.05)
+    gk.nonlinear_fun.zero_mode_fix = False
+    ck = ost.KuramotoSivashinsky(2, 30.0, N, 0.05)
+    ck.nonlinear_fun.zero_mode_fix = False
+    g = g + 3.0  # a mean in the cotangent
+    out = gk.step_vjp(torch.tensor(u, device="cuda"), torch.tensor(g, device="cuda")).cpu().numpy()
+    assert _rel(out, oad.step_vjp(ck, u.astype(np.float64), g.astype(np.float64))) <= RTOL_VJP
+
+
+@pytest.mark.parametrize("dims", [1, 2])
+def test_rollout_vjp_checkpointing(device, dims):
+    """The gradient through a 23-step rollout does not depend on the
+    checkpoint interval and matches the oracle's reverse pass. Covers g_traj
+    with include_init, and g_final."""
+    import torch
+
+    print(f"[case] rollout vjp D={dims}", flush=True)
+    if dims == 1:
+        gpu = ex.stepper.KuramotoSivashinskyConservative(1, 100.0, 256, 0.1)
+        cpu = ost.KuramotoSivashinskyConservative(1, 100.0, 256, 0.1)
+        u0, _ = ics(3, 1, 100.0, 256, 5, seed=41)
+        gt, _ = ics(3 * 24, 1, 100.0, 256, 10, seed=42)
+        gt = gt.reshape(3, 24, 1, 256)
+    else:
+        gpu = ex.stepper.KuramotoSivashinsky(2, 30.0, 64, 0.05)
+        cpu = ost.KuramotoSivashinsky(2, 30.0, 64, 0.05)
+        u0, _ = ics(2, 2, 30.0, 64, 5, seed=43)
+        gt, _ = ics(2 * 24, 2, 30.0, 64, 10, seed=44)
+        gt = gt.reshape((2, 24) + gt.shape[1:])
+    n = 23
+    x = torch.tensor(u0, device="cuda")
+    gtt = torch.tensor(gt, device="cuda")
+    outs = [gpu.rollout_vjp(x, n, g_traj=gtt, include_init=True, checkpoint_every=k).cpu().numpy()
+            for k in (1, 5, 10, 23, 0)]
+    for o in outs[1:]:
+        assert _rel(o, outs[0]) <= 1e-6
+    ref = np.stack([oad.rollout_vjp(cpu, u0[b].astype(np.float64), n, np.moveaxis(gt[b], 0, 0), include_init=True)
+                    for b in range(u0.shape[0])])
+    err = _rel(outs[0], ref)
+    print(f"  rollout vjp rel-L2 {err:.2e}", flush=True)
+    assert err <= RTOL_GRAD
+    # g_final alone equals repeat's gradient
+    gf = gt[:, -1]
+    o = gpu.rollout_vjp(x, n, g_final=torch.tensor(gf, device="cuda"), checkpoint_every=4).cpu().numpy()
+    ref = np.stack([oad.repeat_vjp(cpu, u0[b].astype(np.float64), n, gf[b].astype(np.float64))
+                    for b in range(u0.shape[0])])
+    assert _rel(o, ref) <= RTOL_GRAD
+
+
+def test_autograd_functions(device):
+    """torch.autograd through step, rollout and repeat equals the explicit
+    VJP entry points."""
+    import torch
+
+    print("[case] autograd", flush=True)
+    s = ex.stepper.KuramotoSivashinsky(2, 30.0, 64, 0.05)
+    u0, _ = ics(2, 2, 30.0, 64, 5, seed=51)
+    w, _ = ics(2 * 8, 2, 30.0, 64, 7, seed=52)
+    w = torch.tensor(w.reshape((2, 8) + w.shape[1:]), device="cuda")
+    x = torch.tensor(u0, device="cuda", requires_grad=True)
+    tr = ex.autograd.rollout(s, x, 7, include_init=True, checkpoint_every=3)
+    (tr * w).sum().backward()
+    ref = s.rollout_vjp(x.detach(), 7, g_traj=w, include_init=True, checkpoint_every=7)
+    assert _rel(x.grad.cpu().numpy(), ref.cpu().numpy()) <= 1e-6
+    x.grad = None
+    y = ex.autograd.repeat(s, x, 7, checkpoint_every=2)
+    (y * w[:, 0]).sum().backward()
+    ref = s.rollout_vjp(x.detach(), 7, g_final=w[:, 0])
+    assert _rel(x.grad.cpu().numpy(), ref.cpu().numpy()) <= 1e-6
+    x.grad = None
+    y = ex.autograd.step(s, x)
+    (y * w[:, 1]).sum().backward()
+    assert _rel(x.grad.cpu().numpy(), s.step_vjp(x.detach(), w[:, 1]).cpu().numpy()) <= 1e-7
+
+
+def test_ks2d_mse_gradient_128(device):
+    """The differentiation workload: KS 2-D 128^2, 32 trajectories, 50 steps,
+    checkpoints every 10. The GPU gradient of MSE(u_50, target) matches the
+    oracle's float64 reverse pass and float64 central differences (eps=1e-3)
+    along two random directions."""
+    import torch
+
+    print("[case] ks2d mse gradient 128^2 x 32, 50 steps", flush=True)
+    N, L, dt, n, B = 128, 60.0, 0.05, 50, 32
+    s = ex.stepper.KuramotoSivashinsky(2, L, N, dt)
+    u0, _ = ics(B, 2, L, N, 5, seed=4)
+    tg, _ = ics(B, 2, L, N, 5, seed=40)
+    x = torch.tensor(u0, device="cuda", requires_grad=True)
+    target = torch.tensor(tg, device="cuda")
+    y = ex.autograd.repeat(s, x, n, checkpoint_every=10)
+    loss = ((y - target) ** 2).mean()
+    loss.backward()
+    g = x.grad.cpu().numpy()
+    assert np.isfinite(g).all()
+    cpu = ost.KuramotoSivashinsky(2, L, N, dt)
+    # oracle on two of the trajectories (the loss's mean couples only through 1/size)
+    for b in (0, 17):
+        _, gref = oad.mse_grad(cpu, u0[b].astype(np.float64), tg[b].astype(np.float64), n)
+        gref = gref / B  # mean over the whole batch
+        err = _rel(g[b], gref)
+        print(f"  traj {b}: gradient rel-L2 vs float64 oracle {err:.2e}", flush=True)
+        assert err <= RTOL_GRAD
+
+    def loss64(u):
+        for _ in range(n):
+            u = cpu.step_batch(u)
+        return float(np.mean((u - tg.astype(np.float64)) ** 2))
+
+    rng = np.random.default_rng(41)
+    for _ in range(2):
+        v = rng.standard_normal(u0.shape)
+        eps = 1e-3
+        fd = (loss64(u0.astype(np.float64) + eps * v) - loss64(u0.astype(np.float64) - eps * v)) / (2 * eps)
+        an = float(np.sum(g.astype(np.float64) * v))
+        print(f"  directional derivative: adjoint {an:.8e}, central FD {fd:.8e}", flush=True)
+        assert abs(an - fd) <= RTOL_GRAD * abs(fd)

@@ -31,6 +31,11 @@ def _pair(kind, N, order, L=None, dt=None):
         dt = dt or 1e-3
         return (ex.stepper.Burgers(1, L, N, dt, diffusivity=3e-3, order=order),
                 ost.Burgers(1, L, N, dt, diffusivity=3e-3, order=order, dtype=np.float32), L)
+    if kind == "burgers_exact":
+        # literal u u_x with two inverse transforms (EXMI_FLAG_EXACT_FORM)
+        g, c, L = _pair("burgers", N, order, L, dt)
+        g.exact_form = True
+        return g, c, L
     if kind == "burgers_cons":
         L = L or 1.0
         dt = dt or 1e-3
@@ -42,7 +47,7 @@ def _pair(kind, N, order, L=None, dt=None):
 
 
 @pytest.mark.parametrize("N", [32, 64, 256, 1024, 2048])
-@pytest.mark.parametrize("kind", ["ks_cons", "ks", "burgers", "burgers_cons"])
+@pytest.mark.parametrize("kind", ["ks_cons", "ks", "burgers", "burgers_exact", "burgers_cons"])
 @pytest.mark.parametrize("order", [1, 2, 3, 4])
 def test_step_parity(device, kind, N, order):
     print(f"[case] step kind={kind} N={N} order={order}", flush=True)
