@@ -62,7 +62,7 @@ def test_step_vjp_1d(device, name, mk_gpu, mk_cpu, L, N, order):
     gpu, cpu = mk_gpu(order, 0.1), mk_cpu(order, 0.1)
     u, _ = ics(4, 1, L, N, 6, seed=21)
     g, _ = ics(4, 1, L, N, 12, seed=22)
-    _vjp_case(ex.vmap(gpu), cpu, u, g)
+    _vjp_case(gpu, cpu, u, g)
 
 
 def test_step_vjp_burgers_exact_form(device):
@@ -74,7 +74,7 @@ def test_step_vjp_burgers_exact_form(device):
     cpu = ost.Burgers(1, 1.0, 256, 1e-3, diffusivity=1e-2)
     u, _ = ics(3, 1, 1.0, 256, 6, seed=23)
     g, _ = ics(3, 1, 1.0, 256, 20, seed=24)
-    _vjp_case(ex.vmap(gpu), cpu, u, g)
+    _vjp_case(gpu, cpu, u, g)
 
 
 def test_step_vjp_ensemble(device):
@@ -96,7 +96,7 @@ def test_step_vjp_ks2d(device, N):
     cpu = ost.KuramotoSivashinsky(2, L, N, 0.05)
     u, _ = ics(3, 2, L, N, 5, seed=31)
     g, _ = ics(3, 2, L, N, 9, seed=32)
-    _vjp_case(ex.vmap(gpu), cpu, u, g)
+    _vjp_case(gpu, cpu, u, g)
 
 
 def test_step_vjp_ks2d_ensemble_no_zero_mode_fix(device):
