@@ -22,7 +22,9 @@ Other workloads (--workload): ks1d (Kuramoto-Sivashinsky conservative 1-D,
 N=256, one trajectory, 500 steps), kolmogorov2d (Kolmogorov vorticity,
 32 x 128^2 + 16 x 256^2 + 8 x 512^2 instances with per-instance Re, 2000
 steps in one graph), ks2d (Kuramoto-Sivashinsky 2-D 128^2, 32 trajectories,
-50 steps). They print the same JSON contract for their own workload.
+50 steps), ks2d_grad (the gradient of an MSE loss through those 50 steps,
+checkpoints every 10). They print the same JSON contract for their own
+workload.
 """
 from __future__ import annotations
 
@@ -311,7 +313,84 @@ def wl_ks2d(args, ex, _lib, torch):
     return res
 
 
-WORKLOADS = {"burgers1d": wl_burgers1d, "ks1d": wl_ks1d, "kolmogorov2d": wl_kolmogorov2d, "ks2d": wl_ks2d}
+def wl_ks2d_grad(args, ex, _lib, torch):
+    """Gradient of MSE(u_50, target) w.r.t. u_0 through 50 KS 2-D steps
+    (128^2, 32 trajectories), checkpoints every 10 steps: the forward pass
+    that keeps checkpoints, the segment recomputation and the adjoint steps,
+    all on the GPU. value = trajectories * steps / wall time of one
+    loss-and-gradient evaluation."""
+    N, L, DT = 128, 60.0, 0.05
+    B, K, W, CK = args.batch or 32, args.steps or 50, args.warmup, 10
+    u0 = tfs_ics(B, 2, N, L, 5, np.random.default_rng(4))
+    tg = tfs_ics(B, 2, N, L, 5, np.random.default_rng(40))
+    st = ex.stepper.KuramotoSivashinsky(2, L, N, DT)
+    x = torch.tensor(u0, dtype=torch.float32, device="cuda:0")
+    target = torch.tensor(tg, dtype=torch.float32, device="cuda:0")
+
+    def loss_and_grad():
+        xr = x.clone().requires_grad_(True)
+        y = ex.autograd.repeat(st, xr, K, checkpoint_every=CK)
+        loss = ((y - target) ** 2).mean()
+        loss.backward()
+        return loss, xr.grad
+
+    for _ in range(max(1, W // 10)):
+        loss_and_grad()
+        st.repeat_batch(x, K)
+    torch.cuda.synchronize()
+    out = {}
+    wall_f, _ = timed_rollout(lambda: out.__setitem__("f", st.repeat_batch(x, K)), torch.cuda.synchronize)
+    wall, gpu_s = timed_rollout(lambda: out.__setitem__("g", loss_and_grad()), torch.cuda.synchronize)
+    loss, grad = out["g"]
+    # per state-step: checkpoint pass and recomputation (8 N^2 each) plus the
+    # adjoint step (reads u_t and the cotangent, writes the new cotangent)
+    per_state = 8 * N * N * 2 + 12 * N * N
+    n_launches = K * 12 + 2 * K * 6
+    res = result_line("ks2d_n128_b32_grad50_ckpt10",
+                      {"equation": "KuramotoSivashinsky", "num_points": N, "domain_extent": L, "dt": DT,
+                       "order": 2, "batch": B, "rollout_steps": K, "checkpoint_every": CK,
+                       "loss": "mean((u_50 - target)^2)", "ic": "TFS cutoff 5 seed 4, target seed 40"},
+                      B * K / wall, K, W, wall, gpu_s / n_launches, B * per_state / (n_launches / K), n_launches,
+                      "row2d/col2d (+ adjoint modes) kernels, 12 launches per adjoint step", None)
+    res["unit_note"] = "state-steps/s of a full loss+gradient evaluation"
+    res["forward_only_state_steps_per_s"] = B * K / wall_f
+    res["gradient_cost_over_forward"] = wall / wall_f
+    res["loss"] = float(loss.item())
+    if not args.no_cpu_baseline:
+        from oracle import adjoint as oad
+        from oracle import stepper as ost
+
+        g = grad.cpu().numpy().astype(np.float64)
+        cpu = ost.KuramotoSivashinsky(2, L, N, DT)
+        par = {}
+        t0 = time.perf_counter()
+        nb = 2
+        for b in range(nb):
+            _, gref = oad.mse_grad(cpu, u0[b], tg[b], K)
+            gref = gref / B
+            par[f"grad_rel_l2_fp64_traj{b}"] = float(np.linalg.norm(g[b] - gref) / np.linalg.norm(gref))
+        dt = time.perf_counter() - t0
+
+        def loss64(u):
+            for _ in range(K):
+                u = cpu.step_batch(u)
+            return float(np.mean((u - tg[:1]) ** 2))
+
+        v = np.random.default_rng(41).standard_normal(u0[:1].shape)
+        eps = 1e-3
+        fd = (loss64(u0[:1] + eps * v) - loss64(u0[:1] - eps * v)) / (2 * eps) / B
+        an = float(np.sum(g[:1] * v))
+        par["fd_directional_rel_err"] = abs(an - fd) / abs(fd)
+        res["parity"] = par
+        res["cpu_baseline"] = {"value": nb * K / dt, "unit": "state-steps/s (loss+gradient)",
+                               "cores": os.cpu_count() or 1, "kind": "port",
+                               "sample": f"NumPy float64 reverse pass (oracle/adjoint.py), {nb} trajectories x "
+                                         f"{K} steps ({dt:.1f} s)"}
+    return res
+
+
+WORKLOADS = {"burgers1d": wl_burgers1d, "ks1d": wl_ks1d, "kolmogorov2d": wl_kolmogorov2d, "ks2d": wl_ks2d,
+             "ks2d_grad": wl_ks2d_grad}
 
 
 def main():

@@ -34,7 +34,7 @@ EXPORTED = (
     "exmi_plan_create", "exmi_plan_destroy", "exmi_step", "exmi_step_fourier", "exmi_nonlinear",
     "exmi_rollout", "exmi_rollout_prepare", "exmi_rollout_group", "exmi_repeat", "exmi_rfft", "exmi_irfft", "exmi_plan_num_coefs", "exmi_plan_get_tables",
     "exmi_plan_rollout_strategy", "exmi_plan_set_rollout_strategy", "exmi_last_error", "exmi_info",
-    "exmi_step_vjp", "exmi_rollout_vjp",
+    "exmi_step_vjp", "exmi_rollout_vjp", "exmi_nrmse", "exmi_spectrum", "exmi_map_resolution",
 )
 
 
@@ -96,6 +96,9 @@ def _declare(lib):
     lib.exmi_plan_set_rollout_strategy.argtypes = [vp, i32]
     lib.exmi_step_vjp.argtypes = [vp, vp, i64, vp, i64, vp, i64, vp]
     lib.exmi_rollout_vjp.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, i32, vp]
+    lib.exmi_nrmse.argtypes = [i32, i32, i64, vp, i64, vp, i64, vp, vp]
+    lib.exmi_spectrum.argtypes = [i32, i32, i32, vp, vp, i32, vp]
+    lib.exmi_map_resolution.argtypes = [i32, i32, i32, i32, vp, vp, i32, vp]
     lib.exmi_last_error.restype = C.c_char_p
     lib.exmi_info.argtypes = [C.c_char_p, i32]
     for name in EXPORTED:

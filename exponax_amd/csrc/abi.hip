@@ -156,6 +156,9 @@ static int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+// the error channel for the library's other translation units
+int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
 #define HIP_TRY(expr)                                                                          \
   do {                                                                                         \
     hipError_t e_ = (expr);                                                                    \

@@ -1,0 +1,174 @@
+// GPU counterparts of the functions on either side of the ETDRK path that
+// measure states or change their resolution:
+//   nRMSE                    metrics/_spatial.py:537-586 (spatial_norm, mode "normalized")
+//   get_spectrum (1-D)       _spectral.py:866-1030
+//   map_between_resolutions  _interpolation.py:195-285 (1-D)
+// Transforms go through the library's own real FFT kernels (exmi_rfft /
+// exmi_irfft); the kernels here are the reductions and per-mode maps
+// between them. Workspaces are stream-ordered allocations.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/exmi.h"
+
+namespace exmi {
+int set_error(int code, const char* msg);  // abi.hip: the thread-local error channel
+}
+
+namespace {
+
+constexpr int WG = 256;
+
+// out[b] = sum_c ||p_bc - r_bc|| / ||r_bc|| with float64 accumulation.
+__global__ void __launch_bounds__(WG) nrmse_kernel(const float* __restrict__ p, long long pbs,
+                                                   const float* __restrict__ r, long long rbs, int C, long long n,
+                                                   float* __restrict__ out) {
+  __shared__ double sh[2][WG];
+  const int b = blockIdx.x;
+  double total = 0.0;
+  for (int c = 0; c < C; ++c) {
+    const float* pc = p + (long long)b * pbs + (long long)c * n;
+    const float* rc = r + (long long)b * rbs + (long long)c * n;
+    double se = 0.0, sr = 0.0;
+    for (long long i = threadIdx.x; i < n; i += WG) {
+      const double d = (double)pc[i] - (double)rc[i];
+      se += d * d;
+      sr += (double)rc[i] * (double)rc[i];
+    }
+    sh[0][threadIdx.x] = se;
+    sh[1][threadIdx.x] = sr;
+    __syncthreads();
+    for (int w = WG / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) {
+        sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+        sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+      }
+      __syncthreads();
+    }
+    total += sqrt(sh[0][0]) / sqrt(sh[1][0]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[b] = (float)total;
+}
+
+// 1-D spectrum of X (count, M+1) complex: power 0.5 |X|^2 / (s_rec s_norm),
+// else |X| / s_rec; s_rec = N at k = 0 and k = N/2, N/2 elsewhere; s_norm = N
+// (build_scaling_array "reconstruction" / "norm_compensation").
+__global__ void spectrum_kernel(const float2* __restrict__ X, int M, long long total, int power, float* out) {
+  const float N = 2.f * M;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int k = (int)(i % (M + 1));
+    const float2 x = X[i];
+    const float mag = sqrtf(x.x * x.x + x.y * x.y);
+    const float rec = (k == 0 || k == M) ? N : 0.5f * N;
+    const float m = mag / rec;
+    out[i] = power ? 0.5f * m * (mag / N) : m;
+  }
+}
+
+// Spectral resampling of (count, Mo+1) -> (count, Mn+1): copy the modes
+// k <= min(No, Nn)/2 times Nn/No (norm compensation), zero the old Nyquist
+// before upsampling and the new Nyquist after downsampling (oddball rules).
+__global__ void resample_kernel(const float2* __restrict__ A, int Mo, float2* __restrict__ Bn, int Mn, int count,
+                                int oddball) {
+  const long long total = (long long)count * (Mn + 1);
+  const int kmax = Mo < Mn ? Mo : Mn;
+  const float f = (float)Mn / (float)Mo;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long t = i / (Mn + 1);
+    const int k = (int)(i % (Mn + 1));
+    float2 v = make_float2(0.f, 0.f);
+    if (k <= kmax) {
+      v = A[t * (Mo + 1) + k];
+      v.x *= f;
+      v.y *= f;
+      if (oddball && Mn > Mo && k == Mo) v = make_float2(0.f, 0.f);
+      if (oddball && Mo > Mn && k == Mn) v = make_float2(0.f, 0.f);
+    }
+    Bn[i] = v;
+  }
+}
+
+int grid_for(long long n) {
+  long long g = (n + WG - 1) / WG;
+  if (g > 16384) g = 16384;
+  return g < 1 ? 1 : (int)g;
+}
+
+bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+int exmi_nrmse(int32_t batch, int32_t channels, int64_t n, const float* pred, int64_t pred_bstride, const float* ref,
+               int64_t ref_bstride, float* out, void* stream) {
+  if (!pred || !ref || !out) return exmi::set_error(EXMI_ERR_INVALID, "exmi_nrmse: null argument");
+  if (batch < 0 || channels < 1 || n < 1) return exmi::set_error(EXMI_ERR_INVALID, "exmi_nrmse: bad sizes");
+  if (batch == 0) return EXMI_OK;
+  hipLaunchKernelGGL(nrmse_kernel, dim3(batch), dim3(WG), 0, (hipStream_t)stream, pred, (long long)pred_bstride,
+                     ref, (long long)ref_bstride, channels, (long long)n, out);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? EXMI_OK : exmi::set_error(EXMI_ERR_HIP, hipGetErrorString(e));
+}
+
+int exmi_spectrum(int32_t D, int32_t N, int32_t count, const float* x, float* out, int32_t power, void* stream) {
+  if (D != 1) return exmi::set_error(EXMI_ERR_UNSUPPORTED, "exmi_spectrum: only num_spatial_dims = 1 is built");
+  if (!x || !out) return exmi::set_error(EXMI_ERR_INVALID, "exmi_spectrum: null argument");
+  if (count <= 0) return EXMI_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int M = N / 2;
+  const long long total = (long long)count * (M + 1);
+  float2* X = nullptr;
+  if (hipMallocAsync((void**)&X, sizeof(float2) * total, st) != hipSuccess)
+    return exmi::set_error(EXMI_ERR_ALLOC, "exmi_spectrum: workspace allocation failed");
+  int rc = exmi_rfft(1, N, count, x, reinterpret_cast<float*>(X), stream);
+  if (rc == EXMI_OK) {
+    hipLaunchKernelGGL(spectrum_kernel, dim3(grid_for(total)), dim3(WG), 0, st, X, M, total, power, out);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = exmi::set_error(EXMI_ERR_HIP, hipGetErrorString(e));
+  }
+  (void)hipFreeAsync(X, st);
+  return rc;
+}
+
+int exmi_map_resolution(int32_t D, int32_t n_old, int32_t n_new, int32_t count, const float* x, float* y,
+                        int32_t oddball_zero, void* stream) {
+  if (D != 1) return exmi::set_error(EXMI_ERR_UNSUPPORTED, "exmi_map_resolution: only num_spatial_dims = 1 is built");
+  if (!x || !y) return exmi::set_error(EXMI_ERR_INVALID, "exmi_map_resolution: null argument");
+  if (!pow2(n_old) || !pow2(n_new) || n_old < 16 || n_new < 16 || n_old > 4096 || n_new > 4096)
+    return exmi::set_error(EXMI_ERR_UNSUPPORTED, "exmi_map_resolution: sizes must be powers of two in 16..4096");
+  if (count <= 0) return EXMI_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (n_old == n_new) {
+    const hipError_t e = hipMemcpyAsync(y, x, sizeof(float) * (size_t)count * n_old, hipMemcpyDeviceToDevice, st);
+    return e == hipSuccess ? EXMI_OK : exmi::set_error(EXMI_ERR_HIP, hipGetErrorString(e));
+  }
+  const int Mo = n_old / 2, Mn = n_new / 2;
+  float2 *A = nullptr, *Bn = nullptr;
+  if (hipMallocAsync((void**)&A, sizeof(float2) * (size_t)count * (Mo + 1), st) != hipSuccess)
+    return exmi::set_error(EXMI_ERR_ALLOC, "exmi_map_resolution: workspace allocation failed");
+  if (hipMallocAsync((void**)&Bn, sizeof(float2) * (size_t)count * (Mn + 1), st) != hipSuccess) {
+    (void)hipFreeAsync(A, st);
+    return exmi::set_error(EXMI_ERR_ALLOC, "exmi_map_resolution: workspace allocation failed");
+  }
+  int rc = exmi_rfft(1, n_old, count, x, reinterpret_cast<float*>(A), stream);
+  if (rc == EXMI_OK) {
+    const long long total = (long long)count * (Mn + 1);
+    hipLaunchKernelGGL(resample_kernel, dim3(grid_for(total)), dim3(WG), 0, st, A, Mo, Bn, Mn, count,
+                       oddball_zero ? 1 : 0);
+    const hipError_t e = hipGetLastError();
+    rc = e == hipSuccess ? EXMI_OK : exmi::set_error(EXMI_ERR_HIP, hipGetErrorString(e));
+  }
+  if (rc == EXMI_OK) rc = exmi_irfft(1, n_new, count, reinterpret_cast<const float*>(Bn), y, stream);
+  (void)hipFreeAsync(A, st);
+  (void)hipFreeAsync(Bn, st);
+  return rc;
+}
+
+}  // extern "C"

@@ -139,3 +139,39 @@ def ifft(field_hat, *, num_spatial_dims=None, num_points=None):
     _lib.check(_lib.lib().exmi_irfft(1, num_points, count, z.data_ptr(), out.data_ptr(), _lib.stream_ptr()),
                "exmi_irfft")
     return restore_like(out, back)
+
+
+def get_spectrum(state, *, power: bool = True, radial_binning: str = "sum"):
+    """(Power) spectrum of a 1-D state ``(..., C, N)`` -> ``(..., C, N//2+1)``;
+    ``_spectral.py:866-1030`` (in 1-D there is no radial binning)."""
+    t, back = as_device_tensor(state)
+    if t.dim() < 2:
+        raise ValueError("expected a state with a leading channel axis")
+    if radial_binning not in ("sum", "average"):
+        raise ValueError(f"Invalid radial_binning {radial_binning!r}")
+    import torch
+
+    n = t.shape[-1]
+    x = t.contiguous().to(torch.float32)
+    count = x.numel() // n
+    out = torch.empty(x.shape[:-1] + (n // 2 + 1,), dtype=torch.float32, device=x.device)
+    _lib.check(_lib.lib().exmi_spectrum(1, n, count, x.data_ptr(), out.data_ptr(), int(power), _lib.stream_ptr()),
+               "exmi_spectrum")
+    return restore_like(out, back)
+
+
+def map_between_resolutions(state, new_num_points: int, *, oddball_zero: bool = True):
+    """Spectral up/down-sampling of a 1-D state ``(..., C, N)`` to
+    ``new_num_points``; ``_interpolation.py:195-285``."""
+    t, back = as_device_tensor(state)
+    if t.dim() < 2:
+        raise ValueError("expected a state with a leading channel axis")
+    import torch
+
+    n = t.shape[-1]
+    x = t.contiguous().to(torch.float32)
+    count = x.numel() // n
+    out = torch.empty(x.shape[:-1] + (new_num_points,), dtype=torch.float32, device=x.device)
+    _lib.check(_lib.lib().exmi_map_resolution(1, n, new_num_points, count, x.data_ptr(), out.data_ptr(),
+                                              int(oddball_zero), _lib.stream_ptr()), "exmi_map_resolution")
+    return restore_like(out, back)

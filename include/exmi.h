@@ -196,6 +196,33 @@ int exmi_rfft(int32_t num_spatial_dims, int32_t num_points, int32_t count, const
 int exmi_irfft(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x_hat, float* x,
                void* stream);
 
+/* Normalized RMSE of B state pairs: out[b] = sum_c ||pred_bc - ref_bc||_2 /
+ * ||ref_bc||_2 over the n = N^D points of each channel (the (L/N)^D factor
+ * cancels), float64 accumulation, float32 result. pred/ref (B, C, n) with
+ * batch strides in floats; out (B,) device floats.
+ * Replaces exponax.metrics.nRMSE (metrics/_spatial.py:537-586) under
+ * jax.vmap. */
+int exmi_nrmse(int32_t batch, int32_t channels, int64_t n, const float* pred, int64_t pred_bstride,
+               const float* ref, int64_t ref_bstride, float* out, void* stream);
+
+/* Spectrum of `count` 1-D signals of N points: out (count, N/2+1) float32,
+ * power = 0.5 |u_hat|^2 / (s_rec s_norm) when `power`, else |u_hat| / s_rec,
+ * with the reference's scaling arrays (reconstruction: N at k = 0, N/2 and
+ * N/2 elsewhere; norm compensation: N). 1-D only.
+ * Replaces exponax.get_spectrum (_spectral.py:866-1030) for D = 1. */
+int exmi_spectrum(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x, float* out,
+                  int32_t power, void* stream);
+
+/* Spectral up/down-sampling of `count` 1-D signals from n_old to n_new
+ * points (powers of two): modes k <= min(n_old, n_new)/2 are carried over
+ * with the norm compensation n_new/n_old; with oddball_zero the Nyquist mode
+ * of the even source (upsampling) or target (downsampling) grid is zeroed.
+ * x (count, n_old) -> y (count, n_new), contiguous. 1-D only.
+ * Replaces exponax.map_between_resolutions (_interpolation.py:195-285) for
+ * D = 1. */
+int exmi_map_resolution(int32_t num_spatial_dims, int32_t n_old, int32_t n_new, int32_t count, const float* x,
+                        float* y, int32_t oddball_zero, void* stream);
+
 /* Copy the plan's float32 operator tables to host memory:
  * (lin_batch, lin_channels, ncoef, Nr) with ncoef = exmi_plan_num_coefs().
  * Coefficient order: ETDRK0 [E]; ETDRK1 [E, c1]; ETDRK2 [E, c1, c2];
