@@ -45,7 +45,8 @@ COMMON = [
 
 def _jobs():
     jobs = [("coef", os.path.join(CSRC, "coef.hip"), []), ("abi", os.path.join(CSRC, "abi.hip"), []),
-            ("metrics", os.path.join(CSRC, "metrics.hip"), [])]
+            ("metrics", os.path.join(CSRC, "metrics.hip"), []),
+            ("corrector", os.path.join(CSRC, "corrector.hip"), [])]
     for m, pt in SIZES_1D.items():
         jobs.append((f"step1d_{m}", os.path.join(CSRC, "step1d_inst.hip"), [f"-DEXMI_M={m}", f"-DEXMI_PT={pt}"]))
     for n in SIZES_2D:

@@ -1,0 +1,228 @@
+// Learned corrector: a periodic 1-D CNN (5 Conv1d layers, kernel 5,
+// 1 -> 32 -> 32 -> 32 -> 32 -> 1 channels, GELU in tanh form after layers
+// 1-4, residual output y = x + f(x)) on N points (N in {32, 64, 128}).
+//
+// Arithmetic: weights and the activations entering each convolution are
+// bfloat16; products accumulate in float32; biases, GELU and the residual
+// are float32. The three 32 -> 32 layers (98% of the flops) run on the
+// gfx950 matrix cores (v_mfma_f32_32x32x16_bf16). The 1 -> 32 and 32 -> 1
+// layers run on the VALU.
+//
+// One wavefront owns one sample at a time. Its activations live in LDS as
+// [position][channel] rows of 40 bfloat16 (32 channels + 8 pad, 80 bytes, so
+// each 8-channel group is one aligned 16-byte read). The convolution
+// out[co][p] = sum_{ci,k} W[co][ci][k] in[ci][(p + k - 2) mod N] is a GEMM
+// with M = co, N = p and K = (k, ci) ordered k-major, so one K-step of 16
+// is 16 channels at one shift k:
+//   A (32 x 16): lane l holds W[co = l % 32][ci = 16 (s & 1) + 8 (l / 32) + i][k = s / 2], i < 8
+//   B (16 x 32): lane l holds in[ci = 16 (s & 1) + 8 (l / 32) + i][p = p0 + l % 32 + k - 2]
+//   D (32 x 32): lane l, element i is row 8 (i / 4) + 4 (l / 32) + i % 4, column l % 32
+// A and B use the same (lane, element) -> K assignment, so the product
+// does not depend on the hardware's internal K order.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace exmi {
+namespace cnn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int CH = 32;
+constexpr int KW = 5;
+constexpr int LDA = 40;  // bfloat16 per activation row in LDS
+// flat parameter vector: W1 (32,1,5) b1 W2 (32,32,5) b2 W3 b3 W4 b4 W5 (1,32,5) b5
+constexpr int OW1 = 0, OB1 = 160, OW2 = 192, OB2 = OW2 + 5120, OW3 = OB2 + 32, OB3 = OW3 + 5120;
+constexpr int OW4 = OB3 + 32, OB4 = OW4 + 5120, OW5 = OB4 + 32, OB5 = OW5 + 160;
+constexpr int NPARAM = OB5 + 1;
+static_assert(NPARAM == 15809, "parameter count");
+constexpr int KSTEPS = CH * KW / 16;  // 10 MFMA K-steps per 32 -> 32 layer
+constexpr int NFRAG = 6 * KSTEPS * 64;  // packed fragments: 3 forward + 3 transposed layers
+constexpr int WAVES = 4;                // wavefronts per workgroup
+
+constexpr float GELU_C = 0.7978845608028654f;  // sqrt(2 / pi)
+constexpr float GELU_A = 0.044715f;
+
+__device__ __forceinline__ float bf16r(float v) { return static_cast<float>(static_cast<__bf16>(v)); }
+
+// tanh(u) = 1 - 2 / (1 + e^{2u}); saturates correctly at both ends
+__device__ __forceinline__ float tanh_u(float z) {
+  const float u = GELU_C * fmaf(GELU_A * z * z, z, z);
+  return 1.f - 2.f / (1.f + __expf(2.f * u));
+}
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.f + tanh_u(z)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+  const float t = tanh_u(z);
+  return 0.5f * (1.f + t) + 0.5f * z * (1.f - t * t) * GELU_C * fmaf(3.f * GELU_A, z * z, 1.f);
+}
+
+// Edge-layer parameters and hidden biases staged in LDS (weights rounded to
+// bfloat16, kept as float).
+struct Edge {
+  float w1[CH * KW];  // [co][k]
+  float b1[CH];
+  float w5[CH * KW];  // [ci][k]
+  float b5;
+  float bh[3][CH];  // b2, b3, b4
+};
+
+__device__ inline void load_edge(Edge& e, const float* __restrict__ p) {
+  for (int i = threadIdx.x; i < CH * KW; i += blockDim.x) {
+    e.w1[i] = bf16r(p[OW1 + i]);
+    e.w5[i] = bf16r(p[OW5 + i]);
+  }
+  for (int i = threadIdx.x; i < CH; i += blockDim.x) {
+    e.b1[i] = p[OB1 + i];
+    e.bh[0][i] = p[OB2 + i];
+    e.bh[1][i] = p[OB3 + i];
+    e.bh[2][i] = p[OB4 + i];
+  }
+  if (threadIdx.x == 0) e.b5 = p[OB5];
+}
+
+// Packed fragment index of hidden layer L (0..2 = layers 2..4), K-step s,
+// lane l; T selects the transposed (input-gradient) operand.
+__host__ __device__ constexpr int frag_index(bool T, int L, int s, int l) {
+  return ((T ? 3 + L : L) * KSTEPS + s) * 64 + l;
+}
+
+// 32 x 32 output tile t (positions 32 t ..) of one hidden layer from the
+// activation rows `in` (shift +k - 2 forward, -k + 2 transposed).
+template <int N, bool TRANSPOSED>
+__device__ __forceinline__ f32x16 conv_tile(const bf16x8 (&W)[KSTEPS], const __bf16* in, int t, int lane) {
+  const int c = lane & 31, h = lane >> 5;
+  f32x16 acc = {};
+#pragma unroll
+  for (int s = 0; s < KSTEPS; ++s) {
+    const int k = s >> 1;
+    const int p = (t * 32 + c + (TRANSPOSED ? 2 - k : k - 2)) & (N - 1);
+    const bf16x8 b = *reinterpret_cast<const bf16x8*>(in + p * LDA + (s & 1) * 16 + 8 * h);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[s], b, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Training workspace of `count` samples (see exmi_cnn_workspace_bytes):
+//   Z  float [4][count][32][N]  pre-activations z1..z4
+//   A0 bf16  [count][N]         rounded input
+//   A  bf16  [4][count][32][N]  activations a1..a4
+//   D  bf16  [4][count][32][N]  deltas (gradients of z1..z4)
+struct Workspace {
+  float* Z;
+  __bf16* A0;
+  __bf16* A;
+  __bf16* D;
+};
+
+__host__ __device__ inline long long workspace_bytes(int N, long long count) {
+  return count * N * (4LL * CH * 4 + 2 + 4LL * CH * 2 + 4LL * CH * 2);
+}
+
+__host__ __device__ inline Workspace workspace_at(void* base, int N, long long count) {
+  char* b = static_cast<char*>(base);
+  Workspace w;
+  w.Z = reinterpret_cast<float*>(b);
+  b += 4LL * count * CH * N * 4;
+  w.A0 = reinterpret_cast<__bf16*>(b);
+  b += count * N * 2;
+  w.A = reinterpret_cast<__bf16*>(b);
+  b += 4LL * count * CH * N * 2;
+  w.D = reinterpret_cast<__bf16*>(b);
+  return w;
+}
+
+// Forward of one sample by one wavefront. xs: the sample's N inputs in LDS
+// (float); act0/act1: this wave's two activation buffers; the result
+// y = x + f(x) is written to y (global). With `ws` the training quantities
+// of sample `smp` are stored.
+template <int N, bool TRAIN>
+__device__ __forceinline__ void forward_sample(const Edge& e, const bf16x8 (&W)[3][KSTEPS], const float* xs,
+                                               __bf16* act0, __bf16* act1, float* __restrict__ y, Workspace ws,
+                                               long long smp, long long count, int lane) {
+  const int c = lane & 31, h = lane >> 5;
+  // layer 1 (1 -> 32) on the VALU: lane = position
+  for (int p = lane; p < N; p += 64) {
+    float xv[KW];
+#pragma unroll
+    for (int k = 0; k < KW; ++k) xv[k] = bf16r(xs[(p + k - 2) & (N - 1)]);
+    if constexpr (TRAIN) ws.A0[smp * N + p] = static_cast<__bf16>(xs[p]);
+    __bf16 outv[CH];
+#pragma unroll
+    for (int co = 0; co < CH; ++co) {
+      float z = e.b1[co];
+#pragma unroll
+      for (int k = 0; k < KW; ++k) z = fmaf(e.w1[co * KW + k], xv[k], z);
+      const __bf16 a = static_cast<__bf16>(gelu(z));
+      outv[co] = a;
+      if constexpr (TRAIN) {
+        ws.Z[(smp * CH + co) * N + p] = z;
+        ws.A[(smp * CH + co) * N + p] = a;
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < CH / 8; ++g) {
+      bf16x8 v;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = outv[8 * g + i];
+      *reinterpret_cast<bf16x8*>(act0 + p * LDA + 8 * g) = v;
+    }
+  }
+  wave_sync();
+  // layers 2-4 (32 -> 32) on the matrix cores
+  __bf16* in = act0;
+  __bf16* out = act1;
+#pragma unroll
+  for (int L = 0; L < 3; ++L) {
+#pragma unroll
+    for (int t = 0; t < N / 32; ++t) {
+      const f32x16 acc = conv_tile<N, false>(W[L], in, t, lane);
+      const int pos = t * 32 + c;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        bf16x4 v;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int co = 8 * j + 4 * h + r;
+          const float z = acc[4 * j + r] + e.bh[L][co];
+          v[r] = static_cast<__bf16>(gelu(z));
+          if constexpr (TRAIN) {
+            const long long o = (((long long)(L + 1) * count + smp) * CH + co) * N + pos;
+            ws.Z[o] = z;
+            ws.A[o] = v[r];
+          }
+        }
+        *reinterpret_cast<bf16x4*>(out + pos * LDA + 8 * j + 4 * h) = v;
+      }
+    }
+    wave_sync();
+    __bf16* tmp = in;
+    in = out;
+    out = tmp;
+  }
+  // layer 5 (32 -> 1) on the VALU, residual output
+  for (int p = lane; p < N; p += 64) {
+    float o = e.b5;
+#pragma unroll
+    for (int k = 0; k < KW; ++k) {
+      const __bf16* row = in + ((p + k - 2) & (N - 1)) * LDA;
+#pragma unroll
+      for (int g = 0; g < CH / 8; ++g) {
+        const bf16x8 v = *reinterpret_cast<const bf16x8*>(row + 8 * g);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) o = fmaf(e.w5[(8 * g + i) * KW + k], static_cast<float>(v[i]), o);
+      }
+    }
+    y[p] = xs[p] + o;
+  }
+  wave_sync();
+}
+
+}  // namespace cnn
+}  // namespace exmi

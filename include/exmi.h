@@ -223,6 +223,34 @@ int exmi_spectrum(int32_t num_spatial_dims, int32_t num_points, int32_t count, c
 int exmi_map_resolution(int32_t num_spatial_dims, int32_t n_old, int32_t n_new, int32_t count, const float* x,
                         float* y, int32_t oddball_zero, void* stream);
 
+/* ---- Learned corrector (no reference counterpart: the reference's closest
+ * analogue is the periodic Conv1d emulator of
+ * docs/examples/learning_burgers_autoregressive_neural_operator.ipynb).
+ * A periodic 1-D CNN y = x + f(x), 5 Conv1d layers of kernel 5 with widths
+ * 1 -> 32 -> 32 -> 32 -> 32 -> 1 and GELU (tanh form) after layers 1-4, on
+ * num_points in {32, 64, 128}. Weights and activations enter the
+ * convolutions as bfloat16, accumulation is float32; the 32 -> 32 layers run
+ * on the matrix cores. params: EXMI_CNN_NUM_PARAMS float32 values,
+ * W1 (32,1,5) b1 (32) W2 (32,32,5) b2 W3 b3 W4 b4 W5 (1,32,5) b5 (1), the
+ * PyTorch Conv1d layout. Samples are (count, num_points) contiguous. */
+#define EXMI_CNN_NUM_PARAMS 15809
+
+/* Bytes of the training workspace for `count` samples. */
+int64_t exmi_cnn_workspace_bytes(int32_t num_points, int32_t count);
+
+/* y = x + f(x). With a workspace (exmi_cnn_workspace_bytes) the
+ * intermediates that exmi_cnn_backward needs are stored in it. */
+int exmi_cnn_forward(const float* params, int32_t num_points, int32_t count, const float* x, float* y,
+                     void* workspace, void* stream);
+
+/* Gradients of sum(y * dy) for the samples of the last exmi_cnn_forward into
+ * `workspace`: dparams (EXMI_CNN_NUM_PARAMS, overwritten, summed over the
+ * samples) and, when dx is not NULL, dx (count, num_points). The weight
+ * gradients are reduced over per-wavefront partial sums in a fixed order
+ * (deterministic). */
+int exmi_cnn_backward(const float* params, int32_t num_points, int32_t count, const float* dy, float* dx,
+                      float* dparams, void* workspace, void* stream);
+
 /* Copy the plan's float32 operator tables to host memory:
  * (lin_batch, lin_channels, ncoef, Nr) with ncoef = exmi_plan_num_coefs().
  * Coefficient order: ETDRK0 [E]; ETDRK1 [E, c1]; ETDRK2 [E, c1, c2];

@@ -100,3 +100,14 @@ def test_no_cpu_fallback_without_device():
     s = ex.stepper.Burgers(1, 1.0, 64, 0.01)
     with pytest.raises(ex.ExmiError, match="no CPU fallback"):
         s(np.zeros((1, 64), np.float32))
+
+
+def test_package_ic_recipe_matches_oracle():
+    """The package's IC generator (host code) is the oracle's recipe."""
+    from exponax_amd import ic as pic
+    from oracle import ic as oic
+
+    for d, n, c in [(1, 256, 5), (2, 32, 6)]:
+        a = pic.truncated_fourier_series(3, d, 7.0, n, c, np.random.default_rng(3))
+        b, _ = oic.tfs_batch(3, d, 7.0, n, c, 3)
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-13)

@@ -1,0 +1,166 @@
+"""GPU parity of the corrector network kernels (exmi_cnn_forward /
+exmi_cnn_backward) against the oracle's bfloat16 model (oracle/cnn.py).
+
+That model is pinned to torch's float64 conv1d/autograd in
+tests/test_oracle_cnn.py. Tolerances are relative L2:
+- forward correction f(x): 1e-3;
+- parameter and input gradients: 5e-3.
+Both sides round the same quantities to bfloat16. What remains is the float32
+summation order, and the rare bfloat16 rounding it flips.
+
+Sample counts are odd on purpose, so the per-wavefront sample loops and the
+weight-gradient partial sums get ragged ends.
+"""
+import numpy as np
+import pytest
+
+from oracle import cnn as ocnn
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    p = ocnn.init_params(seed, scale_last=1.0)
+    return (p + 0.02 * rng.standard_normal(p.shape)).astype(np.float32)
+
+
+@pytest.mark.parametrize("N", [32, 64, 128])
+@pytest.mark.parametrize("count", [1, 37])
+def test_forward_parity(device, N, count):
+    import torch
+
+    from exponax_amd import corrector as cr
+
+    print(f"[case] cnn forward N={N} count={count}", flush=True)
+    p = _params(N + count)
+    x = np.random.default_rng(1).standard_normal((count, 1, N)).astype(np.float32)
+    pt = torch.tensor(p, device="cuda")
+    y = cr.cnn_forward(pt, torch.tensor(x, device="cuda")).cpu().numpy()
+    ref, _ = ocnn.forward(p, x, bf16_model=True)
+    ref64, _ = ocnn.forward(p, x, bf16_model=False)
+    err = _rel(y - x, ref - x)
+    err64 = _rel(y - x, ref64 - x)
+    print(f"  f(x) rel-L2 vs bf16 oracle {err:.2e}, vs float64 network {err64:.2e}", flush=True)
+    assert np.isfinite(y).all()
+    assert err <= 1e-3
+    # with a workspace the same outputs come back
+    ws = cr.workspace_for(N, count, "cuda")
+    y2 = cr.cnn_forward(pt, torch.tensor(x, device="cuda"), workspace=ws).cpu().numpy()
+    np.testing.assert_array_equal(y, y2)
+
+
+@pytest.mark.parametrize("N", [32, 64, 128])
+@pytest.mark.parametrize("count", [3, 301])
+def test_backward_parity(device, N, count):
+    import torch
+
+    from exponax_amd import corrector as cr
+
+    print(f"[case] cnn backward N={N} count={count}", flush=True)
+    p = _params(7 * N + count)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((count, 1, N)).astype(np.float32)
+    dy = rng.standard_normal((count, 1, N)).astype(np.float32)
+    pt = torch.tensor(p, device="cuda", requires_grad=True)
+    xt = torch.tensor(x, device="cuda", requires_grad=True)
+    y = cr._CnnFn.apply(xt, pt)
+    (y * torch.tensor(dy, device="cuda")).sum().backward()
+    gp, gx = pt.grad.cpu().numpy(), xt.grad.cpu().numpy()
+    rp, rx = ocnn.backward(p, x, dy, bf16_model=True)
+    ep, ex_ = _rel(gp, rp), _rel(gx - dy, rx - dy)
+    print(f"  d params rel-L2 {ep:.2e}, d x (network part) rel-L2 {ex_:.2e}", flush=True)
+    assert ep <= 5e-3 and ex_ <= 5e-3
+    # per-layer check (a layout slip in one layer cannot hide in the total)
+    o = 0
+    for ws_, bs_ in ocnn.SHAPES:
+        nw, nb = int(np.prod(ws_)), int(np.prod(bs_))
+        assert _rel(gp[o:o + nw], rp[o:o + nw]) <= 1e-2
+        assert _rel(gp[o + nw:o + nw + nb], rp[o + nw:o + nw + nb]) <= 1e-2
+        o += nw + nb
+
+
+def test_backward_deterministic(device):
+    import torch
+
+    from exponax_amd import corrector as cr
+
+    p = torch.tensor(_params(5), device="cuda", requires_grad=True)
+    x = torch.randn(513, 1, 64, device="cuda")
+    grads = []
+    for _ in range(2):
+        p.grad = None
+        (cr._CnnFn.apply(x, p) ** 2).sum().backward()
+        grads.append(p.grad.clone())
+    assert torch.equal(grads[0], grads[1])
+
+
+def _ks_pair():
+    import exponax_amd as ex
+
+    fine = ex.stepper.KuramotoSivashinskyConservative(1, 100.0, 256, 0.1)
+    coarse = ex.stepper.KuramotoSivashinskyConservative(1, 100.0, 64, 0.5)
+    return fine, coarse
+
+
+def test_training_beats_uncorrected_coarse(device):
+    """Ablation: after a short GPU training run on GPU-generated fine data,
+    the corrected coarse stepper (N = 64, dt = 0.5) is more accurate than
+    the same coarse stepper without the network, 10 coarse steps (5 time
+    units) ahead on held-out trajectories."""
+    import torch
+
+    import exponax_amd as ex
+    from exponax_amd import corrector as cr
+
+    print("[case] corrector ablation", flush=True)
+    fine, coarse = _ks_pair()
+    u0 = torch.tensor(ex.ic.truncated_fourier_series(256, 1, 100.0, 256, 5, 5), dtype=torch.float32,
+                      device="cuda")
+    data = cr.downsampled_trajectories(fine, u0, 60, 5, 64)
+    cnn = cr.PeriodicCNN(seed=0)
+    hist = cr.train(cnn, coarse, data, iters=400, batch=256, unroll=2, lr=2e-3, log_every=100)
+    print("  loss history", hist, flush=True)
+    assert hist[-1][1] < hist[0][1]
+    ut = torch.tensor(ex.ic.truncated_fourier_series(64, 1, 100.0, 256, 5, 50), dtype=torch.float32,
+                      device="cuda")
+    ref = cr.downsampled_trajectories(fine, ut, 10, 5, 64)
+    corr = cr.CorrectedStepper(coarse, cnn).rollout_batch(ref[:, 0].contiguous(), 10)
+    plain = coarse.rollout_batch(ref[:, 0].contiguous(), 10)
+    ec = ex.metrics.mean_nRMSE(corr[:, -1], ref[:, -1])
+    ep = ex.metrics.mean_nRMSE(plain[:, -1], ref[:, -1])
+    print(f"  5 time units ahead: corrected {ec:.3e}, uncorrected {ep:.3e}", flush=True)
+    assert ec < 0.5 * ep
+
+
+def test_fallback_recomputes_exactly_the_failed_trajectories(device):
+    """Corrupted coarse initial states (NaN and 1e30, forced from the host)
+    are flagged and recomputed by the fine stepper; the other trajectories
+    are the plain corrected rollout, bit for bit."""
+    import torch
+
+    import exponax_amd as ex
+    from exponax_amd import corrector as cr
+
+    fine, coarse = _ks_pair()
+    cnn = cr.PeriodicCNN(seed=1)
+    cs = cr.CorrectedStepper(coarse, cnn)
+    u0 = torch.tensor(ex.ic.truncated_fourier_series(12, 1, 100.0, 256, 5, 9), dtype=torch.float32, device="cuda")
+    u0c = ex.map_between_resolutions(u0, 64)
+    bad0 = u0c.clone()
+    bad0[3, 0, 5] = float("nan")
+    bad0[7, 0, 11] = 1e30
+    tr, mask = cr.rollout_with_fallback(cs, fine, u0, 6, 5, u0_coarse=bad0)
+    assert mask.cpu().tolist() == [i in (3, 7) for i in range(12)]
+    assert torch.isfinite(tr).all()
+    ref_fine = cr.downsampled_trajectories(fine, u0[[3, 7]].contiguous(), 6, 5, 64)[:, 1:]
+    assert torch.equal(tr[[3, 7]], ref_fine)
+    plain = cs.rollout_batch(u0c, 6)
+    keep = [i for i in range(12) if i not in (3, 7)]
+    assert torch.equal(tr[keep], plain[keep])
