@@ -23,7 +23,9 @@ N=256, one trajectory, 500 steps), kolmogorov2d (Kolmogorov vorticity,
 32 x 128^2 + 16 x 256^2 + 8 x 512^2 instances with per-instance Re, 2000
 steps in one graph), ks2d (Kuramoto-Sivashinsky 2-D 128^2, 32 trajectories,
 50 steps), ks2d_grad (the gradient of an MSE loss through those 50 steps,
-checkpoints every 10). They print the same JSON contract for their own
+checkpoints every 10), corrector (the learned corrector: GPU training data,
+GPU training, corrected coarse rollout of 8192 trajectories against the
+plain fine stepper). They print the same JSON contract for their own
 workload.
 """
 from __future__ import annotations
@@ -389,8 +391,132 @@ def wl_ks2d_grad(args, ex, _lib, torch):
     return res
 
 
+CNN_FLOP_PER_SAMPLE = 2 * (1 * 32 * 5 + 3 * 32 * 32 * 5 + 32 * 1 * 5) * 64  # 2,007,040 on 64 points
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def wl_corrector(args, ex, _lib, torch):
+    """Learned corrector: KS-conservative 1-D, L = 100. The fine stepper
+    (N = 256, dt = 0.1) generates 2048 training trajectories of 150 time
+    units, downsampled to N = 64 every 5th step. The network is trained on
+    the GPU (4-step unrolled loss through the coarse solver's adjoint). The
+    corrected coarse stepper (N = 64, dt = 0.5) then rolls 8192 held-out
+    trajectories 200 steps (100 time units), against the plain fine stepper
+    over the same time (1000 steps). value = state-steps/s of the corrected
+    rollout. Errors are measured against the float64 fine oracle at t = 5, 10
+    and 20."""
+    from exponax_amd import corrector as cr
+
+    L, NF, NC, DTF, DTC, ST = 100.0, 256, 64, 0.1, 0.5, 5
+    BT, B, K = 2048, args.batch or 8192, args.steps or 200
+    dev = torch.device("cuda:0")
+    sync = torch.cuda.synchronize
+    KS = ex.stepper.KuramotoSivashinskyConservative
+    fine, coarse = KS(1, L, NF, DTF), KS(1, L, NC, DTC)
+    t0 = time.perf_counter()
+    u0_tr = torch.tensor(ex.ic.truncated_fourier_series(BT, 1, L, NF, 5, 5), dtype=torch.float32, device=dev)
+    data = cr.downsampled_trajectories(fine, u0_tr, 300, ST, NC)
+    sync()
+    t_data = time.perf_counter() - t0
+    cnn = cr.PeriodicCNN(seed=0)
+    t0 = time.perf_counter()
+    hist = cr.train(cnn, coarse, data, iters=args.train_iters or 3000, batch=512, unroll=4, lr=1e-3,
+                    log_every=500)
+    sync()
+    t_train = time.perf_counter() - t0
+    del data
+
+    u0 = torch.tensor(ex.ic.truncated_fourier_series(B, 1, L, NF, 5, 50), dtype=torch.float32, device=dev)
+    u0c = ex.map_between_resolutions(u0, NC)
+    corrected = cr.CorrectedStepper(coarse, cnn)
+    corrected.rollout_batch(u0c, 2)
+    fine.rollout_batch(u0, 10)
+    coarse.rollout_batch(u0c, 2)
+    out = {}
+    wall_c, _ = timed_rollout(lambda: out.__setitem__("c", corrected.rollout_batch(u0c, K)), sync)
+    wall_f, _ = timed_rollout(lambda: out.__setitem__("f", fine.rollout_batch(u0, K * ST)), sync)
+    wall_p, _ = timed_rollout(lambda: out.__setitem__("p", coarse.rollout_batch(u0c, K)), sync)
+    # the network kernel alone, events on the launching (current) stream
+    x = out["c"][:, -1].contiguous()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    sync()
+    ev0.record()
+    for _ in range(reps):
+        cr.cnn_forward(cnn.params.detach(), x)
+    ev1.record()
+    sync()
+    cnn_s = ev0.elapsed_time(ev1) / 1e3 / reps
+    achieved = B * CNN_FLOP_PER_SAMPLE / cnn_s / 1e12
+    res = {
+        "metric": METRIC, "value": B * K / wall_c, "unit": "state-steps/s", "n_gpus": 1, "steps": K, "warmup": 1,
+        "ms_per_step": wall_c * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 solver, bf16 network (f32 accumulate)",
+        "data": "synthetic (seeded truncated-Fourier ICs); network trained in this run on GPU-generated data",
+        "config": {"workload": "corrector_ks_cons1d_l100_fine256_dt01_coarse64_dt05_b8192_200steps",
+                   "equation": "KuramotoSivashinskyConservative", "domain_extent": L, "fine": [NF, DTF],
+                   "coarse": [NC, DTC], "train_trajectories": BT, "train_horizon_coarse_steps": 300,
+                   "eval_trajectories": B, "eval_coarse_steps": K, "network": "5x Conv1d k5 32ch GELU(tanh), residual"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None, "kernel": "exmi cnn forward_kernel<64>",
+                     "flop_per_launch": B * CNN_FLOP_PER_SAMPLE, "avg_launch_us": cnn_s * 1e6},
+        "wall_s": {"corrected_coarse": wall_c, "plain_fine": wall_f, "plain_coarse": wall_p,
+                   "training": t_train, "training_data": t_data},
+        "learned_vs_plain_fine_speedup": wall_f / wall_c,
+        "train_loss": hist,
+    }
+    # accuracy against the float64 fine oracle on the first trajectories
+    from oracle import spectral as osp
+    from oracle import stepper as ost
+
+    nb = args.cpu_sample_batch or 16
+    horizons = {10: 5.0, 20: 10.0, 40: 20.0}
+    o64 = ost.KuramotoSivashinskyConservative(1, L, NF, DTF)
+    u = u0[:nb].cpu().numpy().astype(np.float64)
+    ref = {}
+    t0 = time.perf_counter()
+    for s_ in range(1, max(horizons) * ST + 1):
+        u = o64.step_batch(u)
+        if s_ % ST == 0 and s_ // ST in horizons:
+            ref[s_ // ST] = np.stack([osp.map_between_resolutions(v, NC) for v in u])
+    t_o = time.perf_counter() - t0
+    trc, trf, trp = out["c"][:nb].cpu().numpy(), out["f"][:nb], out["p"][:nb].cpu().numpy()
+    acc = {}
+    ladder = {"plain_n128_dt05": (128, 0.5), "plain_n128_dt025": (128, 0.25)}
+    lad_out, lad_wall = {}, {}
+    for name, (n, dt) in ladder.items():
+        stp = KS(1, L, n, dt)
+        u0n = ex.map_between_resolutions(u0, n)
+        steps = int(round(K * DTC / dt))
+        stp.rollout_batch(u0n, 2)
+        w_, _ = timed_rollout(lambda: out.__setitem__(name, stp.rollout_batch(u0n, steps)), sync)
+        lad_wall[name] = w_
+        lad_out[name] = (out[name][:nb], int(round(DTC / dt)))
+        del out[name]
+    for k, tphys in horizons.items():
+        r = rel_l2_rows
+        row = {"corrected_coarse": float(np.median(r(trc[:, k - 1], ref[k]))),
+               "plain_coarse": float(np.median(r(trp[:, k - 1], ref[k]))),
+               "plain_fine_gpu_f32": float(np.median(r(
+                   ex.map_between_resolutions(trf[:, k * ST - 1].contiguous(), NC).cpu().numpy(), ref[k])))}
+        for name, (tr_, per) in lad_out.items():
+            row[name] = float(np.median(r(ex.map_between_resolutions(tr_[:, k * per - 1].contiguous(), NC)
+                                          .cpu().numpy(), ref[k])))
+        acc[f"t={tphys:g}"] = row
+    res["median_rel_l2_vs_fp64_fine"] = acc
+    res["ladder_wall_s"] = lad_wall
+    res["nan_trajectories"] = {"corrected": int((~torch.isfinite(out["c"][:, -1]).all(-1).all(-1)).sum().item()),
+                               "plain_coarse": int((~torch.isfinite(out["p"][:, -1]).all(-1).all(-1)).sum().item())}
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = {"value": nb * max(horizons) * ST / t_o, "unit": "state-steps/s (fine, float64)",
+                               "cores": os.cpu_count() or 1, "kind": "port",
+                               "sample": f"NumPy/SciPy float64 fine stepper, {nb} trajectories x "
+                                         f"{max(horizons) * ST} steps ({t_o:.1f} s)"}
+    return res
+
+
 WORKLOADS = {"burgers1d": wl_burgers1d, "ks1d": wl_ks1d, "kolmogorov2d": wl_kolmogorov2d, "ks2d": wl_ks2d,
-             "ks2d_grad": wl_ks2d_grad}
+             "ks2d_grad": wl_ks2d_grad, "corrector": wl_corrector}
 
 
 def main():
@@ -403,6 +529,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-batch", type=int, default=0)
     ap.add_argument("--cpu-sample-steps", type=int, default=0)
+    ap.add_argument("--train-iters", type=int, default=0, help="corrector workload: training iterations")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()

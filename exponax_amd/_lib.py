@@ -35,7 +35,7 @@ EXPORTED = (
     "exmi_rollout", "exmi_rollout_prepare", "exmi_rollout_group", "exmi_repeat", "exmi_rfft", "exmi_irfft", "exmi_plan_num_coefs", "exmi_plan_get_tables",
     "exmi_plan_rollout_strategy", "exmi_plan_set_rollout_strategy", "exmi_last_error", "exmi_info",
     "exmi_step_vjp", "exmi_rollout_vjp", "exmi_nrmse", "exmi_spectrum", "exmi_map_resolution",
-    "exmi_cnn_workspace_bytes", "exmi_cnn_forward", "exmi_cnn_backward",
+    "exmi_cnn_workspace_bytes", "exmi_cnn_forward", "exmi_cnn_backward", "exmi_corrected_rollout",
 )
 EXMI_CNN_NUM_PARAMS = 15809
 
@@ -104,6 +104,7 @@ def _declare(lib):
     lib.exmi_cnn_workspace_bytes.argtypes = [i32, i32]
     lib.exmi_cnn_forward.argtypes = [vp, i32, i32, vp, vp, vp, vp]
     lib.exmi_cnn_backward.argtypes = [vp, i32, i32, vp, vp, vp, vp, vp]
+    lib.exmi_corrected_rollout.argtypes = [vp, vp, vp, i64, vp, i32, i32, vp]
     lib.exmi_last_error.restype = C.c_char_p
     lib.exmi_info.argtypes = [C.c_char_p, i32]
     for name in EXPORTED:

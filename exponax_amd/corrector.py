@@ -128,14 +128,18 @@ class CorrectedStepper:
     __call__ = step
 
     def rollout_batch(self, u0, n: int, *, include_init: bool = False):
-        """(B, 1, N) -> (B, n[+1], 1, N)."""
+        """(B, 1, N) -> (B, n[+1], 1, N) through exmi_corrected_rollout."""
         t, back = as_device_tensor(u0)
         u = t.contiguous().to(torch.float32)
-        outs = [u] if include_init else []
-        for _ in range(n):
-            u = self.step(u)
-            outs.append(u)
-        return restore_like(torch.stack(outs, dim=1), back)
+        lead, B = self.coarse._split(u, self.coarse.state_shape)
+        N = self.coarse.num_points
+        T = n + (1 if include_init else 0)
+        out = torch.empty(lead + (T, 1, N), dtype=torch.float32, device=u.device)
+        p = self.coarse.plan(B)
+        _lib.check(_lib.lib().exmi_corrected_rollout(p.handle, self.cnn.params.data_ptr(), u.data_ptr(), N,
+                                                     out.data_ptr(), n, int(include_init), _lib.stream_ptr()),
+                   "exmi_corrected_rollout")
+        return restore_like(out, back)
 
 
 # ------------------------------------------------------------------ training

@@ -994,6 +994,54 @@ int exmi_irfft(int32_t D, int32_t N, int32_t count, const float* x_hat, float* x
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "irfft launch failed: %s", hipGetErrorString(e));
 }
 
+// ------------------------------------------------------- corrected rollout
+}  // extern "C"
+namespace exmi {
+size_t cnn_frag_bytes();
+hipError_t cnn_pack(const float* params, void* frag, hipStream_t st);
+hipError_t cnn_forward_strided(const float* params, const void* frag, int N, long long count, const float* x,
+                               long long xs, float* y, long long ys, hipStream_t st);
+}  // namespace exmi
+extern "C" {
+
+int exmi_corrected_rollout(exmi_plan* p, const float* params, const float* u0, int64_t u0_bs, float* traj,
+                           int32_t n, int32_t include_init, void* stream) {
+  if (!p || !params || !u0 || !traj) return fail(EXMI_ERR_INVALID, "null argument");
+  if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
+  const int N = p->d.num_points;
+  if (p->ndim != 1 || p->d.num_channels != 1 || !(N == 32 || N == 64 || N == 128))
+    return fail(EXMI_ERR_UNSUPPORTED, "corrected rollout: 1-D single-channel plans with 32, 64 or 128 points");
+  hipStream_t st = (hipStream_t)stream;
+  const int B = p->d.batch;
+  const int off = include_init ? 1 : 0;
+  const long long ts = (long long)(n + off) * N;  // trajectory batch stride
+  if (include_init) {
+    hipError_t e = copy_batch(u0, u0_bs, traj, ts, N, B, st);
+    if (e != hipSuccess) return fail(EXMI_ERR_HIP, "copy failed: %s", hipGetErrorString(e));
+  }
+  if (n == 0) return EXMI_OK;
+  void* frag = nullptr;
+  float* tmp = nullptr;
+  if (hipMallocAsync(&frag, cnn_frag_bytes(), st) != hipSuccess)
+    return fail(EXMI_ERR_ALLOC, "corrected rollout: workspace allocation failed");
+  if (hipMallocAsync((void**)&tmp, sizeof(float) * (size_t)B * N, st) != hipSuccess) {
+    (void)hipFreeAsync(frag, st);
+    return fail(EXMI_ERR_ALLOC, "corrected rollout: workspace allocation failed");
+  }
+  int rc = EXMI_OK;
+  hipError_t e = cnn_pack(params, frag, st);
+  for (int t = 0; t < n && rc == EXMI_OK && e == hipSuccess; ++t) {
+    const float* in = t == 0 ? u0 : traj + (long long)(t - 1 + off) * N;
+    const long long ibs = t == 0 ? u0_bs : ts;
+    rc = run_step(p, in, ibs, 0, tmp, N, 0, 0, st);
+    if (rc == EXMI_OK) e = cnn_forward_strided(params, frag, N, B, tmp, N, traj + (long long)(t + off) * N, ts, st);
+  }
+  (void)hipFreeAsync(frag, st);
+  (void)hipFreeAsync(tmp, st);
+  if (rc) return rc;
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "corrected rollout launch failed: %s", hipGetErrorString(e));
+}
+
 // ------------------------------------------------------------------ adjoint
 __global__ void exmi_axpy_kernel(float* __restrict__ dst, long long ds, const float* __restrict__ src, long long ss,
                                  long long n, int B) {

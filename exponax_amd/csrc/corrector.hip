@@ -37,8 +37,9 @@ __global__ void pack_kernel(const float* __restrict__ p, bf16x8* __restrict__ fr
 struct Args {
   const float* params;
   const bf16x8* frag;  // packed fragments (pack_kernel)
-  const float* x;      // forward input (count, N)
-  float* y;            // forward output
+  const float* x;      // forward input, sample i at x + i * xs
+  float* y;            // forward output, sample i at y + i * ys
+  long long xs, ys;    // sample strides in floats (N when contiguous)
   const float* dy;     // backward: cotangent of y (count, N)
   float* dx;           // backward: cotangent of x, or null
   Workspace ws;        // training intermediates, or all null
@@ -65,17 +66,13 @@ __global__ void __launch_bounds__(64 * WAVES) forward_kernel(Args a) {
   load_frags(sh.frag, a.frag, false);
   __syncthreads();
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-  bf16x8 W[3][KSTEPS];
-#pragma unroll
-  for (int L = 0; L < 3; ++L)
-#pragma unroll
-    for (int s = 0; s < KSTEPS; ++s) W[L][s] = sh.frag[(L * KSTEPS + s) * 64 + lane];
+  const bf16x8* W = sh.frag;
   float* xs = sh.vec[w];
   // every wave runs the same bounded loop over its samples
   for (long long smp = (long long)blockIdx.x * WAVES + w; smp < a.count; smp += (long long)gridDim.x * WAVES) {
-    for (int p = lane; p < N; p += 64) xs[p] = a.x[smp * N + p];
+    for (int p = lane; p < N; p += 64) xs[p] = a.x[smp * a.xs + p];
     wave_sync();
-    forward_sample<N, TRAIN>(sh.e, W, xs, sh.act[w][0], sh.act[w][1], a.y + smp * N, a.ws, smp, a.count, lane);
+    forward_sample<N, TRAIN>(sh.e, W, xs, sh.act[w][0], sh.act[w][1], a.y + smp * a.ys, a.ws, smp, a.count, lane);
   }
 }
 
@@ -89,11 +86,7 @@ __global__ void __launch_bounds__(64 * WAVES) backward_data_kernel(Args a) {
   __syncthreads();
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int c = lane & 31, h = lane >> 5;
-  bf16x8 W[3][KSTEPS];
-#pragma unroll
-  for (int L = 0; L < 3; ++L)
-#pragma unroll
-    for (int s = 0; s < KSTEPS; ++s) W[L][s] = sh.frag[(L * KSTEPS + s) * 64 + lane];
+  const bf16x8* W = sh.frag;
   const Edge& e = sh.e;
   float* dys = sh.vec[w];
   const Workspace ws = a.ws;
@@ -129,24 +122,29 @@ __global__ void __launch_bounds__(64 * WAVES) backward_data_kernel(Args a) {
     }
     wave_sync();
     // layers 4, 3, 2 transposed on the matrix cores -> deltas of layers 3, 2, 1
+    constexpr int NTL = tiles_together<N>();
 #pragma unroll
     for (int L = 2; L >= 0; --L) {
 #pragma unroll
-      for (int t = 0; t < N / 32; ++t) {
-        const f32x16 acc = conv_tile<N, true>(W[L], in, t, lane);
-        const int pos = t * 32 + c;
+      for (int t0 = 0; t0 < N / 32; t0 += NTL) {
+        f32x16 acc[NTL];
+        conv_tiles<N, true, NTL>(acc, W + L * KSTEPS * 64, in, t0, lane);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          bf16x4 v;
+        for (int q = 0; q < NTL; ++q) {
+          const int pos = (t0 + q) * 32 + c;
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int ci = 8 * j + 4 * h + r;
-            const long long o = (((long long)L * cnt + smp) * CH + ci) * N + pos;
-            const __bf16 d = static_cast<__bf16>(acc[4 * j + r] * gelu_grad(ws.Z[o]));
-            v[r] = d;
-            ws.D[o] = d;
+          for (int j = 0; j < 4; ++j) {
+            bf16x4 v;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int ci = 8 * j + 4 * h + r;
+              const long long o = (((long long)L * cnt + smp) * CH + ci) * N + pos;
+              const __bf16 d = static_cast<__bf16>(acc[q][4 * j + r] * gelu_grad(ws.Z[o]));
+              v[r] = d;
+              ws.D[o] = d;
+            }
+            *reinterpret_cast<bf16x4*>(out + pos * LDA + 8 * j + 4 * h) = v;
           }
-          *reinterpret_cast<bf16x4*>(out + pos * LDA + 8 * j + 4 * h) = v;
         }
       }
       wave_sync();
@@ -306,6 +304,30 @@ hipError_t launch_backward(const Args& a, int wgrad_groups, hipStream_t st) {
 bool size_ok(int N) { return N == 32 || N == 64 || N == 128; }
 
 }  // namespace cnn
+
+// Internal entry points for the corrected rollout in abi.hip.
+size_t cnn_frag_bytes() { return sizeof(cnn::bf16x8) * cnn::NFRAG; }
+
+hipError_t cnn_pack(const float* params, void* frag, hipStream_t st) {
+  hipLaunchKernelGGL(cnn::pack_kernel, dim3((cnn::NFRAG + 255) / 256), dim3(256), 0, st, params,
+                     static_cast<cnn::bf16x8*>(frag));
+  return hipGetLastError();
+}
+
+hipError_t cnn_forward_strided(const float* params, const void* frag, int N, long long count, const float* x,
+                               long long xs, float* y, long long ys, hipStream_t st) {
+  cnn::Args a{};
+  a.params = params;
+  a.frag = static_cast<const cnn::bf16x8*>(frag);
+  a.x = x;
+  a.y = y;
+  a.xs = xs;
+  a.ys = ys;
+  a.count = count;
+  return N == 32 ? cnn::launch_forward<32>(a, false, st)
+                 : (N == 64 ? cnn::launch_forward<64>(a, false, st) : cnn::launch_forward<128>(a, false, st));
+}
+
 }  // namespace exmi
 
 using namespace exmi::cnn;
@@ -331,6 +353,7 @@ int exmi_cnn_forward(const float* params, int32_t N, int32_t count, const float*
   a.frag = frag;
   a.x = x;
   a.y = y;
+  a.xs = a.ys = N;
   a.count = count;
   const bool train = workspace != nullptr;
   if (train) a.ws = workspace_at(workspace, N, count);

@@ -46,15 +46,18 @@ constexpr float GELU_A = 0.044715f;
 
 __device__ __forceinline__ float bf16r(float v) { return static_cast<float>(static_cast<__bf16>(v)); }
 
-// tanh(u) = 1 - 2 / (1 + e^{2u}); saturates correctly at both ends
-__device__ __forceinline__ float tanh_u(float z) {
+// GELU, tanh form: 0.5 z (1 + tanh(u)), u = c (z + a z^3), written as
+// z * sigmoid(2u) (the same function), with the hardware reciprocal.
+// sigmoid saturates to 0 and 1 without overflow trouble: rcp(inf) = 0.
+__device__ __forceinline__ float sig2u(float z) {
   const float u = GELU_C * fmaf(GELU_A * z * z, z, z);
-  return 1.f - 2.f / (1.f + __expf(2.f * u));
+  return __builtin_amdgcn_rcpf(1.f + __expf(-2.f * u));
 }
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.f + tanh_u(z)); }
+__device__ __forceinline__ float gelu(float z) { return z * sig2u(z); }
+// d/dz z sigmoid(2u) = s + 2 z s (1 - s) c (1 + 3 a z^2)
 __device__ __forceinline__ float gelu_grad(float z) {
-  const float t = tanh_u(z);
-  return 0.5f * (1.f + t) + 0.5f * z * (1.f - t * t) * GELU_C * fmaf(3.f * GELU_A, z * z, 1.f);
+  const float s = sig2u(z);
+  return fmaf(2.f * z * s * (1.f - s), GELU_C * fmaf(3.f * GELU_A, z * z, 1.f), s);
 }
 
 // Edge-layer parameters and hidden biases staged in LDS (weights rounded to
@@ -87,20 +90,33 @@ __host__ __device__ constexpr int frag_index(bool T, int L, int s, int l) {
   return ((T ? 3 + L : L) * KSTEPS + s) * 64 + l;
 }
 
-// 32 x 32 output tile t (positions 32 t ..) of one hidden layer from the
-// activation rows `in` (shift +k - 2 forward, -k + 2 transposed).
-template <int N, bool TRANSPOSED>
-__device__ __forceinline__ f32x16 conv_tile(const bf16x8 (&W)[KSTEPS], const __bf16* in, int t, int lane) {
+// 32 x 32 output tiles t0 .. t0 + NTL - 1 (positions 32 t ..) of one hidden
+// layer from the activation rows `in` (shift +k - 2 forward, -k + 2
+// transposed). W: this layer's A fragments in LDS, [s][lane]. The tiles'
+// accumulation chains are interleaved to hide the MFMA dependency latency.
+template <int N, bool TRANSPOSED, int NTL>
+__device__ __forceinline__ void conv_tiles(f32x16 (&acc)[NTL], const bf16x8* W, const __bf16* in, int t0,
+                                           int lane) {
   const int c = lane & 31, h = lane >> 5;
-  f32x16 acc = {};
+#pragma unroll
+  for (int q = 0; q < NTL; ++q) acc[q] = f32x16{};
 #pragma unroll
   for (int s = 0; s < KSTEPS; ++s) {
     const int k = s >> 1;
-    const int p = (t * 32 + c + (TRANSPOSED ? 2 - k : k - 2)) & (N - 1);
-    const bf16x8 b = *reinterpret_cast<const bf16x8*>(in + p * LDA + (s & 1) * 16 + 8 * h);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[s], b, acc, 0, 0, 0);
+    const bf16x8 a = W[s * 64 + lane];
+#pragma unroll
+    for (int q = 0; q < NTL; ++q) {
+      const int p = ((t0 + q) * 32 + c + (TRANSPOSED ? 2 - k : k - 2)) & (N - 1);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(in + p * LDA + (s & 1) * 16 + 8 * h);
+      acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[q], 0, 0, 0);
+    }
   }
-  return acc;
+}
+
+// tiles processed together: two when the layer has an even number of tiles
+template <int N>
+constexpr int tiles_together() {
+  return (N / 32) % 2 == 0 ? 2 : 1;
 }
 
 __device__ __forceinline__ void wave_sync() {
@@ -143,7 +159,7 @@ __host__ __device__ inline Workspace workspace_at(void* base, int N, long long c
 // y = x + f(x) is written to y (global). With `ws` the training quantities
 // of sample `smp` are stored.
 template <int N, bool TRAIN>
-__device__ __forceinline__ void forward_sample(const Edge& e, const bf16x8 (&W)[3][KSTEPS], const float* xs,
+__device__ __forceinline__ void forward_sample(const Edge& e, const bf16x8* W, const float* xs,
                                                __bf16* act0, __bf16* act1, float* __restrict__ y, Workspace ws,
                                                long long smp, long long count, int lane) {
   const int c = lane & 31, h = lane >> 5;
@@ -178,27 +194,32 @@ __device__ __forceinline__ void forward_sample(const Edge& e, const bf16x8 (&W)[
   // layers 2-4 (32 -> 32) on the matrix cores
   __bf16* in = act0;
   __bf16* out = act1;
+  constexpr int NTL = tiles_together<N>();
 #pragma unroll
   for (int L = 0; L < 3; ++L) {
 #pragma unroll
-    for (int t = 0; t < N / 32; ++t) {
-      const f32x16 acc = conv_tile<N, false>(W[L], in, t, lane);
-      const int pos = t * 32 + c;
+    for (int t0 = 0; t0 < N / 32; t0 += NTL) {
+      f32x16 acc[NTL];
+      conv_tiles<N, false, NTL>(acc, W + L * KSTEPS * 64, in, t0, lane);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        bf16x4 v;
+      for (int q = 0; q < NTL; ++q) {
+        const int pos = (t0 + q) * 32 + c;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int co = 8 * j + 4 * h + r;
-          const float z = acc[4 * j + r] + e.bh[L][co];
-          v[r] = static_cast<__bf16>(gelu(z));
-          if constexpr (TRAIN) {
-            const long long o = (((long long)(L + 1) * count + smp) * CH + co) * N + pos;
-            ws.Z[o] = z;
-            ws.A[o] = v[r];
+        for (int j = 0; j < 4; ++j) {
+          bf16x4 v;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int co = 8 * j + 4 * h + r;
+            const float z = acc[q][4 * j + r] + e.bh[L][co];
+            v[r] = static_cast<__bf16>(gelu(z));
+            if constexpr (TRAIN) {
+              const long long o = (((long long)(L + 1) * count + smp) * CH + co) * N + pos;
+              ws.Z[o] = z;
+              ws.A[o] = v[r];
+            }
           }
+          *reinterpret_cast<bf16x4*>(out + pos * LDA + 8 * j + 4 * h) = v;
         }
-        *reinterpret_cast<bf16x4*>(out + pos * LDA + 8 * j + 4 * h) = v;
       }
     }
     wave_sync();

@@ -251,6 +251,16 @@ int exmi_cnn_forward(const float* params, int32_t num_points, int32_t count, con
 int exmi_cnn_backward(const float* params, int32_t num_points, int32_t count, const float* dy, float* dx,
                       float* dparams, void* workspace, void* stream);
 
+/* Corrected coarse rollout: traj[b, t] = cnn(step(traj[b, t-1])) for
+ * t = 0..n_steps-1 (+1 with include_init), starting from u0, with `plan` a
+ * 1-D single-channel plan of 32, 64 or 128 points and `params` the
+ * network's EXMI_CNN_NUM_PARAMS values. traj (B, n_steps[+1], 1, N)
+ * contiguous. Two launches per step: the plan's fused step and the network.
+ * (No reference counterpart; the rollout semantics are exponax.rollout's,
+ * _utils.py:92-186.) */
+int exmi_corrected_rollout(exmi_plan* plan, const float* params, const float* u0, int64_t u0_bstride, float* traj,
+                           int32_t n_steps, int32_t include_init, void* stream);
+
 /* Copy the plan's float32 operator tables to host memory:
  * (lin_batch, lin_channels, ncoef, Nr) with ncoef = exmi_plan_num_coefs().
  * Coefficient order: ETDRK0 [E]; ETDRK1 [E, c1]; ETDRK2 [E, c1, c2];
