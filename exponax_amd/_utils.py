@@ -116,11 +116,14 @@ def make_grid(num_spatial_dims, domain_extent, num_points, *, full=False, zero_c
     return np.stack(np.meshgrid(*([g] * num_spatial_dims), indexing="ij"))
 
 
-def rollout_group(steppers, u0s, n: int, *, include_init: bool = False, use_graph: bool = True):
-    """Roll several steppers (e.g. ensembles on different grid sizes) in one
-    captured graph whose per-stepper launch chains run as parallel branches.
-    ``u0s[i]`` is a (B_i, C, N_i^D) batch for ``steppers[i]``; returns the
-    list of (B_i, n[+1], C, N_i^D) trajectories."""
+def rollout_group(steppers, u0s, n: int, *, include_init: bool = False, use_graph=True, out=None):
+    """Roll several steppers (e.g. ensembles on different grid sizes)
+    together. ``u0s[i]`` is a (B_i, C, N_i^D) batch for ``steppers[i]``;
+    returns the list of (B_i, n[+1], C, N_i^D) trajectories (written into
+    ``out`` when given). ``use_graph``: False/0 = in order on the current
+    stream; True/1 = one captured graph with a parallel branch per stepper;
+    2 = one cached graph per stepper, launched concurrently on the steppers'
+    own streams; 3 = as 2 without graphs (include/exmi.h)."""
     import ctypes
 
     import torch
@@ -134,7 +137,14 @@ def rollout_group(steppers, u0s, n: int, *, include_init: bool = False, use_grap
         lead, B = st._split(t, st.state_shape)
         x = t.contiguous().to(torch.float32)
         T = n + (1 if include_init else 0)
-        outs.append(torch.empty(lead + (T,) + st.state_shape, dtype=torch.float32, device=x.device))
+        shape = lead + (T,) + st.state_shape
+        if out is not None:
+            o = out[len(outs)]
+            if tuple(o.shape) != shape or o.dtype != torch.float32 or not o.is_contiguous():
+                raise ValueError(f"out[{len(outs)}] must be a contiguous float32 tensor of shape {shape}")
+            outs.append(o)
+        else:
+            outs.append(torch.empty(shape, dtype=torch.float32, device=x.device))
         xs.append(x)
         plans.append(st.plan(B))
         bs.append(int(np.prod(st.state_shape)))

@@ -242,6 +242,7 @@ struct exmi_plan {
   RollFn roll[3] = {nullptr, nullptr, nullptr};  // by transforms per lane group (1, 2)
   int roll_nt = 0;                               // chosen by the autotuner (0 = not yet)
   hipStream_t capture_stream = nullptr;
+  hipStream_t launch_stream = nullptr;  // concurrent group rollouts
   GraphCache gc;
   // 2-D plans
   int ndim = 1;
@@ -444,7 +445,8 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     exmi_plan_destroy(p);
     return fail(EXMI_ERR_HIP, "coefficient formation failed: %s", hipGetErrorString(e));
   }
-  if (hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&p->launch_stream, hipStreamNonBlocking) != hipSuccess) {
     exmi_plan_destroy(p);
     return fail(EXMI_ERR_HIP, "stream creation failed");
   }
@@ -568,7 +570,8 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
     exmi_plan_destroy(p);
     return fail(EXMI_ERR_HIP, "coefficient formation failed: %s", hipGetErrorString(e));
   }
-  if (hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&p->launch_stream, hipStreamNonBlocking) != hipSuccess) {
     exmi_plan_destroy(p);
     return fail(EXMI_ERR_HIP, "stream creation failed");
   }
@@ -593,6 +596,7 @@ int exmi_plan_destroy(exmi_plan* p) {
   }
   if (p->gc.exec) hipGraphExecDestroy(p->gc.exec);
   if (p->capture_stream) hipStreamDestroy(p->capture_stream);
+  if (p->launch_stream) hipStreamDestroy(p->launch_stream);
   hipFree(p->tables);
   hipFree(p->work);
   hipFree(p->S0);
@@ -923,6 +927,38 @@ int exmi_rollout_group(exmi_plan* const* plans, int32_t np, const float* const* 
   }
   hipStream_t st = (hipStream_t)stream;
   if (!use_graph) return enqueue_group(plans, np, u0s, u0_bstrides, trajs, n, include_init, st, false);
+  if (use_graph == 2 || use_graph == 3) {
+    // every plan's rollout runs on its own stream (a hardware queue of its
+    // own), as a cached per-plan graph (2) or as direct launches (3), forked
+    // from and joined back into `stream`
+    if (use_graph == 2) {
+      for (int i = 0; i < np; ++i) {
+        int rc = run_maybe_graph(plans[i], u0s[i], trajs[i], u0_bstrides[i], n, include_init ? 1 : 0, 1, 1,
+                                 nullptr, enq_rollout_cb, false);
+        if (rc) return rc;
+      }
+    }
+    std::vector<hipEvent_t> ev(np + 1);
+    for (auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    int rc = EXMI_OK;
+    if (hipEventRecord(ev[0], st) != hipSuccess) rc = fail(EXMI_ERR_HIP, "fork record failed");
+    for (int i = 0; i < np && !rc; ++i) {
+      hipStream_t ls = plans[i]->launch_stream;
+      if (hipStreamWaitEvent(ls, ev[0], 0) != hipSuccess) rc = fail(EXMI_ERR_HIP, "fork failed");
+      if (!rc) {
+        if (use_graph == 2) {
+          if (hipGraphLaunch(plans[i]->gc.exec, ls) != hipSuccess) rc = fail(EXMI_ERR_HIP, "graph launch failed");
+        } else {
+          rc = enqueue_rollout(plans[i], u0s[i], u0_bstrides[i], trajs[i], n, include_init, ls);
+        }
+      }
+      if (!rc && hipEventRecord(ev[i + 1], ls) != hipSuccess) rc = fail(EXMI_ERR_HIP, "join record failed");
+    }
+    for (int i = 0; i < np && !rc; ++i)
+      if (hipStreamWaitEvent(st, ev[i + 1], 0) != hipSuccess) rc = fail(EXMI_ERR_HIP, "join failed");
+    for (auto& e : ev) hipEventDestroy(e);
+    return rc;
+  }
   std::vector<const void*> key;
   for (int i = 0; i < np; ++i) {
     key.push_back(plans[i]);

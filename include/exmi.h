@@ -146,9 +146,12 @@ int exmi_rollout_prepare(exmi_plan* plan, const float* u0, int64_t u0_bstride, f
 
 /* Rollouts of several plans advanced together, e.g. instances of different
  * grid sizes in one batch: plan i rolls u0s[i] (stride u0_bstrides[i]) into
- * trajs[i] exactly as exmi_rollout would. With use_graph != 0 all of it is
- * one captured hipGraph in which each plan's launches form a parallel branch
- * (cached by the first plan for the same arguments). A plan may appear once.
+ * trajs[i] exactly as exmi_rollout would. use_graph: 0 = every launch in
+ * order on `stream`; 1 = one captured hipGraph in which each plan's launches
+ * form a parallel branch (cached by the first plan for the same arguments);
+ * 2 = one cached graph per plan, the graphs launched concurrently on the
+ * plans' own streams (forked from and joined into `stream`); 3 = as 2 with
+ * direct launches instead of graphs. A plan may appear once.
  * (The reference runs each grid size separately: arrays of different shapes
  * cannot share one jax.vmap, docs/examples/performance_hints.ipynb cell 22.) */
 int exmi_rollout_group(exmi_plan* const* plans, int32_t n_plans, const float* const* u0s,
