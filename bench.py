@@ -193,6 +193,33 @@ def wl_burgers1d(args, ex, _lib, torch):
     res["rollout_strategy"] = strategy
     final = traj[:, K - 1]
     res["nan_trajectories"] = int((~torch.isfinite(final).all(dim=-1).all(dim=-1)).sum().item())
+    if not args.no_fourier_resident:
+        # the Fourier-resident rollout of the same workload (an option of the
+        # library: EXMI_FLAG_FOURIER_RESIDENT), reported beside the default
+        fr_members = [ex.stepper.Burgers(1, L, N, DT, diffusivity=float(nu)) for nu in nus]
+        for m in fr_members:
+            m.fourier_resident = True
+        ens_fr = ex.Ensemble(fr_members)
+        ens_fr.fourier_resident = True
+        plan_fr = ens_fr.plan(B)
+        traj_fr = torch.empty_like(traj)
+        if W > 0:
+            tw = torch.empty((B, W, 1, N), dtype=torch.float32, device=dev)
+            _lib.check(lib.exmi_rollout(plan_fr.handle, u0_d.data_ptr(), N, tw.data_ptr(), W, 0, 1, stream))
+            torch.cuda.synchronize()
+            del tw
+        _lib.check(lib.exmi_rollout_prepare(plan_fr.handle, u0_d.data_ptr(), N, traj_fr.data_ptr(), K, 0))
+        wall_fr, gpu_fr = timed_rollout(
+            lambda: _lib.check(lib.exmi_rollout(plan_fr.handle, u0_d.data_ptr(), N, traj_fr.data_ptr(), K, 0, 1,
+                                                stream)), torch.cuda.synchronize)
+        dev_steps = {s_: float(rel_l2_rows(traj_fr[:, s_ - 1].cpu().numpy(), traj[:, s_ - 1].cpu().numpy()).max())
+                     for s_ in (1, 10) if s_ <= K}
+        res["fourier_resident"] = {
+            "value": B * K / wall_fr, "ms_per_step": wall_fr * 1e3 / K,
+            "roofline_frac": B * K * per_state / gpu_fr / 1e9 / HBM_PEAK_GBS,
+            "strategy": plan_fr.rollout_strategy(),
+            "max_rel_l2_vs_default_path": dev_steps}
+        del traj_fr
     if not args.no_cpu_baseline:
         from oracle import stepper as ost
 
@@ -539,6 +566,8 @@ def main():
     ap.add_argument("--cpu-sample-batch", type=int, default=0)
     ap.add_argument("--cpu-sample-steps", type=int, default=0)
     ap.add_argument("--train-iters", type=int, default=0, help="corrector workload: training iterations")
+    ap.add_argument("--no-fourier-resident", action="store_true",
+                    help="burgers1d: skip the Fourier-resident rollout measured beside the default")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()

@@ -11,6 +11,14 @@ formed in float64 on the device when the first plan is built.
 Batching: ``__call__`` accepts exactly one state (C, N, ..., N) like the
 reference; ``exponax_amd.vmap(stepper)`` (or ``stepper.step`` on a tensor
 with leading batch axes) runs a whole batch in one launch.
+
+Plan options, set as attributes before the first call:
+- ``exact_form = True`` evaluates the non-conservative convection with its
+  two inverse transforms, as the reference does (EXMI_FLAG_EXACT_FORM).
+- ``fourier_resident = True`` makes 1-D ETDRK2 rollouts keep the spectral
+  state on chip between steps (EXMI_FLAG_FOURIER_RESIDENT). This is the
+  Fourier-space rollout of the reference's performance notes; it differs by
+  rounding only.
 """
 from __future__ import annotations
 
@@ -74,7 +82,8 @@ class BaseStepper:
         d.domain_extent = float(self.domain_extent)
         d.dt = float(self.dt)
         d.dealiasing_fraction = -1.0
-        d.flags = _lib.EXMI_FLAG_EXACT_FORM if getattr(self, "exact_form", False) else 0
+        d.flags = ((_lib.EXMI_FLAG_EXACT_FORM if getattr(self, "exact_form", False) else 0)
+                   | (_lib.EXMI_FLAG_FOURIER_RESIDENT if getattr(self, "fourier_resident", False) else 0))
         if self.order > 0:
             self.nonlinear_fun.fill(d)
         return d

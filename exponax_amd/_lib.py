@@ -23,6 +23,7 @@ EXMI_NL_VORTICITY = 5
 EXMI_NL_KOLMOGOROV = 6
 EXMI_MAX_POLY = 8
 EXMI_FLAG_EXACT_FORM = 1
+EXMI_FLAG_FOURIER_RESIDENT = 2
 
 EXMI_ERR_INVALID = -1
 EXMI_ERR_HIP = -2

@@ -29,7 +29,7 @@ hipError_t launch_coef(const double* L, long long total, int nr, double dt, int 
 
 #define EXMI_DECL_SIZE(M)                                                                              \
   StepFn get_step1d_##M(int kind, int order);                                                          \
-  RollFn get_roll1d_##M(int kind, int order, int nt);                                                  \
+  RollFn get_roll1d_##M(int kind, int order, int nt, int fr);                                          \
   VjpFn get_vjp1d_##M(int kind, int order);                                                            \
   hipError_t launch_fft1d_##M(bool inverse, const float* in, float* out, int count, const RealTables& rt, \
                               hipStream_t st);
@@ -108,17 +108,17 @@ static VjpFn get_vjp1d(int M, int kind, int order) {
   }
 }
 
-static RollFn get_roll1d(int M, int kind, int order, int nt) {
+static RollFn get_roll1d(int M, int kind, int order, int nt, int fr) {
   switch (M) {
-    case 8: return get_roll1d_8(kind, order, nt);
-    case 16: return get_roll1d_16(kind, order, nt);
-    case 32: return get_roll1d_32(kind, order, nt);
-    case 64: return get_roll1d_64(kind, order, nt);
-    case 128: return get_roll1d_128(kind, order, nt);
-    case 256: return get_roll1d_256(kind, order, nt);
-    case 512: return get_roll1d_512(kind, order, nt);
-    case 1024: return get_roll1d_1024(kind, order, nt);
-    case 2048: return get_roll1d_2048(kind, order, nt);
+    case 8: return get_roll1d_8(kind, order, nt, fr);
+    case 16: return get_roll1d_16(kind, order, nt, fr);
+    case 32: return get_roll1d_32(kind, order, nt, fr);
+    case 64: return get_roll1d_64(kind, order, nt, fr);
+    case 128: return get_roll1d_128(kind, order, nt, fr);
+    case 256: return get_roll1d_256(kind, order, nt, fr);
+    case 512: return get_roll1d_512(kind, order, nt, fr);
+    case 1024: return get_roll1d_1024(kind, order, nt, fr);
+    case 2048: return get_roll1d_2048(kind, order, nt, fr);
     default: return nullptr;
   }
 }
@@ -241,6 +241,7 @@ struct exmi_plan {
   StepFn nonlin = nullptr;
   RollFn roll[3] = {nullptr, nullptr, nullptr};  // by transforms per lane group (1, 2)
   int roll_nt = 0;                               // chosen by the autotuner (0 = not yet)
+  int fourier_resident = 0;                      // rollouts keep the spectral state on chip
   hipStream_t capture_stream = nullptr;
   hipStream_t launch_stream = nullptr;  // concurrent group rollouts
   GraphCache gc;
@@ -356,6 +357,8 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   if (d->num_channels != 1) return fail(EXMI_ERR_UNSUPPORTED, "2-D plans need num_channels == 1");
   if (d->batch < 1) return fail(EXMI_ERR_INVALID, "batch must be >= 1");
   if (d->order != 2) return fail(EXMI_ERR_UNSUPPORTED, "2-D plans implement ETDRK order 2 (got %d)", d->order);
+  if (d->flags & EXMI_FLAG_FOURIER_RESIDENT)
+    return fail(EXMI_ERR_UNSUPPORTED, "Fourier-resident rollouts are built for 1-D plans");
   if (d->lin_batch != 1 && d->lin_batch != d->batch)
     return fail(EXMI_ERR_INVALID, "lin_batch must be 1 or batch (%d), got %d", d->batch, d->lin_batch);
   if (d->lin_channels != 1) return fail(EXMI_ERR_INVALID, "lin_channels must be 1");
@@ -543,9 +546,16 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
     return fail(EXMI_ERR_UNSUPPORTED, "no kernel for N=%d kind=%d order=%d", N, kind, d->order);
   }
   p->vjp = d->order <= 2 ? get_vjp1d(p->M, d->order == 0 ? NL_ZERO : kind, d->order) : nullptr;
-  p->roll[1] = get_roll1d(p->M, kind, d->order, 1);
-  p->roll[2] = get_roll1d(p->M, kind, d->order, 2);
+  const int fr = (d->flags & EXMI_FLAG_FOURIER_RESIDENT) ? 1 : 0;
+  if (fr && d->order != 2) {
+    delete p;
+    return fail(EXMI_ERR_UNSUPPORTED, "Fourier-resident rollouts are built for ETDRK2");
+  }
+  p->fourier_resident = fr;
+  p->roll[1] = get_roll1d(p->M, kind, d->order, 1, fr);
+  p->roll[2] = get_roll1d(p->M, kind, d->order, 2, fr);
   if (const char* e = getenv("EXMI_ROLL_NT")) p->roll_nt = atoi(e);  // 1, 2; -1 = per-step launches
+  if (p->fourier_resident && p->roll_nt < 0) p->roll_nt = 0;
 
   int rc = get_twiddles(p->M, &p->tw);
   if (rc) {
@@ -618,6 +628,8 @@ int exmi_plan_rollout_strategy(const exmi_plan* p) { return p ? p->roll_nt : fai
 
 int exmi_plan_set_rollout_strategy(exmi_plan* p, int32_t nt) {
   if (!p) return fail(EXMI_ERR_INVALID, "null plan");
+  if (nt == -1 && p->fourier_resident)
+    return fail(EXMI_ERR_INVALID, "Fourier-resident plans roll out with the multi-step kernel only");
   if (!(nt == -1 || nt == 0 || ((nt == 1 || nt == 2) && p->roll[nt])))
     return fail(EXMI_ERR_UNSUPPORTED, "rollout strategy %d not available for this plan", nt);
   p->roll_nt = nt;
@@ -728,6 +740,7 @@ static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u
     }
     return EXMI_OK;
   }
+  if (p->fourier_resident && nt <= 0) nt = p->roll[2] ? 2 : 1;
   if (nt > 0 && p->roll[nt]) return enqueue_roll(p, nt, u0, u0_bs, traj + off * sz, tstride, sz, n, 0, st);
   for (int t = 0; t < n; ++t) {
     const float* in = (t == 0 && !include_init) ? u0 : traj + (long long)(t - 1 + off) * sz;
@@ -757,6 +770,7 @@ static int autotune_rollout(exmi_plan* p, const float* u0, long long u0_bs, floa
   const int cands[3] = {-1, 1, 2};
   for (int ci = 0; ci < 3; ++ci) {
     const int nt = cands[ci];
+    if (nt < 0 && p->fourier_resident) continue;
     if (nt > 0 && !p->roll[nt]) continue;
     float tmin = 1e30f;
     for (int rep = 0; rep < 2; ++rep) {

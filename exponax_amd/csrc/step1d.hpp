@@ -547,7 +547,30 @@ struct Roll1DArgs {
   int final_only;          // 1: write only the last state (traj_tstride unused)
 };
 
-template <int M, int PT, int KIND, int ORDER, int NT>
+// Write the NT physical states (N x, times `scale`) of a step to the
+// trajectory buffer.
+template <int M, int PT, int NT>
+__device__ __forceinline__ void store_states(const cf (*S)[PT], float scale, const Roll1DArgs& ra, const int* tb_,
+                                             const int* cb_, const bool* act, int step, int j) {
+  constexpr int T = M / PT;
+  const long long tofs = ra.final_only ? 0 : (long long)step * ra.traj_tstride;
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    if (act[n]) {
+      float2* out = reinterpret_cast<float2*>(ra.traj + (long long)tb_[n] * ra.traj_bstride + tofs +
+                                              (long long)cb_[n] * 2 * M);
+#pragma unroll
+      for (int s = 0; s < PT; ++s) out[j + s * T] = make_float2(S[n][s].re * scale, S[n][s].im * scale);
+    }
+  }
+}
+
+// FR (Fourier-resident): the spectral state stays on chip from step to step
+// (one forward transform before the loop); each step's physical state is
+// the inverse transform of a copy. It differs from the physical path only in
+// rounding (no float32 round trip of the state through physical space), and
+// it saves one forward transform per step.
+template <int M, int PT, int KIND, int ORDER, int NT, bool FR = false>
 __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(Roll1DArgs ra) {
   const Step1DArgs& a = ra.s;
   constexpr int T = M / PT;
@@ -597,37 +620,43 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
     }
   }
 
+  if constexpr (FR) rfft<M, PT, NT>(S, nyq, ln, lds, rt);
   for (int step = 0; step < ra.nsteps; ++step) {
-    rfft<M, PT, NT>(S, nyq, ln, lds, rt);
+    if constexpr (!FR) rfft<M, PT, NT>(S, nyq, ln, lds, rt);
     etdrk_step<KIND, ORDER, M, PT, NT>(S, nyq, cl, ln, lds, rt, a);
-    irfft<M, PT, NT>(S, nyq, ln, lds, rt);
-    // back to the float32 physical state, exactly as a stored-and-reloaded
-    // state of the single-step path
+    const bool emit = !ra.final_only || step == ra.nsteps - 1;
+    if constexpr (FR) {
+      if (emit) {
+        cf O[NT][PT];
+        cf on[NT];
 #pragma unroll
-    for (int n = 0; n < NT; ++n)
+        for (int n = 0; n < NT; ++n) {
+          on[n] = nyq[n];
 #pragma unroll
-      for (int s = 0; s < PT; ++s) S[n][s] = cf{S[n][s].re * r2n, S[n][s].im * r2n};
-    if (!ra.final_only || step == ra.nsteps - 1) {
-      const long long tofs = ra.final_only ? 0 : (long long)step * ra.traj_tstride;
-#pragma unroll
-      for (int n = 0; n < NT; ++n) {
-        if (act[n]) {
-          float2* out = reinterpret_cast<float2*>(ra.traj + (long long)tb_[n] * ra.traj_bstride + tofs +
-                                                  (long long)cb_[n] * 2 * M);
-#pragma unroll
-          for (int s = 0; s < PT; ++s) out[j + s * T] = make_float2(S[n][s].re, S[n][s].im);
+          for (int s = 0; s < PT; ++s) O[n][s] = S[n][s];
         }
+        irfft<M, PT, NT>(O, on, ln, lds, rt);
+        store_states<M, PT, NT>(O, r2n, ra, tb_, cb_, act, step, j);
       }
+    } else {
+      irfft<M, PT, NT>(S, nyq, ln, lds, rt);
+      // back to the float32 physical state, exactly as a stored-and-reloaded
+      // state of the single-step path
+#pragma unroll
+      for (int n = 0; n < NT; ++n)
+#pragma unroll
+        for (int s = 0; s < PT; ++s) S[n][s] = cf{S[n][s].re * r2n, S[n][s].im * r2n};
+      if (emit) store_states<M, PT, NT>(S, 1.f, ra, tb_, cb_, act, step, j);
     }
   }
 }
 
-template <int M, int PT, int KIND, int ORDER, int NT>
+template <int M, int PT, int KIND, int ORDER, int NT, bool FR = false>
 inline hipError_t launch_roll1d(const Roll1DArgs& ra, hipStream_t st) {
   constexpr int WG = Slots<M, PT, NT>::WG;
   constexpr int PB = Slots<M, PT, NT>::PER_BLOCK;
   const int grid = (ra.s.ntrans + PB - 1) / PB;
-  hipLaunchKernelGGL((roll1d_kernel<M, PT, KIND, ORDER, NT>), dim3(grid), dim3(WG), 0, st, ra);
+  hipLaunchKernelGGL((roll1d_kernel<M, PT, KIND, ORDER, NT, FR>), dim3(grid), dim3(WG), 0, st, ra);
   return hipGetLastError();
 }
 

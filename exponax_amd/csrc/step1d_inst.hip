@@ -118,9 +118,31 @@ static RollFn pick_roll(int kind, int order) {
   }
 }
 
-// Multi-step kernels carrying 1 or 2 transforms per lane group.
-RollFn EXMI_CAT(get_roll1d_, EXMI_M)(int kind, int order, int nt) {
+template <int M, int PT, int NT>
+static RollFn pick_roll_fr(int kind) {
+  switch (kind) {
+    case NL_ZERO: return &launch_roll1d<M, PT, NL_ZERO, 2, NT, true>;
+    case NL_CONV_CONS: return &launch_roll1d<M, PT, NL_CONV_CONS, 2, NT, true>;
+    case NL_CONV_NONCONS: return &launch_roll1d<M, PT, NL_CONV_NONCONS, 2, NT, true>;
+    case NL_GRADNORM: return &launch_roll1d<M, PT, NL_GRADNORM, 2, NT, true>;
+    case NL_POLY: return &launch_roll1d<M, PT, NL_POLY, 2, NT, true>;
+    case NL_GENERAL: return &launch_roll1d<M, PT, NL_GENERAL, 2, NT, true>;
+    default: return nullptr;
+  }
+}
+
+// Multi-step kernels carrying 1 or 2 transforms per lane group; `fr`
+// selects the Fourier-resident variant (ETDRK2 only).
+RollFn EXMI_CAT(get_roll1d_, EXMI_M)(int kind, int order, int nt, int fr) {
   constexpr int M = EXMI_M, PT = EXMI_PT;
+  if (fr) {
+    if (order != 2) return nullptr;
+    if (nt == 1) return pick_roll_fr<M, PT, 1>(kind);
+    if constexpr (M / PT <= 64) {
+      if (nt == 2) return pick_roll_fr<M, PT, 2>(kind);
+    }
+    return nullptr;
+  }
   if (nt == 1) return pick_roll<M, PT, 1>(kind, order);
   if constexpr (M / PT <= 64) {
     if (nt == 2) return pick_roll<M, PT, 2>(kind, order);

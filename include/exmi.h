@@ -57,6 +57,15 @@ enum exmi_nonlin_kind {
  * inverse transform per nonlinear evaluation is saved. */
 #define EXMI_FLAG_EXACT_FORM 1
 
+/* EXMI_FLAG_FOURIER_RESIDENT (1-D, ETDRK2): rollouts and repeats keep each
+ * trajectory's spectral state on chip between steps instead of re-deriving
+ * it from the float32 physical state written for that step, the Fourier-space
+ * rollout of the reference's docs/examples/performance_hints.ipynb
+ * (cells 10-16). The physical states written are the inverse transforms of
+ * the spectral states. Results differ from the default path by rounding only;
+ * one forward transform per step is saved. Single steps are unaffected. */
+#define EXMI_FLAG_FOURIER_RESIDENT 2
+
 /*
  * Stepper description: the constructor arguments of BaseStepper
  * (_base_stepper.py:27-157) plus the nonlinear function's own arguments and

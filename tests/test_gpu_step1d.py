@@ -154,3 +154,34 @@ def test_repeat_matches_rollout_last(device):
     tr = ex.rollout(ex.vmap(gpu), 7)(x)
     last = ex.repeat(ex.vmap(gpu), 7)(x)
     assert max_rel_l2(last.cpu().numpy(), tr[:, -1].cpu().numpy()) <= 1e-6
+
+
+@pytest.mark.parametrize("kind", ["ks_cons", "burgers"])
+def test_fourier_resident_rollout(device, kind):
+    """The Fourier-resident rollout (EXMI_FLAG_FOURIER_RESIDENT) stays within
+    the parity gate of the oracle at steps 1 and 10 and within float32
+    rounding of the physical rollout; both multi-step variants agree."""
+    import torch
+
+    print(f"[case] fourier-resident {kind}", flush=True)
+    N = 1024 if kind == "burgers" else 256
+    gpu, cpu, L = _pair(kind, N, 2)
+    fr, _, _ = _pair(kind, N, 2)
+    fr.fourier_resident = True
+    u0, _ = ics(6, 1, L, N, 5, seed=13)
+    x = torch.tensor(u0, device=device)
+    a = gpu.rollout_batch(x, 10).cpu().numpy()
+    outs = []
+    for nt in (1, 2):
+        fr.plan(6).set_rollout_strategy(nt)
+        outs.append(fr.rollout_batch(x, 10).cpu().numpy())
+    assert max_rel_l2(outs[1][:, -1], outs[0][:, -1]) <= 1e-6
+    b = outs[0]
+    u = u0
+    for s in range(1, 11):
+        u = cpu.step_batch(u)
+        if s in (1, 10):
+            assert max_rel_l2(b[:, s - 1], u) <= RTOL_STEP
+    assert max_rel_l2(b[:, -1], a[:, -1]) <= 1e-6
+    last = fr.repeat_batch(x, 10).cpu().numpy()
+    assert max_rel_l2(last, b[:, -1]) <= 1e-6
