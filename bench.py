@@ -262,6 +262,31 @@ def wl_ks1d(args, ex, _lib, torch):
         mk = lambda dt: ost.KuramotoSivashinskyConservative(1, L, N, DT, dtype=dt)  # noqa: E731
         res["cpu_baseline"], res["parity"] = oracle_baseline(mk, u0, checks, K, f"1 trajectory x {K} steps",
                                                               fp64_maker=mk)
+        # chaotic statistics over the second half of the rollout: the
+        # time-averaged power spectrum (GPU spectrum kernel on the GPU states,
+        # oracle spectrum on the float64 oracle's states) and the mean energy
+        from oracle import spectral as osp
+
+        h0 = K // 2
+        spec_gpu = ex.get_spectrum(tr[0, h0:]).mean(dim=0)[0].cpu().numpy().astype(np.float64)
+        o64 = ost.KuramotoSivashinskyConservative(1, L, N, DT)
+        u = u0[0].astype(np.float64)
+        spec_ref = np.zeros(N // 2 + 1)
+        e_ref = []
+        for s_ in range(1, K + 1):
+            u = o64.step_batch(u)
+            if s_ > h0:
+                spec_ref += osp.get_spectrum(u)[0]
+                e_ref.append(0.5 * np.mean(u * u))
+        spec_ref /= K - h0
+        kmax = int(2 / 3 * (N // 2)) - 1
+        e_gpu = float(0.5 * (tr[0, h0:] ** 2).mean().item())
+        res["chaotic_statistics"] = {
+            "window_steps": [h0 + 1, K],
+            "spectrum_rel_l2_modes_1_to_kmax": float(np.linalg.norm(spec_gpu[1:kmax + 1] - spec_ref[1:kmax + 1])
+                                                      / np.linalg.norm(spec_ref[1:kmax + 1])),
+            "mean_energy_gpu": e_gpu, "mean_energy_fp64_oracle": float(np.mean(e_ref)),
+        }
     return res
 
 

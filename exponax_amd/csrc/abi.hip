@@ -248,7 +248,8 @@ struct exmi_plan {
   // 2-D plans
   int ndim = 1;
   int nf = 0, kc = 0;
-  Launch2D row2d[3] = {nullptr, nullptr, nullptr}, col2d[3] = {nullptr, nullptr, nullptr};
+  Launch2D row2d[3] = {nullptr, nullptr, nullptr};
+  Launch2D col2d[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   Twiddles tw_col;
   cf *S0 = nullptr, *N0 = nullptr, *S1 = nullptr, *F = nullptr, *T2 = nullptr;
   float inj_value = 0.f;
@@ -280,6 +281,7 @@ static void fill_args2d(const exmi_plan* p, Step2DArgs* a) {
   a->F = p->F;
   a->T = p->T2;
   a->tables = p->tables;
+  a->ncoef = p->ncoef;
   a->lin_batch = p->d.lin_batch;
   a->tw_row = p->tw.tw;
   a->fw_row = p->tw.fw;
@@ -356,7 +358,8 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     return fail(EXMI_ERR_UNSUPPORTED, "2-D num_points=%d: supported sizes are powers of two 32..512", N);
   if (d->num_channels != 1) return fail(EXMI_ERR_UNSUPPORTED, "2-D plans need num_channels == 1");
   if (d->batch < 1) return fail(EXMI_ERR_INVALID, "batch must be >= 1");
-  if (d->order != 2) return fail(EXMI_ERR_UNSUPPORTED, "2-D plans implement ETDRK order 2 (got %d)", d->order);
+  if (d->order < 0 || d->order > 2)
+    return fail(EXMI_ERR_UNSUPPORTED, "2-D plans implement ETDRK orders 0-2 (got %d)", d->order);
   if (d->flags & EXMI_FLAG_FOURIER_RESIDENT)
     return fail(EXMI_ERR_UNSUPPORTED, "Fourier-resident rollouts are built for 1-D plans");
   if (d->lin_batch != 1 && d->lin_batch != d->batch)
@@ -366,8 +369,8 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   int kind = 0;
   if (d->nonlin_kind == EXMI_NL_VORTICITY || d->nonlin_kind == EXMI_NL_KOLMOGOROV) {
     kind = NL_VORT;
-  } else if (d->nonlin_kind == EXMI_NL_GRADIENT_NORM) {
-    kind = NL_GRADNORM;
+  } else if (d->nonlin_kind == EXMI_NL_GRADIENT_NORM || d->order == 0) {
+    kind = NL_GRADNORM;  // ETDRK0 evaluates no nonlinear term
   } else {
     return fail(EXMI_ERR_UNSUPPORTED, "2-D plans implement the vorticity and gradient-norm nonlinearities");
   }
@@ -378,7 +381,7 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   p->M = N / 2;
   const int NY = N / 2 + 1;
   p->nr = N * NY;
-  p->ncoef = num_coefs(2);
+  p->ncoef = num_coefs(d->order);
   p->kind = kind;
   p->kmax = dealias_kmax(N, d->dealiasing_fraction);
   p->kc = p->kmax + 1 < NY ? p->kmax + 1 : NY;
@@ -397,15 +400,15 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
       p->inj_value = (float)(-(double)kf * d->injection_scale * (double)N * sy);
     }
   }
-  for (int m = 0; m < 3; ++m) {
-    p->row2d[m] = get_2d(N, kind, 1, m);
-    p->col2d[m] = get_2d(N, kind, 0, m);
-    if (!p->row2d[m] || !p->col2d[m]) {
+  for (int m = 0; m < 3; ++m) p->row2d[m] = get_2d(N, kind, 1, m);
+  for (int m = 0; m < 5; ++m) p->col2d[m] = get_2d(N, kind, 0, m);
+  for (int m = 0; m < 5; ++m) {
+    if ((m < 3 && !p->row2d[m]) || !p->col2d[m]) {
       delete p;
       return fail(EXMI_ERR_UNSUPPORTED, "no 2-D kernel for N=%d", N);
     }
   }
-  if (kind == NL_GRADNORM) {
+  if (kind == NL_GRADNORM && d->order == 2) {
     for (int m = 0; m < 3; ++m) {
       p->rowA[m] = get_adj2d(N, 1, m);
       p->colA[m] = get_adj2d(N, 0, m);
@@ -428,7 +431,7 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
         Lt[(size_t)(b * full + (long long)ky * N + kx)] = d->host_linear_operator[b * full + (long long)kx * NY + ky];
   double* dL = nullptr;
   bool ok = hipMalloc(&dL, sizeof(double) * rows * full) == hipSuccess &&
-            hipMalloc(&p->tables, sizeof(float) * rows * full * 3) == hipSuccess &&
+            hipMalloc(&p->tables, sizeof(float) * rows * full * p->ncoef) == hipSuccess &&
             hipMalloc(&p->S0, sizeof(cf) * B * full) == hipSuccess &&
             hipMalloc(&p->S1, sizeof(cf) * B * full) == hipSuccess &&
             hipMalloc(&p->T2, sizeof(cf) * B * full) == hipSuccess &&
@@ -441,7 +444,8 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   }
   hipError_t e = hipMemcpy(dL, Lt.data(), sizeof(double) * rows * full, hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = launch_coef(dL, rows * full, (int)full, d->dt, 2, d->num_circle_points, d->circle_radius, p->tables, nullptr);
+    e = launch_coef(dL, rows * full, (int)full, d->dt, d->order, d->num_circle_points, d->circle_radius, p->tables,
+                    nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   hipFree(dL);
   if (e != hipSuccess) {
@@ -472,11 +476,19 @@ static int enqueue_step2d(exmi_plan* p, const float* u_in, long long ibs, float*
   const int B = p->d.batch;
   hipError_t e = hipSuccess;
   if (!have_T) e = p->row2d[ROW_RA](a, B, st);
-  if (e == hipSuccess) e = p->col2d[COL_CA](a, B, st);
-  if (e == hipSuccess) e = p->row2d[ROW_RB](a, B, st);
-  if (e == hipSuccess) e = p->col2d[COL_CB](a, B, st);
-  if (e == hipSuccess) e = p->row2d[ROW_RB](a, B, st);
-  if (e == hipSuccess) e = p->col2d[COL_CC](a, B, st);
+  if (p->d.order == 0) {
+    if (e == hipSuccess) e = p->col2d[COL_C0](a, B, st);
+  } else if (p->d.order == 1) {
+    if (e == hipSuccess) e = p->col2d[COL_CA](a, B, st);
+    if (e == hipSuccess) e = p->row2d[ROW_RB](a, B, st);
+    if (e == hipSuccess) e = p->col2d[COL_C1](a, B, st);
+  } else {
+    if (e == hipSuccess) e = p->col2d[COL_CA](a, B, st);
+    if (e == hipSuccess) e = p->row2d[ROW_RB](a, B, st);
+    if (e == hipSuccess) e = p->col2d[COL_CB](a, B, st);
+    if (e == hipSuccess) e = p->row2d[ROW_RB](a, B, st);
+    if (e == hipSuccess) e = p->col2d[COL_CC](a, B, st);
+  }
   if (e == hipSuccess) e = p->row2d[ROW_RC](a, B, st);
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "2-D step launch failed: %s", hipGetErrorString(e));
 }

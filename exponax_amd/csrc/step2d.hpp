@@ -47,7 +47,8 @@ struct Step2DArgs {
   cf* S1;                    // (ninst, Ny, N)   stage state
   cf* F;                     // (ninst, NF, Kc, N) nonlinear fields [ky][x]
   cf* T;                     // (ninst, Ny, N)   row spectra / products [ky][x]
-  const float* tables;       // (lin_batch, 3, Ny, N) E, c1, c2 as [ky][kx]
+  const float* tables;       // (lin_batch, ncoef, Ny, N): ETDRK0 [E], ETDRK1 [E, c1], ETDRK2 [E, c1, c2], [ky][kx]
+  int ncoef;
   int lin_batch;
   const cf* tw_row;          // row real FFT (M = N/2) tables
   const cf* fw_row;
@@ -61,7 +62,9 @@ struct Step2DArgs {
   int write_T;               // RC: also r2c the new state into T (rollout)
 };
 
-enum Col2DMode { COL_CA = 0, COL_CB = 1, COL_CC = 2 };
+// CA/CB/CC: the ETDRK2 stages; C0: ETDRK0 (S' = E S); C1: ETDRK1's update
+// (S' = E S0 + c1 N0, etdrk/_etdrk_1.py:77-82) after CA and RB.
+enum Col2DMode { COL_CA = 0, COL_CB = 1, COL_CC = 2, COL_C0 = 3, COL_C1 = 4 };
 enum Row2DMode { ROW_RA = 0, ROW_RB = 1, ROW_RC = 2 };
 
 // Nonlinear fields of one evaluation. Vorticity: (u, omega_x, v, omega_y),
@@ -84,7 +87,9 @@ struct Fields2D<NL_GRADNORM> {
 template <int N>
 struct RowCfg {
   static constexpr int M = N / 2;
-  static constexpr int PT = 8;
+  // 4 points per lane up to 128^2 (twice the wavefronts per launch and a
+  // shorter per-lane chain for these latency-bound sizes), 8 above
+  static constexpr int PT = N <= 128 ? 4 : 8;
   static constexpr int T = M / PT;
   static constexpr int R = 16;  // rows per workgroup
   static constexpr int WG = R * T;
@@ -235,7 +240,7 @@ inline hipError_t launch_row2d(const Step2DArgs& a, int ninst, hipStream_t st) {
 // owned by T = N/PT lanes; CPB columns per 256-thread workgroup.
 template <int N>
 struct ColCfg {
-  static constexpr int PT = 8;
+  static constexpr int PT = N <= 128 ? 4 : 8;
   static constexpr int T = N / PT;
   static constexpr int WG = 256;
   static constexpr int CPB = WG / T;  // columns per workgroup
@@ -340,7 +345,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     }
   };
 
-  const float* tb = a.tables + (long long)(a.lin_batch > 1 ? inst : 0) * 3 * full;
+  const float* tb = a.tables + (long long)(a.lin_batch > 1 ? inst : 0) * a.ncoef * full;
   auto coef = [&](int i, int s) { return tb[(long long)i * full + (long long)ky * N + j + s * T]; };
 
   if constexpr (MODE == COL_CA) {
@@ -372,6 +377,28 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
       }
     }
     fields_out(S);
+  } else if constexpr (MODE == COL_C0 || MODE == COL_C1) {
+    cf v[1][PT];
+    if constexpr (MODE == COL_C0) {
+      const cf* t = a.T + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) v[0][s] = t[j + s * T];
+      cfft<N, PT, false, 1>(v, ln, lds, tr);
+#pragma unroll
+      for (int s = 0; s < PT; ++s) v[0][s] = coef(0, s) * v[0][s];
+    } else {
+      cf Nh[PT];
+      nonlinear(Nh);
+      const cf* s0 = a.S0 + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) v[0][s] = coef(0, s) * s0[j + s * T] + coef(1, s) * Nh[s];
+    }
+    cfft<N, PT, true, 1>(v, ln, lds, tr);
+    if (active) {
+      cf* t = a.T + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
+    }
   } else {
     cf Nh[PT], v[1][PT];
     nonlinear(Nh);

@@ -28,6 +28,8 @@ static Launch2D pick2d(int is_row, int mode) {
       case COL_CA: return &launch_col2d<N, COL_CA, KIND>;
       case COL_CB: return &launch_col2d<N, COL_CB, KIND>;
       case COL_CC: return &launch_col2d<N, COL_CC, KIND>;
+      case COL_C0: return &launch_col2d<N, COL_C0, KIND>;
+      case COL_C1: return &launch_col2d<N, COL_C1, KIND>;
     }
   }
   return nullptr;

@@ -71,3 +71,39 @@ def test_rollout2d_equals_steps(device):
         assert max_rel_l2(tr[:, t], u.cpu().numpy()) <= 1e-6
     last = gpu.repeat_batch(x, 5).cpu().numpy()
     assert max_rel_l2(last, tr[:, -1]) <= 1e-6
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("kind", ["ks", "kolmogorov"])
+@pytest.mark.parametrize("N", [32, 128, 256])
+def test_2d_etdrk0_etdrk1_parity(device, kind, order, N):
+    """2-D ETDRK0 (E u, etdrk/_etdrk_0.py:30-34) and ETDRK1 (E u + c1 N(u),
+    etdrk/_etdrk_1.py:77-82) against the FP32 oracle, 1 and 10 steps."""
+    print(f"[case] 2-D {kind} order={order} N={N}", flush=True)
+    if kind == "ks":
+        L = 30.0
+        gpu = ex.stepper.KuramotoSivashinsky(2, L, N, 0.02, order=order)
+        cpu = ost.KuramotoSivashinsky(2, L, N, 0.02, order=order, dtype=np.float32)
+    else:
+        L = 2 * np.pi
+        gpu = ex.stepper.KolmogorovFlowVorticity(2, L, N, 0.002, diffusivity=0.01, order=order)
+        cpu = ost.KolmogorovFlowVorticity(2, L, N, 0.002, diffusivity=0.01, order=order, dtype=np.float32)
+    u0, _ = ics(2, 2, L, N, 5, seed=17)
+    _run(ex.vmap(gpu), cpu.step_batch, u0)
+
+
+def test_2d_etdrk0_rollout_exact(device):
+    """ETDRK0 on a linear problem: the 2-D rollout equals exp(L t) applied
+    to the IC in closed form (float32 rounding)."""
+    import torch
+
+    N, L, dt, n = 64, 20.0, 0.05, 20
+    gpu = ex.stepper.KuramotoSivashinsky(2, L, N, dt, order=0)
+    u0, _ = ics(2, 2, L, N, 4, seed=18)
+    tr = gpu.rollout_batch(torch.tensor(u0, device=device), n).cpu().numpy()
+    from oracle import spectral as osp
+
+    lin = gpu.linear_operator.astype(np.complex128)
+    uh = osp.fft(u0.astype(np.float64), num_spatial_dims=2)
+    ref = osp.ifft(np.exp(lin * dt * n) * uh, num_spatial_dims=2, num_points=N)
+    assert max_rel_l2(tr[:, -1], ref) <= 1e-5
