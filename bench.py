@@ -531,7 +531,7 @@ def wl_corrector(args, ex, _lib, torch):
     from oracle import stepper as ost
 
     nb = args.cpu_sample_batch or 16
-    horizons = {10: 5.0, 20: 10.0, 40: 20.0}
+    horizons = {k: k * DTC for k in (10, 20, 40) if k <= K}
     o64 = ost.KuramotoSivashinskyConservative(1, L, NF, DTF)
     u = u0[:nb].cpu().numpy().astype(np.float64)
     ref = {}

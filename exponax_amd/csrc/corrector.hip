@@ -270,11 +270,26 @@ __global__ void __launch_bounds__(64 * WAVES) wgrad_edge_kernel(Args a) {
   }
 }
 
-__global__ void reduce_kernel(const float* __restrict__ partial, long long nw, float* __restrict__ out) {
+// Two-stage fixed-order reduction of the per-wave partial sums: stage 1
+// sums chunks of RCHUNK waves (grid.y = number of chunks), stage 2 sums the
+// chunk results. Deterministic for a given wave count.
+constexpr int RCHUNK = 16;
+
+__global__ void reduce_chunks_kernel(const float* __restrict__ partial, long long nw, float* __restrict__ stage) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= NPARAM) return;
+  const long long w0 = (long long)blockIdx.y * RCHUNK;
+  const long long w1 = w0 + RCHUNK < nw ? w0 + RCHUNK : nw;
+  float s = 0.f;
+  for (long long w = w0; w < w1; ++w) s += partial[w * NPARAM + j];
+  stage[(long long)blockIdx.y * NPARAM + j] = s;
+}
+
+__global__ void reduce_kernel(const float* __restrict__ stage, long long nchunks, float* __restrict__ out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= NPARAM) return;
   float s = 0.f;
-  for (long long w = 0; w < nw; ++w) s += partial[w * NPARAM + j];
+  for (long long c = 0; c < nchunks; ++c) s += stage[c * NPARAM + j];
   out[j] = s;
 }
 
@@ -376,7 +391,8 @@ int exmi_cnn_backward(const float* params, int32_t N, int32_t count, const float
   float* partial = nullptr;
   if (hipMallocAsync((void**)&frag, sizeof(bf16x8) * NFRAG, st) != hipSuccess)
     return exmi::set_error(EXMI_ERR_ALLOC, "exmi_cnn_backward: fragment buffer allocation failed");
-  if (hipMallocAsync((void**)&partial, sizeof(float) * nw * NPARAM, st) != hipSuccess) {
+  const long long nchunks = (nw + RCHUNK - 1) / RCHUNK;
+  if (hipMallocAsync((void**)&partial, sizeof(float) * (nw + nchunks) * NPARAM, st) != hipSuccess) {
     (void)hipFreeAsync(frag, st);
     return exmi::set_error(EXMI_ERR_ALLOC, "exmi_cnn_backward: partial-sum allocation failed");
   }
@@ -392,7 +408,10 @@ int exmi_cnn_backward(const float* params, int32_t N, int32_t count, const float
   hipError_t e = N == 32 ? launch_backward<32>(a, groups, st)
                          : (N == 64 ? launch_backward<64>(a, groups, st) : launch_backward<128>(a, groups, st));
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(reduce_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, st, partial, nw, dparams);
+    float* stage = partial + nw * NPARAM;
+    hipLaunchKernelGGL(reduce_chunks_kernel, dim3((NPARAM + 255) / 256, (unsigned)nchunks), dim3(256), 0, st, partial,
+                       nw, stage);
+    hipLaunchKernelGGL(reduce_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, st, stage, nchunks, dparams);
     e = hipGetLastError();
   }
   (void)hipFreeAsync(frag, st);

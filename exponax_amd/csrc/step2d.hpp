@@ -117,6 +117,31 @@ __device__ __forceinline__ void tile_store(const cf* tl, cf* __restrict__ g, int
   }
 }
 
+// Register-staged copy of one [ky][x0, x0 + R) tile (ky < kc), for software
+// pipelining: load() issues the global reads, store() writes them to the
+// LDS tile later, so the next field's reads overlap the current transform.
+template <int N>
+struct TileRegs {
+  using C = RowCfg<N>;
+  static constexpr int CNT = (C::NY * C::R + C::WG - 1) / C::WG;
+  cf v[CNT];
+  __device__ __forceinline__ void load(const cf* __restrict__ g, int x0, int kc) {
+#pragma unroll
+    for (int q = 0; q < CNT; ++q) {
+      const int i = threadIdx.x + q * C::WG;
+      const int ky = i / C::R, r = i % C::R;
+      v[q] = (i < C::NY * C::R && ky < kc) ? g[(long long)ky * N + x0 + r] : cf{0.f, 0.f};
+    }
+  }
+  __device__ __forceinline__ void store(cf* tl) const {
+#pragma unroll
+    for (int q = 0; q < CNT; ++q) {
+      const int i = threadIdx.x + q * C::WG;
+      if (i < C::NY * C::R) tl[(i / C::R) * C::LD + i % C::R] = v[q];
+    }
+  }
+};
+
 template <int N>
 __device__ __forceinline__ void tile_to_row(const cf* tl, int r, int j, cf (*z)[RowCfg<N>::PT], cf* nyq) {
   using C = RowCfg<N>;
@@ -171,11 +196,14 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
     constexpr int NF = Fields2D<KIND>::NF;
     const cf* F = a.F + (long long)inst * NF * kc * N;
     cf A[PT], P[PT];
+    TileRegs<N> pf;
+    pf.load(F, x0, kc);
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       if (f > 0) __syncthreads();
-      tile_load<N>(lds_t, F + (long long)f * kc * N, x0, kc);
+      pf.store(lds_t);
       __syncthreads();
+      if (f + 1 < NF) pf.load(F + (long long)(f + 1) * kc * N, x0, kc);  // overlaps this field's transform
       tile_to_row<N>(lds_t, r, j, z, nyq);
       irfft<M, PT, 1>(z, nyq, ln, lds, tr);
       if constexpr (KIND == NL_VORT) {
