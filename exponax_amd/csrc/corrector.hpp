@@ -47,11 +47,15 @@ constexpr float GELU_A = 0.044715f;
 __device__ __forceinline__ float bf16r(float v) { return static_cast<float>(static_cast<__bf16>(v)); }
 
 // GELU, tanh form: 0.5 z (1 + tanh(u)), u = c (z + a z^3), written as
-// z * sigmoid(2u) (the same function), with the hardware reciprocal.
-// sigmoid saturates to 0 and 1 without overflow trouble: rcp(inf) = 0.
+// z * sigmoid(2u) (the same function): sigmoid(2u) = 1 / (1 + 2^(z (K1 + K2 z^2)))
+// with K1 = -2 c log2(e), K2 = K1 a, the hardware exp2 and reciprocal
+// (7 VALU operations). It saturates to 0 and 1 without overflow trouble:
+// rcp(inf) = 0.
+constexpr float GELU_K1 = -2.f * GELU_C * 1.4426950408889634f;
+constexpr float GELU_K2 = GELU_K1 * GELU_A;
 __device__ __forceinline__ float sig2u(float z) {
-  const float u = GELU_C * fmaf(GELU_A * z * z, z, z);
-  return __builtin_amdgcn_rcpf(1.f + __expf(-2.f * u));
+  const float w = fmaf(GELU_K2, z * z, GELU_K1);
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z * w));
 }
 __device__ __forceinline__ float gelu(float z) { return z * sig2u(z); }
 // d/dz z sigmoid(2u) = s + 2 z s (1 - s) c (1 + 3 a z^2)
