@@ -576,8 +576,42 @@ def wl_corrector(args, ex, _lib, torch):
     return res
 
 
+def wl_cnn_forward(args, ex, _lib, torch):
+    """The corrector network alone: exmi_cnn_forward on 8192 samples of 64
+    points (seeded parameters and inputs), `--steps` launches timed with
+    events on the launching stream. Its roofline is the BF16 matrix-core one."""
+    from exponax_amd import corrector as cr
+
+    B, K, W = args.batch or 8192, args.steps or 20, args.warmup
+    cnn = cr.PeriodicCNN(seed=0)
+    gen = torch.Generator(device="cpu").manual_seed(0)
+    x = torch.randn(B, 1, 64, generator=gen).to("cuda:0")
+    p = cnn.params.detach()
+    for _ in range(max(W, 1)):
+        cr.cnn_forward(p, x)
+    torch.cuda.synchronize()
+    out = {}
+
+    def run():
+        for _ in range(K):
+            out["y"] = cr.cnn_forward(p, x)
+
+    wall, gpu_s = timed_rollout(run, torch.cuda.synchronize)
+    per = gpu_s / K
+    achieved = B * CNN_FLOP_PER_SAMPLE / per / 1e12
+    return {
+        "metric": METRIC, "value": B * K / wall, "unit": "samples/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (f32 accumulate)", "data": "synthetic (seeded normal inputs, seeded parameters)",
+        "config": {"workload": "cnn_forward_b8192_n64", "network": "5x Conv1d k5 32ch GELU(tanh), residual"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None, "kernel": "exmi cnn forward_kernel<64>",
+                     "flop_per_launch": B * CNN_FLOP_PER_SAMPLE, "avg_launch_us": per * 1e6},
+    }
+
+
 WORKLOADS = {"burgers1d": wl_burgers1d, "ks1d": wl_ks1d, "kolmogorov2d": wl_kolmogorov2d, "ks2d": wl_ks2d,
-             "ks2d_grad": wl_ks2d_grad, "corrector": wl_corrector}
+             "ks2d_grad": wl_ks2d_grad, "corrector": wl_corrector, "cnn_forward": wl_cnn_forward}
 
 
 def main():
