@@ -249,9 +249,9 @@ struct exmi_plan {
   int ndim = 1;
   int nf = 0, kc = 0;
   Launch2D row2d[3] = {nullptr, nullptr, nullptr};
-  Launch2D col2d[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  Launch2D col2d[COL_NUM_MODES] = {};
   Twiddles tw_col;
-  cf *S0 = nullptr, *N0 = nullptr, *S1 = nullptr, *F = nullptr, *T2 = nullptr;
+  cf *S0 = nullptr, *N0 = nullptr, *S1 = nullptr, *F = nullptr, *T2 = nullptr, *ACC = nullptr;
   float inj_value = 0.f;
   int inj_mode = -1;
   // adjoint: 1-D kernel; 2-D modes and workspaces (allocated on first use)
@@ -280,6 +280,7 @@ static void fill_args2d(const exmi_plan* p, Step2DArgs* a) {
   a->S1 = p->S1;
   a->F = p->F;
   a->T = p->T2;
+  a->A = p->ACC;
   a->tables = p->tables;
   a->ncoef = p->ncoef;
   a->lin_batch = p->d.lin_batch;
@@ -358,8 +359,7 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     return fail(EXMI_ERR_UNSUPPORTED, "2-D num_points=%d: supported sizes are powers of two 32..512", N);
   if (d->num_channels != 1) return fail(EXMI_ERR_UNSUPPORTED, "2-D plans need num_channels == 1");
   if (d->batch < 1) return fail(EXMI_ERR_INVALID, "batch must be >= 1");
-  if (d->order < 0 || d->order > 2)
-    return fail(EXMI_ERR_UNSUPPORTED, "2-D plans implement ETDRK orders 0-2 (got %d)", d->order);
+  if (d->order < 0 || d->order > 4) return fail(EXMI_ERR_UNSUPPORTED, "Order %d not implemented.", d->order);
   if (d->flags & EXMI_FLAG_FOURIER_RESIDENT)
     return fail(EXMI_ERR_UNSUPPORTED, "Fourier-resident rollouts are built for 1-D plans");
   if (d->lin_batch != 1 && d->lin_batch != d->batch)
@@ -401,8 +401,8 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     }
   }
   for (int m = 0; m < 3; ++m) p->row2d[m] = get_2d(N, kind, 1, m);
-  for (int m = 0; m < 5; ++m) p->col2d[m] = get_2d(N, kind, 0, m);
-  for (int m = 0; m < 5; ++m) {
+  for (int m = 0; m < COL_NUM_MODES; ++m) p->col2d[m] = get_2d(N, kind, 0, m);
+  for (int m = 0; m < COL_NUM_MODES; ++m) {
     if ((m < 3 && !p->row2d[m]) || !p->col2d[m]) {
       delete p;
       return fail(EXMI_ERR_UNSUPPORTED, "no 2-D kernel for N=%d", N);
@@ -436,7 +436,8 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
             hipMalloc(&p->S1, sizeof(cf) * B * full) == hipSuccess &&
             hipMalloc(&p->T2, sizeof(cf) * B * full) == hipSuccess &&
             hipMalloc(&p->N0, sizeof(cf) * B * p->kc * N) == hipSuccess &&
-            hipMalloc(&p->F, sizeof(cf) * B * p->nf * p->kc * N) == hipSuccess;
+            hipMalloc(&p->F, sizeof(cf) * B * p->nf * p->kc * N) == hipSuccess &&
+            (d->order < 3 || hipMalloc(&p->ACC, sizeof(cf) * B * full) == hipSuccess);
   if (!ok) {
     hipFree(dL);
     exmi_plan_destroy(p);
@@ -483,11 +484,18 @@ static int enqueue_step2d(exmi_plan* p, const float* u_in, long long ibs, float*
     if (e == hipSuccess) e = p->row2d[ROW_RB](a, B, st);
     if (e == hipSuccess) e = p->col2d[COL_C1](a, B, st);
   } else {
+    // ETDRK2 CA RB CB RB CC; ETDRK3 CA RB C3A RB C3B RB C3C;
+    // ETDRK4 CA RB C4A RB C4B RB C4C RB C4D
+    static const int stages[3][4] = {{COL_CB, COL_CC, -1, -1},
+                                     {COL_C3A, COL_C3B, COL_C3C, -1},
+                                     {COL_C4A, COL_C4B, COL_C4C, COL_C4D}};
     if (e == hipSuccess) e = p->col2d[COL_CA](a, B, st);
-    if (e == hipSuccess) e = p->row2d[ROW_RB](a, B, st);
-    if (e == hipSuccess) e = p->col2d[COL_CB](a, B, st);
-    if (e == hipSuccess) e = p->row2d[ROW_RB](a, B, st);
-    if (e == hipSuccess) e = p->col2d[COL_CC](a, B, st);
+    for (int k = 0; k < 4 && e == hipSuccess; ++k) {
+      const int m = stages[p->d.order - 2][k];
+      if (m < 0) break;
+      e = p->row2d[ROW_RB](a, B, st);
+      if (e == hipSuccess) e = p->col2d[m](a, B, st);
+    }
   }
   if (e == hipSuccess) e = p->row2d[ROW_RC](a, B, st);
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "2-D step launch failed: %s", hipGetErrorString(e));
@@ -626,6 +634,7 @@ int exmi_plan_destroy(exmi_plan* p) {
   hipFree(p->N0);
   hipFree(p->F);
   hipFree(p->T2);
+  hipFree(p->ACC);
   hipFree(p->G);
   hipFree(p->LAM);
   hipFree(p->RHO);

@@ -47,6 +47,7 @@ struct Step2DArgs {
   cf* S1;                    // (ninst, Ny, N)   stage state
   cf* F;                     // (ninst, NF, Kc, N) nonlinear fields [ky][x]
   cf* T;                     // (ninst, Ny, N)   row spectra / products [ky][x]
+  cf* A;                     // (ninst, Ny, N)   ETDRK3/4 accumulator of the final combination
   const float* tables;       // (lin_batch, ncoef, Ny, N): ETDRK0 [E], ETDRK1 [E, c1], ETDRK2 [E, c1, c2], [ky][kx]
   int ncoef;
   int lin_batch;
@@ -64,7 +65,15 @@ struct Step2DArgs {
 
 // CA/CB/CC: the ETDRK2 stages; C0: ETDRK0 (S' = E S); C1: ETDRK1's update
 // (S' = E S0 + c1 N0, etdrk/_etdrk_1.py:77-82) after CA and RB.
-enum Col2DMode { COL_CA = 0, COL_CB = 1, COL_CC = 2, COL_C0 = 3, COL_C1 = 4 };
+// C3A..C3C: the three ETDRK3 stages after CA (etdrk/_etdrk_3.py:191-212),
+// C4A..C4D: the four ETDRK4 stages after CA (etdrk/_etdrk_4.py:198-224); each
+// consumes the product of the preceding RB and leaves the next stage's
+// fields (or, last, the new state's inverse column transform in T).
+enum Col2DMode {
+  COL_CA = 0, COL_CB = 1, COL_CC = 2, COL_C0 = 3, COL_C1 = 4,
+  COL_C3A = 5, COL_C3B = 6, COL_C3C = 7, COL_C4A = 8, COL_C4B = 9, COL_C4C = 10, COL_C4D = 11,
+  COL_NUM_MODES = 12
+};
 enum Row2DMode { ROW_RA = 0, ROW_RB = 1, ROW_RC = 2 };
 
 // Nonlinear fields of one evaluation. Vorticity: (u, omega_x, v, omega_y),
@@ -405,6 +414,85 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
       }
     }
     fields_out(S);
+  } else if constexpr (MODE >= COL_C3A && MODE <= COL_C4D) {
+    // column entries of a full [ky][kx] buffer, and N0 (forcing outside the
+    // kept columns)
+    auto ld = [&](const cf* buf, cf* v) {
+      const cf* p = buf + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) v[s] = p[j + s * T];
+    };
+    auto st = [&](cf* buf, const cf* v) {
+      if (!active) return;
+      cf* p = buf + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) p[j + s * T] = v[s];
+    };
+    auto ld_n0 = [&](cf* v) {
+      const cf* n0 = a.N0 + ((long long)inst * kc + ky) * N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) v[s] = kept_col ? n0[j + s * T] : forcing(s);
+    };
+    cf Nh[PT], U0[PT], Ac[PT], S[PT];
+    nonlinear(Nh);
+    ld(a.S0, U0);
+    if constexpr (MODE == COL_C3A || MODE == COL_C4A) {
+      // A = E U0 + (c3 | c4) N0; S = E_half U0 + c1 N0
+      constexpr int cA = MODE == COL_C3A ? 4 : 3;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) {
+        Ac[s] = coef(0, s) * U0[s] + coef(cA, s) * Nh[s];
+        S[s] = coef(1, s) * U0[s] + coef(2, s) * Nh[s];
+      }
+      st(a.A, Ac);
+      if (active && kept_col) {
+        cf* n0 = a.N0 + ((long long)inst * kc + ky) * N;
+#pragma unroll
+        for (int s = 0; s < PT; ++s) n0[j + s * T] = Nh[s];
+      }
+      if constexpr (MODE == COL_C4A) st(a.S1, S);
+      fields_out(S);
+    } else if constexpr (MODE == COL_C3B) {
+      // A += c4 N1; S = E U0 + c2 (2 N1 - N0)
+      cf N0v[PT];
+      ld(a.A, Ac);
+      ld_n0(N0v);
+#pragma unroll
+      for (int s = 0; s < PT; ++s) {
+        Ac[s] = Ac[s] + coef(5, s) * Nh[s];
+        S[s] = coef(0, s) * U0[s] + coef(3, s) * (2.f * Nh[s] - N0v[s]);
+      }
+      st(a.A, Ac);
+      fields_out(S);
+    } else if constexpr (MODE == COL_C4B || MODE == COL_C4C) {
+      // A += 2 c5 N_k; stage 2: S = E_half U0 + c1 N1;
+      // stage 3: S = E_half S1 + c1 (2 N2 - N0)
+      ld(a.A, Ac);
+#pragma unroll
+      for (int s = 0; s < PT; ++s) Ac[s] = Ac[s] + (coef(4, s) * 2.f) * Nh[s];
+      st(a.A, Ac);
+      if constexpr (MODE == COL_C4B) {
+#pragma unroll
+        for (int s = 0; s < PT; ++s) S[s] = coef(1, s) * U0[s] + coef(2, s) * Nh[s];
+      } else {
+        cf N0v[PT], S1v[PT];
+        ld_n0(N0v);
+        ld(a.S1, S1v);
+#pragma unroll
+        for (int s = 0; s < PT; ++s) S[s] = coef(1, s) * S1v[s] + coef(2, s) * (2.f * Nh[s] - N0v[s]);
+      }
+      fields_out(S);
+    } else {
+      // last stage: S' = A + (c5 | c6) N_last -> inverse column transform -> T
+      constexpr int cL = MODE == COL_C3C ? 6 : 5;
+      cf v[1][PT];
+      ld(a.A, Ac);
+#pragma unroll
+      for (int s = 0; s < PT; ++s) v[0][s] = Ac[s] + coef(cL, s) * Nh[s];
+      cfft<N, PT, true, 1>(v, ln, lds, tr);
+      st(a.T, v[0]);
+    }
+    (void)U0;
   } else if constexpr (MODE == COL_C0 || MODE == COL_C1) {
     cf v[1][PT];
     if constexpr (MODE == COL_C0) {

@@ -30,6 +30,13 @@ static Launch2D pick2d(int is_row, int mode) {
       case COL_CC: return &launch_col2d<N, COL_CC, KIND>;
       case COL_C0: return &launch_col2d<N, COL_C0, KIND>;
       case COL_C1: return &launch_col2d<N, COL_C1, KIND>;
+      case COL_C3A: return &launch_col2d<N, COL_C3A, KIND>;
+      case COL_C3B: return &launch_col2d<N, COL_C3B, KIND>;
+      case COL_C3C: return &launch_col2d<N, COL_C3C, KIND>;
+      case COL_C4A: return &launch_col2d<N, COL_C4A, KIND>;
+      case COL_C4B: return &launch_col2d<N, COL_C4B, KIND>;
+      case COL_C4C: return &launch_col2d<N, COL_C4C, KIND>;
+      case COL_C4D: return &launch_col2d<N, COL_C4D, KIND>;
     }
   }
   return nullptr;

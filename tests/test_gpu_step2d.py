@@ -73,12 +73,13 @@ def test_rollout2d_equals_steps(device):
     assert max_rel_l2(last, tr[:, -1]) <= 1e-6
 
 
-@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("order", [0, 1, 3, 4])
 @pytest.mark.parametrize("kind", ["ks", "kolmogorov"])
 @pytest.mark.parametrize("N", [32, 128, 256])
-def test_2d_etdrk0_etdrk1_parity(device, kind, order, N):
-    """2-D ETDRK0 (E u, etdrk/_etdrk_0.py:30-34) and ETDRK1 (E u + c1 N(u),
-    etdrk/_etdrk_1.py:77-82) against the FP32 oracle, 1 and 10 steps."""
+def test_2d_etdrk_orders_parity(device, kind, order, N):
+    """2-D ETDRK0 (E u, etdrk/_etdrk_0.py:30-34), ETDRK1 (E u + c1 N(u),
+    etdrk/_etdrk_1.py:77-82), ETDRK3 (etdrk/_etdrk_3.py:191-212) and ETDRK4
+    (etdrk/_etdrk_4.py:198-224) against the FP32 oracle, 1 and 10 steps."""
     print(f"[case] 2-D {kind} order={order} N={N}", flush=True)
     if kind == "ks":
         L = 30.0
@@ -107,3 +108,18 @@ def test_2d_etdrk0_rollout_exact(device):
     uh = osp.fft(u0.astype(np.float64), num_spatial_dims=2)
     ref = osp.ifft(np.exp(lin * dt * n) * uh, num_spatial_dims=2, num_points=N)
     assert max_rel_l2(tr[:, -1], ref) <= 1e-5
+
+
+@pytest.mark.parametrize("order", [3, 4])
+def test_2d_high_order_rollout_equals_steps(device, order):
+    import torch
+
+    N = 64
+    gpu = ex.stepper.KolmogorovFlowVorticity(2, 2 * np.pi, N, 0.002, diffusivity=0.01, order=order)
+    u0, _ = ics(2, 2, 2 * np.pi, N, 5, seed=19)
+    x = torch.tensor(u0, device=device)
+    tr = gpu.rollout_batch(x, 4).cpu().numpy()
+    u = x
+    for t in range(4):
+        u = gpu.step(u)
+        assert max_rel_l2(tr[:, t], u.cpu().numpy()) <= 1e-6
