@@ -775,8 +775,60 @@ static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u
 
 // Autotuner: time each available rollout strategy on a short probe into the
 // caller's own buffer (overwritten by the real run) and keep the fastest.
+// Autotuner cache: with EXMI_TUNE_CACHE naming a file, chosen strategies
+// persist across processes as plain-text lines "<key> <strategy>", keyed by
+// GPU architecture and the plan's shape (D, N, C, batch, kernel kind, order,
+// per-trajectory tables, Fourier-resident).
+static std::mutex g_tune_mu;
+
+static std::string tune_key(const exmi_plan* p) {
+  hipDeviceProp_t prop;
+  char arch[64] = "unknown";
+  if (hipGetDeviceProperties(&prop, p->device) == hipSuccess) {
+    snprintf(arch, sizeof arch, "%s", prop.gcnArchName);
+    for (char* c = arch; *c; ++c)
+      if (*c == ' ' || *c == ':') *c = '_';
+  }
+  char key[256];
+  snprintf(key, sizeof key, "%s|D%d|N%d|C%d|B%d|k%d|o%d|lb%d|fr%d", arch, p->ndim, p->d.num_points,
+           p->d.num_channels, p->d.batch, p->kind, p->d.order, p->d.lin_batch, p->fourier_resident);
+  return key;
+}
+
+static bool tune_lookup(exmi_plan* p) {
+  const char* path = getenv("EXMI_TUNE_CACHE");
+  if (!path || !*path) return false;
+  const std::string key = tune_key(p);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  FILE* f = fopen(path, "r");
+  if (!f) return false;
+  char k[256];
+  int v = 0;
+  bool found = false;
+  while (fscanf(f, "%255s %d", k, &v) == 2) {  // the last entry for a key wins
+    if (key == k && (v == -1 || ((v == 1 || v == 2) && p->roll[v])) && !(v == -1 && p->fourier_resident)) {
+      p->roll_nt = v;
+      found = true;
+    }
+  }
+  fclose(f);
+  return found;
+}
+
+static void tune_store(const exmi_plan* p) {
+  const char* path = getenv("EXMI_TUNE_CACHE");
+  if (!path || !*path) return;
+  const std::string key = tune_key(p);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (FILE* f = fopen(path, "a")) {
+    fprintf(f, "%s %d\n", key.c_str(), p->roll_nt);
+    fclose(f);
+  }
+}
+
 static int autotune_rollout(exmi_plan* p, const float* u0, long long u0_bs, float* traj, int n, int include_init) {
   if (p->roll_nt != 0) return EXMI_OK;
+  if (tune_lookup(p)) return EXMI_OK;
   const int probe = n < 16 ? n : 16;
   if (probe == 0) {
     p->roll_nt = p->roll[2] ? 2 : 1;
@@ -816,6 +868,7 @@ static int autotune_rollout(exmi_plan* p, const float* u0, long long u0_bs, floa
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   p->roll_nt = best_nt;
+  tune_store(p);
   return EXMI_OK;
 }
 

@@ -286,8 +286,13 @@ int exmi_plan_get_tables(const exmi_plan* plan, float* host_out, int64_t host_co
  * kernel), 1 or 2 = one multi-step launch keeping 1 or 2 transforms per lane
  * group on chip for all steps (same per-step arithmetic), 0 = not chosen
  * yet (the first exmi_rollout / exmi_rollout_prepare times the candidates on
- * probe of min(n_steps, 16) steps and keeps the fastest). Setting a strategy drops a cached
- * graph. (No reference counterpart: tuning knobs of this implementation.) */
+ * a probe of min(n_steps, 16) steps and keeps the fastest). Setting a
+ * strategy drops a cached graph. With the environment variable
+ * EXMI_TUNE_CACHE naming a file, choices persist across processes: the
+ * first rollout of a plan looks its key up (GPU architecture, D, N, C, batch,
+ * kernel kind, order, per-trajectory tables, Fourier-resident) before
+ * probing, and appends a new choice as a "<key> <strategy>" line.
+ * (No reference counterpart: tuning knobs of this implementation.) */
 int exmi_plan_rollout_strategy(const exmi_plan* plan);
 int exmi_plan_set_rollout_strategy(exmi_plan* plan, int32_t strategy);
 

@@ -185,3 +185,41 @@ def test_fourier_resident_rollout(device, kind):
     assert max_rel_l2(b[:, -1], a[:, -1]) <= 1e-6
     last = fr.repeat_batch(x, 10).cpu().numpy()
     assert max_rel_l2(last, b[:, -1]) <= 1e-6
+
+
+@pytest.mark.parametrize("kind,N", [("ks_cons", 256), ("burgers", 1024)])
+def test_rollout_resume_equals_continuous(device, kind, N):
+    """Checkpoint / resume: a rollout of 100 steps resumed from its last state
+    for 100 more equals the continuous 200-step rollout. The multi-step kernel
+    does the same arithmetic per step, so the match is bit for bit."""
+    import torch
+
+    gpu, _, L = _pair(kind, N, 2)
+    u0, _ = ics(5, 1, L, N, 5, seed=14)
+    x = torch.tensor(u0, device=device)
+    full = gpu.rollout_batch(x, 200)
+    a = gpu.rollout_batch(x, 100)
+    b = gpu.rollout_batch(a[:, -1].contiguous(), 100)
+    torch.testing.assert_close(torch.cat([a, b], dim=1), full, rtol=0, atol=0)
+
+
+def test_autotuner_cache_roundtrip(device, tmp_path, monkeypatch):
+    """EXMI_TUNE_CACHE: a tuned strategy is written once and a new plan of
+    the same shape takes it from the file without probing."""
+    import torch
+
+    path = tmp_path / "tune.txt"
+    monkeypatch.setenv("EXMI_TUNE_CACHE", str(path))
+    u0, _ = ics(3, 1, 60.0, 128, 5, seed=15)
+    x = torch.tensor(u0, device=device)
+    s1 = ex.stepper.KuramotoSivashinskyConservative(1, 60.0, 128, 0.1)
+    s1.rollout_batch(x, 20)
+    chosen = s1.plan(3).rollout_strategy()
+    lines = path.read_text().split("\n")
+    assert len([ln for ln in lines if ln.strip()]) == 1 and lines[0].endswith(f" {chosen}")
+    path.write_text(lines[0].rsplit(" ", 1)[0] + " 1\n")  # force a different, valid choice
+    s2 = ex.stepper.KuramotoSivashinskyConservative(1, 60.0, 128, 0.1)
+    out = s2.rollout_batch(x, 20)
+    assert s2.plan(3).rollout_strategy() == 1
+    assert len([ln for ln in path.read_text().split("\n") if ln.strip()]) == 1  # nothing re-tuned
+    assert max_rel_l2(out[:, -1].cpu().numpy(), s1.rollout_batch(x, 20)[:, -1].cpu().numpy()) <= 1e-5

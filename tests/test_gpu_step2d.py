@@ -123,3 +123,16 @@ def test_2d_high_order_rollout_equals_steps(device, order):
     for t in range(4):
         u = gpu.step(u)
         assert max_rel_l2(tr[:, t], u.cpu().numpy()) <= 1e-6
+
+
+def test_2d_rollout_resume_equals_continuous(device):
+    import torch
+
+    gpu = ex.stepper.KuramotoSivashinsky(2, 30.0, 64, 0.05)
+    u0, _ = ics(2, 2, 30.0, 64, 5, seed=20)
+    x = torch.tensor(u0, device=device)
+    full = gpu.rollout_batch(x, 40).cpu().numpy()
+    a = gpu.rollout_batch(x, 20)
+    b = gpu.rollout_batch(a[:, -1].contiguous(), 20)
+    got = torch.cat([a, b], dim=1).cpu().numpy()
+    assert max_rel_l2(got[:, -1], full[:, -1]) <= 1e-6

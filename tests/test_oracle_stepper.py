@@ -159,3 +159,15 @@ def test_ic_recipe():
     uh = sp.fft(u2, num_spatial_dims=2)
     wn = sp.build_wavenumbers(2, 32)
     assert np.abs(uh[0][np.max(np.abs(wn), axis=0) > 3]).max() < 1e-9
+
+
+def test_rollout_resume_bit_exact():
+    """Checkpoint / resume: 100 steps, then 100 more from the (step, state)
+    snapshot, equal a continuous 200-step rollout bit for bit."""
+    s = st.KuramotoSivashinskyConservative(1, 100.0, 64, 0.1, dtype=np.float32)
+    u0 = ic.tfs(1, 100.0, 64, 5, np.random.default_rng(12)).astype(np.float32)
+    full = st.rollout(s, 200)(u0)
+    first = st.rollout(s, 100)(u0)
+    snap = (100, first[-1].copy())
+    second = st.rollout(s, 100)(snap[1])
+    np.testing.assert_array_equal(np.concatenate([first, second]), full)
