@@ -253,7 +253,7 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) vjp1d_kernel(Vj
     const float* tb = a.tables + ((long long)(a.lin_batch > 1 ? b : 0) * a.lin_channels +
                                   (a.lin_channels > 1 ? c : 0)) * (long long)NCO * NR;
     CoefRegs<NCO, PT, 1> cr;
-    cr.load(0, tb, j, T, M);
+    cr.template load<M, NATURAL>(0, tb, j);
     rfft<M, PT, 1>(U, un, ln, lds, rt);
 #pragma unroll
     for (int i = 0; i < NCO; ++i)

@@ -389,4 +389,132 @@ __device__ __forceinline__ void irfft(cf (*z)[PT], const cf* nyq, const Lane<M, 
   cfft<M, PT, true, NT>(z, ln, lds, tr);
 }
 
+// ------------------------------------------------------ spectral layouts
+// Where the M+1 modes of a real transform's spectrum live (T lanes x PT
+// slots + one extra register per lane, meaningful in lane 0):
+//   NATURAL: slot s of lane j holds mode j + s*T; the extra holds X[M].
+//   PAIRED:  slots s < H = PT/2 hold k = j + s*T (k < M/2) and slot H + h
+//            holds the partner M - (j + h*T); the extra holds X[M/2]. Lane 0
+//            slot H is the Nyquist mode M.
+// The paired layout keeps every mode next to its partner M - k, which the
+// real pre/post-processing combines: one formula then yields both modes of a
+// pair, and the exchange with the partner lane moves half the data.
+enum SpecLayout { NATURAL = 0, PAIRED = 1 };
+
+template <int M, int PT, int LAY>
+struct Modes {
+  static constexpr int T = M / PT;
+  static constexpr int H = PT / 2;
+  static constexpr bool lower(int s) { return LAY == NATURAL || s < H; }
+  // mode of slot s in lane j
+  __device__ __forceinline__ static int k(int j, int s) { return lower(s) ? j + s * T : M - j - (s - H) * T; }
+  // smallest / largest mode of slot s over the lanes (uniform across the wave)
+  __host__ __device__ static constexpr int klo(int s) { return lower(s) ? s * T : M - (s - H) * T - (T - 1); }
+  __host__ __device__ static constexpr int khi(int s) { return lower(s) ? s * T + T - 1 : M - (s - H) * T; }
+  static constexpr int kextra = LAY == NATURAL ? M : M / 2;
+};
+
+// r2c into the paired layout: z (packed samples, lane-strided) -> 2 X, and
+// xm = 2 X[M/2] (lane 0; zero elsewhere). Modes above `kcut` are not formed
+// where a whole slot lies above it.
+//   2 X[k]   = s + P,   2 X[M-k] = conj(s - P),
+//   s = Z[k] + Z*[M-k], P = fw[k] (Z[k] - Z*[M-k]),
+// the second from fw[M-k] = conj(fw[k]); at k = 0 the pair is (X[0], X[M])
+// with Z[0] its own partner, and 2 X[M/2] = 2 conj(Z[M/2]) since
+// fw[M/2] = -1.
+template <int M, int PT, int NT>
+__device__ __forceinline__ void rfft_paired(cf (*z)[PT], cf* xm, const Lane<M, PT>& ln, cf* lds,
+                                            const TwRegs<M, PT>& tr, int kcut = M) {
+  constexpr int T = M / PT, H = PT / 2;
+  using MD = Modes<M, PT, PAIRED>;
+  cfft<M, PT, false, NT>(z, ln, lds, tr);
+  cf zp[NT][H];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+#pragma unroll
+    for (int s = H; s < PT; ++s) lds[ln.at(s)] = z[n][s];
+    xsync<T>();
+#pragma unroll
+    for (int h = 0; h < H; ++h) zp[n][h] = lds[ln.mirror(h)];
+    xsync<T>();
+  }
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    const bool l0 = ln.j == 0;
+    if (l0) zp[n][0] = z[n][0];  // element 0 is not exchanged
+    xm[n] = (l0 && M / 2 <= kcut) ? cf{2.f * z[n][H].re, -2.f * z[n][H].im} : cf{0.f, 0.f};
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const bool lo_drop = MD::klo(h) > kcut, hi_drop = MD::klo(H + h) > kcut;
+      if (lo_drop && hi_drop) {
+        z[n][h] = z[n][H + h] = cf{0.f, 0.f};
+        continue;
+      }
+      const cf a = z[n][h], b = conj(zp[n][h]);
+      const cf sm = a + b, p = cmul(tr.fw[h], a - b);
+      z[n][h] = lo_drop ? cf{0.f, 0.f} : sm + p;
+      z[n][H + h] = hi_drop ? cf{0.f, 0.f} : conj(sm - p);
+    }
+  }
+}
+
+// c2r from the paired layout: X (+ xm = X[M/2] in lane 0) -> packed samples
+// times N. Imaginary parts of X[0] and X[M] are ignored. `kin`: modes above
+// it are zero on entry (a dealiased spectrum); pairs whose partner slot lies
+// wholly above it use the shorter formula.
+//   Z'[k] = s + P,  Z'[M-k] = conj(s - P),
+//   s = X[k] + X*[M-k], P = conj(fw[k]) (X[k] - X*[M-k]).
+template <int M, int PT, int NT>
+__device__ __forceinline__ void irfft_paired(cf (*z)[PT], const cf* xm, const Lane<M, PT>& ln, cf* lds,
+                                             const TwRegs<M, PT>& tr, int kin = M) {
+  constexpr int T = M / PT, H = PT / 2;
+  using MD = Modes<M, PT, PAIRED>;
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    if (ln.j == 0) {
+      z[n][0].im = 0.f;
+      z[n][H].im = 0.f;
+    }
+    cf send[H];
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const cf a = z[n][h];
+      if (MD::klo(H + h) > kin) {  // X[M-k] = 0 for every lane
+        const cf p = cmulc(a, tr.fw[h]);
+        z[n][h] = a + p;
+        send[h] = conj(a - p);
+      } else {
+        const cf b = conj(z[n][H + h]);
+        const cf sm = a + b, p = cmulc(a - b, tr.fw[h]);
+        z[n][h] = sm + p;
+        send[h] = conj(sm - p);
+      }
+    }
+    // Z'[M-k] to the natural slot of element M-k (lane 0, h = 0 writes the
+    // meaningless Z'[M] to element 0's slot, which is not read back)
+#pragma unroll
+    for (int h = 0; h < H; ++h) lds[ln.mirror(h)] = send[h];
+    xsync<T>();
+#pragma unroll
+    for (int s = H; s < PT; ++s) z[n][s] = lds[ln.at(s)];
+    xsync<T>();
+    if (ln.j == 0) z[n][H] = cf{2.f * xm[n].re, -2.f * xm[n].im};  // Z'[M/2], not exchanged
+  }
+  cfft<M, PT, true, NT>(z, ln, lds, tr);
+}
+
+// Layout dispatch.
+template <int LAY, int M, int PT, int NT>
+__device__ __forceinline__ void rfft_l(cf (*z)[PT], cf* x, const Lane<M, PT>& ln, cf* lds, const TwRegs<M, PT>& tr,
+                                       int kcut = M) {
+  if constexpr (LAY == PAIRED) rfft_paired<M, PT, NT>(z, x, ln, lds, tr, kcut);
+  else rfft<M, PT, NT>(z, x, ln, lds, tr, kcut);
+}
+template <int LAY, int M, int PT, int NT>
+__device__ __forceinline__ void irfft_l(cf (*z)[PT], const cf* x, const Lane<M, PT>& ln, cf* lds,
+                                        const TwRegs<M, PT>& tr, int kin = M) {
+  if constexpr (LAY == PAIRED) irfft_paired<M, PT, NT>(z, x, ln, lds, tr, kin);
+  else irfft<M, PT, NT>(z, x, ln, lds, tr);
+}
+
 }  // namespace exmi

@@ -51,75 +51,82 @@ constexpr int ncoef_of(int order) {
 }
 
 // ----------------------------------------------------------------- helpers
-// Dealias mask classes of slot s (modes j + s*T, j < T) for a uniform kmax:
-// every mode kept, every mode dropped, or a per-lane boundary. The branches
-// are uniform across the wave, so fully kept or dropped slots cost no VALU
-// selects.
-template <int T>
-__device__ __forceinline__ bool slot_kept(int s, int kmax) { return (s + 1) * T - 1 <= kmax; }
-template <int T>
-__device__ __forceinline__ bool slot_dropped(int s, int kmax) { return s * T > kmax; }
+// Dealias mask classes of slot s for a uniform kmax: every mode kept, every
+// mode dropped, or a per-lane boundary. The branches are uniform across the
+// wave, so fully kept or dropped slots cost no VALU selects. Slot s of lane j
+// holds mode Modes<M, PT, LAY>::k(j, s) (fft1d.hpp); the extra register holds
+// mode Modes::kextra.
+template <int M, int PT, int LAY>
+__device__ __forceinline__ bool slot_kept(int s, int kmax) { return Modes<M, PT, LAY>::khi(s) <= kmax; }
+template <int M, int PT, int LAY>
+__device__ __forceinline__ bool slot_dropped(int s, int kmax) { return Modes<M, PT, LAY>::klo(s) > kmax; }
+
+// c * k for slot s of lane j: cj = c * j, cjm = c * (M - j) (exact integer
+// mode numbers; one fused multiply-add per slot)
+template <int M, int PT, int LAY>
+__device__ __forceinline__ float scaled_mode(float c, float cj, float cjm, int s) {
+  constexpr int T = M / PT, H = PT / 2;
+  if constexpr (LAY == NATURAL) return fmaf(c, (float)(s * T), cj);
+  else return s < H ? fmaf(c, (float)(s * T), cj) : fmaf(-c, (float)((s - H) * T), cjm);
+}
 
 // Z = mask * S (dealias before an inverse transform)
-template <int M, int PT>
+template <int M, int PT, int LAY = NATURAL>
 __device__ __forceinline__ void masked_copy(const cf* S, cf nyq, cf* Z, cf& zn, int j, int kmax) {
-  constexpr int T = M / PT;
+  using MD = Modes<M, PT, LAY>;
 #pragma unroll
   for (int s = 0; s < PT; ++s) {
-    if (slot_kept<T>(s, kmax)) {
+    if (slot_kept<M, PT, LAY>(s, kmax)) {
       Z[s] = S[s];
-    } else if (slot_dropped<T>(s, kmax)) {
+    } else if (slot_dropped<M, PT, LAY>(s, kmax)) {
       Z[s] = cf{0.f, 0.f};
     } else {
-      const int k = j + s * T;
-      Z[s] = (k <= kmax) ? S[s] : cf{0.f, 0.f};
+      Z[s] = (MD::k(j, s) <= kmax) ? S[s] : cf{0.f, 0.f};
     }
   }
-  zn = (M <= kmax) ? nyq : cf{0.f, 0.f};
+  zn = (MD::kextra <= kmax) ? nyq : cf{0.f, 0.f};
 }
 
 // Z = mask * (i kappa) * S
-template <int M, int PT>
+template <int M, int PT, int LAY = NATURAL>
 __device__ __forceinline__ void masked_deriv(const cf* S, cf nyq, cf* Z, cf& zn, int j, int kmax,
                                              float kscale) {
-  constexpr int T = M / PT;
-  const float kj = kscale * (float)j;
+  using MD = Modes<M, PT, LAY>;
+  const float kj = kscale * (float)j, kjm = kscale * (float)(M - j);
 #pragma unroll
   for (int s = 0; s < PT; ++s) {
-    if (slot_dropped<T>(s, kmax)) {
+    if (slot_dropped<M, PT, LAY>(s, kmax)) {
       Z[s] = cf{0.f, 0.f};
       continue;
     }
-    const float kap = fmaf(kscale, (float)(s * T), kj);  // kscale * k, k = j + s*T
+    const float kap = scaled_mode<M, PT, LAY>(kscale, kj, kjm, s);  // kscale * k
     const cf z = cf{-kap * S[s].im, kap * S[s].re};
-    if (slot_kept<T>(s, kmax)) {
+    if (slot_kept<M, PT, LAY>(s, kmax)) {
       Z[s] = z;
     } else {
-      const int k = j + s * T;
-      Z[s] = (k <= kmax) ? z : cf{0.f, 0.f};
+      Z[s] = (MD::k(j, s) <= kmax) ? z : cf{0.f, 0.f};
     }
   }
-  const float kn = kscale * (float)M;
-  zn = (M <= kmax) ? cf{-kn * nyq.im, kn * nyq.re} : cf{0.f, 0.f};
+  const float kn = kscale * (float)MD::kextra;
+  zn = (MD::kextra <= kmax) ? cf{-kn * nyq.im, kn * nyq.re} : cf{0.f, 0.f};
 }
 
 // Nn = mask * f * X for a real per-mode factor f(k) (dealias after a forward
 // transform). Fully dropped slots are already zero from rfft's kcut.
-template <int M, int PT, class F>
+template <int M, int PT, int LAY = NATURAL, class F>
 __device__ __forceinline__ void masked_scale(const cf* X, cf* Nn, int j, int kmax, F f) {
-  constexpr int T = M / PT;
+  using MD = Modes<M, PT, LAY>;
 #pragma unroll
   for (int s = 0; s < PT; ++s) {
-    if (slot_dropped<T>(s, kmax)) {
+    if (slot_dropped<M, PT, LAY>(s, kmax)) {
       Nn[s] = cf{0.f, 0.f};
       continue;
     }
     const cf y = f(s, X[s]);
-    if (slot_kept<T>(s, kmax)) {
+    if (slot_kept<M, PT, LAY>(s, kmax)) {
       Nn[s] = y;
     } else {
-      const int k = j + s * T;
-      Nn[s] = (k <= kmax) ? y : cf{0.f, 0.f};
+      Nn[s] = (MD::k(j, s) <= kmax) ? y : cf{0.f, 0.f};
     }
   }
 }
@@ -130,11 +137,12 @@ __device__ __forceinline__ void masked_scale(const cf* X, cf* Nn, int j, int kma
 // transforms of 2 u_hat comes back from the forward transform as
 // 8 N^2 F[product]; the factor q2 = 2 / (8 N^2) restores 2 N_hat (powers of
 // two, exact).
-template <int KIND, int M, int PT, int NT>
+template <int KIND, int M, int PT, int NT, int LAY = NATURAL>
 __device__ __forceinline__ void nl_eval(const cf (*S)[PT], const cf* nyq, cf (*Nn)[PT], cf* nn,
                                         const Lane<M, PT>& ln, cf* lds, const TwRegs<M, PT>& rt,
                                         const Step1DArgs& a) {
-  constexpr int T = M / PT;
+  using MD = Modes<M, PT, LAY>;
+  constexpr int KX = MD::kextra;
   constexpr float NN = 2.f * M;
   constexpr float q2 = 2.f / (8.f * NN * NN);
   const int j = ln.j;
@@ -152,22 +160,22 @@ __device__ __forceinline__ void nl_eval(const cf (*S)[PT], const cf* nyq, cf (*N
     cf Z[NT][PT];
     cf zn[NT];
 #pragma unroll
-    for (int n = 0; n < NT; ++n) masked_copy<M, PT>(S[n], nyq[n], Z[n], zn[n], j, a.kmax);
-    irfft<M, PT, NT>(Z, zn, ln, lds, rt);
+    for (int n = 0; n < NT; ++n) masked_copy<M, PT, LAY>(S[n], nyq[n], Z[n], zn[n], j, a.kmax);
+    irfft_l<LAY, M, PT, NT>(Z, zn, ln, lds, rt, a.kmax);
 #pragma unroll
     for (int n = 0; n < NT; ++n)
 #pragma unroll
       for (int s = 0; s < PT; ++s) Z[n][s] = cf{Z[n][s].re * Z[n][s].re, Z[n][s].im * Z[n][s].im};
-    rfft<M, PT, NT>(Z, zn, ln, lds, rt, kc);
+    rfft_l<LAY, M, PT, NT>(Z, zn, ln, lds, rt, kc);
     const float c = -a.scale * 0.5f * a.kscale * q2;
-    const float cj = c * (float)j;
+    const float cj = c * (float)j, cjm = c * (float)(M - j);
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      masked_scale<M, PT>(Z[n], Nn[n], j, a.kmax, [&](int s, cf x) {
-        const float f = fmaf(c, (float)(s * T), cj);  // c * k
+      masked_scale<M, PT, LAY>(Z[n], Nn[n], j, a.kmax, [&](int s, cf x) {
+        const float f = scaled_mode<M, PT, LAY>(c, cj, cjm, s);  // c * k
         return cf{-f * x.im, f * x.re};
       });
-      const float fn = (M <= a.kmax) ? c * (float)M : 0.f;
+      const float fn = (KX <= a.kmax) ? c * (float)KX : 0.f;
       nn[n] = cf{-fn * zn[n].im, fn * zn[n].re};
     }
   } else if constexpr (KIND == NL_CONV_NONCONS) {
@@ -176,21 +184,21 @@ __device__ __forceinline__ void nl_eval(const cf (*S)[PT], const cf* nyq, cf (*N
     cf un[NT], vn[NT];
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      masked_copy<M, PT>(S[n], nyq[n], U[n], un[n], j, a.kmax);
-      masked_deriv<M, PT>(S[n], nyq[n], V[n], vn[n], j, a.kmax, a.kscale);
+      masked_copy<M, PT, LAY>(S[n], nyq[n], U[n], un[n], j, a.kmax);
+      masked_deriv<M, PT, LAY>(S[n], nyq[n], V[n], vn[n], j, a.kmax, a.kscale);
     }
-    irfft<M, PT, NT>(U, un, ln, lds, rt);
-    irfft<M, PT, NT>(V, vn, ln, lds, rt);
+    irfft_l<LAY, M, PT, NT>(U, un, ln, lds, rt, a.kmax);
+    irfft_l<LAY, M, PT, NT>(V, vn, ln, lds, rt, a.kmax);
 #pragma unroll
     for (int n = 0; n < NT; ++n)
 #pragma unroll
       for (int s = 0; s < PT; ++s) U[n][s] = cf{U[n][s].re * V[n][s].re, U[n][s].im * V[n][s].im};
-    rfft<M, PT, NT>(U, un, ln, lds, rt, kc);
+    rfft_l<LAY, M, PT, NT>(U, un, ln, lds, rt, kc);
     const float c = -a.scale * q2;
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      masked_scale<M, PT>(U[n], Nn[n], j, a.kmax, [&](int, cf x) { return c * x; });
-      nn[n] = ((M <= a.kmax) ? c : 0.f) * un[n];
+      masked_scale<M, PT, LAY>(U[n], Nn[n], j, a.kmax, [&](int, cf x) { return c * x; });
+      nn[n] = ((KX <= a.kmax) ? c : 0.f) * un[n];
     }
   } else if constexpr (KIND == NL_GRADNORM) {
     // -b/2 F[(u_x)^2 - mean]  (nonlin_fun/_gradient_norm.py:84-101); the mean
@@ -198,19 +206,19 @@ __device__ __forceinline__ void nl_eval(const cf (*S)[PT], const cf* nyq, cf (*N
     cf V[NT][PT];
     cf vn[NT];
 #pragma unroll
-    for (int n = 0; n < NT; ++n) masked_deriv<M, PT>(S[n], nyq[n], V[n], vn[n], j, a.kmax, a.kscale);
-    irfft<M, PT, NT>(V, vn, ln, lds, rt);
+    for (int n = 0; n < NT; ++n) masked_deriv<M, PT, LAY>(S[n], nyq[n], V[n], vn[n], j, a.kmax, a.kscale);
+    irfft_l<LAY, M, PT, NT>(V, vn, ln, lds, rt, a.kmax);
 #pragma unroll
     for (int n = 0; n < NT; ++n)
 #pragma unroll
       for (int s = 0; s < PT; ++s) V[n][s] = cf{V[n][s].re * V[n][s].re, V[n][s].im * V[n][s].im};
-    rfft<M, PT, NT>(V, vn, ln, lds, rt, kc);
+    rfft_l<LAY, M, PT, NT>(V, vn, ln, lds, rt, kc);
     const float c = -a.scale * 0.5f * q2;
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      masked_scale<M, PT>(V[n], Nn[n], j, a.kmax, [&](int, cf x) { return c * x; });
+      masked_scale<M, PT, LAY>(V[n], Nn[n], j, a.kmax, [&](int, cf x) { return c * x; });
       if (j == 0 && a.zero_mode_fix) Nn[n][0] = cf{0.f, 0.f};
-      nn[n] = ((M <= a.kmax) ? c : 0.f) * vn[n];
+      nn[n] = ((KX <= a.kmax) ? c : 0.f) * vn[n];
     }
   } else if constexpr (KIND == NL_POLY) {
     // F[sum_i c_i u^i]  (nonlin_fun/_polynomial.py:64-76), same power loop;
@@ -218,8 +226,8 @@ __device__ __forceinline__ void nl_eval(const cf (*S)[PT], const cf* nyq, cf (*N
     cf U[NT][PT];
     cf un[NT];
 #pragma unroll
-    for (int n = 0; n < NT; ++n) masked_copy<M, PT>(S[n], nyq[n], U[n], un[n], j, a.kmax);
-    irfft<M, PT, NT>(U, un, ln, lds, rt);
+    for (int n = 0; n < NT; ++n) masked_copy<M, PT, LAY>(S[n], nyq[n], U[n], un[n], j, a.kmax);
+    irfft_l<LAY, M, PT, NT>(U, un, ln, lds, rt, a.kmax);
     constexpr float r2n = 1.f / (2.f * NN);
 #pragma unroll
     for (int n = 0; n < NT; ++n)
@@ -235,11 +243,11 @@ __device__ __forceinline__ void nl_eval(const cf (*S)[PT], const cf* nyq, cf (*N
         }
         U[n][s] = cf{ar, ai};
       }
-    rfft<M, PT, NT>(U, un, ln, lds, rt, kc);
+    rfft_l<LAY, M, PT, NT>(U, un, ln, lds, rt, kc);
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      masked_scale<M, PT>(U[n], Nn[n], j, a.kmax, [&](int, cf x) { return x; });
-      nn[n] = (M <= a.kmax) ? un[n] : cf{0.f, 0.f};
+      masked_scale<M, PT, LAY>(U[n], Nn[n], j, a.kmax, [&](int, cf x) { return x; });
+      nn[n] = (KX <= a.kmax) ? un[n] : cf{0.f, 0.f};
     }
   } else if constexpr (KIND == NL_GENERAL) {
     // b0 F[u^2] + (b1/2)(i kappa) F[u^2] + (b2/2) F[u_x^2 - mean]
@@ -249,11 +257,11 @@ __device__ __forceinline__ void nl_eval(const cf (*S)[PT], const cf* nyq, cf (*N
     cf un[NT], vn[NT];
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      masked_copy<M, PT>(S[n], nyq[n], U[n], un[n], j, a.kmax);
-      masked_deriv<M, PT>(S[n], nyq[n], V[n], vn[n], j, a.kmax, a.kscale);
+      masked_copy<M, PT, LAY>(S[n], nyq[n], U[n], un[n], j, a.kmax);
+      masked_deriv<M, PT, LAY>(S[n], nyq[n], V[n], vn[n], j, a.kmax, a.kscale);
     }
-    irfft<M, PT, NT>(U, un, ln, lds, rt);
-    irfft<M, PT, NT>(V, vn, ln, lds, rt);
+    irfft_l<LAY, M, PT, NT>(U, un, ln, lds, rt, a.kmax);
+    irfft_l<LAY, M, PT, NT>(V, vn, ln, lds, rt, a.kmax);
 #pragma unroll
     for (int n = 0; n < NT; ++n)
 #pragma unroll
@@ -261,14 +269,14 @@ __device__ __forceinline__ void nl_eval(const cf (*S)[PT], const cf* nyq, cf (*N
         U[n][s] = cf{U[n][s].re * U[n][s].re, U[n][s].im * U[n][s].im};
         V[n][s] = cf{V[n][s].re * V[n][s].re, V[n][s].im * V[n][s].im};
       }
-    rfft<M, PT, NT>(U, un, ln, lds, rt, kc);
-    rfft<M, PT, NT>(V, vn, ln, lds, rt, kc);
+    rfft_l<LAY, M, PT, NT>(U, un, ln, lds, rt, kc);
+    rfft_l<LAY, M, PT, NT>(V, vn, ln, lds, rt, kc);
     const float g0 = a.g0 * q2, hb1 = 0.5f * a.g1 * a.kscale * q2, hb2 = 0.5f * a.g2 * q2;
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
 #pragma unroll
       for (int s = 0; s < PT; ++s) {
-        const int k = j + s * T;
+        const int k = MD::k(j, s);
         const bool keep = k <= a.kmax;
         const float ck = hb1 * (float)k;
         const float cg = (k == 0 && a.zero_mode_fix) ? 0.f : hb2;
@@ -276,10 +284,10 @@ __device__ __forceinline__ void nl_eval(const cf (*S)[PT], const cf* nyq, cf (*N
         cf r = cf{g0 * u.re - ck * u.im + cg * v.re, g0 * u.im + ck * u.re + cg * v.im};
         Nn[n][s] = keep ? r : cf{0.f, 0.f};
       }
-      const float ck = hb1 * (float)M;
+      const float ck = hb1 * (float)KX;
       const cf u = un[n], v = vn[n];
       cf r = cf{g0 * u.re - ck * u.im + hb2 * v.re, g0 * u.im + ck * u.re + hb2 * v.im};
-      nn[n] = (M <= a.kmax) ? r : cf{0.f, 0.f};
+      nn[n] = (KX <= a.kmax) ? r : cf{0.f, 0.f};
     }
   }
 }
@@ -291,12 +299,14 @@ template <int NCO, int PT, int NT>
 struct CoefRegs {
   float v[NT][NCO][PT + 1];
   __device__ __forceinline__ float operator()(int n, int i, int s) const { return v[n][i][s]; }
-  __device__ __forceinline__ void load(int n, const float* __restrict__ tb, int j, int T, int M) {
+  template <int M, int LAY>
+  __device__ __forceinline__ void load(int n, const float* __restrict__ tb, int j) {
+    using MD = Modes<M, PT, LAY>;
 #pragma unroll
     for (int i = 0; i < NCO; ++i) {
 #pragma unroll
-      for (int s = 0; s < PT; ++s) v[n][i][s] = tb[i * (M + 1) + j + s * T];
-      v[n][i][PT] = tb[i * (M + 1) + M];
+      for (int s = 0; s < PT; ++s) v[n][i][s] = tb[i * (M + 1) + MD::k(j, s)];
+      v[n][i][PT] = tb[i * (M + 1) + MD::kextra];
     }
   }
 };
@@ -322,7 +332,7 @@ struct CoefLds {
 // _etdrk_1.py:77-82, _etdrk_2.py:91-102, _etdrk_3.py:191-212,
 // _etdrk_4.py:198-224. Table order: ETDRK0 [E]; ETDRK1 [E, c1];
 // ETDRK2 [E, c1, c2]; ETDRK3 [E, E/2, c1..c5]; ETDRK4 [E, E/2, c1, c4, c5, c6].
-template <int KIND, int ORDER, int M, int PT, int NT, class Coef>
+template <int KIND, int ORDER, int M, int PT, int NT, class Coef, int LAY = NATURAL>
 __device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co, const Lane<M, PT>& ln, cf* lds,
                                            const TwRegs<M, PT>& rt, const Step1DArgs& a) {
 #define EXMI_EACH(body)                   \
@@ -335,18 +345,18 @@ __device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co,
   } else if constexpr (ORDER == 1) {
     cf N0[NT][PT];
     cf n0[NT];
-    nl_eval<KIND, M, PT, NT>(S, nyq, N0, n0, ln, lds, rt, a);
+    nl_eval<KIND, M, PT, NT, LAY>(S, nyq, N0, n0, ln, lds, rt, a);
     EXMI_EACH(S[n][s] = co(n, 0, s) * S[n][s] + co(n, 1, s) * N0[n][s]);
 #pragma unroll
     for (int n = 0; n < NT; ++n) nyq[n] = co(n, 0, PT) * nyq[n] + co(n, 1, PT) * n0[n];
   } else if constexpr (ORDER == 2) {
     cf N0[NT][PT], N1[NT][PT];
     cf n0[NT], n1[NT];
-    nl_eval<KIND, M, PT, NT>(S, nyq, N0, n0, ln, lds, rt, a);
+    nl_eval<KIND, M, PT, NT, LAY>(S, nyq, N0, n0, ln, lds, rt, a);
     EXMI_EACH(S[n][s] = co(n, 0, s) * S[n][s] + co(n, 1, s) * N0[n][s]);
 #pragma unroll
     for (int n = 0; n < NT; ++n) nyq[n] = co(n, 0, PT) * nyq[n] + co(n, 1, PT) * n0[n];
-    nl_eval<KIND, M, PT, NT>(S, nyq, N1, n1, ln, lds, rt, a);
+    nl_eval<KIND, M, PT, NT, LAY>(S, nyq, N1, n1, ln, lds, rt, a);
     EXMI_EACH(S[n][s] = S[n][s] + co(n, 2, s) * (N1[n][s] - N0[n][s]));
 #pragma unroll
     for (int n = 0; n < NT; ++n) nyq[n] = nyq[n] + co(n, 2, PT) * (n1[n] - n0[n]);
@@ -356,7 +366,7 @@ __device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co,
     EXMI_EACH(U0[n][s] = S[n][s]);
 #pragma unroll
     for (int n = 0; n < NT; ++n) u0n[n] = nyq[n];
-    nl_eval<KIND, M, PT, NT>(S, nyq, N0, n0, ln, lds, rt, a);
+    nl_eval<KIND, M, PT, NT, LAY>(S, nyq, N0, n0, ln, lds, rt, a);
     // accumulate the final combination E u + c3 N0 + c4 N1 + c5 N2
     EXMI_EACH(A[n][s] = co(n, 0, s) * U0[n][s] + co(n, 4, s) * N0[n][s];
               S[n][s] = co(n, 1, s) * U0[n][s] + co(n, 2, s) * N0[n][s]);  // stage 1
@@ -365,7 +375,7 @@ __device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co,
       an[n] = co(n, 0, PT) * u0n[n] + co(n, 4, PT) * n0[n];
       nyq[n] = co(n, 1, PT) * u0n[n] + co(n, 2, PT) * n0[n];
     }
-    nl_eval<KIND, M, PT, NT>(S, nyq, Nk, nk, ln, lds, rt, a);  // N1
+    nl_eval<KIND, M, PT, NT, LAY>(S, nyq, Nk, nk, ln, lds, rt, a);  // N1
     EXMI_EACH(A[n][s] = A[n][s] + co(n, 5, s) * Nk[n][s];
               S[n][s] = co(n, 0, s) * U0[n][s] + co(n, 3, s) * (2.f * Nk[n][s] - N0[n][s]));  // stage 2
 #pragma unroll
@@ -373,7 +383,7 @@ __device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co,
       an[n] = an[n] + co(n, 5, PT) * nk[n];
       nyq[n] = co(n, 0, PT) * u0n[n] + co(n, 3, PT) * (2.f * nk[n] - n0[n]);
     }
-    nl_eval<KIND, M, PT, NT>(S, nyq, Nk, nk, ln, lds, rt, a);  // N2
+    nl_eval<KIND, M, PT, NT, LAY>(S, nyq, Nk, nk, ln, lds, rt, a);  // N2
     EXMI_EACH(S[n][s] = A[n][s] + co(n, 6, s) * Nk[n][s]);
 #pragma unroll
     for (int n = 0; n < NT; ++n) nyq[n] = an[n] + co(n, 6, PT) * nk[n];
@@ -383,7 +393,7 @@ __device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co,
     EXMI_EACH(U0[n][s] = S[n][s]);
 #pragma unroll
     for (int n = 0; n < NT; ++n) u0n[n] = nyq[n];
-    nl_eval<KIND, M, PT, NT>(S, nyq, N0, n0, ln, lds, rt, a);
+    nl_eval<KIND, M, PT, NT, LAY>(S, nyq, N0, n0, ln, lds, rt, a);
     EXMI_EACH(A[n][s] = co(n, 0, s) * U0[n][s] + co(n, 3, s) * N0[n][s];
               S1[n][s] = co(n, 1, s) * U0[n][s] + co(n, 2, s) * N0[n][s]);
 #pragma unroll
@@ -391,7 +401,7 @@ __device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co,
       an[n] = co(n, 0, PT) * u0n[n] + co(n, 3, PT) * n0[n];
       s1n[n] = co(n, 1, PT) * u0n[n] + co(n, 2, PT) * n0[n];
     }
-    nl_eval<KIND, M, PT, NT>(S1, s1n, Nk, nk, ln, lds, rt, a);  // N1
+    nl_eval<KIND, M, PT, NT, LAY>(S1, s1n, Nk, nk, ln, lds, rt, a);  // N1
     EXMI_EACH(A[n][s] = A[n][s] + (co(n, 4, s) * 2.f) * Nk[n][s];
               S[n][s] = co(n, 1, s) * U0[n][s] + co(n, 2, s) * Nk[n][s]);  // stage 2
 #pragma unroll
@@ -399,7 +409,7 @@ __device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co,
       an[n] = an[n] + (co(n, 4, PT) * 2.f) * nk[n];
       nyq[n] = co(n, 1, PT) * u0n[n] + co(n, 2, PT) * nk[n];
     }
-    nl_eval<KIND, M, PT, NT>(S, nyq, Nk, nk, ln, lds, rt, a);  // N2
+    nl_eval<KIND, M, PT, NT, LAY>(S, nyq, Nk, nk, ln, lds, rt, a);  // N2
     EXMI_EACH(A[n][s] = A[n][s] + (co(n, 4, s) * 2.f) * Nk[n][s];
               S[n][s] = co(n, 1, s) * S1[n][s] + co(n, 2, s) * (2.f * Nk[n][s] - N0[n][s]));  // stage 3
 #pragma unroll
@@ -407,7 +417,7 @@ __device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co,
       an[n] = an[n] + (co(n, 4, PT) * 2.f) * nk[n];
       nyq[n] = co(n, 1, PT) * s1n[n] + co(n, 2, PT) * (2.f * nk[n] - n0[n]);
     }
-    nl_eval<KIND, M, PT, NT>(S, nyq, Nk, nk, ln, lds, rt, a);  // N3
+    nl_eval<KIND, M, PT, NT, LAY>(S, nyq, Nk, nk, ln, lds, rt, a);  // N3
     EXMI_EACH(S[n][s] = A[n][s] + co(n, 5, s) * Nk[n][s]);
 #pragma unroll
     for (int n = 0; n < NT; ++n) nyq[n] = an[n] + co(n, 5, PT) * nk[n];
@@ -429,13 +439,18 @@ struct Slots {
 #ifndef EXMI_STEP_WAVES_PER_EU
 #define EXMI_STEP_WAVES_PER_EU 1
 #endif
+// Spectral layout of the step and rollout kernels (fft1d.hpp).
+#ifndef EXMI_LAYOUT_1D
+#define EXMI_LAYOUT_1D PAIRED
+#endif
 
 // One step (or one nonlinear evaluation) of every transform of the batch.
 // Tables are staged through lane-private LDS to keep register pressure low.
-template <int M, int PT, int KIND, int ORDER>
+template <int M, int PT, int KIND, int ORDER, int LAY = EXMI_LAYOUT_1D>
 __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
     __attribute__((amdgpu_waves_per_eu(EXMI_STEP_WAVES_PER_EU))) step1d_kernel(Step1DArgs a) {
   constexpr int T = M / PT;
+  using MD = Modes<M, PT, LAY>;
   constexpr int G = Slots<M, PT, 1>::G;
   constexpr int LDSN = LdsSize<M>::value;
   constexpr int NR = M + 1;
@@ -464,8 +479,8 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
   if (a.in_fourier) {
     const cf* in = reinterpret_cast<const cf*>(a.in) + (long long)b * a.in_bstride + (long long)c * NR;
 #pragma unroll
-    for (int s = 0; s < PT; ++s) S[0][s] = 2.f * in[j + s * T];
-    nyq[0] = 2.f * in[M];
+    for (int s = 0; s < PT; ++s) S[0][s] = 2.f * in[MD::k(j, s)];
+    nyq[0] = 2.f * in[MD::kextra];
   } else {
     const float2* in = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(a.in) +
                                                        (long long)b * a.in_bstride + (long long)c * 2 * M);
@@ -481,25 +496,25 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
     const float* tb = a.tables + ((long long)(a.lin_batch > 1 ? b : 0) * a.lin_channels +
                                   (a.lin_channels > 1 ? c : 0)) * (long long)NCO * NR;
     CoefRegs<NCO, PT, 1> cr;
-    cr.load(0, tb, j, T, M);
-    if (!a.in_fourier) rfft<M, PT, 1>(S, nyq, ln, lds, rt);
+    cr.template load<M, LAY>(0, tb, j);
+    if (!a.in_fourier) rfft_l<LAY, M, PT, 1>(S, nyq, ln, lds, rt);
 #pragma unroll
     for (int i = 0; i < NCO; ++i)
 #pragma unroll
       for (int s = 0; s <= PT; ++s) cl.put(0, i, s, cr(0, i, s));
   } else if (!a.in_fourier) {
-    rfft<M, PT, 1>(S, nyq, ln, lds, rt);
+    rfft_l<LAY, M, PT, 1>(S, nyq, ln, lds, rt);
   }
 
   if (a.nonlin_only) {
     cf Nn[1][PT];
     cf nn[1];
-    nl_eval<KIND, M, PT, 1>(S, nyq, Nn, nn, ln, lds, rt, a);
+    nl_eval<KIND, M, PT, 1, LAY>(S, nyq, Nn, nn, ln, lds, rt, a);
 #pragma unroll
     for (int s = 0; s < PT; ++s) S[0][s] = Nn[0][s];
     nyq[0] = nn[0];
   } else {
-    etdrk_step<KIND, ORDER, M, PT, 1>(S, nyq, cl, ln, lds, rt, a);
+    etdrk_step<KIND, ORDER, M, PT, 1, CL, LAY>(S, nyq, cl, ln, lds, rt, a);
   }
 
   // ---- store (2 u_hat -> u_hat, or N * 2 u -> u)
@@ -507,11 +522,11 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
     if (active) {
       cf* out = reinterpret_cast<cf*>(a.out) + (long long)b * a.out_bstride + (long long)c * NR;
 #pragma unroll
-      for (int s = 0; s < PT; ++s) out[j + s * T] = 0.5f * S[0][s];
-      if (j == 0) out[M] = 0.5f * nyq[0];
+      for (int s = 0; s < PT; ++s) out[MD::k(j, s)] = 0.5f * S[0][s];
+      if (j == 0) out[MD::kextra] = 0.5f * nyq[0];
     }
   } else {
-    irfft<M, PT, 1>(S, nyq, ln, lds, rt);
+    irfft_l<LAY, M, PT, 1>(S, nyq, ln, lds, rt);
     if (active) {
       constexpr float r2n = 1.f / (2.f * NN);
       float2* out = reinterpret_cast<float2*>(reinterpret_cast<float*>(a.out) + (long long)b * a.out_bstride +
@@ -522,12 +537,12 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
   }
 }
 
-template <int M, int PT, int KIND, int ORDER>
+template <int M, int PT, int KIND, int ORDER, int LAY = EXMI_LAYOUT_1D>
 inline hipError_t launch_step1d(const Step1DArgs& a, hipStream_t st) {
   constexpr int WG = Slots<M, PT, 1>::WG;
   constexpr int G = Slots<M, PT, 1>::G;
   const int grid = (a.ntrans + G - 1) / G;
-  hipLaunchKernelGGL((step1d_kernel<M, PT, KIND, ORDER>), dim3(grid), dim3(WG), 0, st, a);
+  hipLaunchKernelGGL((step1d_kernel<M, PT, KIND, ORDER, LAY>), dim3(grid), dim3(WG), 0, st, a);
   return hipGetLastError();
 }
 
@@ -570,10 +585,11 @@ __device__ __forceinline__ void store_states(const cf (*S)[PT], float scale, con
 // the inverse transform of a copy. It differs from the physical path only in
 // rounding (no float32 round trip of the state through physical space), and
 // it saves one forward transform per step.
-template <int M, int PT, int KIND, int ORDER, int NT, bool FR = false>
+template <int M, int PT, int KIND, int ORDER, int NT, bool FR = false, int LAY = EXMI_LAYOUT_1D>
 __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(Roll1DArgs ra) {
   const Step1DArgs& a = ra.s;
   constexpr int T = M / PT;
+  using MD = Modes<M, PT, LAY>;
   constexpr int G = Slots<M, PT, NT>::G;
   constexpr int LDSN = LdsSize<M>::value;
   constexpr int NR = M + 1;
@@ -615,15 +631,15 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
 #pragma unroll
     for (int i = 0; i < NCO; ++i) {
 #pragma unroll
-      for (int s = 0; s < PT; ++s) cl.put(n, i, s, tb[i * NR + j + s * T]);
-      cl.put(n, i, PT, tb[i * NR + M]);
+      for (int s = 0; s < PT; ++s) cl.put(n, i, s, tb[i * NR + MD::k(j, s)]);
+      cl.put(n, i, PT, tb[i * NR + MD::kextra]);
     }
   }
 
-  if constexpr (FR) rfft<M, PT, NT>(S, nyq, ln, lds, rt);
+  if constexpr (FR) rfft_l<LAY, M, PT, NT>(S, nyq, ln, lds, rt);
   for (int step = 0; step < ra.nsteps; ++step) {
-    if constexpr (!FR) rfft<M, PT, NT>(S, nyq, ln, lds, rt);
-    etdrk_step<KIND, ORDER, M, PT, NT>(S, nyq, cl, ln, lds, rt, a);
+    if constexpr (!FR) rfft_l<LAY, M, PT, NT>(S, nyq, ln, lds, rt);
+    etdrk_step<KIND, ORDER, M, PT, NT, CL, LAY>(S, nyq, cl, ln, lds, rt, a);
     const bool emit = !ra.final_only || step == ra.nsteps - 1;
     if constexpr (FR) {
       if (emit) {
@@ -635,11 +651,11 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
 #pragma unroll
           for (int s = 0; s < PT; ++s) O[n][s] = S[n][s];
         }
-        irfft<M, PT, NT>(O, on, ln, lds, rt);
+        irfft_l<LAY, M, PT, NT>(O, on, ln, lds, rt);
         store_states<M, PT, NT>(O, r2n, ra, tb_, cb_, act, step, j);
       }
     } else {
-      irfft<M, PT, NT>(S, nyq, ln, lds, rt);
+      irfft_l<LAY, M, PT, NT>(S, nyq, ln, lds, rt);
       // back to the float32 physical state, exactly as a stored-and-reloaded
       // state of the single-step path
 #pragma unroll
@@ -651,12 +667,12 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
   }
 }
 
-template <int M, int PT, int KIND, int ORDER, int NT, bool FR = false>
+template <int M, int PT, int KIND, int ORDER, int NT, bool FR = false, int LAY = EXMI_LAYOUT_1D>
 inline hipError_t launch_roll1d(const Roll1DArgs& ra, hipStream_t st) {
   constexpr int WG = Slots<M, PT, NT>::WG;
   constexpr int PB = Slots<M, PT, NT>::PER_BLOCK;
   const int grid = (ra.s.ntrans + PB - 1) / PB;
-  hipLaunchKernelGGL((roll1d_kernel<M, PT, KIND, ORDER, NT, FR>), dim3(grid), dim3(WG), 0, st, ra);
+  hipLaunchKernelGGL((roll1d_kernel<M, PT, KIND, ORDER, NT, FR, LAY>), dim3(grid), dim3(WG), 0, st, ra);
   return hipGetLastError();
 }
 
