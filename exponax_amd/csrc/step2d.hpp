@@ -35,6 +35,18 @@
 #include "exmi_internal.hpp"
 #include "fft1d.hpp"
 
+// 4 points per lane up to these sizes, 8 above (build-time knobs; the
+// defaults were the fastest measured for the mixed 128^2/256^2/512^2 batch)
+#ifndef EXMI_ROW_PT4_MAX
+#define EXMI_ROW_PT4_MAX 128
+#endif
+#ifndef EXMI_ROW_R
+#define EXMI_ROW_R 16
+#endif
+#ifndef EXMI_COL_PT4_MAX
+#define EXMI_COL_PT4_MAX 128
+#endif
+
 namespace exmi {
 
 struct Step2DArgs {
@@ -96,11 +108,11 @@ struct Fields2D<NL_GRADNORM> {
 template <int N>
 struct RowCfg {
   static constexpr int M = N / 2;
-  // 4 points per lane up to 128^2 (twice the wavefronts per launch and a
-  // shorter per-lane chain for these latency-bound sizes), 8 above
-  static constexpr int PT = N <= 128 ? 4 : 8;
+  // 4 points per lane at small N: twice the wavefronts per launch and a
+  // shorter per-lane chain for these latency-bound sizes
+  static constexpr int PT = N <= EXMI_ROW_PT4_MAX ? 4 : 8;
   static constexpr int T = M / PT;
-  static constexpr int R = 16;  // rows per workgroup
+  static constexpr int R = EXMI_ROW_R;  // rows per workgroup
   static constexpr int WG = R * T;
   static constexpr int NY = M + 1;
   static constexpr int LD = R + 1;  // padded tile row
@@ -277,7 +289,7 @@ inline hipError_t launch_row2d(const Step2DArgs& a, int ninst, hipStream_t st) {
 // owned by T = N/PT lanes; CPB columns per 256-thread workgroup.
 template <int N>
 struct ColCfg {
-  static constexpr int PT = N <= 128 ? 4 : 8;
+  static constexpr int PT = N <= EXMI_COL_PT4_MAX ? 4 : 8;
   static constexpr int T = N / PT;
   static constexpr int WG = 256;
   static constexpr int CPB = WG / T;  // columns per workgroup
