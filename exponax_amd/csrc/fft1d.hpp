@@ -40,6 +40,43 @@ __device__ __forceinline__ cf rot90(cf a) {
 }
 
 // ---------------------------------------------------------------- butterflies
+// Twiddled passes multiply inputs 1..R-1 of every butterfly by twiddles w
+// (conj(w) for the inverse). Where a first-stage radix-2 butterfly meets a
+// twiddled input, a + b w is formed with four fused multiply-adds and
+// a - b w as 2 a - (a + b w): 6 operations instead of the 8 of a complex
+// multiply followed by an add and a subtract.
+template <bool INV>
+__device__ __forceinline__ cf twid(cf b, cf w) {
+  return INV ? cmulc(b, w) : cmul(b, w);
+}
+// a + b w (a + b conj(w) for INV) in four fused multiply-adds
+template <bool INV>
+__device__ __forceinline__ cf cfma(cf a, cf b, cf w) {
+  if constexpr (INV)  // b conj(w) = (b.re w.re + b.im w.im, b.im w.re - b.re w.im)
+    return cf{fmaf(b.im, w.im, fmaf(b.re, w.re, a.re)), fmaf(-b.re, w.im, fmaf(b.im, w.re, a.im))};
+  else  // b w = (b.re w.re - b.im w.im, b.re w.im + b.im w.re)
+    return cf{fmaf(-b.im, w.im, fmaf(b.re, w.re, a.re)), fmaf(b.im, w.re, fmaf(b.re, w.im, a.im))};
+}
+template <bool INV>
+__device__ __forceinline__ void bfly_tw(cf a, cf b, cf w, cf& s, cf& d) {
+  s = cfma<INV>(a, b, w);
+  d = cf{fmaf(2.f, a.re, -s.re), fmaf(2.f, a.im, -s.im)};
+}
+
+// DFT4 of (y0, y1, y2, y3) with y_r = x_r w_r; w0 = 1 unless TW0.
+// The pairs (y0, y2) and (y1, y3) are the first-stage butterflies.
+template <bool INV, bool TW0>
+__device__ __forceinline__ void dft4_tw(cf* v, cf w0, cf w1, cf w2, cf w3) {
+  cf t0, t1, t2, t3;
+  bfly_tw<INV>(TW0 ? twid<INV>(v[0], w0) : v[0], v[2], w2, t0, t1);
+  bfly_tw<INV>(twid<INV>(v[1], w1), v[3], w3, t2, t3);
+  t3 = rot90<INV>(t3);
+  v[0] = t0 + t2;
+  v[2] = t0 - t2;
+  v[1] = t1 + t3;
+  v[3] = t1 - t3;
+}
+
 template <int R, bool INV>
 struct Dft;
 
@@ -50,6 +87,8 @@ struct Dft<2, INV> {
     v[0] = a + b;
     v[1] = a - b;
   }
+  // w[r - 1] multiplies input r
+  __device__ __forceinline__ static void run_tw(cf* v, const cf* w) { bfly_tw<INV>(v[0], v[1], w[0], v[0], v[1]); }
 };
 
 template <bool INV>
@@ -62,6 +101,9 @@ struct Dft<4, INV> {
     v[1] = t1 + t3;
     v[3] = t1 - t3;
   }
+  __device__ __forceinline__ static void run_tw(cf* v, const cf* w) {
+    dft4_tw<INV, false>(v, cf{1.f, 0.f}, w[0], w[1], w[2]);
+  }
 };
 
 template <bool INV>
@@ -70,11 +112,21 @@ struct Dft<8, INV> {
   // (h (1 -+ i), h (-1 -+ i)) are folded into fused multiply-adds of the
   // final butterfly.
   __device__ __forceinline__ static void run(cf* v) {
-    const float h = 0.70710678118654752440f;
     cf e[4] = {v[0], v[2], v[4], v[6]};
     cf o[4] = {v[1], v[3], v[5], v[7]};
     Dft<4, INV>::run(e);
     Dft<4, INV>::run(o);
+    combine(e, o, v);
+  }
+  __device__ __forceinline__ static void run_tw(cf* v, const cf* w) {
+    cf e[4] = {v[0], v[2], v[4], v[6]};
+    cf o[4] = {v[1], v[3], v[5], v[7]};
+    dft4_tw<INV, false>(e, cf{1.f, 0.f}, w[1], w[3], w[5]);
+    dft4_tw<INV, true>(o, w[0], w[2], w[4], w[6]);
+    combine(e, o, v);
+  }
+  __device__ __forceinline__ static void combine(const cf* e, const cf* o, cf* v) {
+    const float h = 0.70710678118654752440f;
     // forward: W8 o1 = h ((o1.re + o1.im) + i (o1.im - o1.re)),
     //          W8^3 o3 = h ((o3.im - o3.re) - i (o3.re + o3.im));
     // inverse: conjugate factors.
@@ -97,14 +149,23 @@ struct Dft<8, INV> {
 template <bool INV>
 struct Dft<16, INV> {
   // 4 x 4 decomposition: column DFT4s, inner twiddles W16^(r*c), row DFT4s.
-  __device__ __forceinline__ static void run(cf* v) {
+  __device__ __forceinline__ static void run(cf* v) { run_impl<false>(v, nullptr); }
+  __device__ __forceinline__ static void run_tw(cf* v, const cf* w) { run_impl<true>(v, w); }
+  template <bool TW>
+  __device__ __forceinline__ static void run_impl(cf* v, const cf* w) {
     const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
     const float h = 0.70710678118654752440f;
     cf a[4][4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       cf t[4] = {v[c], v[c + 4], v[c + 8], v[c + 12]};
-      Dft<4, INV>::run(t);
+      if constexpr (TW) {
+        const cf one{1.f, 0.f};
+        if (c == 0) dft4_tw<INV, false>(t, one, w[3], w[7], w[11]);
+        else dft4_tw<INV, true>(t, w[c - 1], w[c + 3], w[c + 7], w[c + 11]);
+      } else {
+        Dft<4, INV>::run(t);
+      }
 #pragma unroll
       for (int r = 0; r < 4; ++r) a[r][c] = t[r];
     }
@@ -201,14 +262,8 @@ __device__ __forceinline__ void stockham_pass(cf (*v)[PT], const Lane<M, PT>& ln
       cf x[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) x[r] = v[n][q + r * NB];  // x[b + r*M/R], b = j + q*T
-      if constexpr (NS > 1) {
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          const cf w = pw[q * (R - 1) + r - 1];
-          x[r] = INV ? cmulc(x[r], w) : cmul(x[r], w);
-        }
-      }
-      Dft<R, INV>::run(x);
+      if constexpr (NS > 1) Dft<R, INV>::run_tw(x, pw + q * (R - 1));
+      else Dft<R, INV>::run(x);
 #pragma unroll
       for (int r = 0; r < R; ++r) v[n][q + r * NB] = x[r];
     }
@@ -344,7 +399,7 @@ __device__ __forceinline__ void rfft(cf (*z)[PT], cf* nyq, const Lane<M, PT>& ln
       }
       const cf a = z[n][s], b = conj(zm[n][s]);
       // at k = 0 this yields 2 X[0] = 2 (Re Z0 + Im Z0) since fw[0] = -i
-      z[n][s] = (a + b) + cmul(tr.fw[s], a - b);
+      z[n][s] = cfma<false>(a + b, a - b, tr.fw[s]);
     }
   }
 }
@@ -384,7 +439,7 @@ __device__ __forceinline__ void irfft(cf (*z)[PT], const cf* nyq, const Lane<M, 
 #pragma unroll
     for (int s = 0; s < PT; ++s) {
       const cf a = z[n][s], b = conj(xm[n][s]);
-      z[n][s] = (a + b) + cmulc(a - b, tr.fw[s]);
+      z[n][s] = cfma<true>(a + b, a - b, tr.fw[s]);
     }
   cfft<M, PT, true, NT>(z, ln, lds, tr);
 }
@@ -417,7 +472,7 @@ struct Modes {
 // r2c into the paired layout: z (packed samples, lane-strided) -> 2 X, and
 // xm = 2 X[M/2] (lane 0; zero elsewhere). Modes above `kcut` are not formed
 // where a whole slot lies above it.
-//   2 X[k]   = s + P,   2 X[M-k] = conj(s - P),
+//   2 X[k]   = s + P,   2 X[M-k] = conj(s - P) = conj(2 s - (s + P)),
 //   s = Z[k] + Z*[M-k], P = fw[k] (Z[k] - Z*[M-k]),
 // the second from fw[M-k] = conj(fw[k]); at k = 0 the pair is (X[0], X[M])
 // with Z[0] its own partner, and 2 X[M/2] = 2 conj(Z[M/2]) since
@@ -451,9 +506,15 @@ __device__ __forceinline__ void rfft_paired(cf (*z)[PT], cf* xm, const Lane<M, P
         continue;
       }
       const cf a = z[n][h], b = conj(zp[n][h]);
-      const cf sm = a + b, p = cmul(tr.fw[h], a - b);
-      z[n][h] = lo_drop ? cf{0.f, 0.f} : sm + p;
-      z[n][H + h] = hi_drop ? cf{0.f, 0.f} : conj(sm - p);
+      if (hi_drop) {
+        z[n][h] = cfma<false>(a + b, a - b, tr.fw[h]);
+        z[n][H + h] = cf{0.f, 0.f};
+      } else {
+        cf x0, x1;
+        bfly_tw<false>(a + b, a - b, tr.fw[h], x0, x1);
+        z[n][h] = lo_drop ? cf{0.f, 0.f} : x0;
+        z[n][H + h] = conj(x1);
+      }
     }
   }
 }
@@ -479,16 +540,15 @@ __device__ __forceinline__ void irfft_paired(cf (*z)[PT], const cf* xm, const La
 #pragma unroll
     for (int h = 0; h < H; ++h) {
       const cf a = z[n][h];
+      cf x0, x1;
       if (MD::klo(H + h) > kin) {  // X[M-k] = 0 for every lane
-        const cf p = cmulc(a, tr.fw[h]);
-        z[n][h] = a + p;
-        send[h] = conj(a - p);
+        bfly_tw<true>(a, a, tr.fw[h], x0, x1);
       } else {
         const cf b = conj(z[n][H + h]);
-        const cf sm = a + b, p = cmulc(a - b, tr.fw[h]);
-        z[n][h] = sm + p;
-        send[h] = conj(sm - p);
+        bfly_tw<true>(a + b, a - b, tr.fw[h], x0, x1);
       }
+      z[n][h] = x0;
+      send[h] = conj(x1);
     }
     // Z'[M-k] to the natural slot of element M-k (lane 0, h = 0 writes the
     // meaningless Z'[M] to element 0's slot, which is not read back)
