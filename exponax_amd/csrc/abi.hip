@@ -29,7 +29,7 @@ hipError_t launch_coef(const double* L, long long total, int nr, double dt, int 
 
 #define EXMI_DECL_SIZE(M)                                                                              \
   StepFn get_step1d_##M(int kind, int order);                                                          \
-  RollFn get_roll1d_##M(int kind, int order, int nt, int fr);                                          \
+  RollFn get_roll1d_##M(int kind, int order, int nt, int fr, int std_kc);                              \
   VjpFn get_vjp1d_##M(int kind, int order);                                                            \
   hipError_t launch_fft1d_##M(bool inverse, const float* in, float* out, int count, const RealTables& rt, \
                               hipStream_t st);
@@ -108,17 +108,17 @@ static VjpFn get_vjp1d(int M, int kind, int order) {
   }
 }
 
-static RollFn get_roll1d(int M, int kind, int order, int nt, int fr) {
+static RollFn get_roll1d(int M, int kind, int order, int nt, int fr, int std_kc) {
   switch (M) {
-    case 8: return get_roll1d_8(kind, order, nt, fr);
-    case 16: return get_roll1d_16(kind, order, nt, fr);
-    case 32: return get_roll1d_32(kind, order, nt, fr);
-    case 64: return get_roll1d_64(kind, order, nt, fr);
-    case 128: return get_roll1d_128(kind, order, nt, fr);
-    case 256: return get_roll1d_256(kind, order, nt, fr);
-    case 512: return get_roll1d_512(kind, order, nt, fr);
-    case 1024: return get_roll1d_1024(kind, order, nt, fr);
-    case 2048: return get_roll1d_2048(kind, order, nt, fr);
+    case 8: return get_roll1d_8(kind, order, nt, fr, std_kc);
+    case 16: return get_roll1d_16(kind, order, nt, fr, std_kc);
+    case 32: return get_roll1d_32(kind, order, nt, fr, std_kc);
+    case 64: return get_roll1d_64(kind, order, nt, fr, std_kc);
+    case 128: return get_roll1d_128(kind, order, nt, fr, std_kc);
+    case 256: return get_roll1d_256(kind, order, nt, fr, std_kc);
+    case 512: return get_roll1d_512(kind, order, nt, fr, std_kc);
+    case 1024: return get_roll1d_1024(kind, order, nt, fr, std_kc);
+    case 2048: return get_roll1d_2048(kind, order, nt, fr, std_kc);
     default: return nullptr;
   }
 }
@@ -572,8 +572,9 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
     return fail(EXMI_ERR_UNSUPPORTED, "Fourier-resident rollouts are built for ETDRK2");
   }
   p->fourier_resident = fr;
-  p->roll[1] = get_roll1d(p->M, kind, d->order, 1, fr);
-  p->roll[2] = get_roll1d(p->M, kind, d->order, 2, fr);
+  const int std_kc = p->kmax == default_kmax(p->M) ? 1 : 0;
+  p->roll[1] = get_roll1d(p->M, kind, d->order, 1, fr, std_kc);
+  p->roll[2] = get_roll1d(p->M, kind, d->order, 2, fr, std_kc);
   if (const char* e = getenv("EXMI_ROLL_NT")) p->roll_nt = atoi(e);  // 1, 2; -1 = per-step launches
   if (p->fourier_resident && p->roll_nt < 0) p->roll_nt = 0;
 

@@ -444,6 +444,12 @@ struct Slots {
 #define EXMI_LAYOUT_1D PAIRED
 #endif
 
+// Dealias cutoff of the default dealiasing_fraction 2/3 for N = 2M points:
+// floor(2/3 * M - 1) (nonlin_fun/_base.py:59-71; never an integer boundary
+// for power-of-two M). Kernels instantiated with KC >= 0 take the cutoff as a
+// compile-time constant, so the per-slot mask classes resolve statically.
+constexpr int default_kmax(int M) { return (2 * M - 3) / 3; }
+
 // One step (or one nonlinear evaluation) of every transform of the batch.
 // Tables are staged through lane-private LDS to keep register pressure low.
 template <int M, int PT, int KIND, int ORDER, int LAY = EXMI_LAYOUT_1D>
@@ -585,9 +591,10 @@ __device__ __forceinline__ void store_states(const cf (*S)[PT], float scale, con
 // the inverse transform of a copy. It differs from the physical path only in
 // rounding (no float32 round trip of the state through physical space), and
 // it saves one forward transform per step.
-template <int M, int PT, int KIND, int ORDER, int NT, bool FR = false, int LAY = EXMI_LAYOUT_1D>
+template <int M, int PT, int KIND, int ORDER, int NT, bool FR = false, int KC = -1, int LAY = EXMI_LAYOUT_1D>
 __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(Roll1DArgs ra) {
-  const Step1DArgs& a = ra.s;
+  Step1DArgs a = ra.s;
+  if constexpr (KC >= 0) a.kmax = KC;  // the caller launches KC only when it equals the plan's cutoff
   constexpr int T = M / PT;
   using MD = Modes<M, PT, LAY>;
   constexpr int G = Slots<M, PT, NT>::G;
@@ -667,12 +674,13 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
   }
 }
 
-template <int M, int PT, int KIND, int ORDER, int NT, bool FR = false, int LAY = EXMI_LAYOUT_1D>
+template <int M, int PT, int KIND, int ORDER, int NT, bool FR = false, int KC = -1, int LAY = EXMI_LAYOUT_1D>
 inline hipError_t launch_roll1d(const Roll1DArgs& ra, hipStream_t st) {
   constexpr int WG = Slots<M, PT, NT>::WG;
   constexpr int PB = Slots<M, PT, NT>::PER_BLOCK;
   const int grid = (ra.s.ntrans + PB - 1) / PB;
-  hipLaunchKernelGGL((roll1d_kernel<M, PT, KIND, ORDER, NT, FR, LAY>), dim3(grid), dim3(WG), 0, st, ra);
+  if (KC >= 0 && ra.s.kmax != KC) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((roll1d_kernel<M, PT, KIND, ORDER, NT, FR, KC, LAY>), dim3(grid), dim3(WG), 0, st, ra);
   return hipGetLastError();
 }
 

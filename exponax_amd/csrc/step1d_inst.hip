@@ -131,10 +131,32 @@ static RollFn pick_roll_fr(int kind) {
   }
 }
 
+// ETDRK2 rollouts of the common nonlinear kinds with the default 2/3
+// dealias cutoff as a compile-time constant.
+template <int M, int PT, int NT, bool FR>
+static RollFn pick_roll_kc(int kind) {
+  constexpr int KC = default_kmax(M);
+  switch (kind) {
+    case NL_CONV_CONS: return &launch_roll1d<M, PT, NL_CONV_CONS, 2, NT, FR, KC>;
+    case NL_CONV_NONCONS: return &launch_roll1d<M, PT, NL_CONV_NONCONS, 2, NT, FR, KC>;
+    case NL_GRADNORM: return &launch_roll1d<M, PT, NL_GRADNORM, 2, NT, FR, KC>;
+    default: return nullptr;
+  }
+}
+
 // Multi-step kernels carrying 1 or 2 transforms per lane group; `fr`
-// selects the Fourier-resident variant (ETDRK2 only).
-RollFn EXMI_CAT(get_roll1d_, EXMI_M)(int kind, int order, int nt, int fr) {
+// selects the Fourier-resident variant (ETDRK2 only); `std_kc` says that the
+// plan's dealias cutoff is default_kmax(M).
+RollFn EXMI_CAT(get_roll1d_, EXMI_M)(int kind, int order, int nt, int fr, int std_kc) {
   constexpr int M = EXMI_M, PT = EXMI_PT;
+  if (std_kc && order == 2) {
+    RollFn f = nullptr;
+    if (nt == 1) f = fr ? pick_roll_kc<M, PT, 1, true>(kind) : pick_roll_kc<M, PT, 1, false>(kind);
+    if constexpr (M / PT <= 64) {
+      if (nt == 2) f = fr ? pick_roll_kc<M, PT, 2, true>(kind) : pick_roll_kc<M, PT, 2, false>(kind);
+    }
+    if (f) return f;
+  }
   if (fr) {
     if (order != 2) return nullptr;
     if (nt == 1) return pick_roll_fr<M, PT, 1>(kind);
