@@ -470,7 +470,7 @@ struct Modes {
 };
 
 // r2c into the paired layout: z (packed samples, lane-strided) -> 2 X, and
-// xm = 2 X[M/2] (lane 0; zero elsewhere). Modes above `kcut` are not formed
+// xm = 2 X[M/2] (lane 0; an unused value elsewhere). Modes above `kcut` are not formed
 // where a whole slot lies above it.
 //   2 X[k]   = s + P,   2 X[M-k] = conj(s - P) = conj(2 s - (s + P)),
 //   s = Z[k] + Z*[M-k], P = fw[k] (Z[k] - Z*[M-k]),
@@ -493,11 +493,13 @@ __device__ __forceinline__ void rfft_paired(cf (*z)[PT], cf* xm, const Lane<M, P
     for (int h = 0; h < H; ++h) zp[n][h] = lds[ln.mirror(h)];
     xsync<T>();
   }
+  // Lane-0 special cases are selects, not branches: a divergent branch would
+  // split the interleaved instruction streams of the NT transforms.
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    const bool l0 = ln.j == 0;
-    if (l0) zp[n][0] = z[n][0];  // element 0 is not exchanged
-    xm[n] = (l0 && M / 2 <= kcut) ? cf{2.f * z[n][H].re, -2.f * z[n][H].im} : cf{0.f, 0.f};
+    if (ln.j == 0) zp[n][0] = z[n][0];  // element 0 is its own partner and is not exchanged
+    // meaningful in lane 0 (Z[M/2]); other lanes carry a finite unused value
+    xm[n] = M / 2 <= kcut ? cf{2.f * z[n][H].re, -2.f * z[n][H].im} : cf{0.f, 0.f};
 #pragma unroll
     for (int h = 0; h < H; ++h) {
       const bool lo_drop = MD::klo(h) > kcut, hi_drop = MD::klo(H + h) > kcut;

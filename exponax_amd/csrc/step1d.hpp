@@ -425,6 +425,115 @@ __device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co,
 #undef EXMI_EACH
 }
 
+// ------------------------------------------- folded ETDRK2 (constant cutoff)
+// For the convection and gradient-norm kinds the nonlinear term is a fixed
+// per-mode operator applied to the forward transform X of a product:
+// 2 N_hat = f(k) m(k) op(X) with op(X) = i X, f = c k (conservative
+// convection) or op(X) = X, f = c (non-conservative convection in its exact
+// form; gradient norm, f(0) = 0 with the zero-mode fix), m the dealias mask.
+// The rollout kernels with a compile-time cutoff fold f m into the ETDRK2
+// coefficients when they stage the tables, so a nonlinear evaluation ends at
+// the forward transform. With sc = 1/(2N) (physical rollouts; sc = 1 for the
+// Fourier-resident one) they also fold the output scale in:
+//   S1 = (sc E) S + (sc c1 f m) op(X0)                 = sc (E S + c1 N0)
+//   S' = S1 + (sc c2 f m) op(K X1 - X0),  K = 1/sc^2   = sc S_next
+// X1, the transform of the product of the scaled stage state, carries sc^2,
+// and K X1 - X0 is one fused multiply-add with an exact power-of-two K. The
+// inverse transform of S' is then the new physical state itself. Only the
+// rounding of the per-mode table products changes (etdrk/_etdrk_2.py:91-102).
+// The tables stay three floats per mode, as the unfolded ones: the launch
+// fills every CU's LDS at the batch sizes of interest.
+template <int KIND>
+constexpr bool folded_kind() {
+  return KIND == NL_CONV_CONS || KIND == NL_CONV_NONCONS || KIND == NL_GRADNORM;
+}
+
+// X: forward transform of the nonlinear product of 2 u_hat (paired layout),
+// modes above KC left unmasked (their folded coefficients are zero).
+template <int KIND, int M, int PT, int NT, int KC>
+__device__ __forceinline__ void nl_product(const cf (*S)[PT], const cf* nyq, cf (*X)[PT], cf* xn,
+                                           const Lane<M, PT>& ln, cf* lds, const TwRegs<M, PT>& rt, float kscale) {
+  const int j = ln.j;
+  if constexpr (KIND == NL_CONV_CONS) {  // u^2
+#pragma unroll
+    for (int n = 0; n < NT; ++n) masked_copy<M, PT, PAIRED>(S[n], nyq[n], X[n], xn[n], j, KC);
+    irfft_paired<M, PT, NT>(X, xn, ln, lds, rt, KC);
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+      for (int s = 0; s < PT; ++s) X[n][s] = cf{X[n][s].re * X[n][s].re, X[n][s].im * X[n][s].im};
+  } else if constexpr (KIND == NL_CONV_NONCONS) {  // u u_x
+    cf V[NT][PT];
+    cf vn[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      masked_copy<M, PT, PAIRED>(S[n], nyq[n], X[n], xn[n], j, KC);
+      masked_deriv<M, PT, PAIRED>(S[n], nyq[n], V[n], vn[n], j, KC, kscale);
+    }
+    irfft_paired<M, PT, NT>(X, xn, ln, lds, rt, KC);
+    irfft_paired<M, PT, NT>(V, vn, ln, lds, rt, KC);
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+      for (int s = 0; s < PT; ++s) X[n][s] = cf{X[n][s].re * V[n][s].re, X[n][s].im * V[n][s].im};
+  } else {  // (u_x)^2
+#pragma unroll
+    for (int n = 0; n < NT; ++n) masked_deriv<M, PT, PAIRED>(S[n], nyq[n], X[n], xn[n], j, KC, kscale);
+    irfft_paired<M, PT, NT>(X, xn, ln, lds, rt, KC);
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+      for (int s = 0; s < PT; ++s) X[n][s] = cf{X[n][s].re * X[n][s].re, X[n][s].im * X[n][s].im};
+  }
+  rfft_paired<M, PT, NT>(X, xn, ln, lds, rt, KC);
+}
+
+// Folded coefficients of one mode k: co = [sc E, sc c1 f m, sc c2 f m].
+template <int KIND, int KC>
+__device__ __forceinline__ void fold_coefs(int k, float E, float c1, float c2, float cop, float sc, int zero_mode_fix,
+                                           float* co) {
+  float f = KIND == NL_CONV_CONS ? cop * (float)k : cop;
+  if (KIND == NL_GRADNORM && k == 0 && zero_mode_fix) f = 0.f;
+  if (k > KC) f = 0.f;
+  co[0] = sc * E;
+  co[1] = (sc * c1) * f;
+  co[2] = (sc * c2) * f;
+}
+
+// K = 1/sc^2 (1 for the Fourier-resident rollout)
+template <int KIND, int M, int PT, int NT, int KC, class Coef>
+__device__ __forceinline__ void etdrk2_folded(float K, cf (*S)[PT], cf* nyq, const Coef& co, const Lane<M, PT>& ln, cf* lds,
+                                              const TwRegs<M, PT>& rt, float kscale) {
+  using MD = Modes<M, PT, PAIRED>;
+  static_assert(MD::kextra <= KC, "the mode M/2 is kept by every 2/3 cutoff");
+  // acc + g op(x) in two fused multiply-adds
+  auto fop = [](cf acc, float g, cf x) -> cf {
+    return KIND == NL_CONV_CONS ? cf{fmaf(-g, x.im, acc.re), fmaf(g, x.re, acc.im)}
+                                : cf{fmaf(g, x.re, acc.re), fmaf(g, x.im, acc.im)};
+  };
+  cf X0[NT][PT], X1[NT][PT];
+  cf x0n[NT], x1n[NT];
+  nl_product<KIND, M, PT, NT, KC>(S, nyq, X0, x0n, ln, lds, rt, kscale);
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+#pragma unroll
+    for (int s = 0; s < PT; ++s) {
+      const cf es = co(n, 0, s) * S[n][s];
+      S[n][s] = MD::klo(s) > KC ? es : fop(es, co(n, 1, s), X0[n][s]);
+    }
+    nyq[n] = fop(co(n, 0, PT) * nyq[n], co(n, 1, PT), x0n[n]);
+  }
+  nl_product<KIND, M, PT, NT, KC>(S, nyq, X1, x1n, ln, lds, rt, kscale);
+  auto kdiff = [K](cf x1, cf x0) { return cf{fmaf(K, x1.re, -x0.re), fmaf(K, x1.im, -x0.im)}; };
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+#pragma unroll
+    for (int s = 0; s < PT; ++s)
+      if (MD::klo(s) <= KC) S[n][s] = fop(S[n][s], co(n, 2, s), kdiff(X1[n][s], X0[n][s]));
+    nyq[n] = fop(nyq[n], co(n, 2, PT), kdiff(x1n[n], x0n[n]));
+  }
+}
+
 // ------------------------------------------------------------- indexing
 // Transform n of a workgroup: G groups of T lanes, each group NT transforms.
 template <int M, int PT, int NT>
@@ -601,6 +710,7 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
   constexpr int LDSN = LdsSize<M>::value;
   constexpr int NR = M + 1;
   constexpr int NCO = ncoef_of(ORDER);
+  constexpr bool FOLD = KC >= 0 && ORDER == 2 && LAY == PAIRED && folded_kind<KIND>();
   using CL = CoefLds<M, PT, NCO>;
   constexpr float NN = 2.f * M;
   constexpr float r2n = 1.f / (2.f * NN);
@@ -635,18 +745,34 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
     }
     const float* tb = a.tables + ((long long)(a.lin_batch > 1 ? tb_[n] : 0) * a.lin_channels +
                                   (a.lin_channels > 1 ? cb_[n] : 0)) * (long long)NCO * NR;
+    if constexpr (FOLD) {
+      constexpr float q2 = 2.f / (8.f * NN * NN);
+      const float cop = KIND == NL_CONV_CONS ? -a.scale * 0.5f * a.kscale * q2
+                                             : (KIND == NL_GRADNORM ? -a.scale * 0.5f * q2 : -a.scale * q2);
+      const float sc = FR ? 1.f : r2n;
 #pragma unroll
-    for (int i = 0; i < NCO; ++i) {
+      for (int s = 0; s <= PT; ++s) {
+        const int k = s < PT ? MD::k(j, s) : MD::kextra;
+        float co[3];
+        fold_coefs<KIND, KC>(k, tb[k], tb[NR + k], tb[2 * NR + k], cop, sc, a.zero_mode_fix, co);
 #pragma unroll
-      for (int s = 0; s < PT; ++s) cl.put(n, i, s, tb[i * NR + MD::k(j, s)]);
-      cl.put(n, i, PT, tb[i * NR + MD::kextra]);
+        for (int i = 0; i < 3; ++i) cl.put(n, i, s, co[i]);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NCO; ++i) {
+#pragma unroll
+        for (int s = 0; s < PT; ++s) cl.put(n, i, s, tb[i * NR + MD::k(j, s)]);
+        cl.put(n, i, PT, tb[i * NR + MD::kextra]);
+      }
     }
   }
 
   if constexpr (FR) rfft_l<LAY, M, PT, NT>(S, nyq, ln, lds, rt);
   for (int step = 0; step < ra.nsteps; ++step) {
     if constexpr (!FR) rfft_l<LAY, M, PT, NT>(S, nyq, ln, lds, rt);
-    etdrk_step<KIND, ORDER, M, PT, NT, CL, LAY>(S, nyq, cl, ln, lds, rt, a);
+    if constexpr (FOLD) etdrk2_folded<KIND, M, PT, NT, KC>(FR ? 1.f : 1.f / (r2n * r2n), S, nyq, cl, ln, lds, rt, a.kscale);
+    else etdrk_step<KIND, ORDER, M, PT, NT, CL, LAY>(S, nyq, cl, ln, lds, rt, a);
     const bool emit = !ra.final_only || step == ra.nsteps - 1;
     if constexpr (FR) {
       if (emit) {
@@ -664,11 +790,14 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
     } else {
       irfft_l<LAY, M, PT, NT>(S, nyq, ln, lds, rt);
       // back to the float32 physical state, exactly as a stored-and-reloaded
-      // state of the single-step path
+      // state of the single-step path (the folded step's state carries the
+      // scale already)
+      if constexpr (!FOLD) {
 #pragma unroll
-      for (int n = 0; n < NT; ++n)
+        for (int n = 0; n < NT; ++n)
 #pragma unroll
-        for (int s = 0; s < PT; ++s) S[n][s] = cf{S[n][s].re * r2n, S[n][s].im * r2n};
+          for (int s = 0; s < PT; ++s) S[n][s] = cf{S[n][s].re * r2n, S[n][s].im * r2n};
+      }
       if (emit) store_states<M, PT, NT>(S, 1.f, ra, tb_, cb_, act, step, j);
     }
   }
