@@ -19,6 +19,8 @@
 // _etdrk_2.py:91-102, _etdrk_3.py:191-212, _etdrk_4.py:198-224) and the
 // nonlinear functions of nonlin_fun/ (see nl_eval below).
 #pragma once
+#include <type_traits>
+
 #include "exmi_internal.hpp"
 #include "fft1d.hpp"
 
@@ -500,6 +502,37 @@ __device__ __forceinline__ void fold_coefs(int k, float E, float c1, float c2, f
   co[2] = (sc * c2) * f;
 }
 
+// Compact staging of the folded tables: sc E for every slot, the two
+// nonlinear coefficients only for slots not wholly above the cutoff (they
+// are zero there), lane-private in LDS; the mode M/2 (the extra register,
+// meaningful in lane 0) keeps its three in registers. co(n, i, s) as
+// CoefLds.
+template <int M, int PT, int NT, int KC>
+struct FoldTables {
+  using MD = Modes<M, PT, PAIRED>;
+  static constexpr int T = M / PT;
+  __host__ __device__ static constexpr int kept_before(int s) {
+    return s == 0 ? 0 : kept_before(s - 1) + (MD::klo(s - 1) <= KC ? 1 : 0);
+  }
+  static constexpr int NK = kept_before(PT);
+  static constexpr int PER = PT + 2 * NK;  // floats per lane and transform
+  static constexpr int FLOATS = PER * T;   // per transform
+  float* base;
+  int j;
+  float ex[NT][3];
+  __device__ __forceinline__ int at(int n, int i, int s) const {
+    const int row = i == 0 ? s : PT + (i - 1) * NK + kept_before(s);
+    return (n * PER + row) * T + j;
+  }
+  __device__ __forceinline__ float operator()(int n, int i, int s) const {
+    return s == PT ? ex[n][i] : base[at(n, i, s)];
+  }
+  __device__ __forceinline__ void put(int n, int i, int s, float x) {
+    if (s == PT) ex[n][i] = x;
+    else if (i == 0 || MD::klo(s) <= KC) base[at(n, i, s)] = x;
+  }
+};
+
 // K = 1/sc^2 (1 for the Fourier-resident rollout)
 template <int KIND, int M, int PT, int NT, int KC, class Coef>
 __device__ __forceinline__ void etdrk2_folded(float K, cf (*S)[PT], cf* nyq, const Coef& co, const Lane<M, PT>& ln, cf* lds,
@@ -711,7 +744,7 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
   constexpr int NR = M + 1;
   constexpr int NCO = ncoef_of(ORDER);
   constexpr bool FOLD = KC >= 0 && ORDER == 2 && LAY == PAIRED && folded_kind<KIND>();
-  using CL = CoefLds<M, PT, NCO>;
+  using CL = std::conditional_t<FOLD, FoldTables<M, PT, NT, (KC < 0 ? 0 : KC)>, CoefLds<M, PT, NCO>>;
   constexpr float NN = 2.f * M;
   constexpr float r2n = 1.f / (2.f * NN);
   __shared__ cf lds_all[G * LDSN];
@@ -720,7 +753,7 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
   const int g = threadIdx.x / T;
   const int j = threadIdx.x % T;
   cf* lds = lds_all + g * LDSN;
-  const CL cl{lds_coef + g * NT * CL::FLOATS, j};
+  CL cl{lds_coef + g * NT * CL::FLOATS, j};
   const Lane<M, PT> ln(j);
   TwRegs<M, PT> rt;
   rt.load(j, RealTables{a.tw, a.fw});
