@@ -36,6 +36,13 @@ def _pair(kind, N, order, L=None, dt=None):
         g, c, L = _pair("burgers", N, order, L, dt)
         g.exact_form = True
         return g, c, L
+    if kind == "burgers_df05":
+        # a non-default dealias cutoff: the rollout kernels take it at run time
+        L = L or 1.0
+        dt = dt or 1e-3
+        return (ex.stepper.Burgers(1, L, N, dt, diffusivity=3e-3, order=order, dealiasing_fraction=0.5),
+                ost.Burgers(1, L, N, dt, diffusivity=3e-3, order=order, dealiasing_fraction=0.5,
+                            dtype=np.float32), L)
     if kind == "burgers_cons":
         L = L or 1.0
         dt = dt or 1e-3
@@ -111,12 +118,15 @@ def test_shape_error(device):
 
 
 @pytest.mark.parametrize("N", [64, 256, 1024, 2048])
-@pytest.mark.parametrize("kind", ["ks_cons", "burgers"])
+@pytest.mark.parametrize("kind", ["ks_cons", "ks", "burgers", "burgers_exact", "burgers_df05"])
 @pytest.mark.parametrize("order", [2, 4])
 def test_rollout_strategies_agree(device, kind, N, order):
     """The multi-step rollout kernels (1 or 2 transforms per lane group) and
     the per-step launches compute the same steps; all match the oracle
-    rollout (reference _utils.py:92-186) at steps 1 and 10."""
+    rollout (reference _utils.py:92-186) at steps 1 and 10. ETDRK2 with the
+    default 2/3 cutoff runs the folded constant-cutoff kernels for the three
+    kinds that have them (conservative convection, the exact u u_x form,
+    gradient norm); the 0.5 cutoff runs the general ones."""
     import torch
 
     print(f"[case] rollout kind={kind} N={N} order={order}", flush=True)
