@@ -9,6 +9,7 @@ from __future__ import annotations
 import numpy as np
 
 from ._base_stepper import BaseStepper, Ensemble
+from ._repeated_stepper import RepeatedStepper
 
 
 class vmap:
@@ -19,7 +20,7 @@ class vmap:
         self.stepper = stepper
 
     def __call__(self, u):
-        if isinstance(self.stepper, BaseStepper):
+        if isinstance(self.stepper, (BaseStepper, RepeatedStepper)):
             return self.stepper.step(u)
         return _stack([self.stepper(x) for x in u])
 
@@ -33,11 +34,11 @@ def _stack(xs):
 
 
 def _native(stepper_fn):
-    if isinstance(stepper_fn, vmap) and isinstance(stepper_fn.stepper, BaseStepper):
+    if isinstance(stepper_fn, vmap) and isinstance(stepper_fn.stepper, (BaseStepper, RepeatedStepper)):
         return stepper_fn.stepper, True
     if isinstance(stepper_fn, Ensemble):
         return stepper_fn, True
-    if isinstance(stepper_fn, BaseStepper):
+    if isinstance(stepper_fn, (BaseStepper, RepeatedStepper)):
         return stepper_fn, False
     return None, False
 

@@ -111,3 +111,18 @@ def test_package_ic_recipe_matches_oracle():
         a = pic.truncated_fourier_series(3, d, 7.0, n, c, np.random.default_rng(3))
         b, _ = oic.tfs_batch(3, d, 7.0, n, c, 3)
         np.testing.assert_allclose(a, b, rtol=0, atol=1e-13)
+
+
+def test_repeated_stepper_attributes():
+    """RepeatedStepper's constructor (reference _repeated_stepper.py:20-54)
+    needs no device: dt = sub-step dt * num_sub_steps, geometry copied."""
+    import exponax_amd as ex
+
+    sub = ex.stepper.Burgers(1, 3.0, 32, 0.05, diffusivity=0.1)
+    rs = ex.RepeatedStepper(sub, 8)
+    assert rs.dt == pytest.approx(0.4, abs=1e-10)
+    assert (rs.num_spatial_dims, rs.domain_extent, rs.num_points, rs.num_channels, rs.dx) == (
+        1, 3.0, 32, 1, sub.dx)
+    assert rs.state_shape == (1, 32)
+    with pytest.raises(ValueError):
+        ex.RepeatedStepper(sub, -1)

@@ -233,3 +233,38 @@ def test_autotuner_cache_roundtrip(device, tmp_path, monkeypatch):
     assert s2.plan(3).rollout_strategy() == 1
     assert len([ln for ln in path.read_text().split("\n") if ln.strip()]) == 1  # nothing re-tuned
     assert max_rel_l2(out[:, -1].cpu().numpy(), s1.rollout_batch(x, 20)[:, -1].cpu().numpy()) <= 1e-5
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_repeated_stepper(device, order):
+    """RepeatedStepper (reference _repeated_stepper.py:9-143): dt is the
+    sub-step dt times num_sub_steps; a call equals the oracle's transform-once
+    sub-stepping (fft, num_sub_steps x step_fourier, ifft) and the repeat of
+    the plain stepper; rollouts of the wrapper return every call's state."""
+    import torch
+
+    from oracle import spectral as osp
+
+    k = 6
+    gpu, cpu, L = _pair("burgers", 256, order)
+    rs = ex.RepeatedStepper(gpu, k)
+    assert rs.dt == pytest.approx(gpu.dt * k, abs=1e-12)
+    u0, _ = ics(3, 1, L, 256, 5, seed=31)
+    x = torch.tensor(u0, device=device)
+    out = rs.step(x).cpu().numpy()
+    uh = osp.fft(u0, num_spatial_dims=1).astype(np.complex64)
+    for _ in range(k):
+        uh = cpu.step_fourier(uh)
+    ref = osp.ifft(uh, num_spatial_dims=1, num_points=256).astype(np.float32)
+    assert max_rel_l2(out, ref) <= RTOL_STEP
+    assert max_rel_l2(out, gpu.repeat_batch(x, k).cpu().numpy()) <= 1e-6
+    with pytest.raises(ValueError):
+        rs(x)  # a batch through __call__
+    single = rs(x[0]).cpu().numpy()
+    assert max_rel_l2(single[None], out[:1]) <= 1e-6
+    traj = ex.rollout(ex.vmap(rs), 3, include_init=True)(x).cpu().numpy()
+    assert traj.shape == (3, 4, 1, 256)
+    np.testing.assert_array_equal(traj[:, 0], u0)
+    np.testing.assert_array_equal(traj[:, 1], out)
+    last = ex.repeat(ex.vmap(rs), 3)(x).cpu().numpy()
+    np.testing.assert_array_equal(last, traj[:, 3])
