@@ -126,3 +126,27 @@ def test_repeated_stepper_attributes():
     assert rs.state_shape == (1, 32)
     with pytest.raises(ValueError):
         ex.RepeatedStepper(sub, -1)
+
+
+def test_trajectory_file_format(tmp_path):
+    """open_trajectories: read-only memory map in the rollout layout
+    (B, T, C, N); a sidecar whose shape disagrees is rejected."""
+    import json
+
+    import numpy as np
+
+    from exponax_amd import io as exio
+
+    p = str(tmp_path / "traj.npy")
+    data = np.arange(2 * 3 * 1 * 8, dtype=np.float32).reshape(2, 3, 1, 8)
+    np.save(p, data)
+    arr, side = exio.open_trajectories(p)
+    assert side is None and arr.shape == (2, 3, 1, 8) and not arr.flags.writeable
+    np.testing.assert_array_equal(arr, data)
+    with open(p + ".json", "w") as f:
+        json.dump({"shape": [2, 3, 1, 8]}, f)
+    assert exio.open_trajectories(p)[1]["shape"] == [2, 3, 1, 8]
+    with open(p + ".json", "w") as f:
+        json.dump({"shape": [2, 4, 1, 8]}, f)
+    with pytest.raises(ValueError):
+        exio.open_trajectories(p)

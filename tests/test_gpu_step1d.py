@@ -268,3 +268,26 @@ def test_repeated_stepper(device, order):
     np.testing.assert_array_equal(traj[:, 1], out)
     last = ex.repeat(ex.vmap(rs), 3)(x).cpu().numpy()
     np.testing.assert_array_equal(last, traj[:, 3])
+
+
+def test_rollout_to_file(device, tmp_path):
+    """Chunked device rollouts streamed to a .npy dataset equal one
+    continuous rollout bit for bit; the sidecar records the stepper."""
+    import torch
+
+    gpu, _, L = _pair("ks_cons", 256, 2)
+    u0, _ = ics(4, 1, L, 256, 5, seed=33)
+    x = torch.tensor(u0, device=device)
+    path = str(tmp_path / "ks.npy")
+    ex.io.rollout_to_file(gpu, x, 25, path, chunk=7, meta={"ic": "TFS cutoff 5 seed 33"})
+    arr, side = ex.io.open_trajectories(path)
+    full = gpu.rollout_batch(x, 25, include_init=True).cpu().numpy()
+    np.testing.assert_array_equal(np.asarray(arr), full)
+    assert side["stepper"] == "KuramotoSivashinskyConservative" and side["steps"] == 25
+    assert side["shape"] == [4, 26, 1, 256] and side["user"]["ic"].startswith("TFS")
+    rs = ex.RepeatedStepper(gpu, 3)
+    p2 = str(tmp_path / "ks_rep.npy")
+    ex.io.rollout_to_file(rs, x, 4, p2, chunk=3)
+    arr2, side2 = ex.io.open_trajectories(p2)
+    np.testing.assert_array_equal(np.asarray(arr2), rs.rollout_batch(x, 4, include_init=True).cpu().numpy())
+    assert side2["num_sub_steps"] == 3 and side2["sub_stepper"] == "KuramotoSivashinskyConservative"
