@@ -191,7 +191,7 @@ class VorticityConvection2d(BaseNonlinearFun):
         d64 = np.asarray(derivative_operator)
         self.derivative_operator = self._op(d64)
         lap = sp.build_laplace_operator(d64, order=2)
-        with np.errstate(divide="ignore"):
+        with np.errstate(divide="ignore", invalid="ignore"):  # the mean mode (lap = 0) is replaced by 1
             inv = np.where(lap == 0, 1.0, 1.0 / lap)
         self.inv_laplacian = self._op(inv)
 
