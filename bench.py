@@ -323,7 +323,8 @@ def wl_kolmogorov2d(args, ex, _lib, torch):
                        "domain_extent": "2 pi", "dt": 0.01, "order": 2, "reynolds": "U[100, 1000]",
                        "rollout_steps": K, "ic": "TFS cutoff 6 seed 3"},
                       nst * K / wall, K, W, wall, gpu_s / K, per_step_bytes, K,
-                      "row2d/col2d kernels (6 launches per step per grid size)", None)
+                      "row2d/col2d kernels (6 launches per step per grid size)",
+                      read_traffic("kolmogorov2d_mixed", nst))
     res["nan_trajectories"] = int(sum((~torch.isfinite(t[:, -1]).reshape(t.shape[0], -1).all(1)).sum().item()
                                       for t in trs))
     res["launch_mode"] = modes[best]
