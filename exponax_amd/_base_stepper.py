@@ -50,11 +50,12 @@ class BaseStepper:
             # _base_stepper.py:111-116
             raise ValueError(f"Expected linear operator to have shape {single} or {multi}, got {lin.shape}.")
         if np.iscomplexobj(lin):
+            # odd derivatives (advection, dispersion) make L complex; an
+            # imaginary part that is zero to rounding is dropped
             scale = max(1.0, float(np.max(np.abs(lin))))
-            if np.max(np.abs(lin.imag)) > 1e-12 * scale:
-                raise NotImplementedError("complex-valued linear operators (odd derivatives) are not built")
-            lin = lin.real
-        self.linear_operator = np.ascontiguousarray(lin, dtype=np.float64)
+            if np.max(np.abs(lin.imag)) <= 1e-12 * scale:
+                lin = lin.real
+        self.linear_operator = np.ascontiguousarray(lin, dtype=np.complex128 if np.iscomplexobj(lin) else np.float64)
         if order not in (0, 1, 2, 3, 4):
             raise NotImplementedError(f"Order {order} not implemented.")  # _base_stepper.py:156-157
         self.nonlinear_fun = self._build_nonlinear_fun(derivative_operator)

@@ -69,6 +69,7 @@ class PlanDesc(C.Structure):
         ("poly", C.c_double * EXMI_MAX_POLY),
         ("injection_scale", C.c_double),
         ("host_linear_operator", C.POINTER(C.c_double)),
+        ("host_linear_operator_imag", C.POINTER(C.c_double)),
     ]
 
 
@@ -169,12 +170,18 @@ def stream_ptr(stream=None) -> int:
 
 
 class Plan:
-    """Owns one ``exmi_plan``; created from a PlanDesc plus a float64 L table."""
+    """Owns one ``exmi_plan``; created from a PlanDesc plus the L table
+    (float64, or complex128 for odd-derivative operators)."""
 
     def __init__(self, desc: PlanDesc, linear_operator: np.ndarray):
         L = lib()
-        self._L = np.ascontiguousarray(linear_operator, dtype=np.float64)
+        lin = np.asarray(linear_operator)
+        self._L = np.ascontiguousarray(lin.real, dtype=np.float64)
         desc.host_linear_operator = self._L.ctypes.data_as(C.POINTER(C.c_double))
+        self._Li = None
+        if np.iscomplexobj(lin):
+            self._Li = np.ascontiguousarray(lin.imag, dtype=np.float64)
+            desc.host_linear_operator_imag = self._Li.ctypes.data_as(C.POINTER(C.c_double))
         self.desc = desc
         h = C.c_void_p()
         check(L.exmi_plan_create(C.byref(desc), C.byref(h)), "exmi_plan_create")

@@ -24,12 +24,15 @@ namespace exmi {
 using StepFn = hipError_t (*)(const Step1DArgs&, hipStream_t);
 using RollFn = hipError_t (*)(const Roll1DArgs&, hipStream_t);
 using VjpFn = hipError_t (*)(const Vjp1DArgs&, hipStream_t);
-hipError_t launch_coef(const double* L, long long total, int nr, double dt, int order, int ncirc, double radius,
+hipError_t launch_coef(const double* L, const double* Li, long long total, int nr, double dt, int order, int ncirc,
+                       double radius,
                        float* out, hipStream_t st);
 
 #define EXMI_DECL_SIZE(M)                                                                              \
   StepFn get_step1d_##M(int kind, int order);                                                          \
   RollFn get_roll1d_##M(int kind, int order, int nt, int fr, int std_kc);                              \
+  StepFn get_step1d_c_##M(int kind, int order);                                                        \
+  RollFn get_roll1d_c_##M(int kind, int order);                                                        \
   VjpFn get_vjp1d_##M(int kind, int order);                                                            \
   hipError_t launch_fft1d_##M(bool inverse, const float* in, float* out, int count, const RealTables& rt, \
                               hipStream_t st);
@@ -104,6 +107,26 @@ static VjpFn get_vjp1d(int M, int kind, int order) {
     case 512: return get_vjp1d_512(kind, order);
     case 1024: return get_vjp1d_1024(kind, order);
     case 2048: return get_vjp1d_2048(kind, order);
+    default: return nullptr;
+  }
+}
+
+// Kernels for a complex linear operator (1-D, ETDRK orders 0-2, N <= 1024).
+static StepFn get_step1d_c(int M, int kind, int order) {
+  switch (M) {
+#define EXMI_CASE(MM) \
+  case MM: return get_step1d_c_##MM(kind, order);
+    EXMI_CASE(8) EXMI_CASE(16) EXMI_CASE(32) EXMI_CASE(64) EXMI_CASE(128) EXMI_CASE(256) EXMI_CASE(512)
+#undef EXMI_CASE
+    default: return nullptr;
+  }
+}
+static RollFn get_roll1d_c(int M, int kind, int order) {
+  switch (M) {
+#define EXMI_CASE(MM) \
+  case MM: return get_roll1d_c_##MM(kind, order);
+    EXMI_CASE(8) EXMI_CASE(16) EXMI_CASE(32) EXMI_CASE(64) EXMI_CASE(128) EXMI_CASE(256) EXMI_CASE(512)
+#undef EXMI_CASE
     default: return nullptr;
   }
 }
@@ -366,6 +389,8 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     return fail(EXMI_ERR_INVALID, "lin_batch must be 1 or batch (%d), got %d", d->batch, d->lin_batch);
   if (d->lin_channels != 1) return fail(EXMI_ERR_INVALID, "lin_channels must be 1");
   if (!d->host_linear_operator) return fail(EXMI_ERR_INVALID, "host_linear_operator is null");
+  if (d->host_linear_operator_imag)
+    return fail(EXMI_ERR_UNSUPPORTED, "complex linear operators are built for 1-D plans");
   int kind = 0;
   if (d->nonlin_kind == EXMI_NL_VORTICITY || d->nonlin_kind == EXMI_NL_KOLMOGOROV) {
     kind = NL_VORT;
@@ -445,7 +470,8 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   }
   hipError_t e = hipMemcpy(dL, Lt.data(), sizeof(double) * rows * full, hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = launch_coef(dL, rows * full, (int)full, d->dt, d->order, d->num_circle_points, d->circle_radius, p->tables,
+    e = launch_coef(dL, nullptr, rows * full, (int)full, d->dt, d->order, d->num_circle_points, d->circle_radius,
+                    p->tables,
                     nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   hipFree(dL);
@@ -537,6 +563,11 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
                 d->lin_channels);
   if (!d->host_linear_operator) return fail(EXMI_ERR_INVALID, "host_linear_operator is null");
   if (d->order > 0 && d->num_circle_points < 1) return fail(EXMI_ERR_INVALID, "num_circle_points must be >= 1");
+  const bool cplx = d->host_linear_operator_imag != nullptr;
+  if (cplx && (d->order > 2 || N > 1024 || (d->flags & EXMI_FLAG_FOURIER_RESIDENT)))
+    return fail(EXMI_ERR_UNSUPPORTED,
+                "complex linear operators are built for ETDRK orders 0-2, N <= 1024, without the Fourier-resident "
+                "flag");
   int kind = NL_ZERO;
   if (d->order > 0) {
     int rc = kernel_kind(d, &kind);
@@ -552,20 +583,21 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   exmi_plan* p = new exmi_plan();
   p->d = *d;
   p->d.host_linear_operator = nullptr;
+  p->d.host_linear_operator_imag = nullptr;
   p->M = N / 2;
   p->nr = p->M + 1;
-  p->ncoef = num_coefs(d->order);
+  p->ncoef = num_coefs_c(d->order, cplx);
   p->kind = kind;
   p->kmax = kmax;
   p->kscale = (float)(2.0 * 3.14159265358979323846 / d->domain_extent);
   hipGetDevice(&p->device);
-  p->step = get_step1d(p->M, kind, d->order);
+  p->step = cplx ? get_step1d_c(p->M, kind, d->order) : get_step1d(p->M, kind, d->order);
   p->nonlin = d->order > 0 ? get_step1d(p->M, nl_kind, 1) : get_step1d(p->M, NL_ZERO, 1);
   if (!p->step || !p->nonlin) {
     delete p;
     return fail(EXMI_ERR_UNSUPPORTED, "no kernel for N=%d kind=%d order=%d", N, kind, d->order);
   }
-  p->vjp = d->order <= 2 ? get_vjp1d(p->M, d->order == 0 ? NL_ZERO : kind, d->order) : nullptr;
+  p->vjp = d->order <= 2 && !cplx ? get_vjp1d(p->M, d->order == 0 ? NL_ZERO : kind, d->order) : nullptr;
   const int fr = (d->flags & EXMI_FLAG_FOURIER_RESIDENT) ? 1 : 0;
   if (fr && d->order != 2) {
     delete p;
@@ -573,8 +605,8 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   }
   p->fourier_resident = fr;
   const int std_kc = p->kmax == default_kmax(p->M) ? 1 : 0;
-  p->roll[1] = get_roll1d(p->M, kind, d->order, 1, fr, std_kc);
-  p->roll[2] = get_roll1d(p->M, kind, d->order, 2, fr, std_kc);
+  p->roll[1] = cplx ? get_roll1d_c(p->M, kind, d->order) : get_roll1d(p->M, kind, d->order, 1, fr, std_kc);
+  p->roll[2] = cplx ? nullptr : get_roll1d(p->M, kind, d->order, 2, fr, std_kc);
   if (const char* e = getenv("EXMI_ROLL_NT")) p->roll_nt = atoi(e);  // 1, 2; -1 = per-step launches
   if (p->fourier_resident && p->roll_nt < 0) p->roll_nt = 0;
 
@@ -586,15 +618,18 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   const long long rows = (long long)d->lin_batch * d->lin_channels;
   const long long total = rows * p->nr;
   double* dL = nullptr;
-  if (hipMalloc(&dL, sizeof(double) * total) != hipSuccess ||
+  if (hipMalloc(&dL, sizeof(double) * total * (cplx ? 2 : 1)) != hipSuccess ||
       hipMalloc(&p->tables, sizeof(float) * total * p->ncoef) != hipSuccess) {
     hipFree(dL);
     exmi_plan_destroy(p);
     return fail(EXMI_ERR_ALLOC, "table allocation failed (%lld modes)", total);
   }
   hipError_t e = hipMemcpy(dL, d->host_linear_operator, sizeof(double) * total, hipMemcpyHostToDevice);
+  if (e == hipSuccess && cplx)
+    e = hipMemcpy(dL + total, d->host_linear_operator_imag, sizeof(double) * total, hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = launch_coef(dL, total, p->nr, d->dt, d->order, d->num_circle_points, d->circle_radius, p->tables, nullptr);
+    e = launch_coef(dL, cplx ? dL + total : nullptr, total, p->nr, d->dt, d->order, d->num_circle_points,
+                    d->circle_radius, p->tables, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   hipFree(dL);
   if (e != hipSuccess) {

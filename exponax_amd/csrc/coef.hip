@@ -6,7 +6,9 @@
 // multiplying the mean by dt, exactly the reductions of
 // etdrk/_etdrk_1.py:66-76, _etdrk_2.py:71-89, _etdrk_3.py:156-189 and
 // _etdrk_4.py:170-196 (roots from etdrk/_utils.py:9-23). Results are stored
-// as float32 in the layout (row, coefficient, mode).
+// as float32 in the layout (row, coefficient, mode). With a complex L (Li not
+// null) E and E_half are complex; their imaginary parts follow the real
+// coefficients (num_coefs_c), the phi-coefficients are real as above.
 #include <hip/hip_runtime.h>
 
 #include "exmi_internal.hpp"
@@ -38,17 +40,28 @@ __device__ inline zd zexp(zd z) {
   return {e * c, e * s};
 }
 
-__global__ void coef_kernel(const double* __restrict__ L, long long total, int nr, double dt, int order,
-                            int ncirc, double radius, float* __restrict__ out) {
+__global__ void coef_kernel(const double* __restrict__ L, const double* __restrict__ Li, long long total, int nr,
+                            double dt, int order, int ncirc, double radius, float* __restrict__ out) {
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= total) return;
   const long long row = idx / nr;
   const int k = (int)(idx % nr);
   const int nco = num_coefs(order);
-  float* o = out + row * (long long)nco * nr + k;
+  float* o = out + row * (long long)num_coefs_c(order, Li != nullptr) * nr + k;
   const double l = L[idx];
-  const zd ldt = {l * dt, 0.0};
-  o[0] = (float)exp(l * dt);
+  const zd ldt = {l * dt, Li ? Li[idx] * dt : 0.0};
+  if (Li) {
+    const zd E = zexp(ldt);
+    o[0] = (float)E.re;
+    o[nco * nr] = (float)E.im;
+    if (order >= 3) {
+      const zd Eh = zexp(0.5 * ldt);
+      o[nr] = (float)Eh.re;
+      o[(nco + 1) * nr] = (float)Eh.im;
+    }
+  } else {
+    o[0] = (float)exp(l * dt);
+  }
   if (order == 0) return;
   double acc[5] = {0, 0, 0, 0, 0};
   const double pi = 3.14159265358979323846;
@@ -83,19 +96,19 @@ __global__ void coef_kernel(const double* __restrict__ L, long long total, int n
     o[nr] = (float)(dt * (acc[0] / ncirc));
     o[2 * nr] = (float)(dt * (acc[1] / ncirc));
   } else if (order == 3) {
-    o[nr] = (float)exp(0.5 * dt * l);
+    if (!Li) o[nr] = (float)exp(0.5 * dt * l);
     for (int i = 0; i < 5; ++i) o[(2 + i) * nr] = (float)(dt * (acc[i] / ncirc));
   } else {
-    o[nr] = (float)exp(0.5 * dt * l);
+    if (!Li) o[nr] = (float)exp(0.5 * dt * l);
     for (int i = 0; i < 4; ++i) o[(2 + i) * nr] = (float)(dt * (acc[i] / ncirc));
   }
 }
 
-hipError_t launch_coef(const double* L, long long total, int nr, double dt, int order, int ncirc, double radius,
-                       float* out, hipStream_t st) {
+hipError_t launch_coef(const double* L, const double* Li, long long total, int nr, double dt, int order, int ncirc,
+                       double radius, float* out, hipStream_t st) {
   const int bs = 256;
   const long long grid = (total + bs - 1) / bs;
-  hipLaunchKernelGGL(coef_kernel, dim3((unsigned)grid), dim3(bs), 0, st, L, total, nr, dt, order, ncirc, radius,
+  hipLaunchKernelGGL(coef_kernel, dim3((unsigned)grid), dim3(bs), 0, st, L, Li, total, nr, dt, order, ncirc, radius,
                      out);
   return hipGetLastError();
 }

@@ -28,5 +28,10 @@ __host__ __device__ inline int num_coefs(int order) {
     default: return 0;
   }
 }
+// With a complex linear operator the imaginary parts of E (and E_half for
+// orders 3 and 4) follow the real coefficients.
+__host__ __device__ inline int num_coefs_c(int order, bool cplx) {
+  return num_coefs(order) + (cplx ? (order >= 3 ? 2 : 1) : 0);
+}
 
 }  // namespace exmi

@@ -51,6 +51,8 @@ struct Step1DArgs {
 constexpr int ncoef_of(int order) {
   return order == 0 ? 1 : (order == 1 ? 2 : (order == 2 ? 3 : (order == 3 ? 7 : 6)));
 }
+// table entries per mode, with Im E (and Im E_half) for a complex linear operator
+constexpr int ncoef_of(int order, bool cplx) { return ncoef_of(order) + (cplx ? (order >= 3 ? 2 : 1) : 0); }
 
 // ----------------------------------------------------------------- helpers
 // Dealias mask classes of slot s for a uniform kmax: every mode kept, every
@@ -334,30 +336,42 @@ struct CoefLds {
 // _etdrk_1.py:77-82, _etdrk_2.py:91-102, _etdrk_3.py:191-212,
 // _etdrk_4.py:198-224. Table order: ETDRK0 [E]; ETDRK1 [E, c1];
 // ETDRK2 [E, c1, c2]; ETDRK3 [E, E/2, c1..c5]; ETDRK4 [E, E/2, c1, c4, c5, c6].
-template <int KIND, int ORDER, int M, int PT, int NT, class Coef, int LAY = NATURAL>
+template <int KIND, int ORDER, int M, int PT, int NT, class Coef, int LAY = NATURAL, bool CPLX = false>
 __device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co, const Lane<M, PT>& ln, cf* lds,
                                            const TwRegs<M, PT>& rt, const Step1DArgs& a) {
+  // E x: real E, or complex E with Im E stored after the real coefficients
+  static_assert(!CPLX || ORDER <= 2, "complex linear operators are built for ETDRK orders 0-2");
+  constexpr int NRE = ncoef_of(ORDER);
+  auto emul = [&](int n, int s, cf x) -> cf {
+    const float er = co(n, 0, s);
+    if constexpr (CPLX) {
+      const float ei = co(n, NRE, s);
+      return cf{fmaf(er, x.re, -ei * x.im), fmaf(er, x.im, ei * x.re)};
+    } else {
+      return er * x;
+    }
+  };
 #define EXMI_EACH(body)                   \
   _Pragma("unroll") for (int n = 0; n < NT; ++n) { \
     _Pragma("unroll") for (int s = 0; s < PT; ++s) { body; } }
   if constexpr (ORDER == 0) {
-    EXMI_EACH(S[n][s] = co(n, 0, s) * S[n][s]);
+    EXMI_EACH(S[n][s] = emul(n, s, S[n][s]));
 #pragma unroll
-    for (int n = 0; n < NT; ++n) nyq[n] = co(n, 0, PT) * nyq[n];
+    for (int n = 0; n < NT; ++n) nyq[n] = emul(n, PT, nyq[n]);
   } else if constexpr (ORDER == 1) {
     cf N0[NT][PT];
     cf n0[NT];
     nl_eval<KIND, M, PT, NT, LAY>(S, nyq, N0, n0, ln, lds, rt, a);
-    EXMI_EACH(S[n][s] = co(n, 0, s) * S[n][s] + co(n, 1, s) * N0[n][s]);
+    EXMI_EACH(S[n][s] = emul(n, s, S[n][s]) + co(n, 1, s) * N0[n][s]);
 #pragma unroll
-    for (int n = 0; n < NT; ++n) nyq[n] = co(n, 0, PT) * nyq[n] + co(n, 1, PT) * n0[n];
+    for (int n = 0; n < NT; ++n) nyq[n] = emul(n, PT, nyq[n]) + co(n, 1, PT) * n0[n];
   } else if constexpr (ORDER == 2) {
     cf N0[NT][PT], N1[NT][PT];
     cf n0[NT], n1[NT];
     nl_eval<KIND, M, PT, NT, LAY>(S, nyq, N0, n0, ln, lds, rt, a);
-    EXMI_EACH(S[n][s] = co(n, 0, s) * S[n][s] + co(n, 1, s) * N0[n][s]);
+    EXMI_EACH(S[n][s] = emul(n, s, S[n][s]) + co(n, 1, s) * N0[n][s]);
 #pragma unroll
-    for (int n = 0; n < NT; ++n) nyq[n] = co(n, 0, PT) * nyq[n] + co(n, 1, PT) * n0[n];
+    for (int n = 0; n < NT; ++n) nyq[n] = emul(n, PT, nyq[n]) + co(n, 1, PT) * n0[n];
     nl_eval<KIND, M, PT, NT, LAY>(S, nyq, N1, n1, ln, lds, rt, a);
     EXMI_EACH(S[n][s] = S[n][s] + co(n, 2, s) * (N1[n][s] - N0[n][s]));
 #pragma unroll
@@ -594,7 +608,7 @@ constexpr int default_kmax(int M) { return (2 * M - 3) / 3; }
 
 // One step (or one nonlinear evaluation) of every transform of the batch.
 // Tables are staged through lane-private LDS to keep register pressure low.
-template <int M, int PT, int KIND, int ORDER, int LAY = EXMI_LAYOUT_1D>
+template <int M, int PT, int KIND, int ORDER, int LAY = EXMI_LAYOUT_1D, bool CPLX = false>
 __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
     __attribute__((amdgpu_waves_per_eu(EXMI_STEP_WAVES_PER_EU))) step1d_kernel(Step1DArgs a) {
   constexpr int T = M / PT;
@@ -602,7 +616,7 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
   constexpr int G = Slots<M, PT, 1>::G;
   constexpr int LDSN = LdsSize<M>::value;
   constexpr int NR = M + 1;
-  constexpr int NCO = ncoef_of(ORDER);
+  constexpr int NCO = ncoef_of(ORDER, CPLX);
   using CL = CoefLds<M, PT, NCO>;
   constexpr float NN = 2.f * M;
   __shared__ cf lds_all[G * LDSN];
@@ -662,7 +676,7 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
     for (int s = 0; s < PT; ++s) S[0][s] = Nn[0][s];
     nyq[0] = nn[0];
   } else {
-    etdrk_step<KIND, ORDER, M, PT, 1, CL, LAY>(S, nyq, cl, ln, lds, rt, a);
+    etdrk_step<KIND, ORDER, M, PT, 1, CL, LAY, CPLX>(S, nyq, cl, ln, lds, rt, a);
   }
 
   // ---- store (2 u_hat -> u_hat, or N * 2 u -> u)
@@ -685,12 +699,12 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
   }
 }
 
-template <int M, int PT, int KIND, int ORDER, int LAY = EXMI_LAYOUT_1D>
+template <int M, int PT, int KIND, int ORDER, int LAY = EXMI_LAYOUT_1D, bool CPLX = false>
 inline hipError_t launch_step1d(const Step1DArgs& a, hipStream_t st) {
   constexpr int WG = Slots<M, PT, 1>::WG;
   constexpr int G = Slots<M, PT, 1>::G;
   const int grid = (a.ntrans + G - 1) / G;
-  hipLaunchKernelGGL((step1d_kernel<M, PT, KIND, ORDER, LAY>), dim3(grid), dim3(WG), 0, st, a);
+  hipLaunchKernelGGL((step1d_kernel<M, PT, KIND, ORDER, LAY, CPLX>), dim3(grid), dim3(WG), 0, st, a);
   return hipGetLastError();
 }
 
@@ -733,7 +747,8 @@ __device__ __forceinline__ void store_states(const cf (*S)[PT], float scale, con
 // the inverse transform of a copy. It differs from the physical path only in
 // rounding (no float32 round trip of the state through physical space), and
 // it saves one forward transform per step.
-template <int M, int PT, int KIND, int ORDER, int NT, bool FR = false, int KC = -1, int LAY = EXMI_LAYOUT_1D>
+template <int M, int PT, int KIND, int ORDER, int NT, bool FR = false, int KC = -1, int LAY = EXMI_LAYOUT_1D,
+          bool CPLX = false>
 __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(Roll1DArgs ra) {
   Step1DArgs a = ra.s;
   if constexpr (KC >= 0) a.kmax = KC;  // the caller launches KC only when it equals the plan's cutoff
@@ -742,8 +757,8 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
   constexpr int G = Slots<M, PT, NT>::G;
   constexpr int LDSN = LdsSize<M>::value;
   constexpr int NR = M + 1;
-  constexpr int NCO = ncoef_of(ORDER);
-  constexpr bool FOLD = KC >= 0 && ORDER == 2 && LAY == PAIRED && folded_kind<KIND>();
+  constexpr int NCO = ncoef_of(ORDER, CPLX);
+  constexpr bool FOLD = KC >= 0 && ORDER == 2 && LAY == PAIRED && !CPLX && folded_kind<KIND>();
   using CL = std::conditional_t<FOLD, FoldTables<M, PT, NT, (KC < 0 ? 0 : KC)>, CoefLds<M, PT, NCO>>;
   constexpr float NN = 2.f * M;
   constexpr float r2n = 1.f / (2.f * NN);
@@ -805,7 +820,7 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
   for (int step = 0; step < ra.nsteps; ++step) {
     if constexpr (!FR) rfft_l<LAY, M, PT, NT>(S, nyq, ln, lds, rt);
     if constexpr (FOLD) etdrk2_folded<KIND, M, PT, NT, KC>(FR ? 1.f : 1.f / (r2n * r2n), S, nyq, cl, ln, lds, rt, a.kscale);
-    else etdrk_step<KIND, ORDER, M, PT, NT, CL, LAY>(S, nyq, cl, ln, lds, rt, a);
+    else etdrk_step<KIND, ORDER, M, PT, NT, CL, LAY, CPLX>(S, nyq, cl, ln, lds, rt, a);
     const bool emit = !ra.final_only || step == ra.nsteps - 1;
     if constexpr (FR) {
       if (emit) {
@@ -836,13 +851,14 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
   }
 }
 
-template <int M, int PT, int KIND, int ORDER, int NT, bool FR = false, int KC = -1, int LAY = EXMI_LAYOUT_1D>
+template <int M, int PT, int KIND, int ORDER, int NT, bool FR = false, int KC = -1, int LAY = EXMI_LAYOUT_1D,
+          bool CPLX = false>
 inline hipError_t launch_roll1d(const Roll1DArgs& ra, hipStream_t st) {
   constexpr int WG = Slots<M, PT, NT>::WG;
   constexpr int PB = Slots<M, PT, NT>::PER_BLOCK;
   const int grid = (ra.s.ntrans + PB - 1) / PB;
   if (KC >= 0 && ra.s.kmax != KC) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((roll1d_kernel<M, PT, KIND, ORDER, NT, FR, KC, LAY>), dim3(grid), dim3(WG), 0, st, ra);
+  hipLaunchKernelGGL((roll1d_kernel<M, PT, KIND, ORDER, NT, FR, KC, LAY, CPLX>), dim3(grid), dim3(WG), 0, st, ra);
   return hipGetLastError();
 }
 

@@ -172,6 +172,59 @@ RollFn EXMI_CAT(get_roll1d_, EXMI_M)(int kind, int order, int nt, int fr, int st
   return nullptr;
 }
 
+// Complex linear operator (exp(L dt) complex, real phi-coefficients): ETDRK
+// orders 0-2, one transform per lane group, sizes up to M = 512.
+template <int M, int PT, int KIND>
+static StepFn pick_step_c(int order) {
+  switch (order) {
+    case 1: return &launch_step1d<M, PT, KIND, 1, EXMI_LAYOUT_1D, true>;
+    case 2: return &launch_step1d<M, PT, KIND, 2, EXMI_LAYOUT_1D, true>;
+    default: return nullptr;
+  }
+}
+template <int M, int PT, int KIND>
+static RollFn pick_roll_c(int order) {
+  switch (order) {
+    case 1: return &launch_roll1d<M, PT, KIND, 1, 1, false, -1, EXMI_LAYOUT_1D, true>;
+    case 2: return &launch_roll1d<M, PT, KIND, 2, 1, false, -1, EXMI_LAYOUT_1D, true>;
+    default: return nullptr;
+  }
+}
+
+StepFn EXMI_CAT(get_step1d_c_, EXMI_M)(int kind, int order) {
+  constexpr int M = EXMI_M, PT = EXMI_PT;
+  if constexpr (M <= 512) {
+    if (order == 0) return &launch_step1d<M, PT, NL_ZERO, 0, EXMI_LAYOUT_1D, true>;
+    switch (kind) {
+      case NL_ZERO: return pick_step_c<M, PT, NL_ZERO>(order);
+      case NL_CONV_CONS: return pick_step_c<M, PT, NL_CONV_CONS>(order);
+      case NL_CONV_NONCONS: return pick_step_c<M, PT, NL_CONV_NONCONS>(order);
+      case NL_GRADNORM: return pick_step_c<M, PT, NL_GRADNORM>(order);
+      case NL_POLY: return pick_step_c<M, PT, NL_POLY>(order);
+      case NL_GENERAL: return pick_step_c<M, PT, NL_GENERAL>(order);
+      default: return nullptr;
+    }
+  }
+  return nullptr;
+}
+
+RollFn EXMI_CAT(get_roll1d_c_, EXMI_M)(int kind, int order) {
+  constexpr int M = EXMI_M, PT = EXMI_PT;
+  if constexpr (M <= 512) {
+    if (order == 0) return &launch_roll1d<M, PT, NL_ZERO, 0, 1, false, -1, EXMI_LAYOUT_1D, true>;
+    switch (kind) {
+      case NL_ZERO: return pick_roll_c<M, PT, NL_ZERO>(order);
+      case NL_CONV_CONS: return pick_roll_c<M, PT, NL_CONV_CONS>(order);
+      case NL_CONV_NONCONS: return pick_roll_c<M, PT, NL_CONV_NONCONS>(order);
+      case NL_GRADNORM: return pick_roll_c<M, PT, NL_GRADNORM>(order);
+      case NL_POLY: return pick_roll_c<M, PT, NL_POLY>(order);
+      case NL_GENERAL: return pick_roll_c<M, PT, NL_GENERAL>(order);
+      default: return nullptr;
+    }
+  }
+  return nullptr;
+}
+
 using VjpFn = hipError_t (*)(const Vjp1DArgs&, hipStream_t);
 
 template <int M, int PT, int KIND>

@@ -100,6 +100,15 @@ typedef struct exmi_plan_desc {
   double poly[EXMI_MAX_POLY]; /* PolynomialNonlinearFun coefficients         */
   double injection_scale;     /* Kolmogorov forcing amplitude                */
   const double* host_linear_operator; /* (lin_batch, lin_channels, Nr) f64  */
+  /* NULL for a real L; else the imaginary parts of L, same shape. A complex L
+   * (odd derivatives: advection, dispersion; stepper/_advection.py:88-95,
+   * _dispersion.py:102-117, _korteweg_de_vries.py) is built for 1-D plans of
+   * ETDRK orders 0-2, N <= 1024, without EXMI_FLAG_FOURIER_RESIDENT. exp(L dt)
+   * is then complex while the phi-coefficients stay real: the reference keeps
+   * the real part of every contour sample (etdrk/_etdrk_1.py:66-76,
+   * _etdrk_2.py:71-89). The table layout gains Im E after the real
+   * coefficients (exmi_plan_num_coefs counts it). No adjoint. */
+  const double* host_linear_operator_imag;
 } exmi_plan_desc;
 
 typedef struct exmi_plan exmi_plan;

@@ -162,6 +162,137 @@ class Burgers(BaseStepper):
                                          conservative=self.conservative, dtype=self._kw_dtype)
 
 
+def _vec(value, d):
+    return np.ones(d) * value if np.isscalar(value) else np.asarray(value, dtype=np.float64)
+
+
+def _diff_matrix(value, d):
+    if np.isscalar(value):
+        return np.eye(d) * value
+    value = np.asarray(value, dtype=np.float64)
+    return np.diag(value) if value.ndim == 1 else value
+
+
+class _LinearStepper(BaseStepper):
+    """ETDRK0 with a zero nonlinear term (the linear steppers of
+    ``stepper/_advection.py``, ``_diffusion.py``, ``_advection_diffusion.py``,
+    ``_dispersion.py``, ``_hyper_diffusion.py``)."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, **kw):
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=1, order=0, **kw)
+
+    def _build_nonlinear_fun(self, d):
+        return nl.ZeroNonlinearFun(self.num_spatial_dims, self.num_points)
+
+
+class Advection(_LinearStepper):
+    """``stepper/_advection.py:15-104``: L = -sum_d c_d (i kappa_d)."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, velocity=1.0, **kw):
+        self.velocity = _vec(velocity, num_spatial_dims)  # :80-82
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, **kw)
+
+    def _build_linear_operator(self, d):
+        return -sp.build_gradient_inner_product_operator(d, self.velocity, order=1)  # :88-95
+
+
+class Diffusion(_LinearStepper):
+    """``stepper/_diffusion.py:13-122``: L = sum_ij A_ij (i kappa_i)(i kappa_j)."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity=0.01, **kw):
+        self.diffusivity = _diff_matrix(diffusivity, num_spatial_dims)  # :84-88
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, **kw)
+
+    def _build_linear_operator(self, d):
+        return np.einsum("ij,ij...->...", self.diffusivity, d[:, None] * d[None, :])[None]  # :99-113
+
+
+class AdvectionDiffusion(_LinearStepper):
+    """``stepper/_advection_diffusion.py:15-136``."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, velocity=1.0, diffusivity=0.01, **kw):
+        self.velocity = _vec(velocity, num_spatial_dims)
+        self.diffusivity = _diff_matrix(diffusivity, num_spatial_dims)
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, **kw)
+
+    def _build_linear_operator(self, d):  # :107-128
+        diffusion = np.einsum("ij,ij...->...", self.diffusivity, d[:, None] * d[None, :])[None]
+        return -sp.build_gradient_inner_product_operator(d, self.velocity, order=1) + diffusion
+
+
+class Dispersion(_LinearStepper):
+    """``stepper/_dispersion.py:15-126``."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, dispersivity=1.0,
+                 advect_on_diffusion=False, **kw):
+        self.dispersivity = _vec(dispersivity, num_spatial_dims)
+        self.advect_on_diffusion = advect_on_diffusion
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, **kw)
+
+    def _build_linear_operator(self, d):  # :102-117
+        if self.advect_on_diffusion:
+            return (sp.build_gradient_inner_product_operator(d, self.dispersivity, order=1)
+                    * sp.build_laplace_operator(d))
+        return sp.build_gradient_inner_product_operator(d, self.dispersivity, order=3)
+
+
+class HyperDiffusion(_LinearStepper):
+    """``stepper/_hyper_diffusion.py:10-120``."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, hyper_diffusivity=0.0001,
+                 diffuse_on_diffuse=False, **kw):
+        self.hyper_diffusivity = hyper_diffusivity
+        self.diffuse_on_diffuse = diffuse_on_diffuse
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, **kw)
+
+    def _build_linear_operator(self, d):  # :95-110
+        if self.diffuse_on_diffuse:
+            lap = sp.build_laplace_operator(d)
+            return -self.hyper_diffusivity * lap * lap
+        return -self.hyper_diffusivity * sp.build_laplace_operator(d, order=4)
+
+
+class KortewegDeVries(BaseStepper):
+    """``stepper/_korteweg_de_vries.py:21-224``."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, convection_scale=-6.0, diffusivity=0.0,
+                 dispersivity=1.0, hyper_diffusivity=0.01, advect_over_diffuse=False, diffuse_over_diffuse=False,
+                 single_channel=False, conservative=False, order=2, dealiasing_fraction=2 / 3,
+                 num_circle_points=16, circle_radius=1.0, **kw):
+        self.convection_scale = convection_scale
+        self.diffusivity = diffusivity
+        self.dispersivity = dispersivity
+        self.hyper_diffusivity = hyper_diffusivity
+        self.advect_over_diffuse = advect_over_diffuse
+        self.diffuse_over_diffuse = diffuse_over_diffuse
+        self.single_channel = single_channel
+        self.conservative = conservative
+        self.dealiasing_fraction = dealiasing_fraction
+        self._kw_dtype = kw.get("dtype", np.float64)
+        nc = 1 if single_channel else num_spatial_dims
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=nc, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius, **kw)
+
+    def _build_linear_operator(self, d):  # :170-205
+        velocity = self.dispersivity * np.ones(self.num_spatial_dims)
+        lap = sp.build_laplace_operator(d, order=2)
+        if self.advect_over_diffuse:
+            dispersion = -sp.build_gradient_inner_product_operator(d, velocity, order=1) * lap
+        else:
+            dispersion = -sp.build_gradient_inner_product_operator(d, velocity, order=3)
+        if self.diffuse_over_diffuse:
+            hyper = -self.hyper_diffusivity * lap * lap
+        else:
+            hyper = -self.hyper_diffusivity * sp.build_laplace_operator(d, order=4)
+        return self.diffusivity * lap + dispersion + hyper
+
+    def _build_nonlinear_fun(self, d):  # :207-224
+        return nl.ConvectionNonlinearFun(self.num_spatial_dims, self.num_points, derivative_operator=d,
+                                         dealiasing_fraction=self.dealiasing_fraction,
+                                         scale=self.convection_scale, single_channel=self.single_channel,
+                                         conservative=self.conservative, dtype=self._kw_dtype)
+
+
 class NavierStokesVorticity(BaseStepper):
     """``stepper/_navier_stokes.py:13-153``."""
 

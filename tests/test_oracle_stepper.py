@@ -171,3 +171,54 @@ def test_rollout_resume_bit_exact():
     snap = (100, first[-1].copy())
     second = st.rollout(s, 100)(snap[1])
     np.testing.assert_array_equal(np.concatenate([first, second]), full)
+
+
+# ---------------------------------------------------------------- linear steppers and KdV
+# Analytic pins of the linear operators (ETDRK0 is the exact exponential) and
+# of the KdV signs (a travelling soliton).
+def _grid(L, N):
+    return np.linspace(0.0, L, N, endpoint=False)
+
+
+@pytest.mark.parametrize("velocity", [0.7, -1.3])
+def test_advection_exact_shift(velocity):
+    L, N, dt = 2.0, 64, 0.1
+    x = _grid(L, N)
+    adv = st.Advection(1, L, N, dt, velocity=velocity)
+    u = adv.step(np.sin(2 * np.pi * 3 * x / L)[None])
+    np.testing.assert_allclose(u[0], np.sin(2 * np.pi * 3 * (x - velocity * dt) / L), atol=1e-12)
+
+
+def test_diffusion_and_hyperdiffusion_exact_decay():
+    L, N, dt, k = 3.0, 48, 0.2, 4
+    x = _grid(L, N)
+    kap = 2 * np.pi * k / L
+    u0 = np.cos(kap * x)[None]
+    np.testing.assert_allclose(st.Diffusion(1, L, N, dt, diffusivity=0.05).step(u0)[0],
+                               np.exp(-0.05 * kap**2 * dt) * u0[0], atol=1e-12)
+    np.testing.assert_allclose(st.HyperDiffusion(1, L, N, dt, hyper_diffusivity=1e-3).step(u0)[0],
+                               np.exp(-1e-3 * kap**4 * dt) * u0[0], atol=1e-12)
+    ad = st.AdvectionDiffusion(1, L, N, dt, velocity=0.5, diffusivity=0.05).step(u0)[0]
+    np.testing.assert_allclose(ad, np.exp(-0.05 * kap**2 * dt) * np.cos(kap * (x - 0.5 * dt)), atol=1e-12)
+
+
+def test_dispersion_exact_phase():
+    # u_t = a u_xxx: e^{i k x} -> e^{i k (x - a k^2 t)}
+    L, N, dt, a, k = 2 * np.pi, 64, 0.05, 0.3, 3
+    x = _grid(L, N)
+    u = st.Dispersion(1, L, N, dt, dispersivity=a).step(np.sin(k * x)[None])
+    np.testing.assert_allclose(u[0], np.sin(k * (x - a * k**2 * dt)), atol=1e-12)
+
+
+def test_kdv_soliton():
+    """u_t - 6 u u_x + u_xxx = 0 (convection_scale -6, dispersivity 1, no
+    hyper-diffusion) carries u = -(c/2) sech^2(sqrt(c)/2 (x - c t - x0))."""
+    L, N, dt, c, x0, n = 40.0, 256, 1e-3, 1.0, 15.0, 400
+    x = _grid(L, N)
+    sol = lambda t: -0.5 * c / np.cosh(0.5 * np.sqrt(c) * (x - c * t - x0)) ** 2  # noqa: E731
+    kdv = st.KortewegDeVries(1, L, N, dt, hyper_diffusivity=0.0, dealiasing_fraction=1.0)
+    u = sol(0.0)[None]
+    for _ in range(n):
+        u = kdv.step(u)
+    # 2.5e-4 with the reference's convection_scale -6; the opposite sign moves it the wrong way (0.28)
+    assert np.abs(u[0] - sol(n * dt)).max() < 2e-3
