@@ -1,10 +1,11 @@
 """Concrete steppers with exponax's constructor signatures."""
+from . import generic
 from ._burgers import Burgers
 from ._korteweg_de_vries import KortewegDeVries
 from ._kuramoto_sivashinsky import KuramotoSivashinsky, KuramotoSivashinskyConservative
 from ._linear import Advection, AdvectionDiffusion, Diffusion, Dispersion, HyperDiffusion
 from ._navier_stokes import KolmogorovFlowVorticity, NavierStokesVorticity
 
-__all__ = ["Advection", "AdvectionDiffusion", "Burgers", "Diffusion", "Dispersion", "HyperDiffusion",
+__all__ = ["generic", "Advection", "AdvectionDiffusion", "Burgers", "Diffusion", "Dispersion", "HyperDiffusion",
            "KortewegDeVries", "KuramotoSivashinsky", "KuramotoSivashinskyConservative", "KolmogorovFlowVorticity",
            "NavierStokesVorticity"]

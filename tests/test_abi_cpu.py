@@ -150,3 +150,40 @@ def test_trajectory_file_format(tmp_path):
         json.dump({"shape": [2, 4, 1, 8]}, f)
     with pytest.raises(ValueError):
         exio.open_trajectories(p)
+
+
+def test_generic_parametrisations():
+    """Normalized / difficulty conversions (reference stepper/generic/_utils.py):
+    round trips, and every Difficulty* default maps onto the matching
+    Normalized* default (the reference chose them consistently for N = 48,
+    D = 1), which fixes the N^j 2^(j-1) D and M N D / M N^2 D factors."""
+    import numpy as np
+
+    from exponax_amd.stepper import generic as g
+
+    c = (0.3, -1.2, 0.01, 0.002)
+    n = g.normalize_coefficients(c, domain_extent=3.0, dt=0.1)
+    np.testing.assert_allclose(g.denormalize_coefficients(n, domain_extent=3.0, dt=0.1), c, rtol=1e-14)
+    np.testing.assert_allclose(n, [0.03, -0.04, 0.01 * 0.1 / 9, 0.002 * 0.1 / 27], rtol=1e-14)
+    kw = dict(num_spatial_dims=2, num_points=64)
+    d = g.reduce_normalized_coefficients_to_difficulty(n, **kw)
+    np.testing.assert_allclose(g.extract_normalized_coefficients_from_difficulty(d, **kw), n, rtol=1e-14)
+    assert g.denormalize_convection_scale(g.normalize_convection_scale(2.0, domain_extent=5.0, dt=0.2),
+                                          domain_extent=5.0, dt=0.2) == pytest.approx(2.0)
+    assert g.normalize_gradient_norm_scale(1.0, domain_extent=60.0, dt=0.1) == pytest.approx(0.1 / 3600)
+    # defaults: Difficulty -> Normalized
+    pairs = [(g.DifficultyPolynomialStepper(), g.NormalizedPolynomialStepper(1, 48)),
+             (g.DifficultyNonlinearStepper(), g.NormalizedNonlinearStepper(1, 48)),
+             (g.DifficultyGradientNormStepper(), g.NormalizedGradientNormStepper(1, 48))]
+    for dif, nor in pairs:
+        np.testing.assert_allclose(dif.linear_coefficients, nor.linear_coefficients, rtol=1e-12, atol=1e-18)
+    np.testing.assert_allclose(pairs[1][0].nonlinear_coefficients, pairs[1][1].nonlinear_coefficients, rtol=1e-12)
+    assert pairs[2][0].gradient_norm_scale == pytest.approx(pairs[2][1].gradient_norm_scale, rel=1e-12)
+    s = g.DifficultyLinearStepperSimple(difficulty=-2.0, order=1)
+    assert s.linear_difficulties == (0.0, -2.0)
+    with pytest.warns(DeprecationWarning):
+        g.DiffultyLinearStepperSimple()
+    with pytest.raises(ValueError):
+        g.GeneralVorticityConvectionStepper(1, 1.0, 32, 0.01)
+    with pytest.raises(ValueError):
+        g.GeneralNonlinearStepper(1, 1.0, 32, 0.1, nonlinear_coefficients=(0.0, 1.0))
