@@ -1,0 +1,73 @@
+"""Reaction-diffusion steppers with polynomial reactions
+(``stepper/reaction/_allen_cahn.py``, ``_fisher_kpp.py``,
+``_swift_hohenberg.py``): a real linear operator and the polynomial
+nonlinear function (1-D kernels)."""
+from .._base_stepper import BaseStepper
+from ..nonlin_fun import PolynomialNonlinearFun
+from ..spectral import build_laplace_operator
+
+
+class _Polynomial(BaseStepper):
+    def _build_nonlinear_fun(self, derivative_operator):
+        return PolynomialNonlinearFun(self.num_spatial_dims, self.num_points,
+                                      dealiasing_fraction=self.dealiasing_fraction,
+                                      coefficients=self._poly())
+
+
+class AllenCahn(_Polynomial):
+    """u_t = nu Laplace u + c1 u + c3 u^3 (``_allen_cahn.py``)."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity=5e-3,
+                 first_order_coefficient=1.0, third_order_coefficient=-1.0, order=2, dealiasing_fraction=1 / 2,
+                 num_circle_points=16, circle_radius=1.0):
+        self.diffusivity = diffusivity
+        self.first_order_coefficient = first_order_coefficient
+        self.third_order_coefficient = third_order_coefficient
+        self.dealiasing_fraction = dealiasing_fraction
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=1, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius)
+
+    def _build_linear_operator(self, derivative_operator):
+        return self.diffusivity * build_laplace_operator(derivative_operator, order=2) + self.first_order_coefficient
+
+    def _poly(self):
+        return [0.0, 0.0, 0.0, self.third_order_coefficient]
+
+
+class FisherKPP(_Polynomial):
+    """u_t = nu Laplace u + r u (1 - u) (``_fisher_kpp.py``)."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity=0.01, reactivity=1.0, order=2,
+                 dealiasing_fraction=2 / 3, num_circle_points=16, circle_radius=1.0):
+        self.diffusivity = diffusivity
+        self.reactivity = reactivity
+        self.dealiasing_fraction = dealiasing_fraction
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=1, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius)
+
+    def _build_linear_operator(self, derivative_operator):
+        return self.diffusivity * build_laplace_operator(derivative_operator, order=2) + self.reactivity
+
+    def _poly(self):
+        return [0.0, 0.0, -self.reactivity]
+
+
+class SwiftHohenberg(_Polynomial):
+    """u_t = r u - (k + Laplace)^2 u + sum_j p_j u^j (``_swift_hohenberg.py``)."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, reactivity=0.7, critical_number=1.0,
+                 polynomial_coefficients=(0.0, 0.0, 1.0, -1.0), order=2, dealiasing_fraction=1 / 2,
+                 num_circle_points=16, circle_radius=1.0):
+        self.reactivity = reactivity
+        self.critical_number = critical_number
+        self.polynomial_coefficients = tuple(polynomial_coefficients)
+        self.dealiasing_fraction = dealiasing_fraction
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=1, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius)
+
+    def _build_linear_operator(self, derivative_operator):
+        lap = build_laplace_operator(derivative_operator, order=2)
+        return self.reactivity - (self.critical_number + lap) ** 2
+
+    def _poly(self):
+        return list(self.polynomial_coefficients)

@@ -293,6 +293,72 @@ class KortewegDeVries(BaseStepper):
                                          conservative=self.conservative, dtype=self._kw_dtype)
 
 
+class _PolynomialReaction(BaseStepper):
+    """Real linear operator + ``PolynomialNonlinearFun`` (``stepper/reaction/``)."""
+
+    def _build_nonlinear_fun(self, d):
+        return nl.PolynomialNonlinearFun(self.num_spatial_dims, self.num_points,
+                                         dealiasing_fraction=self.dealiasing_fraction, coefficients=self._poly(),
+                                         dtype=self._kw_dtype)
+
+
+class AllenCahn(_PolynomialReaction):
+    """``stepper/reaction/_allen_cahn.py``: L = nu Laplace + c1, N = c3 u^3."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity=5e-3,
+                 first_order_coefficient=1.0, third_order_coefficient=-1.0, order=2, dealiasing_fraction=1 / 2,
+                 num_circle_points=16, circle_radius=1.0, **kw):
+        self.diffusivity, self.c1, self.c3 = diffusivity, first_order_coefficient, third_order_coefficient
+        self.dealiasing_fraction = dealiasing_fraction
+        self._kw_dtype = kw.get("dtype", np.float64)
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=1, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius, **kw)
+
+    def _build_linear_operator(self, d):
+        return self.diffusivity * sp.build_laplace_operator(d, order=2) + self.c1
+
+    def _poly(self):
+        return [0.0, 0.0, 0.0, self.c3]
+
+
+class FisherKPP(_PolynomialReaction):
+    """``stepper/reaction/_fisher_kpp.py``: L = nu Laplace + r, N = -r u^2."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity=0.01, reactivity=1.0,
+                 order=2, dealiasing_fraction=2 / 3, num_circle_points=16, circle_radius=1.0, **kw):
+        self.diffusivity, self.reactivity = diffusivity, reactivity
+        self.dealiasing_fraction = dealiasing_fraction
+        self._kw_dtype = kw.get("dtype", np.float64)
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=1, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius, **kw)
+
+    def _build_linear_operator(self, d):
+        return self.diffusivity * sp.build_laplace_operator(d, order=2) + self.reactivity
+
+    def _poly(self):
+        return [0.0, 0.0, -self.reactivity]
+
+
+class SwiftHohenberg(_PolynomialReaction):
+    """``stepper/reaction/_swift_hohenberg.py``: L = r - (k + Laplace)^2."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, reactivity=0.7, critical_number=1.0,
+                 polynomial_coefficients=(0.0, 0.0, 1.0, -1.0), order=2, dealiasing_fraction=1 / 2,
+                 num_circle_points=16, circle_radius=1.0, **kw):
+        self.reactivity, self.critical_number = reactivity, critical_number
+        self.polynomial_coefficients = tuple(polynomial_coefficients)
+        self.dealiasing_fraction = dealiasing_fraction
+        self._kw_dtype = kw.get("dtype", np.float64)
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=1, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius, **kw)
+
+    def _build_linear_operator(self, d):
+        return self.reactivity - (self.critical_number + sp.build_laplace_operator(d, order=2)) ** 2
+
+    def _poly(self):
+        return list(self.polynomial_coefficients)
+
+
 class NavierStokesVorticity(BaseStepper):
     """``stepper/_navier_stokes.py:13-153``."""
 

@@ -86,3 +86,27 @@ def test_complex_operator_limits(device):
     kdv = ex.stepper.KortewegDeVries(1, 20.0, 64, 1e-3)
     with pytest.raises(NotImplementedError):
         kdv.step_vjp(torch.zeros((1, 64), device=device), torch.zeros((1, 64), device=device))
+
+
+@pytest.mark.parametrize("N", [64, 256])
+@pytest.mark.parametrize("name", ["AllenCahn", "FisherKPP", "SwiftHohenberg"])
+def test_polynomial_reaction_parity(device, name, N):
+    """stepper/reaction/_allen_cahn.py, _fisher_kpp.py, _swift_hohenberg.py:
+    polynomial reactions (cubic ones with the 1/2 dealiasing default, a
+    run-time cutoff) against the FP32 oracle at steps 1 and 10."""
+    import torch
+
+    L, dt = 10.0, 0.01
+    gpu = getattr(ex.stepper.reaction, name)(1, L, N, dt)
+    cpu = getattr(ost, name)(1, L, N, dt, dtype=np.float32)
+    u0, _ = ics(3, 1, L, N, 4, seed=71)
+    u0 = (0.5 * u0 + 0.3).astype(np.float32)  # a positive field for the logistic reaction
+    x = torch.tensor(u0, device=device)
+    traj = gpu.rollout_batch(x, 10).cpu().numpy()
+    uc = u0
+    for s in range(1, 11):
+        uc = cpu.step_batch(uc)
+        if s in (1, 10):
+            err = max_rel_l2(traj[:, s - 1], uc)
+            print(f"  {name} N={N} step {s}: {err:.2e}", flush=True)
+            assert err <= RTOL_STEP

@@ -222,3 +222,38 @@ def test_kdv_soliton():
         u = kdv.step(u)
     # 2.5e-4 with the reference's convection_scale -6; the opposite sign moves it the wrong way (0.28)
     assert np.abs(u[0] - sol(n * dt)).max() < 2e-3
+
+
+# ---------------------------------------------------------------- polynomial reactions
+# A spatially uniform state follows the reaction ODE u' = L(0) u + p(u).
+def _uniform_run(stepper, u0, n):
+    u = np.full((1, stepper.num_points), u0)
+    for _ in range(n):
+        u = stepper.step(u)
+    assert np.ptp(u) < 1e-12
+    return float(u[0, 0])
+
+
+def test_fisher_kpp_logistic():
+    r, u0, dt, n = 1.0, 0.1, 0.01, 100
+    got = _uniform_run(st.FisherKPP(1, 1.0, 32, dt, reactivity=r), u0, n)
+    t = n * dt
+    exact = u0 * np.exp(r * t) / (1 - u0 + u0 * np.exp(r * t))
+    assert abs(got - exact) < 2e-5
+
+
+def test_allen_cahn_cubic():
+    u0, dt, n = 0.2, 0.01, 100
+    got = _uniform_run(st.AllenCahn(1, 1.0, 32, dt), u0, n)  # u' = u - u^3
+    t = n * dt
+    exact = u0 * np.exp(t) / np.sqrt(1 + u0**2 * (np.exp(2 * t) - 1))
+    assert abs(got - exact) < 2e-5
+
+
+def test_swift_hohenberg_uniform():
+    from scipy.integrate import solve_ivp
+
+    u0, dt, n = 0.5, 0.01, 100
+    got = _uniform_run(st.SwiftHohenberg(1, 10.0, 32, dt), u0, n)  # u' = (0.7 - 1) u + u^2 - u^3
+    ref = solve_ivp(lambda t, u: -0.3 * u + u**2 - u**3, (0.0, n * dt), [u0], rtol=1e-12, atol=1e-14).y[0, -1]
+    assert abs(got - ref) < 2e-5
