@@ -23,8 +23,10 @@ LIBDIR = os.path.join(HERE, "_lib")
 BUILDDIR = os.path.join(HERE, "_build")
 LIBNAME = "libexmi.so"
 
-# transform size M = N/2 -> points per lane
-SIZES_1D = {8: 8, 16: 8, 32: 8, 64: 8, 128: 8, 256: 8, 512: 8, 1024: 16, 2048: 16}
+# transform size M = N/2 -> points per lane (24 and 25: single-lane
+# mixed-radix transforms for N = 48 and N = 50; odd M keeps the natural
+# spectral layout, the paired one needs an even number of slots)
+SIZES_1D = {8: 8, 16: 8, 24: 24, 25: 25, 32: 8, 64: 8, 128: 8, 256: 8, 512: 8, 1024: 16, 2048: 16}
 # 2-D grid sizes N (N x N)
 SIZES_2D = (32, 64, 128, 256, 512)
 
@@ -48,7 +50,8 @@ def _jobs():
             ("metrics", os.path.join(CSRC, "metrics.hip"), []),
             ("corrector", os.path.join(CSRC, "corrector.hip"), [])]
     for m, pt in SIZES_1D.items():
-        jobs.append((f"step1d_{m}", os.path.join(CSRC, "step1d_inst.hip"), [f"-DEXMI_M={m}", f"-DEXMI_PT={pt}"]))
+        defs = [f"-DEXMI_M={m}", f"-DEXMI_PT={pt}"] + (["-DEXMI_LAYOUT_1D=NATURAL"] if m % 2 else [])
+        jobs.append((f"step1d_{m}", os.path.join(CSRC, "step1d_inst.hip"), defs))
     for n in SIZES_2D:
         jobs.append((f"step2d_{n}", os.path.join(CSRC, "step2d_inst.hip"), [f"-DEXMI_N2={n}"]))
     # longest compiles first

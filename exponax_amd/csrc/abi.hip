@@ -37,6 +37,8 @@ hipError_t launch_coef(const double* L, const double* Li, long long total, int n
   hipError_t launch_fft1d_##M(bool inverse, const float* in, float* out, int count, const RealTables& rt, \
                               hipStream_t st);
 EXMI_DECL_SIZE(8)
+EXMI_DECL_SIZE(24)
+EXMI_DECL_SIZE(25)
 EXMI_DECL_SIZE(16)
 EXMI_DECL_SIZE(32)
 EXMI_DECL_SIZE(64)
@@ -84,6 +86,8 @@ static Launch2D get_2d(int N, int kind, int is_row, int mode) {
 static StepFn get_step1d(int M, int kind, int order) {
   switch (M) {
     case 8: return get_step1d_8(kind, order);
+    case 24: return get_step1d_24(kind, order);
+    case 25: return get_step1d_25(kind, order);
     case 16: return get_step1d_16(kind, order);
     case 32: return get_step1d_32(kind, order);
     case 64: return get_step1d_64(kind, order);
@@ -99,6 +103,8 @@ static StepFn get_step1d(int M, int kind, int order) {
 static VjpFn get_vjp1d(int M, int kind, int order) {
   switch (M) {
     case 8: return get_vjp1d_8(kind, order);
+    case 24: return get_vjp1d_24(kind, order);
+    case 25: return get_vjp1d_25(kind, order);
     case 16: return get_vjp1d_16(kind, order);
     case 32: return get_vjp1d_32(kind, order);
     case 64: return get_vjp1d_64(kind, order);
@@ -116,7 +122,8 @@ static StepFn get_step1d_c(int M, int kind, int order) {
   switch (M) {
 #define EXMI_CASE(MM) \
   case MM: return get_step1d_c_##MM(kind, order);
-    EXMI_CASE(8) EXMI_CASE(16) EXMI_CASE(32) EXMI_CASE(64) EXMI_CASE(128) EXMI_CASE(256) EXMI_CASE(512)
+    EXMI_CASE(8) EXMI_CASE(16) EXMI_CASE(24) EXMI_CASE(25) EXMI_CASE(32) EXMI_CASE(64) EXMI_CASE(128)
+    EXMI_CASE(256) EXMI_CASE(512)
 #undef EXMI_CASE
     default: return nullptr;
   }
@@ -125,7 +132,8 @@ static RollFn get_roll1d_c(int M, int kind, int order) {
   switch (M) {
 #define EXMI_CASE(MM) \
   case MM: return get_roll1d_c_##MM(kind, order);
-    EXMI_CASE(8) EXMI_CASE(16) EXMI_CASE(32) EXMI_CASE(64) EXMI_CASE(128) EXMI_CASE(256) EXMI_CASE(512)
+    EXMI_CASE(8) EXMI_CASE(16) EXMI_CASE(24) EXMI_CASE(25) EXMI_CASE(32) EXMI_CASE(64) EXMI_CASE(128)
+    EXMI_CASE(256) EXMI_CASE(512)
 #undef EXMI_CASE
     default: return nullptr;
   }
@@ -134,6 +142,8 @@ static RollFn get_roll1d_c(int M, int kind, int order) {
 static RollFn get_roll1d(int M, int kind, int order, int nt, int fr, int std_kc) {
   switch (M) {
     case 8: return get_roll1d_8(kind, order, nt, fr, std_kc);
+    case 24: return get_roll1d_24(kind, order, nt, fr, std_kc);
+    case 25: return get_roll1d_25(kind, order, nt, fr, std_kc);
     case 16: return get_roll1d_16(kind, order, nt, fr, std_kc);
     case 32: return get_roll1d_32(kind, order, nt, fr, std_kc);
     case 64: return get_roll1d_64(kind, order, nt, fr, std_kc);
@@ -153,6 +163,8 @@ static hipError_t launch_fft1d(int M, bool inverse, const float* in, float* out,
   case MM: return launch_fft1d_##MM(inverse, in, out, count, rt, st);
     EXMI_CASE(8)
     EXMI_CASE(16)
+    EXMI_CASE(24)
+    EXMI_CASE(25)
     EXMI_CASE(32)
     EXMI_CASE(64)
     EXMI_CASE(128)
@@ -227,6 +239,9 @@ static int get_twiddles(int M, Twiddles* out) {
 }
 
 static bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+// 1-D transform sizes: powers of two 16..4096, and 48 and 50 (single-lane
+// mixed-radix transforms of 24 = 8 x 3 and 25 = 5 x 5 points)
+static bool size_1d(int n) { return (pow2(n) && n >= 16 && n <= 4096) || n == 48 || n == 50; }
 
 // Dealias cutoff of nonlin_fun/_base.py:59-71: keep |k| <= frac*(N//2) - 1,
 // compared in float32 as the reference's float32 wavenumber grid is.
@@ -552,8 +567,8 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   const int D = d->num_spatial_dims, N = d->num_points;
   if (D == 2) return plan_create_2d(d, out);
   if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "num_spatial_dims=%d: this build supports 1 and 2", D);
-  if (!pow2(N) || N < 16 || N > 4096)
-    return fail(EXMI_ERR_UNSUPPORTED, "num_points=%d: supported sizes are powers of two 16..4096", N);
+  if (!size_1d(N))
+    return fail(EXMI_ERR_UNSUPPORTED, "num_points=%d: supported sizes are powers of two 16..4096, 48 and 50", N);
   if (d->num_channels < 1 || d->batch < 1) return fail(EXMI_ERR_INVALID, "num_channels and batch must be >= 1");
   if (d->order < 0 || d->order > 4) return fail(EXMI_ERR_UNSUPPORTED, "Order %d not implemented.", d->order);
   if (d->lin_batch != 1 && d->lin_batch != d->batch)
@@ -1134,7 +1149,7 @@ int exmi_repeat(exmi_plan* p, const float* u0, int64_t u0_bs, float* u_out, int3
 
 int exmi_rfft(int32_t D, int32_t N, int32_t count, const float* x, float* x_hat, void* stream) {
   if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "exmi_rfft: num_spatial_dims=%d not supported", D);
-  if (!pow2(N) || N < 16 || N > 4096) return fail(EXMI_ERR_UNSUPPORTED, "exmi_rfft: num_points=%d", N);
+  if (!size_1d(N)) return fail(EXMI_ERR_UNSUPPORTED, "exmi_rfft: num_points=%d", N);
   if (count <= 0) return EXMI_OK;
   Twiddles tw;
   int rc = get_twiddles(N / 2, &tw);
@@ -1145,7 +1160,7 @@ int exmi_rfft(int32_t D, int32_t N, int32_t count, const float* x, float* x_hat,
 
 int exmi_irfft(int32_t D, int32_t N, int32_t count, const float* x_hat, float* x, void* stream) {
   if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "exmi_irfft: num_spatial_dims=%d not supported", D);
-  if (!pow2(N) || N < 16 || N > 4096) return fail(EXMI_ERR_UNSUPPORTED, "exmi_irfft: num_points=%d", N);
+  if (!size_1d(N)) return fail(EXMI_ERR_UNSUPPORTED, "exmi_irfft: num_points=%d", N);
   if (count <= 0) return EXMI_OK;
   Twiddles tw;
   int rc = get_twiddles(N / 2, &tw);

@@ -107,6 +107,52 @@ struct Dft<4, INV> {
 };
 
 template <bool INV>
+struct Dft<3, INV> {
+  // X1, X2 = a - (b + c)/2 -+ i s (b - c) (forward), s = sin(2 pi / 3)
+  __device__ __forceinline__ static void run(cf* v) {
+    const float s3 = 0.86602540378443864676f;
+    const cf a = v[0], t1 = v[1] + v[2], d = v[1] - v[2];
+    const cf t2 = cf{fmaf(-0.5f, t1.re, a.re), fmaf(-0.5f, t1.im, a.im)};
+    const cf rd = rot90<INV>(cf{s3 * d.re, s3 * d.im});  // -i s d forward, +i s d inverse
+    v[0] = a + t1;
+    v[1] = t2 + rd;
+    v[2] = t2 - rd;
+  }
+  __device__ __forceinline__ static void run_tw(cf* v, const cf* w) {
+    v[1] = twid<INV>(v[1], w[0]);
+    v[2] = twid<INV>(v[2], w[1]);
+    run(v);
+  }
+};
+
+template <bool INV>
+struct Dft<5, INV> {
+  // t1 = x1 + x4, t2 = x2 + x3, t3 = x1 - x4, t4 = x2 - x3;
+  // X1,4 = x0 + c1 t1 + c2 t2 -+ i (s1 t3 + s2 t4),
+  // X2,3 = x0 + c2 t1 + c1 t2 -+ i (s2 t3 - s1 t4) (forward; + - for inverse),
+  // c_j = cos(2 pi j / 5), s_j = sin(2 pi j / 5).
+  __device__ __forceinline__ static void run(cf* v) {
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+    const cf x0 = v[0], t1 = v[1] + v[4], t2 = v[2] + v[3], t3 = v[1] - v[4], t4 = v[2] - v[3];
+    const cf a1 = cf{fmaf(c1, t1.re, fmaf(c2, t2.re, x0.re)), fmaf(c1, t1.im, fmaf(c2, t2.im, x0.im))};
+    const cf a2 = cf{fmaf(c2, t1.re, fmaf(c1, t2.re, x0.re)), fmaf(c2, t1.im, fmaf(c1, t2.im, x0.im))};
+    const cf b1 = rot90<INV>(cf{fmaf(s1, t3.re, s2 * t4.re), fmaf(s1, t3.im, s2 * t4.im)});
+    const cf b2 = rot90<INV>(cf{fmaf(s2, t3.re, -s1 * t4.re), fmaf(s2, t3.im, -s1 * t4.im)});
+    v[0] = x0 + t1 + t2;
+    v[1] = a1 + b1;
+    v[4] = a1 - b1;
+    v[2] = a2 + b2;
+    v[3] = a2 - b2;
+  }
+  __device__ __forceinline__ static void run_tw(cf* v, const cf* w) {
+#pragma unroll
+    for (int r = 1; r < 5; ++r) v[r] = twid<INV>(v[r], w[r - 1]);
+    run(v);
+  }
+};
+
+template <bool INV>
 struct Dft<8, INV> {
   // radix-2 split into two DFT4s; the odd outputs' W8^1 and W8^3 factors
   // (h (1 -+ i), h (-1 -+ i)) are folded into fused multiply-adds of the
@@ -210,6 +256,22 @@ struct Plan {
   }
 };
 
+// Single-lane transforms of the non-power-of-two sizes N = 48 and N = 50
+// (every point of a transform in one lane's registers, T = 1): 24 = 8 x 3,
+// 25 = 5 x 5.
+template <>
+struct Plan<24, 24> {
+  static constexpr int NPASS = 2;
+  __host__ __device__ static constexpr int radix(int p) { return p == 0 ? 8 : 3; }
+  __host__ __device__ static constexpr int stride(int p) { return p == 0 ? 1 : 8; }
+};
+template <>
+struct Plan<25, 25> {
+  static constexpr int NPASS = 2;
+  __host__ __device__ static constexpr int radix(int) { return 5; }
+  __host__ __device__ static constexpr int stride(int p) { return p == 0 ? 1 : 5; }
+};
+
 // LDS index padding: one pad slot every 16 complex entries.
 __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
 template <int M>
@@ -268,7 +330,20 @@ __device__ __forceinline__ void stockham_pass(cf (*v)[PT], const Lane<M, PT>& ln
       for (int r = 0; r < R; ++r) v[n][q + r * NB] = x[r];
     }
   }
-  if constexpr (!LAST) {
+  if constexpr (!LAST && T == 1) {
+    // one lane holds the whole transform: the exchange is a compile-time
+    // permutation of its registers
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      cf y[PT];
+#pragma unroll
+      for (int q = 0; q < NB; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r) y[(q / NS) * NS * R + (q % NS) + r * NS] = v[n][q + r * NB];
+#pragma unroll
+      for (int s = 0; s < PT; ++s) v[n][s] = y[s];
+    }
+  } else if constexpr (!LAST) {
     // output y[(b/NS)*NS*R + b%NS + r*NS] = v[n][q + r*NB]
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
@@ -380,12 +455,17 @@ __device__ __forceinline__ void rfft(cf (*z)[PT], cf* nyq, const Lane<M, PT>& ln
   cf zm[NT][PT];
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
+    if constexpr (T == 1) {
 #pragma unroll
-    for (int s = 0; s < PT; ++s) lds[ln.at(s)] = z[n][s];
-    xsync<T>();
+      for (int s = 0; s < PT; ++s) zm[n][s] = z[n][(PT - s) % PT];
+    } else {
 #pragma unroll
-    for (int s = 0; s < PT; ++s) zm[n][s] = lds[ln.mirror(s)];
-    xsync<T>();
+      for (int s = 0; s < PT; ++s) lds[ln.at(s)] = z[n][s];
+      xsync<T>();
+#pragma unroll
+      for (int s = 0; s < PT; ++s) zm[n][s] = lds[ln.mirror(s)];
+      xsync<T>();
+    }
   }
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
@@ -420,12 +500,17 @@ __device__ __forceinline__ void irfft(cf (*z)[PT], const cf* nyq, const Lane<M, 
   cf xm[NT][PT];
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
+    if constexpr (T == 1) {
 #pragma unroll
-    for (int s = 0; s < PT; ++s) lds[ln.at(s)] = z[n][s];
-    xsync<T>();
+      for (int s = 0; s < PT; ++s) xm[n][s] = z[n][(PT - s) % PT];
+    } else {
 #pragma unroll
-    for (int s = 0; s < PT; ++s) xm[n][s] = lds[ln.mirror(s)];
-    xsync<T>();
+      for (int s = 0; s < PT; ++s) lds[ln.at(s)] = z[n][s];
+      xsync<T>();
+#pragma unroll
+      for (int s = 0; s < PT; ++s) xm[n][s] = lds[ln.mirror(s)];
+      xsync<T>();
+    }
   }
   if (ln.j == 0) {
 #pragma unroll
@@ -486,12 +571,17 @@ __device__ __forceinline__ void rfft_paired(cf (*z)[PT], cf* xm, const Lane<M, P
   cf zp[NT][H];
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
+    if constexpr (T == 1) {
 #pragma unroll
-    for (int s = H; s < PT; ++s) lds[ln.at(s)] = z[n][s];
-    xsync<T>();
+      for (int h = 0; h < H; ++h) zp[n][h] = z[n][(PT - h) % PT];
+    } else {
 #pragma unroll
-    for (int h = 0; h < H; ++h) zp[n][h] = lds[ln.mirror(h)];
-    xsync<T>();
+      for (int s = H; s < PT; ++s) lds[ln.at(s)] = z[n][s];
+      xsync<T>();
+#pragma unroll
+      for (int h = 0; h < H; ++h) zp[n][h] = lds[ln.mirror(h)];
+      xsync<T>();
+    }
   }
   // Lane-0 special cases are selects, not branches: a divergent branch would
   // split the interleaved instruction streams of the NT transforms.
@@ -554,12 +644,17 @@ __device__ __forceinline__ void irfft_paired(cf (*z)[PT], const cf* xm, const La
     }
     // Z'[M-k] to the natural slot of element M-k (lane 0, h = 0 writes the
     // meaningless Z'[M] to element 0's slot, which is not read back)
+    if constexpr (T == 1) {
 #pragma unroll
-    for (int h = 0; h < H; ++h) lds[ln.mirror(h)] = send[h];
-    xsync<T>();
+      for (int h = 1; h < H; ++h) z[n][PT - h] = send[h];
+    } else {
 #pragma unroll
-    for (int s = H; s < PT; ++s) z[n][s] = lds[ln.at(s)];
-    xsync<T>();
+      for (int h = 0; h < H; ++h) lds[ln.mirror(h)] = send[h];
+      xsync<T>();
+#pragma unroll
+      for (int s = H; s < PT; ++s) z[n][s] = lds[ln.at(s)];
+      xsync<T>();
+    }
     if (ln.j == 0) z[n][H] = cf{2.f * xm[n].re, -2.f * xm[n].im};  // Z'[M/2], not exchanged
   }
   cfft<M, PT, true, NT>(z, ln, lds, tr);

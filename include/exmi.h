@@ -77,7 +77,7 @@ enum exmi_nonlin_kind {
  */
 typedef struct exmi_plan_desc {
   int32_t num_spatial_dims;   /* D: 1 or 2                                   */
-  int32_t num_points;         /* N (power of two, 16..4096 in 1-D)           */
+  int32_t num_points;         /* N: 1-D powers of two 16..4096, 48 and 50    */
   int32_t num_channels;       /* C                                           */
   int32_t batch;              /* B                                           */
   int32_t order;              /* ETDRK order 0..4                            */
@@ -210,7 +210,8 @@ int exmi_rollout_vjp(exmi_plan* plan, const float* u0, int64_t u0_bstride, int32
 
 /* Batched real FFTs with the reference conventions: forward rfftn over the
  * last D axes, norm="backward"; inverse irfftn with 1/N^D scaling, Im of the
- * last axis' DC/Nyquist ignored. `count` transforms, contiguous.
+ * last axis' DC/Nyquist ignored. `count` transforms, contiguous. 1-D sizes
+ * as for plans: powers of two 16..4096, 48 and 50.
  * Replace exponax.fft / exponax.ifft (_spectral.py:614-656, 659-721). */
 int exmi_rfft(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x, float* x_hat,
               void* stream);
