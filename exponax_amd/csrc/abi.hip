@@ -285,6 +285,7 @@ struct exmi_plan {
   GraphCache gc;
   // 2-D plans
   int ndim = 1;
+  int cplx = 0;  // 2-D: complex linear operator (Im E follows the real coefficients)
   int nf = 0, kc = 0;
   Launch2D row2d[3] = {nullptr, nullptr, nullptr};
   Launch2D col2d[COL_NUM_MODES] = {};
@@ -331,6 +332,9 @@ static void fill_args2d(const exmi_plan* p, Step2DArgs* a) {
   a->zero_mode_fix = p->d.zero_mode_fix;
   a->inj_mode = p->inj_mode;
   a->inj_value = p->inj_value;
+  a->e_im = p->cplx ? num_coefs(p->d.order) : -1;
+  a->npoly = p->d.num_poly;
+  for (int i = 0; i < EXMI_MAX_POLY; ++i) a->poly[i] = (float)p->d.poly[i];
 }
 
 static void fill_args(const exmi_plan* p, Step1DArgs* a) {
@@ -389,8 +393,9 @@ static int kernel_kind(const exmi_plan_desc* d, int* kind) {
 int exmi_plan_destroy(exmi_plan* p);
 static int ensure_work(exmi_plan* p);
 
-// 2-D plans: ETDRK2 with the vorticity (Kolmogorov) or gradient-norm
-// nonlinearity, one channel, N a power of two in 32..512.
+// 2-D plans: ETDRK0-4 with the vorticity (Kolmogorov), gradient-norm or
+// polynomial nonlinearity (or none), one channel, N a power of two in
+// 32..512; complex linear operators for orders 0-2.
 static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   const int N = d->num_points;
   if (!pow2(N) || N < 32 || N > 512)
@@ -404,29 +409,42 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     return fail(EXMI_ERR_INVALID, "lin_batch must be 1 or batch (%d), got %d", d->batch, d->lin_batch);
   if (d->lin_channels != 1) return fail(EXMI_ERR_INVALID, "lin_channels must be 1");
   if (!d->host_linear_operator) return fail(EXMI_ERR_INVALID, "host_linear_operator is null");
-  if (d->host_linear_operator_imag)
-    return fail(EXMI_ERR_UNSUPPORTED, "complex linear operators are built for 1-D plans");
+  const bool cplx = d->host_linear_operator_imag != nullptr;
+  if (cplx && d->order > 2)
+    return fail(EXMI_ERR_UNSUPPORTED, "2-D plans with a complex linear operator are built for ETDRK orders 0-2");
   int kind = 0;
+  bool zero_poly = false;
   if (d->nonlin_kind == EXMI_NL_VORTICITY || d->nonlin_kind == EXMI_NL_KOLMOGOROV) {
     kind = NL_VORT;
   } else if (d->nonlin_kind == EXMI_NL_GRADIENT_NORM || d->order == 0) {
     kind = NL_GRADNORM;  // ETDRK0 evaluates no nonlinear term
+  } else if (d->nonlin_kind == EXMI_NL_POLYNOMIAL) {
+    if (d->num_poly < 0 || d->num_poly > EXMI_MAX_POLY)
+      return fail(EXMI_ERR_UNSUPPORTED, "at most %d polynomial coefficients", EXMI_MAX_POLY);
+    kind = NL_POLY;
+  } else if (d->nonlin_kind == EXMI_NL_ZERO) {
+    kind = NL_POLY;  // the empty polynomial: N = 0 exactly
+    zero_poly = true;
   } else {
-    return fail(EXMI_ERR_UNSUPPORTED, "2-D plans implement the vorticity and gradient-norm nonlinearities");
+    return fail(EXMI_ERR_UNSUPPORTED,
+                "2-D plans implement the vorticity, gradient-norm and polynomial nonlinearities (or none)");
   }
   exmi_plan* p = new exmi_plan();
   p->d = *d;
   p->d.host_linear_operator = nullptr;
+  p->d.host_linear_operator_imag = nullptr;
   p->ndim = 2;
   p->M = N / 2;
   const int NY = N / 2 + 1;
   p->nr = N * NY;
-  p->ncoef = num_coefs(d->order);
+  p->ncoef = num_coefs_c(d->order, cplx);
+  p->cplx = cplx;
   p->kind = kind;
+  if (zero_poly) p->d.num_poly = 0;
   p->kmax = dealias_kmax(N, d->dealiasing_fraction);
   p->kc = p->kmax + 1 < NY ? p->kmax + 1 : NY;
   if (p->kc < 1) p->kc = 1;
-  p->nf = kind == NL_VORT ? 4 : 2;
+  p->nf = kind == NL_VORT ? 4 : kind == NL_POLY ? 1 : 2;
   p->kscale = (float)(2.0 * 3.14159265358979323846 / d->domain_extent);
   p->roll_nt = -1;
   hipGetDevice(&p->device);
@@ -442,13 +460,13 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   }
   for (int m = 0; m < 3; ++m) p->row2d[m] = get_2d(N, kind, 1, m);
   for (int m = 0; m < COL_NUM_MODES; ++m) p->col2d[m] = get_2d(N, kind, 0, m);
-  for (int m = 0; m < COL_NUM_MODES; ++m) {
+  for (int m = 0; m < COL_FWD; ++m) {
     if ((m < 3 && !p->row2d[m]) || !p->col2d[m]) {
       delete p;
       return fail(EXMI_ERR_UNSUPPORTED, "no 2-D kernel for N=%d", N);
     }
   }
-  if (kind == NL_GRADNORM && d->order == 2) {
+  if (kind == NL_GRADNORM && d->order == 2 && !cplx) {
     for (int m = 0; m < 3; ++m) {
       p->rowA[m] = get_adj2d(N, 1, m);
       p->colA[m] = get_adj2d(N, 0, m);
@@ -464,13 +482,19 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   const long long rows = d->lin_batch;
   // L arrives as the reference's [kx][ky] table; the column kernels read
   // [ky][kx]
-  std::vector<double> Lt((size_t)(rows * full));
-  for (long long b = 0; b < rows; ++b)
-    for (int kx = 0; kx < N; ++kx)
-      for (int ky = 0; ky < NY; ++ky)
-        Lt[(size_t)(b * full + (long long)ky * N + kx)] = d->host_linear_operator[b * full + (long long)kx * NY + ky];
+  // [ky][kx]; with a complex L the imaginary part follows the real one
+  const int nparts = cplx ? 2 : 1;
+  std::vector<double> Lt((size_t)(nparts * rows * full));
+  for (int part = 0; part < nparts; ++part) {
+    const double* src = part ? d->host_linear_operator_imag : d->host_linear_operator;
+    double* dst = Lt.data() + (size_t)part * rows * full;
+    for (long long b = 0; b < rows; ++b)
+      for (int kx = 0; kx < N; ++kx)
+        for (int ky = 0; ky < NY; ++ky)
+          dst[(size_t)(b * full + (long long)ky * N + kx)] = src[b * full + (long long)kx * NY + ky];
+  }
   double* dL = nullptr;
-  bool ok = hipMalloc(&dL, sizeof(double) * rows * full) == hipSuccess &&
+  bool ok = hipMalloc(&dL, sizeof(double) * nparts * rows * full) == hipSuccess &&
             hipMalloc(&p->tables, sizeof(float) * rows * full * p->ncoef) == hipSuccess &&
             hipMalloc(&p->S0, sizeof(cf) * B * full) == hipSuccess &&
             hipMalloc(&p->S1, sizeof(cf) * B * full) == hipSuccess &&
@@ -483,11 +507,10 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     exmi_plan_destroy(p);
     return fail(EXMI_ERR_ALLOC, "2-D workspace allocation failed (batch %lld, N %d)", B, N);
   }
-  hipError_t e = hipMemcpy(dL, Lt.data(), sizeof(double) * rows * full, hipMemcpyHostToDevice);
+  hipError_t e = hipMemcpy(dL, Lt.data(), sizeof(double) * nparts * rows * full, hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = launch_coef(dL, nullptr, rows * full, (int)full, d->dt, d->order, d->num_circle_points, d->circle_radius,
-                    p->tables,
-                    nullptr);
+    e = launch_coef(dL, cplx ? dL + rows * full : nullptr, rows * full, (int)full, d->dt, d->order,
+                    d->num_circle_points, d->circle_radius, p->tables, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   hipFree(dL);
   if (e != hipSuccess) {
@@ -1147,7 +1170,53 @@ int exmi_repeat(exmi_plan* p, const float* u0, int64_t u0_bs, float* u_out, int3
   return run_maybe_graph(p, u0, u_out, u0_bs, n, 0, 2, use_graph, (hipStream_t)stream, enq_repeat_cb);
 }
 
+}  // extern "C"
+
+// 2-D transforms: the row kernel (r2c along y) and the column kernel (c2c
+// along x) of the 2-D steppers, through a stream-ordered [ky][x] buffer.
+static int transform2d(bool inverse, int N, int count, const float* x, cf* xh, hipStream_t st) {
+  if (!pow2(N) || N < 32 || N > 512)
+    return fail(EXMI_ERR_UNSUPPORTED, "2-D transforms: num_points=%d: supported sizes are powers of two 32..512", N);
+  if (count > 65535) return fail(EXMI_ERR_UNSUPPORTED, "2-D transforms: at most 65535 fields per call");
+  const Launch2D row = get_2d(N, NL_GRADNORM, 1, inverse ? ROW_RC : ROW_RA);
+  const Launch2D col = get_2d(N, NL_GRADNORM, 0, inverse ? COL_INV : COL_FWD);
+  if (!row || !col) return fail(EXMI_ERR_UNSUPPORTED, "no 2-D transform kernel for N=%d", N);
+  Twiddles tr, tc;
+  int rc = get_twiddles(N / 2, &tr);
+  if (!rc) rc = get_twiddles(N, &tc);
+  if (rc) return rc;
+  const long long full = (long long)(N / 2 + 1) * N;
+  Step2DArgs a;
+  memset(&a, 0, sizeof a);
+  a.tw_row = tr.tw;
+  a.fw_row = tr.fw;
+  a.tw_col = tc.tw;
+  a.kmax = N;  // no dealiasing
+  a.e_im = -1;
+  a.spec = xh;
+  a.spec_bstride = full;
+  HIP_TRY(hipMallocAsync((void**)&a.T, sizeof(cf) * full * count, st));
+  hipError_t e;
+  if (!inverse) {
+    a.u_in = x;
+    a.u_in_bstride = (long long)N * N;
+    e = row(a, count, st);
+    if (e == hipSuccess) e = col(a, count, st);
+  } else {
+    a.u_out = const_cast<float*>(x);
+    a.u_out_bstride = (long long)N * N;
+    e = col(a, count, st);
+    if (e == hipSuccess) e = row(a, count, st);
+  }
+  const hipError_t ef = hipFreeAsync(a.T, st);
+  if (e == hipSuccess) e = ef;
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "2-D transform launch failed: %s", hipGetErrorString(e));
+}
+
+extern "C" {
+
 int exmi_rfft(int32_t D, int32_t N, int32_t count, const float* x, float* x_hat, void* stream) {
+  if (D == 2) return count <= 0 ? EXMI_OK : transform2d(false, N, count, x, (cf*)x_hat, (hipStream_t)stream);
   if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "exmi_rfft: num_spatial_dims=%d not supported", D);
   if (!size_1d(N)) return fail(EXMI_ERR_UNSUPPORTED, "exmi_rfft: num_points=%d", N);
   if (count <= 0) return EXMI_OK;
@@ -1159,6 +1228,8 @@ int exmi_rfft(int32_t D, int32_t N, int32_t count, const float* x, float* x_hat,
 }
 
 int exmi_irfft(int32_t D, int32_t N, int32_t count, const float* x_hat, float* x, void* stream) {
+  if (D == 2)
+    return count <= 0 ? EXMI_OK : transform2d(true, N, count, x, (cf*)const_cast<float*>(x_hat), (hipStream_t)stream);
   if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "exmi_irfft: num_spatial_dims=%d not supported", D);
   if (!size_1d(N)) return fail(EXMI_ERR_UNSUPPORTED, "exmi_irfft: num_points=%d", N);
   if (count <= 0) return EXMI_OK;

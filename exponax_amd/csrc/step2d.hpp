@@ -29,8 +29,13 @@
 // the nonlinear coefficients absorb the powers of two (exact).
 //
 // Nonlinear functions: VorticityConvection2d(+Kolmogorov)
-// (nonlin_fun/_vorticity_convection.py:65-99, 178-182) and
-// GradientNormNonlinearFun in 2-D (nonlin_fun/_gradient_norm.py:84-101).
+// (nonlin_fun/_vorticity_convection.py:65-99, 178-182),
+// GradientNormNonlinearFun in 2-D (nonlin_fun/_gradient_norm.py:84-101) and
+// PolynomialNonlinearFun (nonlin_fun/_polynomial.py:64-76).
+//
+// The same row and column kernels also serve the standalone 2-D transforms:
+// rfftn is RA then COL_FWD (column FFT written in the reference's [kx][ky]
+// layout), irfftn is COL_INV (column inverse FFT from [kx][ky]) then RC.
 #pragma once
 #include "exmi_internal.hpp"
 #include "fft1d.hpp"
@@ -73,6 +78,11 @@ struct Step2DArgs {
   int inj_mode;              // Kolmogorov forcing wavenumber (ky index); -1: none
   float inj_value;           // forcing value at (kx = 0, ky = inj_mode), reference scaling
   int write_T;               // RC: also r2c the new state into T (rollout)
+  int e_im;                  // coefficient index of Im E (complex L, orders 0-2); -1: real L
+  int npoly;                 // polynomial coefficients c_0..c_{npoly-1}
+  float poly[EXMI_MAX_POLY];
+  cf* spec;                  // standalone transforms: spectra (ninst, N, N/2+1), [kx][ky]
+  long long spec_bstride;    // complex entries between instances
 };
 
 // CA/CB/CC: the ETDRK2 stages; C0: ETDRK0 (S' = E S); C1: ETDRK1's update
@@ -84,13 +94,14 @@ struct Step2DArgs {
 enum Col2DMode {
   COL_CA = 0, COL_CB = 1, COL_CC = 2, COL_C0 = 3, COL_C1 = 4,
   COL_C3A = 5, COL_C3B = 6, COL_C3C = 7, COL_C4A = 8, COL_C4B = 9, COL_C4C = 10, COL_C4D = 11,
-  COL_NUM_MODES = 12
+  COL_FWD = 12, COL_INV = 13,
+  COL_NUM_MODES = 14
 };
 enum Row2DMode { ROW_RA = 0, ROW_RB = 1, ROW_RC = 2 };
 
 // Nonlinear fields of one evaluation. Vorticity: (u, omega_x, v, omega_y),
 // product u omega_x + v omega_y. Gradient norm: (u_x, u_y), product
-// u_x^2 + u_y^2.
+// u_x^2 + u_y^2. Polynomial: (u), product sum_i c_i u^i.
 template <int KIND>
 struct Fields2D;
 template <>
@@ -100,6 +111,10 @@ struct Fields2D<NL_VORT> {
 template <>
 struct Fields2D<NL_GRADNORM> {
   static constexpr int NF = 2;
+};
+template <>
+struct Fields2D<NL_POLY> {
+  static constexpr int NF = 1;
 };
 
 // ------------------------------------------------------------ row kernels
@@ -239,8 +254,22 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
             P[s] = cf{fmaf(A[s].re, q.re, P[s].re), fmaf(A[s].im, q.im, P[s].im)};
           }
         }
+      } else if constexpr (KIND == NL_POLY) {
+        // the same power loop as the reference, on u = (2 N^2 u) / (2 N^2)
+#pragma unroll
+        for (int s = 0; s < PT; ++s) {
+          const float ur = z[0][s].re * rn2, ui = z[0][s].im * rn2;
+          float pr = 1.f, pi = 1.f, ar = 0.f, ai = 0.f;
+          for (int i = 0; i < a.npoly; ++i) {
+            ar += a.poly[i] * pr;
+            ai += a.poly[i] * pi;
+            pr *= ur;
+            pi *= ui;
+          }
+          P[s] = cf{ar, ai};
+        }
       } else {
-        static_assert(KIND == NL_VORT || KIND == NL_GRADNORM, "unsupported 2-D nonlinear kind");
+        static_assert(KIND == NL_VORT || KIND == NL_GRADNORM || KIND == NL_POLY, "unsupported 2-D nonlinear kind");
 #pragma unroll
         for (int s = 0; s < PT; ++s) {
           const cf q = z[0][s];
@@ -347,7 +376,9 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
 #pragma unroll
       for (int s = 0; s < PT; ++s) v[0][s] = p[j + s * T];
       cfft<N, PT, false, 1>(v, ln, lds, tr);
-      const float cs = (KIND == NL_VORT) ? -a.scale * q2 : -a.scale * 0.5f * q2;
+      // the polynomial product is formed at unit scale, so its forward
+      // transforms give 2 N_hat directly (no sign: nonlin_fun/_polynomial.py:76)
+      const float cs = (KIND == NL_VORT) ? -a.scale * q2 : (KIND == NL_POLY) ? 1.f : -a.scale * 0.5f * q2;
 #pragma unroll
       for (int s = 0; s < PT; ++s) {
         Nh[s] = keep_x[s] ? cs * v[0][s] : cf{0.f, 0.f};
@@ -379,6 +410,8 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
         v[1][s] = cf{-kx * w.im, kx * w.re};
         v[2][s] = cf{kx * psi.im, -kx * psi.re};
         v[3][s] = cf{-kyv * w.im, kyv * w.re};
+      } else if constexpr (KIND == NL_POLY) {
+        v[0][s] = w;
       } else {
         v[0][s] = cf{-kx * w.im, kx * w.re};
         v[1][s] = cf{-kyv * w.im, kyv * w.re};
@@ -396,6 +429,13 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
 
   const float* tb = a.tables + (long long)(a.lin_batch > 1 ? inst : 0) * a.ncoef * full;
   auto coef = [&](int i, int s) { return tb[(long long)i * full + (long long)ky * N + j + s * T]; };
+  // E x with E = exp(L dt), complex when L is (odd derivatives)
+  auto emul = [&](int s, cf x) -> cf {
+    const float er = coef(0, s);
+    if (a.e_im < 0) return er * x;
+    const float ei = coef(a.e_im, s);
+    return cf{fmaf(er, x.re, -ei * x.im), fmaf(er, x.im, ei * x.re)};
+  };
 
   if constexpr (MODE == COL_CA) {
     cf v[1][PT];
@@ -414,7 +454,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     nonlinear(Nh);
     const cf* s0 = a.S0 + (long long)inst * full + (long long)ky * N;
 #pragma unroll
-    for (int s = 0; s < PT; ++s) S[s] = coef(0, s) * s0[j + s * T] + coef(1, s) * Nh[s];
+    for (int s = 0; s < PT; ++s) S[s] = emul(s, s0[j + s * T]) + coef(1, s) * Nh[s];
     if (active) {
       cf* s1 = a.S1 + (long long)inst * full + (long long)ky * N;
 #pragma unroll
@@ -513,14 +553,39 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
       for (int s = 0; s < PT; ++s) v[0][s] = t[j + s * T];
       cfft<N, PT, false, 1>(v, ln, lds, tr);
 #pragma unroll
-      for (int s = 0; s < PT; ++s) v[0][s] = coef(0, s) * v[0][s];
+      for (int s = 0; s < PT; ++s) v[0][s] = emul(s, v[0][s]);
     } else {
       cf Nh[PT];
       nonlinear(Nh);
       const cf* s0 = a.S0 + (long long)inst * full + (long long)ky * N;
 #pragma unroll
-      for (int s = 0; s < PT; ++s) v[0][s] = coef(0, s) * s0[j + s * T] + coef(1, s) * Nh[s];
+      for (int s = 0; s < PT; ++s) v[0][s] = emul(s, s0[j + s * T]) + coef(1, s) * Nh[s];
     }
+    cfft<N, PT, true, 1>(v, ln, lds, tr);
+    if (active) {
+      cf* t = a.T + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
+    }
+  } else if constexpr (MODE == COL_FWD) {
+    // rfftn: the row spectra of column ky -> X[kx][ky] (rows gave 2 X)
+    cf v[1][PT];
+    const cf* t = a.T + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+    for (int s = 0; s < PT; ++s) v[0][s] = t[j + s * T];
+    cfft<N, PT, false, 1>(v, ln, lds, tr);
+    if (active) {
+      cf* o = a.spec + (long long)inst * a.spec_bstride + ky;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) o[(long long)(j + s * T) * C::NY] = 0.5f * v[0][s];
+    }
+  } else if constexpr (MODE == COL_INV) {
+    // irfftn: X[kx][ky] -> inverse column transform of 2 X -> T (RC then
+    // finishes with the row c2r and the 1/(2 N^2) scaling)
+    cf v[1][PT];
+    const cf* in = a.spec + (long long)inst * a.spec_bstride + ky;
+#pragma unroll
+    for (int s = 0; s < PT; ++s) v[0][s] = 2.f * in[(long long)(j + s * T) * C::NY];
     cfft<N, PT, true, 1>(v, ln, lds, tr);
     if (active) {
       cf* t = a.T + (long long)inst * full + (long long)ky * N;

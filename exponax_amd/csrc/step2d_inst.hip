@@ -37,6 +37,9 @@ static Launch2D pick2d(int is_row, int mode) {
       case COL_C4B: return &launch_col2d<N, COL_C4B, KIND>;
       case COL_C4C: return &launch_col2d<N, COL_C4C, KIND>;
       case COL_C4D: return &launch_col2d<N, COL_C4D, KIND>;
+      // the standalone transforms do not depend on the nonlinear kind
+      case COL_FWD: return KIND == NL_GRADNORM ? &launch_col2d<N, COL_FWD, NL_GRADNORM> : nullptr;
+      case COL_INV: return KIND == NL_GRADNORM ? &launch_col2d<N, COL_INV, NL_GRADNORM> : nullptr;
     }
   }
   return nullptr;
@@ -67,6 +70,7 @@ Launch2D EXMI_CAT(get_2d_, EXMI_N2)(int kind, int is_row, int mode) {
   constexpr int N = EXMI_N2;
   if (kind == NL_VORT) return pick2d<N, NL_VORT>(is_row, mode);
   if (kind == NL_GRADNORM) return pick2d<N, NL_GRADNORM>(is_row, mode);
+  if (kind == NL_POLY) return pick2d<N, NL_POLY>(is_row, mode);
   return nullptr;
 }
 

@@ -105,15 +105,18 @@ def fft(field, *, num_spatial_dims=None):
     t, back = as_device_tensor(field)
     if num_spatial_dims is None:
         num_spatial_dims = t.dim() - 1
-    if num_spatial_dims != 1:
-        raise NotImplementedError("fft: only num_spatial_dims == 1 is built")
+    if num_spatial_dims not in (1, 2):
+        raise NotImplementedError("fft: num_spatial_dims 1 and 2 are built")
     import torch
 
     n = t.shape[-1]
+    if num_spatial_dims == 2 and t.shape[-2] != n:
+        raise ValueError(f"expected a square grid on the last two axes, got {tuple(t.shape[-2:])}")
     x = t.contiguous().to(torch.float32)
-    count = x.numel() // n
+    count = x.numel() // n ** num_spatial_dims
     out = torch.empty(x.shape[:-1] + (n // 2 + 1,), dtype=torch.complex64, device=x.device)
-    _lib.check(_lib.lib().exmi_rfft(1, n, count, x.data_ptr(), out.data_ptr(), _lib.stream_ptr()), "exmi_rfft")
+    _lib.check(_lib.lib().exmi_rfft(num_spatial_dims, n, count, x.data_ptr(), out.data_ptr(), _lib.stream_ptr()),
+               "exmi_rfft")
     return restore_like(out, back)
 
 
@@ -127,17 +130,19 @@ def ifft(field_hat, *, num_spatial_dims=None, num_points=None):
             num_points = t.shape[-2]
         else:
             raise ValueError("num_points must be provided if num_spatial_dims == 1.")
-    if num_spatial_dims != 1:
-        raise NotImplementedError("ifft: only num_spatial_dims == 1 is built")
+    if num_spatial_dims not in (1, 2):
+        raise NotImplementedError("ifft: num_spatial_dims 1 and 2 are built")
     import torch
 
     if t.shape[-1] != num_points // 2 + 1:
         raise ValueError(f"expected {num_points // 2 + 1} modes on the last axis, got {t.shape[-1]}")
+    if num_spatial_dims == 2 and t.shape[-2] != num_points:
+        raise ValueError(f"expected {num_points} modes on the second-to-last axis, got {t.shape[-2]}")
     z = t.contiguous().to(torch.complex64)
-    count = z.numel() // z.shape[-1]
+    count = z.numel() // (z.shape[-1] * (num_points if num_spatial_dims == 2 else 1))
     out = torch.empty(z.shape[:-1] + (num_points,), dtype=torch.float32, device=z.device)
-    _lib.check(_lib.lib().exmi_irfft(1, num_points, count, z.data_ptr(), out.data_ptr(), _lib.stream_ptr()),
-               "exmi_irfft")
+    _lib.check(_lib.lib().exmi_irfft(num_spatial_dims, num_points, count, z.data_ptr(), out.data_ptr(),
+                                     _lib.stream_ptr()), "exmi_irfft")
     return restore_like(out, back)
 
 

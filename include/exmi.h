@@ -77,7 +77,10 @@ enum exmi_nonlin_kind {
  */
 typedef struct exmi_plan_desc {
   int32_t num_spatial_dims;   /* D: 1 or 2                                   */
-  int32_t num_points;         /* N: 1-D powers of two 16..4096, 48 and 50    */
+  int32_t num_points;         /* N: 1-D powers of two 16..4096, 48 and 50;
+                                 2-D powers of two 32..512 (one channel;
+                                 vorticity, gradient-norm, polynomial or no
+                                 nonlinearity)                               */
   int32_t num_channels;       /* C                                           */
   int32_t batch;              /* B                                           */
   int32_t order;              /* ETDRK order 0..4                            */
@@ -102,8 +105,9 @@ typedef struct exmi_plan_desc {
   const double* host_linear_operator; /* (lin_batch, lin_channels, Nr) f64  */
   /* NULL for a real L; else the imaginary parts of L, same shape. A complex L
    * (odd derivatives: advection, dispersion; stepper/_advection.py:88-95,
-   * _dispersion.py:102-117, _korteweg_de_vries.py) is built for 1-D plans of
-   * ETDRK orders 0-2, N <= 1024, without EXMI_FLAG_FOURIER_RESIDENT. exp(L dt)
+   * _dispersion.py:102-117, _korteweg_de_vries.py) is built for ETDRK orders
+   * 0-2: 1-D plans with N <= 1024 and without EXMI_FLAG_FOURIER_RESIDENT, and
+   * 2-D plans. exp(L dt)
    * is then complex while the phi-coefficients stay real: the reference keeps
    * the real part of every contour sample (etdrk/_etdrk_1.py:66-76,
    * _etdrk_2.py:71-89). The table layout gains Im E after the real
@@ -210,8 +214,11 @@ int exmi_rollout_vjp(exmi_plan* plan, const float* u0, int64_t u0_bstride, int32
 
 /* Batched real FFTs with the reference conventions: forward rfftn over the
  * last D axes, norm="backward"; inverse irfftn with 1/N^D scaling, Im of the
- * last axis' DC/Nyquist ignored. `count` transforms, contiguous. 1-D sizes
- * as for plans: powers of two 16..4096, 48 and 50.
+ * last axis' DC/Nyquist ignored. `count` transforms, contiguous: real
+ * (count, N^D), spectra (count, N^(D-1), N/2+1) in the reference's axis
+ * order. D = 1: powers of two 16..4096, 48 and 50. D = 2: powers of two
+ * 32..512, count <= 65535, through a stream-ordered (hipMallocAsync)
+ * intermediate of the spectra's size.
  * Replace exponax.fft / exponax.ifft (_spectral.py:614-656, 659-721). */
 int exmi_rfft(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x, float* x_hat,
               void* stream);

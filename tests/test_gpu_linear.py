@@ -77,8 +77,9 @@ def test_complex_operator_limits(device):
     library's message."""
     import torch
 
-    with pytest.raises(NotImplementedError):
-        ex.stepper.Advection(2, 1.0, 32, 0.01).step(torch.zeros((1, 32, 32), device=device))
+    with pytest.raises(NotImplementedError):  # 2-D complex L: ETDRK orders 0-2
+        ex.stepper.generic.GeneralPolynomialStepper(2, 1.0, 32, 0.01, linear_coefficients=(0.0, 0.5, 0.01),
+                                                    order=3).step(torch.zeros((1, 32, 32), device=device))
     with pytest.raises(NotImplementedError):
         ex.stepper.Advection(1, 1.0, 2048, 0.01).step(torch.zeros((1, 2048), device=device))
     with pytest.raises(NotImplementedError):

@@ -257,3 +257,38 @@ def test_swift_hohenberg_uniform():
     got = _uniform_run(st.SwiftHohenberg(1, 10.0, 32, dt), u0, n)  # u' = (0.7 - 1) u + u^2 - u^3
     ref = solve_ivp(lambda t, u: -0.3 * u + u**2 - u**3, (0.0, n * dt), [u0], rtol=1e-12, atol=1e-14).y[0, -1]
     assert abs(got - ref) < 2e-5
+
+
+# ---- 2-D: the same closed forms on a 2-D grid (tests/test_validation.py:13-211
+# covers advection and diffusion in 1/2/3-D)
+def _grid2(L, N):
+    x = _grid(L, N)
+    return np.meshgrid(x, x, indexing="ij")
+
+
+def test_advection_2d_exact_shift():
+    L, N, dt, c = 2.0, 32, 0.1, (0.7, -0.4)
+    X, Y = _grid2(L, N)
+    f = lambda x, y: np.sin(2 * np.pi * (3 * x + 2 * y) / L)  # noqa: E731
+    u = st.Advection(2, L, N, dt, velocity=np.array(c)).step(f(X, Y)[None])
+    np.testing.assert_allclose(u[0], f(X - c[0] * dt, Y - c[1] * dt), atol=1e-12)
+
+
+def test_diffusion_2d_exact_decay():
+    L, N, dt, nu = 3.0, 32, 0.2, 0.05
+    X, Y = _grid2(L, N)
+    kx, ky = 2 * np.pi * 3 / L, 2 * np.pi * 2 / L
+    u0 = np.cos(kx * X) * np.sin(ky * Y)
+    u = st.Diffusion(2, L, N, dt, diffusivity=nu).step(u0[None])
+    np.testing.assert_allclose(u[0], np.exp(-nu * (kx**2 + ky**2) * dt) * u0, atol=1e-12)
+
+
+def test_allen_cahn_2d_uniform():
+    u0, dt, n = 0.2, 0.01, 100
+    ac = st.AllenCahn(2, 1.0, 16, dt)
+    u = np.full((1, 16, 16), u0)
+    for _ in range(n):
+        u = ac.step(u)
+    assert np.ptp(u) < 1e-12
+    t = n * dt
+    assert abs(u[0, 0, 0] - u0 * np.exp(t) / np.sqrt(1 + u0**2 * (np.exp(2 * t) - 1))) < 2e-5
