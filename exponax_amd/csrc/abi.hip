@@ -287,6 +287,8 @@ struct exmi_plan {
   int ndim = 1;
   int cplx = 0;  // 2-D: complex linear operator (Im E follows the real coefficients)
   int nf = 0, kc = 0;
+  int ninst = 0;          // 2-D: instances = batch * channels
+  long long tstride = 0;  // 2-D: complex entries of T per instance
   Launch2D row2d[3] = {nullptr, nullptr, nullptr};
   Launch2D col2d[COL_NUM_MODES] = {};
   Twiddles tw_col;
@@ -332,6 +334,11 @@ static void fill_args2d(const exmi_plan* p, Step2DArgs* a) {
   a->zero_mode_fix = p->d.zero_mode_fix;
   a->inj_mode = p->inj_mode;
   a->inj_value = p->inj_value;
+  a->channels = p->d.num_channels;
+  a->tstride = p->tstride;
+  a->g0 = (float)p->d.general_scales[0];
+  a->g1 = (float)p->d.general_scales[1];
+  a->g2 = (float)p->d.general_scales[2];
   a->e_im = p->cplx ? num_coefs(p->d.order) : -1;
   a->npoly = p->d.num_poly;
   for (int i = 0; i < EXMI_MAX_POLY; ++i) a->poly[i] = (float)p->d.poly[i];
@@ -393,14 +400,15 @@ static int kernel_kind(const exmi_plan_desc* d, int* kind) {
 int exmi_plan_destroy(exmi_plan* p);
 static int ensure_work(exmi_plan* p);
 
-// 2-D plans: ETDRK0-4 with the vorticity (Kolmogorov), gradient-norm or
-// polynomial nonlinearity (or none), one channel, N a power of two in
-// 32..512; complex linear operators for orders 0-2.
+// 2-D plans: ETDRK0-4 with the vorticity (Kolmogorov), gradient-norm,
+// polynomial, two-channel convection or general nonlinearity (or none), N a
+// power of two in 32..512; complex linear operators for orders 0-2.
 static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   const int N = d->num_points;
+  const int C = d->num_channels;
   if (!pow2(N) || N < 32 || N > 512)
     return fail(EXMI_ERR_UNSUPPORTED, "2-D num_points=%d: supported sizes are powers of two 32..512", N);
-  if (d->num_channels != 1) return fail(EXMI_ERR_UNSUPPORTED, "2-D plans need num_channels == 1");
+  if (C < 1) return fail(EXMI_ERR_INVALID, "num_channels must be >= 1");
   if (d->batch < 1) return fail(EXMI_ERR_INVALID, "batch must be >= 1");
   if (d->order < 0 || d->order > 4) return fail(EXMI_ERR_UNSUPPORTED, "Order %d not implemented.", d->order);
   if (d->flags & EXMI_FLAG_FOURIER_RESIDENT)
@@ -413,9 +421,25 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   if (cplx && d->order > 2)
     return fail(EXMI_ERR_UNSUPPORTED, "2-D plans with a complex linear operator are built for ETDRK orders 0-2");
   int kind = 0;
-  bool zero_poly = false;
+  bool zero_poly = false, conv_general = false;
   if (d->nonlin_kind == EXMI_NL_VORTICITY || d->nonlin_kind == EXMI_NL_KOLMOGOROV) {
+    if (C != 1) return fail(EXMI_ERR_INVALID, "vorticity plans need num_channels == 1");
     kind = NL_VORT;
+  } else if (d->nonlin_kind == EXMI_NL_CONVECTION && d->order > 0) {
+    if (d->conv_single_channel) {
+      if (C != 1) return fail(EXMI_ERR_UNSUPPORTED, "single-channel convection needs num_channels == 1");
+      if (!d->conv_conservative)
+        return fail(EXMI_ERR_UNSUPPORTED, "2-D single-channel convection is built in the conservative form");
+      kind = NL_GENERAL;  // -b/2 (sum_d i kappa_d) F[u^2] is the general form with scales (0, -b, 0)
+      conv_general = true;
+    } else {
+      if (C != 2)  // nonlin_fun/_convection.py:125-129
+        return fail(EXMI_ERR_INVALID, "Number of channels in u_hat should match number of spatial dimensions");
+      kind = d->conv_conservative ? NL_CONV_CONS : NL_CONV_NONCONS;
+    }
+  } else if (d->nonlin_kind == EXMI_NL_GENERAL && d->order > 0) {
+    if (C != 1) return fail(EXMI_ERR_UNSUPPORTED, "general nonlinear function needs num_channels == 1");
+    kind = NL_GENERAL;
   } else if (d->nonlin_kind == EXMI_NL_GRADIENT_NORM || d->order == 0) {
     kind = NL_GRADNORM;  // ETDRK0 evaluates no nonlinear term
   } else if (d->nonlin_kind == EXMI_NL_POLYNOMIAL) {
@@ -426,8 +450,7 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     kind = NL_POLY;  // the empty polynomial: N = 0 exactly
     zero_poly = true;
   } else {
-    return fail(EXMI_ERR_UNSUPPORTED,
-                "2-D plans implement the vorticity, gradient-norm and polynomial nonlinearities (or none)");
+    return fail(EXMI_ERR_UNSUPPORTED, "unknown 2-D nonlinear kind %d", d->nonlin_kind);
   }
   exmi_plan* p = new exmi_plan();
   p->d = *d;
@@ -441,10 +464,21 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   p->cplx = cplx;
   p->kind = kind;
   if (zero_poly) p->d.num_poly = 0;
+  if (conv_general) {
+    p->d.general_scales[0] = 0.0;
+    p->d.general_scales[1] = -d->scale;
+    p->d.general_scales[2] = 0.0;
+  }
   p->kmax = dealias_kmax(N, d->dealiasing_fraction);
   p->kc = p->kmax + 1 < NY ? p->kmax + 1 : NY;
   if (p->kc < 1) p->kc = 1;
-  p->nf = kind == NL_VORT ? 4 : kind == NL_POLY ? 1 : 2;
+  {
+    const int np = (kind == NL_CONV_CONS || kind == NL_GENERAL) ? 2 : 1;
+    p->nf = kind == NL_VORT ? 4 : kind == NL_GRADNORM ? 2 : (kind == NL_POLY || kind == NL_CONV_CONS) ? 1 : 3;
+    const long long fullc = (long long)NY * N, prod = (long long)np * p->kc * N;
+    p->tstride = fullc > prod ? fullc : prod;
+    p->ninst = d->batch * C;
+  }
   p->kscale = (float)(2.0 * 3.14159265358979323846 / d->domain_extent);
   p->roll_nt = -1;
   hipGetDevice(&p->device);
@@ -466,7 +500,7 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
       return fail(EXMI_ERR_UNSUPPORTED, "no 2-D kernel for N=%d", N);
     }
   }
-  if (kind == NL_GRADNORM && d->order == 2 && !cplx) {
+  if (kind == NL_GRADNORM && d->order == 2 && !cplx && C == 1) {
     for (int m = 0; m < 3; ++m) {
       p->rowA[m] = get_adj2d(N, 1, m);
       p->colA[m] = get_adj2d(N, 0, m);
@@ -478,7 +512,7 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     delete p;
     return rc;
   }
-  const long long B = d->batch, full = (long long)NY * N;
+  const long long B = p->ninst, full = (long long)NY * N;  // allocations per instance
   const long long rows = d->lin_batch;
   // L arrives as the reference's [kx][ky] table; the column kernels read
   // [ky][kx]
@@ -498,7 +532,7 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
             hipMalloc(&p->tables, sizeof(float) * rows * full * p->ncoef) == hipSuccess &&
             hipMalloc(&p->S0, sizeof(cf) * B * full) == hipSuccess &&
             hipMalloc(&p->S1, sizeof(cf) * B * full) == hipSuccess &&
-            hipMalloc(&p->T2, sizeof(cf) * B * full) == hipSuccess &&
+            hipMalloc(&p->T2, sizeof(cf) * B * p->tstride) == hipSuccess &&
             hipMalloc(&p->N0, sizeof(cf) * B * p->kc * N) == hipSuccess &&
             hipMalloc(&p->F, sizeof(cf) * B * p->nf * p->kc * N) == hipSuccess &&
             (d->order < 3 || hipMalloc(&p->ACC, sizeof(cf) * B * full) == hipSuccess);
@@ -538,7 +572,7 @@ static int enqueue_step2d(exmi_plan* p, const float* u_in, long long ibs, float*
   a.u_out = u_out;
   a.u_out_bstride = obs;
   a.write_T = write_T ? 1 : 0;
-  const int B = p->d.batch;
+  const int B = p->ninst;
   hipError_t e = hipSuccess;
   if (!have_T) e = p->row2d[ROW_RA](a, B, st);
   if (p->d.order == 0) {
@@ -1193,6 +1227,8 @@ static int transform2d(bool inverse, int N, int count, const float* x, cf* xh, h
   a.tw_col = tc.tw;
   a.kmax = N;  // no dealiasing
   a.e_im = -1;
+  a.channels = 1;  // one instance per field
+  a.tstride = full;
   a.spec = xh;
   a.spec_bstride = full;
   HIP_TRY(hipMallocAsync((void**)&a.T, sizeof(cf) * full * count, st));

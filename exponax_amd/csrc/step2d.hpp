@@ -30,8 +30,16 @@
 //
 // Nonlinear functions: VorticityConvection2d(+Kolmogorov)
 // (nonlin_fun/_vorticity_convection.py:65-99, 178-182),
-// GradientNormNonlinearFun in 2-D (nonlin_fun/_gradient_norm.py:84-101) and
-// PolynomialNonlinearFun (nonlin_fun/_polynomial.py:64-76).
+// GradientNormNonlinearFun in 2-D (nonlin_fun/_gradient_norm.py:84-101),
+// PolynomialNonlinearFun (nonlin_fun/_polynomial.py:64-76), the two-channel
+// convections of a velocity field (nonlin_fun/_convection.py:104-171) and
+// GeneralNonlinearFun (nonlin_fun/_general_nonlinear.py:89-119).
+//
+// Channels: every (trajectory, channel) pair is an instance of its own with
+// its own spectral state; instance i is channel i % C of trajectory i / C.
+// The convections couple the channels only through the row kernel's
+// products, which read the velocity fields of all channels of the
+// trajectory.
 //
 // The same row and column kernels also serve the standalone 2-D transforms:
 // rfftn is RA then COL_FWD (column FFT written in the reference's [kx][ky]
@@ -55,7 +63,10 @@
 namespace exmi {
 
 struct Step2DArgs {
-  const float* u_in;         // physical (ninst, N, N)
+  int channels;              // C: instances per trajectory
+  long long tstride;         // complex entries of T per instance (>= Ny N and the products' NP Kc N)
+  float g0, g1, g2;          // GeneralNonlinearFun scales, signs folded in
+  const float* u_in;         // physical (batch, C, N, N) with a batch stride
   long long u_in_bstride;    // floats between instances
   float* u_out;              // physical output of RC
   long long u_out_bstride;
@@ -104,17 +115,22 @@ enum Row2DMode { ROW_RA = 0, ROW_RB = 1, ROW_RC = 2 };
 // u_x^2 + u_y^2. Polynomial: (u), product sum_i c_i u^i.
 template <int KIND>
 struct Fields2D;
-template <>
-struct Fields2D<NL_VORT> {
-  static constexpr int NF = 4;
-};
-template <>
-struct Fields2D<NL_GRADNORM> {
-  static constexpr int NF = 2;
-};
-template <>
-struct Fields2D<NL_POLY> {
-  static constexpr int NF = 1;
+// NF fields leave a column kernel per instance; a row kernel transforms NIN
+// fields back and forms NP products.
+// Two-channel conservative convection: field u_c; inputs (u_0, u_1) of the
+// trajectory; products u_c u_0, u_c u_1 (the terms of 1/2 sum_j i kappa_j
+// F[u_c u_j]). Non-conservative: fields (u_c, d_x u_c, d_y u_c); inputs
+// (u_0, u_1, d_x u_c, d_y u_c); product u_0 d_x u_c + u_1 d_y u_c. General:
+// fields (u, u_x, u_y); products u^2 and u_x^2 + u_y^2.
+template <int KIND>
+struct Fields2D {
+  static constexpr int NF = KIND == NL_VORT ? 4 : KIND == NL_GRADNORM ? 2 : KIND == NL_POLY ? 1
+                            : KIND == NL_CONV_CONS ? 1 : 3;
+  static constexpr int NIN = KIND == NL_CONV_CONS ? 2 : KIND == NL_CONV_NONCONS ? 4 : NF;
+  static constexpr int NP = (KIND == NL_CONV_CONS || KIND == NL_GENERAL) ? 2 : 1;
+  static_assert(KIND == NL_VORT || KIND == NL_GRADNORM || KIND == NL_POLY || KIND == NL_CONV_CONS ||
+                    KIND == NL_CONV_NONCONS || KIND == NL_GENERAL,
+                "unsupported 2-D nonlinear kind");
 };
 
 // ------------------------------------------------------------ row kernels
@@ -210,15 +226,18 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
   const Lane<M, PT> ln(j);
   TwRegs<M, PT> tr;
   tr.load(j, RealTables{a.tw_row, a.fw_row});
-  const long long fstride = (long long)C::NY * N;  // one full [ky][x] field
+  const long long fstride = a.tstride;  // T of one instance
   const int kc = a.kmax + 1 < C::NY ? a.kmax + 1 : C::NY;
+  // physical row x of this instance in a (batch, C, N, N) state
+  const long long urow = (long long)(inst % a.channels) * N * N + (long long)x * N;
+  const int traj = inst / a.channels;
   constexpr float NN = (float)N;
   constexpr float rn2 = 1.f / (2.f * NN * NN);  // inverse of 2 u_hat gives 2 N^2 u
 
   cf z[1][PT];
   cf nyq[1];
   if constexpr (MODE == ROW_RA) {
-    const float2* u = reinterpret_cast<const float2*>(a.u_in + (long long)inst * a.u_in_bstride + (long long)x * N);
+    const float2* u = reinterpret_cast<const float2*>(a.u_in + (long long)traj * a.u_in_bstride + urow);
 #pragma unroll
     for (int s = 0; s < PT; ++s) {
       const float2 v = u[j + s * T];
@@ -229,29 +248,86 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
     __syncthreads();
     tile_store<N>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
   } else if constexpr (MODE == ROW_RB) {
-    constexpr int NF = Fields2D<KIND>::NF;
-    const cf* F = a.F + (long long)inst * NF * kc * N;
-    cf A[PT], P[PT];
+    using FD = Fields2D<KIND>;
+    constexpr int NF = FD::NF, NIN = FD::NIN, NP = FD::NP;
+    const long long fsz = (long long)kc * N;
+    const int ch = inst % a.channels;
+    // input f: this instance's field f, or for the convections the velocity
+    // field u_j of channel j of the same trajectory
+    auto src = [&](int f) -> const cf* {
+      int i = inst, fld = f;
+      if constexpr (KIND == NL_CONV_CONS) {
+        i = inst - ch + f;
+        fld = 0;
+      } else if constexpr (KIND == NL_CONV_NONCONS) {
+        if (f < 2) {
+          i = inst - ch + f;
+          fld = 0;
+        } else {
+          fld = f - 1;
+        }
+      }
+      return a.F + ((long long)i * NF + fld) * fsz;
+    };
+    cf A[PT], B2[PT], P[NP][PT];
     TileRegs<N> pf;
-    pf.load(F, x0, kc);
+    pf.load(src(0), x0, kc);
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
+    for (int f = 0; f < NIN; ++f) {
       if (f > 0) __syncthreads();
       pf.store(lds_t);
       __syncthreads();
-      if (f + 1 < NF) pf.load(F + (long long)(f + 1) * kc * N, x0, kc);  // overlaps this field's transform
+      if (f + 1 < NIN) pf.load(src(f + 1), x0, kc);  // overlaps this field's transform
       tile_to_row<N>(lds_t, r, j, z, nyq);
       irfft<M, PT, 1>(z, nyq, ln, lds, tr);
-      if constexpr (KIND == NL_VORT) {
+      if constexpr (KIND == NL_CONV_CONS) {
+#pragma unroll
+        for (int s = 0; s < PT; ++s) {
+          const cf q = z[0][s];
+          if (f == 0) {
+            A[s] = q;
+          } else {
+            const cf uc = ch == 0 ? A[s] : q;
+            P[0][s] = cf{uc.re * A[s].re, uc.im * A[s].im};
+            P[1][s] = cf{uc.re * q.re, uc.im * q.im};
+          }
+        }
+      } else if constexpr (KIND == NL_CONV_NONCONS) {
+#pragma unroll
+        for (int s = 0; s < PT; ++s) {
+          const cf q = z[0][s];
+          if (f == 0) {
+            A[s] = q;
+          } else if (f == 1) {
+            B2[s] = q;
+          } else if (f == 2) {
+            P[0][s] = cf{A[s].re * q.re, A[s].im * q.im};
+          } else {
+            P[0][s] = cf{fmaf(B2[s].re, q.re, P[0][s].re), fmaf(B2[s].im, q.im, P[0][s].im)};
+          }
+        }
+      } else if constexpr (KIND == NL_GENERAL) {
+#pragma unroll
+        for (int s = 0; s < PT; ++s) {
+          const cf q = z[0][s];
+          if (f == 0) {
+            P[0][s] = cf{q.re * q.re, q.im * q.im};
+          } else if (f == 1) {
+            P[1][s] = cf{q.re * q.re, q.im * q.im};
+          } else {
+            P[1][s] = cf{fmaf(q.re, q.re, P[1][s].re), fmaf(q.im, q.im, P[1][s].im)};
+          }
+        }
+      } else if constexpr (KIND == NL_VORT) {
 #pragma unroll
         for (int s = 0; s < PT; ++s) {
           const cf q = z[0][s];
           if (f == 0 || f == 2) {
             A[s] = q;
           } else if (f == 1) {
-            P[s] = cf{A[s].re * q.re, A[s].im * q.im};
+            P[0][s] = cf{A[s].re * q.re, A[s].im * q.im};
           } else {
-            P[s] = cf{fmaf(A[s].re, q.re, P[s].re), fmaf(A[s].im, q.im, P[s].im)};
+            P[0][s] = cf{fmaf(A[s].re, q.re, P[0][s].re), fmaf(A[s].im, q.im, P[0][s].im)};
           }
         }
       } else if constexpr (KIND == NL_POLY) {
@@ -266,25 +342,29 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
             pr *= ur;
             pi *= ui;
           }
-          P[s] = cf{ar, ai};
+          P[0][s] = cf{ar, ai};
         }
       } else {
-        static_assert(KIND == NL_VORT || KIND == NL_GRADNORM || KIND == NL_POLY, "unsupported 2-D nonlinear kind");
 #pragma unroll
         for (int s = 0; s < PT; ++s) {
           const cf q = z[0][s];
-          P[s] = f == 0 ? cf{q.re * q.re, q.im * q.im} : cf{fmaf(q.re, q.re, P[s].re), fmaf(q.im, q.im, P[s].im)};
+          P[0][s] = f == 0 ? cf{q.re * q.re, q.im * q.im}
+                           : cf{fmaf(q.re, q.re, P[0][s].re), fmaf(q.im, q.im, P[0][s].im)};
         }
       }
     }
-#pragma unroll
-    for (int s = 0; s < PT; ++s) z[0][s] = P[s];
     (void)A;
-    rfft<M, PT, 1>(z, nyq, ln, lds, tr, a.kmax);  // only ky <= kmax are kept
-    __syncthreads();
-    row_to_tile<N>(lds_t, r, j, z, nyq);
-    __syncthreads();
-    tile_store<N>(lds_t, a.T + (long long)inst * fstride, x0, kc);
+    (void)B2;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+#pragma unroll
+      for (int s = 0; s < PT; ++s) z[0][s] = P[p][s];
+      rfft<M, PT, 1>(z, nyq, ln, lds, tr, a.kmax);  // only ky <= kmax are kept
+      __syncthreads();
+      row_to_tile<N>(lds_t, r, j, z, nyq);
+      __syncthreads();
+      tile_store<N>(lds_t, a.T + (long long)inst * fstride + p * fsz, x0, kc);
+    }
   } else {
     tile_load<N>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
     __syncthreads();
@@ -292,7 +372,7 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
     irfft<M, PT, 1>(z, nyq, ln, lds, tr);
 #pragma unroll
     for (int s = 0; s < PT; ++s) z[0][s] = cf{z[0][s].re * rn2, z[0][s].im * rn2};
-    float2* u = reinterpret_cast<float2*>(a.u_out + (long long)inst * a.u_out_bstride + (long long)x * N);
+    float2* u = reinterpret_cast<float2*>(a.u_out + (long long)traj * a.u_out_bstride + urow);
 #pragma unroll
     for (int s = 0; s < PT; ++s) u[j + s * T] = make_float2(z[0][s].re, z[0][s].im);
     if (a.write_T) {
@@ -368,21 +448,42 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     return (a.inj_mode == ky && j + s * T == 0) ? cf{2.f * a.inj_value, 0.f} : cf{0.f, 0.f};
   };
 
-  // nonlinear term 2 N_hat of this column from the product P (ky < kc)
+  // nonlinear term 2 N_hat of this column from the products (ky < kc)
   auto nonlinear = [&](cf* Nh) {
     if (kept_col) {
-      cf v[1][PT];
-      const cf* p = a.T + (long long)inst * full + (long long)ky * N;
+      constexpr int NP = Fields2D<KIND>::NP;
+      cf v[NP][PT];
 #pragma unroll
-      for (int s = 0; s < PT; ++s) v[0][s] = p[j + s * T];
-      cfft<N, PT, false, 1>(v, ln, lds, tr);
+      for (int p = 0; p < NP; ++p) {
+        const cf* pp = a.T + (long long)inst * a.tstride + ((long long)p * kc + ky) * N;
+#pragma unroll
+        for (int s = 0; s < PT; ++s) v[p][s] = pp[j + s * T];
+      }
+      cfft<N, PT, false, NP>(v, ln, lds, tr);
       // the polynomial product is formed at unit scale, so its forward
       // transforms give 2 N_hat directly (no sign: nonlin_fun/_polynomial.py:76)
-      const float cs = (KIND == NL_VORT) ? -a.scale * q2 : (KIND == NL_POLY) ? 1.f : -a.scale * 0.5f * q2;
+      const float cs = (KIND == NL_VORT || KIND == NL_CONV_NONCONS) ? -a.scale * q2
+                       : (KIND == NL_POLY)                          ? 1.f
+                                                                    : -a.scale * 0.5f * q2;
 #pragma unroll
       for (int s = 0; s < PT; ++s) {
-        Nh[s] = keep_x[s] ? cs * v[0][s] : cf{0.f, 0.f};
-        if (KIND == NL_GRADNORM && a.zero_mode_fix && ky == 0 && j + s * T == 0) Nh[s] = cf{0.f, 0.f};
+        cf r;
+        if constexpr (KIND == NL_CONV_CONS) {
+          // -b/2 (i kx F[u_c u_0] + i ky F[u_c u_1])
+          const cf w0 = v[0][s], w1 = v[NP - 1][s];
+          r = cs * cf{-(kap_x[s] * w0.im + kap_y * w1.im), fmaf(kap_x[s], w0.re, kap_y * w1.re)};
+        } else if constexpr (KIND == NL_GENERAL) {
+          // g0 F[u^2] + (g1/2) i (kx + ky) F[u^2] + (g2/2) F[|grad u|^2] (mean-free)
+          const cf w0 = v[0][s], w1 = v[NP - 1][s];
+          const float ck = 0.5f * a.g1 * q2 * (kap_x[s] + kap_y);
+          const float cg = (a.zero_mode_fix && ky == 0 && j + s * T == 0) ? 0.f : 0.5f * a.g2 * q2;
+          const float c0 = a.g0 * q2;
+          r = cf{fmaf(c0, w0.re, fmaf(-ck, w0.im, cg * w1.re)), fmaf(c0, w0.im, fmaf(ck, w0.re, cg * w1.im))};
+        } else {
+          r = cs * v[0][s];
+          if (KIND == NL_GRADNORM && a.zero_mode_fix && ky == 0 && j + s * T == 0) r = cf{0.f, 0.f};
+        }
+        Nh[s] = keep_x[s] ? r : cf{0.f, 0.f};
         Nh[s] = Nh[s] + forcing(s);
       }
     } else {
@@ -410,8 +511,12 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
         v[1][s] = cf{-kx * w.im, kx * w.re};
         v[2][s] = cf{kx * psi.im, -kx * psi.re};
         v[3][s] = cf{-kyv * w.im, kyv * w.re};
-      } else if constexpr (KIND == NL_POLY) {
+      } else if constexpr (KIND == NL_POLY || KIND == NL_CONV_CONS) {
         v[0][s] = w;
+      } else if constexpr (KIND == NL_CONV_NONCONS || KIND == NL_GENERAL) {
+        v[0][s] = w;
+        v[1][s] = cf{-kx * w.im, kx * w.re};
+        v[2][s] = cf{-kyv * w.im, kyv * w.re};
       } else {
         v[0][s] = cf{-kx * w.im, kx * w.re};
         v[1][s] = cf{-kyv * w.im, kyv * w.re};
@@ -427,7 +532,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     }
   };
 
-  const float* tb = a.tables + (long long)(a.lin_batch > 1 ? inst : 0) * a.ncoef * full;
+  const float* tb = a.tables + (long long)(a.lin_batch > 1 ? inst / a.channels : 0) * a.ncoef * full;
   auto coef = [&](int i, int s) { return tb[(long long)i * full + (long long)ky * N + j + s * T]; };
   // E x with E = exp(L dt), complex when L is (odd derivatives)
   auto emul = [&](int s, cf x) -> cf {
@@ -439,7 +544,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
 
   if constexpr (MODE == COL_CA) {
     cf v[1][PT];
-    const cf* t = a.T + (long long)inst * full + (long long)ky * N;
+    const cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
 #pragma unroll
     for (int s = 0; s < PT; ++s) v[0][s] = t[j + s * T];
     cfft<N, PT, false, 1>(v, ln, lds, tr);
@@ -542,13 +647,17 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
 #pragma unroll
       for (int s = 0; s < PT; ++s) v[0][s] = Ac[s] + coef(cL, s) * Nh[s];
       cfft<N, PT, true, 1>(v, ln, lds, tr);
-      st(a.T, v[0]);
+      if (active) {
+        cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
+#pragma unroll
+        for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
+      }
     }
     (void)U0;
   } else if constexpr (MODE == COL_C0 || MODE == COL_C1) {
     cf v[1][PT];
     if constexpr (MODE == COL_C0) {
-      const cf* t = a.T + (long long)inst * full + (long long)ky * N;
+      const cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
 #pragma unroll
       for (int s = 0; s < PT; ++s) v[0][s] = t[j + s * T];
       cfft<N, PT, false, 1>(v, ln, lds, tr);
@@ -563,14 +672,14 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     }
     cfft<N, PT, true, 1>(v, ln, lds, tr);
     if (active) {
-      cf* t = a.T + (long long)inst * full + (long long)ky * N;
+      cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
 #pragma unroll
       for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
     }
   } else if constexpr (MODE == COL_FWD) {
     // rfftn: the row spectra of column ky -> X[kx][ky] (rows gave 2 X)
     cf v[1][PT];
-    const cf* t = a.T + (long long)inst * full + (long long)ky * N;
+    const cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
 #pragma unroll
     for (int s = 0; s < PT; ++s) v[0][s] = t[j + s * T];
     cfft<N, PT, false, 1>(v, ln, lds, tr);
@@ -588,7 +697,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     for (int s = 0; s < PT; ++s) v[0][s] = 2.f * in[(long long)(j + s * T) * C::NY];
     cfft<N, PT, true, 1>(v, ln, lds, tr);
     if (active) {
-      cf* t = a.T + (long long)inst * full + (long long)ky * N;
+      cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
 #pragma unroll
       for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
     }
@@ -604,7 +713,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     }
     cfft<N, PT, true, 1>(v, ln, lds, tr);
     if (active) {
-      cf* t = a.T + (long long)inst * full + (long long)ky * N;
+      cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
 #pragma unroll
       for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
     }

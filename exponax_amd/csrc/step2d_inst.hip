@@ -71,6 +71,9 @@ Launch2D EXMI_CAT(get_2d_, EXMI_N2)(int kind, int is_row, int mode) {
   if (kind == NL_VORT) return pick2d<N, NL_VORT>(is_row, mode);
   if (kind == NL_GRADNORM) return pick2d<N, NL_GRADNORM>(is_row, mode);
   if (kind == NL_POLY) return pick2d<N, NL_POLY>(is_row, mode);
+  if (kind == NL_CONV_CONS) return pick2d<N, NL_CONV_CONS>(is_row, mode);
+  if (kind == NL_CONV_NONCONS) return pick2d<N, NL_CONV_NONCONS>(is_row, mode);
+  if (kind == NL_GENERAL) return pick2d<N, NL_GENERAL>(is_row, mode);
   return nullptr;
 }
 

@@ -78,9 +78,10 @@ enum exmi_nonlin_kind {
 typedef struct exmi_plan_desc {
   int32_t num_spatial_dims;   /* D: 1 or 2                                   */
   int32_t num_points;         /* N: 1-D powers of two 16..4096, 48 and 50;
-                                 2-D powers of two 32..512 (one channel;
-                                 vorticity, gradient-norm, polynomial or no
-                                 nonlinearity)                               */
+                                 2-D powers of two 32..512 (any nonlinear
+                                 kind; two channels for the velocity
+                                 convections, one for vorticity and
+                                 general)                                    */
   int32_t num_channels;       /* C                                           */
   int32_t batch;              /* B                                           */
   int32_t order;              /* ETDRK order 0..4                            */

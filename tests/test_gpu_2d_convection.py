@@ -1,0 +1,80 @@
+"""Two-channel convection and the general nonlinearity in 2-D on the GPU.
+
+Burgers in 2-D (velocity (u, v), both forms of nonlin_fun/_convection.py:
+104-171; stepper/_burgers.py), Kuramoto-Sivashinsky in conservative form in
+2-D (two channels), single-channel conservative Burgers and
+GeneralNonlinearStepper in 2-D (nonlin_fun/_general_nonlinear.py:89-119)
+against the FP32 oracle at steps 1 and 10, through single steps and the
+rollout, for ETDRK orders 1-4 and per-trajectory coefficients.
+"""
+import numpy as np
+import pytest
+
+import exponax_amd as ex
+from oracle import stepper as ost
+from tests.helpers import RTOL_STEP, ics, max_rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(gpu, cpu, u0, device, label):
+    import torch
+
+    x = torch.tensor(u0, device=device)
+    traj = gpu.rollout_batch(x, 10).cpu().numpy()
+    ug, uc = x, u0
+    for s in range(1, 11):
+        ug = gpu.step(ug)
+        uc = cpu.step_batch(uc)
+        if s in (1, 10):
+            e_step = max_rel_l2(ug.cpu().numpy(), uc)
+            e_roll = max_rel_l2(traj[:, s - 1], uc)
+            print(f"  {label} step {s}: steps {e_step:.2e}, rollout {e_roll:.2e}", flush=True)
+            assert e_step <= RTOL_STEP and e_roll <= RTOL_STEP
+
+
+@pytest.mark.parametrize("N", [64, 128])
+@pytest.mark.parametrize("conservative,order", [(False, 2), (True, 2), (False, 1), (True, 3), (False, 4)])
+def test_burgers_2d_parity(device, conservative, order, N):
+    print(f"[case] 2-D Burgers conservative={conservative} order {order} N={N}", flush=True)
+    kw = dict(diffusivity=0.01, conservative=conservative, order=order)
+    gpu = ex.stepper.Burgers(2, 1.0, N, 1e-3, **kw)
+    cpu = ost.Burgers(2, 1.0, N, 1e-3, dtype=np.float32, **kw)
+    u0, _ = ics(3, 2, 1.0, N, 4, seed=71, channels=2)
+    _check(gpu, cpu, u0, device, "burgers2d")
+
+
+def test_burgers_2d_ensemble(device):
+    """Per-trajectory viscosity with two channels per trajectory."""
+    print("[case] 2-D Burgers ensemble", flush=True)
+    nus = [0.005, 0.01, 0.02, 0.04]
+    gpu = ex.ensemble(lambda nu: ex.stepper.Burgers(2, 1.0, 64, 1e-3, diffusivity=nu), nus)
+    cpu = ost.Ensemble([ost.Burgers(2, 1.0, 64, 1e-3, diffusivity=nu, dtype=np.float32) for nu in nus],
+                       dtype=np.float32)
+    u0, _ = ics(4, 2, 1.0, 64, 4, seed=72, channels=2)
+    _check(gpu, cpu, u0, device, "ensemble")
+
+
+def test_ks_conservative_2d_parity(device):
+    print("[case] 2-D KS conservative", flush=True)
+    gpu = ex.stepper.KuramotoSivashinskyConservative(2, 30.0, 64, 0.05)
+    cpu = ost.KuramotoSivashinskyConservative(2, 30.0, 64, 0.05, dtype=np.float32)
+    u0, _ = ics(2, 2, 30.0, 64, 3, seed=73, channels=2)
+    _check(gpu, cpu, 0.3 * u0, device, "ks_cons2d")
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_single_channel_burgers_and_general_2d(device, order):
+    print(f"[case] 2-D single-channel Burgers / general order {order}", flush=True)
+    kw = dict(diffusivity=0.01, single_channel=True, conservative=True, order=order)
+    gpu = ex.stepper.Burgers(2, 1.0, 64, 1e-3, **kw)
+    cpu = ost.Burgers(2, 1.0, 64, 1e-3, dtype=np.float32, **kw)
+    u0, _ = ics(3, 2, 1.0, 64, 4, seed=74)
+    _check(gpu, cpu, u0, device, "burgers_single")
+    lin, nonlin = (0.0, 0.0, 0.01), (0.3, -1.0, -0.5)
+    gpu = ex.stepper.generic.GeneralNonlinearStepper(2, 2.0, 64, 1e-3, linear_coefficients=lin,
+                                                     nonlinear_coefficients=nonlin, order=order)
+    cpu = ost.GeneralStepper(2, 2.0, 64, 1e-3, linear_coefficients=lin, nonlinear_scales=nonlin, order=order,
+                             dtype=np.float32)
+    u0, _ = ics(3, 2, 2.0, 64, 4, seed=75)
+    _check(gpu, cpu, u0, device, "general")

@@ -292,3 +292,47 @@ def test_allen_cahn_2d_uniform():
     assert np.ptp(u) < 1e-12
     t = n * dt
     assert abs(u[0, 0, 0] - u0 * np.exp(t) / np.sqrt(1 + u0**2 * (np.exp(2 * t) - 1))) < 2e-5
+
+
+@pytest.mark.parametrize("conservative", [False, True])
+def test_burgers_2d_reduces_to_1d(conservative):
+    """A velocity field (f(x), 0) or (0, g(y)) of 2-D Burgers evolves as 1-D
+    Burgers along its axis (the y-independent modes have ky = 0 only)."""
+    L, N, dt, nu = 1.0, 32, 0.01, 0.02
+    x = _grid(L, N)
+    f = 0.6 * np.sin(2 * np.pi * x / L) + 0.3 * np.cos(2 * np.pi * 3 * x / L)
+    one = st.Burgers(1, L, N, dt, diffusivity=nu, conservative=conservative).step(f[None])[0]
+    two = st.Burgers(2, L, N, dt, diffusivity=nu, conservative=conservative)
+    ux = two.step(np.stack([np.repeat(f[:, None], N, axis=1), np.zeros((N, N))]))
+    np.testing.assert_allclose(ux[0], np.repeat(one[:, None], N, axis=1), atol=1e-12)
+    np.testing.assert_allclose(ux[1], 0.0, atol=1e-12)
+    vy = two.step(np.stack([np.zeros((N, N)), np.repeat(f[None, :], N, axis=0)]))
+    np.testing.assert_allclose(vy[1], np.repeat(one[None, :], N, axis=0), atol=1e-12)
+    np.testing.assert_allclose(vy[0], 0.0, atol=1e-12)
+
+
+@pytest.mark.parametrize("conservative", [False, True])
+def test_burgers_2d_axis_swap_equivariance(conservative):
+    """Swapping the axes together with the velocity components commutes with
+    a 2-D Burgers step."""
+    from oracle import ic as oic
+
+    u0, _ = oic.tfs_batch(1, 2, 1.0, 32, 4, 5, num_channels=2)
+    u0 = u0[0]
+    bu = st.Burgers(2, 1.0, 32, 0.01, diffusivity=0.02, conservative=conservative)
+    swap = lambda u: np.stack([u[1].T, u[0].T])  # noqa: E731
+    np.testing.assert_allclose(bu.step(swap(u0)), swap(bu.step(u0)), atol=1e-12)
+
+
+def test_general_and_single_channel_convection_2d_reduce_to_1d():
+    L, N, dt = 2.0, 32, 0.01
+    x = _grid(L, N)
+    f = 0.5 * np.sin(2 * np.pi * x / L) + 0.2 * np.sin(2 * np.pi * 4 * x / L)
+    kw = dict(linear_coefficients=(0.0, 0.0, 0.01), nonlinear_scales=(0.3, -1.0, -0.5))
+    one = st.GeneralStepper(1, L, N, dt, **kw).step(f[None])[0]
+    two = st.GeneralStepper(2, L, N, dt, **kw).step(np.repeat(f[:, None], N, axis=1)[None])[0]
+    np.testing.assert_allclose(two, np.repeat(one[:, None], N, axis=1), atol=1e-12)
+    kw = dict(diffusivity=0.01, single_channel=True, conservative=True)
+    one = st.Burgers(1, L, N, dt, **kw).step(f[None])[0]
+    two = st.Burgers(2, L, N, dt, **kw).step(np.repeat(f[None, :], N, axis=0)[None])[0]
+    np.testing.assert_allclose(two, np.repeat(one[None, :], N, axis=0), atol=1e-12)
