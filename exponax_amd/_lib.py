@@ -21,6 +21,7 @@ EXMI_NL_POLYNOMIAL = 3
 EXMI_NL_GENERAL = 4
 EXMI_NL_VORTICITY = 5
 EXMI_NL_KOLMOGOROV = 6
+EXMI_NL_CAHN_HILLIARD = 7
 EXMI_MAX_POLY = 8
 EXMI_FLAG_EXACT_FORM = 1
 EXMI_FLAG_FOURIER_RESIDENT = 2

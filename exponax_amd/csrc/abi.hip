@@ -340,6 +340,7 @@ static void fill_args2d(const exmi_plan* p, Step2DArgs* a) {
   a->g1 = (float)p->d.general_scales[1];
   a->g2 = (float)p->d.general_scales[2];
   a->e_im = p->cplx ? num_coefs(p->d.order) : -1;
+  a->poly_lap = p->d.nonlin_kind == EXMI_NL_CAHN_HILLIARD;
   a->npoly = p->d.num_poly;
   for (int i = 0; i < EXMI_MAX_POLY; ++i) a->poly[i] = (float)p->d.poly[i];
 }
@@ -363,6 +364,15 @@ static void fill_args(const exmi_plan* p, Step1DArgs* a) {
   a->zero_mode_fix = p->d.zero_mode_fix;
   a->npoly = p->d.num_poly;
   for (int i = 0; i < EXMI_MAX_POLY; ++i) a->poly[i] = (float)p->d.poly[i];
+  a->poly_lap = p->d.nonlin_kind == EXMI_NL_CAHN_HILLIARD;
+  a->poly_lap_c = (float)(-p->d.scale * (double)p->kscale * (double)p->kscale);
+}
+
+// Cahn-Hilliard's u^3 as the polynomial (0, 0, 0, 1); the kernels apply
+// scale * Laplace to its transform.
+static void cahn_hilliard_poly(exmi_plan_desc* d) {
+  d->num_poly = 4;
+  for (int i = 0; i < EXMI_MAX_POLY; ++i) d->poly[i] = i == 3 ? 1.0 : 0.0;
 }
 
 static int kernel_kind(const exmi_plan_desc* d, int* kind) {
@@ -382,6 +392,10 @@ static int kernel_kind(const exmi_plan_desc* d, int* kind) {
     case EXMI_NL_POLYNOMIAL:
       if (d->num_poly < 0 || d->num_poly > EXMI_MAX_POLY)
         return fail(EXMI_ERR_UNSUPPORTED, "at most %d polynomial coefficients", EXMI_MAX_POLY);
+      *kind = NL_POLY;
+      return EXMI_OK;
+    case EXMI_NL_CAHN_HILLIARD:
+      if (C != 1) return fail(EXMI_ERR_UNSUPPORTED, "Cahn-Hilliard needs num_channels == 1");
       *kind = NL_POLY;
       return EXMI_OK;
     case EXMI_NL_GENERAL:
@@ -446,6 +460,9 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     if (d->num_poly < 0 || d->num_poly > EXMI_MAX_POLY)
       return fail(EXMI_ERR_UNSUPPORTED, "at most %d polynomial coefficients", EXMI_MAX_POLY);
     kind = NL_POLY;
+  } else if (d->nonlin_kind == EXMI_NL_CAHN_HILLIARD) {
+    if (C != 1) return fail(EXMI_ERR_UNSUPPORTED, "Cahn-Hilliard needs num_channels == 1");
+    kind = NL_POLY;
   } else if (d->nonlin_kind == EXMI_NL_ZERO) {
     kind = NL_POLY;  // the empty polynomial: N = 0 exactly
     zero_poly = true;
@@ -464,6 +481,7 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   p->cplx = cplx;
   p->kind = kind;
   if (zero_poly) p->d.num_poly = 0;
+  if (d->nonlin_kind == EXMI_NL_CAHN_HILLIARD) cahn_hilliard_poly(&p->d);
   if (conv_general) {
     p->d.general_scales[0] = 0.0;
     p->d.general_scales[1] = -d->scale;
@@ -656,6 +674,7 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   p->d = *d;
   p->d.host_linear_operator = nullptr;
   p->d.host_linear_operator_imag = nullptr;
+  if (d->nonlin_kind == EXMI_NL_CAHN_HILLIARD) cahn_hilliard_poly(&p->d);
   p->M = N / 2;
   p->nr = p->M + 1;
   p->ncoef = num_coefs_c(d->order, cplx);

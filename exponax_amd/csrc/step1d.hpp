@@ -46,6 +46,8 @@ struct Step1DArgs {
   int zero_mode_fix;
   int npoly;
   float poly[EXMI_MAX_POLY];
+  int poly_lap;          // 1: the polynomial's transform times poly_lap_c k^2 (Cahn-Hilliard)
+  float poly_lap_c;      // -scale (2 pi / L)^2: scale * Laplace in units of k^2
 };
 
 constexpr int ncoef_of(int order) {
@@ -250,8 +252,16 @@ __device__ __forceinline__ void nl_eval(const cf (*S)[PT], const cf* nyq, cf (*N
     rfft_l<LAY, M, PT, NT>(U, un, ln, lds, rt, kc);
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      masked_scale<M, PT, LAY>(U[n], Nn[n], j, a.kmax, [&](int, cf x) { return x; });
-      nn[n] = (KX <= a.kmax) ? un[n] : cf{0.f, 0.f};
+      if (a.poly_lap) {
+        masked_scale<M, PT, LAY>(U[n], Nn[n], j, a.kmax, [&](int s, cf x) {
+          const float k = (float)MD::k(j, s);
+          return (a.poly_lap_c * k * k) * x;
+        });
+        nn[n] = (KX <= a.kmax) ? (a.poly_lap_c * (float)KX * (float)KX) * un[n] : cf{0.f, 0.f};
+      } else {
+        masked_scale<M, PT, LAY>(U[n], Nn[n], j, a.kmax, [&](int, cf x) { return x; });
+        nn[n] = (KX <= a.kmax) ? un[n] : cf{0.f, 0.f};
+      }
     }
   } else if constexpr (KIND == NL_GENERAL) {
     // b0 F[u^2] + (b1/2)(i kappa) F[u^2] + (b2/2) F[u_x^2 - mean]

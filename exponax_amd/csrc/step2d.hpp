@@ -92,6 +92,7 @@ struct Step2DArgs {
   int e_im;                  // coefficient index of Im E (complex L, orders 0-2); -1: real L
   int npoly;                 // polynomial coefficients c_0..c_{npoly-1}
   float poly[EXMI_MAX_POLY];
+  int poly_lap;              // 1: scale * Laplace of the polynomial's transform (Cahn-Hilliard)
   cf* spec;                  // standalone transforms: spectra (ninst, N, N/2+1), [kx][ky]
   long long spec_bstride;    // complex entries between instances
 };
@@ -479,6 +480,10 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
           const float cg = (a.zero_mode_fix && ky == 0 && j + s * T == 0) ? 0.f : 0.5f * a.g2 * q2;
           const float c0 = a.g0 * q2;
           r = cf{fmaf(c0, w0.re, fmaf(-ck, w0.im, cg * w1.re)), fmaf(c0, w0.im, fmaf(ck, w0.re, cg * w1.im))};
+        } else if constexpr (KIND == NL_POLY) {
+          // scale * Laplace F[u^3] = -scale (kx^2 + ky^2) F[u^3] for Cahn-Hilliard
+          const float lap = -(kap_x[s] * kap_x[s] + kap_y * kap_y);
+          r = a.poly_lap ? (a.scale * lap) * v[0][s] : v[0][s];
         } else {
           r = cs * v[0][s];
           if (KIND == NL_GRADNORM && a.zero_mode_fix && ky == 0 && j + s * T == 0) r = cf{0.f, 0.f};

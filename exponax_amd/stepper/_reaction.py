@@ -1,9 +1,11 @@
 """Reaction-diffusion steppers with polynomial reactions
 (``stepper/reaction/_allen_cahn.py``, ``_fisher_kpp.py``,
-``_swift_hohenberg.py``): a real linear operator and the polynomial
-nonlinear function (1-D kernels)."""
+``_swift_hohenberg.py``, ``_cahn_hilliard.py``): a real linear operator and
+the polynomial nonlinear function (Cahn-Hilliard: the Laplacian of a cubic),
+in 1-D and 2-D."""
+from .. import _lib
 from .._base_stepper import BaseStepper
-from ..nonlin_fun import PolynomialNonlinearFun
+from ..nonlin_fun import BaseNonlinearFun, PolynomialNonlinearFun
 from ..spectral import build_laplace_operator
 
 
@@ -71,3 +73,47 @@ class SwiftHohenberg(_Polynomial):
 
     def _poly(self):
         return list(self.polynomial_coefficients)
+
+
+class CahnHilliardNonlinearFun(BaseNonlinearFun):
+    """``_cahn_hilliard.py:10-41``: scale * Laplace F[u^3] (the state's
+    first channel, dealiased before the inverse and after the forward
+    transform)."""
+
+    kind = _lib.EXMI_NL_CAHN_HILLIARD
+
+    def __init__(self, num_spatial_dims, num_points, *, derivative_operator, scale, dealiasing_fraction):
+        from ..nonlin_fun import domain_extent_of
+
+        super().__init__(num_spatial_dims, num_points, dealiasing_fraction=dealiasing_fraction)
+        self.domain_extent = domain_extent_of(derivative_operator, num_spatial_dims, num_points)
+        self.scale = scale
+
+    def fill(self, d):
+        super().fill(d)
+        d.scale = float(self.scale)
+
+
+class CahnHilliard(BaseStepper):
+    """u_t = nu Laplace (c3 u^3 + c1 u - gamma Laplace u) (``_cahn_hilliard.py:44-196``):
+    L = nu Laplace (c1 - gamma Laplace), N = nu c3 Laplace F[u^3]."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity=1e-2, gamma=1e-3,
+                 first_order_coefficient=-1.0, third_order_coefficient=1.0, order=2, dealiasing_fraction=1 / 2,
+                 num_circle_points=16, circle_radius=1.0):
+        self.diffusivity = diffusivity
+        self.gamma = gamma
+        self.first_order_coefficient = first_order_coefficient
+        self.third_order_coefficient = third_order_coefficient
+        self.dealiasing_fraction = dealiasing_fraction
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=1, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius)
+
+    def _build_linear_operator(self, derivative_operator):
+        lap = build_laplace_operator(derivative_operator, order=2)
+        return self.diffusivity * lap * (self.first_order_coefficient - self.gamma * lap)
+
+    def _build_nonlinear_fun(self, derivative_operator):
+        return CahnHilliardNonlinearFun(self.num_spatial_dims, self.num_points, derivative_operator=derivative_operator,
+                                        dealiasing_fraction=self.dealiasing_fraction,
+                                        scale=self.diffusivity * self.third_order_coefficient)

@@ -42,7 +42,10 @@ enum exmi_nonlin_kind {
   EXMI_NL_POLYNOMIAL = 3,  /* PolynomialNonlinearFun  nonlin_fun/_polynomial.py:6      */
   EXMI_NL_GENERAL = 4,     /* GeneralNonlinearFun     nonlin_fun/_general_nonlinear.py:9 */
   EXMI_NL_VORTICITY = 5,   /* VorticityConvection2d   nonlin_fun/_vorticity_convection.py:8 */
-  EXMI_NL_KOLMOGOROV = 6   /* VorticityConvection2dKolmogorov  ..._vorticity_convection.py:102 */
+  EXMI_NL_KOLMOGOROV = 6,  /* VorticityConvection2dKolmogorov  ..._vorticity_convection.py:102 */
+  /* scale * Laplace F[u^3]: CahnHilliardNonlinearFun
+   * stepper/reaction/_cahn_hilliard.py:10-41 (scale = nu c3) */
+  EXMI_NL_CAHN_HILLIARD = 7
 };
 
 #define EXMI_MAX_POLY 8

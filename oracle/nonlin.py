@@ -155,6 +155,24 @@ class PolynomialNonlinearFun(BaseNonlinearFun):
         return self.fft(acc)
 
 
+class CahnHilliardNonlinearFun(BaseNonlinearFun):
+    """``stepper/reaction/_cahn_hilliard.py:10-41``: scale * Laplace F[u_0^3]
+    (first channel), dealiased before the inverse and after the forward
+    transform."""
+
+    def __init__(self, num_spatial_dims, num_points, *, derivative_operator, scale, dealiasing_fraction,
+                 dtype=np.float64):
+        super().__init__(num_spatial_dims, num_points, dealiasing_fraction=dealiasing_fraction, dtype=dtype)
+        self.laplace_operator = self._op(sp.build_laplace_operator(derivative_operator, order=2))
+        self.scale = scale
+
+    def __call__(self, u_hat):
+        u = self.ifft(self.dealias(u_hat))
+        first = np.take(u, [0], axis=u.ndim - self.num_spatial_dims - 1)  # channel axis, batched or not
+        power_hat = self.fft(first**3)
+        return (self.laplace_operator * power_hat) * self.dtype.type(self.scale)
+
+
 class GeneralNonlinearFun(BaseNonlinearFun):
     """``nonlin_fun/_general_nonlinear.py:9-119``: b0 u^2 + conv + grad-norm."""
 

@@ -359,6 +359,30 @@ class SwiftHohenberg(_PolynomialReaction):
         return list(self.polynomial_coefficients)
 
 
+class CahnHilliard(BaseStepper):
+    """``stepper/reaction/_cahn_hilliard.py:44-196``: L = nu Laplace (c1 -
+    gamma Laplace), N = nu c3 Laplace F[u^3]."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity=1e-2, gamma=1e-3,
+                 first_order_coefficient=-1.0, third_order_coefficient=1.0, order=2, dealiasing_fraction=1 / 2,
+                 num_circle_points=16, circle_radius=1.0, **kw):
+        self.diffusivity, self.gamma = diffusivity, gamma
+        self.c1, self.c3 = first_order_coefficient, third_order_coefficient
+        self.dealiasing_fraction = dealiasing_fraction
+        self._kw_dtype = kw.get("dtype", np.float64)
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=1, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius, **kw)
+
+    def _build_linear_operator(self, d):
+        lap = sp.build_laplace_operator(d, order=2)
+        return self.diffusivity * lap * (self.c1 - self.gamma * lap)
+
+    def _build_nonlinear_fun(self, d):
+        return nl.CahnHilliardNonlinearFun(self.num_spatial_dims, self.num_points, derivative_operator=d,
+                                           scale=self.diffusivity * self.c3,
+                                           dealiasing_fraction=self.dealiasing_fraction, dtype=self._kw_dtype)
+
+
 class NavierStokesVorticity(BaseStepper):
     """``stepper/_navier_stokes.py:13-153``."""
 

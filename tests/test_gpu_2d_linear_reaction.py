@@ -139,3 +139,15 @@ def test_polynomial_2d_complex_linear(device, order):
     cpu = _OraclePolynomial(2, L, N, dt, lin, poly, order)
     u0, _ = ics(3, 2, L, N, 4, seed=63)
     _check(gpu, cpu, u0, device, "polynomial")
+
+
+@pytest.mark.parametrize("D,N,order", [(1, 64, 2), (1, 256, 2), (1, 128, 4), (2, 64, 2), (2, 128, 1)])
+def test_cahn_hilliard_parity(device, D, N, order):
+    """stepper/reaction/_cahn_hilliard.py: the Laplacian of the cubic (a
+    polynomial whose transform the kernels multiply by scale * Laplace)."""
+    print(f"[case] Cahn-Hilliard D={D} N={N} order {order}", flush=True)
+    kw = dict(diffusivity=1e-2, gamma=1e-3, order=order)
+    gpu = ex.stepper.reaction.CahnHilliard(D, 2.0, N, 1e-3, **kw)
+    cpu = ost.CahnHilliard(D, 2.0, N, 1e-3, dtype=np.float32, **kw)
+    u0, _ = ics(3, D, 2.0, N, 4, seed=64)
+    _check(gpu, cpu, u0, device, "cahn_hilliard")

@@ -336,3 +336,25 @@ def test_general_and_single_channel_convection_2d_reduce_to_1d():
     one = st.Burgers(1, L, N, dt, **kw).step(f[None])[0]
     two = st.Burgers(2, L, N, dt, **kw).step(np.repeat(f[None, :], N, axis=0)[None])[0]
     np.testing.assert_allclose(two, np.repeat(one[None, :], N, axis=0), atol=1e-12)
+
+
+@pytest.mark.parametrize("D", [1, 2])
+def test_cahn_hilliard_linearised_growth(D):
+    """u = a + eps cos(kappa x): u^3 = a^3 + 3 a^2 eps cos + O(eps^2), so the
+    cosine amplitude grows at -nu kappa^2 (3 c3 a^2 + c1 + gamma kappa^2)."""
+    L, N, dt, n, nu, gam, c1, c3, a, eps, m = 2.0, 32, 1e-3, 200, 1e-2, 1e-3, -1.0, 1.0, 0.3, 1e-7, 2
+    x = _grid(L, N)
+    kap = 2 * np.pi * m / L
+    prof = np.cos(kap * x)
+    u = (a + eps * (prof if D == 1 else np.repeat(prof[:, None], N, axis=1)))[None]
+    ch = st.CahnHilliard(D, L, N, dt, diffusivity=nu, gamma=gam, first_order_coefficient=c1,
+                         third_order_coefficient=c3, order=2)
+    for _ in range(n):
+        u = ch.step(u)
+    lam = -nu * kap**2 * (3 * c3 * a**2 + c1 + gam * kap**2)
+    row = u[0] if D == 1 else u[0][:, 0]
+    amp = 2 * np.abs(np.fft.rfft(row)[m]) / N
+    assert abs(row.mean() - a) < 1e-12
+    assert abs(amp / (eps * np.exp(lam * n * dt)) - 1) < 1e-4
+    # the wrong sign of the nonlinear Laplacian moves the rate by 2 * 3 nu kappa^2 c3 a^2
+    assert abs(np.exp(6 * nu * kap**2 * c3 * a**2 * n * dt) - 1) > 1e-3
