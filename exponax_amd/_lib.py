@@ -22,6 +22,8 @@ EXMI_NL_GENERAL = 4
 EXMI_NL_VORTICITY = 5
 EXMI_NL_KOLMOGOROV = 6
 EXMI_NL_CAHN_HILLIARD = 7
+EXMI_NL_GRAY_SCOTT = 8
+EXMI_NL_BELOUSOV_ZHABOTINSKY = 9
 EXMI_MAX_POLY = 8
 EXMI_FLAG_EXACT_FORM = 1
 EXMI_FLAG_FOURIER_RESIDENT = 2
@@ -71,6 +73,7 @@ class PlanDesc(C.Structure):
         ("injection_scale", C.c_double),
         ("host_linear_operator", C.POINTER(C.c_double)),
         ("host_linear_operator_imag", C.POINTER(C.c_double)),
+        ("reaction_rates", C.c_double * 2),
     ]
 
 

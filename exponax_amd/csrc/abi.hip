@@ -33,6 +33,7 @@ hipError_t launch_coef(const double* L, const double* Li, long long total, int n
   RollFn get_roll1d_##M(int kind, int order, int nt, int fr, int std_kc);                              \
   StepFn get_step1d_c_##M(int kind, int order);                                                        \
   RollFn get_roll1d_c_##M(int kind, int order);                                                        \
+  RollFn get_roll1d_r_##M(int kind, int order);                                                        \
   VjpFn get_vjp1d_##M(int kind, int order);                                                            \
   hipError_t launch_fft1d_##M(bool inverse, const float* in, float* out, int count, const RealTables& rt, \
                               hipStream_t st);
@@ -132,6 +133,18 @@ static RollFn get_roll1d_c(int M, int kind, int order) {
   switch (M) {
 #define EXMI_CASE(MM) \
   case MM: return get_roll1d_c_##MM(kind, order);
+    EXMI_CASE(8) EXMI_CASE(16) EXMI_CASE(24) EXMI_CASE(25) EXMI_CASE(32) EXMI_CASE(64) EXMI_CASE(128)
+    EXMI_CASE(256) EXMI_CASE(512)
+#undef EXMI_CASE
+    default: return nullptr;
+  }
+}
+
+// channel-coupled reactions: the channels of a trajectory share a lane group
+static RollFn get_roll1d_r(int M, int kind, int order) {
+  switch (M) {
+#define EXMI_CASE(MM) \
+  case MM: return get_roll1d_r_##MM(kind, order);
     EXMI_CASE(8) EXMI_CASE(16) EXMI_CASE(24) EXMI_CASE(25) EXMI_CASE(32) EXMI_CASE(64) EXMI_CASE(128)
     EXMI_CASE(256) EXMI_CASE(512)
 #undef EXMI_CASE
@@ -286,6 +299,7 @@ struct exmi_plan {
   // 2-D plans
   int ndim = 1;
   int cplx = 0;  // 2-D: complex linear operator (Im E follows the real coefficients)
+  int coupled = 0;  // 1-D channel-coupled reaction: steps through the multi-step kernel
   int nf = 0, kc = 0;
   int ninst = 0;          // 2-D: instances = batch * channels
   long long tstride = 0;  // 2-D: complex entries of T per instance
@@ -335,6 +349,9 @@ static void fill_args2d(const exmi_plan* p, Step2DArgs* a) {
   a->inj_mode = p->inj_mode;
   a->inj_value = p->inj_value;
   a->channels = p->d.num_channels;
+  a->lin_channels = p->d.lin_channels;
+  a->feed = (float)p->d.reaction_rates[0];
+  a->kill = (float)p->d.reaction_rates[1];
   a->tstride = p->tstride;
   a->g0 = (float)p->d.general_scales[0];
   a->g1 = (float)p->d.general_scales[1];
@@ -366,6 +383,8 @@ static void fill_args(const exmi_plan* p, Step1DArgs* a) {
   for (int i = 0; i < EXMI_MAX_POLY; ++i) a->poly[i] = (float)p->d.poly[i];
   a->poly_lap = p->d.nonlin_kind == EXMI_NL_CAHN_HILLIARD;
   a->poly_lap_c = (float)(-p->d.scale * (double)p->kscale * (double)p->kscale);
+  a->feed = (float)p->d.reaction_rates[0];
+  a->kill = (float)p->d.reaction_rates[1];
 }
 
 // Cahn-Hilliard's u^3 as the polynomial (0, 0, 0, 1); the kernels apply
@@ -398,6 +417,14 @@ static int kernel_kind(const exmi_plan_desc* d, int* kind) {
       if (C != 1) return fail(EXMI_ERR_UNSUPPORTED, "Cahn-Hilliard needs num_channels == 1");
       *kind = NL_POLY;
       return EXMI_OK;
+    case EXMI_NL_GRAY_SCOTT:  // _gray_scott.py:35-36
+      if (C != 2) return fail(EXMI_ERR_INVALID, "num_channels must be 2");
+      *kind = NL_GRAY_SCOTT;
+      return EXMI_OK;
+    case EXMI_NL_BELOUSOV_ZHABOTINSKY:  // _belousov_zhabotinsky.py:35-37
+      if (C != 3) return fail(EXMI_ERR_INVALID, "num_channels must be 3");
+      *kind = NL_BZ;
+      return EXMI_OK;
     case EXMI_NL_GENERAL:
       if (C != 1) return fail(EXMI_ERR_UNSUPPORTED, "general nonlinear function needs num_channels == 1");
       *kind = NL_GENERAL;
@@ -429,7 +456,8 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     return fail(EXMI_ERR_UNSUPPORTED, "Fourier-resident rollouts are built for 1-D plans");
   if (d->lin_batch != 1 && d->lin_batch != d->batch)
     return fail(EXMI_ERR_INVALID, "lin_batch must be 1 or batch (%d), got %d", d->batch, d->lin_batch);
-  if (d->lin_channels != 1) return fail(EXMI_ERR_INVALID, "lin_channels must be 1");
+  if (d->lin_channels != 1 && d->lin_channels != C)
+    return fail(EXMI_ERR_INVALID, "lin_channels must be 1 or num_channels (%d), got %d", C, d->lin_channels);
   if (!d->host_linear_operator) return fail(EXMI_ERR_INVALID, "host_linear_operator is null");
   const bool cplx = d->host_linear_operator_imag != nullptr;
   if (cplx && d->order > 2)
@@ -463,6 +491,12 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   } else if (d->nonlin_kind == EXMI_NL_CAHN_HILLIARD) {
     if (C != 1) return fail(EXMI_ERR_UNSUPPORTED, "Cahn-Hilliard needs num_channels == 1");
     kind = NL_POLY;
+  } else if (d->nonlin_kind == EXMI_NL_GRAY_SCOTT) {
+    if (C != 2) return fail(EXMI_ERR_INVALID, "num_channels must be 2");
+    kind = NL_GRAY_SCOTT;
+  } else if (d->nonlin_kind == EXMI_NL_BELOUSOV_ZHABOTINSKY) {
+    if (C != 3) return fail(EXMI_ERR_INVALID, "num_channels must be 3");
+    kind = NL_BZ;
   } else if (d->nonlin_kind == EXMI_NL_ZERO) {
     kind = NL_POLY;  // the empty polynomial: N = 0 exactly
     zero_poly = true;
@@ -492,7 +526,10 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   if (p->kc < 1) p->kc = 1;
   {
     const int np = (kind == NL_CONV_CONS || kind == NL_GENERAL) ? 2 : 1;
-    p->nf = kind == NL_VORT ? 4 : kind == NL_GRADNORM ? 2 : (kind == NL_POLY || kind == NL_CONV_CONS) ? 1 : 3;
+    p->nf = kind == NL_VORT ? 4
+            : kind == NL_GRADNORM ? 2
+            : (kind == NL_POLY || kind == NL_CONV_CONS || kind == NL_GRAY_SCOTT || kind == NL_BZ) ? 1
+                                                                                                  : 3;
     const long long fullc = (long long)NY * N, prod = (long long)np * p->kc * N;
     p->tstride = fullc > prod ? fullc : prod;
     p->ninst = d->batch * C;
@@ -531,7 +568,7 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     return rc;
   }
   const long long B = p->ninst, full = (long long)NY * N;  // allocations per instance
-  const long long rows = d->lin_batch;
+  const long long rows = (long long)d->lin_batch * d->lin_channels;
   // L arrives as the reference's [kx][ky] table; the column kernels read
   // [ky][kx]
   // [ky][kx]; with a complex L the imaginary part follows the real one
@@ -682,9 +719,19 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   p->kmax = kmax;
   p->kscale = (float)(2.0 * 3.14159265358979323846 / d->domain_extent);
   hipGetDevice(&p->device);
-  p->step = cplx ? get_step1d_c(p->M, kind, d->order) : get_step1d(p->M, kind, d->order);
-  p->nonlin = d->order > 0 ? get_step1d(p->M, nl_kind, 1) : get_step1d(p->M, NL_ZERO, 1);
-  if (!p->step || !p->nonlin) {
+  // Channel-coupled reactions: every step runs through the multi-step kernel
+  // with a trajectory's channels in one lane group (no single-step kernel)
+  const bool coupled = kind == NL_GRAY_SCOTT || kind == NL_BZ;
+  if (coupled && (cplx || p->M > 512 || (d->flags & EXMI_FLAG_FOURIER_RESIDENT))) {
+    delete p;
+    return fail(EXMI_ERR_UNSUPPORTED,
+                "channel-coupled reactions are built for real linear operators, N <= 1024, without the "
+                "Fourier-resident flag");
+  }
+  p->coupled = coupled;
+  p->step = coupled ? nullptr : cplx ? get_step1d_c(p->M, kind, d->order) : get_step1d(p->M, kind, d->order);
+  p->nonlin = coupled ? nullptr : d->order > 0 ? get_step1d(p->M, nl_kind, 1) : get_step1d(p->M, NL_ZERO, 1);
+  if (!coupled && (!p->step || !p->nonlin)) {
     delete p;
     return fail(EXMI_ERR_UNSUPPORTED, "no kernel for N=%d kind=%d order=%d", N, kind, d->order);
   }
@@ -696,10 +743,19 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   }
   p->fourier_resident = fr;
   const int std_kc = p->kmax == default_kmax(p->M) ? 1 : 0;
-  p->roll[1] = cplx ? get_roll1d_c(p->M, kind, d->order) : get_roll1d(p->M, kind, d->order, 1, fr, std_kc);
-  p->roll[2] = cplx ? nullptr : get_roll1d(p->M, kind, d->order, 2, fr, std_kc);
+  p->roll[1] = coupled ? get_roll1d_r(p->M, kind, d->order)
+              : cplx  ? get_roll1d_c(p->M, kind, d->order)
+                      : get_roll1d(p->M, kind, d->order, 1, fr, std_kc);
+  p->roll[2] = (cplx || coupled) ? nullptr : get_roll1d(p->M, kind, d->order, 2, fr, std_kc);
   if (const char* e = getenv("EXMI_ROLL_NT")) p->roll_nt = atoi(e);  // 1, 2; -1 = per-step launches
   if (p->fourier_resident && p->roll_nt < 0) p->roll_nt = 0;
+  if (coupled) {
+    if (!p->roll[1]) {
+      delete p;
+      return fail(EXMI_ERR_UNSUPPORTED, "no coupled-reaction kernel for N=%d order=%d", N, d->order);
+    }
+    p->roll_nt = 1;  // the only strategy
+  }
 
   int rc = get_twiddles(p->M, &p->tw);
   if (rc) {
@@ -799,6 +855,9 @@ int exmi_plan_get_tables(const exmi_plan* p, float* host_out, int64_t host_count
   return EXMI_OK;
 }
 
+static int enqueue_roll(exmi_plan* p, int nt, const float* u0, long long u0_bs, float* out, long long out_bs,
+                        long long out_ts, int n, int final_only, hipStream_t st);
+
 static int run_step(exmi_plan* p, const void* in, long long in_bs, int in_f, void* out, long long out_bs, int out_f,
                     int nonlin_only, hipStream_t st) {
   Step1DArgs a;
@@ -810,6 +869,12 @@ static int run_step(exmi_plan* p, const void* in, long long in_bs, int in_f, voi
   a.in_fourier = in_f;
   a.out_fourier = out_f;
   a.nonlin_only = nonlin_only;
+  if (p->coupled) {
+    if (nonlin_only || in_f || out_f)
+      return fail(EXMI_ERR_UNSUPPORTED, "channel-coupled reaction plans step physical states only");
+    const long long sz = (long long)p->d.num_channels * p->d.num_points;
+    return enqueue_roll(p, 1, (const float*)in, in_bs, (float*)out, out_bs, sz, 1, 0, st);
+  }
   hipError_t e = nonlin_only ? p->nonlin(a, st) : p->step(a, st);
   if (e != hipSuccess) return fail(EXMI_ERR_HIP, "step launch failed: %s", hipGetErrorString(e));
   return EXMI_OK;

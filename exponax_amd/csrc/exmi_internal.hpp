@@ -15,8 +15,25 @@ enum KernelKind {
   NL_POLY = 4,          // polynomial reaction, 1 + 1
   NL_GENERAL = 5,       // b0 u^2 + conservative convection + gradient norm, 2 + 2
   NL_VORT = 6,          // 2-D vorticity convection (+ Kolmogorov forcing)
-  NL_NUM_KINDS = 7
+  NL_GRAY_SCOTT = 7,    // two-channel Gray-Scott reaction, channels coupled
+  NL_BZ = 8,            // three-channel Belousov-Zhabotinsky reaction, channels coupled
+  NL_NUM_KINDS = 9
 };
+
+// Channel-coupled reactions at one grid point, unit scale: Gray-Scott
+// (stepper/reaction/_gray_scott.py:36-46) and Belousov-Zhabotinsky
+// (_belousov_zhabotinsky.py:38-45), in the reference's operation order.
+__host__ __device__ __forceinline__ void gray_scott(float u0, float u1, float f, float k, float& r0, float& r1) {
+  const float uvv = u0 * u1 * u1;
+  r0 = f * (1.f - u0) - uvv;
+  r1 = -(f + k) * u1 + uvv;
+}
+__host__ __device__ __forceinline__ void belousov_zhabotinsky(float u0, float u1, float u2, float& r0, float& r1,
+                                                             float& r2) {
+  r0 = u0 + u1 - u0 * u1 - u0 * u0;
+  r1 = u2 - u1 - u0 * u1;
+  r2 = u0 - u2;
+}
 
 __host__ __device__ inline int num_coefs(int order) {
   switch (order) {

@@ -48,7 +48,9 @@ struct Step1DArgs {
   float poly[EXMI_MAX_POLY];
   int poly_lap;          // 1: the polynomial's transform times poly_lap_c k^2 (Cahn-Hilliard)
   float poly_lap_c;      // -scale (2 pi / L)^2: scale * Laplace in units of k^2
+  float feed, kill;      // Gray-Scott rates
 };
+
 
 constexpr int ncoef_of(int order) {
   return order == 0 ? 1 : (order == 1 ? 2 : (order == 2 ? 3 : (order == 3 ? 7 : 6)));
@@ -262,6 +264,42 @@ __device__ __forceinline__ void nl_eval(const cf (*S)[PT], const cf* nyq, cf (*N
         masked_scale<M, PT, LAY>(U[n], Nn[n], j, a.kmax, [&](int, cf x) { return x; });
         nn[n] = (KX <= a.kmax) ? un[n] : cf{0.f, 0.f};
       }
+    }
+  } else if constexpr (KIND == NL_GRAY_SCOTT || KIND == NL_BZ) {
+    // the NT transforms of a group are the C channels of one trajectory
+    static_assert(NT == (KIND == NL_GRAY_SCOTT ? 2 : 3), "one trajectory's channels per lane group");
+    cf U[NT][PT];
+    cf un[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) masked_copy<M, PT, LAY>(S[n], nyq[n], U[n], un[n], j, a.kmax);
+    irfft_l<LAY, M, PT, NT>(U, un, ln, lds, rt, a.kmax);
+    constexpr float r2n = 1.f / (2.f * NN);
+#pragma unroll
+    for (int s = 0; s < PT; ++s) {
+      // real and imaginary parts hold two grid points each
+      float v[NT][2];
+#pragma unroll
+      for (int n = 0; n < NT; ++n) {
+        v[n][0] = U[n][s].re * r2n;
+        v[n][1] = U[n][s].im * r2n;
+      }
+      float r[NT][2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if constexpr (KIND == NL_GRAY_SCOTT) {
+          gray_scott(v[0][h], v[1][h], a.feed, a.kill, r[0][h], r[1][h]);
+        } else {
+          belousov_zhabotinsky(v[0][h], v[1][h], v[2][h], r[0][h], r[1][h], r[2][h]);
+        }
+      }
+#pragma unroll
+      for (int n = 0; n < NT; ++n) U[n][s] = cf{r[n][0], r[n][1]};
+    }
+    rfft_l<LAY, M, PT, NT>(U, un, ln, lds, rt, kc);
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      masked_scale<M, PT, LAY>(U[n], Nn[n], j, a.kmax, [&](int, cf x) { return x; });
+      nn[n] = (KX <= a.kmax) ? un[n] : cf{0.f, 0.f};
     }
   } else if constexpr (KIND == NL_GENERAL) {
     // b0 F[u^2] + (b1/2)(i kappa) F[u^2] + (b2/2) F[u_x^2 - mean]

@@ -208,6 +208,28 @@ StepFn EXMI_CAT(get_step1d_c_, EXMI_M)(int kind, int order) {
   return nullptr;
 }
 
+// Channel-coupled reactions (ETDRK orders 1-4): one trajectory's channels
+// are the transforms of one lane group (NT = channels).
+template <int M, int PT, int KIND, int NT>
+static RollFn pick_roll_r(int order) {
+  switch (order) {
+    case 1: return &launch_roll1d<M, PT, KIND, 1, NT>;
+    case 2: return &launch_roll1d<M, PT, KIND, 2, NT>;
+    case 3: return &launch_roll1d<M, PT, KIND, 3, NT>;
+    case 4: return &launch_roll1d<M, PT, KIND, 4, NT>;
+    default: return nullptr;
+  }
+}
+
+RollFn EXMI_CAT(get_roll1d_r_, EXMI_M)(int kind, int order) {
+  constexpr int M = EXMI_M, PT = EXMI_PT;
+  if constexpr (M <= 512) {
+    if (kind == NL_GRAY_SCOTT) return pick_roll_r<M, PT, NL_GRAY_SCOTT, 2>(order);
+    if (kind == NL_BZ) return pick_roll_r<M, PT, NL_BZ, 3>(order);
+  }
+  return nullptr;
+}
+
 RollFn EXMI_CAT(get_roll1d_c_, EXMI_M)(int kind, int order) {
   constexpr int M = EXMI_M, PT = EXMI_PT;
   if constexpr (M <= 512) {

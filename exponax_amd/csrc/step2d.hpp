@@ -93,6 +93,8 @@ struct Step2DArgs {
   int npoly;                 // polynomial coefficients c_0..c_{npoly-1}
   float poly[EXMI_MAX_POLY];
   int poly_lap;              // 1: scale * Laplace of the polynomial's transform (Cahn-Hilliard)
+  int lin_channels;          // 1 or C: per-channel linear operators
+  float feed, kill;          // Gray-Scott rates
   cf* spec;                  // standalone transforms: spectra (ninst, N, N/2+1), [kx][ky]
   long long spec_bstride;    // complex entries between instances
 };
@@ -122,15 +124,19 @@ struct Fields2D;
 // trajectory; products u_c u_0, u_c u_1 (the terms of 1/2 sum_j i kappa_j
 // F[u_c u_j]). Non-conservative: fields (u_c, d_x u_c, d_y u_c); inputs
 // (u_0, u_1, d_x u_c, d_y u_c); product u_0 d_x u_c + u_1 d_y u_c. General:
-// fields (u, u_x, u_y); products u^2 and u_x^2 + u_y^2.
+// fields (u, u_x, u_y); products u^2 and u_x^2 + u_y^2. Coupled reactions
+// (Gray-Scott, Belousov-Zhabotinsky): field u_c; inputs all channels of the
+// trajectory; product the channel's reaction.
 template <int KIND>
 struct Fields2D {
-  static constexpr int NF = KIND == NL_VORT ? 4 : KIND == NL_GRADNORM ? 2 : KIND == NL_POLY ? 1
-                            : KIND == NL_CONV_CONS ? 1 : 3;
-  static constexpr int NIN = KIND == NL_CONV_CONS ? 2 : KIND == NL_CONV_NONCONS ? 4 : NF;
+  static constexpr bool COUPLED = KIND == NL_GRAY_SCOTT || KIND == NL_BZ;
+  static constexpr int NF = KIND == NL_VORT ? 4 : KIND == NL_GRADNORM ? 2
+                            : (KIND == NL_POLY || KIND == NL_CONV_CONS || COUPLED) ? 1 : 3;
+  static constexpr int NIN = (KIND == NL_CONV_CONS || KIND == NL_GRAY_SCOTT) ? 2
+                             : KIND == NL_BZ ? 3 : KIND == NL_CONV_NONCONS ? 4 : NF;
   static constexpr int NP = (KIND == NL_CONV_CONS || KIND == NL_GENERAL) ? 2 : 1;
   static_assert(KIND == NL_VORT || KIND == NL_GRADNORM || KIND == NL_POLY || KIND == NL_CONV_CONS ||
-                    KIND == NL_CONV_NONCONS || KIND == NL_GENERAL,
+                    KIND == NL_CONV_NONCONS || KIND == NL_GENERAL || COUPLED,
                 "unsupported 2-D nonlinear kind");
 };
 
@@ -257,7 +263,7 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
     // field u_j of channel j of the same trajectory
     auto src = [&](int f) -> const cf* {
       int i = inst, fld = f;
-      if constexpr (KIND == NL_CONV_CONS) {
+      if constexpr (KIND == NL_CONV_CONS || FD::COUPLED) {
         i = inst - ch + f;
         fld = 0;
       } else if constexpr (KIND == NL_CONV_NONCONS) {
@@ -281,7 +287,32 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
       if (f + 1 < NIN) pf.load(src(f + 1), x0, kc);  // overlaps this field's transform
       tile_to_row<N>(lds_t, r, j, z, nyq);
       irfft<M, PT, 1>(z, nyq, ln, lds, tr);
-      if constexpr (KIND == NL_CONV_CONS) {
+      if constexpr (FD::COUPLED) {
+        // the channel's reaction at unit scale (inputs are 2 N^2 u)
+#pragma unroll
+        for (int s = 0; s < PT; ++s) {
+          const cf q = z[0][s];
+          if (f == 0) {
+            A[s] = q;
+          } else if (f + 1 < NIN) {
+            B2[s] = q;
+          } else {
+            float r[3][2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              const float u0 = (h ? A[s].im : A[s].re) * rn2;
+              if constexpr (KIND == NL_GRAY_SCOTT) {
+                gray_scott(u0, (h ? q.im : q.re) * rn2, a.feed, a.kill, r[0][h], r[1][h]);
+                r[2][h] = 0.f;
+              } else {
+                belousov_zhabotinsky(u0, (h ? B2[s].im : B2[s].re) * rn2, (h ? q.im : q.re) * rn2, r[0][h], r[1][h],
+                                     r[2][h]);
+              }
+            }
+            P[0][s] = ch == 0 ? cf{r[0][0], r[0][1]} : ch == 1 ? cf{r[1][0], r[1][1]} : cf{r[2][0], r[2][1]};
+          }
+        }
+      } else if constexpr (KIND == NL_CONV_CONS) {
 #pragma unroll
         for (int s = 0; s < PT; ++s) {
           const cf q = z[0][s];
@@ -464,8 +495,8 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
       // the polynomial product is formed at unit scale, so its forward
       // transforms give 2 N_hat directly (no sign: nonlin_fun/_polynomial.py:76)
       const float cs = (KIND == NL_VORT || KIND == NL_CONV_NONCONS) ? -a.scale * q2
-                       : (KIND == NL_POLY)                          ? 1.f
-                                                                    : -a.scale * 0.5f * q2;
+                       : (KIND == NL_POLY || Fields2D<KIND>::COUPLED) ? 1.f
+                                                                      : -a.scale * 0.5f * q2;
 #pragma unroll
       for (int s = 0; s < PT; ++s) {
         cf r;
@@ -516,7 +547,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
         v[1][s] = cf{-kx * w.im, kx * w.re};
         v[2][s] = cf{kx * psi.im, -kx * psi.re};
         v[3][s] = cf{-kyv * w.im, kyv * w.re};
-      } else if constexpr (KIND == NL_POLY || KIND == NL_CONV_CONS) {
+      } else if constexpr (KIND == NL_POLY || KIND == NL_CONV_CONS || Fields2D<KIND>::COUPLED) {
         v[0][s] = w;
       } else if constexpr (KIND == NL_CONV_NONCONS || KIND == NL_GENERAL) {
         v[0][s] = w;
@@ -537,7 +568,10 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     }
   };
 
-  const float* tb = a.tables + (long long)(a.lin_batch > 1 ? inst / a.channels : 0) * a.ncoef * full;
+  const int traj = inst / a.channels, chan = inst % a.channels;
+  const float* tb =
+      a.tables + ((long long)(a.lin_batch > 1 ? traj : 0) * a.lin_channels + (a.lin_channels > 1 ? chan : 0)) *
+                     a.ncoef * full;
   auto coef = [&](int i, int s) { return tb[(long long)i * full + (long long)ky * N + j + s * T]; };
   // E x with E = exp(L dt), complex when L is (odd derivatives)
   auto emul = [&](int s, cf x) -> cf {

@@ -74,6 +74,8 @@ Launch2D EXMI_CAT(get_2d_, EXMI_N2)(int kind, int is_row, int mode) {
   if (kind == NL_CONV_CONS) return pick2d<N, NL_CONV_CONS>(is_row, mode);
   if (kind == NL_CONV_NONCONS) return pick2d<N, NL_CONV_NONCONS>(is_row, mode);
   if (kind == NL_GENERAL) return pick2d<N, NL_GENERAL>(is_row, mode);
+  if (kind == NL_GRAY_SCOTT) return pick2d<N, NL_GRAY_SCOTT>(is_row, mode);
+  if (kind == NL_BZ) return pick2d<N, NL_BZ>(is_row, mode);
   return nullptr;
 }
 

@@ -117,3 +117,79 @@ class CahnHilliard(BaseStepper):
         return CahnHilliardNonlinearFun(self.num_spatial_dims, self.num_points, derivative_operator=derivative_operator,
                                         dealiasing_fraction=self.dealiasing_fraction,
                                         scale=self.diffusivity * self.third_order_coefficient)
+
+
+class _CoupledReactionFun(BaseNonlinearFun):
+    """Reactions that couple the channels (evaluated per grid point on the
+    dealiased physical fields, transformed back and dealiased)."""
+
+    def __init__(self, num_spatial_dims, num_points, *, dealiasing_fraction, rates=(0.0, 0.0)):
+        super().__init__(num_spatial_dims, num_points, dealiasing_fraction=dealiasing_fraction)
+        self.rates = tuple(float(r) for r in rates)
+
+    def fill(self, d):
+        super().fill(d)
+        d.reaction_rates[0], d.reaction_rates[1] = self.rates
+
+
+class GrayScottNonlinearFun(_CoupledReactionFun):
+    """``_gray_scott.py:9-49``: (f (1 - u0) - u0 u1^2, -(f + k) u1 + u0 u1^2)."""
+
+    kind = _lib.EXMI_NL_GRAY_SCOTT
+
+    def __init__(self, num_spatial_dims, num_points, *, dealiasing_fraction, feed_rate, kill_rate):
+        super().__init__(num_spatial_dims, num_points, dealiasing_fraction=dealiasing_fraction,
+                         rates=(feed_rate, kill_rate))
+        self.feed_rate, self.kill_rate = feed_rate, kill_rate
+
+
+class BelousovZhabotinskyNonlinearFun(_CoupledReactionFun):
+    """``_belousov_zhabotinsky.py:9-47``: (u0 + u1 - u0 u1 - u0^2,
+    u2 - u1 - u0 u1, u0 - u2)."""
+
+    kind = _lib.EXMI_NL_BELOUSOV_ZHABOTINSKY
+
+
+class GrayScott(BaseStepper):
+    """``_gray_scott.py:52-172``: two channels, L = (nu1 Laplace, nu2 Laplace)."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity_1=2e-5, diffusivity_2=1e-5,
+                 feed_rate=0.04, kill_rate=0.06, order=2, dealiasing_fraction=1 / 2, num_circle_points=16,
+                 circle_radius=1.0):
+        self.diffusivity_1, self.diffusivity_2 = diffusivity_1, diffusivity_2
+        self.feed_rate, self.kill_rate = feed_rate, kill_rate
+        self.dealiasing_fraction = dealiasing_fraction
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=2, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius)
+
+    def _build_linear_operator(self, derivative_operator):
+        import numpy as np
+
+        lap = build_laplace_operator(derivative_operator, order=2)
+        return np.concatenate([self.diffusivity_1 * lap, self.diffusivity_2 * lap])
+
+    def _build_nonlinear_fun(self, derivative_operator):
+        return GrayScottNonlinearFun(self.num_spatial_dims, self.num_points,
+                                     dealiasing_fraction=self.dealiasing_fraction, feed_rate=self.feed_rate,
+                                     kill_rate=self.kill_rate)
+
+
+class BelousovZhabotinsky(BaseStepper):
+    """``_belousov_zhabotinsky.py:50-105``: three channels, L = diag(nu_i Laplace)."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivities=(1e-5, 2e-5, 1e-5), order=2,
+                 dealiasing_fraction=1 / 2, num_circle_points=16, circle_radius=1.0):
+        self.diffusivities = tuple(diffusivities)
+        self.dealiasing_fraction = dealiasing_fraction
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=3, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius)
+
+    def _build_linear_operator(self, derivative_operator):
+        import numpy as np
+
+        lap = build_laplace_operator(derivative_operator, order=2)
+        return np.concatenate([nu * lap for nu in self.diffusivities])
+
+    def _build_nonlinear_fun(self, derivative_operator):
+        return BelousovZhabotinskyNonlinearFun(self.num_spatial_dims, self.num_points,
+                                               dealiasing_fraction=self.dealiasing_fraction)

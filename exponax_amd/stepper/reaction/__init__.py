@@ -1,4 +1,4 @@
 """Reaction-diffusion steppers (``exponax/stepper/reaction/``)."""
-from .._reaction import AllenCahn, CahnHilliard, FisherKPP, SwiftHohenberg
+from .._reaction import AllenCahn, BelousovZhabotinsky, CahnHilliard, FisherKPP, GrayScott, SwiftHohenberg
 
-__all__ = ["AllenCahn", "CahnHilliard", "FisherKPP", "SwiftHohenberg"]
+__all__ = ["AllenCahn", "BelousovZhabotinsky", "CahnHilliard", "FisherKPP", "GrayScott", "SwiftHohenberg"]

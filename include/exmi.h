@@ -45,7 +45,17 @@ enum exmi_nonlin_kind {
   EXMI_NL_KOLMOGOROV = 6,  /* VorticityConvection2dKolmogorov  ..._vorticity_convection.py:102 */
   /* scale * Laplace F[u^3]: CahnHilliardNonlinearFun
    * stepper/reaction/_cahn_hilliard.py:10-41 (scale = nu c3) */
-  EXMI_NL_CAHN_HILLIARD = 7
+  EXMI_NL_CAHN_HILLIARD = 7,
+  /* Channel-coupled reactions (two and three channels), one per-channel
+   * linear operator row each (lin_channels = num_channels):
+   * GrayScottNonlinearFun stepper/reaction/_gray_scott.py:9-49 (feed and kill
+   * rates in reaction_rates), BelousovZhabotinskyNonlinearFun
+   * stepper/reaction/_belousov_zhabotinsky.py:9-47. 1-D plans run every
+   * step, rollout and repeat through the multi-step kernel with the channels
+   * of a trajectory in one wave (N <= 1024; no exmi_step_fourier /
+   * exmi_nonlinear); 2-D plans couple them in the row kernel. */
+  EXMI_NL_GRAY_SCOTT = 8,
+  EXMI_NL_BELOUSOV_ZHABOTINSKY = 9
 };
 
 #define EXMI_MAX_POLY 8
@@ -117,6 +127,7 @@ typedef struct exmi_plan_desc {
    * _etdrk_2.py:71-89). The table layout gains Im E after the real
    * coefficients (exmi_plan_num_coefs counts it). No adjoint. */
   const double* host_linear_operator_imag;
+  double reaction_rates[2];   /* Gray-Scott feed and kill rates              */
 } exmi_plan_desc;
 
 typedef struct exmi_plan exmi_plan;

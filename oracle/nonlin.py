@@ -173,6 +173,37 @@ class CahnHilliardNonlinearFun(BaseNonlinearFun):
         return (self.laplace_operator * power_hat) * self.dtype.type(self.scale)
 
 
+class GrayScottNonlinearFun(BaseNonlinearFun):
+    """``stepper/reaction/_gray_scott.py:9-49``."""
+
+    def __init__(self, num_spatial_dims, num_points, *, dealiasing_fraction, feed_rate, kill_rate, dtype=np.float64):
+        super().__init__(num_spatial_dims, num_points, dealiasing_fraction=dealiasing_fraction, dtype=dtype)
+        self.feed_rate, self.kill_rate = feed_rate, kill_rate
+
+    def __call__(self, u_hat):
+        u = self.ifft(self.dealias(u_hat))
+        ax = u.ndim - self.num_spatial_dims - 1
+        if u.shape[ax] != 2:
+            raise ValueError("num_channels must be 2")
+        u0, u1 = np.take(u, 0, axis=ax), np.take(u, 1, axis=ax)
+        f, k = self.dtype.type(self.feed_rate), self.dtype.type(self.kill_rate)
+        power = np.stack([f * (1 - u0) - u0 * u1**2, -(f + k) * u1 + u0 * u1**2], axis=ax)
+        return self.fft(power)
+
+
+class BelousovZhabotinskyNonlinearFun(BaseNonlinearFun):
+    """``stepper/reaction/_belousov_zhabotinsky.py:9-47``."""
+
+    def __call__(self, u_hat):
+        u = self.ifft(self.dealias(u_hat))
+        ax = u.ndim - self.num_spatial_dims - 1
+        if u.shape[ax] != 3:
+            raise ValueError("num_channels must be 3")
+        u0, u1, u2 = (np.take(u, i, axis=ax) for i in range(3))
+        power = np.stack([u0 + u1 - u0 * u1 - u0**2, u2 - u1 - u0 * u1, u0 - u2], axis=ax)
+        return self.fft(power)
+
+
 class GeneralNonlinearFun(BaseNonlinearFun):
     """``nonlin_fun/_general_nonlinear.py:9-119``: b0 u^2 + conv + grad-norm."""
 

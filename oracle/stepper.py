@@ -383,6 +383,49 @@ class CahnHilliard(BaseStepper):
                                            dealiasing_fraction=self.dealiasing_fraction, dtype=self._kw_dtype)
 
 
+class GrayScott(BaseStepper):
+    """``stepper/reaction/_gray_scott.py:52-172``: L = (nu1 Laplace, nu2 Laplace)."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity_1=2e-5, diffusivity_2=1e-5,
+                 feed_rate=0.04, kill_rate=0.06, order=2, dealiasing_fraction=1 / 2, num_circle_points=16,
+                 circle_radius=1.0, **kw):
+        self.nu1, self.nu2, self.feed_rate, self.kill_rate = diffusivity_1, diffusivity_2, feed_rate, kill_rate
+        self.dealiasing_fraction = dealiasing_fraction
+        self._kw_dtype = kw.get("dtype", np.float64)
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=2, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius, **kw)
+
+    def _build_linear_operator(self, d):
+        lap = sp.build_laplace_operator(d, order=2)
+        return np.concatenate([self.nu1 * lap, self.nu2 * lap])
+
+    def _build_nonlinear_fun(self, d):
+        return nl.GrayScottNonlinearFun(self.num_spatial_dims, self.num_points,
+                                        dealiasing_fraction=self.dealiasing_fraction, feed_rate=self.feed_rate,
+                                        kill_rate=self.kill_rate, dtype=self._kw_dtype)
+
+
+class BelousovZhabotinsky(BaseStepper):
+    """``stepper/reaction/_belousov_zhabotinsky.py:50-105``: L = diag(nu_i Laplace)."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivities=(1e-5, 2e-5, 1e-5), order=2,
+                 dealiasing_fraction=1 / 2, num_circle_points=16, circle_radius=1.0, **kw):
+        self.diffusivities = tuple(diffusivities)
+        self.dealiasing_fraction = dealiasing_fraction
+        self._kw_dtype = kw.get("dtype", np.float64)
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=3, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius, **kw)
+
+    def _build_linear_operator(self, d):
+        lap = sp.build_laplace_operator(d, order=2)
+        return np.concatenate([nu * lap for nu in self.diffusivities])
+
+    def _build_nonlinear_fun(self, d):
+        return nl.BelousovZhabotinskyNonlinearFun(self.num_spatial_dims, self.num_points,
+                                                  dealiasing_fraction=self.dealiasing_fraction,
+                                                  dtype=self._kw_dtype)
+
+
 class NavierStokesVorticity(BaseStepper):
     """``stepper/_navier_stokes.py:13-153``."""
 

@@ -358,3 +358,48 @@ def test_cahn_hilliard_linearised_growth(D):
     assert abs(amp / (eps * np.exp(lam * n * dt)) - 1) < 1e-4
     # the wrong sign of the nonlinear Laplacian moves the rate by 2 * 3 nu kappa^2 c3 a^2
     assert abs(np.exp(6 * nu * kap**2 * c3 * a**2 * n * dt) - 1) > 1e-3
+
+
+def _uniform_channels(stepper, values, n):
+    u = np.stack([np.full(stepper.num_points, v) for v in values])
+    for _ in range(n):
+        u = stepper.step(u)
+    assert np.ptp(u, axis=1).max() < 1e-12
+    return u[:, 0]
+
+
+def test_gray_scott_uniform_ode():
+    from scipy.integrate import solve_ivp
+
+    f, k, dt, n, y0 = 0.04, 0.06, 0.01, 200, (0.6, 0.3)
+    got = _uniform_channels(st.GrayScott(1, 1.0, 16, dt, feed_rate=f, kill_rate=k), y0, n)
+    rhs = lambda t, y: [f * (1 - y[0]) - y[0] * y[1] ** 2, -(f + k) * y[1] + y[0] * y[1] ** 2]  # noqa: E731
+    ref = solve_ivp(rhs, (0.0, n * dt), list(y0), rtol=1e-12, atol=1e-14).y[:, -1]
+    np.testing.assert_allclose(got, ref, atol=1e-7)
+
+
+def test_gray_scott_linearised_decay_per_channel():
+    """Around (1, 0) a small cosine decays at nu1 kappa^2 + f in channel 0 and
+    nu2 kappa^2 + f + k in channel 1 (per-channel diffusivities)."""
+    L, N, dt, n, nu1, nu2, f, k, eps, m = 1.0, 32, 0.01, 100, 2e-3, 5e-4, 0.04, 0.06, 1e-7, 3
+    x = _grid(L, N)
+    kap = 2 * np.pi * m / L
+    c = np.cos(kap * x)
+    u = np.stack([1 + eps * c, eps * c])
+    gs = st.GrayScott(1, L, N, dt, diffusivity_1=nu1, diffusivity_2=nu2, feed_rate=f, kill_rate=k)
+    for _ in range(n):
+        u = gs.step(u)
+    t = n * dt
+    amp = 2 * np.abs(np.fft.rfft(u, axis=1)[:, m]) / N
+    np.testing.assert_allclose(amp / eps, [np.exp(-(nu1 * kap**2 + f) * t), np.exp(-(nu2 * kap**2 + f + k) * t)],
+                               rtol=1e-5)
+
+
+def test_belousov_zhabotinsky_uniform_ode():
+    from scipy.integrate import solve_ivp
+
+    dt, n, y0 = 0.005, 200, (0.5, 0.2, 0.3)
+    got = _uniform_channels(st.BelousovZhabotinsky(1, 1.0, 16, dt), y0, n)
+    rhs = lambda t, y: [y[0] + y[1] - y[0] * y[1] - y[0] ** 2, y[2] - y[1] - y[0] * y[1], y[0] - y[2]]  # noqa: E731
+    ref = solve_ivp(rhs, (0.0, n * dt), list(y0), rtol=1e-12, atol=1e-14).y[:, -1]
+    np.testing.assert_allclose(got, ref, atol=2e-6)
