@@ -28,6 +28,11 @@ hipError_t launch_coef(const double* L, const double* Li, long long total, int n
                        double radius,
                        float* out, hipStream_t st);
 
+hipError_t launch_lines(int N, bool inv, cf* data, long long nlines, long long inner, long long outer_stride,
+                        long long pstride, float scale, const cf* tw, hipStream_t st);
+hipError_t launch_etdrk0_spec(cf* X, long long ninst, long long nr, int channels, const float* tables, int ncoef,
+                              int lin_batch, int lin_channels, int e_im, hipStream_t st);
+
 #define EXMI_DECL_SIZE(M)                                                                              \
   StepFn get_step1d_##M(int kind, int order);                                                          \
   RollFn get_roll1d_##M(int kind, int order, int nt, int fr, int std_kc);                              \
@@ -301,8 +306,10 @@ struct exmi_plan {
   int cplx = 0;  // 2-D: complex linear operator (Im E follows the real coefficients)
   int coupled = 0;  // 1-D channel-coupled reaction: steps through the multi-step kernel
   int nf = 0, kc = 0;
-  int ninst = 0;          // 2-D: instances = batch * channels
+  int ninst = 0;          // 2-D and 3-D: instances = batch * channels
   long long tstride = 0;  // 2-D: complex entries of T per instance
+  cf* W3 = nullptr;       // 3-D: spectra of every instance
+  float* P3 = nullptr;    // 3-D: contiguous staging of strided physical states
   Launch2D row2d[3] = {nullptr, nullptr, nullptr};
   Launch2D col2d[COL_NUM_MODES] = {};
   Twiddles tw_col;
@@ -618,6 +625,132 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
 // One 2-D ETDRK2 step: u_in (stride ibs) -> u_out (stride obs). With
 // have_T the row spectra of u_in are already in T (rollout); with write_T
 // the last launch also leaves the new state's row spectra in T.
+// floats of one state (C, N^D)
+static long long state_size(const exmi_plan* p) {
+  long long s = p->d.num_channels;
+  for (int i = 0; i < p->ndim; ++i) s *= p->d.num_points;
+  return s;
+}
+
+// ----------------------------------------------------------------- 3-D
+static bool size_3d(int N) { return pow2(N) && N >= 16 && N <= 256; }
+static hipError_t copy_batch(const float* src, long long ss, float* dst, long long ds, long long n, int B,
+                             hipStream_t st);
+
+// rfftn of I contiguous N^3 fields x into spectra X (I, N, N, N/2+1): z rows
+// (r2c), then y lines and x lines (c2c) in place.
+static hipError_t rfft3(int N, long long I, const float* x, cf* X, const Twiddles& tz, const Twiddles& tl,
+                        hipStream_t st) {
+  const long long NZ = N / 2 + 1;
+  hipError_t e = launch_fft1d(N / 2, false, x, (float*)X, (int)(I * N * N), tz.rt(), st);
+  if (e == hipSuccess) e = launch_lines(N, false, X, I * N * NZ, NZ, N * NZ, NZ, 1.f, tl.tw, st);
+  if (e == hipSuccess) e = launch_lines(N, false, X, I * N * NZ, N * NZ, (long long)N * N * NZ, N * NZ, 1.f, tl.tw, st);
+  return e;
+}
+
+// irfftn: the x and y inverse lines in place on X (each scaled by 1/N), then
+// the z rows c2r (scaled by 1/N, Im of kz = 0 and N/2 ignored) into x.
+static hipError_t irfft3(int N, long long I, cf* X, float* x, const Twiddles& tz, const Twiddles& tl,
+                         hipStream_t st) {
+  const long long NZ = N / 2 + 1;
+  const float r = 1.f / (float)N;
+  hipError_t e = launch_lines(N, true, X, I * N * NZ, N * NZ, (long long)N * N * NZ, N * NZ, r, tl.tw, st);
+  if (e == hipSuccess) e = launch_lines(N, true, X, I * N * NZ, NZ, N * NZ, NZ, r, tl.tw, st);
+  if (e == hipSuccess) e = launch_fft1d(N / 2, true, (const float*)X, x, (int)(I * N * N), tz.rt(), st);
+  return e;
+}
+
+// 3-D plans: ETDRK0 (the linear steppers) with real or complex L.
+static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
+  const int N = d->num_points, C = d->num_channels;
+  if (!size_3d(N))
+    return fail(EXMI_ERR_UNSUPPORTED, "3-D num_points=%d: supported sizes are powers of two 16..256", N);
+  if (C < 1 || d->batch < 1) return fail(EXMI_ERR_INVALID, "num_channels and batch must be >= 1");
+  if (d->order != 0)
+    return fail(EXMI_ERR_UNSUPPORTED, "3-D plans are built for ETDRK order 0 (linear steppers), got order %d",
+                d->order);
+  if (d->flags & EXMI_FLAG_FOURIER_RESIDENT)
+    return fail(EXMI_ERR_UNSUPPORTED, "Fourier-resident rollouts are built for 1-D plans");
+  if (d->lin_batch != 1 && d->lin_batch != d->batch)
+    return fail(EXMI_ERR_INVALID, "lin_batch must be 1 or batch (%d), got %d", d->batch, d->lin_batch);
+  if (d->lin_channels != 1 && d->lin_channels != C)
+    return fail(EXMI_ERR_INVALID, "lin_channels must be 1 or num_channels (%d), got %d", C, d->lin_channels);
+  if (!d->host_linear_operator) return fail(EXMI_ERR_INVALID, "host_linear_operator is null");
+  const bool cplx = d->host_linear_operator_imag != nullptr;
+  exmi_plan* p = new exmi_plan();
+  p->d = *d;
+  p->d.host_linear_operator = nullptr;
+  p->d.host_linear_operator_imag = nullptr;
+  p->ndim = 3;
+  p->M = N / 2;
+  p->nr = N * N * (N / 2 + 1);
+  p->ncoef = num_coefs_c(0, cplx);
+  p->cplx = cplx;
+  p->ninst = d->batch * C;
+  p->kscale = (float)(2.0 * 3.14159265358979323846 / d->domain_extent);
+  p->roll_nt = -1;  // per-step launches; no autotuning
+  hipGetDevice(&p->device);
+  int rc = get_twiddles(N / 2, &p->tw);
+  if (!rc) rc = get_twiddles(N, &p->tw_col);
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  const long long rows = (long long)d->lin_batch * d->lin_channels, total = rows * p->nr;
+  double* dL = nullptr;
+  bool ok = hipMalloc(&dL, sizeof(double) * total * (cplx ? 2 : 1)) == hipSuccess &&
+            hipMalloc(&p->tables, sizeof(float) * total * p->ncoef) == hipSuccess &&
+            hipMalloc(&p->W3, sizeof(cf) * p->ninst * p->nr) == hipSuccess &&
+            hipMalloc(&p->P3, sizeof(float) * p->ninst * (long long)N * N * N) == hipSuccess;
+  if (!ok) {
+    hipFree(dL);
+    exmi_plan_destroy(p);
+    return fail(EXMI_ERR_ALLOC, "3-D workspace allocation failed (batch %d, N %d)", d->batch, N);
+  }
+  hipError_t e = hipMemcpy(dL, d->host_linear_operator, sizeof(double) * total, hipMemcpyHostToDevice);
+  if (e == hipSuccess && cplx)
+    e = hipMemcpy(dL + total, d->host_linear_operator_imag, sizeof(double) * total, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_coef(dL, cplx ? dL + total : nullptr, total, (int)p->nr, d->dt, 0, d->num_circle_points,
+                    d->circle_radius, p->tables, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  hipFree(dL);
+  if (e != hipSuccess) {
+    exmi_plan_destroy(p);
+    return fail(EXMI_ERR_HIP, "coefficient formation failed: %s", hipGetErrorString(e));
+  }
+  if (hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&p->launch_stream, hipStreamNonBlocking) != hipSuccess) {
+    exmi_plan_destroy(p);
+    return fail(EXMI_ERR_HIP, "stream creation failed");
+  }
+  *out = p;
+  return EXMI_OK;
+}
+
+// One 3-D step u_in (batch stride ibs) -> u_out (obs): forward transform,
+// E multiply, inverse transform; strided states pass through the staging
+// buffer.
+static int enqueue_step3d(exmi_plan* p, const float* u_in, long long ibs, float* u_out, long long obs,
+                          hipStream_t st) {
+  const int N = p->d.num_points, B = p->d.batch;
+  const long long sz = (long long)p->d.num_channels * N * N * N;
+  hipError_t e = hipSuccess;
+  const float* src = u_in;
+  if (ibs != sz) {
+    e = copy_batch(u_in, ibs, p->P3, sz, sz, B, st);
+    src = p->P3;
+  }
+  if (e == hipSuccess) e = rfft3(N, p->ninst, src, p->W3, p->tw, p->tw_col, st);
+  if (e == hipSuccess)
+    e = launch_etdrk0_spec(p->W3, p->ninst, p->nr, p->d.num_channels, p->tables, p->ncoef, p->d.lin_batch,
+                           p->d.lin_channels, p->cplx ? num_coefs(0) : -1, st);
+  float* dst = obs == sz ? u_out : p->P3;
+  if (e == hipSuccess) e = irfft3(N, p->ninst, p->W3, dst, p->tw, p->tw_col, st);
+  if (e == hipSuccess && dst != u_out) e = copy_batch(p->P3, sz, u_out, obs, sz, B, st);
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "3-D step launch failed: %s", hipGetErrorString(e));
+}
+
 static int enqueue_step2d(exmi_plan* p, const float* u_in, long long ibs, float* u_out, long long obs, bool have_T,
                           bool write_T, hipStream_t st) {
   Step2DArgs a;
@@ -678,7 +811,8 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   *out = nullptr;
   const int D = d->num_spatial_dims, N = d->num_points;
   if (D == 2) return plan_create_2d(d, out);
-  if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "num_spatial_dims=%d: this build supports 1 and 2", D);
+  if (D == 3) return plan_create_3d(d, out);
+  if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "num_spatial_dims=%d: this build supports 1, 2 and 3", D);
   if (!size_1d(N))
     return fail(EXMI_ERR_UNSUPPORTED, "num_points=%d: supported sizes are powers of two 16..4096, 48 and 50", N);
   if (d->num_channels < 1 || d->batch < 1) return fail(EXMI_ERR_INVALID, "num_channels and batch must be >= 1");
@@ -812,6 +946,8 @@ int exmi_plan_destroy(exmi_plan* p) {
   if (p->launch_stream) hipStreamDestroy(p->launch_stream);
   hipFree(p->tables);
   hipFree(p->work);
+  hipFree(p->W3);
+  hipFree(p->P3);
   hipFree(p->S0);
   hipFree(p->S1);
   hipFree(p->N0);
@@ -882,6 +1018,7 @@ static int run_step(exmi_plan* p, const void* in, long long in_bs, int in_f, voi
 
 int exmi_step(exmi_plan* p, const float* u_in, int64_t in_bs, float* u_out, int64_t out_bs, void* stream) {
   if (!p || !u_in || !u_out) return fail(EXMI_ERR_INVALID, "null argument");
+  if (p->ndim == 3) return enqueue_step3d(p, u_in, in_bs, u_out, out_bs, (hipStream_t)stream);
   if (p->ndim == 2) return enqueue_step2d(p, u_in, in_bs, u_out, out_bs, false, false, (hipStream_t)stream);
   return run_step(p, u_in, in_bs, 0, u_out, out_bs, 0, 0, (hipStream_t)stream);
 }
@@ -889,14 +1026,14 @@ int exmi_step(exmi_plan* p, const float* u_in, int64_t in_bs, float* u_out, int6
 int exmi_step_fourier(exmi_plan* p, const float* uh_in, int64_t in_bs, float* uh_out, int64_t out_bs,
                       void* stream) {
   if (!p || !uh_in || !uh_out) return fail(EXMI_ERR_INVALID, "null argument");
-  if (p->ndim == 2) return fail(EXMI_ERR_UNSUPPORTED, "exmi_step_fourier: 2-D plans step physical states only");
+  if (p->ndim >= 2) return fail(EXMI_ERR_UNSUPPORTED, "exmi_step_fourier: 2-D and 3-D plans step physical states only");
   return run_step(p, uh_in, in_bs, 1, uh_out, out_bs, 1, 0, (hipStream_t)stream);
 }
 
 int exmi_nonlinear(exmi_plan* p, const float* uh_in, int64_t in_bs, float* nh_out, int64_t out_bs, void* stream) {
   if (!p || !uh_in || !nh_out) return fail(EXMI_ERR_INVALID, "null argument");
   if (p->d.order == 0) return fail(EXMI_ERR_INVALID, "order-0 plan has no nonlinear function");
-  if (p->ndim == 2) return fail(EXMI_ERR_UNSUPPORTED, "exmi_nonlinear: not built for 2-D plans");
+  if (p->ndim >= 2) return fail(EXMI_ERR_UNSUPPORTED, "exmi_nonlinear: not built for 2-D and 3-D plans");
   return run_step(p, uh_in, in_bs, 1, nh_out, out_bs, 1, 1, (hipStream_t)stream);
 }
 
@@ -937,7 +1074,7 @@ static int enqueue_roll(exmi_plan* p, int nt, const float* u0, long long u0_bs, 
 // nt transforms per lane group.
 static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u0_bs, float* traj, int n,
                               int include_init, hipStream_t st) {
-  const long long sz = (long long)p->d.num_channels * p->d.num_points * (p->ndim == 2 ? p->d.num_points : 1);
+  const long long sz = state_size(p);
   const long long tstride = (long long)(n + (include_init ? 1 : 0)) * sz;
   int off = 0;
   if (include_init) {
@@ -946,6 +1083,15 @@ static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u
     off = 1;
   }
   if (n == 0) return EXMI_OK;
+  if (p->ndim == 3) {
+    for (int t = 0; t < n; ++t) {
+      const float* in = (t == 0 && !include_init) ? u0 : traj + (long long)(t - 1 + off) * sz;
+      const long long ibs = (t == 0 && !include_init) ? u0_bs : tstride;
+      int rc = enqueue_step3d(p, in, ibs, traj + (long long)(t + off) * sz, tstride, st);
+      if (rc) return rc;
+    }
+    return EXMI_OK;
+  }
   if (p->ndim == 2) {
     for (int t = 0; t < n; ++t) {
       int rc = enqueue_step2d(p, u0, u0_bs, traj + (long long)(t + off) * sz, tstride, t > 0, t < n - 1, st);
@@ -1113,10 +1259,17 @@ static int enq_rollout_cb(exmi_plan* p, const void* a, const void* b, long long 
 static int enq_repeat_cb(exmi_plan* p, const void* a, const void* b, long long s, int n, int, hipStream_t st) {
   const float* u0 = (const float*)a;
   float* out = (float*)b;
-  const long long sz = (long long)p->d.num_channels * p->d.num_points * (p->ndim == 2 ? p->d.num_points : 1);
+  const long long sz = state_size(p);
   if (n == 0) {
     hipError_t e = copy_batch(u0, s, out, sz, sz, p->d.batch, st);
     return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "copy failed: %s", hipGetErrorString(e));
+  }
+  if (p->ndim == 3) {
+    for (int t = 0; t < n; ++t) {
+      int rc = enqueue_step3d(p, t == 0 ? u0 : out, t == 0 ? s : sz, out, sz, st);
+      if (rc) return rc;
+    }
+    return EXMI_OK;
   }
   if (p->ndim == 2) {
     for (int t = 0; t < n; ++t) {
@@ -1337,6 +1490,16 @@ extern "C" {
 
 int exmi_rfft(int32_t D, int32_t N, int32_t count, const float* x, float* x_hat, void* stream) {
   if (D == 2) return count <= 0 ? EXMI_OK : transform2d(false, N, count, x, (cf*)x_hat, (hipStream_t)stream);
+  if (D == 3) {
+    if (!size_3d(N)) return fail(EXMI_ERR_UNSUPPORTED, "exmi_rfft: 3-D num_points=%d (powers of two 16..256)", N);
+    if (count <= 0) return EXMI_OK;
+    Twiddles tz, tl;
+    int rc = get_twiddles(N / 2, &tz);
+    if (!rc) rc = get_twiddles(N, &tl);
+    if (rc) return rc;
+    hipError_t e = rfft3(N, count, x, (cf*)x_hat, tz, tl, (hipStream_t)stream);
+    return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "3-D rfft launch failed: %s", hipGetErrorString(e));
+  }
   if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "exmi_rfft: num_spatial_dims=%d not supported", D);
   if (!size_1d(N)) return fail(EXMI_ERR_UNSUPPORTED, "exmi_rfft: num_points=%d", N);
   if (count <= 0) return EXMI_OK;
@@ -1350,6 +1513,24 @@ int exmi_rfft(int32_t D, int32_t N, int32_t count, const float* x, float* x_hat,
 int exmi_irfft(int32_t D, int32_t N, int32_t count, const float* x_hat, float* x, void* stream) {
   if (D == 2)
     return count <= 0 ? EXMI_OK : transform2d(true, N, count, x, (cf*)const_cast<float*>(x_hat), (hipStream_t)stream);
+  if (D == 3) {
+    if (!size_3d(N)) return fail(EXMI_ERR_UNSUPPORTED, "exmi_irfft: 3-D num_points=%d (powers of two 16..256)", N);
+    if (count <= 0) return EXMI_OK;
+    Twiddles tz, tl;
+    int rc = get_twiddles(N / 2, &tz);
+    if (!rc) rc = get_twiddles(N, &tl);
+    if (rc) return rc;
+    // the inverse line transforms run in place: on a stream-ordered copy
+    const hipStream_t st = (hipStream_t)stream;
+    const size_t bytes = sizeof(cf) * (size_t)count * N * N * (N / 2 + 1);
+    cf* tmp = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&tmp, bytes, st));
+    hipError_t e = hipMemcpyAsync(tmp, x_hat, bytes, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = irfft3(N, count, tmp, x, tz, tl, st);
+    const hipError_t ef = hipFreeAsync(tmp, st);
+    if (e == hipSuccess) e = ef;
+    return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "3-D irfft launch failed: %s", hipGetErrorString(e));
+  }
   if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "exmi_irfft: num_spatial_dims=%d not supported", D);
   if (!size_1d(N)) return fail(EXMI_ERR_UNSUPPORTED, "exmi_irfft: num_points=%d", N);
   if (count <= 0) return EXMI_OK;
@@ -1551,7 +1732,7 @@ int exmi_rollout_vjp(exmi_plan* p, const float* u0, int64_t u0_bs, int32_t n, in
                                       "gradient norm; 2-D: gradient norm)");
   hipStream_t st = (hipStream_t)stream;
   const int B = p->d.batch;
-  const long long sz = (long long)p->d.num_channels * p->d.num_points * (p->ndim == 2 ? p->d.num_points : 1);
+  const long long sz = state_size(p);
   const long long bsz = (long long)B * sz;
   const int off = include_init ? 1 : 0;
   const long long gts = (long long)(n + off) * sz;  // g_traj batch stride

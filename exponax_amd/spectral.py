@@ -105,13 +105,13 @@ def fft(field, *, num_spatial_dims=None):
     t, back = as_device_tensor(field)
     if num_spatial_dims is None:
         num_spatial_dims = t.dim() - 1
-    if num_spatial_dims not in (1, 2):
-        raise NotImplementedError("fft: num_spatial_dims 1 and 2 are built")
+    if num_spatial_dims not in (1, 2, 3):
+        raise NotImplementedError("fft: num_spatial_dims 1, 2 and 3 are built")
     import torch
 
     n = t.shape[-1]
-    if num_spatial_dims == 2 and t.shape[-2] != n:
-        raise ValueError(f"expected a square grid on the last two axes, got {tuple(t.shape[-2:])}")
+    if any(s != n for s in t.shape[t.dim() - num_spatial_dims:]):
+        raise ValueError(f"expected N points on each of the last {num_spatial_dims} axes, got {tuple(t.shape)}")
     x = t.contiguous().to(torch.float32)
     count = x.numel() // n ** num_spatial_dims
     out = torch.empty(x.shape[:-1] + (n // 2 + 1,), dtype=torch.complex64, device=x.device)
@@ -130,16 +130,16 @@ def ifft(field_hat, *, num_spatial_dims=None, num_points=None):
             num_points = t.shape[-2]
         else:
             raise ValueError("num_points must be provided if num_spatial_dims == 1.")
-    if num_spatial_dims not in (1, 2):
-        raise NotImplementedError("ifft: num_spatial_dims 1 and 2 are built")
+    if num_spatial_dims not in (1, 2, 3):
+        raise NotImplementedError("ifft: num_spatial_dims 1, 2 and 3 are built")
     import torch
 
     if t.shape[-1] != num_points // 2 + 1:
         raise ValueError(f"expected {num_points // 2 + 1} modes on the last axis, got {t.shape[-1]}")
-    if num_spatial_dims == 2 and t.shape[-2] != num_points:
-        raise ValueError(f"expected {num_points} modes on the second-to-last axis, got {t.shape[-2]}")
+    if any(s != num_points for s in t.shape[t.dim() - num_spatial_dims:-1]):
+        raise ValueError(f"expected {num_points} modes on the leading spectral axes, got {tuple(t.shape)}")
     z = t.contiguous().to(torch.complex64)
-    count = z.numel() // (z.shape[-1] * (num_points if num_spatial_dims == 2 else 1))
+    count = z.numel() // (z.shape[-1] * num_points ** (num_spatial_dims - 1))
     out = torch.empty(z.shape[:-1] + (num_points,), dtype=torch.float32, device=z.device)
     _lib.check(_lib.lib().exmi_irfft(num_spatial_dims, num_points, count, z.data_ptr(), out.data_ptr(),
                                      _lib.stream_ptr()), "exmi_irfft")

@@ -89,7 +89,7 @@ enum exmi_nonlin_kind {
  * (per-trajectory coefficients, the eqx.filter_vmap case).
  */
 typedef struct exmi_plan_desc {
-  int32_t num_spatial_dims;   /* D: 1 or 2                                   */
+  int32_t num_spatial_dims;   /* D: 1, 2 or 3 (3: ETDRK0, N 16..256)         */
   int32_t num_points;         /* N: 1-D powers of two 16..4096, 48 and 50;
                                  2-D powers of two 32..512 (any nonlinear
                                  kind; two channels for the velocity
@@ -231,7 +231,8 @@ int exmi_rollout_vjp(exmi_plan* plan, const float* u0, int64_t u0_bstride, int32
  * last D axes, norm="backward"; inverse irfftn with 1/N^D scaling, Im of the
  * last axis' DC/Nyquist ignored. `count` transforms, contiguous: real
  * (count, N^D), spectra (count, N^(D-1), N/2+1) in the reference's axis
- * order. D = 1: powers of two 16..4096, 48 and 50. D = 2: powers of two
+ * order. D = 3: powers of two 16..256 (the inverse through a stream-ordered
+ * copy of the spectra). D = 1: powers of two 16..4096, 48 and 50. D = 2: powers of two
  * 32..512, count <= 65535, through a stream-ordered (hipMallocAsync)
  * intermediate of the spectra's size.
  * Replace exponax.fft / exponax.ifft (_spectral.py:614-656, 659-721). */

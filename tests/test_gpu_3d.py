@@ -1,0 +1,89 @@
+"""3-D on the GPU: rfftn / irfftn over the last three axes (_spectral.py:
+614-721; z rows r2c, then y and x line FFTs) against numpy, and the linear
+steppers in 3-D (ETDRK0, real and complex exp(L dt)) against the FP32 oracle
+at steps 1 and 10 through single steps and the rollout, plus exact
+transport."""
+import numpy as np
+import pytest
+
+import exponax_amd as ex
+from oracle import stepper as ost
+from tests.helpers import RTOL_STEP, ics, max_rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("N", [16, 32, 64, 128])
+def test_fft3d_against_numpy(device, N):
+    import torch
+
+    print(f"[case] fft3d N={N}", flush=True)
+    rng = np.random.default_rng(N)
+    u = rng.standard_normal((2, 1, N, N, N)).astype(np.float32)
+    uh = ex.spectral.fft(torch.tensor(u, device=device), num_spatial_dims=3)
+    want = np.fft.rfftn(u.astype(np.float64), axes=(-3, -2, -1))
+    err = np.abs(uh.cpu().numpy() - want).max() / np.abs(want).max()
+    back = ex.spectral.ifft(uh, num_spatial_dims=3).cpu().numpy()
+    shp = (2, N, N, N // 2 + 1)
+    z = (rng.standard_normal(shp) + 1j * rng.standard_normal(shp)).astype(np.complex64)
+    got_i = ex.spectral.ifft(torch.tensor(z, device=device), num_spatial_dims=3).cpu().numpy()
+    want_i = np.fft.irfftn(z.astype(np.complex128), s=(N, N, N), axes=(-3, -2, -1))
+    err_i = np.abs(got_i - want_i).max() / np.abs(want_i).max()
+    print(f"  rfftn err {err:.2e}, roundtrip {np.abs(back - u).max():.2e}, irfftn err {err_i:.2e}", flush=True)
+    assert err <= 2e-6 and err_i <= 2e-6
+    np.testing.assert_allclose(back, u, atol=2e-5)
+
+
+def _pair(name, N):
+    L = 2.0
+    if name == "advection":
+        kw = dict(velocity=np.array([0.8, -0.3, 0.5]))
+        return ex.stepper.Advection(3, L, N, 0.01, **kw), ost.Advection(3, L, N, 0.01, dtype=np.float32, **kw)
+    if name == "advection_diffusion":
+        kw = dict(velocity=-0.5, diffusivity=0.01)
+        return (ex.stepper.AdvectionDiffusion(3, L, N, 0.01, **kw),
+                ost.AdvectionDiffusion(3, L, N, 0.01, dtype=np.float32, **kw))
+    if name == "dispersion":
+        return (ex.stepper.Dispersion(3, L, N, 1e-4, dispersivity=0.2),
+                ost.Dispersion(3, L, N, 1e-4, dispersivity=0.2, dtype=np.float32))
+    if name == "diffusion":
+        return (ex.stepper.Diffusion(3, L, N, 0.01, diffusivity=0.02),
+                ost.Diffusion(3, L, N, 0.01, diffusivity=0.02, dtype=np.float32))
+    return (ex.stepper.HyperDiffusion(3, L, N, 0.01, hyper_diffusivity=1e-4),
+            ost.HyperDiffusion(3, L, N, 0.01, hyper_diffusivity=1e-4, dtype=np.float32))
+
+
+@pytest.mark.parametrize("N", [16, 32])
+@pytest.mark.parametrize("name", ["advection", "advection_diffusion", "dispersion", "diffusion", "hyper_diffusion"])
+def test_linear_3d_parity(device, name, N):
+    import torch
+
+    print(f"[case] 3-D {name} N={N}", flush=True)
+    gpu, cpu = _pair(name, N)
+    u0, _ = ics(2, 3, 2.0, N, 3, seed=101)
+    x = torch.tensor(u0, device=device)
+    traj = gpu.rollout_batch(x, 10).cpu().numpy()
+    ug, uc = x, u0
+    for s in range(1, 11):
+        ug = gpu.step(ug)
+        uc = cpu.step_batch(uc)
+        if s in (1, 10):
+            e_step = max_rel_l2(ug.cpu().numpy(), uc)
+            e_roll = max_rel_l2(traj[:, s - 1], uc)
+            print(f"  step {s}: steps {e_step:.2e}, rollout {e_roll:.2e}", flush=True)
+            assert e_step <= RTOL_STEP and e_roll <= RTOL_STEP
+
+
+def test_advection_3d_exact_transport(device):
+    import torch
+
+    print("[case] 3-D exact transport", flush=True)
+    L, N, dt, c = 2.0, 32, 0.05, np.array([0.7, -0.4, 0.3])
+    x = np.linspace(0.0, L, N, endpoint=False)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    f = lambda x, y, z: np.sin(2 * np.pi * (2 * x + y - 3 * z) / L)  # noqa: E731
+    st = ex.stepper.Advection(3, L, N, dt, velocity=c)
+    u = st.rollout_batch(torch.tensor(f(X, Y, Z)[None, None], dtype=torch.float32, device=device), 20)
+    got = u[0, -1, 0].cpu().numpy()
+    t = 20 * dt
+    np.testing.assert_allclose(got, f(X - c[0] * t, Y - c[1] * t, Z - c[2] * t), atol=2e-5)

@@ -403,3 +403,17 @@ def test_belousov_zhabotinsky_uniform_ode():
     rhs = lambda t, y: [y[0] + y[1] - y[0] * y[1] - y[0] ** 2, y[2] - y[1] - y[0] * y[1], y[0] - y[2]]  # noqa: E731
     ref = solve_ivp(rhs, (0.0, n * dt), list(y0), rtol=1e-12, atol=1e-14).y[:, -1]
     np.testing.assert_allclose(got, ref, atol=2e-6)
+
+
+def test_linear_3d_exact():
+    """3-D advection and diffusion against their closed forms
+    (tests/test_validation.py:13-211 covers 1/2/3-D)."""
+    L, N, dt, c, nu = 2.0, 16, 0.1, (0.7, -0.4, 0.3), 0.05
+    x = _grid(L, N)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    f = lambda x, y, z: np.sin(2 * np.pi * (2 * x + y - 3 * z) / L)  # noqa: E731
+    u = st.Advection(3, L, N, dt, velocity=np.array(c)).step(f(X, Y, Z)[None])
+    np.testing.assert_allclose(u[0], f(X - c[0] * dt, Y - c[1] * dt, Z - c[2] * dt), atol=1e-12)
+    k2 = (2 * np.pi / L) ** 2 * (4 + 1 + 9)
+    u = st.Diffusion(3, L, N, dt, diffusivity=nu).step(f(X, Y, Z)[None])
+    np.testing.assert_allclose(u[0], np.exp(-nu * k2 * dt) * f(X, Y, Z), atol=1e-12)
