@@ -32,6 +32,29 @@ hipError_t launch_lines(int N, bool inv, cf* data, long long nlines, long long i
                         long long pstride, float scale, const cf* tw, hipStream_t st);
 hipError_t launch_etdrk0_spec(cf* X, long long ninst, long long nr, int channels, const float* tables, int ncoef,
                               int lin_batch, int lin_channels, int e_im, hipStream_t st);
+struct NL3DArgs {
+  int N, NZ, channels, kind, kmax, nf, np;
+  long long Nr, N3, ninst;
+  float kscale, scale, g0, g1, g2, feed, kill;
+  int zero_mode_fix, npoly, poly_lap;
+  float poly[EXMI_MAX_POLY];
+};
+enum Stage3D { ST1 = 0, ST2A, ST2B, ST3A, ST3B, ST3C, ST4A, ST4B, ST4C, ST4D };
+struct Stage3DArgs {
+  long long ninst, nr;
+  int channels, ncoef, lin_batch, lin_channels, e_im, mode;
+  const float* tables;
+  const cf* U0;
+  const cf* N0;
+  const cf* NC;
+  cf* S;
+  cf* S1;
+  cf* A;
+};
+hipError_t launch_fields3d(const NL3DArgs& a, const cf* S, cf* F, hipStream_t st);
+hipError_t launch_products3d(const NL3DArgs& a, const float* R, float* PR, hipStream_t st);
+hipError_t launch_combine3d(const NL3DArgs& a, const cf* PH, cf* Nh, hipStream_t st);
+hipError_t launch_stage3d(const Stage3DArgs& a, hipStream_t st);
 
 #define EXMI_DECL_SIZE(M)                                                                              \
   StepFn get_step1d_##M(int kind, int order);                                                          \
@@ -308,8 +331,12 @@ struct exmi_plan {
   int nf = 0, kc = 0;
   int ninst = 0;          // 2-D and 3-D: instances = batch * channels
   long long tstride = 0;  // 2-D: complex entries of T per instance
-  cf* W3 = nullptr;       // 3-D: spectra of every instance
+  cf* W3 = nullptr;       // 3-D: spectra of every instance (the step's U0)
   float* P3 = nullptr;    // 3-D: contiguous staging of strided physical states
+  // 3-D nonlinear steps: stage spectra, nonlinear terms and evaluation buffers
+  cf *N03 = nullptr, *NC3 = nullptr, *S3 = nullptr, *S13 = nullptr, *A3 = nullptr, *F3 = nullptr;
+  float *R3 = nullptr, *PR3 = nullptr;
+  int nf3 = 0, np3 = 0, kind3 = 0;
   Launch2D row2d[3] = {nullptr, nullptr, nullptr};
   Launch2D col2d[COL_NUM_MODES] = {};
   Twiddles tw_col;
@@ -660,15 +687,34 @@ static hipError_t irfft3(int N, long long I, cf* X, float* x, const Twiddles& tz
   return e;
 }
 
-// 3-D plans: ETDRK0 (the linear steppers) with real or complex L.
+// 3-D plans: ETDRK0-4 (complex L for orders 0-2) with the polynomial
+// (Cahn-Hilliard), gradient-norm, three-channel convection, general or
+// coupled-reaction nonlinearity (or none).
 static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
   const int N = d->num_points, C = d->num_channels;
   if (!size_3d(N))
     return fail(EXMI_ERR_UNSUPPORTED, "3-D num_points=%d: supported sizes are powers of two 16..256", N);
   if (C < 1 || d->batch < 1) return fail(EXMI_ERR_INVALID, "num_channels and batch must be >= 1");
-  if (d->order != 0)
-    return fail(EXMI_ERR_UNSUPPORTED, "3-D plans are built for ETDRK order 0 (linear steppers), got order %d",
-                d->order);
+  if (d->order < 0 || d->order > 4) return fail(EXMI_ERR_UNSUPPORTED, "Order %d not implemented.", d->order);
+  if (d->host_linear_operator_imag && d->order > 2)
+    return fail(EXMI_ERR_UNSUPPORTED, "3-D plans with a complex linear operator are built for ETDRK orders 0-2");
+  int kind = NL_ZERO;
+  bool conv_general = false;
+  if (d->order > 0) {
+    if (d->nonlin_kind == EXMI_NL_ZERO) {
+      kind = NL_POLY;  // the empty polynomial
+    } else if (d->nonlin_kind == EXMI_NL_CONVECTION && d->conv_single_channel) {
+      if (C != 1) return fail(EXMI_ERR_UNSUPPORTED, "single-channel convection needs num_channels == 1");
+      if (!d->conv_conservative)
+        return fail(EXMI_ERR_UNSUPPORTED, "3-D single-channel convection is built in the conservative form");
+      kind = NL_GENERAL;
+      conv_general = true;
+    } else {
+      int rc = kernel_kind(d, &kind);
+      if (rc) return rc;
+      if (kind == NL_VORT) return fail(EXMI_ERR_INVALID, "vorticity convection is two-dimensional");
+    }
+  }
   if (d->flags & EXMI_FLAG_FOURIER_RESIDENT)
     return fail(EXMI_ERR_UNSUPPORTED, "Fourier-resident rollouts are built for 1-D plans");
   if (d->lin_batch != 1 && d->lin_batch != d->batch)
@@ -684,9 +730,20 @@ static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
   p->ndim = 3;
   p->M = N / 2;
   p->nr = N * N * (N / 2 + 1);
-  p->ncoef = num_coefs_c(0, cplx);
+  p->ncoef = num_coefs_c(d->order, cplx);
   p->cplx = cplx;
   p->ninst = d->batch * C;
+  p->kind3 = kind;
+  p->kmax = dealias_kmax(N, d->dealiasing_fraction);
+  if (d->order > 0 && d->nonlin_kind == EXMI_NL_ZERO) p->d.num_poly = 0;
+  if (d->nonlin_kind == EXMI_NL_CAHN_HILLIARD) cahn_hilliard_poly(&p->d);
+  if (conv_general) {
+    p->d.general_scales[0] = 0.0;
+    p->d.general_scales[1] = -d->scale;
+    p->d.general_scales[2] = 0.0;
+  }
+  p->nf3 = kind == NL_GRADNORM ? 3 : (kind == NL_CONV_NONCONS || kind == NL_GENERAL) ? 4 : 1;
+  p->np3 = kind == NL_CONV_CONS ? 3 : kind == NL_GENERAL ? 2 : 1;
   p->kscale = (float)(2.0 * 3.14159265358979323846 / d->domain_extent);
   p->roll_nt = -1;  // per-step launches; no autotuning
   hipGetDevice(&p->device);
@@ -698,10 +755,20 @@ static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
   }
   const long long rows = (long long)d->lin_batch * d->lin_channels, total = rows * p->nr;
   double* dL = nullptr;
+  const long long I = p->ninst, n3 = (long long)N * N * N, nfp = p->nf3 > p->np3 ? p->nf3 : p->np3;
+  const bool nl = d->order > 0;
   bool ok = hipMalloc(&dL, sizeof(double) * total * (cplx ? 2 : 1)) == hipSuccess &&
             hipMalloc(&p->tables, sizeof(float) * total * p->ncoef) == hipSuccess &&
-            hipMalloc(&p->W3, sizeof(cf) * p->ninst * p->nr) == hipSuccess &&
-            hipMalloc(&p->P3, sizeof(float) * p->ninst * (long long)N * N * N) == hipSuccess;
+            hipMalloc(&p->W3, sizeof(cf) * I * p->nr) == hipSuccess &&
+            hipMalloc(&p->P3, sizeof(float) * I * n3) == hipSuccess &&
+            (!nl || (hipMalloc(&p->N03, sizeof(cf) * I * p->nr) == hipSuccess &&
+                     hipMalloc(&p->NC3, sizeof(cf) * I * p->nr) == hipSuccess &&
+                     hipMalloc(&p->S3, sizeof(cf) * I * p->nr) == hipSuccess &&
+                     hipMalloc(&p->F3, sizeof(cf) * I * nfp * p->nr) == hipSuccess &&
+                     hipMalloc(&p->R3, sizeof(float) * I * p->nf3 * n3) == hipSuccess &&
+                     hipMalloc(&p->PR3, sizeof(float) * I * p->np3 * n3) == hipSuccess)) &&
+            (d->order < 3 || hipMalloc(&p->A3, sizeof(cf) * I * p->nr) == hipSuccess) &&
+            (d->order < 4 || hipMalloc(&p->S13, sizeof(cf) * I * p->nr) == hipSuccess);
   if (!ok) {
     hipFree(dL);
     exmi_plan_destroy(p);
@@ -711,7 +778,7 @@ static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
   if (e == hipSuccess && cplx)
     e = hipMemcpy(dL + total, d->host_linear_operator_imag, sizeof(double) * total, hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = launch_coef(dL, cplx ? dL + total : nullptr, total, (int)p->nr, d->dt, 0, d->num_circle_points,
+    e = launch_coef(dL, cplx ? dL + total : nullptr, total, (int)p->nr, d->dt, d->order, d->num_circle_points,
                     d->circle_radius, p->tables, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   hipFree(dL);
@@ -728,6 +795,40 @@ static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
   return EXMI_OK;
 }
 
+// N(X) of every instance into outN (the four passes of spectral3d.hip).
+static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t st) {
+  const int N = p->d.num_points;
+  NL3DArgs a;
+  memset(&a, 0, sizeof a);
+  a.N = N;
+  a.NZ = N / 2 + 1;
+  a.channels = p->d.num_channels;
+  a.kind = p->kind3;
+  a.kmax = p->kmax;
+  a.nf = p->nf3;
+  a.np = p->np3;
+  a.Nr = p->nr;
+  a.N3 = (long long)N * N * N;
+  a.ninst = p->ninst;
+  a.kscale = p->kscale;
+  a.scale = (float)p->d.scale;
+  a.g0 = (float)p->d.general_scales[0];
+  a.g1 = (float)p->d.general_scales[1];
+  a.g2 = (float)p->d.general_scales[2];
+  a.feed = (float)p->d.reaction_rates[0];
+  a.kill = (float)p->d.reaction_rates[1];
+  a.zero_mode_fix = p->d.zero_mode_fix;
+  a.npoly = p->d.num_poly;
+  for (int i = 0; i < EXMI_MAX_POLY; ++i) a.poly[i] = (float)p->d.poly[i];
+  a.poly_lap = p->d.nonlin_kind == EXMI_NL_CAHN_HILLIARD;
+  hipError_t e = launch_fields3d(a, X, p->F3, st);
+  if (e == hipSuccess) e = irfft3(N, p->ninst * p->nf3, p->F3, p->R3, p->tw, p->tw_col, st);
+  if (e == hipSuccess) e = launch_products3d(a, p->R3, p->PR3, st);
+  if (e == hipSuccess) e = rfft3(N, p->ninst * p->np3, p->PR3, p->F3, p->tw, p->tw_col, st);
+  if (e == hipSuccess) e = launch_combine3d(a, p->F3, outN, st);
+  return e;
+}
+
 // One 3-D step u_in (batch stride ibs) -> u_out (obs): forward transform,
 // E multiply, inverse transform; strided states pass through the staging
 // buffer.
@@ -742,11 +843,52 @@ static int enqueue_step3d(exmi_plan* p, const float* u_in, long long ibs, float*
     src = p->P3;
   }
   if (e == hipSuccess) e = rfft3(N, p->ninst, src, p->W3, p->tw, p->tw_col, st);
-  if (e == hipSuccess)
-    e = launch_etdrk0_spec(p->W3, p->ninst, p->nr, p->d.num_channels, p->tables, p->ncoef, p->d.lin_batch,
-                           p->d.lin_channels, p->cplx ? num_coefs(0) : -1, st);
+  cf* X = p->W3;  // the new state's spectrum
+  if (p->d.order == 0) {
+    if (e == hipSuccess)
+      e = launch_etdrk0_spec(p->W3, p->ninst, p->nr, p->d.num_channels, p->tables, p->ncoef, p->d.lin_batch,
+                             p->d.lin_channels, p->cplx ? num_coefs(0) : -1, st);
+  } else {
+    Stage3DArgs sa;
+    sa.ninst = p->ninst;
+    sa.nr = p->nr;
+    sa.channels = p->d.num_channels;
+    sa.ncoef = p->ncoef;
+    sa.lin_batch = p->d.lin_batch;
+    sa.lin_channels = p->d.lin_channels;
+    sa.e_im = p->cplx ? num_coefs(p->d.order) : -1;
+    sa.tables = p->tables;
+    sa.U0 = p->W3;
+    sa.N0 = p->N03;
+    sa.NC = p->NC3;
+    sa.S = p->S3;
+    sa.S1 = p->S13;
+    sa.A = p->A3;
+    auto stage = [&](int mode) {
+      if (e != hipSuccess) return;
+      sa.mode = mode;
+      e = launch_stage3d(sa, st);
+    };
+    auto nlev = [&](const cf* in, cf* outN) {
+      if (e == hipSuccess) e = nonlinear3d(p, in, outN, st);
+    };
+    // etdrk/_etdrk_{1,2,3,4}.py: each NC evaluation follows the stage that forms its argument S
+    static const int seq[4][7] = {{ST1, -1}, {ST2A, 1, ST2B, -1}, {ST3A, 1, ST3B, 1, ST3C, -1},
+                                  {ST4A, 1, ST4B, 1, ST4C, 1, ST4D}};
+    nlev(p->W3, p->N03);
+    for (int i = 0; i < 7; ++i) {
+      const int m = seq[p->d.order - 1][i];
+      if (m < 0) break;
+      if (i % 2 == 1) {
+        nlev(p->S3, p->NC3);
+      } else {
+        stage(m);
+      }
+    }
+    X = p->S3;
+  }
   float* dst = obs == sz ? u_out : p->P3;
-  if (e == hipSuccess) e = irfft3(N, p->ninst, p->W3, dst, p->tw, p->tw_col, st);
+  if (e == hipSuccess) e = irfft3(N, p->ninst, X, dst, p->tw, p->tw_col, st);
   if (e == hipSuccess && dst != u_out) e = copy_batch(p->P3, sz, u_out, obs, sz, B, st);
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "3-D step launch failed: %s", hipGetErrorString(e));
 }
@@ -948,6 +1090,14 @@ int exmi_plan_destroy(exmi_plan* p) {
   hipFree(p->work);
   hipFree(p->W3);
   hipFree(p->P3);
+  hipFree(p->N03);
+  hipFree(p->NC3);
+  hipFree(p->S3);
+  hipFree(p->S13);
+  hipFree(p->A3);
+  hipFree(p->F3);
+  hipFree(p->R3);
+  hipFree(p->PR3);
   hipFree(p->S0);
   hipFree(p->S1);
   hipFree(p->N0);

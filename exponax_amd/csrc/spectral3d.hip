@@ -106,4 +106,252 @@ hipError_t launch_etdrk0_spec(cf* X, long long ninst, long long nr, int channels
   return hipGetLastError();
 }
 
+
+// ------------------------------------------------------ 3-D nonlinear terms
+// One evaluation N(S) of a batch of 3-D spectra (instance i = channel i % C
+// of trajectory i / C) in four passes around the transforms:
+//   fields   : the dealiased spectral fields of each instance (the state and
+//              its derivatives i kappa_d S)        -> F (NF per instance)
+//   (irfftn of every field)                        -> R
+//   products : the pointwise products, reading the fields of the
+//              trajectory's other channels where the kind couples them
+//                                                  -> PR (NP per instance)
+//   (rfftn of every product)                       -> F
+//   combine  : derivative multiplies, scales, the post-dealias -> N
+// The kinds and their sign conventions follow the 1-D and 2-D kernels:
+// polynomial (nonlin_fun/_polynomial.py; with the Laplacian for
+// Cahn-Hilliard), gradient norm (_gradient_norm.py), the three-channel
+// convections (_convection.py:104-171), the general form
+// (_general_nonlinear.py) and the coupled reactions.
+struct NL3DArgs {
+  int N, NZ, channels, kind, kmax, nf, np;
+  long long Nr, N3, ninst;
+  float kscale, scale, g0, g1, g2, feed, kill;
+  int zero_mode_fix, npoly, poly_lap;
+  float poly[EXMI_MAX_POLY];
+};
+
+struct Mode3D {
+  float kx, ky, kz;  // kappa_d = (2 pi / L) k_d
+  bool keep;         // |k_d| <= kmax on every axis
+  bool zero;         // the mean mode
+};
+
+__device__ __forceinline__ Mode3D mode3d(const NL3DArgs& a, long long k) {
+  const int iz = (int)(k % a.NZ);
+  const int iy = (int)((k / a.NZ) % a.N);
+  const int ix = (int)(k / ((long long)a.NZ * a.N));
+  const int kx = ix < a.N / 2 ? ix : ix - a.N, ky = iy < a.N / 2 ? iy : iy - a.N;
+  Mode3D m;
+  m.kx = a.kscale * (float)kx;
+  m.ky = a.kscale * (float)ky;
+  m.kz = a.kscale * (float)iz;
+  m.keep = (kx < 0 ? -kx : kx) <= a.kmax && (ky < 0 ? -ky : ky) <= a.kmax && iz <= a.kmax;
+  m.zero = kx == 0 && ky == 0 && iz == 0;
+  return m;
+}
+
+__device__ __forceinline__ cf imul(float kap, cf w) { return cf{-kap * w.im, kap * w.re}; }
+
+__global__ void fields3d_kernel(NL3DArgs a, const cf* __restrict__ S, cf* __restrict__ F) {
+  const long long total = a.ninst * a.Nr;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const long long inst = idx / a.Nr, k = idx % a.Nr;
+    const Mode3D m = mode3d(a, k);
+    const cf w = m.keep ? S[idx] : cf{0.f, 0.f};
+    cf* f = F + inst * a.nf * a.Nr + k;
+    if (a.kind == NL_GRADNORM) {
+      f[0] = imul(m.kx, w);
+      f[a.Nr] = imul(m.ky, w);
+      f[2 * a.Nr] = imul(m.kz, w);
+    } else if (a.kind == NL_CONV_NONCONS || a.kind == NL_GENERAL) {
+      f[0] = w;
+      f[a.Nr] = imul(m.kx, w);
+      f[2 * a.Nr] = imul(m.ky, w);
+      f[3 * a.Nr] = imul(m.kz, w);
+    } else {
+      f[0] = w;
+    }
+  }
+}
+
+__global__ void products3d_kernel(NL3DArgs a, const float* __restrict__ R, float* __restrict__ PR) {
+  const long long total = a.ninst * a.N3;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const long long inst = idx / a.N3, p = idx % a.N3;
+    const long long base = inst - inst % a.channels;
+    const int ch = (int)(inst % a.channels);
+    auto fld = [&](long long i, int f) { return R[(i * a.nf + f) * a.N3 + p]; };
+    float* out = PR + inst * a.np * a.N3 + p;
+    switch (a.kind) {
+      case NL_POLY: {
+        const float u = fld(inst, 0);
+        float pw = 1.f, acc = 0.f;
+        for (int i = 0; i < a.npoly; ++i) {
+          acc += a.poly[i] * pw;
+          pw *= u;
+        }
+        out[0] = acc;
+        break;
+      }
+      case NL_GRADNORM: {
+        const float gx = fld(inst, 0), gy = fld(inst, 1), gz = fld(inst, 2);
+        out[0] = gx * gx + gy * gy + gz * gz;
+        break;
+      }
+      case NL_CONV_CONS: {
+        const float uc = fld(inst, 0);
+        for (int j = 0; j < 3; ++j) out[(long long)j * a.N3] = uc * fld(base + j, 0);
+        break;
+      }
+      case NL_CONV_NONCONS: {
+        float acc = 0.f;
+        for (int j = 0; j < 3; ++j) acc += fld(base + j, 0) * fld(inst, 1 + j);
+        out[0] = acc;
+        break;
+      }
+      case NL_GENERAL: {
+        const float u = fld(inst, 0), gx = fld(inst, 1), gy = fld(inst, 2), gz = fld(inst, 3);
+        out[0] = u * u;
+        out[a.N3] = gx * gx + gy * gy + gz * gz;
+        break;
+      }
+      case NL_GRAY_SCOTT: {
+        float r0, r1;
+        gray_scott(fld(base, 0), fld(base + 1, 0), a.feed, a.kill, r0, r1);
+        out[0] = ch == 0 ? r0 : r1;
+        break;
+      }
+      case NL_BZ: {
+        float r0, r1, r2;
+        belousov_zhabotinsky(fld(base, 0), fld(base + 1, 0), fld(base + 2, 0), r0, r1, r2);
+        out[0] = ch == 0 ? r0 : ch == 1 ? r1 : r2;
+        break;
+      }
+      default: out[0] = 0.f;
+    }
+  }
+}
+
+__global__ void combine3d_kernel(NL3DArgs a, const cf* __restrict__ PH, cf* __restrict__ Nh) {
+  const long long total = a.ninst * a.Nr;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const long long inst = idx / a.Nr, k = idx % a.Nr;
+    const Mode3D m = mode3d(a, k);
+    const cf* ph = PH + inst * a.np * a.Nr + k;
+    cf r = cf{0.f, 0.f};
+    switch (a.kind) {
+      case NL_POLY:
+        r = a.poly_lap ? (a.scale * -(m.kx * m.kx + m.ky * m.ky + m.kz * m.kz)) * ph[0] : ph[0];
+        break;
+      case NL_GRADNORM: r = (a.zero_mode_fix && m.zero) ? cf{0.f, 0.f} : (-0.5f * a.scale) * ph[0]; break;
+      case NL_CONV_CONS:
+        r = (-0.5f * a.scale) * (imul(m.kx, ph[0]) + imul(m.ky, ph[a.Nr]) + imul(m.kz, ph[2 * a.Nr]));
+        break;
+      case NL_CONV_NONCONS: r = -a.scale * ph[0]; break;
+      case NL_GENERAL: {
+        const float cg = (a.zero_mode_fix && m.zero) ? 0.f : 0.5f * a.g2;
+        r = a.g0 * ph[0] + imul(0.5f * a.g1 * (m.kx + m.ky + m.kz), ph[0]) + cg * ph[a.Nr];
+        break;
+      }
+      default: r = ph[0];
+    }
+    Nh[idx] = m.keep ? r : cf{0.f, 0.f};
+  }
+}
+
+static unsigned grid_of(long long total) {
+  long long g = (total + 255) / 256;
+  return (unsigned)(g > 65536 ? 65536 : g);
+}
+
+hipError_t launch_fields3d(const NL3DArgs& a, const cf* S, cf* F, hipStream_t st) {
+  hipLaunchKernelGGL(fields3d_kernel, dim3(grid_of(a.ninst * a.Nr)), dim3(256), 0, st, a, S, F);
+  return hipGetLastError();
+}
+hipError_t launch_products3d(const NL3DArgs& a, const float* R, float* PR, hipStream_t st) {
+  hipLaunchKernelGGL(products3d_kernel, dim3(grid_of(a.ninst * a.N3)), dim3(256), 0, st, a, R, PR);
+  return hipGetLastError();
+}
+hipError_t launch_combine3d(const NL3DArgs& a, const cf* PH, cf* Nh, hipStream_t st) {
+  hipLaunchKernelGGL(combine3d_kernel, dim3(grid_of(a.ninst * a.Nr)), dim3(256), 0, st, a, PH, Nh);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------ 3-D ETDRK stages
+// The stage combinations of etdrk/_etdrk_{1,2,3,4}.py on whole spectra,
+// coefficient c_i of the plan's tables (orders 1-2: [E, c1, c2] with Im E
+// at e_im for a complex L; order 3: 7 entries, order 4: 6, as coef.hip
+// forms them). X is the new state's spectrum.
+enum Stage3D { ST1 = 0, ST2A, ST2B, ST3A, ST3B, ST3C, ST4A, ST4B, ST4C, ST4D };
+
+struct Stage3DArgs {
+  long long ninst, nr;
+  int channels, ncoef, lin_batch, lin_channels, e_im, mode;
+  const float* tables;
+  const cf* U0;
+  const cf* N0;
+  const cf* NC;
+  cf* S;
+  cf* S1;
+  cf* A;
+};
+
+__global__ void stage3d_kernel(Stage3DArgs a) {
+  const long long total = a.ninst * a.nr;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const long long inst = idx / a.nr, k = idx % a.nr;
+    const long long traj = inst / a.channels, ch = inst % a.channels;
+    const float* tb = a.tables + ((a.lin_batch > 1 ? traj : 0) * a.lin_channels + (a.lin_channels > 1 ? ch : 0)) *
+                                     (long long)a.ncoef * a.nr;
+    auto c = [&](int i) { return tb[(long long)i * a.nr + k]; };
+    auto emul = [&](cf x) -> cf {
+      const float er = c(0);
+      if (a.e_im < 0) return er * x;
+      const float ei = c(a.e_im);
+      return cf{fmaf(er, x.re, -ei * x.im), fmaf(er, x.im, ei * x.re)};
+    };
+    const cf u0 = a.U0[idx];
+    switch (a.mode) {
+      case ST1:
+      case ST2A: a.S[idx] = emul(u0) + c(1) * a.N0[idx]; break;
+      case ST2B: a.S[idx] = a.S[idx] + c(2) * (a.NC[idx] - a.N0[idx]); break;
+      case ST3A:
+        a.A[idx] = c(0) * u0 + c(4) * a.N0[idx];
+        a.S[idx] = c(1) * u0 + c(2) * a.N0[idx];
+        break;
+      case ST3B:
+        a.A[idx] = a.A[idx] + c(5) * a.NC[idx];
+        a.S[idx] = c(0) * u0 + c(3) * (2.f * a.NC[idx] - a.N0[idx]);
+        break;
+      case ST3C: a.S[idx] = a.A[idx] + c(6) * a.NC[idx]; break;
+      case ST4A: {
+        a.A[idx] = c(0) * u0 + c(3) * a.N0[idx];
+        const cf s = c(1) * u0 + c(2) * a.N0[idx];
+        a.S1[idx] = s;
+        a.S[idx] = s;
+        break;
+      }
+      case ST4B:
+        a.A[idx] = a.A[idx] + (2.f * c(4)) * a.NC[idx];
+        a.S[idx] = c(1) * u0 + c(2) * a.NC[idx];
+        break;
+      case ST4C:
+        a.A[idx] = a.A[idx] + (2.f * c(4)) * a.NC[idx];
+        a.S[idx] = c(1) * a.S1[idx] + c(2) * (2.f * a.NC[idx] - a.N0[idx]);
+        break;
+      case ST4D: a.S[idx] = a.A[idx] + c(5) * a.NC[idx]; break;
+    }
+  }
+}
+
+hipError_t launch_stage3d(const Stage3DArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(stage3d_kernel, dim3(grid_of(a.ninst * a.nr)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
 }  // namespace exmi

@@ -87,3 +87,66 @@ def test_advection_3d_exact_transport(device):
     got = u[0, -1, 0].cpu().numpy()
     t = 20 * dt
     np.testing.assert_allclose(got, f(X - c[0] * t, Y - c[1] * t, Z - c[2] * t), atol=2e-5)
+
+
+def _check(gpu, cpu, u0, device, label):
+    import torch
+
+    x = torch.tensor(u0, device=device)
+    traj = gpu.rollout_batch(x, 10).cpu().numpy()
+    ug, uc = x, u0
+    for s in range(1, 11):
+        ug = gpu.step(ug)
+        uc = cpu.step_batch(uc)
+        if s in (1, 10):
+            e_step = max_rel_l2(ug.cpu().numpy(), uc)
+            e_roll = max_rel_l2(traj[:, s - 1], uc)
+            print(f"  {label} step {s}: steps {e_step:.2e}, rollout {e_roll:.2e}", flush=True)
+            assert e_step <= RTOL_STEP and e_roll <= RTOL_STEP
+
+
+@pytest.mark.parametrize("N", [16, 32])
+@pytest.mark.parametrize("conservative,order", [(False, 2), (True, 2), (False, 1), (True, 4), (False, 3)])
+def test_burgers_3d_parity(device, conservative, order, N):
+    print(f"[case] 3-D Burgers conservative={conservative} order {order} N={N}", flush=True)
+    kw = dict(diffusivity=0.01, conservative=conservative, order=order)
+    gpu = ex.stepper.Burgers(3, 1.0, N, 1e-3, **kw)
+    cpu = ost.Burgers(3, 1.0, N, 1e-3, dtype=np.float32, **kw)
+    u0, _ = ics(2, 3, 1.0, N, 3, seed=111, channels=3)
+    _check(gpu, cpu, u0, device, "burgers3d")
+
+
+def _nl3d_pair(name, N):
+    if name == "ks":
+        return ex.stepper.KuramotoSivashinsky(3, 30.0, N, 0.05), ost.KuramotoSivashinsky(
+            3, 30.0, N, 0.05, dtype=np.float32), 30.0, 1
+    if name == "allen_cahn":
+        return ex.stepper.reaction.AllenCahn(3, 1.0, N, 1e-3), ost.AllenCahn(3, 1.0, N, 1e-3,
+                                                                             dtype=np.float32), 1.0, 1
+    if name == "cahn_hilliard":
+        return ex.stepper.reaction.CahnHilliard(3, 2.0, N, 1e-3), ost.CahnHilliard(3, 2.0, N, 1e-3,
+                                                                                   dtype=np.float32), 2.0, 1
+    if name == "gray_scott":
+        kw = dict(diffusivity_1=2e-3, diffusivity_2=1e-3)
+        return (ex.stepper.reaction.GrayScott(3, 1.0, N, 0.5, **kw),
+                ost.GrayScott(3, 1.0, N, 0.5, dtype=np.float32, **kw), 1.0, 2)
+    if name == "general":
+        kw = dict(linear_coefficients=(0.0, 0.0, 0.01))
+        return (ex.stepper.generic.GeneralNonlinearStepper(3, 2.0, N, 1e-3, nonlinear_coefficients=(0.3, -1.0, -0.5),
+                                                           **kw),
+                ost.GeneralStepper(3, 2.0, N, 1e-3, nonlinear_scales=(0.3, -1.0, -0.5), dtype=np.float32, **kw),
+                2.0, 1)
+    kw = dict(diffusivity=0.01, single_channel=True, conservative=True)
+    return (ex.stepper.Burgers(3, 1.0, N, 1e-3, **kw), ost.Burgers(3, 1.0, N, 1e-3, dtype=np.float32, **kw), 1.0, 1)
+
+
+@pytest.mark.parametrize("name", ["ks", "allen_cahn", "cahn_hilliard", "gray_scott", "general", "burgers_single"])
+def test_nonlinear_3d_parity(device, name):
+    N = 32
+    print(f"[case] 3-D {name} N={N}", flush=True)
+    gpu, cpu, L, C = _nl3d_pair(name, N)
+    u0, _ = ics(2, 3, L, N, 3, seed=112, channels=C)
+    if name == "gray_scott":
+        u0 = (np.array([0.7, 0.2], dtype=np.float32).reshape(1, 2, 1, 1, 1) + 0.2 * u0 / np.abs(u0).max()).astype(
+            np.float32)
+    _check(gpu, cpu, u0, device, name)

@@ -417,3 +417,29 @@ def test_linear_3d_exact():
     k2 = (2 * np.pi / L) ** 2 * (4 + 1 + 9)
     u = st.Diffusion(3, L, N, dt, diffusivity=nu).step(f(X, Y, Z)[None])
     np.testing.assert_allclose(u[0], np.exp(-nu * k2 * dt) * f(X, Y, Z), atol=1e-12)
+
+
+@pytest.mark.parametrize("conservative", [False, True])
+def test_burgers_3d_reduces_to_1d(conservative):
+    L, N, dt, nu = 1.0, 16, 0.01, 0.02
+    x = _grid(L, N)
+    f = 0.6 * np.sin(2 * np.pi * x / L) + 0.3 * np.cos(2 * np.pi * 3 * x / L)
+    one = st.Burgers(1, L, N, dt, diffusivity=nu, conservative=conservative).step(f[None])[0]
+    three = st.Burgers(3, L, N, dt, diffusivity=nu, conservative=conservative)
+    z = np.zeros((N, N, N))
+    out = three.step(np.stack([z, z, np.broadcast_to(f, (N, N, N))]))  # w(z) along the last axis
+    np.testing.assert_allclose(out[2], np.broadcast_to(one, (N, N, N)), atol=1e-12)
+    np.testing.assert_allclose(out[:2], 0.0, atol=1e-12)
+
+
+def test_ks_and_general_3d_reduce_to_1d():
+    L, N, dt = 30.0, 16, 0.05
+    x = _grid(L, N)
+    f = 0.5 * np.sin(2 * np.pi * x / L) + 0.2 * np.sin(2 * np.pi * 3 * x / L)
+    one = st.KuramotoSivashinsky(1, L, N, dt).step(f[None])[0]
+    three = st.KuramotoSivashinsky(3, L, N, dt).step(np.broadcast_to(f[:, None, None], (N, N, N))[None])[0]
+    np.testing.assert_allclose(three, np.broadcast_to(one[:, None, None], (N, N, N)), atol=1e-12)
+    kw = dict(linear_coefficients=(0.0, 0.0, 0.01), nonlinear_scales=(0.3, -1.0, -0.5))
+    one = st.GeneralStepper(1, 2.0, N, 0.01, **kw).step(f[None])[0]
+    three = st.GeneralStepper(3, 2.0, N, 0.01, **kw).step(np.broadcast_to(f[None, :, None], (N, N, N))[None])[0]
+    np.testing.assert_allclose(three, np.broadcast_to(one[None, :, None], (N, N, N)), atol=1e-12)
