@@ -24,6 +24,8 @@ EXMI_NL_KOLMOGOROV = 6
 EXMI_NL_CAHN_HILLIARD = 7
 EXMI_NL_GRAY_SCOTT = 8
 EXMI_NL_BELOUSOV_ZHABOTINSKY = 9
+EXMI_NL_PROJECTED_CONVECTION = 10
+EXMI_NL_PROJECTED_CONVECTION_KOLMOGOROV = 11
 EXMI_MAX_POLY = 8
 EXMI_FLAG_EXACT_FORM = 1
 EXMI_FLAG_FOURIER_RESIDENT = 2

@@ -38,6 +38,8 @@ struct NL3DArgs {
   float kscale, scale, g0, g1, g2, feed, kill;
   int zero_mode_fix, npoly, poly_lap;
   float poly[EXMI_MAX_POLY];
+  int inj_mode;
+  float inj_value;
 };
 enum Stage3D { ST1 = 0, ST2A, ST2B, ST3A, ST3B, ST3C, ST4A, ST4B, ST4C, ST4D };
 struct Stage3DArgs {
@@ -709,6 +711,10 @@ static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
         return fail(EXMI_ERR_UNSUPPORTED, "3-D single-channel convection is built in the conservative form");
       kind = NL_GENERAL;
       conv_general = true;
+    } else if (d->nonlin_kind == EXMI_NL_PROJECTED_CONVECTION ||
+               d->nonlin_kind == EXMI_NL_PROJECTED_CONVECTION_KOLMOGOROV) {
+      if (C != 3) return fail(EXMI_ERR_INVALID, "the projected convection needs num_channels == 3");
+      kind = NL_PROJ3D;
     } else {
       int rc = kernel_kind(d, &kind);
       if (rc) return rc;
@@ -742,7 +748,18 @@ static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
     p->d.general_scales[1] = -d->scale;
     p->d.general_scales[2] = 0.0;
   }
-  p->nf3 = kind == NL_GRADNORM ? 3 : (kind == NL_CONV_NONCONS || kind == NL_GENERAL) ? 4 : 1;
+  p->nf3 = kind == NL_GRADNORM ? 3 : (kind == NL_CONV_NONCONS || kind == NL_GENERAL) ? 4 : kind == NL_PROJ3D ? 2 : 1;
+  if (d->nonlin_kind == EXMI_NL_PROJECTED_CONVECTION_KOLMOGOROV) {
+    // _projected_convection.py:195-216: injection_scale times the
+    // coef_extraction scaling (_spectral.py:403-527) at (0, k_f, 0): N on
+    // the zero-wavenumber axes, N/2 on the k_f axis unless k_f is 0 or N/2
+    const int kf = d->injection_mode;
+    if (kf >= 0 && kf < N / 2) {
+      const double sy = kf == 0 ? N : N / 2.0;
+      p->inj_mode = kf;
+      p->inj_value = (float)(d->injection_scale * (double)N * sy * (double)N);
+    }
+  }
   p->np3 = kind == NL_CONV_CONS ? 3 : kind == NL_GENERAL ? 2 : 1;
   p->kscale = (float)(2.0 * 3.14159265358979323846 / d->domain_extent);
   p->roll_nt = -1;  // per-step launches; no autotuning
@@ -821,6 +838,8 @@ static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t s
   a.npoly = p->d.num_poly;
   for (int i = 0; i < EXMI_MAX_POLY; ++i) a.poly[i] = (float)p->d.poly[i];
   a.poly_lap = p->d.nonlin_kind == EXMI_NL_CAHN_HILLIARD;
+  a.inj_mode = p->inj_mode;
+  a.inj_value = p->inj_value;
   hipError_t e = launch_fields3d(a, X, p->F3, st);
   if (e == hipSuccess) e = irfft3(N, p->ninst * p->nf3, p->F3, p->R3, p->tw, p->tw_col, st);
   if (e == hipSuccess) e = launch_products3d(a, p->R3, p->PR3, st);

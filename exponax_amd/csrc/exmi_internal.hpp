@@ -17,7 +17,8 @@ enum KernelKind {
   NL_VORT = 6,          // 2-D vorticity convection (+ Kolmogorov forcing)
   NL_GRAY_SCOTT = 7,    // two-channel Gray-Scott reaction, channels coupled
   NL_BZ = 8,            // three-channel Belousov-Zhabotinsky reaction, channels coupled
-  NL_NUM_KINDS = 9
+  NL_PROJ3D = 9,        // 3-D Leray-projected rotational convection (+ Kolmogorov forcing)
+  NL_NUM_KINDS = 10
 };
 
 // Channel-coupled reactions at one grid point, unit scale: Gray-Scott

@@ -129,12 +129,15 @@ struct NL3DArgs {
   float kscale, scale, g0, g1, g2, feed, kill;
   int zero_mode_fix, npoly, poly_lap;
   float poly[EXMI_MAX_POLY];
+  int inj_mode;     // projected convection: Kolmogorov forcing at (0, inj_mode, 0) of channel 0; -1: none
+  float inj_value;  // injection_scale * coef_extraction scaling
 };
 
 struct Mode3D {
   float kx, ky, kz;  // kappa_d = (2 pi / L) k_d
   bool keep;         // |k_d| <= kmax on every axis
   bool zero;         // the mean mode
+  bool inj;          // the Kolmogorov forcing mode (0, inj_mode, 0)
 };
 
 __device__ __forceinline__ Mode3D mode3d(const NL3DArgs& a, long long k) {
@@ -148,6 +151,7 @@ __device__ __forceinline__ Mode3D mode3d(const NL3DArgs& a, long long k) {
   m.kz = a.kscale * (float)iz;
   m.keep = (kx < 0 ? -kx : kx) <= a.kmax && (ky < 0 ? -ky : ky) <= a.kmax && iz <= a.kmax;
   m.zero = kx == 0 && ky == 0 && iz == 0;
+  m.inj = kx == 0 && ky == a.inj_mode && iz == 0;
   return m;
 }
 
@@ -161,7 +165,18 @@ __global__ void fields3d_kernel(NL3DArgs a, const cf* __restrict__ S, cf* __rest
     const Mode3D m = mode3d(a, k);
     const cf w = m.keep ? S[idx] : cf{0.f, 0.f};
     cf* f = F + inst * a.nf * a.Nr + k;
-    if (a.kind == NL_GRADNORM) {
+    if (a.kind == NL_PROJ3D) {
+      // u_c and the curl component omega_c = (i kappa x u_hat)_c
+      // (_projected_convection.py:8-16, 120-123), of the dealiased spectra
+      const long long b = (inst - inst % a.channels) * a.Nr + k;
+      const cf w0 = m.keep ? S[b] : cf{0.f, 0.f}, w1 = m.keep ? S[b + a.Nr] : cf{0.f, 0.f};
+      const cf w2 = m.keep ? S[b + 2 * a.Nr] : cf{0.f, 0.f};
+      const int c = (int)(inst % a.channels);
+      f[0] = w;
+      f[a.Nr] = c == 0 ? imul(m.ky, w2) - imul(m.kz, w1)
+                : c == 1 ? imul(m.kz, w0) - imul(m.kx, w2)
+                         : imul(m.kx, w1) - imul(m.ky, w0);
+    } else if (a.kind == NL_GRADNORM) {
       f[0] = imul(m.kx, w);
       f[a.Nr] = imul(m.ky, w);
       f[2 * a.Nr] = imul(m.kz, w);
@@ -230,6 +245,12 @@ __global__ void products3d_kernel(NL3DArgs a, const float* __restrict__ R, float
         out[0] = ch == 0 ? r0 : ch == 1 ? r1 : r2;
         break;
       }
+      case NL_PROJ3D: {
+        // (u x omega)_c
+        const int c1 = (ch + 1) % 3, c2 = (ch + 2) % 3;
+        out[0] = fld(base + c1, 0) * fld(base + c2, 1) - fld(base + c2, 0) * fld(base + c1, 1);
+        break;
+      }
       default: out[0] = 0.f;
     }
   }
@@ -257,9 +278,22 @@ __global__ void combine3d_kernel(NL3DArgs a, const cf* __restrict__ PH, cf* __re
         r = a.g0 * ph[0] + imul(0.5f * a.g1 * (m.kx + m.ky + m.kz), ph[0]) + cg * ph[a.Nr];
         break;
       }
+      case NL_PROJ3D: {
+        // Leray projection (_leray.py:86-103): c + i kappa_c p with
+        // p = -div / Laplace = (i kappa . c) / |kappa|^2 (p = 0 at the mean mode)
+        const long long b = (inst - inst % a.channels) * a.Nr + k;  // np = 1
+        const cf p0 = PH[b], p1 = PH[b + a.Nr], p2 = PH[b + 2 * a.Nr];
+        const cf div = imul(m.kx, p0) + imul(m.ky, p1) + imul(m.kz, p2);
+        const float k2 = m.kx * m.kx + m.ky * m.ky + m.kz * m.kz;
+        const cf pr = k2 != 0.f ? (1.f / k2) * div : cf{0.f, 0.f};
+        const int c = (int)(inst % a.channels);
+        r = ph[0] + imul(c == 0 ? m.kx : c == 1 ? m.ky : m.kz, pr);
+        break;
+      }
       default: r = ph[0];
     }
     Nh[idx] = m.keep ? r : cf{0.f, 0.f};
+    if (a.kind == NL_PROJ3D && m.inj && inst % a.channels == 0) Nh[idx] = Nh[idx] + cf{a.inj_value, 0.f};
   }
 }
 

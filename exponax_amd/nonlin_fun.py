@@ -224,3 +224,36 @@ class VorticityConvection2dKolmogorov(VorticityConvection2d):
         super().fill(d)
         d.injection_mode = int(self.injection_mode)
         d.injection_scale = float(self.injection_scale)
+
+
+class ProjectedConvection3d(BaseNonlinearFun):
+    """``nonlin_fun/_projected_convection.py:19-136``: P(u x omega) with the
+    Leray projection P (``_leray.py``), three channels, 3-D only."""
+
+    kind = _lib.EXMI_NL_PROJECTED_CONVECTION
+
+    def __init__(self, num_spatial_dims, num_points, *, derivative_operator, dealiasing_fraction=2 / 3):
+        if num_spatial_dims != 3:
+            raise ValueError("ProjectedConvection3d only supports 3 spatial dimensions.")
+        super().__init__(num_spatial_dims, num_points, dealiasing_fraction=dealiasing_fraction)
+        self.domain_extent = domain_extent_of(derivative_operator, num_spatial_dims, num_points)
+
+
+class ProjectedConvection3dKolmogorov(ProjectedConvection3d):
+    """``nonlin_fun/_projected_convection.py:139-226``: plus the forcing of
+    intensity injection_scale at wavenumber (0, injection_mode, 0) of the
+    first velocity channel."""
+
+    kind = _lib.EXMI_NL_PROJECTED_CONVECTION_KOLMOGOROV
+
+    def __init__(self, num_spatial_dims, num_points, *, injection_mode=4, injection_scale=1.0, derivative_operator,
+                 dealiasing_fraction):
+        super().__init__(num_spatial_dims, num_points, derivative_operator=derivative_operator,
+                         dealiasing_fraction=dealiasing_fraction)
+        self.injection_mode = injection_mode
+        self.injection_scale = injection_scale
+
+    def fill(self, d):
+        super().fill(d)
+        d.injection_mode = int(self.injection_mode)
+        d.injection_scale = float(self.injection_scale)

@@ -4,10 +4,10 @@ from ._burgers import Burgers
 from ._korteweg_de_vries import KortewegDeVries
 from ._kuramoto_sivashinsky import KuramotoSivashinsky, KuramotoSivashinskyConservative
 from ._linear import Advection, AdvectionDiffusion, Diffusion, Dispersion, HyperDiffusion
-from ._navier_stokes import KolmogorovFlowVorticity, NavierStokesVorticity
+from ._navier_stokes import KolmogorovFlowVelocity, KolmogorovFlowVorticity, NavierStokesVelocity, NavierStokesVorticity
 from ._reaction import AllenCahn, FisherKPP, SwiftHohenberg
 from . import reaction
 
 __all__ = ["generic", "reaction", "AllenCahn", "FisherKPP", "SwiftHohenberg", "Advection", "AdvectionDiffusion", "Burgers", "Diffusion", "Dispersion", "HyperDiffusion",
            "KortewegDeVries", "KuramotoSivashinsky", "KuramotoSivashinskyConservative", "KolmogorovFlowVorticity",
-           "NavierStokesVorticity"]
+           "NavierStokesVorticity", "NavierStokesVelocity", "KolmogorovFlowVelocity"]

@@ -1,6 +1,7 @@
 """2-D vorticity steppers (``stepper/_navier_stokes.py:13-327``)."""
 from .._base_stepper import BaseStepper
-from ..nonlin_fun import VorticityConvection2d, VorticityConvection2dKolmogorov
+from ..nonlin_fun import (ProjectedConvection3d, ProjectedConvection3dKolmogorov, VorticityConvection2d,
+                          VorticityConvection2dKolmogorov)
 from ..spectral import build_laplace_operator
 
 
@@ -57,6 +58,59 @@ class KolmogorovFlowVorticity(BaseStepper):
     def _build_nonlinear_fun(self, derivative_operator):
         return VorticityConvection2dKolmogorov(self.num_spatial_dims, self.num_points,
                                                convection_scale=self.convection_scale,
+                                               injection_mode=self.injection_mode,
+                                               injection_scale=self.injection_scale,
+                                               derivative_operator=derivative_operator,
+                                               dealiasing_fraction=self.dealiasing_fraction)
+
+
+class NavierStokesVelocity(BaseStepper):
+    """``stepper/_navier_stokes.py:330-450``: 3-D incompressible Navier-Stokes
+    in velocity form, u_t = P(u x omega) + nu Laplace u + lambda u (three
+    channels)."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity=0.01, drag=0.0, order=2,
+                 dealiasing_fraction=2 / 3, num_circle_points=16, circle_radius=1.0):
+        if num_spatial_dims != 3:
+            raise ValueError(f"NavierStokesVelocity is three-dimensional, got num_spatial_dims={num_spatial_dims}")
+        self.diffusivity = diffusivity
+        self.drag = drag
+        self.dealiasing_fraction = dealiasing_fraction
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=3, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius)
+
+    def _build_linear_operator(self, derivative_operator):
+        return (self.diffusivity * build_laplace_operator(derivative_operator, order=2)
+                + self.drag * build_laplace_operator(derivative_operator, order=0))
+
+    def _build_nonlinear_fun(self, derivative_operator):
+        return ProjectedConvection3d(self.num_spatial_dims, self.num_points, derivative_operator=derivative_operator,
+                                     dealiasing_fraction=self.dealiasing_fraction)
+
+
+class KolmogorovFlowVelocity(BaseStepper):
+    """``stepper/_navier_stokes.py:451-598``: NavierStokesVelocity with the
+    Kolmogorov forcing in the first channel."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity=0.01, drag=0.0,
+                 injection_mode=4, injection_scale=1.0, order=2, dealiasing_fraction=2 / 3, num_circle_points=16,
+                 circle_radius=1.0):
+        if num_spatial_dims != 3:
+            raise ValueError(f"KolmogorovFlowVelocity is three-dimensional, got num_spatial_dims={num_spatial_dims}")
+        self.diffusivity = diffusivity
+        self.drag = drag
+        self.injection_mode = injection_mode
+        self.injection_scale = injection_scale
+        self.dealiasing_fraction = dealiasing_fraction
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=3, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius)
+
+    def _build_linear_operator(self, derivative_operator):
+        return (self.diffusivity * build_laplace_operator(derivative_operator, order=2)
+                + self.drag * build_laplace_operator(derivative_operator, order=0))
+
+    def _build_nonlinear_fun(self, derivative_operator):
+        return ProjectedConvection3dKolmogorov(self.num_spatial_dims, self.num_points,
                                                injection_mode=self.injection_mode,
                                                injection_scale=self.injection_scale,
                                                derivative_operator=derivative_operator,

@@ -55,7 +55,14 @@ enum exmi_nonlin_kind {
    * of a trajectory in one wave (N <= 1024; no exmi_step_fourier /
    * exmi_nonlinear); 2-D plans couple them in the row kernel. */
   EXMI_NL_GRAY_SCOTT = 8,
-  EXMI_NL_BELOUSOV_ZHABOTINSKY = 9
+  EXMI_NL_BELOUSOV_ZHABOTINSKY = 9,
+  /* P(u x omega), three channels, 3-D plans: ProjectedConvection3d
+   * nonlin_fun/_projected_convection.py:19-136 (Leray projection
+   * nonlin_fun/_leray.py:86-103); the Kolmogorov variant (:139-226) adds
+   * injection_scale * coef_extraction scaling at (0, injection_mode, 0) of
+   * the first channel */
+  EXMI_NL_PROJECTED_CONVECTION = 10,
+  EXMI_NL_PROJECTED_CONVECTION_KOLMOGOROV = 11
 };
 
 #define EXMI_MAX_POLY 8

@@ -204,6 +204,69 @@ class BelousovZhabotinskyNonlinearFun(BaseNonlinearFun):
         return self.fft(power)
 
 
+def _cross3(a, b, ax):
+    """Cross product of two 3-vectors stored along axis ax."""
+    g = lambda x, i: np.take(x, i, axis=ax)  # noqa: E731
+    return np.stack([g(a, 1) * g(b, 2) - g(a, 2) * g(b, 1), g(a, 2) * g(b, 0) - g(a, 0) * g(b, 2),
+                     g(a, 0) * g(b, 1) - g(a, 1) * g(b, 0)], axis=ax)
+
+
+class Leray:
+    """``nonlin_fun/_leray.py:8-103``: u_hat + i kappa p_hat with p_hat =
+    -Laplace^-1 (i kappa . u_hat), Laplace^-1 = 0 at the mean mode."""
+
+    def __init__(self, derivative_operator, dtype=np.float64):
+        lap = sp.build_laplace_operator(derivative_operator, order=2)
+        with np.errstate(divide="ignore"):
+            self.inv_laplacian = np.where(lap != 0, 1.0 / np.where(lap != 0, lap, 1.0), 0.0).astype(
+                sp.complex_of(dtype))
+        self.derivative_operator = derivative_operator.astype(sp.complex_of(dtype))
+
+    def __call__(self, u_hat):
+        ax = u_hat.ndim - self.derivative_operator.ndim
+        div = np.sum(self.derivative_operator * u_hat, axis=ax, keepdims=True)
+        return u_hat + self.derivative_operator * (-self.inv_laplacian * div)
+
+
+class ProjectedConvection3d(BaseNonlinearFun):
+    """``nonlin_fun/_projected_convection.py:19-136``: P(u x omega), omega =
+    i kappa x u_hat, the cross product in physical space."""
+
+    def __init__(self, num_spatial_dims, num_points, *, derivative_operator, dealiasing_fraction=2 / 3,
+                 dtype=np.float64):
+        if num_spatial_dims != 3:
+            raise ValueError("ProjectedConvection3d only supports 3 spatial dimensions.")
+        super().__init__(num_spatial_dims, num_points, dealiasing_fraction=dealiasing_fraction, dtype=dtype)
+        self.derivative_operator = self._op(derivative_operator)
+        self.leray = Leray(derivative_operator, dtype=dtype)
+
+    def __call__(self, u_hat):
+        ax = u_hat.ndim - 4  # the channel axis
+        dop = self.derivative_operator if ax == 0 else self.derivative_operator[None]
+        curl_hat = _cross3(dop, u_hat, ax)
+        curl = self.ifft(curl_hat)
+        velocity = self.ifft(u_hat)
+        return self.leray(self.fft(_cross3(velocity, curl, ax)))
+
+
+class ProjectedConvection3dKolmogorov(ProjectedConvection3d):
+    """``nonlin_fun/_projected_convection.py:139-226``."""
+
+    def __init__(self, num_spatial_dims, num_points, *, injection_mode=4, injection_scale=1.0, derivative_operator,
+                 dealiasing_fraction, dtype=np.float64):
+        super().__init__(num_spatial_dims, num_points, derivative_operator=derivative_operator,
+                         dealiasing_fraction=dealiasing_fraction, dtype=dtype)
+        wn = sp.build_wavenumbers(num_spatial_dims, num_points)
+        mask = (wn[0] == 0) & (wn[1] == injection_mode) & (wn[2] == 0)
+        single = np.where(mask, injection_scale * sp.build_scaling_array(num_spatial_dims, num_points,
+                                                                         mode="coef_extraction"), 0.0)
+        zeros = np.zeros_like(single)
+        self.injection = np.concatenate([single, zeros, zeros], axis=0).astype(self.cdtype)
+
+    def __call__(self, u_hat):
+        return super().__call__(u_hat) + self.injection
+
+
 class GeneralNonlinearFun(BaseNonlinearFun):
     """``nonlin_fun/_general_nonlinear.py:9-119``: b0 u^2 + conv + grad-norm."""
 

@@ -481,6 +481,34 @@ class KolmogorovFlowVorticity(BaseStepper):
             dealiasing_fraction=self.dealiasing_fraction, dtype=self._kw_dtype)
 
 
+class NavierStokesVelocity(BaseStepper):
+    """``stepper/_navier_stokes.py:330-450``: L = nu Laplace + lambda,
+    N = P(u x omega) (+ the Kolmogorov forcing with injection_mode set)."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, diffusivity=0.01, drag=0.0,
+                 injection_mode=None, injection_scale=1.0, order=2, dealiasing_fraction=2 / 3, num_circle_points=16,
+                 circle_radius=1.0, **kw):
+        self.diffusivity, self.drag = diffusivity, drag
+        self.injection_mode, self.injection_scale = injection_mode, injection_scale
+        self.dealiasing_fraction = dealiasing_fraction
+        self._kw_dtype = kw.get("dtype", np.float64)
+        super().__init__(num_spatial_dims, domain_extent, num_points, dt, num_channels=3, order=order,
+                         num_circle_points=num_circle_points, circle_radius=circle_radius, **kw)
+
+    def _build_linear_operator(self, d):
+        return self.diffusivity * sp.build_laplace_operator(d, order=2) + self.drag * sp.build_laplace_operator(
+            d, order=0)
+
+    def _build_nonlinear_fun(self, d):
+        if self.injection_mode is None:
+            return nl.ProjectedConvection3d(self.num_spatial_dims, self.num_points, derivative_operator=d,
+                                            dealiasing_fraction=self.dealiasing_fraction, dtype=self._kw_dtype)
+        return nl.ProjectedConvection3dKolmogorov(self.num_spatial_dims, self.num_points,
+                                                  injection_mode=self.injection_mode,
+                                                  injection_scale=self.injection_scale, derivative_operator=d,
+                                                  dealiasing_fraction=self.dealiasing_fraction, dtype=self._kw_dtype)
+
+
 class GeneralStepper(BaseStepper):
     """Linear part sum_j a_j sum_d (i kappa_d)^j (even j here) + ``GeneralNonlinearFun``.
 

@@ -150,3 +150,34 @@ def test_nonlinear_3d_parity(device, name):
         u0 = (np.array([0.7, 0.2], dtype=np.float32).reshape(1, 2, 1, 1, 1) + 0.2 * u0 / np.abs(u0).max()).astype(
             np.float32)
     _check(gpu, cpu, u0, device, name)
+
+
+@pytest.mark.parametrize("N,order", [(16, 2), (32, 2), (32, 1), (32, 4)])
+def test_navier_stokes_velocity_parity(device, N, order):
+    """stepper/_navier_stokes.py:330-598: P(u x omega) (Leray-projected
+    rotational convection) and, for N = 32 order 2, the Kolmogorov forcing."""
+    print(f"[case] 3-D Navier-Stokes velocity N={N} order {order}", flush=True)
+    L = 2 * np.pi
+    kw = dict(diffusivity=0.02, order=order)
+    gpu = ex.stepper.NavierStokesVelocity(3, L, N, 0.01, **kw)
+    cpu = ost.NavierStokesVelocity(3, L, N, 0.01, dtype=np.float32, **kw)
+    u0, _ = ics(2, 3, L, N, 3, seed=121, channels=3)
+    _check(gpu, cpu, u0, device, "ns_velocity")
+    if N == 32 and order == 2:
+        kw.update(injection_mode=2, injection_scale=0.5, drag=-0.1)
+        gpu = ex.stepper.KolmogorovFlowVelocity(3, L, N, 0.01, **kw)
+        cpu = ost.NavierStokesVelocity(3, L, N, 0.01, dtype=np.float32, **kw)
+        _check(gpu, cpu, u0, device, "kolmogorov_velocity")
+
+
+def test_abc_flow_decays_viscously(device):
+    import torch
+
+    print("[case] 3-D ABC flow", flush=True)
+    N, dt, nu, n = 32, 0.05, 0.02, 20
+    x = np.linspace(0.0, 2 * np.pi, N, endpoint=False)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    u0 = np.stack([np.sin(Z) + 0.4 * np.cos(Y), 0.7 * np.sin(X) + np.cos(Z), 0.4 * np.sin(Y) + 0.7 * np.cos(X)])
+    st = ex.stepper.NavierStokesVelocity(3, 2 * np.pi, N, dt, diffusivity=nu)
+    got = st.repeat_batch(torch.tensor(u0[None], dtype=torch.float32, device=device), n)[0].cpu().numpy()
+    np.testing.assert_allclose(got, np.exp(-nu * n * dt) * u0, atol=2e-5)

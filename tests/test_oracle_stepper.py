@@ -443,3 +443,71 @@ def test_ks_and_general_3d_reduce_to_1d():
     one = st.GeneralStepper(1, 2.0, N, 0.01, **kw).step(f[None])[0]
     three = st.GeneralStepper(3, 2.0, N, 0.01, **kw).step(np.broadcast_to(f[None, :, None], (N, N, N))[None])[0]
     np.testing.assert_allclose(three, np.broadcast_to(one[None, :, None], (N, N, N)), atol=1e-12)
+
+
+def _abc(N, A=1.0, B=0.7, C=0.4):
+    x = _grid(2 * np.pi, N)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    return np.stack([A * np.sin(Z) + C * np.cos(Y), B * np.sin(X) + A * np.cos(Z), C * np.sin(Y) + B * np.cos(X)])
+
+
+def test_navier_stokes_velocity_abc_flow_decays_viscously():
+    """The ABC flow is a Beltrami field (curl u = u), so u x omega = 0 and
+    every mode (|k| = 1) decays as exp(-nu t)."""
+    N, dt, nu, n = 16, 0.05, 0.02, 10
+    u0 = _abc(N)
+    ns = st.NavierStokesVelocity(3, 2 * np.pi, N, dt, diffusivity=nu)
+    u = u0
+    for _ in range(n):
+        u = ns.step(u)
+    np.testing.assert_allclose(u, np.exp(-nu * n * dt) * u0, atol=1e-12)
+
+
+def test_leray_projection_properties():
+    from oracle import nonlin as onl
+    from oracle import spectral as osp
+
+    N = 16
+    d = osp.build_derivative_operator(3, 2 * np.pi, N)
+    P = onl.Leray(d)
+    rng = np.random.default_rng(0)
+    u_hat = osp.fft(rng.standard_normal((3, N, N, N)), num_spatial_dims=3)
+    pu = P(u_hat)
+    np.testing.assert_allclose(P(pu), pu, atol=1e-9)  # idempotent
+    assert np.abs(np.sum(d * pu, axis=0)).max() < 1e-9 * np.abs(u_hat).max()  # divergence-free
+    phi_hat = osp.fft(rng.standard_normal((1, N, N, N)), num_spatial_dims=3)
+    assert np.abs(P(d * phi_hat)).max() < 1e-9 * np.abs(d * phi_hat).max()  # gradients vanish
+
+
+def test_navier_stokes_velocity_keeps_divergence_free():
+    from oracle import nonlin as onl
+    from oracle import spectral as osp
+
+    N, L = 16, 2 * np.pi
+    d = osp.build_derivative_operator(3, L, N)
+    rng = np.random.default_rng(1)
+    k = osp.build_wavenumbers(3, N)
+    low = (np.abs(k) <= 3).all(axis=0)
+    u_hat = onl.Leray(d)(osp.fft(rng.standard_normal((3, N, N, N)), num_spatial_dims=3) * low)
+    u = osp.ifft(u_hat, num_spatial_dims=3, num_points=N)
+    ns = st.NavierStokesVelocity(3, L, N, 0.01, diffusivity=0.01)
+    for _ in range(5):
+        u = ns.step(u)
+    uh = osp.fft(u, num_spatial_dims=3)
+    assert np.abs(np.sum(d * uh, axis=0)).max() < 1e-10 * np.abs(uh).max()
+
+
+def test_kolmogorov_velocity_from_rest():
+    """From rest, N(0) is the forcing and the second stage's u x omega is a
+    gradient (u = (f(y), 0, 0)), so one ETDRK2 step gives c1 * forcing."""
+    from oracle import spectral as osp
+
+    N, L, dt, nu, kf = 16, 2 * np.pi, 0.1, 0.05, 2
+    ks = st.NavierStokesVelocity(3, L, N, dt, diffusivity=nu, injection_mode=kf, injection_scale=0.5)
+    u1 = ks.step(np.zeros((3, N, N, N)))
+    f_hat = ks.nonlinear_fun.injection
+    lam = -nu * (2 * np.pi / L * kf) ** 2
+    # the state is real: compare after the same physical round trip
+    want = osp.ifft((np.exp(lam * dt) - 1) / lam * f_hat, num_spatial_dims=3, num_points=N)
+    np.testing.assert_allclose(u1, want, atol=1e-9 * np.abs(want).max())
+    assert np.abs(want).max() > 0.01
