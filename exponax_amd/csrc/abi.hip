@@ -16,6 +16,7 @@
 #include "adjoint1d.hpp"
 #include "adjoint2d.hpp"
 #include "fft1d.hpp"
+#include "spectral3d.hpp"
 #include "step1d.hpp"
 #include "step2d.hpp"
 
@@ -32,27 +33,6 @@ hipError_t launch_lines(int N, bool inv, cf* data, long long nlines, long long i
                         long long pstride, float scale, const cf* tw, hipStream_t st);
 hipError_t launch_etdrk0_spec(cf* X, long long ninst, long long nr, int channels, const float* tables, int ncoef,
                               int lin_batch, int lin_channels, int e_im, hipStream_t st);
-struct NL3DArgs {
-  int N, NZ, channels, kind, kmax, nf, np;
-  long long Nr, N3, ninst;
-  float kscale, scale, g0, g1, g2, feed, kill;
-  int zero_mode_fix, npoly, poly_lap;
-  float poly[EXMI_MAX_POLY];
-  int inj_mode;
-  float inj_value;
-};
-enum Stage3D { ST1 = 0, ST2A, ST2B, ST3A, ST3B, ST3C, ST4A, ST4B, ST4C, ST4D };
-struct Stage3DArgs {
-  long long ninst, nr;
-  int channels, ncoef, lin_batch, lin_channels, e_im, mode;
-  const float* tables;
-  const cf* U0;
-  const cf* N0;
-  const cf* NC;
-  cf* S;
-  cf* S1;
-  cf* A;
-};
 hipError_t launch_fields3d(const NL3DArgs& a, const cf* S, cf* F, hipStream_t st);
 hipError_t launch_products3d(const NL3DArgs& a, const float* R, float* PR, hipStream_t st);
 hipError_t launch_combine3d(const NL3DArgs& a, const cf* PH, cf* Nh, hipStream_t st);

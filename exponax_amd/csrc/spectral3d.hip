@@ -10,6 +10,7 @@
 // slot the groups of a wavefront read neighbouring addresses.
 #include "exmi_internal.hpp"
 #include "fft1d.hpp"
+#include "spectral3d.hpp"
 
 namespace exmi {
 
@@ -123,16 +124,6 @@ hipError_t launch_etdrk0_spec(cf* X, long long ninst, long long nr, int channels
 // Cahn-Hilliard), gradient norm (_gradient_norm.py), the three-channel
 // convections (_convection.py:104-171), the general form
 // (_general_nonlinear.py) and the coupled reactions.
-struct NL3DArgs {
-  int N, NZ, channels, kind, kmax, nf, np;
-  long long Nr, N3, ninst;
-  float kscale, scale, g0, g1, g2, feed, kill;
-  int zero_mode_fix, npoly, poly_lap;
-  float poly[EXMI_MAX_POLY];
-  int inj_mode;     // projected convection: Kolmogorov forcing at (0, inj_mode, 0) of channel 0; -1: none
-  float inj_value;  // injection_scale * coef_extraction scaling
-};
-
 struct Mode3D {
   float kx, ky, kz;  // kappa_d = (2 pi / L) k_d
   bool keep;         // |k_d| <= kmax on every axis
@@ -320,20 +311,6 @@ hipError_t launch_combine3d(const NL3DArgs& a, const cf* PH, cf* Nh, hipStream_t
 // coefficient c_i of the plan's tables (orders 1-2: [E, c1, c2] with Im E
 // at e_im for a complex L; order 3: 7 entries, order 4: 6, as coef.hip
 // forms them). X is the new state's spectrum.
-enum Stage3D { ST1 = 0, ST2A, ST2B, ST3A, ST3B, ST3C, ST4A, ST4B, ST4C, ST4D };
-
-struct Stage3DArgs {
-  long long ninst, nr;
-  int channels, ncoef, lin_batch, lin_channels, e_im, mode;
-  const float* tables;
-  const cf* U0;
-  const cf* N0;
-  const cf* NC;
-  cf* S;
-  cf* S1;
-  cf* A;
-};
-
 __global__ void stage3d_kernel(Stage3DArgs a) {
   const long long total = a.ninst * a.nr;
   for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;

@@ -1,0 +1,33 @@
+// Launch arguments of the 3-D nonlinear and ETDRK stage kernels
+// (spectral3d.hip), shared with the C-ABI layer.
+#pragma once
+#include "exmi_internal.hpp"
+
+namespace exmi {
+
+struct NL3DArgs {
+  int N, NZ, channels, kind, kmax, nf, np;
+  long long Nr, N3, ninst;
+  float kscale, scale, g0, g1, g2, feed, kill;
+  int zero_mode_fix, npoly, poly_lap;
+  float poly[EXMI_MAX_POLY];
+  int inj_mode;     // projected convection: Kolmogorov forcing at (0, inj_mode, 0) of channel 0; -1: none
+  float inj_value;  // injection_scale * coef_extraction scaling
+};
+
+// The stage combinations of etdrk/_etdrk_{1,2,3,4}.py on whole spectra.
+enum Stage3D { ST1 = 0, ST2A, ST2B, ST3A, ST3B, ST3C, ST4A, ST4B, ST4C, ST4D };
+
+struct Stage3DArgs {
+  long long ninst, nr;
+  int channels, ncoef, lin_batch, lin_channels, e_im, mode;
+  const float* tables;
+  const cf* U0;
+  const cf* N0;
+  const cf* NC;
+  cf* S;
+  cf* S1;
+  cf* A;
+};
+
+}  // namespace exmi

@@ -2,7 +2,9 @@
 
 ``BaseStepper`` follows ``exponax/_base_stepper.py:17-271``; the concrete
 steppers follow ``exponax/stepper/_kuramoto_sivashinsky.py``,
-``_burgers.py`` and ``_navier_stokes.py:13-327``; ``rollout``/``repeat``
+``_burgers.py``, ``_navier_stokes.py`` (vorticity and velocity forms), the
+linear steppers (``_advection.py`` ... ``_korteweg_de_vries.py``) and
+``stepper/reaction/``; ``rollout``/``repeat``
 follow ``exponax/_utils.py:92-254``; ``nrmse`` follows
 ``exponax/metrics/_spatial.py:537-586``.
 
