@@ -40,7 +40,8 @@ EXPORTED = (
     "exmi_plan_create", "exmi_plan_destroy", "exmi_step", "exmi_step_fourier", "exmi_nonlinear",
     "exmi_rollout", "exmi_rollout_prepare", "exmi_rollout_group", "exmi_repeat", "exmi_rfft", "exmi_irfft", "exmi_plan_num_coefs", "exmi_plan_get_tables",
     "exmi_plan_rollout_strategy", "exmi_plan_set_rollout_strategy", "exmi_last_error", "exmi_info",
-    "exmi_step_vjp", "exmi_rollout_vjp", "exmi_nrmse", "exmi_spectrum", "exmi_map_resolution",
+    "exmi_step_vjp", "exmi_rollout_vjp", "exmi_nrmse", "exmi_spectrum", "exmi_spectrum_binned",
+    "exmi_map_resolution",
     "exmi_cnn_workspace_bytes", "exmi_cnn_forward", "exmi_cnn_backward", "exmi_corrected_rollout",
 )
 EXMI_CNN_NUM_PARAMS = 15809
@@ -108,6 +109,7 @@ def _declare(lib):
     lib.exmi_rollout_vjp.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, i32, vp]
     lib.exmi_nrmse.argtypes = [i32, i32, i64, vp, i64, vp, i64, vp, vp]
     lib.exmi_spectrum.argtypes = [i32, i32, i32, vp, vp, i32, vp]
+    lib.exmi_spectrum_binned.argtypes = [i32, i32, i32, vp, vp, i32, i32, vp]
     lib.exmi_map_resolution.argtypes = [i32, i32, i32, i32, vp, vp, i32, vp]
     lib.exmi_cnn_workspace_bytes.argtypes = [i32, i32]
     lib.exmi_cnn_forward.argtypes = [vp, i32, i32, vp, vp, vp, vp]

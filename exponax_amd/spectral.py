@@ -146,9 +146,22 @@ def ifft(field_hat, *, num_spatial_dims=None, num_points=None):
     return restore_like(out, back)
 
 
-def get_spectrum(state, *, power: bool = True, radial_binning: str = "sum"):
-    """(Power) spectrum of a 1-D state ``(..., C, N)`` -> ``(..., C, N//2+1)``;
-    ``_spectral.py:866-1030`` (in 1-D there is no radial binning)."""
+def _spatial_dims(shape) -> int:
+    """Trailing axes of the last axis' length, at most len(shape) - 1: the
+    reference's D = ndim - 1 for a single state (C, N, ..., N), and the same D
+    for leading batch axes of another length."""
+    d = 0
+    while d < len(shape) - 1 and shape[len(shape) - 1 - d] == shape[-1]:
+        d += 1
+    return max(d, 1)
+
+
+def get_spectrum(state, *, power: bool = True, radial_binning: str = "sum", num_spatial_dims=None):
+    """(Power) spectrum of a state ``(C, N, ..., N)`` -> ``(C, N//2+1)``;
+    ``_spectral.py:866-1030``. In 2-D and 3-D the modes are binned by the
+    norm of their wavenumber ("sum" or "average"). Leading batch axes are
+    allowed; ``num_spatial_dims`` defaults to the number of trailing axes of
+    length N, at most ``state.ndim - 1`` (the reference's single state)."""
     t, back = as_device_tensor(state)
     if t.dim() < 2:
         raise ValueError("expected a state with a leading channel axis")
@@ -156,27 +169,34 @@ def get_spectrum(state, *, power: bool = True, radial_binning: str = "sum"):
         raise ValueError(f"Invalid radial_binning {radial_binning!r}")
     import torch
 
+    D = _spatial_dims(t.shape) if num_spatial_dims is None else int(num_spatial_dims)
     n = t.shape[-1]
+    if any(s != n for s in t.shape[t.dim() - D:]):
+        raise ValueError(f"expected N points on each of the last {D} axes, got {tuple(t.shape)}")
     x = t.contiguous().to(torch.float32)
-    count = x.numel() // n
-    out = torch.empty(x.shape[:-1] + (n // 2 + 1,), dtype=torch.float32, device=x.device)
-    _lib.check(_lib.lib().exmi_spectrum(1, n, count, x.data_ptr(), out.data_ptr(), int(power), _lib.stream_ptr()),
-               "exmi_spectrum")
+    count = x.numel() // n**D
+    out = torch.empty(x.shape[:t.dim() - D] + (n // 2 + 1,), dtype=torch.float32, device=x.device)
+    _lib.check(_lib.lib().exmi_spectrum_binned(D, n, count, x.data_ptr(), out.data_ptr(), int(power),
+                                               int(radial_binning == "average"), _lib.stream_ptr()), "exmi_spectrum")
     return restore_like(out, back)
 
 
-def map_between_resolutions(state, new_num_points: int, *, oddball_zero: bool = True):
-    """Spectral up/down-sampling of a 1-D state ``(..., C, N)`` to
-    ``new_num_points``; ``_interpolation.py:195-285``."""
+def map_between_resolutions(state, new_num_points: int, *, oddball_zero: bool = True, num_spatial_dims=None):
+    """Spectral up/down-sampling of a state ``(C, N, ..., N)`` to
+    ``new_num_points`` per axis; ``_interpolation.py:195-285``. Leading batch
+    axes are allowed; ``num_spatial_dims`` defaults as in ``get_spectrum``."""
     t, back = as_device_tensor(state)
     if t.dim() < 2:
         raise ValueError("expected a state with a leading channel axis")
     import torch
 
+    D = _spatial_dims(t.shape) if num_spatial_dims is None else int(num_spatial_dims)
     n = t.shape[-1]
+    if any(s != n for s in t.shape[t.dim() - D:]):
+        raise ValueError(f"expected N points on each of the last {D} axes, got {tuple(t.shape)}")
     x = t.contiguous().to(torch.float32)
-    count = x.numel() // n
-    out = torch.empty(x.shape[:-1] + (new_num_points,), dtype=torch.float32, device=x.device)
-    _lib.check(_lib.lib().exmi_map_resolution(1, n, new_num_points, count, x.data_ptr(), out.data_ptr(),
+    count = x.numel() // n**D
+    out = torch.empty(x.shape[:t.dim() - D] + (new_num_points,) * D, dtype=torch.float32, device=x.device)
+    _lib.check(_lib.lib().exmi_map_resolution(D, n, new_num_points, count, x.data_ptr(), out.data_ptr(),
                                               int(oddball_zero), _lib.stream_ptr()), "exmi_map_resolution")
     return restore_like(out, back)

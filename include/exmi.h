@@ -257,11 +257,18 @@ int exmi_irfft(int32_t num_spatial_dims, int32_t num_points, int32_t count, cons
 int exmi_nrmse(int32_t batch, int32_t channels, int64_t n, const float* pred, int64_t pred_bstride,
                const float* ref, int64_t ref_bstride, float* out, void* stream);
 
-/* Spectrum of `count` 1-D signals of N points: out (count, N/2+1) float32,
+/* Spectrum of `count` fields of N^D points: out (count, N/2+1) float32,
  * power = 0.5 |u_hat|^2 / (s_rec s_norm) when `power`, else |u_hat| / s_rec,
  * with the reference's scaling arrays (reconstruction: N at k = 0, N/2 and
- * N/2 elsewhere; norm compensation: N). 1-D only.
- * Replaces exponax.get_spectrum (_spectral.py:866-1030) for D = 1. */
+ * N/2 elsewhere on the last axis, N on the others; norm compensation:
+ * N^D). For D = 2, 3 the modes are summed over radial buckets
+ * [b - 1/2, b + 1/2) of the wavenumber norm, b = 0..N/2
+ * (radial_binning="sum"); D = 2: N 32..512, D = 3: N 16..256.
+ * Replaces exponax.get_spectrum (_spectral.py:866-1030). */
+/* exmi_spectrum with radial_binning="average" when `average` (NaN for an
+ * empty bucket, as nanmean); D = 1 ignores it. */
+int exmi_spectrum_binned(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x, float* out,
+                         int32_t power, int32_t average, void* stream);
 int exmi_spectrum(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x, float* out,
                   int32_t power, void* stream);
 

@@ -54,3 +54,30 @@ def test_downsample_fine_trajectory(device):
     got = ex.map_between_resolutions(u, 64)
     ref = np.stack([osp.map_between_resolutions(u[i].astype(np.float64), 64) for i in range(8)])
     assert np.abs(got - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("D,N", [(2, 32), (2, 128), (3, 16), (3, 32)])
+@pytest.mark.parametrize("power,binning", [(True, "sum"), (False, "sum"), (True, "average")])
+def test_spectrum_radial(device, D, N, power, binning):
+    """2-D and 3-D spectra binned by the wavenumber norm (_spectral.py:866-1030)."""
+    print(f"[case] spectrum D={D} N={N} power={power} {binning}", flush=True)
+    rng = np.random.default_rng(D * N)
+    u = rng.standard_normal((3, 2) + (N,) * D).astype(np.float32)
+    got = ex.get_spectrum(u, power=power, radial_binning=binning)
+    ref = np.stack([osp.get_spectrum(u[i].astype(np.float64), power=power, radial_binning=binning)
+                    for i in range(3)])
+    assert got.shape == ref.shape == (3, 2, N // 2 + 1)
+    np.testing.assert_allclose(got, ref, rtol=2e-5, atol=1e-7 * np.nanmax(np.abs(ref)))
+
+
+@pytest.mark.parametrize("D,n_old,n_new", [(2, 64, 32), (2, 32, 128), (2, 128, 64), (3, 32, 16), (3, 16, 32)])
+@pytest.mark.parametrize("oddball", [True, False])
+def test_map_between_resolutions_nd(device, D, n_old, n_new, oddball):
+    print(f"[case] resample D={D} {n_old}->{n_new} oddball={oddball}", flush=True)
+    rng = np.random.default_rng(n_old * 7 + n_new + D)
+    u = rng.standard_normal((2, 1) + (n_old,) * D).astype(np.float32)  # full band, Nyquist modes included
+    got = ex.map_between_resolutions(u, n_new, oddball_zero=oddball)
+    ref = np.stack([osp.map_between_resolutions(u[i].astype(np.float64), n_new, oddball_zero=oddball)
+                    for i in range(2)])
+    assert got.shape == ref.shape
+    np.testing.assert_allclose(got, ref, atol=3e-6 * np.abs(ref).max(), rtol=0)
