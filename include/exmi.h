@@ -272,13 +272,14 @@ int exmi_spectrum_binned(int32_t num_spatial_dims, int32_t num_points, int32_t c
 int exmi_spectrum(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x, float* out,
                   int32_t power, void* stream);
 
-/* Spectral up/down-sampling of `count` 1-D signals from n_old to n_new
- * points (powers of two): modes k <= min(n_old, n_new)/2 are carried over
- * with the norm compensation n_new/n_old; with oddball_zero the Nyquist mode
- * of the even source (upsampling) or target (downsampling) grid is zeroed.
- * x (count, n_old) -> y (count, n_new), contiguous. 1-D only.
- * Replaces exponax.map_between_resolutions (_interpolation.py:195-285) for
- * D = 1. */
+/* Spectral up/down-sampling of `count` fields from n_old to n_new points
+ * per axis (powers of two): the modes of the reference's blocks
+ * (_spectral.py:530-611; 1-D: k <= min(n_old, n_new)/2) are carried over
+ * with the norm compensation (n_new/n_old)^D; with oddball_zero the Nyquist
+ * modes of the even source (upsampling) or target (downsampling) grid are
+ * zeroed. x (count, n_old^D) -> y (count, n_new^D), contiguous. D = 1:
+ * 16..4096, D = 2: 32..512, D = 3: 16..256.
+ * Replaces exponax.map_between_resolutions (_interpolation.py:195-285). */
 int exmi_map_resolution(int32_t num_spatial_dims, int32_t n_old, int32_t n_new, int32_t count, const float* x,
                         float* y, int32_t oddball_zero, void* stream);
 
