@@ -6,7 +6,8 @@ The reference transforms once, repeats ``step_fourier`` and transforms back
 - 1-D ETDRK2 steppers sub-step with the Fourier-resident repeat kernel: one
   launch keeps each trajectory's spectrum on chip for all sub-steps, the same
   transform-once sequence;
-- other 1-D orders transform once and launch ``step_fourier`` per sub-step;
+- other 1-D orders and 3-D steppers transform once and launch
+  ``step_fourier`` per sub-step;
 - 2-D steppers (the library steps 2-D states physically) repeat the physical
   step, which differs from the spectral sub-stepping by rounding only.
 """
@@ -66,12 +67,13 @@ class RepeatedStepper:
         if self.num_sub_steps == 0:
             t, back = as_device_tensor(u)
             return restore_like(t.clone(), back)
-        if self.num_spatial_dims == 1 and self.stepper.order != 2:
+        D = self.num_spatial_dims
+        if (D == 1 and self.stepper.order != 2) or D == 3:
             from .spectral import fft, ifft
 
             t, back = as_device_tensor(u)
-            uh = fft(t.to(torch.float32), num_spatial_dims=1)
-            return restore_like(ifft(self.step_fourier(uh), num_spatial_dims=1, num_points=self.num_points), back)
+            uh = fft(t.to(torch.float32), num_spatial_dims=D)
+            return restore_like(ifft(self.step_fourier(uh), num_spatial_dims=D, num_points=self.num_points), back)
         return self._sub.repeat_batch(u, self.num_sub_steps)
 
     def __call__(self, u):

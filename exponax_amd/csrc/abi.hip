@@ -831,19 +831,30 @@ static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t s
 // One 3-D step u_in (batch stride ibs) -> u_out (obs): forward transform,
 // E multiply, inverse transform; strided states pass through the staging
 // buffer.
+// With in_f / out_f the input / output are spectra (C, N, N, N/2+1) with
+// batch strides in complex entries (exmi_step_fourier); nonlin_only
+// evaluates N(u_hat) instead of a step (exmi_nonlinear).
 static int enqueue_step3d(exmi_plan* p, const float* u_in, long long ibs, float* u_out, long long obs,
-                          hipStream_t st) {
+                          hipStream_t st, bool in_f = false, bool out_f = false, bool nonlin_only = false) {
   const int N = p->d.num_points, B = p->d.batch;
   const long long sz = (long long)p->d.num_channels * N * N * N;
+  const long long szs = 2LL * p->d.num_channels * p->nr;  // floats of one spectrum
   hipError_t e = hipSuccess;
-  const float* src = u_in;
-  if (ibs != sz) {
-    e = copy_batch(u_in, ibs, p->P3, sz, sz, B, st);
-    src = p->P3;
+  if (in_f) {
+    e = copy_batch(u_in, 2 * ibs, (float*)p->W3, szs, szs, B, st);
+  } else {
+    const float* src = u_in;
+    if (ibs != sz) {
+      e = copy_batch(u_in, ibs, p->P3, sz, sz, B, st);
+      src = p->P3;
+    }
+    if (e == hipSuccess) e = rfft3(N, p->ninst, src, p->W3, p->tw, p->tw_col, st);
   }
-  if (e == hipSuccess) e = rfft3(N, p->ninst, src, p->W3, p->tw, p->tw_col, st);
   cf* X = p->W3;  // the new state's spectrum
-  if (p->d.order == 0) {
+  if (nonlin_only) {
+    if (e == hipSuccess) e = nonlinear3d(p, p->W3, p->N03, st);
+    X = p->N03;
+  } else if (p->d.order == 0) {
     if (e == hipSuccess)
       e = launch_etdrk0_spec(p->W3, p->ninst, p->nr, p->d.num_channels, p->tables, p->ncoef, p->d.lin_batch,
                              p->d.lin_channels, p->cplx ? num_coefs(0) : -1, st);
@@ -886,9 +897,13 @@ static int enqueue_step3d(exmi_plan* p, const float* u_in, long long ibs, float*
     }
     X = p->S3;
   }
-  float* dst = obs == sz ? u_out : p->P3;
-  if (e == hipSuccess) e = irfft3(N, p->ninst, X, dst, p->tw, p->tw_col, st);
-  if (e == hipSuccess && dst != u_out) e = copy_batch(p->P3, sz, u_out, obs, sz, B, st);
+  if (out_f) {
+    if (e == hipSuccess) e = copy_batch((const float*)X, szs, u_out, 2 * obs, szs, B, st);
+  } else {
+    float* dst = obs == sz ? u_out : p->P3;
+    if (e == hipSuccess) e = irfft3(N, p->ninst, X, dst, p->tw, p->tw_col, st);
+    if (e == hipSuccess && dst != u_out) e = copy_batch(p->P3, sz, u_out, obs, sz, B, st);
+  }
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "3-D step launch failed: %s", hipGetErrorString(e));
 }
 
@@ -1175,14 +1190,16 @@ int exmi_step(exmi_plan* p, const float* u_in, int64_t in_bs, float* u_out, int6
 int exmi_step_fourier(exmi_plan* p, const float* uh_in, int64_t in_bs, float* uh_out, int64_t out_bs,
                       void* stream) {
   if (!p || !uh_in || !uh_out) return fail(EXMI_ERR_INVALID, "null argument");
-  if (p->ndim >= 2) return fail(EXMI_ERR_UNSUPPORTED, "exmi_step_fourier: 2-D and 3-D plans step physical states only");
+  if (p->ndim == 3) return enqueue_step3d(p, uh_in, in_bs, uh_out, out_bs, (hipStream_t)stream, true, true);
+  if (p->ndim == 2) return fail(EXMI_ERR_UNSUPPORTED, "exmi_step_fourier: 2-D plans step physical states only");
   return run_step(p, uh_in, in_bs, 1, uh_out, out_bs, 1, 0, (hipStream_t)stream);
 }
 
 int exmi_nonlinear(exmi_plan* p, const float* uh_in, int64_t in_bs, float* nh_out, int64_t out_bs, void* stream) {
   if (!p || !uh_in || !nh_out) return fail(EXMI_ERR_INVALID, "null argument");
   if (p->d.order == 0) return fail(EXMI_ERR_INVALID, "order-0 plan has no nonlinear function");
-  if (p->ndim >= 2) return fail(EXMI_ERR_UNSUPPORTED, "exmi_nonlinear: not built for 2-D and 3-D plans");
+  if (p->ndim == 3) return enqueue_step3d(p, uh_in, in_bs, nh_out, out_bs, (hipStream_t)stream, true, true, true);
+  if (p->ndim == 2) return fail(EXMI_ERR_UNSUPPORTED, "exmi_nonlinear: not built for 2-D plans");
   return run_step(p, uh_in, in_bs, 1, nh_out, out_bs, 1, 1, (hipStream_t)stream);
 }
 

@@ -181,3 +181,34 @@ def test_abc_flow_decays_viscously(device):
     st = ex.stepper.NavierStokesVelocity(3, 2 * np.pi, N, dt, diffusivity=nu)
     got = st.repeat_batch(torch.tensor(u0[None], dtype=torch.float32, device=device), n)[0].cpu().numpy()
     np.testing.assert_allclose(got, np.exp(-nu * n * dt) * u0, atol=2e-5)
+
+
+def test_step_fourier_nonlinear_and_repeated_3d(device):
+    """3-D step_fourier (_base_stepper.py:222-239), the nonlinear function on
+    spectra and RepeatedStepper (transform once, sub-step in Fourier space)
+    against the oracle."""
+    import torch
+
+    from oracle import spectral as osp
+
+    print("[case] 3-D step_fourier / nonlinear / repeated", flush=True)
+    N, L = 16, 1.0
+    kw = dict(diffusivity=0.01, order=2)
+    gpu = ex.stepper.Burgers(3, L, N, 1e-3, **kw)
+    cpu = ost.Burgers(3, L, N, 1e-3, dtype=np.float32, **kw)
+    u0, _ = ics(2, 3, L, N, 3, seed=131, channels=3)
+    uh = osp.fft(u0, num_spatial_dims=3).astype(np.complex64)
+    got = gpu.step_fourier(torch.tensor(uh, device=device)).cpu().numpy()
+    want = cpu.step_fourier(uh)
+    e_sf = np.abs(got - want).max() / np.abs(want).max()
+    got_n = gpu.nonlinear_fun(torch.tensor(uh, device=device)).cpu().numpy()
+    want_n = cpu.nonlinear_fun(uh)
+    e_nl = np.abs(got_n - want_n).max() / np.abs(want_n).max()
+    rep = ex.RepeatedStepper(gpu, 5).step(torch.tensor(u0, device=device)).cpu().numpy()
+    vh = uh
+    for _ in range(5):
+        vh = cpu.step_fourier(vh)
+    want_r = osp.ifft(vh, num_spatial_dims=3, num_points=N)
+    e_rep = max_rel_l2(rep, want_r)
+    print(f"  step_fourier {e_sf:.2e}, nonlinear {e_nl:.2e}, repeated {e_rep:.2e}", flush=True)
+    assert e_sf <= 1e-5 and e_nl <= 1e-5 and e_rep <= RTOL_STEP
