@@ -49,7 +49,8 @@ def _jobs():
     jobs = [("coef", os.path.join(CSRC, "coef.hip"), []), ("abi", os.path.join(CSRC, "abi.hip"), []),
             ("metrics", os.path.join(CSRC, "metrics.hip"), []),
             ("corrector", os.path.join(CSRC, "corrector.hip"), []),
-            ("spectral3d", os.path.join(CSRC, "spectral3d.hip"), [])]
+            ("spectral3d", os.path.join(CSRC, "spectral3d.hip"), []),
+            ("ic", os.path.join(CSRC, "ic.hip"), [])]
     for m, pt in SIZES_1D.items():
         defs = [f"-DEXMI_M={m}", f"-DEXMI_PT={pt}"] + (["-DEXMI_LAYOUT_1D=NATURAL"] if m % 2 else [])
         jobs.append((f"step1d_{m}", os.path.join(CSRC, "step1d_inst.hip"), defs))

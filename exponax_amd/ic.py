@@ -43,3 +43,35 @@ def truncated_fourier_series(batch, num_spatial_dims, domain_extent, num_points,
             u += a[m] * np.cos(phase_base[m] + ph[m])
         out[b, 0] = u / np.max(np.abs(u))
     return out
+
+
+def truncated_fourier_series_device(batch, num_spatial_dims, domain_extent, num_points, cutoff, seed, *,
+                                    num_channels=1, device="cuda"):
+    """The same recipe on the GPU: (batch, C, N, ..., N) float32 torch tensor.
+
+    The amplitudes and phases are drawn on the host from the same
+    ``numpy.random.default_rng(seed)`` stream and in the same order as the
+    host recipe (per trajectory, per channel: amplitudes, then phases); the
+    device writes the spectra, inverse-transforms them and normalises each
+    field by its max |u| (``exmi_ic_tfs``). ``domain_extent`` does not enter:
+    on the grid x = i L / N the phases are 2 pi k . i / N."""
+    import torch
+
+    from . import _lib
+
+    if cutoff < 1 or cutoff >= num_points // 2:
+        raise ValueError(f"cutoff must be in 1..{num_points // 2 - 1}, got {cutoff}")
+    rng = seed if isinstance(seed, np.random.Generator) else np.random.default_rng(seed)
+    modes = np.ascontiguousarray(_half_plane_modes(num_spatial_dims, cutoff).astype(np.int32))
+    nm, nf = len(modes), batch * num_channels
+    amp = np.empty((nf, nm), dtype=np.float32)
+    phase = np.empty((nf, nm), dtype=np.float32)
+    for f in range(nf):
+        amp[f] = rng.standard_normal(nm)
+        phase[f] = rng.uniform(0.0, 2 * np.pi, nm)
+    a = torch.tensor(amp, device=device)
+    p = torch.tensor(phase, device=device)
+    out = torch.empty((batch, num_channels) + (num_points,) * num_spatial_dims, dtype=torch.float32, device=device)
+    _lib.check(_lib.lib().exmi_ic_tfs(num_spatial_dims, num_points, nf, nm, modes.ctypes.data, a.data_ptr(),
+                                      p.data_ptr(), out.data_ptr(), _lib.stream_ptr()), "exmi_ic_tfs")
+    return out

@@ -265,6 +265,16 @@ int exmi_nrmse(int32_t batch, int32_t channels, int64_t n, const float* pred, in
  * [b - 1/2, b + 1/2) of the wavenumber norm, b = 0..N/2
  * (radial_binning="sum"); D = 2: N 32..512, D = 3: N 16..256.
  * Replaces exponax.get_spectrum (_spectral.py:866-1030). */
+/* Seeded truncated-Fourier-series initial conditions on the device (the
+ * role of ic/_truncated_fourier_series.py:17-100 with the library's own
+ * draws): out (count, N^D) = sum_m amp[f, m] cos(2 pi k_m . x / L +
+ * phase[f, m]) divided by its max |u| per field f. `modes` is a HOST array
+ * (nmodes, D) of distinct half-plane representatives (last non-zero
+ * component positive) with every |k_d| < N/2; amp and phase are device
+ * arrays (count, nmodes). Sizes as for exmi_irfft. */
+int exmi_ic_tfs(int32_t num_spatial_dims, int32_t num_points, int32_t count, int32_t nmodes, const int32_t* modes,
+                const float* amp, const float* phase, float* out, void* stream);
+
 /* exmi_spectrum with radial_binning="average" when `average` (NaN for an
  * empty bucket, as nanmean); D = 1 ignores it. */
 int exmi_spectrum_binned(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x, float* out,

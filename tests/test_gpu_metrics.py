@@ -81,3 +81,19 @@ def test_map_between_resolutions_nd(device, D, n_old, n_new, oddball):
                     for i in range(2)])
     assert got.shape == ref.shape
     np.testing.assert_allclose(got, ref, atol=3e-6 * np.abs(ref).max(), rtol=0)
+
+
+@pytest.mark.parametrize("D,N,C", [(1, 256, 1), (1, 1024, 2), (2, 64, 1), (2, 128, 2), (3, 16, 1), (3, 32, 3)])
+def test_ic_generation_on_device(device, D, N, C):
+    """The seeded truncated-Fourier-series ICs built on the GPU equal the
+    host recipe drawn from the same stream."""
+    from exponax_amd import ic as xic
+    from oracle import ic as oic
+
+    print(f"[case] device ICs D={D} N={N} C={C}", flush=True)
+    cutoff = 3 if D == 3 else 5
+    got = xic.truncated_fourier_series_device(4, D, 2.0, N, cutoff, 17, num_channels=C).cpu().numpy()
+    want, _ = oic.tfs_batch(4, D, 2.0, N, cutoff, 17, num_channels=C)
+    assert got.shape == want.shape
+    np.testing.assert_allclose(got, want, atol=3e-6, rtol=0)
+    assert np.abs(np.abs(got).max(axis=tuple(range(2, got.ndim))) - 1).max() < 1e-6
