@@ -908,6 +908,36 @@ static int enqueue_step3d(exmi_plan* p, const float* u_in, long long ibs, float*
 }
 
 static int enqueue_step2d(exmi_plan* p, const float* u_in, long long ibs, float* u_out, long long obs, bool have_T,
+                          bool write_T, hipStream_t st);
+static int transform2d(bool inverse, int N, int count, const float* x, cf* xh, hipStream_t st);
+
+// 2-D step_fourier: the physical step between the standalone 2-D transforms
+// (irfftn, step, rfftn). For the spectrum of a real field this is the
+// reference's step_fourier up to rounding; imaginary parts that irfftn
+// ignores (the DC and Nyquist columns of the last axis) do not survive.
+static int step_fourier2d(exmi_plan* p, const cf* in, long long ibs, cf* out, long long obs, hipStream_t st) {
+  const int N = p->d.num_points, B = p->d.batch, C = p->d.num_channels;
+  const long long nr = (long long)N * (N / 2 + 1), sz = (long long)C * N * N, szs = (long long)C * nr;
+  float* phys = nullptr;
+  cf* spec = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&phys, sizeof(float) * sz * B, st));
+  hipError_t e = hipMallocAsync((void**)&spec, sizeof(cf) * szs * B, st);
+  int rc = EXMI_OK;
+  if (e == hipSuccess) e = copy_batch((const float*)in, 2 * ibs, (float*)spec, 2 * szs, 2 * szs, B, st);
+  if (e != hipSuccess) rc = fail(EXMI_ERR_HIP, "2-D step_fourier: %s", hipGetErrorString(e));
+  if (!rc) rc = transform2d(true, N, B * C, phys, spec, st);
+  if (!rc) rc = enqueue_step2d(p, phys, sz, phys, sz, false, false, st);
+  if (!rc) rc = transform2d(false, N, B * C, phys, spec, st);
+  if (!rc) {
+    e = copy_batch((const float*)spec, 2 * szs, (float*)out, 2 * obs, 2 * szs, B, st);
+    if (e != hipSuccess) rc = fail(EXMI_ERR_HIP, "2-D step_fourier: %s", hipGetErrorString(e));
+  }
+  (void)hipFreeAsync(phys, st);
+  if (spec) (void)hipFreeAsync(spec, st);
+  return rc;
+}
+
+static int enqueue_step2d(exmi_plan* p, const float* u_in, long long ibs, float* u_out, long long obs, bool have_T,
                           bool write_T, hipStream_t st) {
   Step2DArgs a;
   fill_args2d(p, &a);
@@ -1191,7 +1221,7 @@ int exmi_step_fourier(exmi_plan* p, const float* uh_in, int64_t in_bs, float* uh
                       void* stream) {
   if (!p || !uh_in || !uh_out) return fail(EXMI_ERR_INVALID, "null argument");
   if (p->ndim == 3) return enqueue_step3d(p, uh_in, in_bs, uh_out, out_bs, (hipStream_t)stream, true, true);
-  if (p->ndim == 2) return fail(EXMI_ERR_UNSUPPORTED, "exmi_step_fourier: 2-D plans step physical states only");
+  if (p->ndim == 2) return step_fourier2d(p, (const cf*)uh_in, in_bs, (cf*)uh_out, out_bs, (hipStream_t)stream);
   return run_step(p, uh_in, in_bs, 1, uh_out, out_bs, 1, 0, (hipStream_t)stream);
 }
 

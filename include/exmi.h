@@ -160,13 +160,18 @@ int exmi_step(exmi_plan* plan, const float* u_in, int64_t in_bstride, float* u_o
               int64_t out_bstride, void* stream);
 
 /* One step in Fourier space on (B, C, Nr) complex64 spectra (float pairs);
- * strides in complex elements.
+ * strides in complex elements. 2-D plans step physically between the
+ * standalone transforms (irfftn, step, rfftn): exact up to rounding for the
+ * spectrum of a real field (imaginary parts irfftn ignores are dropped);
+ * coupled-reaction 1-D plans return EXMI_ERR_UNSUPPORTED.
  * Replaces BaseStepper.step_fourier (_base_stepper.py:222-239) /
  * BaseETDRK.step_fourier (etdrk/_base_etdrk.py:64-80). */
 int exmi_step_fourier(exmi_plan* plan, const float* uh_in, int64_t in_bstride, float* uh_out,
                       int64_t out_bstride, void* stream);
 
-/* Dealiased nonlinear term N(u_hat) on (B, C, Nr) complex64 spectra.
+/* Dealiased nonlinear term N(u_hat) on (B, C, Nr) complex64 spectra (1-D
+ * and 3-D plans; 2-D plans and coupled-reaction 1-D plans return
+ * EXMI_ERR_UNSUPPORTED).
  * Replaces BaseNonlinearFun.__call__ (nonlin_fun/_base.py:139-159) of the
  * plan's nonlinear function. */
 int exmi_nonlinear(exmi_plan* plan, const float* uh_in, int64_t in_bstride, float* nh_out,

@@ -78,3 +78,23 @@ def test_single_channel_burgers_and_general_2d(device, order):
                              dtype=np.float32)
     u0, _ = ics(3, 2, 2.0, 64, 4, seed=75)
     _check(gpu, cpu, u0, device, "general")
+
+
+def test_step_fourier_2d(device):
+    """2-D step_fourier (_base_stepper.py:222-239) on the spectrum of a real
+    field against the oracle's."""
+    import torch
+
+    from oracle import spectral as osp
+
+    print("[case] 2-D step_fourier", flush=True)
+    kw = dict(diffusivity=0.01)
+    gpu = ex.stepper.Burgers(2, 1.0, 64, 1e-3, **kw)
+    cpu = ost.Burgers(2, 1.0, 64, 1e-3, dtype=np.float32, **kw)
+    u0, _ = ics(3, 2, 1.0, 64, 4, seed=76, channels=2)
+    uh = osp.fft(u0, num_spatial_dims=2).astype(np.complex64)
+    got = gpu.step_fourier(torch.tensor(uh, device=device)).cpu().numpy()
+    want = cpu.step_fourier(uh)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"  max rel {err:.2e}", flush=True)
+    assert err <= 1e-5
