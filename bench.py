@@ -25,8 +25,9 @@ steps in one graph), ks2d (Kuramoto-Sivashinsky 2-D 128^2, 32 trajectories,
 50 steps), ks2d_grad (the gradient of an MSE loss through those 50 steps,
 checkpoints every 10), corrector (the learned corrector: GPU training data,
 GPU training, corrected coarse rollout of 8192 trajectories against the
-plain fine stepper). They print the same JSON contract for their own
-workload.
+plain fine stepper), ns3d (Navier-Stokes velocity 3-D, 64^3, 8 trajectories
+of three channels, 50 steps). They print the same JSON contract for their
+own workload.
 """
 from __future__ import annotations
 
@@ -377,6 +378,37 @@ def wl_ks2d(args, ex, _lib, torch):
     return res
 
 
+def wl_ns3d(args, ex, _lib, torch):
+    """3-D Navier-Stokes in velocity form (Leray-projected rotational
+    convection), N = 64, L = 2 pi, three channels, ETDRK2."""
+    N, L, DT = 64, 2 * np.pi, 0.01
+    B, K, W = args.batch or 8, args.steps or 50, args.warmup
+    # the seeded recipe built on the device (three channels per trajectory)
+    x = ex.ic.truncated_fourier_series_device(B, 3, L, N, 3, 7, num_channels=3, device="cuda:0")
+    u0 = x.cpu().numpy()
+    st = ex.stepper.NavierStokesVelocity(3, L, N, DT, diffusivity=0.01)
+    if W > 0:
+        st.rollout_batch(x, W)
+    torch.cuda.synchronize()
+    out = {}
+    wall, gpu_s = timed_rollout(lambda: out.__setitem__("tr", st.rollout_batch(x, K)), torch.cuda.synchronize)
+    tr = out["tr"]
+    res = result_line("navier_stokes_velocity3d_n64_b8_etdrk2",
+                      {"equation": "NavierStokesVelocity", "num_points": N, "domain_extent": L, "dt": DT,
+                       "order": 2, "batch": B, "rollout_steps": K, "ic": "TFS cutoff 3 seed 7 (device)"},
+                      B * K / wall, K, W, wall, gpu_s / K, B * 8 * 3 * N ** 3, K,
+                      "3-D transform, field, product, combine and stage passes (per step)", None)
+    if not args.no_cpu_baseline:
+        from oracle import stepper as ost
+
+        nb = 1
+        g = tr[:nb].cpu().numpy()
+        checks = {s: g[:, s - 1] for s in (1, 10)}
+        mk = lambda dt: ost.NavierStokesVelocity(3, L, N, DT, diffusivity=0.01, dtype=dt)  # noqa: E731
+        res["cpu_baseline"], res["parity"] = oracle_baseline(mk, u0[:nb], checks, 10, f"{nb} trajectory x 10 steps")
+    return res
+
+
 def wl_ks2d_grad(args, ex, _lib, torch):
     """Gradient of MSE(u_50, target) w.r.t. u_0 through 50 KS 2-D steps
     (128^2, 32 trajectories), checkpoints every 10 steps: the forward pass
@@ -612,7 +644,8 @@ def wl_cnn_forward(args, ex, _lib, torch):
 
 
 WORKLOADS = {"burgers1d": wl_burgers1d, "ks1d": wl_ks1d, "kolmogorov2d": wl_kolmogorov2d, "ks2d": wl_ks2d,
-             "ks2d_grad": wl_ks2d_grad, "corrector": wl_corrector, "cnn_forward": wl_cnn_forward}
+             "ks2d_grad": wl_ks2d_grad, "corrector": wl_corrector, "cnn_forward": wl_cnn_forward,
+             "ns3d": wl_ns3d}
 
 
 def main():
