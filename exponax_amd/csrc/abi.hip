@@ -476,8 +476,6 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     return fail(EXMI_ERR_INVALID, "lin_channels must be 1 or num_channels (%d), got %d", C, d->lin_channels);
   if (!d->host_linear_operator) return fail(EXMI_ERR_INVALID, "host_linear_operator is null");
   const bool cplx = d->host_linear_operator_imag != nullptr;
-  if (cplx && d->order > 2)
-    return fail(EXMI_ERR_UNSUPPORTED, "2-D plans with a complex linear operator are built for ETDRK orders 0-2");
   int kind = 0;
   bool zero_poly = false, conv_general = false;
   if (d->nonlin_kind == EXMI_NL_VORTICITY || d->nonlin_kind == EXMI_NL_KOLMOGOROV) {
@@ -678,8 +676,7 @@ static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
     return fail(EXMI_ERR_UNSUPPORTED, "3-D num_points=%d: supported sizes are powers of two 16..256", N);
   if (C < 1 || d->batch < 1) return fail(EXMI_ERR_INVALID, "num_channels and batch must be >= 1");
   if (d->order < 0 || d->order > 4) return fail(EXMI_ERR_UNSUPPORTED, "Order %d not implemented.", d->order);
-  if (d->host_linear_operator_imag && d->order > 2)
-    return fail(EXMI_ERR_UNSUPPORTED, "3-D plans with a complex linear operator are built for ETDRK orders 0-2");
+
   int kind = NL_ZERO;
   bool conv_general = false;
   if (d->order > 0) {
@@ -1011,10 +1008,9 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   if (!d->host_linear_operator) return fail(EXMI_ERR_INVALID, "host_linear_operator is null");
   if (d->order > 0 && d->num_circle_points < 1) return fail(EXMI_ERR_INVALID, "num_circle_points must be >= 1");
   const bool cplx = d->host_linear_operator_imag != nullptr;
-  if (cplx && (d->order > 2 || N > 1024 || (d->flags & EXMI_FLAG_FOURIER_RESIDENT)))
+  if (cplx && (N > 1024 || (d->flags & EXMI_FLAG_FOURIER_RESIDENT)))
     return fail(EXMI_ERR_UNSUPPORTED,
-                "complex linear operators are built for ETDRK orders 0-2, N <= 1024, without the Fourier-resident "
-                "flag");
+                "complex linear operators are built for N <= 1024, without the Fourier-resident flag");
   int kind = NL_ZERO;
   if (d->order > 0) {
     int rc = kernel_kind(d, &kind);

@@ -320,40 +320,42 @@ __global__ void stage3d_kernel(Stage3DArgs a) {
     const float* tb = a.tables + ((a.lin_batch > 1 ? traj : 0) * a.lin_channels + (a.lin_channels > 1 ? ch : 0)) *
                                      (long long)a.ncoef * a.nr;
     auto c = [&](int i) { return tb[(long long)i * a.nr + k]; };
-    auto emul = [&](cf x) -> cf {
-      const float er = c(0);
+    // coefficient i (0: E, 1: E_half for orders 3-4) times x, complex when L is
+    auto cmul = [&](int i, cf x) -> cf {
+      const float er = c(i);
       if (a.e_im < 0) return er * x;
-      const float ei = c(a.e_im);
+      const float ei = c(a.e_im + i);
       return cf{fmaf(er, x.re, -ei * x.im), fmaf(er, x.im, ei * x.re)};
     };
+    auto emul = [&](cf x) -> cf { return cmul(0, x); };
     const cf u0 = a.U0[idx];
     switch (a.mode) {
       case ST1:
       case ST2A: a.S[idx] = emul(u0) + c(1) * a.N0[idx]; break;
       case ST2B: a.S[idx] = a.S[idx] + c(2) * (a.NC[idx] - a.N0[idx]); break;
       case ST3A:
-        a.A[idx] = c(0) * u0 + c(4) * a.N0[idx];
-        a.S[idx] = c(1) * u0 + c(2) * a.N0[idx];
+        a.A[idx] = emul(u0) + c(4) * a.N0[idx];
+        a.S[idx] = cmul(1, u0) + c(2) * a.N0[idx];
         break;
       case ST3B:
         a.A[idx] = a.A[idx] + c(5) * a.NC[idx];
-        a.S[idx] = c(0) * u0 + c(3) * (2.f * a.NC[idx] - a.N0[idx]);
+        a.S[idx] = emul(u0) + c(3) * (2.f * a.NC[idx] - a.N0[idx]);
         break;
       case ST3C: a.S[idx] = a.A[idx] + c(6) * a.NC[idx]; break;
       case ST4A: {
-        a.A[idx] = c(0) * u0 + c(3) * a.N0[idx];
-        const cf s = c(1) * u0 + c(2) * a.N0[idx];
+        a.A[idx] = emul(u0) + c(3) * a.N0[idx];
+        const cf s = cmul(1, u0) + c(2) * a.N0[idx];
         a.S1[idx] = s;
         a.S[idx] = s;
         break;
       }
       case ST4B:
         a.A[idx] = a.A[idx] + (2.f * c(4)) * a.NC[idx];
-        a.S[idx] = c(1) * u0 + c(2) * a.NC[idx];
+        a.S[idx] = cmul(1, u0) + c(2) * a.NC[idx];
         break;
       case ST4C:
         a.A[idx] = a.A[idx] + (2.f * c(4)) * a.NC[idx];
-        a.S[idx] = c(1) * a.S1[idx] + c(2) * (2.f * a.NC[idx] - a.N0[idx]);
+        a.S[idx] = cmul(1, a.S1[idx]) + c(2) * (2.f * a.NC[idx] - a.N0[idx]);
         break;
       case ST4D: a.S[idx] = a.A[idx] + c(5) * a.NC[idx]; break;
     }

@@ -387,18 +387,20 @@ struct CoefLds {
 template <int KIND, int ORDER, int M, int PT, int NT, class Coef, int LAY = NATURAL, bool CPLX = false>
 __device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co, const Lane<M, PT>& ln, cf* lds,
                                            const TwRegs<M, PT>& rt, const Step1DArgs& a) {
-  // E x: real E, or complex E with Im E stored after the real coefficients
-  static_assert(!CPLX || ORDER <= 2, "complex linear operators are built for ETDRK orders 0-2");
+  // E x and E_half x: real, or complex with Im E (and Im E_half for orders
+  // 3-4) stored after the real coefficients
   constexpr int NRE = ncoef_of(ORDER);
-  auto emul = [&](int n, int s, cf x) -> cf {
-    const float er = co(n, 0, s);
+  auto cmul = [&](int n, int i, int s, cf x) -> cf {
+    const float er = co(n, i, s);
     if constexpr (CPLX) {
-      const float ei = co(n, NRE, s);
+      const float ei = co(n, NRE + i, s);
       return cf{fmaf(er, x.re, -ei * x.im), fmaf(er, x.im, ei * x.re)};
     } else {
       return er * x;
     }
   };
+  auto emul = [&](int n, int s, cf x) -> cf { return cmul(n, 0, s, x); };
+  auto hmul = [&](int n, int s, cf x) -> cf { return cmul(n, 1, s, x); };
 #define EXMI_EACH(body)                   \
   _Pragma("unroll") for (int n = 0; n < NT; ++n) { \
     _Pragma("unroll") for (int s = 0; s < PT; ++s) { body; } }
@@ -432,20 +434,20 @@ __device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co,
     for (int n = 0; n < NT; ++n) u0n[n] = nyq[n];
     nl_eval<KIND, M, PT, NT, LAY>(S, nyq, N0, n0, ln, lds, rt, a);
     // accumulate the final combination E u + c3 N0 + c4 N1 + c5 N2
-    EXMI_EACH(A[n][s] = co(n, 0, s) * U0[n][s] + co(n, 4, s) * N0[n][s];
-              S[n][s] = co(n, 1, s) * U0[n][s] + co(n, 2, s) * N0[n][s]);  // stage 1
+    EXMI_EACH(A[n][s] = emul(n, s, U0[n][s]) + co(n, 4, s) * N0[n][s];
+              S[n][s] = hmul(n, s, U0[n][s]) + co(n, 2, s) * N0[n][s]);  // stage 1
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      an[n] = co(n, 0, PT) * u0n[n] + co(n, 4, PT) * n0[n];
-      nyq[n] = co(n, 1, PT) * u0n[n] + co(n, 2, PT) * n0[n];
+      an[n] = emul(n, PT, u0n[n]) + co(n, 4, PT) * n0[n];
+      nyq[n] = hmul(n, PT, u0n[n]) + co(n, 2, PT) * n0[n];
     }
     nl_eval<KIND, M, PT, NT, LAY>(S, nyq, Nk, nk, ln, lds, rt, a);  // N1
     EXMI_EACH(A[n][s] = A[n][s] + co(n, 5, s) * Nk[n][s];
-              S[n][s] = co(n, 0, s) * U0[n][s] + co(n, 3, s) * (2.f * Nk[n][s] - N0[n][s]));  // stage 2
+              S[n][s] = emul(n, s, U0[n][s]) + co(n, 3, s) * (2.f * Nk[n][s] - N0[n][s]));  // stage 2
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
       an[n] = an[n] + co(n, 5, PT) * nk[n];
-      nyq[n] = co(n, 0, PT) * u0n[n] + co(n, 3, PT) * (2.f * nk[n] - n0[n]);
+      nyq[n] = emul(n, PT, u0n[n]) + co(n, 3, PT) * (2.f * nk[n] - n0[n]);
     }
     nl_eval<KIND, M, PT, NT, LAY>(S, nyq, Nk, nk, ln, lds, rt, a);  // N2
     EXMI_EACH(S[n][s] = A[n][s] + co(n, 6, s) * Nk[n][s]);
@@ -458,28 +460,28 @@ __device__ __forceinline__ void etdrk_step(cf (*S)[PT], cf* nyq, const Coef& co,
 #pragma unroll
     for (int n = 0; n < NT; ++n) u0n[n] = nyq[n];
     nl_eval<KIND, M, PT, NT, LAY>(S, nyq, N0, n0, ln, lds, rt, a);
-    EXMI_EACH(A[n][s] = co(n, 0, s) * U0[n][s] + co(n, 3, s) * N0[n][s];
-              S1[n][s] = co(n, 1, s) * U0[n][s] + co(n, 2, s) * N0[n][s]);
+    EXMI_EACH(A[n][s] = emul(n, s, U0[n][s]) + co(n, 3, s) * N0[n][s];
+              S1[n][s] = hmul(n, s, U0[n][s]) + co(n, 2, s) * N0[n][s]);
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      an[n] = co(n, 0, PT) * u0n[n] + co(n, 3, PT) * n0[n];
-      s1n[n] = co(n, 1, PT) * u0n[n] + co(n, 2, PT) * n0[n];
+      an[n] = emul(n, PT, u0n[n]) + co(n, 3, PT) * n0[n];
+      s1n[n] = hmul(n, PT, u0n[n]) + co(n, 2, PT) * n0[n];
     }
     nl_eval<KIND, M, PT, NT, LAY>(S1, s1n, Nk, nk, ln, lds, rt, a);  // N1
     EXMI_EACH(A[n][s] = A[n][s] + (co(n, 4, s) * 2.f) * Nk[n][s];
-              S[n][s] = co(n, 1, s) * U0[n][s] + co(n, 2, s) * Nk[n][s]);  // stage 2
+              S[n][s] = hmul(n, s, U0[n][s]) + co(n, 2, s) * Nk[n][s]);  // stage 2
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
       an[n] = an[n] + (co(n, 4, PT) * 2.f) * nk[n];
-      nyq[n] = co(n, 1, PT) * u0n[n] + co(n, 2, PT) * nk[n];
+      nyq[n] = hmul(n, PT, u0n[n]) + co(n, 2, PT) * nk[n];
     }
     nl_eval<KIND, M, PT, NT, LAY>(S, nyq, Nk, nk, ln, lds, rt, a);  // N2
     EXMI_EACH(A[n][s] = A[n][s] + (co(n, 4, s) * 2.f) * Nk[n][s];
-              S[n][s] = co(n, 1, s) * S1[n][s] + co(n, 2, s) * (2.f * Nk[n][s] - N0[n][s]));  // stage 3
+              S[n][s] = hmul(n, s, S1[n][s]) + co(n, 2, s) * (2.f * Nk[n][s] - N0[n][s]));  // stage 3
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
       an[n] = an[n] + (co(n, 4, PT) * 2.f) * nk[n];
-      nyq[n] = co(n, 1, PT) * s1n[n] + co(n, 2, PT) * (2.f * nk[n] - n0[n]);
+      nyq[n] = hmul(n, PT, s1n[n]) + co(n, 2, PT) * (2.f * nk[n] - n0[n]);
     }
     nl_eval<KIND, M, PT, NT, LAY>(S, nyq, Nk, nk, ln, lds, rt, a);  // N3
     EXMI_EACH(S[n][s] = A[n][s] + co(n, 5, s) * Nk[n][s]);

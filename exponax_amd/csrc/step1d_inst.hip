@@ -179,6 +179,8 @@ static StepFn pick_step_c(int order) {
   switch (order) {
     case 1: return &launch_step1d<M, PT, KIND, 1, EXMI_LAYOUT_1D, true>;
     case 2: return &launch_step1d<M, PT, KIND, 2, EXMI_LAYOUT_1D, true>;
+    case 3: return &launch_step1d<M, PT, KIND, 3, EXMI_LAYOUT_1D, true>;
+    case 4: return &launch_step1d<M, PT, KIND, 4, EXMI_LAYOUT_1D, true>;
     default: return nullptr;
   }
 }
@@ -187,6 +189,8 @@ static RollFn pick_roll_c(int order) {
   switch (order) {
     case 1: return &launch_roll1d<M, PT, KIND, 1, 1, false, -1, EXMI_LAYOUT_1D, true>;
     case 2: return &launch_roll1d<M, PT, KIND, 2, 1, false, -1, EXMI_LAYOUT_1D, true>;
+    case 3: return &launch_roll1d<M, PT, KIND, 3, 1, false, -1, EXMI_LAYOUT_1D, true>;
+    case 4: return &launch_roll1d<M, PT, KIND, 4, 1, false, -1, EXMI_LAYOUT_1D, true>;
     default: return nullptr;
   }
 }

@@ -89,7 +89,7 @@ struct Step2DArgs {
   int inj_mode;              // Kolmogorov forcing wavenumber (ky index); -1: none
   float inj_value;           // forcing value at (kx = 0, ky = inj_mode), reference scaling
   int write_T;               // RC: also r2c the new state into T (rollout)
-  int e_im;                  // coefficient index of Im E (complex L, orders 0-2); -1: real L
+  int e_im;                  // coefficient index of Im E (complex L; Im E_half follows for orders 3-4); -1: real L
   int npoly;                 // polynomial coefficients c_0..c_{npoly-1}
   float poly[EXMI_MAX_POLY];
   int poly_lap;              // 1: scale * Laplace of the polynomial's transform (Cahn-Hilliard)
@@ -574,12 +574,15 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
                      a.ncoef * full;
   auto coef = [&](int i, int s) { return tb[(long long)i * full + (long long)ky * N + j + s * T]; };
   // E x with E = exp(L dt), complex when L is (odd derivatives)
-  auto emul = [&](int s, cf x) -> cf {
-    const float er = coef(0, s);
+  // coefficient i (0: E, 1: E_half for orders 3-4) times x, complex when
+  // the plan's L is
+  auto cmul = [&](int i, int s, cf x) -> cf {
+    const float er = coef(i, s);
     if (a.e_im < 0) return er * x;
-    const float ei = coef(a.e_im, s);
+    const float ei = coef(a.e_im + i, s);
     return cf{fmaf(er, x.re, -ei * x.im), fmaf(er, x.im, ei * x.re)};
   };
+  auto emul = [&](int s, cf x) -> cf { return cmul(0, s, x); };
 
   if constexpr (MODE == COL_CA) {
     cf v[1][PT];
@@ -637,8 +640,8 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
       constexpr int cA = MODE == COL_C3A ? 4 : 3;
 #pragma unroll
       for (int s = 0; s < PT; ++s) {
-        Ac[s] = coef(0, s) * U0[s] + coef(cA, s) * Nh[s];
-        S[s] = coef(1, s) * U0[s] + coef(2, s) * Nh[s];
+        Ac[s] = cmul(0, s, U0[s]) + coef(cA, s) * Nh[s];
+        S[s] = cmul(1, s, U0[s]) + coef(2, s) * Nh[s];
       }
       st(a.A, Ac);
       if (active && kept_col) {
@@ -656,7 +659,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
 #pragma unroll
       for (int s = 0; s < PT; ++s) {
         Ac[s] = Ac[s] + coef(5, s) * Nh[s];
-        S[s] = coef(0, s) * U0[s] + coef(3, s) * (2.f * Nh[s] - N0v[s]);
+        S[s] = cmul(0, s, U0[s]) + coef(3, s) * (2.f * Nh[s] - N0v[s]);
       }
       st(a.A, Ac);
       fields_out(S);
@@ -669,13 +672,13 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
       st(a.A, Ac);
       if constexpr (MODE == COL_C4B) {
 #pragma unroll
-        for (int s = 0; s < PT; ++s) S[s] = coef(1, s) * U0[s] + coef(2, s) * Nh[s];
+        for (int s = 0; s < PT; ++s) S[s] = cmul(1, s, U0[s]) + coef(2, s) * Nh[s];
       } else {
         cf N0v[PT], S1v[PT];
         ld_n0(N0v);
         ld(a.S1, S1v);
 #pragma unroll
-        for (int s = 0; s < PT; ++s) S[s] = coef(1, s) * S1v[s] + coef(2, s) * (2.f * Nh[s] - N0v[s]);
+        for (int s = 0; s < PT; ++s) S[s] = cmul(1, s, S1v[s]) + coef(2, s) * (2.f * Nh[s] - N0v[s]);
       }
       fields_out(S);
     } else {

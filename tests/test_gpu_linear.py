@@ -77,13 +77,8 @@ def test_complex_operator_limits(device):
     library's message."""
     import torch
 
-    with pytest.raises(NotImplementedError):  # 2-D complex L: ETDRK orders 0-2
-        ex.stepper.generic.GeneralPolynomialStepper(2, 1.0, 32, 0.01, linear_coefficients=(0.0, 0.5, 0.01),
-                                                    order=3).step(torch.zeros((1, 32, 32), device=device))
     with pytest.raises(NotImplementedError):
         ex.stepper.Advection(1, 1.0, 2048, 0.01).step(torch.zeros((1, 2048), device=device))
-    with pytest.raises(NotImplementedError):
-        ex.stepper.KortewegDeVries(1, 20.0, 64, 1e-3, order=3).step(torch.zeros((1, 64), device=device))
     kdv = ex.stepper.KortewegDeVries(1, 20.0, 64, 1e-3)
     with pytest.raises(NotImplementedError):
         kdv.step_vjp(torch.zeros((1, 64), device=device), torch.zeros((1, 64), device=device))
@@ -111,3 +106,37 @@ def test_polynomial_reaction_parity(device, name, N):
             err = max_rel_l2(traj[:, s - 1], uc)
             print(f"  {name} N={N} step {s}: {err:.2e}", flush=True)
             assert err <= RTOL_STEP
+
+
+@pytest.mark.parametrize("D,N,order", [(1, 64, 3), (1, 256, 4), (1, 1024, 3), (2, 64, 3), (2, 64, 4), (3, 16, 3),
+                                       (3, 16, 4)])
+def test_complex_linear_operator_orders_3_4(device, D, N, order):
+    """Complex exp(L dt) and exp(L dt / 2) in the ETDRK3/4 stages
+    (etdrk/_etdrk_3.py:191-212, _etdrk_4.py:198-224): Korteweg-de Vries in
+    1-D, a general nonlinear stepper with an odd derivative term in 2-D and
+    3-D, against the FP32 oracle."""
+    import torch
+
+    print(f"[case] complex L, D={D} N={N} order {order}", flush=True)
+    if D == 1:
+        gpu = ex.stepper.KortewegDeVries(1, 20.0, N, 1e-3, order=order)
+        cpu = ost.KortewegDeVries(1, 20.0, N, 1e-3, order=order, dtype=np.float32)
+        L = 20.0
+    else:
+        kw = dict(linear_coefficients=(0.0, 0.5, 0.01), order=order)
+        gpu = ex.stepper.generic.GeneralNonlinearStepper(D, 2.0, N, 1e-3, nonlinear_coefficients=(0.3, -1.0, -0.5),
+                                                         **kw)
+        cpu = ost.GeneralStepper(D, 2.0, N, 1e-3, nonlinear_scales=(0.3, -1.0, -0.5), dtype=np.float32, **kw)
+        L = 2.0
+    u0, _ = ics(3, D, L, N, 4, seed=140 + N)
+    x = torch.tensor(u0, device=device)
+    traj = gpu.rollout_batch(x, 10).cpu().numpy()
+    ug, uc = x, u0
+    for s in range(1, 11):
+        ug = gpu.step(ug)
+        uc = cpu.step_batch(uc)
+        if s in (1, 10):
+            e_step = max_rel_l2(ug.cpu().numpy(), uc)
+            e_roll = max_rel_l2(traj[:, s - 1], uc)
+            print(f"  step {s}: steps {e_step:.2e}, rollout {e_roll:.2e}", flush=True)
+            assert e_step <= RTOL_STEP and e_roll <= RTOL_STEP
