@@ -10,6 +10,9 @@ import numpy as np
 
 from ._base_stepper import BaseStepper, Ensemble
 from ._repeated_stepper import RepeatedStepper
+from .stepper._wave import Wave
+
+_STEPPERS = (BaseStepper, RepeatedStepper, Wave)
 
 
 class vmap:
@@ -20,7 +23,7 @@ class vmap:
         self.stepper = stepper
 
     def __call__(self, u):
-        if isinstance(self.stepper, (BaseStepper, RepeatedStepper)):
+        if isinstance(self.stepper, _STEPPERS):
             return self.stepper.step(u)
         return _stack([self.stepper(x) for x in u])
 
@@ -34,11 +37,11 @@ def _stack(xs):
 
 
 def _native(stepper_fn):
-    if isinstance(stepper_fn, vmap) and isinstance(stepper_fn.stepper, (BaseStepper, RepeatedStepper)):
+    if isinstance(stepper_fn, vmap) and isinstance(stepper_fn.stepper, _STEPPERS):
         return stepper_fn.stepper, True
     if isinstance(stepper_fn, Ensemble):
         return stepper_fn, True
-    if isinstance(stepper_fn, (BaseStepper, RepeatedStepper)):
+    if isinstance(stepper_fn, _STEPPERS):
         return stepper_fn, False
     return None, False
 

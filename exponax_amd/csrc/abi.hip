@@ -34,6 +34,8 @@ hipError_t launch_lines(int N, bool inv, cf* data, long long nlines, long long i
 hipError_t launch_etdrk0_spec(cf* X, long long ninst, long long nr, int channels, const float* tables, int ncoef,
                               int lin_batch, int lin_channels, int e_im, hipStream_t st);
 hipError_t launch_fields3d(const NL3DArgs& a, const cf* S, cf* F, hipStream_t st);
+hipError_t launch_wave_spec(cf* X, long long batch, int D, int N, double kscale, double c, double dt,
+                            hipStream_t st);
 hipError_t launch_products3d(const NL3DArgs& a, const float* R, float* PR, hipStream_t st);
 hipError_t launch_combine3d(const NL3DArgs& a, const cf* PH, cf* Nh, hipStream_t st);
 hipError_t launch_stage3d(const Stage3DArgs& a, hipStream_t st);
@@ -1700,6 +1702,17 @@ int exmi_rfft(int32_t D, int32_t N, int32_t count, const float* x, float* x_hat,
   if (rc) return rc;
   hipError_t e = launch_fft1d(N / 2, false, x, x_hat, count, tw.rt(), (hipStream_t)stream);
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "rfft launch failed: %s", hipGetErrorString(e));
+}
+
+int exmi_wave_step_fourier(int32_t D, int32_t N, int32_t batch, double domain_extent, double speed_of_sound,
+                           double dt, float* uh, void* stream) {
+  if (D < 1 || D > 3) return fail(EXMI_ERR_UNSUPPORTED, "exmi_wave_step_fourier: num_spatial_dims 1, 2, 3");
+  if (!uh) return fail(EXMI_ERR_INVALID, "null argument");
+  if (N < 2 || batch < 0 || !(domain_extent > 0)) return fail(EXMI_ERR_INVALID, "exmi_wave_step_fourier: sizes");
+  if (batch == 0) return EXMI_OK;
+  const hipError_t e = launch_wave_spec((cf*)uh, batch, D, N, 2.0 * 3.14159265358979323846 / domain_extent,
+                                        speed_of_sound, dt, (hipStream_t)stream);
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "wave step launch failed: %s", hipGetErrorString(e));
 }
 
 int exmi_irfft(int32_t D, int32_t N, int32_t count, const float* x_hat, float* x, void* stream) {

@@ -368,3 +368,55 @@ hipError_t launch_stage3d(const Stage3DArgs& a, hipStream_t st) {
 }
 
 }  // namespace exmi
+
+namespace exmi {
+
+// The wave equation's exact step in Fourier space (stepper/_wave.py:130-197)
+// on (batch, 2, Nr) spectra of (h, v) in D = 1, 2, 3. The reference rotates
+// into the modes (w +- v)/sqrt 2 with w = i c |k| h, advances them by
+// exp(+- i c |k| dt), rotates back and divides by i c |k|; multiplied out
+// this is, with theta = c |k| dt,
+//   h' = cos(theta) h + sin(theta) / (c |k|) v,  v' = -c |k| sin(theta) h + cos(theta) v,
+// and at the mean mode h' = h + dt v, v' = v (the reference's drift term).
+// The product form avoids the float32 cancellation of w e^{i theta} -
+// w e^{-i theta} when c |k| |h| >> |v| (the reference's own float32 error
+// grows with c |k|); the coefficients are formed in float64.
+__global__ void wave_spec_kernel(cf* X, long long batch, int D, int N, double kscale, double c, double dt) {
+  const int NZ = N / 2 + 1;
+  long long nr = NZ;
+  for (int d = 1; d < D; ++d) nr *= N;
+  const long long total = batch * nr;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const long long b = idx / nr, k = idx % nr;
+    long long r = k / NZ;
+    const int kz = (int)(k % NZ);
+    double k2 = (double)kz * kz;
+    for (int d = 1; d < D; ++d) {
+      const int i = (int)(r % N);
+      r /= N;
+      const int kk = i < N / 2 ? i : i - N;
+      k2 += (double)kk * kk;
+    }
+    const double ck = c * kscale * sqrt(k2);
+    double sn, cs;
+    sincos(ck * dt, &sn, &cs);
+    const bool dc = k2 == 0.0;
+    const float a = (float)cs, bh = (float)(dc ? dt : sn / ck), bv = (float)(-ck * sn);
+    cf* h = X + (b * 2) * nr + k;
+    cf* v = X + (b * 2 + 1) * nr + k;
+    const cf h0 = *h, v0 = *v;
+    *h = cf{fmaf(a, h0.re, bh * v0.re), fmaf(a, h0.im, bh * v0.im)};
+    *v = cf{fmaf(bv, h0.re, a * v0.re), fmaf(bv, h0.im, a * v0.im)};
+  }
+}
+
+hipError_t launch_wave_spec(cf* X, long long batch, int D, int N, double kscale, double c, double dt,
+                            hipStream_t st) {
+  long long nr = N / 2 + 1;
+  for (int d = 1; d < D; ++d) nr *= N;
+  hipLaunchKernelGGL(wave_spec_kernel, dim3(grid_of(batch * nr)), dim3(256), 0, st, X, batch, D, N, kscale, c, dt);
+  return hipGetLastError();
+}
+
+}  // namespace exmi

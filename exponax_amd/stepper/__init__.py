@@ -7,7 +7,8 @@ from ._linear import Advection, AdvectionDiffusion, Diffusion, Dispersion, Hyper
 from ._navier_stokes import KolmogorovFlowVelocity, KolmogorovFlowVorticity, NavierStokesVelocity, NavierStokesVorticity
 from ._reaction import AllenCahn, FisherKPP, SwiftHohenberg
 from . import reaction
+from ._wave import Wave
 
 __all__ = ["generic", "reaction", "AllenCahn", "FisherKPP", "SwiftHohenberg", "Advection", "AdvectionDiffusion", "Burgers", "Diffusion", "Dispersion", "HyperDiffusion",
            "KortewegDeVries", "KuramotoSivashinsky", "KuramotoSivashinskyConservative", "KolmogorovFlowVorticity",
-           "NavierStokesVorticity", "NavierStokesVelocity", "KolmogorovFlowVelocity"]
+           "NavierStokesVorticity", "NavierStokesVelocity", "KolmogorovFlowVelocity", "Wave"]

@@ -270,6 +270,14 @@ int exmi_nrmse(int32_t batch, int32_t channels, int64_t n, const float* pred, in
  * [b - 1/2, b + 1/2) of the wavenumber norm, b = 0..N/2
  * (radial_binning="sum"); D = 2: N 32..512, D = 3: N 16..256.
  * Replaces exponax.get_spectrum (_spectral.py:866-1030). */
+/* The wave equation's exact Fourier-space step, in place on `batch`
+ * contiguous (2, N^(D-1), N/2+1) complex64 spectra of (h, v): the
+ * diagonalisation, exp(+- i c |k| dt), the inverse diagonalisation and the
+ * mean mode's drift h += dt v.
+ * Replaces Wave.step_fourier (stepper/_wave.py:130-197). */
+int exmi_wave_step_fourier(int32_t num_spatial_dims, int32_t num_points, int32_t batch, double domain_extent,
+                           double speed_of_sound, double dt, float* uh, void* stream);
+
 /* Seeded truncated-Fourier-series initial conditions on the device (the
  * role of ic/_truncated_fourier_series.py:17-100 with the library's own
  * draws): out (count, N^D) = sum_m amp[f, m] cos(2 pi k_m . x / L +

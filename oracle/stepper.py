@@ -511,6 +511,49 @@ class NavierStokesVelocity(BaseStepper):
                                                   dealiasing_fraction=self.dealiasing_fraction, dtype=self._kw_dtype)
 
 
+class Wave:
+    """``stepper/_wave.py:9-197``: h_t = v, v_t = c^2 Laplace h, stepped by
+    the reference's diagonalisation: w = i c |k| h (|k| -> 1 at the mean
+    mode), modes (w +- v)/sqrt 2 advanced by exp(+- i c |k| dt) (ETDRK0 with
+    L = (i c |k|, -i c |k|)), rotated back, h = w / (i c |k|), and the mean
+    mode's drift h += dt v."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, dt, *, speed_of_sound=1.0, dtype=np.float64):
+        self.num_spatial_dims, self.num_points, self.dt = num_spatial_dims, num_points, dt
+        self.speed_of_sound = speed_of_sound
+        self.dtype = np.dtype(dtype)
+        cdt = sp.complex_of(dtype)
+        k = sp.build_wavenumbers(num_spatial_dims, num_points) * (2 * np.pi / domain_extent)
+        self.knorm = np.linalg.norm(k, axis=0, keepdims=True)
+        lin = 1j * speed_of_sound * self.knorm
+        self.exp_term = np.exp(dt * np.concatenate([lin, -lin])).astype(cdt)
+        self.kguard = np.where(self.knorm == 0, 1.0, self.knorm)
+
+    def step_fourier(self, u_hat):
+        cdt = u_hat.dtype
+        ax = u_hat.ndim - self.num_spatial_dims - 1
+        h, v = np.take(u_hat, [0], axis=ax), np.take(u_hat, [1], axis=ax)
+        w = (1j * self.speed_of_sound * self.kguard).astype(cdt) * h
+        r = cdt.type(1 / np.sqrt(2))
+        waves = np.concatenate([r * (w + v), r * (w - v)], axis=ax) * self.exp_term
+        p, q = np.take(waves, [0], axis=ax), np.take(waves, [1], axis=ax)
+        w1, v1 = r * (p + q), r * (p - q)
+        h1 = w1 / (1j * self.speed_of_sound * self.kguard).astype(cdt)
+        out = np.concatenate([h1, v1], axis=ax)
+        dc = (slice(None),) * ax + (0,) + (0,) * self.num_spatial_dims
+        dcv = (slice(None),) * ax + (1,) + (0,) * self.num_spatial_dims
+        out[dc] = out[dc] + cdt.type(self.dt) * u_hat[dcv]
+        return out
+
+    def step_batch(self, u):
+        u = np.asarray(u, dtype=self.dtype)
+        uh = sp.fft(u, num_spatial_dims=self.num_spatial_dims).astype(sp.complex_of(self.dtype))
+        return sp.ifft(self.step_fourier(uh), num_spatial_dims=self.num_spatial_dims,
+                       num_points=self.num_points).astype(self.dtype, copy=False)
+
+    step = step_batch
+
+
 class GeneralStepper(BaseStepper):
     """Linear part sum_j a_j sum_d (i kappa_d)^j (even j here) + ``GeneralNonlinearFun``.
 

@@ -140,3 +140,44 @@ def test_complex_linear_operator_orders_3_4(device, D, N, order):
             e_roll = max_rel_l2(traj[:, s - 1], uc)
             print(f"  step {s}: steps {e_step:.2e}, rollout {e_roll:.2e}", flush=True)
             assert e_step <= RTOL_STEP and e_roll <= RTOL_STEP
+
+
+@pytest.mark.parametrize("D,N", [(1, 64), (1, 1024), (2, 64), (3, 16)])
+def test_wave_parity_and_exact(device, D, N):
+    """stepper/_wave.py: the exact per-mode step against the FP64 oracle at
+    steps 1 and 10 (single steps, rollout, vmap) and the standing wave's
+    closed form. v' = -c|k| sin(c|k| dt) h + ... amplifies the float32
+    rounding of the transforms at high modes by ~(c|k|)^2 dt (4e4 at
+    N = 1024, c = 1.3, L = 2): the reference's own float32 arithmetic (the
+    FP32 oracle) is 4e-5 from the FP64 oracle there. The gate is the FP64
+    oracle at max(1e-5, 2x the FP32 oracle's own distance)."""
+    import torch
+
+    print(f"[case] wave D={D} N={N}", flush=True)
+    L, dt, c = 2.0, 0.01, 1.3
+    gpu = ex.stepper.Wave(D, L, N, dt, speed_of_sound=c)
+    c64 = ost.Wave(D, L, N, dt, speed_of_sound=c, dtype=np.float64)
+    c32 = ost.Wave(D, L, N, dt, speed_of_sound=c, dtype=np.float32)
+    u0, _ = ics(3, D, L, N, 4, seed=150 + N, channels=2)
+    x = torch.tensor(u0, device=device)
+    traj = ex.rollout(ex.vmap(gpu), 10)(x).cpu().numpy()
+    ug, u64, u32 = x, u0.astype(np.float64), u0
+    for s in range(1, 11):
+        ug = ex.vmap(gpu)(ug)
+        u64 = c64.step_batch(u64)
+        u32 = c32.step_batch(u32)
+        if s in (1, 10):
+            e_step = max_rel_l2(ug.cpu().numpy(), u64)
+            e_roll = max_rel_l2(traj[:, s - 1], u64)
+            e_ref = max_rel_l2(u32, u64)
+            gate = max(RTOL_STEP, 2 * e_ref)
+            print(f"  step {s}: steps {e_step:.2e}, rollout {e_roll:.2e}, FP32 oracle {e_ref:.2e} (vs FP64)",
+                  flush=True)
+            assert e_step <= gate and e_roll <= gate
+    xs = np.linspace(0.0, L, N, endpoint=False)
+    kap = 2 * np.pi * 3 / L
+    prof = np.cos(kap * xs).reshape((N,) + (1,) * (D - 1)) * np.ones((N,) * D)
+    st0 = torch.tensor(np.stack([prof, np.zeros_like(prof)]), dtype=torch.float32, device=device)
+    got = ex.repeat(gpu, 50)(st0).cpu().numpy()
+    t = 50 * dt
+    np.testing.assert_allclose(got[0], prof * np.cos(c * kap * t), atol=2e-5)

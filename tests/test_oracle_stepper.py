@@ -511,3 +511,44 @@ def test_kolmogorov_velocity_from_rest():
     want = osp.ifft((np.exp(lam * dt) - 1) / lam * f_hat, num_spatial_dims=3, num_points=N)
     np.testing.assert_allclose(u1, want, atol=1e-9 * np.abs(want).max())
     assert np.abs(want).max() > 0.01
+
+
+@pytest.mark.parametrize("D", [1, 2])
+def test_wave_exact_standing_wave_and_drift(D):
+    """h = cos(kappa x), v = 0 gives h = cos(kappa x) cos(c kappa t), v = -c kappa
+    cos(kappa x) sin(c kappa t); a uniform (a, b) drifts as h = a + b t."""
+    L, N, dt, c, m, n = 2.0, 16, 0.07, 1.3, 3, 9
+    x = _grid(L, N)
+    kap = 2 * np.pi * m / L
+    prof = np.cos(kap * x) if D == 1 else np.cos(kap * x)[:, None] * np.ones(N)
+    wave = st.Wave(D, L, N, dt, speed_of_sound=c)
+    u = np.stack([prof, np.zeros_like(prof)])
+    for _ in range(n):
+        u = wave.step(u)
+    t = n * dt
+    np.testing.assert_allclose(u[0], prof * np.cos(c * kap * t), atol=1e-12)
+    np.testing.assert_allclose(u[1], -c * kap * prof * np.sin(c * kap * t), atol=1e-11)
+    u = np.stack([np.full_like(prof, 0.4), np.full_like(prof, -0.3)])
+    for _ in range(n):
+        u = wave.step(u)
+    np.testing.assert_allclose(u[0], 0.4 - 0.3 * t, atol=1e-12)
+    np.testing.assert_allclose(u[1], -0.3, atol=1e-12)
+
+
+def test_wave_diagonalisation_equals_product_form():
+    """The reference's rotation (w +- v)/sqrt 2, w = i c |k| h, advanced by
+    exp(+- i c |k| dt) and rotated back equals h' = cos h + sin/(c|k|) v,
+    v' = -c|k| sin h + cos v (the GPU's form), and h' = h + dt v at k = 0."""
+    from oracle import spectral as osp
+
+    N, L, dt, c = 32, 2.0, 0.03, 1.3
+    wave = st.Wave(2, L, N, dt, speed_of_sound=c)
+    rng = np.random.default_rng(0)
+    uh = osp.fft(rng.standard_normal((2, N, N)), num_spatial_dims=2)
+    got = wave.step_fourier(uh)
+    ck = c * wave.knorm[0]
+    th = ck * dt
+    with np.errstate(divide="ignore", invalid="ignore"):
+        bh = np.where(ck == 0, dt, np.sin(th) / np.where(ck == 0, 1.0, ck))
+    want = np.stack([np.cos(th) * uh[0] + bh * uh[1], -ck * np.sin(th) * uh[0] + np.cos(th) * uh[1]])
+    np.testing.assert_allclose(got, want, atol=1e-10 * np.abs(uh).max())
