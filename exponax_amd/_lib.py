@@ -41,7 +41,7 @@ EXPORTED = (
     "exmi_rollout", "exmi_rollout_prepare", "exmi_rollout_group", "exmi_repeat", "exmi_rfft", "exmi_irfft", "exmi_plan_num_coefs", "exmi_plan_get_tables",
     "exmi_plan_rollout_strategy", "exmi_plan_set_rollout_strategy", "exmi_last_error", "exmi_info",
     "exmi_step_vjp", "exmi_rollout_vjp", "exmi_nrmse", "exmi_spectrum", "exmi_spectrum_binned",
-    "exmi_map_resolution", "exmi_ic_tfs", "exmi_wave_step_fourier",
+    "exmi_map_resolution", "exmi_ic_tfs", "exmi_wave_step_fourier", "exmi_poisson_fourier", "exmi_axpy",
     "exmi_cnn_workspace_bytes", "exmi_cnn_forward", "exmi_cnn_backward", "exmi_corrected_rollout",
 )
 EXMI_CNN_NUM_PARAMS = 15809
@@ -112,6 +112,8 @@ def _declare(lib):
     lib.exmi_spectrum_binned.argtypes = [i32, i32, i32, vp, vp, i32, i32, vp]
     lib.exmi_ic_tfs.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, vp]
     lib.exmi_wave_step_fourier.argtypes = [i32, i32, i32, C.c_double, C.c_double, C.c_double, vp, vp]
+    lib.exmi_poisson_fourier.argtypes = [i32, i32, i32, C.c_double, i32, vp, vp]
+    lib.exmi_axpy.argtypes = [C.c_int64, C.c_float, vp, vp, vp, vp]
     lib.exmi_map_resolution.argtypes = [i32, i32, i32, i32, vp, vp, i32, vp]
     lib.exmi_cnn_workspace_bytes.argtypes = [i32, i32]
     lib.exmi_cnn_forward.argtypes = [vp, i32, i32, vp, vp, vp, vp]

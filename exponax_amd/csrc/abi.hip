@@ -36,6 +36,8 @@ hipError_t launch_etdrk0_spec(cf* X, long long ninst, long long nr, int channels
 hipError_t launch_fields3d(const NL3DArgs& a, const cf* S, cf* F, hipStream_t st);
 hipError_t launch_wave_spec(cf* X, long long batch, int D, int N, double kscale, double c, double dt,
                             hipStream_t st);
+hipError_t launch_poisson_spec(cf* X, long long count, int D, int N, double kscale, int order, hipStream_t st);
+hipError_t launch_axpy(long long n, float a, const float* x, const float* y, float* out, hipStream_t st);
 hipError_t launch_products3d(const NL3DArgs& a, const float* R, float* PR, hipStream_t st);
 hipError_t launch_combine3d(const NL3DArgs& a, const cf* PH, cf* Nh, hipStream_t st);
 hipError_t launch_stage3d(const Stage3DArgs& a, hipStream_t st);
@@ -1702,6 +1704,25 @@ int exmi_rfft(int32_t D, int32_t N, int32_t count, const float* x, float* x_hat,
   if (rc) return rc;
   hipError_t e = launch_fft1d(N / 2, false, x, x_hat, count, tw.rt(), (hipStream_t)stream);
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "rfft launch failed: %s", hipGetErrorString(e));
+}
+
+int exmi_poisson_fourier(int32_t D, int32_t N, int32_t count, double domain_extent, int32_t order, float* uh,
+                         void* stream) {
+  if (D < 1 || D > 3) return fail(EXMI_ERR_UNSUPPORTED, "exmi_poisson_fourier: num_spatial_dims 1, 2, 3");
+  if (!uh) return fail(EXMI_ERR_INVALID, "null argument");
+  if (N < 2 || count < 0 || !(domain_extent > 0) || order < 2 || order % 2)
+    return fail(EXMI_ERR_INVALID, "exmi_poisson_fourier: sizes, or an odd order");
+  if (count == 0) return EXMI_OK;
+  const hipError_t e = launch_poisson_spec((cf*)uh, count, D, N, 2.0 * 3.14159265358979323846 / domain_extent, order,
+                                           (hipStream_t)stream);
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "Poisson launch failed: %s", hipGetErrorString(e));
+}
+
+int exmi_axpy(int64_t n, float a, const float* x, const float* y, float* out, void* stream) {
+  if (n < 0 || (n > 0 && (!x || !y || !out))) return fail(EXMI_ERR_INVALID, "exmi_axpy: arguments");
+  if (n == 0) return EXMI_OK;
+  const hipError_t e = launch_axpy(n, a, x, y, out, (hipStream_t)stream);
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "axpy launch failed: %s", hipGetErrorString(e));
 }
 
 int exmi_wave_step_fourier(int32_t D, int32_t N, int32_t batch, double domain_extent, double speed_of_sound,

@@ -420,3 +420,53 @@ hipError_t launch_wave_spec(cf* X, long long batch, int D, int N, double kscale,
 }
 
 }  // namespace exmi
+
+namespace exmi {
+
+// Poisson solve in Fourier space (_poisson.py): u_hat = -f_hat / Laplace_p(k),
+// Laplace_p = sum_d (i kappa_d)^p (p even), and 0 where that operator is 0
+// (the mean-zero solution). count spectra of N^(D-1) (N/2+1) modes, in place.
+__global__ void poisson_spec_kernel(cf* X, long long count, int D, int N, double kscale, int order) {
+  const int NZ = N / 2 + 1;
+  long long nr = NZ;
+  for (int d = 1; d < D; ++d) nr *= N;
+  const long long total = count * nr;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const long long k = idx % nr;
+    long long r = k / NZ;
+    const int kz = (int)(k % NZ);
+    // (i kappa)^p = (-1)^(p/2) kappa^p for even p
+    const double sgn = (order / 2) % 2 ? -1.0 : 1.0;
+    double op = sgn * pow(kscale * kz, order);
+    for (int d = 1; d < D; ++d) {
+      const int i = (int)(r % N);
+      r /= N;
+      const int kk = i < N / 2 ? i : i - N;
+      op += sgn * pow(kscale * kk, order);
+    }
+    const float f = op == 0.0 ? 0.f : (float)(-1.0 / op);
+    X[idx] = f * X[idx];
+  }
+}
+
+hipError_t launch_poisson_spec(cf* X, long long count, int D, int N, double kscale, int order, hipStream_t st) {
+  long long nr = N / 2 + 1;
+  for (int d = 1; d < D; ++d) nr *= N;
+  hipLaunchKernelGGL(poisson_spec_kernel, dim3(grid_of(count * nr)), dim3(256), 0, st, X, count, D, N, kscale, order);
+  return hipGetLastError();
+}
+
+// out = x + a y over n floats (the forcing of ForcedStepper, _forced_stepper.py:33-56)
+__global__ void axpy_kernel(long long n, float a, const float* __restrict__ x, const float* __restrict__ y,
+                            float* __restrict__ out) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = fmaf(a, y[i], x[i]);
+}
+
+hipError_t launch_axpy(long long n, float a, const float* x, const float* y, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(axpy_kernel, dim3(grid_of(n)), dim3(256), 0, st, n, a, x, y, out);
+  return hipGetLastError();
+}
+
+}  // namespace exmi

@@ -270,6 +270,18 @@ int exmi_nrmse(int32_t batch, int32_t channels, int64_t n, const float* pred, in
  * [b - 1/2, b + 1/2) of the wavenumber norm, b = 0..N/2
  * (radial_binning="sum"); D = 2: N 32..512, D = 3: N 16..256.
  * Replaces exponax.get_spectrum (_spectral.py:866-1030). */
+/* Poisson solve in Fourier space, in place on `count` contiguous
+ * (N^(D-1), N/2+1) complex64 spectra: u_hat = -f_hat / sum_d (i kappa_d)^p
+ * (p = order, even), 0 where that operator vanishes (mean-zero solution).
+ * Replaces Poisson.step_fourier (_poisson.py:12-80). */
+int exmi_poisson_fourier(int32_t num_spatial_dims, int32_t num_points, int32_t count, double domain_extent,
+                         int32_t order, float* uh, void* stream);
+
+/* out = x + a y over n floats (device arrays; out may alias x or y).
+ * Replaces the forcing of ForcedStepper.step / step_fourier
+ * (_forced_stepper.py:33-56): u + dt f. */
+int exmi_axpy(int64_t n, float a, const float* x, const float* y, float* out, void* stream);
+
 /* The wave equation's exact Fourier-space step, in place on `batch`
  * contiguous (2, N^(D-1), N/2+1) complex64 spectra of (h, v): the
  * diagonalisation, exp(+- i c |k| dt), the inverse diagonalisation and the

@@ -554,6 +554,37 @@ class Wave:
     step = step_batch
 
 
+class Poisson:
+    """``_poisson.py:12-80``: u_hat = -f_hat / Laplace_order, 0 where the
+    operator vanishes."""
+
+    def __init__(self, num_spatial_dims, domain_extent, num_points, *, order=2, dtype=np.float64):
+        self.num_spatial_dims, self.num_points, self.dtype = num_spatial_dims, num_points, np.dtype(dtype)
+        d = sp.build_derivative_operator(num_spatial_dims, domain_extent, num_points)
+        op = sp.build_laplace_operator(d, order=order)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            self.inv = np.where(op == 0, 0.0, 1.0 / np.where(op == 0, 1.0, op)).astype(sp.complex_of(dtype))
+
+    def step_fourier(self, f_hat):
+        return -self.inv * f_hat
+
+    def step(self, f):
+        f = np.asarray(f, dtype=self.dtype)
+        fh = sp.fft(f, num_spatial_dims=self.num_spatial_dims).astype(sp.complex_of(self.dtype))
+        return sp.ifft(self.step_fourier(fh), num_spatial_dims=self.num_spatial_dims,
+                       num_points=self.num_points).astype(self.dtype, copy=False)
+
+
+class ForcedStepper:
+    """``_forced_stepper.py:8-56``: stepper(u + dt f)."""
+
+    def __init__(self, stepper):
+        self.stepper = stepper
+
+    def step_batch(self, u, f):
+        return self.stepper.step_batch(u + self.stepper.dt * f)
+
+
 class GeneralStepper(BaseStepper):
     """Linear part sum_j a_j sum_d (i kappa_d)^j (even j here) + ``GeneralNonlinearFun``.
 

@@ -181,3 +181,39 @@ def test_wave_parity_and_exact(device, D, N):
     got = ex.repeat(gpu, 50)(st0).cpu().numpy()
     t = 50 * dt
     np.testing.assert_allclose(got[0], prof * np.cos(c * kap * t), atol=2e-5)
+
+
+@pytest.mark.parametrize("D,N,order", [(1, 256, 2), (2, 64, 2), (2, 64, 4), (3, 32, 2)])
+def test_poisson_parity(device, D, N, order):
+    """_poisson.py: the spectral Poisson solve against the FP32 oracle and
+    the mean-zero convention."""
+    import torch
+
+    print(f"[case] Poisson D={D} N={N} order {order}", flush=True)
+    f, _ = ics(3, D, 3.0, N, 4, seed=160 + N)
+    f = f + 0.3
+    got = ex.Poisson(D, 3.0, N, order=order).step(torch.tensor(f, device=device)).cpu().numpy()
+    want = ost.Poisson(D, 3.0, N, order=order, dtype=np.float32).step(f)
+    err = max_rel_l2(got, want)
+    print(f"  rel-L2 {err:.2e}, max |mean| {np.abs(got.mean(axis=tuple(range(2, got.ndim)))).max():.1e}", flush=True)
+    assert err <= RTOL_STEP
+    assert np.abs(got.mean(axis=tuple(range(2, got.ndim)))).max() < 1e-5
+
+
+def test_forced_stepper_parity(device):
+    """_forced_stepper.py: stepper(u + dt f) around 1-D Burgers and 2-D KS."""
+    import torch
+
+    print("[case] ForcedStepper", flush=True)
+    for D, N, L, mk, mko in [
+            (1, 256, 1.0, lambda: ex.stepper.Burgers(1, 1.0, 256, 1e-3, diffusivity=0.01),
+             lambda: ost.Burgers(1, 1.0, 256, 1e-3, diffusivity=0.01, dtype=np.float32)),
+            (2, 64, 30.0, lambda: ex.stepper.KuramotoSivashinsky(2, 30.0, 64, 0.05),
+             lambda: ost.KuramotoSivashinsky(2, 30.0, 64, 0.05, dtype=np.float32))]:
+        u0, _ = ics(3, D, L, N, 4, seed=170 + N)
+        f, _ = ics(3, D, L, N, 3, seed=171 + N)
+        got = ex.ForcedStepper(mk()).step(torch.tensor(u0, device=device), torch.tensor(f, device=device))
+        want = ost.ForcedStepper(mko()).step_batch(u0, f)
+        err = max_rel_l2(got.cpu().numpy(), want)
+        print(f"  D={D}: rel-L2 {err:.2e}", flush=True)
+        assert err <= RTOL_STEP

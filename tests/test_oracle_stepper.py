@@ -552,3 +552,32 @@ def test_wave_diagonalisation_equals_product_form():
         bh = np.where(ck == 0, dt, np.sin(th) / np.where(ck == 0, 1.0, ck))
     want = np.stack([np.cos(th) * uh[0] + bh * uh[1], -ck * np.sin(th) * uh[0] + np.cos(th) * uh[1]])
     np.testing.assert_allclose(got, want, atol=1e-10 * np.abs(uh).max())
+
+
+@pytest.mark.parametrize("D", [1, 2, 3])
+def test_poisson_closed_form(D):
+    """-Laplace u = f for f = sin(2 pi (k . x) / L) gives u = f / |kappa|^2
+    (tests/test_validation.py:214-292); order 4 divides by |kappa|^4 summed per
+    axis; the mean is dropped."""
+    L, N = 3.0, 16
+    x = _grid(L, N)
+    grids = np.meshgrid(*([x] * D), indexing="ij")
+    ks = [2, -1, 3][:D]
+    arg = sum(k * g for k, g in zip(ks, grids)) * 2 * np.pi / L
+    f = np.sin(arg) + 0.7
+    kap2 = sum((2 * np.pi * k / L) ** 2 for k in ks)
+    u = st.Poisson(D, L, N).step(f[None])[0]
+    np.testing.assert_allclose(u, np.sin(arg) / kap2, atol=1e-12)
+    kap4 = sum((2 * np.pi * k / L) ** 4 for k in ks)
+    u4 = st.Poisson(D, L, N, order=4).step(f[None])[0]
+    np.testing.assert_allclose(u4, -np.sin(arg) / kap4, atol=1e-12)
+
+
+def test_forced_diffusion_closed_form():
+    """ForcedStepper around exact diffusion: u' = E (u + dt f) per mode."""
+    L, N, dt, nu, m = 2.0, 32, 0.05, 0.03, 2
+    x = _grid(L, N)
+    kap = 2 * np.pi * m / L
+    u, f = np.cos(kap * x)[None], 0.5 * np.cos(kap * x)[None]
+    got = st.ForcedStepper(st.Diffusion(1, L, N, dt, diffusivity=nu)).step_batch(u, f)
+    np.testing.assert_allclose(got, np.exp(-nu * kap**2 * dt) * (1 + 0.5 * dt) * u, atol=1e-12)
