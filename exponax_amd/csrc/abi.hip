@@ -1229,7 +1229,24 @@ int exmi_nonlinear(exmi_plan* p, const float* uh_in, int64_t in_bs, float* nh_ou
   if (!p || !uh_in || !nh_out) return fail(EXMI_ERR_INVALID, "null argument");
   if (p->d.order == 0) return fail(EXMI_ERR_INVALID, "order-0 plan has no nonlinear function");
   if (p->ndim == 3) return enqueue_step3d(p, uh_in, in_bs, nh_out, out_bs, (hipStream_t)stream, true, true, true);
-  if (p->ndim == 2) return fail(EXMI_ERR_UNSUPPORTED, "exmi_nonlinear: not built for 2-D plans");
+  if (p->ndim == 2) {
+    // the nonlinear evaluation of the 2-D step chain, from and to spectra
+    if (p->cplx) return fail(EXMI_ERR_UNSUPPORTED, "exmi_nonlinear: 2-D plans with a complex L");
+    const int N = p->d.num_points;
+    const long long szs = (long long)p->d.num_channels * N * (N / 2 + 1);
+    if (in_bs != szs || out_bs != szs)
+      return fail(EXMI_ERR_UNSUPPORTED, "exmi_nonlinear: 2-D spectra must be contiguous");
+    Step2DArgs a;
+    fill_args2d(p, &a);
+    a.spec = (cf*)const_cast<float*>(uh_in);
+    a.spec_out = (cf*)nh_out;
+    a.spec_bstride = (long long)N * (N / 2 + 1);
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = p->col2d[COL_CA_SPEC](a, p->ninst, st);
+    if (e == hipSuccess) e = p->row2d[ROW_RB](a, p->ninst, st);
+    if (e == hipSuccess) e = p->col2d[COL_NOUT](a, p->ninst, st);
+    return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "2-D nonlinear launch failed: %s", hipGetErrorString(e));
+  }
   return run_step(p, uh_in, in_bs, 1, nh_out, out_bs, 1, 1, (hipStream_t)stream);
 }
 

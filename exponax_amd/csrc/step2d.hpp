@@ -96,6 +96,7 @@ struct Step2DArgs {
   int lin_channels;          // 1 or C: per-channel linear operators
   float feed, kill;          // Gray-Scott rates
   cf* spec;                  // standalone transforms: spectra (ninst, N, N/2+1), [kx][ky]
+  cf* spec_out;              // exmi_nonlinear: N_hat (ninst, N, N/2+1), [kx][ky]
   long long spec_bstride;    // complex entries between instances
 };
 
@@ -109,7 +110,8 @@ enum Col2DMode {
   COL_CA = 0, COL_CB = 1, COL_CC = 2, COL_C0 = 3, COL_C1 = 4,
   COL_C3A = 5, COL_C3B = 6, COL_C3C = 7, COL_C4A = 8, COL_C4B = 9, COL_C4C = 10, COL_C4D = 11,
   COL_FWD = 12, COL_INV = 13,
-  COL_NUM_MODES = 14
+  COL_CA_SPEC = 14, COL_NOUT = 15,
+  COL_NUM_MODES = 16
 };
 enum Row2DMode { ROW_RA = 0, ROW_RB = 1, ROW_RC = 2 };
 
@@ -717,6 +719,27 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
       cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
 #pragma unroll
       for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
+    }
+  } else if constexpr (MODE == COL_CA_SPEC) {
+    // CA from a given spectrum u_hat [kx][ky] (exmi_nonlinear): S0 = 2 u_hat
+    cf v[1][PT];
+    const cf* in = a.spec + (long long)inst * a.spec_bstride + ky;
+#pragma unroll
+    for (int s = 0; s < PT; ++s) v[0][s] = 2.f * in[(long long)(j + s * T) * C::NY];
+    if (active) {
+      cf* s0 = a.S0 + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) s0[j + s * T] = v[0][s];
+    }
+    fields_out(v[0]);
+  } else if constexpr (MODE == COL_NOUT) {
+    // the nonlinear term N_hat of the products in T, written as [kx][ky]
+    cf Nh[PT];
+    nonlinear(Nh);
+    if (active) {
+      cf* o = a.spec_out + (long long)inst * a.spec_bstride + ky;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) o[(long long)(j + s * T) * C::NY] = 0.5f * Nh[s];
     }
   } else if constexpr (MODE == COL_FWD) {
     // rfftn: the row spectra of column ky -> X[kx][ky] (rows gave 2 X)

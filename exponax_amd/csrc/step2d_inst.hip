@@ -40,6 +40,8 @@ static Launch2D pick2d(int is_row, int mode) {
       // the standalone transforms do not depend on the nonlinear kind
       case COL_FWD: return KIND == NL_GRADNORM ? &launch_col2d<N, COL_FWD, NL_GRADNORM> : nullptr;
       case COL_INV: return KIND == NL_GRADNORM ? &launch_col2d<N, COL_INV, NL_GRADNORM> : nullptr;
+      case COL_CA_SPEC: return &launch_col2d<N, COL_CA_SPEC, KIND>;
+      case COL_NOUT: return &launch_col2d<N, COL_NOUT, KIND>;
     }
   }
   return nullptr;

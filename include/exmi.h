@@ -169,8 +169,8 @@ int exmi_step(exmi_plan* plan, const float* u_in, int64_t in_bstride, float* u_o
 int exmi_step_fourier(exmi_plan* plan, const float* uh_in, int64_t in_bstride, float* uh_out,
                       int64_t out_bstride, void* stream);
 
-/* Dealiased nonlinear term N(u_hat) on (B, C, Nr) complex64 spectra (1-D
- * and 3-D plans; 2-D plans and coupled-reaction 1-D plans return
+/* Dealiased nonlinear term N(u_hat) on (B, C, Nr) complex64 spectra (2-D:
+ * contiguous spectra, real L; coupled-reaction 1-D plans return
  * EXMI_ERR_UNSUPPORTED).
  * Replaces BaseNonlinearFun.__call__ (nonlin_fun/_base.py:139-159) of the
  * plan's nonlinear function. */

@@ -98,3 +98,30 @@ def test_step_fourier_2d(device):
     err = np.abs(got - want).max() / np.abs(want).max()
     print(f"  max rel {err:.2e}", flush=True)
     assert err <= 1e-5
+
+
+def test_nonlinear_fun_2d(device):
+    """The nonlinear functions on 2-D spectra (nonlin_fun/*.py __call__)
+    against the oracle: two-channel convection, gradient norm, vorticity
+    convection with the Kolmogorov forcing, polynomial."""
+    import torch
+
+    from oracle import spectral as osp
+
+    print("[case] 2-D nonlinear functions", flush=True)
+    cases = [
+        (ex.stepper.Burgers(2, 1.0, 64, 1e-3), ost.Burgers(2, 1.0, 64, 1e-3, dtype=np.float32), 1.0, 2),
+        (ex.stepper.KuramotoSivashinsky(2, 30.0, 64, 0.05),
+         ost.KuramotoSivashinsky(2, 30.0, 64, 0.05, dtype=np.float32), 30.0, 1),
+        (ex.stepper.KolmogorovFlowVorticity(2, 2 * np.pi, 64, 0.01),
+         ost.KolmogorovFlowVorticity(2, 2 * np.pi, 64, 0.01, dtype=np.float32), 2 * np.pi, 1),
+        (ex.stepper.reaction.AllenCahn(2, 1.0, 64, 1e-3), ost.AllenCahn(2, 1.0, 64, 1e-3, dtype=np.float32), 1.0, 1),
+    ]
+    for gpu, cpu, L, C in cases:
+        u0, _ = ics(3, 2, L, 64, 4, seed=77, channels=C)
+        uh = osp.fft(u0, num_spatial_dims=2).astype(np.complex64)
+        got = gpu.nonlinear_fun(torch.tensor(uh, device=device)).cpu().numpy()
+        want = cpu.nonlinear_fun(uh)
+        err = np.abs(got - want).max() / np.abs(want).max()
+        print(f"  {type(gpu).__name__}: max rel {err:.2e}", flush=True)
+        assert err <= 1e-5
