@@ -1055,7 +1055,10 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
     delete p;
     return fail(EXMI_ERR_UNSUPPORTED, "no kernel for N=%d kind=%d order=%d", N, kind, d->order);
   }
-  p->vjp = d->order <= 2 && !cplx ? get_vjp1d(p->M, d->order == 0 ? NL_ZERO : kind, d->order) : nullptr;
+  // the Laplacian of Cahn-Hilliard's cubic has no transpose kernel here
+  p->vjp = d->order <= 2 && !cplx && d->nonlin_kind != EXMI_NL_CAHN_HILLIARD
+               ? get_vjp1d(p->M, d->order == 0 ? NL_ZERO : kind, d->order)
+               : nullptr;
   const int fr = (d->flags & EXMI_FLAG_FOURIER_RESIDENT) ? 1 : 0;
   if (fr && d->order != 2) {
     delete p;
@@ -1958,7 +1961,7 @@ int exmi_step_vjp(exmi_plan* p, const float* u, int64_t u_bs, const float* g, in
                   int64_t gin_bs, void* stream) {
   if (!p || !u || !g || !g_in) return fail(EXMI_ERR_INVALID, "null argument");
   if (!has_adjoint(p))
-    return fail(EXMI_ERR_UNSUPPORTED, "no adjoint kernel for this plan (1-D: ETDRK orders 0-2 with convection or "
+    return fail(EXMI_ERR_UNSUPPORTED, "no adjoint kernel for this plan (1-D: ETDRK orders 0-2 with convection, polynomial or "
                                       "gradient norm; 2-D: gradient norm)");
   int rc = ensure_adjoint_ws(p);
   if (rc) return rc;
@@ -1971,7 +1974,7 @@ int exmi_rollout_vjp(exmi_plan* p, const float* u0, int64_t u0_bs, int32_t n, in
   if (!p || !u0 || !g_u0) return fail(EXMI_ERR_INVALID, "null argument");
   if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
   if (!has_adjoint(p))
-    return fail(EXMI_ERR_UNSUPPORTED, "no adjoint kernel for this plan (1-D: ETDRK orders 0-2 with convection or "
+    return fail(EXMI_ERR_UNSUPPORTED, "no adjoint kernel for this plan (1-D: ETDRK orders 0-2 with convection, polynomial or "
                                       "gradient norm; 2-D: gradient norm)");
   hipStream_t st = (hipStream_t)stream;
   const int B = p->d.batch;

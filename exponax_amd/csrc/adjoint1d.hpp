@@ -17,6 +17,8 @@
 //     J^T mu = -b P (F[w_x h] + (-i kappa) F[w h]),  h = F^-1[P mu]
 //   gradient norm              N = -b/2 P Z F[w_x^2]       (_gradient_norm.py:84-101)
 //     J^T mu = -b P (-i kappa) F[w_x F^-1[P Z mu]]
+//   polynomial                 N = P F[p(w)]               (_polynomial.py:64-76)
+//     J^T mu = P F[p'(w) F^-1[P mu]]
 //
 // ETDRK2 (_etdrk_2.py:91-102), cotangent lam of the new state:
 //   rho = lam + J(S1)^T (c2 lam),   lam_in = E rho + J(u_hat)^T (c1 rho - c2 lam)
@@ -48,7 +50,7 @@ constexpr int lin_fields() {
 template <int KIND, int PT>
 struct Lin1D {
   static constexpr int NFL = lin_fields<KIND>() > 0 ? lin_fields<KIND>() : 1;
-  cf f[NFL][PT];  // physical fields (2N x): CONS w; NONCONS w, w_x; GRADNORM w_x
+  cf f[NFL][PT];  // physical fields (2N x): CONS w; NONCONS w, w_x; GRADNORM w_x; POLY w
 };
 
 // Fields of the linearisation point S (2 u_hat) and, with WANT_N, the
@@ -107,6 +109,32 @@ __device__ __forceinline__ void lin_eval(const cf* S, cf nyq, Lin1D<KIND, PT>& L
       const float c = -a.scale * q2;
       masked_scale<M, PT>(U[0], Nn, j, a.kmax, [&](int, cf x) { return c * x; });
       nn = ((M <= a.kmax) ? c : 0.f) * un[0];
+    }
+  } else if constexpr (KIND == NL_POLY) {
+    cf Z[1][PT];
+    cf zn[1];
+    masked_copy<M, PT>(S, nyq, Z[0], zn[0], j, a.kmax);
+    irfft<M, PT, 1>(Z, zn, ln, lds, rt);
+#pragma unroll
+    for (int s = 0; s < PT; ++s) L.f[0][s] = Z[0][s];
+    if constexpr (WANT_N) {
+      // nl_eval's power loop on u = (2 N u) / (2 N)
+      constexpr float r2n = 1.f / (2.f * NN);
+#pragma unroll
+      for (int s = 0; s < PT; ++s) {
+        const float ur = Z[0][s].re * r2n, ui = Z[0][s].im * r2n;
+        float pr = 1.f, pi = 1.f, ar = 0.f, ai = 0.f;
+        for (int i = 0; i < a.npoly; ++i) {
+          ar += a.poly[i] * pr;
+          ai += a.poly[i] * pi;
+          pr *= ur;
+          pi *= ui;
+        }
+        Z[0][s] = cf{ar, ai};
+      }
+      rfft<M, PT, 1>(Z, zn, ln, lds, rt, kc);
+      masked_scale<M, PT>(Z[0], Nn, j, a.kmax, [&](int, cf x) { return x; });
+      nn = (M <= a.kmax) ? zn[0] : cf{0.f, 0.f};
     }
   } else {
     static_assert(KIND == NL_GRADNORM, "no 1-D adjoint for this nonlinear kind");
@@ -186,6 +214,31 @@ __device__ __forceinline__ void adj_eval(const Lin1D<KIND, PT>& L, const cf* X, 
     });
     an = (M <= a.kmax) ? cf{fmaf(cs, zn[0].re, cs * kn * zn[1].im), fmaf(cs, zn[0].im, -cs * kn * zn[1].re)}
                        : cf{0.f, 0.f};
+  } else if constexpr (KIND == NL_POLY) {
+    // h = F^-1[P mu] (2N h from X = 2 mu); A = P F[p'(w) h]: the forward
+    // transform of p'(w) (2N h) gives 4N F[p' h], so A = 2 J^T mu needs 1/(2N)
+    constexpr float r2n = 1.f / (2.f * NN);
+    cf Z[1][PT];
+    cf zn[1];
+    masked_copy<M, PT>(X, xn, Z[0], zn[0], j, a.kmax);
+    irfft<M, PT, 1>(Z, zn, ln, lds, rt);
+#pragma unroll
+    for (int s = 0; s < PT; ++s) {
+      const float wr = L.f[0][s].re * r2n, wi = L.f[0][s].im * r2n;
+      float pr = 1.f, pi = 1.f, dr = 0.f, di = 0.f;
+      for (int i = 1; i < a.npoly; ++i) {
+        dr += (float)i * a.poly[i] * pr;
+        di += (float)i * a.poly[i] * pi;
+        pr *= wr;
+        pi *= wi;
+      }
+      Z[0][s] = cf{dr * Z[0][s].re, di * Z[0][s].im};
+    }
+    rfft<M, PT, 1>(Z, zn, ln, lds, rt, kc);
+    masked_scale<M, PT>(Z[0], A, j, a.kmax, [&](int, cf x) { return r2n * x; });
+    an = ((M <= a.kmax) ? r2n : 0.f) * zn[0];
+    (void)cs;
+    (void)kn;
   } else {
     static_assert(KIND == NL_GRADNORM, "no 1-D adjoint for this nonlinear kind");
     // h = F^-1[P Z mu]; A = cs P (-i kappa) F[w_x h]

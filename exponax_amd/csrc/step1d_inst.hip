@@ -272,6 +272,7 @@ VjpFn EXMI_CAT(get_vjp1d_, EXMI_M)(int kind, int order) {
     case NL_CONV_CONS: return pick_vjp_order<M, PT, NL_CONV_CONS>(order);
     case NL_CONV_NONCONS: return pick_vjp_order<M, PT, NL_CONV_NONCONS>(order);
     case NL_GRADNORM: return pick_vjp_order<M, PT, NL_GRADNORM>(order);
+    case NL_POLY: return pick_vjp_order<M, PT, NL_POLY>(order);
     default: return nullptr;
   }
 }

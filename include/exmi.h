@@ -212,7 +212,8 @@ int exmi_rollout_group(exmi_plan* const* plans, int32_t n_plans, const float* co
 int exmi_repeat(exmi_plan* plan, const float* u0, int64_t u0_bstride, float* u_out, int32_t n_steps,
                 int32_t use_graph, void* stream);
 
-/* Adjoint of one step: g_in = (d step(u) / d u)^T g_out at u = u_in, all
+/* Adjoint of one step (1-D: also the polynomial reactions): g_in =
+ * (d step(u) / d u)^T g_out at u = u_in, all
  * physical (B, C, N^D) float32 with batch strides in floats; g_in may equal
  * g_out (not u_in). Inner products are the plain sums over grid points, as
  * jax.grad of a loss sum(step(u) * g_out) would give.

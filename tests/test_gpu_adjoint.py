@@ -52,6 +52,10 @@ CASES_1D = [
      lambda o, dt: ost.KuramotoSivashinsky(1, 60.0, 128, dt, order=o), 60.0, 128),
     ("burgers", lambda o, dt: ex.stepper.Burgers(1, 1.0, 1024, dt * 1e-2, diffusivity=3e-3, order=o),
      lambda o, dt: ost.Burgers(1, 1.0, 1024, dt * 1e-2, diffusivity=3e-3, order=o), 1.0, 1024),
+    ("allen_cahn", lambda o, dt: ex.stepper.reaction.AllenCahn(1, 1.0, 256, dt * 1e-2, order=o),
+     lambda o, dt: ost.AllenCahn(1, 1.0, 256, dt * 1e-2, order=o), 1.0, 256),
+    ("swift_hohenberg", lambda o, dt: ex.stepper.reaction.SwiftHohenberg(1, 20.0, 128, dt, order=o),
+     lambda o, dt: ost.SwiftHohenberg(1, 20.0, 128, dt, order=o), 20.0, 128),
 ]
 
 
