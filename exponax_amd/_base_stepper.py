@@ -235,6 +235,16 @@ class BaseStepper:
         return self.plan(batch).tables()
 
 
+def _nonlinear_signature(stepper):
+    """The nonlinear function's parameters as the plan carries them (one set
+    per plan: the kernels take per-trajectory linear operators only)."""
+    d = _lib.PlanDesc()
+    stepper.nonlinear_fun.fill(d)
+    return (type(stepper.nonlinear_fun), d.nonlin_kind, d.dealiasing_fraction, d.scale, d.conv_single_channel,
+            d.conv_conservative, d.zero_mode_fix, d.num_poly, tuple(d.poly), tuple(d.general_scales),
+            d.injection_mode, d.injection_scale, tuple(d.reaction_rates))
+
+
 class Ensemble(BaseStepper):
     """A batch of steppers that differ only in their linear operator
     (per-trajectory coefficients), run in one launch per step — the
@@ -253,6 +263,9 @@ class Ensemble(BaseStepper):
                     raise ValueError(f"ensemble members differ in {attr}")
             if type(m.nonlinear_fun) is not type(m0.nonlinear_fun):
                 raise ValueError("ensemble members differ in their nonlinear function")
+            if _nonlinear_signature(m) != _nonlinear_signature(m0):
+                raise NotImplementedError("ensemble members differ in their nonlinear function's parameters; "
+                                          "per-trajectory coefficients are built for the linear operator only")
         self.__dict__.update({k: v for k, v in m0.__dict__.items() if k != "_plans"})
         self.members = members
         self.batch = len(members)

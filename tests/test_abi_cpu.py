@@ -187,3 +187,19 @@ def test_generic_parametrisations():
         g.GeneralVorticityConvectionStepper(1, 1.0, 32, 0.01)
     with pytest.raises(ValueError):
         g.GeneralNonlinearStepper(1, 1.0, 32, 0.1, nonlinear_coefficients=(0.0, 1.0))
+
+
+def test_ensemble_rejects_differing_nonlinear_parameters():
+    """Per-trajectory coefficients are built for the linear operator; members
+    that differ in a nonlinear parameter fail loudly instead of silently
+    taking the first member's."""
+    import pytest
+
+    import exponax_amd as ex
+
+    ok = ex.ensemble(lambda nu: ex.stepper.Burgers(1, 1.0, 64, 1e-3, diffusivity=nu), [0.01, 0.02])
+    assert ok.batch == 2
+    with pytest.raises(NotImplementedError):
+        ex.ensemble(lambda b: ex.stepper.Burgers(1, 1.0, 64, 1e-3, convection_scale=b), [1.0, 0.5])
+    with pytest.raises(NotImplementedError):
+        ex.ensemble(lambda f: ex.stepper.reaction.GrayScott(1, 1.0, 64, 0.5, feed_rate=f), [0.04, 0.05])

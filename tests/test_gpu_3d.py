@@ -212,3 +212,26 @@ def test_step_fourier_nonlinear_and_repeated_3d(device):
     e_rep = max_rel_l2(rep, want_r)
     print(f"  step_fourier {e_sf:.2e}, nonlinear {e_nl:.2e}, repeated {e_rep:.2e}", flush=True)
     assert e_sf <= 1e-5 and e_nl <= 1e-5 and e_rep <= RTOL_STEP
+
+
+def test_ensembles_3d_and_coupled_2d(device):
+    """Per-trajectory linear operators with several channels: 3-D Burgers
+    (three channels, shared L) and 2-D Gray-Scott (per-channel L) ensembles
+    against the oracle's members."""
+    import torch
+
+    print("[case] 3-D Burgers / 2-D Gray-Scott ensembles", flush=True)
+    nus = [0.005, 0.02]
+    gpu = ex.ensemble(lambda nu: ex.stepper.Burgers(3, 1.0, 16, 1e-3, diffusivity=nu), nus)
+    cpu = ost.Ensemble([ost.Burgers(3, 1.0, 16, 1e-3, diffusivity=nu, dtype=np.float32) for nu in nus],
+                       dtype=np.float32)
+    u0, _ = ics(2, 3, 1.0, 16, 3, seed=181, channels=3)
+    _check(gpu, cpu, u0, device, "burgers3d_ensemble")
+    d1 = [1e-3, 4e-3]
+    gpu = ex.ensemble(lambda d: ex.stepper.reaction.GrayScott(2, 1.0, 64, 0.5, diffusivity_1=d, diffusivity_2=d / 2),
+                      d1)
+    cpu = ost.Ensemble([ost.GrayScott(2, 1.0, 64, 0.5, diffusivity_1=d, diffusivity_2=d / 2, dtype=np.float32)
+                        for d in d1], dtype=np.float32)
+    u0, _ = ics(2, 2, 1.0, 64, 4, seed=182, channels=2)
+    u0 = (np.array([0.7, 0.2], dtype=np.float32).reshape(1, 2, 1, 1) + 0.2 * u0 / np.abs(u0).max()).astype(np.float32)
+    _check(gpu, cpu, u0, device, "gray_scott_ensemble")
