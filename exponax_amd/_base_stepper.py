@@ -86,6 +86,12 @@ class BaseStepper:
         d.flags = ((_lib.EXMI_FLAG_EXACT_FORM if getattr(self, "exact_form", False) else 0)
                    | (_lib.EXMI_FLAG_FOURIER_RESIDENT if getattr(self, "fourier_resident", False) else 0))
         if self.order > 0:
+            from .nonlin_fun import BaseNonlinearFun
+
+            if not isinstance(self.nonlinear_fun, BaseNonlinearFun):
+                raise NotImplementedError(
+                    f"{type(self.nonlinear_fun).__name__}: only the library's nonlinear functions "
+                    "(exponax_amd.nonlin_fun and the steppers' own) run in the fused kernels")
             self.nonlinear_fun.fill(d)
         return d
 
