@@ -6,10 +6,11 @@ The reference transforms once, repeats ``step_fourier`` and transforms back
 - 1-D ETDRK2 steppers sub-step with the Fourier-resident repeat kernel: one
   launch keeps each trajectory's spectrum on chip for all sub-steps, the same
   transform-once sequence;
-- other 1-D orders and 3-D steppers transform once and launch
-  ``step_fourier`` per sub-step;
-- 2-D steppers (the library steps 2-D states physically) repeat the physical
-  step, which differs from the spectral sub-stepping by rounding only.
+- other 1-D plans (orders other than 2, complex linear operators) and 3-D
+  steppers transform once and launch ``step_fourier`` per sub-step;
+- 2-D steppers and 1-D channel-coupled reactions (stepped physically)
+  repeat the physical step, which differs from the spectral sub-stepping by
+  rounding only.
 """
 from __future__ import annotations
 
@@ -33,10 +34,23 @@ class RepeatedStepper:
         self._sub = self._spectral_twin(stepper)
 
     @staticmethod
+    def _fourier_resident_ok(st):
+        """The Fourier-resident kernels are built for 1-D ETDRK2 plans with a
+        real linear operator and an uncoupled nonlinearity."""
+        import numpy as np
+
+        from . import _lib
+
+        nl = getattr(st, "nonlinear_fun", None)
+        coupled = getattr(nl, "kind", None) in (_lib.EXMI_NL_GRAY_SCOTT, _lib.EXMI_NL_BELOUSOV_ZHABOTINSKY)
+        return (st.num_spatial_dims == 1 and st.order == 2 and st.num_points <= 4096 and not coupled
+                and not np.iscomplexobj(getattr(st, "linear_operator", np.zeros(1))))
+
+    @staticmethod
     def _spectral_twin(st):
         """A copy of `st` whose plans keep the spectral state on chip between
-        sub-steps (1-D ETDRK2), else `st` itself."""
-        if st.num_spatial_dims == 1 and st.order == 2:
+        sub-steps (1-D ETDRK2, real L), else `st` itself."""
+        if RepeatedStepper._fourier_resident_ok(st):
             if getattr(st, "fourier_resident", False):
                 return st
             twin = copy.copy(st)
@@ -68,7 +82,11 @@ class RepeatedStepper:
             t, back = as_device_tensor(u)
             return restore_like(t.clone(), back)
         D = self.num_spatial_dims
-        if (D == 1 and self.stepper.order != 2) or D == 3:
+        nl = getattr(self.stepper, "nonlinear_fun", None)
+        from . import _lib
+
+        coupled = getattr(nl, "kind", None) in (_lib.EXMI_NL_GRAY_SCOTT, _lib.EXMI_NL_BELOUSOV_ZHABOTINSKY)
+        if (D == 1 and not coupled and not self._fourier_resident_ok(self.stepper)) or D == 3:
             from .spectral import fft, ifft
 
             t, back = as_device_tensor(u)

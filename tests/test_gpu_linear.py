@@ -217,3 +217,32 @@ def test_forced_stepper_parity(device):
         err = max_rel_l2(got.cpu().numpy(), want)
         print(f"  D={D}: rel-L2 {err:.2e}", flush=True)
         assert err <= RTOL_STEP
+
+
+def test_repeated_stepper_complex_and_coupled(device):
+    """RepeatedStepper around a complex-L stepper (KdV: Fourier-space
+    sub-steps) and a 1-D coupled reaction (Gray-Scott: physical sub-steps)
+    against the oracle's repeated steps."""
+    import torch
+
+    print("[case] RepeatedStepper KdV / Gray-Scott", flush=True)
+    u0, _ = ics(2, 1, 20.0, 128, 4, seed=191)
+    got = ex.RepeatedStepper(ex.stepper.KortewegDeVries(1, 20.0, 128, 1e-3), 5).step(
+        torch.tensor(u0, device=device)).cpu().numpy()
+    cpu = ost.KortewegDeVries(1, 20.0, 128, 1e-3, dtype=np.float32)
+    want = u0
+    for _ in range(5):
+        want = cpu.step_batch(want)
+    e1 = max_rel_l2(got, want)
+    g0, _ = ics(2, 1, 1.0, 64, 4, seed=192, channels=2)
+    g0 = (np.array([0.7, 0.2], dtype=np.float32).reshape(1, 2, 1) + 0.2 * g0 / np.abs(g0).max()).astype(np.float32)
+    kw = dict(diffusivity_1=2e-3, diffusivity_2=1e-3)
+    got = ex.RepeatedStepper(ex.stepper.reaction.GrayScott(1, 1.0, 64, 0.5, **kw), 4).step(
+        torch.tensor(g0, device=device)).cpu().numpy()
+    cpu = ost.GrayScott(1, 1.0, 64, 0.5, dtype=np.float32, **kw)
+    want = g0
+    for _ in range(4):
+        want = cpu.step_batch(want)
+    e2 = max_rel_l2(got, want)
+    print(f"  KdV {e1:.2e}, Gray-Scott {e2:.2e}", flush=True)
+    assert e1 <= RTOL_STEP and e2 <= RTOL_STEP
