@@ -35,21 +35,22 @@ class RepeatedStepper:
 
     @staticmethod
     def _fourier_resident_ok(st):
-        """The Fourier-resident kernels are built for 1-D ETDRK2 plans with a
-        real linear operator and an uncoupled nonlinearity."""
+        """The Fourier-resident kernels are built for 1-D ETDRK2 plans with an
+        uncoupled nonlinearity (complex linear operators: N <= 1024)."""
         import numpy as np
 
         from . import _lib
 
         nl = getattr(st, "nonlinear_fun", None)
         coupled = getattr(nl, "kind", None) in (_lib.EXMI_NL_GRAY_SCOTT, _lib.EXMI_NL_BELOUSOV_ZHABOTINSKY)
-        return (st.num_spatial_dims == 1 and st.order == 2 and st.num_points <= 4096 and not coupled
-                and not np.iscomplexobj(getattr(st, "linear_operator", np.zeros(1))))
+        cplx = np.iscomplexobj(getattr(st, "linear_operator", np.zeros(1)))
+        return (st.num_spatial_dims == 1 and st.order == 2 and not coupled
+                and st.num_points <= (1024 if cplx else 4096))
 
     @staticmethod
     def _spectral_twin(st):
         """A copy of `st` whose plans keep the spectral state on chip between
-        sub-steps (1-D ETDRK2, real L), else `st` itself."""
+        sub-steps (1-D ETDRK2, uncoupled), else `st` itself."""
         if RepeatedStepper._fourier_resident_ok(st):
             if getattr(st, "fourier_resident", False):
                 return st

@@ -46,7 +46,7 @@ hipError_t launch_stage3d(const Stage3DArgs& a, hipStream_t st);
   StepFn get_step1d_##M(int kind, int order);                                                          \
   RollFn get_roll1d_##M(int kind, int order, int nt, int fr, int std_kc);                              \
   StepFn get_step1d_c_##M(int kind, int order);                                                        \
-  RollFn get_roll1d_c_##M(int kind, int order);                                                        \
+  RollFn get_roll1d_c_##M(int kind, int order, int fr);                                                \
   RollFn get_roll1d_r_##M(int kind, int order);                                                        \
   VjpFn get_vjp1d_##M(int kind, int order);                                                            \
   hipError_t launch_fft1d_##M(bool inverse, const float* in, float* out, int count, const RealTables& rt, \
@@ -143,10 +143,10 @@ static StepFn get_step1d_c(int M, int kind, int order) {
     default: return nullptr;
   }
 }
-static RollFn get_roll1d_c(int M, int kind, int order) {
+static RollFn get_roll1d_c(int M, int kind, int order, int fr) {
   switch (M) {
 #define EXMI_CASE(MM) \
-  case MM: return get_roll1d_c_##MM(kind, order);
+  case MM: return get_roll1d_c_##MM(kind, order, fr);
     EXMI_CASE(8) EXMI_CASE(16) EXMI_CASE(24) EXMI_CASE(25) EXMI_CASE(32) EXMI_CASE(64) EXMI_CASE(128)
     EXMI_CASE(256) EXMI_CASE(512)
 #undef EXMI_CASE
@@ -1012,9 +1012,8 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   if (!d->host_linear_operator) return fail(EXMI_ERR_INVALID, "host_linear_operator is null");
   if (d->order > 0 && d->num_circle_points < 1) return fail(EXMI_ERR_INVALID, "num_circle_points must be >= 1");
   const bool cplx = d->host_linear_operator_imag != nullptr;
-  if (cplx && (N > 1024 || (d->flags & EXMI_FLAG_FOURIER_RESIDENT)))
-    return fail(EXMI_ERR_UNSUPPORTED,
-                "complex linear operators are built for N <= 1024, without the Fourier-resident flag");
+  if (cplx && N > 1024)
+    return fail(EXMI_ERR_UNSUPPORTED, "complex linear operators are built for N <= 1024");
   int kind = NL_ZERO;
   if (d->order > 0) {
     int rc = kernel_kind(d, &kind);
@@ -1067,7 +1066,7 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   p->fourier_resident = fr;
   const int std_kc = p->kmax == default_kmax(p->M) ? 1 : 0;
   p->roll[1] = coupled ? get_roll1d_r(p->M, kind, d->order)
-              : cplx  ? get_roll1d_c(p->M, kind, d->order)
+              : cplx  ? get_roll1d_c(p->M, kind, d->order, fr)
                       : get_roll1d(p->M, kind, d->order, 1, fr, std_kc);
   p->roll[2] = (cplx || coupled) ? nullptr : get_roll1d(p->M, kind, d->order, 2, fr, std_kc);
   if (const char* e = getenv("EXMI_ROLL_NT")) p->roll_nt = atoi(e);  // 1, 2; -1 = per-step launches

@@ -234,9 +234,27 @@ RollFn EXMI_CAT(get_roll1d_r_, EXMI_M)(int kind, int order) {
   return nullptr;
 }
 
-RollFn EXMI_CAT(get_roll1d_c_, EXMI_M)(int kind, int order) {
+// complex L; `fr`: the Fourier-resident ETDRK2 rollout
+template <int M, int PT, int KIND>
+static RollFn pick_roll_cf() {
+  return &launch_roll1d<M, PT, KIND, 2, 1, true, -1, EXMI_LAYOUT_1D, true>;
+}
+
+RollFn EXMI_CAT(get_roll1d_c_, EXMI_M)(int kind, int order, int fr) {
   constexpr int M = EXMI_M, PT = EXMI_PT;
   if constexpr (M <= 512) {
+    if (fr) {
+      if (order != 2) return nullptr;
+      switch (kind) {
+        case NL_ZERO: return pick_roll_cf<M, PT, NL_ZERO>();
+        case NL_CONV_CONS: return pick_roll_cf<M, PT, NL_CONV_CONS>();
+        case NL_CONV_NONCONS: return pick_roll_cf<M, PT, NL_CONV_NONCONS>();
+        case NL_GRADNORM: return pick_roll_cf<M, PT, NL_GRADNORM>();
+        case NL_POLY: return pick_roll_cf<M, PT, NL_POLY>();
+        case NL_GENERAL: return pick_roll_cf<M, PT, NL_GENERAL>();
+        default: return nullptr;
+      }
+    }
     if (order == 0) return &launch_roll1d<M, PT, NL_ZERO, 0, 1, false, -1, EXMI_LAYOUT_1D, true>;
     switch (kind) {
       case NL_ZERO: return pick_roll_c<M, PT, NL_ZERO>(order);

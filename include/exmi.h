@@ -127,7 +127,7 @@ typedef struct exmi_plan_desc {
   /* NULL for a real L; else the imaginary parts of L, same shape. A complex L
    * (odd derivatives: advection, dispersion; stepper/_advection.py:88-95,
    * _dispersion.py:102-117, _korteweg_de_vries.py) is built for ETDRK orders
-   * 0-4: 1-D plans with N <= 1024 and without EXMI_FLAG_FOURIER_RESIDENT, 2-D
+   * 0-4: 1-D plans with N <= 1024 (EXMI_FLAG_FOURIER_RESIDENT: ETDRK2), 2-D
    * and 3-D plans. exp(L dt) (and exp(L dt / 2) for orders 3-4)
    * is then complex while the phi-coefficients stay real: the reference keeps
    * the real part of every contour sample (etdrk/_etdrk_1.py:66-76,

@@ -220,7 +220,7 @@ def test_forced_stepper_parity(device):
 
 
 def test_repeated_stepper_complex_and_coupled(device):
-    """RepeatedStepper around a complex-L stepper (KdV: Fourier-space
+    """RepeatedStepper around a complex-L stepper (KdV: Fourier-resident
     sub-steps) and a 1-D coupled reaction (Gray-Scott: physical sub-steps)
     against the oracle's repeated steps."""
     import torch
@@ -246,3 +246,24 @@ def test_repeated_stepper_complex_and_coupled(device):
     e2 = max_rel_l2(got, want)
     print(f"  KdV {e1:.2e}, Gray-Scott {e2:.2e}", flush=True)
     assert e1 <= RTOL_STEP and e2 <= RTOL_STEP
+
+
+def test_fourier_resident_complex(device):
+    """The Fourier-resident ETDRK2 rollout with a complex L (KdV) against the
+    oracle's physical steps (the Im of the Nyquist mode it keeps never
+    reaches the physical states)."""
+    import torch
+
+    print("[case] Fourier-resident KdV", flush=True)
+    gpu = ex.stepper.KortewegDeVries(1, 20.0, 256, 1e-3)
+    gpu.fourier_resident = True
+    cpu = ost.KortewegDeVries(1, 20.0, 256, 1e-3, dtype=np.float32)
+    u0, _ = ics(3, 1, 20.0, 256, 4, seed=193)
+    traj = gpu.rollout_batch(torch.tensor(u0, device=device), 10).cpu().numpy()
+    uc = u0
+    for s in range(1, 11):
+        uc = cpu.step_batch(uc)
+        if s in (1, 10):
+            e = max_rel_l2(traj[:, s - 1], uc)
+            print(f"  step {s}: {e:.2e}", flush=True)
+            assert e <= RTOL_STEP
