@@ -1233,7 +1233,7 @@ int exmi_nonlinear(exmi_plan* p, const float* uh_in, int64_t in_bs, float* nh_ou
   if (p->ndim == 3) return enqueue_step3d(p, uh_in, in_bs, nh_out, out_bs, (hipStream_t)stream, true, true, true);
   if (p->ndim == 2) {
     // the nonlinear evaluation of the 2-D step chain, from and to spectra
-    if (p->cplx) return fail(EXMI_ERR_UNSUPPORTED, "exmi_nonlinear: 2-D plans with a complex L");
+    // (no exponential tables are read, so real and complex L plans share it)
     const int N = p->d.num_points;
     const long long szs = (long long)p->d.num_channels * N * (N / 2 + 1);
     if (in_bs != szs || out_bs != szs)

@@ -103,7 +103,8 @@ def test_step_fourier_2d(device):
 def test_nonlinear_fun_2d(device):
     """The nonlinear functions on 2-D spectra (nonlin_fun/*.py __call__)
     against the oracle: two-channel convection, gradient norm, vorticity
-    convection with the Kolmogorov forcing, polynomial."""
+    convection with the Kolmogorov forcing, polynomial, and the two-channel
+    convection of a complex-L plan (KdV)."""
     import torch
 
     from oracle import spectral as osp
@@ -116,6 +117,8 @@ def test_nonlinear_fun_2d(device):
         (ex.stepper.KolmogorovFlowVorticity(2, 2 * np.pi, 64, 0.01),
          ost.KolmogorovFlowVorticity(2, 2 * np.pi, 64, 0.01, dtype=np.float32), 2 * np.pi, 1),
         (ex.stepper.reaction.AllenCahn(2, 1.0, 64, 1e-3), ost.AllenCahn(2, 1.0, 64, 1e-3, dtype=np.float32), 1.0, 1),
+        (ex.stepper.KortewegDeVries(2, 20.0, 64, 1e-3),
+         ost.KortewegDeVries(2, 20.0, 64, 1e-3, dtype=np.float32), 20.0, 2),
     ]
     for gpu, cpu, L, C in cases:
         u0, _ = ics(3, 2, L, 64, 4, seed=77, channels=C)
