@@ -48,7 +48,7 @@ hipError_t launch_stage3d(const Stage3DArgs& a, hipStream_t st);
   StepFn get_step1d_c_##M(int kind, int order);                                                        \
   RollFn get_roll1d_c_##M(int kind, int order, int fr);                                                \
   RollFn get_roll1d_r_##M(int kind, int order);                                                        \
-  VjpFn get_vjp1d_##M(int kind, int order);                                                            \
+  VjpFn get_vjp1d_##M(int kind, int order, int cplx);                                                            \
   hipError_t launch_fft1d_##M(bool inverse, const float* in, float* out, int count, const RealTables& rt, \
                               hipStream_t st);
 EXMI_DECL_SIZE(8)
@@ -115,19 +115,19 @@ static StepFn get_step1d(int M, int kind, int order) {
   }
 }
 
-static VjpFn get_vjp1d(int M, int kind, int order) {
+static VjpFn get_vjp1d(int M, int kind, int order, int cplx) {
   switch (M) {
-    case 8: return get_vjp1d_8(kind, order);
-    case 24: return get_vjp1d_24(kind, order);
-    case 25: return get_vjp1d_25(kind, order);
-    case 16: return get_vjp1d_16(kind, order);
-    case 32: return get_vjp1d_32(kind, order);
-    case 64: return get_vjp1d_64(kind, order);
-    case 128: return get_vjp1d_128(kind, order);
-    case 256: return get_vjp1d_256(kind, order);
-    case 512: return get_vjp1d_512(kind, order);
-    case 1024: return get_vjp1d_1024(kind, order);
-    case 2048: return get_vjp1d_2048(kind, order);
+    case 8: return get_vjp1d_8(kind, order, cplx);
+    case 24: return get_vjp1d_24(kind, order, cplx);
+    case 25: return get_vjp1d_25(kind, order, cplx);
+    case 16: return get_vjp1d_16(kind, order, cplx);
+    case 32: return get_vjp1d_32(kind, order, cplx);
+    case 64: return get_vjp1d_64(kind, order, cplx);
+    case 128: return get_vjp1d_128(kind, order, cplx);
+    case 256: return get_vjp1d_256(kind, order, cplx);
+    case 512: return get_vjp1d_512(kind, order, cplx);
+    case 1024: return get_vjp1d_1024(kind, order, cplx);
+    case 2048: return get_vjp1d_2048(kind, order, cplx);
     default: return nullptr;
   }
 }
@@ -1055,8 +1055,8 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
     return fail(EXMI_ERR_UNSUPPORTED, "no kernel for N=%d kind=%d order=%d", N, kind, d->order);
   }
   // the Laplacian of Cahn-Hilliard's cubic has no transpose kernel here
-  p->vjp = d->order <= 2 && !cplx && d->nonlin_kind != EXMI_NL_CAHN_HILLIARD
-               ? get_vjp1d(p->M, d->order == 0 ? NL_ZERO : kind, d->order)
+  p->vjp = d->order <= 2 && d->nonlin_kind != EXMI_NL_CAHN_HILLIARD
+               ? get_vjp1d(p->M, d->order == 0 ? NL_ZERO : kind, d->order, cplx ? 1 : 0)
                : nullptr;
   const int fr = (d->flags & EXMI_FLAG_FOURIER_RESIDENT) ? 1 : 0;
   if (fr && d->order != 2) {

@@ -6,6 +6,9 @@
 // spectra. Under them:
 // - the forward and inverse real transforms are mutual adjoints;
 // - a real per-mode factor (E, c1, c2, the dealias mask P) is its own adjoint;
+// - a complex E (odd-derivative L) has adjoint conj(E): E is Hermitian in k
+//   (E(-k) = conj E(k)), and at the Nyquist mode, whose imaginary part the
+//   inverse transform drops, both E and conj(E) act as Re E;
 // - i kappa has adjoint -i kappa;
 // - the mean removal is self-adjoint.
 //
@@ -261,14 +264,15 @@ __device__ __forceinline__ void adj_eval(const Lin1D<KIND, PT>& L, const cf* X, 
   }
 }
 
-template <int M, int PT, int KIND, int ORDER>
+template <int M, int PT, int KIND, int ORDER, bool CPLX = false>
 __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) vjp1d_kernel(Vjp1DArgs va) {
   const Step1DArgs& a = va.s;
   constexpr int T = M / PT;
   constexpr int G = Slots<M, PT, 1>::G;
   constexpr int LDSN = LdsSize<M>::value;
   constexpr int NR = M + 1;
-  constexpr int NCO = ncoef_of(ORDER);
+  constexpr int NRE = ncoef_of(ORDER);
+  constexpr int NCO = NRE + (CPLX ? 1 : 0);  // Im E follows the real coefficients
   using CL = CoefLds<M, PT, NCO>;
   constexpr float NN = 2.f * M;
   constexpr float r2n = 1.f / (2.f * NN);
@@ -282,6 +286,25 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) vjp1d_kernel(Vj
   const int t = active ? t_raw : 0;
   cf* lds = lds_all + g * LDSN;
   const CL co{lds_coef + g * CL::FLOATS, j};
+  // E x (the forward stage) and conj(E) x (the transpose)
+  auto emul = [&](int s, cf x) -> cf {
+    const float er = co(0, 0, s);
+    if constexpr (CPLX) {
+      const float ei = co(0, NRE, s);
+      return cf{fmaf(er, x.re, -ei * x.im), fmaf(er, x.im, ei * x.re)};
+    } else {
+      return er * x;
+    }
+  };
+  auto etmul = [&](int s, cf x) -> cf {
+    const float er = co(0, 0, s);
+    if constexpr (CPLX) {
+      const float ei = co(0, NRE, s);
+      return cf{fmaf(er, x.re, ei * x.im), fmaf(er, x.im, -ei * x.re)};
+    } else {
+      return er * x;
+    }
+  };
   const int b = t / a.channels, c = t % a.channels;
   const Lane<M, PT> ln(j);
   TwRegs<M, PT> rt;
@@ -319,8 +342,8 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) vjp1d_kernel(Vj
   cf on[1];
   if constexpr (ORDER == 0) {
 #pragma unroll
-    for (int s = 0; s < PT; ++s) O[0][s] = co(0, 0, s) * Lm[0][s];
-    on[0] = co(0, 0, PT) * lmn[0];
+    for (int s = 0; s < PT; ++s) O[0][s] = etmul(s, Lm[0][s]);
+    on[0] = etmul(PT, lmn[0]);
   } else if constexpr (ORDER == 1) {
     Lin1D<KIND, PT> L0;
     cf dum[PT], dn;
@@ -331,8 +354,8 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) vjp1d_kernel(Vj
     const cf xn = co(0, 1, PT) * lmn[0];
     adj_eval<KIND, M, PT>(L0, X, xn, A, an, ln, lds, rt, a);
 #pragma unroll
-    for (int s = 0; s < PT; ++s) O[0][s] = co(0, 0, s) * Lm[0][s] + A[s];
-    on[0] = co(0, 0, PT) * lmn[0] + an;
+    for (int s = 0; s < PT; ++s) O[0][s] = etmul(s, Lm[0][s]) + A[s];
+    on[0] = etmul(PT, lmn[0]) + an;
   } else {
     static_assert(ORDER == 2, "1-D adjoint implements ETDRK orders 0-2");
     Lin1D<KIND, PT> L0, L1;
@@ -340,8 +363,8 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) vjp1d_kernel(Vj
     lin_eval<KIND, true, M, PT>(U[0], un[0], L0, N0, n0, ln, lds, rt, a);
     cf S1[PT];
 #pragma unroll
-    for (int s = 0; s < PT; ++s) S1[s] = co(0, 0, s) * U[0][s] + co(0, 1, s) * N0[s];
-    const cf s1n = co(0, 0, PT) * un[0] + co(0, 1, PT) * n0;
+    for (int s = 0; s < PT; ++s) S1[s] = emul(s, U[0][s]) + co(0, 1, s) * N0[s];
+    const cf s1n = emul(PT, un[0]) + co(0, 1, PT) * n0;
     lin_eval<KIND, false, M, PT>(S1, s1n, L1, N0, n0, ln, lds, rt, a);
     // rho = lam + J(S1)^T (c2 lam)
     cf X[PT], A[PT], an;
@@ -359,8 +382,8 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) vjp1d_kernel(Vj
     xn = co(0, 1, PT) * rn - co(0, 2, PT) * lmn[0];
     adj_eval<KIND, M, PT>(L0, X, xn, A, an, ln, lds, rt, a);
 #pragma unroll
-    for (int s = 0; s < PT; ++s) O[0][s] = co(0, 0, s) * R[s] + A[s];
-    on[0] = co(0, 0, PT) * rn + an;
+    for (int s = 0; s < PT; ++s) O[0][s] = etmul(s, R[s]) + A[s];
+    on[0] = etmul(PT, rn) + an;
   }
   irfft<M, PT, 1>(O, on, ln, lds, rt);
   if (active) {
@@ -370,12 +393,12 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) vjp1d_kernel(Vj
   }
 }
 
-template <int M, int PT, int KIND, int ORDER>
+template <int M, int PT, int KIND, int ORDER, bool CPLX = false>
 inline hipError_t launch_vjp1d(const Vjp1DArgs& va, hipStream_t st) {
   constexpr int WG = Slots<M, PT, 1>::WG;
   constexpr int G = Slots<M, PT, 1>::G;
   const int grid = (va.s.ntrans + G - 1) / G;
-  hipLaunchKernelGGL((vjp1d_kernel<M, PT, KIND, ORDER>), dim3(grid), dim3(WG), 0, st, va);
+  hipLaunchKernelGGL((vjp1d_kernel<M, PT, KIND, ORDER, CPLX>), dim3(grid), dim3(WG), 0, st, va);
   return hipGetLastError();
 }
 

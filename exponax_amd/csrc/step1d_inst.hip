@@ -271,19 +271,31 @@ RollFn EXMI_CAT(get_roll1d_c_, EXMI_M)(int kind, int order, int fr) {
 
 using VjpFn = hipError_t (*)(const Vjp1DArgs&, hipStream_t);
 
-template <int M, int PT, int KIND>
+template <int M, int PT, int KIND, bool CPLX = false>
 static VjpFn pick_vjp_order(int order) {
   switch (order) {
-    case 1: return &launch_vjp1d<M, PT, KIND, 1>;
-    case 2: return &launch_vjp1d<M, PT, KIND, 2>;
+    case 1: return &launch_vjp1d<M, PT, KIND, 1, CPLX>;
+    case 2: return &launch_vjp1d<M, PT, KIND, 2, CPLX>;
     default: return nullptr;
   }
 }
 
-// Adjoint step kernels: ETDRK orders 0-2, convection (both forms) and
-// gradient norm.
-VjpFn EXMI_CAT(get_vjp1d_, EXMI_M)(int kind, int order) {
+// Adjoint step kernels: ETDRK orders 0-2, convection (both forms), gradient
+// norm and polynomials; with a complex L (N <= 1024): linear and convection.
+VjpFn EXMI_CAT(get_vjp1d_, EXMI_M)(int kind, int order, int cplx) {
   constexpr int M = EXMI_M, PT = EXMI_PT;
+  if (cplx) {
+    if constexpr (M <= 512) {
+      if (order == 0) return &launch_vjp1d<M, PT, NL_ZERO, 0, true>;
+      switch (kind) {
+        case NL_ZERO: return pick_vjp_order<M, PT, NL_ZERO, true>(order);
+        case NL_CONV_CONS: return pick_vjp_order<M, PT, NL_CONV_CONS, true>(order);
+        case NL_CONV_NONCONS: return pick_vjp_order<M, PT, NL_CONV_NONCONS, true>(order);
+        default: return nullptr;
+      }
+    }
+    return nullptr;
+  }
   if (order == 0) return &launch_vjp1d<M, PT, NL_ZERO, 0>;
   switch (kind) {
     case NL_ZERO: return pick_vjp_order<M, PT, NL_ZERO>(order);

@@ -56,7 +56,23 @@ CASES_1D = [
      lambda o, dt: ost.AllenCahn(1, 1.0, 256, dt * 1e-2, order=o), 1.0, 256),
     ("swift_hohenberg", lambda o, dt: ex.stepper.reaction.SwiftHohenberg(1, 20.0, 128, dt, order=o),
      lambda o, dt: ost.SwiftHohenberg(1, 20.0, 128, dt, order=o), 20.0, 128),
+    # complex L: the transpose multiplies by conj(exp(L dt))
+    ("kdv", lambda o, dt: ex.stepper.KortewegDeVries(1, 20.0, 256, dt * 1e-2, order=o),
+     lambda o, dt: ost.KortewegDeVries(1, 20.0, 256, dt * 1e-2, order=o), 20.0, 256),
 ]
+
+
+def test_step_vjp_linear_complex(device):
+    """Dispersion and advection (complex L, ETDRK0) against the oracle's VJP."""
+    print("[case] vjp1d dispersion / advection", flush=True)
+    for gpu, cpu in ((ex.stepper.Dispersion(1, 1.0, 512, 1e-3, dispersivity=0.01),
+                      ost.Dispersion(1, 1.0, 512, 1e-3, dispersivity=0.01)),
+                     (ex.stepper.Advection(1, 1.0, 128, 0.01, velocity=0.3),
+                      ost.Advection(1, 1.0, 128, 0.01, velocity=0.3))):
+        N = gpu.num_points
+        u, _ = ics(4, 1, 1.0, N, 6, seed=23)
+        g, _ = ics(4, 1, 1.0, N, 12, seed=24)
+        _vjp_case(gpu, cpu, u, g)
 
 
 @pytest.mark.parametrize("order", [0, 1, 2])
@@ -125,15 +141,21 @@ def test_step_vjp_ks2d_ensemble_no_zero_mode_fix(device):
     assert _rel(out, oad.step_vjp(ck, u.astype(np.float64), g.astype(np.float64))) <= RTOL_VJP
 
 
-@pytest.mark.parametrize("dims", [1, 2])
+@pytest.mark.parametrize("dims", [1, 2, "kdv"])
 def test_rollout_vjp_checkpointing(device, dims):
     """The gradient through a 23-step rollout does not depend on the
     checkpoint interval and matches the oracle's reverse pass. Covers g_traj
-    with include_init, and g_final."""
+    with include_init, and g_final; 1-D, 2-D, and a complex L (KdV)."""
     import torch
 
     print(f"[case] rollout vjp D={dims}", flush=True)
-    if dims == 1:
+    if dims == "kdv":
+        gpu = ex.stepper.KortewegDeVries(1, 20.0, 256, 1e-3)
+        cpu = ost.KortewegDeVries(1, 20.0, 256, 1e-3)
+        u0, _ = ics(3, 1, 20.0, 256, 5, seed=45)
+        gt, _ = ics(3 * 24, 1, 20.0, 256, 10, seed=46)
+        gt = gt.reshape(3, 24, 1, 256)
+    elif dims == 1:
         gpu = ex.stepper.KuramotoSivashinskyConservative(1, 100.0, 256, 0.1)
         cpu = ost.KuramotoSivashinskyConservative(1, 100.0, 256, 0.1)
         u0, _ = ics(3, 1, 100.0, 256, 5, seed=41)

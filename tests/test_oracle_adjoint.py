@@ -114,6 +114,10 @@ def _steppers():
         ("kolmogorov", st.KolmogorovFlowVorticity(2, 2 * np.pi, 16, 0.01, diffusivity=0.01), 1),
         ("ks1d_order1", st.KuramotoSivashinsky(1, 30.0, 64, 0.02, order=1), 1),
         ("ks1d_order0", st.KuramotoSivashinsky(1, 30.0, 64, 0.02, order=0), 1),
+        # complex L: the transpose multiplies by conj(E)
+        ("kdv1d", st.KortewegDeVries(1, 20.0, 64, 1e-3), 1),
+        ("kdv1d_order1", st.KortewegDeVries(1, 20.0, 64, 1e-3, order=1), 1),
+        ("dispersion1d", st.Dispersion(1, 1.0, 64, 1e-3, dispersivity=0.01), 1),
     ]
 
 
