@@ -196,7 +196,7 @@ def rollout_with_fallback(corrected: CorrectedStepper, fine, u0_fine, n_coarse: 
     """Corrected coarse rollout of (B, 1, Nf) initial states. Trajectories
     that leave [-bound, bound] or become non-finite are recomputed by the
     plain fine stepper and downsampled, so the fine solver stays the fallback
-    and the judge. ``u0_coarse`` (default: u0_fine mapped to the coarse grid)
+    and the reference. ``u0_coarse`` (default: u0_fine mapped to the coarse grid)
     are the coarse initial states. Returns (trajectory (B, n_coarse, 1, Nc),
     mask of the trajectories that fell back)."""
     from .spectral import map_between_resolutions
