@@ -22,6 +22,10 @@
 //     J^T mu = -b P (-i kappa) F[w_x F^-1[P Z mu]]
 //   polynomial                 N = P F[p(w)]               (_polynomial.py:64-76)
 //     J^T mu = P F[p'(w) F^-1[P mu]]
+//   general  N = g0 P F[w^2] + (g1/2) P i kappa F[w^2] + (g2/2) P Z F[w_x^2]
+//                                                          (_general_nonlinear.py:89-119)
+//     J^T mu = P (F[w (2 g0 h + g1 hd)] + g2 (-i kappa) F[w_x hz]),
+//     h = F^-1[P mu], hd = F^-1[P (-i kappa) mu], hz = F^-1[P Z mu]
 //
 // ETDRK2 (_etdrk_2.py:91-102), cotangent lam of the new state:
 //   rho = lam + J(S1)^T (c2 lam),   lam_in = E rho + J(u_hat)^T (c1 rho - c2 lam)
@@ -47,13 +51,13 @@ struct Vjp1DArgs {
 
 template <int KIND>
 constexpr int lin_fields() {
-  return KIND == NL_CONV_NONCONS ? 2 : (KIND == NL_ZERO ? 0 : 1);
+  return (KIND == NL_CONV_NONCONS || KIND == NL_GENERAL) ? 2 : (KIND == NL_ZERO ? 0 : 1);
 }
 
 template <int KIND, int PT>
 struct Lin1D {
   static constexpr int NFL = lin_fields<KIND>() > 0 ? lin_fields<KIND>() : 1;
-  cf f[NFL][PT];  // physical fields (2N x): CONS w; NONCONS w, w_x; GRADNORM w_x; POLY w
+  cf f[NFL][PT];  // physical fields (2N x): CONS w; NONCONS, GENERAL w, w_x; GRADNORM w_x; POLY w
 };
 
 // Fields of the linearisation point S (2 u_hat) and, with WANT_N, the
@@ -138,6 +142,39 @@ __device__ __forceinline__ void lin_eval(const cf* S, cf nyq, Lin1D<KIND, PT>& L
       rfft<M, PT, 1>(Z, zn, ln, lds, rt, kc);
       masked_scale<M, PT>(Z[0], Nn, j, a.kmax, [&](int, cf x) { return x; });
       nn = (M <= a.kmax) ? zn[0] : cf{0.f, 0.f};
+    }
+  } else if constexpr (KIND == NL_GENERAL) {
+    cf U[1][PT], V[1][PT];
+    cf un[1], vn[1];
+    masked_copy<M, PT>(S, nyq, U[0], un[0], j, a.kmax);
+    masked_deriv<M, PT>(S, nyq, V[0], vn[0], j, a.kmax, a.kscale);
+    irfft<M, PT, 1>(U, un, ln, lds, rt);
+    irfft<M, PT, 1>(V, vn, ln, lds, rt);
+#pragma unroll
+    for (int s = 0; s < PT; ++s) {
+      L.f[0][s] = U[0][s];
+      L.f[1][s] = V[0][s];
+    }
+    if constexpr (WANT_N) {
+#pragma unroll
+      for (int s = 0; s < PT; ++s) {
+        U[0][s] = cf{U[0][s].re * U[0][s].re, U[0][s].im * U[0][s].im};
+        V[0][s] = cf{V[0][s].re * V[0][s].re, V[0][s].im * V[0][s].im};
+      }
+      rfft<M, PT, 1>(U, un, ln, lds, rt, kc);
+      rfft<M, PT, 1>(V, vn, ln, lds, rt, kc);
+      const float g0 = a.g0 * q2, hb1 = 0.5f * a.g1 * a.kscale * q2, hb2 = 0.5f * a.g2 * q2;
+      masked_scale<M, PT>(U[0], Nn, j, a.kmax, [&](int s, cf u) {
+        const int k = j + s * T;
+        const float ck = hb1 * (float)k;
+        const float cg = (k == 0 && a.zero_mode_fix) ? 0.f : hb2;
+        const cf v = V[0][s];
+        return cf{g0 * u.re - ck * u.im + cg * v.re, g0 * u.im + ck * u.re + cg * v.im};
+      });
+      const float ckn = hb1 * (float)M;
+      nn = (M <= a.kmax) ? cf{g0 * un[0].re - ckn * un[0].im + hb2 * vn[0].re,
+                              g0 * un[0].im + ckn * un[0].re + hb2 * vn[0].im}
+                         : cf{0.f, 0.f};
     }
   } else {
     static_assert(KIND == NL_GRADNORM, "no 1-D adjoint for this nonlinear kind");
@@ -242,6 +279,40 @@ __device__ __forceinline__ void adj_eval(const Lin1D<KIND, PT>& L, const cf* X, 
     an = ((M <= a.kmax) ? r2n : 0.f) * zn[0];
     (void)cs;
     (void)kn;
+  } else if constexpr (KIND == NL_GENERAL) {
+    cf H[1][PT], D[1][PT], Q[1][PT];
+    cf hn[1], dn[1], qn[1];
+    masked_copy<M, PT>(X, xn, H[0], hn[0], j, a.kmax);
+    masked_deriv<M, PT>(X, xn, D[0], dn[0], j, a.kmax, a.kscale);
+#pragma unroll
+    for (int s = 0; s < PT; ++s) {
+      D[0][s] = cf{-D[0][s].re, -D[0][s].im};
+      Q[0][s] = H[0][s];
+    }
+    dn[0] = cf{-dn[0].re, -dn[0].im};
+    qn[0] = hn[0];
+    if (j == 0 && a.zero_mode_fix) Q[0][0] = cf{0.f, 0.f};
+    irfft<M, PT, 1>(H, hn, ln, lds, rt);
+    irfft<M, PT, 1>(D, dn, ln, lds, rt);
+    irfft<M, PT, 1>(Q, qn, ln, lds, rt);
+    const float c0 = 2.f * a.g0 * q2, c1 = a.g1 * q2, c2 = a.g2 * q2;
+#pragma unroll
+    for (int s = 0; s < PT; ++s) {
+      const cf w = L.f[0][s], wx = L.f[1][s];
+      H[0][s] = cf{w.re * fmaf(c0, H[0][s].re, c1 * D[0][s].re), w.im * fmaf(c0, H[0][s].im, c1 * D[0][s].im)};
+      Q[0][s] = cf{wx.re * Q[0][s].re, wx.im * Q[0][s].im};
+    }
+    rfft<M, PT, 1>(H, hn, ln, lds, rt, kc);
+    rfft<M, PT, 1>(Q, qn, ln, lds, rt, kc);
+    const float cj = c2 * a.kscale * (float)j, ck = c2 * a.kscale;
+    masked_scale<M, PT>(H[0], A, j, a.kmax, [&](int s, cf x) {
+      const float f = fmaf(ck, (float)(s * T), cj);  // g2 q2 kappa_k
+      const cf y = Q[0][s];
+      return cf{fmaf(f, y.im, x.re), fmaf(-f, y.re, x.im)};  // x + f (-i) y
+    });
+    const float fn = c2 * kn;
+    an = (M <= a.kmax) ? cf{fmaf(fn, qn[0].im, hn[0].re), fmaf(-fn, qn[0].re, hn[0].im)} : cf{0.f, 0.f};
+    (void)cs;
   } else {
     static_assert(KIND == NL_GRADNORM, "no 1-D adjoint for this nonlinear kind");
     // h = F^-1[P Z mu]; A = cs P (-i kappa) F[w_x h]

@@ -281,7 +281,8 @@ static VjpFn pick_vjp_order(int order) {
 }
 
 // Adjoint step kernels: ETDRK orders 0-2, convection (both forms), gradient
-// norm and polynomials; with a complex L (N <= 1024): linear and convection.
+// norm, polynomials and the general form; with a complex L (N <= 1024):
+// linear, convection and the general form.
 VjpFn EXMI_CAT(get_vjp1d_, EXMI_M)(int kind, int order, int cplx) {
   constexpr int M = EXMI_M, PT = EXMI_PT;
   if (cplx) {
@@ -291,6 +292,7 @@ VjpFn EXMI_CAT(get_vjp1d_, EXMI_M)(int kind, int order, int cplx) {
         case NL_ZERO: return pick_vjp_order<M, PT, NL_ZERO, true>(order);
         case NL_CONV_CONS: return pick_vjp_order<M, PT, NL_CONV_CONS, true>(order);
         case NL_CONV_NONCONS: return pick_vjp_order<M, PT, NL_CONV_NONCONS, true>(order);
+        case NL_GENERAL: return pick_vjp_order<M, PT, NL_GENERAL, true>(order);
         default: return nullptr;
       }
     }
@@ -303,6 +305,7 @@ VjpFn EXMI_CAT(get_vjp1d_, EXMI_M)(int kind, int order, int cplx) {
     case NL_CONV_NONCONS: return pick_vjp_order<M, PT, NL_CONV_NONCONS>(order);
     case NL_GRADNORM: return pick_vjp_order<M, PT, NL_GRADNORM>(order);
     case NL_POLY: return pick_vjp_order<M, PT, NL_POLY>(order);
+    case NL_GENERAL: return pick_vjp_order<M, PT, NL_GENERAL>(order);
     default: return nullptr;
   }
 }

@@ -59,6 +59,15 @@ CASES_1D = [
     # complex L: the transpose multiplies by conj(exp(L dt))
     ("kdv", lambda o, dt: ex.stepper.KortewegDeVries(1, 20.0, 256, dt * 1e-2, order=o),
      lambda o, dt: ost.KortewegDeVries(1, 20.0, 256, dt * 1e-2, order=o), 20.0, 256),
+    # the general form (square, convection, gradient norm), real and complex L
+    ("general", lambda o, dt: ex.stepper.generic.GeneralNonlinearStepper(
+        1, 2.0, 128, dt * 1e-2, linear_coefficients=(0.0, 0.0, 0.01), nonlinear_coefficients=(0.3, -1.0, -0.5), order=o),
+     lambda o, dt: ost.GeneralStepper(1, 2.0, 128, dt * 1e-2, linear_coefficients=(0.0, 0.0, 0.01),
+                                      nonlinear_scales=(0.3, -1.0, -0.5), order=o), 2.0, 128),
+    ("general_complex", lambda o, dt: ex.stepper.generic.GeneralNonlinearStepper(
+        1, 2.0, 512, dt * 1e-2, linear_coefficients=(0.0, 0.1, 0.01), nonlinear_coefficients=(0.3, -1.0, -0.5), order=o),
+     lambda o, dt: ost.GeneralStepper(1, 2.0, 512, dt * 1e-2, linear_coefficients=(0.0, 0.1, 0.01),
+                                      nonlinear_scales=(0.3, -1.0, -0.5), order=o), 2.0, 512),
 ]
 
 

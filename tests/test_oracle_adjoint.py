@@ -118,6 +118,8 @@ def _steppers():
         ("kdv1d", st.KortewegDeVries(1, 20.0, 64, 1e-3), 1),
         ("kdv1d_order1", st.KortewegDeVries(1, 20.0, 64, 1e-3, order=1), 1),
         ("dispersion1d", st.Dispersion(1, 1.0, 64, 1e-3, dispersivity=0.01), 1),
+        ("general1d_complex", st.GeneralStepper(1, 2.0, 64, 1e-3, linear_coefficients=(0.0, 0.1, 0.01),
+                                                nonlinear_scales=(0.3, -1.0, -0.5)), 1),
     ]
 
 
