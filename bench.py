@@ -3,16 +3,25 @@
 Default workload (the batched Burgers rollout): Burgers 1-D, N=1024, L=1,
 dt=1e-3, ETDRK2, 4096 trajectories advanced together, per-trajectory
 viscosity nu log-uniform in [1e-4, 1e-2], float32 truncated-Fourier ICs
-(cutoff 8, seed 2), rollout of --steps steps captured in one hipGraph, every
+(cutoff 8, seed 2), a rollout of 1000 steps captured in one hipGraph, every
 state written to the trajectory buffer (rollout semantics).
 
-A bench step is one ETDRK step of every trajectory of the workload; `value`
-is state-steps/s = trajectories * steps / wall time of the timed graph
-replay, with the ICs already resident in HBM. The rollout runs as the
-strategy the library's autotuner picks (one multi-step launch, or per-step
-launches). The roofline uses the algorithmic bytes of one launch:
-trajectories * steps-per-launch * (read + write of the state + 12 B per mode
-of per-trajectory ETDRK2 tables).
+A bench step is one replay of the workload's rollout graph (1000 ETDRK steps
+of all 4096 trajectories, the configuration's unit of work); `value` is
+state-steps/s = trajectories * 1000 * bench steps / wall time of the timed
+replays, with the ICs already resident in HBM. The rollout runs as the
+library's `rollout` does by default: for this plan (real linear operator,
+Nyquist mode removed by the 2/3 dealias mask) the per-step forward transform
+of the emitted state is an exact identity, so the multi-step kernel keeps
+each trajectory's spectrum on chip and writes its inverse transform every
+step; the literal per-step physical round trip of the reference is measured
+beside it (`physical_roundtrip`). The strategy is the one the library's
+autotuner picks (one multi-step launch, or per-step launches). The roofline
+uses the algorithmic bytes of one launch: trajectories * steps-per-launch *
+(read + write of the state + 12 B per mode of per-trajectory ETDRK2 tables);
+`roofline.traffic` is the HBM traffic of the same launch from the committed
+rocprofv3 FETCH_SIZE/WRITE_SIZE summary (bytes per state-step, copy-probe
+calibrated), scaled to this run's state-steps (`traffic_source`).
 
 The CPU baseline (rank 0 only) runs the NumPy/SciPy FP32 restatement of the
 same step on a bounded sample of the workload and doubles as the parity
@@ -69,14 +78,20 @@ def tfs_ics(batch, d, n, length, cutoff, rng):
 
 
 def read_traffic(workload, state_steps):
-    """Measured HBM bytes for this run's launch from the committed
-    rocprofv3 --pmc summary (bytes per state-step times state-steps)."""
-    path = os.path.join(HERE, "profiles", f"traffic_{workload}.json")
+    """HBM bytes of this run's launch from the committed rocprofv3 --pmc
+    summary (bytes per state-step times state-steps), and where they come
+    from."""
+    name = f"traffic_{workload}.json"
+    path = os.path.join(HERE, "profiles", name)
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
         per_step = json.load(f).get("bytes_per_state_step")
-    return None if per_step is None else per_step * state_steps
+    if per_step is None:
+        return None, None
+    return per_step * state_steps, (f"profiles/{name}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the same "
+                                    f"launch ({per_step:.0f} B per state-step, copy-probe calibrated), times this "
+                                    "run's state-steps")
 
 
 def rel_l2_rows(g, ref):
@@ -144,7 +159,8 @@ def oracle_baseline(make_oracle, u0, gpu_checks, sample_steps, label, fp64_maker
 
 def result_line(workload, config, value, K, W, wall, per_launch_s, alg_bytes, n_launches, kernel, traffic):
     achieved = alg_bytes / per_launch_s / 1e9
-    return {
+    traffic, traffic_source = traffic if isinstance(traffic, tuple) else (traffic, None)
+    res = {
         "metric": METRIC, "value": value, "unit": "state-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded truncated-Fourier ICs, seeded coefficients)",
@@ -153,83 +169,105 @@ def result_line(workload, config, value, K, W, wall, per_launch_s, alg_bytes, n_
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes_per_launch": alg_bytes,
                      "avg_launch_us": per_launch_s * 1e6, "launches": n_launches, "kernel": kernel},
     }
+    if traffic_source:
+        res["roofline"]["traffic_source"] = traffic_source
+    return res
 
 
 # ------------------------------------------------------------------ workloads
+def timed_replays(run, K, sync):
+    """Time K back-to-back calls of `run` (graph replays) with events on the
+    launching stream and host wall time, synchronized on both sides."""
+    import torch
+
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    sync()
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(K):
+        run()
+    ev1.record()
+    sync()
+    return time.perf_counter() - t0, ev0.elapsed_time(ev1) / 1e3
+
+
 def wl_burgers1d(args, ex, _lib, torch):
+    """One bench step is one replay of the workload's rollout graph: 4096
+    trajectories x R ETDRK steps (R = 1000, the configuration's graph length;
+    --rollout-steps changes it), every state written to the trajectory
+    buffer. value = 4096 * R * K / wall time of K replays."""
     N, L, DT = 1024, 1.0, 1e-3
-    B, K, W = args.batch or 4096, args.steps or 1000, args.warmup
+    B, K, W = args.batch or 4096, args.steps or 20, args.warmup
+    R = args.rollout_steps or 1000
     rng = np.random.default_rng(2)
     u0 = tfs_ics(B, 1, N, L, 8, rng)
     nus = 10.0 ** rng.uniform(-4.0, -2.0, B)
-    ens = ex.ensemble(lambda nu: ex.stepper.Burgers(1, L, N, DT, diffusivity=float(nu)), nus)
-    plan = ens.plan(B)
     lib = _lib.lib()
     dev = torch.device("cuda:0")
     u0_d = torch.tensor(u0, dtype=torch.float32, device=dev)
     stream = _lib.stream_ptr()
-    if W > 0:  # warmup: W steps through a captured graph of their own (includes the autotuner's probe)
-        tw = torch.empty((B, W, 1, N), dtype=torch.float32, device=dev)
-        _lib.check(lib.exmi_rollout(plan.handle, u0_d.data_ptr(), N, tw.data_ptr(), W, 0, 1, stream))
-        torch.cuda.synchronize()
-        del tw
-    traj = torch.empty((B, K, 1, N), dtype=torch.float32, device=dev)
-    _lib.check(lib.exmi_rollout_prepare(plan.handle, u0_d.data_ptr(), N, traj.data_ptr(), K, 0))
-    wall, gpu_s = timed_rollout(
-        lambda: _lib.check(lib.exmi_rollout(plan.handle, u0_d.data_ptr(), N, traj.data_ptr(), K, 0, 1, stream)),
-        torch.cuda.synchronize)
-    strategy = plan.rollout_strategy()
-    n_launches = K if strategy < 0 else 1
-    spl = K // n_launches
     per_state = 4 * N + 4 * N + 12 * (N // 2 + 1)
+
+    def measure(physical):
+        members = [ex.stepper.Burgers(1, L, N, DT, diffusivity=float(nu)) for nu in nus]
+        if physical:  # the literal per-step round trip (EXMI_FLAG_PHYSICAL_ROLLOUT)
+            for m in members:
+                m.fourier_resident = False
+        ens = ex.Ensemble(members)
+        plan = ens.plan(B)
+        traj = torch.empty((B, R, 1, N), dtype=torch.float32, device=dev)
+        # the autotuner's probe, capture and instantiation of the graph, then W untimed replays
+        _lib.check(lib.exmi_rollout_prepare(plan.handle, u0_d.data_ptr(), N, traj.data_ptr(), R, 0))
+        replay = lambda: _lib.check(lib.exmi_rollout(plan.handle, u0_d.data_ptr(), N, traj.data_ptr(), R, 0, 1,  # noqa: E731
+                                                     stream))
+        for _ in range(W):
+            replay()
+        torch.cuda.synchronize()
+        wall, gpu_s = timed_replays(replay, K, torch.cuda.synchronize)
+        strategy = plan.rollout_strategy()
+        return traj, wall, gpu_s, strategy
+
+    traj, wall, gpu_s, strategy = measure(False)
+    n_launches = R if strategy < 0 else 1  # per replay
+    spl = R // n_launches
     kernel = ("step1d_kernel<512,8,NL_CONV_CONS,2>" if strategy < 0 else
-              f"roll1d_kernel<512,8,NL_CONV_CONS,2,{strategy}> ({spl} steps per launch)")
+              f"roll1d_kernel<512,8,NL_CONV_CONS,2,{strategy},FR,KC=340> ({spl} steps per launch)")
     res = result_line("burgers1d_n1024_b4096_etdrk2_graph",
                       {"equation": "Burgers", "num_points": N, "domain_extent": L, "dt": DT, "order": 2,
-                       "batch": B, "rollout_steps": K, "per_trajectory_viscosity": "log-uniform [1e-4, 1e-2]",
-                       "ic": "TFS cutoff 8 seed 2", "hipgraph": True},
-                      B * K / wall, K, W, wall, gpu_s / n_launches, B * spl * per_state, n_launches, kernel,
+                       "batch": B, "rollout_steps_per_graph": R, "bench_step": f"one replay of the {R}-step graph",
+                       "per_trajectory_viscosity": "log-uniform [1e-4, 1e-2]", "ic": "TFS cutoff 8 seed 2",
+                       "hipgraph": True},
+                      B * R * K / wall, K, W, wall, gpu_s / (K * n_launches), B * spl * per_state, n_launches, kernel,
                       read_traffic("burgers1d_n1024_b4096", B * spl))
     res["roofline"]["alg_bytes_per_state_step"] = per_state
+    res["us_per_etdrk_step"] = wall * 1e6 / (K * R)
     res["rollout_strategy"] = strategy
-    final = traj[:, K - 1]
-    res["nan_trajectories"] = int((~torch.isfinite(final).all(dim=-1).all(dim=-1)).sum().item())
-    if not args.no_fourier_resident:
-        # the Fourier-resident rollout of the same workload (an option of the
-        # library: EXMI_FLAG_FOURIER_RESIDENT), reported beside the default
-        fr_members = [ex.stepper.Burgers(1, L, N, DT, diffusivity=float(nu)) for nu in nus]
-        for m in fr_members:
-            m.fourier_resident = True
-        ens_fr = ex.Ensemble(fr_members)
-        ens_fr.fourier_resident = True
-        plan_fr = ens_fr.plan(B)
-        traj_fr = torch.empty_like(traj)
-        if W > 0:
-            tw = torch.empty((B, W, 1, N), dtype=torch.float32, device=dev)
-            _lib.check(lib.exmi_rollout(plan_fr.handle, u0_d.data_ptr(), N, tw.data_ptr(), W, 0, 1, stream))
-            torch.cuda.synchronize()
-            del tw
-        _lib.check(lib.exmi_rollout_prepare(plan_fr.handle, u0_d.data_ptr(), N, traj_fr.data_ptr(), K, 0))
-        wall_fr, gpu_fr = timed_rollout(
-            lambda: _lib.check(lib.exmi_rollout(plan_fr.handle, u0_d.data_ptr(), N, traj_fr.data_ptr(), K, 0, 1,
-                                                stream)), torch.cuda.synchronize)
-        dev_steps = {s_: float(rel_l2_rows(traj_fr[:, s_ - 1].cpu().numpy(), traj[:, s_ - 1].cpu().numpy()).max())
-                     for s_ in (1, 10) if s_ <= K}
-        res["fourier_resident"] = {
-            "value": B * K / wall_fr, "ms_per_step": wall_fr * 1e3 / K,
-            "roofline_frac": B * K * per_state / gpu_fr / 1e9 / HBM_PEAK_GBS,
-            "strategy": plan_fr.rollout_strategy(),
-            "max_rel_l2_vs_default_path": dev_steps}
-        del traj_fr
+    final = traj[:, R - 1]
+    res["nan_trajectories"] = {"count": int((~torch.isfinite(final).all(dim=-1).all(dim=-1)).sum().item()),
+                               "at_step": R, "note": "low-viscosity trajectories blow up, as the FP64 oracle's"}
+    checks = {s: traj[:, s - 1].cpu().numpy() for s in (1, 10) if s <= R}
+    if not args.no_physical_roundtrip:
+        traj_p, wall_p, gpu_p, sp = measure(True)
+        nlp = R if sp < 0 else 1
+        res["physical_roundtrip"] = {
+            "value": B * R * K / wall_p, "us_per_etdrk_step": wall_p * 1e6 / (K * R),
+            "roofline_frac": B * (R // nlp) * per_state / (gpu_p / (K * nlp)) / 1e9 / HBM_PEAK_GBS,
+            "strategy": sp,
+            "kernel": ("step1d_kernel<512,8,NL_CONV_CONS,2>" if sp < 0 else
+                       f"roll1d_kernel<512,8,NL_CONV_CONS,2,{sp},physical,KC=340>"),
+            "max_rel_l2_vs_default_path": {s_: float(rel_l2_rows(checks[s_], traj_p[:, s_ - 1].cpu().numpy()).max())
+                                           for s_ in checks}}
+        del traj_p
+    del traj
     if not args.no_cpu_baseline:
         from oracle import stepper as ost
 
         nb, ns = min(args.cpu_sample_batch or 4096, B), args.cpu_sample_steps or 300
-        checks = {s: traj[:nb, s - 1].cpu().numpy() for s in (1, 10) if s <= K}
         mk = lambda dt: ost.Ensemble([ost.Burgers(1, L, N, DT, diffusivity=float(nu), dtype=dt)  # noqa: E731
                                       for nu in nus[:nb]], dtype=dt)
-        res["cpu_baseline"], res["parity"] = oracle_baseline(mk, u0[:nb], checks, ns,
-                                                              f"first {nb} trajectories x {ns} steps")
+        res["cpu_baseline"], res["parity"] = oracle_baseline(mk, u0[:nb], {s_: g[:nb] for s_, g in checks.items()},
+                                                              ns, f"first {nb} trajectories x {ns} steps")
     return res
 
 
@@ -659,8 +697,10 @@ def main():
     ap.add_argument("--cpu-sample-batch", type=int, default=0)
     ap.add_argument("--cpu-sample-steps", type=int, default=0)
     ap.add_argument("--train-iters", type=int, default=0, help="corrector workload: training iterations")
-    ap.add_argument("--no-fourier-resident", action="store_true",
-                    help="burgers1d: skip the Fourier-resident rollout measured beside the default")
+    ap.add_argument("--rollout-steps", type=int, default=0,
+                    help="burgers1d: ETDRK steps per rollout graph (default 1000)")
+    ap.add_argument("--no-physical-roundtrip", action="store_true",
+                    help="burgers1d: skip the per-step physical round-trip rollout measured beside the default")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()

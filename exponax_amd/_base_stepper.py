@@ -15,10 +15,17 @@ with leading batch axes) runs a whole batch in one launch.
 Plan options, set as attributes before the first call:
 - ``exact_form = True`` evaluates the non-conservative convection with its
   two inverse transforms, as the reference does (EXMI_FLAG_EXACT_FORM).
-- ``fourier_resident = True`` makes 1-D ETDRK2 rollouts keep the spectral
-  state on chip between steps (EXMI_FLAG_FOURIER_RESIDENT). This is the
-  Fourier-space rollout of the reference's performance notes; it differs by
-  rounding only.
+- ``fourier_resident``: how 1-D ETDRK2 rollouts and repeats carry the state
+  from step to step. ``None`` (default): the plan keeps each trajectory's
+  spectrum on chip and writes its inverse transform where the reference's
+  per-step forward transform of that state would return the same spectrum
+  exactly (real linear operator, Nyquist mode removed from the nonlinear term
+  by the dealias mask), else it re-derives the spectrum from the written
+  physical state every step as the reference does. ``True`` forces the
+  spectrum-resident form (EXMI_FLAG_FOURIER_RESIDENT: the Fourier-space
+  rollout of the reference's performance notes), ``False`` the per-step
+  physical round trip (EXMI_FLAG_PHYSICAL_ROLLOUT). The forms differ by
+  float32 rounding only where the default picks the resident one.
 """
 from __future__ import annotations
 
@@ -83,8 +90,10 @@ class BaseStepper:
         d.domain_extent = float(self.domain_extent)
         d.dt = float(self.dt)
         d.dealiasing_fraction = -1.0
+        fr = getattr(self, "fourier_resident", None)
         d.flags = ((_lib.EXMI_FLAG_EXACT_FORM if getattr(self, "exact_form", False) else 0)
-                   | (_lib.EXMI_FLAG_FOURIER_RESIDENT if getattr(self, "fourier_resident", False) else 0))
+                   | (_lib.EXMI_FLAG_FOURIER_RESIDENT if fr is True else 0)
+                   | (_lib.EXMI_FLAG_PHYSICAL_ROLLOUT if fr is False else 0))
         if self.order > 0:
             from .nonlin_fun import BaseNonlinearFun
 

@@ -29,6 +29,7 @@ EXMI_NL_PROJECTED_CONVECTION_KOLMOGOROV = 11
 EXMI_MAX_POLY = 8
 EXMI_FLAG_EXACT_FORM = 1
 EXMI_FLAG_FOURIER_RESIDENT = 2
+EXMI_FLAG_PHYSICAL_ROLLOUT = 4
 
 EXMI_ERR_INVALID = -1
 EXMI_ERR_HIP = -2

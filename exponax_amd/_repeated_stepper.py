@@ -5,12 +5,12 @@ The reference transforms once, repeats ``step_fourier`` and transforms back
 (``_repeated_stepper.py:56-99``). Here:
 - 1-D ETDRK2 steppers sub-step with the Fourier-resident repeat kernel: one
   launch keeps each trajectory's spectrum on chip for all sub-steps, the same
-  transform-once sequence;
-- other 1-D plans (orders other than 2, complex linear operators) and 3-D
-  steppers transform once and launch ``step_fourier`` per sub-step;
-- 2-D steppers and 1-D channel-coupled reactions (stepped physically)
-  repeat the physical step, which differs from the spectral sub-stepping by
-  rounding only.
+  transform-once sequence (complex linear operators included: the spectrum's
+  imaginary Nyquist part is carried from sub-step to sub-step);
+- other 1-D plans (orders other than 2) and 2-D and 3-D steppers transform
+  once and launch ``step_fourier`` per sub-step;
+- 1-D channel-coupled reactions, which have no Fourier-space step, repeat
+  the physical step.
 """
 from __future__ import annotations
 
@@ -87,7 +87,7 @@ class RepeatedStepper:
         from . import _lib
 
         coupled = getattr(nl, "kind", None) in (_lib.EXMI_NL_GRAY_SCOTT, _lib.EXMI_NL_BELOUSOV_ZHABOTINSKY)
-        if (D == 1 and not coupled and not self._fourier_resident_ok(self.stepper)) or D == 3:
+        if (D == 1 and not coupled and not self._fourier_resident_ok(self.stepper)) or D >= 2:
             from .spectral import fft, ifft
 
             t, back = as_device_tensor(u)

@@ -46,8 +46,19 @@ def _native(stepper_fn):
     return None, False
 
 
+def _time_first(traj, lead_ndim):
+    """(lead..., T, ...) -> (T, lead..., ...): the stacking of ``lax.scan``
+    over a batched carry."""
+    return traj.movedim(lead_ndim, 0) if hasattr(traj, "movedim") else np.moveaxis(traj, lead_ndim, 0)
+
+
 def rollout(stepper_fn, n: int, *, include_init: bool = False, takes_aux: bool = False, constant_aux: bool = True):
-    """``_utils.py:92-186``. Returns a function u0 -> trajectory (n[+1], ...)."""
+    """``_utils.py:92-186``. Returns a function u0 -> trajectory (n[+1], ...):
+    time is the leading axis, also for a batched stepper (``vmap(stepper)``
+    or an :class:`Ensemble`), whose trajectory is (n[+1], B, ...) as the
+    reference's ``rollout(jax.vmap(stepper), n)`` scans over the batched
+    state. ``stepper.rollout_batch`` keeps the batch axis first
+    ((B, n[+1], ...), the layout of ``jax.vmap(rollout(stepper, n))``)."""
     if takes_aux:
         def run_aux(u0, aux):
             out = [u0] if include_init else []
@@ -67,7 +78,9 @@ def rollout(stepper_fn, n: int, *, include_init: bool = False, takes_aux: bool =
                 shape = tuple(getattr(u0, "shape", ()))
                 if shape != st.state_shape:
                     raise ValueError(f"Expected shape {st.state_shape}, got {shape}.")
-            return st.rollout_batch(u0, n, include_init=include_init)
+                return st.rollout_batch(u0, n, include_init=include_init)
+            lead_ndim = len(getattr(u0, "shape", ())) - len(st.state_shape)
+            return _time_first(st.rollout_batch(u0, n, include_init=include_init), lead_ndim)
         out = [u0] if include_init else []
         u = u0
         for _ in range(n):

@@ -237,6 +237,17 @@ struct Twiddles {
 static std::mutex g_tw_mu;
 static std::map<std::pair<int, int>, Twiddles> g_tw;  // (device, M)
 
+// Host values of the two tables of transform size M, formed in double
+// precision and rounded once to float.
+static void twiddle_values(int M, cf* h, cf* hf) {
+  const double pi = 3.14159265358979323846;
+  for (int m = 0; m < M; ++m) h[m] = cf{(float)std::cos(2 * pi * m / M), (float)-std::sin(2 * pi * m / M)};
+  for (int k = 0; k < M; ++k) {
+    const double c = std::cos(pi * k / M), s = -std::sin(pi * k / M);  // w_k
+    hf[k] = cf{(float)s, (float)(-c)};                                  // -i w_k
+  }
+}
+
 static int get_twiddles(int M, Twiddles* out) {
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
@@ -247,14 +258,8 @@ static int get_twiddles(int M, Twiddles* out) {
     *out = it->second;
     return EXMI_OK;
   }
-  // tables formed in double precision, rounded once to float
   std::vector<cf> h(M), hf(M);
-  const double pi = 3.14159265358979323846;
-  for (int m = 0; m < M; ++m) h[m] = cf{(float)std::cos(2 * pi * m / M), (float)-std::sin(2 * pi * m / M)};
-  for (int k = 0; k < M; ++k) {
-    const double c = std::cos(pi * k / M), s = -std::sin(pi * k / M);  // w_k
-    hf[k] = cf{(float)s, (float)(-c)};                                  // -i w_k
-  }
+  twiddle_values(M, h.data(), hf.data());
   Twiddles t;
   if (hipMalloc(&t.tw, sizeof(cf) * M) != hipSuccess || hipMalloc(&t.fw, sizeof(cf) * M) != hipSuccess)
     return fail(EXMI_ERR_ALLOC, "twiddle allocation failed");
@@ -307,6 +312,7 @@ struct exmi_plan {
   RollFn roll[3] = {nullptr, nullptr, nullptr};  // by transforms per lane group (1, 2)
   int roll_nt = 0;                               // chosen by the autotuner (0 = not yet)
   int fourier_resident = 0;                      // rollouts keep the spectral state on chip
+  int fr_auto = 0;  // chosen by the plan (exact identity); per-step launches stay allowed
   hipStream_t capture_stream = nullptr;
   hipStream_t launch_stream = nullptr;  // concurrent group rollouts
   GraphCache gc;
@@ -315,6 +321,7 @@ struct exmi_plan {
   int cplx = 0;  // 2-D: complex linear operator (Im E follows the real coefficients)
   int coupled = 0;  // 1-D channel-coupled reaction: steps through the multi-step kernel
   int nf = 0, kc = 0;
+  int prod_nc = 0;  // 2-D/3-D general kind: first product u sum_d d_d u (single-channel non-conservative)
   int ninst = 0;          // 2-D and 3-D: instances = batch * channels
   long long tstride = 0;  // 2-D: complex entries of T per instance
   cf* W3 = nullptr;       // 3-D: spectra of every instance (the step's U0)
@@ -376,6 +383,7 @@ static void fill_args2d(const exmi_plan* p, Step2DArgs* a) {
   a->g0 = (float)p->d.general_scales[0];
   a->g1 = (float)p->d.general_scales[1];
   a->g2 = (float)p->d.general_scales[2];
+  a->prod_nc = p->prod_nc;
   a->e_im = p->cplx ? num_coefs(p->d.order) : -1;
   a->poly_lap = p->d.nonlin_kind == EXMI_NL_CAHN_HILLIARD;
   a->npoly = p->d.num_poly;
@@ -488,9 +496,10 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   } else if (d->nonlin_kind == EXMI_NL_CONVECTION && d->order > 0) {
     if (d->conv_single_channel) {
       if (C != 1) return fail(EXMI_ERR_UNSUPPORTED, "single-channel convection needs num_channels == 1");
-      if (!d->conv_conservative)
-        return fail(EXMI_ERR_UNSUPPORTED, "2-D single-channel convection is built in the conservative form");
-      kind = NL_GENERAL;  // -b/2 (sum_d i kappa_d) F[u^2] is the general form with scales (0, -b, 0)
+      // -b/2 (sum_d i kappa_d) F[u^2] is the general form with scales
+      // (0, -b, 0); the non-conservative -b F[u sum_d d_d u] is the general
+      // form's fields with that product and scales (-b, 0, 0)
+      kind = NL_GENERAL;
       conv_general = true;
     } else {
       if (C != 2)  // nonlin_fun/_convection.py:125-129
@@ -535,8 +544,9 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   if (zero_poly) p->d.num_poly = 0;
   if (d->nonlin_kind == EXMI_NL_CAHN_HILLIARD) cahn_hilliard_poly(&p->d);
   if (conv_general) {
-    p->d.general_scales[0] = 0.0;
-    p->d.general_scales[1] = -d->scale;
+    p->prod_nc = d->conv_conservative ? 0 : 1;
+    p->d.general_scales[0] = p->prod_nc ? -d->scale : 0.0;
+    p->d.general_scales[1] = p->prod_nc ? 0.0 : -d->scale;
     p->d.general_scales[2] = 0.0;
   }
   p->kmax = dealias_kmax(N, d->dealiasing_fraction);
@@ -688,9 +698,7 @@ static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
       kind = NL_POLY;  // the empty polynomial
     } else if (d->nonlin_kind == EXMI_NL_CONVECTION && d->conv_single_channel) {
       if (C != 1) return fail(EXMI_ERR_UNSUPPORTED, "single-channel convection needs num_channels == 1");
-      if (!d->conv_conservative)
-        return fail(EXMI_ERR_UNSUPPORTED, "3-D single-channel convection is built in the conservative form");
-      kind = NL_GENERAL;
+      kind = NL_GENERAL;  // as in 2-D: the general form's fields, products and scales
       conv_general = true;
     } else if (d->nonlin_kind == EXMI_NL_PROJECTED_CONVECTION ||
                d->nonlin_kind == EXMI_NL_PROJECTED_CONVECTION_KOLMOGOROV) {
@@ -725,8 +733,9 @@ static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
   if (d->order > 0 && d->nonlin_kind == EXMI_NL_ZERO) p->d.num_poly = 0;
   if (d->nonlin_kind == EXMI_NL_CAHN_HILLIARD) cahn_hilliard_poly(&p->d);
   if (conv_general) {
-    p->d.general_scales[0] = 0.0;
-    p->d.general_scales[1] = -d->scale;
+    p->prod_nc = d->conv_conservative ? 0 : 1;
+    p->d.general_scales[0] = p->prod_nc ? -d->scale : 0.0;
+    p->d.general_scales[1] = p->prod_nc ? 0.0 : -d->scale;
     p->d.general_scales[2] = 0.0;
   }
   p->nf3 = kind == NL_GRADNORM ? 3 : (kind == NL_CONV_NONCONS || kind == NL_GENERAL) ? 4 : kind == NL_PROJ3D ? 2 : 1;
@@ -811,6 +820,7 @@ static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t s
   a.kscale = p->kscale;
   a.scale = (float)p->d.scale;
   a.g0 = (float)p->d.general_scales[0];
+  a.prod_nc = p->prod_nc;
   a.g1 = (float)p->d.general_scales[1];
   a.g2 = (float)p->d.general_scales[2];
   a.feed = (float)p->d.reaction_rates[0];
@@ -912,30 +922,46 @@ static int enqueue_step2d(exmi_plan* p, const float* u_in, long long ibs, float*
                           bool write_T, hipStream_t st);
 static int transform2d(bool inverse, int N, int count, const float* x, cf* xh, hipStream_t st);
 
-// 2-D step_fourier: the physical step between the standalone 2-D transforms
-// (irfftn, step, rfftn). For the spectrum of a real field this is the
-// reference's step_fourier up to rounding; imaginary parts that irfftn
-// ignores (the DC and Nyquist columns of the last axis) do not survive.
+// 2-D step_fourier (_base_stepper.py:222-239, etdrk/_base_etdrk.py:64-80):
+// the step chain's column passes from the given spectrum to the new one, no
+// outer transforms: the first column pass reads u_hat [kx][ky] (COL_CA_SPEC,
+// or COL_C0 for ETDRK0) and the last stage writes u_hat' [kx][ky]; the row
+// passes in between form the nonlinear products as in a physical step.
+// Imaginary parts of the DC and Nyquist columns are carried like every other
+// mode, as the reference's step_fourier does. Contiguous spectra; in and out
+// may be the same buffer (every read of `in` happens in the first launch,
+// every write of `out` in the last).
 static int step_fourier2d(exmi_plan* p, const cf* in, long long ibs, cf* out, long long obs, hipStream_t st) {
-  const int N = p->d.num_points, B = p->d.batch, C = p->d.num_channels;
-  const long long nr = (long long)N * (N / 2 + 1), sz = (long long)C * N * N, szs = (long long)C * nr;
-  float* phys = nullptr;
-  cf* spec = nullptr;
-  HIP_TRY(hipMallocAsync((void**)&phys, sizeof(float) * sz * B, st));
-  hipError_t e = hipMallocAsync((void**)&spec, sizeof(cf) * szs * B, st);
-  int rc = EXMI_OK;
-  if (e == hipSuccess) e = copy_batch((const float*)in, 2 * ibs, (float*)spec, 2 * szs, 2 * szs, B, st);
-  if (e != hipSuccess) rc = fail(EXMI_ERR_HIP, "2-D step_fourier: %s", hipGetErrorString(e));
-  if (!rc) rc = transform2d(true, N, B * C, phys, spec, st);
-  if (!rc) rc = enqueue_step2d(p, phys, sz, phys, sz, false, false, st);
-  if (!rc) rc = transform2d(false, N, B * C, phys, spec, st);
-  if (!rc) {
-    e = copy_batch((const float*)spec, 2 * szs, (float*)out, 2 * obs, 2 * szs, B, st);
-    if (e != hipSuccess) rc = fail(EXMI_ERR_HIP, "2-D step_fourier: %s", hipGetErrorString(e));
+  const int N = p->d.num_points;
+  const long long nr = (long long)N * (N / 2 + 1), szs = (long long)p->d.num_channels * nr;
+  if (ibs != szs || obs != szs) return fail(EXMI_ERR_UNSUPPORTED, "exmi_step_fourier: 2-D spectra must be contiguous");
+  Step2DArgs a;
+  fill_args2d(p, &a);
+  a.spec = const_cast<cf*>(in);
+  a.spec_out = out;
+  a.spec_bstride = nr;
+  a.spec_io = 1;
+  const int B = p->ninst;
+  hipError_t e = hipSuccess;
+  if (p->d.order == 0) {
+    e = p->col2d[COL_C0](a, B, st);
+  } else if (p->d.order == 1) {
+    e = p->col2d[COL_CA_SPEC](a, B, st);
+    if (e == hipSuccess) e = p->row2d[ROW_RB](a, B, st);
+    if (e == hipSuccess) e = p->col2d[COL_C1](a, B, st);
+  } else {
+    static const int stages[3][4] = {{COL_CB, COL_CC, -1, -1},
+                                     {COL_C3A, COL_C3B, COL_C3C, -1},
+                                     {COL_C4A, COL_C4B, COL_C4C, COL_C4D}};
+    e = p->col2d[COL_CA_SPEC](a, B, st);
+    for (int k = 0; k < 4 && e == hipSuccess; ++k) {
+      const int m = stages[p->d.order - 2][k];
+      if (m < 0) break;
+      e = p->row2d[ROW_RB](a, B, st);
+      if (e == hipSuccess) e = p->col2d[m](a, B, st);
+    }
   }
-  (void)hipFreeAsync(phys, st);
-  if (spec) (void)hipFreeAsync(spec, st);
-  return rc;
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "2-D step_fourier launch failed: %s", hipGetErrorString(e));
 }
 
 static int enqueue_step2d(exmi_plan* p, const float* u_in, long long ibs, float* u_out, long long obs, bool have_T,
@@ -1058,19 +1084,31 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   p->vjp = d->order <= 2 && d->nonlin_kind != EXMI_NL_CAHN_HILLIARD
                ? get_vjp1d(p->M, d->order == 0 ? NL_ZERO : kind, d->order, cplx ? 1 : 0)
                : nullptr;
-  const int fr = (d->flags & EXMI_FLAG_FOURIER_RESIDENT) ? 1 : 0;
-  if (fr && d->order != 2) {
+  if ((d->flags & EXMI_FLAG_FOURIER_RESIDENT) && (d->flags & EXMI_FLAG_PHYSICAL_ROLLOUT)) {
+    delete p;
+    return fail(EXMI_ERR_INVALID, "EXMI_FLAG_FOURIER_RESIDENT and EXMI_FLAG_PHYSICAL_ROLLOUT exclude each other");
+  }
+  const int fr_forced = (d->flags & EXMI_FLAG_FOURIER_RESIDENT) ? 1 : 0;
+  if (fr_forced && d->order != 2) {
     delete p;
     return fail(EXMI_ERR_UNSUPPORTED, "Fourier-resident rollouts are built for ETDRK2");
   }
+  // The forward transform of an emitted state returns the spectrum it came
+  // from exactly (in exact arithmetic) when every spectrum the step produces
+  // has real DC and Nyquist bins: a real L (real E at both), and a nonlinear
+  // term whose Nyquist mode the dealias mask removes (the DC bin of every
+  // kind is real or zero). Such ETDRK2 rollouts keep the spectrum on chip.
+  const bool fr_exact = !cplx && !coupled && d->order == 2 && (kind == NL_ZERO || kmax < p->M);
+  const int fr = fr_forced ? 1 : (!(d->flags & EXMI_FLAG_PHYSICAL_ROLLOUT) && fr_exact) ? 1 : 0;
   p->fourier_resident = fr;
+  p->fr_auto = fr && !fr_forced;
   const int std_kc = p->kmax == default_kmax(p->M) ? 1 : 0;
   p->roll[1] = coupled ? get_roll1d_r(p->M, kind, d->order)
               : cplx  ? get_roll1d_c(p->M, kind, d->order, fr)
                       : get_roll1d(p->M, kind, d->order, 1, fr, std_kc);
   p->roll[2] = (cplx || coupled) ? nullptr : get_roll1d(p->M, kind, d->order, 2, fr, std_kc);
   if (const char* e = getenv("EXMI_ROLL_NT")) p->roll_nt = atoi(e);  // 1, 2; -1 = per-step launches
-  if (p->fourier_resident && p->roll_nt < 0) p->roll_nt = 0;
+  if (p->fourier_resident && !p->fr_auto && p->roll_nt < 0) p->roll_nt = 0;
   if (coupled) {
     if (!p->roll[1]) {
       delete p;
@@ -1164,7 +1202,7 @@ int exmi_plan_rollout_strategy(const exmi_plan* p) { return p ? p->roll_nt : fai
 
 int exmi_plan_set_rollout_strategy(exmi_plan* p, int32_t nt) {
   if (!p) return fail(EXMI_ERR_INVALID, "null plan");
-  if (nt == -1 && p->fourier_resident)
+  if (nt == -1 && p->fourier_resident && !p->fr_auto)
     return fail(EXMI_ERR_INVALID, "Fourier-resident plans roll out with the multi-step kernel only");
   if (!(nt == -1 || nt == 0 || ((nt == 1 || nt == 2) && p->roll[nt])))
     return fail(EXMI_ERR_UNSUPPORTED, "rollout strategy %d not available for this plan", nt);
@@ -1314,7 +1352,9 @@ static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u
     }
     return EXMI_OK;
   }
-  if (p->fourier_resident && nt <= 0) nt = p->roll[2] ? 2 : 1;
+  // nt = -1 on a plan that chose the Fourier-resident kernels itself: the
+  // per-step launches (the literal physical round trip) stay available
+  if (p->fourier_resident && (nt == 0 || (nt < 0 && !p->fr_auto))) nt = p->roll[2] ? 2 : 1;
   if (nt > 0 && p->roll[nt]) return enqueue_roll(p, nt, u0, u0_bs, traj + off * sz, tstride, sz, n, 0, st);
   for (int t = 0; t < n; ++t) {
     const float* in = (t == 0 && !include_init) ? u0 : traj + (long long)(t - 1 + off) * sz;
@@ -1344,7 +1384,8 @@ static std::string tune_key(const exmi_plan* p) {
   }
   char key[256];
   snprintf(key, sizeof key, "%s|D%d|N%d|C%d|B%d|k%d|o%d|lb%d|fr%d", arch, p->ndim, p->d.num_points,
-           p->d.num_channels, p->d.batch, p->kind, p->d.order, p->d.lin_batch, p->fourier_resident);
+           p->d.num_channels, p->d.batch, p->kind, p->d.order, p->d.lin_batch,
+           p->fourier_resident ? (p->fr_auto ? 2 : 1) : 0);
   return key;
 }
 
@@ -1359,7 +1400,7 @@ static bool tune_lookup(exmi_plan* p) {
   int v = 0;
   bool found = false;
   while (fscanf(f, "%255s %d", k, &v) == 2) {  // the last entry for a key wins
-    if (key == k && (v == -1 || ((v == 1 || v == 2) && p->roll[v])) && !(v == -1 && p->fourier_resident)) {
+    if (key == k && (v == -1 || ((v == 1 || v == 2) && p->roll[v])) && !(v == -1 && p->fourier_resident && !p->fr_auto)) {
       p->roll_nt = v;
       found = true;
     }
@@ -1396,7 +1437,7 @@ static int autotune_rollout(exmi_plan* p, const float* u0, long long u0_bs, floa
   const int cands[3] = {-1, 1, 2};
   for (int ci = 0; ci < 3; ++ci) {
     const int nt = cands[ci];
-    if (nt < 0 && p->fourier_resident) continue;
+    if (nt < 0 && p->fourier_resident && !p->fr_auto) continue;
     if (nt > 0 && !p->roll[nt]) continue;
     float tmin = 1e30f;
     for (int rep = 0; rep < 4; ++rep) {  // the first launch of each candidate is not timed
@@ -1659,18 +1700,24 @@ int exmi_repeat(exmi_plan* p, const float* u0, int64_t u0_bs, float* u_out, int3
 }  // extern "C"
 
 // 2-D transforms: the row kernel (r2c along y) and the column kernel (c2c
-// along x) of the 2-D steppers, through a stream-ordered [ky][x] buffer.
-static int transform2d(bool inverse, int N, int count, const float* x, cf* xh, hipStream_t st) {
+// along x) of the 2-D steppers, through a [ky][x] intermediate of
+// (N/2+1) N complex entries per field: `T` when given (the caller's
+// scratch), else a stream-ordered allocation. Twiddles `tr` (row size N/2)
+// and `tc` (column size N).
+static int check_2d_transform(int N, int count) {
   if (!pow2(N) || N < 32 || N > 512)
     return fail(EXMI_ERR_UNSUPPORTED, "2-D transforms: num_points=%d: supported sizes are powers of two 32..512", N);
   if (count > 65535) return fail(EXMI_ERR_UNSUPPORTED, "2-D transforms: at most 65535 fields per call");
+  return EXMI_OK;
+}
+
+static int transform2d_with(bool inverse, int N, int count, const float* x, cf* xh, const Twiddles& tr,
+                            const Twiddles& tc, cf* T, hipStream_t st) {
+  int rc = check_2d_transform(N, count);
+  if (rc) return rc;
   const Launch2D row = get_2d(N, NL_GRADNORM, 1, inverse ? ROW_RC : ROW_RA);
   const Launch2D col = get_2d(N, NL_GRADNORM, 0, inverse ? COL_INV : COL_FWD);
   if (!row || !col) return fail(EXMI_ERR_UNSUPPORTED, "no 2-D transform kernel for N=%d", N);
-  Twiddles tr, tc;
-  int rc = get_twiddles(N / 2, &tr);
-  if (!rc) rc = get_twiddles(N, &tc);
-  if (rc) return rc;
   const long long full = (long long)(N / 2 + 1) * N;
   Step2DArgs a;
   memset(&a, 0, sizeof a);
@@ -1683,7 +1730,8 @@ static int transform2d(bool inverse, int N, int count, const float* x, cf* xh, h
   a.tstride = full;
   a.spec = xh;
   a.spec_bstride = full;
-  HIP_TRY(hipMallocAsync((void**)&a.T, sizeof(cf) * full * count, st));
+  if (T) a.T = T;
+  else HIP_TRY(hipMallocAsync((void**)&a.T, sizeof(cf) * full * count, st));
   hipError_t e;
   if (!inverse) {
     a.u_in = x;
@@ -1696,9 +1744,21 @@ static int transform2d(bool inverse, int N, int count, const float* x, cf* xh, h
     e = col(a, count, st);
     if (e == hipSuccess) e = row(a, count, st);
   }
-  const hipError_t ef = hipFreeAsync(a.T, st);
-  if (e == hipSuccess) e = ef;
+  if (!T) {
+    const hipError_t ef = hipFreeAsync(a.T, st);
+    if (e == hipSuccess) e = ef;
+  }
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "2-D transform launch failed: %s", hipGetErrorString(e));
+}
+
+static int transform2d(bool inverse, int N, int count, const float* x, cf* xh, hipStream_t st) {
+  int rc = check_2d_transform(N, count);
+  if (rc) return rc;
+  Twiddles tr, tc;
+  rc = get_twiddles(N / 2, &tr);
+  if (!rc) rc = get_twiddles(N, &tc);
+  if (rc) return rc;
+  return transform2d_with(inverse, N, count, x, xh, tr, tc, nullptr, st);
 }
 
 extern "C" {
@@ -1956,12 +2016,19 @@ static int ensure_work(exmi_plan* p) {
   return EXMI_OK;
 }
 
+// The plans the adjoint kernels cover, for the error of every other plan.
+static int adjoint_unsupported() {
+  return fail(EXMI_ERR_UNSUPPORTED,
+              "no adjoint kernel for this plan (built: 1-D ETDRK orders 0-2 with no nonlinearity, convection, "
+              "gradient norm, the polynomial reactions or the general form, real L, and with a complex L (N <= "
+              "1024) the linear, convection and general plans; 2-D ETDRK2 with the gradient norm)");
+}
+
 int exmi_step_vjp(exmi_plan* p, const float* u, int64_t u_bs, const float* g, int64_t g_bs, float* g_in,
                   int64_t gin_bs, void* stream) {
   if (!p || !u || !g || !g_in) return fail(EXMI_ERR_INVALID, "null argument");
   if (!has_adjoint(p))
-    return fail(EXMI_ERR_UNSUPPORTED, "no adjoint kernel for this plan (1-D: ETDRK orders 0-2 with convection, polynomial or "
-                                      "gradient norm; 2-D: gradient norm)");
+    return adjoint_unsupported();
   int rc = ensure_adjoint_ws(p);
   if (rc) return rc;
   return enqueue_step_vjp(p, u, u_bs, g, g_bs, g_in, gin_bs, (hipStream_t)stream);
@@ -1973,8 +2040,7 @@ int exmi_rollout_vjp(exmi_plan* p, const float* u0, int64_t u0_bs, int32_t n, in
   if (!p || !u0 || !g_u0) return fail(EXMI_ERR_INVALID, "null argument");
   if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
   if (!has_adjoint(p))
-    return fail(EXMI_ERR_UNSUPPORTED, "no adjoint kernel for this plan (1-D: ETDRK orders 0-2 with convection, polynomial or "
-                                      "gradient norm; 2-D: gradient norm)");
+    return adjoint_unsupported();
   hipStream_t st = (hipStream_t)stream;
   const int B = p->d.batch;
   const long long sz = state_size(p);

@@ -220,7 +220,8 @@ __global__ void products3d_kernel(NL3DArgs a, const float* __restrict__ R, float
       }
       case NL_GENERAL: {
         const float u = fld(inst, 0), gx = fld(inst, 1), gy = fld(inst, 2), gz = fld(inst, 3);
-        out[0] = u * u;
+        // single-channel non-conservative convection (nonlin_fun/_convection.py:204-231)
+        out[0] = a.prod_nc ? u * (gx + gy + gz) : u * u;
         out[a.N3] = gx * gx + gy * gy + gz * gz;
         break;
       }

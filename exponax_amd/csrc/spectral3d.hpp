@@ -10,6 +10,7 @@ struct NL3DArgs {
   long long Nr, N3, ninst;
   float kscale, scale, g0, g1, g2, feed, kill;
   int zero_mode_fix, npoly, poly_lap;
+  int prod_nc;      // general kind: first product u sum_d d_d u instead of u^2
   float poly[EXMI_MAX_POLY];
   int inj_mode;     // projected convection: Kolmogorov forcing at (0, inj_mode, 0) of channel 0; -1: none
   float inj_value;  // injection_scale * coef_extraction scaling

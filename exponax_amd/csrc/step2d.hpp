@@ -66,6 +66,8 @@ struct Step2DArgs {
   int channels;              // C: instances per trajectory
   long long tstride;         // complex entries of T per instance (>= Ny N and the products' NP Kc N)
   float g0, g1, g2;          // GeneralNonlinearFun scales, signs folded in
+  int prod_nc;               // general kind: first product u (u_x + u_y) instead of u^2
+                             // (single-channel non-conservative convection)
   const float* u_in;         // physical (batch, C, N, N) with a batch stride
   long long u_in_bstride;    // floats between instances
   float* u_out;              // physical output of RC
@@ -98,6 +100,8 @@ struct Step2DArgs {
   cf* spec;                  // standalone transforms: spectra (ninst, N, N/2+1), [kx][ky]
   cf* spec_out;              // exmi_nonlinear: N_hat (ninst, N, N/2+1), [kx][ky]
   long long spec_bstride;    // complex entries between instances
+  int spec_io;               // exmi_step_fourier: COL_C0 reads u_hat from spec, and the last stage
+                             // (C0, C1, CC, C3C, C4D) writes u_hat' to spec_out, both [kx][ky]
 };
 
 // CA/CB/CC: the ETDRK2 stages; C0: ETDRK0 (S' = E S); C1: ETDRK1's update
@@ -341,15 +345,21 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
           }
         }
       } else if constexpr (KIND == NL_GENERAL) {
+        // u^2 and u_x^2 + u_y^2; with prod_nc the first product is
+        // u (u_x + u_y) (nonlin_fun/_convection.py:204-231)
 #pragma unroll
         for (int s = 0; s < PT; ++s) {
           const cf q = z[0][s];
           if (f == 0) {
+            A[s] = q;
             P[0][s] = cf{q.re * q.re, q.im * q.im};
           } else if (f == 1) {
             P[1][s] = cf{q.re * q.re, q.im * q.im};
+            B2[s] = cf{A[s].re * q.re, A[s].im * q.im};
           } else {
             P[1][s] = cf{fmaf(q.re, q.re, P[1][s].re), fmaf(q.im, q.im, P[1][s].im)};
+            const cf nc = cf{fmaf(A[s].re, q.re, B2[s].re), fmaf(A[s].im, q.im, B2[s].im)};
+            if (a.prod_nc) P[0][s] = nc;
           }
         }
       } else if constexpr (KIND == NL_VORT) {
@@ -585,6 +595,25 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     return cf{fmaf(er, x.re, -ei * x.im), fmaf(er, x.im, ei * x.re)};
   };
   auto emul = [&](int s, cf x) -> cf { return cmul(0, s, x); };
+  // the last stage's new state 2 u_hat' of this column: inverse column
+  // transform into T (physical steps; RC follows), or u_hat' itself into
+  // spec_out [kx][ky] (Fourier-space steps)
+  auto finish = [&](cf (*v)[PT]) {
+    if (a.spec_io) {
+      if (active) {
+        cf* o = a.spec_out + (long long)inst * a.spec_bstride + ky;
+#pragma unroll
+        for (int s = 0; s < PT; ++s) o[(long long)(j + s * T) * C::NY] = 0.5f * v[0][s];
+      }
+      return;
+    }
+    cfft<N, PT, true, 1>(v, ln, lds, tr);
+    if (active) {
+      cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
+    }
+  };
 
   if constexpr (MODE == COL_CA) {
     cf v[1][PT];
@@ -690,21 +719,22 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
       ld(a.A, Ac);
 #pragma unroll
       for (int s = 0; s < PT; ++s) v[0][s] = Ac[s] + coef(cL, s) * Nh[s];
-      cfft<N, PT, true, 1>(v, ln, lds, tr);
-      if (active) {
-        cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
-#pragma unroll
-        for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
-      }
+      finish(v);
     }
     (void)U0;
   } else if constexpr (MODE == COL_C0 || MODE == COL_C1) {
     cf v[1][PT];
     if constexpr (MODE == COL_C0) {
-      const cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
+      if (a.spec_io) {
+        const cf* in = a.spec + (long long)inst * a.spec_bstride + ky;
 #pragma unroll
-      for (int s = 0; s < PT; ++s) v[0][s] = t[j + s * T];
-      cfft<N, PT, false, 1>(v, ln, lds, tr);
+        for (int s = 0; s < PT; ++s) v[0][s] = 2.f * in[(long long)(j + s * T) * C::NY];
+      } else {
+        const cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
+#pragma unroll
+        for (int s = 0; s < PT; ++s) v[0][s] = t[j + s * T];
+        cfft<N, PT, false, 1>(v, ln, lds, tr);
+      }
 #pragma unroll
       for (int s = 0; s < PT; ++s) v[0][s] = emul(s, v[0][s]);
     } else {
@@ -714,12 +744,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
 #pragma unroll
       for (int s = 0; s < PT; ++s) v[0][s] = emul(s, s0[j + s * T]) + coef(1, s) * Nh[s];
     }
-    cfft<N, PT, true, 1>(v, ln, lds, tr);
-    if (active) {
-      cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
-#pragma unroll
-      for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
-    }
+    finish(v);
   } else if constexpr (MODE == COL_CA_SPEC) {
     // CA from a given spectrum u_hat [kx][ky] (exmi_nonlinear): S0 = 2 u_hat
     cf v[1][PT];
@@ -776,12 +801,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
       const cf N0s = kept_col ? n0[j + s * T] : forcing(s);
       v[0][s] = s1[j + s * T] + coef(2, s) * (Nh[s] - N0s);
     }
-    cfft<N, PT, true, 1>(v, ln, lds, tr);
-    if (active) {
-      cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
-#pragma unroll
-      for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
-    }
+    finish(v);
   }
 }
 

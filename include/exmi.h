@@ -77,26 +77,37 @@ enum exmi_nonlin_kind {
  * inverse transform per nonlinear evaluation is saved. */
 #define EXMI_FLAG_EXACT_FORM 1
 
-/* EXMI_FLAG_FOURIER_RESIDENT (1-D, ETDRK2): rollouts and repeats keep each
- * trajectory's spectral state on chip between steps instead of re-deriving
- * it from the float32 physical state written for that step, the Fourier-space
- * rollout of the reference's docs/examples/performance_hints.ipynb
- * (cells 10-16). The physical states written are the inverse transforms of
- * the spectral states. Results differ from the default path by rounding only;
- * one forward transform per step is saved. Single steps are unaffected. */
+/* Rollout forms of 1-D ETDRK2 plans. The reference's rollout
+ * (_utils.py:167-172 over _base_stepper.py:201-220) transforms every state
+ * forward, steps, and transforms back. When every spectrum the step produces
+ * has real DC and Nyquist bins (a real L, and a nonlinear term whose Nyquist
+ * mode the dealias mask removes), the forward transform of an emitted state
+ * returns the spectrum it came from exactly in exact arithmetic; for such
+ * plans rollouts and repeats keep each trajectory's spectrum on chip between
+ * steps and write its inverse transform ("Fourier-resident"). Results differ
+ * from the per-step round trip by float32 rounding only, and one forward
+ * transform per step is saved. This is the default for those plans; the
+ * per-step launches (exmi_plan_set_rollout_strategy(-1)) keep the round trip.
+ * EXMI_FLAG_FOURIER_RESIDENT forces the Fourier-resident form for any 1-D
+ * ETDRK2 plan (complex L included: the Fourier-space rollout of
+ * docs/examples/performance_hints.ipynb cells 10-16, which keeps imaginary
+ * Nyquist parts that a physical round trip drops), and
+ * EXMI_FLAG_PHYSICAL_ROLLOUT forces the literal round trip for every
+ * strategy. Single steps are unaffected. */
 #define EXMI_FLAG_FOURIER_RESIDENT 2
+#define EXMI_FLAG_PHYSICAL_ROLLOUT 4
 
 /*
  * Stepper description: the constructor arguments of BaseStepper
  * (_base_stepper.py:27-157) plus the nonlinear function's own arguments and
- * the batch. The linear operator is given as a host float64 table of the
- * real-valued diagonal L (every config stepper's L is real: even powers of
- * i*kappa plus a constant) of shape (lin_batch, lin_channels, Nr),
- * Nr = N^(D-1) * (N/2+1). lin_batch is 1 (shared) or `batch`
- * (per-trajectory coefficients, the eqx.filter_vmap case).
+ * the batch. The linear operator is given as host float64 tables of the
+ * diagonal L: its real part (host_linear_operator) and, for odd derivative
+ * terms, its imaginary part (host_linear_operator_imag), each of shape
+ * (lin_batch, lin_channels, Nr), Nr = N^(D-1) * (N/2+1). lin_batch is 1
+ * (shared) or `batch` (per-trajectory coefficients, the eqx.filter_vmap case).
  */
 typedef struct exmi_plan_desc {
-  int32_t num_spatial_dims;   /* D: 1, 2 or 3 (3: ETDRK0, N 16..256)         */
+  int32_t num_spatial_dims;   /* D: 1, 2 or 3 (3: ETDRK0-4, N 16..256)       */
   int32_t num_points;         /* N: 1-D powers of two 16..4096, 48 and 50;
                                  2-D powers of two 32..512 (any nonlinear
                                  kind; two channels for the velocity
@@ -132,7 +143,9 @@ typedef struct exmi_plan_desc {
    * is then complex while the phi-coefficients stay real: the reference keeps
    * the real part of every contour sample (etdrk/_etdrk_1.py:66-76,
    * _etdrk_2.py:71-89). The table layout gains Im E after the real
-   * coefficients (exmi_plan_num_coefs counts it). No adjoint. */
+   * coefficients (exmi_plan_num_coefs counts it). The 1-D adjoint of the
+   * linear, convection and general plans multiplies by conj(exp(L dt))
+   * (exmi_step_vjp). */
   const double* host_linear_operator_imag;
   double reaction_rates[2];   /* Gray-Scott feed and kill rates              */
 } exmi_plan_desc;
@@ -160,10 +173,11 @@ int exmi_step(exmi_plan* plan, const float* u_in, int64_t in_bstride, float* u_o
               int64_t out_bstride, void* stream);
 
 /* One step in Fourier space on (B, C, Nr) complex64 spectra (float pairs);
- * strides in complex elements. 2-D plans step physically between the
- * standalone transforms (irfftn, step, rfftn): exact up to rounding for the
- * spectrum of a real field (imaginary parts irfftn ignores are dropped);
- * coupled-reaction 1-D plans return EXMI_ERR_UNSUPPORTED.
+ * strides in complex elements (2-D: contiguous spectra). The step runs on
+ * the spectra with no outer transforms: every mode is carried, imaginary
+ * parts of the DC and Nyquist bins included, as the reference's
+ * step_fourier carries them; uh_in may equal uh_out. Coupled-reaction 1-D
+ * plans return EXMI_ERR_UNSUPPORTED.
  * Replaces BaseStepper.step_fourier (_base_stepper.py:222-239) /
  * BaseETDRK.step_fourier (etdrk/_base_etdrk.py:64-80). */
 int exmi_step_fourier(exmi_plan* plan, const float* uh_in, int64_t in_bstride, float* uh_out,
@@ -265,14 +279,6 @@ int exmi_irfft(int32_t num_spatial_dims, int32_t num_points, int32_t count, cons
 int exmi_nrmse(int32_t batch, int32_t channels, int64_t n, const float* pred, int64_t pred_bstride,
                const float* ref, int64_t ref_bstride, float* out, void* stream);
 
-/* Spectrum of `count` fields of N^D points: out (count, N/2+1) float32,
- * power = 0.5 |u_hat|^2 / (s_rec s_norm) when `power`, else |u_hat| / s_rec,
- * with the reference's scaling arrays (reconstruction: N at k = 0, N/2 and
- * N/2 elsewhere on the last axis, N on the others; norm compensation:
- * N^D). For D = 2, 3 the modes are summed over radial buckets
- * [b - 1/2, b + 1/2) of the wavenumber norm, b = 0..N/2
- * (radial_binning="sum"); D = 2: N 32..512, D = 3: N 16..256.
- * Replaces exponax.get_spectrum (_spectral.py:866-1030). */
 /* Poisson solve in Fourier space, in place on `count` contiguous
  * (N^(D-1), N/2+1) complex64 spectra: u_hat = -f_hat / sum_d (i kappa_d)^p
  * (p = order, even), 0 where that operator vanishes (mean-zero solution).
@@ -303,8 +309,16 @@ int exmi_wave_step_fourier(int32_t num_spatial_dims, int32_t num_points, int32_t
 int exmi_ic_tfs(int32_t num_spatial_dims, int32_t num_points, int32_t count, int32_t nmodes, const int32_t* modes,
                 const float* amp, const float* phase, float* out, void* stream);
 
-/* exmi_spectrum with radial_binning="average" when `average` (NaN for an
- * empty bucket, as nanmean); D = 1 ignores it. */
+/* Spectrum of `count` fields of N^D points: out (count, N/2+1) float32,
+ * power = 0.5 |u_hat|^2 / (s_rec s_norm) when `power`, else |u_hat| / s_rec,
+ * with the reference's scaling arrays (reconstruction: N at k = 0, N/2 and
+ * N/2 elsewhere on the last axis, N on the others; norm compensation:
+ * N^D). For D = 2, 3 the modes are summed over radial buckets
+ * [b - 1/2, b + 1/2) of the wavenumber norm, b = 0..N/2
+ * (radial_binning="sum"; exmi_spectrum_binned with `average`:
+ * radial_binning="average", NaN for an empty bucket, as nanmean; D = 1
+ * ignores it); D = 2: N 32..512, D = 3: N 16..256.
+ * Replaces exponax.get_spectrum (_spectral.py:866-1030). */
 int exmi_spectrum_binned(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x, float* out,
                          int32_t power, int32_t average, void* stream);
 int exmi_spectrum(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x, float* out,
@@ -369,8 +383,10 @@ int exmi_plan_num_coefs(const exmi_plan* plan);
 int exmi_plan_get_tables(const exmi_plan* plan, float* host_out, int64_t host_count);
 
 /* Rollout strategy of a plan: -1 = one launch per step (each a fused step
- * kernel), 1 or 2 = one multi-step launch keeping 1 or 2 transforms per lane
- * group on chip for all steps (same per-step arithmetic), 0 = not chosen
+ * kernel; the per-step physical round trip), 1 or 2 = one multi-step launch
+ * keeping 1 or 2 transforms per lane group on chip for all steps (the
+ * Fourier-resident form where the plan has it, see
+ * EXMI_FLAG_PHYSICAL_ROLLOUT), 0 = not chosen
  * yet (the first exmi_rollout / exmi_rollout_prepare times the candidates on
  * a probe of min(n_steps, 16) steps and keeps the fastest). Setting a
  * strategy drops a cached graph. With the environment variable

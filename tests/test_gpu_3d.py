@@ -136,11 +136,12 @@ def _nl3d_pair(name, N):
                                                            **kw),
                 ost.GeneralStepper(3, 2.0, N, 1e-3, nonlinear_scales=(0.3, -1.0, -0.5), dtype=np.float32, **kw),
                 2.0, 1)
-    kw = dict(diffusivity=0.01, single_channel=True, conservative=True)
+    kw = dict(diffusivity=0.01, single_channel=True, conservative=name == "burgers_single")
     return (ex.stepper.Burgers(3, 1.0, N, 1e-3, **kw), ost.Burgers(3, 1.0, N, 1e-3, dtype=np.float32, **kw), 1.0, 1)
 
 
-@pytest.mark.parametrize("name", ["ks", "allen_cahn", "cahn_hilliard", "gray_scott", "general", "burgers_single"])
+@pytest.mark.parametrize("name", ["ks", "allen_cahn", "cahn_hilliard", "gray_scott", "general", "burgers_single",
+                                  "burgers_single_nc"])
 def test_nonlinear_3d_parity(device, name):
     N = 32
     print(f"[case] 3-D {name} N={N}", flush=True)

@@ -143,10 +143,18 @@ def test_rollout_strategies_agree(device, kind, N, order):
             continue
         outs[strat] = gpu.rollout_batch(x, n, include_init=True).cpu().numpy()
     ref = outs[-1]
+    # ETDRK2 plans whose dealias mask removes the Nyquist mode keep the
+    # spectrum on chip in the multi-step kernels (exact identity of the
+    # per-step round trip, float32 rounding apart); the per-step launches
+    # (-1) keep the round trip
+    tol_rt = 5e-6 if order == 2 else 1e-6
     for strat, tr in outs.items():
         assert tr.shape == (B, n + 1, 1, N)
         np.testing.assert_array_equal(tr[:, 0], u0)
-        assert max_rel_l2(tr[:, 1:].reshape(B * n, 1, N), ref[:, 1:].reshape(B * n, 1, N)) <= 1e-6, strat
+        tol = 1e-6 if strat == -1 else tol_rt
+        assert max_rel_l2(tr[:, 1:].reshape(B * n, 1, N), ref[:, 1:].reshape(B * n, 1, N)) <= tol, strat
+    if 1 in outs and 2 in outs:
+        assert max_rel_l2(outs[1][:, 1:].reshape(B * n, 1, N), outs[2][:, 1:].reshape(B * n, 1, N)) <= 1e-6
     u = u0
     for s in range(1, 11):
         u = cpu.step_batch(u)
@@ -162,8 +170,11 @@ def test_repeat_matches_rollout_last(device):
     u0, _ = ics(3, 1, L, 256, 5, seed=5)
     x = torch.tensor(u0, device=device)
     tr = ex.rollout(ex.vmap(gpu), 7)(x)
+    assert tuple(tr.shape) == (7, 3, 1, 256)  # time first, as lax.scan over the batched state
     last = ex.repeat(ex.vmap(gpu), 7)(x)
-    assert max_rel_l2(last.cpu().numpy(), tr[:, -1].cpu().numpy()) <= 1e-6
+    assert max_rel_l2(last.cpu().numpy(), tr[-1].cpu().numpy()) <= 1e-6
+    assert max_rel_l2(tr.movedim(0, 1).cpu().numpy().reshape(21, 1, 256),
+                      gpu.rollout_batch(x, 7).cpu().numpy().reshape(21, 1, 256)) == 0.0
 
 
 @pytest.mark.parametrize("kind", ["ks_cons", "burgers"])
@@ -176,6 +187,7 @@ def test_fourier_resident_rollout(device, kind):
     print(f"[case] fourier-resident {kind}", flush=True)
     N = 1024 if kind == "burgers" else 256
     gpu, cpu, L = _pair(kind, N, 2)
+    gpu.fourier_resident = False  # the per-step physical round trip
     fr, _, _ = _pair(kind, N, 2)
     fr.fourier_resident = True
     u0, _ = ics(6, 1, L, N, 5, seed=13)
@@ -198,19 +210,28 @@ def test_fourier_resident_rollout(device, kind):
 
 
 @pytest.mark.parametrize("kind,N", [("ks_cons", 256), ("burgers", 1024)])
-def test_rollout_resume_equals_continuous(device, kind, N):
+@pytest.mark.parametrize("physical", [True, False])
+def test_rollout_resume_equals_continuous(device, kind, N, physical):
     """Checkpoint / resume: a rollout of 100 steps resumed from its last state
-    for 100 more equals the continuous 200-step rollout. The multi-step kernel
-    does the same arithmetic per step, so the match is bit for bit."""
+    for 100 more equals the continuous 200-step rollout. With the per-step
+    physical round trip the multi-step kernel does the same arithmetic per
+    step, so the match is bit for bit; the default (spectrum kept on chip)
+    restarts from the rounded physical state and matches to rounding."""
     import torch
 
     gpu, _, L = _pair(kind, N, 2)
+    if physical:
+        gpu.fourier_resident = False
     u0, _ = ics(5, 1, L, N, 5, seed=14)
     x = torch.tensor(u0, device=device)
     full = gpu.rollout_batch(x, 200)
     a = gpu.rollout_batch(x, 100)
     b = gpu.rollout_batch(a[:, -1].contiguous(), 100)
-    torch.testing.assert_close(torch.cat([a, b], dim=1), full, rtol=0, atol=0)
+    if physical:
+        torch.testing.assert_close(torch.cat([a, b], dim=1), full, rtol=0, atol=0)
+    else:
+        got = torch.cat([a, b], dim=1).cpu().numpy()
+        assert max_rel_l2(got.reshape(-1, 1, N), full.cpu().numpy().reshape(-1, 1, N)) <= 1e-5
 
 
 def test_autotuner_cache_roundtrip(device, tmp_path, monkeypatch):
@@ -263,21 +284,31 @@ def test_repeated_stepper(device, order):
     single = rs(x[0]).cpu().numpy()
     assert max_rel_l2(single[None], out[:1]) <= 1e-6
     traj = ex.rollout(ex.vmap(rs), 3, include_init=True)(x).cpu().numpy()
-    assert traj.shape == (3, 4, 1, 256)
-    np.testing.assert_array_equal(traj[:, 0], u0)
-    np.testing.assert_array_equal(traj[:, 1], out)
+    assert traj.shape == (4, 3, 1, 256)  # time first
+    np.testing.assert_array_equal(traj[0], u0)
+    np.testing.assert_array_equal(traj[1], out)
     last = ex.repeat(ex.vmap(rs), 3)(x).cpu().numpy()
-    np.testing.assert_array_equal(last, traj[:, 3])
+    np.testing.assert_array_equal(last, traj[3])
 
 
 def test_rollout_to_file(device, tmp_path):
     """Chunked device rollouts streamed to a .npy dataset equal one
-    continuous rollout bit for bit; the sidecar records the stepper."""
+    continuous rollout: bit for bit with the per-step physical round trip,
+    to float32 rounding with the default spectrum-resident rollout (each
+    chunk restarts from the stored float32 state); the sidecar records the
+    stepper."""
     import torch
 
     gpu, _, L = _pair("ks_cons", 256, 2)
     u0, _ = ics(4, 1, L, 256, 5, seed=33)
     x = torch.tensor(u0, device=device)
+    path = str(tmp_path / "ks_default.npy")
+    ex.io.rollout_to_file(gpu, x, 25, path, chunk=7)
+    arr, _ = ex.io.open_trajectories(path)
+    full = gpu.rollout_batch(x, 25, include_init=True).cpu().numpy()
+    assert max_rel_l2(np.asarray(arr).reshape(-1, 1, 256), full.reshape(-1, 1, 256)) <= 1e-6
+    gpu, _, _ = _pair("ks_cons", 256, 2)
+    gpu.fourier_resident = False  # before the first plan is built
     path = str(tmp_path / "ks.npy")
     ex.io.rollout_to_file(gpu, x, 25, path, chunk=7, meta={"ic": "TFS cutoff 5 seed 33"})
     arr, side = ex.io.open_trajectories(path)
@@ -291,3 +322,125 @@ def test_rollout_to_file(device, tmp_path):
     arr2, side2 = ex.io.open_trajectories(p2)
     np.testing.assert_array_equal(np.asarray(arr2), rs.rollout_batch(x, 4, include_init=True).cpu().numpy())
     assert side2["num_sub_steps"] == 3 and side2["sub_stepper"] == "KuramotoSivashinskyConservative"
+
+
+def test_ks_cons_chaotic_statistics_500_steps(device):
+    """The plain configuration's long rollout (KS-conservative 1-D, L = 100,
+    N = 256, dt = 0.1, ETDRK2, IC TFS cutoff 5 seed 1, 500 steps), judged by
+    statistics as a chaotic trajectory: the time-averaged power spectrum
+    (get_spectrum, _spectral.py:866-1030) over steps 251-500 against the
+    float64 oracle's over the same window, modes 1..kmax, rel-L2 <= 1e-2, and
+    the mean energy 0.5 mean(u^2) over that window within 1e-3 relative.
+    The float32 oracle itself sits at 4.8e-4 and 3e-5 from the float64 one
+    on this window (the gates are ~20x that). The per-step distance to the
+    float64 oracle is printed at steps 1, 10, 100 and 500."""
+    import torch
+
+    from oracle import spectral as osp
+
+    L, N, DT, K = 100.0, 256, 0.1, 500
+    u0, _ = ics(1, 1, L, N, 5, seed=1)
+    gpu = ex.stepper.KuramotoSivashinskyConservative(1, L, N, DT)
+    tr = gpu.rollout_batch(torch.tensor(u0, device=device), K)
+    h0 = K // 2
+    spec_gpu = ex.get_spectrum(tr[0, h0:]).mean(dim=0)[0].cpu().numpy().astype(np.float64)
+    e_gpu = float(0.5 * (tr[0, h0:].double() ** 2).mean().item())
+    g = tr[0].cpu().numpy().astype(np.float64)
+    o64 = ost.KuramotoSivashinskyConservative(1, L, N, DT)
+    u = u0[0].astype(np.float64)
+    spec_ref = np.zeros(N // 2 + 1)
+    e_ref = []
+    for s in range(1, K + 1):
+        u = o64.step_batch(u)
+        if s in (1, 10, 100, 500):
+            print(f"  step {s}: rel-L2 vs float64 oracle {ost.nrmse(g[s - 1], u):.2e}", flush=True)
+        if s > h0:
+            spec_ref += osp.get_spectrum(u)[0]
+            e_ref.append(0.5 * np.mean(u * u))
+    spec_ref /= K - h0
+    kmax = int(2 / 3 * (N // 2)) - 1
+    es = np.linalg.norm(spec_gpu[1:kmax + 1] - spec_ref[1:kmax + 1]) / np.linalg.norm(spec_ref[1:kmax + 1])
+    ee = abs(e_gpu - np.mean(e_ref)) / np.mean(e_ref)
+    print(f"  time-averaged spectrum rel-L2 {es:.2e}, mean energy rel {ee:.2e}", flush=True)
+    assert np.isfinite(g).all()
+    assert es <= 1e-2 and ee <= 1e-3
+
+
+@pytest.mark.parametrize("name", ["ks_cons", "ks", "burgers", "burgers_cons", "burgers_df05", "allen_cahn",
+                                  "cahn_hilliard", "general", "kdv"])
+def test_nonlinear_fun_1d(device, name):
+    """The nonlinear functions on 1-D spectra (BaseNonlinearFun.__call__,
+    nonlin_fun/_base.py:139-159) against the oracle's: single-channel
+    conservative convection, gradient norm, the non-conservative convection
+    (in its literal two-transform form), a 0.5 dealias fraction, the
+    polynomial, Cahn-Hilliard's Laplacian of the cubic, the general form and
+    the convection of a complex-L plan (KdV); max |error| / max |N| <= 1e-5."""
+    import torch
+
+    from oracle import spectral as osp
+
+    print(f"[case] 1-D nonlinear function {name}", flush=True)
+    N = 256
+    if name in ("ks_cons", "ks", "burgers", "burgers_cons", "burgers_df05"):
+        gpu, cpu, L = _pair(name, N, 2)
+    elif name == "allen_cahn":
+        L = 1.0
+        gpu, cpu = ex.stepper.reaction.AllenCahn(1, L, N, 1e-3), ost.AllenCahn(1, L, N, 1e-3, dtype=np.float32)
+    elif name == "cahn_hilliard":
+        L = 1.0
+        gpu = ex.stepper.reaction.CahnHilliard(1, L, N, 1e-5)
+        cpu = ost.CahnHilliard(1, L, N, 1e-5, dtype=np.float32)
+    elif name == "general":
+        L = 2.0
+        gpu = ex.stepper.generic.GeneralNonlinearStepper(1, L, N, 1e-3, nonlinear_coefficients=(0.3, -1.0, -0.5))
+        cpu = ost.GeneralStepper(1, L, N, 1e-3, nonlinear_scales=(0.3, -1.0, -0.5), dtype=np.float32)
+    else:
+        L = 20.0
+        gpu, cpu = ex.stepper.KortewegDeVries(1, L, N, 1e-3), ost.KortewegDeVries(1, L, N, 1e-3, dtype=np.float32)
+    u0, _ = ics(5, 1, L, N, 8, seed=88)
+    uh = osp.fft(u0, num_spatial_dims=1).astype(np.complex64)
+    got = gpu.nonlinear_fun(torch.tensor(uh, device=device)).cpu().numpy()
+    want = cpu.nonlinear_fun(uh)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"  max rel {err:.2e}", flush=True)
+    assert got.shape == want.shape
+    assert err <= 1e-5
+
+
+def test_dlpack_producer_round_trip(device):
+    """Arrays from a DLPack producer that is neither torch nor NumPy (the
+    role of a JAX array): a device producer is used in place (zero copy), a
+    host producer is moved to the device, and both step like the torch
+    tensor they wrap."""
+    import torch
+
+    class Producer:
+        """The DLPack protocol and a shape, as a JAX array has."""
+
+        def __init__(self, t):
+            self._t = t
+            self.shape = tuple(t.shape)
+
+        def __dlpack__(self, stream=None):
+            return self._t.__dlpack__(stream=stream) if stream is not None else self._t.__dlpack__()
+
+        def __dlpack_device__(self):
+            return self._t.__dlpack_device__()
+
+    gpu, _, L = _pair("ks_cons", 256, 2)
+    u0, _ = ics(3, 1, L, 256, 5, seed=41)
+    x = torch.tensor(u0, device=device)
+    want = gpu.step(x).cpu().numpy()
+    from exponax_amd._tensor import as_device_tensor
+
+    t, info = as_device_tensor(Producer(x))
+    assert info[0] == "dlpack" and t.data_ptr() == x.data_ptr()  # no copy of a device array
+    got = gpu.step(Producer(x))
+    got = got.cpu().numpy() if hasattr(got, "cpu") else np.asarray(got)
+    np.testing.assert_array_equal(got, want)
+    got_host = gpu.step(Producer(torch.tensor(u0)))
+    got_host = got_host.cpu().numpy() if hasattr(got_host, "cpu") else np.asarray(got_host)
+    np.testing.assert_array_equal(got_host, want)
+    one = gpu(Producer(x[0]))
+    one = one.cpu().numpy() if hasattr(one, "cpu") else np.asarray(one)
+    np.testing.assert_array_equal(one, want[0])

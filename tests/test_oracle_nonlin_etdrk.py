@@ -25,25 +25,42 @@ def test_convection_single_channel_sine():
     np.testing.assert_allclose(r, -(np.pi / L) * np.sin(2 * 2 * np.pi * g / L), atol=1e-4)
 
 
-@pytest.mark.parametrize("single", [True, False])
-def test_conservative_equals_nonconservative(single):
-    """tests/test_nonlinear_funs.py:55-93 (1-D, smooth input); with 2/3
+@pytest.mark.parametrize("single,D", [(True, 1), (False, 1), (True, 2), (True, 3)])
+def test_conservative_equals_nonconservative(single, D):
+    """tests/test_nonlinear_funs.py:55-93 (smooth input); with 2/3
     dealiasing both forms are the same alias-free convolution, so they agree
-    to rounding for any band-limited input, which the GPU path relies on."""
-    n, L = 64, 4.0
-    d = sp.build_derivative_operator(1, L, n)
+    to rounding for any band-limited input, which the GPU path relies on.
+    In 2-D and 3-D this pins the single-channel non-conservative form
+    (_convection.py:204-231) to the conservative one (:173-202)."""
+    n, L = {1: 64, 2: 32, 3: 16}[D], 4.0
+    d = sp.build_derivative_operator(D, L, n)
     kw = dict(derivative_operator=d, dealiasing_fraction=2 / 3, scale=1.0, single_channel=single)
-    cons = nl.ConvectionNonlinearFun(1, n, conservative=True, **kw)
-    noncons = nl.ConvectionNonlinearFun(1, n, conservative=False, **kw)
-    u = np.random.default_rng(0).standard_normal((1, n))
-    uh = sp.fft(u, num_spatial_dims=1)
-    np.testing.assert_allclose(cons(uh), noncons(uh), atol=1e-12 * np.abs(uh).max() ** 2 * n)
+    cons = nl.ConvectionNonlinearFun(D, n, conservative=True, **kw)
+    noncons = nl.ConvectionNonlinearFun(D, n, conservative=False, **kw)
+    u = np.random.default_rng(0).standard_normal((1,) + (n,) * D)
+    uh = sp.fft(u, num_spatial_dims=D)
+    np.testing.assert_allclose(cons(uh), noncons(uh), atol=1e-12 * np.abs(uh).max() ** 2 * n ** D)
     # without dealiasing the forms differ (aliasing), so the check is not vacuous
-    c0 = nl.ConvectionNonlinearFun(1, n, conservative=True, derivative_operator=d, dealiasing_fraction=1.0,
+    c0 = nl.ConvectionNonlinearFun(D, n, conservative=True, derivative_operator=d, dealiasing_fraction=1.0,
                                    single_channel=single)
-    n0 = nl.ConvectionNonlinearFun(1, n, conservative=False, derivative_operator=d, dealiasing_fraction=1.0,
+    n0 = nl.ConvectionNonlinearFun(D, n, conservative=False, derivative_operator=d, dealiasing_fraction=1.0,
                                    single_channel=single)
     assert np.abs(c0(uh) - n0(uh)).max() > 1e-6
+
+
+def test_single_channel_nonconservative_2d_closed_form():
+    """-u (u_x + u_y) for u = sin(2 pi x / L) + sin(2 pi y / L):
+    -(2 pi / L) (sin a + sin b)(cos a + cos b)."""
+    n, L = 32, 3.0
+    d = sp.build_derivative_operator(2, L, n)
+    f = nl.ConvectionNonlinearFun(2, n, derivative_operator=d, dealiasing_fraction=2 / 3, scale=1.0,
+                                  single_channel=True, conservative=False)
+    g = sp.make_grid(2, L, n)
+    a, b = 2 * np.pi * g[0] / L, 2 * np.pi * g[1] / L
+    u = (np.sin(a) + np.sin(b))[None]
+    r = sp.ifft(f(sp.fft(u, num_spatial_dims=2)), num_spatial_dims=2, num_points=n)
+    want = -(2 * np.pi / L) * (np.sin(a) + np.sin(b)) * (np.cos(a) + np.cos(b))
+    np.testing.assert_allclose(r[0], want, atol=1e-10)
 
 
 def test_gradient_norm_zero_mode_fix():
