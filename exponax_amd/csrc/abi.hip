@@ -1846,6 +1846,126 @@ int exmi_irfft(int32_t D, int32_t N, int32_t count, const float* x_hat, float* x
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "irfft launch failed: %s", hipGetErrorString(e));
 }
 
+}  // extern "C"
+
+// ------------------------------------------- allocation-free inverse transform
+// exmi_irfft_noalloc keeps every buffer it needs beyond the caller's input
+// and output in two file-scope device arrays: `scratch` (the 2-D [ky][x]
+// intermediate, or the copy of the 3-D spectra that the in-place line
+// transforms work on) and `noalloc_twiddles` (the twiddle tables of every
+// transform size, the same values get_twiddles forms). Each GPU holds its
+// own copy of both until the process ends. In host code their names are
+// stand-ins, not device addresses: the set-up takes each device's addresses
+// with hipGetSymbolAddress, and only those are passed to kernels and copies.
+static constexpr size_t kNoAllocScratchBytes = (size_t)EXMI_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static constexpr int kNoAllocSizes[] = {8, 16, 24, 25, 32, 64, 128, 256, 512, 1024, 2048};
+static constexpr int noalloc_tw_total() {
+  int t = 0;
+  for (int m : kNoAllocSizes) t += 2 * m;
+  return t;
+}
+static __device__ __align__(256) cf noalloc_twiddles[noalloc_tw_total()];
+
+struct NoAllocDevice {
+  unsigned char* scratch = nullptr;  // this device's `scratch`
+  cf* tw = nullptr;                  // this device's `noalloc_twiddles`
+};
+static std::mutex g_noalloc_mu;
+static std::map<int, NoAllocDevice> g_noalloc;  // by device
+
+// Once per device, with that device current: the arrays' addresses and the
+// twiddle values (a synchronous host-to-device copy, no allocation).
+static int noalloc_setup(NoAllocDevice* out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  auto it = g_noalloc.find(dev);
+  if (it != g_noalloc.end()) {
+    *out = it->second;
+    return EXMI_OK;
+  }
+  NoAllocDevice d;
+  hipError_t e = hipGetSymbolAddress((void**)&d.scratch, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return fail(EXMI_ERR_HIP, "scratch symbol address: %s", hipGetErrorString(e));
+  e = hipGetSymbolAddress((void**)&d.tw, HIP_SYMBOL(noalloc_twiddles));
+  if (e != hipSuccess) return fail(EXMI_ERR_HIP, "twiddle symbol address: %s", hipGetErrorString(e));
+  std::vector<cf> h(noalloc_tw_total());
+  int off = 0;
+  for (int m : kNoAllocSizes) {
+    twiddle_values(m, &h[off], &h[off + m]);
+    off += 2 * m;
+  }
+  e = hipMemcpy(d.tw, h.data(), sizeof(cf) * h.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(EXMI_ERR_HIP, "twiddle upload: %s", hipGetErrorString(e));
+  g_noalloc[dev] = d;
+  *out = d;
+  return EXMI_OK;
+}
+
+static bool noalloc_twiddles_of(const NoAllocDevice& d, int M, Twiddles* t) {
+  int off = 0;
+  for (int m : kNoAllocSizes) {
+    if (m == M) {
+      t->tw = d.tw + off;
+      t->fw = d.tw + off + m;
+      return true;
+    }
+    off += 2 * m;
+  }
+  return false;
+}
+
+extern "C" {
+
+int exmi_irfft_noalloc(int32_t D, int32_t N, int32_t count, const float* x_hat, float* x, void* stream) {
+  NoAllocDevice nd;
+  int rc = noalloc_setup(&nd);
+  if (rc) return rc;
+  if (count < 0) return fail(EXMI_ERR_INVALID, "exmi_irfft_noalloc: count must be >= 0");
+  if (D < 1 || D > 3) return fail(EXMI_ERR_UNSUPPORTED, "exmi_irfft_noalloc: num_spatial_dims=%d not supported", D);
+  if (count == 0) return EXMI_OK;
+  if (!x_hat || !x) return fail(EXMI_ERR_INVALID, "null argument");
+  const hipStream_t st = (hipStream_t)stream;
+  if (D == 1) {
+    if (!size_1d(N)) return fail(EXMI_ERR_UNSUPPORTED, "exmi_irfft_noalloc: num_points=%d", N);
+    Twiddles tw;
+    if (!noalloc_twiddles_of(nd, N / 2, &tw)) return fail(EXMI_ERR_UNSUPPORTED, "no twiddles for N=%d", N);
+    const hipError_t e = launch_fft1d(N / 2, true, x_hat, x, count, tw.rt(), st);
+    return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "irfft launch failed: %s", hipGetErrorString(e));
+  }
+  if (D == 2) {
+    rc = check_2d_transform(N, count);
+    if (rc) return rc;
+    const size_t need = sizeof(cf) * (size_t)(N / 2 + 1) * N * count;
+    if (need > kNoAllocScratchBytes)
+      return fail(EXMI_ERR_INVALID,
+                  "exmi_irfft_noalloc: %d 2-D fields of N=%d need %zu bytes of scratch, more than its %zu bytes "
+                  "(at most %zu fields per call)",
+                  count, N, need, kNoAllocScratchBytes, kNoAllocScratchBytes / (sizeof(cf) * (N / 2 + 1) * N));
+    Twiddles tr, tc;
+    if (!noalloc_twiddles_of(nd, N / 2, &tr) || !noalloc_twiddles_of(nd, N, &tc))
+      return fail(EXMI_ERR_UNSUPPORTED, "no twiddles for N=%d", N);
+    return transform2d_with(true, N, count, x, (cf*)const_cast<float*>(x_hat), tr, tc, (cf*)nd.scratch, st);
+  }
+  if (!size_3d(N)) return fail(EXMI_ERR_UNSUPPORTED, "exmi_irfft_noalloc: 3-D num_points=%d (powers of two 16..256)", N);
+  const size_t per = sizeof(cf) * (size_t)N * N * (N / 2 + 1);
+  const size_t need = per * count;
+  if (need > kNoAllocScratchBytes)
+    return fail(EXMI_ERR_INVALID,
+                "exmi_irfft_noalloc: %d 3-D fields of N=%d need %zu bytes of scratch, more than its %zu bytes "
+                "(at most %zu fields per call)",
+                count, N, need, kNoAllocScratchBytes, kNoAllocScratchBytes / per);
+  Twiddles tz, tl;
+  if (!noalloc_twiddles_of(nd, N / 2, &tz) || !noalloc_twiddles_of(nd, N, &tl))
+    return fail(EXMI_ERR_UNSUPPORTED, "no twiddles for N=%d", N);
+  // the inverse line transforms run in place: on a copy in the scratch
+  cf* X = (cf*)nd.scratch;
+  hipError_t e = copy_batch(x_hat, 0, (float*)X, 0, (long long)(need / sizeof(float)), 1, st);
+  if (e == hipSuccess) e = irfft3(N, count, X, x, tz, tl, st);
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "3-D irfft launch failed: %s", hipGetErrorString(e));
+}
+
 // ------------------------------------------------------- corrected rollout
 }  // extern "C"
 namespace exmi {

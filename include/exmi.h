@@ -270,6 +270,24 @@ int exmi_rfft(int32_t num_spatial_dims, int32_t num_points, int32_t count, const
 int exmi_irfft(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x_hat, float* x,
                void* stream);
 
+/* exmi_irfft that allocates nothing, for callers that cannot call hipMalloc
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation). It has no counterpart in the reference. Same contract and
+ * results as exmi_irfft; the 2-D [ky][x] intermediate and the copy of the
+ * 3-D spectra (the line transforms run in place) live in a library-owned
+ * device array of EXMI_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends, and a request needing more fails with EXMI_ERR_INVALID before any
+ * work is enqueued (2-D: (N/2+1) N complex64 per field, so 31 fields of
+ * 512^2; 3-D: N^2 (N/2+1) complex64 per field, so 3 fields of 128^3 and
+ * none of 256^3). Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams, other threads) race. The first call on
+ * a device runs a synchronous set-up (symbol addresses, a host-to-device
+ * copy of the twiddle tables); make it outside graph capture, e.g. with
+ * count = 0. */
+#define EXMI_NOALLOC_SCRATCH_BYTES (32u << 20)
+int exmi_irfft_noalloc(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x_hat, float* x,
+                       void* stream);
+
 /* Normalized RMSE of B state pairs: out[b] = sum_c ||pred_bc - ref_bc||_2 /
  * ||ref_bc||_2 over the n = N^D points of each channel (the (L/N)^D factor
  * cancels), float64 accumulation, float32 result. pred/ref (B, C, n) with
