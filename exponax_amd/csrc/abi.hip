@@ -18,6 +18,7 @@
 #include "fft1d.hpp"
 #include "spectral3d.hpp"
 #include "step1d.hpp"
+#include "roll2d.hpp"
 #include "step2d.hpp"
 
 namespace exmi {
@@ -69,12 +70,25 @@ Launch2D get_2d_64(int kind, int is_row, int mode);
 Launch2D get_2d_128(int kind, int is_row, int mode);
 Launch2D get_2d_256(int kind, int is_row, int mode);
 Launch2D get_2d_512(int kind, int is_row, int mode);
+using LaunchRoll2D = hipError_t (*)(const Roll2DArgs&, hipStream_t);
+LaunchRoll2D get_roll2d_32(int kind);
+LaunchRoll2D get_roll2d_64(int kind);
+LaunchRoll2D get_roll2d_128(int kind);
 using LaunchAdj2D = hipError_t (*)(const Adj2DArgs&, int, hipStream_t);
 LaunchAdj2D get_adj2d_32(int is_row, int mode);
 LaunchAdj2D get_adj2d_64(int is_row, int mode);
 LaunchAdj2D get_adj2d_128(int is_row, int mode);
 LaunchAdj2D get_adj2d_256(int is_row, int mode);
 LaunchAdj2D get_adj2d_512(int is_row, int mode);
+
+static LaunchRoll2D get_roll2d(int N, int kind) {
+  switch (N) {
+    case 32: return get_roll2d_32(kind);
+    case 64: return get_roll2d_64(kind);
+    case 128: return get_roll2d_128(kind);
+    default: return nullptr;
+  }
+}
 
 static LaunchAdj2D get_adj2d(int N, int is_row, int mode) {
   switch (N) {
@@ -330,7 +344,9 @@ struct exmi_plan {
   cf *N03 = nullptr, *NC3 = nullptr, *S3 = nullptr, *S13 = nullptr, *A3 = nullptr, *F3 = nullptr;
   float *R3 = nullptr, *PR3 = nullptr;
   int nf3 = 0, np3 = 0, kind3 = 0;
-  Launch2D row2d[3] = {nullptr, nullptr, nullptr};
+  Launch2D row2d[ROW_NUM_MODES] = {};
+  LaunchRoll2D roll2d = nullptr;  // 2-D workgroup-resident rollout (strategy 1)
+  int fr2d = 0;                   // 2-D spectrum-resident launch chain available (strategy 2)
   Launch2D col2d[COL_NUM_MODES] = {};
   Twiddles tw_col;
   cf *S0 = nullptr, *N0 = nullptr, *S1 = nullptr, *F = nullptr, *T2 = nullptr, *ACC = nullptr;
@@ -565,6 +581,23 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   p->kscale = (float)(2.0 * 3.14159265358979323846 / d->domain_extent);
   p->roll_nt = -1;
   hipGetDevice(&p->device);
+  // Workgroup-resident ETDRK2 rollouts (one workgroup keeps an instance's
+  // spectrum on chip for all steps): N <= 128, single-channel gradient norm
+  // or vorticity, real L, dealias cutoff up to the default 2/3 one (the
+  // Nyquist row and column then never reach the nonlinear term, so the
+  // per-step forward transform of the emitted state is an exact identity),
+  // forcing below the Nyquist column. Other plans, and every plan with
+  // EXMI_FLAG_PHYSICAL_ROLLOUT, step through the row/column launches.
+  if (d->order == 2 && !cplx && C == 1 && N <= 128 && !(d->flags & EXMI_FLAG_PHYSICAL_ROLLOUT) &&
+      (d->nonlin_kind == EXMI_NL_GRADIENT_NORM || kind == NL_VORT) && p->kmax >= 0 &&
+      p->kmax <= (2 * (N / 2) - 3) / 3 && d->injection_mode < N / 2)
+    p->roll2d = get_roll2d(N, kind);
+  // The same exactness lets the row/column launches carry S' in the column
+  // layout from step to step (ETDRK2, every kind: the products of real
+  // fields have Hermitian transforms): four launches per step instead of
+  // six (no forward transform of the emitted state).
+  p->fr2d = d->order == 2 && !cplx && p->kmax >= 0 && p->kmax < N / 2 && !(d->flags & EXMI_FLAG_PHYSICAL_ROLLOUT);
+  if (p->roll2d || p->fr2d) p->roll_nt = 0;  // the autotuner picks among them and the per-step launches
   if (d->nonlin_kind == EXMI_NL_KOLMOGOROV) {
     // _vorticity_convection.py:166-176: -k_f gamma scaling(coef_extraction)
     // at wavenumber (0, k_f); scaling = N at k = 0 or N/2, else N/2 per axis
@@ -575,10 +608,11 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
       p->inj_value = (float)(-(double)kf * d->injection_scale * (double)N * sy);
     }
   }
-  for (int m = 0; m < 3; ++m) p->row2d[m] = get_2d(N, kind, 1, m);
+  for (int m = 0; m < ROW_NUM_MODES; ++m) p->row2d[m] = get_2d(N, kind, 1, m);
   for (int m = 0; m < COL_NUM_MODES; ++m) p->col2d[m] = get_2d(N, kind, 0, m);
-  for (int m = 0; m < COL_FWD; ++m) {
-    if ((m < 3 && !p->row2d[m]) || !p->col2d[m]) {
+  for (int m = 0; m < COL_NUM_MODES; ++m) {
+    if (m == COL_FWD || m == COL_INV) continue;  // the standalone transforms' modes
+    if ((m < ROW_NUM_MODES && !p->row2d[m]) || !p->col2d[m]) {
       delete p;
       return fail(EXMI_ERR_UNSUPPORTED, "no 2-D kernel for N=%d", N);
     }
@@ -1198,14 +1232,25 @@ int exmi_plan_destroy(exmi_plan* p) {
   return EXMI_OK;
 }
 
+// Strategies a plan can roll out with: -1 per-step launches (not for plans
+// forced to keep the spectrum on chip), 1 and 2 the multi-step kernels (2-D:
+// 1 = the workgroup-resident kernel).
+static bool strategy_ok(const exmi_plan* p, int nt) {
+  if (nt == -1) return !(p->fourier_resident && !p->fr_auto);
+  if (p->ndim == 2) return (nt == 1 && p->roll2d != nullptr) || (nt == 2 && p->fr2d);
+  if (p->ndim == 3) return false;
+  return (nt == 1 || nt == 2) && p->roll[nt] != nullptr;
+}
+
 int exmi_plan_rollout_strategy(const exmi_plan* p) { return p ? p->roll_nt : fail(EXMI_ERR_INVALID, "null plan"); }
 
 int exmi_plan_set_rollout_strategy(exmi_plan* p, int32_t nt) {
   if (!p) return fail(EXMI_ERR_INVALID, "null plan");
   if (nt == -1 && p->fourier_resident && !p->fr_auto)
     return fail(EXMI_ERR_INVALID, "Fourier-resident plans roll out with the multi-step kernel only");
-  if (!(nt == -1 || nt == 0 || ((nt == 1 || nt == 2) && p->roll[nt])))
+  if (!(nt == 0 || strategy_ok(p, nt)))
     return fail(EXMI_ERR_UNSUPPORTED, "rollout strategy %d not available for this plan", nt);
+  if (nt == 0 && p->ndim != 1 && !p->roll2d && !p->fr2d) nt = -1;  // nothing to choose from
   p->roll_nt = nt;
   if (p->gc.exec) {
     hipGraphExecDestroy(p->gc.exec);
@@ -1323,6 +1368,59 @@ static int enqueue_roll(exmi_plan* p, int nt, const float* u0, long long u0_bs, 
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "rollout launch failed: %s", hipGetErrorString(e));
 }
 
+// 2-D workgroup-resident rollout: one launch for all n steps, instance i's
+// states to out + i * out_bs + t * out_ts (only the last with final_only).
+static int enqueue_roll2d(exmi_plan* p, const float* u0, long long u0_bs, float* out, long long out_bs,
+                          long long out_ts, int n, int final_only, hipStream_t st) {
+  Roll2DArgs ra;
+  fill_args2d(p, &ra.s);
+  ra.u0 = u0;
+  ra.u0_bstride = u0_bs;
+  ra.traj = out;
+  ra.traj_bstride = out_bs;
+  ra.traj_tstride = out_ts;
+  ra.nsteps = n;
+  ra.final_only = final_only;
+  ra.ninst = p->ninst;
+  const hipError_t e = p->roll2d(ra, st);
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "2-D rollout launch failed: %s", hipGetErrorString(e));
+}
+
+// 2-D spectrum-resident launch chain: RA CA once, then per step
+// RB|RCB CB RB CCA, and a final RC. RCB emits the previous step's state
+// while forming the products; CCA keeps S' as the next S0 with its fields.
+// States go to out + i * out_bs + t * out_ts (only the last, at out, with
+// final_only).
+static int enqueue_chain_fr2d(exmi_plan* p, const float* u0, long long u0_bs, float* out, long long out_bs,
+                              long long out_ts, int n, int final_only, hipStream_t st) {
+  Step2DArgs a;
+  fill_args2d(p, &a);
+  a.u_in = u0;
+  a.u_in_bstride = u0_bs;
+  a.u_out_bstride = out_bs;
+  a.write_T = 0;
+  const int B = p->ninst;
+  hipError_t e = p->row2d[ROW_RA](a, B, st);
+  if (e == hipSuccess) e = p->col2d[COL_CA](a, B, st);
+  for (int t = 0; t < n && e == hipSuccess; ++t) {
+    if (t == 0 || final_only) {
+      e = p->row2d[ROW_RB](a, B, st);
+    } else {
+      a.u_out = out + (long long)(t - 1) * out_ts;
+      e = p->row2d[ROW_RCB](a, B, st);
+    }
+    if (e == hipSuccess) e = p->col2d[COL_CB](a, B, st);
+    if (e == hipSuccess) e = p->row2d[ROW_RB](a, B, st);
+    a.next_fields = t < n - 1 ? 1 : 0;
+    if (e == hipSuccess) e = p->col2d[COL_CCA](a, B, st);
+  }
+  if (e == hipSuccess) {
+    a.u_out = final_only ? out : out + (long long)(n - 1) * out_ts;
+    e = p->row2d[ROW_RC](a, B, st);
+  }
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "2-D rollout launch failed: %s", hipGetErrorString(e));
+}
+
 // Rollout as per-step launches (nt <= 0) or one multi-step launch carrying
 // nt transforms per lane group.
 static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u0_bs, float* traj, int n,
@@ -1346,6 +1444,8 @@ static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u
     return EXMI_OK;
   }
   if (p->ndim == 2) {
+    if (nt == 1 && p->roll2d) return enqueue_roll2d(p, u0, u0_bs, traj + off * sz, tstride, sz, n, 0, st);
+    if (nt == 2 && p->fr2d) return enqueue_chain_fr2d(p, u0, u0_bs, traj + off * sz, tstride, sz, n, 0, st);
     for (int t = 0; t < n; ++t) {
       int rc = enqueue_step2d(p, u0, u0_bs, traj + (long long)(t + off) * sz, tstride, t > 0, t < n - 1, st);
       if (rc) return rc;
@@ -1400,7 +1500,7 @@ static bool tune_lookup(exmi_plan* p) {
   int v = 0;
   bool found = false;
   while (fscanf(f, "%255s %d", k, &v) == 2) {  // the last entry for a key wins
-    if (key == k && (v == -1 || ((v == 1 || v == 2) && p->roll[v])) && !(v == -1 && p->fourier_resident && !p->fr_auto)) {
+    if (key == k && strategy_ok(p, v)) {
       p->roll_nt = v;
       found = true;
     }
@@ -1425,7 +1525,7 @@ static int autotune_rollout(exmi_plan* p, const float* u0, long long u0_bs, floa
   if (tune_lookup(p)) return EXMI_OK;
   const int probe = n < 16 ? n : 16;
   if (probe == 0) {
-    p->roll_nt = p->roll[2] ? 2 : 1;
+    p->roll_nt = strategy_ok(p, 2) ? 2 : 1;
     return EXMI_OK;
   }
   hipStream_t st = p->capture_stream;
@@ -1437,8 +1537,7 @@ static int autotune_rollout(exmi_plan* p, const float* u0, long long u0_bs, floa
   const int cands[3] = {-1, 1, 2};
   for (int ci = 0; ci < 3; ++ci) {
     const int nt = cands[ci];
-    if (nt < 0 && p->fourier_resident && !p->fr_auto) continue;
-    if (nt > 0 && !p->roll[nt]) continue;
+    if (!strategy_ok(p, nt)) continue;
     float tmin = 1e30f;
     for (int rep = 0; rep < 4; ++rep) {  // the first launch of each candidate is not timed
       HIP_TRY(hipEventRecord(e0, st));
@@ -1528,6 +1627,9 @@ static int enq_repeat_cb(exmi_plan* p, const void* a, const void* b, long long s
     return EXMI_OK;
   }
   if (p->ndim == 2) {
+    const int nt2 = p->roll_nt != 0 ? p->roll_nt : (p->fr2d ? 2 : -1);
+    if (nt2 == 1 && p->roll2d) return enqueue_roll2d(p, u0, s, out, sz, 0, n, 1, st);
+    if (nt2 == 2 && p->fr2d) return enqueue_chain_fr2d(p, u0, s, out, sz, 0, n, 1, st);
     for (int t = 0; t < n; ++t) {
       int rc = enqueue_step2d(p, u0, s, out, sz, t > 0, t < n - 1, st);
       if (rc) return rc;
@@ -1608,7 +1710,7 @@ int exmi_rollout_group(exmi_plan* const* plans, int32_t np, const float* const* 
     if (!plans[i] || !u0s[i] || !trajs[i]) return fail(EXMI_ERR_INVALID, "null plan or buffer in group");
     for (int k = 0; k < i; ++k)
       if (plans[k] == plans[i]) return fail(EXMI_ERR_INVALID, "a plan appears twice in the group");
-    if (plans[i]->ndim == 1 && plans[i]->roll_nt == 0) {
+    if (plans[i]->roll_nt == 0) {
       HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
       int rc = autotune_rollout(plans[i], u0s[i], u0_bstrides[i], trajs[i], n, include_init);
       if (rc) return rc;

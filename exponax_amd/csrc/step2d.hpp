@@ -100,11 +100,15 @@ struct Step2DArgs {
   cf* spec;                  // standalone transforms: spectra (ninst, N, N/2+1), [kx][ky]
   cf* spec_out;              // exmi_nonlinear: N_hat (ninst, N, N/2+1), [kx][ky]
   long long spec_bstride;    // complex entries between instances
+  int next_fields;           // COL_CCA: also form the fields of S' (the next step's first evaluation)
   int spec_io;               // exmi_step_fourier: COL_C0 reads u_hat from spec, and the last stage
                              // (C0, C1, CC, C3C, C4D) writes u_hat' to spec_out, both [kx][ky]
 };
 
-// CA/CB/CC: the ETDRK2 stages; C0: ETDRK0 (S' = E S); C1: ETDRK1's update
+// CA/CB/CC: the ETDRK2 stages; CCA: CC that also keeps S' as the next
+// step's S0 and forms its fields (the spectrum-resident chain, where the
+// next step's forward transform of the emitted state is an exact identity
+// and is skipped: CA of the next step fused into CC); C0: ETDRK0 (S' = E S); C1: ETDRK1's update
 // (S' = E S0 + c1 N0, etdrk/_etdrk_1.py:77-82) after CA and RB.
 // C3A..C3C: the three ETDRK3 stages after CA (etdrk/_etdrk_3.py:191-212),
 // C4A..C4D: the four ETDRK4 stages after CA (etdrk/_etdrk_4.py:198-224); each
@@ -115,9 +119,12 @@ enum Col2DMode {
   COL_C3A = 5, COL_C3B = 6, COL_C3C = 7, COL_C4A = 8, COL_C4B = 9, COL_C4C = 10, COL_C4D = 11,
   COL_FWD = 12, COL_INV = 13,
   COL_CA_SPEC = 14, COL_NOUT = 15,
-  COL_NUM_MODES = 16
+  COL_CCA = 16,
+  COL_NUM_MODES = 17
 };
-enum Row2DMode { ROW_RA = 0, ROW_RB = 1, ROW_RC = 2 };
+// ROW_RCB: RC (the inverse row transforms of T into u_out) then RB (the
+// products of the fields F into T), in one pass over the rows.
+enum Row2DMode { ROW_RA = 0, ROW_RB = 1, ROW_RC = 2, ROW_RCB = 3, ROW_NUM_MODES = 4 };
 
 // Nonlinear fields of one evaluation. Vorticity: (u, omega_x, v, omega_y),
 // product u omega_x + v omega_y. Gradient norm: (u_x, u_y), product
@@ -260,7 +267,30 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
     row_to_tile<N>(lds_t, r, j, z, nyq);
     __syncthreads();
     tile_store<N>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
-  } else if constexpr (MODE == ROW_RB) {
+  }
+  if constexpr (MODE == ROW_RC || MODE == ROW_RCB) {
+    tile_load<N>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
+    __syncthreads();
+    tile_to_row<N>(lds_t, r, j, z, nyq);
+    irfft<M, PT, 1>(z, nyq, ln, lds, tr);
+#pragma unroll
+    for (int s = 0; s < PT; ++s) z[0][s] = cf{z[0][s].re * rn2, z[0][s].im * rn2};
+    float2* u = reinterpret_cast<float2*>(a.u_out + (long long)traj * a.u_out_bstride + urow);
+#pragma unroll
+    for (int s = 0; s < PT; ++s) u[j + s * T] = make_float2(z[0][s].re, z[0][s].im);
+    if (MODE == ROW_RC && a.write_T) {
+      // the next step's forward transform of the float32 state just written
+      rfft<M, PT, 1>(z, nyq, ln, lds, tr);
+      __syncthreads();
+      row_to_tile<N>(lds_t, r, j, z, nyq);
+      __syncthreads();
+      tile_store<N>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
+    }
+    // RCB: the products below overwrite T's rows ky < kc of the same x
+    // range, which every thread of the workgroup has read into LDS by now
+    if constexpr (MODE == ROW_RCB) __syncthreads();
+  }
+  if constexpr (MODE == ROW_RB || MODE == ROW_RCB) {
     using FD = Fields2D<KIND>;
     constexpr int NF = FD::NF, NIN = FD::NIN, NP = FD::NP;
     const long long fsz = (long long)kc * N;
@@ -408,24 +438,6 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
       row_to_tile<N>(lds_t, r, j, z, nyq);
       __syncthreads();
       tile_store<N>(lds_t, a.T + (long long)inst * fstride + p * fsz, x0, kc);
-    }
-  } else {
-    tile_load<N>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
-    __syncthreads();
-    tile_to_row<N>(lds_t, r, j, z, nyq);
-    irfft<M, PT, 1>(z, nyq, ln, lds, tr);
-#pragma unroll
-    for (int s = 0; s < PT; ++s) z[0][s] = cf{z[0][s].re * rn2, z[0][s].im * rn2};
-    float2* u = reinterpret_cast<float2*>(a.u_out + (long long)traj * a.u_out_bstride + urow);
-#pragma unroll
-    for (int s = 0; s < PT; ++s) u[j + s * T] = make_float2(z[0][s].re, z[0][s].im);
-    if (a.write_T) {
-      // the next step's forward transform of the float32 state just written
-      rfft<M, PT, 1>(z, nyq, ln, lds, tr);
-      __syncthreads();
-      row_to_tile<N>(lds_t, r, j, z, nyq);
-      __syncthreads();
-      tile_store<N>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
     }
   }
 }
@@ -792,6 +804,9 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
       for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
     }
   } else {
+    // CC: S' = S1 + c2 (N1 - N0) -> inverse column transform -> T. CCA
+    // also keeps S' as the next step's S0 and, with next_fields, forms
+    // its fields (the next step's CA)
     cf Nh[PT], v[1][PT];
     nonlinear(Nh);
     const cf* s1 = a.S1 + (long long)inst * full + (long long)ky * N;
@@ -800,6 +815,14 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     for (int s = 0; s < PT; ++s) {
       const cf N0s = kept_col ? n0[j + s * T] : forcing(s);
       v[0][s] = s1[j + s * T] + coef(2, s) * (Nh[s] - N0s);
+    }
+    if constexpr (MODE == COL_CCA) {
+      if (active) {
+        cf* s0 = a.S0 + (long long)inst * full + (long long)ky * N;
+#pragma unroll
+        for (int s = 0; s < PT; ++s) s0[j + s * T] = v[0][s];
+      }
+      if (a.next_fields) fields_out(v[0]);
     }
     finish(v);
   }

@@ -2,6 +2,7 @@
 // compile time with -DEXMI_N2=<N>. The build compiles this file once per
 // supported size.
 #include "adjoint2d.hpp"
+#include "roll2d.hpp"
 #include "step2d.hpp"
 
 #ifndef EXMI_N2
@@ -22,6 +23,7 @@ static Launch2D pick2d(int is_row, int mode) {
       case ROW_RA: return &launch_row2d<N, ROW_RA, KIND>;
       case ROW_RB: return &launch_row2d<N, ROW_RB, KIND>;
       case ROW_RC: return &launch_row2d<N, ROW_RC, KIND>;
+      case ROW_RCB: return &launch_row2d<N, ROW_RCB, KIND>;
     }
   } else {
     switch (mode) {
@@ -42,6 +44,7 @@ static Launch2D pick2d(int is_row, int mode) {
       case COL_INV: return KIND == NL_GRADNORM ? &launch_col2d<N, COL_INV, NL_GRADNORM> : nullptr;
       case COL_CA_SPEC: return &launch_col2d<N, COL_CA_SPEC, KIND>;
       case COL_NOUT: return &launch_col2d<N, COL_NOUT, KIND>;
+      case COL_CCA: return &launch_col2d<N, COL_CCA, KIND>;
     }
   }
   return nullptr;
@@ -78,6 +81,19 @@ Launch2D EXMI_CAT(get_2d_, EXMI_N2)(int kind, int is_row, int mode) {
   if (kind == NL_GENERAL) return pick2d<N, NL_GENERAL>(is_row, mode);
   if (kind == NL_GRAY_SCOTT) return pick2d<N, NL_GRAY_SCOTT>(is_row, mode);
   if (kind == NL_BZ) return pick2d<N, NL_BZ>(is_row, mode);
+  return nullptr;
+}
+
+using LaunchRoll2D = hipError_t (*)(const Roll2DArgs&, hipStream_t);
+
+// workgroup-resident ETDRK2 rollouts (N <= 128; gradient norm and vorticity)
+LaunchRoll2D EXMI_CAT(get_roll2d_, EXMI_N2)(int kind) {
+  constexpr int N = EXMI_N2;
+  if constexpr (N <= 128) {
+    if (kind == NL_VORT) return &launch_roll2d<N, NL_VORT>;
+    if (kind == NL_GRADNORM) return &launch_roll2d<N, NL_GRADNORM>;
+  }
+  (void)kind;
   return nullptr;
 }
 
