@@ -177,6 +177,27 @@ class BaseStepper:
                                            int(use_graph), _lib.stream_ptr()), "exmi_rollout")
         return restore_like(out, back)
 
+    def rollout_checked(self, u0, n: int, *, bound: float = 0.0, include_init: bool = False):
+        """``rollout_batch`` with failure detection: returns (trajectory,
+        first_bad) where first_bad (B,) int32 holds, per trajectory, the first
+        time index whose state has a non-finite value or one above `bound`
+        in magnitude (bound <= 0: non-finite only), -1 if none
+        (exmi_rollout_checked)."""
+        import torch
+
+        t, back = as_device_tensor(u0)
+        lead, B = self._split(t, self.state_shape)
+        x = t.contiguous().to(torch.float32)
+        T = n + (1 if include_init else 0)
+        out = torch.empty(lead + (T,) + self.state_shape, dtype=torch.float32, device=x.device)
+        fb = torch.empty(B, dtype=torch.int32, device=x.device)
+        sz = int(np.prod(self.state_shape))
+        p = self.plan(B)
+        _lib.check(_lib.lib().exmi_rollout_checked(p.handle, x.data_ptr(), sz, out.data_ptr(), n, int(include_init),
+                                                   float(bound), fb.data_ptr(), _lib.stream_ptr()),
+                   "exmi_rollout_checked")
+        return restore_like(out, back), restore_like(fb.reshape(lead) if lead else fb[0], back)
+
     def repeat_batch(self, u0, n: int, *, use_graph: bool = True):
         import torch
 

@@ -44,7 +44,7 @@ EXPORTED = (
     "exmi_step_vjp", "exmi_rollout_vjp", "exmi_nrmse", "exmi_spectrum", "exmi_spectrum_binned",
     "exmi_map_resolution", "exmi_ic_tfs", "exmi_wave_step_fourier", "exmi_poisson_fourier", "exmi_axpy",
     "exmi_cnn_workspace_bytes", "exmi_cnn_forward", "exmi_cnn_backward", "exmi_corrected_rollout",
-    "exmi_irfft_noalloc",
+    "exmi_irfft_noalloc", "exmi_rollout_checked", "exmi_first_bad_step",
 )
 EXMI_NOALLOC_SCRATCH_BYTES = 32 << 20
 EXMI_CNN_NUM_PARAMS = 15809
@@ -105,6 +105,8 @@ def _declare(lib):
     lib.exmi_rfft.argtypes = [i32, i32, i32, vp, vp, vp]
     lib.exmi_irfft.argtypes = [i32, i32, i32, vp, vp, vp]
     lib.exmi_irfft_noalloc.argtypes = [i32, i32, i32, vp, vp, vp]
+    lib.exmi_rollout_checked.argtypes = [vp, vp, i64, vp, i32, i32, C.c_float, vp, vp]
+    lib.exmi_first_bad_step.argtypes = [i32, i32, i64, vp, i64, i64, C.c_float, vp, vp]
     lib.exmi_plan_num_coefs.argtypes = [vp]
     lib.exmi_plan_get_tables.argtypes = [vp, vp, i64]
     lib.exmi_plan_rollout_strategy.argtypes = [vp]

@@ -201,13 +201,18 @@ def rollout_with_fallback(corrected: CorrectedStepper, fine, u0_fine, n_coarse: 
     mask of the trajectories that fell back)."""
     from .spectral import map_between_resolutions
 
+    from ._base_stepper import Ensemble
+    from .metrics import first_bad_step
+
     nc = corrected.coarse.num_points
     u0c = map_between_resolutions(u0_fine, nc) if u0_coarse is None else u0_coarse
     tr = corrected.rollout_batch(u0c, n_coarse)
-    flat = tr.reshape(tr.shape[0], -1)
-    bad = ~(torch.isfinite(flat).all(dim=1) & (flat.abs() <= bound).all(dim=1))
+    bad = first_bad_step(tr, bound) >= 0  # exmi_first_bad_step on the device
     if bool(bad.any()):
         idx = torch.nonzero(bad).flatten()
-        ft = downsampled_trajectories(fine, u0_fine[idx].contiguous(), n_coarse, stride, nc)
+        sub = fine
+        if isinstance(fine, Ensemble):  # the failed trajectories' own members (per-trajectory coefficients)
+            sub = Ensemble([fine.members[i] for i in idx.tolist()])
+        ft = downsampled_trajectories(sub, u0_fine[idx].contiguous(), n_coarse, stride, nc)
         tr[idx] = ft[:, 1:]
     return tr, bad

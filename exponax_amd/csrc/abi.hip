@@ -12,6 +12,8 @@
 #include <string>
 #include <vector>
 
+#include <climits>
+
 #include "exmi_internal.hpp"
 #include "adjoint1d.hpp"
 #include "adjoint2d.hpp"
@@ -1271,7 +1273,8 @@ int exmi_plan_get_tables(const exmi_plan* p, float* host_out, int64_t host_count
 }
 
 static int enqueue_roll(exmi_plan* p, int nt, const float* u0, long long u0_bs, float* out, long long out_bs,
-                        long long out_ts, int n, int final_only, hipStream_t st);
+                        long long out_ts, int n, int final_only, hipStream_t st, int* first_bad = nullptr,
+                        float bound = 0.f, int bad_offset = 0);
 
 static int run_step(exmi_plan* p, const void* in, long long in_bs, int in_f, void* out, long long out_bs, int out_f,
                     int nonlin_only, hipStream_t st) {
@@ -1354,9 +1357,13 @@ static hipError_t copy_batch(const float* src, long long ss, float* dst, long lo
 }
 
 static int enqueue_roll(exmi_plan* p, int nt, const float* u0, long long u0_bs, float* out, long long out_bs,
-                        long long out_ts, int n, int final_only, hipStream_t st) {
+                        long long out_ts, int n, int final_only, hipStream_t st, int* first_bad,
+                        float bound, int bad_offset) {
   Roll1DArgs ra;
   fill_args(p, &ra.s);
+  ra.first_bad = first_bad;
+  ra.bound = bound;
+  ra.bad_offset = bad_offset;
   ra.s.in = u0;
   ra.s.in_bstride = u0_bs;
   ra.traj = out;
@@ -1672,6 +1679,66 @@ int exmi_rollout_prepare(exmi_plan* p, const float* u0, int64_t u0_bs, float* tr
     if (rc) return rc;
   }
   return run_maybe_graph(p, u0, traj, u0_bs, n, include_init ? 1 : 0, 1, 1, nullptr, enq_rollout_cb, false);
+}
+
+// ---- rollouts with failure detection (exmi_rollout_checked)
+__global__ void exmi_fill_int_kernel(int* p, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void exmi_bad_finalize_kernel(int* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (p[i] == INT_MAX) p[i] = -1;
+}
+static unsigned small_grid(int n) { return (unsigned)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024); }
+
+}  // extern "C"
+namespace exmi {
+int launch_first_bad(int batch, int n_times, long long n, const float* x, long long bs, long long ts, float bound,
+                     int offset, int* first_bad, hipStream_t st);
+}
+extern "C" {
+
+int exmi_rollout_checked(exmi_plan* p, const float* u0, int64_t u0_bs, float* traj, int32_t n, int32_t include_init,
+                         float bound, int32_t* first_bad, void* stream) {
+  if (!p || !u0 || !traj || !first_bad) return fail(EXMI_ERR_INVALID, "null argument");
+  if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
+  hipStream_t st = (hipStream_t)stream;
+  if (p->roll_nt == 0) {
+    HIP_TRY(hipStreamSynchronize(st));
+    int rc = autotune_rollout(p, u0, u0_bs, traj, n, include_init);
+    if (rc) return rc;
+  }
+  const int B = p->d.batch;
+  const float bnd = bound > 0.f ? bound : 3.402823466e38f;  // non-finite values fail every bound
+  const long long sz = state_size(p);
+  const int off = include_init ? 1 : 0;
+  const long long tstride = (long long)(n + off) * sz;
+  if (n == 0) {
+    hipLaunchKernelGGL(exmi_fill_int_kernel, dim3(small_grid(B)), dim3(256), 0, st, (int*)first_bad, B, -1);
+    HIP_TRY(hipGetLastError());
+    return enqueue_rollout_nt(p, p->roll_nt, u0, u0_bs, traj, n, include_init, st);
+  }
+  int nt = p->roll_nt;
+  if (p->ndim == 1 && p->fourier_resident && (nt == 0 || (nt < 0 && !p->fr_auto))) nt = p->roll[2] ? 2 : 1;
+  if (p->ndim == 1 && nt > 0 && p->roll[nt]) {
+    // the multi-step kernel checks every state it writes
+    if (include_init) {
+      hipError_t e = copy_batch(u0, u0_bs, traj, tstride, sz, B, st);
+      if (e != hipSuccess) return fail(EXMI_ERR_HIP, "copy launch failed: %s", hipGetErrorString(e));
+    }
+    hipLaunchKernelGGL(exmi_fill_int_kernel, dim3(small_grid(B)), dim3(256), 0, st, (int*)first_bad, B, INT_MAX);
+    HIP_TRY(hipGetLastError());
+    int rc = enqueue_roll(p, nt, u0, u0_bs, traj + off * sz, tstride, sz, n, 0, st, (int*)first_bad, bnd, off);
+    if (rc) return rc;
+    hipLaunchKernelGGL(exmi_bad_finalize_kernel, dim3(small_grid(B)), dim3(256), 0, st, (int*)first_bad, B);
+    HIP_TRY(hipGetLastError());
+    return EXMI_OK;
+  }
+  // other strategies and dimensions: the rollout, then one scan of the
+  // states it wrote
+  int rc = enqueue_rollout_nt(p, p->roll_nt, u0, u0_bs, traj, n, include_init, st);
+  if (rc) return rc;
+  return launch_first_bad(B, n, sz, traj + off * sz, tstride, sz, bnd, off, (int*)first_bad, st);
 }
 
 // ---- grouped rollouts (see exmi_rollout_group)

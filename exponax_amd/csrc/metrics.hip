@@ -1,6 +1,8 @@
 // GPU counterparts of the functions on either side of the ETDRK path that
 // measure states or change their resolution:
 //   nRMSE                    metrics/_spatial.py:537-586 (spatial_norm, mode "normalized")
+//   first non-finite / out-of-bound state of every trajectory (failure
+//   detection; no reference counterpart)
 //   get_spectrum (1-D)       _spectral.py:866-1030
 //   map_between_resolutions  _interpolation.py:195-285 (1-D)
 // Transforms go through the library's own real FFT kernels (exmi_rfft /
@@ -197,6 +199,34 @@ __global__ void resample_nd_kernel(const float2* __restrict__ A, int No, float2*
   }
 }
 
+// first_bad[b] = the smallest t < T whose state x[b, t, 0:n] has a value
+// that is not finite or exceeds `bound` in magnitude, -1 when there is none.
+// One workgroup per trajectory; every thread keeps the smallest bad t of the
+// elements it reads, then a min-reduction.
+__global__ void __launch_bounds__(WG) first_bad_kernel(const float* __restrict__ x, long long bs, long long ts,
+                                                       int T, long long n, float bound, int offset,
+                                                       int* __restrict__ first_bad) {
+  __shared__ int sh[WG];
+  const int b = blockIdx.x;
+  int best = T;
+  for (int t = 0; t < T; ++t) {
+    const float* xs = x + (long long)b * bs + (long long)t * ts;
+    bool bad = false;
+    for (long long i = threadIdx.x; i < n; i += WG) bad |= !(fabsf(xs[i]) <= bound);
+    if (bad) {  // this thread's first failing time; the minimum over the threads is the trajectory's
+      best = t;
+      break;
+    }
+  }
+  sh[threadIdx.x] = best;
+  __syncthreads();
+  for (int w = WG / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w && sh[threadIdx.x + w] < sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) first_bad[b] = sh[0] < T ? offset + sh[0] : -1;
+}
+
 int grid_for(long long n) {
   long long g = (n + WG - 1) / WG;
   if (g > 16384) g = 16384;
@@ -206,6 +236,16 @@ int grid_for(long long n) {
 bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 }  // namespace
+
+namespace exmi {
+int launch_first_bad(int batch, int n_times, long long n, const float* x, long long bs, long long ts, float bound,
+                     int offset, int* first_bad, hipStream_t st) {
+  if (batch == 0) return EXMI_OK;
+  hipLaunchKernelGGL(first_bad_kernel, dim3(batch), dim3(WG), 0, st, x, bs, ts, n_times, n, bound, offset, first_bad);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? EXMI_OK : set_error(EXMI_ERR_HIP, hipGetErrorString(e));
+}
+}  // namespace exmi
 
 extern "C" {
 
@@ -218,6 +258,16 @@ int exmi_nrmse(int32_t batch, int32_t channels, int64_t n, const float* pred, in
                      ref, (long long)ref_bstride, channels, (long long)n, out);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? EXMI_OK : exmi::set_error(EXMI_ERR_HIP, hipGetErrorString(e));
+}
+
+int exmi_first_bad_step(int32_t batch, int32_t n_times, int64_t n, const float* x, int64_t bstride, int64_t tstride,
+                        float bound, int32_t* first_bad, void* stream) {
+  if (!x || !first_bad) return exmi::set_error(EXMI_ERR_INVALID, "exmi_first_bad_step: null argument");
+  if (batch < 0 || n_times < 0 || n < 1) return exmi::set_error(EXMI_ERR_INVALID, "exmi_first_bad_step: bad sizes");
+  if (batch == 0) return EXMI_OK;
+  const float b = bound > 0.f ? bound : 3.402823466e38f;  // non-finite values fail every bound
+  return exmi::launch_first_bad(batch, n_times, (long long)n, x, (long long)bstride, (long long)tstride, b, 0,
+                                (int*)first_bad, (hipStream_t)stream);
 }
 
 int exmi_spectrum_binned(int32_t D, int32_t N, int32_t count, const float* x, float* out, int32_t power,

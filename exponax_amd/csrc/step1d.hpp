@@ -19,6 +19,7 @@
 // _etdrk_2.py:91-102, _etdrk_3.py:191-212, _etdrk_4.py:198-224) and the
 // nonlinear functions of nonlin_fun/ (see nl_eval below).
 #pragma once
+#include <climits>
 #include <type_traits>
 
 #include "exmi_internal.hpp"
@@ -772,6 +773,11 @@ struct Roll1DArgs {
   long long traj_tstride;  // floats between consecutive outputs of one trajectory
   int nsteps;
   int final_only;          // 1: write only the last state (traj_tstride unused)
+  int* first_bad = nullptr;  // failure detection, or nullptr: per trajectory, atomicMin of the first
+                             // written slot (bad_offset + step) holding a value that is not finite or
+                             // exceeds `bound` in magnitude
+  float bound = 0.f;
+  int bad_offset = 0;
 };
 
 // Write the NT physical states (N x, times `scale`) of a step to the
@@ -866,6 +872,23 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
     }
   }
 
+  // failure detection: the first step whose written state fails the bound
+  // (a lane's own values; the trajectory's minimum is taken at the end)
+  int fb[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) fb[n] = INT_MAX;
+  auto check = [&](const cf (*V)[PT], float scale, int step) {
+    if (!ra.first_bad) return;
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      bool bad = false;
+#pragma unroll
+      for (int s = 0; s < PT; ++s)
+        bad |= !(fabsf(V[n][s].re * scale) <= ra.bound) | !(fabsf(V[n][s].im * scale) <= ra.bound);
+      if (bad && fb[n] == INT_MAX) fb[n] = step;
+    }
+  };
+
   if constexpr (FR) rfft_l<LAY, M, PT, NT>(S, nyq, ln, lds, rt);
   for (int step = 0; step < ra.nsteps; ++step) {
     if constexpr (!FR) rfft_l<LAY, M, PT, NT>(S, nyq, ln, lds, rt);
@@ -884,6 +907,7 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
         }
         irfft_l<LAY, M, PT, NT>(O, on, ln, lds, rt);
         store_states<M, PT, NT>(O, r2n, ra, tb_, cb_, act, step, j);
+        check(O, r2n, step);
       }
     } else {
       irfft_l<LAY, M, PT, NT>(S, nyq, ln, lds, rt);
@@ -896,8 +920,16 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64) roll1d_kernel(R
 #pragma unroll
           for (int s = 0; s < PT; ++s) S[n][s] = cf{S[n][s].re * r2n, S[n][s].im * r2n};
       }
-      if (emit) store_states<M, PT, NT>(S, 1.f, ra, tb_, cb_, act, step, j);
+      if (emit) {
+        store_states<M, PT, NT>(S, 1.f, ra, tb_, cb_, act, step, j);
+        check(S, 1.f, step);
+      }
     }
+  }
+  if (ra.first_bad) {
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+      if (act[n] && fb[n] != INT_MAX) atomicMin(ra.first_bad + tb_[n], ra.bad_offset + (ra.final_only ? 0 : fb[n]));
   }
 }
 

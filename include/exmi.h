@@ -200,6 +200,18 @@ int exmi_nonlinear(exmi_plan* plan, const float* uh_in, int64_t in_bstride, floa
 int exmi_rollout(exmi_plan* plan, const float* u0, int64_t u0_bstride, float* traj, int32_t n_steps,
                  int32_t include_init, int32_t use_graph, void* stream);
 
+/* exmi_rollout (no graph) with failure detection: first_bad (B,) device
+ * int32 receives, per trajectory, the first time index t of traj whose
+ * state holds a value that is not finite or exceeds `bound` in magnitude
+ * (bound <= 0: non-finite values only), or -1; with include_init the given
+ * initial state (t = 0) is not checked. The multi-step 1-D kernels check
+ * every state as they write it (no extra pass); other strategies and
+ * dimensions scan the written states once (exmi_first_bad_step).
+ * (No reference counterpart: failure detection for long batched rollouts,
+ * in which trajectories blow up one by one.) */
+int exmi_rollout_checked(exmi_plan* plan, const float* u0, int64_t u0_bstride, float* traj, int32_t n_steps,
+                         int32_t include_init, float bound, int32_t* first_bad, void* stream);
+
 /* Capture and instantiate the rollout graph for these arguments without
  * launching it, so that a later exmi_rollout(..., use_graph=1) with the same
  * arguments only replays. Synchronous on the host; enqueues no GPU work.
@@ -341,6 +353,15 @@ int exmi_spectrum_binned(int32_t num_spatial_dims, int32_t num_points, int32_t c
                          int32_t power, int32_t average, void* stream);
 int exmi_spectrum(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x, float* out,
                   int32_t power, void* stream);
+
+/* Per trajectory b of a batch of trajectories x (b, t, 0..n-1) with batch
+ * and time strides in floats: first_bad[b] (device int32) = the smallest
+ * t < n_times whose state holds a value that is not finite or exceeds
+ * `bound` in magnitude (bound <= 0: non-finite values only), or -1.
+ * (No reference counterpart: the failure detection of exmi_rollout_checked
+ * for any trajectory buffer, e.g. a corrected coarse rollout.) */
+int exmi_first_bad_step(int32_t batch, int32_t n_times, int64_t n, const float* x, int64_t bstride, int64_t tstride,
+                        float bound, int32_t* first_bad, void* stream);
 
 /* Spectral up/down-sampling of `count` fields from n_old to n_new points
  * per axis (powers of two): the modes of the reference's blocks

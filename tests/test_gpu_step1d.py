@@ -444,3 +444,43 @@ def test_dlpack_producer_round_trip(device):
     one = gpu(Producer(x[0]))
     one = one.cpu().numpy() if hasattr(one, "cpu") else np.asarray(one)
     np.testing.assert_array_equal(one, want[0])
+
+
+@pytest.mark.parametrize("physical", [False, True])
+def test_rollout_checked_flags_first_bad_step(device, physical):
+    """exmi_rollout_checked: the multi-step kernel (or, with the per-step
+    launches, a scan of the written states) reports per trajectory the first
+    time index whose state is non-finite or leaves the bound, -1 otherwise;
+    the trajectory equals the unchecked rollout's. A state forced to
+    blow up from the host (a huge initial amplitude) and a bound crossed by
+    another one are caught at the step the oracle's own states show it."""
+    import torch
+
+    gpu, cpu, L = _pair("burgers", 256, 2)
+    gpu.fourier_resident = None if not physical else False
+    u0, _ = ics(4, 1, L, 256, 5, seed=57)
+    u0[1] *= 1e18  # overflows to inf within a few steps
+    u0[2] *= 3.0
+    x = torch.tensor(u0, device=device)
+    n = 12
+    plan = gpu.plan(4)
+    plan.set_rollout_strategy(-1 if physical else 1)
+    tr, fb = gpu.rollout_checked(x, n, bound=2.5, include_init=True)
+    ref = gpu.rollout_batch(x, n, include_init=True)
+    torch.testing.assert_close(tr[:, 1:], ref[:, 1:], rtol=0, atol=0, equal_nan=True)
+    # expected from the written states themselves (t >= 1)
+    want = []
+    for b in range(4):
+        s = tr[b, 1:].reshape(n, -1)
+        bad = ~(s.abs() <= 2.5).all(dim=1)
+        want.append(int(torch.nonzero(bad)[0].item()) + 1 if bool(bad.any()) else -1)
+    print(f"  first bad {fb.tolist()} (expected {want})", flush=True)
+    assert fb.tolist() == want and want[0] == -1 and want[1] >= 1 and want[2] >= 1
+    # the scan on its own, non-finite values only
+    fb2 = ex.metrics.first_bad_step(tr[:, 1:])
+    want2 = []
+    for b in range(4):
+        s = tr[b, 1:].reshape(n, -1)
+        bad = ~torch.isfinite(s).all(dim=1)
+        want2.append(int(torch.nonzero(bad)[0].item()) if bool(bad.any()) else -1)
+    assert fb2.tolist() == want2 and want2[1] >= 0 and want2[0] == -1
