@@ -44,7 +44,8 @@ EXPORTED = (
     "exmi_step_vjp", "exmi_rollout_vjp", "exmi_nrmse", "exmi_spectrum", "exmi_spectrum_binned",
     "exmi_map_resolution", "exmi_ic_tfs", "exmi_wave_step_fourier", "exmi_poisson_fourier", "exmi_axpy",
     "exmi_cnn_workspace_bytes", "exmi_cnn_forward", "exmi_cnn_backward", "exmi_corrected_rollout",
-    "exmi_irfft_noalloc", "exmi_rollout_checked", "exmi_first_bad_step",
+    "exmi_irfft_noalloc", "exmi_rollout_checked", "exmi_first_bad_step", "exmi_plan_fft_config",
+    "exmi_plan_set_fft_config",
 )
 EXMI_NOALLOC_SCRATCH_BYTES = 32 << 20
 EXMI_CNN_NUM_PARAMS = 15809
@@ -80,6 +81,7 @@ class PlanDesc(C.Structure):
         ("host_linear_operator", C.POINTER(C.c_double)),
         ("host_linear_operator_imag", C.POINTER(C.c_double)),
         ("reaction_rates", C.c_double * 2),
+        ("host_nonlin_factor", C.POINTER(C.c_double)),
     ]
 
 
@@ -111,6 +113,8 @@ def _declare(lib):
     lib.exmi_plan_get_tables.argtypes = [vp, vp, i64]
     lib.exmi_plan_rollout_strategy.argtypes = [vp]
     lib.exmi_plan_set_rollout_strategy.argtypes = [vp, i32]
+    lib.exmi_plan_fft_config.argtypes = [vp, vp, i32]
+    lib.exmi_plan_set_fft_config.argtypes = [vp, i32]
     lib.exmi_step_vjp.argtypes = [vp, vp, i64, vp, i64, vp, i64, vp]
     lib.exmi_rollout_vjp.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, i32, vp]
     lib.exmi_nrmse.argtypes = [i32, i32, i64, vp, i64, vp, i64, vp, vp]
@@ -191,8 +195,12 @@ class Plan:
     """Owns one ``exmi_plan``; created from a PlanDesc plus the L table
     (float64, or complex128 for odd-derivative operators)."""
 
-    def __init__(self, desc: PlanDesc, linear_operator: np.ndarray):
+    def __init__(self, desc: PlanDesc, linear_operator: np.ndarray, nonlin_factor=None):
         L = lib()
+        self._F = None
+        if nonlin_factor is not None:
+            self._F = np.ascontiguousarray(nonlin_factor, dtype=np.float64)
+            desc.host_nonlin_factor = self._F.ctypes.data_as(C.POINTER(C.c_double))
         lin = np.asarray(linear_operator)
         self._L = np.ascontiguousarray(lin.real, dtype=np.float64)
         desc.host_linear_operator = self._L.ctypes.data_as(C.POINTER(C.c_double))
@@ -226,6 +234,23 @@ class Plan:
 
     def set_rollout_strategy(self, strategy: int) -> None:
         check(self._lib.exmi_plan_set_rollout_strategy(self._h, int(strategy)), "exmi_plan_set_rollout_strategy")
+
+    def fft_config(self) -> dict | None:
+        """The runtime-size transforms' knobs (None for plans on the fused
+        kernels): code = max_radix * 100000 + points per workgroup, where it
+        came from (default / timed / cache / set), lines per workgroup and
+        the pass radices."""
+        buf = (C.c_int32 * 32)()
+        n = self._lib.exmi_plan_fft_config(self._h, buf, 32)
+        check(min(n, 0), "exmi_plan_fft_config")
+        if n == 0:
+            return None
+        v = list(buf[:n])
+        return {"code": v[0], "source": ("default", "timed", "cache", "set")[v[1]], "lines_per_workgroup": v[2],
+                "radices": v[4:4 + v[3]]}
+
+    def set_fft_config(self, code: int) -> None:
+        check(self._lib.exmi_plan_set_fft_config(self._h, int(code)), "exmi_plan_set_fft_config")
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -35,8 +35,9 @@ class RepeatedStepper:
 
     @staticmethod
     def _fourier_resident_ok(st):
-        """The Fourier-resident kernels are built for 1-D ETDRK2 plans with an
-        uncoupled nonlinearity (complex linear operators: N <= 1024)."""
+        """The Fourier-resident kernels are built for 1-D ETDRK2 plans of the
+        fused sizes with an uncoupled nonlinearity (complex linear operators:
+        N <= 1024)."""
         import numpy as np
 
         from . import _lib
@@ -44,8 +45,10 @@ class RepeatedStepper:
         nl = getattr(st, "nonlinear_fun", None)
         coupled = getattr(nl, "kind", None) in (_lib.EXMI_NL_GRAY_SCOTT, _lib.EXMI_NL_BELOUSOV_ZHABOTINSKY)
         cplx = np.iscomplexobj(getattr(st, "linear_operator", np.zeros(1)))
-        return (st.num_spatial_dims == 1 and st.order == 2 and not coupled
-                and st.num_points <= (1024 if cplx else 4096))
+        n = st.num_points
+        fused = (n & (n - 1)) == 0 and 16 <= n <= 4096 or n in (48, 50)
+        return (st.num_spatial_dims == 1 and st.order == 2 and not coupled and fused
+                and n <= (1024 if cplx else 4096))
 
     @staticmethod
     def _spectral_twin(st):

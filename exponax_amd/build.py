@@ -50,6 +50,7 @@ def _jobs():
             ("metrics", os.path.join(CSRC, "metrics.hip"), []),
             ("corrector", os.path.join(CSRC, "corrector.hip"), []),
             ("spectral3d", os.path.join(CSRC, "spectral3d.hip"), []),
+            ("gfft", os.path.join(CSRC, "gfft.hip"), []),
             ("ic", os.path.join(CSRC, "ic.hip"), [])]
     for m, pt in SIZES_1D.items():
         defs = [f"-DEXMI_M={m}", f"-DEXMI_PT={pt}"] + (["-DEXMI_LAYOUT_1D=NATURAL"] if m % 2 else [])

@@ -18,6 +18,7 @@
 #include "adjoint1d.hpp"
 #include "adjoint2d.hpp"
 #include "fft1d.hpp"
+#include "gfft.hpp"
 #include "spectral3d.hpp"
 #include "step1d.hpp"
 #include "roll2d.hpp"
@@ -29,8 +30,7 @@ using StepFn = hipError_t (*)(const Step1DArgs&, hipStream_t);
 using RollFn = hipError_t (*)(const Roll1DArgs&, hipStream_t);
 using VjpFn = hipError_t (*)(const Vjp1DArgs&, hipStream_t);
 hipError_t launch_coef(const double* L, const double* Li, long long total, int nr, double dt, int order, int ncirc,
-                       double radius,
-                       float* out, hipStream_t st);
+                       double radius, const double* nf, float* out, hipStream_t st);
 
 hipError_t launch_lines(int N, bool inv, cf* data, long long nlines, long long inner, long long outer_stride,
                         long long pstride, float scale, const cf* tw, hipStream_t st);
@@ -293,6 +293,24 @@ static bool size_1d(int n) { return (pow2(n) && n >= 16 && n <= 4096) || n == 48
 
 // Dealias cutoff of nonlin_fun/_base.py:59-71: keep |k| <= frac*(N//2) - 1,
 // compared in float32 as the reference's float32 wavenumber grid is.
+// Per-row factors of the nonlinear-term coefficients: row r of the tables
+// (lin_batch, lin_channels) takes host_nonlin_factor[r / lin_channels];
+// nullptr without factors. The caller frees *out.
+static int upload_nonlin_factor(const exmi_plan_desc* d, long long rows, double** out) {
+  *out = nullptr;
+  if (!d->host_nonlin_factor) return EXMI_OK;
+  std::vector<double> h((size_t)rows);
+  for (long long r = 0; r < rows; ++r) h[(size_t)r] = d->host_nonlin_factor[r / d->lin_channels];
+  HIP_TRY(hipMalloc(out, sizeof(double) * rows));
+  const hipError_t e = hipMemcpy(*out, h.data(), sizeof(double) * rows, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(*out);
+    *out = nullptr;
+    return fail(EXMI_ERR_HIP, "factor upload failed: %s", hipGetErrorString(e));
+  }
+  return EXMI_OK;
+}
+
 static int dealias_kmax(int N, double frac) {
   if (frac < 0) return 1 << 30;
   const double cutoff = frac * (double)(N / 2) - 1.0;
@@ -346,6 +364,16 @@ struct exmi_plan {
   cf *N03 = nullptr, *NC3 = nullptr, *S3 = nullptr, *S13 = nullptr, *A3 = nullptr, *F3 = nullptr;
   float *R3 = nullptr, *PR3 = nullptr;
   int nf3 = 0, np3 = 0, kind3 = 0;
+  // multi-pass spectral plans (fields / products / combine / stage passes
+  // around whole-field transforms): 3-D plans, and plans of any dimension
+  // whose grid size is outside the fused kernels' set
+  int md = 0;
+  int gen = 0;           // runtime-size transforms (gfft.hip)
+  int gnx = 1, gny = 1;  // sizes of the first two of the three embedded axes
+  long long n3 = 0;      // points of one field, N^D
+  GfftCfg gcfg;          // runtime-size transforms: radices and lines per workgroup
+  int gcode = 0;         // their knobs, encoded (autotuned at plan creation)
+  int gtuned = 0;        // 0: default knobs, 1: timed at plan creation, 2: from the tuning cache, 3: set
   Launch2D row2d[ROW_NUM_MODES] = {};
   LaunchRoll2D roll2d = nullptr;  // 2-D workgroup-resident rollout (strategy 1)
   int fr2d = 0;                   // 2-D spectrum-resident launch chain available (strategy 2)
@@ -660,12 +688,20 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     exmi_plan_destroy(p);
     return fail(EXMI_ERR_ALLOC, "2-D workspace allocation failed (batch %lld, N %d)", B, N);
   }
+  double* dF = nullptr;
+  int frc = upload_nonlin_factor(d, rows, &dF);
+  if (frc) {
+    hipFree(dL);
+    exmi_plan_destroy(p);
+    return frc;
+  }
   hipError_t e = hipMemcpy(dL, Lt.data(), sizeof(double) * nparts * rows * full, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = launch_coef(dL, cplx ? dL + rows * full : nullptr, rows * full, (int)full, d->dt, d->order,
-                    d->num_circle_points, d->circle_radius, p->tables, nullptr);
+                    d->num_circle_points, d->circle_radius, dF, p->tables, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   hipFree(dL);
+  hipFree(dF);
   if (e != hipSuccess) {
     exmi_plan_destroy(p);
     return fail(EXMI_ERR_HIP, "coefficient formation failed: %s", hipGetErrorString(e));
@@ -689,8 +725,13 @@ static long long state_size(const exmi_plan* p) {
   return s;
 }
 
-// ----------------------------------------------------------------- 3-D
+// ------------------------------------------------- multi-pass plans (3-D, any size)
 static bool size_3d(int N) { return pow2(N) && N >= 16 && N <= 256; }
+static bool size_2d(int N) { return pow2(N) && N >= 32 && N <= 512; }
+// Sizes of the fused kernels; every other N in 2..GFFT_MAX_N runs through
+// the multi-pass plans with the runtime-size transforms of gfft.hip.
+static bool fused_size(int D, int N) { return D == 1 ? size_1d(N) : D == 2 ? size_2d(N) : size_3d(N); }
+static bool gen_size(int N) { return N >= 2 && N <= GFFT_MAX_N; }
 static hipError_t copy_batch(const float* src, long long ss, float* dst, long long ds, long long n, int B,
                              hipStream_t st);
 
@@ -717,13 +758,58 @@ static hipError_t irfft3(int N, long long I, cf* X, float* x, const Twiddles& tz
   return e;
 }
 
-// 3-D plans: ETDRK0-4 (complex L for orders 0-2) with the polynomial
-// (Cahn-Hilliard), gradient-norm, three-channel convection, general or
+// rfftn over the last D axes of I contiguous N^D fields with the
+// runtime-size transforms: r2c along the last axis, then c2c in place along
+// the others (innermost first). tw: the N-th roots of unity.
+static hipError_t rfft_gen(const GfftCfg& c, int D, long long I, const float* x, cf* X, const cf* tw,
+                           hipStream_t st) {
+  const int N = c.n;
+  const long long NZ = N / 2 + 1;
+  long long total = I * NZ;  // complex entries of all spectra
+  for (int d = 1; d < D; ++d) total *= N;
+  hipError_t e = launch_gfft(c, GFFT_R2C, x, X, total / NZ, 1, 0, 1, 1.f, tw, st);
+  long long inner = NZ;  // entries between neighbouring points of the axis
+  for (int ax = D - 2; ax >= 0 && e == hipSuccess; --ax) {
+    e = launch_gfft(c, GFFT_FWD, X, X, total / N, inner, inner * N, inner, 1.f, tw, st);
+    inner *= N;
+  }
+  return e;
+}
+
+// irfftn: the inverse c2c lines in place on X (outermost axis first, each
+// scaled by 1/N), then c2r along the last axis into x.
+static hipError_t irfft_gen(const GfftCfg& c, int D, long long I, cf* X, float* x, const cf* tw, hipStream_t st) {
+  const int N = c.n;
+  const long long NZ = N / 2 + 1;
+  const float r = 1.f / (float)N;
+  long long total = I * NZ, inner = NZ;
+  for (int d = 1; d < D; ++d) total *= N;
+  for (int d = 0; d < D - 2; ++d) inner *= N;
+  hipError_t e = hipSuccess;
+  for (int ax = 0; ax <= D - 2 && e == hipSuccess; ++ax) {
+    e = launch_gfft(c, GFFT_INV, X, X, total / N, inner, inner * N, inner, r, tw, st);
+    inner /= N;
+  }
+  if (e == hipSuccess) e = launch_gfft(c, GFFT_C2R, X, x, total / NZ, 1, 0, 1, r, tw, st);
+  return e;
+}
+
+static hipError_t rfft_md(const exmi_plan* p, long long I, const float* x, cf* X, hipStream_t st);
+static hipError_t irfft_md(const exmi_plan* p, long long I, cf* X, float* x, hipStream_t st);
+static int tune_gfft(exmi_plan* p);
+
+// Multi-pass plans: 3-D grids of the fused sizes (powers of two 16..256),
+// and grids of any dimension whose size is outside the fused kernels' set
+// (gen: runtime-size transforms, N in 2..GFFT_MAX_N). ETDRK0-4 (real or
+// complex L) with the polynomial (Cahn-Hilliard), gradient-norm, convection
+// (all four forms), general, vorticity (2-D), projected-convection (3-D) or
 // coupled-reaction nonlinearity (or none).
-static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
-  const int N = d->num_points, C = d->num_channels;
-  if (!size_3d(N))
+static int plan_create_md(const exmi_plan_desc* d, exmi_plan** out, bool gen) {
+  const int N = d->num_points, C = d->num_channels, D = d->num_spatial_dims;
+  if (!gen && !size_3d(N))
     return fail(EXMI_ERR_UNSUPPORTED, "3-D num_points=%d: supported sizes are powers of two 16..256", N);
+  if (gen && !gen_size(N))
+    return fail(EXMI_ERR_UNSUPPORTED, "%d-D num_points=%d: supported sizes are 2..%d", D, N, GFFT_MAX_N);
   if (C < 1 || d->batch < 1) return fail(EXMI_ERR_INVALID, "num_channels and batch must be >= 1");
   if (d->order < 0 || d->order > 4) return fail(EXMI_ERR_UNSUPPORTED, "Order %d not implemented.", d->order);
 
@@ -732,35 +818,45 @@ static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
   if (d->order > 0) {
     if (d->nonlin_kind == EXMI_NL_ZERO) {
       kind = NL_POLY;  // the empty polynomial
-    } else if (d->nonlin_kind == EXMI_NL_CONVECTION && d->conv_single_channel) {
+    } else if (d->nonlin_kind == EXMI_NL_CONVECTION && d->conv_single_channel && D > 1) {
       if (C != 1) return fail(EXMI_ERR_UNSUPPORTED, "single-channel convection needs num_channels == 1");
       kind = NL_GENERAL;  // as in 2-D: the general form's fields, products and scales
       conv_general = true;
     } else if (d->nonlin_kind == EXMI_NL_PROJECTED_CONVECTION ||
                d->nonlin_kind == EXMI_NL_PROJECTED_CONVECTION_KOLMOGOROV) {
+      if (D != 3) return fail(EXMI_ERR_INVALID, "the projected convection is three-dimensional");
       if (C != 3) return fail(EXMI_ERR_INVALID, "the projected convection needs num_channels == 3");
       kind = NL_PROJ3D;
+    } else if (d->nonlin_kind == EXMI_NL_VORTICITY || d->nonlin_kind == EXMI_NL_KOLMOGOROV) {
+      if (D != 2) return fail(EXMI_ERR_INVALID, "Expected num_spatial_dims = 2, got %d.", D);
+      if (C != 1) return fail(EXMI_ERR_INVALID, "vorticity plans need num_channels == 1");
+      kind = NL_VORT;
     } else {
       int rc = kernel_kind(d, &kind);
       if (rc) return rc;
-      if (kind == NL_VORT) return fail(EXMI_ERR_INVALID, "vorticity convection is two-dimensional");
     }
   }
   if (d->flags & EXMI_FLAG_FOURIER_RESIDENT)
-    return fail(EXMI_ERR_UNSUPPORTED, "Fourier-resident rollouts are built for 1-D plans");
+    return fail(EXMI_ERR_UNSUPPORTED, "Fourier-resident rollouts are built for 1-D plans of the fused sizes");
   if (d->lin_batch != 1 && d->lin_batch != d->batch)
     return fail(EXMI_ERR_INVALID, "lin_batch must be 1 or batch (%d), got %d", d->batch, d->lin_batch);
   if (d->lin_channels != 1 && d->lin_channels != C)
     return fail(EXMI_ERR_INVALID, "lin_channels must be 1 or num_channels (%d), got %d", C, d->lin_channels);
   if (!d->host_linear_operator) return fail(EXMI_ERR_INVALID, "host_linear_operator is null");
+  if (d->order > 0 && d->num_circle_points < 1) return fail(EXMI_ERR_INVALID, "num_circle_points must be >= 1");
   const bool cplx = d->host_linear_operator_imag != nullptr;
   exmi_plan* p = new exmi_plan();
   p->d = *d;
   p->d.host_linear_operator = nullptr;
   p->d.host_linear_operator_imag = nullptr;
-  p->ndim = 3;
+  p->ndim = D;
+  p->md = 1;
+  p->gen = gen ? 1 : 0;
+  p->gnx = D == 3 ? N : 1;
+  p->gny = D >= 2 ? N : 1;
   p->M = N / 2;
-  p->nr = N * N * (N / 2 + 1);
+  p->n3 = (long long)p->gnx * p->gny * N;
+  p->nr = (int)((long long)p->gnx * p->gny * (N / 2 + 1));
   p->ncoef = num_coefs_c(d->order, cplx);
   p->cplx = cplx;
   p->ninst = d->batch * C;
@@ -774,7 +870,11 @@ static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
     p->d.general_scales[1] = p->prod_nc ? 0.0 : -d->scale;
     p->d.general_scales[2] = 0.0;
   }
-  p->nf3 = kind == NL_GRADNORM ? 3 : (kind == NL_CONV_NONCONS || kind == NL_GENERAL) ? 4 : kind == NL_PROJ3D ? 2 : 1;
+  p->nf3 = kind == NL_GRADNORM ? D
+           : (kind == NL_CONV_NONCONS || kind == NL_GENERAL) ? 1 + D
+           : kind == NL_PROJ3D ? 2
+           : kind == NL_VORT ? 4
+                             : 1;
   if (d->nonlin_kind == EXMI_NL_PROJECTED_CONVECTION_KOLMOGOROV) {
     // _projected_convection.py:195-216: injection_scale times the
     // coef_extraction scaling (_spectral.py:403-527) at (0, k_f, 0): N on
@@ -786,19 +886,32 @@ static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
       p->inj_value = (float)(d->injection_scale * (double)N * sy * (double)N);
     }
   }
-  p->np3 = kind == NL_CONV_CONS ? 3 : kind == NL_GENERAL ? 2 : 1;
+  if (d->nonlin_kind == EXMI_NL_KOLMOGOROV) {
+    // _vorticity_convection.py:166-176: -k_f gamma scaling(coef_extraction)
+    // at wavenumber (0, k_f); scaling = N at k = 0 (and at the Nyquist mode
+    // of an even N), else N/2 per axis
+    const int kf = d->injection_mode;
+    if (kf >= 0 && kf <= N / 2) {
+      const double sy = (kf == 0 || 2 * kf == N) ? N : N / 2.0;
+      p->inj_mode = kf;
+      p->inj_value = (float)(-(double)kf * d->injection_scale * (double)N * sy);
+    }
+  }
+  p->np3 = kind == NL_CONV_CONS ? D : kind == NL_GENERAL ? 2 : 1;
   p->kscale = (float)(2.0 * 3.14159265358979323846 / d->domain_extent);
-  p->roll_nt = -1;  // per-step launches; no autotuning
+  p->roll_nt = -1;  // per-step launches; no rollout strategies to choose from
   hipGetDevice(&p->device);
-  int rc = get_twiddles(N / 2, &p->tw);
+  int rc = gen ? EXMI_OK : get_twiddles(N / 2, &p->tw);
   if (!rc) rc = get_twiddles(N, &p->tw_col);
   if (rc) {
     delete p;
     return rc;
   }
+  if (gen) gfft_cfg_from_code(N, gfft_default_code(), &p->gcfg);
+  p->gcode = gen ? gfft_default_code() : 0;
   const long long rows = (long long)d->lin_batch * d->lin_channels, total = rows * p->nr;
   double* dL = nullptr;
-  const long long I = p->ninst, n3 = (long long)N * N * N, nfp = p->nf3 > p->np3 ? p->nf3 : p->np3;
+  const long long I = p->ninst, n3 = p->n3, nfp = p->nf3 > p->np3 ? p->nf3 : p->np3;
   const bool nl = d->order > 0;
   bool ok = hipMalloc(&dL, sizeof(double) * total * (cplx ? 2 : 1)) == hipSuccess &&
             hipMalloc(&p->tables, sizeof(float) * total * p->ncoef) == hipSuccess &&
@@ -815,16 +928,24 @@ static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
   if (!ok) {
     hipFree(dL);
     exmi_plan_destroy(p);
-    return fail(EXMI_ERR_ALLOC, "3-D workspace allocation failed (batch %d, N %d)", d->batch, N);
+    return fail(EXMI_ERR_ALLOC, "%d-D workspace allocation failed (batch %d, N %d)", D, d->batch, N);
+  }
+  double* dF = nullptr;
+  int frc = upload_nonlin_factor(d, rows, &dF);
+  if (frc) {
+    hipFree(dL);
+    exmi_plan_destroy(p);
+    return frc;
   }
   hipError_t e = hipMemcpy(dL, d->host_linear_operator, sizeof(double) * total, hipMemcpyHostToDevice);
   if (e == hipSuccess && cplx)
     e = hipMemcpy(dL + total, d->host_linear_operator_imag, sizeof(double) * total, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = launch_coef(dL, cplx ? dL + total : nullptr, total, (int)p->nr, d->dt, d->order, d->num_circle_points,
-                    d->circle_radius, p->tables, nullptr);
+                    d->circle_radius, dF, p->tables, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   hipFree(dL);
+  hipFree(dF);
   if (e != hipSuccess) {
     exmi_plan_destroy(p);
     return fail(EXMI_ERR_HIP, "coefficient formation failed: %s", hipGetErrorString(e));
@@ -834,8 +955,24 @@ static int plan_create_3d(const exmi_plan_desc* d, exmi_plan** out) {
     exmi_plan_destroy(p);
     return fail(EXMI_ERR_HIP, "stream creation failed");
   }
+  if (gen) {
+    rc = tune_gfft(p);
+    if (rc) {
+      exmi_plan_destroy(p);
+      return rc;
+    }
+  }
   *out = p;
   return EXMI_OK;
+}
+
+static hipError_t rfft_md(const exmi_plan* p, long long I, const float* x, cf* X, hipStream_t st) {
+  if (p->gen) return rfft_gen(p->gcfg, p->ndim, I, x, X, p->tw_col.tw, st);
+  return rfft3(p->d.num_points, I, x, X, p->tw, p->tw_col, st);
+}
+static hipError_t irfft_md(const exmi_plan* p, long long I, cf* X, float* x, hipStream_t st) {
+  if (p->gen) return irfft_gen(p->gcfg, p->ndim, I, X, x, p->tw_col.tw, st);
+  return irfft3(p->d.num_points, I, X, x, p->tw, p->tw_col, st);
 }
 
 // N(X) of every instance into outN (the four passes of spectral3d.hip).
@@ -845,13 +982,16 @@ static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t s
   memset(&a, 0, sizeof a);
   a.N = N;
   a.NZ = N / 2 + 1;
+  a.dims = p->ndim;
+  a.nx = p->gnx;
+  a.ny = p->gny;
   a.channels = p->d.num_channels;
   a.kind = p->kind3;
   a.kmax = p->kmax;
   a.nf = p->nf3;
   a.np = p->np3;
   a.Nr = p->nr;
-  a.N3 = (long long)N * N * N;
+  a.N3 = p->n3;
   a.ninst = p->ninst;
   a.kscale = p->kscale;
   a.scale = (float)p->d.scale;
@@ -866,25 +1006,26 @@ static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t s
   for (int i = 0; i < EXMI_MAX_POLY; ++i) a.poly[i] = (float)p->d.poly[i];
   a.poly_lap = p->d.nonlin_kind == EXMI_NL_CAHN_HILLIARD;
   a.inj_mode = p->inj_mode;
+  a.inj_axis = p->kind3 == NL_VORT ? 2 : 1;
   a.inj_value = p->inj_value;
   hipError_t e = launch_fields3d(a, X, p->F3, st);
-  if (e == hipSuccess) e = irfft3(N, p->ninst * p->nf3, p->F3, p->R3, p->tw, p->tw_col, st);
+  if (e == hipSuccess) e = irfft_md(p, p->ninst * p->nf3, p->F3, p->R3, st);
   if (e == hipSuccess) e = launch_products3d(a, p->R3, p->PR3, st);
-  if (e == hipSuccess) e = rfft3(N, p->ninst * p->np3, p->PR3, p->F3, p->tw, p->tw_col, st);
+  if (e == hipSuccess) e = rfft_md(p, p->ninst * p->np3, p->PR3, p->F3, st);
   if (e == hipSuccess) e = launch_combine3d(a, p->F3, outN, st);
   return e;
 }
 
-// One 3-D step u_in (batch stride ibs) -> u_out (obs): forward transform,
-// E multiply, inverse transform; strided states pass through the staging
-// buffer.
-// With in_f / out_f the input / output are spectra (C, N, N, N/2+1) with
+// One multi-pass step u_in (batch stride ibs) -> u_out (obs): forward
+// transform, ETDRK stages, inverse transform; strided states pass through
+// the staging buffer.
+// With in_f / out_f the input / output are spectra (C, N^(D-1), N/2+1) with
 // batch strides in complex entries (exmi_step_fourier); nonlin_only
 // evaluates N(u_hat) instead of a step (exmi_nonlinear).
 static int enqueue_step3d(exmi_plan* p, const float* u_in, long long ibs, float* u_out, long long obs,
                           hipStream_t st, bool in_f = false, bool out_f = false, bool nonlin_only = false) {
-  const int N = p->d.num_points, B = p->d.batch;
-  const long long sz = (long long)p->d.num_channels * N * N * N;
+  const int B = p->d.batch;
+  const long long sz = (long long)p->d.num_channels * p->n3;
   const long long szs = 2LL * p->d.num_channels * p->nr;  // floats of one spectrum
   hipError_t e = hipSuccess;
   if (in_f) {
@@ -895,7 +1036,7 @@ static int enqueue_step3d(exmi_plan* p, const float* u_in, long long ibs, float*
       e = copy_batch(u_in, ibs, p->P3, sz, sz, B, st);
       src = p->P3;
     }
-    if (e == hipSuccess) e = rfft3(N, p->ninst, src, p->W3, p->tw, p->tw_col, st);
+    if (e == hipSuccess) e = rfft_md(p, p->ninst, src, p->W3, st);
   }
   cf* X = p->W3;  // the new state's spectrum
   if (nonlin_only) {
@@ -948,10 +1089,10 @@ static int enqueue_step3d(exmi_plan* p, const float* u_in, long long ibs, float*
     if (e == hipSuccess) e = copy_batch((const float*)X, szs, u_out, 2 * obs, szs, B, st);
   } else {
     float* dst = obs == sz ? u_out : p->P3;
-    if (e == hipSuccess) e = irfft3(N, p->ninst, X, dst, p->tw, p->tw_col, st);
+    if (e == hipSuccess) e = irfft_md(p, p->ninst, X, dst, st);
     if (e == hipSuccess && dst != u_out) e = copy_batch(p->P3, sz, u_out, obs, sz, B, st);
   }
-  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "3-D step launch failed: %s", hipGetErrorString(e));
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "%d-D step launch failed: %s", p->ndim, hipGetErrorString(e));
 }
 
 static int enqueue_step2d(exmi_plan* p, const float* u_in, long long ibs, float* u_out, long long obs, bool have_T,
@@ -1059,11 +1200,12 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
   if (!d || !out) return fail(EXMI_ERR_INVALID, "null argument");
   *out = nullptr;
   const int D = d->num_spatial_dims, N = d->num_points;
-  if (D == 2) return plan_create_2d(d, out);
-  if (D == 3) return plan_create_3d(d, out);
+  if (d->host_nonlin_factor && d->lin_batch != d->batch)
+    return fail(EXMI_ERR_INVALID, "host_nonlin_factor needs per-trajectory tables (lin_batch == batch)");
+  if (D == 2) return size_2d(N) ? plan_create_2d(d, out) : plan_create_md(d, out, true);
+  if (D == 3) return plan_create_md(d, out, !size_3d(N));
   if (D != 1) return fail(EXMI_ERR_UNSUPPORTED, "num_spatial_dims=%d: this build supports 1, 2 and 3", D);
-  if (!size_1d(N))
-    return fail(EXMI_ERR_UNSUPPORTED, "num_points=%d: supported sizes are powers of two 16..4096, 48 and 50", N);
+  if (!size_1d(N)) return plan_create_md(d, out, true);
   if (d->num_channels < 1 || d->batch < 1) return fail(EXMI_ERR_INVALID, "num_channels and batch must be >= 1");
   if (d->order < 0 || d->order > 4) return fail(EXMI_ERR_UNSUPPORTED, "Order %d not implemented.", d->order);
   if (d->lin_batch != 1 && d->lin_batch != d->batch)
@@ -1167,14 +1309,22 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
     exmi_plan_destroy(p);
     return fail(EXMI_ERR_ALLOC, "table allocation failed (%lld modes)", total);
   }
+  double* dF = nullptr;
+  rc = upload_nonlin_factor(d, rows, &dF);
+  if (rc) {
+    hipFree(dL);
+    exmi_plan_destroy(p);
+    return rc;
+  }
   hipError_t e = hipMemcpy(dL, d->host_linear_operator, sizeof(double) * total, hipMemcpyHostToDevice);
   if (e == hipSuccess && cplx)
     e = hipMemcpy(dL + total, d->host_linear_operator_imag, sizeof(double) * total, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = launch_coef(dL, cplx ? dL + total : nullptr, total, p->nr, d->dt, d->order, d->num_circle_points,
-                    d->circle_radius, p->tables, nullptr);
+                    d->circle_radius, dF, p->tables, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   hipFree(dL);
+  hipFree(dF);
   if (e != hipSuccess) {
     exmi_plan_destroy(p);
     return fail(EXMI_ERR_HIP, "coefficient formation failed: %s", hipGetErrorString(e));
@@ -1239,12 +1389,42 @@ int exmi_plan_destroy(exmi_plan* p) {
 // 1 = the workgroup-resident kernel).
 static bool strategy_ok(const exmi_plan* p, int nt) {
   if (nt == -1) return !(p->fourier_resident && !p->fr_auto);
+  if (p->md) return false;
   if (p->ndim == 2) return (nt == 1 && p->roll2d != nullptr) || (nt == 2 && p->fr2d);
-  if (p->ndim == 3) return false;
   return (nt == 1 || nt == 2) && p->roll[nt] != nullptr;
 }
 
 int exmi_plan_rollout_strategy(const exmi_plan* p) { return p ? p->roll_nt : fail(EXMI_ERR_INVALID, "null plan"); }
+
+int exmi_plan_fft_config(const exmi_plan* p, int32_t* info, int32_t len) {
+  if (!p || (!info && len > 0)) return fail(EXMI_ERR_INVALID, "null argument");
+  if (!p->gen) return 0;
+  int32_t v[4 + GFFT_MAX_PASS];
+  int n = 0;
+  v[n++] = p->gcode;
+  v[n++] = p->gtuned;
+  v[n++] = p->gcfg.lpb;
+  v[n++] = p->gcfg.npass;
+  for (int i = 0; i < p->gcfg.npass; ++i) v[n++] = p->gcfg.radix[i];
+  for (int i = 0; i < n && i < len; ++i) info[i] = v[i];
+  return n;
+}
+
+int exmi_plan_set_fft_config(exmi_plan* p, int32_t code) {
+  if (!p) return fail(EXMI_ERR_INVALID, "null plan");
+  if (!p->gen) return fail(EXMI_ERR_UNSUPPORTED, "the plan's transforms are the fused kernels' (no knobs)");
+  GfftCfg c;
+  if (!gfft_cfg_from_code(p->d.num_points, code, &c))
+    return fail(EXMI_ERR_INVALID, "transform knobs %d: max_radix * 100000 + points_per_workgroup expected", code);
+  p->gcfg = c;
+  p->gcode = code;
+  p->gtuned = 3;
+  if (p->gc.exec) {
+    hipGraphExecDestroy(p->gc.exec);
+    p->gc = GraphCache();
+  }
+  return EXMI_OK;
+}
 
 int exmi_plan_set_rollout_strategy(exmi_plan* p, int32_t nt) {
   if (!p) return fail(EXMI_ERR_INVALID, "null plan");
@@ -1252,7 +1432,7 @@ int exmi_plan_set_rollout_strategy(exmi_plan* p, int32_t nt) {
     return fail(EXMI_ERR_INVALID, "Fourier-resident plans roll out with the multi-step kernel only");
   if (!(nt == 0 || strategy_ok(p, nt)))
     return fail(EXMI_ERR_UNSUPPORTED, "rollout strategy %d not available for this plan", nt);
-  if (nt == 0 && p->ndim != 1 && !p->roll2d && !p->fr2d) nt = -1;  // nothing to choose from
+  if (nt == 0 && (p->md || (p->ndim != 1 && !p->roll2d && !p->fr2d))) nt = -1;  // nothing to choose from
   p->roll_nt = nt;
   if (p->gc.exec) {
     hipGraphExecDestroy(p->gc.exec);
@@ -1300,7 +1480,7 @@ static int run_step(exmi_plan* p, const void* in, long long in_bs, int in_f, voi
 
 int exmi_step(exmi_plan* p, const float* u_in, int64_t in_bs, float* u_out, int64_t out_bs, void* stream) {
   if (!p || !u_in || !u_out) return fail(EXMI_ERR_INVALID, "null argument");
-  if (p->ndim == 3) return enqueue_step3d(p, u_in, in_bs, u_out, out_bs, (hipStream_t)stream);
+  if (p->md) return enqueue_step3d(p, u_in, in_bs, u_out, out_bs, (hipStream_t)stream);
   if (p->ndim == 2) return enqueue_step2d(p, u_in, in_bs, u_out, out_bs, false, false, (hipStream_t)stream);
   return run_step(p, u_in, in_bs, 0, u_out, out_bs, 0, 0, (hipStream_t)stream);
 }
@@ -1308,7 +1488,7 @@ int exmi_step(exmi_plan* p, const float* u_in, int64_t in_bs, float* u_out, int6
 int exmi_step_fourier(exmi_plan* p, const float* uh_in, int64_t in_bs, float* uh_out, int64_t out_bs,
                       void* stream) {
   if (!p || !uh_in || !uh_out) return fail(EXMI_ERR_INVALID, "null argument");
-  if (p->ndim == 3) return enqueue_step3d(p, uh_in, in_bs, uh_out, out_bs, (hipStream_t)stream, true, true);
+  if (p->md) return enqueue_step3d(p, uh_in, in_bs, uh_out, out_bs, (hipStream_t)stream, true, true);
   if (p->ndim == 2) return step_fourier2d(p, (const cf*)uh_in, in_bs, (cf*)uh_out, out_bs, (hipStream_t)stream);
   return run_step(p, uh_in, in_bs, 1, uh_out, out_bs, 1, 0, (hipStream_t)stream);
 }
@@ -1316,7 +1496,7 @@ int exmi_step_fourier(exmi_plan* p, const float* uh_in, int64_t in_bs, float* uh
 int exmi_nonlinear(exmi_plan* p, const float* uh_in, int64_t in_bs, float* nh_out, int64_t out_bs, void* stream) {
   if (!p || !uh_in || !nh_out) return fail(EXMI_ERR_INVALID, "null argument");
   if (p->d.order == 0) return fail(EXMI_ERR_INVALID, "order-0 plan has no nonlinear function");
-  if (p->ndim == 3) return enqueue_step3d(p, uh_in, in_bs, nh_out, out_bs, (hipStream_t)stream, true, true, true);
+  if (p->md) return enqueue_step3d(p, uh_in, in_bs, nh_out, out_bs, (hipStream_t)stream, true, true, true);
   if (p->ndim == 2) {
     // the nonlinear evaluation of the 2-D step chain, from and to spectra
     // (no exponential tables are read, so real and complex L plans share it)
@@ -1441,7 +1621,7 @@ static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u
     off = 1;
   }
   if (n == 0) return EXMI_OK;
-  if (p->ndim == 3) {
+  if (p->md) {
     for (int t = 0; t < n; ++t) {
       const float* in = (t == 0 && !include_init) ? u0 : traj + (long long)(t - 1 + off) * sz;
       const long long ibs = (t == 0 && !include_init) ? u0_bs : tstride;
@@ -1525,6 +1705,104 @@ static void tune_store(const exmi_plan* p) {
     fprintf(f, "%s %d\n", key.c_str(), p->roll_nt);
     fclose(f);
   }
+}
+
+// Runtime-size transforms: the radices (largest pass max_radix) and the
+// lines per workgroup are timed at plan creation on a forward and an
+// inverse transform of the plan's own batch, and the fastest kept; with
+// EXMI_TUNE_CACHE the choice is looked up and stored like the rollout
+// strategies ("<arch>|gfft|D|N|count <code>").
+static std::string gfft_tune_key(const exmi_plan* p) {
+  std::string k = tune_key(p);
+  char key[256];
+  snprintf(key, sizeof key, "%s|gfft|D%d|N%d|I%d", k.substr(0, k.find('|')).c_str(), p->ndim, p->d.num_points,
+           p->ninst);
+  return key;
+}
+
+static bool same_cfg(const GfftCfg& a, const GfftCfg& b) {
+  if (a.n != b.n || a.npass != b.npass || a.lpb != b.lpb) return false;
+  for (int i = 0; i < a.npass; ++i)
+    if (a.radix[i] != b.radix[i]) return false;
+  return true;
+}
+
+static int tune_gfft(exmi_plan* p) {
+  const int N = p->d.num_points;
+  const std::string key = gfft_tune_key(p);
+  const char* path = getenv("EXMI_TUNE_CACHE");
+  if (path && *path) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    bool found = false;
+    if (FILE* f = fopen(path, "r")) {
+      char k[256];
+      int v = 0;
+      GfftCfg c;
+      while (fscanf(f, "%255s %d", k, &v) == 2) {  // the last entry for a key wins
+        if (key == k && gfft_cfg_from_code(N, v, &c)) {
+          p->gcode = v;
+          p->gcfg = c;
+          found = true;
+        }
+      }
+      fclose(f);
+    }
+    if (found) {
+      p->gtuned = 2;
+      return EXMI_OK;
+    }
+  }
+  // probe on up to ~4M points of the plan's batch, in its own buffers
+  long long I = (1LL << 22) / p->n3;
+  if (I < 1) I = 1;
+  if (I > p->ninst) I = p->ninst;
+  hipStream_t st = p->capture_stream;
+  HIP_TRY(hipMemsetAsync(p->P3, 0, sizeof(float) * I * p->n3, st));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  std::vector<GfftCfg> seen;
+  float best = 1e30f;
+  int best_code = p->gcode;
+  hipError_t err = hipSuccess;
+  for (int i = 0; i < gfft_num_candidates() && err == hipSuccess; ++i) {
+    const int code = gfft_candidate(i);
+    GfftCfg c;
+    if (!gfft_cfg_from_code(N, code, &c)) continue;
+    bool dup = false;
+    for (const GfftCfg& o : seen) dup = dup || same_cfg(o, c);
+    if (dup) continue;
+    seen.push_back(c);
+    float tmin = 1e30f;
+    for (int rep = 0; rep < 3 && err == hipSuccess; ++rep) {  // the first run of each candidate is not timed
+      err = hipEventRecord(e0, st);
+      if (err == hipSuccess) err = rfft_gen(c, p->ndim, I, p->P3, p->W3, p->tw_col.tw, st);
+      if (err == hipSuccess) err = irfft_gen(c, p->ndim, I, p->W3, p->P3, p->tw_col.tw, st);
+      if (err == hipSuccess) err = hipEventRecord(e1, st);
+      if (err == hipSuccess) err = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+      if (rep > 0 && ms < tmin) tmin = ms;
+    }
+    if (tmin < best) {
+      best = tmin;
+      best_code = code;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (err != hipSuccess) return fail(EXMI_ERR_HIP, "transform autotuning failed: %s", hipGetErrorString(err));
+  gfft_cfg_from_code(N, best_code, &p->gcfg);
+  p->gcode = best_code;
+  p->gtuned = 1;
+  if (path && *path) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (FILE* f = fopen(path, "a")) {
+      fprintf(f, "%s %d\n", key.c_str(), p->gcode);
+      fclose(f);
+    }
+  }
+  return EXMI_OK;
 }
 
 static int autotune_rollout(exmi_plan* p, const float* u0, long long u0_bs, float* traj, int n, int include_init) {
@@ -1626,7 +1904,7 @@ static int enq_repeat_cb(exmi_plan* p, const void* a, const void* b, long long s
     hipError_t e = copy_batch(u0, s, out, sz, sz, p->d.batch, st);
     return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "copy failed: %s", hipGetErrorString(e));
   }
-  if (p->ndim == 3) {
+  if (p->md) {
     for (int t = 0; t < n; ++t) {
       int rc = enqueue_step3d(p, t == 0 ? u0 : out, t == 0 ? s : sz, out, sz, st);
       if (rc) return rc;
@@ -1719,8 +1997,9 @@ int exmi_rollout_checked(exmi_plan* p, const float* u0, int64_t u0_bs, float* tr
     return enqueue_rollout_nt(p, p->roll_nt, u0, u0_bs, traj, n, include_init, st);
   }
   int nt = p->roll_nt;
-  if (p->ndim == 1 && p->fourier_resident && (nt == 0 || (nt < 0 && !p->fr_auto))) nt = p->roll[2] ? 2 : 1;
-  if (p->ndim == 1 && nt > 0 && p->roll[nt]) {
+  if (p->ndim == 1 && !p->md && p->fourier_resident && (nt == 0 || (nt < 0 && !p->fr_auto)))
+    nt = p->roll[2] ? 2 : 1;
+  if (p->ndim == 1 && !p->md && nt > 0 && p->roll[nt]) {
     // the multi-step kernel checks every state it writes
     if (include_init) {
       hipError_t e = copy_batch(u0, u0_bs, traj, tstride, sz, B, st);
@@ -1930,9 +2209,43 @@ static int transform2d(bool inverse, int N, int count, const float* x, cf* xh, h
   return transform2d_with(inverse, N, count, x, xh, tr, tc, nullptr, st);
 }
 
+// Standalone transforms of the sizes outside the fused kernels' set, with the
+// runtime-size kernels and their default knobs; the inverse of D >= 2 runs
+// its in-place line transforms on a stream-ordered copy of the spectra.
+static int gen_transform(bool inverse, int D, int N, int count, const float* in, float* out, hipStream_t st) {
+  if (!gen_size(N))
+    return fail(EXMI_ERR_UNSUPPORTED, "%s: num_points=%d: supported sizes are 2..%d", inverse ? "exmi_irfft" : "exmi_rfft",
+                N, GFFT_MAX_N);
+  if (count <= 0) return EXMI_OK;
+  if (!in || !out) return fail(EXMI_ERR_INVALID, "null argument");
+  Twiddles tw;
+  int rc = get_twiddles(N, &tw);
+  if (rc) return rc;
+  GfftCfg c;
+  gfft_cfg_from_code(N, gfft_default_code(), &c);
+  hipError_t e = hipSuccess;
+  if (!inverse) {
+    e = rfft_gen(c, D, count, in, (cf*)out, tw.tw, st);
+  } else if (D == 1) {
+    e = launch_gfft(c, GFFT_C2R, in, out, count, 1, 0, 1, 1.f / (float)N, tw.tw, st);
+  } else {
+    size_t nr = (size_t)(N / 2 + 1);
+    for (int d = 1; d < D; ++d) nr *= (size_t)N;
+    const size_t bytes = sizeof(cf) * nr * (size_t)count;
+    cf* tmp = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&tmp, bytes, st));
+    e = hipMemcpyAsync(tmp, in, bytes, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = irfft_gen(c, D, count, tmp, out, tw.tw, st);
+    const hipError_t ef = hipFreeAsync(tmp, st);
+    if (e == hipSuccess) e = ef;
+  }
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "runtime-size transform launch failed: %s", hipGetErrorString(e));
+}
+
 extern "C" {
 
 int exmi_rfft(int32_t D, int32_t N, int32_t count, const float* x, float* x_hat, void* stream) {
+  if (D >= 1 && D <= 3 && !fused_size(D, N)) return gen_transform(false, D, N, count, x, x_hat, (hipStream_t)stream);
   if (D == 2) return count <= 0 ? EXMI_OK : transform2d(false, N, count, x, (cf*)x_hat, (hipStream_t)stream);
   if (D == 3) {
     if (!size_3d(N)) return fail(EXMI_ERR_UNSUPPORTED, "exmi_rfft: 3-D num_points=%d (powers of two 16..256)", N);
@@ -1985,6 +2298,7 @@ int exmi_wave_step_fourier(int32_t D, int32_t N, int32_t batch, double domain_ex
 }
 
 int exmi_irfft(int32_t D, int32_t N, int32_t count, const float* x_hat, float* x, void* stream) {
+  if (D >= 1 && D <= 3 && !fused_size(D, N)) return gen_transform(true, D, N, count, x_hat, x, (hipStream_t)stream);
   if (D == 2)
     return count <= 0 ? EXMI_OK : transform2d(true, N, count, x, (cf*)const_cast<float*>(x_hat), (hipStream_t)stream);
   if (D == 3) {
@@ -2296,7 +2610,7 @@ static int enqueue_step_vjp(exmi_plan* p, const float* u, long long ubs, const f
 
 static int ensure_work(exmi_plan* p) {
   const size_t need = (size_t)p->d.batch * p->d.num_channels * p->d.num_points;
-  if (p->ndim != 1 || p->work_elems >= need) return EXMI_OK;
+  if (p->ndim != 1 || p->md || p->work_elems >= need) return EXMI_OK;
   hipFree(p->work);
   p->work = nullptr;
   p->work_elems = 0;

@@ -9,6 +9,10 @@
 // as float32 in the layout (row, coefficient, mode). With a complex L (Li not
 // null) E and E_half are complex; their imaginary parts follow the real
 // coefficients (num_coefs_c), the phi-coefficients are real as above.
+// With per-row factors nf (not null) every coefficient that multiplies a
+// nonlinear term (all but E and E_half) is multiplied by its row's factor in
+// float64 before rounding: the ETDRK stages are linear in the nonlinear
+// term, so this steps N' = f N, a per-trajectory scale of the nonlinearity.
 #include <hip/hip_runtime.h>
 
 #include "exmi_internal.hpp"
@@ -41,7 +45,8 @@ __device__ inline zd zexp(zd z) {
 }
 
 __global__ void coef_kernel(const double* __restrict__ L, const double* __restrict__ Li, long long total, int nr,
-                            double dt, int order, int ncirc, double radius, float* __restrict__ out) {
+                            double dt, int order, int ncirc, double radius, const double* __restrict__ nf,
+                            float* __restrict__ out) {
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= total) return;
   const long long row = idx / nr;
@@ -90,26 +95,27 @@ __global__ void coef_kernel(const double* __restrict__ L, const double* __restri
       acc[3] += zdiv(((-4.0 + (-3.0) * z) - z2) + ez * (4.0 + (-1.0) * z), z3).re;
     }
   }
+  const double f = nf ? nf[row] : 1.0;
   if (order == 1) {
-    o[nr] = (float)(dt * (acc[0] / ncirc));
+    o[nr] = (float)(f * (dt * (acc[0] / ncirc)));
   } else if (order == 2) {
-    o[nr] = (float)(dt * (acc[0] / ncirc));
-    o[2 * nr] = (float)(dt * (acc[1] / ncirc));
+    o[nr] = (float)(f * (dt * (acc[0] / ncirc)));
+    o[2 * nr] = (float)(f * (dt * (acc[1] / ncirc)));
   } else if (order == 3) {
     if (!Li) o[nr] = (float)exp(0.5 * dt * l);
-    for (int i = 0; i < 5; ++i) o[(2 + i) * nr] = (float)(dt * (acc[i] / ncirc));
+    for (int i = 0; i < 5; ++i) o[(2 + i) * nr] = (float)(f * (dt * (acc[i] / ncirc)));
   } else {
     if (!Li) o[nr] = (float)exp(0.5 * dt * l);
-    for (int i = 0; i < 4; ++i) o[(2 + i) * nr] = (float)(dt * (acc[i] / ncirc));
+    for (int i = 0; i < 4; ++i) o[(2 + i) * nr] = (float)(f * (dt * (acc[i] / ncirc)));
   }
 }
 
 hipError_t launch_coef(const double* L, const double* Li, long long total, int nr, double dt, int order, int ncirc,
-                       double radius, float* out, hipStream_t st) {
+                       double radius, const double* nf, float* out, hipStream_t st) {
   const int bs = 256;
   const long long grid = (total + bs - 1) / bs;
   hipLaunchKernelGGL(coef_kernel, dim3((unsigned)grid), dim3(bs), 0, st, L, Li, total, nr, dt, order, ncirc, radius,
-                     out);
+                     nf, out);
   return hipGetLastError();
 }
 

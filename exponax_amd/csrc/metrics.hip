@@ -56,17 +56,18 @@ __global__ void __launch_bounds__(WG) nrmse_kernel(const float* __restrict__ p, 
   if (threadIdx.x == 0) out[b] = (float)total;
 }
 
-// 1-D spectrum of X (count, M+1) complex: power 0.5 |X|^2 / (s_rec s_norm),
-// else |X| / s_rec; s_rec = N at k = 0 and k = N/2, N/2 elsewhere; s_norm = N
-// (build_scaling_array "reconstruction" / "norm_compensation").
-__global__ void spectrum_kernel(const float2* __restrict__ X, int M, long long total, int power, float* out) {
-  const float N = 2.f * M;
+// 1-D spectrum of X (count, N/2+1) complex: power 0.5 |X|^2 / (s_rec s_norm),
+// else |X| / s_rec; s_rec = N at k = 0 and (even N) k = N/2, N/2 elsewhere;
+// s_norm = N (build_scaling_array "reconstruction" / "norm_compensation").
+__global__ void spectrum_kernel(const float2* __restrict__ X, int n, long long total, int power, float* out) {
+  const int nk = n / 2 + 1;
+  const float N = (float)n;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (long long)gridDim.x * blockDim.x) {
-    const int k = (int)(i % (M + 1));
+    const int k = (int)(i % nk);
     const float2 x = X[i];
     const float mag = sqrtf(x.x * x.x + x.y * x.y);
-    const float rec = (k == 0 || k == M) ? N : 0.5f * N;
+    const float rec = (k == 0 || 2 * k == n) ? N : 0.5f * N;
     const float m = mag / rec;
     out[i] = power ? 0.5f * m * (mag / N) : m;
   }
@@ -96,11 +97,11 @@ __global__ void __launch_bounds__(WG) radial_spectrum_kernel(const float2* __res
     const int kz = (int)(m % NZ);
     long long r = m / NZ;
     float k2 = (float)kz * (float)kz;
-    float rec = (kz == 0 || kz == N / 2) ? (float)N : 0.5f * (float)N;
+    float rec = (kz == 0 || 2 * kz == N) ? (float)N : 0.5f * (float)N;
     for (int d = 1; d < D; ++d) {
       const int i = (int)(r % N);
       r /= N;
-      const int k = i < N / 2 ? i : i - N;
+      const int k = i <= (N - 1) / 2 ? i : i - N;
       k2 += (float)k * (float)k;
       rec *= (float)N;
     }
@@ -306,7 +307,7 @@ int exmi_spectrum(int32_t D, int32_t N, int32_t count, const float* x, float* ou
     return exmi::set_error(EXMI_ERR_ALLOC, "exmi_spectrum: workspace allocation failed");
   int rc = exmi_rfft(1, N, count, x, reinterpret_cast<float*>(X), stream);
   if (rc == EXMI_OK) {
-    hipLaunchKernelGGL(spectrum_kernel, dim3(grid_for(total)), dim3(WG), 0, st, X, M, total, power, out);
+    hipLaunchKernelGGL(spectrum_kernel, dim3(grid_for(total)), dim3(WG), 0, st, X, N, total, power, out);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = exmi::set_error(EXMI_ERR_HIP, hipGetErrorString(e));
   }

@@ -125,25 +125,36 @@ hipError_t launch_etdrk0_spec(cf* X, long long ninst, long long nr, int channels
 // convections (_convection.py:104-171), the general form
 // (_general_nonlinear.py) and the coupled reactions.
 struct Mode3D {
-  float kx, ky, kz;  // kappa_d = (2 pi / L) k_d
-  bool keep;         // |k_d| <= kmax on every axis
+  float kx, ky, kz;  // kappa = (2 pi / L) k along the three axes (0 on absent ones)
+  bool keep;         // |k| <= kmax on every axis
   bool zero;         // the mean mode
-  bool inj;          // the Kolmogorov forcing mode (0, inj_mode, 0)
+  bool inj;          // the Kolmogorov forcing mode
 };
+
+// Signed wavenumber of index i on a full (fftfreq) axis of n points: i for
+// i <= (n - 1) / 2, else i - n (even n: the Nyquist index n/2 is -n/2).
+__device__ __forceinline__ int signed_k(int i, int n) { return i <= (n - 1) / 2 ? i : i - n; }
 
 __device__ __forceinline__ Mode3D mode3d(const NL3DArgs& a, long long k) {
   const int iz = (int)(k % a.NZ);
-  const int iy = (int)((k / a.NZ) % a.N);
-  const int ix = (int)(k / ((long long)a.NZ * a.N));
-  const int kx = ix < a.N / 2 ? ix : ix - a.N, ky = iy < a.N / 2 ? iy : iy - a.N;
+  const long long r = k / a.NZ;
+  const int iy = (int)(r % a.ny);
+  const int ix = (int)(r / a.ny);
+  const int kx = signed_k(ix, a.nx), ky = signed_k(iy, a.ny);
   Mode3D m;
   m.kx = a.kscale * (float)kx;
   m.ky = a.kscale * (float)ky;
   m.kz = a.kscale * (float)iz;
   m.keep = (kx < 0 ? -kx : kx) <= a.kmax && (ky < 0 ? -ky : ky) <= a.kmax && iz <= a.kmax;
   m.zero = kx == 0 && ky == 0 && iz == 0;
-  m.inj = kx == 0 && ky == a.inj_mode && iz == 0;
+  m.inj = a.inj_axis == 2 ? (kx == 0 && ky == 0 && iz == a.inj_mode) : (kx == 0 && ky == a.inj_mode && iz == 0);
   return m;
+}
+
+// kappa of spatial derivative d (axis 3 - D + d)
+__device__ __forceinline__ float kap_d(const NL3DArgs& a, const Mode3D& m, int d) {
+  const int axis = 3 - a.dims + d;
+  return axis == 0 ? m.kx : axis == 1 ? m.ky : m.kz;
 }
 
 __device__ __forceinline__ cf imul(float kap, cf w) { return cf{-kap * w.im, kap * w.re}; }
@@ -168,14 +179,21 @@ __global__ void fields3d_kernel(NL3DArgs a, const cf* __restrict__ S, cf* __rest
                 : c == 1 ? imul(m.kz, w0) - imul(m.kx, w2)
                          : imul(m.kx, w1) - imul(m.ky, w0);
     } else if (a.kind == NL_GRADNORM) {
-      f[0] = imul(m.kx, w);
-      f[a.Nr] = imul(m.ky, w);
-      f[2 * a.Nr] = imul(m.kz, w);
+      for (int d = 0; d < a.dims; ++d) f[d * a.Nr] = imul(kap_d(a, m, d), w);
     } else if (a.kind == NL_CONV_NONCONS || a.kind == NL_GENERAL) {
       f[0] = w;
-      f[a.Nr] = imul(m.kx, w);
-      f[2 * a.Nr] = imul(m.ky, w);
-      f[3 * a.Nr] = imul(m.kz, w);
+      for (int d = 0; d < a.dims; ++d) f[(1 + d) * a.Nr] = imul(kap_d(a, m, d), w);
+    } else if (a.kind == NL_VORT) {
+      // 2-D (_vorticity_convection.py:78-99): psi = Laplace^-1 omega (1 at
+      // the mean mode), u = d_y psi, v = -d_x psi, omega_x, omega_y; x is
+      // axis 1, y the last axis
+      const float kx = m.ky, ky = m.kz;
+      const float lap = -(kx * kx + ky * ky);
+      const cf psi = (lap == 0.f ? 1.f : 1.f / lap) * w;
+      f[0] = imul(ky, psi);
+      f[a.Nr] = imul(kx, w);
+      f[2 * a.Nr] = imul(-kx, psi);
+      f[3 * a.Nr] = imul(ky, w);
     } else {
       f[0] = w;
     }
@@ -203,28 +221,39 @@ __global__ void products3d_kernel(NL3DArgs a, const float* __restrict__ R, float
         break;
       }
       case NL_GRADNORM: {
-        const float gx = fld(inst, 0), gy = fld(inst, 1), gz = fld(inst, 2);
-        out[0] = gx * gx + gy * gy + gz * gz;
+        float acc = 0.f;
+        for (int d = 0; d < a.dims; ++d) {
+          const float g = fld(inst, d);
+          acc = fmaf(g, g, acc);
+        }
+        out[0] = acc;
         break;
       }
       case NL_CONV_CONS: {
         const float uc = fld(inst, 0);
-        for (int j = 0; j < 3; ++j) out[(long long)j * a.N3] = uc * fld(base + j, 0);
+        for (int j = 0; j < a.dims; ++j) out[(long long)j * a.N3] = uc * fld(base + j, 0);
         break;
       }
       case NL_CONV_NONCONS: {
         float acc = 0.f;
-        for (int j = 0; j < 3; ++j) acc += fld(base + j, 0) * fld(inst, 1 + j);
+        for (int j = 0; j < a.dims; ++j) acc += fld(base + j, 0) * fld(inst, 1 + j);
         out[0] = acc;
         break;
       }
       case NL_GENERAL: {
-        const float u = fld(inst, 0), gx = fld(inst, 1), gy = fld(inst, 2), gz = fld(inst, 3);
+        const float u = fld(inst, 0);
+        float gs = 0.f, g2 = 0.f;
+        for (int d = 0; d < a.dims; ++d) {
+          const float g = fld(inst, 1 + d);
+          gs += g;
+          g2 = fmaf(g, g, g2);
+        }
         // single-channel non-conservative convection (nonlin_fun/_convection.py:204-231)
-        out[0] = a.prod_nc ? u * (gx + gy + gz) : u * u;
-        out[a.N3] = gx * gx + gy * gy + gz * gz;
+        out[0] = a.prod_nc ? u * gs : u * u;
+        out[a.N3] = g2;
         break;
       }
+      case NL_VORT: out[0] = fld(inst, 0) * fld(inst, 1) + fld(inst, 2) * fld(inst, 3); break;
       case NL_GRAY_SCOTT: {
         float r0, r1;
         gray_scott(fld(base, 0), fld(base + 1, 0), a.feed, a.kill, r0, r1);
@@ -261,10 +290,14 @@ __global__ void combine3d_kernel(NL3DArgs a, const cf* __restrict__ PH, cf* __re
         r = a.poly_lap ? (a.scale * -(m.kx * m.kx + m.ky * m.ky + m.kz * m.kz)) * ph[0] : ph[0];
         break;
       case NL_GRADNORM: r = (a.zero_mode_fix && m.zero) ? cf{0.f, 0.f} : (-0.5f * a.scale) * ph[0]; break;
-      case NL_CONV_CONS:
-        r = (-0.5f * a.scale) * (imul(m.kx, ph[0]) + imul(m.ky, ph[a.Nr]) + imul(m.kz, ph[2 * a.Nr]));
+      case NL_CONV_CONS: {
+        cf acc = cf{0.f, 0.f};
+        for (int j = 0; j < a.dims; ++j) acc = acc + imul(kap_d(a, m, j), ph[j * a.Nr]);
+        r = (-0.5f * a.scale) * acc;
         break;
-      case NL_CONV_NONCONS: r = -a.scale * ph[0]; break;
+      }
+      case NL_CONV_NONCONS:
+      case NL_VORT: r = -a.scale * ph[0]; break;
       case NL_GENERAL: {
         const float cg = (a.zero_mode_fix && m.zero) ? 0.f : 0.5f * a.g2;
         r = a.g0 * ph[0] + imul(0.5f * a.g1 * (m.kx + m.ky + m.kz), ph[0]) + cg * ph[a.Nr];
@@ -285,7 +318,8 @@ __global__ void combine3d_kernel(NL3DArgs a, const cf* __restrict__ PH, cf* __re
       default: r = ph[0];
     }
     Nh[idx] = m.keep ? r : cf{0.f, 0.f};
-    if (a.kind == NL_PROJ3D && m.inj && inst % a.channels == 0) Nh[idx] = Nh[idx] + cf{a.inj_value, 0.f};
+    if ((a.kind == NL_PROJ3D || a.kind == NL_VORT) && m.inj && inst % a.channels == 0)
+      Nh[idx] = Nh[idx] + cf{a.inj_value, 0.f};
   }
 }
 

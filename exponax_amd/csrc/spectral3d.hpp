@@ -5,14 +5,19 @@
 
 namespace exmi {
 
+// A grid of any dimension D <= 3 is held as three axes (nx, ny, N) with the
+// real transform along the last one: D = 3 is (N, N, N), D = 2 (1, N, N),
+// D = 1 (1, 1, N). Spatial derivative d < D acts along axis 3 - D + d.
 struct NL3DArgs {
   int N, NZ, channels, kind, kmax, nf, np;
+  int dims, nx, ny;
   long long Nr, N3, ninst;
   float kscale, scale, g0, g1, g2, feed, kill;
   int zero_mode_fix, npoly, poly_lap;
   int prod_nc;      // general kind: first product u sum_d d_d u instead of u^2
   float poly[EXMI_MAX_POLY];
-  int inj_mode;     // projected convection: Kolmogorov forcing at (0, inj_mode, 0) of channel 0; -1: none
+  int inj_mode;     // Kolmogorov forcing of channel 0 at mode inj_mode of axis inj_axis, 0 on the others; -1: none
+  int inj_axis;     // 1 (3-D projected convection, the y axis) or 2 (2-D vorticity, the last axis)
   float inj_value;  // injection_scale * coef_extraction scaling
 };
 

@@ -108,11 +108,15 @@ enum exmi_nonlin_kind {
  */
 typedef struct exmi_plan_desc {
   int32_t num_spatial_dims;   /* D: 1, 2 or 3 (3: ETDRK0-4, N 16..256)       */
-  int32_t num_points;         /* N: 1-D powers of two 16..4096, 48 and 50;
-                                 2-D powers of two 32..512 (any nonlinear
-                                 kind; two channels for the velocity
-                                 convections, one for vorticity and
-                                 general)                                    */
+  int32_t num_points;         /* N: any size 2..4096 in every dimension.
+                                 The fused kernels cover 1-D powers of two
+                                 16..4096, 48 and 50, 2-D powers of two
+                                 32..512 and 3-D powers of two 16..256; other
+                                 sizes run the multi-pass plan with the
+                                 runtime-size mixed-radix transforms (any
+                                 factorisation, e.g. N = 100, 513 or an odd
+                                 prime), per-step launches, no adjoint and
+                                 no Fourier-resident rollout               */
   int32_t num_channels;       /* C                                           */
   int32_t batch;              /* B                                           */
   int32_t order;              /* ETDRK order 0..4                            */
@@ -148,6 +152,13 @@ typedef struct exmi_plan_desc {
    * (exmi_step_vjp). */
   const double* host_linear_operator_imag;
   double reaction_rates[2];   /* Gray-Scott feed and kill rates              */
+  /* NULL, or (batch,) host float64 factors f_b: trajectory b steps with the
+   * nonlinear term f_b N(u) (the plan's nonlinear parameters times f_b, e.g.
+   * per-trajectory convection or gradient-norm scales in an ensemble,
+   * eqx.filter_vmap over them). Folded into the coefficients that multiply
+   * N (every ETDRK stage is linear in N); needs lin_batch == batch.
+   * exmi_nonlinear returns the unscaled N. */
+  const double* host_nonlin_factor;
 } exmi_plan_desc;
 
 typedef struct exmi_plan exmi_plan;
@@ -275,7 +286,9 @@ int exmi_rollout_vjp(exmi_plan* plan, const float* u0, int64_t u0_bstride, int32
  * order. D = 3: powers of two 16..256 (the inverse through a stream-ordered
  * copy of the spectra). D = 1: powers of two 16..4096, 48 and 50. D = 2: powers of two
  * 32..512, count <= 65535, through a stream-ordered (hipMallocAsync)
- * intermediate of the spectra's size.
+ * intermediate of the spectra's size. Every other N in 2..4096 (any D) runs
+ * the runtime-size mixed-radix transforms (the inverse of D >= 2 through a
+ * stream-ordered copy of the spectra).
  * Replace exponax.fft / exponax.ifft (_spectral.py:614-656, 659-721). */
 int exmi_rfft(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x, float* x_hat,
               void* stream);
@@ -347,7 +360,7 @@ int exmi_ic_tfs(int32_t num_spatial_dims, int32_t num_points, int32_t count, int
  * [b - 1/2, b + 1/2) of the wavenumber norm, b = 0..N/2
  * (radial_binning="sum"; exmi_spectrum_binned with `average`:
  * radial_binning="average", NaN for an empty bucket, as nanmean; D = 1
- * ignores it); D = 2: N 32..512, D = 3: N 16..256.
+ * ignores it); sizes as for exmi_rfft.
  * Replaces exponax.get_spectrum (_spectral.py:866-1030). */
 int exmi_spectrum_binned(int32_t num_spatial_dims, int32_t num_points, int32_t count, const float* x, float* out,
                          int32_t power, int32_t average, void* stream);
@@ -436,6 +449,22 @@ int exmi_plan_get_tables(const exmi_plan* plan, float* host_out, int64_t host_co
  * (No reference counterpart: tuning knobs of this implementation.) */
 int exmi_plan_rollout_strategy(const exmi_plan* plan);
 int exmi_plan_set_rollout_strategy(exmi_plan* plan, int32_t strategy);
+
+/* Transform knobs of a plan on the runtime-size transforms (grid sizes
+ * outside the fused kernels' set): the Stockham radices (passes of at most
+ * max_radix points; a larger prime factor is a direct DFT pass of its own)
+ * and the lines one workgroup holds in LDS. exmi_plan_create times a set of
+ * candidates on the plan's batch and keeps the fastest (with
+ * EXMI_TUNE_CACHE set, it looks the choice up first and stores a new one,
+ * keyed by GPU architecture, D, N and the number of fields).
+ * exmi_plan_fft_config writes [code, source, lines per workgroup, passes,
+ * radix_1 .. radix_passes] into info (up to len entries) and returns the
+ * entry count, 0 for plans on the fused kernels; code = max_radix * 100000
+ * + points per workgroup; source 0 default, 1 timed, 2 tuning cache, 3 set.
+ * exmi_plan_set_fft_config applies a code (drops a cached graph).
+ * (No reference counterpart: XLA picks the reference's FFT plans.) */
+int exmi_plan_fft_config(const exmi_plan* plan, int32_t* info, int32_t len);
+int exmi_plan_set_fft_config(exmi_plan* plan, int32_t code);
 
 /* Message for the last error on this thread ("" if none). */
 const char* exmi_last_error(void);

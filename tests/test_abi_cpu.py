@@ -189,17 +189,24 @@ def test_generic_parametrisations():
         g.GeneralNonlinearStepper(1, 1.0, 32, 0.1, nonlinear_coefficients=(0.0, 1.0))
 
 
-def test_ensemble_rejects_differing_nonlinear_parameters():
-    """Per-trajectory coefficients are built for the linear operator; members
-    that differ in a nonlinear parameter fail loudly instead of silently
-    taking the first member's."""
-    import pytest
+def test_ensemble_nonlinear_parameters_factor_or_group():
+    """Members whose nonlinear terms differ by a common factor (a convection
+    scale, the polynomial coefficients together) fold it into per-trajectory
+    coefficient tables (one plan); members that differ otherwise (one
+    coefficient of several, reaction rates) run as groups of equal
+    parameters. The split is decided on the host."""
+    import numpy as np
 
     import exponax_amd as ex
 
     ok = ex.ensemble(lambda nu: ex.stepper.Burgers(1, 1.0, 64, 1e-3, diffusivity=nu), [0.01, 0.02])
-    assert ok.batch == 2
-    with pytest.raises(NotImplementedError):
-        ex.ensemble(lambda b: ex.stepper.Burgers(1, 1.0, 64, 1e-3, convection_scale=b), [1.0, 0.5])
-    with pytest.raises(NotImplementedError):
-        ex.ensemble(lambda f: ex.stepper.reaction.GrayScott(1, 1.0, 64, 0.5, feed_rate=f), [0.04, 0.05])
+    assert ok.batch == 2 and ok._factors is None and ok._groups is None
+    sc = ex.ensemble(lambda b: ex.stepper.Burgers(1, 1.0, 64, 1e-3, convection_scale=b), [1.0, 0.5, -2.0])
+    assert sc._groups is None and np.allclose(sc._factors, [1.0, 0.5, -2.0])
+    fk = ex.ensemble(lambda r: ex.stepper.reaction.FisherKPP(1, 1.0, 64, 1e-3, reactivity=r), [1.0, 3.0])
+    assert fk._groups is None and np.allclose(fk._factors, [1.0, 3.0])
+    gs = ex.ensemble(lambda f: ex.stepper.reaction.GrayScott(1, 1.0, 64, 0.5, feed_rate=f), [0.04, 0.05, 0.04])
+    assert gs._factors is None and sorted(len(ix) for ix, _ in gs._groups) == [1, 2]
+    gen = ex.ensemble(lambda c: ex.stepper.generic.GeneralNonlinearStepper(
+        1, 1.0, 64, 1e-3, linear_coefficients=(0.0, 0.0, 0.01), nonlinear_coefficients=(c, -1.0, 0.0)), [0.0, 0.5])
+    assert gen._factors is None and len(gen._groups) == 2

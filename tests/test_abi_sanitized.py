@@ -54,9 +54,9 @@ int main(void) {
   exmi_plan_desc d;
   report("null desc", exmi_plan_create(0, &p));
   d = base(4, 64); d.host_linear_operator = L; report("D=4", exmi_plan_create(&d, &p));
-  d = base(1, 100); d.host_linear_operator = L; report("1-D N=100", exmi_plan_create(&d, &p));
-  d = base(2, 96); d.host_linear_operator = L; report("2-D N=96", exmi_plan_create(&d, &p));
-  d = base(3, 8); d.host_linear_operator = L; report("3-D N=8", exmi_plan_create(&d, &p));
+  d = base(1, 5000); d.host_linear_operator = L; report("1-D N=5000", exmi_plan_create(&d, &p));
+  d = base(2, 1); d.host_linear_operator = L; report("2-D N=1", exmi_plan_create(&d, &p));
+  d = base(3, 4097); d.host_linear_operator = L; report("3-D N=4097", exmi_plan_create(&d, &p));
   d = base(1, 64); d.host_linear_operator = L; d.order = 5; report("order 5", exmi_plan_create(&d, &p));
   d = base(1, 64); d.host_linear_operator = L; d.lin_batch = 3; report("lin_batch", exmi_plan_create(&d, &p));
   d = base(1, 64); report("no L", exmi_plan_create(&d, &p));
@@ -78,8 +78,12 @@ int main(void) {
   report("repeat null", exmi_repeat(0, 0, 0, 0, 1, 1, 0));
   report("strategy null", exmi_plan_set_rollout_strategy(0, 1));
   report("rfft D=4", exmi_rfft(4, 64, 1, 0, 0, 0));
-  report("rfft N=100", exmi_rfft(1, 100, 1, 0, 0, 0));
-  report("irfft 2-D N=1024", exmi_irfft(2, 1024, 1, 0, 0, 0));
+  report("rfft N=5000", exmi_rfft(1, 5000, 1, 0, 0, 0));
+  report("irfft 2-D N=1", exmi_irfft(2, 1, 1, 0, 0, 0));
+  d = base(2, 100); d.host_linear_operator = L; report("valid 2-D N=100", exmi_plan_create(&d, &p));
+  d = base(1, 11); d.host_linear_operator = L; report("valid 1-D N=11", exmi_plan_create(&d, &p));
+  report("fft_config null", exmi_plan_fft_config(0, 0, 0));
+  report("set_fft_config null", exmi_plan_set_fft_config(0, 800512));
   report("irfft_noalloc", exmi_irfft_noalloc(2, 64, 1, 0, 0, 0));
   report("nrmse null", exmi_nrmse(1, 1, 8, 0, 8, 0, 8, 0, 0));
   report("first_bad null", exmi_first_bad_step(1, 1, 8, 0, 8, 8, 0.f, 0, 0));
@@ -123,19 +127,20 @@ def test_abi_layer_under_address_and_ub_sanitizers(tmp_path):
     msg = {k: v.split("msg=", 1)[1] for k, v in (ln.split(": rc=", 1) for ln in p.stdout.splitlines() if ": rc=" in ln)}
     invalid, hip, unsupported = -1, -2, -4
     expected = {
-        "null desc": invalid, "D=4": unsupported, "1-D N=100": unsupported, "2-D N=96": unsupported,
-        "3-D N=8": unsupported, "order 5": unsupported, "lin_batch": invalid, "no L": invalid, "flags": invalid,
+        "null desc": invalid, "D=4": unsupported, "1-D N=5000": unsupported, "2-D N=1": unsupported,
+        "3-D N=4097": unsupported, "order 5": unsupported, "lin_batch": invalid, "no L": invalid, "flags": invalid,
         "2-D convection C=3": invalid, "polynomial 9": unsupported,
         # valid descriptors: no device here, so the set-up fails at its first HIP call and cleans up
         "valid 1-D": hip, "valid 2-D": hip, "valid 3-D": hip, "irfft_noalloc": hip,
         "destroy null": 0, "step null": invalid, "step_fourier null": invalid, "rollout null": invalid,
         "rollout_checked null": invalid, "repeat null": invalid, "strategy null": invalid,
-        "rfft D=4": unsupported, "rfft N=100": unsupported, "irfft 2-D N=1024": unsupported,
+        "rfft D=4": unsupported, "rfft N=5000": unsupported, "irfft 2-D N=1": unsupported,
+        "valid 2-D N=100": hip, "valid 1-D N=11": hip, "fft_config null": invalid, "set_fft_config null": invalid,
         "nrmse null": invalid, "first_bad null": invalid, "poisson odd order": invalid,
         "map_resolution D=5": unsupported, "info": 0,
     }
     assert rc == expected, rc
-    assert "supported sizes" in msg["1-D N=100"] and "lin_batch" in msg["lin_batch"]
+    assert "supported sizes are 2..4096" in msg["1-D N=5000"] and "lin_batch" in msg["lin_batch"]
     assert "exclude each other" in msg["flags"]
-    assert "checks=30" in p.stdout
+    assert "checks=34" in p.stdout
     shutil.rmtree(tmp_path, ignore_errors=True)
