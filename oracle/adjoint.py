@@ -127,22 +127,50 @@ def nonlin_vjp(fun, u_hat, m_hat):
 
 # ------------------------------------------------------------------ steps
 def step_fourier_jvp(integ, u_hat, d_hat):
-    """Derivative of the ETDRK0/1/2 step_fourier along d_hat
-    (etdrk/_etdrk_0.py:33, _etdrk_1.py:78-82, _etdrk_2.py:91-102)."""
+    """Derivative of the ETDRK0-4 step_fourier along d_hat
+    (etdrk/_etdrk_0.py:33, _etdrk_1.py:78-82, _etdrk_2.py:91-102,
+    _etdrk_3.py:191-212, _etdrk_4.py:198-224; the stage names follow
+    oracle/etdrk.py)."""
     E = integ._exp_term
     if isinstance(integ, et.ETDRK0):
         return E * d_hat
     f = integ._nonlinear_fun
+    jv = lambda w, v: nonlin_jvp(f, w, v)  # noqa: E731
     if isinstance(integ, et.ETDRK1):
-        return E * d_hat + integ._coef_1 * nonlin_jvp(f, u_hat, d_hat)
+        return E * d_hat + integ._coef_1 * jv(u_hat, d_hat)
     if isinstance(integ, et.ETDRK2):
         c1, c2 = integ._coef_1, integ._coef_2
         n0 = f(u_hat)
         s1 = E * u_hat + c1 * n0
-        dn0 = nonlin_jvp(f, u_hat, d_hat)
+        dn0 = jv(u_hat, d_hat)
         ds1 = E * d_hat + c1 * dn0
-        dn1 = nonlin_jvp(f, s1, ds1)
+        dn1 = jv(s1, ds1)
         return ds1 + c2 * (dn1 - dn0)
+    Eh = integ._half_exp_term
+    if isinstance(integ, et.ETDRK3):
+        c1, c2, c3, c4, c5 = integ._coef_1, integ._coef_2, integ._coef_3, integ._coef_4, integ._coef_5
+        n0 = f(u_hat)
+        ua = Eh * u_hat + c1 * n0
+        n1 = f(ua)
+        ub = E * u_hat + c2 * (2 * n1 - n0)
+        dn0 = jv(u_hat, d_hat)
+        dn1 = jv(ua, Eh * d_hat + c1 * dn0)
+        dn2 = jv(ub, E * d_hat + c2 * (2 * dn1 - dn0))
+        return E * d_hat + c3 * dn0 + c4 * dn1 + c5 * dn2
+    if isinstance(integ, et.ETDRK4):
+        c1, c4, c5, c6 = integ._coef_1, integ._coef_4, integ._coef_5, integ._coef_6
+        n0 = f(u_hat)
+        ua = Eh * u_hat + c1 * n0
+        n1 = f(ua)
+        ub = Eh * u_hat + c1 * n1
+        n2 = f(ub)
+        uc = Eh * ua + c1 * (2 * n2 - n0)
+        dn0 = jv(u_hat, d_hat)
+        dua = Eh * d_hat + c1 * dn0
+        dn1 = jv(ua, dua)
+        dn2 = jv(ub, Eh * d_hat + c1 * dn1)
+        dn3 = jv(uc, Eh * dua + c1 * (2 * dn2 - dn0))
+        return E * d_hat + c4 * dn0 + 2 * c5 * (dn1 + dn2) + c6 * dn3
     raise NotImplementedError(type(integ).__name__)
 
 
@@ -152,20 +180,53 @@ def step_fourier_vjp(integ, u_hat, lam):
     ETDRK2, with N0 = N(u), S1 = E u + c1 N0, u' = S1 + c2 (N(S1) - N0):
         rho = lam + J(S1)^T (c2 lam)
         out = E rho + J(u)^T (c1 rho - c2 lam)
-    The coefficients are real for real linear operators; conj() keeps the
-    formula right for complex ones.
+    ETDRK3 (ua = Eh u + c1 N0, ub = E u + c2 (2 N(ua) - N0),
+    u' = E u + c3 N0 + c4 N(ua) + c5 N(ub)):
+        l_ub = J(ub)^T (c5 lam)
+        l_ua = J(ua)^T (c4 lam + 2 c2 l_ub)
+        out = E (lam + l_ub) + Eh l_ua + J(u)^T (c3 lam - c2 l_ub + c1 l_ua)
+    ETDRK4 (ua = Eh u + c1 N0, ub = Eh u + c1 N(ua),
+    uc = Eh ua + c1 (2 N(ub) - N0),
+    u' = E u + c4 N0 + 2 c5 (N(ua) + N(ub)) + c6 N(uc)):
+        l_uc = J(uc)^T (c6 lam)
+        l_ub = J(ub)^T (2 c5 lam + 2 c1 l_uc)
+        l_ua = Eh l_uc + J(ua)^T (2 c5 lam + c1 l_ub)
+        out = E lam + Eh (l_ub + l_ua) + J(u)^T (c4 lam - c1 l_uc + c1 l_ua)
+    Every multiplier is conjugated: the coefficients are real for real
+    linear operators, and conj() keeps the formulas right for complex ones.
     """
     E = np.conj(integ._exp_term)
     if isinstance(integ, et.ETDRK0):
         return E * lam
     f = integ._nonlinear_fun
+    vj = lambda w, m: nonlin_vjp(f, w, m)  # noqa: E731
     if isinstance(integ, et.ETDRK1):
-        return E * lam + nonlin_vjp(f, u_hat, np.conj(integ._coef_1) * lam)
+        return E * lam + vj(u_hat, np.conj(integ._coef_1) * lam)
     if isinstance(integ, et.ETDRK2):
         c1, c2 = np.conj(integ._coef_1), np.conj(integ._coef_2)
         s1 = integ._exp_term * u_hat + integ._coef_1 * f(u_hat)
-        rho = lam + nonlin_vjp(f, s1, c2 * lam)
-        return E * rho + nonlin_vjp(f, u_hat, c1 * rho - c2 * lam)
+        rho = lam + vj(s1, c2 * lam)
+        return E * rho + vj(u_hat, c1 * rho - c2 * lam)
+    Eh = np.conj(integ._half_exp_term)
+    if isinstance(integ, et.ETDRK3):
+        n0 = f(u_hat)
+        ua = integ._half_exp_term * u_hat + integ._coef_1 * n0
+        ub = integ._exp_term * u_hat + integ._coef_2 * (2 * f(ua) - n0)
+        c1, c2, c3, c4, c5 = (np.conj(c) for c in (integ._coef_1, integ._coef_2, integ._coef_3, integ._coef_4,
+                                                      integ._coef_5))
+        l_ub = vj(ub, c5 * lam)
+        l_ua = vj(ua, c4 * lam + 2 * c2 * l_ub)
+        return E * (lam + l_ub) + Eh * l_ua + vj(u_hat, c3 * lam - c2 * l_ub + c1 * l_ua)
+    if isinstance(integ, et.ETDRK4):
+        n0 = f(u_hat)
+        ua = integ._half_exp_term * u_hat + integ._coef_1 * n0
+        ub = integ._half_exp_term * u_hat + integ._coef_1 * f(ua)
+        uc = integ._half_exp_term * ua + integ._coef_1 * (2 * f(ub) - n0)
+        c1, c4, c5, c6 = (np.conj(c) for c in (integ._coef_1, integ._coef_4, integ._coef_5, integ._coef_6))
+        l_uc = vj(uc, c6 * lam)
+        l_ub = vj(ub, 2 * c5 * lam + 2 * c1 * l_uc)
+        l_ua = Eh * l_uc + vj(ua, 2 * c5 * lam + c1 * l_ub)
+        return E * lam + Eh * (l_ub + l_ua) + vj(u_hat, c4 * lam - c1 * l_uc + c1 * l_ua)
     raise NotImplementedError(type(integ).__name__)
 
 

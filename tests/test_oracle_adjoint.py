@@ -120,6 +120,14 @@ def _steppers():
         ("dispersion1d", st.Dispersion(1, 1.0, 64, 1e-3, dispersivity=0.01), 1),
         ("general1d_complex", st.GeneralStepper(1, 2.0, 64, 1e-3, linear_coefficients=(0.0, 0.1, 0.01),
                                                 nonlinear_scales=(0.3, -1.0, -0.5)), 1),
+        # ETDRK3 and ETDRK4 (etdrk/_etdrk_3.py:191-212, _etdrk_4.py:198-224)
+        ("ks1d_order3", st.KuramotoSivashinsky(1, 30.0, 64, 0.05, order=3), 1),
+        ("ks1d_order4", st.KuramotoSivashinsky(1, 30.0, 64, 0.05, order=4), 1),
+        ("kdv1d_order4", st.KortewegDeVries(1, 20.0, 64, 1e-3, order=4), 1),
+        ("burgers2d_order3", st.Burgers(2, 1.0, 16, 1e-3, diffusivity=0.02, order=3), 2),
+        ("kolmogorov_order4", st.KolmogorovFlowVorticity(2, 2 * np.pi, 16, 0.01, diffusivity=0.01, order=4), 1),
+        ("general2d_order4", st.GeneralStepper(2, 2.0, 16, 1e-3, linear_coefficients=(0.0, 0.1, 0.01),
+                                               nonlinear_scales=(0.3, -1.0, -0.5), order=4), 1),
     ]
 
 
