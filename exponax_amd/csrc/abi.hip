@@ -10,6 +10,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <initializer_list>
 #include <vector>
 
 #include <climits>
@@ -44,6 +45,10 @@ hipError_t launch_axpy(long long n, float a, const float* x, const float* y, flo
 hipError_t launch_products3d(const NL3DArgs& a, const float* R, float* PR, hipStream_t st);
 hipError_t launch_combine3d(const NL3DArgs& a, const cf* PH, cf* Nh, hipStream_t st);
 hipError_t launch_stage3d(const Stage3DArgs& a, hipStream_t st);
+hipError_t launch_adj_cofields3d(const NL3DArgs& a, const cf* MU, cf* H, hipStream_t st);
+hipError_t launch_adj_products3d(const NL3DArgs& a, const float* R, const float* HR, float* Q, hipStream_t st);
+hipError_t launch_adj_combine3d(const NL3DArgs& a, const cf* QH, cf* out, hipStream_t st);
+hipError_t launch_adj_linear3d(const AdjLinArgs& a, hipStream_t st);
 
 #define EXMI_DECL_SIZE(M)                                                                              \
   StepFn get_step1d_##M(int kind, int order);                                                          \
@@ -374,6 +379,18 @@ struct exmi_plan {
   GfftCfg gcfg;          // runtime-size transforms: radices and lines per workgroup
   int gcode = 0;         // their knobs, encoded (autotuned at plan creation)
   int gtuned = 0;        // 0: default knobs, 1: timed at plan creation, 2: from the tuning cache, 3: set
+  // multi-pass adjoint: stage states, cotangents and the nonlinear VJP's
+  // cotangent fields / products (allocated on first use)
+  cf *XA3 = nullptr, *XB3 = nullptr, *XC3 = nullptr, *LAM3 = nullptr, *MU3 = nullptr, *T3 = nullptr;
+  cf *LA3 = nullptr, *LB3 = nullptr, *LC3 = nullptr, *FH3 = nullptr;
+  float *RH3 = nullptr, *Q3 = nullptr;
+  int adj_nh = 0, adj_nq = 0;
+  // fused plans whose adjoint the fused kernels do not cover: the
+  // description and host operator tables kept to build a multi-pass twin
+  // plan of the same stepper on first use
+  exmi_plan* twin = nullptr;
+  exmi_plan_desc d0{};
+  std::vector<double> hL, hLi, hF;
   Launch2D row2d[ROW_NUM_MODES] = {};
   LaunchRoll2D roll2d = nullptr;  // 2-D workgroup-resident rollout (strategy 1)
   int fr2d = 0;                   // 2-D spectrum-resident launch chain available (strategy 2)
@@ -514,6 +531,7 @@ static int kernel_kind(const exmi_plan_desc* d, int* kind) {
 
 int exmi_plan_destroy(exmi_plan* p);
 static int ensure_work(exmi_plan* p);
+static void keep_for_twin(exmi_plan* p, const exmi_plan_desc* d, long long total);
 
 // 2-D plans: ETDRK0-4 with the vorticity (Kolmogorov), gradient-norm,
 // polynomial, two-channel convection or general nonlinearity (or none), N a
@@ -711,6 +729,7 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
     exmi_plan_destroy(p);
     return fail(EXMI_ERR_HIP, "stream creation failed");
   }
+  if (!p->rowA[0]) keep_for_twin(p, d, rows * full);  // the multi-pass adjoint, on first use
   *out = p;
   return EXMI_OK;
 }
@@ -975,10 +994,8 @@ static hipError_t irfft_md(const exmi_plan* p, long long I, cf* X, float* x, hip
   return irfft3(p->d.num_points, I, X, x, p->tw, p->tw_col, st);
 }
 
-// N(X) of every instance into outN (the four passes of spectral3d.hip).
-static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t st) {
+static void fill_nl3d(const exmi_plan* p, NL3DArgs& a) {
   const int N = p->d.num_points;
-  NL3DArgs a;
   memset(&a, 0, sizeof a);
   a.N = N;
   a.NZ = N / 2 + 1;
@@ -1008,6 +1025,14 @@ static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t s
   a.inj_mode = p->inj_mode;
   a.inj_axis = p->kind3 == NL_VORT ? 2 : 1;
   a.inj_value = p->inj_value;
+  a.nh = p->adj_nh;
+  a.nq = p->adj_nq;
+}
+
+// N(X) of every instance into outN (the four passes of spectral3d.hip).
+static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t st) {
+  NL3DArgs a;
+  fill_nl3d(p, a);
   hipError_t e = launch_fields3d(a, X, p->F3, st);
   if (e == hipSuccess) e = irfft_md(p, p->ninst * p->nf3, p->F3, p->R3, st);
   if (e == hipSuccess) e = launch_products3d(a, p->R3, p->PR3, st);
@@ -1093,6 +1118,269 @@ static int enqueue_step3d(exmi_plan* p, const float* u_in, long long ibs, float*
     if (e == hipSuccess && dst != u_out) e = copy_batch(p->P3, sz, u_out, obs, sz, B, st);
   }
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "%d-D step launch failed: %s", p->ndim, hipGetErrorString(e));
+}
+
+// ------------------------------------------------------- multi-pass adjoint
+// The transposed ETDRK0-4 step on whole spectra (etdrk/_etdrk_{0,1,2,3,4}.py;
+// the reverse-mode derivative jax.grad takes of step_fourier), for every
+// multi-pass plan whose nonlinear kind has a transpose (adj3d_fields), and,
+// through a multi-pass twin of the same stepper, for fused plans the fused
+// adjoint kernels do not cover (1-D ETDRK3/4, 2-D kinds other than the
+// gradient norm, 2-D ETDRK orders other than 2). With lam the cotangent of
+// the new state and J(x)^T the nonlinear transpose at x (all multipliers
+// conjugated; the phi-coefficients c_i are real):
+//   ETDRK1: E lam + J(u)^T (c1 lam)
+//   ETDRK2 (ua = E u + c1 N(u)): rho = lam + J(ua)^T (c2 lam),
+//           E rho + J(u)^T (c1 rho - c2 lam)
+//   ETDRK3 (ua = Eh u + c2 N0, ub = E u + c3 (2 N(ua) - N0),
+//           u' = E u + c4 N0 + c5 N(ua) + c6 N(ub); table indices):
+//           l_ub = J(ub)^T (c6 lam), l_ua = J(ua)^T (c5 lam + 2 c3 l_ub),
+//           E (lam + l_ub) + Eh l_ua + J(u)^T (c4 lam - c3 l_ub + c2 l_ua)
+//   ETDRK4 (ua = Eh u + c2 N0, ub = Eh u + c2 N(ua), uc = Eh ua + c2 (2 N(ub) - N0),
+//           u' = E u + c3 N0 + 2 c4 (N(ua) + N(ub)) + c5 N(uc)):
+//           l_uc = J(uc)^T (c5 lam), l_ub = J(ub)^T (2 c4 lam + 2 c2 l_uc),
+//           l_ua = Eh l_uc + J(ua)^T (2 c4 lam + c2 l_ub),
+//           E lam + Eh (l_ub + l_ua) + J(u)^T (c3 lam - c2 l_uc + c2 l_ua)
+// The stage states ua, ub, uc are recomputed from u with the forward stage
+// kernels.
+static bool md_adjoint_ok(const exmi_plan* p) {
+  if (p->d.order == 0) return true;
+  int nh = 0, nq = 0;
+  return adj3d_fields(p->kind3, p->ndim, p->d.nonlin_kind == EXMI_NL_CAHN_HILLIARD, &nh, &nq) > 0;
+}
+
+static int ensure_adj_md(exmi_plan* p) {
+  if (p->LAM3) return EXMI_OK;
+  int nh = 0, nq = 0;
+  const int order = p->d.order;
+  if (order > 0) adj3d_fields(p->kind3, p->ndim, p->d.nonlin_kind == EXMI_NL_CAHN_HILLIARD, &nh, &nq);
+  p->adj_nh = nh;
+  p->adj_nq = nq;
+  const long long I = p->ninst, sp = I * p->nr, nfh = nh > nq ? nh : nq;
+  auto al = [&](cf** b, bool need) { return !need || hipMalloc(b, sizeof(cf) * sp) == hipSuccess; };
+  bool ok = al(&p->LAM3, true) && al(&p->MU3, true) && al(&p->T3, true) && al(&p->XA3, order >= 2) &&
+            al(&p->LA3, order >= 2) && al(&p->XB3, order >= 3) && al(&p->LB3, order >= 3) &&
+            al(&p->XC3, order >= 4) && al(&p->LC3, order >= 4) &&
+            (order == 0 || (hipMalloc(&p->FH3, sizeof(cf) * nfh * sp) == hipSuccess &&
+                            hipMalloc(&p->RH3, sizeof(float) * I * nh * p->n3) == hipSuccess &&
+                            hipMalloc(&p->Q3, sizeof(float) * I * nq * p->n3) == hipSuccess));
+  if (!ok) {
+    for (cf** b : {&p->XA3, &p->XB3, &p->XC3, &p->LAM3, &p->MU3, &p->T3, &p->LA3, &p->LB3, &p->LC3, &p->FH3}) {
+      hipFree(*b);
+      *b = nullptr;
+    }
+    hipFree(p->RH3);
+    hipFree(p->Q3);
+    p->RH3 = p->Q3 = nullptr;
+    return fail(EXMI_ERR_ALLOC, "adjoint workspace allocation failed (batch %d, N %d)", p->d.batch,
+                p->d.num_points);
+  }
+  return EXMI_OK;
+}
+
+// out = J(W)^T MU for every instance (out aliases neither W nor MU).
+static hipError_t nonlinear_vjp3d(exmi_plan* p, const cf* W, const cf* MU, cf* out, hipStream_t st) {
+  NL3DArgs a;
+  fill_nl3d(p, a);
+  hipError_t e = launch_fields3d(a, W, p->F3, st);
+  if (e == hipSuccess) e = irfft_md(p, p->ninst * p->nf3, p->F3, p->R3, st);
+  if (e == hipSuccess) e = launch_adj_cofields3d(a, MU, p->FH3, st);
+  if (e == hipSuccess) e = irfft_md(p, p->ninst * p->adj_nh, p->FH3, p->RH3, st);
+  if (e == hipSuccess) e = launch_adj_products3d(a, p->R3, p->RH3, p->Q3, st);
+  if (e == hipSuccess) e = rfft_md(p, p->ninst * p->adj_nq, p->Q3, p->FH3, st);
+  if (e == hipSuccess) e = launch_adj_combine3d(a, p->FH3, out, st);
+  return e;
+}
+
+struct AdjTerm {
+  const cf* x;
+  int op;  // -1 identity, 100 conj(E), 101 conj(E_half), i >= 0 the real coefficient i
+  float mult;
+};
+
+static hipError_t adj_lin(const exmi_plan* p, cf* out, std::initializer_list<AdjTerm> terms, hipStream_t st) {
+  AdjLinArgs a;
+  memset(&a, 0, sizeof a);
+  a.ninst = p->ninst;
+  a.nr = p->nr;
+  a.channels = p->d.num_channels;
+  a.ncoef = p->ncoef;
+  a.lin_batch = p->d.lin_batch;
+  a.lin_channels = p->d.lin_channels;
+  a.e_im = p->cplx ? num_coefs(p->d.order) : -1;
+  a.tables = p->tables;
+  if (terms.size() > 4) return hipErrorInvalidValue;
+  for (const AdjTerm& t : terms) {
+    const bool ok = t.op == -1 || (t.op >= 0 && t.op < num_coefs(p->d.order)) || t.op == 100 ||
+                    (t.op == 101 && p->d.order >= 3);
+    if (!ok) return hipErrorInvalidValue;  // would read outside the tables
+    a.x[a.nt] = t.x;
+    a.op[a.nt] = t.op;
+    a.mult[a.nt] = t.mult;
+    ++a.nt;
+  }
+  a.out = out;
+  return launch_adj_linear3d(a, st);
+}
+
+static int enqueue_vjp_md(exmi_plan* p, const float* u, long long ubs, const float* g, long long gbs, float* gout,
+                          long long obs, hipStream_t st) {
+  const int B = p->d.batch, order = p->d.order;
+  const long long sz = (long long)p->d.num_channels * p->n3;
+  hipError_t e = hipSuccess;
+  auto to_spec = [&](const float* x, long long xbs, cf* X) {
+    const float* src = x;
+    if (e == hipSuccess && xbs != sz) {
+      e = copy_batch(x, xbs, p->P3, sz, sz, B, st);
+      src = p->P3;
+    }
+    if (e == hipSuccess) e = rfft_md(p, p->ninst, src, X, st);
+  };
+  to_spec(u, ubs, p->W3);
+  to_spec(g, gbs, p->LAM3);
+  cf *U = p->W3, *LAM = p->LAM3, *MU = p->MU3, *T = p->T3;
+  auto vjp = [&](const cf* W, const cf* M, cf* out) {
+    if (e == hipSuccess) e = nonlinear_vjp3d(p, W, M, out, st);
+  };
+  auto lin = [&](cf* out, std::initializer_list<AdjTerm> terms) {
+    if (e == hipSuccess) e = adj_lin(p, out, terms, st);
+  };
+  if (order >= 2) {  // the forward stage states
+    Stage3DArgs sa;
+    sa.ninst = p->ninst;
+    sa.nr = p->nr;
+    sa.channels = p->d.num_channels;
+    sa.ncoef = p->ncoef;
+    sa.lin_batch = p->d.lin_batch;
+    sa.lin_channels = p->d.lin_channels;
+    sa.e_im = p->cplx ? num_coefs(order) : -1;
+    sa.tables = p->tables;
+    sa.U0 = U;
+    sa.N0 = p->N03;
+    sa.NC = p->NC3;
+    sa.S1 = p->S13;
+    sa.A = p->A3;
+    auto stage = [&](int mode, cf* S) {
+      sa.S = S;
+      sa.mode = mode;
+      if (e == hipSuccess) e = launch_stage3d(sa, st);
+    };
+    auto nlev = [&](const cf* in, cf* outN) {
+      if (e == hipSuccess) e = nonlinear3d(p, in, outN, st);
+    };
+    nlev(U, p->N03);
+    if (order == 2) {
+      stage(ST2A, p->XA3);
+    } else if (order == 3) {
+      stage(ST3A, p->XA3);
+      nlev(p->XA3, p->NC3);
+      stage(ST3B, p->XB3);
+    } else {
+      stage(ST4A, p->XA3);
+      nlev(p->XA3, p->NC3);
+      stage(ST4B, p->XB3);
+      nlev(p->XB3, p->NC3);
+      stage(ST4C, p->XC3);
+    }
+  }
+  cf* OUT = MU;  // the last combination reads neither MU nor OUT
+  switch (order) {
+    case 0: lin(OUT, {{LAM, 100, 1.f}}); break;
+    case 1:
+      lin(MU, {{LAM, 1, 1.f}});
+      vjp(U, MU, T);
+      lin(OUT, {{LAM, 100, 1.f}, {T, -1, 1.f}});
+      break;
+    case 2: {
+      cf* RHO = p->LA3;
+      lin(MU, {{LAM, 2, 1.f}});
+      vjp(p->XA3, MU, T);
+      lin(RHO, {{LAM, -1, 1.f}, {T, -1, 1.f}});
+      lin(MU, {{RHO, 1, 1.f}, {LAM, 2, -1.f}});
+      vjp(U, MU, T);
+      lin(OUT, {{RHO, 100, 1.f}, {T, -1, 1.f}});
+      break;
+    }
+    case 3:
+      lin(MU, {{LAM, 6, 1.f}});
+      vjp(p->XB3, MU, p->LB3);
+      lin(MU, {{LAM, 5, 1.f}, {p->LB3, 3, 2.f}});
+      vjp(p->XA3, MU, p->LA3);
+      lin(MU, {{LAM, 4, 1.f}, {p->LB3, 3, -1.f}, {p->LA3, 2, 1.f}});
+      vjp(U, MU, T);
+      lin(OUT, {{LAM, 100, 1.f}, {p->LB3, 100, 1.f}, {p->LA3, 101, 1.f}, {T, -1, 1.f}});
+      break;
+    default:
+      lin(MU, {{LAM, 5, 1.f}});
+      vjp(p->XC3, MU, p->LC3);
+      lin(MU, {{LAM, 4, 2.f}, {p->LC3, 2, 2.f}});
+      vjp(p->XB3, MU, p->LB3);
+      lin(MU, {{LAM, 4, 2.f}, {p->LB3, 2, 1.f}});
+      vjp(p->XA3, MU, T);
+      lin(p->LA3, {{p->LC3, 101, 1.f}, {T, -1, 1.f}});
+      lin(MU, {{LAM, 3, 1.f}, {p->LC3, 2, -1.f}, {p->LA3, 2, 1.f}});
+      vjp(U, MU, T);
+      lin(OUT, {{LAM, 100, 1.f}, {p->LB3, 101, 1.f}, {p->LA3, 101, 1.f}, {T, -1, 1.f}});
+      break;
+  }
+  float* dst = obs == sz ? gout : p->P3;
+  if (e == hipSuccess) e = irfft_md(p, p->ninst, OUT, dst, st);
+  if (e == hipSuccess && dst != gout) e = copy_batch(p->P3, sz, gout, obs, sz, B, st);
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "adjoint step launch failed: %s", hipGetErrorString(e));
+}
+
+// Fused plans without a fused adjoint build a multi-pass twin of the same
+// stepper (same description and operator tables) the first time an adjoint
+// is asked for; synchronous.
+static bool twin_kind_ok(const exmi_plan_desc* d) {
+  if (d->order == 0) return true;
+  switch (d->nonlin_kind) {
+    case EXMI_NL_ZERO:
+    case EXMI_NL_CONVECTION:
+    case EXMI_NL_GRADIENT_NORM:
+    case EXMI_NL_POLYNOMIAL:
+    case EXMI_NL_GENERAL:
+    case EXMI_NL_VORTICITY:
+    case EXMI_NL_KOLMOGOROV: return true;
+    default: return false;
+  }
+}
+
+// Keep what the twin needs (called at the end of a fused plan's creation).
+static void keep_for_twin(exmi_plan* p, const exmi_plan_desc* d, long long total) {
+  if (!twin_kind_ok(d)) return;
+  p->d0 = *d;
+  p->d0.host_linear_operator = nullptr;
+  p->d0.host_linear_operator_imag = nullptr;
+  p->d0.host_nonlin_factor = nullptr;
+  p->hL.assign(d->host_linear_operator, d->host_linear_operator + total);
+  if (d->host_linear_operator_imag)
+    p->hLi.assign(d->host_linear_operator_imag, d->host_linear_operator_imag + total);
+  if (d->host_nonlin_factor) p->hF.assign(d->host_nonlin_factor, d->host_nonlin_factor + d->batch);
+}
+
+static int ensure_twin(exmi_plan* p) {
+  if (p->twin) return EXMI_OK;
+  if (p->hL.empty()) return fail(EXMI_ERR_UNSUPPORTED, "no adjoint for this plan");
+  exmi_plan_desc d = p->d0;
+  d.host_linear_operator = p->hL.data();
+  d.host_linear_operator_imag = p->hLi.empty() ? nullptr : p->hLi.data();
+  d.host_nonlin_factor = p->hF.empty() ? nullptr : p->hF.data();
+  d.flags &= ~(EXMI_FLAG_FOURIER_RESIDENT | EXMI_FLAG_PHYSICAL_ROLLOUT);
+  exmi_plan* t = nullptr;
+  int rc = plan_create_md(&d, &t, true);
+  if (rc) return rc;
+  if (!md_adjoint_ok(t)) {
+    exmi_plan_destroy(t);
+    return fail(EXMI_ERR_UNSUPPORTED, "no adjoint for this plan's nonlinear function");
+  }
+  rc = ensure_adj_md(t);
+  if (rc) {
+    exmi_plan_destroy(t);
+    return rc;
+  }
+  p->twin = t;
+  return EXMI_OK;
 }
 
 static int enqueue_step2d(exmi_plan* p, const float* u_in, long long ibs, float* u_out, long long obs, bool have_T,
@@ -1334,6 +1622,7 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
     exmi_plan_destroy(p);
     return fail(EXMI_ERR_HIP, "stream creation failed");
   }
+  if (!p->vjp) keep_for_twin(p, d, total);  // the multi-pass adjoint, on first use
   *out = p;
   return EXMI_OK;
 }
@@ -1380,6 +1669,10 @@ int exmi_plan_destroy(exmi_plan* p) {
   hipFree(p->H);
   hipFree(p->P);
   hipFree(p->ckpt);
+  for (cf* b : {p->XA3, p->XB3, p->XC3, p->LAM3, p->MU3, p->T3, p->LA3, p->LB3, p->LC3, p->FH3}) hipFree(b);
+  hipFree(p->RH3);
+  hipFree(p->Q3);
+  if (p->twin) exmi_plan_destroy(p->twin);
   delete p;
   return EXMI_OK;
 }
@@ -2528,11 +2821,18 @@ static hipError_t fill(float* dst, long long n, float v, hipStream_t st) {
   return hipGetLastError();
 }
 
+static bool fused_adjoint(const exmi_plan* p) {
+  return !p->md && (p->ndim == 1 ? p->vjp != nullptr : p->rowA[0] != nullptr);
+}
+
 static bool has_adjoint(const exmi_plan* p) {
-  return p->ndim == 1 ? p->vjp != nullptr : p->rowA[0] != nullptr;
+  if (p->md) return md_adjoint_ok(p);
+  return fused_adjoint(p) || !p->hL.empty();
 }
 
 static int ensure_adjoint_ws(exmi_plan* p) {
+  if (p->md) return ensure_adj_md(p);
+  if (!fused_adjoint(p)) return ensure_twin(p);
   if (p->ndim != 2 || p->G) return EXMI_OK;
   const int N = p->d.num_points;
   const long long B = p->d.batch, full = (long long)(N / 2 + 1) * N;
@@ -2557,6 +2857,11 @@ static int ensure_adjoint_ws(exmi_plan* p) {
 // gout = (d step / d u)^T g at u; gout may alias g (g is read first).
 static int enqueue_step_vjp(exmi_plan* p, const float* u, long long ubs, const float* g, long long gbs, float* gout,
                             long long obs, hipStream_t st) {
+  if (p->md) return enqueue_vjp_md(p, u, ubs, g, gbs, gout, obs, st);
+  if (!fused_adjoint(p)) {
+    if (!p->twin) return fail(EXMI_ERR_UNSUPPORTED, "adjoint twin plan not built");
+    return enqueue_vjp_md(p->twin, u, ubs, g, gbs, gout, obs, st);
+  }
   const int B = p->d.batch;
   hipError_t e = hipSuccess;
   if (p->ndim == 1) {
@@ -2622,9 +2927,10 @@ static int ensure_work(exmi_plan* p) {
 // The plans the adjoint kernels cover, for the error of every other plan.
 static int adjoint_unsupported() {
   return fail(EXMI_ERR_UNSUPPORTED,
-              "no adjoint kernel for this plan (built: 1-D ETDRK orders 0-2 with no nonlinearity, convection, "
-              "gradient norm, the polynomial reactions or the general form, real L, and with a complex L (N <= "
-              "1024) the linear, convection and general plans; 2-D ETDRK2 with the gradient norm)");
+              "no adjoint for this plan (built for ETDRK orders 0-4 in 1-D, 2-D and 3-D with no nonlinearity, "
+              "convection, the gradient norm, the polynomial reactions, the general form or the 2-D vorticity "
+              "convection, real or complex L; not for Cahn-Hilliard, the coupled reactions and the projected "
+              "3-D convection)");
 }
 
 int exmi_step_vjp(exmi_plan* p, const float* u, int64_t u_bs, const float* g, int64_t g_bs, float* g_in,

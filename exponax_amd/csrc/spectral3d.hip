@@ -323,6 +323,208 @@ __global__ void combine3d_kernel(NL3DArgs a, const cf* __restrict__ PH, cf* __re
   }
 }
 
+// ------------------------------------------------ adjoint of the nonlinear term
+// J(w)^T mu on whole spectra in three passes around the transforms, the
+// transposes of the forward passes above under the inner products sum(f g)
+// (fields) and sum(irfftn(a) irfftn(b)) (spectra), in which rfftn and
+// irfftn are mutual adjoints, a Hermitian per-mode factor has adjoint its
+// conjugate (i kappa -> -i kappa), and the dealias mask P and the mean
+// removal Z are self-adjoint:
+//   cofields : the cotangent fields of mu (P mu, conj-derivatives, Z)   -> H
+//   (irfftn of every cotangent field, and of the forward fields of w)
+//   products : the pointwise adjoint products of the two                -> Q
+//   (rfftn of every product)
+//   combine  : conj-derivative multiplies, scales, P                    -> out
+// Per kind (h = irfftn(P mu); w, g_d = d_d w, u, v, omega_x, omega_y the
+// forward fields of w):
+//   gradient norm   -b/2 P Z F[sum g_d^2]      -> -b P sum_d conj(D_d) F[g_d irfftn(Z P mu)]
+//   convection, conservative, C = D channels   -> -(b/2) P F[sum_j (h_aj + h_ja) w_j],
+//                                                  h_cj = irfftn(P conj(D_j) mu_c)
+//   convection, u . grad u                     -> -b P (F[sum_c h_c d_a w_c] + sum_j conj(D_j) F[h_a w_j])
+//   polynomial                                 -> P F[p'(w) h]
+//   general                                    -> P (F[w (2 g0 h + g1 hd)] + g2 sum_d conj(D_d) F[g_d hz]),
+//                                                  hd = irfftn(P conj(sum_d D_d) mu), hz = irfftn(Z P mu);
+//                                                  single-channel u sum_d d_d u: g0 (F[h sum_d g_d] + conj(sum_d D_d) F[h w])
+//   vorticity (2-D) -b (u omega_x + v omega_y) -> -b P (conj(D_y) Linv F[omega_x h] - conj(D_x) Linv F[omega_y h]
+//                                                        + conj(D_x) F[u h] + conj(D_y) F[v h])
+// (oracle/adjoint.py restates each from the forward formulas of
+// nonlin_fun/_gradient_norm.py, _convection.py, _polynomial.py,
+// _general_nonlinear.py and _vorticity_convection.py:78-99.)
+int adj3d_fields(int kind, int dims, int poly_lap, int* nh, int* nq) {
+  *nh = 0;
+  *nq = 0;
+  switch (kind) {
+    case NL_GRADNORM: *nh = 1; *nq = dims; break;
+    case NL_CONV_CONS: *nh = dims; *nq = 1; break;
+    case NL_CONV_NONCONS: *nh = 1; *nq = 1 + dims; break;
+    case NL_POLY:
+      if (poly_lap) return 0;
+      *nh = 1;
+      *nq = 1;
+      break;
+    case NL_GENERAL: *nh = 3; *nq = 2 + dims; break;
+    case NL_VORT: *nh = 1; *nq = 4; break;
+    default: return 0;
+  }
+  return *nh;
+}
+
+__global__ void adj_cofields3d_kernel(NL3DArgs a, const cf* __restrict__ MU, cf* __restrict__ H) {
+  const long long total = a.ninst * a.Nr;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const long long inst = idx / a.Nr, k = idx % a.Nr;
+    const Mode3D m = mode3d(a, k);
+    const cf mu = m.keep ? MU[idx] : cf{0.f, 0.f};
+    cf* h = H + inst * a.nh * a.Nr + k;
+    switch (a.kind) {
+      case NL_GRADNORM: h[0] = (a.zero_mode_fix && m.zero) ? cf{0.f, 0.f} : mu; break;
+      case NL_CONV_CONS:
+        for (int j = 0; j < a.dims; ++j) h[j * a.Nr] = imul(-kap_d(a, m, j), mu);
+        break;
+      case NL_GENERAL: {
+        h[0] = mu;
+        h[a.Nr] = imul(-(m.kx + m.ky + m.kz), mu);
+        h[2 * a.Nr] = (a.zero_mode_fix && m.zero) ? cf{0.f, 0.f} : mu;
+        break;
+      }
+      default: h[0] = mu;
+    }
+  }
+}
+
+__global__ void adj_products3d_kernel(NL3DArgs a, const float* __restrict__ R, const float* __restrict__ HR,
+                                      float* __restrict__ Q) {
+  const long long total = a.ninst * a.N3;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const long long inst = idx / a.N3, p = idx % a.N3;
+    const long long base = inst - inst % a.channels;
+    const int ch = (int)(inst % a.channels);
+    auto fld = [&](long long i, int f) { return R[(i * a.nf + f) * a.N3 + p]; };
+    auto cot = [&](long long i, int f) { return HR[(i * a.nh + f) * a.N3 + p]; };
+    float* q = Q + inst * a.nq * a.N3 + p;
+    switch (a.kind) {
+      case NL_GRADNORM: {
+        const float h = cot(inst, 0);
+        for (int d = 0; d < a.dims; ++d) q[(long long)d * a.N3] = fld(inst, d) * h;
+        break;
+      }
+      case NL_CONV_CONS: {
+        float acc = 0.f;
+        for (int j = 0; j < a.dims; ++j) acc = fmaf(cot(inst, j) + cot(base + j, ch), fld(base + j, 0), acc);
+        q[0] = acc;
+        break;
+      }
+      case NL_CONV_NONCONS: {
+        const float ha = cot(inst, 0);
+        float acc = 0.f;
+        for (int c = 0; c < a.dims; ++c) acc = fmaf(cot(base + c, 0), fld(base + c, 1 + ch), acc);
+        q[0] = acc;
+        for (int j = 0; j < a.dims; ++j) q[(long long)(1 + j) * a.N3] = ha * fld(base + j, 0);
+        break;
+      }
+      case NL_POLY: {
+        const float u = fld(inst, 0);
+        float dp = 0.f, pw = 1.f;
+        for (int i = 1; i < a.npoly; ++i) {
+          dp = fmaf((float)i * a.poly[i], pw, dp);
+          pw *= u;
+        }
+        q[0] = dp * cot(inst, 0);
+        break;
+      }
+      case NL_GENERAL: {
+        const float w = fld(inst, 0), h = cot(inst, 0), hd = cot(inst, 1), hz = cot(inst, 2);
+        if (a.prod_nc) {
+          float s = 0.f;
+          for (int d = 0; d < a.dims; ++d) s += fld(inst, 1 + d);
+          q[0] = a.g0 * h * s;
+          q[a.N3] = a.g0 * h * w;
+        } else {
+          q[0] = w * fmaf(2.f * a.g0, h, a.g1 * hd);
+          q[a.N3] = 0.f;
+        }
+        for (int d = 0; d < a.dims; ++d) q[(long long)(2 + d) * a.N3] = a.g2 * fld(inst, 1 + d) * hz;
+        break;
+      }
+      case NL_VORT: {
+        const float h = cot(inst, 0);
+        q[0] = fld(inst, 1) * h;           // omega_x h
+        q[a.N3] = fld(inst, 3) * h;        // omega_y h
+        q[2 * a.N3] = fld(inst, 0) * h;    // u h
+        q[3 * a.N3] = fld(inst, 2) * h;    // v h
+        break;
+      }
+      default: q[0] = 0.f;
+    }
+  }
+}
+
+__global__ void adj_combine3d_kernel(NL3DArgs a, const cf* __restrict__ QH, cf* __restrict__ out) {
+  const long long total = a.ninst * a.Nr;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const long long inst = idx / a.Nr, k = idx % a.Nr;
+    const Mode3D m = mode3d(a, k);
+    const cf* q = QH + inst * a.nq * a.Nr + k;
+    cf r = cf{0.f, 0.f};
+    switch (a.kind) {
+      case NL_GRADNORM:
+        for (int d = 0; d < a.dims; ++d) r = r + imul(-kap_d(a, m, d), q[d * a.Nr]);
+        r = -a.scale * r;
+        break;
+      case NL_CONV_CONS: r = (-0.5f * a.scale) * q[0]; break;
+      case NL_CONV_NONCONS:
+        r = q[0];
+        for (int j = 0; j < a.dims; ++j) r = r + imul(-kap_d(a, m, j), q[(1 + j) * a.Nr]);
+        r = -a.scale * r;
+        break;
+      case NL_GENERAL:
+        r = q[0] + imul(-(m.kx + m.ky + m.kz), q[a.Nr]);
+        for (int d = 0; d < a.dims; ++d) r = r + imul(-kap_d(a, m, d), q[(2 + d) * a.Nr]);
+        break;
+      case NL_VORT: {
+        const float kx = m.ky, ky = m.kz;
+        const float lap = -(kx * kx + ky * ky);
+        const float inv = lap == 0.f ? 1.f : 1.f / lap;
+        r = imul(-ky, inv * q[0]) + imul(kx, inv * q[a.Nr]) + imul(-kx, q[2 * a.Nr]) + imul(-ky, q[3 * a.Nr]);
+        r = -a.scale * r;
+        break;
+      }
+      default: r = q[0];
+    }
+    out[idx] = m.keep ? r : cf{0.f, 0.f};
+  }
+}
+
+__global__ void adj_linear3d_kernel(AdjLinArgs a) {
+  const long long total = a.ninst * a.nr;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const long long inst = idx / a.nr, k = idx % a.nr;
+    const long long traj = inst / a.channels, ch = inst % a.channels;
+    const float* tb = a.tables + ((a.lin_batch > 1 ? traj : 0) * a.lin_channels + (a.lin_channels > 1 ? ch : 0)) *
+                                     (long long)a.ncoef * a.nr;
+    cf acc = cf{0.f, 0.f};
+    for (int t = 0; t < a.nt; ++t) {
+      const cf x = a.x[t][idx];
+      const int op = a.op[t];
+      cf v = x;
+      if (op >= 100) {  // conj(E) x (100) or conj(E_half) x (101)
+        const int ei_ = op - 100;
+        const float er = tb[(long long)ei_ * a.nr + k];
+        const float ei = a.e_im < 0 ? 0.f : tb[(long long)(a.e_im + ei_) * a.nr + k];
+        v = cf{fmaf(er, x.re, ei * x.im), fmaf(er, x.im, -ei * x.re)};
+      } else if (op >= 0) {  // a real coefficient of the tables
+        v = tb[(long long)op * a.nr + k] * x;
+      }
+      acc = acc + a.mult[t] * v;
+    }
+    a.out[idx] = acc;
+  }
+}
+
 static unsigned grid_of(long long total) {
   long long g = (total + 255) / 256;
   return (unsigned)(g > 65536 ? 65536 : g);
@@ -334,6 +536,22 @@ hipError_t launch_fields3d(const NL3DArgs& a, const cf* S, cf* F, hipStream_t st
 }
 hipError_t launch_products3d(const NL3DArgs& a, const float* R, float* PR, hipStream_t st) {
   hipLaunchKernelGGL(products3d_kernel, dim3(grid_of(a.ninst * a.N3)), dim3(256), 0, st, a, R, PR);
+  return hipGetLastError();
+}
+hipError_t launch_adj_cofields3d(const NL3DArgs& a, const cf* MU, cf* H, hipStream_t st) {
+  hipLaunchKernelGGL(adj_cofields3d_kernel, dim3(grid_of(a.ninst * a.Nr)), dim3(256), 0, st, a, MU, H);
+  return hipGetLastError();
+}
+hipError_t launch_adj_products3d(const NL3DArgs& a, const float* R, const float* HR, float* Q, hipStream_t st) {
+  hipLaunchKernelGGL(adj_products3d_kernel, dim3(grid_of(a.ninst * a.N3)), dim3(256), 0, st, a, R, HR, Q);
+  return hipGetLastError();
+}
+hipError_t launch_adj_combine3d(const NL3DArgs& a, const cf* QH, cf* out, hipStream_t st) {
+  hipLaunchKernelGGL(adj_combine3d_kernel, dim3(grid_of(a.ninst * a.Nr)), dim3(256), 0, st, a, QH, out);
+  return hipGetLastError();
+}
+hipError_t launch_adj_linear3d(const AdjLinArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(adj_linear3d_kernel, dim3(grid_of(a.ninst * a.nr)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 hipError_t launch_combine3d(const NL3DArgs& a, const cf* PH, cf* Nh, hipStream_t st) {
