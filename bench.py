@@ -260,6 +260,8 @@ def wl_burgers1d(args, ex, _lib, torch):
                                            for s_ in checks}}
         del traj_p
     del traj
+    if not args.no_corrector_timing:
+        res["learned_vs_plain_fine_speedup"] = corrector_inference_timing(ex, torch)
     if not args.no_cpu_baseline:
         from oracle import stepper as ost
 
@@ -362,8 +364,10 @@ def wl_kolmogorov2d(args, ex, _lib, torch):
                        "domain_extent": "2 pi", "dt": 0.01, "order": 2, "reynolds": "U[100, 1000]",
                        "rollout_steps": K, "ic": "TFS cutoff 6 seed 3"},
                       nst * K / wall, K, W, wall, gpu_s / K, per_step_bytes, K,
-                      "row2d/col2d kernels (6 launches per step per grid size)",
+                      "row2d/col2d kernels per grid size: the spectrum-carrying chain (4 launches per step), "
+                      "the workgroup-resident kernel or the six-launch chain, as the autotuner picks",
                       read_traffic("kolmogorov2d_mixed", nst))
+    res["rollout_strategy_by_grid"] = {f"{n}^2": st.plan(b).rollout_strategy() for st, (n, b) in zip(steppers, groups)}
     res["nan_trajectories"] = int(sum((~torch.isfinite(t[:, -1]).reshape(t.shape[0], -1).all(1)).sum().item()
                                       for t in trs))
     res["launch_mode"] = modes[best]
@@ -537,6 +541,49 @@ def wl_ks2d_grad(args, ex, _lib, torch):
 
 CNN_FLOP_PER_SAMPLE = 2 * (1 * 32 * 5 + 3 * 32 * 32 * 5 + 32 * 1 * 5) * 64  # 2,007,040 on 64 points
 BF16_PEAK_TFLOPS = 2500.0
+
+
+def corrector_inference_timing(ex, torch):
+    """The learned-corrector part of the metric, timed: the corrected coarse
+    stepper (KS-conservative 1-D, L = 100, N = 64, dt = 0.5, 200 steps, one
+    fused step and one network launch per step) against the plain fine
+    stepper (N = 256, dt = 0.1, 1000 steps) over the same 100 time units on
+    8192 trajectories, and the network kernel's fraction of the BF16 MFMA
+    peak. The network has the trained one's shape with seeded weights: the
+    wall times do not depend on the weights; the accuracy at equal rel-L2 of
+    the trained network is measured by --workload corrector."""
+    from exponax_amd import corrector as cr
+
+    L, NF, NC, DTF, DTC, ST, B, K = 100.0, 256, 64, 0.1, 0.5, 5, 8192, 200
+    sync = torch.cuda.synchronize
+    KS = ex.stepper.KuramotoSivashinskyConservative
+    fine, coarse = KS(1, L, NF, DTF), KS(1, L, NC, DTC)
+    u0 = torch.tensor(ex.ic.truncated_fourier_series(B, 1, L, NF, 5, 50), dtype=torch.float32, device="cuda:0")
+    u0c = ex.map_between_resolutions(u0, NC)
+    cnn = cr.PeriodicCNN(seed=0)
+    corrected = cr.CorrectedStepper(coarse, cnn)
+    corrected.rollout_batch(u0c, 2)
+    fine.rollout_batch(u0, 10)
+    keep = {}
+    wall_c, _ = timed_rollout(lambda: keep.__setitem__("c", corrected.rollout_batch(u0c, K)), sync)
+    wall_f, _ = timed_rollout(lambda: keep.__setitem__("f", fine.rollout_batch(u0, K * ST)), sync)
+    x = keep["c"][:, -1].contiguous()
+    del keep
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    sync()
+    ev0.record()
+    for _ in range(reps):
+        cr.cnn_forward(cnn.params.detach(), x)
+    ev1.record()
+    sync()
+    cnn_s = ev0.elapsed_time(ev1) / 1e3 / reps
+    tflops = B * CNN_FLOP_PER_SAMPLE / cnn_s / 1e12
+    return {"value": wall_f / wall_c, "corrected_coarse_wall_s": wall_c, "plain_fine_wall_s": wall_f,
+            "trajectories": B, "coarse_steps": K, "fine_steps": K * ST,
+            "cnn_mfma_frac": tflops / BF16_PEAK_TFLOPS, "cnn_tflops": tflops, "cnn_us_per_launch": cnn_s * 1e6,
+            "note": "wall-time ratio of the inference rollouts (seeded weights; the trained network's "
+                    "accuracy is measured by --workload corrector)"}
 
 
 def wl_corrector(args, ex, _lib, torch):
@@ -713,12 +760,14 @@ def main():
                     help="burgers1d: ETDRK steps per rollout graph (default 1000)")
     ap.add_argument("--no-physical-roundtrip", action="store_true",
                     help="burgers1d: skip the per-step physical round-trip rollout measured beside the default")
+    ap.add_argument("--no-corrector-timing", action="store_true",
+                    help="burgers1d: skip the learned-corrector inference timing added to the line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (trajectories shard across GPUs "
-              "without communication, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the trajectories of a batch "
+              "shard across GPUs without communication, so one GPU's share is the measured unit)", file=sys.stderr)
         return 2
 
     import torch

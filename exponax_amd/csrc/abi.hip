@@ -350,6 +350,7 @@ struct exmi_plan {
   StepFn nonlin = nullptr;
   RollFn roll[3] = {nullptr, nullptr, nullptr};  // by transforms per lane group (1, 2)
   int roll_nt = 0;                               // chosen by the autotuner (0 = not yet)
+  int tune_short = 0;  // chosen on a probe shorter than kTuneProbe steps: timed again on a longer rollout
   int fourier_resident = 0;                      // rollouts keep the spectral state on chip
   int fr_auto = 0;  // chosen by the plan (exact identity); per-step launches stay allowed
   hipStream_t capture_stream = nullptr;
@@ -1727,6 +1728,7 @@ int exmi_plan_set_rollout_strategy(exmi_plan* p, int32_t nt) {
     return fail(EXMI_ERR_UNSUPPORTED, "rollout strategy %d not available for this plan", nt);
   if (nt == 0 && (p->md || (p->ndim != 1 && !p->roll2d && !p->fr2d))) nt = -1;  // nothing to choose from
   p->roll_nt = nt;
+  p->tune_short = 0;
   if (p->gc.exec) {
     hipGraphExecDestroy(p->gc.exec);
     p->gc = GraphCache();
@@ -1954,6 +1956,12 @@ static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u
 // per-trajectory tables, Fourier-resident).
 static std::mutex g_tune_mu;
 
+// Rollout strategies are timed on a probe of kTuneProbe steps (fewer when the
+// first rollout is shorter; such a choice is provisional and timed again on
+// the first rollout of at least kTuneProbe steps).
+static constexpr int kTuneProbe = 16;
+static bool needs_tune(const exmi_plan* p, int n) { return p->roll_nt == 0 || (p->tune_short && n >= kTuneProbe); }
+
 static std::string tune_key(const exmi_plan* p) {
   hipDeviceProp_t prop;
   char arch[64] = "unknown";
@@ -2099,11 +2107,12 @@ static int tune_gfft(exmi_plan* p) {
 }
 
 static int autotune_rollout(exmi_plan* p, const float* u0, long long u0_bs, float* traj, int n, int include_init) {
-  if (p->roll_nt != 0) return EXMI_OK;
-  if (tune_lookup(p)) return EXMI_OK;
-  const int probe = n < 16 ? n : 16;
+  if (!needs_tune(p, n)) return EXMI_OK;
+  if (p->roll_nt == 0 && tune_lookup(p)) return EXMI_OK;
+  const int probe = n < kTuneProbe ? n : kTuneProbe;
+  p->tune_short = probe < kTuneProbe;
   if (probe == 0) {
-    p->roll_nt = strategy_ok(p, 2) ? 2 : 1;
+    p->roll_nt = strategy_ok(p, 2) ? 2 : strategy_ok(p, 1) ? 1 : -1;
     return EXMI_OK;
   }
   hipStream_t st = p->capture_stream;
@@ -2139,7 +2148,7 @@ static int autotune_rollout(exmi_plan* p, const float* u0, long long u0_bs, floa
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   p->roll_nt = best_nt;
-  tune_store(p);
+  if (!p->tune_short) tune_store(p);
   return EXMI_OK;
 }
 
@@ -2230,7 +2239,7 @@ int exmi_rollout(exmi_plan* p, const float* u0, int64_t u0_bs, float* traj, int3
                  int32_t use_graph, void* stream) {
   if (!p || !u0 || !traj) return fail(EXMI_ERR_INVALID, "null argument");
   if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
-  if (p->roll_nt == 0) {
+  if (needs_tune(p, n)) {
     // the probe writes into traj; order it after the caller's pending work
     HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
     int rc = autotune_rollout(p, u0, u0_bs, traj, n, include_init);
@@ -2244,7 +2253,7 @@ int exmi_rollout_prepare(exmi_plan* p, const float* u0, int64_t u0_bs, float* tr
                          int32_t include_init) {
   if (!p || !u0 || !traj) return fail(EXMI_ERR_INVALID, "null argument");
   if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
-  if (p->roll_nt == 0) {
+  if (needs_tune(p, n)) {
     HIP_TRY(hipDeviceSynchronize());
     int rc = autotune_rollout(p, u0, u0_bs, traj, n, include_init);
     if (rc) return rc;
@@ -2274,7 +2283,7 @@ int exmi_rollout_checked(exmi_plan* p, const float* u0, int64_t u0_bs, float* tr
   if (!p || !u0 || !traj || !first_bad) return fail(EXMI_ERR_INVALID, "null argument");
   if (n < 0) return fail(EXMI_ERR_INVALID, "n_steps must be >= 0");
   hipStream_t st = (hipStream_t)stream;
-  if (p->roll_nt == 0) {
+  if (needs_tune(p, n)) {
     HIP_TRY(hipStreamSynchronize(st));
     int rc = autotune_rollout(p, u0, u0_bs, traj, n, include_init);
     if (rc) return rc;
@@ -2349,7 +2358,7 @@ int exmi_rollout_group(exmi_plan* const* plans, int32_t np, const float* const* 
     if (!plans[i] || !u0s[i] || !trajs[i]) return fail(EXMI_ERR_INVALID, "null plan or buffer in group");
     for (int k = 0; k < i; ++k)
       if (plans[k] == plans[i]) return fail(EXMI_ERR_INVALID, "a plan appears twice in the group");
-    if (plans[i]->roll_nt == 0) {
+    if (needs_tune(plans[i], n)) {
       HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
       int rc = autotune_rollout(plans[i], u0s[i], u0_bstrides[i], trajs[i], n, include_init);
       if (rc) return rc;

@@ -256,6 +256,27 @@ def test_autotuner_cache_roundtrip(device, tmp_path, monkeypatch):
     assert max_rel_l2(out[:, -1].cpu().numpy(), s1.rollout_batch(x, 20)[:, -1].cpu().numpy()) <= 1e-5
 
 
+def test_autotuner_short_first_rollout_is_provisional(device, tmp_path, monkeypatch):
+    """A first rollout shorter than the 16-step probe chooses a strategy for
+    itself only: nothing is stored, and the first long rollout times the
+    candidates again and stores that choice (2-D Kolmogorov at 128^2, where
+    launch counts weigh on short probes)."""
+    import torch
+
+    path = tmp_path / "tune.txt"
+    monkeypatch.setenv("EXMI_TUNE_CACHE", str(path))
+    st = ex.stepper.KolmogorovFlowVorticity(2, 2 * np.pi, 128, 0.01, diffusivity=1e-2)
+    u0, _ = ics(4, 2, 2 * np.pi, 128, 6, seed=16)
+    x = torch.tensor(u0, device=device)
+    short = st.rollout_batch(x, 2)
+    assert st.plan(4).rollout_strategy() != 0
+    assert not path.exists() or not path.read_text().strip()
+    long = st.rollout_batch(x, 20)
+    lines = [ln for ln in path.read_text().split("\n") if ln.strip()]
+    assert len(lines) == 1 and lines[0].endswith(f" {st.plan(4).rollout_strategy()}")
+    assert max_rel_l2(long[:, 1].cpu().numpy(), short[:, 1].cpu().numpy()) <= 1e-5
+
+
 @pytest.mark.parametrize("order", [2, 3])
 def test_repeated_stepper(device, order):
     """RepeatedStepper (reference _repeated_stepper.py:9-143): dt is the
