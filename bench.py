@@ -463,6 +463,45 @@ def wl_ns3d(args, ex, _lib, torch):
     return res
 
 
+def wl_kolmogorov2d_513(args, ex, _lib, torch):
+    """A grid size outside the fused kernels' set: the 2-D Kolmogorov flow at
+    the reference validation notebook's N = 513 (= 27 x 19), L = 2 pi,
+    dt = 0.01, ETDRK2, 8 instances with Re in [100, 1000]: the multi-pass
+    plan on the runtime-size mixed-radix transforms (radix/tile chosen by the
+    plan's autotuner). value = instance-steps/s of a K-step rollout."""
+    N, L, DT = 513, 2 * np.pi, 0.01
+    B, K, W = args.batch or 8, args.steps or 50, args.warmup
+    rng = np.random.default_rng(6)
+    re = rng.uniform(100.0, 1000.0, B)
+    u0 = tfs_ics(B, 2, N, L, 6, rng)
+    st = ex.ensemble(lambda r: ex.stepper.KolmogorovFlowVorticity(2, L, N, DT, diffusivity=1.0 / r), re)
+    x = torch.tensor(u0, dtype=torch.float32, device="cuda:0")
+    if W > 0:
+        st.rollout_batch(x, W)
+    torch.cuda.synchronize()
+    out = {}
+    wall, gpu_s = timed_rollout(lambda: out.__setitem__("tr", st.rollout_batch(x, K)), torch.cuda.synchronize)
+    tr = out["tr"]
+    nr = N * (N // 2 + 1)
+    res = result_line("kolmogorov2d_n513_b8_etdrk2",
+                      {"equation": "KolmogorovFlowVorticity", "num_points": N, "domain_extent": "2 pi", "dt": DT,
+                       "order": 2, "batch": B, "rollout_steps": K, "reynolds": "U[100, 1000]",
+                       "ic": "TFS cutoff 6 seed 6"},
+                      B * K / wall, K, W, wall, gpu_s / K, B * (8 * N * N + 12 * nr), K,
+                      "runtime-size transforms (gfft_kernel) and the field, product, combine and stage passes", None)
+    res["fft_config"] = st.fft_config(batch=B)
+    if not args.no_cpu_baseline:
+        from oracle import stepper as ost
+
+        nb = 1
+        g = tr[:nb].cpu().numpy()
+        checks = {s: g[:, s - 1] for s in (1, 10)}
+        mk = lambda dt: ost.Ensemble([ost.KolmogorovFlowVorticity(2, L, N, DT, diffusivity=1.0 / re[0],  # noqa: E731
+                                                                   dtype=dt)], dtype=dt)
+        res["cpu_baseline"], res["parity"] = oracle_baseline(mk, u0[:nb], checks, 10, f"{nb} instance x 10 steps")
+    return res
+
+
 def wl_ks2d_grad(args, ex, _lib, torch):
     """Gradient of MSE(u_50, target) w.r.t. u_0 through 50 KS 2-D steps
     (128^2, 32 trajectories), checkpoints every 10 steps: the forward pass
@@ -742,7 +781,7 @@ def wl_cnn_forward(args, ex, _lib, torch):
 
 WORKLOADS = {"burgers1d": wl_burgers1d, "ks1d": wl_ks1d, "kolmogorov2d": wl_kolmogorov2d, "ks2d": wl_ks2d,
              "ks2d_grad": wl_ks2d_grad, "corrector": wl_corrector, "cnn_forward": wl_cnn_forward,
-             "ns3d": wl_ns3d}
+             "ns3d": wl_ns3d, "kolmogorov2d_513": wl_kolmogorov2d_513}
 
 
 def main():

@@ -237,7 +237,8 @@ class Plan:
 
     def fft_config(self) -> dict | None:
         """The runtime-size transforms' knobs (None for plans on the fused
-        kernels): code = max_radix * 100000 + points per workgroup, where it
+        kernels): code = reg * 100000000 + max_radix * 100000 + points per
+        workgroup (reg 1: radices 9..17 and 19 as register butterflies), where it
         came from (default / timed / cache / set), lines per workgroup and
         the pass radices."""
         buf = (C.c_int32 * 32)()

@@ -2023,6 +2023,9 @@ static std::string gfft_tune_key(const exmi_plan* p) {
 
 static bool same_cfg(const GfftCfg& a, const GfftCfg& b) {
   if (a.n != b.n || a.npass != b.npass || a.lpb != b.lpb) return false;
+  bool big = false;
+  for (int i = 0; i < a.npass; ++i) big = big || (a.radix[i] > 8 && (a.radix[i] <= 17 || a.radix[i] == 19));
+  if (big && a.reg19 != b.reg19) return false;
   for (int i = 0; i < a.npass; ++i)
     if (a.radix[i] != b.radix[i]) return false;
   return true;

@@ -19,6 +19,7 @@ struct GfftCfg {
   int npass = 0;
   int radix[GFFT_MAX_PASS] = {};
   int lpb = 1;  // lines per workgroup
+  int reg19 = 0;  // 1: radices 9..17 and 19 as register butterflies (a kernel of lower occupancy)
 };
 
 // Transforms of `nlines` lines.
@@ -35,8 +36,8 @@ enum GfftMode { GFFT_FWD = 0, GFFT_INV = 1, GFFT_R2C = 2, GFFT_C2R = 3 };
 // target number of points per workgroup. Returns false when n is outside
 // 2..GFFT_MAX_N.
 bool gfft_make_cfg(int n, int max_radix, int points_per_wg, GfftCfg* cfg);
-// Encoded knobs (max_radix * 100000 + points_per_wg) of the candidates the
-// autotuner times, and the default used without tuning.
+// Encoded knobs (reg19 * 100000000 + max_radix * 100000 + points_per_wg) of
+// the candidates the autotuner times, and the default used without tuning.
 int gfft_num_candidates();
 int gfft_candidate(int i);
 int gfft_default_code();

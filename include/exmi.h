@@ -462,8 +462,11 @@ int exmi_plan_set_rollout_strategy(exmi_plan* plan, int32_t strategy);
  * keyed by GPU architecture, D, N and the number of fields).
  * exmi_plan_fft_config writes [code, source, lines per workgroup, passes,
  * radix_1 .. radix_passes] into info (up to len entries) and returns the
- * entry count, 0 for plans on the fused kernels; code = max_radix * 100000
- * + points per workgroup; source 0 default, 1 timed, 2 tuning cache, 3 set.
+ * entry count, 0 for plans on the fused kernels; code = reg * 100000000 +
+ * max_radix * 100000 + points per workgroup (reg 1: passes of radix 9..17
+ * and 19 as register butterflies, in a kernel of lower occupancy; reg 0:
+ * such passes as direct sums); source 0 default, 1 timed, 2 tuning cache,
+ * 3 set.
  * exmi_plan_set_fft_config applies a code (drops a cached graph).
  * (No reference counterpart: XLA picks the reference's FFT plans.) */
 int exmi_plan_fft_config(const exmi_plan* plan, int32_t* info, int32_t len);

@@ -27,7 +27,7 @@ def _rel(a, b):
 
 
 # prime sizes run one direct-DFT pass of their own length
-FFT_SIZES_1D = [2, 3, 5, 7, 11, 12, 24, 97, 100, 210, 243, 343, 513, 1000, 1009, 2310, 4095]
+FFT_SIZES_1D = [2, 3, 5, 7, 11, 12, 24, 97, 100, 210, 243, 343, 513, 1000, 1009, 2310, 4095, 17 * 19, 13 * 11 * 9]
 
 
 @pytest.mark.parametrize("N", FFT_SIZES_1D)
@@ -271,7 +271,8 @@ def test_transform_autotuner_choice_persisted_and_every_candidate(device, tmp_pa
             ref = cpu.step_batch(ref)
         plan = gpu.plan(4)
         for code in (400000 + 1024, 800000 + 1024, 1600000 + 1024, 400000 + 2048, 800000 + 2048, 1600000 + 2048,
-                     800000 + 512, 800000 + 4096, 200000 + 100, 10000000 + 100):
+                     800000 + 512, 800000 + 4096, 200000 + 100, 10000000 + 100, 100000000 + 1600000 + 1024,
+                     100000000 + 800000 + 2048):
             plan.set_fft_config(code)
             got = gpu.rollout_batch(torch.tensor(u0, device=device), 10).cpu().numpy()[:, -1]
             e = max_rel_l2(got, ref)
