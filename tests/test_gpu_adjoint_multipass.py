@@ -184,3 +184,27 @@ def test_adjoint_unsupported_kinds_fail_loudly(device):
     gs = ex.stepper.reaction.GrayScott(1, 2.5, 64, 1.0)
     with pytest.raises(NotImplementedError):
         gs.step_vjp(torch.zeros(1, 2, 64, device="cuda"), torch.zeros(1, 2, 64, device="cuda"))
+
+
+def test_rollout_vjp_3d_and_any_size(device):
+    """Checkpointed rollout gradients of multi-pass plans: 3-D KS (N = 16)
+    and 1-D Burgers at N = 100, against the oracle's reverse pass."""
+    import torch
+
+    print("[case] rollout vjp 3-D and N=100", flush=True)
+    cases = [(ex.stepper.KuramotoSivashinsky(3, 20.0, 16, 0.05), ost.KuramotoSivashinsky(3, 20.0, 16, 0.05),
+              ics(2, 3, 20.0, 16, 3, seed=83)[0], ics(2, 3, 20.0, 16, 6, seed=84)[0]),
+             (ex.stepper.Burgers(1, 1.0, 100, 1e-3, diffusivity=0.01), ost.Burgers(1, 1.0, 100, 1e-3, diffusivity=0.01),
+              ics(2, 1, 1.0, 100, 5, seed=85)[0], ics(2, 1, 1.0, 100, 10, seed=86)[0])]
+    for gpu, cpu, u0, gf in cases:
+        n = 7
+        x = torch.tensor(u0, device="cuda")
+        outs = [gpu.rollout_vjp(x, n, g_final=torch.tensor(gf, device="cuda"), checkpoint_every=k).cpu().numpy()
+                for k in (1, 3, 0)]
+        for o in outs[1:]:
+            assert _rel(o, outs[0]) <= 1e-6
+        ref = np.stack([oad.repeat_vjp(cpu, u0[b].astype(np.float64), n, gf[b].astype(np.float64))
+                        for b in range(2)])
+        err = _rel(outs[0], ref)
+        print(f"  D={gpu.num_spatial_dims} N={gpu.num_points}: rollout vjp rel-L2 {err:.2e}", flush=True)
+        assert err <= RTOL_GRAD

@@ -283,3 +283,50 @@ def test_transform_autotuner_choice_persisted_and_every_candidate(device, tmp_pa
             os.environ.pop("EXMI_TUNE_CACHE", None)
         else:
             os.environ["EXMI_TUNE_CACHE"] = old
+
+
+def test_any_size_rollout_forms(device):
+    """Rollouts of a multi-pass plan: include_init, the captured graph and
+    direct launches agree, repeat equals the rollout's last state, a group
+    mixes a runtime size with a fused one, and the failure-detecting rollout
+    finds nothing on a stable run."""
+    import torch
+
+    print("[case] any-size rollout forms", flush=True)
+    st = ex.stepper.KuramotoSivashinskyConservative(1, 30.0, 100, 0.05)
+    u0, _ = ics(3, 1, 30.0, 100, 5, seed=81)
+    x = torch.tensor(u0, device=device)
+    a = st.rollout_batch(x, 12, include_init=True)
+    b = st.rollout_batch(x, 12, use_graph=False)
+    assert torch.equal(a[:, 0], x)
+    assert max_rel_l2(a[:, 1:].cpu().numpy().reshape(3, -1), b.cpu().numpy().reshape(3, -1)) <= 1e-6
+    r = st.repeat_batch(x, 12)
+    assert max_rel_l2(r.cpu().numpy(), b[:, -1].cpu().numpy()) <= 1e-6
+    traj, fb = st.rollout_checked(x, 12)
+    assert (fb.cpu().numpy() == -1).all()
+    fused = ex.stepper.KuramotoSivashinskyConservative(1, 30.0, 128, 0.05)
+    u1, _ = ics(2, 1, 30.0, 128, 5, seed=82)
+    y = torch.tensor(u1, device=device)
+    for mode in (0, 1, 2, 3):
+        ga, gb = ex.rollout_group([st, fused], [x, y], 12, use_graph=mode)
+        assert max_rel_l2(ga.cpu().numpy().reshape(3, -1), b.cpu().numpy().reshape(3, -1)) <= 1e-6
+        assert max_rel_l2(gb.cpu().numpy().reshape(2, -1),
+                          fused.rollout_batch(y, 12).cpu().numpy().reshape(2, -1)) <= 1e-6
+
+
+def test_any_size_noalloc_inverse_rejected(device):
+    """The allocation-free inverse transform covers the fused sizes only: a
+    runtime size is an error, not a silent allocation."""
+    import ctypes
+
+    import torch
+
+    from exponax_amd import _lib
+
+    print("[case] noalloc runtime size rejected", flush=True)
+    L = _lib.lib()
+    xh = torch.zeros(2, 51, dtype=torch.complex64, device=device)
+    out = torch.zeros(2, 100, dtype=torch.float32, device=device)
+    rc = L.exmi_irfft_noalloc(1, 100, 2, ctypes.c_void_p(xh.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                              ctypes.c_void_p(_lib.stream_ptr()))
+    assert rc == _lib.EXMI_ERR_UNSUPPORTED
