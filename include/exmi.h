@@ -115,8 +115,8 @@ typedef struct exmi_plan_desc {
                                  sizes run the multi-pass plan with the
                                  runtime-size mixed-radix transforms (any
                                  factorisation, e.g. N = 100, 513 or an odd
-                                 prime), per-step launches, no adjoint and
-                                 no Fourier-resident rollout               */
+                                 prime), per-step launches, the multi-pass
+                                 adjoint, no Fourier-resident rollout      */
   int32_t num_channels;       /* C                                           */
   int32_t batch;              /* B                                           */
   int32_t order;              /* ETDRK order 0..4                            */
