@@ -19,12 +19,32 @@ namespace cnn {
 __global__ void pack_kernel(const float* __restrict__ p, bf16x8* __restrict__ frag) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= NFRAG) return;
+  bf16x8 v;
+  if (idx >= FRAG_L5) {  // layer 5: A[row 0][K = (tap, ci)] = W5[ci][k], other rows 0
+    const int l = (idx - FRAG_L5) % 64, s = (idx - FRAG_L5) / 64, k = s >> 1;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int ci = (s & 1) * 16 + 8 * (l >> 5) + i;
+      v[i] = static_cast<__bf16>((l & 31) == 0 ? p[OW5 + ci * KW + k] : 0.f);
+    }
+    frag[idx] = v;
+    return;
+  }
+  if (idx >= FRAG_L1) {  // layer 1: A[co][K = tap] = W1[co][k] for k < 5, else 0
+    const int l = idx - FRAG_L1, co = l & 31;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = 8 * (l >> 5) + i;
+      v[i] = static_cast<__bf16>(k < KW ? p[OW1 + co * KW + k] : 0.f);
+    }
+    frag[idx] = v;
+    return;
+  }
   const int l = idx % 64, s = (idx / 64) % KSTEPS, LT = idx / (64 * KSTEPS);
   const bool T = LT >= 3;
   const int L = T ? LT - 3 : LT;
   const int ow = L == 0 ? OW2 : (L == 1 ? OW3 : OW4);
   const int row = l & 31, k = s >> 1;
-  bf16x8 v;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int col = (s & 1) * 16 + 8 * (l >> 5) + i;
@@ -72,7 +92,8 @@ __global__ void __launch_bounds__(64 * WAVES) forward_kernel(Args a) {
   for (long long smp = (long long)blockIdx.x * WAVES + w; smp < a.count; smp += (long long)gridDim.x * WAVES) {
     for (int p = lane; p < N; p += 64) xs[p] = a.x[smp * a.xs + p];
     wave_sync();
-    forward_sample<N, TRAIN>(sh.e, W, xs, sh.act[w][0], sh.act[w][1], a.y + smp * a.ys, a.ws, smp, a.count, lane);
+    forward_sample<N, TRAIN>(sh.e, W, a.frag + FRAG_L1, a.frag + FRAG_L5, xs, sh.act[w][0], sh.act[w][1],
+                             a.y + smp * a.ys, a.ws, smp, a.count, lane);
   }
 }
 

@@ -4,9 +4,12 @@
 //
 // Arithmetic: weights and the activations entering each convolution are
 // bfloat16; products accumulate in float32; biases, GELU and the residual
-// are float32. The three 32 -> 32 layers (98% of the flops) run on the
-// gfx950 matrix cores (v_mfma_f32_32x32x16_bf16). The 1 -> 32 and 32 -> 1
-// layers run on the VALU.
+// are float32. In the forward pass every layer runs on the gfx950 matrix
+// cores (v_mfma_f32_32x32x16_bf16): the three 32 -> 32 layers (98% of the
+// flops), the 1 -> 32 layer as one K-step whose K = (tap) holds the 5 taps
+// and 11 zeros, and the 32 -> 1 layer as the hidden layers' GEMM with one
+// non-zero output row, so the VALU is left with the biases, GELU and the
+// residual. The backward passes run the edge layers on the VALU.
 //
 // One wavefront owns one sample at a time. Its activations live in LDS as
 // [position][channel] rows of 40 bfloat16 (32 channels + 8 pad, 80 bytes, so
@@ -38,7 +41,11 @@ constexpr int OW4 = OB3 + 32, OB4 = OW4 + 5120, OW5 = OB4 + 32, OB5 = OW5 + 160;
 constexpr int NPARAM = OB5 + 1;
 static_assert(NPARAM == 15809, "parameter count");
 constexpr int KSTEPS = CH * KW / 16;  // 10 MFMA K-steps per 32 -> 32 layer
-constexpr int NFRAG = 6 * KSTEPS * 64;  // packed fragments: 3 forward + 3 transposed layers
+// packed fragments: 3 forward + 3 transposed hidden layers, then the forward
+// edge layers (layer 1: one K-step; layer 5: KSTEPS K-steps, row 0 only)
+constexpr int FRAG_L1 = 6 * KSTEPS * 64;
+constexpr int FRAG_L5 = FRAG_L1 + 64;
+constexpr int NFRAG = FRAG_L5 + KSTEPS * 64;
 constexpr int WAVES = 4;                // wavefronts per workgroup
 
 constexpr float GELU_C = 0.7978845608028654f;  // sqrt(2 / pi)
@@ -162,89 +169,97 @@ __host__ __device__ inline Workspace workspace_at(void* base, int N, long long c
 // (float); act0/act1: this wave's two activation buffers; the result
 // y = x + f(x) is written to y (global). With `ws` the training quantities
 // of sample `smp` are stored.
-template <int N, bool TRAIN>
-__device__ __forceinline__ void forward_sample(const Edge& e, const bf16x8* W, const float* xs,
-                                               __bf16* act0, __bf16* act1, float* __restrict__ y, Workspace ws,
-                                               long long smp, long long count, int lane) {
+// Bias, GELU and the bfloat16 activation rows of 32 x 32 output tiles
+// t0.. (layer index Lz of the training workspace).
+template <int N, bool TRAIN, int NTL>
+__device__ __forceinline__ void act_epilogue(const f32x16 (&acc)[NTL], const float* bias, __bf16* out, int t0,
+                                             int lane, Workspace ws, long long smp, long long count, int Lz) {
   const int c = lane & 31, h = lane >> 5;
-  // layer 1 (1 -> 32) on the VALU: lane = position
-  for (int p = lane; p < N; p += 64) {
-    float xv[KW];
 #pragma unroll
-    for (int k = 0; k < KW; ++k) xv[k] = bf16r(xs[(p + k - 2) & (N - 1)]);
-    if constexpr (TRAIN) ws.A0[smp * N + p] = static_cast<__bf16>(xs[p]);
-    __bf16 outv[CH];
+  for (int q = 0; q < NTL; ++q) {
+    const int pos = (t0 + q) * 32 + c;
 #pragma unroll
-    for (int co = 0; co < CH; ++co) {
-      float z = e.b1[co];
+    for (int j = 0; j < 4; ++j) {
+      bf16x4 v;
 #pragma unroll
-      for (int k = 0; k < KW; ++k) z = fmaf(e.w1[co * KW + k], xv[k], z);
-      const __bf16 a = static_cast<__bf16>(gelu(z));
-      outv[co] = a;
-      if constexpr (TRAIN) {
-        ws.Z[(smp * CH + co) * N + p] = z;
-        ws.A[(smp * CH + co) * N + p] = a;
+      for (int r = 0; r < 4; ++r) {
+        const int co = 8 * j + 4 * h + r;
+        const float z = acc[q][4 * j + r] + bias[co];
+        v[r] = static_cast<__bf16>(gelu(z));
+        if constexpr (TRAIN) {
+          const long long o = (((long long)Lz * count + smp) * CH + co) * N + pos;
+          ws.Z[o] = z;
+          ws.A[o] = v[r];
+        }
       }
+      *reinterpret_cast<bf16x4*>(out + pos * LDA + 8 * j + 4 * h) = v;
     }
+  }
+}
+
+// Forward of one sample by one wavefront. xs: the sample's N inputs in LDS
+// (float); act0/act1: this wave's two activation buffers; W: the hidden
+// layers' A fragments (LDS); F1, F5: the edge layers' A fragments (global,
+// cached); the result y = x + f(x) is written to y (global). With `ws` the
+// training quantities of sample `smp` are stored.
+template <int N, bool TRAIN>
+__device__ __forceinline__ void forward_sample(const Edge& e, const bf16x8* W, const bf16x8* F1, const bf16x8* F5,
+                                               const float* xs, __bf16* act0, __bf16* act1, float* __restrict__ y,
+                                               Workspace ws, long long smp, long long count, int lane) {
+  const int c = lane & 31, h = lane >> 5;
+  constexpr int NTL = tiles_together<N>();
+  if constexpr (TRAIN) {
+    for (int p = lane; p < N; p += 64) ws.A0[smp * N + p] = static_cast<__bf16>(xs[p]);
+  }
+  // layer 1 (1 -> 32): K = tap 0..4 (lanes of the upper half and taps >= 5 hold zeros)
+  {
+    const bf16x8 a1 = F1[lane];
 #pragma unroll
-    for (int g = 0; g < CH / 8; ++g) {
-      bf16x8 v;
+    for (int t0 = 0; t0 < N / 32; t0 += NTL) {
+      f32x16 acc[NTL];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) v[i] = outv[8 * g + i];
-      *reinterpret_cast<bf16x8*>(act0 + p * LDA + 8 * g) = v;
+      for (int q = 0; q < NTL; ++q) {
+        const int pos = (t0 + q) * 32 + c;
+        bf16x8 b;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int k = 8 * h + i;
+          b[i] = k < KW ? static_cast<__bf16>(xs[(pos + k - 2) & (N - 1)]) : static_cast<__bf16>(0.f);
+        }
+        acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, f32x16{}, 0, 0, 0);
+      }
+      act_epilogue<N, TRAIN, NTL>(acc, e.b1, act0, t0, lane, ws, smp, count, 0);
     }
   }
   wave_sync();
-  // layers 2-4 (32 -> 32) on the matrix cores
+  // layers 2-4 (32 -> 32)
   __bf16* in = act0;
   __bf16* out = act1;
-  constexpr int NTL = tiles_together<N>();
 #pragma unroll
   for (int L = 0; L < 3; ++L) {
 #pragma unroll
     for (int t0 = 0; t0 < N / 32; t0 += NTL) {
       f32x16 acc[NTL];
       conv_tiles<N, false, NTL>(acc, W + L * KSTEPS * 64, in, t0, lane);
-#pragma unroll
-      for (int q = 0; q < NTL; ++q) {
-        const int pos = (t0 + q) * 32 + c;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          bf16x4 v;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int co = 8 * j + 4 * h + r;
-            const float z = acc[q][4 * j + r] + e.bh[L][co];
-            v[r] = static_cast<__bf16>(gelu(z));
-            if constexpr (TRAIN) {
-              const long long o = (((long long)(L + 1) * count + smp) * CH + co) * N + pos;
-              ws.Z[o] = z;
-              ws.A[o] = v[r];
-            }
-          }
-          *reinterpret_cast<bf16x4*>(out + pos * LDA + 8 * j + 4 * h) = v;
-        }
-      }
+      act_epilogue<N, TRAIN, NTL>(acc, e.bh[L], out, t0, lane, ws, smp, count, L + 1);
     }
     wave_sync();
     __bf16* tmp = in;
     in = out;
     out = tmp;
   }
-  // layer 5 (32 -> 1) on the VALU, residual output
-  for (int p = lane; p < N; p += 64) {
-    float o = e.b5;
+  // layer 5 (32 -> 1): output row 0 of the hidden layers' GEMM, residual output
 #pragma unroll
-    for (int k = 0; k < KW; ++k) {
-      const __bf16* row = in + ((p + k - 2) & (N - 1)) * LDA;
+  for (int t0 = 0; t0 < N / 32; t0 += NTL) {
+    f32x16 acc[NTL];
+    conv_tiles<N, false, NTL>(acc, F5, in, t0, lane);
+    if (h == 0) {
 #pragma unroll
-      for (int g = 0; g < CH / 8; ++g) {
-        const bf16x8 v = *reinterpret_cast<const bf16x8*>(row + 8 * g);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) o = fmaf(e.w5[(8 * g + i) * KW + k], static_cast<float>(v[i]), o);
+      for (int q = 0; q < NTL; ++q) {
+        const int pos = (t0 + q) * 32 + c;
+        y[pos] = xs[pos] + (e.b5 + acc[q][0]);
       }
     }
-    y[p] = xs[p] + o;
   }
   wave_sync();
 }
