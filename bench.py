@@ -399,6 +399,8 @@ def wl_ks2d(args, ex, _lib, torch):
     x = torch.tensor(u0, dtype=torch.float32, device="cuda:0")
     lib = _lib.lib()
     plan = st.plan(B)
+    if args.strategy:
+        plan.set_rollout_strategy(args.strategy)
     tr = torch.empty((B, K, 1, N, N), dtype=torch.float32, device="cuda:0")
     sz = N * N
     # autotuner probe, graph capture; then W untimed replays of the K-step graph
@@ -412,8 +414,9 @@ def wl_ks2d(args, ex, _lib, torch):
     strategy = plan.rollout_strategy()
     kernel = {-1: "row2d/col2d kernels, per-step physical round trip (6 launches per step)",
               1: "roll2d_kernel<128, gradient norm> (workgroup-resident, one launch)",
-              2: "row2d/col2d kernels, spectrum kept in the column layout (4 launches per step)"}[strategy]
-    n_launches = 1 if strategy == 1 else K
+              2: "row2d/col2d kernels, spectrum kept in the column layout (4 launches per step)",
+              3: "coop2d_kernel<128, gradient norm> (the 4-pass chain as one cooperative launch)"}[strategy]
+    n_launches = 1 if strategy in (1, 3) else K
     res = result_line("ks2d_n128_b32_etdrk2",
                       {"equation": "KuramotoSivashinsky", "num_points": N, "domain_extent": L, "dt": DT,
                        "order": 2, "batch": B, "rollout_steps": K, "ic": "TFS cutoff 5 seed 4"},
@@ -792,6 +795,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="burgers1d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--strategy", type=int, default=0,
+                    help="ks2d: force a rollout strategy (-1, 1, 2, 3; default: the autotuner's)")
     ap.add_argument("--cpu-sample-batch", type=int, default=0)
     ap.add_argument("--cpu-sample-steps", type=int, default=0)
     ap.add_argument("--train-iters", type=int, default=0, help="corrector workload: training iterations")

@@ -22,6 +22,7 @@
 #include "gfft.hpp"
 #include "spectral3d.hpp"
 #include "step1d.hpp"
+#include "coop2d.hpp"
 #include "roll2d.hpp"
 #include "step2d.hpp"
 
@@ -81,6 +82,13 @@ using LaunchRoll2D = hipError_t (*)(const Roll2DArgs&, hipStream_t);
 LaunchRoll2D get_roll2d_32(int kind);
 LaunchRoll2D get_roll2d_64(int kind);
 LaunchRoll2D get_roll2d_128(int kind);
+using LaunchCoop2D = hipError_t (*)(const Coop2DArgs&, hipStream_t);
+using Coop2DCapacity = long long (*)();
+LaunchCoop2D get_coop2d_32(int kind, Coop2DCapacity* cap);
+LaunchCoop2D get_coop2d_64(int kind, Coop2DCapacity* cap);
+LaunchCoop2D get_coop2d_128(int kind, Coop2DCapacity* cap);
+LaunchCoop2D get_coop2d_256(int kind, Coop2DCapacity* cap);
+LaunchCoop2D get_coop2d_512(int kind, Coop2DCapacity* cap);
 using LaunchAdj2D = hipError_t (*)(const Adj2DArgs&, int, hipStream_t);
 LaunchAdj2D get_adj2d_32(int is_row, int mode);
 LaunchAdj2D get_adj2d_64(int is_row, int mode);
@@ -93,6 +101,15 @@ static LaunchRoll2D get_roll2d(int N, int kind) {
     case 32: return get_roll2d_32(kind);
     case 64: return get_roll2d_64(kind);
     case 128: return get_roll2d_128(kind);
+    default: return nullptr;
+  }
+}
+
+static LaunchCoop2D get_coop2d(int N, int kind, Coop2DCapacity* cap) {
+  *cap = nullptr;
+  switch (N) {
+    case 128: return get_coop2d_128(kind, cap);
+    case 256: return get_coop2d_256(kind, cap);
     default: return nullptr;
   }
 }
@@ -395,6 +412,8 @@ struct exmi_plan {
   Launch2D row2d[ROW_NUM_MODES] = {};
   LaunchRoll2D roll2d = nullptr;  // 2-D workgroup-resident rollout (strategy 1)
   int fr2d = 0;                   // 2-D spectrum-resident launch chain available (strategy 2)
+  LaunchCoop2D coop2d = nullptr;  // the chain as one cooperative launch (strategy 3)
+  unsigned* coop_sync = nullptr;  // its arrival counters and failure word
   Launch2D col2d[COL_NUM_MODES] = {};
   Twiddles tw_col;
   cf *S0 = nullptr, *N0 = nullptr, *S1 = nullptr, *F = nullptr, *T2 = nullptr, *ACC = nullptr;
@@ -646,6 +665,17 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   // fields have Hermitian transforms): four launches per step instead of
   // six (no forward transform of the emitted state).
   p->fr2d = d->order == 2 && !cplx && p->kmax >= 0 && p->kmax < N / 2 && !(d->flags & EXMI_FLAG_PHYSICAL_ROLLOUT);
+  // The chain as one launch whose workgroups meet at barriers between the
+  // passes (single-channel gradient norm and vorticity, N = 128 and 256),
+  // where the whole grid fits the device at once.
+  if (p->fr2d && C == 1) {
+    Coop2DCapacity cap = nullptr;
+    LaunchCoop2D f = get_coop2d(N, kind, &cap);
+    if (f && cap && p->ninst <= cap() &&
+        hipMalloc(&p->coop_sync, sizeof(unsigned) * (64 * (size_t)p->ninst + 64)) == hipSuccess)
+      p->coop2d = f;
+    (void)hipGetLastError();
+  }
   if (p->roll2d || p->fr2d) p->roll_nt = 0;  // the autotuner picks among them and the per-step launches
   if (d->nonlin_kind == EXMI_NL_KOLMOGOROV) {
     // _vorticity_convection.py:166-176: -k_f gamma scaling(coef_extraction)
@@ -1673,6 +1703,7 @@ int exmi_plan_destroy(exmi_plan* p) {
   for (cf* b : {p->XA3, p->XB3, p->XC3, p->LAM3, p->MU3, p->T3, p->LA3, p->LB3, p->LC3, p->FH3}) hipFree(b);
   hipFree(p->RH3);
   hipFree(p->Q3);
+  hipFree(p->coop_sync);
   if (p->twin) exmi_plan_destroy(p->twin);
   delete p;
   return EXMI_OK;
@@ -1684,7 +1715,7 @@ int exmi_plan_destroy(exmi_plan* p) {
 static bool strategy_ok(const exmi_plan* p, int nt) {
   if (nt == -1) return !(p->fourier_resident && !p->fr_auto);
   if (p->md) return false;
-  if (p->ndim == 2) return (nt == 1 && p->roll2d != nullptr) || (nt == 2 && p->fr2d);
+  if (p->ndim == 2) return (nt == 1 && p->roll2d != nullptr) || (nt == 2 && p->fr2d) || (nt == 3 && p->coop2d);
   return (nt == 1 || nt == 2) && p->roll[nt] != nullptr;
 }
 
@@ -1903,6 +1934,26 @@ static int enqueue_chain_fr2d(exmi_plan* p, const float* u0, long long u0_bs, fl
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "2-D rollout launch failed: %s", hipGetErrorString(e));
 }
 
+// The chain as one cooperative launch for all n steps (states as in
+// enqueue_chain_fr2d).
+static int enqueue_coop2d(exmi_plan* p, const float* u0, long long u0_bs, float* out, long long out_bs,
+                          long long out_ts, int n, int final_only, hipStream_t st) {
+  Coop2DArgs ca;
+  fill_args2d(p, &ca.s);
+  ca.u0 = u0;
+  ca.u0_bstride = u0_bs;
+  ca.traj = out;
+  ca.traj_bstride = out_bs;
+  ca.traj_tstride = out_ts;
+  ca.nsteps = n;
+  ca.final_only = final_only;
+  ca.ninst = p->ninst;
+  ca.sync = p->coop_sync;
+  ca.spin_limit = 1 << 20;
+  const hipError_t e = p->coop2d(ca, st);
+  return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "2-D cooperative rollout launch failed: %s", hipGetErrorString(e));
+}
+
 // Rollout as per-step launches (nt <= 0) or one multi-step launch carrying
 // nt transforms per lane group.
 static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u0_bs, float* traj, int n,
@@ -1928,6 +1979,7 @@ static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u
   if (p->ndim == 2) {
     if (nt == 1 && p->roll2d) return enqueue_roll2d(p, u0, u0_bs, traj + off * sz, tstride, sz, n, 0, st);
     if (nt == 2 && p->fr2d) return enqueue_chain_fr2d(p, u0, u0_bs, traj + off * sz, tstride, sz, n, 0, st);
+    if (nt == 3 && p->coop2d) return enqueue_coop2d(p, u0, u0_bs, traj + off * sz, tstride, sz, n, 0, st);
     for (int t = 0; t < n; ++t) {
       int rc = enqueue_step2d(p, u0, u0_bs, traj + (long long)(t + off) * sz, tstride, t > 0, t < n - 1, st);
       if (rc) return rc;
@@ -2124,8 +2176,8 @@ static int autotune_rollout(exmi_plan* p, const float* u0, long long u0_bs, floa
   HIP_TRY(hipEventCreate(&e1));
   float best = 1e30f;
   int best_nt = -1;
-  const int cands[3] = {-1, 1, 2};
-  for (int ci = 0; ci < 3; ++ci) {
+  const int cands[4] = {-1, 1, 2, 3};
+  for (int ci = 0; ci < 4; ++ci) {
     const int nt = cands[ci];
     if (!strategy_ok(p, nt)) continue;
     float tmin = 1e30f;
@@ -2220,6 +2272,7 @@ static int enq_repeat_cb(exmi_plan* p, const void* a, const void* b, long long s
     const int nt2 = p->roll_nt != 0 ? p->roll_nt : (p->fr2d ? 2 : -1);
     if (nt2 == 1 && p->roll2d) return enqueue_roll2d(p, u0, s, out, sz, 0, n, 1, st);
     if (nt2 == 2 && p->fr2d) return enqueue_chain_fr2d(p, u0, s, out, sz, 0, n, 1, st);
+    if (nt2 == 3 && p->coop2d) return enqueue_coop2d(p, u0, s, out, sz, 0, n, 1, st);
     for (int t = 0; t < n; ++t) {
       int rc = enqueue_step2d(p, u0, s, out, sz, t > 0, t < n - 1, st);
       if (rc) return rc;

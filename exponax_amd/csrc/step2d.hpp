@@ -170,29 +170,57 @@ struct RowCfg {
   static_assert(T <= 64, "row transform must fit one wavefront");
 };
 
+// Global accesses of the row/column interface buffers (T, F). COH = true
+// (the cooperative rollout, where another workgroup of the same launch reads
+// what this one wrote after a barrier) makes each access coherent across the
+// device's L2 caches by itself: a relaxed agent-scope 64-bit atomic load or
+// store, which the hardware serves at the device's point of coherence, so
+// the barrier needs no cache-wide writeback or invalidate.
+template <bool COH>
+__device__ __forceinline__ cf gld(const cf* p) {
+  if constexpr (COH) {
+    const unsigned long long v =
+        __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return cf{__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32))};
+  } else {
+    return *p;
+  }
+}
+
+template <bool COH>
+__device__ __forceinline__ void gst(cf* p, cf v) {
+  if constexpr (COH) {
+    const unsigned long long w =
+        (unsigned long long)__float_as_uint(v.re) | ((unsigned long long)__float_as_uint(v.im) << 32);
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    *p = v;
+  }
+}
+
 // Tile [ky][r] (padded) of R rows <-> column-major global field rows x0..x0+R.
-template <int N>
+template <int N, bool COH = false>
 __device__ __forceinline__ void tile_load(cf* tl, const cf* __restrict__ g, int x0, int kc) {
   using C = RowCfg<N>;
   for (int i = threadIdx.x; i < C::NY * C::R; i += C::WG) {
     const int ky = i / C::R, r = i % C::R;
-    tl[ky * C::LD + r] = ky < kc ? g[(long long)ky * N + x0 + r] : cf{0.f, 0.f};
+    tl[ky * C::LD + r] = ky < kc ? gld<COH>(g + (long long)ky * N + x0 + r) : cf{0.f, 0.f};
   }
 }
 
-template <int N>
+template <int N, bool COH = false>
 __device__ __forceinline__ void tile_store(const cf* tl, cf* __restrict__ g, int x0, int kc) {
   using C = RowCfg<N>;
   for (int i = threadIdx.x; i < kc * C::R; i += C::WG) {
     const int ky = i / C::R, r = i % C::R;
-    g[(long long)ky * N + x0 + r] = tl[ky * C::LD + r];
+    gst<COH>(g + (long long)ky * N + x0 + r, tl[ky * C::LD + r]);
   }
 }
 
 // Register-staged copy of one [ky][x0, x0 + R) tile (ky < kc), for software
 // pipelining: load() issues the global reads, store() writes them to the
 // LDS tile later, so the next field's reads overlap the current transform.
-template <int N>
+template <int N, bool COH = false>
 struct TileRegs {
   using C = RowCfg<N>;
   static constexpr int CNT = (C::NY * C::R + C::WG - 1) / C::WG;
@@ -202,7 +230,7 @@ struct TileRegs {
     for (int q = 0; q < CNT; ++q) {
       const int i = threadIdx.x + q * C::WG;
       const int ky = i / C::R, r = i % C::R;
-      v[q] = (i < C::NY * C::R && ky < kc) ? g[(long long)ky * N + x0 + r] : cf{0.f, 0.f};
+      v[q] = (i < C::NY * C::R && ky < kc) ? gld<COH>(g + (long long)ky * N + x0 + r) : cf{0.f, 0.f};
     }
   }
   __device__ __forceinline__ void store(cf* tl) const {
@@ -230,16 +258,20 @@ __device__ __forceinline__ void row_to_tile(cf* tl, int r, int j, const cf (*z)[
   if (j == 0) tl[C::M * C::LD + r] = nyq[0];
 }
 
-template <int N, int MODE, int KIND>
-__global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
+template <int N>
+struct RowLds {
+  static constexpr int X = RowCfg<N>::R * LdsSize<RowCfg<N>::M>::value;  // transforms' exchange regions
+  static constexpr int TL = RowCfg<N>::NY * RowCfg<N>::LD;               // transpose tile
+};
+
+// One row workgroup's pass: rows x0 = bx R .. x0 + R of instance inst.
+template <int N, int MODE, int KIND, bool COH>
+__device__ __forceinline__ void row2d_body(const Step2DArgs& a, int bx, int inst, cf* lds_x, cf* lds_t) {
   using C = RowCfg<N>;
   constexpr int M = C::M, PT = C::PT, T = C::T, R = C::R;
   constexpr int LDSN = LdsSize<M>::value;
-  __shared__ cf lds_x[R * LDSN];
-  __shared__ cf lds_t[C::NY * C::LD];
 
-  const int inst = blockIdx.y;
-  const int x0 = blockIdx.x * R;
+  const int x0 = bx * R;
   const int r = threadIdx.x / T, j = threadIdx.x % T;
   const int x = x0 + r;
   cf* lds = lds_x + r * LDSN;
@@ -266,10 +298,10 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
     rfft<M, PT, 1>(z, nyq, ln, lds, tr);
     row_to_tile<N>(lds_t, r, j, z, nyq);
     __syncthreads();
-    tile_store<N>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
+    tile_store<N, COH>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
   }
   if constexpr (MODE == ROW_RC || MODE == ROW_RCB) {
-    tile_load<N>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
+    tile_load<N, COH>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
     __syncthreads();
     tile_to_row<N>(lds_t, r, j, z, nyq);
     irfft<M, PT, 1>(z, nyq, ln, lds, tr);
@@ -284,7 +316,7 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
       __syncthreads();
       row_to_tile<N>(lds_t, r, j, z, nyq);
       __syncthreads();
-      tile_store<N>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
+      tile_store<N, COH>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
     }
     // RCB: the products below overwrite T's rows ky < kc of the same x
     // range, which every thread of the workgroup has read into LDS by now
@@ -313,7 +345,7 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
       return a.F + ((long long)i * NF + fld) * fsz;
     };
     cf A[PT], B2[PT], P[NP][PT];
-    TileRegs<N> pf;
+    TileRegs<N, COH> pf;
     pf.load(src(0), x0, kc);
 #pragma unroll
     for (int f = 0; f < NIN; ++f) {
@@ -437,9 +469,16 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
       __syncthreads();
       row_to_tile<N>(lds_t, r, j, z, nyq);
       __syncthreads();
-      tile_store<N>(lds_t, a.T + (long long)inst * fstride + p * fsz, x0, kc);
+      tile_store<N, COH>(lds_t, a.T + (long long)inst * fstride + p * fsz, x0, kc);
     }
   }
+}
+
+template <int N, int MODE, int KIND>
+__global__ void __launch_bounds__(RowCfg<N>::WG) row2d_kernel(Step2DArgs a) {
+  __shared__ cf lds_x[RowLds<N>::X];
+  __shared__ cf lds_t[RowLds<N>::TL];
+  row2d_body<N, MODE, KIND, false>(a, blockIdx.x, blockIdx.y, lds_x, lds_t);
 }
 
 template <int N, int MODE, int KIND>
@@ -462,8 +501,14 @@ struct ColCfg {
   static_assert(T <= 64, "column transform must fit one wavefront");
 };
 
-template <int N, int MODE, int KIND>
-__global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
+template <int N>
+struct ColLds {
+  static constexpr int X = ColCfg<N>::CPB * LdsSize<N>::value;
+};
+
+// One column workgroup's pass: columns ky = bx CPB .. of instance inst.
+template <int N, int MODE, int KIND, bool COH>
+__device__ __forceinline__ void col2d_body(const Step2DArgs& a, int bx, int inst, cf* lds_x) {
   using C = ColCfg<N>;
   constexpr int PT = C::PT, T = C::T;
   constexpr int LDSN = LdsSize<N>::value;
@@ -471,11 +516,9 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
   constexpr float NN = (float)N;
   // 2 N_hat from the forward transform of the product (8 N^4 F[product])
   constexpr float q2 = 2.f / (8.f * NN * NN * NN * NN);
-  __shared__ cf lds_x[C::CPB * LDSN];
 
-  const int inst = blockIdx.y;
   const int c = threadIdx.x / T, j = threadIdx.x % T;
-  const int ky_raw = blockIdx.x * C::CPB + c;
+  const int ky_raw = bx * C::CPB + c;
   const bool active = ky_raw < C::NY;
   const int ky = active ? ky_raw : 0;  // inactive groups mirror column 0, store nothing
   cf* lds = lds_x + c * LDSN;
@@ -513,7 +556,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
       for (int p = 0; p < NP; ++p) {
         const cf* pp = a.T + (long long)inst * a.tstride + ((long long)p * kc + ky) * N;
 #pragma unroll
-        for (int s = 0; s < PT; ++s) v[p][s] = pp[j + s * T];
+        for (int s = 0; s < PT; ++s) v[p][s] = gld<COH>(pp + j + s * T);
       }
       cfft<N, PT, false, NP>(v, ln, lds, tr);
       // the polynomial product is formed at unit scale, so its forward
@@ -588,7 +631,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     for (int f = 0; f < NF; ++f) {
       cf* o = F + ((long long)f * kc + ky) * N;
 #pragma unroll
-      for (int s = 0; s < PT; ++s) o[j + s * T] = v[f][s];
+      for (int s = 0; s < PT; ++s) gst<COH>(o + j + s * T, v[f][s]);
     }
   };
 
@@ -623,7 +666,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     if (active) {
       cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
 #pragma unroll
-      for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
+      for (int s = 0; s < PT; ++s) gst<COH>(t + j + s * T, v[0][s]);
     }
   };
 
@@ -631,7 +674,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     cf v[1][PT];
     const cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
 #pragma unroll
-    for (int s = 0; s < PT; ++s) v[0][s] = t[j + s * T];
+    for (int s = 0; s < PT; ++s) v[0][s] = gld<COH>(t + j + s * T);
     cfft<N, PT, false, 1>(v, ln, lds, tr);
     if (active) {
       cf* s0 = a.S0 + (long long)inst * full + (long long)ky * N;
@@ -744,7 +787,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
       } else {
         const cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
 #pragma unroll
-        for (int s = 0; s < PT; ++s) v[0][s] = t[j + s * T];
+        for (int s = 0; s < PT; ++s) v[0][s] = gld<COH>(t + j + s * T);
         cfft<N, PT, false, 1>(v, ln, lds, tr);
       }
 #pragma unroll
@@ -783,7 +826,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     cf v[1][PT];
     const cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
 #pragma unroll
-    for (int s = 0; s < PT; ++s) v[0][s] = t[j + s * T];
+    for (int s = 0; s < PT; ++s) v[0][s] = gld<COH>(t + j + s * T);
     cfft<N, PT, false, 1>(v, ln, lds, tr);
     if (active) {
       cf* o = a.spec + (long long)inst * a.spec_bstride + ky;
@@ -801,7 +844,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     if (active) {
       cf* t = a.T + (long long)inst * a.tstride + (long long)ky * N;
 #pragma unroll
-      for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
+      for (int s = 0; s < PT; ++s) gst<COH>(t + j + s * T, v[0][s]);
     }
   } else {
     // CC: S' = S1 + c2 (N1 - N0) -> inverse column transform -> T. CCA
@@ -826,6 +869,12 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
     }
     finish(v);
   }
+}
+
+template <int N, int MODE, int KIND>
+__global__ void __launch_bounds__(ColCfg<N>::WG) col2d_kernel(Step2DArgs a) {
+  __shared__ cf lds_x[ColLds<N>::X];
+  col2d_body<N, MODE, KIND, false>(a, blockIdx.x, blockIdx.y, lds_x);
 }
 
 template <int N, int MODE, int KIND>

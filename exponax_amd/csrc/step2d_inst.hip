@@ -2,6 +2,7 @@
 // compile time with -DEXMI_N2=<N>. The build compiles this file once per
 // supported size.
 #include "adjoint2d.hpp"
+#include "coop2d.hpp"
 #include "roll2d.hpp"
 #include "step2d.hpp"
 
@@ -94,6 +95,28 @@ LaunchRoll2D EXMI_CAT(get_roll2d_, EXMI_N2)(int kind) {
     if (kind == NL_GRADNORM) return &launch_roll2d<N, NL_GRADNORM>;
   }
   (void)kind;
+  return nullptr;
+}
+
+using LaunchCoop2D = hipError_t (*)(const Coop2DArgs&, hipStream_t);
+using Coop2DCapacity = long long (*)();
+
+// cooperative ETDRK2 rollouts (N = 128 and 256; gradient norm and
+// vorticity); *cap: instances one launch can hold
+LaunchCoop2D EXMI_CAT(get_coop2d_, EXMI_N2)(int kind, Coop2DCapacity* cap) {
+  constexpr int N = EXMI_N2;
+  if constexpr (N == 128 || N == 256) {
+    if (kind == NL_VORT) {
+      *cap = &coop2d_max_instances<N, NL_VORT>;
+      return &launch_coop2d<N, NL_VORT>;
+    }
+    if (kind == NL_GRADNORM) {
+      *cap = &coop2d_max_instances<N, NL_GRADNORM>;
+      return &launch_coop2d<N, NL_GRADNORM>;
+    }
+  }
+  (void)kind;
+  *cap = nullptr;
   return nullptr;
 }
 

@@ -173,8 +173,9 @@ def test_resident_rollout_2d(device, kind, N):
     transform of the emitted state is an exact identity (real L, Nyquist
     row and column dealiased): the workgroup-resident kernel (strategy 1,
     N <= 128, gradient norm and vorticity) and the four-launch chain that
-    carries S' in the column layout (strategy 2, every kind), against the
-    per-step row/column launches (strategy -1: the float32 physical round
+    carries S' in the column layout (strategy 2, every kind) and the same
+    chain as one cooperative launch (strategy 3: N = 128 and 256, gradient
+    norm and vorticity), against the per-step row/column launches (strategy -1: the float32 physical round
     trip every step) and the FP32 oracle at steps 1 and 10, with
     per-instance operator tables (Kolmogorov ensemble), two channels
     (Burgers) and shared tables; repeat equals the rollout's last state;
@@ -188,7 +189,7 @@ def test_resident_rollout_2d(device, kind, N):
     x = torch.tensor(u0, device=device)
     plan = gpu.plan(B)
     outs = {}
-    for strat in (-1, 1, 2):
+    for strat in (-1, 1, 2, 3):
         try:
             plan.set_rollout_strategy(strat)
         except NotImplementedError:
@@ -196,6 +197,12 @@ def test_resident_rollout_2d(device, kind, N):
         outs[strat] = gpu.rollout_batch(x, n, include_init=True).cpu().numpy()
         np.testing.assert_array_equal(outs[strat][:, 0], u0)
     assert 2 in outs and (1 in outs) == (N <= 128 and kind in ("ks", "kolmogorov"))
+    assert (3 in outs) == (N in (128, 256) and kind in ("ks", "kolmogorov"))
+    if 3 in outs:
+        # the cooperative launch runs the chain's passes: the same arithmetic
+        d = max_rel_l2(outs[3][:, 1:].reshape(B * n, C, N, N), outs[2][:, 1:].reshape(B * n, C, N, N))
+        print(f"  strategy 3 vs 2: {d:.2e} (bit-identical: {np.array_equal(outs[3], outs[2])})", flush=True)
+        assert d <= 1e-6
     ref = outs[-1]
     for strat, tr in outs.items():
         d = max_rel_l2(tr[:, 1:].reshape(B * n, C, N, N), ref[:, 1:].reshape(B * n, C, N, N))
