@@ -367,7 +367,10 @@ def wl_kolmogorov2d(args, ex, _lib, torch):
                       "row2d/col2d kernels per grid size: the spectrum-carrying chain (4 launches per step), "
                       "the workgroup-resident kernel or the six-launch chain, as the autotuner picks",
                       read_traffic("kolmogorov2d_mixed", nst))
-    res["rollout_strategy_by_grid"] = {f"{n}^2": st.plan(b).rollout_strategy() for st, (n, b) in zip(steppers, groups)}
+    # a member tuned to the cooperative launch (3) runs the launch chain (2)
+    # inside a group of several plans
+    res["rollout_strategy_by_grid"] = {f"{n}^2": (lambda s: 2 if s == 3 else s)(st.plan(b).rollout_strategy())
+                                       for st, (n, b) in zip(steppers, groups)}
     res["nan_trajectories"] = int(sum((~torch.isfinite(t[:, -1]).reshape(t.shape[0], -1).all(1)).sum().item()
                                       for t in trs))
     res["launch_mode"] = modes[best]

@@ -2380,11 +2380,28 @@ int exmi_rollout_checked(exmi_plan* p, const float* u0, int64_t u0_bs, float* tr
 
 // ---- grouped rollouts (see exmi_rollout_group)
 
+// A member of a group of several plans rolls out with its tuned strategy,
+// except that the cooperative launch (3) runs as the launch chain (2) it
+// equals bit for bit: its workgroups stay resident for the whole rollout
+// and, beside the other members' launches on concurrent streams, slowed
+// the mixed Kolmogorov batch (32 x 128^2 + 16 x 256^2 + 8 x 512^2: 99.4
+// against 86-92 us per step).
+static int member_nt(const exmi_plan* p, int np) { return (np > 1 && p->roll_nt == 3) ? 2 : p->roll_nt; }
+
+static int enqueue_member(exmi_plan* p, int np, const float* u0, long long u0_bs, float* traj, int n, int include_init,
+                          hipStream_t st) {
+  return enqueue_rollout_nt(p, member_nt(p, np), u0, u0_bs, traj, n, include_init, st);
+}
+
+static int enq_member_cb(exmi_plan* p, const void* a, const void* b, long long s, int n, int flag, hipStream_t st) {
+  return enqueue_member(p, 2, (const float*)a, s, (float*)b, n, flag, st);
+}
+
 static int enqueue_group(exmi_plan* const* plans, int np, const float* const* u0s, const int64_t* bs,
                          float* const* trajs, int n, int include_init, hipStream_t main, bool fork) {
   if (!fork || np == 1) {
     for (int i = 0; i < np; ++i) {
-      int rc = enqueue_rollout(plans[i], u0s[i], bs[i], trajs[i], n, include_init, main);
+      int rc = enqueue_member(plans[i], np, u0s[i], bs[i], trajs[i], n, include_init, main);
       if (rc) return rc;
     }
     return EXMI_OK;
@@ -2396,10 +2413,10 @@ static int enqueue_group(exmi_plan* const* plans, int np, const float* const* u0
   for (int i = 1; i < np && !rc; ++i) {
     hipStream_t s = plans[i]->capture_stream;
     if (hipStreamWaitEvent(s, ev[0], 0) != hipSuccess) rc = fail(EXMI_ERR_HIP, "fork failed");
-    if (!rc) rc = enqueue_rollout(plans[i], u0s[i], bs[i], trajs[i], n, include_init, s);
+    if (!rc) rc = enqueue_member(plans[i], np, u0s[i], bs[i], trajs[i], n, include_init, s);
     if (!rc && hipEventRecord(ev[i], s) != hipSuccess) rc = fail(EXMI_ERR_HIP, "join record failed");
   }
-  if (!rc) rc = enqueue_rollout(plans[0], u0s[0], bs[0], trajs[0], n, include_init, main);
+  if (!rc) rc = enqueue_member(plans[0], np, u0s[0], bs[0], trajs[0], n, include_init, main);
   for (int i = 1; i < np && !rc; ++i)
     if (hipStreamWaitEvent(main, ev[i], 0) != hipSuccess) rc = fail(EXMI_ERR_HIP, "join failed");
   for (auto& e : ev) hipEventDestroy(e);
@@ -2428,8 +2445,11 @@ int exmi_rollout_group(exmi_plan* const* plans, int32_t np, const float* const* 
     // from and joined back into `stream`
     if (use_graph == 2) {
       for (int i = 0; i < np; ++i) {
-        int rc = run_maybe_graph(plans[i], u0s[i], trajs[i], u0_bstrides[i], n, include_init ? 1 : 0, 1, 1,
-                                 nullptr, enq_rollout_cb, false);
+        // graph kind 3 (a group member's rollout) when its strategy differs
+        // from the plan's own, so the plan's cached rollout graph is not reused
+        const bool differs = member_nt(plans[i], np) != plans[i]->roll_nt;
+        int rc = run_maybe_graph(plans[i], u0s[i], trajs[i], u0_bstrides[i], n, include_init ? 1 : 0, differs ? 3 : 1,
+                                 1, nullptr, differs ? enq_member_cb : enq_rollout_cb, false);
         if (rc) return rc;
       }
     }
@@ -2444,7 +2464,7 @@ int exmi_rollout_group(exmi_plan* const* plans, int32_t np, const float* const* 
         if (use_graph == 2) {
           if (hipGraphLaunch(plans[i]->gc.exec, ls) != hipSuccess) rc = fail(EXMI_ERR_HIP, "graph launch failed");
         } else {
-          rc = enqueue_rollout(plans[i], u0s[i], u0_bstrides[i], trajs[i], n, include_init, ls);
+          rc = enqueue_member(plans[i], np, u0s[i], u0_bstrides[i], trajs[i], n, include_init, ls);
         }
       }
       if (!rc && hipEventRecord(ev[i + 1], ls) != hipSuccess) rc = fail(EXMI_ERR_HIP, "join record failed");

@@ -441,7 +441,11 @@ int exmi_plan_get_tables(const exmi_plan* plan, float* host_out, int64_t host_co
  * kernel; the per-step physical round trip), 1 or 2 = one multi-step launch
  * keeping 1 or 2 transforms per lane group on chip for all steps (the
  * Fourier-resident form where the plan has it, see
- * EXMI_FLAG_PHYSICAL_ROLLOUT), 0 = not chosen
+ * EXMI_FLAG_PHYSICAL_ROLLOUT). 2-D: 1 = the workgroup-resident kernel, 2 =
+ * the spectrum-carrying launch chain, 3 = the same chain as one cooperative
+ * launch (single-channel gradient norm and vorticity, N = 128 and 256, when
+ * the whole grid fits the device; inside exmi_rollout_group with several
+ * plans it runs as 2). 0 = not chosen
  * yet (the first exmi_rollout / exmi_rollout_prepare times the candidates on
  * a probe of min(n_steps, 16) steps and keeps the fastest). Setting a
  * strategy drops a cached graph. With the environment variable
