@@ -279,8 +279,9 @@ def wl_ks1d(args, ex, _lib, torch):
     u0 = tfs_ics(1, 1, N, L, 5, np.random.default_rng(1))
     st = ex.stepper.KuramotoSivashinskyConservative(1, L, N, DT)
     x = torch.tensor(u0, dtype=torch.float32, device="cuda:0")
-    if W > 0:
-        st.rollout_batch(x, W)
+    # untimed: the autotuner's probe (a rollout of at least 16 steps, so its
+    # choice is final) and the warm-up
+    st.rollout_batch(x, max(W, 16))
     torch.cuda.synchronize()
     out = {}
     wall, gpu_s = timed_rollout(lambda: out.__setitem__("tr", st.rollout_batch(x, K)), torch.cuda.synchronize)
@@ -447,8 +448,9 @@ def wl_ns3d(args, ex, _lib, torch):
     x = ex.ic.truncated_fourier_series_device(B, 3, L, N, 3, 7, num_channels=3, device="cuda:0")
     u0 = x.cpu().numpy()
     st = ex.stepper.NavierStokesVelocity(3, L, N, DT, diffusivity=0.01)
-    if W > 0:
-        st.rollout_batch(x, W)
+    # untimed: the autotuner's probe (a rollout of at least 16 steps, so its
+    # choice is final) and the warm-up
+    st.rollout_batch(x, max(W, 16))
     torch.cuda.synchronize()
     out = {}
     wall, gpu_s = timed_rollout(lambda: out.__setitem__("tr", st.rollout_batch(x, K)), torch.cuda.synchronize)
@@ -458,6 +460,7 @@ def wl_ns3d(args, ex, _lib, torch):
                        "order": 2, "batch": B, "rollout_steps": K, "ic": "TFS cutoff 3 seed 7 (device)"},
                       B * K / wall, K, W, wall, gpu_s / K, B * 8 * 3 * N ** 3, K,
                       "3-D transform, field, product, combine and stage passes (per step)", None)
+    res["rollout_strategy"] = st.plan(B).rollout_strategy()
     if not args.no_cpu_baseline:
         from oracle import stepper as ost
 
@@ -482,8 +485,9 @@ def wl_kolmogorov2d_513(args, ex, _lib, torch):
     u0 = tfs_ics(B, 2, N, L, 6, rng)
     st = ex.ensemble(lambda r: ex.stepper.KolmogorovFlowVorticity(2, L, N, DT, diffusivity=1.0 / r), re)
     x = torch.tensor(u0, dtype=torch.float32, device="cuda:0")
-    if W > 0:
-        st.rollout_batch(x, W)
+    # untimed: the autotuner's probe (a rollout of at least 16 steps, so its
+    # choice is final) and the warm-up
+    st.rollout_batch(x, max(W, 16))
     torch.cuda.synchronize()
     out = {}
     wall, gpu_s = timed_rollout(lambda: out.__setitem__("tr", st.rollout_batch(x, K)), torch.cuda.synchronize)
@@ -496,6 +500,7 @@ def wl_kolmogorov2d_513(args, ex, _lib, torch):
                       B * K / wall, K, W, wall, gpu_s / K, B * (8 * N * N + 12 * nr), K,
                       "runtime-size transforms (gfft_kernel) and the field, product, combine and stage passes", None)
     res["fft_config"] = st.fft_config(batch=B)
+    res["rollout_strategy"] = st.plan(B).rollout_strategy()
     if not args.no_cpu_baseline:
         from oracle import stepper as ost
 

@@ -35,7 +35,7 @@ hipError_t launch_coef(const double* L, const double* Li, long long total, int n
                        double radius, const double* nf, float* out, hipStream_t st);
 
 hipError_t launch_lines(int N, bool inv, cf* data, long long nlines, long long inner, long long outer_stride,
-                        long long pstride, float scale, const cf* tw, hipStream_t st);
+                        long long pstride, float scale, const cf* tw, hipStream_t st, const cf* src = nullptr);
 hipError_t launch_etdrk0_spec(cf* X, long long ninst, long long nr, int channels, const float* tables, int ncoef,
                               int lin_batch, int lin_channels, int e_im, hipStream_t st);
 hipError_t launch_fields3d(const NL3DArgs& a, const cf* S, cf* F, hipStream_t st);
@@ -382,6 +382,8 @@ struct exmi_plan {
   int ninst = 0;          // 2-D and 3-D: instances = batch * channels
   long long tstride = 0;  // 2-D: complex entries of T per instance
   cf* W3 = nullptr;       // 3-D: spectra of every instance (the step's U0)
+  cf* K3 = nullptr;       // multi-pass spectrum-carrying rollout: the emitted states' inverse transform
+  int md_carry = 0;       // multi-pass plans: the spectrum-carrying rollout (strategy 2) available
   float* P3 = nullptr;    // 3-D: contiguous staging of strided physical states
   // 3-D nonlinear steps: stage spectra, nonlinear terms and evaluation buffers
   cf *N03 = nullptr, *NC3 = nullptr, *S3 = nullptr, *S13 = nullptr, *A3 = nullptr, *F3 = nullptr;
@@ -797,12 +799,14 @@ static hipError_t rfft3(int N, long long I, const float* x, cf* X, const Twiddle
 }
 
 // irfftn: the x and y inverse lines in place on X (each scaled by 1/N), then
-// the z rows c2r (scaled by 1/N, Im of kz = 0 and N/2 ignored) into x.
+// the z rows c2r (scaled by 1/N, Im of kz = 0 and N/2 ignored) into x. With
+// src, the x lines read src instead and X is only the intermediate (src is
+// left unchanged).
 static hipError_t irfft3(int N, long long I, cf* X, float* x, const Twiddles& tz, const Twiddles& tl,
-                         hipStream_t st) {
+                         hipStream_t st, const cf* src = nullptr) {
   const long long NZ = N / 2 + 1;
   const float r = 1.f / (float)N;
-  hipError_t e = launch_lines(N, true, X, I * N * NZ, N * NZ, (long long)N * N * NZ, N * NZ, r, tl.tw, st);
+  hipError_t e = launch_lines(N, true, X, I * N * NZ, N * NZ, (long long)N * N * NZ, N * NZ, r, tl.tw, st, src);
   if (e == hipSuccess) e = launch_lines(N, true, X, I * N * NZ, NZ, N * NZ, NZ, r, tl.tw, st);
   if (e == hipSuccess) e = launch_fft1d(N / 2, true, (const float*)X, x, (int)(I * N * N), tz.rt(), st);
   return e;
@@ -827,8 +831,11 @@ static hipError_t rfft_gen(const GfftCfg& c, int D, long long I, const float* x,
 }
 
 // irfftn: the inverse c2c lines in place on X (outermost axis first, each
-// scaled by 1/N), then c2r along the last axis into x.
-static hipError_t irfft_gen(const GfftCfg& c, int D, long long I, cf* X, float* x, const cf* tw, hipStream_t st) {
+// scaled by 1/N), then c2r along the last axis into x. With src, the first
+// pass reads src instead and X is only the intermediate (src is left
+// unchanged; in 1-D the c2r reads src).
+static hipError_t irfft_gen(const GfftCfg& c, int D, long long I, cf* X, float* x, const cf* tw, hipStream_t st,
+                            const cf* src = nullptr) {
   const int N = c.n;
   const long long NZ = N / 2 + 1;
   const float r = 1.f / (float)N;
@@ -836,11 +843,13 @@ static hipError_t irfft_gen(const GfftCfg& c, int D, long long I, cf* X, float* 
   for (int d = 1; d < D; ++d) total *= N;
   for (int d = 0; d < D - 2; ++d) inner *= N;
   hipError_t e = hipSuccess;
+  const cf* in = src ? src : X;
   for (int ax = 0; ax <= D - 2 && e == hipSuccess; ++ax) {
-    e = launch_gfft(c, GFFT_INV, X, X, total / N, inner, inner * N, inner, r, tw, st);
+    e = launch_gfft(c, GFFT_INV, in, X, total / N, inner, inner * N, inner, r, tw, st);
+    in = X;
     inner /= N;
   }
-  if (e == hipSuccess) e = launch_gfft(c, GFFT_C2R, X, x, total / NZ, 1, 0, 1, r, tw, st);
+  if (e == hipSuccess) e = launch_gfft(c, GFFT_C2R, in, x, total / NZ, 1, 0, 1, r, tw, st);
   return e;
 }
 
@@ -949,7 +958,14 @@ static int plan_create_md(const exmi_plan_desc* d, exmi_plan** out, bool gen) {
   }
   p->np3 = kind == NL_CONV_CONS ? D : kind == NL_GENERAL ? 2 : 1;
   p->kscale = (float)(2.0 * 3.14159265358979323846 / d->domain_extent);
-  p->roll_nt = -1;  // per-step launches; no rollout strategies to choose from
+  // Spectrum-carrying rollouts (strategy 2): with a real L and, for an even
+  // N, the Nyquist modes dealiased out of the nonlinear term, the forward
+  // transform of an emitted state returns the spectrum it came from (exact
+  // arithmetic), so the rollout keeps the spectrum from step to step and
+  // only inverse-transforms each state it emits; the autotuner times it
+  // against the per-step round trip (-1).
+  p->md_carry = !cplx && (N % 2 == 1 || p->kmax < N / 2) && !(d->flags & EXMI_FLAG_PHYSICAL_ROLLOUT);
+  p->roll_nt = p->md_carry ? 0 : -1;
   hipGetDevice(&p->device);
   int rc = gen ? EXMI_OK : get_twiddles(N / 2, &p->tw);
   if (!rc) rc = get_twiddles(N, &p->tw_col);
@@ -967,6 +983,7 @@ static int plan_create_md(const exmi_plan_desc* d, exmi_plan** out, bool gen) {
             hipMalloc(&p->tables, sizeof(float) * total * p->ncoef) == hipSuccess &&
             hipMalloc(&p->W3, sizeof(cf) * I * p->nr) == hipSuccess &&
             hipMalloc(&p->P3, sizeof(float) * I * n3) == hipSuccess &&
+            (!p->md_carry || hipMalloc(&p->K3, sizeof(cf) * I * p->nr) == hipSuccess) &&
             (!nl || (hipMalloc(&p->N03, sizeof(cf) * I * p->nr) == hipSuccess &&
                      hipMalloc(&p->NC3, sizeof(cf) * I * p->nr) == hipSuccess &&
                      hipMalloc(&p->S3, sizeof(cf) * I * p->nr) == hipSuccess &&
@@ -1023,6 +1040,11 @@ static hipError_t rfft_md(const exmi_plan* p, long long I, const float* x, cf* X
 static hipError_t irfft_md(const exmi_plan* p, long long I, cf* X, float* x, hipStream_t st) {
   if (p->gen) return irfft_gen(p->gcfg, p->ndim, I, X, x, p->tw_col.tw, st);
   return irfft3(p->d.num_points, I, X, x, p->tw, p->tw_col, st);
+}
+// irfftn of the spectra X that leaves X unchanged (the intermediate in K)
+static hipError_t irfft_md_keep(const exmi_plan* p, long long I, const cf* X, cf* K, float* x, hipStream_t st) {
+  if (p->gen) return irfft_gen(p->gcfg, p->ndim, I, K, x, p->tw_col.tw, st, X);
+  return irfft3(p->d.num_points, I, K, x, p->tw, p->tw_col, st, X);
 }
 
 static void fill_nl3d(const exmi_plan* p, NL3DArgs& a) {
@@ -1149,6 +1171,72 @@ static int enqueue_step3d(exmi_plan* p, const float* u_in, long long ibs, float*
     if (e == hipSuccess && dst != u_out) e = copy_batch(p->P3, sz, u_out, obs, sz, B, st);
   }
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "%d-D step launch failed: %s", p->ndim, hipGetErrorString(e));
+}
+
+// Multi-pass spectrum-carrying rollout (strategy 2): the initial states are
+// transformed once; each step advances the spectra (the stages of
+// enqueue_step3d, alternating the state between W3 and S3) and emits its
+// state with the inverse transform that leaves the spectrum in place. States
+// go to out + i * out_bs + t * out_ts (only the last, at out, with
+// final_only).
+static int enqueue_md_carry(exmi_plan* p, const float* u0, long long u0_bs, float* out, long long out_bs,
+                            long long out_ts, int n, int final_only, hipStream_t st) {
+  const int B = p->d.batch;
+  const long long sz = (long long)p->d.num_channels * p->n3;
+  hipError_t e = hipSuccess;
+  const float* src = u0;
+  if (u0_bs != sz) {
+    e = copy_batch(u0, u0_bs, p->P3, sz, sz, B, st);
+    src = p->P3;
+  }
+  if (e == hipSuccess) e = rfft_md(p, p->ninst, src, p->W3, st);
+  cf* cur = p->W3;
+  cf* nxt = p->S3;
+  Stage3DArgs sa;
+  sa.ninst = p->ninst;
+  sa.nr = p->nr;
+  sa.channels = p->d.num_channels;
+  sa.ncoef = p->ncoef;
+  sa.lin_batch = p->d.lin_batch;
+  sa.lin_channels = p->d.lin_channels;
+  sa.e_im = -1;  // real L
+  sa.tables = p->tables;
+  sa.N0 = p->N03;
+  sa.NC = p->NC3;
+  sa.S1 = p->S13;
+  sa.A = p->A3;
+  static const int seq[4][7] = {{ST1, -1}, {ST2A, 1, ST2B, -1}, {ST3A, 1, ST3B, 1, ST3C, -1},
+                                {ST4A, 1, ST4B, 1, ST4C, 1, ST4D}};
+  for (int t = 0; t < n && e == hipSuccess; ++t) {
+    if (p->d.order == 0) {
+      e = launch_etdrk0_spec(cur, p->ninst, p->nr, p->d.num_channels, p->tables, p->ncoef, p->d.lin_batch,
+                             p->d.lin_channels, -1, st);
+    } else {
+      sa.U0 = cur;
+      sa.S = nxt;
+      e = nonlinear3d(p, cur, p->N03, st);
+      for (int i = 0; i < 7 && e == hipSuccess; ++i) {
+        const int m = seq[p->d.order - 1][i];
+        if (m < 0) break;
+        if (i % 2 == 1) {
+          e = nonlinear3d(p, nxt, p->NC3, st);
+        } else {
+          sa.mode = m;
+          e = launch_stage3d(sa, st);
+        }
+      }
+      cf* t2 = cur;
+      cur = nxt;
+      nxt = t2;
+    }
+    if (e != hipSuccess || (final_only && t < n - 1)) continue;
+    float* dst = final_only ? out : out + (long long)t * out_ts;
+    const bool direct = out_bs == sz;
+    if (e == hipSuccess) e = irfft_md_keep(p, p->ninst, cur, p->K3, direct ? dst : p->P3, st);
+    if (e == hipSuccess && !direct) e = copy_batch(p->P3, sz, dst, out_bs, sz, B, st);
+  }
+  return e == hipSuccess ? EXMI_OK
+                         : fail(EXMI_ERR_HIP, "%d-D rollout launch failed: %s", p->ndim, hipGetErrorString(e));
 }
 
 // ------------------------------------------------------- multi-pass adjoint
@@ -1679,6 +1767,7 @@ int exmi_plan_destroy(exmi_plan* p) {
   hipFree(p->tables);
   hipFree(p->work);
   hipFree(p->W3);
+  hipFree(p->K3);
   hipFree(p->P3);
   hipFree(p->N03);
   hipFree(p->NC3);
@@ -1714,7 +1803,7 @@ int exmi_plan_destroy(exmi_plan* p) {
 // 1 = the workgroup-resident kernel).
 static bool strategy_ok(const exmi_plan* p, int nt) {
   if (nt == -1) return !(p->fourier_resident && !p->fr_auto);
-  if (p->md) return false;
+  if (p->md) return nt == 2 && p->md_carry;
   if (p->ndim == 2) return (nt == 1 && p->roll2d != nullptr) || (nt == 2 && p->fr2d) || (nt == 3 && p->coop2d);
   return (nt == 1 || nt == 2) && p->roll[nt] != nullptr;
 }
@@ -1757,7 +1846,8 @@ int exmi_plan_set_rollout_strategy(exmi_plan* p, int32_t nt) {
     return fail(EXMI_ERR_INVALID, "Fourier-resident plans roll out with the multi-step kernel only");
   if (!(nt == 0 || strategy_ok(p, nt)))
     return fail(EXMI_ERR_UNSUPPORTED, "rollout strategy %d not available for this plan", nt);
-  if (nt == 0 && (p->md || (p->ndim != 1 && !p->roll2d && !p->fr2d))) nt = -1;  // nothing to choose from
+  if (nt == 0 && ((p->md && !p->md_carry) || (!p->md && p->ndim != 1 && !p->roll2d && !p->fr2d)))
+    nt = -1;  // nothing to choose from
   p->roll_nt = nt;
   p->tune_short = 0;
   if (p->gc.exec) {
@@ -1968,6 +2058,7 @@ static int enqueue_rollout_nt(exmi_plan* p, int nt, const float* u0, long long u
   }
   if (n == 0) return EXMI_OK;
   if (p->md) {
+    if (nt == 2 && p->md_carry) return enqueue_md_carry(p, u0, u0_bs, traj + off * sz, tstride, sz, n, 0, st);
     for (int t = 0; t < n; ++t) {
       const float* in = (t == 0 && !include_init) ? u0 : traj + (long long)(t - 1 + off) * sz;
       const long long ibs = (t == 0 && !include_init) ? u0_bs : tstride;
@@ -2262,6 +2353,7 @@ static int enq_repeat_cb(exmi_plan* p, const void* a, const void* b, long long s
     return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "copy failed: %s", hipGetErrorString(e));
   }
   if (p->md) {
+    if (p->roll_nt == 2 && p->md_carry) return enqueue_md_carry(p, u0, s, out, sz, 0, n, 1, st);
     for (int t = 0; t < n; ++t) {
       int rc = enqueue_step3d(p, t == 0 ? u0 : out, t == 0 ? s : sz, out, sz, st);
       if (rc) return rc;

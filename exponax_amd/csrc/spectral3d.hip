@@ -24,10 +24,12 @@ struct LineCfg {
 };
 
 // Line l = (outer o, inner i) with o = l / inner, i = l % inner starts at
-// o * outer_stride + i; its points are pstride apart. In place, times scale.
+// o * outer_stride + i; its points are pstride apart. Read from src, written
+// to data at the same offsets (in place when src == data), times scale.
 template <int N, bool INV>
-__global__ void __launch_bounds__(256) lines_kernel(cf* data, long long nlines, long long inner, long long outer_stride,
-                                                    long long pstride, float scale, const cf* __restrict__ tw) {
+__global__ void __launch_bounds__(256) lines_kernel(const cf* src, cf* data, long long nlines, long long inner,
+                                                    long long outer_stride, long long pstride, float scale,
+                                                    const cf* __restrict__ tw) {
   using C = LineCfg<N>;
   constexpr int PT = C::PT, T = C::T;
   constexpr int LDSN = LdsSize<N>::value;
@@ -43,7 +45,7 @@ __global__ void __launch_bounds__(256) lines_kernel(cf* data, long long nlines, 
   tr.template load_pass<1>(j, tw);
   cf v[1][PT];
 #pragma unroll
-  for (int s = 0; s < PT; ++s) v[0][s] = data[base + (long long)(j + s * T) * pstride];
+  for (int s = 0; s < PT; ++s) v[0][s] = src[base + (long long)(j + s * T) * pstride];
   cfft<N, PT, INV, 1>(v, ln, lds, tr);
   if (active) {
 #pragma unroll
@@ -52,22 +54,23 @@ __global__ void __launch_bounds__(256) lines_kernel(cf* data, long long nlines, 
 }
 
 template <int N, bool INV>
-static hipError_t launch_lines_n(cf* data, long long nlines, long long inner, long long outer_stride,
+static hipError_t launch_lines_n(const cf* src, cf* data, long long nlines, long long inner, long long outer_stride,
                                  long long pstride, float scale, const cf* tw, hipStream_t st) {
   using C = LineCfg<N>;
   const long long grid = (nlines + C::CPB - 1) / C::CPB;
-  hipLaunchKernelGGL((lines_kernel<N, INV>), dim3((unsigned)grid), dim3(C::WG), 0, st, data, nlines, inner,
+  hipLaunchKernelGGL((lines_kernel<N, INV>), dim3((unsigned)grid), dim3(C::WG), 0, st, src, data, nlines, inner,
                      outer_stride, pstride, scale, tw);
   return hipGetLastError();
 }
 
 hipError_t launch_lines(int N, bool inv, cf* data, long long nlines, long long inner, long long outer_stride,
-                        long long pstride, float scale, const cf* tw, hipStream_t st) {
+                        long long pstride, float scale, const cf* tw, hipStream_t st, const cf* src) {
+  if (!src) src = data;
   switch (N) {
 #define EXMI_LINES(NN)                                                                                        \
   case NN:                                                                                                    \
-    return inv ? launch_lines_n<NN, true>(data, nlines, inner, outer_stride, pstride, scale, tw, st)         \
-               : launch_lines_n<NN, false>(data, nlines, inner, outer_stride, pstride, scale, tw, st);
+    return inv ? launch_lines_n<NN, true>(src, data, nlines, inner, outer_stride, pstride, scale, tw, st)    \
+               : launch_lines_n<NN, false>(src, data, nlines, inner, outer_stride, pstride, scale, tw, st);
     EXMI_LINES(16) EXMI_LINES(32) EXMI_LINES(64) EXMI_LINES(128) EXMI_LINES(256)
 #undef EXMI_LINES
     default: return hipErrorInvalidValue;

@@ -330,3 +330,59 @@ def test_any_size_noalloc_inverse_rejected(device):
     rc = L.exmi_irfft_noalloc(1, 100, 2, ctypes.c_void_p(xh.data_ptr()), ctypes.c_void_p(out.data_ptr()),
                               ctypes.c_void_p(_lib.stream_ptr()))
     assert rc == _lib.EXMI_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("case", ["burgers1d_101", "kolmogorov2d_100", "ks2d_96", "ns3d_32", "burgers3d_12"])
+def test_multipass_spectrum_carrying_rollout(device, case):
+    """Multi-pass plans with a real L (and, for an even N, the Nyquist modes
+    dealiased): the rollout that keeps the spectrum from step to step and
+    only inverse-transforms the states it emits (strategy 2) against the
+    per-step round trip (-1) and the FP32 oracle at steps 1 and 10; repeat
+    equals its last state and include_init keeps u0. A complex L offers only
+    the per-step round trip."""
+    import torch
+
+    print(f"[case] multi-pass spectrum-carrying rollout {case}", flush=True)
+    if case == "burgers1d_101":
+        kw = dict(diffusivity=0.01)
+        gpu, cpu = ex.stepper.Burgers(1, 1.0, 101, 1e-3, **kw), ost.Burgers(1, 1.0, 101, 1e-3, dtype=np.float32, **kw)
+        u0 = ics(3, 1, 1.0, 101, 6, seed=91)[0]
+    elif case == "kolmogorov2d_100":
+        gpu, cpu, u0 = _pair2d("kolmogorov", 100, 2)
+    elif case == "ks2d_96":
+        gpu, cpu, u0 = _pair2d("ks", 96, 2)
+    elif case == "ns3d_32":
+        L, kw = 2 * np.pi, dict(diffusivity=0.02)
+        gpu = ex.stepper.NavierStokesVelocity(3, L, 32, 0.01, **kw)
+        cpu = ost.NavierStokesVelocity(3, L, 32, 0.01, dtype=np.float32, **kw)
+        u0 = ics(2, 3, L, 32, 3, seed=121, channels=3)[0]
+    else:
+        kw = dict(diffusivity=0.01)
+        gpu, cpu = ex.stepper.Burgers(3, 1.0, 12, 1e-3, **kw), ost.Burgers(3, 1.0, 12, 1e-3, dtype=np.float32, **kw)
+        u0 = ics(2, 3, 1.0, 12, 3, seed=92, channels=3)[0]
+    B, n = u0.shape[0], 12
+    x = torch.tensor(u0, device=device)
+    plan = gpu.plan(B)
+    outs = {}
+    for strat in (-1, 2):
+        plan.set_rollout_strategy(strat)
+        outs[strat] = gpu.rollout_batch(x, n, include_init=True).cpu().numpy()
+        np.testing.assert_array_equal(outs[strat][:, 0], u0)
+        last = gpu.repeat_batch(x, n).cpu().numpy()
+        np.testing.assert_array_equal(last, outs[strat][:, -1])
+    d = max(max_rel_l2(outs[2][:, s], outs[-1][:, s]) for s in range(1, n + 1))
+    print(f"  carried vs per-step round trip, max over {n} steps: {d:.2e}", flush=True)
+    u, errs = u0, []
+    for s in range(1, 11):
+        u = cpu.step_batch(u)
+        if s in (1, 10):
+            for strat, tr in outs.items():
+                e = max_rel_l2(tr[:, s], u)
+                print(f"  strategy {strat} step {s}: rel-L2 vs oracle {e:.2e}", flush=True)
+                errs.append(e)
+    # both are float32 computations of the same step; they drift apart by
+    # rounding (a few 1e-7 per step), as each drifts from the oracle
+    assert d <= RTOL_STEP and max(errs) <= RTOL_STEP
+    cplx = ex.stepper.Dispersion(2, 2.0, 100, 1e-3, dispersivity=0.01)
+    with pytest.raises(NotImplementedError):
+        cplx.plan(1).set_rollout_strategy(2)
