@@ -35,10 +35,13 @@ hipError_t launch_coef(const double* L, const double* Li, long long total, int n
                        double radius, const double* nf, float* out, hipStream_t st);
 
 hipError_t launch_lines(int N, bool inv, cf* data, long long nlines, long long inner, long long outer_stride,
-                        long long pstride, float scale, const cf* tw, hipStream_t st, const cf* src = nullptr);
+                        long long pstride, float scale, const cf* tw, hipStream_t st, const cf* src = nullptr,
+                        int kkeep = 0);
 hipError_t launch_etdrk0_spec(cf* X, long long ninst, long long nr, int channels, const float* tables, int ncoef,
                               int lin_batch, int lin_channels, int e_im, hipStream_t st);
 hipError_t launch_fields3d(const NL3DArgs& a, const cf* S, cf* F, hipStream_t st);
+hipError_t launch_fields_lines3d(const NL3DArgs& a, const cf* S, cf* F, float scale, const cf* tw, int kkeep,
+                                 hipStream_t st);
 hipError_t launch_wave_spec(cf* X, long long batch, int D, int N, double kscale, double c, double dt,
                             hipStream_t st);
 hipError_t launch_poisson_spec(cf* X, long long count, int D, int N, double kscale, int order, hipStream_t st);
@@ -46,6 +49,7 @@ hipError_t launch_axpy(long long n, float a, const float* x, const float* y, flo
 hipError_t launch_products3d(const NL3DArgs& a, const float* R, float* PR, hipStream_t st);
 hipError_t launch_combine3d(const NL3DArgs& a, const cf* PH, cf* Nh, hipStream_t st);
 hipError_t launch_stage3d(const Stage3DArgs& a, hipStream_t st);
+hipError_t launch_stage_nl3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* PH, cf* N0out, hipStream_t st);
 hipError_t launch_adj_cofields3d(const NL3DArgs& a, const cf* MU, cf* H, hipStream_t st);
 hipError_t launch_adj_products3d(const NL3DArgs& a, const float* R, const float* HR, float* Q, hipStream_t st);
 hipError_t launch_adj_combine3d(const NL3DArgs& a, const cf* QH, cf* out, hipStream_t st);
@@ -59,7 +63,7 @@ hipError_t launch_adj_linear3d(const AdjLinArgs& a, hipStream_t st);
   RollFn get_roll1d_r_##M(int kind, int order);                                                        \
   VjpFn get_vjp1d_##M(int kind, int order, int cplx);                                                            \
   hipError_t launch_fft1d_##M(bool inverse, const float* in, float* out, int count, const RealTables& rt, \
-                              hipStream_t st);
+                              hipStream_t st, int kcut);
 EXMI_DECL_SIZE(8)
 EXMI_DECL_SIZE(24)
 EXMI_DECL_SIZE(25)
@@ -222,10 +226,10 @@ static RollFn get_roll1d(int M, int kind, int order, int nt, int fr, int std_kc)
 }
 
 static hipError_t launch_fft1d(int M, bool inverse, const float* in, float* out, int count, const RealTables& rt,
-                               hipStream_t st) {
+                               hipStream_t st, int kcut = 1 << 30) {
   switch (M) {
 #define EXMI_CASE(MM) \
-  case MM: return launch_fft1d_##MM(inverse, in, out, count, rt, st);
+  case MM: return launch_fft1d_##MM(inverse, in, out, count, rt, st, kcut);
     EXMI_CASE(8)
     EXMI_CASE(16)
     EXMI_CASE(24)
@@ -1082,15 +1086,64 @@ static void fill_nl3d(const exmi_plan* p, NL3DArgs& a) {
   a.nq = p->adj_nq;
 }
 
-// N(X) of every instance into outN (the four passes of spectral3d.hip).
+// The product spectra of N(X) of every instance into F3 (the passes of
+// spectral3d.hip up to the combine).
+static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf* X, hipStream_t st) {
+  if (p->gen) {
+    hipError_t e = launch_fields3d(a, X, p->F3, st);
+    if (e == hipSuccess) e = irfft_md(p, p->ninst * p->nf3, p->F3, p->R3, st);
+    if (e == hipSuccess) e = launch_products3d(a, p->R3, p->PR3, st);
+    if (e == hipSuccess) e = rfft_md(p, p->ninst * p->np3, p->PR3, p->F3, st);
+    return e;
+  }
+  // 3-D grids of the fused sizes. The fields are dealiased and the combine
+  // keeps only |k| <= kmax, so every pass skips the kz planes above the
+  // cutoff: the inverse x lines (fused with the fields pass) and y lines run
+  // on planes kz <= kmax only, the z rows read the planes above as zero; the
+  // forward z rows store, and the y and x lines transform, planes kz <= kmax
+  // only (the combine never reads the others).
+  const int N = p->d.num_points;
+  const long long NZ = N / 2 + 1, I = p->ninst * p->nf3, IP = p->ninst * p->np3;
+  const int kz = p->kmax < (int)NZ - 1 ? (p->kmax < 0 ? 0 : p->kmax) : (int)NZ - 1;  // last kept plane
+  const float r = 1.f / (float)N;
+  const cf* tl = p->tw_col.tw;
+  hipError_t e = launch_fields_lines3d(a, X, p->F3, r, tl, kz + 1, st);
+  if (e == hipSuccess) e = launch_lines(N, true, p->F3, I * N * NZ, NZ, N * NZ, NZ, r, tl, st, nullptr, kz + 1);
+  if (e == hipSuccess)
+    e = launch_fft1d(N / 2, true, (const float*)p->F3, p->R3, (int)(I * N * N), p->tw.rt(), st, p->kmax);
+  if (e == hipSuccess) e = launch_products3d(a, p->R3, p->PR3, st);
+  if (e == hipSuccess) e = launch_fft1d(N / 2, false, p->PR3, (float*)p->F3, (int)(IP * N * N), p->tw.rt(), st, p->kmax);
+  if (e == hipSuccess) e = launch_lines(N, false, p->F3, IP * N * NZ, NZ, N * NZ, NZ, 1.f, tl, st, nullptr, kz + 1);
+  if (e == hipSuccess)
+    e = launch_lines(N, false, p->F3, IP * N * NZ, N * NZ, (long long)N * N * NZ, N * NZ, 1.f, tl, st, nullptr, kz + 1);
+  return e;
+}
+
+// N(X) of every instance into outN.
 static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t st) {
   NL3DArgs a;
   fill_nl3d(p, a);
-  hipError_t e = launch_fields3d(a, X, p->F3, st);
-  if (e == hipSuccess) e = irfft_md(p, p->ninst * p->nf3, p->F3, p->R3, st);
-  if (e == hipSuccess) e = launch_products3d(a, p->R3, p->PR3, st);
-  if (e == hipSuccess) e = rfft_md(p, p->ninst * p->np3, p->PR3, p->F3, st);
+  hipError_t e = nonlinear3d_products(p, a, X, st);
   if (e == hipSuccess) e = launch_combine3d(a, p->F3, outN, st);
+  return e;
+}
+
+// The stages of one ETDRK1-4 step on the spectra in sa (U0 the state, S the
+// result): each stage follows the evaluation of N at its argument (U0 for
+// the first, then the previous stage's S), the combine pass fused into the
+// stage (the first also keeps N0).
+static hipError_t stages3d(exmi_plan* p, Stage3DArgs& sa, hipStream_t st) {
+  static const int seq[4][4] = {{ST1, -1}, {ST2A, ST2B, -1}, {ST3A, ST3B, ST3C, -1}, {ST4A, ST4B, ST4C, ST4D}};
+  NL3DArgs a;
+  fill_nl3d(p, a);
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < 4 && e == hipSuccess; ++i) {
+    const int m = seq[p->d.order - 1][i];
+    if (m < 0) break;
+    e = nonlinear3d_products(p, a, i == 0 ? sa.U0 : sa.S, st);
+    sa.mode = m;
+    if (e == hipSuccess) e = launch_stage_nl3d(a, sa, p->F3, i == 0 ? p->N03 : nullptr, st);
+  }
   return e;
 }
 
@@ -1140,27 +1193,8 @@ static int enqueue_step3d(exmi_plan* p, const float* u_in, long long ibs, float*
     sa.S = p->S3;
     sa.S1 = p->S13;
     sa.A = p->A3;
-    auto stage = [&](int mode) {
-      if (e != hipSuccess) return;
-      sa.mode = mode;
-      e = launch_stage3d(sa, st);
-    };
-    auto nlev = [&](const cf* in, cf* outN) {
-      if (e == hipSuccess) e = nonlinear3d(p, in, outN, st);
-    };
-    // etdrk/_etdrk_{1,2,3,4}.py: each NC evaluation follows the stage that forms its argument S
-    static const int seq[4][7] = {{ST1, -1}, {ST2A, 1, ST2B, -1}, {ST3A, 1, ST3B, 1, ST3C, -1},
-                                  {ST4A, 1, ST4B, 1, ST4C, 1, ST4D}};
-    nlev(p->W3, p->N03);
-    for (int i = 0; i < 7; ++i) {
-      const int m = seq[p->d.order - 1][i];
-      if (m < 0) break;
-      if (i % 2 == 1) {
-        nlev(p->S3, p->NC3);
-      } else {
-        stage(m);
-      }
-    }
+    // etdrk/_etdrk_{1,2,3,4}.py: each stage after the evaluation of N at its argument
+    if (e == hipSuccess) e = stages3d(p, sa, st);
     X = p->S3;
   }
   if (out_f) {
@@ -1205,8 +1239,6 @@ static int enqueue_md_carry(exmi_plan* p, const float* u0, long long u0_bs, floa
   sa.NC = p->NC3;
   sa.S1 = p->S13;
   sa.A = p->A3;
-  static const int seq[4][7] = {{ST1, -1}, {ST2A, 1, ST2B, -1}, {ST3A, 1, ST3B, 1, ST3C, -1},
-                                {ST4A, 1, ST4B, 1, ST4C, 1, ST4D}};
   for (int t = 0; t < n && e == hipSuccess; ++t) {
     if (p->d.order == 0) {
       e = launch_etdrk0_spec(cur, p->ninst, p->nr, p->d.num_channels, p->tables, p->ncoef, p->d.lin_batch,
@@ -1214,17 +1246,7 @@ static int enqueue_md_carry(exmi_plan* p, const float* u0, long long u0_bs, floa
     } else {
       sa.U0 = cur;
       sa.S = nxt;
-      e = nonlinear3d(p, cur, p->N03, st);
-      for (int i = 0; i < 7 && e == hipSuccess; ++i) {
-        const int m = seq[p->d.order - 1][i];
-        if (m < 0) break;
-        if (i % 2 == 1) {
-          e = nonlinear3d(p, nxt, p->NC3, st);
-        } else {
-          sa.mode = m;
-          e = launch_stage3d(sa, st);
-        }
-      }
+      e = stages3d(p, sa, st);
       cf* t2 = cur;
       cur = nxt;
       nxt = t2;

@@ -14,9 +14,13 @@
 
 namespace exmi {
 
+// 16 points per lane up to N = 128 (T = N/16 lanes per line): a wavefront
+// then holds 64/T neighbouring lines, and for a fixed slot it reads 64/T
+// adjacent entries (128 bytes at N = 64) instead of four; 10% faster 3-D
+// steps than 4 points per lane (Navier-Stokes velocity 64^3).
 template <int N>
 struct LineCfg {
-  static constexpr int PT = N <= 128 ? 4 : 8;
+  static constexpr int PT = N <= 128 ? 16 : 8;
   static constexpr int T = N / PT;
   static constexpr int WG = 256;
   static constexpr int CPB = WG / T;  // lines per workgroup
@@ -26,18 +30,24 @@ struct LineCfg {
 // Line l = (outer o, inner i) with o = l / inner, i = l % inner starts at
 // o * outer_stride + i; its points are pstride apart. Read from src, written
 // to data at the same offsets (in place when src == data), times scale.
+// With kkeep < nz (inner a multiple of nz, a line's last-axis index l % nz),
+// only the lines whose last-axis index is below kkeep run (nlines counts
+// all lines): the kz planes above a dealias cutoff, zero on the way in and
+// discarded on the way out.
 template <int N, bool INV>
 __global__ void __launch_bounds__(256) lines_kernel(const cf* src, cf* data, long long nlines, long long inner,
                                                     long long outer_stride, long long pstride, float scale,
-                                                    const cf* __restrict__ tw) {
+                                                    const cf* __restrict__ tw, int nz, int kkeep) {
   using C = LineCfg<N>;
   constexpr int PT = C::PT, T = C::T;
   constexpr int LDSN = LdsSize<N>::value;
   __shared__ cf lds_x[C::CPB * LDSN];
   const int c = threadIdx.x / T, j = threadIdx.x % T;
   const long long raw = (long long)blockIdx.x * C::CPB + c;
-  const bool active = raw < nlines;
-  const long long line = active ? raw : 0;  // inactive groups mirror line 0 and store nothing
+  const long long nrun = nlines / nz * kkeep;
+  const bool active = raw < nrun;
+  const long long run = active ? raw : 0;  // inactive groups mirror line 0 and store nothing
+  const long long line = kkeep == nz ? run : (run / kkeep) * nz + run % kkeep;
   const long long base = (line / inner) * outer_stride + line % inner;
   cf* lds = lds_x + c * LDSN;
   const Lane<N, PT> ln(j);
@@ -55,22 +65,27 @@ __global__ void __launch_bounds__(256) lines_kernel(const cf* src, cf* data, lon
 
 template <int N, bool INV>
 static hipError_t launch_lines_n(const cf* src, cf* data, long long nlines, long long inner, long long outer_stride,
-                                 long long pstride, float scale, const cf* tw, hipStream_t st) {
+                                 long long pstride, float scale, const cf* tw, int nz, int kkeep, hipStream_t st) {
   using C = LineCfg<N>;
-  const long long grid = (nlines + C::CPB - 1) / C::CPB;
+  const long long grid = (nlines / nz * kkeep + C::CPB - 1) / C::CPB;
   hipLaunchKernelGGL((lines_kernel<N, INV>), dim3((unsigned)grid), dim3(C::WG), 0, st, src, data, nlines, inner,
-                     outer_stride, pstride, scale, tw);
+                     outer_stride, pstride, scale, tw, nz, kkeep);
   return hipGetLastError();
 }
 
 hipError_t launch_lines(int N, bool inv, cf* data, long long nlines, long long inner, long long outer_stride,
-                        long long pstride, float scale, const cf* tw, hipStream_t st, const cf* src) {
+                        long long pstride, float scale, const cf* tw, hipStream_t st, const cf* src, int kkeep) {
   if (!src) src = data;
+  const int nz = N / 2 + 1;
+  if (kkeep <= 0 || kkeep > nz) kkeep = nz;
+  if (inner % nz != 0 || nlines % nz != 0) kkeep = nz;
   switch (N) {
 #define EXMI_LINES(NN)                                                                                        \
   case NN:                                                                                                    \
-    return inv ? launch_lines_n<NN, true>(src, data, nlines, inner, outer_stride, pstride, scale, tw, st)    \
-               : launch_lines_n<NN, false>(src, data, nlines, inner, outer_stride, pstride, scale, tw, st);
+    return inv ? launch_lines_n<NN, true>(src, data, nlines, inner, outer_stride, pstride, scale, tw, nz,    \
+                                          kkeep, st)                                                          \
+               : launch_lines_n<NN, false>(src, data, nlines, inner, outer_stride, pstride, scale, tw, nz,   \
+                                           kkeep, st);
     EXMI_LINES(16) EXMI_LINES(32) EXMI_LINES(64) EXMI_LINES(128) EXMI_LINES(256)
 #undef EXMI_LINES
     default: return hipErrorInvalidValue;
@@ -138,11 +153,7 @@ struct Mode3D {
 // i <= (n - 1) / 2, else i - n (even n: the Nyquist index n/2 is -n/2).
 __device__ __forceinline__ int signed_k(int i, int n) { return i <= (n - 1) / 2 ? i : i - n; }
 
-__device__ __forceinline__ Mode3D mode3d(const NL3DArgs& a, long long k) {
-  const int iz = (int)(k % a.NZ);
-  const long long r = k / a.NZ;
-  const int iy = (int)(r % a.ny);
-  const int ix = (int)(r / a.ny);
+__device__ __forceinline__ Mode3D mode3d_at(const NL3DArgs& a, int ix, int iy, int iz) {
   const int kx = signed_k(ix, a.nx), ky = signed_k(iy, a.ny);
   Mode3D m;
   m.kx = a.kscale * (float)kx;
@@ -154,6 +165,12 @@ __device__ __forceinline__ Mode3D mode3d(const NL3DArgs& a, long long k) {
   return m;
 }
 
+__device__ __forceinline__ Mode3D mode3d(const NL3DArgs& a, long long k) {
+  const int iz = (int)(k % a.NZ);
+  const long long r = k / a.NZ;
+  return mode3d_at(a, (int)(r / a.ny), (int)(r % a.ny), iz);
+}
+
 // kappa of spatial derivative d (axis 3 - D + d)
 __device__ __forceinline__ float kap_d(const NL3DArgs& a, const Mode3D& m, int d) {
   const int axis = 3 - a.dims + d;
@@ -162,44 +179,85 @@ __device__ __forceinline__ float kap_d(const NL3DArgs& a, const Mode3D& m, int d
 
 __device__ __forceinline__ cf imul(float kap, cf w) { return cf{-kap * w.im, kap * w.re}; }
 
+// Field f of instance inst at entry k (mode m) of the spectra S.
+__device__ __forceinline__ cf field_at(const NL3DArgs& a, const cf* __restrict__ S, long long inst, int f,
+                                       long long k, const Mode3D& m) {
+  if (!m.keep) return cf{0.f, 0.f};
+  if (a.kind == NL_PROJ3D && f == 1) {
+    // the curl component omega_c = (i kappa x u_hat)_c of the dealiased
+    // spectra (_projected_convection.py:8-16, 120-123); field 0 is u_c
+    const long long b = (inst - inst % a.channels) * a.Nr + k;
+    const int c = (int)(inst % a.channels);
+    const cf w0 = S[b], w1 = S[b + a.Nr], w2 = S[b + 2 * a.Nr];
+    return c == 0 ? imul(m.ky, w2) - imul(m.kz, w1) : c == 1 ? imul(m.kz, w0) - imul(m.kx, w2)
+                                                           : imul(m.kx, w1) - imul(m.ky, w0);
+  }
+  const cf w = S[inst * a.Nr + k];
+  if (a.kind == NL_GRADNORM) return imul(kap_d(a, m, f), w);
+  if (a.kind == NL_CONV_NONCONS || a.kind == NL_GENERAL) return f == 0 ? w : imul(kap_d(a, m, f - 1), w);
+  if (a.kind == NL_VORT) {
+    // 2-D (_vorticity_convection.py:78-99): psi = Laplace^-1 omega (1 at
+    // the mean mode), u = d_y psi, v = -d_x psi, omega_x, omega_y; x is
+    // axis 1, y the last axis
+    const float kx = m.ky, ky = m.kz;
+    const float lap = -(kx * kx + ky * ky);
+    const cf psi = (lap == 0.f ? 1.f : 1.f / lap) * w;
+    return f == 0 ? imul(ky, psi) : f == 1 ? imul(kx, w) : f == 2 ? imul(-kx, psi) : imul(ky, w);
+  }
+  return w;
+}
+
 __global__ void fields3d_kernel(NL3DArgs a, const cf* __restrict__ S, cf* __restrict__ F) {
   const long long total = a.ninst * a.Nr;
   for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (long long)gridDim.x * blockDim.x) {
     const long long inst = idx / a.Nr, k = idx % a.Nr;
     const Mode3D m = mode3d(a, k);
-    const cf w = m.keep ? S[idx] : cf{0.f, 0.f};
     cf* f = F + inst * a.nf * a.Nr + k;
-    if (a.kind == NL_PROJ3D) {
-      // u_c and the curl component omega_c = (i kappa x u_hat)_c
-      // (_projected_convection.py:8-16, 120-123), of the dealiased spectra
-      const long long b = (inst - inst % a.channels) * a.Nr + k;
-      const cf w0 = m.keep ? S[b] : cf{0.f, 0.f}, w1 = m.keep ? S[b + a.Nr] : cf{0.f, 0.f};
-      const cf w2 = m.keep ? S[b + 2 * a.Nr] : cf{0.f, 0.f};
-      const int c = (int)(inst % a.channels);
-      f[0] = w;
-      f[a.Nr] = c == 0 ? imul(m.ky, w2) - imul(m.kz, w1)
-                : c == 1 ? imul(m.kz, w0) - imul(m.kx, w2)
-                         : imul(m.kx, w1) - imul(m.ky, w0);
-    } else if (a.kind == NL_GRADNORM) {
-      for (int d = 0; d < a.dims; ++d) f[d * a.Nr] = imul(kap_d(a, m, d), w);
-    } else if (a.kind == NL_CONV_NONCONS || a.kind == NL_GENERAL) {
-      f[0] = w;
-      for (int d = 0; d < a.dims; ++d) f[(1 + d) * a.Nr] = imul(kap_d(a, m, d), w);
-    } else if (a.kind == NL_VORT) {
-      // 2-D (_vorticity_convection.py:78-99): psi = Laplace^-1 omega (1 at
-      // the mean mode), u = d_y psi, v = -d_x psi, omega_x, omega_y; x is
-      // axis 1, y the last axis
-      const float kx = m.ky, ky = m.kz;
-      const float lap = -(kx * kx + ky * ky);
-      const cf psi = (lap == 0.f ? 1.f : 1.f / lap) * w;
-      f[0] = imul(ky, psi);
-      f[a.Nr] = imul(kx, w);
-      f[2 * a.Nr] = imul(-kx, psi);
-      f[3 * a.Nr] = imul(ky, w);
-    } else {
-      f[0] = w;
-    }
+    for (int q = 0; q < a.nf; ++q) f[q * a.Nr] = field_at(a, S, inst, q, k, m);
+  }
+}
+
+// The fields pass fused with the first inverse transform pass of a 3-D grid
+// (the x lines): line l = (field g = inst * nf + f, iy, iz) reads the state
+// spectra S along x, forms field f of instance inst at every point, and
+// writes its inverse FFT (times scale) to F. Lines adjacent in memory sit in
+// adjacent lane groups, as in lines_kernel. Only the lines with iz < kkeep
+// run (the fields are zero above the dealias cutoff; the later passes read
+// those planes as zero or skip them).
+template <int N>
+__global__ void __launch_bounds__(256) fields_lines_kernel(NL3DArgs a, const cf* __restrict__ S, cf* __restrict__ F,
+                                                           float scale, const cf* __restrict__ tw, int kkeep) {
+  using C = LineCfg<N>;
+  constexpr int PT = C::PT, T = C::T;
+  constexpr int LDSN = LdsSize<N>::value;
+  __shared__ cf lds_x[C::CPB * LDSN];
+  const int c = threadIdx.x / T, j = threadIdx.x % T;
+  const long long inner = (long long)N * a.NZ;  // lines per field
+  const long long nrun = a.ninst * a.nf * N * kkeep;
+  const long long raw = (long long)blockIdx.x * C::CPB + c;
+  const bool active = raw < nrun;
+  const long long run = active ? raw : 0;  // inactive groups mirror line 0 and store nothing
+  const long long line = (run / kkeep) * a.NZ + run % kkeep;
+  const long long g = line / inner, li = line % inner;
+  const long long inst = g / a.nf;
+  const int f = (int)(g % a.nf);
+  const int iy = (int)(li / a.NZ), iz = (int)(li % a.NZ);
+  cf* lds = lds_x + c * LDSN;
+  const Lane<N, PT> ln(j);
+  TwRegs<N, PT> tr;
+  tr.template load_pass<1>(j, tw);
+  cf v[1][PT];
+#pragma unroll
+  for (int s = 0; s < PT; ++s) {
+    const int ix = j + s * T;
+    v[0][s] = field_at(a, S, inst, f, (long long)ix * inner + li, mode3d_at(a, ix, iy, iz));
+  }
+  cfft<N, PT, true, 1>(v, ln, lds, tr);
+  if (active) {
+    cf* o = F + g * a.Nr + li;
+#pragma unroll
+    for (int s = 0; s < PT; ++s) o[(long long)(j + s * T) * inner] = scale * v[0][s];
   }
 }
 
@@ -280,10 +338,9 @@ __global__ void products3d_kernel(NL3DArgs a, const float* __restrict__ R, float
   }
 }
 
-__global__ void combine3d_kernel(NL3DArgs a, const cf* __restrict__ PH, cf* __restrict__ Nh) {
-  const long long total = a.ninst * a.Nr;
-  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-       idx += (long long)gridDim.x * blockDim.x) {
+// N_hat at entry idx (instance idx / Nr) from the product spectra PH.
+__device__ __forceinline__ cf combine_at(const NL3DArgs& a, const cf* __restrict__ PH, long long idx) {
+  {
     const long long inst = idx / a.Nr, k = idx % a.Nr;
     const Mode3D m = mode3d(a, k);
     const cf* ph = PH + inst * a.np * a.Nr + k;
@@ -320,10 +377,18 @@ __global__ void combine3d_kernel(NL3DArgs a, const cf* __restrict__ PH, cf* __re
       }
       default: r = ph[0];
     }
-    Nh[idx] = m.keep ? r : cf{0.f, 0.f};
+    cf out = m.keep ? r : cf{0.f, 0.f};
     if ((a.kind == NL_PROJ3D || a.kind == NL_VORT) && m.inj && inst % a.channels == 0)
-      Nh[idx] = Nh[idx] + cf{a.inj_value, 0.f};
+      out = out + cf{a.inj_value, 0.f};
+    return out;
   }
+}
+
+__global__ void combine3d_kernel(NL3DArgs a, const cf* __restrict__ PH, cf* __restrict__ Nh) {
+  const long long total = a.ninst * a.Nr;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x)
+    Nh[idx] = combine_at(a, PH, idx);
 }
 
 // ------------------------------------------------ adjoint of the nonlinear term
@@ -533,6 +598,23 @@ static unsigned grid_of(long long total) {
   return (unsigned)(g > 65536 ? 65536 : g);
 }
 
+hipError_t launch_fields_lines3d(const NL3DArgs& a, const cf* S, cf* F, float scale, const cf* tw, int kkeep,
+                                 hipStream_t st) {
+  auto go = [&](auto kern, int cpb) {
+    const long long nrun = a.ninst * a.nf * (long long)a.N * kkeep;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((nrun + cpb - 1) / cpb)), dim3(256), 0, st, a, S, F, scale, tw, kkeep);
+    return hipGetLastError();
+  };
+  switch (a.N) {
+    case 16: return go(fields_lines_kernel<16>, LineCfg<16>::CPB);
+    case 32: return go(fields_lines_kernel<32>, LineCfg<32>::CPB);
+    case 64: return go(fields_lines_kernel<64>, LineCfg<64>::CPB);
+    case 128: return go(fields_lines_kernel<128>, LineCfg<128>::CPB);
+    case 256: return go(fields_lines_kernel<256>, LineCfg<256>::CPB);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_fields3d(const NL3DArgs& a, const cf* S, cf* F, hipStream_t st) {
   hipLaunchKernelGGL(fields3d_kernel, dim3(grid_of(a.ninst * a.Nr)), dim3(256), 0, st, a, S, F);
   return hipGetLastError();
@@ -567,10 +649,17 @@ hipError_t launch_combine3d(const NL3DArgs& a, const cf* PH, cf* Nh, hipStream_t
 // coefficient c_i of the plan's tables (orders 1-2: [E, c1, c2] with Im E
 // at e_im for a complex L; order 3: 7 entries, order 4: 6, as coef.hip
 // forms them). X is the new state's spectrum.
-__global__ void stage3d_kernel(Stage3DArgs a) {
-  const long long total = a.ninst * a.nr;
-  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-       idx += (long long)gridDim.x * blockDim.x) {
+// Stages that read the first nonlinear term N0, and those that read the
+// latest one NC.
+__device__ __forceinline__ bool stage_reads_n0(int m) {
+  return m == ST1 || m == ST2A || m == ST2B || m == ST3A || m == ST3B || m == ST4A || m == ST4C;
+}
+__device__ __forceinline__ bool stage_reads_nc(int m) { return !(m == ST1 || m == ST2A || m == ST3A || m == ST4A); }
+
+// The stage combination at entry idx with the nonlinear terms n0 (N0) and
+// nc (NC) given.
+__device__ __forceinline__ void stage_at(const Stage3DArgs& a, long long idx, cf n0, cf nc) {
+  {
     const long long inst = idx / a.nr, k = idx % a.nr;
     const long long traj = inst / a.channels, ch = inst % a.channels;
     const float* tb = a.tables + ((a.lin_batch > 1 ? traj : 0) * a.lin_channels + (a.lin_channels > 1 ? ch : 0)) *
@@ -587,39 +676,71 @@ __global__ void stage3d_kernel(Stage3DArgs a) {
     const cf u0 = a.U0[idx];
     switch (a.mode) {
       case ST1:
-      case ST2A: a.S[idx] = emul(u0) + c(1) * a.N0[idx]; break;
-      case ST2B: a.S[idx] = a.S[idx] + c(2) * (a.NC[idx] - a.N0[idx]); break;
+      case ST2A: a.S[idx] = emul(u0) + c(1) * n0; break;
+      case ST2B: a.S[idx] = a.S[idx] + c(2) * (nc - n0); break;
       case ST3A:
-        a.A[idx] = emul(u0) + c(4) * a.N0[idx];
-        a.S[idx] = cmul(1, u0) + c(2) * a.N0[idx];
+        a.A[idx] = emul(u0) + c(4) * n0;
+        a.S[idx] = cmul(1, u0) + c(2) * n0;
         break;
       case ST3B:
-        a.A[idx] = a.A[idx] + c(5) * a.NC[idx];
-        a.S[idx] = emul(u0) + c(3) * (2.f * a.NC[idx] - a.N0[idx]);
+        a.A[idx] = a.A[idx] + c(5) * nc;
+        a.S[idx] = emul(u0) + c(3) * (2.f * nc - n0);
         break;
-      case ST3C: a.S[idx] = a.A[idx] + c(6) * a.NC[idx]; break;
+      case ST3C: a.S[idx] = a.A[idx] + c(6) * nc; break;
       case ST4A: {
-        a.A[idx] = emul(u0) + c(3) * a.N0[idx];
-        const cf s = cmul(1, u0) + c(2) * a.N0[idx];
+        a.A[idx] = emul(u0) + c(3) * n0;
+        const cf s = cmul(1, u0) + c(2) * n0;
         a.S1[idx] = s;
         a.S[idx] = s;
         break;
       }
       case ST4B:
-        a.A[idx] = a.A[idx] + (2.f * c(4)) * a.NC[idx];
-        a.S[idx] = cmul(1, u0) + c(2) * a.NC[idx];
+        a.A[idx] = a.A[idx] + (2.f * c(4)) * nc;
+        a.S[idx] = cmul(1, u0) + c(2) * nc;
         break;
       case ST4C:
-        a.A[idx] = a.A[idx] + (2.f * c(4)) * a.NC[idx];
-        a.S[idx] = cmul(1, a.S1[idx]) + c(2) * (2.f * a.NC[idx] - a.N0[idx]);
+        a.A[idx] = a.A[idx] + (2.f * c(4)) * nc;
+        a.S[idx] = cmul(1, a.S1[idx]) + c(2) * (2.f * nc - n0);
         break;
-      case ST4D: a.S[idx] = a.A[idx] + c(5) * a.NC[idx]; break;
+      case ST4D: a.S[idx] = a.A[idx] + c(5) * nc; break;
+    }
+  }
+}
+
+__global__ void stage3d_kernel(Stage3DArgs a) {
+  const long long total = a.ninst * a.nr;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const cf n0 = stage_reads_n0(a.mode) ? a.N0[idx] : cf{0.f, 0.f};
+    const cf nc = stage_reads_nc(a.mode) ? a.NC[idx] : cf{0.f, 0.f};
+    stage_at(a, idx, n0, nc);
+  }
+}
+
+// The combine pass fused with the stage that consumes its result: N_hat is
+// formed in registers from the product spectra PH and enters the stage as
+// N0 (first: also stored to N0out for the later stages) or as NC.
+__global__ void stage_nl3d_kernel(NL3DArgs na, Stage3DArgs a, const cf* __restrict__ PH, cf* __restrict__ N0out) {
+  const long long total = a.ninst * a.nr;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const cf r = combine_at(na, PH, idx);
+    if (N0out) {
+      N0out[idx] = r;
+      stage_at(a, idx, r, cf{0.f, 0.f});
+    } else {
+      stage_at(a, idx, stage_reads_n0(a.mode) ? a.N0[idx] : cf{0.f, 0.f}, r);
     }
   }
 }
 
 hipError_t launch_stage3d(const Stage3DArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(stage3d_kernel, dim3(grid_of(a.ninst * a.nr)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_stage_nl3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* PH, cf* N0out, hipStream_t st) {
+  hipLaunchKernelGGL(stage_nl3d_kernel, dim3(grid_of(a.ninst * a.nr)), dim3(256), 0, st, na, a, PH, N0out);
   return hipGetLastError();
 }
 

@@ -13,9 +13,13 @@ namespace exmi {
 
 using StepFn = hipError_t (*)(const Step1DArgs&, hipStream_t);
 
+// Batched real FFTs of count rows of 2M points (forward: to M + 1 complex,
+// inverse: from them). kcut: the inverse reads the entries above kcut as
+// zero and the forward does not store them (the planes above a dealias
+// cutoff; kcut >= M: all).
 template <int M, int PT, bool INV>
 __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
-    fft1d_kernel(const float* __restrict__ in, float* __restrict__ out, int count, RealTables rt) {
+    fft1d_kernel(const float* __restrict__ in, float* __restrict__ out, int count, RealTables rt, int kcut) {
   constexpr int T = M / PT;
   constexpr int WG = T > 64 ? T : 64;
   constexpr int G = WG / T;
@@ -42,14 +46,15 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
     if (active) {
       cf* y = reinterpret_cast<cf*>(out) + (long long)t * (M + 1);
 #pragma unroll
-      for (int s = 0; s < PT; ++s) y[j + s * T] = 0.5f * v[0][s];
-      if (j == 0) y[M] = 0.5f * nyq[0];
+      for (int s = 0; s < PT; ++s)
+        if (j + s * T <= kcut) y[j + s * T] = 0.5f * v[0][s];
+      if (j == 0 && M <= kcut) y[M] = 0.5f * nyq[0];
     }
   } else {
     const cf* x = reinterpret_cast<const cf*>(in) + (long long)t * (M + 1);
 #pragma unroll
-    for (int s = 0; s < PT; ++s) v[0][s] = x[j + s * T];
-    nyq[0] = x[M];
+    for (int s = 0; s < PT; ++s) v[0][s] = j + s * T <= kcut ? x[j + s * T] : cf{0.f, 0.f};
+    nyq[0] = M <= kcut ? x[M] : cf{0.f, 0.f};
     irfft<M, PT, 1>(v, nyq, ln, lds, tr);  // N x
     if (active) {
       constexpr float rn = 1.f / (2.f * M);
@@ -311,16 +316,16 @@ VjpFn EXMI_CAT(get_vjp1d_, EXMI_M)(int kind, int order, int cplx) {
 }
 
 hipError_t EXMI_CAT(launch_fft1d_, EXMI_M)(bool inverse, const float* in, float* out, int count,
-                                           const RealTables& rt, hipStream_t st) {
+                                           const RealTables& rt, hipStream_t st, int kcut) {
   constexpr int M = EXMI_M, PT = EXMI_PT;
   constexpr int T = M / PT;
   constexpr int WG = T > 64 ? T : 64;
   constexpr int G = WG / T;
   const int grid = (count + G - 1) / G;
   if (inverse)
-    hipLaunchKernelGGL((fft1d_kernel<M, PT, true>), dim3(grid), dim3(WG), 0, st, in, out, count, rt);
+    hipLaunchKernelGGL((fft1d_kernel<M, PT, true>), dim3(grid), dim3(WG), 0, st, in, out, count, rt, kcut);
   else
-    hipLaunchKernelGGL((fft1d_kernel<M, PT, false>), dim3(grid), dim3(WG), 0, st, in, out, count, rt);
+    hipLaunchKernelGGL((fft1d_kernel<M, PT, false>), dim3(grid), dim3(WG), 0, st, in, out, count, rt, kcut);
   return hipGetLastError();
 }
 
