@@ -42,6 +42,8 @@ hipError_t launch_etdrk0_spec(cf* X, long long ninst, long long nr, int channels
 hipError_t launch_fields3d(const NL3DArgs& a, const cf* S, cf* F, hipStream_t st);
 hipError_t launch_fields_lines3d(const NL3DArgs& a, const cf* S, cf* F, float scale, const cf* tw, int kkeep,
                                  hipStream_t st);
+hipError_t launch_zcross3d(int N, const cf* F, cf* P, long long ntraj, long long Nr, int kcut, const RealTables& rt,
+                           hipStream_t st);
 hipError_t launch_wave_spec(cf* X, long long batch, int D, int N, double kscale, double c, double dt,
                             hipStream_t st);
 hipError_t launch_poisson_spec(cf* X, long long count, int D, int N, double kscale, int order, hipStream_t st);
@@ -63,7 +65,7 @@ hipError_t launch_adj_linear3d(const AdjLinArgs& a, hipStream_t st);
   RollFn get_roll1d_r_##M(int kind, int order);                                                        \
   VjpFn get_vjp1d_##M(int kind, int order, int cplx);                                                            \
   hipError_t launch_fft1d_##M(bool inverse, const float* in, float* out, int count, const RealTables& rt, \
-                              hipStream_t st, int kcut);
+                              hipStream_t st, int kcut, long long rpg, long long gstride);
 EXMI_DECL_SIZE(8)
 EXMI_DECL_SIZE(24)
 EXMI_DECL_SIZE(25)
@@ -226,10 +228,10 @@ static RollFn get_roll1d(int M, int kind, int order, int nt, int fr, int std_kc)
 }
 
 static hipError_t launch_fft1d(int M, bool inverse, const float* in, float* out, int count, const RealTables& rt,
-                               hipStream_t st, int kcut = 1 << 30) {
+                               hipStream_t st, int kcut = 1 << 30, long long rpg = 0, long long gstride = 0) {
   switch (M) {
 #define EXMI_CASE(MM) \
-  case MM: return launch_fft1d_##MM(inverse, in, out, count, rt, st, kcut);
+  case MM: return launch_fft1d_##MM(inverse, in, out, count, rt, st, kcut, rpg, gstride);
     EXMI_CASE(8)
     EXMI_CASE(16)
     EXMI_CASE(24)
@@ -807,12 +809,13 @@ static hipError_t rfft3(int N, long long I, const float* x, cf* X, const Twiddle
 // src, the x lines read src instead and X is only the intermediate (src is
 // left unchanged).
 static hipError_t irfft3(int N, long long I, cf* X, float* x, const Twiddles& tz, const Twiddles& tl,
-                         hipStream_t st, const cf* src = nullptr) {
+                         hipStream_t st, const cf* src = nullptr, long long rpg = 0, long long gstride = 0) {
   const long long NZ = N / 2 + 1;
   const float r = 1.f / (float)N;
   hipError_t e = launch_lines(N, true, X, I * N * NZ, N * NZ, (long long)N * N * NZ, N * NZ, r, tl.tw, st, src);
   if (e == hipSuccess) e = launch_lines(N, true, X, I * N * NZ, NZ, N * NZ, NZ, r, tl.tw, st);
-  if (e == hipSuccess) e = launch_fft1d(N / 2, true, (const float*)X, x, (int)(I * N * N), tz.rt(), st);
+  if (e == hipSuccess)
+    e = launch_fft1d(N / 2, true, (const float*)X, x, (int)(I * N * N), tz.rt(), st, 1 << 30, rpg, gstride);
   return e;
 }
 
@@ -1050,6 +1053,13 @@ static hipError_t irfft_md_keep(const exmi_plan* p, long long I, const cf* X, cf
   if (p->gen) return irfft_gen(p->gcfg, p->ndim, I, K, x, p->tw_col.tw, st, X);
   return irfft3(p->d.num_points, I, K, x, p->tw, p->tw_col, st, X);
 }
+// The same for a fused 3-D size, each state (C N^3 floats) written to x +
+// b * xstride (a strided batch such as a trajectory buffer).
+static hipError_t irfft3_keep_strided(const exmi_plan* p, const cf* X, cf* K, float* x, long long xstride,
+                                      hipStream_t st) {
+  const int N = p->d.num_points;
+  return irfft3(N, p->ninst, K, x, p->tw, p->tw_col, st, X, (long long)p->d.num_channels * N * N, xstride);
+}
 
 static void fill_nl3d(const exmi_plan* p, NL3DArgs& a) {
   const int N = p->d.num_points;
@@ -1088,7 +1098,11 @@ static void fill_nl3d(const exmi_plan* p, NL3DArgs& a) {
 
 // The product spectra of N(X) of every instance into F3 (the passes of
 // spectral3d.hip up to the combine).
-static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf* X, hipStream_t st) {
+// *ph: where the product spectra are (F3, or for the projected convection
+// of a fused size R3 as spectra).
+static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf* X, hipStream_t st,
+                                       const cf** ph) {
+  *ph = p->F3;
   if (p->gen) {
     hipError_t e = launch_fields3d(a, X, p->F3, st);
     if (e == hipSuccess) e = irfft_md(p, p->ninst * p->nf3, p->F3, p->R3, st);
@@ -1109,6 +1123,17 @@ static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf
   const cf* tl = p->tw_col.tw;
   hipError_t e = launch_fields_lines3d(a, X, p->F3, r, tl, kz + 1, st);
   if (e == hipSuccess) e = launch_lines(N, true, p->F3, I * N * NZ, NZ, N * NZ, NZ, r, tl, st, nullptr, kz + 1);
+  if (p->kind3 == NL_PROJ3D) {
+    // the z rows with the cross products fused (zcross3d_kernel), the
+    // product spectra in R3 (which holds I N^3 >= ninst N^2 (N + 2) floats)
+    cf* P = (cf*)p->R3;
+    if (e == hipSuccess) e = launch_zcross3d(N, p->F3, P, p->d.batch, p->nr, kz, p->tw.rt(), st);
+    if (e == hipSuccess) e = launch_lines(N, false, P, IP * N * NZ, NZ, N * NZ, NZ, 1.f, tl, st, nullptr, kz + 1);
+    if (e == hipSuccess)
+      e = launch_lines(N, false, P, IP * N * NZ, N * NZ, (long long)N * N * NZ, N * NZ, 1.f, tl, st, nullptr, kz + 1);
+    *ph = P;
+    return e;
+  }
   if (e == hipSuccess)
     e = launch_fft1d(N / 2, true, (const float*)p->F3, p->R3, (int)(I * N * N), p->tw.rt(), st, p->kmax);
   if (e == hipSuccess) e = launch_products3d(a, p->R3, p->PR3, st);
@@ -1123,8 +1148,9 @@ static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf
 static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t st) {
   NL3DArgs a;
   fill_nl3d(p, a);
-  hipError_t e = nonlinear3d_products(p, a, X, st);
-  if (e == hipSuccess) e = launch_combine3d(a, p->F3, outN, st);
+  const cf* ph = nullptr;
+  hipError_t e = nonlinear3d_products(p, a, X, st, &ph);
+  if (e == hipSuccess) e = launch_combine3d(a, ph, outN, st);
   return e;
 }
 
@@ -1140,9 +1166,10 @@ static hipError_t stages3d(exmi_plan* p, Stage3DArgs& sa, hipStream_t st) {
   for (int i = 0; i < 4 && e == hipSuccess; ++i) {
     const int m = seq[p->d.order - 1][i];
     if (m < 0) break;
-    e = nonlinear3d_products(p, a, i == 0 ? sa.U0 : sa.S, st);
+    const cf* ph = nullptr;
+    e = nonlinear3d_products(p, a, i == 0 ? sa.U0 : sa.S, st, &ph);
     sa.mode = m;
-    if (e == hipSuccess) e = launch_stage_nl3d(a, sa, p->F3, i == 0 ? p->N03 : nullptr, st);
+    if (e == hipSuccess) e = launch_stage_nl3d(a, sa, ph, i == 0 ? p->N03 : nullptr, st);
   }
   return e;
 }
@@ -1253,6 +1280,10 @@ static int enqueue_md_carry(exmi_plan* p, const float* u0, long long u0_bs, floa
     }
     if (e != hipSuccess || (final_only && t < n - 1)) continue;
     float* dst = final_only ? out : out + (long long)t * out_ts;
+    if (!p->gen) {
+      if (e == hipSuccess) e = irfft3_keep_strided(p, cur, p->K3, dst, out_bs, st);
+      continue;
+    }
     const bool direct = out_bs == sz;
     if (e == hipSuccess) e = irfft_md_keep(p, p->ninst, cur, p->K3, direct ? dst : p->P3, st);
     if (e == hipSuccess && !direct) e = copy_batch(p->P3, sz, dst, out_bs, sz, B, st);

@@ -391,6 +391,83 @@ __global__ void combine3d_kernel(NL3DArgs a, const cf* __restrict__ PH, cf* __re
     Nh[idx] = combine_at(a, PH, idx);
 }
 
+// The z rows of the projected convection (Navier-Stokes velocity, three
+// channels), fused: one group of T lanes owns the row (trajectory, ix, iy)
+// of the six fields (u_c, omega_c) after their inverse x and y lines, runs
+// their six c2r transforms along z (the planes above kcut read as zero),
+// forms the cross products (u x omega)_c = u_c1 omega_c2 - u_c2 omega_c1
+// (c1 = c + 1, c2 = c + 2 mod 3; _projected_convection.py:120-123) at every
+// point in registers, and stores their r2c transforms along z for kz <= kcut
+// to P (instance traj * 3 + c, the layout of the spectra). Replaces the z
+// c2r pass, the products pass and the z r2c pass, and the physical fields'
+// round trip through memory.
+template <int N, int PT>
+__global__ void __launch_bounds__(256) zcross3d_kernel(const cf* __restrict__ F, cf* __restrict__ P, long long nrows,
+                                                       long long Nr, int kcut, RealTables rt) {
+  constexpr int M = N / 2, T = M / PT, G = 256 / T, NZ = M + 1;
+  constexpr int LDSN = LdsSize<M>::value;
+  __shared__ cf lds_all[G * LDSN];
+  const int g = threadIdx.x / T, j = threadIdx.x % T;
+  const long long raw = (long long)blockIdx.x * G + g;
+  const bool active = raw < nrows;
+  const long long row = active ? raw : 0;  // inactive groups mirror row 0 and store nothing
+  const long long traj = row / ((long long)N * N), rr = row % ((long long)N * N);
+  cf* lds = lds_all + g * LDSN;
+  const Lane<M, PT> ln(j);
+  TwRegs<M, PT> tr;
+  tr.load(j, rt);
+  cf v[6][PT];
+  cf nyq[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const cf* x = F + (traj * 6 + q) * Nr + rr * NZ;  // field q = 2 c + f of instance traj * 3 + c
+#pragma unroll
+    for (int s = 0; s < PT; ++s) v[q][s] = j + s * T <= kcut ? x[j + s * T] : cf{0.f, 0.f};
+    nyq[q] = M <= kcut ? x[M] : cf{0.f, 0.f};
+  }
+  irfft<M, PT, 6>(v, nyq, ln, lds, tr);  // N times the physical pairs
+  constexpr float rn2 = 1.f / ((float)N * (float)N);
+  cf pr[3][PT];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+#pragma unroll
+    for (int s = 0; s < PT; ++s) {
+      const cf u1 = v[2 * c1][s], w2 = v[2 * c2 + 1][s], u2 = v[2 * c2][s], w1 = v[2 * c1 + 1][s];
+      pr[c][s] = cf{(u1.re * w2.re - u2.re * w1.re) * rn2, (u1.im * w2.im - u2.im * w1.im) * rn2};
+    }
+  }
+  cf pn[3];
+  rfft<M, PT, 3>(pr, pn, ln, lds, tr, kcut);  // 2 X
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      cf* y = P + (traj * 3 + c) * Nr + rr * NZ;
+#pragma unroll
+      for (int s = 0; s < PT; ++s)
+        if (j + s * T <= kcut) y[j + s * T] = 0.5f * pr[c][s];
+      if (j == 0 && M <= kcut) y[M] = 0.5f * pn[c];
+    }
+  }
+}
+
+hipError_t launch_zcross3d(int N, const cf* F, cf* P, long long ntraj, long long Nr, int kcut, const RealTables& rt,
+                           hipStream_t st) {
+  auto go = [&](auto kern, int G) {
+    const long long nrows = ntraj * N * N;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((nrows + G - 1) / G)), dim3(256), 0, st, F, P, nrows, Nr, kcut, rt);
+    return hipGetLastError();
+  };
+  switch (N) {
+    case 16: return go(zcross3d_kernel<16, 8>, 256 / 1);
+    case 32: return go(zcross3d_kernel<32, 8>, 256 / 2);
+    case 64: return go(zcross3d_kernel<64, 8>, 256 / 4);
+    case 128: return go(zcross3d_kernel<128, 8>, 256 / 8);
+    case 256: return go(zcross3d_kernel<256, 8>, 256 / 16);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // ------------------------------------------------ adjoint of the nonlinear term
 // J(w)^T mu on whole spectra in three passes around the transforms, the
 // transposes of the forward passes above under the inner products sum(f g)
@@ -739,8 +816,45 @@ hipError_t launch_stage3d(const Stage3DArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
+// The same for the projected convection (three channels): one thread per
+// (trajectory, mode) projects the three product spectra once and runs the
+// three channels' stages.
+__global__ void stage_nl3d_proj_kernel(NL3DArgs na, Stage3DArgs a, const cf* __restrict__ PH, cf* __restrict__ N0out) {
+  const long long total = a.ninst / 3 * a.nr;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const long long traj = idx / a.nr, k = idx % a.nr;
+    const Mode3D m = mode3d(na, k);
+    const long long b = traj * 3 * a.nr + k;
+    const cf p0 = PH[b], p1 = PH[b + a.nr], p2 = PH[b + 2 * a.nr];
+    // Leray projection (_leray.py:86-103): c + i kappa_c p with
+    // p = -div / Laplace = (i kappa . c) / |kappa|^2 (p = 0 at the mean mode)
+    const cf div = imul(m.kx, p0) + imul(m.ky, p1) + imul(m.kz, p2);
+    const float k2 = m.kx * m.kx + m.ky * m.ky + m.kz * m.kz;
+    const cf pr = k2 != 0.f ? (1.f / k2) * div : cf{0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const cf pc = c == 0 ? p0 : c == 1 ? p1 : p2;
+      cf r = m.keep ? pc + imul(c == 0 ? m.kx : c == 1 ? m.ky : m.kz, pr) : cf{0.f, 0.f};
+      if (c == 0 && m.inj) r = r + cf{na.inj_value, 0.f};
+      const long long i = b + c * a.nr;
+      if (N0out) {
+        N0out[i] = r;
+        stage_at(a, i, r, cf{0.f, 0.f});
+      } else {
+        stage_at(a, i, stage_reads_n0(a.mode) ? a.N0[i] : cf{0.f, 0.f}, r);
+      }
+    }
+  }
+}
+
 hipError_t launch_stage_nl3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* PH, cf* N0out, hipStream_t st) {
-  hipLaunchKernelGGL(stage_nl3d_kernel, dim3(grid_of(a.ninst * a.nr)), dim3(256), 0, st, na, a, PH, N0out);
+  if (na.kind == NL_PROJ3D && a.channels == 3) {
+    hipLaunchKernelGGL(stage_nl3d_proj_kernel, dim3(grid_of(a.ninst / 3 * a.nr)), dim3(256), 0, st, na, a, PH,
+                       N0out);
+  } else {
+    hipLaunchKernelGGL(stage_nl3d_kernel, dim3(grid_of(a.ninst * a.nr)), dim3(256), 0, st, na, a, PH, N0out);
+  }
   return hipGetLastError();
 }
 

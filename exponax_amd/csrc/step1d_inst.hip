@@ -16,10 +16,13 @@ using StepFn = hipError_t (*)(const Step1DArgs&, hipStream_t);
 // Batched real FFTs of count rows of 2M points (forward: to M + 1 complex,
 // inverse: from them). kcut: the inverse reads the entries above kcut as
 // zero and the forward does not store them (the planes above a dealias
-// cutoff; kcut >= M: all).
+// cutoff; kcut >= M: all). rpg > 0: the inverse's output rows come in
+// groups of rpg rows, group q starting at out + q * gstride floats (a
+// strided batch of states); rpg = 0: contiguous rows.
 template <int M, int PT, bool INV>
 __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
-    fft1d_kernel(const float* __restrict__ in, float* __restrict__ out, int count, RealTables rt, int kcut) {
+    fft1d_kernel(const float* __restrict__ in, float* __restrict__ out, int count, RealTables rt, int kcut,
+                 long long rpg, long long gstride) {
   constexpr int T = M / PT;
   constexpr int WG = T > 64 ? T : 64;
   constexpr int G = WG / T;
@@ -58,7 +61,8 @@ __global__ void __launch_bounds__((M / PT) > 64 ? (M / PT) : 64)
     irfft<M, PT, 1>(v, nyq, ln, lds, tr);  // N x
     if (active) {
       constexpr float rn = 1.f / (2.f * M);
-      float2* y = reinterpret_cast<float2*>(out + (long long)t * 2 * M);
+      const long long ro = rpg > 0 ? (t / rpg) * gstride + (t % rpg) * 2 * M : (long long)t * 2 * M;
+      float2* y = reinterpret_cast<float2*>(out + ro);
 #pragma unroll
       for (int s = 0; s < PT; ++s) y[j + s * T] = make_float2(v[0][s].re * rn, v[0][s].im * rn);
     }
@@ -316,16 +320,19 @@ VjpFn EXMI_CAT(get_vjp1d_, EXMI_M)(int kind, int order, int cplx) {
 }
 
 hipError_t EXMI_CAT(launch_fft1d_, EXMI_M)(bool inverse, const float* in, float* out, int count,
-                                           const RealTables& rt, hipStream_t st, int kcut) {
+                                           const RealTables& rt, hipStream_t st, int kcut, long long rpg,
+                                           long long gstride) {
   constexpr int M = EXMI_M, PT = EXMI_PT;
   constexpr int T = M / PT;
   constexpr int WG = T > 64 ? T : 64;
   constexpr int G = WG / T;
   const int grid = (count + G - 1) / G;
   if (inverse)
-    hipLaunchKernelGGL((fft1d_kernel<M, PT, true>), dim3(grid), dim3(WG), 0, st, in, out, count, rt, kcut);
+    hipLaunchKernelGGL((fft1d_kernel<M, PT, true>), dim3(grid), dim3(WG), 0, st, in, out, count, rt, kcut, rpg,
+                       gstride);
   else
-    hipLaunchKernelGGL((fft1d_kernel<M, PT, false>), dim3(grid), dim3(WG), 0, st, in, out, count, rt, kcut);
+    hipLaunchKernelGGL((fft1d_kernel<M, PT, false>), dim3(grid), dim3(WG), 0, st, in, out, count, rt, kcut, 0LL,
+                       0LL);
   return hipGetLastError();
 }
 
