@@ -300,6 +300,30 @@ __device__ __forceinline__ void row2d_body(const Step2DArgs& a, int bx, int inst
     __syncthreads();
     tile_store<N, COH>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
   }
+  using FD = Fields2D<KIND>;
+  constexpr int NF = FD::NF, NIN = FD::NIN, NP = FD::NP;
+  const long long fsz = (long long)kc * N;
+  const int ch = inst % a.channels;
+  // input f of the products: this instance's field f, or for the
+  // convections the velocity field u_j of channel j of the same trajectory
+  auto src = [&](int f) -> const cf* {
+    int i = inst, fld = f;
+    if constexpr (KIND == NL_CONV_CONS || FD::COUPLED) {
+      i = inst - ch + f;
+      fld = 0;
+    } else if constexpr (KIND == NL_CONV_NONCONS) {
+      if (f < 2) {
+        i = inst - ch + f;
+        fld = 0;
+      } else {
+        fld = f - 1;
+      }
+    }
+    return a.F + ((long long)i * NF + fld) * fsz;
+  };
+  TileRegs<N, COH> pf;
+  // RCB: the first field's reads are issued before RC, whose transform they overlap
+  if constexpr (MODE == ROW_RCB) pf.load(src(0), x0, kc);
   if constexpr (MODE == ROW_RC || MODE == ROW_RCB) {
     tile_load<N, COH>(lds_t, a.T + (long long)inst * fstride, x0, C::NY);
     __syncthreads();
@@ -323,30 +347,8 @@ __device__ __forceinline__ void row2d_body(const Step2DArgs& a, int bx, int inst
     if constexpr (MODE == ROW_RCB) __syncthreads();
   }
   if constexpr (MODE == ROW_RB || MODE == ROW_RCB) {
-    using FD = Fields2D<KIND>;
-    constexpr int NF = FD::NF, NIN = FD::NIN, NP = FD::NP;
-    const long long fsz = (long long)kc * N;
-    const int ch = inst % a.channels;
-    // input f: this instance's field f, or for the convections the velocity
-    // field u_j of channel j of the same trajectory
-    auto src = [&](int f) -> const cf* {
-      int i = inst, fld = f;
-      if constexpr (KIND == NL_CONV_CONS || FD::COUPLED) {
-        i = inst - ch + f;
-        fld = 0;
-      } else if constexpr (KIND == NL_CONV_NONCONS) {
-        if (f < 2) {
-          i = inst - ch + f;
-          fld = 0;
-        } else {
-          fld = f - 1;
-        }
-      }
-      return a.F + ((long long)i * NF + fld) * fsz;
-    };
     cf A[PT], B2[PT], P[NP][PT];
-    TileRegs<N, COH> pf;
-    pf.load(src(0), x0, kc);
+    if constexpr (MODE == ROW_RB) pf.load(src(0), x0, kc);
 #pragma unroll
     for (int f = 0; f < NIN; ++f) {
       if (f > 0) __syncthreads();
