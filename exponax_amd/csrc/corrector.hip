@@ -79,9 +79,31 @@ __device__ inline void load_frags(bf16x8* dst, const bf16x8* __restrict__ src, b
   for (int i = threadIdx.x; i < 3 * KSTEPS * 64; i += blockDim.x) dst[i] = src[(transposed ? 3 * KSTEPS * 64 : 0) + i];
 }
 
+// The forward pass's workgroup: where one layer's output tiles are all
+// formed before any is written (N / 32 tiles, all taken together), a wave
+// keeps one activation buffer and overwrites it in place (its reads of a
+// layer's input precede its writes of the output in program order, and a
+// wave's LDS operations complete in order); the LDS saved holds eight waves
+// per workgroup, two workgroups per CU: four waves per SIMD to overlap one
+// wave's GELU with another's matrix-core work.
+template <int N>
+struct FwdCfg {
+  static constexpr int NB = N / 32 == tiles_together<N>() ? 1 : 2;  // activation buffers per wave
+  static constexpr int NW = NB == 1 ? 8 : WAVES;                    // waves per workgroup
+};
+
+template <int N>
+struct LdsFwd {
+  Edge e;
+  bf16x8 frag[3 * KSTEPS * 64];
+  __attribute__((aligned(16))) __bf16 act[FwdCfg<N>::NW][FwdCfg<N>::NB][N * LDA];
+  float vec[FwdCfg<N>::NW][N];
+};
+
 template <int N, bool TRAIN>
-__global__ void __launch_bounds__(64 * WAVES) forward_kernel(Args a) {
-  __shared__ Lds<N> sh;
+__global__ void __launch_bounds__(64 * FwdCfg<N>::NW) forward_kernel(Args a) {
+  constexpr int NW = FwdCfg<N>::NW, NB = FwdCfg<N>::NB;
+  __shared__ LdsFwd<N> sh;
   load_edge(sh.e, a.params);
   load_frags(sh.frag, a.frag, false);
   __syncthreads();
@@ -89,10 +111,10 @@ __global__ void __launch_bounds__(64 * WAVES) forward_kernel(Args a) {
   const bf16x8* W = sh.frag;
   float* xs = sh.vec[w];
   // every wave runs the same bounded loop over its samples
-  for (long long smp = (long long)blockIdx.x * WAVES + w; smp < a.count; smp += (long long)gridDim.x * WAVES) {
+  for (long long smp = (long long)blockIdx.x * NW + w; smp < a.count; smp += (long long)gridDim.x * NW) {
     for (int p = lane; p < N; p += 64) xs[p] = a.x[smp * a.xs + p];
     wave_sync();
-    forward_sample<N, TRAIN>(sh.e, W, a.frag + FRAG_L1, a.frag + FRAG_L5, xs, sh.act[w][0], sh.act[w][1],
+    forward_sample<N, TRAIN>(sh.e, W, a.frag + FRAG_L1, a.frag + FRAG_L5, xs, sh.act[w][0], sh.act[w][NB - 1],
                              a.y + smp * a.ys, a.ws, smp, a.count, lane);
   }
 }
@@ -314,18 +336,19 @@ __global__ void reduce_kernel(const float* __restrict__ stage, long long nchunks
   out[j] = s;
 }
 
-int grid_for(long long count) {
-  long long g = (count + WAVES * 4 - 1) / (WAVES * 4);  // about 4 samples per wave
+int grid_for(long long count, int waves = WAVES) {
+  long long g = (count + waves * 4 - 1) / (waves * 4);  // about 4 samples per wave
   if (g > 2048) g = 2048;
   return g < 1 ? 1 : (int)g;
 }
 
 template <int N>
 hipError_t launch_forward(const Args& a, bool train, hipStream_t st) {
+  constexpr int NW = FwdCfg<N>::NW;
   if (train)
-    hipLaunchKernelGGL((forward_kernel<N, true>), dim3(grid_for(a.count)), dim3(64 * WAVES), 0, st, a);
+    hipLaunchKernelGGL((forward_kernel<N, true>), dim3(grid_for(a.count, NW)), dim3(64 * NW), 0, st, a);
   else
-    hipLaunchKernelGGL((forward_kernel<N, false>), dim3(grid_for(a.count)), dim3(64 * WAVES), 0, st, a);
+    hipLaunchKernelGGL((forward_kernel<N, false>), dim3(grid_for(a.count, NW)), dim3(64 * NW), 0, st, a);
   return hipGetLastError();
 }
 

@@ -198,7 +198,8 @@ __device__ __forceinline__ void act_epilogue(const f32x16 (&acc)[NTL], const flo
 }
 
 // Forward of one sample by one wavefront. xs: the sample's N inputs in LDS
-// (float); act0/act1: this wave's two activation buffers; W: the hidden
+// (float); act0/act1: this wave's two activation buffers (one and the same
+// when every output tile of a layer is formed before any is written); W: the hidden
 // layers' A fragments (LDS); F1, F5: the edge layers' A fragments (global,
 // cached); the result y = x + f(x) is written to y (global). With `ws` the
 // training quantities of sample `smp` are stored.
