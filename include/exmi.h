@@ -445,7 +445,9 @@ int exmi_plan_get_tables(const exmi_plan* plan, float* host_out, int64_t host_co
  * the spectrum-carrying launch chain, 3 = the same chain as one cooperative
  * launch (single-channel gradient norm and vorticity, N = 128 and 256, when
  * the whole grid fits the device; inside exmi_rollout_group with several
- * plans it runs as 2). 0 = not chosen
+ * plans it runs as 2). 3-D and runtime-size (multi-pass) plans: 2 = the
+ * spectra carried from step to step, each emitted state inverse-transformed
+ * (real L; for an even N the Nyquist modes dealiased). 0 = not chosen
  * yet (the first exmi_rollout / exmi_rollout_prepare times the candidates on
  * a probe of min(n_steps, 16) steps and keeps the fastest). Setting a
  * strategy drops a cached graph. With the environment variable
