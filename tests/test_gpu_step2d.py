@@ -218,3 +218,26 @@ def test_resident_rollout_2d(device, kind, N):
         plan.set_rollout_strategy(strat)
         last = gpu.repeat_batch(x, n).cpu().numpy()
         np.testing.assert_array_equal(last, tr[:, -1])
+
+
+def test_cooperative_rollout_capacity_and_group(device):
+    """The cooperative 2-D launch needs every workgroup of its grid resident:
+    a batch beyond what the device holds at once (9 workgroups per 128^2
+    instance) does not offer it. A plan tuned to it runs the launch chain
+    inside a group of several plans, with the same states."""
+    import torch
+
+    print("[case] cooperative rollout capacity and group", flush=True)
+    st = ex.stepper.KuramotoSivashinsky(2, 60.0, 128, 0.05)
+    small, big = st.plan(4), st.plan(512)
+    small.set_rollout_strategy(3)
+    with pytest.raises(NotImplementedError):
+        big.set_rollout_strategy(3)
+    u0 = torch.tensor(ics(4, 2, 60.0, 128, 5, seed=61)[0], device=device)
+    alone = st.rollout_batch(u0, 12).cpu().numpy()
+    other = ex.stepper.KuramotoSivashinsky(2, 60.0, 64, 0.05)
+    v0 = torch.tensor(ics(2, 2, 60.0, 64, 5, seed=62)[0], device=device)
+    for mode in (1, 2, 3):
+        a, _ = ex.rollout_group([st, other], [u0, v0], 12, use_graph=mode)
+        np.testing.assert_array_equal(a.cpu().numpy(), alone)
+    assert small.rollout_strategy() == 3
