@@ -111,6 +111,22 @@ static LaunchRoll2D get_roll2d(int N, int kind) {
   }
 }
 
+using LaunchCoopAdj2D = hipError_t (*)(const CoopAdj2DArgs&, hipStream_t);
+LaunchCoopAdj2D get_coop_adj2d_32(Coop2DCapacity* cap);
+LaunchCoopAdj2D get_coop_adj2d_64(Coop2DCapacity* cap);
+LaunchCoopAdj2D get_coop_adj2d_128(Coop2DCapacity* cap);
+LaunchCoopAdj2D get_coop_adj2d_256(Coop2DCapacity* cap);
+LaunchCoopAdj2D get_coop_adj2d_512(Coop2DCapacity* cap);
+
+static LaunchCoopAdj2D get_coop_adj2d(int N, Coop2DCapacity* cap) {
+  *cap = nullptr;
+  switch (N) {
+    case 128: return get_coop_adj2d_128(cap);
+    case 256: return get_coop_adj2d_256(cap);
+    default: return nullptr;
+  }
+}
+
 static LaunchCoop2D get_coop2d(int N, int kind, Coop2DCapacity* cap) {
   *cap = nullptr;
   switch (N) {
@@ -421,6 +437,7 @@ struct exmi_plan {
   LaunchRoll2D roll2d = nullptr;  // 2-D workgroup-resident rollout (strategy 1)
   int fr2d = 0;                   // 2-D spectrum-resident launch chain available (strategy 2)
   LaunchCoop2D coop2d = nullptr;  // the chain as one cooperative launch (strategy 3)
+  LaunchCoopAdj2D coop_adj = nullptr;  // transposed rollout segments as one cooperative launch
   unsigned* coop_sync = nullptr;  // its arrival counters and failure word
   Launch2D col2d[COL_NUM_MODES] = {};
   Twiddles tw_col;
@@ -709,6 +726,12 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
       p->rowA[m] = get_adj2d(N, 1, m);
       p->colA[m] = get_adj2d(N, 0, m);
     }
+    // with the cooperative forward, the reverse pass of a rollout gradient
+    // runs each checkpoint segment as one cooperative launch too
+    Coop2DCapacity cap = nullptr;
+    LaunchCoopAdj2D f = p->coop2d ? get_coop_adj2d(N, &cap) : nullptr;
+    if (f && cap && p->ninst <= cap()) p->coop_adj = f;
+    (void)hipGetLastError();
   }
   int rc = get_twiddles(N / 2, &p->tw);
   if (!rc) rc = get_twiddles(N, &p->tw_col);
@@ -2397,6 +2420,22 @@ static int enq_rollout_cb(exmi_plan* p, const void* a, const void* b, long long 
   return enqueue_rollout(p, (const float*)a, s, (float*)b, n, flag, st);
 }
 
+// The strategy a plan runs with before the autotuner has chosen one (repeat
+// and the rollout gradient, which do not probe): the multi-step kernels
+// where the plan has them, by the measurements behind the autotuner's
+// usual picks (2-D: the workgroup-resident kernel up to 64^2, then the
+// cooperative launch, then the launch chain).
+static int effective_nt(const exmi_plan* p) {
+  if (p->roll_nt != 0) return p->roll_nt;
+  if (p->md) return p->md_carry ? 2 : -1;
+  if (p->ndim == 2) {
+    if (p->roll2d && p->d.num_points <= 64) return 1;
+    if (p->coop2d) return 3;
+    return p->fr2d ? 2 : -1;
+  }
+  return p->roll[2] ? 2 : (p->roll[1] ? 1 : -1);
+}
+
 static int enq_repeat_cb(exmi_plan* p, const void* a, const void* b, long long s, int n, int, hipStream_t st) {
   const float* u0 = (const float*)a;
   float* out = (float*)b;
@@ -2406,7 +2445,7 @@ static int enq_repeat_cb(exmi_plan* p, const void* a, const void* b, long long s
     return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "copy failed: %s", hipGetErrorString(e));
   }
   if (p->md) {
-    if (p->roll_nt == 2 && p->md_carry) return enqueue_md_carry(p, u0, s, out, sz, 0, n, 1, st);
+    if (effective_nt(p) == 2 && p->md_carry) return enqueue_md_carry(p, u0, s, out, sz, 0, n, 1, st);
     for (int t = 0; t < n; ++t) {
       int rc = enqueue_step3d(p, t == 0 ? u0 : out, t == 0 ? s : sz, out, sz, st);
       if (rc) return rc;
@@ -2414,7 +2453,7 @@ static int enq_repeat_cb(exmi_plan* p, const void* a, const void* b, long long s
     return EXMI_OK;
   }
   if (p->ndim == 2) {
-    const int nt2 = p->roll_nt != 0 ? p->roll_nt : (p->fr2d ? 2 : -1);
+    const int nt2 = effective_nt(p);
     if (nt2 == 1 && p->roll2d) return enqueue_roll2d(p, u0, s, out, sz, 0, n, 1, st);
     if (nt2 == 2 && p->fr2d) return enqueue_chain_fr2d(p, u0, s, out, sz, 0, n, 1, st);
     if (nt2 == 3 && p->coop2d) return enqueue_coop2d(p, u0, s, out, sz, 0, n, 1, st);
@@ -2424,7 +2463,7 @@ static int enq_repeat_cb(exmi_plan* p, const void* a, const void* b, long long s
     }
     return EXMI_OK;
   }
-  const int nt = p->roll_nt != 0 ? p->roll_nt : (p->roll[2] ? 2 : 1);
+  const int nt = effective_nt(p);
   if (nt > 0 && p->roll[nt]) return enqueue_roll(p, nt, u0, s, out, sz, 0, n, 1, st);
   for (int t = 0; t < n; ++t) {
     const float* in = t == 0 ? u0 : (((n - t) % 2 == 0) ? out : p->work);
@@ -3219,9 +3258,33 @@ int exmi_rollout_vjp(exmi_plan* p, const float* u0, int64_t u0_bs, int32_t n, in
     for (int sidx = nseg - 1; sidx >= 0; --sidx) {
       const int t0 = sidx * K;
       const int len = (n - t0) < K ? (n - t0) : K;
-      rc = enqueue_rollout_nt(p, p->roll_nt, cp + (long long)sidx * bsz, sz, seg, len - 1, 1, st);
+      rc = enqueue_rollout_nt(p, effective_nt(p), cp + (long long)sidx * bsz, sz, seg, len - 1, 1, st);
       if (rc) return rc;
       const long long segs = (long long)len * sz;  // segment batch stride
+      if (p->coop_adj && effective_nt(p) == 3 && !g_traj) {
+        // the segment's transposed steps as one cooperative launch (the
+        // plan's forward runs cooperatively too)
+        CoopAdj2DArgs ca;
+        memset(&ca, 0, sizeof ca);
+        fill_args2d(p, &ca.aa.s);
+        ca.aa.G = p->G;
+        ca.aa.LAM = p->LAM;
+        ca.aa.RHO = p->RHO;
+        ca.aa.H = p->H;
+        ca.aa.P = p->P;
+        ca.seg = seg;
+        ca.seg_bstride = segs;
+        ca.seg_tstride = sz;
+        ca.len = len;
+        ca.acc = acc;
+        ca.acc_bstride = sz;
+        ca.ninst = p->ninst;
+        ca.sync = p->coop_sync;
+        ca.spin_limit = 1 << 20;
+        e = p->coop_adj(ca, st);
+        if (e != hipSuccess) return fail(EXMI_ERR_HIP, "cooperative adjoint launch failed: %s", hipGetErrorString(e));
+        continue;
+      }
       for (int i = len - 1; i >= 0; --i) {
         const int t = t0 + i;
         if (g_traj) {

@@ -44,17 +44,16 @@ enum Col2DAdjMode { COLA_CL = 0, COLA_CD = 1, COLA_CE = 2 };
 
 // RBG: the forward RB (gradient norm) that also keeps the physical gradient
 // fields; RG: the fields only; RP: the products g_d h transformed along y.
-template <int N, int MODE>
-__global__ void __launch_bounds__(RowCfg<N>::WG) row2d_adj_kernel(Adj2DArgs aa) {
+// One row workgroup's pass (rows x0 = bx R .. of instance inst); COH as in
+// step2d.hpp (the buffers other workgroups read or wrote: F, T, H, P).
+template <int N, int MODE, bool COH>
+__device__ __forceinline__ void row2d_adj_body(const Adj2DArgs& aa, int bx, int inst, cf* lds_x, cf* lds_t) {
   using C = RowCfg<N>;
   constexpr int M = C::M, PT = C::PT, T = C::T, R = C::R;
   constexpr int LDSN = LdsSize<M>::value;
-  __shared__ cf lds_x[R * LDSN];
-  __shared__ cf lds_t[C::NY * C::LD];
   const Step2DArgs& a = aa.s;
 
-  const int inst = blockIdx.y;
-  const int x0 = blockIdx.x * R;
+  const int x0 = bx * R;
   const int r = threadIdx.x / T, j = threadIdx.x % T;
   const int x = x0 + r;
   cf* lds = lds_x + r * LDSN;
@@ -74,7 +73,7 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_adj_kernel(Adj2DArgs aa) 
 #pragma unroll
     for (int f = 0; f < 2; ++f) {
       if (f > 0) __syncthreads();
-      tile_load<N>(lds_t, F + (long long)f * kc * N, x0, kc);
+      tile_load<N, COH>(lds_t, F + (long long)f * kc * N, x0, kc);
       __syncthreads();
       tile_to_row<N>(lds_t, r, j, z, nyq);
       irfft<M, PT, 1>(z, nyq, ln, lds, tr);
@@ -96,12 +95,12 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_adj_kernel(Adj2DArgs aa) 
       __syncthreads();
       row_to_tile<N>(lds_t, r, j, z, nyq);
       __syncthreads();
-      tile_store<N>(lds_t, a.T + (long long)inst * fstride, x0, kc);
+      tile_store<N, COH>(lds_t, a.T + (long long)inst * fstride, x0, kc);
     }
     (void)P;
   } else {
     static_assert(MODE == ROWA_RP, "unknown adjoint row mode");
-    tile_load<N>(lds_t, aa.H + (long long)inst * kc * N, x0, kc);
+    tile_load<N, COH>(lds_t, aa.H + (long long)inst * kc * N, x0, kc);
     __syncthreads();
     tile_to_row<N>(lds_t, r, j, z, nyq);
     irfft<M, PT, 1>(z, nyq, ln, lds, tr);
@@ -119,9 +118,16 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row2d_adj_kernel(Adj2DArgs aa) 
       __syncthreads();
       row_to_tile<N>(lds_t, r, j, z, nyq);
       __syncthreads();
-      tile_store<N>(lds_t, aa.P + ((long long)inst * 2 + d) * kc * N, x0, kc);
+      tile_store<N, COH>(lds_t, aa.P + ((long long)inst * 2 + d) * kc * N, x0, kc);
     }
   }
+}
+
+template <int N, int MODE>
+__global__ void __launch_bounds__(RowCfg<N>::WG) row2d_adj_kernel(Adj2DArgs aa) {
+  __shared__ cf lds_x[RowLds<N>::X];
+  __shared__ cf lds_t[RowLds<N>::TL];
+  row2d_adj_body<N, MODE, false>(aa, blockIdx.x, blockIdx.y, lds_x, lds_t);
 }
 
 template <int N, int MODE>
@@ -134,19 +140,17 @@ inline hipError_t launch_row2d_adj(const Adj2DArgs& a, int ninst, hipStream_t st
 // CL: lam = column transform of RA(g); H <- columns of P Z (c2 lam).
 // CD: rho = lam + J(S1)^T (c2 lam); H <- columns of P Z (c1 rho - c2 lam).
 // CE: lam_in = E rho + J(u_hat)^T (...); inverse transform along x -> T.
-template <int N, int MODE>
-__global__ void __launch_bounds__(ColCfg<N>::WG) col2d_adj_kernel(Adj2DArgs aa) {
+template <int N, int MODE, bool COH>
+__device__ __forceinline__ void col2d_adj_body(const Adj2DArgs& aa, int bx, int inst, cf* lds_x) {
   using C = ColCfg<N>;
   constexpr int PT = C::PT, T = C::T;
   constexpr int LDSN = LdsSize<N>::value;
   constexpr float NN = (float)N;
   constexpr float q2 = 2.f / (8.f * NN * NN * NN * NN);
-  __shared__ cf lds_x[C::CPB * LDSN];
   const Step2DArgs& a = aa.s;
 
-  const int inst = blockIdx.y;
   const int c = threadIdx.x / T, j = threadIdx.x % T;
-  const int ky_raw = blockIdx.x * C::CPB + c;
+  const int ky_raw = bx * C::CPB + c;
   const bool active = ky_raw < C::NY;
   const int ky = active ? ky_raw : 0;  // inactive groups mirror column 0, store nothing
   cf* lds = lds_x + c * LDSN;
@@ -182,7 +186,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_adj_kernel(Adj2DArgs aa) 
     cfft<N, PT, true, 1>(v, ln, lds, tr);
     cf* h = aa.H + ((long long)inst * kc + ky) * N;
 #pragma unroll
-    for (int s = 0; s < PT; ++s) h[j + s * T] = v[0][s];
+    for (int s = 0; s < PT; ++s) gst<COH>(h + j + s * T, v[0][s]);
   };
 
   // 2 J^T mu of this column from the product spectra (zero outside the kept
@@ -194,8 +198,8 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_adj_kernel(Adj2DArgs aa) 
       const cf* p1 = p0 + (long long)kc * N;
 #pragma unroll
       for (int s = 0; s < PT; ++s) {
-        v[0][s] = p0[j + s * T];
-        v[1][s] = p1[j + s * T];
+        v[0][s] = gld<COH>(p0 + j + s * T);
+        v[1][s] = gld<COH>(p1 + j + s * T);
       }
       cfft<N, PT, false, 2>(v, ln, lds, tr);
       const float cs = -a.scale * q2;
@@ -215,7 +219,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_adj_kernel(Adj2DArgs aa) 
     cf v[1][PT];
     const cf* t = a.T + at;
 #pragma unroll
-    for (int s = 0; s < PT; ++s) v[0][s] = t[j + s * T];
+    for (int s = 0; s < PT; ++s) v[0][s] = gld<COH>(t + j + s * T);
     cfft<N, PT, false, 1>(v, ln, lds, tr);
     if (active) {
       cf* lam = aa.LAM + at;
@@ -253,9 +257,15 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col2d_adj_kernel(Adj2DArgs aa) 
     if (active) {
       cf* t = a.T + at;
 #pragma unroll
-      for (int s = 0; s < PT; ++s) t[j + s * T] = v[0][s];
+      for (int s = 0; s < PT; ++s) gst<COH>(t + j + s * T, v[0][s]);
     }
   }
+}
+
+template <int N, int MODE>
+__global__ void __launch_bounds__(ColCfg<N>::WG) col2d_adj_kernel(Adj2DArgs aa) {
+  __shared__ cf lds_x[ColLds<N>::X];
+  col2d_adj_body<N, MODE, false>(aa, blockIdx.x, blockIdx.y, lds_x);
 }
 
 template <int N, int MODE>

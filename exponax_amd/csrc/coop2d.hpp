@@ -33,6 +33,7 @@
 // (Kolmogorov), N = 128 and 256, where the row and column passes both run
 // 256-thread workgroups.
 #pragma once
+#include "adjoint2d.hpp"
 #include "step2d.hpp"
 
 namespace exmi {
@@ -154,6 +155,108 @@ __global__ void __launch_bounds__(CoopCfg<N>::WG) coop2d_kernel(Coop2DArgs ca) {
   }
 }
 
+// The transposed steps of one rollout segment (gradient norm, ETDRK2) as
+// one cooperative launch: for i = len - 1 .. 0 the VJP of the step at the
+// segment's state i applied to the cotangent acc (in place), the launch
+// sequence of the adjoint step (adjoint2d.hpp: RA(u) CA RBG CB RG, then
+// RA(g) CL RP CD RP CE RC) with barriers between passes; RG shares a pass
+// with RA(g), and a step's RC with the next step's RA(u) (each row
+// workgroup reads its tile of T before it writes T). The gradient fields G,
+// the cotangent acc and the column states (S0, S1, N0, LAM, RHO) stay with
+// the workgroup that wrote them; T, F, H and P are accessed coherently.
+struct CoopAdj2DArgs {
+  Adj2DArgs aa;
+  const float* seg;        // the segment's states: state i of instance b at seg + b * seg_bstride + i * seg_tstride
+  long long seg_bstride, seg_tstride;
+  int len;
+  float* acc;              // cotangent (ninst, N, N), in place
+  long long acc_bstride;
+  int ninst;
+  unsigned* sync;
+  int spin_limit;
+};
+
+template <int N>
+__global__ void __launch_bounds__(CoopCfg<N>::WG) coop_adj2d_kernel(CoopAdj2DArgs ca) {
+  using CC = CoopCfg<N>;
+  constexpr int K = NL_GRADNORM;
+  __shared__ cf lds[CC::LDS];
+  __shared__ int flag;
+  const int inst = blockIdx.x / CC::CG, g = blockIdx.x % CC::CG;
+  if (inst >= ca.ninst) return;  // the launcher sizes the grid exactly; never taken
+  unsigned* ctr = ca.sync + (long long)inst * 64;
+  unsigned* fail = ca.sync + (long long)ca.ninst * 64;
+  unsigned arrivals = 0;
+  auto meet = [&]() {
+    arrivals += CC::CG;
+    return coop_barrier(ctr, arrivals, fail, ca.spin_limit, &flag);
+  };
+  Adj2DArgs aa = ca.aa;
+  Step2DArgs& a = aa.s;
+  a.write_T = 0;
+  cf* rx = lds;
+  cf* rt = lds + RowLds<N>::X;
+  const bool row = g < CC::RG;
+  // after a barrier gave up: NaN in this workgroup's rows of the cotangent
+  auto give_up = [&]() {
+    if (!row) return;
+    float* o = ca.acc + (long long)inst * ca.acc_bstride + (long long)g * RowCfg<N>::R * N;
+    for (int i = threadIdx.x; i < RowCfg<N>::R * N; i += CC::WG) o[i] = __builtin_nanf("");
+  };
+  auto emit = [&]() {  // RC: the transposed step's result into acc
+    a.u_out = ca.acc;
+    a.u_out_bstride = ca.acc_bstride;
+    row2d_body<N, ROW_RC, K, true>(a, g, inst, rx, rt);
+  };
+  for (int i = ca.len - 1; i >= 0; --i) {
+    if (row) {
+      if (i < ca.len - 1) {
+        emit();
+        __syncthreads();
+      }
+      a.u_in = ca.seg + (long long)i * ca.seg_tstride;
+      a.u_in_bstride = ca.seg_bstride;
+      row2d_body<N, ROW_RA, K, true>(a, g, inst, rx, rt);
+    }
+    if (!meet()) return give_up();
+    col2d_body<N, COL_CA, K, true>(a, g, inst, lds);
+    if (!meet()) return give_up();
+    if (row) {
+      aa.stage = 0;
+      row2d_adj_body<N, ROWA_RBG, true>(aa, g, inst, rx, rt);
+    }
+    if (!meet()) return give_up();
+    col2d_body<N, COL_CB, K, true>(a, g, inst, lds);
+    if (!meet()) return give_up();
+    if (row) {
+      aa.stage = 1;
+      row2d_adj_body<N, ROWA_RG, true>(aa, g, inst, rx, rt);
+      __syncthreads();
+      a.u_in = ca.acc;
+      a.u_in_bstride = ca.acc_bstride;
+      row2d_body<N, ROW_RA, K, true>(a, g, inst, rx, rt);
+    }
+    if (!meet()) return give_up();
+    col2d_adj_body<N, COLA_CL, true>(aa, g, inst, lds);
+    if (!meet()) return give_up();
+    if (row) {
+      aa.stage = 1;
+      row2d_adj_body<N, ROWA_RP, true>(aa, g, inst, rx, rt);
+    }
+    if (!meet()) return give_up();
+    col2d_adj_body<N, COLA_CD, true>(aa, g, inst, lds);
+    if (!meet()) return give_up();
+    if (row) {
+      aa.stage = 0;
+      row2d_adj_body<N, ROWA_RP, true>(aa, g, inst, rx, rt);
+    }
+    if (!meet()) return give_up();
+    col2d_adj_body<N, COLA_CE, true>(aa, g, inst, lds);
+    if (!meet()) return give_up();
+  }
+  if (row) emit();
+}
+
 template <int N>
 __global__ void coop2d_zero_kernel(unsigned* sync, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) sync[i] = 0u;
@@ -178,6 +281,33 @@ inline long long coop2d_max_instances() {
   }
   cached = (long long)per_cu * cus / CoopCfg<N>::CG;
   return cached;
+}
+
+template <int N>
+inline long long coop_adj2d_max_instances() {
+  static long long cached = -1;
+  if (cached >= 0) return cached;
+  int dev = 0, per_cu = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, coop_adj2d_kernel<N>, CoopCfg<N>::WG, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  cached = (long long)per_cu * cus / CoopCfg<N>::CG;
+  return cached;
+}
+
+template <int N>
+inline hipError_t launch_coop_adj2d(const CoopAdj2DArgs& a, hipStream_t st) {
+  if (a.ninst < 1 || a.len < 1) return hipErrorInvalidValue;
+  if (a.ninst > coop_adj2d_max_instances<N>()) return hipErrorCooperativeLaunchTooLarge;
+  const int nsync = a.ninst * 64 + 64;
+  hipLaunchKernelGGL(coop2d_zero_kernel<N>, dim3((nsync + 255) / 256), dim3(256), 0, st, a.sync, nsync);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((coop_adj2d_kernel<N>), dim3(a.ninst * CoopCfg<N>::CG), dim3(CoopCfg<N>::WG), 0, st, a);
+  return hipGetLastError();
 }
 
 template <int N, int KIND>

@@ -120,4 +120,17 @@ LaunchCoop2D EXMI_CAT(get_coop2d_, EXMI_N2)(int kind, Coop2DCapacity* cap) {
   return nullptr;
 }
 
+using LaunchCoopAdj2D = hipError_t (*)(const CoopAdj2DArgs&, hipStream_t);
+
+// cooperative transposed segments (gradient norm; N = 128 and 256)
+LaunchCoopAdj2D EXMI_CAT(get_coop_adj2d_, EXMI_N2)(Coop2DCapacity* cap) {
+  constexpr int N = EXMI_N2;
+  if constexpr (N == 128 || N == 256) {
+    *cap = &coop_adj2d_max_instances<N>;
+    return &launch_coop_adj2d<N>;
+  }
+  *cap = nullptr;
+  return nullptr;
+}
+
 }  // namespace exmi

@@ -263,3 +263,33 @@ def test_ks2d_mse_gradient_128(device):
         an = float(np.sum(g.astype(np.float64) * v))
         print(f"  directional derivative: adjoint {an:.8e}, central FD {fd:.8e}", flush=True)
         assert abs(an - fd) <= RTOL_GRAD * abs(fd)
+
+
+def test_cooperative_adjoint_segments(device):
+    """KS 2-D at N = 128 with the cooperative forward (strategy 3): the
+    reverse pass runs each checkpoint segment's transposed steps as one
+    cooperative launch, with the same passes as the per-step launch chain
+    (strategy 2), so the gradients agree bit for bit; against the float64
+    oracle's reverse pass as the other 2-D gradients."""
+    import torch
+
+    print("[case] cooperative adjoint segments", flush=True)
+    N, L, dt, n = 128, 60.0, 0.05, 12
+    gpu = ex.stepper.KuramotoSivashinsky(2, L, N, dt)
+    cpu = ost.KuramotoSivashinsky(2, L, N, dt)
+    u0, _ = ics(3, 2, L, N, 5, seed=47)
+    gf, _ = ics(3, 2, L, N, 8, seed=48)
+    x = torch.tensor(u0, device="cuda")
+    g = torch.tensor(gf, device="cuda")
+    plan = gpu.plan(3)
+    out = {}
+    for strat in (2, 3):
+        plan.set_rollout_strategy(strat)
+        out[strat] = [gpu.rollout_vjp(x, n, g_final=g, checkpoint_every=k).cpu().numpy() for k in (5, 12)]
+    for a, b in zip(out[2], out[3]):
+        np.testing.assert_array_equal(a, b)
+    assert _rel(out[3][0], out[3][1]) <= 1e-6
+    ref = np.stack([oad.repeat_vjp(cpu, u0[b].astype(np.float64), n, gf[b].astype(np.float64)) for b in range(3)])
+    err = _rel(out[3][0], ref)
+    print(f"  cooperative rollout vjp rel-L2 vs float64 {err:.2e}", flush=True)
+    assert err <= RTOL_GRAD
