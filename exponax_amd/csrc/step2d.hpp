@@ -686,10 +686,18 @@ __device__ __forceinline__ void col2d_body(const Step2DArgs& a, int bx, int inst
     fields_out(v[0]);
   } else if constexpr (MODE == COL_CB) {
     cf Nh[PT], S[PT];
-    nonlinear(Nh);
+    // the state and the coefficients are read before the products, so
+    // their latency overlaps the products' transform
     const cf* s0 = a.S0 + (long long)inst * full + (long long)ky * N;
+    float c1[PT];
 #pragma unroll
-    for (int s = 0; s < PT; ++s) S[s] = emul(s, s0[j + s * T]) + coef(1, s) * Nh[s];
+    for (int s = 0; s < PT; ++s) {
+      S[s] = emul(s, s0[j + s * T]);
+      c1[s] = coef(1, s);
+    }
+    nonlinear(Nh);
+#pragma unroll
+    for (int s = 0; s < PT; ++s) S[s] = S[s] + c1[s] * Nh[s];
     if (active) {
       cf* s1 = a.S1 + (long long)inst * full + (long long)ky * N;
 #pragma unroll
@@ -852,15 +860,20 @@ __device__ __forceinline__ void col2d_body(const Step2DArgs& a, int bx, int inst
     // CC: S' = S1 + c2 (N1 - N0) -> inverse column transform -> T. CCA
     // also keeps S' as the next step's S0 and, with next_fields, forms
     // its fields (the next step's CA)
-    cf Nh[PT], v[1][PT];
-    nonlinear(Nh);
+    cf Nh[PT], v[1][PT], N0s[PT];
+    float c2[PT];
+    // S1, N0 and the coefficients are read before the products (as in CB)
     const cf* s1 = a.S1 + (long long)inst * full + (long long)ky * N;
     const cf* n0 = a.N0 + ((long long)inst * kc + ky) * N;
 #pragma unroll
     for (int s = 0; s < PT; ++s) {
-      const cf N0s = kept_col ? n0[j + s * T] : forcing(s);
-      v[0][s] = s1[j + s * T] + coef(2, s) * (Nh[s] - N0s);
+      N0s[s] = kept_col ? n0[j + s * T] : forcing(s);
+      v[0][s] = s1[j + s * T];
+      c2[s] = coef(2, s);
     }
+    nonlinear(Nh);
+#pragma unroll
+    for (int s = 0; s < PT; ++s) v[0][s] = v[0][s] + c2[s] * (Nh[s] - N0s[s]);
     if constexpr (MODE == COL_CCA) {
       if (active) {
         cf* s0 = a.S0 + (long long)inst * full + (long long)ky * N;
