@@ -426,6 +426,8 @@ struct exmi_plan {
   cf *XA3 = nullptr, *XB3 = nullptr, *XC3 = nullptr, *LAM3 = nullptr, *MU3 = nullptr, *T3 = nullptr;
   cf *LA3 = nullptr, *LB3 = nullptr, *LC3 = nullptr, *FH3 = nullptr;
   float *RH3 = nullptr, *Q3 = nullptr;
+  float* Gseg = nullptr;  // 2-D rollout gradient: one segment's gradient fields from the forward launch
+  long long gseg_elems = 0;
   int adj_nh = 0, adj_nq = 0;
   // fused plans whose adjoint the fused kernels do not cover: the
   // description and host operator tables kept to build a multi-pass twin
@@ -1869,6 +1871,7 @@ int exmi_plan_destroy(exmi_plan* p) {
   hipFree(p->RH3);
   hipFree(p->Q3);
   hipFree(p->coop_sync);
+  hipFree(p->Gseg);
   if (p->twin) exmi_plan_destroy(p->twin);
   delete p;
   return EXMI_OK;
@@ -2103,9 +2106,13 @@ static int enqueue_chain_fr2d(exmi_plan* p, const float* u0, long long u0_bs, fl
 // The chain as one cooperative launch for all n steps (states as in
 // enqueue_chain_fr2d).
 static int enqueue_coop2d(exmi_plan* p, const float* u0, long long u0_bs, float* out, long long out_bs,
-                          long long out_ts, int n, int final_only, hipStream_t st) {
+                          long long out_ts, int n, int final_only, hipStream_t st, float* G = nullptr,
+                          long long G_tstride = 0) {
   Coop2DArgs ca;
+  memset(&ca, 0, sizeof ca);
   fill_args2d(p, &ca.s);
+  ca.G = G;
+  ca.G_tstride = G_tstride;
   ca.u0 = u0;
   ca.u0_bstride = u0_bs;
   ca.traj = out;
@@ -3241,6 +3248,19 @@ int exmi_rollout_vjp(exmi_plan* p, const float* u0, int64_t u0_bs, int32_t n, in
       return fail(EXMI_ERR_ALLOC, "rollout-gradient workspace allocation failed (%zu floats)", need);
     p->ckpt_elems = need;
   }
+  // the cooperative reverse pass takes each segment's gradient fields from
+  // its forward recomputation (ninst x 4 fields of N^2 per step)
+  const bool coop = p->coop_adj && effective_nt(p) == 3 && !g_traj && n > 0;
+  const long long gblk = coop ? (long long)p->ninst * 4 * p->d.num_points * p->d.num_points : 0;
+  if (coop && p->gseg_elems < (long long)K * gblk) {
+    HIP_TRY(hipStreamSynchronize(st));
+    hipFree(p->Gseg);
+    p->Gseg = nullptr;
+    p->gseg_elems = 0;
+    if (hipMalloc(&p->Gseg, sizeof(float) * K * gblk) != hipSuccess)
+      return fail(EXMI_ERR_ALLOC, "rollout-gradient field workspace allocation failed (%lld floats)", K * gblk);
+    p->gseg_elems = K * gblk;
+  }
   float* cp = p->ckpt;
   float* seg = cp + (long long)nseg * bsz;
   float* acc = seg + (long long)K * bsz;
@@ -3258,10 +3278,18 @@ int exmi_rollout_vjp(exmi_plan* p, const float* u0, int64_t u0_bs, int32_t n, in
     for (int sidx = nseg - 1; sidx >= 0; --sidx) {
       const int t0 = sidx * K;
       const int len = (n - t0) < K ? (n - t0) : K;
-      rc = enqueue_rollout_nt(p, effective_nt(p), cp + (long long)sidx * bsz, sz, seg, len - 1, 1, st);
-      if (rc) return rc;
+      if (coop) {
+        // the segment's len steps as one cooperative launch that keeps the
+        // gradient fields of every step (its states are not needed; the last
+        // one lands in seg), then the transposed steps from those fields
+        rc = enqueue_coop2d(p, cp + (long long)sidx * bsz, sz, seg, sz, 0, len, 1, st, p->Gseg, gblk);
+        if (rc) return rc;
+      } else {
+        rc = enqueue_rollout_nt(p, effective_nt(p), cp + (long long)sidx * bsz, sz, seg, len - 1, 1, st);
+        if (rc) return rc;
+      }
       const long long segs = (long long)len * sz;  // segment batch stride
-      if (p->coop_adj && effective_nt(p) == 3 && !g_traj) {
+      if (coop) {
         // the segment's transposed steps as one cooperative launch (the
         // plan's forward runs cooperatively too)
         CoopAdj2DArgs ca;
@@ -3272,6 +3300,8 @@ int exmi_rollout_vjp(exmi_plan* p, const float* u0, int64_t u0_bs, int32_t n, in
         ca.aa.RHO = p->RHO;
         ca.aa.H = p->H;
         ca.aa.P = p->P;
+        ca.Gseg = p->Gseg;
+        ca.G_tstride = gblk;
         ca.seg = seg;
         ca.seg_bstride = segs;
         ca.seg_tstride = sz;

@@ -51,6 +51,8 @@ struct Coop2DArgs {
   unsigned* sync;          // arrival counter of instance i at sync[64 i] (zero at launch);
                            // sync[64 ninst]: failure word (nonzero: a barrier gave up)
   int spin_limit;          // polls of a barrier before it gives up
+  float* G;                // gradient norm: step t's gradient fields of both stages to G + t G_tstride
+  long long G_tstride;     // (the adjoint's layout, see Step2DArgs::Gout), or G null
 };
 
 template <int N>
@@ -131,6 +133,8 @@ __global__ void __launch_bounds__(CoopCfg<N>::WG) coop2d_kernel(Coop2DArgs ca) {
   col2d_body<N, COL_CA, KIND, true>(a, g, inst, lds);
   if (!meet()) return give_up();
   for (int t = 0; t < n; ++t) {
+    a.Gout = ca.G ? ca.G + (long long)t * ca.G_tstride : nullptr;
+    a.gstage = 0;
     if (row) {
       if (t == 0 || ca.final_only) {
         row2d_body<N, ROW_RB, KIND, true>(a, g, inst, rx, rt);
@@ -143,6 +147,7 @@ __global__ void __launch_bounds__(CoopCfg<N>::WG) coop2d_kernel(Coop2DArgs ca) {
     if (!meet()) return give_up();
     col2d_body<N, COL_CB, KIND, true>(a, g, inst, lds);
     if (!meet()) return give_up();
+    a.gstage = 1;
     if (row) row2d_body<N, ROW_RB, KIND, true>(a, g, inst, rx, rt);
     if (!meet()) return give_up();
     a.next_fields = t < n - 1 ? 1 : 0;
@@ -164,8 +169,13 @@ __global__ void __launch_bounds__(CoopCfg<N>::WG) coop2d_kernel(Coop2DArgs ca) {
 // workgroup reads its tile of T before it writes T). The gradient fields G,
 // the cotangent acc and the column states (S0, S1, N0, LAM, RHO) stay with
 // the workgroup that wrote them; T, F, H and P are accessed coherently.
+// With Gseg (the gradient fields of the segment's steps, stored by the
+// forward launch that recomputed the segment) the recompute passes are
+// skipped: a step is RC + RA(g), CL, RP, CD, RP, CE (six passes).
 struct CoopAdj2DArgs {
   Adj2DArgs aa;
+  const float* Gseg;       // the segment's gradient fields from the forward launch (step i at Gseg +
+  long long G_tstride;     // i G_tstride), or null: recompute them from the segment's states
   const float* seg;        // the segment's states: state i of instance b at seg + b * seg_bstride + i * seg_tstride
   long long seg_bstride, seg_tstride;
   int len;
@@ -209,34 +219,50 @@ __global__ void __launch_bounds__(CoopCfg<N>::WG) coop_adj2d_kernel(CoopAdj2DArg
     row2d_body<N, ROW_RC, K, true>(a, g, inst, rx, rt);
   };
   for (int i = ca.len - 1; i >= 0; --i) {
-    if (row) {
-      if (i < ca.len - 1) {
-        emit();
-        __syncthreads();
+    if (ca.Gseg) {
+      // the gradient fields of both stages come from the forward launch:
+      // only the transposed stages run (RC of the previous step, RA(g))
+      aa.G = const_cast<float*>(ca.Gseg) + (long long)i * ca.G_tstride;
+      if (row) {
+        if (i < ca.len - 1) {
+          emit();
+          __syncthreads();
+        }
+        a.u_in = ca.acc;
+        a.u_in_bstride = ca.acc_bstride;
+        row2d_body<N, ROW_RA, K, true>(a, g, inst, rx, rt);
       }
-      a.u_in = ca.seg + (long long)i * ca.seg_tstride;
-      a.u_in_bstride = ca.seg_bstride;
-      row2d_body<N, ROW_RA, K, true>(a, g, inst, rx, rt);
+      if (!meet()) return give_up();
+    } else {
+      if (row) {
+        if (i < ca.len - 1) {
+          emit();
+          __syncthreads();
+        }
+        a.u_in = ca.seg + (long long)i * ca.seg_tstride;
+        a.u_in_bstride = ca.seg_bstride;
+        row2d_body<N, ROW_RA, K, true>(a, g, inst, rx, rt);
+      }
+      if (!meet()) return give_up();
+      col2d_body<N, COL_CA, K, true>(a, g, inst, lds);
+      if (!meet()) return give_up();
+      if (row) {
+        aa.stage = 0;
+        row2d_adj_body<N, ROWA_RBG, true>(aa, g, inst, rx, rt);
+      }
+      if (!meet()) return give_up();
+      col2d_body<N, COL_CB, K, true>(a, g, inst, lds);
+      if (!meet()) return give_up();
+      if (row) {
+        aa.stage = 1;
+        row2d_adj_body<N, ROWA_RG, true>(aa, g, inst, rx, rt);
+        __syncthreads();
+        a.u_in = ca.acc;
+        a.u_in_bstride = ca.acc_bstride;
+        row2d_body<N, ROW_RA, K, true>(a, g, inst, rx, rt);
+      }
+      if (!meet()) return give_up();
     }
-    if (!meet()) return give_up();
-    col2d_body<N, COL_CA, K, true>(a, g, inst, lds);
-    if (!meet()) return give_up();
-    if (row) {
-      aa.stage = 0;
-      row2d_adj_body<N, ROWA_RBG, true>(aa, g, inst, rx, rt);
-    }
-    if (!meet()) return give_up();
-    col2d_body<N, COL_CB, K, true>(a, g, inst, lds);
-    if (!meet()) return give_up();
-    if (row) {
-      aa.stage = 1;
-      row2d_adj_body<N, ROWA_RG, true>(aa, g, inst, rx, rt);
-      __syncthreads();
-      a.u_in = ca.acc;
-      a.u_in_bstride = ca.acc_bstride;
-      row2d_body<N, ROW_RA, K, true>(a, g, inst, rx, rt);
-    }
-    if (!meet()) return give_up();
     col2d_adj_body<N, COLA_CL, true>(aa, g, inst, lds);
     if (!meet()) return give_up();
     if (row) {

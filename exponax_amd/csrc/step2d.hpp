@@ -101,6 +101,9 @@ struct Step2DArgs {
   cf* spec_out;              // exmi_nonlinear: N_hat (ninst, N, N/2+1), [kx][ky]
   long long spec_bstride;    // complex entries between instances
   int next_fields;           // COL_CCA: also form the fields of S' (the next step's first evaluation)
+  float* Gout;               // gradient norm, RB/RCB: also store the physical gradient fields (the
+                             // adjoint's G layout: (ninst, 2 stages, 2 fields, N, N)), or null
+  int gstage;                // the stage those fields belong to (0: the state, 1: the stage state S1)
   int spec_io;               // exmi_step_fourier: COL_C0 reads u_hat from spec, and the last stage
                              // (C0, C1, CC, C3C, C4D) writes u_hat' to spec_out, both [kx][ky]
 };
@@ -357,6 +360,14 @@ __device__ __forceinline__ void row2d_body(const Step2DArgs& a, int bx, int inst
       if (f + 1 < NIN) pf.load(src(f + 1), x0, kc);  // overlaps this field's transform
       tile_to_row<N>(lds_t, r, j, z, nyq);
       irfft<M, PT, 1>(z, nyq, ln, lds, tr);
+      if constexpr (KIND == NL_GRADNORM) {
+        if (a.Gout) {  // the adjoint's gradient fields (as its RBG/RG passes store them)
+          float2* G = reinterpret_cast<float2*>(a.Gout + ((long long)inst * 4 + a.gstage * 2 + f) * N * N +
+                                                (long long)x * N);
+#pragma unroll
+          for (int s = 0; s < PT; ++s) G[j + s * T] = make_float2(z[0][s].re, z[0][s].im);
+        }
+      }
       if constexpr (FD::COUPLED) {
         // the channel's reaction at unit scale (inputs are 2 N^2 u)
 #pragma unroll

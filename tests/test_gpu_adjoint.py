@@ -266,11 +266,12 @@ def test_ks2d_mse_gradient_128(device):
 
 
 def test_cooperative_adjoint_segments(device):
-    """KS 2-D at N = 128 with the cooperative forward (strategy 3): the
-    reverse pass runs each checkpoint segment's transposed steps as one
-    cooperative launch, with the same passes as the per-step launch chain
-    (strategy 2), so the gradients agree bit for bit; against the float64
-    oracle's reverse pass as the other 2-D gradients."""
+    """KS 2-D at N = 128 with the cooperative forward (strategy 3): each
+    checkpoint segment is recomputed by one cooperative launch that keeps
+    every step's gradient fields, and its transposed steps run as one
+    cooperative launch from those fields. Against the per-step launch chain
+    (strategy 2, which recomputes the fields from the physical states: the
+    two differ by float32 rounding) and the float64 oracle's reverse pass."""
     import torch
 
     print("[case] cooperative adjoint segments", flush=True)
@@ -287,8 +288,10 @@ def test_cooperative_adjoint_segments(device):
         plan.set_rollout_strategy(strat)
         out[strat] = [gpu.rollout_vjp(x, n, g_final=g, checkpoint_every=k).cpu().numpy() for k in (5, 12)]
     for a, b in zip(out[2], out[3]):
-        np.testing.assert_array_equal(a, b)
-    assert _rel(out[3][0], out[3][1]) <= 1e-6
+        d = _rel(b, a)
+        print(f"  cooperative vs per-step launches: {d:.2e}", flush=True)
+        assert d <= 1e-5
+    assert _rel(out[3][0], out[3][1]) <= 1e-5
     ref = np.stack([oad.repeat_vjp(cpu, u0[b].astype(np.float64), n, gf[b].astype(np.float64)) for b in range(3)])
     err = _rel(out[3][0], ref)
     print(f"  cooperative rollout vjp rel-L2 vs float64 {err:.2e}", flush=True)
