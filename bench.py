@@ -545,13 +545,21 @@ def wl_ks2d_grad(args, ex, _lib, torch):
     # per state-step: checkpoint pass and recomputation (8 N^2 each) plus the
     # adjoint step (reads u_t and the cotangent, writes the new cotangent)
     per_state = 8 * N * N * 2 + 12 * N * N
-    n_launches = K * 12 + 2 * K * 6
+    coop = st.plan(B).rollout_strategy() in (0, 3)  # untuned plans run the cooperative launches too
+    nseg = (K + CK - 1) // CK
+    # cooperative: checkpoint pass (nseg - 1 launches), per segment one
+    # recompute and one reverse-pass launch; otherwise six launches per
+    # forward step and twelve per transposed step
+    n_launches = (nseg - 1) + 2 * nseg if coop else K * 12 + 2 * K * 6
+    kernel = ("coop2d_kernel<128, gradient norm> (checkpoint pass; segment recompute keeping the gradient "
+              "fields) + coop_adj2d_kernel<128> (a segment's transposed steps, one launch)") if coop else \
+        "row2d/col2d (+ adjoint modes) kernels, 12 launches per adjoint step"
     res = result_line("ks2d_n128_b32_grad50_ckpt10",
                       {"equation": "KuramotoSivashinsky", "num_points": N, "domain_extent": L, "dt": DT,
                        "order": 2, "batch": B, "rollout_steps": K, "checkpoint_every": CK,
                        "loss": "mean((u_50 - target)^2)", "ic": "TFS cutoff 5 seed 4, target seed 40"},
-                      B * K / wall, K, W, wall, gpu_s / n_launches, B * per_state / (n_launches / K), n_launches,
-                      "row2d/col2d (+ adjoint modes) kernels, 12 launches per adjoint step", None)
+                      B * K / wall, K, W, wall, gpu_s / n_launches, B * per_state * K / n_launches, n_launches,
+                      kernel, None)
     res["unit_note"] = "state-steps/s of a full loss+gradient evaluation"
     res["forward_only_state_steps_per_s"] = B * K / wall_f
     res["gradient_cost_over_forward"] = wall / wall_f
