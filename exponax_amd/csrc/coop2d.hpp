@@ -72,7 +72,11 @@ struct CoopCfg {
 // false when this workgroup gave up or found the launch's failure word set.
 __device__ __forceinline__ bool coop_barrier(unsigned* ctr, unsigned target, unsigned* fail, int spin_limit,
                                              int* flag) {
-  __builtin_amdgcn_s_waitcnt(0);  // this lane's stores have completed
+  // the pass's stores stay above the wait (a compiler-level ordering: a
+  // wavefront-scope release needs no cache maintenance), and the wait
+  // returns once this lane's stores have completed
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
