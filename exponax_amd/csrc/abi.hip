@@ -441,6 +441,7 @@ struct exmi_plan {
   LaunchCoop2D coop2d = nullptr;  // the chain as one cooperative launch (strategy 3)
   LaunchCoopAdj2D coop_adj = nullptr;  // transposed rollout segments as one cooperative launch
   unsigned* coop_sync = nullptr;  // its arrival counters and failure word
+  int coop_spin = 1 << 20;        // polls of a cooperative barrier before it gives up
   Launch2D col2d[COL_NUM_MODES] = {};
   Twiddles tw_col;
   cf *S0 = nullptr, *N0 = nullptr, *S1 = nullptr, *F = nullptr, *T2 = nullptr, *ACC = nullptr;
@@ -696,6 +697,9 @@ static int plan_create_2d(const exmi_plan_desc* d, exmi_plan** out) {
   // passes (single-channel gradient norm and vorticity, N = 128 and 256),
   // where the whole grid fits the device at once.
   if (p->fr2d && C == 1) {
+    // EXMI_COOP_SPIN_LIMIT: the barriers' poll limit (tests force the
+    // give-up path with a tiny one; the launch then ends with NaN states)
+    if (const char* sl = getenv("EXMI_COOP_SPIN_LIMIT")) p->coop_spin = atoi(sl) > 0 ? atoi(sl) : 1;
     Coop2DCapacity cap = nullptr;
     LaunchCoop2D f = get_coop2d(N, kind, &cap);
     if (f && cap && p->ninst <= cap() &&
@@ -2122,7 +2126,7 @@ static int enqueue_coop2d(exmi_plan* p, const float* u0, long long u0_bs, float*
   ca.final_only = final_only;
   ca.ninst = p->ninst;
   ca.sync = p->coop_sync;
-  ca.spin_limit = 1 << 20;
+  ca.spin_limit = p->coop_spin;
   const hipError_t e = p->coop2d(ca, st);
   return e == hipSuccess ? EXMI_OK : fail(EXMI_ERR_HIP, "2-D cooperative rollout launch failed: %s", hipGetErrorString(e));
 }
@@ -3310,7 +3314,7 @@ int exmi_rollout_vjp(exmi_plan* p, const float* u0, int64_t u0_bs, int32_t n, in
         ca.acc_bstride = sz;
         ca.ninst = p->ninst;
         ca.sync = p->coop_sync;
-        ca.spin_limit = 1 << 20;
+        ca.spin_limit = p->coop_spin;
         e = p->coop_adj(ca, st);
         if (e != hipSuccess) return fail(EXMI_ERR_HIP, "cooperative adjoint launch failed: %s", hipGetErrorString(e));
         continue;

@@ -241,3 +241,30 @@ def test_cooperative_rollout_capacity_and_group(device):
         a, _ = ex.rollout_group([st, other], [u0, v0], 12, use_graph=mode)
         np.testing.assert_array_equal(a.cpu().numpy(), alone)
     assert small.rollout_strategy() == 3
+
+
+def test_cooperative_barrier_give_up(device, monkeypatch):
+    """A cooperative barrier that runs past its poll limit gives up: the
+    launch ends normally and the states it has not emitted are NaN rather
+    than stale data (forced from the host with a poll limit of 1, so the
+    workgroups that wait on slower ones give up). The next launch starts
+    from fresh counters: a plan with the default limit is unaffected."""
+    import torch
+
+    print("[case] cooperative barrier give-up", flush=True)
+    u0 = torch.tensor(ics(4, 2, 60.0, 128, 5, seed=63)[0], device=device)
+    monkeypatch.setenv("EXMI_COOP_SPIN_LIMIT", "1")
+    bad = ex.stepper.KuramotoSivashinsky(2, 60.0, 128, 0.05)
+    bad.plan(4).set_rollout_strategy(3)
+    tr = bad.rollout_batch(u0, 12).cpu().numpy()
+    assert not np.isfinite(tr).all()
+    g = bad.rollout_vjp(u0, 12, g_final=u0, checkpoint_every=6).cpu().numpy()
+    assert not np.isfinite(g).all()
+    monkeypatch.delenv("EXMI_COOP_SPIN_LIMIT")
+    good = ex.stepper.KuramotoSivashinsky(2, 60.0, 128, 0.05)
+    good.plan(4).set_rollout_strategy(3)
+    a = good.rollout_batch(u0, 12).cpu().numpy()
+    good.plan(4).set_rollout_strategy(2)
+    b = good.rollout_batch(u0, 12).cpu().numpy()
+    assert np.isfinite(a).all()
+    np.testing.assert_array_equal(a, b)
