@@ -44,6 +44,7 @@ hipError_t launch_fields_lines3d(const NL3DArgs& a, const cf* S, cf* F, float sc
                                  hipStream_t st);
 hipError_t launch_zcross3d(int N, const cf* F, cf* P, long long ntraj, long long Nr, int kcut, const RealTables& rt,
                            hipStream_t st);
+hipError_t launch_zprod3d(const NL3DArgs& a, const cf* F, cf* P, int kcut, const RealTables& rt, hipStream_t st);
 hipError_t launch_wave_spec(cf* X, long long batch, int D, int N, double kscale, double c, double dt,
                             hipStream_t st);
 hipError_t launch_poisson_spec(cf* X, long long count, int D, int N, double kscale, int order, hipStream_t st);
@@ -1157,6 +1158,19 @@ static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf
     // product spectra in R3 (which holds I N^3 >= ninst N^2 (N + 2) floats)
     cf* P = (cf*)p->R3;
     if (e == hipSuccess) e = launch_zcross3d(N, p->F3, P, p->d.batch, p->nr, kz, p->tw.rt(), st);
+    if (e == hipSuccess) e = launch_lines(N, false, P, IP * N * NZ, NZ, N * NZ, NZ, 1.f, tl, st, nullptr, kz + 1);
+    if (e == hipSuccess)
+      e = launch_lines(N, false, P, IP * N * NZ, N * NZ, (long long)N * N * NZ, N * NZ, 1.f, tl, st, nullptr, kz + 1);
+    *ph = P;
+    return e;
+  }
+  const int k3 = p->kind3;
+  if (N <= 128 && (k3 == NL_GRADNORM || k3 == NL_GENERAL || k3 == NL_POLY || k3 == NL_GRAY_SCOTT || k3 == NL_BZ)) {
+    // the z rows with the products fused (zprod3d_kernel); one field and one
+    // product per instance: in place in F3, else into R3 (which holds
+    // I NF N^3 >= ninst NP N^2 (N + 2) floats)
+    cf* P = p->nf3 == p->np3 ? p->F3 : (cf*)p->R3;
+    if (e == hipSuccess) e = launch_zprod3d(a, p->F3, P, kz, p->tw.rt(), st);
     if (e == hipSuccess) e = launch_lines(N, false, P, IP * N * NZ, NZ, N * NZ, NZ, 1.f, tl, st, nullptr, kz + 1);
     if (e == hipSuccess)
       e = launch_lines(N, false, P, IP * N * NZ, N * NZ, (long long)N * N * NZ, N * NZ, 1.f, tl, st, nullptr, kz + 1);

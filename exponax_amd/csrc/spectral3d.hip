@@ -451,6 +451,126 @@ __global__ void __launch_bounds__(256) zcross3d_kernel(const cf* __restrict__ F,
   }
 }
 
+// The z rows of the other nonlinear kinds, fused the same way (the c2r of
+// the trajectory's fields, the products of products3d_kernel at every point
+// in registers, the r2c of the products): C instances per trajectory, NF
+// fields and NP products per instance. Gradient norm (3 fields, 1 product),
+// the general form (u and its 3 derivatives; u^2 or u sum_d d_d u, and
+// |grad u|^2), the polynomial and the coupled reactions (one field and one
+// product per channel).
+template <int KIND>
+struct ZProd {
+  static constexpr int C = KIND == NL_GRAY_SCOTT ? 2 : KIND == NL_BZ ? 3 : 1;
+  static constexpr int NF = KIND == NL_GRADNORM ? 3 : KIND == NL_GENERAL ? 4 : 1;
+  static constexpr int NP = KIND == NL_GENERAL ? 2 : 1;
+};
+
+template <int N, int PT, int KIND>
+__global__ void __launch_bounds__(256) zprod3d_kernel(NL3DArgs a, const cf* __restrict__ F, cf* __restrict__ P,
+                                                      long long nrows, int kcut, RealTables rt) {
+  using Z = ZProd<KIND>;
+  constexpr int C = Z::C, NF = Z::NF, NP = Z::NP, NIN = C * NF, NOUT = C * NP;
+  constexpr int M = N / 2, T = M / PT, G = 256 / T, NZ = M + 1;
+  constexpr int LDSN = LdsSize<M>::value;
+  __shared__ cf lds_all[G * LDSN];
+  const int g = threadIdx.x / T, j = threadIdx.x % T;
+  const long long raw = (long long)blockIdx.x * G + g;
+  const bool active = raw < nrows;
+  const long long row = active ? raw : 0;  // inactive groups mirror row 0 and store nothing
+  const long long traj = row / ((long long)N * N), rr = row % ((long long)N * N);
+  cf* lds = lds_all + g * LDSN;
+  const Lane<M, PT> ln(j);
+  TwRegs<M, PT> tr;
+  tr.load(j, rt);
+  cf v[NIN][PT];
+  cf nyq[NIN];
+#pragma unroll
+  for (int q = 0; q < NIN; ++q) {
+    const cf* x = F + (traj * NIN + q) * a.Nr + rr * NZ;  // field q % NF of instance traj * C + q / NF
+#pragma unroll
+    for (int s = 0; s < PT; ++s) v[q][s] = j + s * T <= kcut ? x[j + s * T] : cf{0.f, 0.f};
+    nyq[q] = M <= kcut ? x[M] : cf{0.f, 0.f};
+  }
+  irfft<M, PT, NIN>(v, nyq, ln, lds, tr);  // N times the physical pairs
+  constexpr float rn = 1.f / (float)N;
+  cf pr[NOUT][PT];
+#pragma unroll
+  for (int s = 0; s < PT; ++s) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {  // the two real points of the slot
+      float f[NIN], o[NOUT];
+#pragma unroll
+      for (int q = 0; q < NIN; ++q) f[q] = (h ? v[q][s].im : v[q][s].re) * rn;
+      if constexpr (KIND == NL_GRADNORM) {
+        o[0] = fmaf(f[2], f[2], fmaf(f[1], f[1], f[0] * f[0]));
+      } else if constexpr (KIND == NL_GENERAL) {
+        const float gs = f[1] + f[2] + f[3];
+        o[0] = a.prod_nc ? f[0] * gs : f[0] * f[0];
+        o[1] = fmaf(f[3], f[3], fmaf(f[2], f[2], f[1] * f[1]));
+      } else if constexpr (KIND == NL_POLY) {
+        float pw = 1.f, acc = 0.f;
+        for (int i = 0; i < a.npoly; ++i) {
+          acc += a.poly[i] * pw;
+          pw *= f[0];
+        }
+        o[0] = acc;
+      } else if constexpr (KIND == NL_GRAY_SCOTT) {
+        gray_scott(f[0], f[1], a.feed, a.kill, o[0], o[1]);
+      } else {
+        belousov_zhabotinsky(f[0], f[1], f[2], o[0], o[1], o[2]);
+      }
+#pragma unroll
+      for (int q = 0; q < NOUT; ++q) {
+        if (h) pr[q][s].im = o[q];
+        else pr[q][s].re = o[q];
+      }
+    }
+  }
+  cf pn[NOUT];
+  rfft<M, PT, NOUT>(pr, pn, ln, lds, tr, kcut);  // 2 X
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < NOUT; ++q) {
+      cf* y = P + (traj * NOUT + q) * a.Nr + rr * NZ;  // product q % NP of instance traj * C + q / NP
+#pragma unroll
+      for (int s = 0; s < PT; ++s)
+        if (j + s * T <= kcut) y[j + s * T] = 0.5f * pr[q][s];
+      if (j == 0 && M <= kcut) y[M] = 0.5f * pn[q];
+    }
+  }
+}
+
+// Fused z rows of a 3-D kind other than the projected convection; returns
+// hipErrorInvalidValue for the kinds it does not cover (the caller then runs
+// the separate passes). P may equal F when every instance has one field and
+// one product (each row group reads its rows before it writes them).
+hipError_t launch_zprod3d(const NL3DArgs& a, const cf* F, cf* P, int kcut, const RealTables& rt, hipStream_t st) {
+  const int N = a.N;
+  auto go = [&](auto kern, int G) {
+    const long long nrows = a.ninst / (a.kind == NL_GRAY_SCOTT ? 2 : a.kind == NL_BZ ? 3 : 1) * N * N;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((nrows + G - 1) / G)), dim3(256), 0, st, a, F, P, nrows, kcut, rt);
+    return hipGetLastError();
+  };
+#define EXMI_ZPROD(NN, TT)                                                       \
+  case NN:                                                                       \
+    switch (a.kind) {                                                            \
+      case NL_GRADNORM: return go(zprod3d_kernel<NN, 8, NL_GRADNORM>, 256 / TT);  \
+      case NL_GENERAL: return go(zprod3d_kernel<NN, 8, NL_GENERAL>, 256 / TT);    \
+      case NL_POLY: return go(zprod3d_kernel<NN, 8, NL_POLY>, 256 / TT);          \
+      case NL_GRAY_SCOTT: return go(zprod3d_kernel<NN, 8, NL_GRAY_SCOTT>, 256 / TT); \
+      case NL_BZ: return go(zprod3d_kernel<NN, 8, NL_BZ>, 256 / TT);              \
+      default: return hipErrorInvalidValue;                                      \
+    }
+  switch (N) {
+    EXMI_ZPROD(16, 1)
+    EXMI_ZPROD(32, 2)
+    EXMI_ZPROD(64, 4)
+    EXMI_ZPROD(128, 8)
+    default: return hipErrorInvalidValue;
+  }
+#undef EXMI_ZPROD
+}
+
 hipError_t launch_zcross3d(int N, const cf* F, cf* P, long long ntraj, long long Nr, int kcut, const RealTables& rt,
                            hipStream_t st) {
   auto go = [&](auto kern, int G) {
