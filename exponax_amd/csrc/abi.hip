@@ -53,6 +53,8 @@ hipError_t launch_products3d(const NL3DArgs& a, const float* R, float* PR, hipSt
 hipError_t launch_combine3d(const NL3DArgs& a, const cf* PH, cf* Nh, hipStream_t st);
 hipError_t launch_stage3d(const Stage3DArgs& a, hipStream_t st);
 hipError_t launch_stage_nl3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* PH, cf* N0out, hipStream_t st);
+hipError_t launch_xstage_proj3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P, cf* N0out, int kcut,
+                                const cf* tw, hipStream_t st);
 hipError_t launch_adj_cofields3d(const NL3DArgs& a, const cf* MU, cf* H, hipStream_t st);
 hipError_t launch_adj_products3d(const NL3DArgs& a, const float* R, const float* HR, float* Q, hipStream_t st);
 hipError_t launch_adj_combine3d(const NL3DArgs& a, const cf* QH, cf* out, hipStream_t st);
@@ -1129,9 +1131,10 @@ static void fill_nl3d(const exmi_plan* p, NL3DArgs& a) {
 // The product spectra of N(X) of every instance into F3 (the passes of
 // spectral3d.hip up to the combine).
 // *ph: where the product spectra are (F3, or for the projected convection
-// of a fused size R3 as spectra).
+// of a fused size R3 as spectra). no_x: the projected convection's forward
+// x lines are left to the caller (stages3d fuses them with the stage).
 static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf* X, hipStream_t st,
-                                       const cf** ph) {
+                                       const cf** ph, bool no_x = false) {
   *ph = p->F3;
   if (p->gen) {
     hipError_t e = launch_fields3d(a, X, p->F3, st);
@@ -1159,7 +1162,7 @@ static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf
     cf* P = (cf*)p->R3;
     if (e == hipSuccess) e = launch_zcross3d(N, p->F3, P, p->d.batch, p->nr, kz, p->tw.rt(), st);
     if (e == hipSuccess) e = launch_lines(N, false, P, IP * N * NZ, NZ, N * NZ, NZ, 1.f, tl, st, nullptr, kz + 1);
-    if (e == hipSuccess)
+    if (e == hipSuccess && !no_x)
       e = launch_lines(N, false, P, IP * N * NZ, N * NZ, (long long)N * N * NZ, N * NZ, 1.f, tl, st, nullptr, kz + 1);
     *ph = P;
     return e;
@@ -1201,18 +1204,26 @@ static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t s
 // result): each stage follows the evaluation of N at its argument (U0 for
 // the first, then the previous stage's S), the combine pass fused into the
 // stage (the first also keeps N0).
+// For the projected convection of the fused sizes up to 128 the forward x
+// lines of the product spectra run fused with the projection and the stage
+// (xstage_proj3d_kernel).
 static hipError_t stages3d(exmi_plan* p, Stage3DArgs& sa, hipStream_t st) {
   static const int seq[4][4] = {{ST1, -1}, {ST2A, ST2B, -1}, {ST3A, ST3B, ST3C, -1}, {ST4A, ST4B, ST4C, ST4D}};
   NL3DArgs a;
   fill_nl3d(p, a);
+  const int N = p->d.num_points, NZ = N / 2 + 1;
+  const bool xs = !p->gen && p->kind3 == NL_PROJ3D && N <= 128;
+  const int kz = p->kmax < NZ - 1 ? (p->kmax < 0 ? 0 : p->kmax) : NZ - 1;
   hipError_t e = hipSuccess;
   for (int i = 0; i < 4 && e == hipSuccess; ++i) {
     const int m = seq[p->d.order - 1][i];
     if (m < 0) break;
     const cf* ph = nullptr;
-    e = nonlinear3d_products(p, a, i == 0 ? sa.U0 : sa.S, st, &ph);
+    e = nonlinear3d_products(p, a, i == 0 ? sa.U0 : sa.S, st, &ph, xs);
     sa.mode = m;
-    if (e == hipSuccess) e = launch_stage_nl3d(a, sa, ph, i == 0 ? p->N03 : nullptr, st);
+    cf* n0 = i == 0 ? p->N03 : nullptr;
+    if (e == hipSuccess)
+      e = xs ? launch_xstage_proj3d(a, sa, ph, n0, kz, p->tw_col.tw, st) : launch_stage_nl3d(a, sa, ph, n0, st);
   }
   return e;
 }

@@ -8,6 +8,8 @@
 // lanes (lane j holds points j + s T); CPB lines share a 256-thread workgroup,
 // and lines adjacent in memory sit in adjacent lane groups, so for a fixed
 // slot the groups of a wavefront read neighbouring addresses.
+#include <type_traits>
+
 #include "exmi_internal.hpp"
 #include "fft1d.hpp"
 #include "spectral3d.hpp"
@@ -853,55 +855,84 @@ __device__ __forceinline__ bool stage_reads_n0(int m) {
 }
 __device__ __forceinline__ bool stage_reads_nc(int m) { return !(m == ST1 || m == ST2A || m == ST3A || m == ST4A); }
 
+// The coefficient table of instance (trajectory traj, channel ch).
+__device__ __forceinline__ const float* stage_table(const Stage3DArgs& a, long long traj, long long ch) {
+  return a.tables +
+         ((a.lin_batch > 1 ? traj : 0) * a.lin_channels + (a.lin_channels > 1 ? ch : 0)) * (long long)a.ncoef * a.nr;
+}
+
+// The stage combination of stage MODE at entry idx (mode k of its instance,
+// tb the instance's coefficient table) with the nonlinear terms n0 (N0) and
+// nc (NC) given; U0, S, S1, A as in Stage3DArgs.
+template <int MODE>
+__device__ __forceinline__ void stage_core(int e_im, long long nr, const float* tb, long long k, long long idx,
+                                           const cf* U0, cf* S, cf* S1, cf* A, cf n0, cf nc) {
+  auto c = [&](int i) { return tb[(long long)i * nr + k]; };
+  // coefficient i (0: E, 1: E_half for orders 3-4) times x, complex when L is
+  auto cmul = [&](int i, cf x) -> cf {
+    const float er = c(i);
+    if (e_im < 0) return er * x;
+    const float ei = c(e_im + i);
+    return cf{fmaf(er, x.re, -ei * x.im), fmaf(er, x.im, ei * x.re)};
+  };
+  auto emul = [&](cf x) -> cf { return cmul(0, x); };
+  if constexpr (MODE == ST1 || MODE == ST2A) {
+    S[idx] = emul(U0[idx]) + c(1) * n0;
+  } else if constexpr (MODE == ST2B) {
+    S[idx] = S[idx] + c(2) * (nc - n0);
+  } else if constexpr (MODE == ST3A) {
+    const cf u0 = U0[idx];
+    A[idx] = emul(u0) + c(4) * n0;
+    S[idx] = cmul(1, u0) + c(2) * n0;
+  } else if constexpr (MODE == ST3B) {
+    const cf u0 = U0[idx];
+    A[idx] = A[idx] + c(5) * nc;
+    S[idx] = emul(u0) + c(3) * (2.f * nc - n0);
+  } else if constexpr (MODE == ST3C) {
+    S[idx] = A[idx] + c(6) * nc;
+  } else if constexpr (MODE == ST4A) {
+    const cf u0 = U0[idx];
+    A[idx] = emul(u0) + c(3) * n0;
+    const cf s = cmul(1, u0) + c(2) * n0;
+    S1[idx] = s;
+    S[idx] = s;
+  } else if constexpr (MODE == ST4B) {
+    const cf u0 = U0[idx];
+    A[idx] = A[idx] + (2.f * c(4)) * nc;
+    S[idx] = cmul(1, u0) + c(2) * nc;
+  } else if constexpr (MODE == ST4C) {
+    A[idx] = A[idx] + (2.f * c(4)) * nc;
+    S[idx] = cmul(1, S1[idx]) + c(2) * (2.f * nc - n0);
+  } else if constexpr (MODE == ST4D) {
+    S[idx] = A[idx] + c(5) * nc;
+  }
+}
+
+// Calls f(std::integral_constant<int, mode>) for the runtime stage mode.
+template <class F>
+__device__ __forceinline__ void with_stage_mode(int mode, F&& f) {
+  switch (mode) {
+    case ST1: f(std::integral_constant<int, ST1>{}); break;
+    case ST2A: f(std::integral_constant<int, ST2A>{}); break;
+    case ST2B: f(std::integral_constant<int, ST2B>{}); break;
+    case ST3A: f(std::integral_constant<int, ST3A>{}); break;
+    case ST3B: f(std::integral_constant<int, ST3B>{}); break;
+    case ST3C: f(std::integral_constant<int, ST3C>{}); break;
+    case ST4A: f(std::integral_constant<int, ST4A>{}); break;
+    case ST4B: f(std::integral_constant<int, ST4B>{}); break;
+    case ST4C: f(std::integral_constant<int, ST4C>{}); break;
+    case ST4D: f(std::integral_constant<int, ST4D>{}); break;
+  }
+}
+
 // The stage combination at entry idx with the nonlinear terms n0 (N0) and
 // nc (NC) given.
 __device__ __forceinline__ void stage_at(const Stage3DArgs& a, long long idx, cf n0, cf nc) {
-  {
-    const long long inst = idx / a.nr, k = idx % a.nr;
-    const long long traj = inst / a.channels, ch = inst % a.channels;
-    const float* tb = a.tables + ((a.lin_batch > 1 ? traj : 0) * a.lin_channels + (a.lin_channels > 1 ? ch : 0)) *
-                                     (long long)a.ncoef * a.nr;
-    auto c = [&](int i) { return tb[(long long)i * a.nr + k]; };
-    // coefficient i (0: E, 1: E_half for orders 3-4) times x, complex when L is
-    auto cmul = [&](int i, cf x) -> cf {
-      const float er = c(i);
-      if (a.e_im < 0) return er * x;
-      const float ei = c(a.e_im + i);
-      return cf{fmaf(er, x.re, -ei * x.im), fmaf(er, x.im, ei * x.re)};
-    };
-    auto emul = [&](cf x) -> cf { return cmul(0, x); };
-    const cf u0 = a.U0[idx];
-    switch (a.mode) {
-      case ST1:
-      case ST2A: a.S[idx] = emul(u0) + c(1) * n0; break;
-      case ST2B: a.S[idx] = a.S[idx] + c(2) * (nc - n0); break;
-      case ST3A:
-        a.A[idx] = emul(u0) + c(4) * n0;
-        a.S[idx] = cmul(1, u0) + c(2) * n0;
-        break;
-      case ST3B:
-        a.A[idx] = a.A[idx] + c(5) * nc;
-        a.S[idx] = emul(u0) + c(3) * (2.f * nc - n0);
-        break;
-      case ST3C: a.S[idx] = a.A[idx] + c(6) * nc; break;
-      case ST4A: {
-        a.A[idx] = emul(u0) + c(3) * n0;
-        const cf s = cmul(1, u0) + c(2) * n0;
-        a.S1[idx] = s;
-        a.S[idx] = s;
-        break;
-      }
-      case ST4B:
-        a.A[idx] = a.A[idx] + (2.f * c(4)) * nc;
-        a.S[idx] = cmul(1, u0) + c(2) * nc;
-        break;
-      case ST4C:
-        a.A[idx] = a.A[idx] + (2.f * c(4)) * nc;
-        a.S[idx] = cmul(1, a.S1[idx]) + c(2) * (2.f * nc - n0);
-        break;
-      case ST4D: a.S[idx] = a.A[idx] + c(5) * nc; break;
-    }
-  }
+  const long long inst = idx / a.nr, k = idx % a.nr;
+  const float* tb = stage_table(a, inst / a.channels, inst % a.channels);
+  with_stage_mode(a.mode, [&](auto m) {
+    stage_core<decltype(m)::value>(a.e_im, a.nr, tb, k, idx, a.U0, a.S, a.S1, a.A, n0, nc);
+  });
 }
 
 __global__ void stage3d_kernel(Stage3DArgs a) {
@@ -965,6 +996,127 @@ __global__ void stage_nl3d_proj_kernel(NL3DArgs na, Stage3DArgs a, const cf* __r
         stage_at(a, i, stage_reads_n0(a.mode) ? a.N0[i] : cf{0.f, 0.f}, r);
       }
     }
+  }
+}
+
+// The forward x lines of the projected convection's three product spectra
+// fused with the projection and the stage: one lane group owns the line
+// (trajectory, iy, kz) of all three channels (P holds them after the z rows
+// and the forward y lines), transforms them along x (planes kz > kcut hold
+// no products and are not read: their N_hat is zero), and applies at every
+// point the projection and the stage of stage_nl3d_proj_kernel. The state
+// buffers come in as separate non-aliasing arguments so that the stage's
+// loads of later points need not wait for the stores of earlier ones.
+// 8 points per lane from N = 64 on: twice the wavefronts of the line
+// kernels' 16, for the latency of the stage's accesses.
+template <int N>
+struct XStageCfg {
+  static constexpr int PT = N >= 64 ? 8 : LineCfg<N>::PT;
+  static constexpr int T = N / PT;
+  static constexpr int CPB = 256 / T;
+};
+
+template <int N>
+__global__ void __launch_bounds__(256)
+    xstage_proj3d_kernel(NL3DArgs na, Stage3DArgs a, const cf* __restrict__ P, cf* __restrict__ N0out, int kcut,
+                         const cf* __restrict__ tw, const cf* __restrict__ U0, const cf* __restrict__ N0,
+                         cf* __restrict__ S, cf* __restrict__ S1, cf* __restrict__ A) {
+  using C = XStageCfg<N>;
+  constexpr int PT = C::PT, T = C::T;
+  constexpr int LDSN = LdsSize<N>::value;
+  __shared__ cf lds_x[C::CPB * LDSN];
+  const int c = threadIdx.x / T, j = threadIdx.x % T;
+  const int NZ = N / 2 + 1;
+  const long long inner = (long long)N * NZ;  // lines (iy, kz) per trajectory
+  const long long nlines = a.ninst / 3 * inner;
+  const long long raw = (long long)blockIdx.x * C::CPB + c;
+  const long long line = raw < nlines ? raw : 0;  // inactive groups mirror line 0 and store nothing
+  const long long traj = line / inner;
+  const int li = (int)(line - traj * inner);
+  const int iy = li / NZ, iz = li - iy * NZ;
+  const bool kept = iz <= kcut;
+  cf* lds = lds_x + c * LDSN;
+  const Lane<N, PT> ln(j);
+  TwRegs<N, PT> tr;
+  tr.template load_pass<1>(j, tw);
+  cf v[3][PT];
+  const long long b0 = traj * 3 * a.nr + li;  // channel 0's entry of this line at ix = 0
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {
+#pragma unroll
+    for (int s = 0; s < PT; ++s)
+      v[ch][s] = kept ? P[b0 + ch * a.nr + (long long)(j + s * T) * inner] : cf{0.f, 0.f};
+  }
+  cfft<N, PT, false, 3>(v, ln, lds, tr);
+  // the projection (as in stage_nl3d_proj_kernel), in place
+#pragma unroll
+  for (int s = 0; s < PT; ++s) {
+    const Mode3D m = mode3d_at(na, j + s * T, iy, iz);
+    const cf p0 = v[0][s], p1 = v[1][s], p2 = v[2][s];
+    const cf div = imul(m.kx, p0) + imul(m.ky, p1) + imul(m.kz, p2);
+    const float k2 = m.kx * m.kx + m.ky * m.ky + m.kz * m.kz;
+    const cf pr = k2 != 0.f ? (1.f / k2) * div : cf{0.f, 0.f};
+    const cf z{0.f, 0.f};
+    v[0][s] = m.keep ? p0 + imul(m.kx, pr) : z;
+    v[1][s] = m.keep ? p1 + imul(m.ky, pr) : z;
+    v[2][s] = m.keep ? p2 + imul(m.kz, pr) : z;
+    if (m.inj) v[0][s] = v[0][s] + cf{na.inj_value, 0.f};
+  }
+  // The stage runs transposed through LDS: thread (q0, cl) takes the points
+  // ix = q0 + q T of the workgroup's line cl, so that a wavefront's accesses
+  // to the state buffers cover neighbouring lines (contiguous entries).
+  const int cl = threadIdx.x % C::CPB, q0 = threadIdx.x / C::CPB;
+  const long long sraw = (long long)blockIdx.x * C::CPB + cl;
+  const bool sact = sraw < nlines;
+  const long long sline = sact ? sraw : 0;
+  const long long straj = sline / inner, sli = sline - straj * inner;
+  const long long sb0 = straj * 3 * a.nr + sli;
+  // one copy of the stage loop per stage mode (a per-point switch would put
+  // every mode's code into every point's path)
+  with_stage_mode(a.mode, [&](auto mc) {
+    constexpr int M = decltype(mc)::value;
+    constexpr bool first = !(M == ST2B || M == ST3B || M == ST3C || M == ST4B || M == ST4C || M == ST4D);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      if (ch > 0) __syncthreads();  // the previous channel's reads are done
+#pragma unroll
+      for (int s = 0; s < PT; ++s) lds[j + s * T] = v[ch][s];
+      __syncthreads();
+      const float* tb = stage_table(a, straj, ch);
+#pragma unroll
+      for (int q = 0; q < PT; ++q) {
+        const int ix = q0 + q * T;
+        const cf r = lds_x[cl * LDSN + ix];
+        const long long k = sli + (long long)ix * inner;
+        const long long i = sb0 + ch * a.nr + (long long)ix * inner;
+        if (!sact) continue;
+        if constexpr (first) {
+          N0out[i] = r;
+          stage_core<M>(a.e_im, a.nr, tb, k, i, U0, S, S1, A, r, cf{0.f, 0.f});
+        } else {
+          stage_core<M>(a.e_im, a.nr, tb, k, i, U0, S, S1, A, stage_reads_n0(M) ? N0[i] : cf{0.f, 0.f}, r);
+        }
+      }
+    }
+  });
+}
+
+hipError_t launch_xstage_proj3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P, cf* N0out, int kcut,
+                                const cf* tw, hipStream_t st) {
+  const bool first = a.mode == ST1 || a.mode == ST2A || a.mode == ST3A || a.mode == ST4A;
+  if (first != (N0out != nullptr)) return hipErrorInvalidValue;  // the first stage keeps N0, the others read it
+  auto go = [&](auto kern, int cpb) {
+    const long long nlines = a.ninst / 3 * (long long)na.N * na.NZ;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((nlines + cpb - 1) / cpb)), dim3(256), 0, st, na, a, P, N0out, kcut,
+                       tw, a.U0, a.N0, a.S, a.S1, a.A);
+    return hipGetLastError();
+  };
+  switch (na.N) {
+    case 16: return go(xstage_proj3d_kernel<16>, XStageCfg<16>::CPB);
+    case 32: return go(xstage_proj3d_kernel<32>, XStageCfg<32>::CPB);
+    case 64: return go(xstage_proj3d_kernel<64>, XStageCfg<64>::CPB);
+    case 128: return go(xstage_proj3d_kernel<128>, XStageCfg<128>::CPB);
+    default: return hipErrorInvalidValue;
   }
 }
 
