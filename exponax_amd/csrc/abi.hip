@@ -55,6 +55,8 @@ hipError_t launch_stage3d(const Stage3DArgs& a, hipStream_t st);
 hipError_t launch_stage_nl3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* PH, cf* N0out, hipStream_t st);
 hipError_t launch_xstage_proj3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P, cf* N0out, int kcut,
                                 const cf* tw, hipStream_t st);
+hipError_t launch_xstage3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P, cf* N0out, int kcut, const cf* tw,
+                           hipStream_t st);
 hipError_t launch_adj_cofields3d(const NL3DArgs& a, const cf* MU, cf* H, hipStream_t st);
 hipError_t launch_adj_products3d(const NL3DArgs& a, const float* R, const float* HR, float* Q, hipStream_t st);
 hipError_t launch_adj_combine3d(const NL3DArgs& a, const cf* QH, cf* out, hipStream_t st);
@@ -1128,11 +1130,25 @@ static void fill_nl3d(const exmi_plan* p, NL3DArgs& a) {
   a.nq = p->adj_nq;
 }
 
+// Fused 3-D plans whose z rows run with the products fused (zprod3d_kernel).
+static bool zprod_kind(const exmi_plan* p) {
+  const int k3 = p->kind3;
+  return !p->gen && p->d.num_points <= 128 &&
+         (k3 == NL_GRADNORM || k3 == NL_GENERAL || k3 == NL_POLY || k3 == NL_GRAY_SCOTT || k3 == NL_BZ);
+}
+
+// Fused 3-D plans whose forward x lines run with the combine and the stage
+// (xstage3d_kernel, xstage_proj3d_kernel) inside an ETDRK step.
+static bool xstage_kind(const exmi_plan* p) {
+  return zprod_kind(p) || (!p->gen && p->kind3 == NL_PROJ3D && p->d.num_points <= 128);
+}
+
 // The product spectra of N(X) of every instance into F3 (the passes of
 // spectral3d.hip up to the combine).
 // *ph: where the product spectra are (F3, or for the projected convection
-// of a fused size R3 as spectra). no_x: the projected convection's forward
-// x lines are left to the caller (stages3d fuses them with the stage).
+// of a fused size R3 as spectra). no_x: the forward x lines of the kinds
+// with fused z rows are left to the caller (stages3d fuses them with the
+// stage; see xstage_kind).
 static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf* X, hipStream_t st,
                                        const cf** ph, bool no_x = false) {
   *ph = p->F3;
@@ -1167,15 +1183,14 @@ static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf
     *ph = P;
     return e;
   }
-  const int k3 = p->kind3;
-  if (N <= 128 && (k3 == NL_GRADNORM || k3 == NL_GENERAL || k3 == NL_POLY || k3 == NL_GRAY_SCOTT || k3 == NL_BZ)) {
+  if (zprod_kind(p)) {
     // the z rows with the products fused (zprod3d_kernel); one field and one
     // product per instance: in place in F3, else into R3 (which holds
     // I NF N^3 >= ninst NP N^2 (N + 2) floats)
     cf* P = p->nf3 == p->np3 ? p->F3 : (cf*)p->R3;
     if (e == hipSuccess) e = launch_zprod3d(a, p->F3, P, kz, p->tw.rt(), st);
     if (e == hipSuccess) e = launch_lines(N, false, P, IP * N * NZ, NZ, N * NZ, NZ, 1.f, tl, st, nullptr, kz + 1);
-    if (e == hipSuccess)
+    if (e == hipSuccess && !no_x)
       e = launch_lines(N, false, P, IP * N * NZ, N * NZ, (long long)N * N * NZ, N * NZ, 1.f, tl, st, nullptr, kz + 1);
     *ph = P;
     return e;
@@ -1204,15 +1219,14 @@ static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t s
 // result): each stage follows the evaluation of N at its argument (U0 for
 // the first, then the previous stage's S), the combine pass fused into the
 // stage (the first also keeps N0).
-// For the projected convection of the fused sizes up to 128 the forward x
-// lines of the product spectra run fused with the projection and the stage
-// (xstage_proj3d_kernel).
+// For the fused sizes up to 128 the forward x lines of the product spectra
+// run fused with the combine (the projection) and the stage (xstage_kind).
 static hipError_t stages3d(exmi_plan* p, Stage3DArgs& sa, hipStream_t st) {
   static const int seq[4][4] = {{ST1, -1}, {ST2A, ST2B, -1}, {ST3A, ST3B, ST3C, -1}, {ST4A, ST4B, ST4C, ST4D}};
   NL3DArgs a;
   fill_nl3d(p, a);
   const int N = p->d.num_points, NZ = N / 2 + 1;
-  const bool xs = !p->gen && p->kind3 == NL_PROJ3D && N <= 128;
+  const bool xs = xstage_kind(p);
   const int kz = p->kmax < NZ - 1 ? (p->kmax < 0 ? 0 : p->kmax) : NZ - 1;
   hipError_t e = hipSuccess;
   for (int i = 0; i < 4 && e == hipSuccess; ++i) {
@@ -1223,7 +1237,9 @@ static hipError_t stages3d(exmi_plan* p, Stage3DArgs& sa, hipStream_t st) {
     sa.mode = m;
     cf* n0 = i == 0 ? p->N03 : nullptr;
     if (e == hipSuccess)
-      e = xs ? launch_xstage_proj3d(a, sa, ph, n0, kz, p->tw_col.tw, st) : launch_stage_nl3d(a, sa, ph, n0, st);
+      e = !xs                          ? launch_stage_nl3d(a, sa, ph, n0, st)
+          : p->kind3 == NL_PROJ3D ? launch_xstage_proj3d(a, sa, ph, n0, kz, p->tw_col.tw, st)
+                                  : launch_xstage3d(a, sa, ph, n0, kz, p->tw_col.tw, st);
   }
   return e;
 }

@@ -1120,6 +1120,131 @@ hipError_t launch_xstage_proj3d(const NL3DArgs& na, const Stage3DArgs& a, const 
   }
 }
 
+// N_hat at one mode from the product spectra values ph0 (and ph1 for the
+// general form) of the kinds with fused z rows: combine_at's formulas for
+// these kinds, the kind fixed at compile time.
+template <int KIND>
+__device__ __forceinline__ cf combine_vals(const NL3DArgs& a, const Mode3D& m, cf ph0, cf ph1) {
+  cf r;
+  if constexpr (KIND == NL_POLY) {
+    r = a.poly_lap ? (a.scale * -(m.kx * m.kx + m.ky * m.ky + m.kz * m.kz)) * ph0 : ph0;
+  } else if constexpr (KIND == NL_GRADNORM) {
+    r = (a.zero_mode_fix && m.zero) ? cf{0.f, 0.f} : (-0.5f * a.scale) * ph0;
+  } else if constexpr (KIND == NL_GENERAL) {
+    const float cg = (a.zero_mode_fix && m.zero) ? 0.f : 0.5f * a.g2;
+    r = a.g0 * ph0 + imul(0.5f * a.g1 * (m.kx + m.ky + m.kz), ph0) + cg * ph1;
+  } else {
+    r = ph0;  // the coupled reactions
+  }
+  return m.keep ? r : cf{0.f, 0.f};
+}
+
+// The forward x lines of the product spectra of the kinds with fused z rows
+// (gradient norm, polynomial, general form, coupled reactions) fused with
+// the combine and the stage, as xstage_proj3d_kernel does for the projected
+// convection: a lane group owns the line (instance, iy, kz) of the
+// instance's NP product spectra, transforms them along x (planes kz > kcut
+// are not read), forms N_hat per point, and the workgroup runs the stage
+// transposed through LDS, one loop per stage mode.
+template <int N, int KIND>
+__global__ void __launch_bounds__(256)
+    xstage3d_kernel(NL3DArgs na, Stage3DArgs a, const cf* __restrict__ P, cf* __restrict__ N0out, int kcut,
+                    const cf* __restrict__ tw, const cf* __restrict__ U0, const cf* __restrict__ N0,
+                    cf* __restrict__ S, cf* __restrict__ S1, cf* __restrict__ A) {
+  using C = XStageCfg<N>;
+  constexpr int PT = C::PT, T = C::T, NP = ZProd<KIND>::NP;
+  constexpr int LDSN = LdsSize<N>::value;
+  __shared__ cf lds_x[C::CPB * LDSN];
+  const int c = threadIdx.x / T, j = threadIdx.x % T;
+  const int NZ = N / 2 + 1;
+  const long long inner = (long long)N * NZ;  // lines (iy, kz) per instance
+  const long long nlines = a.ninst * inner;
+  const long long raw = (long long)blockIdx.x * C::CPB + c;
+  const long long line = raw < nlines ? raw : 0;  // inactive groups mirror line 0 and store nothing
+  const long long inst = line / inner;
+  const int li = (int)(line - inst * inner);
+  const int iy = li / NZ, iz = li - iy * NZ;
+  const bool kept = iz <= kcut;
+  cf* lds = lds_x + c * LDSN;
+  const Lane<N, PT> ln(j);
+  TwRegs<N, PT> tr;
+  tr.template load_pass<1>(j, tw);
+  cf v[NP][PT];
+  const long long b0 = inst * NP * a.nr + li;  // product 0 of this line at ix = 0
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+#pragma unroll
+    for (int s = 0; s < PT; ++s)
+      v[q][s] = kept ? P[b0 + q * a.nr + (long long)(j + s * T) * inner] : cf{0.f, 0.f};
+  }
+  cfft<N, PT, false, NP>(v, ln, lds, tr);
+#pragma unroll
+  for (int s = 0; s < PT; ++s)
+    v[0][s] = combine_vals<KIND>(na, mode3d_at(na, j + s * T, iy, iz), v[0][s], v[NP - 1][s]);
+  // the stage, transposed through LDS (thread (q0, cl) takes the points
+  // ix = q0 + q T of the workgroup's line cl)
+  const int cl = threadIdx.x % C::CPB, q0 = threadIdx.x / C::CPB;
+  const long long sraw = (long long)blockIdx.x * C::CPB + cl;
+  const bool sact = sraw < nlines;
+  const long long sline = sact ? sraw : 0;
+  const long long sinst = sline / inner, sli = sline - sinst * inner;
+  const long long sb0 = sinst * a.nr + sli;
+  const float* tb = stage_table(a, sinst / a.channels, sinst % a.channels);
+#pragma unroll
+  for (int s = 0; s < PT; ++s) lds[j + s * T] = v[0][s];
+  __syncthreads();
+  with_stage_mode(a.mode, [&](auto mc) {
+    constexpr int M = decltype(mc)::value;
+    constexpr bool first = !(M == ST2B || M == ST3B || M == ST3C || M == ST4B || M == ST4C || M == ST4D);
+#pragma unroll
+    for (int q = 0; q < PT; ++q) {
+      const int ix = q0 + q * T;
+      const cf r = lds_x[cl * LDSN + ix];
+      const long long k = sli + (long long)ix * inner;
+      const long long i = sb0 + (long long)ix * inner;
+      if (!sact) continue;
+      if constexpr (first) {
+        N0out[i] = r;
+        stage_core<M>(a.e_im, a.nr, tb, k, i, U0, S, S1, A, r, cf{0.f, 0.f});
+      } else {
+        stage_core<M>(a.e_im, a.nr, tb, k, i, U0, S, S1, A, stage_reads_n0(M) ? N0[i] : cf{0.f, 0.f}, r);
+      }
+    }
+  });
+}
+
+// Fused forward x lines, combine and stage of the kinds with fused z rows;
+// hipErrorInvalidValue for the other kinds and sizes.
+hipError_t launch_xstage3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P, cf* N0out, int kcut, const cf* tw,
+                           hipStream_t st) {
+  const bool first = a.mode == ST1 || a.mode == ST2A || a.mode == ST3A || a.mode == ST4A;
+  if (first != (N0out != nullptr)) return hipErrorInvalidValue;  // the first stage keeps N0, the others read it
+  auto go = [&](auto kern, int cpb) {
+    const long long nlines = a.ninst * (long long)na.N * na.NZ;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((nlines + cpb - 1) / cpb)), dim3(256), 0, st, na, a, P, N0out, kcut,
+                       tw, a.U0, a.N0, a.S, a.S1, a.A);
+    return hipGetLastError();
+  };
+#define EXMI_XSTAGE(NN)                                                                      \
+  case NN:                                                                                   \
+    switch (na.kind) {                                                                       \
+      case NL_GRADNORM: return go(xstage3d_kernel<NN, NL_GRADNORM>, XStageCfg<NN>::CPB);     \
+      case NL_GENERAL: return go(xstage3d_kernel<NN, NL_GENERAL>, XStageCfg<NN>::CPB);       \
+      case NL_POLY: return go(xstage3d_kernel<NN, NL_POLY>, XStageCfg<NN>::CPB);             \
+      case NL_GRAY_SCOTT: return go(xstage3d_kernel<NN, NL_GRAY_SCOTT>, XStageCfg<NN>::CPB); \
+      case NL_BZ: return go(xstage3d_kernel<NN, NL_BZ>, XStageCfg<NN>::CPB);                 \
+      default: return hipErrorInvalidValue;                                                  \
+    }
+  switch (na.N) {
+    EXMI_XSTAGE(16)
+    EXMI_XSTAGE(32)
+    EXMI_XSTAGE(64)
+    EXMI_XSTAGE(128)
+    default: return hipErrorInvalidValue;
+  }
+#undef EXMI_XSTAGE
+}
+
 hipError_t launch_stage_nl3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* PH, cf* N0out, hipStream_t st) {
   if (na.kind == NL_PROJ3D && a.channels == 3) {
     hipLaunchKernelGGL(stage_nl3d_proj_kernel, dim3(grid_of(a.ninst / 3 * a.nr)), dim3(256), 0, st, na, a, PH,
