@@ -153,7 +153,21 @@ def test_nonlinear_3d_parity(device, name):
     _check(gpu, cpu, u0, device, name)
 
 
-@pytest.mark.parametrize("N,order", [(16, 2), (32, 2), (32, 1), (32, 4)])
+@pytest.mark.parametrize("name", ["ks", "cahn_hilliard", "gray_scott", "general"])
+def test_nonlinear_3d_parity_n64(device, name):
+    """N = 64, where the kernel that fuses the forward x lines with the
+    combine and the ETDRK stage holds 8 points per lane (16 below)."""
+    N = 64
+    print(f"[case] 3-D {name} N={N}", flush=True)
+    gpu, cpu, L, C = _nl3d_pair(name, N)
+    u0, _ = ics(2, 3, L, N, 3, seed=113, channels=C)
+    if name == "gray_scott":
+        u0 = (np.array([0.7, 0.2], dtype=np.float32).reshape(1, 2, 1, 1, 1) + 0.2 * u0 / np.abs(u0).max()).astype(
+            np.float32)
+    _check(gpu, cpu, u0, device, name)
+
+
+@pytest.mark.parametrize("N,order", [(16, 2), (32, 2), (32, 1), (32, 4), (64, 2), (64, 3)])
 def test_navier_stokes_velocity_parity(device, N, order):
     """stepper/_navier_stokes.py:330-598: P(u x omega) (Leray-projected
     rotational convection) and, for N = 32 order 2, the Kolmogorov forcing."""
