@@ -1030,7 +1030,8 @@ static int plan_create_md(const exmi_plan_desc* d, exmi_plan** out, bool gen) {
                      hipMalloc(&p->S3, sizeof(cf) * I * p->nr) == hipSuccess &&
                      hipMalloc(&p->F3, sizeof(cf) * I * nfp * p->nr) == hipSuccess &&
                      hipMalloc(&p->R3, sizeof(float) * I * p->nf3 * n3) == hipSuccess &&
-                     hipMalloc(&p->PR3, sizeof(float) * I * p->np3 * n3) == hipSuccess)) &&
+                     hipMalloc(&p->PR3, sizeof(float) * I * p->np3 * (2 * p->nr > n3 ? 2 * p->nr : n3)) ==
+                         hipSuccess)) &&
             (d->order < 3 || hipMalloc(&p->A3, sizeof(cf) * I * p->nr) == hipSuccess) &&
             (d->order < 4 || hipMalloc(&p->S13, sizeof(cf) * I * p->nr) == hipSuccess);
   if (!ok) {
@@ -1134,7 +1135,8 @@ static void fill_nl3d(const exmi_plan* p, NL3DArgs& a) {
 static bool zprod_kind(const exmi_plan* p) {
   const int k3 = p->kind3;
   return !p->gen && p->d.num_points <= 128 &&
-         (k3 == NL_GRADNORM || k3 == NL_GENERAL || k3 == NL_POLY || k3 == NL_GRAY_SCOTT || k3 == NL_BZ);
+         (k3 == NL_GRADNORM || k3 == NL_GENERAL || k3 == NL_POLY || k3 == NL_GRAY_SCOTT || k3 == NL_BZ ||
+          k3 == NL_CONV_CONS || k3 == NL_CONV_NONCONS);
 }
 
 // Fused 3-D plans whose forward x lines run with the combine and the stage
@@ -1185,9 +1187,11 @@ static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf
   }
   if (zprod_kind(p)) {
     // the z rows with the products fused (zprod3d_kernel); one field and one
-    // product per instance: in place in F3, else into R3 (which holds
-    // I NF N^3 >= ninst NP N^2 (N + 2) floats)
-    cf* P = p->nf3 == p->np3 ? p->F3 : (cf*)p->R3;
+    // product per instance: in place in F3; three products per instance
+    // (the conservative convection): into PR3 (allocated to hold them as
+    // spectra); else into R3 (which holds I NF N^3 >= ninst NP N^2 (N + 2)
+    // floats)
+    cf* P = p->nf3 == p->np3 ? p->F3 : p->kind3 == NL_CONV_CONS ? (cf*)p->PR3 : (cf*)p->R3;
     if (e == hipSuccess) e = launch_zprod3d(a, p->F3, P, kz, p->tw.rt(), st);
     if (e == hipSuccess) e = launch_lines(N, false, P, IP * N * NZ, NZ, N * NZ, NZ, 1.f, tl, st, nullptr, kz + 1);
     if (e == hipSuccess && !no_x)

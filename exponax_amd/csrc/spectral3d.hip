@@ -459,12 +459,15 @@ __global__ void __launch_bounds__(256) zcross3d_kernel(const cf* __restrict__ F,
 // fields and NP products per instance. Gradient norm (3 fields, 1 product),
 // the general form (u and its 3 derivatives; u^2 or u sum_d d_d u, and
 // |grad u|^2), the polynomial and the coupled reactions (one field and one
-// product per channel).
+// product per channel), the three-channel convections (conservative: u_c,
+// and u_c u_j for every j; non-conservative: u_c and its 3 derivatives, and
+// sum_j u_j d_j u_c).
 template <int KIND>
 struct ZProd {
-  static constexpr int C = KIND == NL_GRAY_SCOTT ? 2 : KIND == NL_BZ ? 3 : 1;
-  static constexpr int NF = KIND == NL_GRADNORM ? 3 : KIND == NL_GENERAL ? 4 : 1;
-  static constexpr int NP = KIND == NL_GENERAL ? 2 : 1;
+  static constexpr bool CONV = KIND == NL_CONV_CONS || KIND == NL_CONV_NONCONS;
+  static constexpr int C = KIND == NL_GRAY_SCOTT ? 2 : (KIND == NL_BZ || CONV) ? 3 : 1;
+  static constexpr int NF = KIND == NL_GRADNORM ? 3 : (KIND == NL_GENERAL || KIND == NL_CONV_NONCONS) ? 4 : 1;
+  static constexpr int NP = KIND == NL_GENERAL ? 2 : KIND == NL_CONV_CONS ? 3 : 1;
 };
 
 template <int N, int PT, int KIND>
@@ -518,6 +521,14 @@ __global__ void __launch_bounds__(256) zprod3d_kernel(NL3DArgs a, const cf* __re
         o[0] = acc;
       } else if constexpr (KIND == NL_GRAY_SCOTT) {
         gray_scott(f[0], f[1], a.feed, a.kill, o[0], o[1]);
+      } else if constexpr (KIND == NL_CONV_CONS) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int jj = 0; jj < 3; ++jj) o[c * 3 + jj] = f[c] * f[jj];
+      } else if constexpr (KIND == NL_CONV_NONCONS) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[c] = fmaf(f[8], f[c * 4 + 3], fmaf(f[4], f[c * 4 + 2], f[0] * f[c * 4 + 1]));
       } else {
         belousov_zhabotinsky(f[0], f[1], f[2], o[0], o[1], o[2]);
       }
@@ -549,19 +560,24 @@ __global__ void __launch_bounds__(256) zprod3d_kernel(NL3DArgs a, const cf* __re
 hipError_t launch_zprod3d(const NL3DArgs& a, const cf* F, cf* P, int kcut, const RealTables& rt, hipStream_t st) {
   const int N = a.N;
   auto go = [&](auto kern, int G) {
-    const long long nrows = a.ninst / (a.kind == NL_GRAY_SCOTT ? 2 : a.kind == NL_BZ ? 3 : 1) * N * N;
+    const int C = a.kind == NL_GRAY_SCOTT ? 2 : (a.kind == NL_BZ || a.kind == NL_CONV_CONS || a.kind == NL_CONV_NONCONS) ? 3 : 1;
+    const long long nrows = a.ninst / C * N * N;
     hipLaunchKernelGGL(kern, dim3((unsigned)((nrows + G - 1) / G)), dim3(256), 0, st, a, F, P, nrows, kcut, rt);
     return hipGetLastError();
   };
-#define EXMI_ZPROD(NN, TT)                                                       \
-  case NN:                                                                       \
-    switch (a.kind) {                                                            \
-      case NL_GRADNORM: return go(zprod3d_kernel<NN, 8, NL_GRADNORM>, 256 / TT);  \
-      case NL_GENERAL: return go(zprod3d_kernel<NN, 8, NL_GENERAL>, 256 / TT);    \
-      case NL_POLY: return go(zprod3d_kernel<NN, 8, NL_POLY>, 256 / TT);          \
-      case NL_GRAY_SCOTT: return go(zprod3d_kernel<NN, 8, NL_GRAY_SCOTT>, 256 / TT); \
-      case NL_BZ: return go(zprod3d_kernel<NN, 8, NL_BZ>, 256 / TT);              \
-      default: return hipErrorInvalidValue;                                      \
+  // 8 points per lane (TT lanes per row); the convections, with 12 values
+  // per point in flight, 4 (2 TT lanes)
+#define EXMI_ZPROD(NN, TT)                                                                \
+  case NN:                                                                                \
+    switch (a.kind) {                                                                     \
+      case NL_GRADNORM: return go(zprod3d_kernel<NN, 8, NL_GRADNORM>, 256 / TT);           \
+      case NL_GENERAL: return go(zprod3d_kernel<NN, 8, NL_GENERAL>, 256 / TT);             \
+      case NL_POLY: return go(zprod3d_kernel<NN, 8, NL_POLY>, 256 / TT);                   \
+      case NL_GRAY_SCOTT: return go(zprod3d_kernel<NN, 8, NL_GRAY_SCOTT>, 256 / TT);       \
+      case NL_BZ: return go(zprod3d_kernel<NN, 8, NL_BZ>, 256 / TT);                       \
+      case NL_CONV_CONS: return go(zprod3d_kernel<NN, 4, NL_CONV_CONS>, 256 / (2 * TT));   \
+      case NL_CONV_NONCONS: return go(zprod3d_kernel<NN, 4, NL_CONV_NONCONS>, 256 / (2 * TT)); \
+      default: return hipErrorInvalidValue;                                               \
     }
   switch (N) {
     EXMI_ZPROD(16, 1)
@@ -1120,19 +1136,24 @@ hipError_t launch_xstage_proj3d(const NL3DArgs& na, const Stage3DArgs& a, const 
   }
 }
 
-// N_hat at one mode from the product spectra values ph0 (and ph1 for the
-// general form) of the kinds with fused z rows: combine_at's formulas for
-// these kinds, the kind fixed at compile time.
+// N_hat at one mode from the instance's product spectra values ph[0..NP) of
+// the kinds with fused z rows: combine_at's formulas for these kinds, the
+// kind fixed at compile time.
 template <int KIND>
-__device__ __forceinline__ cf combine_vals(const NL3DArgs& a, const Mode3D& m, cf ph0, cf ph1) {
+__device__ __forceinline__ cf combine_vals(const NL3DArgs& a, const Mode3D& m, const cf* ph) {
+  const cf ph0 = ph[0];
   cf r;
-  if constexpr (KIND == NL_POLY) {
+  if constexpr (KIND == NL_CONV_CONS) {
+    r = (-0.5f * a.scale) * (imul(m.kx, ph[0]) + imul(m.ky, ph[1]) + imul(m.kz, ph[2]));
+  } else if constexpr (KIND == NL_CONV_NONCONS) {
+    r = -a.scale * ph0;
+  } else if constexpr (KIND == NL_POLY) {
     r = a.poly_lap ? (a.scale * -(m.kx * m.kx + m.ky * m.ky + m.kz * m.kz)) * ph0 : ph0;
   } else if constexpr (KIND == NL_GRADNORM) {
     r = (a.zero_mode_fix && m.zero) ? cf{0.f, 0.f} : (-0.5f * a.scale) * ph0;
   } else if constexpr (KIND == NL_GENERAL) {
     const float cg = (a.zero_mode_fix && m.zero) ? 0.f : 0.5f * a.g2;
-    r = a.g0 * ph0 + imul(0.5f * a.g1 * (m.kx + m.ky + m.kz), ph0) + cg * ph1;
+    r = a.g0 * ph0 + imul(0.5f * a.g1 * (m.kx + m.ky + m.kz), ph0) + cg * ph[1];
   } else {
     r = ph0;  // the coupled reactions
   }
@@ -1140,7 +1161,8 @@ __device__ __forceinline__ cf combine_vals(const NL3DArgs& a, const Mode3D& m, c
 }
 
 // The forward x lines of the product spectra of the kinds with fused z rows
-// (gradient norm, polynomial, general form, coupled reactions) fused with
+// (gradient norm, polynomial, general form, coupled reactions, three-channel
+// convections) fused with
 // the combine and the stage, as xstage_proj3d_kernel does for the projected
 // convection: a lane group owns the line (instance, iy, kz) of the
 // instance's NP product spectra, transforms them along x (planes kz > kcut
@@ -1179,8 +1201,12 @@ __global__ void __launch_bounds__(256)
   }
   cfft<N, PT, false, NP>(v, ln, lds, tr);
 #pragma unroll
-  for (int s = 0; s < PT; ++s)
-    v[0][s] = combine_vals<KIND>(na, mode3d_at(na, j + s * T, iy, iz), v[0][s], v[NP - 1][s]);
+  for (int s = 0; s < PT; ++s) {
+    cf ph[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) ph[q] = v[q][s];
+    v[0][s] = combine_vals<KIND>(na, mode3d_at(na, j + s * T, iy, iz), ph);
+  }
   // the stage, transposed through LDS (thread (q0, cl) takes the points
   // ix = q0 + q T of the workgroup's line cl)
   const int cl = threadIdx.x % C::CPB, q0 = threadIdx.x / C::CPB;
@@ -1233,6 +1259,8 @@ hipError_t launch_xstage3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P
       case NL_POLY: return go(xstage3d_kernel<NN, NL_POLY>, XStageCfg<NN>::CPB);             \
       case NL_GRAY_SCOTT: return go(xstage3d_kernel<NN, NL_GRAY_SCOTT>, XStageCfg<NN>::CPB); \
       case NL_BZ: return go(xstage3d_kernel<NN, NL_BZ>, XStageCfg<NN>::CPB);                 \
+      case NL_CONV_CONS: return go(xstage3d_kernel<NN, NL_CONV_CONS>, XStageCfg<NN>::CPB);   \
+      case NL_CONV_NONCONS: return go(xstage3d_kernel<NN, NL_CONV_NONCONS>, XStageCfg<NN>::CPB); \
       default: return hipErrorInvalidValue;                                                  \
     }
   switch (na.N) {

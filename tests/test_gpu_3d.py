@@ -105,7 +105,7 @@ def _check(gpu, cpu, u0, device, label):
             assert e_step <= RTOL_STEP and e_roll <= RTOL_STEP
 
 
-@pytest.mark.parametrize("N", [16, 32])
+@pytest.mark.parametrize("N", [16, 32, 64])
 @pytest.mark.parametrize("conservative,order", [(False, 2), (True, 2), (False, 1), (True, 4), (False, 3)])
 def test_burgers_3d_parity(device, conservative, order, N):
     print(f"[case] 3-D Burgers conservative={conservative} order {order} N={N}", flush=True)
