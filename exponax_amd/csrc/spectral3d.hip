@@ -630,9 +630,14 @@ hipError_t launch_zcross3d(int N, const cf* F, cf* P, long long ntraj, long long
 //                                                  single-channel u sum_d d_d u: g0 (F[h sum_d g_d] + conj(sum_d D_d) F[h w])
 //   vorticity (2-D) -b (u omega_x + v omega_y) -> -b P (conj(D_y) Linv F[omega_x h] - conj(D_x) Linv F[omega_y h]
 //                                                        + conj(D_x) F[u h] + conj(D_y) F[v h])
+//   projected convection Pl P F[u x omega],    -> P (F[omega x h] + i kappa x F[h x u]),
+//   omega = irfftn(P i kappa x w_hat)             h = irfftn(Pl P mu) (the Leray projector Pl is
+//                                                  real and symmetric per mode; the cross product
+//                                                  with i kappa is its own conjugate transpose)
 // (oracle/adjoint.py restates each from the forward formulas of
 // nonlin_fun/_gradient_norm.py, _convection.py, _polynomial.py,
-// _general_nonlinear.py and _vorticity_convection.py:78-99.)
+// _general_nonlinear.py, _vorticity_convection.py:78-99 and
+// _projected_convection.py:19-136 with _leray.py:8-103.)
 int adj3d_fields(int kind, int dims, int poly_lap, int* nh, int* nq) {
   *nh = 0;
   *nq = 0;
@@ -647,6 +652,7 @@ int adj3d_fields(int kind, int dims, int poly_lap, int* nh, int* nq) {
       break;
     case NL_GENERAL: *nh = 3; *nq = 2 + dims; break;
     case NL_VORT: *nh = 1; *nq = 4; break;
+    case NL_PROJ3D: *nh = 1; *nq = 2; break;
     default: return 0;
   }
   return *nh;
@@ -669,6 +675,19 @@ __global__ void adj_cofields3d_kernel(NL3DArgs a, const cf* __restrict__ MU, cf*
         h[0] = mu;
         h[a.Nr] = imul(-(m.kx + m.ky + m.kz), mu);
         h[2 * a.Nr] = (a.zero_mode_fix && m.zero) ? cf{0.f, 0.f} : mu;
+        break;
+      }
+      case NL_PROJ3D: {
+        // the Leray projection of the trajectory's three masked cotangents
+        // (as in combine_at; the projector is its own transpose)
+        const long long b = (inst - inst % a.channels) * a.Nr + k;
+        const cf z{0.f, 0.f};
+        const cf m0 = m.keep ? MU[b] : z, m1 = m.keep ? MU[b + a.Nr] : z, m2 = m.keep ? MU[b + 2 * a.Nr] : z;
+        const cf div = imul(m.kx, m0) + imul(m.ky, m1) + imul(m.kz, m2);
+        const float k2 = m.kx * m.kx + m.ky * m.ky + m.kz * m.kz;
+        const cf pr = k2 != 0.f ? (1.f / k2) * div : z;
+        const int c = (int)(inst % a.channels);
+        h[0] = mu + imul(c == 0 ? m.kx : c == 1 ? m.ky : m.kz, pr);
         break;
       }
       default: h[0] = mu;
@@ -739,6 +758,13 @@ __global__ void adj_products3d_kernel(NL3DArgs a, const float* __restrict__ R, c
         q[3 * a.N3] = fld(inst, 2) * h;    // v h
         break;
       }
+      case NL_PROJ3D: {
+        // fields u_c (0) and omega_c (1) of the trajectory's channels
+        const long long c1 = base + (ch + 1) % 3, c2 = base + (ch + 2) % 3;
+        q[0] = fld(c1, 1) * cot(c2, 0) - fld(c2, 1) * cot(c1, 0);     // (omega x h)_c
+        q[a.N3] = cot(c1, 0) * fld(c2, 0) - cot(c2, 0) * fld(c1, 0);  // (h x u)_c
+        break;
+      }
       default: q[0] = 0.f;
     }
   }
@@ -773,6 +799,16 @@ __global__ void adj_combine3d_kernel(NL3DArgs a, const cf* __restrict__ QH, cf* 
         const float inv = lap == 0.f ? 1.f : 1.f / lap;
         r = imul(-ky, inv * q[0]) + imul(kx, inv * q[a.Nr]) + imul(-kx, q[2 * a.Nr]) + imul(-ky, q[3 * a.Nr]);
         r = -a.scale * r;
+        break;
+      }
+      case NL_PROJ3D: {
+        // F[omega x h]_c + (i kappa x F[h x u])_c, the second from the
+        // trajectory's other two channels
+        const int c = (int)(inst % a.channels);
+        const cf* q1 = QH + (inst - c) * a.nq * a.Nr + a.Nr + k;  // channel 0's F[h x u]
+        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        const float k1 = c1 == 0 ? m.kx : c1 == 1 ? m.ky : m.kz, kk2 = c2 == 0 ? m.kx : c2 == 1 ? m.ky : m.kz;
+        r = q[0] + imul(k1, q1[(long long)c2 * a.nq * a.Nr]) - imul(kk2, q1[(long long)c1 * a.nq * a.Nr]);
         break;
       }
       default: r = q[0];

@@ -255,13 +255,14 @@ int exmi_repeat(exmi_plan* plan, const float* u0, int64_t u0_bstride, float* u_o
  * g_out (not u_in). Inner products are the plain sums over grid points, as
  * jax.grad of a loss sum(step(u) * g_out) would give.
  * Built for ETDRK orders 0-4 in 1-D, 2-D and 3-D with no nonlinearity,
- * every convection form, the gradient norm, polynomials, the general form
- * and the 2-D vorticity (Kolmogorov) convection, real or complex L (the
+ * every convection form, the gradient norm, polynomials, the general form,
+ * the 2-D vorticity (Kolmogorov) convection and the projected 3-D
+ * (Navier-Stokes velocity) convection, real or complex L (the
  * transpose multiplies by conj(exp(L dt))): fused kernels for 1-D orders
  * 0-2 and 2-D gradient-norm ETDRK2, the multi-pass transpose otherwise (for
  * a fused plan through a multi-pass twin of the same stepper, built
- * synchronously on the first call). Cahn-Hilliard, the coupled reactions
- * and the projected 3-D convection return EXMI_ERR_UNSUPPORTED.
+ * synchronously on the first call). Cahn-Hilliard and the coupled
+ * reactions return EXMI_ERR_UNSUPPORTED.
  * Replaces jax.vjp of BaseStepper.__call__ (_base_stepper.py:241-271), i.e.
  * the transposes JAX derives for etdrk/_etdrk_2.py:91-102 and the nonlinear
  * functions (nonlin_fun/_convection.py, _gradient_norm.py). */

@@ -77,6 +77,21 @@ def nonlin_vjp(fun, u_hat, m_hat):
         h = -fun.convection_scale * fun.ifft(m_hat)
         return (np.conj(d[1:2] * inv) * fun.fft(wx * h) + np.conj(-d[0:1] * inv) * fun.fft(wy * h)
                 + np.conj(d[0:1]) * fun.fft(uu * h) + np.conj(d[1:2]) * fun.fft(vv * h))
+    if isinstance(fun, nl.ProjectedConvection3d):  # includes the Kolmogorov variant (constant forcing)
+        # N = P fft(u x w), u = ifft(u_hat), w = ifft(dop x u_hat), dop = i kappa.
+        # The Leray projector P is real and symmetric per mode, so P^T = P;
+        # the adjoint of fft (mask, then rfftn) is ifft (mask, then irfftn)
+        # and vice versa; d(u x w) = du x w + u x dw gives the cotangents
+        # w x h of u and h x u of w; the per-mode cross product with dop has
+        # entries e_{cja} dop_j, whose conjugate transpose is itself (dop is
+        # imaginary), so its adjoint is the same cross product.
+        ax = u_hat.ndim - 4
+        dop = fun.derivative_operator if ax == 0 else fun.derivative_operator[None]
+        dop = dop.astype(np.complex128)
+        u = fun.ifft(u_hat)
+        w = fun.ifft(nl._cross3(dop, u_hat, ax))
+        h = fun.ifft(fun.leray(m_hat))
+        return fun.fft(nl._cross3(w, h, ax)) + nl._cross3(dop, fun.fft(nl._cross3(h, u, ax)), ax)
     if isinstance(fun, nl.GradientNormNonlinearFun):
         d = _D(fun)
         grad = fun.ifft(d[None, :] * np.expand_dims(u_hat, -(D + 1)))

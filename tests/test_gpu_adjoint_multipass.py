@@ -132,11 +132,25 @@ def test_step_vjp_2d_kinds(device, kind):
     _vjp_case(gpu, cpu, u, g, f"2-D {kind}")
 
 
-@pytest.mark.parametrize("name,order", [("burgers", 2), ("burgers", 4), ("ks", 3)])
+@pytest.mark.parametrize("name,order", [("burgers", 2), ("burgers", 4), ("ks", 3), ("ns", 2), ("ns", 4),
+                                        ("ns_kolmogorov", 2)])
 def test_step_vjp_3d(device, name, order):
     print(f"[case] 3-D {name} vjp order {order}", flush=True)
     N = 16
-    if name == "burgers":
+    if name.startswith("ns"):
+        # the Leray-projected rotational convection (Navier-Stokes velocity;
+        # the Kolmogorov forcing is a constant and drops out of the VJP)
+        L = 2 * np.pi
+        kw = dict(diffusivity=0.02, order=order)
+        if name == "ns":
+            gpu = ex.stepper.NavierStokesVelocity(3, L, N, 0.01, **kw)
+        else:
+            kw.update(injection_mode=2, injection_scale=0.5, drag=-0.1)
+            gpu = ex.stepper.KolmogorovFlowVelocity(3, L, N, 0.01, **kw)
+        cpu = ost.NavierStokesVelocity(3, L, N, 0.01, **kw)
+        u, _ = ics(2, 3, L, N, 3, seed=77, channels=3)
+        g, _ = ics(2, 3, L, N, 6, seed=78, channels=3)
+    elif name == "burgers":
         gpu = ex.stepper.Burgers(3, 1.0, N, 1e-3, diffusivity=0.02, order=order)
         cpu = ost.Burgers(3, 1.0, N, 1e-3, diffusivity=0.02, order=order)
         u, _ = ics(2, 3, 1.0, N, 3, seed=71, channels=3)
@@ -187,15 +201,19 @@ def test_adjoint_unsupported_kinds_fail_loudly(device):
 
 
 def test_rollout_vjp_3d_and_any_size(device):
-    """Checkpointed rollout gradients of multi-pass plans: 3-D KS (N = 16)
-    and 1-D Burgers at N = 100, against the oracle's reverse pass."""
+    """Checkpointed rollout gradients of multi-pass plans: 3-D KS (N = 16),
+    1-D Burgers at N = 100 and 3-D Navier-Stokes velocity (N = 16), against
+    the oracle's reverse pass."""
     import torch
 
     print("[case] rollout vjp 3-D and N=100", flush=True)
     cases = [(ex.stepper.KuramotoSivashinsky(3, 20.0, 16, 0.05), ost.KuramotoSivashinsky(3, 20.0, 16, 0.05),
               ics(2, 3, 20.0, 16, 3, seed=83)[0], ics(2, 3, 20.0, 16, 6, seed=84)[0]),
              (ex.stepper.Burgers(1, 1.0, 100, 1e-3, diffusivity=0.01), ost.Burgers(1, 1.0, 100, 1e-3, diffusivity=0.01),
-              ics(2, 1, 1.0, 100, 5, seed=85)[0], ics(2, 1, 1.0, 100, 10, seed=86)[0])]
+              ics(2, 1, 1.0, 100, 5, seed=85)[0], ics(2, 1, 1.0, 100, 10, seed=86)[0]),
+             (ex.stepper.NavierStokesVelocity(3, 2 * np.pi, 16, 0.01, diffusivity=0.02),
+              ost.NavierStokesVelocity(3, 2 * np.pi, 16, 0.01, diffusivity=0.02),
+              ics(2, 3, 2 * np.pi, 16, 3, seed=87, channels=3)[0], ics(2, 3, 2 * np.pi, 16, 6, seed=88, channels=3)[0])]
     for gpu, cpu, u0, gf in cases:
         n = 7
         x = torch.tensor(u0, device="cuda")

@@ -55,6 +55,12 @@ def _nonlin_cases():
         ("kolmogorov", 2, 16, 1, nl.VorticityConvection2dKolmogorov(2, 16, derivative_operator=d2,
                                                                     dealiasing_fraction=2 / 3)),
     ]
+    d3 = sp.build_derivative_operator(3, 2 * np.pi, 8)
+    out += [
+        ("projected3d", 3, 8, 3, nl.ProjectedConvection3d(3, 8, derivative_operator=d3, dealiasing_fraction=2 / 3)),
+        ("projected3d_kolmogorov", 3, 8, 3, nl.ProjectedConvection3dKolmogorov(
+            3, 8, injection_mode=1, injection_scale=0.7, derivative_operator=d3, dealiasing_fraction=2 / 3)),
+    ]
     return out
 
 
