@@ -1587,7 +1587,10 @@ static bool twin_kind_ok(const exmi_plan_desc* d) {
     case EXMI_NL_POLYNOMIAL:
     case EXMI_NL_GENERAL:
     case EXMI_NL_VORTICITY:
-    case EXMI_NL_KOLMOGOROV: return true;
+    case EXMI_NL_KOLMOGOROV:
+    case EXMI_NL_CAHN_HILLIARD:
+    case EXMI_NL_GRAY_SCOTT:
+    case EXMI_NL_BELOUSOV_ZHABOTINSKY: return true;
     default: return false;
   }
 }
@@ -1792,7 +1795,8 @@ int exmi_plan_create(const exmi_plan_desc* d, exmi_plan** out) {
     delete p;
     return fail(EXMI_ERR_UNSUPPORTED, "no kernel for N=%d kind=%d order=%d", N, kind, d->order);
   }
-  // the Laplacian of Cahn-Hilliard's cubic has no transpose kernel here
+  // the fused 1-D transpose has no Laplacian of Cahn-Hilliard's cubic (the
+  // multi-pass twin covers it)
   p->vjp = d->order <= 2 && d->nonlin_kind != EXMI_NL_CAHN_HILLIARD
                ? get_vjp1d(p->M, d->order == 0 ? NL_ZERO : kind, d->order, cplx ? 1 : 0)
                : nullptr;
@@ -3253,9 +3257,9 @@ static int ensure_work(exmi_plan* p) {
 static int adjoint_unsupported() {
   return fail(EXMI_ERR_UNSUPPORTED,
               "no adjoint for this plan (built for ETDRK orders 0-4 in 1-D, 2-D and 3-D with no nonlinearity, "
-              "convection, the gradient norm, the polynomial reactions, the general form or the 2-D vorticity "
-              "convection, real or complex L; not for Cahn-Hilliard, the coupled reactions and the projected "
-              "3-D convection)");
+              "convection, the gradient norm, the polynomial reactions, Cahn-Hilliard, the coupled reactions, "
+              "the general form, the 2-D vorticity convection or the projected 3-D convection, real or "
+              "complex L)");
 }
 
 int exmi_step_vjp(exmi_plan* p, const float* u, int64_t u_bs, const float* g, int64_t g_bs, float* g_in,

@@ -625,6 +625,9 @@ hipError_t launch_zcross3d(int N, const cf* F, cf* P, long long ntraj, long long
 //                                                  h_cj = irfftn(P conj(D_j) mu_c)
 //   convection, u . grad u                     -> -b P (F[sum_c h_c d_a w_c] + sum_j conj(D_j) F[h_a w_j])
 //   polynomial                                 -> P F[p'(w) h]
+//   Cahn-Hilliard s Lap P F[w^3]                 -> P F[3 w^2 irfftn(s Lap P mu)]
+//   Gray-Scott, Belousov-Zhabotinsky           -> P F[sum_j dN_j/dw_c h_j] (the reactions'
+//                                                  pointwise Jacobians, transposed)
 //   general                                    -> P (F[w (2 g0 h + g1 hd)] + g2 sum_d conj(D_d) F[g_d hz]),
 //                                                  hd = irfftn(P conj(sum_d D_d) mu), hz = irfftn(Z P mu);
 //                                                  single-channel u sum_d d_d u: g0 (F[h sum_d g_d] + conj(sum_d D_d) F[h w])
@@ -639,14 +642,16 @@ hipError_t launch_zcross3d(int N, const cf* F, cf* P, long long ntraj, long long
 // _general_nonlinear.py, _vorticity_convection.py:78-99 and
 // _projected_convection.py:19-136 with _leray.py:8-103.)
 int adj3d_fields(int kind, int dims, int poly_lap, int* nh, int* nq) {
+  (void)poly_lap;
   *nh = 0;
   *nq = 0;
   switch (kind) {
     case NL_GRADNORM: *nh = 1; *nq = dims; break;
     case NL_CONV_CONS: *nh = dims; *nq = 1; break;
     case NL_CONV_NONCONS: *nh = 1; *nq = 1 + dims; break;
-    case NL_POLY:
-      if (poly_lap) return 0;
+    case NL_POLY:  // Cahn-Hilliard (poly_lap) included
+    case NL_GRAY_SCOTT:
+    case NL_BZ:
       *nh = 1;
       *nq = 1;
       break;
@@ -668,6 +673,9 @@ __global__ void adj_cofields3d_kernel(NL3DArgs a, const cf* __restrict__ MU, cf*
     cf* h = H + inst * a.nh * a.Nr + k;
     switch (a.kind) {
       case NL_GRADNORM: h[0] = (a.zero_mode_fix && m.zero) ? cf{0.f, 0.f} : mu; break;
+      case NL_POLY:  // Cahn-Hilliard: the (real) scale times Laplacian first
+        h[0] = a.poly_lap ? (a.scale * -(m.kx * m.kx + m.ky * m.ky + m.kz * m.kz)) * mu : mu;
+        break;
       case NL_CONV_CONS:
         for (int j = 0; j < a.dims; ++j) h[j * a.Nr] = imul(-kap_d(a, m, j), mu);
         break;
@@ -756,6 +764,19 @@ __global__ void adj_products3d_kernel(NL3DArgs a, const float* __restrict__ R, c
         q[a.N3] = fld(inst, 3) * h;        // omega_y h
         q[2 * a.N3] = fld(inst, 0) * h;    // u h
         q[3 * a.N3] = fld(inst, 2) * h;    // v h
+        break;
+      }
+      case NL_GRAY_SCOTT: {
+        // N0 = f (1 - u0) - u0 u1^2, N1 = -(f + k) u1 + u0 u1^2
+        const float u0 = fld(base, 0), u1 = fld(base + 1, 0), h0 = cot(base, 0), h1 = cot(base + 1, 0);
+        q[0] = ch == 0 ? fmaf(h1 - h0, u1 * u1, -a.feed * h0) : fmaf(2.f * u0 * u1, h1 - h0, -(a.feed + a.kill) * h1);
+        break;
+      }
+      case NL_BZ: {
+        // N0 = u0 + u1 - u0 u1 - u0^2, N1 = u2 - u1 - u0 u1, N2 = u0 - u2
+        const float u0 = fld(base, 0), u1 = fld(base + 1, 0);
+        const float h0 = cot(base, 0), h1 = cot(base + 1, 0), h2 = cot(base + 2, 0);
+        q[0] = ch == 0 ? h0 * (1.f - u1 - 2.f * u0) - h1 * u1 + h2 : ch == 1 ? h0 * (1.f - u0) - h1 * (1.f + u0) : h1 - h2;
         break;
       }
       case NL_PROJ3D: {

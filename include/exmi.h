@@ -261,8 +261,8 @@ int exmi_repeat(exmi_plan* plan, const float* u0, int64_t u0_bstride, float* u_o
  * transpose multiplies by conj(exp(L dt))): fused kernels for 1-D orders
  * 0-2 and 2-D gradient-norm ETDRK2, the multi-pass transpose otherwise (for
  * a fused plan through a multi-pass twin of the same stepper, built
- * synchronously on the first call). Cahn-Hilliard and the coupled
- * reactions return EXMI_ERR_UNSUPPORTED.
+ * synchronously on the first call), Cahn-Hilliard and the coupled
+ * reactions included.
  * Replaces jax.vjp of BaseStepper.__call__ (_base_stepper.py:241-271), i.e.
  * the transposes JAX derives for etdrk/_etdrk_2.py:91-102 and the nonlinear
  * functions (nonlin_fun/_convection.py, _gradient_norm.py). */

@@ -16,10 +16,12 @@ removing the mean is self-adjoint. The gradient of a loss l(u) is then
 irfftn(VJP_spectral(rfftn(dl/du))).
 
 Pins, in tests/test_oracle_adjoint.py:
-- JVP: every nonlinear function here is a polynomial of degree <= 2 in
-  u_hat (plus a constant for the Kolmogorov forcing). So the central
-  difference (N(u + d) - N(u - d)) / 2 is its exact derivative along d up to
-  rounding. ``nonlin_jvp`` uses that, and the step JVP composes it.
+- JVP: every nonlinear function here is a polynomial of degree <= 3 in
+  u_hat (plus a constant: the Kolmogorov forcing, the Gray-Scott feed). The
+  central difference D(h) = (N(u + h d) - N(u - h d)) / (2h) of such a
+  function is J d + h^2 T with T independent of h, so (4 D(1/2) - D(1)) / 3
+  is its exact derivative along d up to rounding. ``nonlin_jvp`` uses that,
+  and the step JVP composes it.
 - VJP: pinned by the dot-product identity <J v, w> = <v, J^T w> against that
   JVP, per nonlinear function and per step.
 - End to end: the gradient of MSE(u_n, target) is checked against float64
@@ -42,8 +44,10 @@ def inner(a_hat, b_hat, num_spatial_dims, num_points):
 
 # ------------------------------------------------------------ nonlinear terms
 def nonlin_jvp(fun, u_hat, d_hat):
-    """J(u_hat) d_hat, exact for the quadratic nonlinear functions."""
-    return 0.5 * (fun(u_hat + d_hat) - fun(u_hat - d_hat))
+    """J(u_hat) d_hat, exact for nonlinear functions of degree <= 3."""
+    d1 = 0.5 * (fun(u_hat + d_hat) - fun(u_hat - d_hat))
+    d2 = fun(u_hat + 0.5 * d_hat) - fun(u_hat - 0.5 * d_hat)
+    return (4.0 * d2 - d1) / 3.0
 
 
 def _D(fun):
@@ -92,6 +96,36 @@ def nonlin_vjp(fun, u_hat, m_hat):
         w = fun.ifft(nl._cross3(dop, u_hat, ax))
         h = fun.ifft(fun.leray(m_hat))
         return fun.fft(nl._cross3(w, h, ax)) + nl._cross3(dop, fun.fft(nl._cross3(h, u, ax)), ax)
+    if isinstance(fun, nl.CahnHilliardNonlinearFun):
+        # N = s Lap fft(u_0^3), u = ifft(dealias(u_hat)): the adjoint of fft
+        # is ifft and that of ifft(dealias(.)) is fft (the mask is binary);
+        # Lap is real
+        ax = u_hat.ndim - D - 1
+        u0 = np.take(fun.ifft(fun.dealias(u_hat)), [0], axis=ax)
+        h = fun.ifft(fun.scale * fun.laplace_operator.astype(np.complex128) * m_hat)
+        g = np.zeros(u_hat.shape[:ax] + (u_hat.shape[ax],) + u0.shape[ax + 1:])
+        idx = [slice(None)] * g.ndim
+        idx[ax] = slice(0, 1)
+        g[tuple(idx)] = 3.0 * u0**2 * h
+        return fun.fft(g)
+    if isinstance(fun, nl.GrayScottNonlinearFun):
+        # N0 = f (1 - u0) - u0 u1^2, N1 = -(f + k) u1 + u0 u1^2
+        ax = u_hat.ndim - D - 1
+        u = fun.ifft(fun.dealias(u_hat))
+        h = fun.ifft(m_hat)
+        u0, u1 = np.take(u, 0, axis=ax), np.take(u, 1, axis=ax)
+        h0, h1 = np.take(h, 0, axis=ax), np.take(h, 1, axis=ax)
+        f, k = fun.feed_rate, fun.kill_rate
+        return fun.fft(np.stack([(h1 - h0) * u1**2 - f * h0, 2.0 * u0 * u1 * (h1 - h0) - (f + k) * h1], axis=ax))
+    if isinstance(fun, nl.BelousovZhabotinskyNonlinearFun):
+        # N0 = u0 + u1 - u0 u1 - u0^2, N1 = u2 - u1 - u0 u1, N2 = u0 - u2
+        ax = u_hat.ndim - D - 1
+        u = fun.ifft(fun.dealias(u_hat))
+        h = fun.ifft(m_hat)
+        u0, u1 = np.take(u, 0, axis=ax), np.take(u, 1, axis=ax)
+        h0, h1, h2 = (np.take(h, i, axis=ax) for i in range(3))
+        return fun.fft(np.stack([h0 * (1.0 - u1 - 2.0 * u0) - h1 * u1 + h2, h0 * (1.0 - u0) - h1 * (1.0 + u0),
+                                 h1 - h2], axis=ax))
     if isinstance(fun, nl.GradientNormNonlinearFun):
         d = _D(fun)
         grad = fun.ifft(d[None, :] * np.expand_dims(u_hat, -(D + 1)))

@@ -1,7 +1,8 @@
 """GPU parity of the multi-pass adjoint (the transposed ETDRK0-4 step on whole
 spectra): 1-D ETDRK3/4 (through the multi-pass twin of a fused plan), any
 grid size, the 2-D vorticity (Kolmogorov) convection and the other 2-D
-kinds, and 3-D, against the oracle's float64 VJP (oracle/adjoint.py, pinned
+kinds, 3-D (Navier-Stokes velocity included), the coupled reactions and
+Cahn-Hilliard, against the oracle's float64 VJP (oracle/adjoint.py, pinned
 by the dot-product identity and central differences). Per step rel-L2
 <= 1e-5; through a checkpointed rollout <= 1e-4."""
 import numpy as np
@@ -189,15 +190,32 @@ def test_rollout_vjp_kolmogorov_and_autograd(device):
     assert _rel(xr.grad.cpu().numpy(), ref) <= RTOL_GRAD
 
 
-def test_adjoint_unsupported_kinds_fail_loudly(device):
-    """Cahn-Hilliard and the coupled reactions have no transpose here: a
-    NotImplementedError, not a wrong gradient."""
-    import torch
-
-    print("[case] adjoint unsupported kinds", flush=True)
-    gs = ex.stepper.reaction.GrayScott(1, 2.5, 64, 1.0)
-    with pytest.raises(NotImplementedError):
-        gs.step_vjp(torch.zeros(1, 2, 64, device="cuda"), torch.zeros(1, 2, 64, device="cuda"))
+@pytest.mark.parametrize("name,D", [("gray_scott", 1), ("gray_scott", 2), ("gray_scott", 3), ("bz", 1), ("bz", 2),
+                                    ("cahn_hilliard", 1), ("cahn_hilliard", 2), ("cahn_hilliard", 3)])
+def test_step_vjp_reactions(device, name, D):
+    """The coupled reactions (stepper/reaction/_gray_scott.py:9-49,
+    _belousov_zhabotinsky.py:9-47) and Cahn-Hilliard's Laplacian of the cube
+    (_cahn_hilliard.py:10-41): the transposed pointwise Jacobians, through
+    the multi-pass adjoint (a twin of the fused plan in 1-D and 2-D)."""
+    print(f"[case] {D}-D {name} vjp", flush=True)
+    N = {1: 128, 2: 32, 3: 16}[D]
+    if name == "gray_scott":
+        kw = dict(diffusivity_1=2e-3, diffusivity_2=1e-3, feed_rate=0.04, kill_rate=0.06)
+        gpu = ex.stepper.reaction.GrayScott(D, 1.0, N, 0.5, **kw)
+        cpu = ost.GrayScott(D, 1.0, N, 0.5, **kw)
+        ch, L = 2, 1.0
+    elif name == "bz":
+        kw = dict(diffusivities=(1e-3, 2e-3, 1e-3))
+        gpu = ex.stepper.reaction.BelousovZhabotinsky(D, 1.0, N, 0.01, **kw)
+        cpu = ost.BelousovZhabotinsky(D, 1.0, N, 0.01, **kw)
+        ch, L = 3, 1.0
+    else:
+        gpu = ex.stepper.reaction.CahnHilliard(D, 2.0, N, 1e-3)
+        cpu = ost.CahnHilliard(D, 2.0, N, 1e-3)
+        ch, L = 1, 2.0
+    u, _ = ics(2, D, L, N, 4, seed=79, channels=ch)
+    g, _ = ics(2, D, L, N, 8, seed=80, channels=ch)
+    _vjp_case(gpu, cpu, u, g, f"{D}-D {name}")
 
 
 def test_rollout_vjp_3d_and_any_size(device):

@@ -55,6 +55,19 @@ def _nonlin_cases():
         ("kolmogorov", 2, 16, 1, nl.VorticityConvection2dKolmogorov(2, 16, derivative_operator=d2,
                                                                     dealiasing_fraction=2 / 3)),
     ]
+    for D in (1, 2):
+        N = 32 if D == 1 else 16
+        d = sp.build_derivative_operator(D, 3.0, N)
+        out += [
+            (f"allen_cahn_cubic{D}", D, N, 1, nl.PolynomialNonlinearFun(D, N, dealiasing_fraction=2 / 3,
+                                                                        coefficients=(0.0, 1.0, 0.0, -1.0))),
+            (f"cahn_hilliard{D}", D, N, 1, nl.CahnHilliardNonlinearFun(D, N, derivative_operator=d, scale=0.6,
+                                                                       dealiasing_fraction=2 / 3)),
+            (f"gray_scott{D}", D, N, 2, nl.GrayScottNonlinearFun(D, N, dealiasing_fraction=2 / 3, feed_rate=0.04,
+                                                                 kill_rate=0.06)),
+            (f"belousov_zhabotinsky{D}", D, N, 3, nl.BelousovZhabotinskyNonlinearFun(D, N,
+                                                                                    dealiasing_fraction=2 / 3)),
+        ]
     d3 = sp.build_derivative_operator(3, 2 * np.pi, 8)
     out += [
         ("projected3d", 3, 8, 3, nl.ProjectedConvection3d(3, 8, derivative_operator=d3, dealiasing_fraction=2 / 3)),
