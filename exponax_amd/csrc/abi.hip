@@ -57,6 +57,8 @@ hipError_t launch_xstage_proj3d(const NL3DArgs& na, const Stage3DArgs& a, const 
                                 const cf* tw, hipStream_t st);
 hipError_t launch_xstage3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P, cf* N0out, int kcut, const cf* tw,
                            hipStream_t st);
+hipError_t launch_fields_unit_lines3d(const NL3DArgs& a, const cf* S, cf* F, float scale, const cf* tw, int kkeep,
+                                      hipStream_t st);
 hipError_t launch_adj_cofields3d(const NL3DArgs& a, const cf* MU, cf* H, hipStream_t st);
 hipError_t launch_adj_products3d(const NL3DArgs& a, const float* R, const float* HR, float* Q, hipStream_t st);
 hipError_t launch_adj_combine3d(const NL3DArgs& a, const cf* QH, cf* out, hipStream_t st);
@@ -1172,7 +1174,13 @@ static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf
   const int kz = p->kmax < (int)NZ - 1 ? (p->kmax < 0 ? 0 : p->kmax) : (int)NZ - 1;  // last kept plane
   const float r = 1.f / (float)N;
   const cf* tl = p->tw_col.tw;
-  hipError_t e = launch_fields_lines3d(a, X, p->F3, r, tl, kz + 1, st);
+  // the fields with the inverse x lines: one lane group per trajectory or
+  // instance line where the fields share their spectra (each spectrum value
+  // read once), else per field line
+  const int k3 = p->kind3;
+  const bool unit = N <= 128 && (k3 == NL_PROJ3D || k3 == NL_GRADNORM || k3 == NL_GENERAL || k3 == NL_CONV_NONCONS);
+  hipError_t e = unit ? launch_fields_unit_lines3d(a, X, p->F3, r, tl, kz + 1, st)
+                      : launch_fields_lines3d(a, X, p->F3, r, tl, kz + 1, st);
   if (e == hipSuccess) e = launch_lines(N, true, p->F3, I * N * NZ, NZ, N * NZ, NZ, r, tl, st, nullptr, kz + 1);
   if (p->kind3 == NL_PROJ3D) {
     // the z rows with the cross products fused (zcross3d_kernel), the

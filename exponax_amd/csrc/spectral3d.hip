@@ -29,6 +29,16 @@ struct LineCfg {
   static_assert(T <= 64, "a line transform must fit one wavefront");
 };
 
+// The fused x-line kernels that carry several lines per lane group or a
+// stage (xstage*, fields_unit_lines): 8 points per lane from N = 64 on,
+// twice the wavefronts of the line kernels' 16.
+template <int N>
+struct XStageCfg {
+  static constexpr int PT = N >= 64 ? 8 : LineCfg<N>::PT;
+  static constexpr int T = N / PT;
+  static constexpr int CPB = 256 / T;
+};
+
 // Line l = (outer o, inner i) with o = l / inner, i = l % inner starts at
 // o * outer_stride + i; its points are pstride apart. Read from src, written
 // to data at the same offsets (in place when src == data), times scale.
@@ -870,6 +880,115 @@ static unsigned grid_of(long long total) {
   return (unsigned)(g > 65536 ? 65536 : g);
 }
 
+// The fields pass fused with the inverse x lines, one lane group per unit
+// line: a unit is a trajectory for the projected convection (its three
+// channels' spectra in, the six fields u_c, omega_c out) and an instance for
+// the kinds whose fields are derivatives of the one spectrum (gradient
+// norm: d_d u; non-conservative convection and the general form: u and
+// d_d u). Each spectrum value is loaded once for all the unit's fields
+// (fields_lines_kernel loads it once per field); the NOUT lines are then
+// transformed together. Lines with iz >= kkeep do not run.
+template <int KIND>
+struct FieldUnit {
+  static constexpr int CIN = KIND == NL_PROJ3D ? 3 : 1;  // spectra read per point
+  static constexpr int NOUT = KIND == NL_PROJ3D ? 6 : KIND == NL_GRADNORM ? 3 : 4;
+};
+
+template <int N, int KIND>
+__global__ void __launch_bounds__(256) fields_unit_lines_kernel(NL3DArgs a, const cf* __restrict__ S,
+                                                                cf* __restrict__ F, float scale,
+                                                                const cf* __restrict__ tw, int kkeep) {
+  using C = XStageCfg<N>;
+  using U = FieldUnit<KIND>;
+  constexpr int PT = C::PT, T = C::T, CIN = U::CIN, NOUT = U::NOUT;
+  constexpr int LDSN = LdsSize<N>::value;
+  __shared__ cf lds_x[C::CPB * LDSN];
+  const int c = threadIdx.x / T, j = threadIdx.x % T;
+  const int NZ = N / 2 + 1;
+  const long long inner = (long long)N * NZ;
+  const long long nrun = a.ninst / CIN * N * kkeep;
+  const long long raw = (long long)blockIdx.x * C::CPB + c;
+  const bool active = raw < nrun;
+  const long long run = active ? raw : 0;  // inactive groups mirror line 0 and store nothing
+  const long long unit = run / ((long long)N * kkeep);
+  const int rr = (int)(run - unit * N * kkeep);
+  const int iy = rr / kkeep, iz = rr - iy * kkeep;
+  const long long li = (long long)iy * NZ + iz;
+  cf* lds = lds_x + c * LDSN;
+  const Lane<N, PT> ln(j);
+  TwRegs<N, PT> tr;
+  tr.template load_pass<1>(j, tw);
+  cf v[NOUT][PT];
+  const cf* src = S + unit * CIN * a.Nr + li;
+#pragma unroll
+  for (int s = 0; s < PT; ++s) {
+    const int ix = j + s * T;
+    const Mode3D m = mode3d_at(a, ix, iy, iz);
+    const cf z{0.f, 0.f};
+    if constexpr (KIND == NL_PROJ3D) {
+      const cf w0 = m.keep ? src[(long long)ix * inner] : z, w1 = m.keep ? src[a.Nr + (long long)ix * inner] : z,
+               w2 = m.keep ? src[2 * a.Nr + (long long)ix * inner] : z;
+      // field 2c + f of channel c: u_c, omega_c = (i kappa x u_hat)_c
+      v[0][s] = w0;
+      v[1][s] = imul(m.ky, w2) - imul(m.kz, w1);
+      v[2][s] = w1;
+      v[3][s] = imul(m.kz, w0) - imul(m.kx, w2);
+      v[4][s] = w2;
+      v[5][s] = imul(m.kx, w1) - imul(m.ky, w0);
+    } else {
+      const cf w = m.keep ? src[(long long)ix * inner] : z;
+      if constexpr (KIND == NL_GRADNORM) {
+        v[0][s] = imul(m.kx, w);
+        v[1][s] = imul(m.ky, w);
+        v[2][s] = imul(m.kz, w);
+      } else {  // u and its three derivatives
+        v[0][s] = w;
+        v[1][s] = imul(m.kx, w);
+        v[2][s] = imul(m.ky, w);
+        v[3][s] = imul(m.kz, w);
+      }
+    }
+  }
+  cfft<N, PT, true, NOUT>(v, ln, lds, tr);
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < NOUT; ++q) {
+      cf* o = F + (unit * NOUT + q) * a.Nr + li;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) o[(long long)(j + s * T) * inner] = scale * v[q][s];
+    }
+  }
+}
+
+// The unit-line fields pass for the kinds it covers (3-D, N <= 128);
+// hipErrorInvalidValue for the others.
+hipError_t launch_fields_unit_lines3d(const NL3DArgs& a, const cf* S, cf* F, float scale, const cf* tw, int kkeep,
+                                      hipStream_t st) {
+  if (a.dims != 3) return hipErrorInvalidValue;
+  auto go = [&](auto kern, int cin, int cpb) {
+    const long long nrun = a.ninst / cin * (long long)a.N * kkeep;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((nrun + cpb - 1) / cpb)), dim3(256), 0, st, a, S, F, scale, tw, kkeep);
+    return hipGetLastError();
+  };
+#define EXMI_FUNIT(NN)                                                                                  \
+  case NN:                                                                                              \
+    switch (a.kind) {                                                                                   \
+      case NL_PROJ3D: return go(fields_unit_lines_kernel<NN, NL_PROJ3D>, 3, XStageCfg<NN>::CPB);         \
+      case NL_GRADNORM: return go(fields_unit_lines_kernel<NN, NL_GRADNORM>, 1, XStageCfg<NN>::CPB);     \
+      case NL_GENERAL: return go(fields_unit_lines_kernel<NN, NL_GENERAL>, 1, XStageCfg<NN>::CPB);       \
+      case NL_CONV_NONCONS: return go(fields_unit_lines_kernel<NN, NL_CONV_NONCONS>, 1, XStageCfg<NN>::CPB); \
+      default: return hipErrorInvalidValue;                                                             \
+    }
+  switch (a.N) {
+    EXMI_FUNIT(16)
+    EXMI_FUNIT(32)
+    EXMI_FUNIT(64)
+    EXMI_FUNIT(128)
+    default: return hipErrorInvalidValue;
+  }
+#undef EXMI_FUNIT
+}
+
 hipError_t launch_fields_lines3d(const NL3DArgs& a, const cf* S, cf* F, float scale, const cf* tw, int kkeep,
                                  hipStream_t st) {
   auto go = [&](auto kern, int cpb) {
@@ -1080,14 +1199,6 @@ __global__ void stage_nl3d_proj_kernel(NL3DArgs na, Stage3DArgs a, const cf* __r
 // point the projection and the stage of stage_nl3d_proj_kernel. The state
 // buffers come in as separate non-aliasing arguments so that the stage's
 // loads of later points need not wait for the stores of earlier ones.
-// 8 points per lane from N = 64 on: twice the wavefronts of the line
-// kernels' 16, for the latency of the stage's accesses.
-template <int N>
-struct XStageCfg {
-  static constexpr int PT = N >= 64 ? 8 : LineCfg<N>::PT;
-  static constexpr int T = N / PT;
-  static constexpr int CPB = 256 / T;
-};
 
 template <int N>
 __global__ void __launch_bounds__(256)
