@@ -59,6 +59,8 @@ hipError_t launch_xstage3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P
                            hipStream_t st);
 hipError_t launch_fields_unit_lines3d(const NL3DArgs& a, const cf* S, cf* F, float scale, const cf* tw, int kkeep,
                                       hipStream_t st);
+hipError_t launch_yz_inverse3d(int N, const cf* X, float* x, long long nplanes, const cf* tw, const RealTables& rt,
+                               long long rpg, long long gstride, hipStream_t st);
 hipError_t launch_adj_cofields3d(const NL3DArgs& a, const cf* MU, cf* H, hipStream_t st);
 hipError_t launch_adj_products3d(const NL3DArgs& a, const float* R, const float* HR, float* Q, hipStream_t st);
 hipError_t launch_adj_combine3d(const NL3DArgs& a, const cf* QH, cf* out, hipStream_t st);
@@ -849,6 +851,8 @@ static hipError_t irfft3(int N, long long I, cf* X, float* x, const Twiddles& tz
   const long long NZ = N / 2 + 1;
   const float r = 1.f / (float)N;
   hipError_t e = launch_lines(N, true, X, I * N * NZ, N * NZ, (long long)N * N * NZ, N * NZ, r, tl.tw, st, src);
+  // up to N = 64 the y lines and the z c2r run as one plane kernel
+  if (N <= 64) return e == hipSuccess ? launch_yz_inverse3d(N, X, x, I * N, tl.tw, tz.rt(), rpg, gstride, st) : e;
   if (e == hipSuccess) e = launch_lines(N, true, X, I * N * NZ, NZ, N * NZ, NZ, r, tl.tw, st);
   if (e == hipSuccess)
     e = launch_fft1d(N / 2, true, (const float*)X, x, (int)(I * N * N), tz.rt(), st, 1 << 30, rpg, gstride);

@@ -880,6 +880,90 @@ static unsigned grid_of(long long total) {
   return (unsigned)(g > 65536 ? 65536 : g);
 }
 
+// The last two passes of irfftn for N <= 64, fused: one workgroup owns the
+// plane (instance, ix) after the inverse x lines, runs its N/2 + 1 inverse
+// y lines (times 1/N) into LDS, then its N c2r rows along z (times 1/N)
+// from LDS into x; row t = plane * N + iy of the output starts at
+// (t / rpg) gstride + (t % rpg) N (rpg > 0: a strided batch such as a
+// trajectory buffer) or t N. Saves the y pass's store and the z pass's
+// load of the plane.
+template <int N>
+struct YZCfg {
+  static constexpr int M = N / 2, NZ = M + 1;
+  static constexpr int PT1 = LineCfg<N>::PT, T1 = N / PT1, CPB1 = 256 / T1;  // y lines
+  static constexpr int PT2 = 8, T2 = M / PT2, G2 = 256 / T2;                    // z rows
+  static constexpr int L1 = CPB1 * LdsSize<N>::value, L2 = G2 * LdsSize<M>::value;
+  static constexpr int SCR = L1 > L2 ? L1 : L2;
+  static_assert(NZ <= CPB1 && N <= G2, "one y line per lane group and one z row per group");
+};
+
+template <int N>
+__global__ void __launch_bounds__(256) yz_inverse3d_kernel(const cf* __restrict__ X, float* __restrict__ x,
+                                                           long long nplanes, const cf* __restrict__ tw,
+                                                           RealTables rt, long long rpg, long long gstride) {
+  using C = YZCfg<N>;
+  constexpr int M = C::M, NZ = C::NZ, PT1 = C::PT1, T1 = C::T1, PT2 = C::PT2, T2 = C::T2;
+  __shared__ cf scr[C::SCR];
+  __shared__ cf Y[N * NZ];
+  const long long plane = blockIdx.x;
+  const cf* src = X + plane * N * NZ;
+  constexpr float rn = 1.f / (float)N;
+  {
+    // the inverse y line kz = c of the plane
+    const int c = threadIdx.x / T1, j = threadIdx.x % T1;
+    const int kz = c < NZ ? c : 0;  // groups past the last line mirror line 0 and store nothing
+    const Lane<N, PT1> ln(j);
+    TwRegs<N, PT1> tr;
+    tr.template load_pass<1>(j, tw);
+    cf v[1][PT1];
+#pragma unroll
+    for (int s = 0; s < PT1; ++s) v[0][s] = src[(j + s * T1) * NZ + kz];
+    cfft<N, PT1, true, 1>(v, ln, scr + c * LdsSize<N>::value, tr);
+    if (c < NZ) {
+#pragma unroll
+      for (int s = 0; s < PT1; ++s) Y[(j + s * T1) * NZ + kz] = rn * v[0][s];
+    }
+  }
+  __syncthreads();
+  {
+    // the c2r row iy = g along z
+    const int g = threadIdx.x / T2, j = threadIdx.x % T2;
+    const int iy = g < N ? g : 0;  // groups past the last row mirror row 0 and store nothing
+    const Lane<M, PT2> ln(j);
+    TwRegs<M, PT2> tr;
+    tr.load(j, rt);
+    cf v[1][PT2];
+    cf nyq[1];
+#pragma unroll
+    for (int s = 0; s < PT2; ++s) v[0][s] = Y[iy * NZ + j + s * T2];
+    nyq[0] = Y[iy * NZ + M];
+    irfft<M, PT2, 1>(v, nyq, ln, scr + g * LdsSize<M>::value, tr);  // N x
+    if (g < N) {
+      const long long t = plane * N + iy;
+      const long long ro = rpg > 0 ? (t / rpg) * gstride + (t % rpg) * N : t * N;
+      float2* y = reinterpret_cast<float2*>(x + ro);
+#pragma unroll
+      for (int s = 0; s < PT2; ++s) y[j + s * T2] = make_float2(v[0][s].re * rn, v[0][s].im * rn);
+    }
+  }
+}
+
+// The fused y and z passes of irfftn over nplanes = I N planes (N = 16, 32
+// or 64); hipErrorInvalidValue for the other sizes.
+hipError_t launch_yz_inverse3d(int N, const cf* X, float* x, long long nplanes, const cf* tw, const RealTables& rt,
+                               long long rpg, long long gstride, hipStream_t st) {
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)nplanes), dim3(256), 0, st, X, x, nplanes, tw, rt, rpg, gstride);
+    return hipGetLastError();
+  };
+  switch (N) {
+    case 16: return go(yz_inverse3d_kernel<16>);
+    case 32: return go(yz_inverse3d_kernel<32>);
+    case 64: return go(yz_inverse3d_kernel<64>);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // The fields pass fused with the inverse x lines, one lane group per unit
 // line: a unit is a trajectory for the projected convection (its three
 // channels' spectra in, the six fields u_c, omega_c out) and an instance for
