@@ -463,6 +463,124 @@ __global__ void __launch_bounds__(256) zcross3d_kernel(const cf* __restrict__ F,
   }
 }
 
+// The inverse y lines of the projected convection's six fields fused with
+// its z rows (zcross3d_kernel) for N <= 64: one workgroup owns the plane
+// (trajectory, ix) after the inverse x lines, runs the 6 (kcut + 1) inverse
+// y lines of the planes kz <= kcut (times 1/N) into LDS, then per row iy the
+// six c2r transforms along z, u x omega and the three r2c transforms, into
+// P as zcross3d_kernel stores them. kcut < N/3 + 1 planes are held (the
+// 2/3 rule's cutoff); the caller checks.
+template <int N>
+struct YZCrossCfg {
+  static constexpr int M = N / 2, NZ = M + 1, KCMAX = N / 3 + 1;
+  static constexpr int PT1 = LineCfg<N>::PT, T1 = N / PT1, G1 = 256 / T1;  // y lines
+  static constexpr int B1 = (6 * KCMAX + G1 - 1) / G1;                      // y lines per group
+  static constexpr int PT2 = 8, T2 = M / PT2, G2 = 256 / T2;                 // z rows
+  static constexpr int L1 = G1 * LdsSize<N>::value, L2 = G2 * LdsSize<M>::value, LY = 6 * N * KCMAX;
+  static constexpr int LDS = LY > L1 ? (LY > L2 ? LY : L2) : (L1 > L2 ? L1 : L2);
+  static_assert(N <= G2, "one z row per group");
+};
+
+template <int N>
+__global__ void __launch_bounds__(256) yzcross3d_kernel(const cf* __restrict__ F, cf* __restrict__ P,
+                                                        long long Nr, int kcut, const cf* __restrict__ tw,
+                                                        RealTables rt) {
+  using C = YZCrossCfg<N>;
+  constexpr int M = C::M, NZ = C::NZ, KC = C::KCMAX, PT1 = C::PT1, T1 = C::T1, G1 = C::G1, B1 = C::B1;
+  constexpr int PT2 = C::PT2, T2 = C::T2;
+  // the y lines' scratch, then the y-transformed fields Y[q][iy][kz], then
+  // the z rows' scratch, in turn (barriers between)
+  __shared__ cf lds[C::LDS];
+  const long long plane = blockIdx.x;  // traj * N + ix
+  const long long traj = plane / N;
+  const int ix = (int)(plane - traj * N);
+  const int nl = 6 * (kcut + 1);
+  constexpr float rn = 1.f / (float)N;
+  {
+    const int c = threadIdx.x / T1, j = threadIdx.x % T1;
+    const Lane<N, PT1> ln(j);
+    TwRegs<N, PT1> tr;
+    tr.template load_pass<1>(j, tw);
+    cf v[B1][PT1];
+#pragma unroll
+    for (int b = 0; b < B1; ++b) {
+      const int l = c + b * G1;
+      const int q = l < nl ? l / (kcut + 1) : 0, kz = l < nl ? l % (kcut + 1) : 0;
+      const cf* x = F + (traj * 6 + q) * Nr + (long long)ix * N * NZ + kz;
+#pragma unroll
+      for (int s = 0; s < PT1; ++s) v[b][s] = l < nl ? x[(j + s * T1) * NZ] : cf{0.f, 0.f};
+    }
+    cfft<N, PT1, true, B1>(v, ln, lds + c * LdsSize<N>::value, tr);
+    __syncthreads();  // every group is done with its scratch before Y overwrites it
+#pragma unroll
+    for (int b = 0; b < B1; ++b) {
+      const int l = c + b * G1;
+      if (l < nl) {
+        const int q = l / (kcut + 1), kz = l % (kcut + 1);
+#pragma unroll
+        for (int s = 0; s < PT1; ++s) lds[(q * N + j + s * T1) * KC + kz] = rn * v[b][s];
+      }
+    }
+  }
+  __syncthreads();
+  const int g = threadIdx.x / T2, j = threadIdx.x % T2;
+  const int iy = g < N ? g : 0;  // groups past the last row mirror row 0 and store nothing
+  cf v[6][PT2];
+  cf nyq[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+#pragma unroll
+    for (int s = 0; s < PT2; ++s) v[q][s] = j + s * T2 <= kcut ? lds[(q * N + iy) * KC + j + s * T2] : cf{0.f, 0.f};
+    nyq[q] = cf{0.f, 0.f};  // kcut < KC <= M
+  }
+  __syncthreads();  // every row is loaded before the z scratch overwrites Y
+  const Lane<M, PT2> ln(j);
+  TwRegs<M, PT2> tr;
+  tr.load(j, rt);
+  cf* scr = lds + g * LdsSize<M>::value;
+  irfft<M, PT2, 6>(v, nyq, ln, scr, tr);  // N times the physical pairs
+  constexpr float rn2 = 1.f / ((float)N * (float)N);
+  cf pr[3][PT2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+#pragma unroll
+    for (int s = 0; s < PT2; ++s) {
+      const cf u1 = v[2 * c1][s], w2 = v[2 * c2 + 1][s], u2 = v[2 * c2][s], w1 = v[2 * c1 + 1][s];
+      pr[c][s] = cf{(u1.re * w2.re - u2.re * w1.re) * rn2, (u1.im * w2.im - u2.im * w1.im) * rn2};
+    }
+  }
+  cf pn[3];
+  rfft<M, PT2, 3>(pr, pn, ln, scr, tr, kcut);  // 2 X
+  if (g < N) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      cf* y = P + (traj * 3 + c) * Nr + ((long long)ix * N + iy) * NZ;
+#pragma unroll
+      for (int s = 0; s < PT2; ++s)
+        if (j + s * T2 <= kcut) y[j + s * T2] = 0.5f * pr[c][s];
+    }
+  }
+}
+
+// The fused y lines and z rows of the projected convection over ntraj N
+// planes (N = 16, 32 or 64 with kcut < N/3 + 1); hipErrorInvalidValue
+// otherwise (the caller runs the y lines and zcross3d_kernel apart).
+hipError_t launch_yzcross3d(int N, const cf* F, cf* P, long long ntraj, long long Nr, int kcut, const cf* tw,
+                            const RealTables& rt, hipStream_t st) {
+  if (N > 64 || kcut < 0 || kcut >= N / 3 + 1) return hipErrorInvalidValue;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)(ntraj * N)), dim3(256), 0, st, F, P, Nr, kcut, tw, rt);
+    return hipGetLastError();
+  };
+  switch (N) {
+    case 16: return go(yzcross3d_kernel<16>);
+    case 32: return go(yzcross3d_kernel<32>);
+    case 64: return go(yzcross3d_kernel<64>);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // The z rows of the other nonlinear kinds, fused the same way (the c2r of
 // the trajectory's fields, the products of products3d_kernel at every point
 // in registers, the r2c of the products): C instances per trajectory, NF
