@@ -1187,8 +1187,8 @@ static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf
   const bool unit = N <= 128 && (k3 == NL_PROJ3D || k3 == NL_GRADNORM || k3 == NL_GENERAL || k3 == NL_CONV_NONCONS);
   hipError_t e = unit ? launch_fields_unit_lines3d(a, X, p->F3, r, tl, kz + 1, st)
                       : launch_fields_lines3d(a, X, p->F3, r, tl, kz + 1, st);
-  // the projected convection up to N = 64 (2/3 cutoff): the inverse y lines
-  // run inside the z-row plane kernel (yzcross3d_kernel)
+  // the projected convection up to N = 64 (2/3 cutoff): the inverse and the
+  // forward y lines run inside the z-row plane kernel (yzcross3d_kernel)
   const bool yz = k3 == NL_PROJ3D && N <= 64 && kz < N / 3 + 1;
   if (e == hipSuccess && !yz)
     e = launch_lines(N, true, p->F3, I * N * NZ, NZ, N * NZ, NZ, r, tl, st, nullptr, kz + 1);
@@ -1199,7 +1199,8 @@ static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf
     if (e == hipSuccess)
       e = yz ? launch_yzcross3d(N, p->F3, P, p->d.batch, p->nr, kz, tl, p->tw.rt(), st)
              : launch_zcross3d(N, p->F3, P, p->d.batch, p->nr, kz, p->tw.rt(), st);
-    if (e == hipSuccess) e = launch_lines(N, false, P, IP * N * NZ, NZ, N * NZ, NZ, 1.f, tl, st, nullptr, kz + 1);
+    if (e == hipSuccess && !yz)
+      e = launch_lines(N, false, P, IP * N * NZ, NZ, N * NZ, NZ, 1.f, tl, st, nullptr, kz + 1);
     if (e == hipSuccess && !no_x)
       e = launch_lines(N, false, P, IP * N * NZ, N * NZ, (long long)N * N * NZ, N * NZ, 1.f, tl, st, nullptr, kz + 1);
     *ph = P;

@@ -463,18 +463,20 @@ __global__ void __launch_bounds__(256) zcross3d_kernel(const cf* __restrict__ F,
   }
 }
 
-// The inverse y lines of the projected convection's six fields fused with
-// its z rows (zcross3d_kernel) for N <= 64: one workgroup owns the plane
-// (trajectory, ix) after the inverse x lines, runs the 6 (kcut + 1) inverse
-// y lines of the planes kz <= kcut (times 1/N) into LDS, then per row iy the
-// six c2r transforms along z, u x omega and the three r2c transforms, into
-// P as zcross3d_kernel stores them. kcut < N/3 + 1 planes are held (the
-// 2/3 rule's cutoff); the caller checks.
+// The projected convection's y lines fused with its z rows
+// (zcross3d_kernel) for N <= 64: one workgroup owns the plane (trajectory,
+// ix) after the inverse x lines, runs the 6 (kcut + 1) inverse y lines of
+// the planes kz <= kcut (times 1/N) into LDS, then per row iy the six c2r
+// transforms along z, u x omega and the three r2c transforms back into
+// LDS, then the 3 (kcut + 1) forward y lines into P, where the forward y
+// line pass would leave them. kcut < N/3 + 1 planes are held (the 2/3
+// rule's cutoff); the caller checks.
 template <int N>
 struct YZCrossCfg {
   static constexpr int M = N / 2, NZ = M + 1, KCMAX = N / 3 + 1;
   static constexpr int PT1 = LineCfg<N>::PT, T1 = N / PT1, G1 = 256 / T1;  // y lines
-  static constexpr int B1 = (6 * KCMAX + G1 - 1) / G1;                      // y lines per group
+  static constexpr int B1 = (6 * KCMAX + G1 - 1) / G1;                      // inverse y lines per group
+  static constexpr int B3 = (3 * KCMAX + G1 - 1) / G1;                      // forward y lines per group
   static constexpr int PT2 = 8, T2 = M / PT2, G2 = 256 / T2;                 // z rows
   static constexpr int L1 = G1 * LdsSize<N>::value, L2 = G2 * LdsSize<M>::value, LY = 6 * N * KCMAX;
   static constexpr int LDS = LY > L1 ? (LY > L2 ? LY : L2) : (L1 > L2 ? L1 : L2);
@@ -487,7 +489,7 @@ __global__ void __launch_bounds__(256) yzcross3d_kernel(const cf* __restrict__ F
                                                         RealTables rt) {
   using C = YZCrossCfg<N>;
   constexpr int M = C::M, NZ = C::NZ, KC = C::KCMAX, PT1 = C::PT1, T1 = C::T1, G1 = C::G1, B1 = C::B1;
-  constexpr int PT2 = C::PT2, T2 = C::T2;
+  constexpr int PT2 = C::PT2, T2 = C::T2, B3 = C::B3;
   // the y lines' scratch, then the y-transformed fields Y[q][iy][kz], then
   // the z rows' scratch, in turn (barriers between)
   __shared__ cf lds[C::LDS];
@@ -552,13 +554,43 @@ __global__ void __launch_bounds__(256) yzcross3d_kernel(const cf* __restrict__ F
   }
   cf pn[3];
   rfft<M, PT2, 3>(pr, pn, ln, scr, tr, kcut);  // 2 X
+  __syncthreads();  // every row is done with its scratch before Y2 overwrites it
   if (g < N) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      cf* y = P + (traj * 3 + c) * Nr + ((long long)ix * N + iy) * NZ;
 #pragma unroll
       for (int s = 0; s < PT2; ++s)
-        if (j + s * T2 <= kcut) y[j + s * T2] = 0.5f * pr[c][s];
+        if (j + s * T2 <= kcut) lds[(c * N + iy) * KC + j + s * T2] = 0.5f * pr[c][s];
+    }
+  }
+  __syncthreads();
+  {
+    // the forward y lines of the three product spectra's planes kz <= kcut,
+    // stored where the forward y line pass would leave them
+    const int c3 = threadIdx.x / T1, j3 = threadIdx.x % T1;
+    const int nl3 = 3 * (kcut + 1);
+    cf w[B3][PT1];
+#pragma unroll
+    for (int b = 0; b < B3; ++b) {
+      const int l = c3 + b * G1;
+      const int q = l < nl3 ? l / (kcut + 1) : 0, kz = l < nl3 ? l % (kcut + 1) : 0;
+#pragma unroll
+      for (int s = 0; s < PT1; ++s) w[b][s] = l < nl3 ? lds[(q * N + j3 + s * T1) * KC + kz] : cf{0.f, 0.f};
+    }
+    __syncthreads();  // every line is loaded before the scratch overwrites Y2
+    const Lane<N, PT1> ln3(j3);
+    TwRegs<N, PT1> tr3;
+    tr3.template load_pass<1>(j3, tw);
+    cfft<N, PT1, false, B3>(w, ln3, lds + c3 * LdsSize<N>::value, tr3);
+#pragma unroll
+    for (int b = 0; b < B3; ++b) {
+      const int l = c3 + b * G1;
+      if (l < nl3) {
+        const int q = l / (kcut + 1), kz = l % (kcut + 1);
+        cf* y = P + (traj * 3 + q) * Nr + (long long)ix * N * NZ + kz;
+#pragma unroll
+        for (int s = 0; s < PT1; ++s) y[(j3 + s * T1) * NZ] = w[b][s];
+      }
     }
   }
 }
