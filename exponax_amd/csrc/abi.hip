@@ -61,8 +61,8 @@ hipError_t launch_fields_unit_lines3d(const NL3DArgs& a, const cf* S, cf* F, flo
                                       hipStream_t st);
 hipError_t launch_yz_inverse3d(int N, const cf* X, float* x, long long nplanes, const cf* tw, const RealTables& rt,
                                long long rpg, long long gstride, hipStream_t st);
-hipError_t launch_yzcross3d(int N, const cf* F, cf* P, long long ntraj, long long Nr, int kcut, const cf* tw,
-                            const RealTables& rt, hipStream_t st);
+hipError_t launch_yzprod3d(const NL3DArgs& a, const cf* F, cf* P, int kcut, const cf* tw, const RealTables& rt,
+                           hipStream_t st);
 hipError_t launch_adj_cofields3d(const NL3DArgs& a, const cf* MU, cf* H, hipStream_t st);
 hipError_t launch_adj_products3d(const NL3DArgs& a, const float* R, const float* HR, float* Q, hipStream_t st);
 hipError_t launch_adj_combine3d(const NL3DArgs& a, const cf* QH, cf* out, hipStream_t st);
@@ -1187,9 +1187,12 @@ static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf
   const bool unit = N <= 128 && (k3 == NL_PROJ3D || k3 == NL_GRADNORM || k3 == NL_GENERAL || k3 == NL_CONV_NONCONS);
   hipError_t e = unit ? launch_fields_unit_lines3d(a, X, p->F3, r, tl, kz + 1, st)
                       : launch_fields_lines3d(a, X, p->F3, r, tl, kz + 1, st);
-  // the projected convection up to N = 64 (2/3 cutoff): the inverse and the
-  // forward y lines run inside the z-row plane kernel (yzcross3d_kernel)
-  const bool yz = k3 == NL_PROJ3D && N <= 64 && kz < N / 3 + 1;
+  // up to N = 64 (2/3 cutoff) the inverse y lines, the z rows with the
+  // products and the forward y lines are one plane kernel (yzprod3d_kernel)
+  // for the kinds it covers
+  const bool yz = N <= 64 && kz < N / 3 + 1 &&
+                  (k3 == NL_PROJ3D || k3 == NL_GRADNORM || k3 == NL_GENERAL || k3 == NL_POLY ||
+                   k3 == NL_GRAY_SCOTT || k3 == NL_BZ);
   if (e == hipSuccess && !yz)
     e = launch_lines(N, true, p->F3, I * N * NZ, NZ, N * NZ, NZ, r, tl, st, nullptr, kz + 1);
   if (p->kind3 == NL_PROJ3D) {
@@ -1197,7 +1200,7 @@ static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf
     // product spectra in R3 (which holds I N^3 >= ninst N^2 (N + 2) floats)
     cf* P = (cf*)p->R3;
     if (e == hipSuccess)
-      e = yz ? launch_yzcross3d(N, p->F3, P, p->d.batch, p->nr, kz, tl, p->tw.rt(), st)
+      e = yz ? launch_yzprod3d(a, p->F3, P, kz, tl, p->tw.rt(), st)
              : launch_zcross3d(N, p->F3, P, p->d.batch, p->nr, kz, p->tw.rt(), st);
     if (e == hipSuccess && !yz)
       e = launch_lines(N, false, P, IP * N * NZ, NZ, N * NZ, NZ, 1.f, tl, st, nullptr, kz + 1);
@@ -1213,8 +1216,10 @@ static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf
     // spectra); else into R3 (which holds I NF N^3 >= ninst NP N^2 (N + 2)
     // floats)
     cf* P = p->nf3 == p->np3 ? p->F3 : p->kind3 == NL_CONV_CONS ? (cf*)p->PR3 : (cf*)p->R3;
-    if (e == hipSuccess) e = launch_zprod3d(a, p->F3, P, kz, p->tw.rt(), st);
-    if (e == hipSuccess) e = launch_lines(N, false, P, IP * N * NZ, NZ, N * NZ, NZ, 1.f, tl, st, nullptr, kz + 1);
+    if (e == hipSuccess) e = yz ? launch_yzprod3d(a, p->F3, P, kz, tl, p->tw.rt(), st)
+                                : launch_zprod3d(a, p->F3, P, kz, p->tw.rt(), st);
+    if (e == hipSuccess && !yz)
+      e = launch_lines(N, false, P, IP * N * NZ, NZ, N * NZ, NZ, 1.f, tl, st, nullptr, kz + 1);
     if (e == hipSuccess && !no_x)
       e = launch_lines(N, false, P, IP * N * NZ, N * NZ, (long long)N * N * NZ, N * NZ, 1.f, tl, st, nullptr, kz + 1);
     *ph = P;

@@ -463,156 +463,6 @@ __global__ void __launch_bounds__(256) zcross3d_kernel(const cf* __restrict__ F,
   }
 }
 
-// The projected convection's y lines fused with its z rows
-// (zcross3d_kernel) for N <= 64: one workgroup owns the plane (trajectory,
-// ix) after the inverse x lines, runs the 6 (kcut + 1) inverse y lines of
-// the planes kz <= kcut (times 1/N) into LDS, then per row iy the six c2r
-// transforms along z, u x omega and the three r2c transforms back into
-// LDS, then the 3 (kcut + 1) forward y lines into P, where the forward y
-// line pass would leave them. kcut < N/3 + 1 planes are held (the 2/3
-// rule's cutoff); the caller checks.
-template <int N>
-struct YZCrossCfg {
-  static constexpr int M = N / 2, NZ = M + 1, KCMAX = N / 3 + 1;
-  static constexpr int PT1 = LineCfg<N>::PT, T1 = N / PT1, G1 = 256 / T1;  // y lines
-  static constexpr int B1 = (6 * KCMAX + G1 - 1) / G1;                      // inverse y lines per group
-  static constexpr int B3 = (3 * KCMAX + G1 - 1) / G1;                      // forward y lines per group
-  static constexpr int PT2 = 8, T2 = M / PT2, G2 = 256 / T2;                 // z rows
-  static constexpr int L1 = G1 * LdsSize<N>::value, L2 = G2 * LdsSize<M>::value, LY = 6 * N * KCMAX;
-  static constexpr int LDS = LY > L1 ? (LY > L2 ? LY : L2) : (L1 > L2 ? L1 : L2);
-  static_assert(N <= G2, "one z row per group");
-};
-
-template <int N>
-__global__ void __launch_bounds__(256) yzcross3d_kernel(const cf* __restrict__ F, cf* __restrict__ P,
-                                                        long long Nr, int kcut, const cf* __restrict__ tw,
-                                                        RealTables rt) {
-  using C = YZCrossCfg<N>;
-  constexpr int M = C::M, NZ = C::NZ, KC = C::KCMAX, PT1 = C::PT1, T1 = C::T1, G1 = C::G1, B1 = C::B1;
-  constexpr int PT2 = C::PT2, T2 = C::T2, B3 = C::B3;
-  // the y lines' scratch, then the y-transformed fields Y[q][iy][kz], then
-  // the z rows' scratch, in turn (barriers between)
-  __shared__ cf lds[C::LDS];
-  const long long plane = blockIdx.x;  // traj * N + ix
-  const long long traj = plane / N;
-  const int ix = (int)(plane - traj * N);
-  const int nl = 6 * (kcut + 1);
-  constexpr float rn = 1.f / (float)N;
-  {
-    const int c = threadIdx.x / T1, j = threadIdx.x % T1;
-    const Lane<N, PT1> ln(j);
-    TwRegs<N, PT1> tr;
-    tr.template load_pass<1>(j, tw);
-    cf v[B1][PT1];
-#pragma unroll
-    for (int b = 0; b < B1; ++b) {
-      const int l = c + b * G1;
-      const int q = l < nl ? l / (kcut + 1) : 0, kz = l < nl ? l % (kcut + 1) : 0;
-      const cf* x = F + (traj * 6 + q) * Nr + (long long)ix * N * NZ + kz;
-#pragma unroll
-      for (int s = 0; s < PT1; ++s) v[b][s] = l < nl ? x[(j + s * T1) * NZ] : cf{0.f, 0.f};
-    }
-    cfft<N, PT1, true, B1>(v, ln, lds + c * LdsSize<N>::value, tr);
-    __syncthreads();  // every group is done with its scratch before Y overwrites it
-#pragma unroll
-    for (int b = 0; b < B1; ++b) {
-      const int l = c + b * G1;
-      if (l < nl) {
-        const int q = l / (kcut + 1), kz = l % (kcut + 1);
-#pragma unroll
-        for (int s = 0; s < PT1; ++s) lds[(q * N + j + s * T1) * KC + kz] = rn * v[b][s];
-      }
-    }
-  }
-  __syncthreads();
-  const int g = threadIdx.x / T2, j = threadIdx.x % T2;
-  const int iy = g < N ? g : 0;  // groups past the last row mirror row 0 and store nothing
-  cf v[6][PT2];
-  cf nyq[6];
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-#pragma unroll
-    for (int s = 0; s < PT2; ++s) v[q][s] = j + s * T2 <= kcut ? lds[(q * N + iy) * KC + j + s * T2] : cf{0.f, 0.f};
-    nyq[q] = cf{0.f, 0.f};  // kcut < KC <= M
-  }
-  __syncthreads();  // every row is loaded before the z scratch overwrites Y
-  const Lane<M, PT2> ln(j);
-  TwRegs<M, PT2> tr;
-  tr.load(j, rt);
-  cf* scr = lds + g * LdsSize<M>::value;
-  irfft<M, PT2, 6>(v, nyq, ln, scr, tr);  // N times the physical pairs
-  constexpr float rn2 = 1.f / ((float)N * (float)N);
-  cf pr[3][PT2];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-#pragma unroll
-    for (int s = 0; s < PT2; ++s) {
-      const cf u1 = v[2 * c1][s], w2 = v[2 * c2 + 1][s], u2 = v[2 * c2][s], w1 = v[2 * c1 + 1][s];
-      pr[c][s] = cf{(u1.re * w2.re - u2.re * w1.re) * rn2, (u1.im * w2.im - u2.im * w1.im) * rn2};
-    }
-  }
-  cf pn[3];
-  rfft<M, PT2, 3>(pr, pn, ln, scr, tr, kcut);  // 2 X
-  __syncthreads();  // every row is done with its scratch before Y2 overwrites it
-  if (g < N) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-#pragma unroll
-      for (int s = 0; s < PT2; ++s)
-        if (j + s * T2 <= kcut) lds[(c * N + iy) * KC + j + s * T2] = 0.5f * pr[c][s];
-    }
-  }
-  __syncthreads();
-  {
-    // the forward y lines of the three product spectra's planes kz <= kcut,
-    // stored where the forward y line pass would leave them
-    const int c3 = threadIdx.x / T1, j3 = threadIdx.x % T1;
-    const int nl3 = 3 * (kcut + 1);
-    cf w[B3][PT1];
-#pragma unroll
-    for (int b = 0; b < B3; ++b) {
-      const int l = c3 + b * G1;
-      const int q = l < nl3 ? l / (kcut + 1) : 0, kz = l < nl3 ? l % (kcut + 1) : 0;
-#pragma unroll
-      for (int s = 0; s < PT1; ++s) w[b][s] = l < nl3 ? lds[(q * N + j3 + s * T1) * KC + kz] : cf{0.f, 0.f};
-    }
-    __syncthreads();  // every line is loaded before the scratch overwrites Y2
-    const Lane<N, PT1> ln3(j3);
-    TwRegs<N, PT1> tr3;
-    tr3.template load_pass<1>(j3, tw);
-    cfft<N, PT1, false, B3>(w, ln3, lds + c3 * LdsSize<N>::value, tr3);
-#pragma unroll
-    for (int b = 0; b < B3; ++b) {
-      const int l = c3 + b * G1;
-      if (l < nl3) {
-        const int q = l / (kcut + 1), kz = l % (kcut + 1);
-        cf* y = P + (traj * 3 + q) * Nr + (long long)ix * N * NZ + kz;
-#pragma unroll
-        for (int s = 0; s < PT1; ++s) y[(j3 + s * T1) * NZ] = w[b][s];
-      }
-    }
-  }
-}
-
-// The fused y lines and z rows of the projected convection over ntraj N
-// planes (N = 16, 32 or 64 with kcut < N/3 + 1); hipErrorInvalidValue
-// otherwise (the caller runs the y lines and zcross3d_kernel apart).
-hipError_t launch_yzcross3d(int N, const cf* F, cf* P, long long ntraj, long long Nr, int kcut, const cf* tw,
-                            const RealTables& rt, hipStream_t st) {
-  if (N > 64 || kcut < 0 || kcut >= N / 3 + 1) return hipErrorInvalidValue;
-  auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3((unsigned)(ntraj * N)), dim3(256), 0, st, F, P, Nr, kcut, tw, rt);
-    return hipGetLastError();
-  };
-  switch (N) {
-    case 16: return go(yzcross3d_kernel<16>);
-    case 32: return go(yzcross3d_kernel<32>);
-    case 64: return go(yzcross3d_kernel<64>);
-    default: return hipErrorInvalidValue;
-  }
-}
-
 // The z rows of the other nonlinear kinds, fused the same way (the c2r of
 // the trajectory's fields, the products of products3d_kernel at every point
 // in registers, the r2c of the products): C instances per trajectory, NF
@@ -625,10 +475,53 @@ hipError_t launch_yzcross3d(int N, const cf* F, cf* P, long long ntraj, long lon
 template <int KIND>
 struct ZProd {
   static constexpr bool CONV = KIND == NL_CONV_CONS || KIND == NL_CONV_NONCONS;
-  static constexpr int C = KIND == NL_GRAY_SCOTT ? 2 : (KIND == NL_BZ || CONV) ? 3 : 1;
-  static constexpr int NF = KIND == NL_GRADNORM ? 3 : (KIND == NL_GENERAL || KIND == NL_CONV_NONCONS) ? 4 : 1;
+  static constexpr int C = KIND == NL_GRAY_SCOTT ? 2 : (KIND == NL_BZ || CONV || KIND == NL_PROJ3D) ? 3 : 1;
+  static constexpr int NF = KIND == NL_GRADNORM ? 3
+                            : (KIND == NL_GENERAL || KIND == NL_CONV_NONCONS) ? 4
+                            : KIND == NL_PROJ3D ? 2
+                                                : 1;
   static constexpr int NP = KIND == NL_GENERAL ? 2 : KIND == NL_CONV_CONS ? 3 : 1;
 };
+
+// The products at one physical point from the trajectory's fields f[q]
+// (q = c NF + field of instance c) into o[q] (q = c NP + product); the
+// formulas of products3d_kernel.
+template <int KIND>
+__device__ __forceinline__ void zpoint(const NL3DArgs& a, const float* f, float* o) {
+  if constexpr (KIND == NL_GRADNORM) {
+    o[0] = fmaf(f[2], f[2], fmaf(f[1], f[1], f[0] * f[0]));
+  } else if constexpr (KIND == NL_GENERAL) {
+    const float gs = f[1] + f[2] + f[3];
+    o[0] = a.prod_nc ? f[0] * gs : f[0] * f[0];
+    o[1] = fmaf(f[3], f[3], fmaf(f[2], f[2], f[1] * f[1]));
+  } else if constexpr (KIND == NL_POLY) {
+    float pw = 1.f, acc = 0.f;
+    for (int i = 0; i < a.npoly; ++i) {
+      acc += a.poly[i] * pw;
+      pw *= f[0];
+    }
+    o[0] = acc;
+  } else if constexpr (KIND == NL_GRAY_SCOTT) {
+    gray_scott(f[0], f[1], a.feed, a.kill, o[0], o[1]);
+  } else if constexpr (KIND == NL_CONV_CONS) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) o[c * 3 + jj] = f[c] * f[jj];
+  } else if constexpr (KIND == NL_CONV_NONCONS) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = fmaf(f[8], f[c * 4 + 3], fmaf(f[4], f[c * 4 + 2], f[0] * f[c * 4 + 1]));
+  } else if constexpr (KIND == NL_PROJ3D) {
+    // (u x omega)_c = u_c1 omega_c2 - u_c2 omega_c1 (_projected_convection.py:120-123)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      o[c] = f[2 * c1] * f[2 * c2 + 1] - f[2 * c2] * f[2 * c1 + 1];
+    }
+  } else {
+    belousov_zhabotinsky(f[0], f[1], f[2], o[0], o[1], o[2]);
+  }
+}
 
 template <int N, int PT, int KIND>
 __global__ void __launch_bounds__(256) zprod3d_kernel(NL3DArgs a, const cf* __restrict__ F, cf* __restrict__ P,
@@ -666,32 +559,7 @@ __global__ void __launch_bounds__(256) zprod3d_kernel(NL3DArgs a, const cf* __re
       float f[NIN], o[NOUT];
 #pragma unroll
       for (int q = 0; q < NIN; ++q) f[q] = (h ? v[q][s].im : v[q][s].re) * rn;
-      if constexpr (KIND == NL_GRADNORM) {
-        o[0] = fmaf(f[2], f[2], fmaf(f[1], f[1], f[0] * f[0]));
-      } else if constexpr (KIND == NL_GENERAL) {
-        const float gs = f[1] + f[2] + f[3];
-        o[0] = a.prod_nc ? f[0] * gs : f[0] * f[0];
-        o[1] = fmaf(f[3], f[3], fmaf(f[2], f[2], f[1] * f[1]));
-      } else if constexpr (KIND == NL_POLY) {
-        float pw = 1.f, acc = 0.f;
-        for (int i = 0; i < a.npoly; ++i) {
-          acc += a.poly[i] * pw;
-          pw *= f[0];
-        }
-        o[0] = acc;
-      } else if constexpr (KIND == NL_GRAY_SCOTT) {
-        gray_scott(f[0], f[1], a.feed, a.kill, o[0], o[1]);
-      } else if constexpr (KIND == NL_CONV_CONS) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int jj = 0; jj < 3; ++jj) o[c * 3 + jj] = f[c] * f[jj];
-      } else if constexpr (KIND == NL_CONV_NONCONS) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) o[c] = fmaf(f[8], f[c * 4 + 3], fmaf(f[4], f[c * 4 + 2], f[0] * f[c * 4 + 1]));
-      } else {
-        belousov_zhabotinsky(f[0], f[1], f[2], o[0], o[1], o[2]);
-      }
+      zpoint<KIND>(a, f, o);
 #pragma unroll
       for (int q = 0; q < NOUT; ++q) {
         if (h) pr[q][s].im = o[q];
@@ -747,6 +615,186 @@ hipError_t launch_zprod3d(const NL3DArgs& a, const cf* F, cf* P, int kcut, const
     default: return hipErrorInvalidValue;
   }
 #undef EXMI_ZPROD
+}
+
+// The y lines of the nonlinear evaluation fused with its z rows for
+// N <= 64: one workgroup owns the plane (trajectory, ix) after the inverse
+// x lines, runs the NIN (kcut + 1) inverse y lines of the trajectory's
+// fields on the planes kz <= kcut (times 1/N) into LDS, then per row iy the
+// fields' c2r transforms along z, the products at every point (zpoint), the
+// products' r2c transforms back into LDS, then the NOUT (kcut + 1) forward
+// y lines into P, where the forward y line pass would leave them: the
+// inverse y pass, the z-row kernel and the forward y pass in one. kcut <
+// N/3 + 1 planes are held (the 2/3 rule's cutoff); the caller checks. P may
+// equal F (a workgroup reads its whole plane before it writes it).
+template <int N, int KIND>
+struct YZCfg3 {
+  using Z = ZProd<KIND>;
+  static constexpr int NIN = Z::C * Z::NF, NOUT = Z::C * Z::NP;
+  static constexpr int M = N / 2, NZ = M + 1, KCMAX = N / 3 + 1;
+  static constexpr int PT1 = LineCfg<N>::PT, T1 = N / PT1, G1 = 256 / T1;  // y lines
+  static constexpr int B1 = (NIN * KCMAX + G1 - 1) / G1;                    // inverse y lines per group
+  static constexpr int B3 = (NOUT * KCMAX + G1 - 1) / G1;                   // forward y lines per group
+  static constexpr int PT2 = 8, T2 = M / PT2, G2 = 256 / T2;                 // z rows
+  static constexpr int L1 = G1 * LdsSize<N>::value, L2 = G2 * LdsSize<M>::value;
+  static constexpr int LY = (NIN > NOUT ? NIN : NOUT) * N * KCMAX;
+  static constexpr int LDS = LY > L1 ? (LY > L2 ? LY : L2) : (L1 > L2 ? L1 : L2);
+  static_assert(N <= G2, "one z row per group");
+};
+
+template <int N, int KIND>
+__global__ void __launch_bounds__(256) yzprod3d_kernel(NL3DArgs a, const cf* F, cf* P, int kcut,
+                                                       const cf* __restrict__ tw, RealTables rt) {
+  using C = YZCfg3<N, KIND>;
+  constexpr int NIN = C::NIN, NOUT = C::NOUT;
+  constexpr int M = C::M, NZ = C::NZ, KC = C::KCMAX, PT1 = C::PT1, T1 = C::T1, G1 = C::G1, B1 = C::B1;
+  constexpr int PT2 = C::PT2, T2 = C::T2, B3 = C::B3;
+  const long long Nr = a.Nr;
+  // the y lines' scratch, the y-transformed fields Y[q][iy][kz], the z rows'
+  // scratch, the z-transformed products, the y lines' scratch, in turn
+  // (barriers between)
+  __shared__ cf lds[C::LDS];
+  const long long plane = blockIdx.x;  // traj * N + ix
+  const long long traj = plane / N;
+  const int ix = (int)(plane - traj * N);
+  constexpr float rn = 1.f / (float)N;
+  {
+    const int nl = NIN * (kcut + 1);
+    const int c = threadIdx.x / T1, j = threadIdx.x % T1;
+    const Lane<N, PT1> ln(j);
+    TwRegs<N, PT1> tr;
+    tr.template load_pass<1>(j, tw);
+    cf v[B1][PT1];
+#pragma unroll
+    for (int b = 0; b < B1; ++b) {
+      const int l = c + b * G1;
+      const int q = l < nl ? l / (kcut + 1) : 0, kz = l < nl ? l % (kcut + 1) : 0;
+      const cf* x = F + (traj * NIN + q) * Nr + (long long)ix * N * NZ + kz;
+#pragma unroll
+      for (int s = 0; s < PT1; ++s) v[b][s] = l < nl ? x[(j + s * T1) * NZ] : cf{0.f, 0.f};
+    }
+    cfft<N, PT1, true, B1>(v, ln, lds + c * LdsSize<N>::value, tr);
+    __syncthreads();  // every group is done with its scratch before Y overwrites it
+#pragma unroll
+    for (int b = 0; b < B1; ++b) {
+      const int l = c + b * G1;
+      if (l < nl) {
+        const int q = l / (kcut + 1), kz = l % (kcut + 1);
+#pragma unroll
+        for (int s = 0; s < PT1; ++s) lds[(q * N + j + s * T1) * KC + kz] = rn * v[b][s];
+      }
+    }
+  }
+  __syncthreads();
+  {
+    const int g = threadIdx.x / T2, j = threadIdx.x % T2;
+    const int iy = g < N ? g : 0;  // groups past the last row mirror row 0 and store nothing
+    cf v[NIN][PT2];
+    cf nyq[NIN];
+#pragma unroll
+    for (int q = 0; q < NIN; ++q) {
+#pragma unroll
+      for (int s = 0; s < PT2; ++s)
+        v[q][s] = j + s * T2 <= kcut ? lds[(q * N + iy) * KC + j + s * T2] : cf{0.f, 0.f};
+      nyq[q] = cf{0.f, 0.f};  // kcut < KC <= M
+    }
+    __syncthreads();  // every row is loaded before the z scratch overwrites Y
+    const Lane<M, PT2> ln(j);
+    TwRegs<M, PT2> tr;
+    tr.load(j, rt);
+    cf* scr = lds + g * LdsSize<M>::value;
+    irfft<M, PT2, NIN>(v, nyq, ln, scr, tr);  // N times the physical pairs
+    cf pr[NOUT][PT2];
+#pragma unroll
+    for (int s = 0; s < PT2; ++s) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {  // the two real points of the slot
+        float f[NIN], o[NOUT];
+#pragma unroll
+        for (int q = 0; q < NIN; ++q) f[q] = (h ? v[q][s].im : v[q][s].re) * rn;
+        zpoint<KIND>(a, f, o);
+#pragma unroll
+        for (int q = 0; q < NOUT; ++q) {
+          if (h) pr[q][s].im = o[q];
+          else pr[q][s].re = o[q];
+        }
+      }
+    }
+    cf pn[NOUT];
+    rfft<M, PT2, NOUT>(pr, pn, ln, scr, tr, kcut);  // 2 X
+    __syncthreads();  // every row is done with its scratch before Y2 overwrites it
+    if (g < N) {
+#pragma unroll
+      for (int q = 0; q < NOUT; ++q) {
+#pragma unroll
+        for (int s = 0; s < PT2; ++s)
+          if (j + s * T2 <= kcut) lds[(q * N + iy) * KC + j + s * T2] = 0.5f * pr[q][s];
+      }
+    }
+  }
+  __syncthreads();
+  {
+    // the forward y lines of the products' planes kz <= kcut
+    const int nl3 = NOUT * (kcut + 1);
+    const int c3 = threadIdx.x / T1, j3 = threadIdx.x % T1;
+    cf w[B3][PT1];
+#pragma unroll
+    for (int b = 0; b < B3; ++b) {
+      const int l = c3 + b * G1;
+      const int q = l < nl3 ? l / (kcut + 1) : 0, kz = l < nl3 ? l % (kcut + 1) : 0;
+#pragma unroll
+      for (int s = 0; s < PT1; ++s) w[b][s] = l < nl3 ? lds[(q * N + j3 + s * T1) * KC + kz] : cf{0.f, 0.f};
+    }
+    __syncthreads();  // every line is loaded before the scratch overwrites Y2
+    const Lane<N, PT1> ln3(j3);
+    TwRegs<N, PT1> tr3;
+    tr3.template load_pass<1>(j3, tw);
+    cfft<N, PT1, false, B3>(w, ln3, lds + c3 * LdsSize<N>::value, tr3);
+#pragma unroll
+    for (int b = 0; b < B3; ++b) {
+      const int l = c3 + b * G1;
+      if (l < nl3) {
+        const int q = l / (kcut + 1), kz = l % (kcut + 1);
+        cf* y = P + (traj * NOUT + q) * Nr + (long long)ix * N * NZ + kz;
+#pragma unroll
+        for (int s = 0; s < PT1; ++s) y[(j3 + s * T1) * NZ] = w[b][s];
+      }
+    }
+  }
+}
+
+// The fused y lines and z rows over the trajectories' N planes each (N = 16,
+// 32 or 64 with kcut < N/3 + 1; the projected convection, gradient norm,
+// general form, polynomial and coupled reactions); hipErrorInvalidValue
+// otherwise (the caller runs the passes apart).
+hipError_t launch_yzprod3d(const NL3DArgs& a, const cf* F, cf* P, int kcut, const cf* tw, const RealTables& rt,
+                           hipStream_t st) {
+  const int N = a.N;
+  if (N > 64 || a.dims != 3 || kcut < 0 || kcut >= N / 3 + 1) return hipErrorInvalidValue;
+  const int C = a.kind == NL_GRAY_SCOTT ? 2 : (a.kind == NL_BZ || a.kind == NL_PROJ3D) ? 3 : 1;
+  const long long nplanes = a.ninst / C * N;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)nplanes), dim3(256), 0, st, a, F, P, kcut, tw, rt);
+    return hipGetLastError();
+  };
+#define EXMI_YZ(NN)                                                         \
+  case NN:                                                                  \
+    switch (a.kind) {                                                       \
+      case NL_PROJ3D: return go(yzprod3d_kernel<NN, NL_PROJ3D>);            \
+      case NL_GRADNORM: return go(yzprod3d_kernel<NN, NL_GRADNORM>);        \
+      case NL_GENERAL: return go(yzprod3d_kernel<NN, NL_GENERAL>);          \
+      case NL_POLY: return go(yzprod3d_kernel<NN, NL_POLY>);                \
+      case NL_GRAY_SCOTT: return go(yzprod3d_kernel<NN, NL_GRAY_SCOTT>);    \
+      case NL_BZ: return go(yzprod3d_kernel<NN, NL_BZ>);                    \
+      default: return hipErrorInvalidValue;                                 \
+    }
+  switch (N) {
+    EXMI_YZ(16)
+    EXMI_YZ(32)
+    EXMI_YZ(64)
+    default: return hipErrorInvalidValue;
+  }
+#undef EXMI_YZ
 }
 
 hipError_t launch_zcross3d(int N, const cf* F, cf* P, long long ntraj, long long Nr, int kcut, const RealTables& rt,
