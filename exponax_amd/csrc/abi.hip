@@ -54,7 +54,7 @@ hipError_t launch_combine3d(const NL3DArgs& a, const cf* PH, cf* Nh, hipStream_t
 hipError_t launch_stage3d(const Stage3DArgs& a, hipStream_t st);
 hipError_t launch_stage_nl3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* PH, cf* N0out, hipStream_t st);
 hipError_t launch_xstage_proj3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P, cf* N0out, int kcut,
-                                const cf* tw, hipStream_t st);
+                                const cf* tw, hipStream_t st, cf* K);
 hipError_t launch_xstage3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P, cf* N0out, int kcut, const cf* tw,
                            hipStream_t st);
 hipError_t launch_fields_unit_lines3d(const NL3DArgs& a, const cf* S, cf* F, float scale, const cf* tw, int kkeep,
@@ -1251,24 +1251,32 @@ static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t s
 // stage (the first also keeps N0).
 // For the fused sizes up to 128 the forward x lines of the product spectra
 // run fused with the combine (the projection) and the stage (xstage_kind).
-static hipError_t stages3d(exmi_plan* p, Stage3DArgs& sa, hipStream_t st) {
+// With emitK (a rollout that emits the new state) the projected
+// convection's last stage also stores the new state's inverse x lines to
+// emitK (the first pass of irfftn); *emitted says whether it did.
+static hipError_t stages3d(exmi_plan* p, Stage3DArgs& sa, hipStream_t st, cf* emitK = nullptr,
+                           bool* emitted = nullptr) {
   static const int seq[4][4] = {{ST1, -1}, {ST2A, ST2B, -1}, {ST3A, ST3B, ST3C, -1}, {ST4A, ST4B, ST4C, ST4D}};
   NL3DArgs a;
   fill_nl3d(p, a);
   const int N = p->d.num_points, NZ = N / 2 + 1;
   const bool xs = xstage_kind(p);
   const int kz = p->kmax < NZ - 1 ? (p->kmax < 0 ? 0 : p->kmax) : NZ - 1;
+  const bool em = emitK && xs && p->kind3 == NL_PROJ3D;
+  if (emitted) *emitted = em;
   hipError_t e = hipSuccess;
   for (int i = 0; i < 4 && e == hipSuccess; ++i) {
     const int m = seq[p->d.order - 1][i];
     if (m < 0) break;
+    const bool last = i == 3 || seq[p->d.order - 1][i + 1] < 0;
     const cf* ph = nullptr;
     e = nonlinear3d_products(p, a, i == 0 ? sa.U0 : sa.S, st, &ph, xs);
     sa.mode = m;
     cf* n0 = i == 0 ? p->N03 : nullptr;
     if (e == hipSuccess)
       e = !xs                          ? launch_stage_nl3d(a, sa, ph, n0, st)
-          : p->kind3 == NL_PROJ3D ? launch_xstage_proj3d(a, sa, ph, n0, kz, p->tw_col.tw, st)
+          : p->kind3 == NL_PROJ3D ? launch_xstage_proj3d(a, sa, ph, n0, kz, p->tw_col.tw, st,
+                                                         em && last ? emitK : nullptr)
                                   : launch_xstage3d(a, sa, ph, n0, kz, p->tw_col.tw, st);
   }
   return e;
@@ -1370,16 +1378,26 @@ static int enqueue_md_carry(exmi_plan* p, const float* u0, long long u0_bs, floa
     if (p->d.order == 0) {
       e = launch_etdrk0_spec(cur, p->ninst, p->nr, p->d.num_channels, p->tables, p->ncoef, p->d.lin_batch,
                              p->d.lin_channels, -1, st);
-    } else {
+    }
+    const bool emit = !(final_only && t < n - 1);
+    bool emitted = false;  // the last stage left the new state's x lines in K3
+    if (p->d.order > 0) {
       sa.U0 = cur;
       sa.S = nxt;
-      e = stages3d(p, sa, st);
+      // up to N = 64 the emitted state's remaining passes are one plane kernel
+      e = stages3d(p, sa, st, emit && !p->gen && p->d.num_points <= 64 ? p->K3 : nullptr, &emitted);
       cf* t2 = cur;
       cur = nxt;
       nxt = t2;
     }
-    if (e != hipSuccess || (final_only && t < n - 1)) continue;
+    if (e != hipSuccess || !emit) continue;
     float* dst = final_only ? out : out + (long long)t * out_ts;
+    if (emitted) {
+      const int N = p->d.num_points;
+      e = launch_yz_inverse3d(N, p->K3, dst, p->ninst * N, p->tw_col.tw, p->tw.rt(),
+                              (long long)p->d.num_channels * N * N, out_bs, st);
+      continue;
+    }
     if (!p->gen) {
       if (e == hipSuccess) e = irfft3_keep_strided(p, cur, p->K3, dst, out_bs, st);
       continue;

@@ -1337,10 +1337,10 @@ __device__ __forceinline__ const float* stage_table(const Stage3DArgs& a, long l
 
 // The stage combination of stage MODE at entry idx (mode k of its instance,
 // tb the instance's coefficient table) with the nonlinear terms n0 (N0) and
-// nc (NC) given; U0, S, S1, A as in Stage3DArgs.
+// nc (NC) given; U0, S, S1, A as in Stage3DArgs. Returns the new S[idx].
 template <int MODE>
-__device__ __forceinline__ void stage_core(int e_im, long long nr, const float* tb, long long k, long long idx,
-                                           const cf* U0, cf* S, cf* S1, cf* A, cf n0, cf nc) {
+__device__ __forceinline__ cf stage_core(int e_im, long long nr, const float* tb, long long k, long long idx,
+                                         const cf* U0, cf* S, cf* S1, cf* A, cf n0, cf nc) {
   auto c = [&](int i) { return tb[(long long)i * nr + k]; };
   // coefficient i (0: E, 1: E_half for orders 3-4) times x, complex when L is
   auto cmul = [&](int i, cf x) -> cf {
@@ -1350,36 +1350,39 @@ __device__ __forceinline__ void stage_core(int e_im, long long nr, const float* 
     return cf{fmaf(er, x.re, -ei * x.im), fmaf(er, x.im, ei * x.re)};
   };
   auto emul = [&](cf x) -> cf { return cmul(0, x); };
+  cf r;
   if constexpr (MODE == ST1 || MODE == ST2A) {
-    S[idx] = emul(U0[idx]) + c(1) * n0;
+    r = emul(U0[idx]) + c(1) * n0;
   } else if constexpr (MODE == ST2B) {
-    S[idx] = S[idx] + c(2) * (nc - n0);
+    r = S[idx] + c(2) * (nc - n0);
   } else if constexpr (MODE == ST3A) {
     const cf u0 = U0[idx];
     A[idx] = emul(u0) + c(4) * n0;
-    S[idx] = cmul(1, u0) + c(2) * n0;
+    r = cmul(1, u0) + c(2) * n0;
   } else if constexpr (MODE == ST3B) {
     const cf u0 = U0[idx];
     A[idx] = A[idx] + c(5) * nc;
-    S[idx] = emul(u0) + c(3) * (2.f * nc - n0);
+    r = emul(u0) + c(3) * (2.f * nc - n0);
   } else if constexpr (MODE == ST3C) {
-    S[idx] = A[idx] + c(6) * nc;
+    r = A[idx] + c(6) * nc;
   } else if constexpr (MODE == ST4A) {
     const cf u0 = U0[idx];
     A[idx] = emul(u0) + c(3) * n0;
-    const cf s = cmul(1, u0) + c(2) * n0;
-    S1[idx] = s;
-    S[idx] = s;
+    r = cmul(1, u0) + c(2) * n0;
+    S1[idx] = r;
   } else if constexpr (MODE == ST4B) {
     const cf u0 = U0[idx];
     A[idx] = A[idx] + (2.f * c(4)) * nc;
-    S[idx] = cmul(1, u0) + c(2) * nc;
+    r = cmul(1, u0) + c(2) * nc;
   } else if constexpr (MODE == ST4C) {
     A[idx] = A[idx] + (2.f * c(4)) * nc;
-    S[idx] = cmul(1, S1[idx]) + c(2) * (2.f * nc - n0);
-  } else if constexpr (MODE == ST4D) {
-    S[idx] = A[idx] + c(5) * nc;
+    r = cmul(1, S1[idx]) + c(2) * (2.f * nc - n0);
+  } else {
+    static_assert(MODE == ST4D, "stage mode");
+    r = A[idx] + c(5) * nc;
   }
+  S[idx] = r;
+  return r;
 }
 
 // Calls f(std::integral_constant<int, mode>) for the runtime stage mode.
@@ -1482,11 +1485,11 @@ __global__ void stage_nl3d_proj_kernel(NL3DArgs na, Stage3DArgs a, const cf* __r
 // buffers come in as separate non-aliasing arguments so that the stage's
 // loads of later points need not wait for the stores of earlier ones.
 
-template <int N>
+template <int N, bool EMIT>
 __global__ void __launch_bounds__(256)
     xstage_proj3d_kernel(NL3DArgs na, Stage3DArgs a, const cf* __restrict__ P, cf* __restrict__ N0out, int kcut,
                          const cf* __restrict__ tw, const cf* __restrict__ U0, const cf* __restrict__ N0,
-                         cf* __restrict__ S, cf* __restrict__ S1, cf* __restrict__ A) {
+                         cf* __restrict__ S, cf* __restrict__ S1, cf* __restrict__ A, cf* __restrict__ K) {
   using C = XStageCfg<N>;
   constexpr int PT = C::PT, T = C::T;
   constexpr int LDSN = LdsSize<N>::value;
@@ -1539,6 +1542,11 @@ __global__ void __launch_bounds__(256)
   const long long sb0 = straj * 3 * a.nr + sli;
   // one copy of the stage loop per stage mode (a per-point switch would put
   // every mode's code into every point's path)
+  // With EMIT (the step's last stage of a rollout that emits this state)
+  // the new state's values go back through LDS into the lane groups' line
+  // order, and their inverse x lines (times 1/N, every kz plane: the first
+  // pass of irfftn) are stored to K.
+  cf e[EMIT ? 3 : 1][PT];
   with_stage_mode(a.mode, [&](auto mc) {
     constexpr int M = decltype(mc)::value;
     constexpr bool first = !(M == ST2B || M == ST3B || M == ST3C || M == ST4B || M == ST4C || M == ST4D);
@@ -1556,34 +1564,60 @@ __global__ void __launch_bounds__(256)
         const long long k = sli + (long long)ix * inner;
         const long long i = sb0 + ch * a.nr + (long long)ix * inner;
         if (!sact) continue;
+        cf sv;
         if constexpr (first) {
           N0out[i] = r;
-          stage_core<M>(a.e_im, a.nr, tb, k, i, U0, S, S1, A, r, cf{0.f, 0.f});
+          sv = stage_core<M>(a.e_im, a.nr, tb, k, i, U0, S, S1, A, r, cf{0.f, 0.f});
         } else {
-          stage_core<M>(a.e_im, a.nr, tb, k, i, U0, S, S1, A, stage_reads_n0(M) ? N0[i] : cf{0.f, 0.f}, r);
+          sv = stage_core<M>(a.e_im, a.nr, tb, k, i, U0, S, S1, A, stage_reads_n0(M) ? N0[i] : cf{0.f, 0.f}, r);
         }
+        if constexpr (EMIT) lds_x[cl * LDSN + ix] = sv;  // the slot this thread alone read
+      }
+      if constexpr (EMIT) {
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < PT; ++s) e[ch][s] = lds[j + s * T];
       }
     }
   });
+  if constexpr (EMIT) {
+    // only this group uses its LDS line from here on
+    cfft<N, PT, true, 3>(e, ln, lds, tr);
+    if (raw < nlines) {
+      constexpr float rn = 1.f / (float)N;
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+#pragma unroll
+        for (int s = 0; s < PT; ++s) K[b0 + ch * a.nr + (long long)(j + s * T) * inner] = rn * e[ch][s];
+      }
+    }
+  }
 }
 
+// K: where the new state's inverse x lines go (the emitting last stage of a
+// rollout step), or nullptr.
 hipError_t launch_xstage_proj3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P, cf* N0out, int kcut,
-                                const cf* tw, hipStream_t st) {
+                                const cf* tw, hipStream_t st, cf* K) {
   const bool first = a.mode == ST1 || a.mode == ST2A || a.mode == ST3A || a.mode == ST4A;
   if (first != (N0out != nullptr)) return hipErrorInvalidValue;  // the first stage keeps N0, the others read it
   auto go = [&](auto kern, int cpb) {
     const long long nlines = a.ninst / 3 * (long long)na.N * na.NZ;
     hipLaunchKernelGGL(kern, dim3((unsigned)((nlines + cpb - 1) / cpb)), dim3(256), 0, st, na, a, P, N0out, kcut,
-                       tw, a.U0, a.N0, a.S, a.S1, a.A);
+                       tw, a.U0, a.N0, a.S, a.S1, a.A, K);
     return hipGetLastError();
   };
+#define EXMI_XSP(NN)                                                                                 \
+  case NN:                                                                                           \
+    return K ? go(xstage_proj3d_kernel<NN, true>, XStageCfg<NN>::CPB)                                \
+             : go(xstage_proj3d_kernel<NN, false>, XStageCfg<NN>::CPB);
   switch (na.N) {
-    case 16: return go(xstage_proj3d_kernel<16>, XStageCfg<16>::CPB);
-    case 32: return go(xstage_proj3d_kernel<32>, XStageCfg<32>::CPB);
-    case 64: return go(xstage_proj3d_kernel<64>, XStageCfg<64>::CPB);
-    case 128: return go(xstage_proj3d_kernel<128>, XStageCfg<128>::CPB);
+    EXMI_XSP(16)
+    EXMI_XSP(32)
+    EXMI_XSP(64)
+    EXMI_XSP(128)
     default: return hipErrorInvalidValue;
   }
+#undef EXMI_XSP
 }
 
 // N_hat at one mode from the instance's product spectra values ph[0..NP) of
