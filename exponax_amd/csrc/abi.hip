@@ -56,7 +56,7 @@ hipError_t launch_stage_nl3d(const NL3DArgs& na, const Stage3DArgs& a, const cf*
 hipError_t launch_xstage_proj3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P, cf* N0out, int kcut,
                                 const cf* tw, hipStream_t st, cf* K);
 hipError_t launch_xstage3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P, cf* N0out, int kcut, const cf* tw,
-                           hipStream_t st);
+                           hipStream_t st, cf* K);
 hipError_t launch_fields_unit_lines3d(const NL3DArgs& a, const cf* S, cf* F, float scale, const cf* tw, int kkeep,
                                       hipStream_t st);
 hipError_t launch_yz_inverse3d(int N, const cf* X, float* x, long long nplanes, const cf* tw, const RealTables& rt,
@@ -1251,9 +1251,9 @@ static hipError_t nonlinear3d(exmi_plan* p, const cf* X, cf* outN, hipStream_t s
 // stage (the first also keeps N0).
 // For the fused sizes up to 128 the forward x lines of the product spectra
 // run fused with the combine (the projection) and the stage (xstage_kind).
-// With emitK (a rollout that emits the new state) the projected
-// convection's last stage also stores the new state's inverse x lines to
-// emitK (the first pass of irfftn); *emitted says whether it did.
+// With emitK (a rollout that emits the new state) the last stage also
+// stores the new state's inverse x lines to emitK (the first pass of
+// irfftn); *emitted says whether it did.
 static hipError_t stages3d(exmi_plan* p, Stage3DArgs& sa, hipStream_t st, cf* emitK = nullptr,
                            bool* emitted = nullptr) {
   static const int seq[4][4] = {{ST1, -1}, {ST2A, ST2B, -1}, {ST3A, ST3B, ST3C, -1}, {ST4A, ST4B, ST4C, ST4D}};
@@ -1262,7 +1262,7 @@ static hipError_t stages3d(exmi_plan* p, Stage3DArgs& sa, hipStream_t st, cf* em
   const int N = p->d.num_points, NZ = N / 2 + 1;
   const bool xs = xstage_kind(p);
   const int kz = p->kmax < NZ - 1 ? (p->kmax < 0 ? 0 : p->kmax) : NZ - 1;
-  const bool em = emitK && xs && p->kind3 == NL_PROJ3D;
+  const bool em = emitK && xs;
   if (emitted) *emitted = em;
   hipError_t e = hipSuccess;
   for (int i = 0; i < 4 && e == hipSuccess; ++i) {
@@ -1277,7 +1277,7 @@ static hipError_t stages3d(exmi_plan* p, Stage3DArgs& sa, hipStream_t st, cf* em
       e = !xs                          ? launch_stage_nl3d(a, sa, ph, n0, st)
           : p->kind3 == NL_PROJ3D ? launch_xstage_proj3d(a, sa, ph, n0, kz, p->tw_col.tw, st,
                                                          em && last ? emitK : nullptr)
-                                  : launch_xstage3d(a, sa, ph, n0, kz, p->tw_col.tw, st);
+                                  : launch_xstage3d(a, sa, ph, n0, kz, p->tw_col.tw, st, em && last ? emitK : nullptr);
   }
   return e;
 }

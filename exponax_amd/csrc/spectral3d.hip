@@ -1652,11 +1652,11 @@ __device__ __forceinline__ cf combine_vals(const NL3DArgs& a, const Mode3D& m, c
 // instance's NP product spectra, transforms them along x (planes kz > kcut
 // are not read), forms N_hat per point, and the workgroup runs the stage
 // transposed through LDS, one loop per stage mode.
-template <int N, int KIND>
+template <int N, int KIND, bool EMIT>
 __global__ void __launch_bounds__(256)
     xstage3d_kernel(NL3DArgs na, Stage3DArgs a, const cf* __restrict__ P, cf* __restrict__ N0out, int kcut,
                     const cf* __restrict__ tw, const cf* __restrict__ U0, const cf* __restrict__ N0,
-                    cf* __restrict__ S, cf* __restrict__ S1, cf* __restrict__ A) {
+                    cf* __restrict__ S, cf* __restrict__ S1, cf* __restrict__ A, cf* __restrict__ K) {
   using C = XStageCfg<N>;
   constexpr int PT = C::PT, T = C::T, NP = ZProd<KIND>::NP;
   constexpr int LDSN = LdsSize<N>::value;
@@ -1713,39 +1713,61 @@ __global__ void __launch_bounds__(256)
       const long long k = sli + (long long)ix * inner;
       const long long i = sb0 + (long long)ix * inner;
       if (!sact) continue;
+      cf sv;
       if constexpr (first) {
         N0out[i] = r;
-        stage_core<M>(a.e_im, a.nr, tb, k, i, U0, S, S1, A, r, cf{0.f, 0.f});
+        sv = stage_core<M>(a.e_im, a.nr, tb, k, i, U0, S, S1, A, r, cf{0.f, 0.f});
       } else {
-        stage_core<M>(a.e_im, a.nr, tb, k, i, U0, S, S1, A, stage_reads_n0(M) ? N0[i] : cf{0.f, 0.f}, r);
+        sv = stage_core<M>(a.e_im, a.nr, tb, k, i, U0, S, S1, A, stage_reads_n0(M) ? N0[i] : cf{0.f, 0.f}, r);
       }
+      if constexpr (EMIT) lds_x[cl * LDSN + ix] = sv;  // the slot this thread alone read
     }
   });
+  if constexpr (EMIT) {
+    // the new state's inverse x lines (times 1/N, every kz plane: the first
+    // pass of irfftn) to K, from the lane groups' line order
+    __syncthreads();
+    cf e[1][PT];
+#pragma unroll
+    for (int s = 0; s < PT; ++s) e[0][s] = lds[j + s * T];
+    cfft<N, PT, true, 1>(e, ln, lds, tr);  // only this group uses its LDS line from here on
+    if (raw < nlines) {
+      constexpr float rn = 1.f / (float)N;
+#pragma unroll
+      for (int s = 0; s < PT; ++s) K[inst * a.nr + li + (long long)(j + s * T) * inner] = rn * e[0][s];
+    }
+  }
 }
 
 // Fused forward x lines, combine and stage of the kinds with fused z rows;
-// hipErrorInvalidValue for the other kinds and sizes.
+// hipErrorInvalidValue for the other kinds and sizes. K: where the new
+// state's inverse x lines go (the emitting last stage of a rollout step), or
+// nullptr.
 hipError_t launch_xstage3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P, cf* N0out, int kcut, const cf* tw,
-                           hipStream_t st) {
+                           hipStream_t st, cf* K) {
   const bool first = a.mode == ST1 || a.mode == ST2A || a.mode == ST3A || a.mode == ST4A;
   if (first != (N0out != nullptr)) return hipErrorInvalidValue;  // the first stage keeps N0, the others read it
   auto go = [&](auto kern, int cpb) {
     const long long nlines = a.ninst * (long long)na.N * na.NZ;
     hipLaunchKernelGGL(kern, dim3((unsigned)((nlines + cpb - 1) / cpb)), dim3(256), 0, st, na, a, P, N0out, kcut,
-                       tw, a.U0, a.N0, a.S, a.S1, a.A);
+                       tw, a.U0, a.N0, a.S, a.S1, a.A, K);
     return hipGetLastError();
   };
-#define EXMI_XSTAGE(NN)                                                                      \
-  case NN:                                                                                   \
-    switch (na.kind) {                                                                       \
-      case NL_GRADNORM: return go(xstage3d_kernel<NN, NL_GRADNORM>, XStageCfg<NN>::CPB);     \
-      case NL_GENERAL: return go(xstage3d_kernel<NN, NL_GENERAL>, XStageCfg<NN>::CPB);       \
-      case NL_POLY: return go(xstage3d_kernel<NN, NL_POLY>, XStageCfg<NN>::CPB);             \
-      case NL_GRAY_SCOTT: return go(xstage3d_kernel<NN, NL_GRAY_SCOTT>, XStageCfg<NN>::CPB); \
-      case NL_BZ: return go(xstage3d_kernel<NN, NL_BZ>, XStageCfg<NN>::CPB);                 \
-      case NL_CONV_CONS: return go(xstage3d_kernel<NN, NL_CONV_CONS>, XStageCfg<NN>::CPB);   \
-      case NL_CONV_NONCONS: return go(xstage3d_kernel<NN, NL_CONV_NONCONS>, XStageCfg<NN>::CPB); \
-      default: return hipErrorInvalidValue;                                                  \
+#define EXMI_XSK(NN, KK)                                                                              \
+  case KK:                                                                                            \
+    return K ? go(xstage3d_kernel<NN, KK, true>, XStageCfg<NN>::CPB)                                  \
+             : go(xstage3d_kernel<NN, KK, false>, XStageCfg<NN>::CPB);
+#define EXMI_XSTAGE(NN)                    \
+  case NN:                                 \
+    switch (na.kind) {                     \
+      EXMI_XSK(NN, NL_GRADNORM)            \
+      EXMI_XSK(NN, NL_GENERAL)             \
+      EXMI_XSK(NN, NL_POLY)                \
+      EXMI_XSK(NN, NL_GRAY_SCOTT)          \
+      EXMI_XSK(NN, NL_BZ)                  \
+      EXMI_XSK(NN, NL_CONV_CONS)           \
+      EXMI_XSK(NN, NL_CONV_NONCONS)        \
+      default: return hipErrorInvalidValue; \
     }
   switch (na.N) {
     EXMI_XSTAGE(16)
@@ -1755,6 +1777,7 @@ hipError_t launch_xstage3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* P
     default: return hipErrorInvalidValue;
   }
 #undef EXMI_XSTAGE
+#undef EXMI_XSK
 }
 
 hipError_t launch_stage_nl3d(const NL3DArgs& na, const Stage3DArgs& a, const cf* PH, cf* N0out, hipStream_t st) {
