@@ -1196,8 +1196,9 @@ static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf
   if (e == hipSuccess && !yz)
     e = launch_lines(N, true, p->F3, I * N * NZ, NZ, N * NZ, NZ, r, tl, st, nullptr, kz + 1);
   if (p->kind3 == NL_PROJ3D) {
-    // the z rows with the cross products fused (zcross3d_kernel), the
-    // product spectra in R3 (which holds I N^3 >= ninst N^2 (N + 2) floats)
+    // the z rows with the cross products fused (zcross3d_kernel, or with
+    // the y lines yzprod3d_kernel), the product spectra in R3 (which holds
+    // I N^3 >= ninst N^2 (N + 2) floats)
     cf* P = (cf*)p->R3;
     if (e == hipSuccess)
       e = yz ? launch_yzprod3d(a, p->F3, P, kz, tl, p->tw.rt(), st)
@@ -1210,8 +1211,9 @@ static hipError_t nonlinear3d_products(exmi_plan* p, const NL3DArgs& a, const cf
     return e;
   }
   if (zprod_kind(p)) {
-    // the z rows with the products fused (zprod3d_kernel); one field and one
-    // product per instance: in place in F3; three products per instance
+    // the z rows with the products fused (zprod3d_kernel, or with the y
+    // lines yzprod3d_kernel); one field and one product per instance: in
+    // place in F3; three products per instance
     // (the conservative convection): into PR3 (allocated to hold them as
     // spectra); else into R3 (which holds I NF N^3 >= ninst NP N^2 (N + 2)
     // floats)
