@@ -1232,12 +1232,30 @@ __global__ void __launch_bounds__(256) fields_unit_lines_kernel(NL3DArgs a, cons
     }
   }
   cfft<N, PT, true, NOUT>(v, ln, lds, tr);
-  if (active) {
+  // the stores run transposed through LDS: thread (q0, cl) stores the
+  // points ix = q0 + q T of the workgroup's line cl, so that a wavefront
+  // writes neighbouring lines (runs of up to kkeep entries) per point
+  const int cl = threadIdx.x % C::CPB, q0 = threadIdx.x / C::CPB;
+  const long long sraw = (long long)blockIdx.x * C::CPB + cl;
+  const bool sact = sraw < nrun;
+  const long long srun = sact ? sraw : 0;
+  const long long sunit = srun / ((long long)N * kkeep);
+  const int srr = (int)(srun - sunit * N * kkeep);
+  const int siy = srr / kkeep, siz = srr - siy * kkeep;
+  const long long sli = (long long)siy * NZ + siz;
 #pragma unroll
-    for (int q = 0; q < NOUT; ++q) {
-      cf* o = F + (unit * NOUT + q) * a.Nr + li;
+  for (int q = 0; q < NOUT; ++q) {
+    if (q > 0) __syncthreads();  // the previous field's reads are done
 #pragma unroll
-      for (int s = 0; s < PT; ++s) o[(long long)(j + s * T) * inner] = scale * v[q][s];
+    for (int s = 0; s < PT; ++s) lds[j + s * T] = scale * v[q][s];
+    __syncthreads();
+    if (sact) {
+      cf* o = F + (sunit * NOUT + q) * a.Nr + sli;
+#pragma unroll
+      for (int qq = 0; qq < PT; ++qq) {
+        const int ix = q0 + qq * T;
+        o[(long long)ix * inner] = lds_x[cl * LDSN + ix];
+      }
     }
   }
 }
