@@ -1521,20 +1521,35 @@ __global__ void __launch_bounds__(256)
   const long long traj = line / inner;
   const int li = (int)(line - traj * inner);
   const int iy = li / NZ, iz = li - iy * NZ;
-  const bool kept = iz <= kcut;
   cf* lds = lds_x + c * LDSN;
   const Lane<N, PT> ln(j);
   TwRegs<N, PT> tr;
   tr.template load_pass<1>(j, tw);
   cf v[3][PT];
   const long long b0 = traj * 3 * a.nr + li;  // channel 0's entry of this line at ix = 0
+  // The loads and the stage run transposed through LDS: thread (q0, cl)
+  // takes the points ix = q0 + q T of the workgroup's line cl, so that a
+  // wavefront's accesses cover neighbouring lines (contiguous entries).
+  const int cl = threadIdx.x % C::CPB, q0 = threadIdx.x / C::CPB;
+  const long long sraw = (long long)blockIdx.x * C::CPB + cl;
+  const bool sact = sraw < nlines;
+  const long long sline = sact ? sraw : 0;
+  const long long straj = sline / inner, sli = sline - straj * inner;
+  const long long sb0 = straj * 3 * a.nr + sli;
+  const bool skept = sli % NZ <= kcut;
 #pragma unroll
   for (int ch = 0; ch < 3; ++ch) {
+    if (ch > 0) __syncthreads();  // the previous channel's reads are done
 #pragma unroll
-    for (int s = 0; s < PT; ++s)
-      v[ch][s] = kept ? P[b0 + ch * a.nr + (long long)(j + s * T) * inner] : cf{0.f, 0.f};
+    for (int q = 0; q < PT; ++q) {
+      const int ix = q0 + q * T;
+      lds_x[cl * LDSN + ix] = skept ? P[sb0 + ch * a.nr + (long long)ix * inner] : cf{0.f, 0.f};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < PT; ++s) v[ch][s] = lds[j + s * T];
   }
-  cfft<N, PT, false, 3>(v, ln, lds, tr);
+  cfft<N, PT, false, 3>(v, ln, lds, tr);  // only this group uses its LDS line until the next barrier
   // the projection (as in stage_nl3d_proj_kernel), in place
 #pragma unroll
   for (int s = 0; s < PT; ++s) {
@@ -1549,15 +1564,6 @@ __global__ void __launch_bounds__(256)
     v[2][s] = m.keep ? p2 + imul(m.kz, pr) : z;
     if (m.inj) v[0][s] = v[0][s] + cf{na.inj_value, 0.f};
   }
-  // The stage runs transposed through LDS: thread (q0, cl) takes the points
-  // ix = q0 + q T of the workgroup's line cl, so that a wavefront's accesses
-  // to the state buffers cover neighbouring lines (contiguous entries).
-  const int cl = threadIdx.x % C::CPB, q0 = threadIdx.x / C::CPB;
-  const long long sraw = (long long)blockIdx.x * C::CPB + cl;
-  const bool sact = sraw < nlines;
-  const long long sline = sact ? sraw : 0;
-  const long long straj = sline / inner, sli = sline - straj * inner;
-  const long long sb0 = straj * 3 * a.nr + sli;
   // one copy of the stage loop per stage mode (a per-point switch would put
   // every mode's code into every point's path)
   // With EMIT (the step's last stage of a rollout that emits this state)
@@ -1599,14 +1605,21 @@ __global__ void __launch_bounds__(256)
     }
   });
   if constexpr (EMIT) {
-    // only this group uses its LDS line from here on
-    cfft<N, PT, true, 3>(e, ln, lds, tr);
-    if (raw < nlines) {
-      constexpr float rn = 1.f / (float)N;
+    cfft<N, PT, true, 3>(e, ln, lds, tr);  // only this group uses its LDS line until the next barrier
+    // stored transposed through LDS, as the loads
+    constexpr float rn = 1.f / (float)N;
 #pragma unroll
-      for (int ch = 0; ch < 3; ++ch) {
+    for (int ch = 0; ch < 3; ++ch) {
+      if (ch > 0) __syncthreads();  // the previous channel's reads are done
 #pragma unroll
-        for (int s = 0; s < PT; ++s) K[b0 + ch * a.nr + (long long)(j + s * T) * inner] = rn * e[ch][s];
+      for (int s = 0; s < PT; ++s) lds[j + s * T] = rn * e[ch][s];
+      __syncthreads();
+      if (sact) {
+#pragma unroll
+        for (int q = 0; q < PT; ++q) {
+          const int ix = q0 + q * T;
+          K[sb0 + ch * a.nr + (long long)ix * inner] = lds_x[cl * LDSN + ix];
+        }
       }
     }
   }
