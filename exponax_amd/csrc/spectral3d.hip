@@ -1761,11 +1761,18 @@ __global__ void __launch_bounds__(256)
     cf e[1][PT];
 #pragma unroll
     for (int s = 0; s < PT; ++s) e[0][s] = lds[j + s * T];
-    cfft<N, PT, true, 1>(e, ln, lds, tr);  // only this group uses its LDS line from here on
-    if (raw < nlines) {
-      constexpr float rn = 1.f / (float)N;
+    cfft<N, PT, true, 1>(e, ln, lds, tr);  // only this group uses its LDS line until the next barrier
+    // stored transposed through LDS, as the stage's accesses
+    constexpr float rn = 1.f / (float)N;
 #pragma unroll
-      for (int s = 0; s < PT; ++s) K[inst * a.nr + li + (long long)(j + s * T) * inner] = rn * e[0][s];
+    for (int s = 0; s < PT; ++s) lds[j + s * T] = rn * e[0][s];
+    __syncthreads();
+    if (sact) {
+#pragma unroll
+      for (int q = 0; q < PT; ++q) {
+        const int ix = q0 + q * T;
+        K[sb0 + (long long)ix * inner] = lds_x[cl * LDSN + ix];
+      }
     }
   }
 }
