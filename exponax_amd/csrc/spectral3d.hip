@@ -1527,9 +1527,11 @@ __global__ void __launch_bounds__(256)
   tr.template load_pass<1>(j, tw);
   cf v[3][PT];
   const long long b0 = traj * 3 * a.nr + li;  // channel 0's entry of this line at ix = 0
-  // The loads and the stage run transposed through LDS: thread (q0, cl)
-  // takes the points ix = q0 + q T of the workgroup's line cl, so that a
-  // wavefront's accesses cover neighbouring lines (contiguous entries).
+  // The stage (and, in the emitting variant, the loads: measured faster
+  // there, slower in the other) runs transposed through LDS: thread
+  // (q0, cl) takes the points ix = q0 + q T of the workgroup's line cl, so
+  // that a wavefront's accesses cover neighbouring lines (contiguous
+  // entries).
   const int cl = threadIdx.x % C::CPB, q0 = threadIdx.x / C::CPB;
   const long long sraw = (long long)blockIdx.x * C::CPB + cl;
   const bool sact = sraw < nlines;
@@ -1537,17 +1539,27 @@ __global__ void __launch_bounds__(256)
   const long long straj = sline / inner, sli = sline - straj * inner;
   const long long sb0 = straj * 3 * a.nr + sli;
   const bool skept = sli % NZ <= kcut;
+  if constexpr (EMIT) {
 #pragma unroll
-  for (int ch = 0; ch < 3; ++ch) {
-    if (ch > 0) __syncthreads();  // the previous channel's reads are done
+    for (int ch = 0; ch < 3; ++ch) {
+      if (ch > 0) __syncthreads();  // the previous channel's reads are done
 #pragma unroll
-    for (int q = 0; q < PT; ++q) {
-      const int ix = q0 + q * T;
-      lds_x[cl * LDSN + ix] = skept ? P[sb0 + ch * a.nr + (long long)ix * inner] : cf{0.f, 0.f};
+      for (int q = 0; q < PT; ++q) {
+        const int ix = q0 + q * T;
+        lds_x[cl * LDSN + ix] = skept ? P[sb0 + ch * a.nr + (long long)ix * inner] : cf{0.f, 0.f};
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < PT; ++s) v[ch][s] = lds[j + s * T];
     }
-    __syncthreads();
+  } else {
+    const bool kept = iz <= kcut;
 #pragma unroll
-    for (int s = 0; s < PT; ++s) v[ch][s] = lds[j + s * T];
+    for (int ch = 0; ch < 3; ++ch) {
+#pragma unroll
+      for (int s = 0; s < PT; ++s)
+        v[ch][s] = kept ? P[b0 + ch * a.nr + (long long)(j + s * T) * inner] : cf{0.f, 0.f};
+    }
   }
   cfft<N, PT, false, 3>(v, ln, lds, tr);  // only this group uses its LDS line until the next barrier
   // the projection (as in stage_nl3d_proj_kernel), in place
