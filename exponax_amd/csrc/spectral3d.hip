@@ -1527,8 +1527,8 @@ __global__ void __launch_bounds__(256)
   tr.template load_pass<1>(j, tw);
   cf v[3][PT];
   const long long b0 = traj * 3 * a.nr + li;  // channel 0's entry of this line at ix = 0
-  // The stage (and, in the emitting variant, the loads: measured faster
-  // there, slower in the other) runs transposed through LDS: thread
+  // The stage (and, in the emitting variant, the loads; the non-emitting
+  // variant measured 1.4 us slower with them) runs transposed through LDS: thread
   // (q0, cl) takes the points ix = q0 + q T of the workgroup's line cl, so
   // that a wavefront's accesses cover neighbouring lines (contiguous
   // entries).
